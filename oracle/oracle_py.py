@@ -1,0 +1,272 @@
+"""ctypes binding of the oracle (liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+The oracle is the CPU restatement of the reference's bulk EMC step used as
+the checker.  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg may import this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+Q = 1.60219e-19
+KB = 1.38066e-23
+ME = 9.11e-31
+
+_D = C.c_double
+_I = C.c_int
+_PD = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_PI32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_PU64 = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+_PSZ = np.ctypeslib.ndpointer(dtype=np.uintp, flags="C_CONTIGUOUS")
+
+
+def lib_path():
+    return os.path.join(_HERE, "liboracle.so")
+
+
+def load():
+    """Load liboracle.so (built by ``make -C oracle``)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = lib_path()
+    if not os.path.exists(path):
+        raise RuntimeError("oracle not built: run `make -C oracle` (%s missing)" % path)
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    sig = {
+        "orc_model_new": (vp, [_I, _D]),
+        "orc_model_delete": (None, [vp]),
+        "orc_model_add_valley_params": (_I, [vp, _I, _D, _D, _D, _D, _I, _D, _D, C.c_void_p]),
+        "orc_model_add_acoustic": (_I, [vp, _I, _D, _D, _D, _D]),
+        "orc_model_add_iv": (_I, [vp, _I, _I, _I, _D, _D, _D, _D, _I, _PI32]),
+        "orc_model_add_froehlich": (_I, [vp, _I, _I, _D, _D, _D, _D, _D]),
+        "orc_model_build_tables": (None, [vp]),
+        "orc_model_n_tables": (_I, [vp, _I]),
+        "orc_model_table_mech": (_I, [vp, _I, _I]),
+        "orc_model_tau0": (_D, [vp, _I]),
+        "orc_model_dE": (_D, [vp]),
+        "orc_model_get_cum": (None, [vp, _I, _PD]),
+        "orc_model_rate": (_D, [vp, _I, _D]),
+        "orc_energy_level": (_I, [vp, _D]),
+        "orc_scatter_one": (_I, [vp, _PD, _PD, _PI32, _PI32, _PU64, C.c_size_t, _PSZ]),
+        "orc_mech_scatter_one": (None, [vp, _I, _PD, _PD, _PI32, _PI32, _PU64, C.c_size_t, _PSZ]),
+        "orc_drift_one": (None, [vp, _I, _I, _D, _PD, _PD, _PD, _PD]),
+        "orc_velocity_one": (None, [vp, _I, _I, _PD, _D, _PD]),
+        "orc_normk_one": (_D, [vp, _I, _D]),
+        "orc_init_particles_replay": (None, [vp, _I, _D, _PU64, C.c_size_t, _PSZ, _PD, _PD, _PD, _PD, _PI32, _PI32]),
+        "orc_run_replay": (_I, [vp, _I, _PD, _PD, _PD, _PD, _PI32, _PI32, _PU64, C.c_size_t, _PSZ, _I, _D, _PD,
+                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "orc_run_shared": (_I, [vp, C.c_uint64, _PD, _PD, _D, _D, _PD, _D, _D, _I, _I, _PD,
+                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "orc_run_philox": (None, [vp, _I, _PD, _PD, _PD, _PI32, _PI32, _I, _D, _PD, _PD, C.c_uint64,
+                                  C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
+        "orc_init_particles_philox": (None, [vp, _I, _D, C.c_uint64, C.c_uint32, _PD, _PD, _PD, _PI32, _PI32]),
+        "orc_run_mt": (None, [vp, _I, C.c_uint64, _D, _I, _D, _PD, _PD, C.c_void_p, _I,
+                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "orc_mt64_fill": (None, [C.c_uint64, _PU64, C.c_size_t]),
+        "orc_mt64_fill_particles": (None, [C.c_uint64, C.c_uint64, C.c_size_t, _PU64, C.c_size_t, C.c_size_t]),
+        "orc_particle_seed": (C.c_uint64, [C.c_uint64, C.c_uint64]),
+        "orc_canonical": (_D, [C.c_uint64]),
+        "orc_philox4x32_10": (None, [np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"),
+                                     np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"),
+                                     np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")]),
+        "orc_philox_u01_f64": (_D, [C.c_uint32, C.c_uint32]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class OracleModel:
+    """A reference particle type (valleys + mechanisms + tables) in the oracle."""
+
+    def __init__(self, spec):
+        L = load()
+        self.L = L
+        self.spec = spec
+        self.h = L.orc_model_new(spec["n_levels"], spec["max_energy"])
+        T = spec["T"]
+        for v in spec["valleys"]:
+            axes = None
+            if "axes" in v:
+                axes = np.ascontiguousarray(np.array(v["axes"], dtype=np.float64).reshape(-1))
+            m = v["mass"]
+            L.orc_model_add_valley_params(self.h, v["kind"], m[0], m[1], m[2], ME, v["deg"],
+                                          v.get("alpha", 0.0), v.get("bottom", 0.0),
+                                          None if axes is None else axes.ctypes.data_as(C.c_void_p))
+            self._keep = axes
+        for mech in spec["mechs"]:
+            k = mech["kind"]
+            if k == 0:
+                L.orc_model_add_acoustic(self.h, mech["valley"], mech["sigma"], spec.get("rho", 2329.0),
+                                         spec.get("v_sound", 9040.0), T)
+            elif k in (1, 2, 3, 4):
+                fmap = np.ascontiguousarray(np.array(mech["fmap"], dtype=np.int32).reshape(-1))
+                Z = len(mech["fmap"][0])
+                L.orc_model_add_iv(self.h, k, mech["valley"], mech["final"], mech["defpot"], mech["hw"],
+                                   spec.get("rho", 2329.0), T, Z, fmap)
+            else:
+                L.orc_model_add_froehlich(self.h, k, mech["valley"], mech["hw"], mech["mrel"],
+                                          mech["eps_hi"], mech["eps_lo"], T)
+        L.orc_model_build_tables(self.h)
+        self.n_valleys = len(spec["valleys"])
+
+    def __del__(self):
+        try:
+            self.L.orc_model_delete(self.h)
+        except Exception:
+            pass
+
+    # -- tables ----------------------------------------------------------
+    def n_tables(self, v=0):
+        return self.L.orc_model_n_tables(self.h, v)
+
+    def tau0(self, v=0):
+        return self.L.orc_model_tau0(self.h, v)
+
+    def cum(self, v=0):
+        out = np.zeros(self.n_tables(v) * self.spec["n_levels"], dtype=np.float64)
+        self.L.orc_model_get_cum(self.h, v, out)
+        return out.reshape(self.n_tables(v), self.spec["n_levels"])
+
+    def rate(self, mech, E):
+        return self.L.orc_model_rate(self.h, mech, E)
+
+    def energy_level(self, E):
+        return self.L.orc_energy_level(self.h, E)
+
+    # -- single operations ---------------------------------------------
+    def drift(self, valley, sub, dt, k, E, pos, force):
+        k = np.array(k, dtype=np.float64)
+        Ea = np.array([E], dtype=np.float64)
+        pos = np.array(pos, dtype=np.float64)
+        self.L.orc_drift_one(self.h, valley, sub, dt, k, Ea, pos, np.array(force, dtype=np.float64))
+        return k, Ea[0], pos
+
+    def velocity(self, valley, sub, k, E):
+        out = np.zeros(3)
+        self.L.orc_velocity_one(self.h, valley, sub, np.array(k, dtype=np.float64), E, out)
+        return out
+
+    def normk(self, valley, E):
+        return self.L.orc_normk_one(self.h, valley, E)
+
+    def mech_scatter(self, mech, k, E, valley, sub, draws):
+        k = np.array(k, dtype=np.float64)
+        Ea = np.array([E], dtype=np.float64)
+        va = np.array([valley], dtype=np.int32)
+        sa = np.array([sub], dtype=np.int32)
+        used = np.zeros(1, dtype=np.uintp)
+        d = np.ascontiguousarray(draws, dtype=np.uint64)
+        self.L.orc_mech_scatter_one(self.h, mech, k, Ea, va, sa, d, d.size, used)
+        return k, Ea[0], int(va[0]), int(sa[0]), int(used[0])
+
+    # -- ensembles -------------------------------------------------------
+    def init_replay(self, draws, cursor, Vt):
+        np_ = draws.shape[0]
+        k = np.zeros(3 * np_)
+        E = np.zeros(np_)
+        tau = np.zeros(np_)
+        gt = np.zeros(np_)
+        v = np.zeros(np_, dtype=np.int32)
+        s = np.zeros(np_, dtype=np.int32)
+        self.L.orc_init_particles_replay(self.h, np_, Vt, draws, draws.shape[1], cursor, k, E, tau, gt, v, s)
+        return dict(k=k, E=E, tau=tau, grain=gt, valley=v, sub=s)
+
+    def run_replay(self, st, draws, cursor, nsteps, dt, force, trace=False):
+        np_ = st["E"].size
+        outs = {}
+        if trace:
+            outs = dict(valley=np.zeros(nsteps * np_, dtype=np.int32), sub=np.zeros(nsteps * np_, dtype=np.int32),
+                        nev=np.zeros(nsteps * np_, dtype=np.int32), k=np.zeros(nsteps * np_ * 3),
+                        E=np.zeros(nsteps * np_))
+        nover = self.L.orc_run_replay(
+            self.h, np_, st["k"], st["E"], st["tau"], st["grain"], st["valley"], st["sub"], draws,
+            draws.shape[1], cursor, nsteps, dt, np.array(force, dtype=np.float64),
+            _ptr(outs.get("valley")), _ptr(outs.get("sub")), _ptr(outs.get("nev")), _ptr(outs.get("k")),
+            _ptr(outs.get("E")), None)
+        return nover, outs
+
+    def run_philox(self, st, nsteps, dt, force, field_dir, seed, instance=0, step0=0, obs=True):
+        np_ = st["E"].size
+        nv = self.n_valleys
+        o = np.zeros(nsteps * 3 * nv) if obs else None
+        self.L.orc_run_philox(self.h, np_, st["k"], st["E"], st["tau"], st["valley"], st["sub"], nsteps, dt,
+                              np.array(force, dtype=np.float64), np.array(field_dir, dtype=np.float64),
+                              seed, instance, step0, _ptr(o), None)
+        return None if o is None else o.reshape(nsteps, 3, nv)
+
+    def init_philox(self, np_, Vt, seed, instance=0):
+        k = np.zeros(3 * np_)
+        E = np.zeros(np_)
+        tau = np.zeros(np_)
+        v = np.zeros(np_, dtype=np.int32)
+        s = np.zeros(np_, dtype=np.int32)
+        self.L.orc_init_particles_philox(self.h, np_, Vt, seed, instance, k, E, tau, v, s)
+        return dict(k=k, E=E, tau=tau, valley=v, sub=s)
+
+    def run_mt(self, np_, seed, Vt, nsteps, dt, force, field_dir, obs=True, threads=1, final=False):
+        nv = self.n_valleys
+        o = np.zeros(nsteps * 3 * nv) if obs else None
+        fin = None
+        if final:
+            fin = dict(k=np.zeros(3 * np_), E=np.zeros(np_), valley=np.zeros(np_, dtype=np.int32),
+                       sub=np.zeros(np_, dtype=np.int32))
+        self.L.orc_run_mt(self.h, np_, seed, Vt, nsteps, dt, np.array(force, dtype=np.float64),
+                          np.array(field_dir, dtype=np.float64), _ptr(o), threads,
+                          _ptr(fin and fin["k"]), _ptr(fin and fin["E"]), _ptr(fin and fin["valley"]),
+                          _ptr(fin and fin["sub"]))
+        return (None if o is None else o.reshape(nsteps, 3, nv)), fin
+
+    def run_shared(self, seed, max_pos, spacing, doping, T, field_dir, field_strength, dt, nsteps,
+                   max_particles=100000):
+        nv = self.n_valleys
+        obs = np.zeros((nsteps + 1) * 3 * nv)
+        n = self.L.orc_run_shared(self.h, seed, np.array(max_pos, dtype=np.float64),
+                                  np.array(spacing, dtype=np.float64), doping, T,
+                                  np.array(field_dir, dtype=np.float64), field_strength, dt, nsteps,
+                                  max_particles, obs, None, None, None, None)
+        return n, obs.reshape(nsteps + 1, 3, nv)
+
+
+def mt64_fill(seed, n):
+    out = np.zeros(n, dtype=np.uint64)
+    load().orc_mt64_fill(seed, out, n)
+    return out
+
+
+def mt64_particle_draws(seed, np_, n, p0=0):
+    out = np.zeros((np_, n), dtype=np.uint64)
+    load().orc_mt64_fill_particles(seed, p0, np_, out, n, n)
+    return out
+
+
+def particle_seed(seed, pid):
+    return load().orc_particle_seed(seed, pid)
+
+
+def canonical(x):
+    return load().orc_canonical(x)
+
+
+def philox(ctr, key):
+    out = np.zeros(4, dtype=np.uint32)
+    load().orc_philox4x32_10(np.array(ctr, dtype=np.uint32), np.array(key, dtype=np.uint32), out)
+    return out
+
+
+def thermal_voltage(T):
+    """kB / q * T (emcDevice.hpp:88)."""
+    return KB / Q * T

@@ -1,0 +1,509 @@
+// ref_harness.cpp -- golden-vector generator built against the ViennaEMC
+// headers where they lie (include path set by oracle/Makefile).
+//
+// TEST INFRASTRUCTURE ONLY.  This driver is our own code: it instantiates
+// the reference's valley, mechanism, particle-type, drift and bulk-handler
+// classes and records what they compute, so that the oracle restatement
+// and the HIP kernels can be checked against the reference itself.  The
+// binary lands in oracle/_ref/ and is run from tests/golden/make_golden.py.
+//
+// Output: a text stream of sections "@<name> <dtype> <count>" followed by
+// one line of values (doubles printed with %.17g, integers as %lld).
+//
+// Usage: ref_harness <outfile>      (run with cwd = a scratch folder: the
+// reference's table initialisation writes *ScatterMechanism.txt files).
+
+#include <cstdio>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include <ParticleType/emcElectron.hpp>
+#include <ScatterMechanisms/emcAcousticScatterMechanism.hpp>
+#include <ScatterMechanisms/emcFirstOrderInterValleyScatterMechanism.hpp>
+#include <ScatterMechanisms/emcFroehlichInteraction.hpp>
+#include <ScatterMechanisms/emcZeroOrderInterValleyScatterMechanism.hpp>
+#include <ValleyTypes/emcNonParabolicAnistropValley.hpp>
+#include <ValleyTypes/emcNonParabolicIsotropValley.hpp>
+#include <ValleyTypes/emcParabolicAnisotropValley.hpp>
+#include <ValleyTypes/emcParabolicIsotropValley.hpp>
+#include <emcConstants.hpp>
+#include <emcDevice.hpp>
+#include <emcParticleDrift.hpp>
+
+#include "../examples/bulkSimulation/basicBulkParticleHandler.hpp"
+
+using T = double;
+using Device = emcDevice<T, 3>;
+using Electron = emcElectron<T, Device>;
+using Handler = basicBulkParticleHandler<T, Device>;
+using TypeMap = Handler::MapIdxToParticleTypes;
+
+static FILE *out = nullptr;
+
+static void sec(const char *name, const std::vector<double> &v) {
+  std::fprintf(out, "@%s f8 %zu\n", name, v.size());
+  for (size_t i = 0; i < v.size(); i++)
+    std::fprintf(out, i ? " %.17g" : "%.17g", v[i]);
+  std::fprintf(out, "\n");
+}
+static void seci(const char *name, const std::vector<long long> &v) {
+  std::fprintf(out, "@%s i8 %zu\n", name, v.size());
+  for (size_t i = 0; i < v.size(); i++)
+    std::fprintf(out, i ? " %lld" : "%lld", v[i]);
+  std::fprintf(out, "\n");
+}
+static void secu(const char *name, const std::vector<uint64_t> &v) {
+  std::fprintf(out, "@%s u8 %zu\n", name, v.size());
+  for (size_t i = 0; i < v.size(); i++)
+    std::fprintf(out, i ? " %llu" : "%llu", (unsigned long long)v[i]);
+  std::fprintf(out, "\n");
+}
+static std::string nm(const std::string &a, const std::string &b) {
+  return a + "." + b;
+}
+
+static uint64_t splitmix64(uint64_t x) {
+  uint64_t z = x + 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+// Silicon material of examples/SiliconFunctions.hpp:21-25
+static emcMaterial<T> silicon() { return emcMaterial<T>{11.8, 2329., 1.45e16, 9040, 1.15}; }
+
+// ---------------------------------------------------------------------------
+// Model descriptions (mirrored by tests/emc_models.py)
+// ---------------------------------------------------------------------------
+struct Model {
+  std::string name;
+  int nLevels;
+  double maxE;
+  double T;
+  double fieldKVcm;
+  std::array<double, 3> fieldDir;
+};
+
+static const double kSubAxes[3][3][3] = {
+    {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}},
+    {{0, 1, 0}, {1, 0, 0}, {0, 0, 1}},
+    {{0, 0, 1}, {0, 1, 0}, {1, 0, 0}}};
+// [111]-type frames for the rotated-valley model
+static const double kAxes111[4][3][3] = {
+    {{1, 1, 1}, {1, -1, 0}, {1, 1, -2}},
+    {{-1, 1, 1}, {1, 1, 0}, {-1, 1, -2}},
+    {{1, -1, 1}, {1, 1, 0}, {1, -1, -2}},
+    {{1, 1, -1}, {1, -1, 0}, {1, 1, 2}}};
+
+static const std::map<SizeType, std::vector<SizeType>> kFinalG = {
+    {0, {0}}, {1, {1}}, {2, {2}}};
+static const std::map<SizeType, std::vector<SizeType>> kFinalF = {
+    {0, {1, 1, 2, 2}}, {1, {0, 0, 2, 2}}, {2, {0, 0, 1, 1}}};
+
+// Jacoboni-Reggiani (RMP 55, 645, 1983) zero-order Si phonons:
+// (Theta [K], D [eV/cm]); g first, then f.
+static const double kJacG[3][2] = {{140, 0.5e8}, {215, 0.8e8}, {720, 11e8}};
+static const double kJacF[3][2] = {{220, 0.3e8}, {550, 2e8}, {685, 2e8}};
+
+static double thetaToEV(double theta) { return constants::kB * theta / constants::q; }
+
+using MechPtr = std::unique_ptr<emcScatterMechanism<T>>;
+
+// Valleys of a model, added to a particle type.
+static void addValleys(const Model &m, Electron &tp) {
+  if (m.name == "si_jacoboni" || m.name == "si_vasileska") {
+    double mt = (m.name == "si_jacoboni") ? 0.19 : 0.196;
+    auto v = std::make_unique<emcNonParabolicAnisotropValley<T>>(
+        std::array<T, 3>{0.916, mt, mt}, tp.getMass(), 3, 0.5);
+    for (int s = 0; s < 3; s++)
+      v->setSubValleyEllipseCoordSystem(
+          s, {kSubAxes[s][0][0], kSubAxes[s][0][1], kSubAxes[s][0][2]},
+          {kSubAxes[s][1][0], kSubAxes[s][1][1], kSubAxes[s][1][2]},
+          {kSubAxes[s][2][0], kSubAxes[s][2][1], kSubAxes[s][2][2]});
+    tp.addValley(std::move(v));
+  } else if (m.name == "polar_gamma") {
+    tp.addValley(std::make_unique<emcNonParabolicIsotropValley<T>>(0.28, tp.getMass(), 1, 0.3));
+  } else if (m.name == "para_iso") {
+    tp.addValley(std::make_unique<emcParabolicIsotropValley<T>>(0.26, tp.getMass(), 1));
+  } else if (m.name == "two_valley") {
+    auto l = std::make_unique<emcParabolicAnisotropValley<T>>(std::array<T, 3>{1.6, 0.12, 0.12}, tp.getMass(), 4);
+    for (int s = 0; s < 4; s++)
+      l->setSubValleyEllipseCoordSystem(
+          s, {kAxes111[s][0][0], kAxes111[s][0][1], kAxes111[s][0][2]},
+          {kAxes111[s][1][0], kAxes111[s][1][1], kAxes111[s][1][2]},
+          {kAxes111[s][2][0], kAxes111[s][2][1], kAxes111[s][2][2]});
+    tp.addValley(std::move(l));
+    tp.addValley(std::make_unique<emcNonParabolicIsotropValley<T>>(0.067, tp.getMass(), 1, 0.6, 0.3));
+  }
+}
+
+// Mechanisms of a model in registration order.
+static std::vector<MechPtr> makeMechs(const Model &m, const Device &device) {
+  std::vector<MechPtr> v;
+  using ZAb = emcZeroOrderInterValleyAbsorptionScatterMechanism<T>;
+  using ZEm = emcZeroOrderInterValleyEmissionScatterMechanism<T>;
+  using FAb = emcFirstOrderInterValleyAbsorptionScatterMechanism<T>;
+  using FEm = emcFirstOrderInterValleyEmissionScatterMechanism<T>;
+  if (m.name == "si_jacoboni") {
+    v.push_back(std::make_unique<emcAcousticScatterMechanism<T>>(0, 9., device));
+    for (int i = 0; i < 3; i++) {
+      double e = thetaToEV(kJacG[i][0]), d = kJacG[i][1] * 100.;
+      std::string sfx = "G" + std::to_string(i);
+      v.push_back(std::make_unique<ZAb>(sfx, 0, kFinalG, d, e, device));
+      v.push_back(std::make_unique<ZEm>(sfx, 0, kFinalG, d, e, device));
+    }
+    for (int i = 0; i < 3; i++) {
+      double e = thetaToEV(kJacF[i][0]), d = kJacF[i][1] * 100.;
+      std::string sfx = "F" + std::to_string(i);
+      v.push_back(std::make_unique<ZAb>(sfx, 0, kFinalF, d, e, device));
+      v.push_back(std::make_unique<ZEm>(sfx, 0, kFinalF, d, e, device));
+    }
+  } else if (m.name == "si_vasileska") {
+    // examples/SiliconFunctions.hpp:78-145 (acoustic, zero- and first-order f/g)
+    v.push_back(std::make_unique<emcAcousticScatterMechanism<T>>(0, 9., device));
+    v.push_back(std::make_unique<ZAb>("F", 0, kFinalF, 5.23e10, 0.06, device));
+    v.push_back(std::make_unique<ZEm>("F", 0, kFinalF, 5.23e10, 0.06, device));
+    v.push_back(std::make_unique<ZAb>("G", 0, kFinalG, 5.23e10, 0.06, device));
+    v.push_back(std::make_unique<ZEm>("G", 0, kFinalG, 5.23e10, 0.06, device));
+    v.push_back(std::make_unique<FAb>("F", 0, kFinalF, 2.5, 0.023, device));
+    v.push_back(std::make_unique<FEm>("F", 0, kFinalF, 2.5, 0.023, device));
+    v.push_back(std::make_unique<FAb>("G", 0, kFinalG, 4., 0.018, device));
+    v.push_back(std::make_unique<FEm>("G", 0, kFinalG, 4., 0.018, device));
+  } else if (m.name == "polar_gamma") {
+    v.push_back(std::make_unique<emcFroehlichAbsorption3D<T>>(0, 0.06, 0.28, 3.5, 10., m.T));
+    v.push_back(std::make_unique<emcFroehlichEmission3D<T>>(0, 0.06, 0.28, 3.5, 10., m.T));
+  } else if (m.name == "para_iso") {
+    std::map<SizeType, std::vector<SizeType>> self = {{0, {0}}};
+    v.push_back(std::make_unique<emcAcousticScatterMechanism<T>>(0, 9., device));
+    v.push_back(std::make_unique<ZAb>("P", 0, self, 5e10, 0.05, device));
+    v.push_back(std::make_unique<ZEm>("P", 0, self, 5e10, 0.05, device));
+  } else if (m.name == "two_valley") {
+    std::map<SizeType, std::vector<SizeType>> lToL = {{0, {1, 2, 3}}, {1, {0, 2, 3}}, {2, {0, 1, 3}}, {3, {0, 1, 2}}};
+    std::map<SizeType, std::vector<SizeType>> lToG = {{0, {0}}, {1, {0}}, {2, {0}}, {3, {0}}};
+    std::map<SizeType, std::vector<SizeType>> gToL = {{0, {0, 1, 2, 3}}};
+    v.push_back(std::make_unique<emcAcousticScatterMechanism<T>>(0, 7., device));
+    v.push_back(std::make_unique<ZAb>("LL", 0, 0, lToL, 3e10, 0.029, device));
+    v.push_back(std::make_unique<ZEm>("LL", 0, 0, lToL, 3e10, 0.029, device));
+    v.push_back(std::make_unique<ZAb>("LG", 0, 1, lToG, 1e11, 0.0278, device));
+    v.push_back(std::make_unique<ZEm>("LG", 0, 1, lToG, 1e11, 0.0278, device));
+    v.push_back(std::make_unique<emcAcousticScatterMechanism<T>>(1, 7., device));
+    v.push_back(std::make_unique<FAb>("GL", 1, 0, gToL, 3.0, 0.0278, device));
+    v.push_back(std::make_unique<FEm>("GL", 1, 0, gToL, 3.0, 0.0278, device));
+  }
+  return v;
+}
+
+// Builds the particle type for a model (valleys + mechanisms in order).
+static std::unique_ptr<Electron> makeType(const Model &m, Device &device) {
+  auto type = std::make_unique<Electron>(m.nLevels, m.maxE, false);
+  addValleys(m, *type);
+  for (auto &mech : makeMechs(m, device))
+    type->addScatterMechanism({0}, std::move(mech));
+  return type;
+}
+
+// ---------------------------------------------------------------------------
+// Recorded pieces
+// ---------------------------------------------------------------------------
+
+// rates of every mechanism (standalone copies bound to the type's valleys)
+// on a sample grid and on the table grid (l+1)*dE, plus tau0 per valley
+static void dumpRates(const Model &m, const Device &device, Electron &type) {
+  auto mechs = makeMechs(m, device);
+  std::vector<double> energies;
+  for (int i = 0; i <= 400; i++)
+    energies.push_back(m.maxE * 1.1 * i / 400.0);
+  std::vector<double> sample, table;
+  std::vector<long long> mv;
+  double dE = m.maxE / m.nLevels;
+  for (auto &mech : mechs) {
+    mech->setPtrValley(type.valleys);
+    mv.push_back((long long)mech->getIdxValley());
+    for (double e : energies)
+      sample.push_back(mech->getScatterRate(e, 0));
+    for (SizeType l = 0; l < (SizeType)m.nLevels; l++)
+      table.push_back(mech->getScatterRate((l + 1) * dE, 0));
+  }
+  sec(nm(m.name, "rate_energies").c_str(), energies);
+  sec(nm(m.name, "rate_sample").c_str(), sample);
+  sec(nm(m.name, "rate_table").c_str(), table);
+  seci(nm(m.name, "mech_valley").c_str(), mv);
+  std::vector<double> tau0;
+  for (SizeType v = 0; v < type.getNrValleys(); v++)
+    tau0.push_back(type.getTau(v, 0));
+  sec(nm(m.name, "tau0").c_str(), tau0);
+}
+
+struct RngCounter {
+  // number of raw draws between two engine states
+  static long long between(emcRNG before, const emcRNG &after) {
+    long long n = 0;
+    while (!(before == after) && n < 1000) {
+      before();
+      n++;
+    }
+    return n;
+  }
+};
+
+static void dumpTrajectories(const Model &m, Device &device, Electron &type,
+                             uint64_t seed, int np, int nsteps, double dt) {
+  std::array<T, 3> dir = m.fieldDir;
+  normalize(dir);
+  std::array<T, 3> force = scale(scale(dir, m.fieldKVcm * 1e5), type.getCharge());
+  std::vector<double> initK, initE, initTau, initGrain;
+  std::vector<long long> initValley, initSub;
+  std::vector<double> kOut, eOut, tauOut;
+  std::vector<long long> vOut, sOut, nevOut, drawsOut;
+  // per event: particle, step, draws consumed, E before, E after,
+  // valley after, sub after
+  std::vector<long long> evPart, evStep, evDraws, evValley, evSub;
+  std::vector<double> evEb, evEa;
+  std::array<SizeType, 3> coord = {1, 1, 1};
+  std::vector<emcParticle<T>> parts(np);
+  std::vector<emcRNG> rngs;
+  for (int p = 0; p < np; p++) {
+    rngs.emplace_back(splitmix64(seed ^ (uint64_t)p));
+    parts[p] = type.generateInitialParticle(coord, device, rngs[p]);
+    initK.insert(initK.end(), parts[p].k.begin(), parts[p].k.end());
+    initE.push_back(parts[p].energy);
+    initTau.push_back(parts[p].tau);
+    initGrain.push_back(parts[p].grainTau);
+    initValley.push_back((long long)parts[p].valley);
+    initSub.push_back((long long)parts[p].subValley);
+  }
+  std::vector<long long> drawsInit;
+  for (int p = 0; p < np; p++)
+    drawsInit.push_back(RngCounter::between(emcRNG(splitmix64(seed ^ (uint64_t)p)), rngs[p]));
+  for (int s = 0; s < nsteps; s++) {
+    for (int p = 0; p < np; p++) {
+      auto &part = parts[p];
+      auto &rng = rngs[p];
+      std::array<T, 3> pos = {0, 0, 0};
+      emcRNG atStart = rng;
+      // the loop body of basicBulkParticleHandler::moveParticles (:195-221)
+      auto valley = type.getValley(part.valley);
+      drift(std::min(part.tau, dt), part, valley, pos, force);
+      T tRemaining = dt - part.tau;
+      long long nev = 0;
+      while (tRemaining > 0) {
+        emcRNG before = rng;
+        double eb = part.energy;
+        type.scatterParticle(part, rng);
+        evPart.push_back(p);
+        evStep.push_back(s);
+        evDraws.push_back(RngCounter::between(before, rng));
+        evEb.push_back(eb);
+        evEa.push_back(part.energy);
+        evValley.push_back((long long)part.valley);
+        evSub.push_back((long long)part.subValley);
+        nev++;
+        T newTau = type.getNewTau(part.valley, part.region, rng);
+        part.tau += newTau;
+        valley = type.getValley(part.valley);
+        drift(std::min(tRemaining, newTau), part, valley, pos, force);
+        tRemaining -= newTau;
+      }
+      part.tau -= dt;
+      part.grainTau -= dt;
+      if (part.grainTau <= 0) {
+        type.scatterParticleAtGrain(part, rng);
+        part.grainTau = type.getNewGrainTau(rng);
+      }
+      kOut.insert(kOut.end(), part.k.begin(), part.k.end());
+      eOut.push_back(part.energy);
+      tauOut.push_back(part.tau);
+      vOut.push_back((long long)part.valley);
+      sOut.push_back((long long)part.subValley);
+      nevOut.push_back(nev);
+      drawsOut.push_back(RngCounter::between(atStart, rng));
+    }
+  }
+  std::string b = m.name + ".traj";
+  seci((b + ".meta").c_str(), {(long long)np, (long long)nsteps, (long long)seed});
+  sec((b + ".dt").c_str(), {dt});
+  sec((b + ".force").c_str(), {force[0], force[1], force[2]});
+  sec((b + ".init_k").c_str(), initK);
+  sec((b + ".init_E").c_str(), initE);
+  sec((b + ".init_tau").c_str(), initTau);
+  sec((b + ".init_grain").c_str(), initGrain);
+  seci((b + ".init_valley").c_str(), initValley);
+  seci((b + ".init_sub").c_str(), initSub);
+  seci((b + ".init_draws").c_str(), drawsInit);
+  sec((b + ".k").c_str(), kOut);
+  sec((b + ".E").c_str(), eOut);
+  sec((b + ".tau").c_str(), tauOut);
+  seci((b + ".valley").c_str(), vOut);
+  seci((b + ".sub").c_str(), sOut);
+  seci((b + ".nevents").c_str(), nevOut);
+  seci((b + ".draws").c_str(), drawsOut);
+  seci((b + ".ev_part").c_str(), evPart);
+  seci((b + ".ev_step").c_str(), evStep);
+  seci((b + ".ev_draws").c_str(), evDraws);
+  sec((b + ".ev_Eb").c_str(), evEb);
+  sec((b + ".ev_Ea").c_str(), evEa);
+  seci((b + ".ev_valley").c_str(), evValley);
+  seci((b + ".ev_sub").c_str(), evSub);
+}
+
+// drift KATs through the reference drift() for each valley/subvalley
+static void dumpDrift(const Model &m, Electron &type) {
+  std::vector<double> rows;
+  emcRNG rng(12345);
+  std::uniform_real_distribution<T> u(-1., 1.);
+  for (SizeType v = 0; v < type.getNrValleys(); v++) {
+    auto valley = type.getValley(v);
+    for (SizeType s = 0; s < valley->getDegeneracyFactor(); s++) {
+      for (int t = 0; t < 6; t++) {
+        emcParticle<T> part;
+        part.valley = v;
+        part.subValley = s;
+        part.k = {u(rng) * 1e9, u(rng) * 1e9, u(rng) * 1e9};
+        if (t == 0)
+          part.k = {0, 0, 0};
+        std::array<T, 3> force = {u(rng) * 1e-13, u(rng) * 1e-13, u(rng) * 1e-13};
+        std::array<T, 3> pos = {1e-9, 2e-9, 3e-9};
+        double dt = (t + 1) * 0.7e-15;
+        rows.insert(rows.end(), {(double)v, (double)s, part.k[0], part.k[1], part.k[2],
+                                 force[0], force[1], force[2], dt});
+        drift(dt, part, valley, pos, force);
+        rows.insert(rows.end(), {part.k[0], part.k[1], part.k[2], part.energy,
+                                 pos[0], pos[1], pos[2]});
+        // velocity at the new state
+        auto vel = valley->getVelocity(part.k, part.energy, s);
+        rows.insert(rows.end(), {vel[0], vel[1], vel[2], valley->getNormWaveVec(part.energy)});
+      }
+    }
+  }
+  sec(nm(m.name, "drift").c_str(), rows);
+}
+
+// final-state KATs: each mechanism's scatterParticle on many states.
+// Rows: mech, valley, sub, k0,k1,k2, E, seed | draws, valley', sub', k', E'
+static void dumpFinalStates(const Model &m, const Device &device, Electron &type) {
+  auto mechs = makeMechs(m, device);
+  std::vector<double> rows;
+  emcRNG rng(777);
+  std::uniform_real_distribution<T> u(0., 1.);
+  for (SizeType j = 0; j < mechs.size(); j++) {
+    auto &mech = mechs[j];
+    mech->setPtrValley(type.valleys);
+    SizeType v = mech->getIdxValley();
+    auto valley = type.getValley(v);
+    // lowest energy with a non-zero rate: keeps emission final states > 0
+    double eMin = 0;
+    for (int i = 1; i <= 20000; i++) {
+      double e = m.maxE * i / 20000.0;
+      if (mech->getScatterRate(e, 0) > 0) {
+        eMin = e;
+        break;
+      }
+    }
+    for (int t = 0; t < 48; t++) {
+      emcParticle<T> part;
+      part.valley = v;
+      part.subValley = t % valley->getDegeneracyFactor();
+      double e = eMin + (m.maxE - eMin) * u(rng);
+      part.energy = e;
+      double dir[3] = {u(rng) - 0.5, u(rng) - 0.5, u(rng) - 0.5};
+      double kn = valley->getNormWaveVec(e);
+      double dn = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+      part.k = {kn * dir[0] / dn, kn * dir[1] / dn, kn * dir[2] / dn};
+      uint64_t sseed = splitmix64(1000 + (uint64_t)t + 7919 * j);
+      emcRNG r(sseed);
+      emcRNG before = r;
+      rows.insert(rows.end(), {(double)j, (double)v, (double)part.subValley, part.k[0],
+                               part.k[1], part.k[2], part.energy});
+      mech->scatterParticle(part, r);
+      rows.insert(rows.end(), {(double)RngCounter::between(before, r), (double)part.valley,
+                               (double)part.subValley, part.k[0], part.k[1], part.k[2],
+                               part.energy});
+    }
+  }
+  sec(nm(m.name, "final").c_str(), rows);
+  // seeds are reproducible: splitmix64(1000 + t + 7919 * j)
+}
+
+// the real basicBulkParticleHandler with one shared stream
+static void dumpHandler(const Model &m, uint64_t seed, int nsteps, double dt,
+                        double doping, double spacing, int nodes) {
+  double mp = spacing * (nodes - 1);
+  Device device{silicon(), {mp, mp, mp}, {spacing, spacing, spacing}, m.T};
+  device.addConstantDopingRegion({0, 0, 0}, {mp, mp, mp}, doping);
+  TypeMap types;
+  types[0] = makeType(m, device);
+  Handler handler(device, types, m.fieldDir, m.fieldKVcm * 1e5, seed);
+  handler.generateInitialParticles();
+  std::vector<double> obs;
+  auto push = [&]() {
+    auto e = handler.getAvgEnergy(0);
+    auto v = handler.getAvgDriftVelocity(0);
+    auto o = handler.getValleyOccupationProbability(0);
+    obs.insert(obs.end(), e.begin(), e.end());
+    obs.insert(obs.end(), v.begin(), v.end());
+    obs.insert(obs.end(), o.begin(), o.end());
+  };
+  push();
+  for (int s = 0; s < nsteps; s++) {
+    handler.moveParticles(dt);
+    push();
+  }
+  std::string b = m.name + ".handler";
+  seci((b + ".meta").c_str(), {(long long)handler.getNrParticles(0), (long long)nsteps,
+                                (long long)seed, (long long)nodes});
+  sec((b + ".geom").c_str(), {dt, doping, spacing, m.T});
+  sec((b + ".obs").c_str(), obs);
+}
+
+int main(int argc, char **argv) {
+  if (argc < 2) {
+    std::fprintf(stderr, "usage: ref_harness <outfile>\n");
+    return 2;
+  }
+  out = std::fopen(argv[1], "w");
+  if (!out)
+    return 1;
+
+  // std::mt19937_64 known answer of [rand.predef]
+  {
+    std::mt19937_64 g;
+    g.discard(9999);
+    secu("mt64.kat10000", {g()});
+    emcRNG r(20261202ULL);
+    std::vector<uint64_t> first;
+    for (int i = 0; i < 8; i++)
+      first.push_back(r());
+    secu("mt64.seed20261202", first);
+    std::uniform_real_distribution<T> d(1e-6, 1.), d01(0., 1.);
+    emcRNG r2(99);
+    std::vector<double> us;
+    for (int i = 0; i < 8; i++)
+      us.push_back(i % 2 ? d(r2) : d01(r2));
+    sec("mt64.uniform99", us);
+  }
+
+  std::vector<Model> models = {
+      {"si_jacoboni", 2000, 2.0, 300., 50., {1, 0, 0}},
+      {"si_vasileska", 1000, 1.0, 300., 10., {-1, 0, 0}},
+      {"polar_gamma", 2000, 2.0, 300., 20., {0, 0, 1}},
+      {"para_iso", 1000, 1.5, 77., 5., {1, 1, 0}},
+      {"two_valley", 1500, 1.5, 300., 30., {1, 1, 1}},
+  };
+  const uint64_t seed = 20261202ULL;
+  for (const auto &m : models) {
+    double mp = 4e-8;
+    Device device{silicon(), {mp, mp, mp}, {1e-8, 1e-8, 1e-8}, m.T};
+    device.addConstantDopingRegion({0, 0, 0}, {mp, mp, mp}, 1e23);
+    auto type = makeType(m, device);
+    type->initScatterTables();
+    dumpRates(m, device, *type);
+    dumpDrift(m, *type);
+    dumpFinalStates(m, device, *type);
+    dumpTrajectories(m, device, *type, seed, 64, 300, 1e-15);
+    dumpHandler(m, seed + 1, 120, 1e-15, 5e24, 1e-8, 4);
+  }
+  std::fclose(out);
+  return 0;
+}

@@ -1,0 +1,139 @@
+"""Model descriptions shared by the golden harness, the oracle and the GPU path.
+
+Each model mirrors one configuration built by ``oracle/ref_harness.cpp`` out of
+the reference's own valley and mechanism classes.  The same dictionaries are
+lowered to the oracle's handle API (``oracle/oracle_py.py``) and to the C-ABI
+descriptors of the HIP library (``viennaemc_amd.emcgpu``), so every parity test
+compares the three on identical physics.
+
+Units: masses relative to m0, energies in eV, deformation potentials in eV
+(acoustic) / eV per m (zero-order) / eV (first-order), temperatures in K,
+fields in kV/cm.
+"""
+
+KB = 1.38066e-23  # include/emcConstants.hpp:15
+Q = 1.60219e-19   # include/emcConstants.hpp:12
+ME = 9.11e-31     # include/emcConstants.hpp:24
+
+PARA_ISO, NONPARA_ISO, PARA_ANISO, NONPARA_ANISO = 0, 1, 2, 3
+ACOUSTIC, ZERO_IV_AB, ZERO_IV_EM, FIRST_IV_AB, FIRST_IV_EM, FROEHLICH_AB, FROEHLICH_EM = range(7)
+
+SI_RHO = 2329.0   # examples/SiliconFunctions.hpp:22
+SI_VSOUND = 9040.0  # examples/SiliconFunctions.hpp:24
+
+# X-valley frames of examples/SiliconFunctions.hpp:66-71
+SI_AXES = [
+    [(1, 0, 0), (0, 1, 0), (0, 0, 1)],
+    [(0, 1, 0), (1, 0, 0), (0, 0, 1)],
+    [(0, 0, 1), (0, 1, 0), (1, 0, 0)],
+]
+# [111]-type frames (rows need normalising; exercises setSubValleyEllipseCoordSystem)
+AXES_111 = [
+    [(1, 1, 1), (1, -1, 0), (1, 1, -2)],
+    [(-1, 1, 1), (1, 1, 0), (-1, 1, -2)],
+    [(1, -1, 1), (1, 1, 0), (1, -1, -2)],
+    [(1, 1, -1), (1, -1, 0), (1, 1, 2)],
+]
+FINAL_G = [[0], [1], [2]]                            # SiliconFunctions.hpp:48-49
+FINAL_F = [[1, 1, 2, 2], [0, 0, 2, 2], [0, 0, 1, 1]]  # SiliconFunctions.hpp:50-51
+
+# Jacoboni & Reggiani, Rev. Mod. Phys. 55, 645 (1983): zero-order Si phonons
+# (Theta [K], D [eV/cm]), g-type then f-type.
+JAC_G = [(140, 0.5e8), (215, 0.8e8), (720, 11e8)]
+JAC_F = [(220, 0.3e8), (550, 2e8), (685, 2e8)]
+
+
+def theta_to_ev(theta):
+    """kB * Theta / q, evaluated in the same order as the harness."""
+    return KB * theta / Q
+
+
+def _si_jacoboni_mechs():
+    mechs = [dict(kind=ACOUSTIC, valley=0, sigma=9.0)]
+    for theta, d in JAC_G:
+        for kind in (ZERO_IV_AB, ZERO_IV_EM):
+            mechs.append(dict(kind=kind, valley=0, final=0, defpot=d * 100.0,
+                              hw=theta_to_ev(theta), fmap=FINAL_G))
+    for theta, d in JAC_F:
+        for kind in (ZERO_IV_AB, ZERO_IV_EM):
+            mechs.append(dict(kind=kind, valley=0, final=0, defpot=d * 100.0,
+                              hw=theta_to_ev(theta), fmap=FINAL_F))
+    return mechs
+
+
+def _si_vasileska_mechs():
+    m = [dict(kind=ACOUSTIC, valley=0, sigma=9.0)]
+    for kind in (ZERO_IV_AB, ZERO_IV_EM):
+        m.append(dict(kind=kind, valley=0, final=0, defpot=5.23e10, hw=0.06, fmap=FINAL_F))
+    for kind in (ZERO_IV_AB, ZERO_IV_EM):
+        m.append(dict(kind=kind, valley=0, final=0, defpot=5.23e10, hw=0.06, fmap=FINAL_G))
+    for kind in (FIRST_IV_AB, FIRST_IV_EM):
+        m.append(dict(kind=kind, valley=0, final=0, defpot=2.5, hw=0.023, fmap=FINAL_F))
+    for kind in (FIRST_IV_AB, FIRST_IV_EM):
+        m.append(dict(kind=kind, valley=0, final=0, defpot=4.0, hw=0.018, fmap=FINAL_G))
+    return m
+
+
+def _two_valley_mechs():
+    l_to_l = [[1, 2, 3], [0, 2, 3], [0, 1, 3], [0, 1, 2]]
+    l_to_g = [[0], [0], [0], [0]]
+    g_to_l = [[0, 1, 2, 3]]
+    return [
+        dict(kind=ACOUSTIC, valley=0, sigma=7.0),
+        dict(kind=ZERO_IV_AB, valley=0, final=0, defpot=3e10, hw=0.029, fmap=l_to_l),
+        dict(kind=ZERO_IV_EM, valley=0, final=0, defpot=3e10, hw=0.029, fmap=l_to_l),
+        dict(kind=ZERO_IV_AB, valley=0, final=1, defpot=1e11, hw=0.0278, fmap=l_to_g),
+        dict(kind=ZERO_IV_EM, valley=0, final=1, defpot=1e11, hw=0.0278, fmap=l_to_g),
+        dict(kind=ACOUSTIC, valley=1, sigma=7.0),
+        dict(kind=FIRST_IV_AB, valley=1, final=0, defpot=3.0, hw=0.0278, fmap=g_to_l),
+        dict(kind=FIRST_IV_EM, valley=1, final=0, defpot=3.0, hw=0.0278, fmap=g_to_l),
+    ]
+
+
+MODELS = {
+    # config 1/2 physics: Si X valleys, Jacoboni zero-order phonons, m_t = 0.19
+    "si_jacoboni": dict(
+        n_levels=2000, max_energy=2.0, T=300.0, field_kvcm=50.0, field_dir=(1, 0, 0),
+        valleys=[dict(kind=NONPARA_ANISO, mass=(0.916, 0.19, 0.19), deg=3, alpha=0.5,
+                      bottom=0.0, axes=SI_AXES)],
+        mechs=_si_jacoboni_mechs()),
+    # the reference's own bulk example (examples/bulkSimulation + SiliconFunctions.hpp)
+    "si_vasileska": dict(
+        n_levels=1000, max_energy=1.0, T=300.0, field_kvcm=10.0, field_dir=(-1, 0, 0),
+        valleys=[dict(kind=NONPARA_ANISO, mass=(0.916, 0.196, 0.196), deg=3, alpha=0.5,
+                      bottom=0.0, axes=SI_AXES)],
+        mechs=_si_vasileska_mechs()),
+    # config 5 physics (equilibrium phonons): one Kane Gamma valley, Froehlich only
+    "polar_gamma": dict(
+        n_levels=2000, max_energy=2.0, T=300.0, field_kvcm=20.0, field_dir=(0, 0, 1),
+        valleys=[dict(kind=NONPARA_ISO, mass=(0.28, 0.28, 0.28), deg=1, alpha=0.3, bottom=0.0)],
+        mechs=[dict(kind=FROEHLICH_AB, valley=0, hw=0.06, mrel=0.28, eps_hi=3.5, eps_lo=10.0),
+               dict(kind=FROEHLICH_EM, valley=0, hw=0.06, mrel=0.28, eps_hi=3.5, eps_lo=10.0)]),
+    # config 4 physics: one parabolic isotropic valley, m* = 0.26
+    "para_iso": dict(
+        n_levels=1000, max_energy=1.5, T=77.0, field_kvcm=5.0, field_dir=(1, 1, 0),
+        valleys=[dict(kind=PARA_ISO, mass=(0.26, 0.26, 0.26), deg=1, alpha=0.0, bottom=0.0)],
+        mechs=[dict(kind=ACOUSTIC, valley=0, sigma=9.0),
+               dict(kind=ZERO_IV_AB, valley=0, final=0, defpot=5e10, hw=0.05, fmap=[[0]]),
+               dict(kind=ZERO_IV_EM, valley=0, final=0, defpot=5e10, hw=0.05, fmap=[[0]])]),
+    # two valleys: parabolic anisotropic [111] valley + raised Kane valley
+    "two_valley": dict(
+        n_levels=1500, max_energy=1.5, T=300.0, field_kvcm=30.0, field_dir=(1, 1, 1),
+        valleys=[dict(kind=PARA_ANISO, mass=(1.6, 0.12, 0.12), deg=4, alpha=0.0, bottom=0.0,
+                      axes=AXES_111),
+                 dict(kind=NONPARA_ISO, mass=(0.067, 0.067, 0.067), deg=1, alpha=0.6,
+                      bottom=0.3)],
+        mechs=_two_valley_mechs()),
+}
+
+MODEL_NAMES = list(MODELS)
+
+
+def field_force(model):
+    """Force vector of the bulk handler: scale(normalize(dir) * E, -q)."""
+    import math
+    d = list(model["field_dir"])
+    n = math.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2])
+    d = [x / n for x in d]
+    e = model["field_kvcm"] * 1e5
+    return [x * e * -Q for x in d], d
