@@ -1,0 +1,145 @@
+#!/usr/bin/env python3
+"""Benchmark: particle-updates/s of the EMC particle step on one MI355X.
+
+Workload (Si bulk, plain form): 10M electrons in the X valleys of Si
+(m_l = 0.916, m_t = 0.19, alpha = 0.5/eV, three subvalley pairs), 300 K,
+acoustic + six zero-order Jacoboni intervalley phonons (13 tables of 2000
+levels on (0, 2] eV), 50 kV/cm along [100], dt = 1 fs.  A "step" advances
+all particles by one dt including every scatter event inside it and the
+per-step ensemble sums (drift velocity, mean energy, subvalley occupancy).
+
+value      = FP32 (batched kernel, fused steps) particle-updates/s with the
+             state resident in HBM; the FP64 plain kernel (one step per
+             launch) is measured beside it.
+roofline   = the FP32 step kernel: algorithmic bytes per launch / average
+             launch time (HIP events on the library's stream).
+cpu_baseline = the oracle's FP64 restatement of the reference loop with the
+             reference engine (one mt19937_64 per particle), all host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "tests")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_FP32 = 48  # k(3) + E + tau + valley|sub, 4 B each, read + write
+BYTES_FP64 = 88  # k(3) + E + tau 8 B each + valley|sub 4 B, read + write
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--particles", type=int, default=10_000_000)
+    ap.add_argument("--spl", type=int, default=0, help="FP32 steps fused per launch (0 = auto)")
+    ap.add_argument("--fp64-steps", type=int, default=0, help="FP64 timed steps (0 = --steps)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--cpu-particles", type=int, default=1_000_000)
+    ap.add_argument("--cpu-steps", type=int, default=300)
+    return ap.parse_args()
+
+
+def run_gpu(emcgpu, spec, precision, n, warmup, steps, spl, seed=20261202):
+    ctx = emcgpu.Context(precision=precision)
+    ctx.setup(0, spec)
+    ctx.resize([n])
+    ctx.init_particles(0, seed, spec["T"])
+    if warmup:
+        ctx.step(1e-15, warmup, seed=seed, step0=0, steps_per_launch=spl)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(1e-15, steps, seed=seed, step0=warmup, steps_per_launch=spl)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    av = ctx.averages(0)
+    # timed pass for the kernel duration (HIP events around every launch)
+    nt = max(spl, 1) * 4 if spl else 64
+    ctx.timing(reset=True)
+    ctx.step(1e-15, nt, seed=seed, step0=warmup + steps, steps_per_launch=spl, timing=True)
+    ms, launches = ctx.timing()
+    ctx.close()
+    return dict(wall=wall, av=av, kernel_ms=ms / max(launches, 1), launches=launches,
+                steps_per_launch=(nt / max(launches, 1)))
+
+
+def cpu_baseline(spec, n, steps):
+    from oracle import oracle_py as O
+    from emc_models import field_force
+    M = O.OracleModel(spec)
+    force, d = field_force(spec)
+    threads = min(os.cpu_count() or 1, 16)
+    t = np.zeros(1)
+    M.run_mt(n, 20261202, O.thermal_voltage(spec["T"]), steps, 1e-15, force, d, obs=False,
+             threads=threads, timing=t)
+    return dict(value=n * steps / t[0], unit="particle-updates/s", cores=threads, kind="port",
+                sample="%d particles x %d steps, FP64, mt19937_64 per particle (step loop only, %.1f s)"
+                       % (n, steps, t[0]))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the particle step has no multi-GPU path in this build "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        return 2
+    from emc_models import MODELS
+    from viennaemc_amd import emcgpu
+    spec = MODELS["si_jacoboni"]
+    n = args.particles
+
+    r32 = run_gpu(emcgpu, spec, emcgpu.FP32, n, args.warmup, args.steps, args.spl)
+    ups32 = n * args.steps / r32["wall"]
+    out = {}
+    if not args.no_fp64:
+        s64 = args.fp64_steps or args.steps
+        r64 = run_gpu(emcgpu, spec, emcgpu.FP64, n, args.warmup, s64, 1)
+        ups64 = n * s64 / r64["wall"]
+        tail = max(1, min(args.steps, s64) // 2)
+        v32, v64 = r32["av"]["avgV"][-tail:, 0].mean(), r64["av"]["avgV"][-tail:, 0].mean()
+        e32, e64 = r32["av"]["avgE"][-tail:, 0].mean(), r64["av"]["avgE"][-tail:, 0].mean()
+        out.update(fp64_plain_updates_per_s=ups64, fp32_over_fp64=ups32 / ups64,
+                   fp64_kernel_ms=r64["kernel_ms"],
+                   fp64_roofline_frac=(BYTES_FP64 * n / (r64["kernel_ms"] * 1e-3)) / 1e9 / HBM_PEAK_GBS,
+                   delta_vd_rel=(v32 - v64) / v64, delta_E_rel=(e32 - e64) / e64,
+                   vd_fp32=v32, vd_fp64=v64, meanE_fp32=e32, meanE_fp64=e64)
+    spl = r32["steps_per_launch"]
+    # algorithmic bytes of one launch: the state is read and written once
+    achieved = BYTES_FP32 * n / (r32["kernel_ms"] * 1e-3) / 1e9
+    res = {
+        "metric": "particle-updates/s (Si bulk, 10M e-) FP32-batched vs FP64 plain; % HBM peak; dv_d",
+        "value": ups32,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["wall"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (thermal Maxwellian init, Philox-4x32-10)",
+        "config": {"workload": "Si bulk X-valley EMC, 10M e-, 50 kV/cm [100], dt 1 fs, 300 K, "
+                               "acoustic + 6 zero-order Jacoboni phonons, 2000-level tables",
+                   "particles": n, "steps_per_launch": spl},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+    }
+    res.update(out)
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(spec, args.cpu_particles, args.cpu_steps)
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

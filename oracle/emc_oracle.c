@@ -1375,11 +1375,19 @@ static void *orc_mt_worker(void *arg) {
   return NULL;
 }
 
+#include <time.h>
+
+static double orc_now(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
 void orc_run_mt(const orc_model *m, int np, uint64_t seed, double Vt,
                 int nsteps, double tStep, const double force[3],
                 const double fieldDir[3], double *obs, int nThreads,
                 double *kOut, double *EOut, int32_t *valleyOut,
-                int32_t *subOut) {
+                int32_t *subOut, double *stepSeconds) {
   orc_particle *parts = (orc_particle *)malloc(sizeof(orc_particle) * (size_t)np);
   orc_rng *rngs = (orc_rng *)malloc(sizeof(orc_rng) * (size_t)np);
   for (int i = 0; i < np; i++) {
@@ -1394,6 +1402,7 @@ void orc_run_mt(const orc_model *m, int np, uint64_t seed, double Vt,
   /* observables need every step: run the threads one step at a time when
    * obs is requested, else let each thread run all its steps */
   int chunkSteps = obs ? 1 : nsteps;
+  double t0 = orc_now();
   for (int s0 = 0; s0 < nsteps; s0 += chunkSteps) {
     pthread_t th[256];
     orc_mt_job jobs[256];
@@ -1417,6 +1426,8 @@ void orc_run_mt(const orc_model *m, int np, uint64_t seed, double Vt,
                       obs + (size_t)s0 * 3 * nv + 2 * nv);
     }
   }
+  if (stepSeconds)
+    *stepSeconds = orc_now() - t0;
   for (int i = 0; i < np; i++) {
     if (kOut)
       orc_unpack(&parts[i], kOut + 3 * i, EOut + i, valleyOut + i, subOut + i);

@@ -321,12 +321,13 @@ void orc_init_particles_philox(const orc_model *m, int np, double Vt,
                                int32_t *sub);
 /* mt19937_64 ensemble (reference RNG, one stream per particle) with
  * per-step observables; the statistical baseline for production mode.
- * nThreads > 1 runs particles in parallel (pthreads). */
+ * nThreads > 1 runs particles in parallel (pthreads); the wall time of
+ * the step loop alone (no initialisation) goes to *stepSeconds. */
 void orc_run_mt(const orc_model *m, int np, uint64_t seed, double Vt,
                 int nsteps, double tStep, const double force[3],
                 const double fieldDir[3], double *obs, int nThreads,
                 double *kOut, double *EOut, int32_t *valleyOut,
-                int32_t *subOut);
+                int32_t *subOut, double *stepSeconds);
 
 #ifdef __cplusplus
 }

@@ -67,7 +67,7 @@ def load():
                                   C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
         "orc_init_particles_philox": (None, [vp, _I, _D, C.c_uint64, C.c_uint32, _PD, _PD, _PD, _PI32, _PI32]),
         "orc_run_mt": (None, [vp, _I, C.c_uint64, _D, _I, _D, _PD, _PD, C.c_void_p, _I,
-                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
         "orc_mt64_fill": (None, [C.c_uint64, _PU64, C.c_size_t]),
         "orc_mt64_fill_particles": (None, [C.c_uint64, C.c_uint64, C.c_size_t, _PU64, C.c_size_t, C.c_size_t]),
         "orc_particle_seed": (C.c_uint64, [C.c_uint64, C.c_uint64]),
@@ -217,7 +217,8 @@ class OracleModel:
         self.L.orc_init_particles_philox(self.h, np_, Vt, seed, instance, k, E, tau, v, s)
         return dict(k=k, E=E, tau=tau, valley=v, sub=s)
 
-    def run_mt(self, np_, seed, Vt, nsteps, dt, force, field_dir, obs=True, threads=1, final=False):
+    def run_mt(self, np_, seed, Vt, nsteps, dt, force, field_dir, obs=True, threads=1, final=False,
+               timing=None):
         nv = self.n_valleys
         o = np.zeros(nsteps * 3 * nv) if obs else None
         fin = None
@@ -227,7 +228,7 @@ class OracleModel:
         self.L.orc_run_mt(self.h, np_, seed, Vt, nsteps, dt, np.array(force, dtype=np.float64),
                           np.array(field_dir, dtype=np.float64), _ptr(o), threads,
                           _ptr(fin and fin["k"]), _ptr(fin and fin["E"]), _ptr(fin and fin["valley"]),
-                          _ptr(fin and fin["sub"]))
+                          _ptr(fin and fin["sub"]), _ptr(timing))
         return (None if o is None else o.reshape(nsteps, 3, nv)), fin
 
     def run_shared(self, seed, max_pos, spacing, doping, T, field_dir, field_strength, dt, nsteps,

@@ -1,0 +1,133 @@
+"""C-ABI library checks that need no GPU.
+
+libemcgpu.so loads, exports every entry point declared in include/emcgpu.h,
+and its host-side table construction (host-only context, device = -1)
+reproduces the reference's FP64 tables; invalid models are rejected with
+the reference's error conditions.
+"""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from emc_models import MODELS
+from viennaemc_amd import emcgpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for hdr in os.listdir(os.path.join(ROOT, "include")):
+        if hdr.endswith(".h"):
+            src = open(os.path.join(ROOT, "include", hdr)).read()
+            names |= set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(emcgpu_\w+)\s*\(", src, re.M))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = emcgpu.LIB_PATH
+    assert os.path.exists(lib), "build libemcgpu.so first"
+    out = subprocess.check_output(["nm", "-D", "--defined-only", lib]).decode()
+    exported = set(re.findall(r" T (emcgpu_\w+)", out))
+    decl = declared_symbols()
+    assert len(decl) >= 18
+    assert decl <= exported, decl - exported
+    L = emcgpu.load()
+    for name in decl:
+        assert hasattr(L, name)
+    assert set(emcgpu.EXPORTS) <= decl
+
+
+def test_struct_layouts_match_header():
+    # sizes computed by the C compiler for the descriptors of include/emcgpu.h
+    assert C.sizeof(emcgpu.ValleyDesc) == 4 + 4 + 24 + 8 + 4 + 4 + 8 + 8 + 8 * 9 * 8
+    assert C.sizeof(emcgpu.MechDesc) == 3 * 4 + 4 + 5 * 8 + 4 + 8 * 8 * 4 + 4
+    assert C.sizeof(emcgpu.StepArgs) == 8 + 4 * 3 + 4 + 8 + 4 + 4 + 8 * 9
+
+
+def host_ctx(n_instances=1):
+    return emcgpu.Context(precision=emcgpu.FP64, n_instances=n_instances, device=-1)
+
+
+@pytest.mark.parametrize("name", list(MODELS))
+def test_host_tables_match_reference(name, golden, oracle_models):
+    spec = MODELS[name]
+    ctx = host_ctx()
+    ctx.set_model(0, spec)
+    ctx.build_tables(0, spec["n_levels"], spec["max_energy"], spec["T"])
+    M = oracle_models[name]
+    for v in range(len(spec["valleys"])):
+        cum, tau0 = ctx.get_tables(0, v, spec["n_levels"])
+        # tau0 is the reference's getTau; the tables are the reference's
+        # normalised cumulative tables (bit-exact FP64)
+        assert tau0 == golden[name + ".tau0"][v]
+        assert (cum == M.cum(v)).all()
+
+
+def test_user_rate_tables_match_builtin(golden):
+    """emcgpu_set_rate_tables with the reference's raw rates == built-in tables."""
+    name = "si_jacoboni"
+    spec = MODELS[name]
+    nm = len(spec["mechs"])
+    raw = golden[name + ".rate_table"].reshape(nm, -1)
+    a = host_ctx()
+    a.set_model(0, spec)
+    a.set_rate_tables(0, spec["max_energy"], raw)
+    b = host_ctx()
+    b.set_model(0, spec)
+    b.build_tables(0, spec["n_levels"], spec["max_energy"], spec["T"])
+    ca, ta = a.get_tables(0, 0, spec["n_levels"])
+    cb, tb = b.get_tables(0, 0, spec["n_levels"])
+    assert ta == tb and (ca == cb).all()
+
+
+def _bad(spec_mut):
+    import copy
+    spec = copy.deepcopy(MODELS["si_jacoboni"])
+    spec_mut(spec)
+    ctx = host_ctx()
+    with pytest.raises(emcgpu.EmcGpuError) as e:
+        ctx.set_model(0, spec)
+    return str(e.value)
+
+
+def test_rejects_non_orthogonal_axes():
+    def m(s):
+        s["valleys"][0]["axes"][1] = [(1, 1, 0), (1, 0, 0), (0, 0, 1)]
+    assert "not orthogonal" in _bad(m)
+
+
+def test_rejects_bad_final_subvalley():
+    def m(s):
+        s["mechs"][1]["fmap"] = [[3], [1], [2]]
+    assert "not valid" in _bad(m)
+
+
+def test_rejects_negative_nonparabolicity():
+    def m(s):
+        s["valleys"][0]["alpha"] = -0.1
+    assert "NonParabolicity" in _bad(m)
+
+
+def test_rejects_bad_valley_index():
+    def m(s):
+        s["mechs"][0]["valley"] = 3
+    assert "invalid" in _bad(m)
+
+
+def test_host_context_refuses_device_work():
+    ctx = host_ctx()
+    with pytest.raises(emcgpu.EmcGpuError) as e:
+        ctx.resize([10])
+    assert e.value.status == 3
+
+
+def test_tables_before_model_is_a_state_error():
+    ctx = host_ctx()
+    with pytest.raises(emcgpu.EmcGpuError) as e:
+        ctx.build_tables(0, 100, 1.0, 300.0)
+    assert e.value.status == 3

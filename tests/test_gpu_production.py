@@ -1,0 +1,179 @@
+"""Production-mode parity on the GPU.
+
+* FP64 kernel vs the oracle's CPU twin on the same Philox-4x32-10 stream:
+  identical discrete history, k and E within 1e-12 relative.
+* GPU-side ensemble reductions vs the same sums taken on the CPU.
+* FP32 and FP64 kernels vs the reference-RNG (mt19937_64) CPU ensemble:
+  drift velocity and mean energy within 4 sigma of the combined statistical
+  error (3 sigma per-step bars widened for the number of compared steps).
+* FP32 vs FP64 GPU runs: observables within 0.5 % or 4 sigma.
+"""
+import numpy as np
+import pytest
+
+from emc_models import MODELS, field_force
+from oracle import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-12
+
+
+def _emc():
+    from viennaemc_amd import emcgpu
+    return emcgpu
+
+
+def _ctx(name, n, precision, n_instances=1):
+    emcgpu = _emc()
+    ctx = emcgpu.Context(precision=precision, n_instances=n_instances)
+    for i in range(n_instances):
+        ctx.setup(i, MODELS[name])
+    ctx.resize([n] * n_instances)
+    return ctx
+
+
+def _rel_k(a, b):
+    return np.linalg.norm(a - b, axis=-1) / np.maximum(np.linalg.norm(b, axis=-1), 1e-300)
+
+
+@pytest.mark.parametrize("name", ["si_jacoboni", "polar_gamma", "two_valley", "para_iso"])
+def test_fp64_production_matches_oracle_twin(name):
+    emcgpu = _emc()
+    spec = MODELS[name]
+    n, ns, seed = 4096, 100, 0x5EED1234ABCD
+    print("fp64 production vs oracle twin:", name, flush=True)
+    M = O.OracleModel(spec)
+    Vt = O.thermal_voltage(spec["T"])
+    ref = M.init_philox(n, Vt, seed)
+    ctx = _ctx(name, n, emcgpu.FP64)
+    ctx.init_particles(0, seed, spec["T"])
+    g0 = ctx.get_particles(0)
+    assert (g0["valley"] == ref["valley"]).all() and (g0["sub"] == ref["sub"]).all()
+    assert _rel_k(g0["k"].reshape(-1, 3), ref["k"].reshape(-1, 3)).max() < REL_TOL
+    assert (np.abs(g0["E"] - ref["E"]) / ref["E"]).max() < REL_TOL
+    assert (np.abs(g0["tau"] - ref["tau"]) / ref["tau"]).max() < REL_TOL
+    # run both from the oracle's state so the comparison isolates the step
+    ctx.set_particles(0, ref["k"], ref["E"], ref["tau"], ref["valley"], ref["sub"])
+    force, d = field_force(spec)
+    ctx.step(1e-15, ns, seed=seed, step0=7, steps_per_launch=32)
+    obs_ref = M.run_philox(ref, ns, 1e-15, force, d, seed, instance=0, step0=7)
+    g = ctx.get_particles(0)
+    assert (g["valley"] == ref["valley"]).all()
+    assert (g["sub"] == ref["sub"]).all()
+    assert _rel_k(g["k"].reshape(-1, 3), ref["k"].reshape(-1, 3)).max() < REL_TOL
+    assert (np.abs(g["E"] - ref["E"]) / ref["E"]).max() < REL_TOL
+    av = ctx.averages(0)
+    nv = len(spec["valleys"])
+    # getAvgEnergy / getAvgDriftVelocity / occupancy per step, GPU Kahan sums
+    # vs the oracle's serial FP64 sums over identical particles
+    assert np.allclose(av["avgE"], obs_ref[:, 0, :nv], rtol=1e-11, atol=0)
+    assert np.allclose(av["avgV"], obs_ref[:, 1, :nv], rtol=1e-9, atol=1e-6)
+    assert (av["occ"] == obs_ref[:, 2, :nv]).all()
+
+
+def test_reductions_match_cpu_sums():
+    """Per-step sums of the last step equal the sums of the returned state."""
+    emcgpu = _emc()
+    name = "two_valley"
+    spec = MODELS[name]
+    n = 100_003  # ragged: not a multiple of the tile
+    ctx = _ctx(name, n, emcgpu.FP64)
+    ctx.init_particles(0, 99, spec["T"])
+    ctx.step(1e-15, 40, seed=99)
+    g = ctx.get_particles(0)
+    av = ctx.averages(0)
+    M = O.OracleModel(spec)
+    _, d = field_force(spec)
+    vdir = np.array([M.velocity(int(v), int(s), k, e) @ np.array(d) for v, s, k, e in
+                     zip(g["valley"], g["sub"], g["k"].reshape(-1, 3), g["E"])])
+    for v in range(len(spec["valleys"])):
+        m = g["valley"] == v
+        assert av["count"][-1, v] == m.sum()
+        assert np.isclose(av["sumE"][-1, v], g["E"][m].sum(), rtol=1e-13)
+        assert np.isclose(av["sumV"][-1, v], vdir[m].sum(), rtol=1e-9, atol=1e-3)
+        for s in range(spec["valleys"][v]["deg"]):
+            assert av["sub_occ"][-1, v, s] * n == pytest.approx(((g["valley"] == v) & (g["sub"] == s)).sum())
+
+
+def _stat_compare(a_mean, a_sig, b_mean, b_sig, nsig=4.0):
+    return np.abs(a_mean - b_mean) <= nsig * np.sqrt(a_sig ** 2 + b_sig ** 2)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_ensemble_matches_reference_rng(precision):
+    """Si, 50 kV/cm: GPU (Philox) vs CPU ensemble on the reference engine."""
+    emcgpu = _emc()
+    name = "si_jacoboni"
+    spec = MODELS[name]
+    nsteps, dt = 800, 1e-15
+    n_gpu, n_cpu = 400_000, 20_000
+    print("ensemble vs reference rng, precision", precision, flush=True)
+    M = O.OracleModel(spec)
+    force, d = field_force(spec)
+    obs_cpu, fin = M.run_mt(n_cpu, 20261202, O.thermal_voltage(spec["T"]), nsteps, dt, force, d,
+                            threads=8, final=True)
+    vel = np.array([M.velocity(int(v), int(s), k, e) @ np.array(d) for v, s, k, e in
+                    zip(fin["valley"], fin["sub"], fin["k"].reshape(-1, 3), fin["E"])])
+    sig_v, sig_e = vel.std(), fin["E"].std()
+    ctx = _ctx(name, n_gpu, precision)
+    ctx.init_particles(0, 4242, spec["T"])
+    ctx.step(dt, nsteps, seed=4242)
+    av = ctx.averages(0)
+    for s in (99, 199, 399, 599, 799):
+        assert _stat_compare(av["avgV"][s, 0], sig_v / np.sqrt(n_gpu),
+                             obs_cpu[s, 1, 0], sig_v / np.sqrt(n_cpu)), (s, av["avgV"][s, 0], obs_cpu[s, 1, 0])
+        assert _stat_compare(av["avgE"][s, 0], sig_e / np.sqrt(n_gpu),
+                             obs_cpu[s, 0, 0], sig_e / np.sqrt(n_cpu)), (s, av["avgE"][s, 0], obs_cpu[s, 0, 0])
+    # the three X subvalley pairs share the occupation equally within 4 sigma
+    occ = av["sub_occ"][-1, 0, :3]
+    assert np.all(np.abs(occ - occ.mean()) < 4 * np.sqrt(occ * (1 - occ) / n_gpu) + 1e-12)
+
+
+def test_fp32_matches_fp64():
+    """FP32 kinematics vs the FP64 kernel on the same seed: v_d and <E>
+    averaged over the last 200 steps agree within 0.5 %."""
+    emcgpu = _emc()
+    name = "si_jacoboni"
+    spec = MODELS[name]
+    n, nsteps = 1_000_000, 600
+    out = {}
+    for p in (emcgpu.FP64, emcgpu.FP32):
+        ctx = _ctx(name, n, p)
+        ctx.init_particles(0, 777, spec["T"])
+        ctx.step(1e-15, nsteps, seed=777)
+        av = ctx.averages(0)
+        out[p] = (av["avgV"][-200:, 0].mean(), av["avgE"][-200:, 0].mean())
+    dv = abs(out[1][0] - out[0][0]) / abs(out[0][0])
+    de = abs(out[1][1] - out[0][1]) / abs(out[0][1])
+    print("fp32 vs fp64: dv/v = %.2e, dE/E = %.2e" % (dv, de), flush=True)
+    assert dv < 5e-3 and de < 5e-3
+
+
+def test_batched_instances_are_independent():
+    """Three instances with ragged counts in one arena: each evolves exactly
+    as it does alone (same instance index, seed and counts)."""
+    emcgpu = _emc()
+    name = "si_jacoboni"
+    spec = MODELS[name]
+    counts = [1000, 257, 3001]
+    ctx = emcgpu.Context(precision=emcgpu.FP64, n_instances=3)
+    for i in range(3):
+        ctx.setup(i, spec)
+    ctx.set_field(1, (0, 1, 0), 20e5)
+    ctx.resize(counts)
+    for i in range(3):
+        ctx.init_particles(i, 11, spec["T"])
+    ctx.step(1e-15, 50, seed=11)
+    M = O.OracleModel(spec)
+    for i, n in enumerate(counts):
+        ref = M.init_philox(n, O.thermal_voltage(spec["T"]), 11, instance=i)
+        if i == 1:
+            force = [0.0, 20e5 * -O.Q, 0.0]
+            d = (0, 1, 0)
+        else:
+            force, d = field_force(spec)
+        M.run_philox(ref, 50, 1e-15, force, d, 11, instance=i, obs=False)
+        g = ctx.get_particles(i)
+        assert (g["valley"] == ref["valley"]).all() and (g["sub"] == ref["sub"]).all()
+        assert _rel_k(g["k"].reshape(-1, 3), ref["k"].reshape(-1, 3)).max() < REL_TOL

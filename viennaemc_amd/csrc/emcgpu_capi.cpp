@@ -1,0 +1,1102 @@
+// emcgpu_capi.cpp -- C ABI of the MI355X EMC particle step (host side).
+//
+// Owns the device memory of a context (SoA particle arena, per-instance
+// parameter blocks, scatter tables, observable buffers), builds scatter
+// tables in FP64 on the host exactly as emcScatterHandler does
+// (include/emcScatterHandler.hpp:220-288), lowers valleys and mechanisms to
+// the kernels' parameter blocks in scaled units, and drives the step,
+// reduction and initialisation kernels on the context's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/emcgpu.h"
+#include "../include/emcgpu/emcgpuPhysics.hpp"
+#include "emcgpu_device.h"
+
+namespace emcgpu {
+template <typename R>
+hipError_t launch_steps(const StepArgs<R> &a, int mode, hipStream_t st);
+template <typename R>
+hipError_t launch_init(const StepArgs<R> &a, hipStream_t st);
+hipError_t launch_reduce(const double *partial, int n_steps, int n_tiles, int n_obs,
+                         const int *inst_tile0, const int *inst_ntiles, int n_inst, double *out,
+                         int out_step0, int out_total, hipStream_t st);
+}  // namespace emcgpu
+
+using namespace emcgpu;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct Instance {
+  std::vector<emcgpu_valley_desc> valleys;
+  std::vector<emcgpu_mech_desc> mechs;
+  std::vector<phys::ValleyMass> vm;
+  // normalised tables per valley: [table][level] (FP64 master) and tau0 [s]
+  int n_levels = 0;
+  double max_energy = 0, dE = 0;
+  std::vector<std::vector<double>> cum;     // per valley
+  std::vector<std::vector<int>> table_mech;  // per valley: mechanism ids
+  std::vector<double> tau0;
+  bool has_tables = false;
+  double temperature = 300;
+  double dir[3] = {1, 0, 0};
+  double force[3] = {0, 0, 0};  // N
+  bool has_field = false;
+  size_t count = 0;
+  long long start = 0;
+};
+
+}  // namespace
+
+struct emcgpu_ctx {
+  int device = 0;
+  int precision = EMCGPU_FP64;
+  int n_inst = 1;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::vector<Instance> inst;
+  // arena
+  long long capacity = 0;  // padded particles
+  int n_tiles = 0;
+  void *kx = nullptr, *ky = nullptr, *kz = nullptr, *E = nullptr, *tau = nullptr;
+  uint32_t *vs = nullptr;
+  int *d_tile_inst = nullptr;
+  long long *d_inst_start = nullptr, *d_inst_count = nullptr;
+  int *d_inst_tile0 = nullptr, *d_inst_ntiles = nullptr;
+  // parameters and tables
+  void *d_params = nullptr;
+  void *d_cum = nullptr;
+  size_t cum_bytes = 0;
+  bool params_dirty = true;
+  double params_dt = 0;
+  // observables
+  int max_v = 1, max_deg = 1, n_obs = 0;
+  double *d_partial = nullptr;
+  size_t partial_bytes = 0;
+  double *d_obs = nullptr;
+  size_t obs_bytes = 0;
+  int obs_steps = 0;
+  // timing
+  double ms_total = 0;
+  long long launches = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+emcgpu_status fail(emcgpu_ctx *c, emcgpu_status st, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c)
+    c->err = buf;
+  else
+    g_create_error = buf;
+  return st;
+}
+
+#define HIPCHK(c, x)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess)                                                                \
+      return fail((c), EMCGPU_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_));      \
+  } while (0)
+
+size_t real_size(const emcgpu_ctx *c) {
+  return c->precision == EMCGPU_FP32 ? sizeof(float) : sizeof(double);
+}
+
+void dfree(void *&p) {
+  if (p) {
+    (void)hipFree(p);
+    p = nullptr;
+  }
+}
+template <typename T>
+void dfreet(T *&p) {
+  if (p) {
+    (void)hipFree(p);
+    p = nullptr;
+  }
+}
+
+emcgpu_status check_instance(emcgpu_ctx *c, int instance) {
+  if (!c)
+    return EMCGPU_ERR_ARG;
+  if (instance < 0 || instance >= c->n_inst)
+    return fail(c, EMCGPU_ERR_ARG, "instance %d out of range [0, %d)", instance, c->n_inst);
+  return EMCGPU_OK;
+}
+
+// ---- scaled-unit lowering of an instance -------------------------------
+// k [1/nm] = k [1/m] * 1e-9, t [fs] = t [s] * 1e15
+constexpr double kToNm = 1e-9;
+constexpr double kToFs = 1e15;
+
+void normalize3(double d[3]) {
+  double n = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  if (n == 0.0)
+    return;
+  for (int i = 0; i < 3; i++)
+    d[i] /= n;
+}
+
+template <typename R>
+void lower_instance(const emcgpu_ctx *c, const Instance &in, double dt_s, InstDev<R> &P,
+                    const std::vector<long long> &table_off) {
+  std::memset(&P, 0, sizeof(P));
+  P.n_valleys = (int)in.valleys.size();
+  P.n_levels = in.n_levels;
+  P.dE = (R)in.dE;
+  P.inv_dE = (R)(1.0 / in.dE);
+  P.dt = (R)(dt_s * kToFs);
+  P.vt = (R)(phys::kB / phys::q * in.temperature);
+  for (int v = 0; v < P.n_valleys; v++) {
+    const emcgpu_valley_desc &vd = in.valleys[v];
+    const phys::ValleyMass &m = in.vm[v];
+    P.tau0[v] = (R)(in.tau0[v] * kToFs);
+    P.n_tables[v] = (int)in.table_mech[v].size();
+    P.table_off[v] = table_off[v];
+    ValleyDev<R> &V = P.valley[v];
+    V.c2 = (R)(phys::hbar * phys::hbar * 1e18 / (2.0 * m.effMassCond * phys::q));
+    V.alpha = (R)m.alpha;
+    V.cv = (R)(phys::hbar * 1e9 / m.effMassCond);
+    V.deg = vd.degeneracy;
+    for (int s = 0; s < vd.degeneracy; s++) {
+      // M = R^T diag(vogt) R with R rows = normalised subvalley axes
+      double M[3][3];
+      if (phys::isAniso(vd.kind)) {
+        double rows[3][3];
+        for (int r = 0; r < 3; r++) {
+          for (int j = 0; j < 3; j++)
+            rows[r][j] = vd.axes[s][3 * r + j];
+          normalize3(rows[r]);
+        }
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) {
+            double acc = 0;
+            for (int r = 0; r < 3; r++)
+              acc += rows[r][i] * m.vogt[r] * rows[r][j];
+            M[i][j] = acc;
+          }
+      } else {
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++)
+            M[i][j] = (i == j) ? 1.0 : 0.0;
+      }
+      for (int i = 0; i < 3; i++) {
+        double a = 0, w = 0;
+        for (int j = 0; j < 3; j++) {
+          a += M[i][j] * in.force[j];
+          w += M[i][j] * in.dir[j];
+        }
+        // dk/dt = M F / hbar  ->  [1/nm per fs]
+        V.accel[s][i] = (R)(a / phys::hbar * kToNm / kToFs);
+        V.wdir[s][i] = (R)w;
+      }
+    }
+    for (int j = 0; j < P.n_tables[v]; j++) {
+      const emcgpu_mech_desc &md = in.mechs[in.table_mech[v][j]];
+      MechDev<R> &mm = P.mech[v][j];
+      mm.final_valley = v;
+      mm.z = 1;
+      switch (md.kind) {
+      case EMCGPU_MECH_ACOUSTIC:
+        mm.final_kind = kFinalIsotropicElastic;
+        break;
+      case EMCGPU_MECH_ZERO_IV_AB:
+      case EMCGPU_MECH_ZERO_IV_EM:
+      case EMCGPU_MECH_FIRST_IV_AB:
+      case EMCGPU_MECH_FIRST_IV_EM: {
+        mm.final_kind = kFinalIsotropicInelastic;
+        mm.final_valley = md.final_valley;
+        mm.z = md.z;
+        const double delta = in.vm[md.final_valley].bottom - in.vm[md.valley].bottom;
+        const bool ab = md.kind == EMCGPU_MECH_ZERO_IV_AB || md.kind == EMCGPU_MECH_FIRST_IV_AB;
+        mm.shift = ab ? (R)(md.phonon_energy - delta) : (R)(-(md.phonon_energy + delta));
+        for (int s = 0; s < kMaxDeg; s++)
+          for (int z = 0; z < kMaxZ; z++)
+            mm.final_sub[s][z] = (uint8_t)md.final_sub[s][z];
+        break;
+      }
+      default:
+        mm.final_kind = kFinalFroehlich;
+        mm.shift = (md.kind == EMCGPU_MECH_FROEHLICH_AB) ? (R)md.phonon_energy
+                                                          : (R)(-md.phonon_energy);
+        break;
+      }
+    }
+  }
+}
+
+// builds the parameter blocks and the table arena of every instance
+template <typename R>
+emcgpu_status upload_params(emcgpu_ctx *c, double dt_s) {
+  if (!c->params_dirty && dt_s == c->params_dt)
+    return EMCGPU_OK;
+  std::vector<InstDev<R>> P(c->n_inst);
+  std::vector<R> cum;
+  for (int i = 0; i < c->n_inst; i++) {
+    const Instance &in = c->inst[i];
+    std::vector<long long> off(kMaxV, 0);
+    if (!in.has_tables)
+      continue;
+    for (size_t v = 0; v < in.valleys.size(); v++) {
+      off[v] = (long long)cum.size();
+      const int nt = (int)in.table_mech[v].size();
+      // device layout [level][table]
+      for (int l = 0; l < in.n_levels; l++)
+        for (int j = 0; j < nt; j++)
+          cum.push_back((R)in.cum[v][(size_t)j * in.n_levels + l]);
+    }
+    lower_instance<R>(c, in, dt_s, P[i], off);
+  }
+  if (cum.empty())
+    cum.push_back(R(0));
+  size_t bytes = cum.size() * sizeof(R);
+  if (bytes > c->cum_bytes) {
+    dfree(c->d_cum);
+    HIPCHK(c, hipMalloc(&c->d_cum, bytes));
+    c->cum_bytes = bytes;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_cum, cum.data(), bytes, hipMemcpyHostToDevice, c->stream));
+  if (!c->d_params)
+    HIPCHK(c, hipMalloc(&c->d_params, sizeof(InstDev<R>) * c->n_inst));
+  HIPCHK(c, hipMemcpyAsync(c->d_params, P.data(), sizeof(InstDev<R>) * c->n_inst,
+                           hipMemcpyHostToDevice, c->stream));
+  // the host vectors die here: wait for the copies
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->params_dirty = false;
+  c->params_dt = dt_s;
+  return EMCGPU_OK;
+}
+
+emcgpu_status ensure_partial(emcgpu_ctx *c, int k_steps) {
+  size_t bytes = (size_t)k_steps * (size_t)std::max(c->n_tiles, 1) * c->n_obs * sizeof(double);
+  if (bytes > c->partial_bytes) {
+    dfreet(c->d_partial);
+    HIPCHK(c, hipMalloc((void **)&c->d_partial, bytes));
+    c->partial_bytes = bytes;
+  }
+  return EMCGPU_OK;
+}
+
+emcgpu_status ensure_obs(emcgpu_ctx *c, int n_steps) {
+  size_t bytes = (size_t)c->n_inst * n_steps * c->n_obs * sizeof(double);
+  if (bytes > c->obs_bytes) {
+    dfreet(c->d_obs);
+    HIPCHK(c, hipMalloc((void **)&c->d_obs, bytes));
+    c->obs_bytes = bytes;
+  }
+  c->obs_steps = n_steps;
+  return EMCGPU_OK;
+}
+
+template <typename R>
+void fill_common(const emcgpu_ctx *c, StepArgs<R> &a) {
+  std::memset(&a, 0, sizeof(a));
+  a.kx = (R *)c->kx;
+  a.ky = (R *)c->ky;
+  a.kz = (R *)c->kz;
+  a.E = (R *)c->E;
+  a.tau = (R *)c->tau;
+  a.vs = c->vs;
+  a.inst = (const InstDev<R> *)c->d_params;
+  a.cum = (const R *)c->d_cum;
+  a.tile_inst = c->d_tile_inst;
+  a.inst_start = c->d_inst_start;
+  a.inst_count = c->d_inst_count;
+  a.n_tiles = c->n_tiles;
+  a.n_obs = c->n_obs;
+  a.max_v = c->max_v;
+  a.max_deg = c->max_deg;
+}
+
+template <typename R>
+emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
+  if (c->n_tiles == 0)
+    return fail(c, EMCGPU_ERR_STATE, "no particles (call emcgpu_resize first)");
+  for (int i = 0; i < c->n_inst; i++) {
+    if (c->inst[i].count && !c->inst[i].has_tables)
+      return fail(c, EMCGPU_ERR_STATE, "instance %d has particles but no scatter tables", i);
+  }
+  const bool replay = s->mode == EMCGPU_MODE_REPLAY;
+  if (replay && c->precision != EMCGPU_FP64)
+    return fail(c, EMCGPU_ERR_ARG, "replay mode needs an FP64 context");
+  if (replay && (!s->draws || !s->cursor || s->draw_stride == 0))
+    return fail(c, EMCGPU_ERR_ARG, "replay mode needs draws, cursor and draw_stride");
+  if (!(s->dt > 0) || s->n_steps < 0)
+    return fail(c, EMCGPU_ERR_ARG, "dt must be > 0 and n_steps >= 0");
+  if (s->n_steps == 0)
+    return EMCGPU_OK;
+  emcgpu_status st = upload_params<R>(c, s->dt);
+  if (st)
+    return st;
+  int K = s->steps_per_launch > 0 ? s->steps_per_launch : std::min(s->n_steps, 16);
+  K = std::min(K, s->n_steps);
+  if ((st = ensure_partial(c, K)) || (st = ensure_obs(c, s->n_steps)))
+    return st;
+
+  StepArgs<R> a;
+  fill_common<R>(c, a);
+  a.key0 = (uint32_t)s->seed;
+  a.key1 = (uint32_t)(s->seed >> 32);
+  a.partial = c->d_partial;
+  a.dt_s = s->dt;
+  const size_t n0 = c->inst[0].count;
+
+  // replay and trace buffers (instance 0)
+  uint64_t *d_draws = nullptr, *d_cursor = nullptr;
+  double *d_grain = nullptr;
+  int *d_over = nullptr;
+  int32_t *d_tv = nullptr, *d_ts = nullptr, *d_te = nullptr;
+  double *d_tk = nullptr, *d_tE = nullptr;
+  auto cleanup = [&]() {
+    dfreet(d_draws);
+    dfreet(d_cursor);
+    dfreet(d_grain);
+    dfreet(d_over);
+    dfreet(d_tv);
+    dfreet(d_ts);
+    dfreet(d_te);
+    dfreet(d_tk);
+    dfreet(d_tE);
+  };
+#define HIPCHK_CLEAN(x)                                                                  \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) {                                                              \
+      cleanup();                                                                         \
+      return fail(c, EMCGPU_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_));        \
+    }                                                                                    \
+  } while (0)
+  if (replay) {
+    if (c->n_inst != 1)
+      return fail(c, EMCGPU_ERR_ARG, "replay mode supports one instance");
+    HIPCHK_CLEAN(hipMalloc((void **)&d_draws, sizeof(uint64_t) * n0 * s->draw_stride + 8));
+    HIPCHK_CLEAN(hipMalloc((void **)&d_cursor, sizeof(uint64_t) * n0 + 8));
+    HIPCHK_CLEAN(hipMalloc((void **)&d_over, sizeof(int)));
+    HIPCHK_CLEAN(hipMemcpyAsync(d_draws, s->draws, sizeof(uint64_t) * n0 * s->draw_stride,
+                                hipMemcpyHostToDevice, c->stream));
+    HIPCHK_CLEAN(hipMemcpyAsync(d_cursor, s->cursor, sizeof(uint64_t) * n0,
+                                hipMemcpyHostToDevice, c->stream));
+    HIPCHK_CLEAN(hipMemsetAsync(d_over, 0, sizeof(int), c->stream));
+    if (s->grain_tau) {
+      HIPCHK_CLEAN(hipMalloc((void **)&d_grain, sizeof(double) * n0 + 8));
+      HIPCHK_CLEAN(hipMemcpyAsync(d_grain, s->grain_tau, sizeof(double) * n0,
+                                  hipMemcpyHostToDevice, c->stream));
+    }
+    a.draws = d_draws;
+    a.draw_stride = (long long)s->draw_stride;
+    a.cursor = d_cursor;
+    a.grain_tau = d_grain;
+    a.overflow = d_over;
+  }
+  const bool trace = s->trace_valley && s->trace_sub && s->trace_events && s->trace_k &&
+                     s->trace_energy;
+  if (trace) {
+    const size_t m = n0 * (size_t)s->n_steps + 1;
+    HIPCHK_CLEAN(hipMalloc((void **)&d_tv, sizeof(int32_t) * m));
+    HIPCHK_CLEAN(hipMalloc((void **)&d_ts, sizeof(int32_t) * m));
+    HIPCHK_CLEAN(hipMalloc((void **)&d_te, sizeof(int32_t) * m));
+    HIPCHK_CLEAN(hipMalloc((void **)&d_tk, sizeof(double) * 3 * m));
+    HIPCHK_CLEAN(hipMalloc((void **)&d_tE, sizeof(double) * m));
+    a.tr_n = (long long)n0;
+  }
+
+  for (int done = 0; done < s->n_steps; done += K) {
+    const int k = std::min(K, s->n_steps - done);
+    a.n_steps = k;
+    a.step0 = s->step0 + (uint32_t)done;
+    if (trace) {
+      const long long o = (long long)done * (long long)n0;
+      a.tr_valley = d_tv + o;
+      a.tr_sub = d_ts + o;
+      a.tr_events = d_te + o;
+      a.tr_k = d_tk + 3 * o;
+      a.tr_E = d_tE + o;
+    }
+    if (s->timing)
+      HIPCHK_CLEAN(hipEventRecord(c->ev0, c->stream));
+    HIPCHK_CLEAN(launch_steps<R>(a, replay ? 1 : 0, c->stream));
+    if (s->timing) {
+      HIPCHK_CLEAN(hipEventRecord(c->ev1, c->stream));
+      HIPCHK_CLEAN(hipEventSynchronize(c->ev1));
+      float ms = 0;
+      HIPCHK_CLEAN(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      c->ms_total += ms;
+      c->launches += 1;
+    }
+    HIPCHK_CLEAN(launch_reduce(c->d_partial, k, c->n_tiles, c->n_obs, c->d_inst_tile0,
+                               c->d_inst_ntiles, c->n_inst, c->d_obs, done, s->n_steps,
+                               c->stream));
+  }
+
+  if (replay) {
+    int over = 0;
+    HIPCHK_CLEAN(hipMemcpyAsync(s->cursor, d_cursor, sizeof(uint64_t) * n0,
+                                hipMemcpyDeviceToHost, c->stream));
+    HIPCHK_CLEAN(hipMemcpyAsync(&over, d_over, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (s->grain_tau)
+      HIPCHK_CLEAN(hipMemcpyAsync(s->grain_tau, d_grain, sizeof(double) * n0,
+                                  hipMemcpyDeviceToHost, c->stream));
+    HIPCHK_CLEAN(hipStreamSynchronize(c->stream));
+    if (over) {
+      cleanup();
+      return fail(c, EMCGPU_ERR_DRAWS, "%d particles ran out of recorded draws", over);
+    }
+  }
+  if (trace) {
+    const size_t m = n0 * (size_t)s->n_steps;
+    HIPCHK_CLEAN(hipMemcpyAsync(s->trace_valley, d_tv, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK_CLEAN(hipMemcpyAsync(s->trace_sub, d_ts, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK_CLEAN(hipMemcpyAsync(s->trace_events, d_te, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK_CLEAN(hipMemcpyAsync(s->trace_k, d_tk, sizeof(double) * 3 * m, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK_CLEAN(hipMemcpyAsync(s->trace_energy, d_tE, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+  }
+  if (replay || trace)
+    HIPCHK_CLEAN(hipStreamSynchronize(c->stream));
+  if (trace) {
+    // the kernels trace k in 1/nm
+    const size_t m = 3 * n0 * (size_t)s->n_steps;
+    for (size_t i = 0; i < m; i++)
+      s->trace_k[i] /= kToNm;
+  }
+  cleanup();
+#undef HIPCHK_CLEAN
+  return EMCGPU_OK;
+}
+
+// FP64 host tables: fillScatterTables + renormalizeTables
+emcgpu_status finish_tables(emcgpu_ctx *c, Instance &in, int n_levels, double max_energy,
+                            const std::vector<std::vector<double>> &raw /*per mech*/) {
+  const int nv = (int)in.valleys.size();
+  in.n_levels = n_levels;
+  in.max_energy = max_energy;
+  in.dE = max_energy / (double)(std::size_t)n_levels;  // emcScatterHandler.hpp:73-75
+  in.cum.assign(nv, {});
+  in.table_mech.assign(nv, {});
+  in.tau0.assign(nv, 2e-15);  // default of getTau (emcScatterHandler.hpp:79-84)
+  for (int j = 0; j < (int)in.mechs.size(); j++)
+    in.table_mech[in.mechs[j].valley].push_back(j);
+  for (int v = 0; v < nv; v++) {
+    const int nt = (int)in.table_mech[v].size();
+    if (nt == 0)
+      continue;
+    std::vector<double> t((size_t)nt * n_levels);
+    for (int j = 0; j < nt; j++)
+      for (int l = 0; l < n_levels; l++)
+        t[(size_t)j * n_levels + l] = raw[in.table_mech[v][j]][l];
+    for (int j = 1; j < nt; j++)
+      for (int l = 0; l < n_levels; l++)
+        t[(size_t)j * n_levels + l] = t[(size_t)j * n_levels + l] + t[(size_t)(j - 1) * n_levels + l];
+    double mx = t[(size_t)(nt - 1) * n_levels];
+    for (int l = 1; l < n_levels; l++)
+      if (mx < t[(size_t)(nt - 1) * n_levels + l])
+        mx = t[(size_t)(nt - 1) * n_levels + l];
+    if (!(mx > 0) || !std::isfinite(mx))
+      return fail(c, EMCGPU_ERR_ARG, "valley %d: maximum total scatter rate is %g", v, mx);
+    for (double &x : t)
+      x /= mx;
+    in.tau0[v] = 1. / mx;
+    in.cum[v] = std::move(t);
+  }
+  in.has_tables = true;
+  c->params_dirty = true;
+  return EMCGPU_OK;
+}
+
+template <typename R>
+emcgpu_status init_thermal(emcgpu_ctx *c, int instance, uint64_t seed) {
+  const Instance &in = c->inst[instance];
+  if (!in.has_tables)
+    return fail(c, EMCGPU_ERR_STATE, "instance %d: build tables before initialising", instance);
+  emcgpu_status st = upload_params<R>(c, 1e-15);
+  if (st)
+    return st;
+  StepArgs<R> a;
+  fill_common<R>(c, a);
+  a.key0 = (uint32_t)seed;
+  a.key1 = (uint32_t)(seed >> 32);
+  // one instance: its tiles only
+  a.tile_base = (int)(in.start / kTile);
+  a.n_tiles = (int)((in.count + kTile - 1) / kTile);
+  HIPCHK(c, launch_init<R>(a, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return EMCGPU_OK;
+}
+
+}  // namespace
+
+namespace {
+template <typename R>
+emcgpu_status set_particles_t(emcgpu_ctx *c, int instance, const double *k, const double *E,
+                              const double *tau, const int32_t *valley, const int32_t *sub) {
+  const Instance &in = c->inst[instance];
+  const size_t n = in.count;
+  if (n == 0)
+    return EMCGPU_OK;
+  std::vector<R> hx(n), hy(n), hz(n), he(n), ht(n);
+  std::vector<uint32_t> hv(n);
+  for (size_t i = 0; i < n; i++) {
+    const int v = valley[i], s = sub[i];
+    if (v < 0 || v >= (int)in.valleys.size() || s < 0 || s >= in.valleys[v].degeneracy)
+      return fail(c, EMCGPU_ERR_ARG, "particle %zu: invalid valley %d / subvalley %d", i, v, s);
+    hx[i] = (R)(k[3 * i] * kToNm);
+    hy[i] = (R)(k[3 * i + 1] * kToNm);
+    hz[i] = (R)(k[3 * i + 2] * kToNm);
+    he[i] = (R)E[i];
+    ht[i] = (R)(tau[i] * kToFs);
+    hv[i] = ((uint32_t)v << 16) | (uint32_t)s;
+  }
+  const size_t off = (size_t)in.start;
+  HIPCHK(c, hipMemcpyAsync((R *)c->kx + off, hx.data(), n * sizeof(R), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync((R *)c->ky + off, hy.data(), n * sizeof(R), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync((R *)c->kz + off, hz.data(), n * sizeof(R), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync((R *)c->E + off, he.data(), n * sizeof(R), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync((R *)c->tau + off, ht.data(), n * sizeof(R), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->vs + off, hv.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return EMCGPU_OK;
+}
+
+template <typename R>
+emcgpu_status get_particles_t(const emcgpu_ctx *c0, int instance, double *k, double *E,
+                              double *tau, int32_t *valley, int32_t *sub) {
+  emcgpu_ctx *c = const_cast<emcgpu_ctx *>(c0);
+  const Instance &in = c->inst[instance];
+  const size_t n = in.count;
+  if (n == 0)
+    return EMCGPU_OK;
+  std::vector<R> hx(n), hy(n), hz(n), he(n), ht(n);
+  std::vector<uint32_t> hv(n);
+  const size_t off = (size_t)in.start;
+  HIPCHK(c, hipMemcpyAsync(hx.data(), (R *)c->kx + off, n * sizeof(R), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(hy.data(), (R *)c->ky + off, n * sizeof(R), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(hz.data(), (R *)c->kz + off, n * sizeof(R), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(he.data(), (R *)c->E + off, n * sizeof(R), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(ht.data(), (R *)c->tau + off, n * sizeof(R), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(hv.data(), c->vs + off, n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < n; i++) {
+    if (k) {
+      k[3 * i] = (double)hx[i] / kToNm;
+      k[3 * i + 1] = (double)hy[i] / kToNm;
+      k[3 * i + 2] = (double)hz[i] / kToNm;
+    }
+    if (E)
+      E[i] = (double)he[i];
+    if (tau)
+      tau[i] = (double)ht[i] / kToFs;
+    if (valley)
+      valley[i] = (int32_t)(hv[i] >> 16);
+    if (sub)
+      sub[i] = (int32_t)(hv[i] & 0xffffu);
+  }
+  return EMCGPU_OK;
+}
+}  // namespace
+
+// ===========================================================================
+// exported C ABI
+// ===========================================================================
+extern "C" {
+
+emcgpu_status emcgpu_create(const emcgpu_config *cfg, emcgpu_ctx **out) {
+  if (!cfg || !out)
+    return fail(nullptr, EMCGPU_ERR_ARG, "cfg and out must be non-NULL");
+  *out = nullptr;
+  if (cfg->n_instances < 1 || cfg->n_instances > 4096)
+    return fail(nullptr, EMCGPU_ERR_ARG, "n_instances must be in [1, 4096]");
+  if (cfg->precision != EMCGPU_FP64 && cfg->precision != EMCGPU_FP32)
+    return fail(nullptr, EMCGPU_ERR_ARG, "unknown precision %d", cfg->precision);
+  if (cfg->device == -1) {
+    // host-only context: model, tables and validation, no device memory
+    emcgpu_ctx *c = new emcgpu_ctx();
+    c->device = -1;
+    c->precision = cfg->precision;
+    c->n_inst = cfg->n_instances;
+    c->inst.resize(c->n_inst);
+    *out = c;
+    return EMCGPU_OK;
+  }
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0)
+    return fail(nullptr, EMCGPU_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+  if (cfg->device < 0 || cfg->device >= ndev)
+    return fail(nullptr, EMCGPU_ERR_ARG, "device %d out of range [0, %d)", cfg->device, ndev);
+  if ((e = hipSetDevice(cfg->device)) != hipSuccess)
+    return fail(nullptr, EMCGPU_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+  emcgpu_ctx *c = new emcgpu_ctx();
+  c->device = cfg->device;
+  c->precision = cfg->precision;
+  c->n_inst = cfg->n_instances;
+  c->inst.resize(c->n_inst);
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess) {
+    delete c;
+    return fail(nullptr, EMCGPU_ERR_HIP, "stream/event creation: %s", hipGetErrorString(e));
+  }
+  *out = c;
+  return EMCGPU_OK;
+}
+
+void emcgpu_destroy(emcgpu_ctx *c) {
+  if (!c)
+    return;
+  if (c->device < 0) {
+    delete c;
+    return;
+  }
+  (void)hipSetDevice(c->device);
+  if (c->stream)
+    (void)hipStreamSynchronize(c->stream);
+  dfree(c->kx);
+  dfree(c->ky);
+  dfree(c->kz);
+  dfree(c->E);
+  dfree(c->tau);
+  dfreet(c->vs);
+  dfreet(c->d_tile_inst);
+  dfreet(c->d_inst_start);
+  dfreet(c->d_inst_count);
+  dfreet(c->d_inst_tile0);
+  dfreet(c->d_inst_ntiles);
+  dfree(c->d_params);
+  dfree(c->d_cum);
+  dfreet(c->d_partial);
+  dfreet(c->d_obs);
+  if (c->ev0)
+    (void)hipEventDestroy(c->ev0);
+  if (c->ev1)
+    (void)hipEventDestroy(c->ev1);
+  if (c->stream)
+    (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *emcgpu_last_error(const emcgpu_ctx *c) {
+  return c ? c->err.c_str() : g_create_error.c_str();
+}
+
+emcgpu_status emcgpu_set_model(emcgpu_ctx *c, int instance, const emcgpu_valley_desc *valleys,
+                               int n_valleys, const emcgpu_mech_desc *mechs, int n_mechs) {
+  emcgpu_status st = check_instance(c, instance);
+  if (st)
+    return st;
+  if (!valleys || n_valleys < 1 || n_valleys > EMCGPU_MAX_VALLEYS)
+    return fail(c, EMCGPU_ERR_ARG, "n_valleys must be in [1, %d]", EMCGPU_MAX_VALLEYS);
+  if (n_mechs < 0 || n_mechs > EMCGPU_MAX_MECH || (n_mechs && !mechs))
+    return fail(c, EMCGPU_ERR_ARG, "n_mechs must be in [0, %d]", EMCGPU_MAX_MECH);
+  for (int v = 0; v < n_valleys; v++) {
+    const emcgpu_valley_desc &d = valleys[v];
+    if (d.kind < 0 || d.kind > 3)
+      return fail(c, EMCGPU_ERR_ARG, "valley %d: unknown kind %d", v, d.kind);
+    // emcAbstractValley::check (emcAbstractValley.hpp:79-90)
+    if (d.degeneracy < 1 || d.degeneracy > EMCGPU_MAX_DEG)
+      return fail(c, EMCGPU_ERR_ARG, "valley %d: degeneracy must be in [1, %d]", v, EMCGPU_MAX_DEG);
+    if (d.alpha < 0)
+      return fail(c, EMCGPU_ERR_ARG, "valley %d: NonParabolicity of a valley has to be positive", v);
+    if (!(d.particle_mass > 0) || !(d.rel_mass[0] > 0) ||
+        (phys::isAniso(d.kind) && (!(d.rel_mass[1] > 0) || !(d.rel_mass[2] > 0))))
+      return fail(c, EMCGPU_ERR_ARG, "valley %d: masses must be positive", v);
+    if (phys::isAniso(d.kind)) {
+      for (int s = 0; s < d.degeneracy; s++) {
+        const double *a = d.axes[s];
+        auto dot = [&](int i, int j) {
+          return a[3 * i] * a[3 * j] + a[3 * i + 1] * a[3 * j + 1] + a[3 * i + 2] * a[3 * j + 2];
+        };
+        // checkOrthogonalBasis (emcNonParabolicAnistropValley.hpp:208-220)
+        if (dot(0, 1) != 0 || dot(0, 2) != 0 || dot(1, 2) != 0)
+          return fail(c, EMCGPU_ERR_ARG,
+                      "valley %d subvalley %d: the given coordinate system is not orthogonal", v, s);
+        if (dot(0, 0) == 0 || dot(1, 1) == 0 || dot(2, 2) == 0)
+          return fail(c, EMCGPU_ERR_ARG, "valley %d subvalley %d: zero axis", v, s);
+      }
+    }
+  }
+  for (int j = 0; j < n_mechs; j++) {
+    const emcgpu_mech_desc &m = mechs[j];
+    if (m.kind < 0 || m.kind > 6)
+      return fail(c, EMCGPU_ERR_ARG, "mechanism %d: unknown kind %d", j, m.kind);
+    if (m.valley < 0 || m.valley >= n_valleys)
+      return fail(c, EMCGPU_ERR_ARG, "mechanism %d: Used index for Valley is invalid", j);
+    if (m.kind >= EMCGPU_MECH_ZERO_IV_AB && m.kind <= EMCGPU_MECH_FIRST_IV_EM) {
+      // emcZeroOrderInterValleyScatterMechanism::check (:132-165)
+      if (m.final_valley < 0 || m.final_valley >= n_valleys)
+        return fail(c, EMCGPU_ERR_ARG, "mechanism %d: idxFinalValley %d is not valid", j, m.final_valley);
+      if (m.z < 1 || m.z > EMCGPU_MAX_Z)
+        return fail(c, EMCGPU_ERR_ARG, "mechanism %d: z must be in [1, %d]", j, EMCGPU_MAX_Z);
+      const int fdeg = valleys[m.final_valley].degeneracy;
+      for (int s = 0; s < valleys[m.valley].degeneracy; s++)
+        for (int z = 0; z < m.z; z++)
+          if (m.final_sub[s][z] < 0 || m.final_sub[s][z] >= fdeg)
+            return fail(c, EMCGPU_ERR_ARG,
+                        "mechanism %d: Used idx %d for valley of degeneracy %d is not valid", j,
+                        m.final_sub[s][z], fdeg);
+    }
+    if ((m.kind >= EMCGPU_MECH_ZERO_IV_AB && m.kind <= EMCGPU_MECH_FROEHLICH_EM) &&
+        !(m.phonon_energy > 0))
+      return fail(c, EMCGPU_ERR_ARG, "mechanism %d: phonon energy must be > 0", j);
+  }
+  Instance &in = c->inst[instance];
+  in.valleys.assign(valleys, valleys + n_valleys);
+  in.mechs.assign(mechs, mechs + n_mechs);
+  in.vm.clear();
+  for (const auto &d : in.valleys)
+    in.vm.emplace_back(d.kind, d.rel_mass, d.particle_mass, d.alpha, d.bottom_energy);
+  in.has_tables = false;
+  // observable layout covers the largest model of the context
+  int mv = 1, md = 1;
+  for (const auto &i2 : c->inst) {
+    mv = std::max(mv, (int)i2.valleys.size());
+    for (const auto &d : i2.valleys)
+      md = std::max(md, d.degeneracy);
+  }
+  c->max_v = mv;
+  c->max_deg = md;
+  c->n_obs = 2 + mv * (3 + md);
+  c->params_dirty = true;
+  return EMCGPU_OK;
+}
+
+emcgpu_status emcgpu_set_rate_tables(emcgpu_ctx *c, int instance, int n_levels,
+                                     double max_energy, const double *rates) {
+  emcgpu_status st = check_instance(c, instance);
+  if (st)
+    return st;
+  Instance &in = c->inst[instance];
+  if (in.valleys.empty())
+    return fail(c, EMCGPU_ERR_STATE, "instance %d: set the model first", instance);
+  if (n_levels < 1 || n_levels > EMCGPU_MAX_LEVELS || !(max_energy > 0) || !rates)
+    return fail(c, EMCGPU_ERR_ARG, "n_levels in [1, %d], max_energy > 0 and rates required",
+                EMCGPU_MAX_LEVELS);
+  std::vector<std::vector<double>> raw(in.mechs.size());
+  for (size_t j = 0; j < in.mechs.size(); j++)
+    raw[j].assign(rates + j * n_levels, rates + (j + 1) * n_levels);
+  return finish_tables(c, in, n_levels, max_energy, raw);
+}
+
+emcgpu_status emcgpu_build_tables(emcgpu_ctx *c, int instance, int n_levels, double max_energy,
+                                  double temperature, const emcgpu_material *mat) {
+  emcgpu_status st = check_instance(c, instance);
+  if (st)
+    return st;
+  Instance &in = c->inst[instance];
+  if (in.valleys.empty())
+    return fail(c, EMCGPU_ERR_STATE, "instance %d: set the model first", instance);
+  if (n_levels < 1 || n_levels > EMCGPU_MAX_LEVELS || !(max_energy > 0) || !mat ||
+      !(temperature > 0))
+    return fail(c, EMCGPU_ERR_ARG,
+                "n_levels in [1, %d], max_energy > 0, temperature > 0 and material required",
+                EMCGPU_MAX_LEVELS);
+  in.temperature = temperature;
+  const double dE = max_energy / (double)(std::size_t)n_levels;
+  std::vector<std::vector<double>> raw(in.mechs.size(), std::vector<double>(n_levels));
+  for (size_t j = 0; j < in.mechs.size(); j++) {
+    const emcgpu_mech_desc &m = in.mechs[j];
+    const phys::ValleyMass &iv = in.vm[m.valley];
+    double C = 0, nq = 0, meff = 0;
+    const bool ab = m.kind == EMCGPU_MECH_ZERO_IV_AB || m.kind == EMCGPU_MECH_FIRST_IV_AB ||
+                    m.kind == EMCGPU_MECH_FROEHLICH_AB;
+    switch (m.kind) {
+    case EMCGPU_MECH_ACOUSTIC:
+      C = phys::acousticConst(m.coupling, mat->rho, mat->v_sound, temperature);
+      break;
+    case EMCGPU_MECH_ZERO_IV_AB:
+    case EMCGPU_MECH_ZERO_IV_EM:
+      C = phys::zeroOrderConst(m.coupling, m.phonon_energy, mat->rho, temperature, m.z, ab);
+      break;
+    case EMCGPU_MECH_FIRST_IV_AB:
+    case EMCGPU_MECH_FIRST_IV_EM:
+      C = phys::firstOrderConst(m.coupling, m.phonon_energy, mat->rho, temperature, m.z, ab);
+      break;
+    default:
+      meff = m.rel_eff_mass * phys::me;
+      C = phys::froehlichConst(m.phonon_energy, meff, m.eps_hi, m.eps_lo);
+      nq = phys::boseEinstein(m.phonon_energy, temperature);
+      break;
+    }
+    for (int l = 0; l < n_levels; l++) {
+      const double E = (double)(std::size_t)(l + 1) * dE;
+      double r = 0;
+      switch (m.kind) {
+      case EMCGPU_MECH_ACOUSTIC:
+        r = phys::acousticRate(C, iv, E);
+        break;
+      case EMCGPU_MECH_ZERO_IV_AB:
+      case EMCGPU_MECH_ZERO_IV_EM: {
+        const phys::ValleyMass &fv = in.vm[m.final_valley];
+        r = phys::zeroOrderRate(C, fv, E, m.phonon_energy, fv.bottom - iv.bottom, ab);
+        break;
+      }
+      case EMCGPU_MECH_FIRST_IV_AB:
+      case EMCGPU_MECH_FIRST_IV_EM: {
+        const phys::ValleyMass &fv = in.vm[m.final_valley];
+        r = phys::firstOrderRate(C, iv, fv, E, m.phonon_energy, fv.bottom - iv.bottom, ab);
+        break;
+      }
+      default:
+        r = phys::froehlichRate(C, nq, meff, iv, E, m.phonon_energy, ab);
+        break;
+      }
+      raw[j][l] = r;
+    }
+  }
+  return finish_tables(c, in, n_levels, max_energy, raw);
+}
+
+emcgpu_status emcgpu_get_tables(const emcgpu_ctx *c, int instance, int valley, double *cum,
+                                double *tau0, int *n_tables) {
+  emcgpu_status st = check_instance(const_cast<emcgpu_ctx *>(c), instance);
+  if (st)
+    return st;
+  const Instance &in = c->inst[instance];
+  if (!in.has_tables)
+    return fail(const_cast<emcgpu_ctx *>(c), EMCGPU_ERR_STATE, "no tables");
+  if (valley < 0 || valley >= (int)in.valleys.size())
+    return fail(const_cast<emcgpu_ctx *>(c), EMCGPU_ERR_ARG, "valley out of range");
+  if (n_tables)
+    *n_tables = (int)in.table_mech[valley].size();
+  if (tau0)
+    *tau0 = in.tau0[valley];
+  if (cum)
+    std::copy(in.cum[valley].begin(), in.cum[valley].end(), cum);
+  return EMCGPU_OK;
+}
+
+emcgpu_status emcgpu_set_field(emcgpu_ctx *c, int instance, const double dir[3], double strength,
+                               double charge) {
+  emcgpu_status st = check_instance(c, instance);
+  if (st)
+    return st;
+  if (!dir)
+    return fail(c, EMCGPU_ERR_ARG, "dir must be non-NULL");
+  Instance &in = c->inst[instance];
+  double d[3] = {dir[0], dir[1], dir[2]};
+  // normalize (include/emcUtil.hpp:41-47), then scale twice as the bulk
+  // handler does (basicBulkParticleHandler.hpp:100-102, :186)
+  double n = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  if (n == 0.0)
+    return fail(c, EMCGPU_ERR_ARG, "field direction must be non-zero");
+  for (int i = 0; i < 3; i++) {
+    d[i] /= n;
+    in.dir[i] = d[i];
+    in.force[i] = d[i] * strength * charge;
+  }
+  in.has_field = true;
+  c->params_dirty = true;
+  return EMCGPU_OK;
+}
+
+emcgpu_status emcgpu_resize(emcgpu_ctx *c, const size_t *counts) {
+  if (!c || !counts)
+    return fail(c, EMCGPU_ERR_ARG, "counts must be non-NULL");
+  if (c->device < 0)
+    return fail(c, EMCGPU_ERR_STATE, "host-only context has no device memory");
+  (void)hipSetDevice(c->device);
+  std::vector<long long> start(c->n_inst), cnt(c->n_inst);
+  std::vector<int> tile0(c->n_inst), ntile(c->n_inst);
+  long long total = 0;
+  for (int i = 0; i < c->n_inst; i++) {
+    start[i] = total;
+    cnt[i] = (long long)counts[i];
+    tile0[i] = (int)(total / kTile);
+    const long long padded = (cnt[i] + kTile - 1) / kTile * kTile;
+    ntile[i] = (int)(padded / kTile);
+    total += padded;
+  }
+  if (total / kTile > (1LL << 30))
+    return fail(c, EMCGPU_ERR_ARG, "too many particles");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const size_t rs = real_size(c);
+  dfree(c->kx);
+  dfree(c->ky);
+  dfree(c->kz);
+  dfree(c->E);
+  dfree(c->tau);
+  dfreet(c->vs);
+  dfreet(c->d_tile_inst);
+  dfreet(c->d_inst_start);
+  dfreet(c->d_inst_count);
+  dfreet(c->d_inst_tile0);
+  dfreet(c->d_inst_ntiles);
+  c->capacity = total;
+  c->n_tiles = (int)(total / kTile);
+  const size_t n = (size_t)std::max<long long>(total, 1);
+  HIPCHK(c, hipMalloc(&c->kx, n * rs));
+  HIPCHK(c, hipMalloc(&c->ky, n * rs));
+  HIPCHK(c, hipMalloc(&c->kz, n * rs));
+  HIPCHK(c, hipMalloc(&c->E, n * rs));
+  HIPCHK(c, hipMalloc(&c->tau, n * rs));
+  HIPCHK(c, hipMalloc((void **)&c->vs, n * sizeof(uint32_t)));
+  HIPCHK(c, hipMemsetAsync(c->kx, 0, n * rs, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->ky, 0, n * rs, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->kz, 0, n * rs, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->E, 0, n * rs, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->tau, 0, n * rs, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->vs, 0, n * sizeof(uint32_t), c->stream));
+  std::vector<int> tile_inst((size_t)std::max(c->n_tiles, 1), 0);
+  for (int i = 0; i < c->n_inst; i++)
+    for (int t = 0; t < ntile[i]; t++)
+      tile_inst[tile0[i] + t] = i;
+  HIPCHK(c, hipMalloc((void **)&c->d_tile_inst, tile_inst.size() * sizeof(int)));
+  HIPCHK(c, hipMalloc((void **)&c->d_inst_start, c->n_inst * sizeof(long long)));
+  HIPCHK(c, hipMalloc((void **)&c->d_inst_count, c->n_inst * sizeof(long long)));
+  HIPCHK(c, hipMalloc((void **)&c->d_inst_tile0, c->n_inst * sizeof(int)));
+  HIPCHK(c, hipMalloc((void **)&c->d_inst_ntiles, c->n_inst * sizeof(int)));
+  HIPCHK(c, hipMemcpyAsync(c->d_tile_inst, tile_inst.data(), tile_inst.size() * sizeof(int),
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_inst_start, start.data(), c->n_inst * sizeof(long long),
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_inst_count, cnt.data(), c->n_inst * sizeof(long long),
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_inst_tile0, tile0.data(), c->n_inst * sizeof(int),
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_inst_ntiles, ntile.data(), c->n_inst * sizeof(int),
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < c->n_inst; i++) {
+    c->inst[i].count = (size_t)cnt[i];
+    c->inst[i].start = start[i];
+  }
+  return EMCGPU_OK;
+}
+
+emcgpu_status emcgpu_get_count(const emcgpu_ctx *c, int instance, size_t *count) {
+  emcgpu_status st = check_instance(const_cast<emcgpu_ctx *>(c), instance);
+  if (st)
+    return st;
+  if (count)
+    *count = c->inst[instance].count;
+  return EMCGPU_OK;
+}
+
+
+emcgpu_status emcgpu_set_particles(emcgpu_ctx *c, int instance, const double *k, const double *E,
+                                   const double *tau, const int32_t *valley, const int32_t *sub) {
+  emcgpu_status st = check_instance(c, instance);
+  if (st)
+    return st;
+  if (!k || !E || !tau || !valley || !sub)
+    return fail(c, EMCGPU_ERR_ARG, "all particle arrays are required");
+  if (c->inst[instance].valleys.empty())
+    return fail(c, EMCGPU_ERR_STATE, "instance %d: set the model first", instance);
+  if (c->device < 0)
+    return fail(c, EMCGPU_ERR_STATE, "host-only context has no device memory");
+  (void)hipSetDevice(c->device);
+  return c->precision == EMCGPU_FP32 ? set_particles_t<float>(c, instance, k, E, tau, valley, sub)
+                                     : set_particles_t<double>(c, instance, k, E, tau, valley, sub);
+}
+
+emcgpu_status emcgpu_get_particles(const emcgpu_ctx *c, int instance, double *k, double *E,
+                                   double *tau, int32_t *valley, int32_t *sub) {
+  emcgpu_status st = check_instance(const_cast<emcgpu_ctx *>(c), instance);
+  if (st)
+    return st;
+  if (c->device < 0)
+    return fail(const_cast<emcgpu_ctx *>(c), EMCGPU_ERR_STATE, "host-only context has no device memory");
+  (void)hipSetDevice(c->device);
+  return c->precision == EMCGPU_FP32 ? get_particles_t<float>(c, instance, k, E, tau, valley, sub)
+                                     : get_particles_t<double>(c, instance, k, E, tau, valley, sub);
+}
+
+emcgpu_status emcgpu_init_particles(emcgpu_ctx *c, int instance, uint64_t seed,
+                                    double temperature) {
+  emcgpu_status st = check_instance(c, instance);
+  if (st)
+    return st;
+  if (!(temperature > 0))
+    return fail(c, EMCGPU_ERR_ARG, "temperature must be > 0");
+  if (c->device < 0)
+    return fail(c, EMCGPU_ERR_STATE, "host-only context has no device memory");
+  (void)hipSetDevice(c->device);
+  c->inst[instance].temperature = temperature;
+  c->params_dirty = true;
+  return c->precision == EMCGPU_FP32 ? init_thermal<float>(c, instance, seed)
+                                     : init_thermal<double>(c, instance, seed);
+}
+
+emcgpu_status emcgpu_step(emcgpu_ctx *c, const emcgpu_step_args *s) {
+  if (!c || !s)
+    return fail(c, EMCGPU_ERR_ARG, "ctx and args must be non-NULL");
+  if (c->device < 0)
+    return fail(c, EMCGPU_ERR_STATE, "host-only context cannot step");
+  (void)hipSetDevice(c->device);
+  for (int i = 0; i < c->n_inst; i++)
+    if (c->inst[i].count && !c->inst[i].has_field)
+      return fail(c, EMCGPU_ERR_STATE, "instance %d: set the field first", i);
+  return c->precision == EMCGPU_FP32 ? run_steps<float>(c, s) : run_steps<double>(c, s);
+}
+
+emcgpu_status emcgpu_get_observables(const emcgpu_ctx *c0, int instance, double *out, int *n_obs,
+                                     int *n_steps) {
+  emcgpu_ctx *c = const_cast<emcgpu_ctx *>(c0);
+  emcgpu_status st = check_instance(c, instance);
+  if (st)
+    return st;
+  if (n_obs)
+    *n_obs = c->n_obs;
+  if (n_steps)
+    *n_steps = c->obs_steps;
+  if (out && c->obs_steps > 0 && c->device >= 0) {
+    (void)hipSetDevice(c->device);
+    const size_t m = (size_t)c->obs_steps * c->n_obs;
+    HIPCHK(c, hipMemcpyAsync(out, c->d_obs + (size_t)instance * m, m * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return EMCGPU_OK;
+}
+
+emcgpu_status emcgpu_obs_layout(const emcgpu_ctx *c, int *n_obs, int *max_valleys, int *max_deg) {
+  if (!c)
+    return EMCGPU_ERR_ARG;
+  if (n_obs)
+    *n_obs = c->n_obs;
+  if (max_valleys)
+    *max_valleys = c->max_v;
+  if (max_deg)
+    *max_deg = c->max_deg;
+  return EMCGPU_OK;
+}
+
+emcgpu_status emcgpu_get_timing(const emcgpu_ctx *c0, double *ms_total, long long *launches,
+                                int reset) {
+  emcgpu_ctx *c = const_cast<emcgpu_ctx *>(c0);
+  if (!c)
+    return EMCGPU_ERR_ARG;
+  if (ms_total)
+    *ms_total = c->ms_total;
+  if (launches)
+    *launches = c->launches;
+  if (reset) {
+    c->ms_total = 0;
+    c->launches = 0;
+  }
+  return EMCGPU_OK;
+}
+
+emcgpu_status emcgpu_synchronize(emcgpu_ctx *c) {
+  if (!c)
+    return EMCGPU_ERR_ARG;
+  if (c->device < 0)
+    return EMCGPU_OK;
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return EMCGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,101 @@
+// emcgpu_device.h -- parameter blocks shared by the host library and the
+// gfx950 kernels of the EMC particle step.
+//
+// Kinematics on the device run in scaled units so that FP32 neither
+// underflows nor overflows (hbar^2 k^2 is ~1e-50 in SI):
+//   k in 1/nm, energy in eV, time in fs, velocity in m/s.
+// Particle state lives in HBM as structure-of-arrays over an arena holding
+// every instance, each instance padded to a whole number of 256-particle
+// tiles so that a tile never straddles two instances.
+#ifndef EMCGPU_DEVICE_H
+#define EMCGPU_DEVICE_H
+
+#include <stdint.h>
+
+#include "../../include/emcgpu.h"
+
+namespace emcgpu {
+
+constexpr int kMaxV = EMCGPU_MAX_VALLEYS;
+constexpr int kMaxDeg = EMCGPU_MAX_DEG;
+constexpr int kMaxZ = EMCGPU_MAX_Z;
+constexpr int kMaxMech = EMCGPU_MAX_MECH;
+constexpr int kTile = 256;  // particles per tile == threads per block
+
+// final-state sampler of a mechanism
+enum FinalKind : int {
+  kFinalIsotropicElastic = 0,  // acoustic: |k| kept, random direction
+  kFinalIsotropicInelastic = 1,  // IV: E += shift, new valley/subvalley
+  kFinalFroehlich = 2            // polar optical: angle wrt old k
+};
+
+template <typename R>
+struct ValleyDev {
+  R c2;     // gamma(E) = c2 * |k|^2   [eV nm^2]
+  R alpha;  // non-parabolicity [1/eV] (0 for parabolic)
+  R cv;     // hbar * 1e9 / m_c        [m/s per 1/nm]
+  int deg;
+  int pad;
+  R accel[kMaxDeg][3];  // dk/dt in device frame [1/nm per fs]
+  R wdir[kMaxDeg][3];   // v.dir = cv / npf * dot(k, wdir[sub])
+};
+
+template <typename R>
+struct MechDev {
+  int final_kind;
+  int final_valley;
+  int z;
+  int pad;
+  R shift;  // energy change of the final state [eV] (IV, Froehlich)
+  uint8_t final_sub[kMaxDeg][kMaxZ];
+};
+
+template <typename R>
+struct InstDev {
+  int n_valleys;
+  int n_levels;
+  R dE;     // table energy step [eV]
+  R inv_dE;
+  R dt;     // step [fs]
+  R vt;     // thermal voltage [V] for initialisation
+  R tau0[kMaxV];  // [fs]
+  int n_tables[kMaxV];
+  long long table_off[kMaxV];  // element offset of the valley's [level][table] block
+  ValleyDev<R> valley[kMaxV];
+  MechDev<R> mech[kMaxV][kMaxMech];  // per valley, in table order
+};
+
+template <typename R>
+struct StepArgs {
+  R *kx, *ky, *kz, *E, *tau;
+  uint32_t *vs;  // valley << 16 | subvalley
+  const InstDev<R> *inst;
+  const R *cum;
+  const int *tile_inst;
+  const long long *inst_start;
+  const long long *inst_count;
+  int n_tiles;
+  int tile_base;  // first tile of the launch (initialisation of one instance)
+  int n_steps;   // steps fused in this launch
+  uint32_t step0;
+  uint32_t key0, key1;
+  double *partial;  // [n_steps][n_tiles][n_obs]
+  int n_obs;
+  int max_v;
+  int max_deg;
+  // replay
+  const uint64_t *draws;
+  long long draw_stride;
+  uint64_t *cursor;
+  double *grain_tau;  // seconds
+  double dt_s;
+  int *overflow;
+  // trace (instance 0)
+  int32_t *tr_valley, *tr_sub, *tr_events;
+  double *tr_k, *tr_E;
+  long long tr_n;
+};
+
+}  // namespace emcgpu
+
+#endif

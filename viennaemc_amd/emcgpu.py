@@ -1,0 +1,343 @@
+"""ctypes binding of the C ABI in include/emcgpu.h (libemcgpu.so).
+
+The HIP library is the only compute path: importing this module does not
+need a GPU, but creating a :class:`Context` loads ``libemcgpu.so`` and fails
+loudly if it is missing or if no HIP device is present.  There is no CPU
+fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libemcgpu.so")
+
+MAX_VALLEYS, MAX_DEG, MAX_Z, MAX_MECH, MAX_LEVELS = 4, 8, 8, 32, 8192
+FP64, FP32 = 0, 1
+PRODUCTION, REPLAY = 0, 1
+PARA_ISO, NONPARA_ISO, PARA_ANISO, NONPARA_ANISO = 0, 1, 2, 3
+(ACOUSTIC, ZERO_IV_AB, ZERO_IV_EM, FIRST_IV_AB, FIRST_IV_EM, FROEHLICH_AB,
+ FROEHLICH_EM) = range(7)
+
+STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_STATE", 4: "ERR_DRAWS", 5: "ERR_NOMEM"}
+
+# include/emcConstants.hpp:9-25
+Q = 1.60219e-19
+ME = 9.11e-31
+KB = 1.38066e-23
+
+
+class EmcGpuError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s: %s" % (STATUS.get(status, status), msg))
+        self.status = status
+
+
+class Config(C.Structure):
+    _fields_ = [("device", C.c_int), ("precision", C.c_int), ("n_instances", C.c_int)]
+
+
+class ValleyDesc(C.Structure):
+    _fields_ = [("kind", C.c_int), ("rel_mass", C.c_double * 3), ("particle_mass", C.c_double),
+                ("degeneracy", C.c_int), ("alpha", C.c_double), ("bottom_energy", C.c_double),
+                ("axes", (C.c_double * 9) * MAX_DEG)]
+
+
+class MechDesc(C.Structure):
+    _fields_ = [("kind", C.c_int), ("valley", C.c_int), ("final_valley", C.c_int),
+                ("phonon_energy", C.c_double), ("coupling", C.c_double),
+                ("rel_eff_mass", C.c_double), ("eps_hi", C.c_double), ("eps_lo", C.c_double),
+                ("z", C.c_int), ("final_sub", (C.c_int * MAX_Z) * MAX_DEG)]
+
+
+class Material(C.Structure):
+    _fields_ = [("eps_r", C.c_double), ("rho", C.c_double), ("ni", C.c_double),
+                ("v_sound", C.c_double)]
+
+
+class StepArgs(C.Structure):
+    _fields_ = [("dt", C.c_double), ("n_steps", C.c_int), ("steps_per_launch", C.c_int),
+                ("mode", C.c_int), ("seed", C.c_uint64), ("step0", C.c_uint32),
+                ("timing", C.c_int), ("draws", C.c_void_p), ("draw_stride", C.c_size_t),
+                ("cursor", C.c_void_p), ("grain_tau", C.c_void_p),
+                ("trace_valley", C.c_void_p), ("trace_sub", C.c_void_p),
+                ("trace_events", C.c_void_p), ("trace_k", C.c_void_p),
+                ("trace_energy", C.c_void_p)]
+
+
+_LIB = None
+EXPORTS = [
+    "emcgpu_create", "emcgpu_destroy", "emcgpu_last_error", "emcgpu_set_model",
+    "emcgpu_build_tables", "emcgpu_set_rate_tables", "emcgpu_get_tables", "emcgpu_set_field",
+    "emcgpu_resize", "emcgpu_get_count", "emcgpu_set_particles", "emcgpu_get_particles",
+    "emcgpu_init_particles", "emcgpu_step", "emcgpu_get_observables", "emcgpu_obs_layout",
+    "emcgpu_get_timing", "emcgpu_synchronize",
+]
+
+
+def load(path=LIB_PATH):
+    """Load libemcgpu.so and declare the C signatures (no device calls)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise RuntimeError("libemcgpu.so is not built (%s); run __graft_entry__.build()" % path)
+    L = C.CDLL(path)
+    vp, i, d, p = C.c_void_p, C.c_int, C.c_double, C.POINTER
+    sig = {
+        "emcgpu_create": (i, [p(Config), p(vp)]),
+        "emcgpu_destroy": (None, [vp]),
+        "emcgpu_last_error": (C.c_char_p, [vp]),
+        "emcgpu_set_model": (i, [vp, i, p(ValleyDesc), i, p(MechDesc), i]),
+        "emcgpu_build_tables": (i, [vp, i, i, d, d, p(Material)]),
+        "emcgpu_set_rate_tables": (i, [vp, i, i, d, vp]),
+        "emcgpu_get_tables": (i, [vp, i, i, vp, p(d), p(i)]),
+        "emcgpu_set_field": (i, [vp, i, p(d), d, d]),
+        "emcgpu_resize": (i, [vp, p(C.c_size_t)]),
+        "emcgpu_get_count": (i, [vp, i, p(C.c_size_t)]),
+        "emcgpu_set_particles": (i, [vp, i, vp, vp, vp, vp, vp]),
+        "emcgpu_get_particles": (i, [vp, i, vp, vp, vp, vp, vp]),
+        "emcgpu_init_particles": (i, [vp, i, C.c_uint64, d]),
+        "emcgpu_step": (i, [vp, p(StepArgs)]),
+        "emcgpu_get_observables": (i, [vp, i, vp, p(i), p(i)]),
+        "emcgpu_obs_layout": (i, [vp, p(i), p(i), p(i)]),
+        "emcgpu_get_timing": (i, [vp, p(d), p(C.c_longlong), i]),
+        "emcgpu_synchronize": (i, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def valley_desc(v, particle_mass=ME):
+    """Lower a valley dictionary (tests/emc_models.py format) to the C descriptor."""
+    d = ValleyDesc()
+    d.kind = v["kind"]
+    for j in range(3):
+        d.rel_mass[j] = v["mass"][j]
+    d.particle_mass = particle_mass
+    d.degeneracy = v["deg"]
+    d.alpha = v.get("alpha", 0.0)
+    d.bottom_energy = v.get("bottom", 0.0)
+    for s, ax in enumerate(v.get("axes", [])):
+        flat = [x for row in ax for x in row]
+        for j in range(9):
+            d.axes[s][j] = flat[j]
+    return d
+
+
+def mech_desc(m):
+    """Lower a mechanism dictionary (tests/emc_models.py format) to the C descriptor."""
+    d = MechDesc()
+    d.kind = m["kind"]
+    d.valley = m["valley"]
+    d.final_valley = m.get("final", m["valley"])
+    if m["kind"] == ACOUSTIC:
+        d.coupling = m["sigma"]
+    elif m["kind"] in (ZERO_IV_AB, ZERO_IV_EM, FIRST_IV_AB, FIRST_IV_EM):
+        d.coupling = m["defpot"]
+        d.phonon_energy = m["hw"]
+        fmap = m["fmap"]
+        d.z = len(fmap[0])
+        for s, row in enumerate(fmap):
+            for z, f in enumerate(row):
+                d.final_sub[s][z] = f
+    else:
+        d.phonon_energy = m["hw"]
+        d.rel_eff_mass = m["mrel"]
+        d.eps_hi = m["eps_hi"]
+        d.eps_lo = m["eps_lo"]
+    return d
+
+
+class Context:
+    """One emcgpu context: particle state + tables on one GPU."""
+
+    def __init__(self, precision=FP64, n_instances=1, device=0):
+        self.L = load()
+        cfg = Config(device, precision, n_instances)
+        h = C.c_void_p()
+        st = self.L.emcgpu_create(C.byref(cfg), C.byref(h))
+        if st:
+            raise EmcGpuError(st, self.L.emcgpu_last_error(None).decode())
+        self.h = h
+        self.precision = precision
+        self.n_instances = n_instances
+        self.counts = [0] * n_instances
+        self.n_valleys = [0] * n_instances
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.emcgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, st):
+        if st:
+            raise EmcGpuError(st, self.L.emcgpu_last_error(self.h).decode())
+
+    # -- model -----------------------------------------------------------
+    def set_model(self, instance, spec, particle_mass=ME):
+        vs = spec["valleys"]
+        ms = spec["mechs"]
+        va = (ValleyDesc * len(vs))(*[valley_desc(v, particle_mass) for v in vs])
+        ma = (MechDesc * max(len(ms), 1))(*[mech_desc(m) for m in ms])
+        self._chk(self.L.emcgpu_set_model(self.h, instance, va, len(vs), ma, len(ms)))
+        self.n_valleys[instance] = len(vs)
+
+    def build_tables(self, instance, n_levels, max_energy, T, rho=2329.0, v_sound=9040.0,
+                     eps_r=11.8, ni=1.45e16):
+        mat = Material(eps_r, rho, ni, v_sound)
+        self._chk(self.L.emcgpu_build_tables(self.h, instance, n_levels, max_energy, T,
+                                             C.byref(mat)))
+
+    def set_rate_tables(self, instance, max_energy, rates):
+        r = np.ascontiguousarray(rates, dtype=np.float64)
+        self._chk(self.L.emcgpu_set_rate_tables(self.h, instance, r.shape[1], max_energy, _ptr(r)))
+
+    def get_tables(self, instance, valley, n_levels):
+        nt = C.c_int()
+        tau0 = C.c_double()
+        self._chk(self.L.emcgpu_get_tables(self.h, instance, valley, None, C.byref(tau0),
+                                           C.byref(nt)))
+        out = np.zeros(nt.value * n_levels)
+        self._chk(self.L.emcgpu_get_tables(self.h, instance, valley, _ptr(out), C.byref(tau0),
+                                           C.byref(nt)))
+        return out.reshape(nt.value, n_levels), tau0.value
+
+    def set_field(self, instance, direction, strength, charge=-Q):
+        d = (C.c_double * 3)(*direction)
+        self._chk(self.L.emcgpu_set_field(self.h, instance, d, strength, charge))
+
+    def setup(self, instance, spec, T=None):
+        """set_model + build_tables + set_field from a model dictionary."""
+        T = spec["T"] if T is None else T
+        self.set_model(instance, spec)
+        self.build_tables(instance, spec["n_levels"], spec["max_energy"], T,
+                          rho=spec.get("rho", 2329.0), v_sound=spec.get("v_sound", 9040.0))
+        self.set_field(instance, spec["field_dir"], spec["field_kvcm"] * 1e5)
+
+    # -- particles -------------------------------------------------------
+    def resize(self, counts):
+        arr = (C.c_size_t * self.n_instances)(*counts)
+        self._chk(self.L.emcgpu_resize(self.h, arr))
+        self.counts = list(counts)
+
+    def set_particles(self, instance, k, E, tau, valley, sub):
+        k = np.ascontiguousarray(k, dtype=np.float64).reshape(-1)
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        tau = np.ascontiguousarray(tau, dtype=np.float64)
+        valley = np.ascontiguousarray(valley, dtype=np.int32)
+        sub = np.ascontiguousarray(sub, dtype=np.int32)
+        n = self.counts[instance]
+        if k.size != 3 * n or E.size != n or tau.size != n or valley.size != n or sub.size != n:
+            raise ValueError("particle arrays must hold %d particles" % n)
+        self._chk(self.L.emcgpu_set_particles(self.h, instance, _ptr(k), _ptr(E), _ptr(tau),
+                                              _ptr(valley), _ptr(sub)))
+
+    def get_particles(self, instance=0):
+        n = self.counts[instance]
+        k = np.zeros(3 * n)
+        E = np.zeros(n)
+        tau = np.zeros(n)
+        v = np.zeros(n, dtype=np.int32)
+        s = np.zeros(n, dtype=np.int32)
+        self._chk(self.L.emcgpu_get_particles(self.h, instance, _ptr(k), _ptr(E), _ptr(tau),
+                                              _ptr(v), _ptr(s)))
+        return dict(k=k, E=E, tau=tau, valley=v, sub=s)
+
+    def init_particles(self, instance, seed, T):
+        self._chk(self.L.emcgpu_init_particles(self.h, instance, seed, T))
+
+    # -- stepping --------------------------------------------------------
+    def step(self, dt, n_steps, mode=PRODUCTION, seed=0, step0=0, steps_per_launch=0,
+             timing=False, draws=None, cursor=None, grain_tau=None, trace=False):
+        a = StepArgs()
+        a.dt = dt
+        a.n_steps = n_steps
+        a.steps_per_launch = steps_per_launch
+        a.mode = mode
+        a.seed = seed
+        a.step0 = step0
+        a.timing = 1 if timing else 0
+        keep = []
+        if mode == REPLAY:
+            draws = np.ascontiguousarray(draws, dtype=np.uint64)
+            if cursor.dtype != np.uint64 or not cursor.flags["C_CONTIGUOUS"]:
+                raise ValueError("cursor must be a contiguous uint64 array")
+            a.draws = _ptr(draws)
+            a.draw_stride = draws.shape[1]
+            a.cursor = _ptr(cursor)
+            if grain_tau is not None:
+                a.grain_tau = _ptr(grain_tau)
+            keep += [draws]
+        tr = None
+        if trace:
+            n = self.counts[0]
+            tr = dict(valley=np.zeros((n_steps, n), dtype=np.int32),
+                      sub=np.zeros((n_steps, n), dtype=np.int32),
+                      events=np.zeros((n_steps, n), dtype=np.int32),
+                      k=np.zeros((n_steps, n, 3)), E=np.zeros((n_steps, n)))
+            a.trace_valley = _ptr(tr["valley"])
+            a.trace_sub = _ptr(tr["sub"])
+            a.trace_events = _ptr(tr["events"])
+            a.trace_k = _ptr(tr["k"])
+            a.trace_energy = _ptr(tr["E"])
+        self._chk(self.L.emcgpu_step(self.h, C.byref(a)))
+        return tr
+
+    def observables(self, instance=0):
+        """Raw per-step sums, shape (n_steps, n_obs)."""
+        n_obs = C.c_int()
+        n_steps = C.c_int()
+        self._chk(self.L.emcgpu_get_observables(self.h, instance, None, C.byref(n_obs),
+                                                C.byref(n_steps)))
+        out = np.zeros(n_obs.value * n_steps.value)
+        self._chk(self.L.emcgpu_get_observables(self.h, instance, _ptr(out), C.byref(n_obs),
+                                                C.byref(n_steps)))
+        return out.reshape(n_steps.value, n_obs.value)
+
+    def obs_layout(self):
+        n, mv, md = C.c_int(), C.c_int(), C.c_int()
+        self._chk(self.L.emcgpu_obs_layout(self.h, C.byref(n), C.byref(mv), C.byref(md)))
+        return n.value, mv.value, md.value
+
+    def averages(self, instance=0):
+        """Per-step getAvgEnergy / getAvgDriftVelocity / getValleyOccupationProbability
+        (basicBulkParticleHandler.hpp:289-347) plus per-subvalley occupancy."""
+        raw = self.observables(instance)
+        _, mv, md = self.obs_layout()
+        nv = self.n_valleys[instance]
+        n = self.counts[instance]
+        cnt = raw[:, 2:2 + 3 * mv:3][:, :nv]
+        sE = raw[:, 3:3 + 3 * mv:3][:, :nv]
+        sV = raw[:, 4:4 + 3 * mv:3][:, :nv]
+        with np.errstate(invalid="ignore", divide="ignore"):
+            avgE = np.where(cnt > 0, sE / np.maximum(cnt, 1), 0.0)
+            avgV = np.where(cnt > 0, sV / np.maximum(cnt, 1), 0.0)
+        occ = cnt / max(n, 1)
+        sub = raw[:, 2 + 3 * mv:].reshape(raw.shape[0], mv, md)[:, :nv, :] / max(n, 1)
+        return dict(avgE=avgE, avgV=avgV, occ=occ, sub_occ=sub, events=raw[:, 0],
+                    real_events=raw[:, 1], sumE=sE, sumV=sV, count=cnt)
+
+    def timing(self, reset=False):
+        ms = C.c_double()
+        n = C.c_longlong()
+        self._chk(self.L.emcgpu_get_timing(self.h, C.byref(ms), C.byref(n), 1 if reset else 0))
+        return ms.value, n.value
+
+    def synchronize(self):
+        self._chk(self.L.emcgpu_synchronize(self.h))
