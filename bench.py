@@ -44,28 +44,48 @@ def parse():
     ap.add_argument("--fp64-steps", type=int, default=0, help="FP64 timed steps (0 = --steps)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the FP32 one-step-per-launch leg")
+    ap.add_argument("--ppl", type=int, default=0,
+                    help="particles per GPU lane of the production kernel (0 = library default)")
     ap.add_argument("--cpu-particles", type=int, default=1_000_000)
     ap.add_argument("--cpu-steps", type=int, default=300)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (rocprof byte-counter calibration)")
     return ap.parse_args()
 
 
-def run_gpu(emcgpu, spec, precision, n, warmup, steps, spl, seed=20261202):
+def copy_calibration(mib):
+    """One device-to-device copy kernel of known size (torch elementwise copy),
+    to scale rocprof's FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}))
+
+
+def run_gpu(emcgpu, spec, precision, n, warmup, steps, spl, seed=20261202, ppl=0):
     ctx = emcgpu.Context(precision=precision)
     ctx.setup(0, spec)
     ctx.resize([n])
     ctx.init_particles(0, seed, spec["T"])
     if warmup:
-        ctx.step(1e-15, warmup, seed=seed, step0=0, steps_per_launch=spl)
+        ctx.step(1e-15, warmup, seed=seed, step0=0, steps_per_launch=spl, ppl=ppl)
     ctx.synchronize()
     t0 = time.perf_counter()
-    ctx.step(1e-15, steps, seed=seed, step0=warmup, steps_per_launch=spl)
+    ctx.step(1e-15, steps, seed=seed, step0=warmup, steps_per_launch=spl, ppl=ppl)
     ctx.synchronize()
     wall = time.perf_counter() - t0
     av = ctx.averages(0)
     # timed pass for the kernel duration (HIP events around every launch)
     nt = max(spl, 1) * 4 if spl else 64
     ctx.timing(reset=True)
-    ctx.step(1e-15, nt, seed=seed, step0=warmup + steps, steps_per_launch=spl, timing=True)
+    ctx.step(1e-15, nt, seed=seed, step0=warmup + steps, steps_per_launch=spl, timing=True,
+             ppl=ppl)
     ms, launches = ctx.timing()
     ctx.close()
     return dict(wall=wall, av=av, kernel_ms=ms / max(launches, 1), launches=launches,
@@ -88,6 +108,9 @@ def cpu_baseline(spec, n, steps):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return 0
     if args.gpus != 1:
         print("bench.py measures one GPU: the particle step has no multi-GPU path in this build "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
@@ -97,24 +120,32 @@ def main():
     spec = MODELS["si_jacoboni"]
     n = args.particles
 
-    r32 = run_gpu(emcgpu, spec, emcgpu.FP32, n, args.warmup, args.steps, args.spl)
+    r32 = run_gpu(emcgpu, spec, emcgpu.FP32, n, args.warmup, args.steps, args.spl,
+                  ppl=args.ppl)
     ups32 = n * args.steps / r32["wall"]
     out = {}
     if not args.no_fp64:
         s64 = args.fp64_steps or args.steps
-        r64 = run_gpu(emcgpu, spec, emcgpu.FP64, n, args.warmup, s64, 1)
+        r64 = run_gpu(emcgpu, spec, emcgpu.FP64, n, args.warmup, s64, 1, ppl=args.ppl)
         ups64 = n * s64 / r64["wall"]
         tail = max(1, min(args.steps, s64) // 2)
         v32, v64 = r32["av"]["avgV"][-tail:, 0].mean(), r64["av"]["avgV"][-tail:, 0].mean()
         e32, e64 = r32["av"]["avgE"][-tail:, 0].mean(), r64["av"]["avgE"][-tail:, 0].mean()
-        out.update(fp64_plain_updates_per_s=ups64, fp32_over_fp64=ups32 / ups64,
+        out.update(fp64_plain_updates_per_s=ups64, fp32_over_fp64_plain=ups32 / ups64,
                    fp64_kernel_ms=r64["kernel_ms"],
                    fp64_roofline_frac=(BYTES_FP64 * n / (r64["kernel_ms"] * 1e-3)) / 1e9 / HBM_PEAK_GBS,
                    delta_vd_rel=(v32 - v64) / v64, delta_E_rel=(e32 - e64) / e64,
                    vd_fp32=v32, vd_fp64=v64, meanE_fp32=e32, meanE_fp64=e64)
+    if not args.no_plain:
+        r1 = run_gpu(emcgpu, spec, emcgpu.FP32, n, 0, min(args.steps, 200), 1, ppl=args.ppl)
+        out["fp32_plain_updates_per_s"] = n * min(args.steps, 200) / r1["wall"]
+        out["fp32_plain_kernel_ms"] = r1["kernel_ms"]
+        out["fp32_plain_roofline_frac"] = BYTES_FP32 * n / (r1["kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
     spl = r32["steps_per_launch"]
-    # algorithmic bytes of one launch: the state is read and written once
-    achieved = BYTES_FP32 * n / (r32["kernel_ms"] * 1e-3) / 1e9
+    # algorithmic bytes per launch: 48 B per particle-update (the SoA state
+    # read and written once per update) x the updates one launch processes;
+    # with fused steps the HBM traffic itself is 1/spl of this
+    achieved = BYTES_FP32 * n * spl / (r32["kernel_ms"] * 1e-3) / 1e9
     res = {
         "metric": "particle-updates/s (Si bulk, 10M e-) FP32-batched vs FP64 plain; % HBM peak; dv_d",
         "value": ups32,
@@ -130,7 +161,7 @@ def main():
         "data": "synthetic (thermal Maxwellian init, Philox-4x32-10)",
         "config": {"workload": "Si bulk X-valley EMC, 10M e-, 50 kV/cm [100], dt 1 fs, 300 K, "
                                "acoustic + 6 zero-order Jacoboni phonons, 2000-level tables",
-                   "particles": n, "steps_per_launch": spl},
+                   "particles": n, "steps_per_launch": spl, "particles_per_lane": args.ppl},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
     }

@@ -117,6 +117,8 @@ typedef struct {
   uint64_t seed;        /* production: Philox key */
   uint32_t step0;       /* production: global index of the first step */
   int timing;           /* non-zero: time every launch with HIP events */
+  int particles_per_lane; /* production kernel: particles owned by each
+                             GPU lane, 1, 2, 4 or 8 (0 = library default) */
   /* replay (FP64 contexts only): per-particle recorded raw 64-bit draws of
    * the reference engine, host memory, particle order of instance 0 */
   const uint64_t *draws; /* [n_particles][draw_stride] */

@@ -46,7 +46,7 @@ def test_struct_layouts_match_header():
     # sizes computed by the C compiler for the descriptors of include/emcgpu.h
     assert C.sizeof(emcgpu.ValleyDesc) == 4 + 4 + 24 + 8 + 4 + 4 + 8 + 8 + 8 * 9 * 8
     assert C.sizeof(emcgpu.MechDesc) == 3 * 4 + 4 + 5 * 8 + 4 + 8 * 8 * 4 + 4
-    assert C.sizeof(emcgpu.StepArgs) == 8 + 4 * 3 + 4 + 8 + 4 + 4 + 8 * 9
+    assert C.sizeof(emcgpu.StepArgs) == 8 + 4 * 3 + 4 + 8 + 4 + 4 + 4 + 4 + 8 * 9
 
 
 def host_ctx(n_instances=1):
@@ -131,3 +131,14 @@ def test_tables_before_model_is_a_state_error():
     with pytest.raises(emcgpu.EmcGpuError) as e:
         ctx.build_tables(0, 100, 1.0, 300.0)
     assert e.value.status == 3
+
+
+def test_rejects_negative_user_rates(golden):
+    spec = MODELS["si_jacoboni"]
+    raw = golden["si_jacoboni.rate_table"].reshape(len(spec["mechs"]), -1).copy()
+    raw[3, 10] = -1.0
+    ctx = host_ctx()
+    ctx.set_model(0, spec)
+    with pytest.raises(emcgpu.EmcGpuError) as e:
+        ctx.set_rate_tables(0, spec["max_energy"], raw)
+    assert "non-negative" in str(e.value)

@@ -37,8 +37,9 @@ def _rel_k(a, b):
     return np.linalg.norm(a - b, axis=-1) / np.maximum(np.linalg.norm(b, axis=-1), 1e-300)
 
 
+@pytest.mark.parametrize("ppl", [1, 2, 4, 8])
 @pytest.mark.parametrize("name", ["si_jacoboni", "polar_gamma", "two_valley", "para_iso"])
-def test_fp64_production_matches_oracle_twin(name):
+def test_fp64_production_matches_oracle_twin(name, ppl):
     emcgpu = _emc()
     spec = MODELS[name]
     n, ns, seed = 4096, 100, 0x5EED1234ABCD
@@ -56,7 +57,7 @@ def test_fp64_production_matches_oracle_twin(name):
     # run both from the oracle's state so the comparison isolates the step
     ctx.set_particles(0, ref["k"], ref["E"], ref["tau"], ref["valley"], ref["sub"])
     force, d = field_force(spec)
-    ctx.step(1e-15, ns, seed=seed, step0=7, steps_per_launch=32)
+    ctx.step(1e-15, ns, seed=seed, step0=7, steps_per_launch=32, ppl=ppl)
     obs_ref = M.run_philox(ref, ns, 1e-15, force, d, seed, instance=0, step0=7)
     g = ctx.get_particles(0)
     assert (g["valley"] == ref["valley"]).all()

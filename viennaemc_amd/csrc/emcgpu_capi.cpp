@@ -22,12 +22,12 @@
 
 namespace emcgpu {
 template <typename R>
-hipError_t launch_steps(const StepArgs<R> &a, int mode, hipStream_t st);
+hipError_t launch_steps(const StepArgs<R> &a, int mode, int ppt, hipStream_t st);
 template <typename R>
 hipError_t launch_init(const StepArgs<R> &a, hipStream_t st);
-hipError_t launch_reduce(const double *partial, int n_steps, int n_tiles, int n_obs,
-                         const int *inst_tile0, const int *inst_ntiles, int n_inst, double *out,
-                         int out_step0, int out_total, hipStream_t st);
+hipError_t launch_reduce(const double *partial, int n_steps, int n_tiles, int n_obs, int tile_size,
+                         const long long *inst_start, const long long *inst_count, int n_inst,
+                         double *out, int out_step0, int out_total, hipStream_t st);
 }  // namespace emcgpu
 
 using namespace emcgpu;
@@ -65,13 +65,11 @@ struct emcgpu_ctx {
   std::string err;
   std::vector<Instance> inst;
   // arena
-  long long capacity = 0;  // padded particles
-  int n_tiles = 0;
+  long long capacity = 0;  // padded particles (multiple of kUnit)
   void *kx = nullptr, *ky = nullptr, *kz = nullptr, *E = nullptr, *tau = nullptr;
   uint32_t *vs = nullptr;
-  int *d_tile_inst = nullptr;
+  int *d_unit_inst = nullptr;
   long long *d_inst_start = nullptr, *d_inst_count = nullptr;
-  int *d_inst_tile0 = nullptr, *d_inst_ntiles = nullptr;
   // parameters and tables
   void *d_params = nullptr;
   void *d_cum = nullptr;
@@ -282,8 +280,12 @@ emcgpu_status upload_params(emcgpu_ctx *c, double dt_s) {
   return EMCGPU_OK;
 }
 
-emcgpu_status ensure_partial(emcgpu_ctx *c, int k_steps) {
-  size_t bytes = (size_t)k_steps * (size_t)std::max(c->n_tiles, 1) * c->n_obs * sizeof(double);
+// particles per lane of the production kernel when the caller leaves it to
+// the library (measured best on MI355X for the Si bulk workload)
+int default_ppt(int precision) { return precision == EMCGPU_FP32 ? 4 : 2; }
+
+emcgpu_status ensure_partial(emcgpu_ctx *c, int k_steps, int n_tiles) {
+  size_t bytes = (size_t)k_steps * (size_t)std::max(n_tiles, 1) * c->n_obs * sizeof(double);
   if (bytes > c->partial_bytes) {
     dfreet(c->d_partial);
     HIPCHK(c, hipMalloc((void **)&c->d_partial, bytes));
@@ -314,10 +316,9 @@ void fill_common(const emcgpu_ctx *c, StepArgs<R> &a) {
   a.vs = c->vs;
   a.inst = (const InstDev<R> *)c->d_params;
   a.cum = (const R *)c->d_cum;
-  a.tile_inst = c->d_tile_inst;
+  a.unit_inst = c->d_unit_inst;
   a.inst_start = c->d_inst_start;
   a.inst_count = c->d_inst_count;
-  a.n_tiles = c->n_tiles;
   a.n_obs = c->n_obs;
   a.max_v = c->max_v;
   a.max_deg = c->max_deg;
@@ -325,7 +326,7 @@ void fill_common(const emcgpu_ctx *c, StepArgs<R> &a) {
 
 template <typename R>
 emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
-  if (c->n_tiles == 0)
+  if (c->capacity == 0)
     return fail(c, EMCGPU_ERR_STATE, "no particles (call emcgpu_resize first)");
   for (int i = 0; i < c->n_inst; i++) {
     if (c->inst[i].count && !c->inst[i].has_tables)
@@ -336,6 +337,9 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
     return fail(c, EMCGPU_ERR_ARG, "replay mode needs an FP64 context");
   if (replay && (!s->draws || !s->cursor || s->draw_stride == 0))
     return fail(c, EMCGPU_ERR_ARG, "replay mode needs draws, cursor and draw_stride");
+  const int ppl = s->particles_per_lane;
+  if (ppl != 0 && ppl != 1 && ppl != 2 && ppl != 4 && ppl != 8)
+    return fail(c, EMCGPU_ERR_ARG, "particles_per_lane must be 0, 1, 2, 4 or 8 (got %d)", ppl);
   if (!(s->dt > 0) || s->n_steps < 0)
     return fail(c, EMCGPU_ERR_ARG, "dt must be > 0 and n_steps >= 0");
   if (s->n_steps == 0)
@@ -345,11 +349,16 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
     return st;
   int K = s->steps_per_launch > 0 ? s->steps_per_launch : std::min(s->n_steps, 16);
   K = std::min(K, s->n_steps);
-  if ((st = ensure_partial(c, K)) || (st = ensure_obs(c, s->n_steps)))
+  // particles per lane of the production kernel (replay: one per lane)
+  const int ppt = replay ? 1 : (ppl ? ppl : default_ppt(c->precision));
+  const int tile_size = kTile * ppt;
+  const int n_tiles = (int)(c->capacity / tile_size);
+  if ((st = ensure_partial(c, K, n_tiles)) || (st = ensure_obs(c, s->n_steps)))
     return st;
 
   StepArgs<R> a;
   fill_common<R>(c, a);
+  a.n_tiles = n_tiles;
   a.key0 = (uint32_t)s->seed;
   a.key1 = (uint32_t)(s->seed >> 32);
   a.partial = c->d_partial;
@@ -429,7 +438,7 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
     }
     if (s->timing)
       HIPCHK_CLEAN(hipEventRecord(c->ev0, c->stream));
-    HIPCHK_CLEAN(launch_steps<R>(a, replay ? 1 : 0, c->stream));
+    HIPCHK_CLEAN(launch_steps<R>(a, replay ? 1 : 0, ppt, c->stream));
     if (s->timing) {
       HIPCHK_CLEAN(hipEventRecord(c->ev1, c->stream));
       HIPCHK_CLEAN(hipEventSynchronize(c->ev1));
@@ -438,8 +447,8 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
       c->ms_total += ms;
       c->launches += 1;
     }
-    HIPCHK_CLEAN(launch_reduce(c->d_partial, k, c->n_tiles, c->n_obs, c->d_inst_tile0,
-                               c->d_inst_ntiles, c->n_inst, c->d_obs, done, s->n_steps,
+    HIPCHK_CLEAN(launch_reduce(c->d_partial, k, n_tiles, c->n_obs, tile_size, c->d_inst_start,
+                               c->d_inst_count, c->n_inst, c->d_obs, done, s->n_steps,
                                c->stream));
   }
 
@@ -669,11 +678,9 @@ void emcgpu_destroy(emcgpu_ctx *c) {
   dfree(c->E);
   dfree(c->tau);
   dfreet(c->vs);
-  dfreet(c->d_tile_inst);
+  dfreet(c->d_unit_inst);
   dfreet(c->d_inst_start);
   dfreet(c->d_inst_count);
-  dfreet(c->d_inst_tile0);
-  dfreet(c->d_inst_ntiles);
   dfree(c->d_params);
   dfree(c->d_cum);
   dfreet(c->d_partial);
@@ -786,6 +793,13 @@ emcgpu_status emcgpu_set_rate_tables(emcgpu_ctx *c, int instance, int n_levels,
   std::vector<std::vector<double>> raw(in.mechs.size());
   for (size_t j = 0; j < in.mechs.size(); j++)
     raw[j].assign(rates + j * n_levels, rates + (j + 1) * n_levels);
+  // selection relies on non-decreasing cumulative rows
+  for (size_t j = 0; j < raw.size(); j++)
+    for (int l = 0; l < n_levels; l++)
+      if (!(raw[j][l] >= 0.0) || !std::isfinite(raw[j][l]))
+        return fail(c, EMCGPU_ERR_ARG,
+                    "mechanism %zu, level %d: scatter rates must be finite and non-negative (%g)",
+                    j, l, raw[j][l]);
   return finish_tables(c, in, n_levels, max_energy, raw);
 }
 
@@ -907,18 +921,15 @@ emcgpu_status emcgpu_resize(emcgpu_ctx *c, const size_t *counts) {
   if (c->device < 0)
     return fail(c, EMCGPU_ERR_STATE, "host-only context has no device memory");
   (void)hipSetDevice(c->device);
+  // each instance starts on a kUnit boundary (tiles never straddle instances)
   std::vector<long long> start(c->n_inst), cnt(c->n_inst);
-  std::vector<int> tile0(c->n_inst), ntile(c->n_inst);
   long long total = 0;
   for (int i = 0; i < c->n_inst; i++) {
     start[i] = total;
     cnt[i] = (long long)counts[i];
-    tile0[i] = (int)(total / kTile);
-    const long long padded = (cnt[i] + kTile - 1) / kTile * kTile;
-    ntile[i] = (int)(padded / kTile);
-    total += padded;
+    total += (cnt[i] + kUnit - 1) / kUnit * kUnit;
   }
-  if (total / kTile > (1LL << 30))
+  if (total / kTile > (1LL << 30) || total > (long long)UINT32_MAX)
     return fail(c, EMCGPU_ERR_ARG, "too many particles");
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const size_t rs = real_size(c);
@@ -928,13 +939,10 @@ emcgpu_status emcgpu_resize(emcgpu_ctx *c, const size_t *counts) {
   dfree(c->E);
   dfree(c->tau);
   dfreet(c->vs);
-  dfreet(c->d_tile_inst);
+  dfreet(c->d_unit_inst);
   dfreet(c->d_inst_start);
   dfreet(c->d_inst_count);
-  dfreet(c->d_inst_tile0);
-  dfreet(c->d_inst_ntiles);
   c->capacity = total;
-  c->n_tiles = (int)(total / kTile);
   const size_t n = (size_t)std::max<long long>(total, 1);
   HIPCHK(c, hipMalloc(&c->kx, n * rs));
   HIPCHK(c, hipMalloc(&c->ky, n * rs));
@@ -948,24 +956,18 @@ emcgpu_status emcgpu_resize(emcgpu_ctx *c, const size_t *counts) {
   HIPCHK(c, hipMemsetAsync(c->E, 0, n * rs, c->stream));
   HIPCHK(c, hipMemsetAsync(c->tau, 0, n * rs, c->stream));
   HIPCHK(c, hipMemsetAsync(c->vs, 0, n * sizeof(uint32_t), c->stream));
-  std::vector<int> tile_inst((size_t)std::max(c->n_tiles, 1), 0);
+  std::vector<int> unit_inst((size_t)std::max<long long>(total / kUnit, 1), 0);
   for (int i = 0; i < c->n_inst; i++)
-    for (int t = 0; t < ntile[i]; t++)
-      tile_inst[tile0[i] + t] = i;
-  HIPCHK(c, hipMalloc((void **)&c->d_tile_inst, tile_inst.size() * sizeof(int)));
+    for (long long u = start[i] / kUnit; u < (start[i] + cnt[i] + kUnit - 1) / kUnit; u++)
+      unit_inst[u] = i;
+  HIPCHK(c, hipMalloc((void **)&c->d_unit_inst, unit_inst.size() * sizeof(int)));
   HIPCHK(c, hipMalloc((void **)&c->d_inst_start, c->n_inst * sizeof(long long)));
   HIPCHK(c, hipMalloc((void **)&c->d_inst_count, c->n_inst * sizeof(long long)));
-  HIPCHK(c, hipMalloc((void **)&c->d_inst_tile0, c->n_inst * sizeof(int)));
-  HIPCHK(c, hipMalloc((void **)&c->d_inst_ntiles, c->n_inst * sizeof(int)));
-  HIPCHK(c, hipMemcpyAsync(c->d_tile_inst, tile_inst.data(), tile_inst.size() * sizeof(int),
+  HIPCHK(c, hipMemcpyAsync(c->d_unit_inst, unit_inst.data(), unit_inst.size() * sizeof(int),
                            hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_inst_start, start.data(), c->n_inst * sizeof(long long),
                            hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_inst_count, cnt.data(), c->n_inst * sizeof(long long),
-                           hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->d_inst_tile0, tile0.data(), c->n_inst * sizeof(int),
-                           hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->d_inst_ntiles, ntile.data(), c->n_inst * sizeof(int),
                            hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (int i = 0; i < c->n_inst; i++) {

@@ -20,7 +20,12 @@ constexpr int kMaxV = EMCGPU_MAX_VALLEYS;
 constexpr int kMaxDeg = EMCGPU_MAX_DEG;
 constexpr int kMaxZ = EMCGPU_MAX_Z;
 constexpr int kMaxMech = EMCGPU_MAX_MECH;
-constexpr int kTile = 256;  // particles per tile == threads per block
+constexpr int kTile = 256;  // threads per block
+// arena unit: every instance starts on a multiple of kUnit particles, so a
+// kernel tile of 256 * PPT particles (PPT in {1, 2, 4, 8}) never straddles
+// two instances
+constexpr int kUnit = 2048;
+constexpr int kMaxPPT = 8;
 
 // final-state sampler of a mechanism
 enum FinalKind : int {
@@ -71,7 +76,7 @@ struct StepArgs {
   uint32_t *vs;  // valley << 16 | subvalley
   const InstDev<R> *inst;
   const R *cum;
-  const int *tile_inst;
+  const int *unit_inst;  // instance of each kUnit-particle arena unit
   const long long *inst_start;
   const long long *inst_count;
   int n_tiles;
@@ -79,7 +84,7 @@ struct StepArgs {
   int n_steps;   // steps fused in this launch
   uint32_t step0;
   uint32_t key0, key1;
-  double *partial;  // [n_steps][n_tiles][n_obs]
+  double *partial;  // [n_steps][n_obs][n_tiles]
   int n_obs;
   int max_v;
   int max_deg;

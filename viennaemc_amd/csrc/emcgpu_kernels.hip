@@ -68,19 +68,19 @@ struct Math<float> {
 // ---------------------------------------------------------------------------
 // Philox-4x32-10 (Salmon et al., SC'11)
 // ---------------------------------------------------------------------------
+// (the 64-bit products compile to one v_mad_u64_u32 each instead of a
+// v_mul_lo_u32 + v_mul_hi_u32 pair)
 __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
-    const uint32_t lo0 = 0xD2511F53u * c[0];
-    const uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
-    const uint32_t lo1 = 0xCD9E8D57u * c[2];
-    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
-    const uint32_t n0 = hi1 ^ c[1] ^ k0;
-    const uint32_t n2 = hi0 ^ c[3] ^ k1;
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
     c[0] = n0;
-    c[1] = lo1;
+    c[1] = (uint32_t)p1;
     c[2] = n2;
-    c[3] = lo0;
+    c[3] = (uint32_t)p0;
     k0 += 0x9E3779B9u;
     k1 += 0xBB67AE85u;
   }
@@ -322,104 +322,118 @@ struct StepRng<R, 1> {  // replay
   ReplayStream rs;
 };
 
+// free flight of duration t in the current valley/subvalley
+// (emcParticleDrift.hpp:11-35: k += F t vogt / hbar in the ellipse frame,
+// precomputed here as one device-frame acceleration per subvalley)
+template <typename R>
+__device__ __forceinline__ void drift(const InstDev<R> *P, R t, R &kx, R &ky, R &kz, R &E, int v,
+                                      int sub) {
+  const ValleyDev<R> &V = P->valley[v];
+  kx += V.accel[sub][0] * t;
+  ky += V.accel[sub][1] * t;
+  kz += V.accel[sub][2] * t;
+  E = energy_of(V, kx, ky, kz);
+}
+
+// one pass of the scatter loop of moveParticles: scatter (selection +
+// final state), draw tau', drift min(rem, tau'), rem -= tau'
+// (basicBulkParticleHandler.hpp:203-212); e counts the events of the step
+template <typename R, int MODE>
+__device__ __forceinline__ void event_once(const InstDev<R> *P, const R *__restrict__ cum, R &kx,
+                                           R &ky, R &kz, R &E, R &tau, R &rem, int &v, int &sub,
+                                           uint32_t &e, StepRng<R, MODE> &rng, int &nreal) {
+  const int nt = P->n_tables[v];
+  R u[4];
+  if constexpr (MODE == 0)
+    EventDraws<R>::get(e, rng.step, rng.pid, rng.inst, rng.k0, rng.k1, u);
+  ++e;
+  if (nt > 0) {
+    const int level = energy_level(P, E);
+    const R *row = cum + P->table_off[v] + (long long)level * nt;
+    R usel;
+    if constexpr (MODE == 0)
+      usel = u[0];
+    else
+      usel = (R)rng.rs.uniform(0., 1.);
+    // emcScatterHandler::scatterParticle (:148-170): self scattering if
+    // usel > cum[nt-1], else the first t with cum[t-1] <= usel < cum[t]
+    // (the early exit beats a branch-free count of all nt entries: most
+    // selections end in the first few, high-rate tables)
+    if (!(usel > row[nt - 1])) {
+      R lower = R(0), upper = R(0);
+      int j = -1;
+      for (int t = 0; t < nt; ++t) {
+        upper = row[t];
+        if (usel >= lower && usel < upper) {
+          j = t;
+          break;
+        }
+        lower = upper;
+      }
+      if (j >= 0) {
+        const MechDev<R> &m = P->mech[v][j];
+        R ua, ub;
+        int zidx = 0;
+        if constexpr (MODE == 0) {
+          ua = u[1];
+          ub = u[2];
+          if (m.final_kind == kFinalIsotropicInelastic) {
+            // the conditional uniform of the selection draw picks the
+            // final subvalley (statistically the reference's rng() % Z)
+            const R uc = Math<R>::div(usel - lower, upper - lower);
+            zidx = (int)(uc * (R)m.z);
+            zidx = zidx < m.z ? zidx : m.z - 1;
+          }
+        } else {
+          // reference draw order (g++ evaluates call arguments right to
+          // left): IV raw subvalley draw, then cos(theta), then phi;
+          // Froehlich: exponent, then phi
+          if (m.final_kind == kFinalIsotropicElastic) {
+            ub = (R)rng.rs.uniform(0., 1.);
+            ua = (R)rng.rs.uniform(0., 1.);
+          } else if (m.final_kind == kFinalIsotropicInelastic) {
+            zidx = (int)(rng.rs.raw() % (uint64_t)m.z);
+            ub = (R)rng.rs.uniform(0., 1.);
+            ua = (R)rng.rs.uniform(0., 1.);
+          } else {
+            ua = (R)rng.rs.uniform(0., 1.);
+            ub = (R)rng.rs.uniform(0., 1.);
+          }
+        }
+        final_state(P, m, kx, ky, kz, E, v, sub, ua, ub, zidx);
+        ++nreal;
+      }
+    }
+  }
+  // getNewTau (emcParticleType.hpp:187-189): -ln U[1e-6, 1) * tau0
+  R ut;
+  if constexpr (MODE == 0) {
+    ut = u[3] * R(1 - 1e-6) + R(1e-6);
+    ut = (ut < UnitBelow<R>::value) ? ut : UnitBelow<R>::value;
+  } else {
+    ut = (R)rng.rs.uniform(1e-6, 1.);
+  }
+  const R newTau = -Math<R>::log(ut) * P->tau0[v];
+  tau += newTau;
+  drift(P, (newTau < rem) ? newTau : rem, kx, ky, kz, E, v, sub);  // std::min(rem, newTau)
+  rem -= newTau;
+}
+
+// one particle, one step: drift min(tau, dt), then the scatter loop
+// (basicBulkParticleHandler.hpp:195-214)
 template <typename R, int MODE>
 __device__ __forceinline__ void move_one(const InstDev<R> *P, const R *__restrict__ cum, R &kx,
                                          R &ky, R &kz, R &E, R &tau, int &v, int &sub,
                                          StepRng<R, MODE> &rng, int &nev, int &nreal) {
   const R dt = P->dt;
-  {
-    const ValleyDev<R> &V = P->valley[v];
-    const R t = (dt < tau) ? dt : tau;  // std::min(tau, dt)
-    kx += V.accel[sub][0] * t;
-    ky += V.accel[sub][1] * t;
-    kz += V.accel[sub][2] * t;
-    E = energy_of(V, kx, ky, kz);
-  }
+  drift(P, (dt < tau) ? dt : tau, kx, ky, kz, E, v, sub);  // std::min(tau, dt)
   R rem = dt - tau;
   uint32_t e = 0;
   // every pass draws tau' > 0 (the uniform is clamped below 1), so rem
   // strictly decreases; the event cap bounds the loop regardless
-  while (rem > R(0) && e < kMaxEventsPerStep) {
-    const int nt = P->n_tables[v];
-    R u[4];
-    bool have = false;
-    if constexpr (MODE == 0) {
-      EventDraws<R>::get(e, rng.step, rng.pid, rng.inst, rng.k0, rng.k1, u);
-      have = true;
-    }
-    ++e;
-    ++nev;
-    if (nt > 0) {
-      const int level = energy_level(P, E);
-      const R *row = cum + P->table_off[v] + (long long)level * nt;
-      R usel;
-      if constexpr (MODE == 0)
-        usel = u[0];
-      else
-        usel = (R)rng.rs.uniform(0., 1.);
-      if (!(usel > row[nt - 1])) {
-        R lower = R(0), upper = R(0);
-        int j = -1;
-        for (int t = 0; t < nt; ++t) {
-          upper = row[t];
-          if (usel >= lower && usel < upper) {
-            j = t;
-            break;
-          }
-          lower = upper;
-        }
-        if (j >= 0) {
-          const MechDev<R> &m = P->mech[v][j];
-          R ua, ub;
-          int zidx = 0;
-          if constexpr (MODE == 0) {
-            ua = u[1];
-            ub = u[2];
-            if (m.final_kind == kFinalIsotropicInelastic) {
-              const R uc = Math<R>::div(usel - lower, upper - lower);
-              zidx = (int)(uc * (R)m.z);
-              zidx = zidx < m.z ? zidx : m.z - 1;
-            }
-          } else {
-            // reference draw order (g++ evaluates call arguments right to
-            // left): IV raw subvalley draw, then cos(theta), then phi;
-            // Froehlich: exponent, then phi
-            if (m.final_kind == kFinalIsotropicElastic) {
-              ub = (R)rng.rs.uniform(0., 1.);
-              ua = (R)rng.rs.uniform(0., 1.);
-            } else if (m.final_kind == kFinalIsotropicInelastic) {
-              zidx = (int)(rng.rs.raw() % (uint64_t)m.z);
-              ub = (R)rng.rs.uniform(0., 1.);
-              ua = (R)rng.rs.uniform(0., 1.);
-            } else {
-              ua = (R)rng.rs.uniform(0., 1.);
-              ub = (R)rng.rs.uniform(0., 1.);
-            }
-          }
-          final_state(P, m, kx, ky, kz, E, v, sub, ua, ub, zidx);
-          ++nreal;
-        }
-      }
-    }
-    // getNewTau (emcParticleType.hpp:187-189): -ln U[1e-6, 1) * tau0
-    R ut;
-    if constexpr (MODE == 0) {
-      ut = u[3] * R(1 - 1e-6) + R(1e-6);
-      ut = (ut < UnitBelow<R>::value) ? ut : UnitBelow<R>::value;
-    } else {
-      ut = (R)rng.rs.uniform(1e-6, 1.);
-    }
-    (void)have;
-    const R newTau = -Math<R>::log(ut) * P->tau0[v];
-    tau += newTau;
-    const ValleyDev<R> &V = P->valley[v];
-    const R t = (newTau < rem) ? newTau : rem;  // std::min(rem, newTau)
-    kx += V.accel[sub][0] * t;
-    ky += V.accel[sub][1] * t;
-    kz += V.accel[sub][2] * t;
-    E = energy_of(V, kx, ky, kz);
-    rem -= newTau;
-  }
+  while (rem > R(0) && e < kMaxEventsPerStep)
+    event_once<R, MODE>(P, cum, kx, ky, kz, E, tau, rem, v, sub, e, rng, nreal);
+  nev += (int)e;
   tau -= dt;
 }
 
@@ -439,175 +453,323 @@ __device__ __forceinline__ int wave_sum_i(int x) {
   return x;
 }
 
-// ---------------------------------------------------------------------------
-// the step kernel
-// ---------------------------------------------------------------------------
-template <typename R, int MODE>
-__global__ __launch_bounds__(kTile) void k_bulk_steps(StepArgs<R> a) {
-  __shared__ double wsum[kTile / 64][2 + kMaxV * (3 + kMaxDeg)];
-  const int tile = blockIdx.x;
-  const int inst = a.tile_inst[tile];
-  const InstDev<R> *P = a.inst + inst;
-  const long long base = a.inst_start[inst];
-  const long long local = (long long)tile * kTile + threadIdx.x - base;
-  const bool valid = local < a.inst_count[inst];
-  const long long gi = base + local;
+constexpr int kObsMax = 2 + kMaxV * (3 + kMaxDeg);
+
+// Per-step ensemble sums of one tile (each lane holds PPT particles): event
+// counts, per-valley count / sum E / sum v.dir, per-(valley, subvalley)
+// counts.  Lanes sum their own particles in FP64, waves reduce with
+// shuffles and ballots, the block's waves are added in fixed order, and the
+// tile writes one FP64 partial per value: partial[step][obs][tile].
+template <typename R, int PPT>
+__device__ __forceinline__ void tile_observables(const StepArgs<R> &a, const InstDev<R> *P,
+                                                 double (*wsum)[kObsMax], int tile, int s,
+                                                 const bool (&valid)[PPT], const int (&v)[PPT],
+                                                 const int (&sub)[PPT], const R (&kx)[PPT],
+                                                 const R (&ky)[PPT], const R (&kz)[PPT],
+                                                 const R (&E)[PPT], int nev, int nreal) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-
-  R kx = 0, ky = 0, kz = 0, E = 0, tau = 0;
-  int v = 0, sub = 0;
-  if (valid) {
-    kx = a.kx[gi];
-    ky = a.ky[gi];
-    kz = a.kz[gi];
-    E = a.E[gi];
-    tau = a.tau[gi];
-    const uint32_t vs = a.vs[gi];
-    v = (int)(vs >> 16);
-    sub = (int)(vs & 0xffffu);
-  }
-  StepRng<R, MODE> rng;
-  double grain = 1.0;
-  if constexpr (MODE == 0) {
-    rng.pid = (uint32_t)local;
-    rng.inst = (uint32_t)inst;
-    rng.k0 = a.key0;
-    rng.k1 = a.key1;
-  } else {
-    rng.rs.p = a.draws + (valid ? local : 0) * a.draw_stride;
-    rng.rs.len = a.draw_stride;
-    rng.rs.pos = valid ? (long long)a.cursor[local] : 0;
-    rng.rs.over = false;
-    if (valid && a.grain_tau)
-      grain = a.grain_tau[local];
-  }
-
   const int nv = P->n_valleys;
-  for (int s = 0; s < a.n_steps; ++s) {
-    int nev = 0, nreal = 0;
-    if constexpr (MODE == 0)
-      rng.step = a.step0 + (uint32_t)s;
-    if (valid) {
-      move_one<R, MODE>(P, a.cum, kx, ky, kz, E, tau, v, sub, rng, nev, nreal);
-      if constexpr (MODE == 1) {
-        // grain bookkeeping (basicBulkParticleHandler.hpp:213-220), no
-        // grain mechanism: a new grainTau of -ln U[1e-6,1) * 1 s
-        grain -= a.dt_s;
-        if (grain <= 0)
-          grain = -::log(rng.rs.uniform(1e-6, 1.)) * 1.0;
-      }
-      if (a.tr_valley && inst == 0) {
-        const long long o = (long long)s * a.tr_n + local;
-        a.tr_valley[o] = v;
-        a.tr_sub[o] = sub;
-        a.tr_events[o] = nev;
-        a.tr_k[3 * o] = (double)kx;
-        a.tr_k[3 * o + 1] = (double)ky;
-        a.tr_k[3 * o + 2] = (double)kz;
-        a.tr_E[o] = (double)E;
-      }
-    }
-    // ---- per-step ensemble sums of this tile ----
-    const int ev = wave_sum_i(nev);
-    const int re = wave_sum_i(nreal);
-    double *ws = wsum[wave];
-    if (lane == 0) {
-      ws[0] = (double)ev;
-      ws[1] = (double)re;
-    }
-    for (int vv = 0; vv < a.max_v; ++vv) {
-      const bool in = valid && v == vv && vv < nv;
-      double se = 0.0, sv = 0.0;
-      if (vv < nv) {
-        const double de = in ? (double)E : 0.0;
-        const double dv = in ? (double)vdir_of(P->valley[vv], sub, kx, ky, kz, E) : 0.0;
-        se = wave_sum(de);
-        sv = wave_sum(dv);
-      }
-      const unsigned long long bal = __ballot(in);
-      if (lane == 0) {
-        ws[2 + 3 * vv] = (double)__popcll(bal);
-        ws[3 + 3 * vv] = se;
-        ws[4 + 3 * vv] = sv;
-      }
-      for (int ss = 0; ss < a.max_deg; ++ss) {
-        const unsigned long long b2 = __ballot(in && sub == ss);
-        if (lane == 0)
-          ws[2 + 3 * a.max_v + vv * a.max_deg + ss] = (double)__popcll(b2);
-      }
-    }
-    __syncthreads();
-    if (threadIdx.x < a.n_obs) {
-      double t = 0.0;
+  const int ev = wave_sum_i(nev);
+  const int re = wave_sum_i(nreal);
+  double *ws = wsum[wave];
+  if (lane == 0) {
+    ws[0] = (double)ev;
+    ws[1] = (double)re;
+  }
+  for (int vv = 0; vv < a.max_v; ++vv) {
+    double se = 0.0, sv = 0.0;
+    int cnt = 0;
+    if (vv < nv) {
+      double de = 0.0, dv = 0.0;
 #pragma unroll
-      for (int w = 0; w < kTile / 64; ++w)
-        t += wsum[w][threadIdx.x];
-      a.partial[((long long)s * a.n_tiles + tile) * a.n_obs + threadIdx.x] = t;
+      for (int j = 0; j < PPT; ++j) {
+        const bool in = valid[j] && v[j] == vv;
+        de += in ? (double)E[j] : 0.0;
+        dv += in ? (double)vdir_of(P->valley[vv], sub[j], kx[j], ky[j], kz[j], E[j]) : 0.0;
+        cnt += (int)__popcll(__ballot(in));
+      }
+      se = wave_sum(de);
+      sv = wave_sum(dv);
     }
-    __syncthreads();
+    if (lane == 0) {
+      ws[2 + 3 * vv] = (double)cnt;
+      ws[3 + 3 * vv] = se;
+      ws[4 + 3 * vv] = sv;
+    }
+    for (int ss = 0; ss < a.max_deg; ++ss) {
+      int c2 = 0;
+#pragma unroll
+      for (int j = 0; j < PPT; ++j)
+        c2 += (int)__popcll(__ballot(valid[j] && v[j] == vv && sub[j] == ss));
+      if (lane == 0)
+        ws[2 + 3 * a.max_v + vv * a.max_deg + ss] = (double)c2;
+    }
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < a.n_obs) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kTile / 64; ++w)
+      t += wsum[w][threadIdx.x];
+    a.partial[((long long)s * a.n_obs + threadIdx.x) * a.n_tiles + tile] = t;
+  }
+  __syncthreads();
+}
+
+// instance of a kernel tile of tile_size particles (tiles never straddle
+// instances: instances start on kUnit boundaries and kUnit % tile_size == 0)
+__device__ __forceinline__ int tile_instance(const int *unit_inst, long long tile, int tile_size) {
+  return unit_inst[(tile * tile_size) / kUnit];
+}
+
+// ---------------------------------------------------------------------------
+// production step kernel: PPT particles per lane
+//
+// Each lane owns PPT particles of a tile of 256*PPT (particle j of lane t is
+// tile_first + j*256 + t, so every load and store is a contiguous 256-wide
+// access).  Per step all PPT particles drift min(tau, dt); then the lane
+// works through its own pending scatter events one at a time, pulling the
+// state of the next particle that still has time left into working
+// registers.  A wavefront therefore runs the scatter body while any lane
+// has any pending event among its PPT particles, which keeps most lanes
+// busy in the body instead of ~20 % (one particle per lane at ~0.24 events
+// per step).  No barriers are involved in the event loop.
+// ---------------------------------------------------------------------------
+template <typename R, int PPT>
+__global__ __launch_bounds__(kTile) void k_bulk_multi(StepArgs<R> a) {
+  __shared__ double wsum[kTile / 64][kObsMax];
+  constexpr int TS = kTile * PPT;
+  const int tile = blockIdx.x;
+  const int inst = tile_instance(a.unit_inst, tile, TS);
+  const InstDev<R> *P = a.inst + inst;
+  const long long base = a.inst_start[inst];
+  const long long count = a.inst_count[inst];
+  const long long first = (long long)tile * TS - base + threadIdx.x;  // local index, j = 0
+
+  R kx[PPT], ky[PPT], kz[PPT], E[PPT], tau[PPT], rem[PPT];
+  int v[PPT], sub[PPT];
+  uint32_t ev[PPT];
+  bool valid[PPT];
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) {
+    const long long local = first + (long long)j * kTile;
+    valid[j] = local < count;
+    kx[j] = ky[j] = kz[j] = E[j] = tau[j] = rem[j] = R(0);
+    v[j] = sub[j] = 0;
+    ev[j] = 0;
+    if (valid[j]) {
+      const long long gi = base + local;
+      kx[j] = a.kx[gi];
+      ky[j] = a.ky[gi];
+      kz[j] = a.kz[gi];
+      E[j] = a.E[gi];
+      tau[j] = a.tau[gi];
+      const uint32_t vs = a.vs[gi];
+      v[j] = (int)(vs >> 16);
+      sub[j] = (int)(vs & 0xffffu);
+    }
+  }
+  StepRng<R, 0> rng;
+  rng.inst = (uint32_t)inst;
+  rng.k0 = a.key0;
+  rng.k1 = a.key1;
+  const R dt = P->dt;
+
+  for (int s = 0; s < a.n_steps; ++s) {
+    rng.step = a.step0 + (uint32_t)s;
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      ev[j] = 0;
+      rem[j] = R(0);
+      if (valid[j]) {
+        drift(P, (dt < tau[j]) ? dt : tau[j], kx[j], ky[j], kz[j], E[j], v[j], sub[j]);
+        rem[j] = dt - tau[j];  // std::min(tau, dt) above, then tRemaining
+      }
+    }
+    int nreal = 0;
+    for (;;) {
+      // next particle of this lane with time left in the step (lowest j)
+      int sel = -1;
+#pragma unroll
+      for (int j = PPT - 1; j >= 0; --j)
+        if (valid[j] && rem[j] > R(0) && ev[j] < kMaxEventsPerStep)
+          sel = j;
+      if (sel < 0)
+        break;
+      R x = kx[0], y = ky[0], z = kz[0], en = E[0], ta = tau[0], rm = rem[0];
+      int vv = v[0], ss = sub[0];
+      uint32_t ee = ev[0];
+#pragma unroll
+      for (int j = 1; j < PPT; ++j)
+        if (sel == j) {
+          x = kx[j];
+          y = ky[j];
+          z = kz[j];
+          en = E[j];
+          ta = tau[j];
+          rm = rem[j];
+          vv = v[j];
+          ss = sub[j];
+          ee = ev[j];
+        }
+      rng.pid = (uint32_t)(first + (long long)sel * kTile);
+      event_once<R, 0>(P, a.cum, x, y, z, en, ta, rm, vv, ss, ee, rng, nreal);
+#pragma unroll
+      for (int j = 0; j < PPT; ++j)
+        if (sel == j) {
+          kx[j] = x;
+          ky[j] = y;
+          kz[j] = z;
+          E[j] = en;
+          tau[j] = ta;
+          rem[j] = rm;
+          v[j] = vv;
+          sub[j] = ss;
+          ev[j] = ee;
+        }
+    }
+    int nev = 0;
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      nev += (int)ev[j];
+      if (valid[j])
+        tau[j] -= dt;
+    }
+    tile_observables<R, PPT>(a, P, wsum, tile, s, valid, v, sub, kx, ky, kz, E, nev, nreal);
   }
 
-  if (valid) {
-    a.kx[gi] = kx;
-    a.ky[gi] = ky;
-    a.kz[gi] = kz;
-    a.E[gi] = E;
-    a.tau[gi] = tau;
-    a.vs[gi] = ((uint32_t)v << 16) | (uint32_t)sub;
-    if constexpr (MODE == 1) {
-      a.cursor[local] = (uint64_t)rng.rs.pos;
-      if (a.grain_tau)
-        a.grain_tau[local] = grain;
-      if (rng.rs.over)
-        atomicAdd(a.overflow, 1);
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) {
+    if (valid[j]) {
+      const long long gi = base + first + (long long)j * kTile;
+      a.kx[gi] = kx[j];
+      a.ky[gi] = ky[j];
+      a.kz[gi] = kz[j];
+      a.E[gi] = E[j];
+      a.tau[gi] = tau[j];
+      a.vs[gi] = ((uint32_t)v[j] << 16) | (uint32_t)sub[j];
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// per-(instance, step) Kahan / TwoSum reduction of the tile partials
+// replay step kernel (FP64): one particle per lane, the reference engine's
+// recorded draws consumed in the reference's order, optional per-step trace
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(kTile) void k_bulk_replay(StepArgs<R> a) {
+  __shared__ double wsum[kTile / 64][kObsMax];
+  const int tile = blockIdx.x;
+  const int inst = tile_instance(a.unit_inst, tile, kTile);
+  const InstDev<R> *P = a.inst + inst;
+  const long long base = a.inst_start[inst];
+  const long long local = (long long)tile * kTile + threadIdx.x - base;
+  bool valid[1] = {local < a.inst_count[inst]};
+  const long long gi = base + local;
+
+  R kx[1] = {0}, ky[1] = {0}, kz[1] = {0}, E[1] = {0}, tau = 0;
+  int v[1] = {0}, sub[1] = {0};
+  if (valid[0]) {
+    kx[0] = a.kx[gi];
+    ky[0] = a.ky[gi];
+    kz[0] = a.kz[gi];
+    E[0] = a.E[gi];
+    tau = a.tau[gi];
+    const uint32_t vs = a.vs[gi];
+    v[0] = (int)(vs >> 16);
+    sub[0] = (int)(vs & 0xffffu);
+  }
+  StepRng<R, 1> rng;
+  rng.rs.p = a.draws + (valid[0] ? local : 0) * a.draw_stride;
+  rng.rs.len = a.draw_stride;
+  rng.rs.pos = valid[0] ? (long long)a.cursor[local] : 0;
+  rng.rs.over = false;
+  double grain = (valid[0] && a.grain_tau) ? a.grain_tau[local] : 1.0;
+
+  for (int s = 0; s < a.n_steps; ++s) {
+    int nev = 0, nreal = 0;
+    if (valid[0]) {
+      move_one<R, 1>(P, a.cum, kx[0], ky[0], kz[0], E[0], tau, v[0], sub[0], rng, nev, nreal);
+      // grain bookkeeping (basicBulkParticleHandler.hpp:213-220), no grain
+      // mechanism: a new grainTau of -ln U[1e-6,1) * 1 s
+      grain -= a.dt_s;
+      if (grain <= 0)
+        grain = -::log(rng.rs.uniform(1e-6, 1.)) * 1.0;
+      if (a.tr_valley && inst == 0) {
+        const long long o = (long long)s * a.tr_n + local;
+        a.tr_valley[o] = v[0];
+        a.tr_sub[o] = sub[0];
+        a.tr_events[o] = nev;
+        a.tr_k[3 * o] = (double)kx[0];
+        a.tr_k[3 * o + 1] = (double)ky[0];
+        a.tr_k[3 * o + 2] = (double)kz[0];
+        a.tr_E[o] = (double)E[0];
+      }
+    }
+    tile_observables<R, 1>(a, P, wsum, tile, s, valid, v, sub, kx, ky, kz, E, nev, nreal);
+  }
+
+  if (valid[0]) {
+    a.kx[gi] = kx[0];
+    a.ky[gi] = ky[0];
+    a.kz[gi] = kz[0];
+    a.E[gi] = E[0];
+    a.tau[gi] = tau;
+    a.vs[gi] = ((uint32_t)v[0] << 16) | (uint32_t)sub[0];
+    a.cursor[local] = (uint64_t)rng.rs.pos;
+    if (a.grain_tau)
+      a.grain_tau[local] = grain;
+    if (rng.rs.over)
+      atomicAdd(a.overflow, 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// per-(step, observable, instance) compensated reduction of the tile partials
+// partial layout [step][obs][tile]: one block per output value, coalesced
+// loads, Neumaier summation per thread, TwoSum tree across the block
+// (deterministic: fixed order for a given tile count)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_reduce_obs(const double *__restrict__ partial,
-                                                    int n_steps, int n_tiles, int n_obs,
-                                                    const int *__restrict__ inst_tile0,
-                                                    const int *__restrict__ inst_ntiles,
+                                                    int n_tiles, int n_obs, int tile_size,
+                                                    const long long *__restrict__ inst_start,
+                                                    const long long *__restrict__ inst_count,
                                                     int n_inst, double *__restrict__ out,
                                                     int out_step0, int out_total) {
   __shared__ double ss[256], sc[256];
-  const int s = blockIdx.x / n_inst;
   const int inst = blockIdx.x % n_inst;
-  const int t0 = inst_tile0[inst], nt = inst_ntiles[inst];
-  for (int o = 0; o < n_obs; ++o) {
-    double sum = 0.0, comp = 0.0;
-    for (int t = threadIdx.x; t < nt; t += 256) {
-      const double x = partial[((long long)s * n_tiles + t0 + t) * n_obs + o];
-      // Kahan-Babuska (Neumaier) step
-      const double y = sum + x;
-      if (fabs(sum) >= fabs(x))
-        comp += (sum - y) + x;
-      else
-        comp += (x - y) + sum;
-      sum = y;
+  const int so = blockIdx.x / n_inst;  // s * n_obs + o
+  const int s = so / n_obs, o = so % n_obs;
+  const long long t0 = inst_start[inst] / tile_size;
+  const long long nt = (inst_count[inst] + tile_size - 1) / tile_size;
+  const double *src = partial + (long long)so * n_tiles + t0;
+  double sum = 0.0, comp = 0.0;
+  for (long long t = threadIdx.x; t < nt; t += 256) {
+    const double x = src[t];
+    const double y = sum + x;
+    if (fabs(sum) >= fabs(x))
+      comp += (sum - y) + x;
+    else
+      comp += (x - y) + sum;
+    sum = y;
+  }
+  ss[threadIdx.x] = sum;
+  sc[threadIdx.x] = comp;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      const double x = ss[threadIdx.x], y0 = ss[threadIdx.x + w];
+      const double y = x + y0;
+      const double bb = y - x;
+      const double err = (x - (y - bb)) + (y0 - bb);  // TwoSum
+      ss[threadIdx.x] = y;
+      sc[threadIdx.x] += sc[threadIdx.x + w] + err;
     }
-    ss[threadIdx.x] = sum;
-    sc[threadIdx.x] = comp;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if (threadIdx.x < w) {
-        const double a = ss[threadIdx.x], b = ss[threadIdx.x + w];
-        const double y = a + b;
-        const double bb = y - a;
-        const double err = (a - (y - bb)) + (b - bb);  // TwoSum
-        ss[threadIdx.x] = y;
-        sc[threadIdx.x] += sc[threadIdx.x + w] + err;
-      }
-      __syncthreads();
-    }
-    if (threadIdx.x == 0)
-      out[((long long)inst * out_total + out_step0 + s) * n_obs + o] = ss[0] + sc[0];
     __syncthreads();
   }
+  if (threadIdx.x == 0)
+    out[((long long)inst * out_total + out_step0 + s) * n_obs + o] = ss[0] + sc[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -615,11 +777,11 @@ __global__ __launch_bounds__(256) void k_reduce_obs(const double *__restrict__ p
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ __launch_bounds__(kTile) void k_init_thermal(StepArgs<R> a) {
-  const int tile = a.tile_base + blockIdx.x;
-  const int inst = a.tile_inst[tile];
+  const long long tile = (long long)a.tile_base + blockIdx.x;
+  const int inst = tile_instance(a.unit_inst, tile, kTile);
   const InstDev<R> *P = a.inst + inst;
   const long long base = a.inst_start[inst];
-  const long long local = (long long)tile * kTile + threadIdx.x - base;
+  const long long local = tile * kTile + threadIdx.x - base;
   if (local >= a.inst_count[inst])
     return;
   const long long gi = base + local;
@@ -650,14 +812,34 @@ __global__ __launch_bounds__(kTile) void k_init_thermal(StepArgs<R> a) {
 // ---------------------------------------------------------------------------
 // launch wrappers (called by the C ABI layer)
 // ---------------------------------------------------------------------------
+// particles per tile of a step-kernel variant (1 = replay / one per lane)
+int tile_size_of(int ppt) { return kTile * ppt; }
+
 template <typename R>
-hipError_t launch_steps(const StepArgs<R> &a, int mode, hipStream_t st) {
+hipError_t launch_steps(const StepArgs<R> &a, int mode, int ppt, hipStream_t st) {
   if (a.n_tiles <= 0)
     return hipSuccess;
-  if (mode == 0)
-    hipLaunchKernelGGL((k_bulk_steps<R, 0>), dim3(a.n_tiles), dim3(kTile), 0, st, a);
-  else
-    hipLaunchKernelGGL((k_bulk_steps<R, 1>), dim3(a.n_tiles), dim3(kTile), 0, st, a);
+  const dim3 g(a.n_tiles), b(kTile);
+  if (mode == 1) {
+    hipLaunchKernelGGL((k_bulk_replay<R>), g, b, 0, st, a);
+  } else {
+    switch (ppt) {
+    case 1:
+      hipLaunchKernelGGL((k_bulk_multi<R, 1>), g, b, 0, st, a);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_bulk_multi<R, 2>), g, b, 0, st, a);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_bulk_multi<R, 4>), g, b, 0, st, a);
+      break;
+    case 8:
+      hipLaunchKernelGGL((k_bulk_multi<R, 8>), g, b, 0, st, a);
+      break;
+    default:
+      return hipErrorInvalidValue;
+    }
+  }
   return hipGetLastError();
 }
 
@@ -669,18 +851,19 @@ hipError_t launch_init(const StepArgs<R> &a, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_reduce(const double *partial, int n_steps, int n_tiles, int n_obs,
-                         const int *inst_tile0, const int *inst_ntiles, int n_inst, double *out,
-                         int out_step0, int out_total, hipStream_t st) {
+hipError_t launch_reduce(const double *partial, int n_steps, int n_tiles, int n_obs, int tile_size,
+                         const long long *inst_start, const long long *inst_count, int n_inst,
+                         double *out, int out_step0, int out_total, hipStream_t st) {
   if (n_steps <= 0 || n_inst <= 0)
     return hipSuccess;
-  hipLaunchKernelGGL(k_reduce_obs, dim3(n_steps * n_inst), dim3(256), 0, st, partial, n_steps,
-                     n_tiles, n_obs, inst_tile0, inst_ntiles, n_inst, out, out_step0, out_total);
+  hipLaunchKernelGGL(k_reduce_obs, dim3(n_steps * n_obs * n_inst), dim3(256), 0, st, partial,
+                     n_tiles, n_obs, tile_size, inst_start, inst_count, n_inst, out, out_step0,
+                     out_total);
   return hipGetLastError();
 }
 
-template hipError_t launch_steps<float>(const StepArgs<float> &, int, hipStream_t);
-template hipError_t launch_steps<double>(const StepArgs<double> &, int, hipStream_t);
+template hipError_t launch_steps<float>(const StepArgs<float> &, int, int, hipStream_t);
+template hipError_t launch_steps<double>(const StepArgs<double> &, int, int, hipStream_t);
 template hipError_t launch_init<float>(const StepArgs<float> &, hipStream_t);
 template hipError_t launch_init<double>(const StepArgs<double> &, hipStream_t);
 
