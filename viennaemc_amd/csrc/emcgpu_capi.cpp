@@ -25,9 +25,11 @@ template <typename R>
 hipError_t launch_steps(const StepArgs<R> &a, int mode, int ppt, hipStream_t st);
 template <typename R>
 hipError_t launch_init(const StepArgs<R> &a, hipStream_t st);
+int reduce_chunks(long long max_count, int tile_size);
 hipError_t launch_reduce(const double *partial, int n_steps, int n_tiles, int n_obs, int tile_size,
                          const long long *inst_start, const long long *inst_count, int n_inst,
-                         double *out, int out_step0, int out_total, hipStream_t st);
+                         int n_chunks, double *chunk_buf, double *out, int out_step0,
+                         int out_total, hipStream_t st);
 }  // namespace emcgpu
 
 using namespace emcgpu;
@@ -80,6 +82,8 @@ struct emcgpu_ctx {
   int max_v = 1, max_deg = 1, n_obs = 0;
   double *d_partial = nullptr;
   size_t partial_bytes = 0;
+  double *d_chunks = nullptr;  // (sum, compensation) pairs of the first reduction pass
+  size_t chunk_bytes = 0;
   double *d_obs = nullptr;
   size_t obs_bytes = 0;
   int obs_steps = 0;
@@ -282,7 +286,7 @@ emcgpu_status upload_params(emcgpu_ctx *c, double dt_s) {
 
 // particles per lane of the production kernel when the caller leaves it to
 // the library (measured best on MI355X for the Si bulk workload)
-int default_ppt(int precision) { return precision == EMCGPU_FP32 ? 4 : 2; }
+int default_ppt(int) { return 4; }
 
 emcgpu_status ensure_partial(emcgpu_ctx *c, int k_steps, int n_tiles) {
   size_t bytes = (size_t)k_steps * (size_t)std::max(n_tiles, 1) * c->n_obs * sizeof(double);
@@ -347,7 +351,7 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
   emcgpu_status st = upload_params<R>(c, s->dt);
   if (st)
     return st;
-  int K = s->steps_per_launch > 0 ? s->steps_per_launch : std::min(s->n_steps, 16);
+  int K = s->steps_per_launch > 0 ? s->steps_per_launch : std::min(s->n_steps, 64);
   K = std::min(K, s->n_steps);
   // particles per lane of the production kernel (replay: one per lane)
   const int ppt = replay ? 1 : (ppl ? ppl : default_ppt(c->precision));
@@ -355,6 +359,18 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
   const int n_tiles = (int)(c->capacity / tile_size);
   if ((st = ensure_partial(c, K, n_tiles)) || (st = ensure_obs(c, s->n_steps)))
     return st;
+  long long max_count = 0;
+  for (const auto &in : c->inst)
+    max_count = std::max<long long>(max_count, (long long)in.count);
+  const int n_chunks = reduce_chunks(max_count, tile_size);
+  {
+    const size_t bytes = (size_t)c->n_inst * K * c->n_obs * n_chunks * 2 * sizeof(double);
+    if (bytes > c->chunk_bytes) {
+      dfreet(c->d_chunks);
+      HIPCHK(c, hipMalloc((void **)&c->d_chunks, bytes));
+      c->chunk_bytes = bytes;
+    }
+  }
 
   StepArgs<R> a;
   fill_common<R>(c, a);
@@ -448,8 +464,8 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
       c->launches += 1;
     }
     HIPCHK_CLEAN(launch_reduce(c->d_partial, k, n_tiles, c->n_obs, tile_size, c->d_inst_start,
-                               c->d_inst_count, c->n_inst, c->d_obs, done, s->n_steps,
-                               c->stream));
+                               c->d_inst_count, c->n_inst, n_chunks, c->d_chunks, c->d_obs, done,
+                               s->n_steps, c->stream));
   }
 
   if (replay) {
@@ -684,6 +700,7 @@ void emcgpu_destroy(emcgpu_ctx *c) {
   dfree(c->d_params);
   dfree(c->d_cum);
   dfreet(c->d_partial);
+  dfreet(c->d_chunks);
   dfreet(c->d_obs);
   if (c->ev0)
     (void)hipEventDestroy(c->ev0);

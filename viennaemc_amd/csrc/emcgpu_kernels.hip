@@ -1,6 +1,6 @@
 // emcgpu_kernels.hip -- gfx950 kernels of the EMC particle step.
 //
-// One work-item owns one particle of a 256-particle tile and keeps it in
+// A work-item owns PPT particles of a tile of 256 * PPT and keeps them in
 // registers for all steps fused into a launch, so HBM sees the SoA state
 // once per launch.  Each step is the free-flight / scatter loop of
 // basicBulkParticleHandler::moveParticles
@@ -8,7 +8,8 @@
 //   drift min(tau, dt); while (dt - tau > 0) { scatter; tau += tau';
 //   drift min(rem, tau'); rem -= tau' }; tau -= dt
 // followed by the per-step ensemble sums (wave ballot / shuffle reductions,
-// one FP64 partial per tile per step, Kahan-combined by k_reduce_obs).
+// one FP64 partial per tile per step, combined by the compensated two-pass
+// reduction k_reduce_chunks / k_reduce_final).
 //
 // Random numbers:
 //   production: Philox-4x32-10 keyed by the seed, counter
@@ -727,47 +728,79 @@ __global__ __launch_bounds__(kTile) void k_bulk_replay(StepArgs<R> a) {
 
 // ---------------------------------------------------------------------------
 // per-(step, observable, instance) compensated reduction of the tile partials
-// partial layout [step][obs][tile]: one block per output value, coalesced
-// loads, Neumaier summation per thread, TwoSum tree across the block
-// (deterministic: fixed order for a given tile count)
+// partial layout [step][obs][tile].  Two passes, both deterministic (fixed
+// order for a given tile count): chunks of kChunk tiles per block with
+// Neumaier summation per thread and a TwoSum tree across the block, then
+// the chunk (sum, compensation) pairs of each output value.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_reduce_obs(const double *__restrict__ partial,
-                                                    int n_tiles, int n_obs, int tile_size,
-                                                    const long long *__restrict__ inst_start,
-                                                    const long long *__restrict__ inst_count,
-                                                    int n_inst, double *__restrict__ out,
-                                                    int out_step0, int out_total) {
-  __shared__ double ss[256], sc[256];
-  const int inst = blockIdx.x % n_inst;
-  const int so = blockIdx.x / n_inst;  // s * n_obs + o
-  const int s = so / n_obs, o = so % n_obs;
-  const long long t0 = inst_start[inst] / tile_size;
-  const long long nt = (inst_count[inst] + tile_size - 1) / tile_size;
-  const double *src = partial + (long long)so * n_tiles + t0;
-  double sum = 0.0, comp = 0.0;
-  for (long long t = threadIdx.x; t < nt; t += 256) {
-    const double x = src[t];
-    const double y = sum + x;
-    if (fabs(sum) >= fabs(x))
-      comp += (sum - y) + x;
-    else
-      comp += (x - y) + sum;
-    sum = y;
-  }
-  ss[threadIdx.x] = sum;
-  sc[threadIdx.x] = comp;
+constexpr int kChunk = 2048;
+
+__device__ __forceinline__ void neumaier_add(double &sum, double &comp, double x) {
+  const double y = sum + x;
+  if (fabs(sum) >= fabs(x))
+    comp += (sum - y) + x;
+  else
+    comp += (x - y) + sum;
+  sum = y;
+}
+
+// TwoSum tree over 256 (sum, comp) pairs in LDS; result in ss[0] + sc[0]
+__device__ __forceinline__ void block_twosum(double *ss, double *sc) {
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
       const double x = ss[threadIdx.x], y0 = ss[threadIdx.x + w];
       const double y = x + y0;
       const double bb = y - x;
-      const double err = (x - (y - bb)) + (y0 - bb);  // TwoSum
+      const double err = (x - (y - bb)) + (y0 - bb);
       ss[threadIdx.x] = y;
       sc[threadIdx.x] += sc[threadIdx.x + w] + err;
     }
     __syncthreads();
   }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_chunks(const double *__restrict__ partial,
+                                                       int n_tiles, int tile_size,
+                                                       const long long *__restrict__ inst_start,
+                                                       const long long *__restrict__ inst_count,
+                                                       int n_chunks, double *__restrict__ chunk_out) {
+  __shared__ double ss[256], sc[256];
+  const int chunk = blockIdx.x, so = blockIdx.y, inst = blockIdx.z;
+  const long long t0 = inst_start[inst] / tile_size;
+  const long long nt = (inst_count[inst] + tile_size - 1) / tile_size;
+  const long long lo = (long long)chunk * kChunk;
+  const long long hi = (lo + kChunk < nt) ? lo + kChunk : nt;
+  const double *src = partial + (long long)so * n_tiles + t0;
+  double sum = 0.0, comp = 0.0;
+  for (long long t = lo + threadIdx.x; t < hi; t += 256)
+    neumaier_add(sum, comp, src[t]);
+  ss[threadIdx.x] = sum;
+  sc[threadIdx.x] = comp;
+  block_twosum(ss, sc);
+  if (threadIdx.x == 0) {
+    double *o = chunk_out + (((long long)inst * gridDim.y + so) * n_chunks + chunk) * 2;
+    o[0] = ss[0];
+    o[1] = sc[0];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_final(const double *__restrict__ chunk_out,
+                                                      int n_chunks, int n_obs,
+                                                      double *__restrict__ out, int out_step0,
+                                                      int out_total) {
+  __shared__ double ss[256], sc[256];
+  const int so = blockIdx.x, inst = blockIdx.y;
+  const int s = so / n_obs, o = so % n_obs;
+  const double *src = chunk_out + ((long long)inst * gridDim.x + so) * n_chunks * 2;
+  double sum = 0.0, comp = 0.0;
+  for (int c = threadIdx.x; c < n_chunks; c += 256) {
+    neumaier_add(sum, comp, src[2 * c]);
+    comp += src[2 * c + 1];
+  }
+  ss[threadIdx.x] = sum;
+  sc[threadIdx.x] = comp;
+  block_twosum(ss, sc);
   if (threadIdx.x == 0)
     out[((long long)inst * out_total + out_step0 + s) * n_obs + o] = ss[0] + sc[0];
 }
@@ -851,14 +884,22 @@ hipError_t launch_init(const StepArgs<R> &a, hipStream_t st) {
   return hipGetLastError();
 }
 
+// number of chunk pairs per output value for the largest instance
+int reduce_chunks(long long max_count, int tile_size) {
+  const long long nt = (max_count + tile_size - 1) / tile_size;
+  return (int)((nt + kChunk - 1) / kChunk > 0 ? (nt + kChunk - 1) / kChunk : 1);
+}
+
 hipError_t launch_reduce(const double *partial, int n_steps, int n_tiles, int n_obs, int tile_size,
                          const long long *inst_start, const long long *inst_count, int n_inst,
-                         double *out, int out_step0, int out_total, hipStream_t st) {
+                         int n_chunks, double *chunk_buf, double *out, int out_step0,
+                         int out_total, hipStream_t st) {
   if (n_steps <= 0 || n_inst <= 0)
     return hipSuccess;
-  hipLaunchKernelGGL(k_reduce_obs, dim3(n_steps * n_obs * n_inst), dim3(256), 0, st, partial,
-                     n_tiles, n_obs, tile_size, inst_start, inst_count, n_inst, out, out_step0,
-                     out_total);
+  hipLaunchKernelGGL(k_reduce_chunks, dim3(n_chunks, n_steps * n_obs, n_inst), dim3(256), 0, st,
+                     partial, n_tiles, tile_size, inst_start, inst_count, n_chunks, chunk_buf);
+  hipLaunchKernelGGL(k_reduce_final, dim3(n_steps * n_obs, n_inst), dim3(256), 0, st, chunk_buf,
+                     n_chunks, n_obs, out, out_step0, out_total);
   return hipGetLastError();
 }
 
