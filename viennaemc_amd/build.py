@@ -57,7 +57,24 @@ def build(verbose=True):
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
+    build_examples(verbose)
     return LIB
+
+
+EXAMPLES = ["bulkSimulationGpu"]
+
+
+def build_examples(verbose=True):
+    """Host-only C++17 programs written against the header-only host API,
+    linked to libemcgpu.so (rpath to the package folder)."""
+    for name in EXAMPLES:
+        src = os.path.join(HERE, "examples", name + ".cpp")
+        out = os.path.join(HERE, "examples", name)
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(HERE, "include"), src,
+               "-o", out, "-L", HERE, "-lemcgpu", "-Wl,-rpath,$ORIGIN/.."]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,253 @@
+// emcgpuBulkParticleHandler.hpp -- particle type and bulk handler of the GPU
+// host API.
+//
+// emcgpuParticleType keeps the reference particle type's registration API
+// (include/ParticleType/emcParticleType.hpp:125-153: addValley,
+// addScatterMechanism(regions, mechanism)) and builds the scatter tables the
+// reference way (fillScatterTables / renormalizeTables,
+// include/emcScatterHandler.hpp:220-288) from the mechanisms' host-side
+// getScatterRate.  emcgpuBulkParticleHandler keeps the bulk handler's
+// methods (examples/bulkSimulation/basicBulkParticleHandler.hpp:93-347:
+// generateInitialParticles, moveParticles, getAvgEnergy,
+// getAvgDriftVelocity, getValleyOccupationProbability, setSeed,
+// resetAppliedFieldStrength) and runs them on the GPU through the C ABI.
+//
+// Errors: the reference prints its message and aborts (emcMessage.hpp:43-65);
+// that is the default here too.  setErrorPolicy(ErrorPolicy::Throw) turns
+// errors into std::runtime_error instead.
+#ifndef EMCGPU_BULK_PARTICLE_HANDLER_HPP
+#define EMCGPU_BULK_PARTICLE_HANDLER_HPP
+
+#include <array>
+#include <cstdio>
+#include <cstdlib>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "emcgpuMechanisms.hpp"
+
+namespace emcgpu {
+
+enum class ErrorPolicy { Abort, Throw };
+
+inline ErrorPolicy &errorPolicy() {
+  static ErrorPolicy p = ErrorPolicy::Abort;
+  return p;
+}
+inline void setErrorPolicy(ErrorPolicy p) { errorPolicy() = p; }
+
+inline void reportError(const std::string &msg) {
+  if (errorPolicy() == ErrorPolicy::Throw)
+    throw std::runtime_error(msg);
+  std::fprintf(stderr, "ERROR: %s\n", msg.c_str());
+  std::abort();
+}
+
+inline void check(emcgpu_status st, const emcgpu_ctx *ctx, const char *what) {
+  if (st != EMCGPU_OK)
+    reportError(std::string(what) + ": " + emcgpu_last_error(ctx));
+}
+
+/// material constants and lattice temperature of a bulk run; provides the
+/// getters the mechanisms use (emcMaterial.hpp:27-37, emcDevice.hpp:185-190)
+template <class T>
+class emcgpuBulkDevice {
+public:
+  struct Material {
+    T epsR, rho, ni, velSound;
+    T getEpsR() const { return epsR; }
+    T getRho() const { return rho; }
+    T getNi() const { return ni; }
+    T getVelSound() const { return velSound; }
+  };
+  emcgpuBulkDevice(Material m, T temperature) : mat(m), temp(temperature) {}
+  const Material &getMaterial() const { return mat; }
+  T getTemperature() const { return temp; }
+  T getThermalVoltage() const { return phys::kB / phys::q * temp; }
+
+private:
+  Material mat;
+  T temp;
+};
+
+template <class T>
+class emcgpuParticleType {
+public:
+  typedef emcAbstractValley<T> AbstractValley;
+  typedef emcScatterMechanism<T> AbstractScatterMechanism;
+
+  emcgpuParticleType(SizeType nrEnergyLevels = 1000, T maxEnergy = 4., T mass = phys::me,
+                     T charge = -phys::q)
+      : nLevels(nrEnergyLevels), maxE(maxEnergy), particleMass(mass), particleCharge(charge) {}
+
+  T getMass() const { return particleMass; }
+  T getCharge() const { return particleCharge; }
+  SizeType getNrValleys() const { return valleys.size(); }
+  const AbstractValley *getValley(SizeType i) const { return valleys.at(i).get(); }
+  SizeType getNrEnergyLevels() const { return nLevels; }
+  T getMaxEnergy() const { return maxE; }
+
+  template <class DerivedValley>
+  void addValley(std::unique_ptr<DerivedValley> &&v) {
+    v->check();
+    valleys.push_back(std::move(v));
+    for (auto &m : mechs)
+      m->setPtrValley(valleys);
+  }
+
+  /// regions are accepted for interface compatibility; bulk runs use region 0
+  template <class DerivedMechanism>
+  void addScatterMechanism(const std::vector<int> & /*regions*/,
+                           std::unique_ptr<DerivedMechanism> &&m) {
+    if (m->getIdxValley() >= valleys.size())
+      reportError("Used index for Valley is invalid.");
+    m->setPtrValley(valleys);
+    m->check();
+    mechs.push_back(std::move(m));
+  }
+
+  /// raw rates rates[mech][level] at energies (level+1) * maxEnergy / nLevels
+  /// (fillScatterTables, emcScatterHandler.hpp:220-235)
+  std::vector<double> rawRates() const {
+    const T dE = maxE / (T)nLevels;
+    std::vector<double> r(mechs.size() * nLevels);
+    for (SizeType j = 0; j < mechs.size(); j++)
+      for (SizeType l = 0; l < nLevels; l++)
+        r[j * nLevels + l] = mechs[j]->getScatterRate((T)(l + 1) * dE, 0);
+    return r;
+  }
+
+  std::vector<emcgpu_valley_desc> valleyDescriptors() const {
+    std::vector<emcgpu_valley_desc> d;
+    for (auto &v : valleys)
+      d.push_back(v->descriptor());
+    for (auto &x : d)
+      x.particle_mass = particleMass;
+    return d;
+  }
+  std::vector<emcgpu_mech_desc> mechanismDescriptors() const {
+    std::vector<emcgpu_mech_desc> d;
+    for (auto &m : mechs)
+      d.push_back(m->finalState());
+    return d;
+  }
+
+private:
+  SizeType nLevels;
+  T maxE, particleMass, particleCharge;
+  std::vector<std::unique_ptr<AbstractValley>> valleys;
+  std::vector<std::unique_ptr<AbstractScatterMechanism>> mechs;
+};
+
+template <class T>
+class emcgpuBulkParticleHandler {
+public:
+  enum Precision { FP64 = EMCGPU_FP64, FP32 = EMCGPU_FP32 };
+
+  /// the basicBulkParticleHandler constructor (basicBulkParticleHandler.hpp:93-120)
+  /// with the particle count of the ensemble and the GPU precision; tables
+  /// are built here (initScatterTables) from the mechanisms' rates
+  template <class DeviceType>
+  emcgpuBulkParticleHandler(const DeviceType &device, emcgpuParticleType<T> &type,
+                            std::array<T, 3> fieldDirection, T fieldStrength, SizeType nrParticles,
+                            unsigned long inSeed = 1, Precision precision = FP32, int gpu = 0)
+      : ptype(type), dir(fieldDirection), seed(inSeed), temperature(device.getTemperature()),
+        nPart(nrParticles) {
+    emcgpu_config cfg{gpu, (int)precision, 1};
+    check(emcgpu_create(&cfg, &ctx), nullptr, "emcgpu_create");
+    auto vd = type.valleyDescriptors();
+    auto md = type.mechanismDescriptors();
+    check(emcgpu_set_model(ctx, 0, vd.data(), (int)vd.size(), md.data(), (int)md.size()), ctx,
+          "emcgpu_set_model");
+    auto rates = type.rawRates();
+    check(emcgpu_set_rate_tables(ctx, 0, (int)type.getNrEnergyLevels(), type.getMaxEnergy(),
+                                 rates.data()),
+          ctx, "emcgpu_set_rate_tables");
+    resetAppliedFieldStrength(fieldStrength);
+    size_t n = nrParticles;
+    check(emcgpu_resize(ctx, &n), ctx, "emcgpu_resize");
+  }
+  ~emcgpuBulkParticleHandler() { emcgpu_destroy(ctx); }
+  emcgpuBulkParticleHandler(const emcgpuBulkParticleHandler &) = delete;
+  emcgpuBulkParticleHandler &operator=(const emcgpuBulkParticleHandler &) = delete;
+
+  void setSeed(SizeType inSeed) { seed = inSeed; }
+  void resetAppliedFieldStrength(T strength) {
+    const double d[3] = {dir[0], dir[1], dir[2]};
+    check(emcgpu_set_field(ctx, 0, d, strength, ptype.getCharge()), ctx, "emcgpu_set_field");
+  }
+  /// thermal initial ensemble at the device temperature (generateInitialParticles)
+  void generateInitialParticles() {
+    check(emcgpu_init_particles(ctx, 0, seed, temperature), ctx, "emcgpu_init_particles");
+    step = 0;
+  }
+  SizeType getNrParticles(SizeType = 0) const { return nPart; }
+
+  /// moveParticles(dt) repeated nSteps times on the GPU; the observables of
+  /// the last step are kept for the getAvg* calls
+  void moveParticles(T dt, int nSteps = 1) {
+    emcgpu_step_args a{};
+    a.dt = dt;
+    a.n_steps = nSteps;
+    a.mode = EMCGPU_MODE_PRODUCTION;
+    a.seed = seed;
+    a.step0 = step;
+    check(emcgpu_step(ctx, &a), ctx, "emcgpu_step");
+    step += (uint32_t)nSteps;
+    int nObs = 0, nS = 0;
+    check(emcgpu_get_observables(ctx, 0, nullptr, &nObs, &nS), ctx, "emcgpu_get_observables");
+    obs.resize((size_t)nObs * nS);
+    check(emcgpu_get_observables(ctx, 0, obs.data(), &nObs, &nS), ctx, "emcgpu_get_observables");
+    lastObs.assign(obs.end() - nObs, obs.end());
+    check(emcgpu_obs_layout(ctx, &nObsL, &maxV, &maxDeg), ctx, "emcgpu_obs_layout");
+  }
+  /// per-step sums of the last moveParticles call: [step][value]
+  const std::vector<double> &stepObservables() const { return obs; }
+  int observablesPerStep() const { return nObsL; }
+
+  /// getAvgEnergy / getAvgDriftVelocity / getValleyOccupationProbability
+  /// (basicBulkParticleHandler.hpp:289-347) of the state after the last step
+  std::vector<T> getAvgEnergy(SizeType = 0) const { return perValley(1); }
+  std::vector<T> getAvgDriftVelocity(SizeType = 0) const { return perValley(2); }
+  std::vector<T> getValleyOccupationProbability(SizeType = 0) const {
+    std::vector<T> r(ptype.getNrValleys());
+    for (SizeType v = 0; v < r.size(); v++)
+      r[v] = lastObs[2 + 3 * v] / (T)nPart;
+    return r;
+  }
+  /// occupation of each (valley, subvalley)
+  std::vector<T> getSubValleyOccupationProbability(SizeType valley) const {
+    std::vector<T> r(ptype.getValley(valley)->getDegeneracyFactor());
+    for (SizeType s = 0; s < r.size(); s++)
+      r[s] = lastObs[2 + 3 * maxV + valley * maxDeg + s] / (T)nPart;
+    return r;
+  }
+
+  emcgpu_ctx *context() { return ctx; }
+
+private:
+  std::vector<T> perValley(int which) const {
+    std::vector<T> r(ptype.getNrValleys());
+    for (SizeType v = 0; v < r.size(); v++) {
+      const double cnt = lastObs[2 + 3 * v];
+      r[v] = cnt > 0 ? lastObs[2 + 3 * v + which] / cnt : 0;
+    }
+    return r;
+  }
+
+  emcgpu_ctx *ctx = nullptr;
+  emcgpuParticleType<T> &ptype;
+  std::array<T, 3> dir;
+  unsigned long seed;
+  T temperature;
+  SizeType nPart;
+  uint32_t step = 0;
+  std::vector<double> obs, lastObs;
+  int nObsL = 0, maxV = 1, maxDeg = 1;
+};
+
+}  // namespace emcgpu
+
+#endif
