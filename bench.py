@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip the FP32 one-step-per-launch leg")
     ap.add_argument("--ppl", type=int, default=0,
                     help="particles per GPU lane of the production kernel (0 = library default)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-instance sweep leg")
+    ap.add_argument("--batched-steps", type=int, default=2000)
     ap.add_argument("--cpu-particles", type=int, default=1_000_000)
     ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -90,6 +92,40 @@ def run_gpu(emcgpu, spec, precision, n, warmup, steps, spl, seed=20261202, ppl=0
     ctx.close()
     return dict(wall=wall, av=av, kernel_ms=ms / max(launches, 1), launches=launches,
                 steps_per_launch=(nt / max(launches, 1)))
+
+
+def run_batched(emcgpu, warmup, steps, seed=20261202):
+    """The 64-instance Si bias/temperature sweep in one FP32 arena (tables
+    built per instance in FP64, stored FP32), every call launched as one HIP
+    graph; whole-sweep particle-updates/s."""
+    from emc_models import batched_instances
+    inst = batched_instances(64, seed)
+    counts = [n for _, n in inst]
+    ctx = emcgpu.Context(precision=emcgpu.FP32, n_instances=len(inst))
+    t0 = time.perf_counter()
+    for i, (m, _) in enumerate(inst):
+        ctx.setup(i, m)
+    ctx.resize(counts)
+    for i, (m, _) in enumerate(inst):
+        ctx.init_particles(i, seed, m["T"])
+    ctx.synchronize()
+    setup_s = time.perf_counter() - t0
+    if warmup:
+        ctx.step(1e-15, warmup, seed=seed, step0=0, graph=True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(1e-15, steps, seed=seed, step0=warmup, graph=True)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    av = [ctx.averages(i) for i in range(len(inst))]
+    ctx.close()
+    total = sum(counts)
+    vd = [float(a["avgV"][-1, 0]) for a in av]
+    return {"updates_per_s": total * steps / wall, "instances": len(inst), "particles": total,
+            "steps": steps, "ms_per_step": wall / steps * 1e3, "setup_s": setup_s,
+            "vd_min": min(vd), "vd_max": max(vd),
+            "workload": "64 Si bulk instances, field log-U[0.1,200] kV/cm along [100]/[111], "
+                        "T U[77,500] K, N log-U[1e4,2e6], FP32, HIP-graph launch"}
 
 
 PROFILE = os.path.join(ROOT, "profiles", "emc_step_si_bulk_10M.md")
@@ -160,6 +196,8 @@ def main():
         out["fp32_plain_updates_per_s"] = n * min(args.steps, 200) / r1["wall"]
         out["fp32_plain_kernel_ms"] = r1["kernel_ms"]
         out["fp32_plain_roofline_frac"] = BYTES_FP32 * n / (r1["kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+    if not args.no_batched:
+        out["batched_sweep"] = run_batched(emcgpu, args.warmup, args.batched_steps)
     spl = r32["steps_per_launch"]
     # algorithmic bytes per launch: 48 B per particle-update (the SoA state
     # read and written once per update) x the updates one launch processes;

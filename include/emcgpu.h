@@ -118,7 +118,12 @@ typedef struct {
   uint32_t step0;       /* production: global index of the first step */
   int timing;           /* non-zero: time every launch with HIP events */
   int particles_per_lane; /* production kernel: particles owned by each
-                             GPU lane, 1, 2, 4 or 8 (0 = library default) */
+                             GPU lane, 1, 2, 4 or 8 (0 = library default,
+                             or the autotuned value) */
+  int use_graph;        /* production: capture the call's launches (step and
+                           reduction kernels) in a HIP graph and launch it;
+                           the executable graph is updated in place on the
+                           next call.  Ignored with timing or trace. */
   /* replay (FP64 contexts only): per-particle recorded raw 64-bit draws of
    * the reference engine, host memory, particle order of instance 0 */
   const uint64_t *draws; /* [n_particles][draw_stride] */

@@ -137,3 +137,41 @@ def field_force(model):
     d = [x / n for x in d]
     e = model["field_kvcm"] * 1e5
     return [x * e * -Q for x in d], d
+
+
+def _splitmix64(state):
+    """One SplitMix64 step: returns (new state, output)."""
+    m = (1 << 64) - 1
+    state = (state + 0x9E3779B97F4A7C15) & m
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    return state, z ^ (z >> 31)
+
+
+def batched_instances(n_instances=64, seed=20261202, n_min=10_000, n_max=2_000_000):
+    """The batched parameter sweep: independent Si bulk instances (the
+    si_jacoboni physics) whose bias, temperature and size are drawn from a
+    host SplitMix64 stream -- field log-uniform in [0.1, 200] kV/cm along
+    [100] or [111] with probability 1/2 each, T uniform in [77, 500] K,
+    particle count log-uniform in [n_min, n_max].  Each instance keeps its
+    own FP64-built 2000-level tables.  Returns a list of (model, count)."""
+    import copy
+    import math
+    state = seed
+    out = []
+
+    def uniform():
+        nonlocal state
+        state, z = _splitmix64(state)
+        return (z >> 11) * 2.0 ** -53
+
+    for _ in range(n_instances):
+        field = math.exp(math.log(0.1) + (math.log(200.0) - math.log(0.1)) * uniform())
+        direction = (1, 0, 0) if uniform() < 0.5 else (1, 1, 1)
+        temperature = 77.0 + (500.0 - 77.0) * uniform()
+        count = int(math.exp(math.log(n_min) + (math.log(n_max) - math.log(n_min)) * uniform()))
+        m = copy.deepcopy(MODELS["si_jacoboni"])
+        m.update(T=temperature, field_kvcm=field, field_dir=direction)
+        out.append((m, count))
+    return out

@@ -1,0 +1,113 @@
+"""Batched parameter sweep: many independent (bias, temperature, size)
+instances of the Si bulk model in one particle arena.
+
+CPU: every instance's FP64 tables (built at its own temperature) equal the
+oracle's tables for that instance's model.
+GPU: heterogeneous instances ([100] and [111] fields, ragged counts,
+different temperatures) each evolve exactly as the oracle's Philox twin of
+that instance; a step sequence launched through a HIP graph reproduces the
+eagerly launched one bit for bit, over repeated calls.
+"""
+import numpy as np
+import pytest
+
+from emc_models import batched_instances, field_force
+from oracle import oracle_py as O
+from viennaemc_amd import emcgpu
+
+REL_TOL = 1e-12
+
+
+def _rel_k(a, b):
+    return np.linalg.norm(a - b, axis=-1) / np.maximum(np.linalg.norm(b, axis=-1), 1e-300)
+
+
+def test_sweep_generator_covers_the_ranges():
+    inst = batched_instances()
+    assert len(inst) == 64
+    fields = np.array([m["field_kvcm"] for m, _ in inst])
+    temps = np.array([m["T"] for m, _ in inst])
+    counts = np.array([n for _, n in inst])
+    assert fields.min() >= 0.1 and fields.max() <= 200.0
+    assert temps.min() >= 77.0 and temps.max() <= 500.0
+    assert counts.min() >= 10_000 and counts.max() <= 2_000_000
+    dirs = {m["field_dir"] for m, _ in inst}
+    assert dirs == {(1, 0, 0), (1, 1, 1)}
+    # the same seed gives the same sweep
+    assert [n for _, n in batched_instances()] == counts.tolist()
+
+
+def test_per_instance_tables_match_oracle():
+    inst = batched_instances(n_instances=6)
+    ctx = emcgpu.Context(precision=emcgpu.FP32, n_instances=len(inst), device=-1)
+    for i, (m, _) in enumerate(inst):
+        ctx.setup(i, m)
+    for i, (m, _) in enumerate(inst):
+        M = O.OracleModel(m)
+        cum, tau0 = ctx.get_tables(i, 0, m["n_levels"])
+        assert (cum == M.cum(0)).all()
+        assert tau0 == M.tau0(0)
+    # temperatures differ, so the tables do too
+    c0, _ = ctx.get_tables(0, 0, inst[0][0]["n_levels"])
+    c1, _ = ctx.get_tables(1, 0, inst[1][0]["n_levels"])
+    assert not (c0 == c1).all()
+
+
+@pytest.mark.gpu
+def test_sweep_instances_match_oracle_twin():
+    inst = batched_instances(n_instances=5, seed=7, n_min=300, n_max=5000)
+    counts = [n for _, n in inst]
+    print("sweep instances vs oracle twin, counts", counts, flush=True)
+    seed, ns = 0xC3C3, 60
+    ctx = emcgpu.Context(precision=emcgpu.FP64, n_instances=len(inst))
+    for i, (m, _) in enumerate(inst):
+        ctx.setup(i, m)
+    ctx.resize(counts)
+    for i, (m, _) in enumerate(inst):
+        ctx.init_particles(i, seed, m["T"])
+    ctx.step(1e-15, ns, seed=seed, steps_per_launch=16)
+    for i, (m, n) in enumerate(inst):
+        M = O.OracleModel(m)
+        ref = M.init_philox(n, O.thermal_voltage(m["T"]), seed, instance=i)
+        force, d = field_force(m)
+        obs = M.run_philox(ref, ns, 1e-15, force, d, seed, instance=i)
+        g = ctx.get_particles(i)
+        assert (g["valley"] == ref["valley"]).all() and (g["sub"] == ref["sub"]).all(), i
+        assert _rel_k(g["k"].reshape(-1, 3), ref["k"].reshape(-1, 3)).max() < REL_TOL, i
+        assert (np.abs(g["E"] - ref["E"]) / ref["E"]).max() < REL_TOL, i
+        av = ctx.averages(i)
+        assert np.allclose(av["avgE"][:, 0], obs[:, 0, 0], rtol=1e-11, atol=0), i
+        assert (av["occ"][:, 0] == obs[:, 2, 0]).all(), i
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_graph_launch_matches_eager(precision):
+    """Three calls through a HIP graph (the second reuses the executable
+    graph with new parameters, the third has a different number of
+    launches) give the same particles and observables as eager launches."""
+    inst = batched_instances(n_instances=3, seed=11, n_min=2000, n_max=40000)
+    counts = [n for _, n in inst]
+    print("graph vs eager, precision", precision, "counts", counts, flush=True)
+    runs = {}
+    for graph in (False, True):
+        ctx = emcgpu.Context(precision=precision, n_instances=len(inst))
+        for i, (m, _) in enumerate(inst):
+            ctx.setup(i, m)
+        ctx.resize(counts)
+        for i, (m, _) in enumerate(inst):
+            ctx.init_particles(i, 99, m["T"])
+        rec = []
+        step0 = 0
+        for ns in (64, 40, 100):
+            ctx.step(1e-15, ns, seed=99, step0=step0, steps_per_launch=32, graph=graph)
+            step0 += ns
+            rec.append(([ctx.get_particles(i) for i in range(len(inst))],
+                        [ctx.observables(i) for i in range(len(inst))]))
+        ctx.close()
+        runs[graph] = rec
+    for (pa, oa), (pb, ob) in zip(runs[False], runs[True]):
+        for i in range(len(inst)):
+            for key in ("k", "E", "tau", "valley", "sub"):
+                assert np.array_equal(pa[i][key], pb[i][key]), (i, key)
+            assert np.array_equal(oa[i], ob[i]), i

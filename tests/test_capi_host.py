@@ -47,6 +47,7 @@ def test_struct_layouts_match_header():
     assert C.sizeof(emcgpu.ValleyDesc) == 4 + 4 + 24 + 8 + 4 + 4 + 8 + 8 + 8 * 9 * 8
     assert C.sizeof(emcgpu.MechDesc) == 3 * 4 + 4 + 5 * 8 + 4 + 8 * 8 * 4 + 4
     assert C.sizeof(emcgpu.StepArgs) == 8 + 4 * 3 + 4 + 8 + 4 + 4 + 4 + 4 + 8 * 9
+    assert emcgpu.StepArgs.draws.offset == 48
 
 
 def host_ctx(n_instances=1):

@@ -91,6 +91,9 @@ struct emcgpu_ctx {
   double ms_total = 0;
   long long launches = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipGraphExec_t gexec = nullptr;  // last captured step sequence
+  // autotuned launch shape of the production kernel (0 = not tuned)
+  int tuned_ppt = 0, tuned_spl = 0;
 };
 
 namespace {
@@ -387,7 +390,16 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
   int *d_over = nullptr;
   int32_t *d_tv = nullptr, *d_ts = nullptr, *d_te = nullptr;
   double *d_tk = nullptr, *d_tE = nullptr;
+  bool capturing = false;
   auto cleanup = [&]() {
+    if (capturing) {  // abandon a partial capture so the stream is usable
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(c->stream, &g);
+      if (g)
+        (void)hipGraphDestroy(g);
+      (void)hipGetLastError();
+      capturing = false;
+    }
     dfreet(d_draws);
     dfreet(d_cursor);
     dfreet(d_grain);
@@ -440,6 +452,11 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
     a.tr_n = (long long)n0;
   }
 
+  const bool graph = s->use_graph && !replay && !trace && !s->timing;
+  if (graph) {
+    HIPCHK_CLEAN(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    capturing = true;
+  }
   for (int done = 0; done < s->n_steps; done += K) {
     const int k = std::min(K, s->n_steps - done);
     a.n_steps = k;
@@ -466,6 +483,29 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
     HIPCHK_CLEAN(launch_reduce(c->d_partial, k, n_tiles, c->n_obs, tile_size, c->d_inst_start,
                                c->d_inst_count, c->n_inst, n_chunks, c->d_chunks, c->d_obs, done,
                                s->n_steps, c->stream));
+  }
+  if (graph) {
+    hipGraph_t g = nullptr;
+    capturing = false;
+    HIPCHK_CLEAN(hipStreamEndCapture(c->stream, &g));
+    bool updated = false;
+    if (c->gexec) {
+      hipGraphNode_t err_node = nullptr;
+      hipGraphExecUpdateResult res;
+      updated = hipGraphExecUpdate(c->gexec, g, &err_node, &res) == hipSuccess &&
+                res == hipGraphExecUpdateSuccess;
+      if (!updated) {
+        (void)hipGetLastError();
+        (void)hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+      }
+    }
+    hipError_t e = hipSuccess;
+    if (!updated)
+      e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK_CLEAN(e);
+    HIPCHK_CLEAN(hipGraphLaunch(c->gexec, c->stream));
   }
 
   if (replay) {
@@ -702,6 +742,8 @@ void emcgpu_destroy(emcgpu_ctx *c) {
   dfreet(c->d_partial);
   dfreet(c->d_chunks);
   dfreet(c->d_obs);
+  if (c->gexec)
+    (void)hipGraphExecDestroy(c->gexec);
   if (c->ev0)
     (void)hipEventDestroy(c->ev0);
   if (c->ev1)

@@ -59,7 +59,8 @@ class Material(C.Structure):
 class StepArgs(C.Structure):
     _fields_ = [("dt", C.c_double), ("n_steps", C.c_int), ("steps_per_launch", C.c_int),
                 ("mode", C.c_int), ("seed", C.c_uint64), ("step0", C.c_uint32),
-                ("timing", C.c_int), ("particles_per_lane", C.c_int), ("draws", C.c_void_p),
+                ("timing", C.c_int), ("particles_per_lane", C.c_int), ("use_graph", C.c_int),
+                ("draws", C.c_void_p),
                 ("draw_stride", C.c_size_t),
                 ("cursor", C.c_void_p), ("grain_tau", C.c_void_p),
                 ("trace_valley", C.c_void_p), ("trace_sub", C.c_void_p),
@@ -265,7 +266,8 @@ class Context:
 
     # -- stepping --------------------------------------------------------
     def step(self, dt, n_steps, mode=PRODUCTION, seed=0, step0=0, steps_per_launch=0,
-             timing=False, draws=None, cursor=None, grain_tau=None, trace=False, ppl=0):
+             timing=False, draws=None, cursor=None, grain_tau=None, trace=False, ppl=0,
+             graph=False):
         a = StepArgs()
         a.dt = dt
         a.n_steps = n_steps
@@ -275,6 +277,7 @@ class Context:
         a.step0 = step0
         a.timing = 1 if timing else 0
         a.particles_per_lane = ppl
+        a.use_graph = 1 if graph else 0
         keep = []
         if mode == REPLAY:
             draws = np.ascontiguousarray(draws, dtype=np.uint64)
