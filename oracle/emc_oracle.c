@@ -482,6 +482,186 @@ void orc_mech_froehlich(orc_mech *m, int kind, int valley,
   m->nBose = 1.0 / (exp(x) - 1.0);
 }
 
+/* emcHotPhononFroehlich{Absorption,Emission}3D constructors
+ * (emcHotPhononFroehlichMechanism.hpp:61-70, :144-153): the Froehlich
+ * prefactor without an equilibrium occupation */
+void orc_mech_hot_froehlich(orc_mech *m, int kind, int valley,
+                            double phononEnergy, double relEffMass,
+                            double epsHi, double epsLo, int bath) {
+  orc_mech_froehlich(m, ORC_MECH_FROEHLICH_AB, valley, phononEnergy,
+                     relEffMass, epsHi, epsLo, 300.0);
+  m->kind = kind;
+  m->nBose = 0.0;
+  m->bath = bath;
+}
+
+/* ======================================================================
+ * LO phonon bath: include/emcPhononBath.hpp
+ * ====================================================================== */
+
+/* planck / planckTemp (:127-137) */
+static double orc_planck(double energyEV, double tempK) {
+  double x = ORC_Q * energyEV / (ORC_KB * tempK);
+  return 1.0 / (exp(x) - 1.0);
+}
+static double orc_planck_temp(double energyEV, double N) {
+  if (N <= 0.0)
+    return 0.0;
+  return ORC_Q * energyEV / (ORC_KB * log(1.0 + 1.0 / N));
+}
+
+/* constructor (:160-192) */
+void orc_bath_init(orc_bath *b, int nBins, double dq, double tauLO,
+                   double phononEnergy, double latticeTemp, double Vsim,
+                   int enableAcoustic, double acPhononEnergy,
+                   double tauAcoustic, double wRidley, double toPhononEnergy,
+                   double tauTO) {
+  memset(b, 0, sizeof(*b));
+  b->nBins = nBins;
+  b->dq = dq;
+  b->tauLO = tauLO;
+  b->Vsim = Vsim;
+  b->Nq = (double *)calloc((size_t)nBins, sizeof(double));
+  b->nEm = (double *)calloc((size_t)nBins, sizeof(double));
+  b->nAbs = (double *)calloc((size_t)nBins, sizeof(double));
+  b->loE = phononEnergy;
+  b->latT = latticeTemp;
+  b->N0 = orc_planck(phononEnergy, latticeTemp);
+  for (int i = 0; i < nBins; i++)
+    b->Nq[i] = b->N0;
+  if (enableAcoustic) {
+    b->hasAc = 1;
+    b->acE = acPhononEnergy;
+    b->tauAc = tauAcoustic;
+    b->NacEq = orc_planck(acPhononEnergy, latticeTemp);
+    b->Nac = b->NacEq;
+  }
+  if (enableAcoustic && wRidley > 0.0 && toPhononEnergy > 0.0) {
+    b->hasRidley = 1;
+    b->wR = (wRidley > 1.0) ? 1.0 : wRidley;
+    b->toE = toPhononEnergy;
+    b->tauTO = tauTO;
+    b->NtoEq = orc_planck(toPhononEnergy, latticeTemp);
+    b->Nto = b->NtoEq;
+  }
+}
+
+void orc_bath_free(orc_bath *b) {
+  free(b->Nq);
+  free(b->nEm);
+  free(b->nAbs);
+  free(b->tauProfile);
+  memset(b, 0, sizeof(*b));
+}
+
+/* binOf (:222-225): truncation of q/dq to size_t, clamped to the last bin */
+int orc_bath_bin(const orc_bath *b, double q) {
+  double r = q / b->dq;
+  if (!(r < (double)b->nBins))
+    return b->nBins - 1;
+  return (int)(size_t)r;
+}
+
+/* recordEmission / recordAbsorption (:237-253) */
+void orc_bath_record(const orc_bath *b, double q, int emission) {
+  int i = orc_bath_bin(b, q);
+  if (emission)
+    b->nEm[i] += 1.0;
+  else
+    b->nAbs[i] += 1.0;
+}
+
+/* update (:264-358), operation for operation */
+void orc_bath_update(orc_bath *b, double dt) {
+  double target = b->N0;
+  if (b->hasAc) {
+    double nKlemens = b->N0;
+    if (b->Nac > b->NacEq) {
+      double tAc = orc_planck_temp(b->acE, b->Nac);
+      nKlemens = orc_planck(b->loE, tAc);
+    }
+    if (b->hasRidley) {
+      double nRidley = b->N0;
+      if (b->Nto > b->NtoEq) {
+        double tTo = orc_planck_temp(b->toE, b->Nto);
+        nRidley = orc_planck(b->loE, tTo);
+      }
+      target = (1.0 - b->wR) * nKlemens + b->wR * nRidley;
+    } else {
+      target = nKlemens;
+    }
+  }
+  const int hasProfile = b->tauProfile != NULL;
+  double sumW = 0.0, sumWdN = 0.0, sumWdNtau = 0.0;
+  for (int i = 0; i < b->nBins; i++) {
+    double q = ((double)(size_t)i + 0.5) * b->dq;
+    double Dph = q * q * b->dq * b->Vsim / (2.0 * ORC_PI * ORC_PI);
+    double g = (Dph > 0.0) ? (b->nEm[i] - b->nAbs[i]) / (Dph * dt) : 0.0;
+    const double tau_i = hasProfile ? b->tauProfile[i] : b->tauLO;
+    if (b->hasAc) {
+      double w = q * q;
+      sumW += w;
+      if (hasProfile)
+        sumWdNtau += w * (b->Nq[i] - target) / tau_i;
+      else
+        sumWdN += w * (b->Nq[i] - target);
+    }
+    b->Nq[i] += g * dt - (dt / tau_i) * (b->Nq[i] - target);
+    if (b->Nq[i] < 0.0)
+      b->Nq[i] = 0.0;
+    b->nEm[i] = 0.0;
+    b->nAbs[i] = 0.0;
+  }
+  if (b->hasAc) {
+    double fKlemens = b->hasRidley ? (1.0 - b->wR) : 1.0;
+    if (!hasProfile) {
+      double dNmean = (sumW > 0.0) ? sumWdN / sumW : 0.0;
+      b->Nac += dt * fKlemens * dNmean / b->tauLO - dt * (b->Nac - b->NacEq) / b->tauAc;
+      if (b->Nac < b->NacEq)
+        b->Nac = b->NacEq;
+      if (b->hasRidley) {
+        b->Nto += dt * b->wR * dNmean / b->tauLO - dt * (b->Nto - b->NtoEq) / b->tauTO;
+        if (b->Nto < b->NtoEq)
+          b->Nto = b->NtoEq;
+      }
+    } else {
+      double flux = (sumW > 0.0) ? sumWdNtau / sumW : 0.0;
+      b->Nac += dt * fKlemens * flux - dt * (b->Nac - b->NacEq) / b->tauAc;
+      if (b->Nac < b->NacEq)
+        b->Nac = b->NacEq;
+      if (b->hasRidley) {
+        b->Nto += dt * b->wR * flux - dt * (b->Nto - b->NtoEq) / b->tauTO;
+        if (b->Nto < b->NtoEq)
+          b->Nto = b->NtoEq;
+      }
+    }
+  }
+}
+
+/* getMeanNq (:461-470) */
+double orc_bath_mean_nq(const orc_bath *b) {
+  double sumW = 0.0, sumWN = 0.0;
+  for (int i = 0; i < b->nBins; i++) {
+    double q = ((double)(size_t)i + 0.5) * b->dq;
+    double w = q * q;
+    sumW += w;
+    sumWN += w * b->Nq[i];
+  }
+  return (sumW > 0.0) ? sumWN / sumW : b->N0;
+}
+
+/* getAcousticTemp / getTOTemp (:480-494) */
+double orc_bath_acoustic_temp(const orc_bath *b) {
+  if (!b->hasAc || b->Nac <= b->NacEq)
+    return b->latT;
+  return orc_planck_temp(b->acE, b->Nac);
+}
+double orc_bath_to_temp(const orc_bath *b) {
+  if (!b->hasRidley || b->Nto <= b->NtoEq)
+    return b->latT;
+  return orc_planck_temp(b->toE, b->Nto);
+}
+
 /* getScatterRate of each mechanism kind */
 double orc_mech_rate(const orc_model *md, orc_mech *m, double energy) {
   const orc_valley *iv = &md->valley[m->valley];
@@ -549,8 +729,49 @@ double orc_mech_rate(const orc_model *md, orc_mech *m, double energy) {
     double ln = log((kI + kF) / (kI - kF));
     return m->scatterConst * (m->nBose + 1.0) / kI * ln;
   }
+  case ORC_MECH_HOT_FROEHLICH_AB: { /* emcHotPhononFroehlichMechanism.hpp:79-91 */
+    double gammaI = orc_valley_gamma(iv, energy);
+    double ef = energy + m->phononEnergy;
+    double gammaF = orc_valley_gamma(iv, ef);
+    if (gammaI <= 0.0 || gammaF <= 0.0)
+      return 0.0;
+    double kI = sqrt(2.0 * m->effMass * gammaI * ORC_Q) / ORC_HBAR;
+    double kF = sqrt(2.0 * m->effMass * gammaF * ORC_Q) / ORC_HBAR;
+    double ln = log((kI + kF) / (kF - kI));
+    double Nq = orc_bath_mean_nq(&md->bath[m->bath]);
+    return m->scatterConst * Nq / kI * ln;
+  }
+  case ORC_MECH_HOT_FROEHLICH_EM: { /* emcHotPhononFroehlichMechanism.hpp:159-173 */
+    if (energy <= m->phononEnergy)
+      return 0.0;
+    double gammaI = orc_valley_gamma(iv, energy);
+    double ef = energy - m->phononEnergy;
+    double gammaF = orc_valley_gamma(iv, ef);
+    if (gammaI <= 0.0 || gammaF <= 0.0)
+      return 0.0;
+    double kI = sqrt(2.0 * m->effMass * gammaI * ORC_Q) / ORC_HBAR;
+    double kF = sqrt(2.0 * m->effMass * gammaF * ORC_Q) / ORC_HBAR;
+    double ln = log((kI + kF) / (kI - kF));
+    double Nq = orc_bath_mean_nq(&md->bath[m->bath]);
+    return m->scatterConst * (Nq + 1.0) / kI * ln;
+  }
   }
   return 0;
+}
+
+static int orc_is_froehlich_ab(int kind) {
+  return kind == ORC_MECH_FROEHLICH_AB || kind == ORC_MECH_HOT_FROEHLICH_AB;
+}
+
+/* hot-phonon mechanisms record |q| = |k_new - k_old| after the final state
+ * (emcHotPhononFroehlichMechanism.hpp:113-116, :195-198) */
+static void orc_hot_record(const orc_model *md, const orc_mech *m,
+                           const double kOld[3], const double kNew[3]) {
+  if (m->kind != ORC_MECH_HOT_FROEHLICH_AB && m->kind != ORC_MECH_HOT_FROEHLICH_EM)
+    return;
+  double qv[3] = {kNew[0] - kOld[0], kNew[1] - kOld[1], kNew[2] - kOld[2]};
+  orc_bath_record(&md->bath[m->bath], orc_norm3(qv),
+                  m->kind == ORC_MECH_HOT_FROEHLICH_EM);
 }
 
 /* std::min / std::max semantics used by the Froehlich clamp */
@@ -588,9 +809,15 @@ void orc_mech_scatter(const orc_model *md, const orc_mech *m,
     return;
   }
   case ORC_MECH_FROEHLICH_AB:
-  case ORC_MECH_FROEHLICH_EM: { /* emcFroehlichInteraction.hpp:109-128 / :183-200 */
+  case ORC_MECH_FROEHLICH_EM:
+  case ORC_MECH_HOT_FROEHLICH_AB:
+  case ORC_MECH_HOT_FROEHLICH_EM: {
+    /* emcFroehlichInteraction.hpp:109-128 / :183-200; the hot-phonon
+     * kinds (emcHotPhononFroehlichMechanism.hpp:94-117 / :175-199) are the
+     * same sampler followed by the bath record */
+    double kOld[3] = {p->k[0], p->k[1], p->k[2]};
     double initEnergy = p->energy;
-    if (m->kind == ORC_MECH_FROEHLICH_AB)
+    if (orc_is_froehlich_ab(m->kind))
       p->energy += m->phononEnergy;
     else
       p->energy -= m->phononEnergy;
@@ -612,6 +839,7 @@ void orc_mech_scatter(const orc_model *md, const orc_mech *m,
       for (int i = 0; i < 3; i++)
         p->k[i] = p->k[i] * s;
     }
+    orc_hot_record(md, m, kOld, p->k);
     return;
   }
   }
@@ -629,6 +857,8 @@ void orc_model_init(orc_model *m, int nLevels, double maxEnergy) {
   m->dE = maxEnergy / (double)(size_t)nLevels;
   for (int v = 0; v < ORC_MAX_VALLEYS; v++)
     m->tau0[v] = 2e-15;
+  m->bathUpdate = 1;
+  m->rebuildEvery = 1;
 }
 
 void orc_model_free(orc_model *m) {
@@ -636,6 +866,32 @@ void orc_model_free(orc_model *m) {
     free(m->cum[v]);
     m->cum[v] = NULL;
   }
+  for (int b = 0; b < m->nBaths; b++)
+    orc_bath_free(&m->bath[b]);
+  m->nBaths = 0;
+}
+
+int orc_model_has_hot(const orc_model *m) {
+  for (int j = 0; j < m->nMech; j++)
+    if (m->mech[j].kind == ORC_MECH_HOT_FROEHLICH_AB ||
+        m->mech[j].kind == ORC_MECH_HOT_FROEHLICH_EM)
+      return 1;
+  return 0;
+}
+
+void orc_model_bath_step(orc_model *m, double dt, int rebuild) {
+  for (int b = 0; b < m->nBaths; b++)
+    orc_bath_update(&m->bath[b], dt);
+  if (rebuild && orc_model_has_hot(m))
+    orc_model_build_tables(m);
+}
+
+void orc_model_end_step(orc_model *m, double dt) {
+  if (m->nBaths == 0 || !m->bathUpdate)
+    return;
+  m->stepCount++;
+  int every = m->rebuildEvery > 1 ? m->rebuildEvery : 1;
+  orc_model_bath_step(m, dt, m->stepCount % every == 0);
 }
 
 int orc_model_add_valley(orc_model *m, const orc_valley *v) {
@@ -769,33 +1025,34 @@ void orc_move_particle(const orc_model *m, orc_particle *p, double tStep,
   }
 }
 
-int orc_run_replay(const orc_model *m, int np, double *k, double *E,
+int orc_run_replay(orc_model *m, int np, double *k, double *E,
                    double *tau, double *grainTau, int32_t *valley,
                    int32_t *sub, const uint64_t *draws, size_t stride,
                    size_t *cursor, int nsteps, double tStep,
                    const double force[3], int32_t *valleyOut,
                    int32_t *subOut, int32_t *nEventsOut, double *kOut,
                    double *EOut, int32_t *evCount) {
-  int nOverflow = 0;
   int ntMax = m->nTables[0];
   for (int v = 1; v < m->nValleys; v++)
     if (m->nTables[v] > ntMax)
       ntMax = m->nTables[v];
-  for (int ip = 0; ip < np; ip++) {
-    orc_particle p;
-    memset(&p, 0, sizeof p);
-    p.k[0] = k[3 * ip];
-    p.k[1] = k[3 * ip + 1];
-    p.k[2] = k[3 * ip + 2];
-    p.energy = E[ip];
-    p.tau = tau[ip];
-    p.grainTau = grainTau ? grainTau[ip] : 1.0;
-    p.valley = valley[ip];
-    p.subValley = sub[ip];
-    orc_rng rng;
-    orc_rng_init_buf(&rng, draws + (size_t)ip * stride, stride);
-    rng.pos = cursor[ip];
-    for (int s = 0; s < nsteps; s++) {
+  /* step-major: a phonon bath couples the particles between steps */
+  char *over = (char *)calloc((size_t)(np > 0 ? np : 1), 1);
+  for (int s = 0; s < nsteps; s++) {
+    for (int ip = 0; ip < np; ip++) {
+      orc_particle p;
+      memset(&p, 0, sizeof p);
+      p.k[0] = k[3 * ip];
+      p.k[1] = k[3 * ip + 1];
+      p.k[2] = k[3 * ip + 2];
+      p.energy = E[ip];
+      p.tau = tau[ip];
+      p.grainTau = grainTau ? grainTau[ip] : 1.0;
+      p.valley = valley[ip];
+      p.subValley = sub[ip];
+      orc_rng rng;
+      orc_rng_init_buf(&rng, draws + (size_t)ip * stride, stride);
+      rng.pos = cursor[ip];
       orc_step_info info;
       orc_move_particle(m, &p, tStep, force, &rng, &info,
                         evCount ? evCount + (size_t)ip * ntMax : NULL);
@@ -813,20 +1070,25 @@ int orc_run_replay(const orc_model *m, int np, double *k, double *E,
       }
       if (EOut)
         EOut[o] = p.energy;
+      if (rng.overflow)
+        over[ip] = 1;
+      cursor[ip] = rng.pos;
+      k[3 * ip] = p.k[0];
+      k[3 * ip + 1] = p.k[1];
+      k[3 * ip + 2] = p.k[2];
+      E[ip] = p.energy;
+      tau[ip] = p.tau;
+      if (grainTau)
+        grainTau[ip] = p.grainTau;
+      valley[ip] = p.valley;
+      sub[ip] = p.subValley;
     }
-    if (rng.overflow)
-      nOverflow++;
-    cursor[ip] = rng.pos;
-    k[3 * ip] = p.k[0];
-    k[3 * ip + 1] = p.k[1];
-    k[3 * ip + 2] = p.k[2];
-    E[ip] = p.energy;
-    tau[ip] = p.tau;
-    if (grainTau)
-      grainTau[ip] = p.grainTau;
-    valley[ip] = p.valley;
-    sub[ip] = p.subValley;
+    orc_model_end_step(m, tStep);
   }
+  int nOverflow = 0;
+  for (int ip = 0; ip < np; ip++)
+    nOverflow += over[ip];
+  free(over);
   return nOverflow;
 }
 
@@ -842,6 +1104,25 @@ void orc_init_particle(const orc_model *m, orc_particle *p, double Vt,
   p->subValley = (int)floor((double)(size_t)v->deg * orc_rng_uniform(rng, 1e-6, 1.));
   p->energy = -1.5 * Vt * log(orc_rng_uniform(rng, 1e-6, 1.));
   double nk = orc_valley_normk(v, p->energy);
+  double r2 = orc_rng_uniform(rng, 0., 1.);
+  double r1 = orc_rng_uniform(rng, 0., 1.);
+  orc_init_random_direction(nk, r1, r2, p->k);
+  p->tau = orc_new_tau(m, p->valley, rng);
+  p->grainTau = -log(orc_rng_uniform(rng, 1e-6, 1.)) * 1.0;
+}
+
+/* emcElectron::generateInitialParticle with initEnergyEV > 0
+ * (emcElectron.hpp:84-85) -> initParticleKSpaceFixed
+ * (emcParticleInitialization.hpp:57-71): no energy draw, E = E0 */
+void orc_init_particle_fixed(const orc_model *m, orc_particle *p, double E0,
+                             orc_rng *rng) {
+  memset(p, 0, sizeof(*p));
+  p->region = 0;
+  p->valley = (int)floor((double)(size_t)m->nValleys * orc_rng_uniform(rng, 1e-6, 1.));
+  const orc_valley *v = &m->valley[p->valley];
+  p->subValley = (int)floor((double)(size_t)v->deg * orc_rng_uniform(rng, 1e-6, 1.));
+  p->energy = E0;
+  double nk = orc_valley_normk(v, E0);
   double r2 = orc_rng_uniform(rng, 0., 1.);
   double r1 = orc_rng_uniform(rng, 0., 1.);
   orc_init_random_direction(nk, r1, r2, p->k);
@@ -887,7 +1168,11 @@ static void orc_shared_add(const orc_model *m, orc_rng *rng, const size_t c[3],
   p.valley = (int)floor((double)(size_t)m->nValleys * orc_rng_uniform(rng, 1e-6, 1.));
   const orc_valley *v = &m->valley[p.valley];
   p.subValley = (int)floor((double)(size_t)v->deg * orc_rng_uniform(rng, 1e-6, 1.));
-  p.energy = -1.5 * Vt * log(orc_rng_uniform(rng, 1e-6, 1.));
+  /* initEnergyEV > 0: initParticleKSpaceFixed (emcParticleInitialization.hpp:57-71) */
+  if (m->initEnergy > 0.0)
+    p.energy = m->initEnergy;
+  else
+    p.energy = -1.5 * Vt * log(orc_rng_uniform(rng, 1e-6, 1.));
   double nk = orc_valley_normk(v, p.energy);
   double r2 = orc_rng_uniform(rng, 0., 1.);
   double r1 = orc_rng_uniform(rng, 0., 1.);
@@ -902,12 +1187,12 @@ static void orc_shared_add(const orc_model *m, orc_rng *rng, const size_t c[3],
 }
 
 /* ---- full reference bulk driver with one shared stream ---------------- */
-int orc_run_shared(const orc_model *m, uint64_t seed, const double maxPos[3],
+int orc_run_shared(orc_model *m, uint64_t seed, const double maxPos[3],
                    const double spacing[3], double doping, double T,
                    const double fieldDir[3], double fieldStrength,
                    double tStep, int nsteps, int maxParticles, double *obs,
                    double *kOut, double *EOut, int32_t *valleyOut,
-                   int32_t *subOut) {
+                   int32_t *subOut, double *bathMeanOut) {
   orc_rng rng;
   orc_rng_init_mt(&rng, seed);
   /* emcDevice: extent = round(maxPos/spacing)+1 (emcUtil.hpp:115-124),
@@ -972,6 +1257,10 @@ int orc_run_shared(const orc_model *m, uint64_t seed, const double maxPos[3],
       orc_move_particle(m, &parts[i], tStep, force, &rng, NULL, NULL);
     double *o = obs + (size_t)s * 3 * nv;
     orc_observables(m, np, parts, dir, o, o + nv, o + 2 * nv);
+    orc_model_end_step(m, tStep);
+    if (bathMeanOut)
+      for (int b = 0; b < m->nBaths; b++)
+        bathMeanOut[(size_t)(s - 1) * m->nBaths + b] = orc_bath_mean_nq(&m->bath[b]);
   }
   for (int i = 0; i < np; i++) {
     if (kOut) {
@@ -1028,6 +1317,42 @@ void orc_init_particle_philox(const orc_model *m, orc_particle *p, double Vt,
   p->grainTau = 1.0;
 }
 
+/* production initialiser at a prescribed energy: E = E0 + sigma * z with
+ * z = sqrt(-2 ln u2) cos(2 pi u6) (Box-Muller on the stream's uniforms 2
+ * and 6, u2 in [1e-6, 1): |z| <= 5.26), |E| taken if the draw is negative,
+ * E0 if it is zero; sigma = 0 gives the reference's fixed-energy
+ * initialisation (emcParticleInitialization.hpp:57-71) on Philox draws.
+ * Draws 0, 1, 3, 4, 5 as in orc_init_particle_philox. */
+void orc_init_particle_philox_energy(const orc_model *m, orc_particle *p,
+                                     double E0, double sigma, uint64_t seed,
+                                     uint32_t inst, uint32_t pid) {
+  const uint32_t s = 0xFFFFFFFFu;
+  memset(p, 0, sizeof(*p));
+  double uv = orc_prod_log_u(orc_prod_u(seed, inst, pid, s, 0));
+  double us = orc_prod_log_u(orc_prod_u(seed, inst, pid, s, 1));
+  double uphi = orc_prod_u(seed, inst, pid, s, 3);
+  double ucos = orc_prod_u(seed, inst, pid, s, 4);
+  double ut = orc_prod_log_u(orc_prod_u(seed, inst, pid, s, 5));
+  p->valley = (int)floor((double)m->nValleys * uv);
+  const orc_valley *v = &m->valley[p->valley];
+  p->subValley = (int)floor((double)v->deg * us);
+  double E = E0;
+  if (sigma > 0.0) {
+    double ue = orc_prod_log_u(orc_prod_u(seed, inst, pid, s, 2));
+    double ua = orc_prod_u(seed, inst, pid, s, 6);
+    double z = sqrt(-2.0 * log(ue)) * cos(2.0 * ORC_PI * ua);
+    E = E0 + sigma * z;
+    if (E < 0.0)
+      E = -E;
+    if (E == 0.0)
+      E = E0;
+  }
+  p->energy = E;
+  orc_init_random_direction(orc_valley_normk(v, p->energy), uphi, ucos, p->k);
+  p->tau = -log(ut) * m->tau0[p->valley];
+  p->grainTau = 1.0;
+}
+
 /* final state with explicit uniforms: ua, ub feed the direction (phi, cos)
  * for the isotropic kinds and (exponent, phi) for Froehlich; the final
  * subvalley index comes from the conditional uniform uc. */
@@ -1058,9 +1383,12 @@ static void orc_mech_scatter_u(const orc_model *md, const orc_mech *m,
     return;
   }
   case ORC_MECH_FROEHLICH_AB:
-  case ORC_MECH_FROEHLICH_EM: {
+  case ORC_MECH_FROEHLICH_EM:
+  case ORC_MECH_HOT_FROEHLICH_AB:
+  case ORC_MECH_HOT_FROEHLICH_EM: {
+    double kOld[3] = {p->k[0], p->k[1], p->k[2]};
     double initEnergy = p->energy;
-    if (m->kind == ORC_MECH_FROEHLICH_AB)
+    if (orc_is_froehlich_ab(m->kind))
       p->energy += m->phononEnergy;
     else
       p->energy -= m->phononEnergy;
@@ -1080,6 +1408,7 @@ static void orc_mech_scatter_u(const orc_model *md, const orc_mech *m,
       for (int i = 0; i < 3; i++)
         p->k[i] = p->k[i] * s;
     }
+    orc_hot_record(md, m, kOld, p->k);
     return;
   }
   }
@@ -1287,7 +1616,10 @@ void orc_init_particles_replay(const orc_model *m, int np, double Vt,
     orc_rng_init_buf(&r, draws + (size_t)i * stride, stride);
     r.pos = cursor[i];
     orc_particle p;
-    orc_init_particle(m, &p, Vt, &r);
+    if (m->initEnergy > 0.0)
+      orc_init_particle_fixed(m, &p, m->initEnergy, &r);
+    else
+      orc_init_particle(m, &p, Vt, &r);
     cursor[i] = r.pos;
     orc_unpack(&p, k + 3 * i, E + i, valley + i, sub + i);
     tau[i] = p.tau;
@@ -1316,7 +1648,7 @@ static void orc_obs_soa(const orc_model *m, int np, const double *k,
   }
 }
 
-void orc_run_philox(const orc_model *m, int np, double *k, double *E,
+void orc_run_philox(orc_model *m, int np, double *k, double *E,
                     double *tau, int32_t *valley, int32_t *sub, int nsteps,
                     double tStep, const double force[3],
                     const double fieldDir[3], uint64_t seed,
@@ -1338,6 +1670,7 @@ void orc_run_philox(const orc_model *m, int np, double *k, double *E,
     }
     if (obs)
       orc_obs_soa(m, np, k, E, valley, sub, fieldDir, obs + (size_t)s * 3 * nv);
+    orc_model_end_step(m, tStep);
   }
 }
 
@@ -1352,6 +1685,98 @@ void orc_init_particles_philox(const orc_model *m, int np, double Vt,
     tau[i] = p.tau;
   }
 }
+
+void orc_init_particles_philox_energy(const orc_model *m, int np, double E0,
+                                      double sigma, uint64_t seed,
+                                      uint32_t instance, double *k, double *E,
+                                      double *tau, int32_t *valley,
+                                      int32_t *sub) {
+  for (int i = 0; i < np; i++) {
+    orc_particle p;
+    orc_init_particle_philox_energy(m, &p, E0, sigma, seed, instance, (uint32_t)i);
+    orc_unpack(&p, k + 3 * i, E + i, valley + i, sub + i);
+    tau[i] = p.tau;
+  }
+}
+
+/* ---- bath handle API ---------------------------------------------------- */
+int orc_model_add_bath(orc_model *m, int nBins, double dq, double tauLO,
+                       double phononEnergy, double latticeTemp, double Vsim,
+                       int enableAcoustic, double acPhononEnergy,
+                       double tauAcoustic, double wRidley,
+                       double toPhononEnergy, double tauTO) {
+  if (m->nBaths >= ORC_MAX_BATHS || nBins < 1)
+    return -1;
+  orc_bath_init(&m->bath[m->nBaths], nBins, dq, tauLO, phononEnergy,
+                latticeTemp, Vsim, enableAcoustic, acPhononEnergy, tauAcoustic,
+                wRidley, toPhononEnergy, tauTO);
+  return m->nBaths++;
+}
+
+/* setTauLOProfile (emcPhononBath.hpp:214-216); NULL restores the uniform
+ * lifetime */
+void orc_model_set_bath_profile(orc_model *m, int b, const double *profile) {
+  orc_bath *bt = &m->bath[b];
+  free(bt->tauProfile);
+  bt->tauProfile = NULL;
+  if (profile) {
+    bt->tauProfile = (double *)malloc(sizeof(double) * (size_t)bt->nBins);
+    memcpy(bt->tauProfile, profile, sizeof(double) * (size_t)bt->nBins);
+  }
+}
+
+int orc_model_add_hot_froehlich(orc_model *m, int kind, int valley,
+                                double phononEnergy, double relEffMass,
+                                double epsHi, double epsLo, int bath) {
+  orc_mech mech;
+  orc_mech_hot_froehlich(&mech, kind, valley, phononEnergy, relEffMass, epsHi,
+                         epsLo, bath);
+  return orc_model_add_mech(m, &mech);
+}
+
+/* scal: getMeanNq, getMeanNac, getMeanNTO, getN0, getAcousticTemp,
+ * getTOTemp */
+void orc_model_get_bath(const orc_model *m, int b, double *Nq, double *nEm,
+                        double *nAbs, double *scal) {
+  const orc_bath *bt = &m->bath[b];
+  size_t n = (size_t)bt->nBins;
+  if (Nq)
+    memcpy(Nq, bt->Nq, n * sizeof(double));
+  if (nEm)
+    memcpy(nEm, bt->nEm, n * sizeof(double));
+  if (nAbs)
+    memcpy(nAbs, bt->nAbs, n * sizeof(double));
+  if (scal) {
+    scal[0] = orc_bath_mean_nq(bt);
+    scal[1] = bt->Nac;
+    scal[2] = bt->Nto;
+    scal[3] = bt->N0;
+    scal[4] = orc_bath_acoustic_temp(bt);
+    scal[5] = orc_bath_to_temp(bt);
+  }
+}
+
+void orc_model_set_bath(orc_model *m, int b, const double *Nq, double Nac,
+                        double Nto) {
+  orc_bath *bt = &m->bath[b];
+  memcpy(bt->Nq, Nq, sizeof(double) * (size_t)bt->nBins);
+  bt->Nac = Nac;
+  bt->Nto = Nto;
+}
+
+void orc_model_bath_record_one(orc_model *m, int b, double q, int emission) {
+  orc_bath_record(&m->bath[b], q, emission);
+}
+void orc_model_bath_update_one(orc_model *m, int b, double dt) {
+  orc_bath_update(&m->bath[b], dt);
+}
+void orc_model_set_bath_mode(orc_model *m, int update, int rebuildEvery,
+                             long long stepCount) {
+  m->bathUpdate = update;
+  m->rebuildEvery = rebuildEvery;
+  m->stepCount = stepCount;
+}
+void orc_model_set_init_energy(orc_model *m, double E0) { m->initEnergy = E0; }
 
 /* ---- mt19937_64 ensemble with per-step observables (threaded) --------- */
 #include <pthread.h>
@@ -1383,7 +1808,7 @@ static double orc_now(void) {
   return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
-void orc_run_mt(const orc_model *m, int np, uint64_t seed, double Vt,
+void orc_run_mt(orc_model *m, int np, uint64_t seed, double Vt,
                 int nsteps, double tStep, const double force[3],
                 const double fieldDir[3], double *obs, int nThreads,
                 double *kOut, double *EOut, int32_t *valleyOut,
@@ -1401,14 +1826,29 @@ void orc_run_mt(const orc_model *m, int np, uint64_t seed, double Vt,
     nThreads = 1;
   /* observables need every step: run the threads one step at a time when
    * obs is requested, else let each thread run all its steps */
-  int chunkSteps = obs ? 1 : nsteps;
+  int chunkSteps = (obs || m->nBaths) ? 1 : nsteps;
+  /* bath tallies: one private copy of the model per thread whose tally
+   * pointers are thread-local, summed in thread order after each step
+   * (integer-valued doubles: the sum is exact in any order) */
+  int ntUsed = nThreads > 256 ? 256 : nThreads;
+  orc_model *locals = NULL;
+  if (m->nBaths)
+    locals = (orc_model *)calloc((size_t)ntUsed, sizeof(orc_model));
   double t0 = orc_now();
   for (int s0 = 0; s0 < nsteps; s0 += chunkSteps) {
     pthread_t th[256];
     orc_mt_job jobs[256];
-    int nt = nThreads > 256 ? 256 : nThreads;
+    int nt = ntUsed;
     for (int t = 0; t < nt; t++) {
       jobs[t].m = m;
+      if (locals) {
+        locals[t] = *m;
+        for (int b = 0; b < m->nBaths; b++) {
+          locals[t].bath[b].nEm = (double *)calloc((size_t)m->bath[b].nBins, sizeof(double));
+          locals[t].bath[b].nAbs = (double *)calloc((size_t)m->bath[b].nBins, sizeof(double));
+        }
+        jobs[t].m = &locals[t];
+      }
       jobs[t].parts = parts;
       jobs[t].i0 = (int)((long long)np * t / nt);
       jobs[t].i1 = (int)((long long)np * (t + 1) / nt);
@@ -1425,7 +1865,20 @@ void orc_run_mt(const orc_model *m, int np, uint64_t seed, double Vt,
                       obs + (size_t)s0 * 3 * nv + nv,
                       obs + (size_t)s0 * 3 * nv + 2 * nv);
     }
+    if (locals) {
+      for (int t = 0; t < nt; t++)
+        for (int b = 0; b < m->nBaths; b++) {
+          for (int i = 0; i < m->bath[b].nBins; i++) {
+            m->bath[b].nEm[i] += locals[t].bath[b].nEm[i];
+            m->bath[b].nAbs[i] += locals[t].bath[b].nAbs[i];
+          }
+          free(locals[t].bath[b].nEm);
+          free(locals[t].bath[b].nAbs);
+        }
+      orc_model_end_step(m, tStep);
+    }
   }
+  free(locals);
   if (stepSeconds)
     *stepSeconds = orc_now() - t0;
   for (int i = 0; i < np; i++) {

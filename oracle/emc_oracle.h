@@ -33,6 +33,7 @@ extern "C" {
 #define ORC_MAX_Z 8
 #define ORC_MAX_VALLEYS 4
 #define ORC_MAX_MECH 32
+#define ORC_MAX_BATHS 4
 
 /* ---- random numbers ---------------------------------------------------- */
 /* std::mt19937_64 (include/emcUtil.hpp:15), the engine is fixed by the C++
@@ -140,7 +141,10 @@ enum {
   ORC_MECH_FIRST_IV_AB = 3,
   ORC_MECH_FIRST_IV_EM = 4,
   ORC_MECH_FROEHLICH_AB = 5,
-  ORC_MECH_FROEHLICH_EM = 6
+  ORC_MECH_FROEHLICH_EM = 6,
+  /* emcHotPhononFroehlichMechanism.hpp:39-118 / :122-200 */
+  ORC_MECH_HOT_FROEHLICH_AB = 7,
+  ORC_MECH_HOT_FROEHLICH_EM = 8
 };
 
 typedef struct {
@@ -154,7 +158,44 @@ typedef struct {
   int finalSub[ORC_MAX_DEG][ORC_MAX_Z];
   double effMass; /* Froehlich: relEffMass * me */
   double nBose;   /* Froehlich */
+  int bath;       /* hot-phonon Froehlich: index of the model's phonon bath */
 } orc_mech;
+
+/* ---- LO phonon bath: include/emcPhononBath.hpp ------------------------
+ * Non-equilibrium occupation N_q per wavevector bin with event tallies
+ * (:56-105), the optional acoustic (Klemens) and TO (Ridley) reservoirs and
+ * an optional per-bin lifetime profile.  The tallies are written through
+ * the pointers, so a const model can still record events. */
+typedef struct {
+  int nBins;
+  double dq, tauLO, N0, Vsim;
+  double *Nq, *nEm, *nAbs;
+  double *tauProfile; /* NULL = uniform tauLO */
+  double loE, latT;
+  int hasAc;
+  double acE, tauAc, Nac, NacEq;
+  int hasRidley;
+  double wR, toE, tauTO, Nto, NtoEq;
+} orc_bath;
+
+/* constructor (:160-192) */
+void orc_bath_init(orc_bath *b, int nBins, double dq, double tauLO,
+                   double phononEnergy, double latticeTemp, double Vsim,
+                   int enableAcoustic, double acPhononEnergy,
+                   double tauAcoustic, double wRidley, double toPhononEnergy,
+                   double tauTO);
+void orc_bath_free(orc_bath *b);
+/* binOf (:222-225) */
+int orc_bath_bin(const orc_bath *b, double q);
+/* recordEmission / recordAbsorption (:237-253) */
+void orc_bath_record(const orc_bath *b, double q, int emission);
+/* update (:264-358) */
+void orc_bath_update(orc_bath *b, double dt);
+/* getMeanNq (:461-470) */
+double orc_bath_mean_nq(const orc_bath *b);
+/* getAcousticTemp / getTOTemp (:480-494) */
+double orc_bath_acoustic_temp(const orc_bath *b);
+double orc_bath_to_temp(const orc_bath *b);
 
 void orc_mech_acoustic(orc_mech *m, int valley, double sigma, double rho,
                        double vSound, double T);
@@ -165,6 +206,11 @@ void orc_mech_iv(orc_mech *m, int kind, int valley, int finalValley,
 void orc_mech_froehlich(orc_mech *m, int kind, int valley,
                         double phononEnergy, double relEffMass, double epsHi,
                         double epsLo, double T);
+/* kind = ORC_MECH_HOT_FROEHLICH_{AB,EM} bound to bath index `bath`
+ * (emcHotPhononFroehlichMechanism.hpp:61-70) */
+void orc_mech_hot_froehlich(orc_mech *m, int kind, int valley,
+                            double phononEnergy, double relEffMass,
+                            double epsHi, double epsLo, int bath);
 
 /* ---- model: particle type + scatter handler (single region) ----------- */
 typedef struct {
@@ -180,6 +226,17 @@ typedef struct {
   int tableMech[ORC_MAX_VALLEYS][ORC_MAX_MECH];
   double *cum[ORC_MAX_VALLEYS];
   double tau0[ORC_MAX_VALLEYS];
+  int nBaths;
+  orc_bath bath[ORC_MAX_BATHS];
+  /* what the runners do at the end of every step when baths exist:
+   * bathUpdate = 1: orc_model_bath_step, rebuilding the tables on steps
+   * whose 1-based count is a multiple of rebuildEvery (the reference's
+   * reinitEvery, hotPhononGa2O3.cpp:104-109,280); 0: tallies only */
+  int bathUpdate, rebuildEvery;
+  long long stepCount;
+  /* emcElectron's initEnergyEV (emcElectron.hpp:27,84-85): > 0 selects the
+   * fixed-energy initialisation in orc_run_shared */
+  double initEnergy;
 } orc_model;
 
 void orc_model_init(orc_model *m, int nLevels, double maxEnergy);
@@ -198,6 +255,15 @@ int orc_scatter(const orc_model *m, orc_particle *p, orc_rng *rng);
 void orc_mech_scatter(const orc_model *m, const orc_mech *mech,
                       orc_particle *p, orc_rng *rng);
 double orc_new_tau(const orc_model *m, int valley, orc_rng *rng);
+int orc_model_has_hot(const orc_model *m);
+/* end of a time step of the reference's hot-phonon drivers
+ * (examples/hotPhononGa2O3/hotPhononGa2O3.cpp:274-281,
+ *  examples/hotCarrierMHP/hotCarrierMHP.cpp:693-703): every bath's
+ * update(dt), then reinitScatterTables (emcScatterHandler.hpp:100-108)
+ * when `rebuild` is set and a mechanism reads a bath */
+void orc_model_bath_step(orc_model *m, double dt, int rebuild);
+/* the runners' end-of-step hook described at orc_model.bathUpdate */
+void orc_model_end_step(orc_model *m, double dt);
 
 /* ---- bulk loop: examples/bulkSimulation/basicBulkParticleHandler.hpp --- */
 typedef struct {
@@ -216,7 +282,7 @@ void orc_move_particle(const orc_model *m, orc_particle *p, double tStep,
  * Outputs per step (optional, may be NULL): valley/sub/nEvents [nsteps*np],
  * kOut [nsteps*np*3], EOut [nsteps*np].  Returns number of particles that
  * ran out of draws. */
-int orc_run_replay(const orc_model *m, int np, double *k, double *E,
+int orc_run_replay(orc_model *m, int np, double *k, double *E,
                    double *tau, double *grainTau, int32_t *valley,
                    int32_t *sub, const uint64_t *draws, size_t stride,
                    size_t *cursor, int nsteps, double tStep,
@@ -229,6 +295,10 @@ int orc_run_replay(const orc_model *m, int np, double *k, double *E,
  * bulk coordinate taken as an interior node (no boundary flips). */
 void orc_init_particle(const orc_model *m, orc_particle *p, double Vt,
                        orc_rng *rng);
+/* the same with initParticleKSpaceFixed (emcParticleInitialization.hpp:
+ * 57-71): E = E0, isotropic k (emcElectron.hpp:84-85) */
+void orc_init_particle_fixed(const orc_model *m, orc_particle *p, double E0,
+                             orc_rng *rng);
 
 /* ensemble observables (basicBulkParticleHandler.hpp:289-347), per valley */
 void orc_observables(const orc_model *m, int np, const orc_particle *parts,
@@ -239,14 +309,16 @@ void orc_observables(const orc_model *m, int np, const orc_particle *parts,
  * mt19937_64 stream (single thread): generateInitialParticles (:143-159)
  * on a 3-D box, then nsteps of moveParticles with the per-step
  * observables of examples/bulkSimulation/bulkSimulation.cpp:150-156.
- * obs[(step)*(3*nValleys)] = avgE[v], avgV[v], occ[v] for step 0..nsteps.
+ * obs[(step)*(3*nValleys)] = avgE[v], avgV[v], occ[v] for step 0..nsteps;
+ * with baths, each bath's getMeanNq after every step goes to
+ * bathMeanOut[(step-1)*nBaths + b] (may be NULL).
  * Returns the number of particles created (or -needed if > maxParticles).*/
-int orc_run_shared(const orc_model *m, uint64_t seed, const double maxPos[3],
+int orc_run_shared(orc_model *m, uint64_t seed, const double maxPos[3],
                    const double spacing[3], double doping, double T,
                    const double fieldDir[3], double fieldStrength,
                    double tStep, int nsteps, int maxParticles, double *obs,
                    double *kOut, double *EOut, int32_t *valleyOut,
-                   int32_t *subOut);
+                   int32_t *subOut, double *bathMeanOut);
 
 /* ---- production mode (Philox draws, FP64) ------------------------------
  * The CPU twin of the GPU production kernel: same algorithm as
@@ -309,7 +381,7 @@ void orc_init_particles_replay(const orc_model *m, int np, double Vt,
                                int32_t *sub);
 /* production-mode twin over np particles (SoA, SI units); evCount may be
  * NULL; obs[(s)*3*nv ...] = avgE, avgV, occ per step (may be NULL) */
-void orc_run_philox(const orc_model *m, int np, double *k, double *E,
+void orc_run_philox(orc_model *m, int np, double *k, double *E,
                     double *tau, int32_t *valley, int32_t *sub, int nsteps,
                     double tStep, const double force[3],
                     const double fieldDir[3], uint64_t seed,
@@ -319,11 +391,38 @@ void orc_init_particles_philox(const orc_model *m, int np, double Vt,
                                uint64_t seed, uint32_t instance, double *k,
                                double *E, double *tau, int32_t *valley,
                                int32_t *sub);
+void orc_init_particle_philox_energy(const orc_model *m, orc_particle *p,
+                                     double E0, double sigma, uint64_t seed,
+                                     uint32_t inst, uint32_t pid);
+void orc_init_particles_philox_energy(const orc_model *m, int np, double E0,
+                                      double sigma, uint64_t seed,
+                                      uint32_t instance, double *k, double *E,
+                                      double *tau, int32_t *valley,
+                                      int32_t *sub);
+/* phonon baths and hot-phonon mechanisms */
+int orc_model_add_bath(orc_model *m, int nBins, double dq, double tauLO,
+                       double phononEnergy, double latticeTemp, double Vsim,
+                       int enableAcoustic, double acPhononEnergy,
+                       double tauAcoustic, double wRidley,
+                       double toPhononEnergy, double tauTO);
+void orc_model_set_bath_profile(orc_model *m, int b, const double *profile);
+int orc_model_add_hot_froehlich(orc_model *m, int kind, int valley,
+                                double phononEnergy, double relEffMass,
+                                double epsHi, double epsLo, int bath);
+void orc_model_get_bath(const orc_model *m, int b, double *Nq, double *nEm,
+                        double *nAbs, double *scal);
+void orc_model_set_bath(orc_model *m, int b, const double *Nq, double Nac,
+                        double Nto);
+void orc_model_bath_record_one(orc_model *m, int b, double q, int emission);
+void orc_model_bath_update_one(orc_model *m, int b, double dt);
+void orc_model_set_bath_mode(orc_model *m, int update, int rebuildEvery,
+                             long long stepCount);
+void orc_model_set_init_energy(orc_model *m, double E0);
 /* mt19937_64 ensemble (reference RNG, one stream per particle) with
  * per-step observables; the statistical baseline for production mode.
  * nThreads > 1 runs particles in parallel (pthreads); the wall time of
  * the step loop alone (no initialisation) goes to *stepSeconds. */
-void orc_run_mt(const orc_model *m, int np, uint64_t seed, double Vt,
+void orc_run_mt(orc_model *m, int np, uint64_t seed, double Vt,
                 int nsteps, double tStep, const double force[3],
                 const double fieldDir[3], double *obs, int nThreads,
                 double *kOut, double *EOut, int32_t *valleyOut,

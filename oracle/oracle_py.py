@@ -62,12 +62,24 @@ def load():
         "orc_run_replay": (_I, [vp, _I, _PD, _PD, _PD, _PD, _PI32, _PI32, _PU64, C.c_size_t, _PSZ, _I, _D, _PD,
                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
         "orc_run_shared": (_I, [vp, C.c_uint64, _PD, _PD, _D, _D, _PD, _D, _D, _I, _I, _PD,
-                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
         "orc_run_philox": (None, [vp, _I, _PD, _PD, _PD, _PI32, _PI32, _I, _D, _PD, _PD, C.c_uint64,
                                   C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
         "orc_init_particles_philox": (None, [vp, _I, _D, C.c_uint64, C.c_uint32, _PD, _PD, _PD, _PI32, _PI32]),
         "orc_run_mt": (None, [vp, _I, C.c_uint64, _D, _I, _D, _PD, _PD, C.c_void_p, _I,
                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "orc_init_particles_philox_energy": (None, [vp, _I, _D, _D, C.c_uint64, C.c_uint32, _PD, _PD, _PD,
+                                                    _PI32, _PI32]),
+        "orc_model_add_bath": (_I, [vp, _I, _D, _D, _D, _D, _D, _I, _D, _D, _D, _D, _D]),
+        "orc_model_set_bath_profile": (None, [vp, _I, C.c_void_p]),
+        "orc_model_add_hot_froehlich": (_I, [vp, _I, _I, _D, _D, _D, _D, _I]),
+        "orc_model_get_bath": (None, [vp, _I, _PD, _PD, _PD, _PD]),
+        "orc_model_set_bath": (None, [vp, _I, _PD, _D, _D]),
+        "orc_model_bath_record_one": (None, [vp, _I, _D, _I]),
+        "orc_model_bath_update_one": (None, [vp, _I, _D]),
+        "orc_model_bath_step": (None, [vp, _D, _I]),
+        "orc_model_set_bath_mode": (None, [vp, _I, _I, C.c_longlong]),
+        "orc_model_set_init_energy": (None, [vp, _D]),
         "orc_mt64_fill": (None, [C.c_uint64, _PU64, C.c_size_t]),
         "orc_mt64_fill_particles": (None, [C.c_uint64, C.c_uint64, C.c_size_t, _PU64, C.c_size_t, C.c_size_t]),
         "orc_particle_seed": (C.c_uint64, [C.c_uint64, C.c_uint64]),
@@ -98,6 +110,18 @@ class OracleModel:
         self.spec = spec
         self.h = L.orc_model_new(spec["n_levels"], spec["max_energy"])
         T = spec["T"]
+        # baths first: the hot-phonon rates read them when the tables are built
+        for b in spec.get("baths", []):
+            idx = L.orc_model_add_bath(self.h, b["n_bins"], b["dq"], b["tau_lo"], b["hw"], b.get("T", T),
+                                       b["volume"], int(b.get("acoustic", False)), b.get("ac_hw", 0.0),
+                                       b.get("tau_ac", 0.0), b.get("w_ridley", 0.0), b.get("to_hw", 0.0),
+                                       b.get("tau_to", 0.0))
+            if b.get("tau_profile") is not None:
+                prof = np.ascontiguousarray(b["tau_profile"], dtype=np.float64)
+                L.orc_model_set_bath_profile(self.h, idx, prof.ctypes.data_as(C.c_void_p))
+        self.n_baths = len(spec.get("baths", []))
+        if spec.get("init_energy"):
+            L.orc_model_set_init_energy(self.h, spec["init_energy"])
         for v in spec["valleys"]:
             axes = None
             if "axes" in v:
@@ -117,6 +141,9 @@ class OracleModel:
                 Z = len(mech["fmap"][0])
                 L.orc_model_add_iv(self.h, k, mech["valley"], mech["final"], mech["defpot"], mech["hw"],
                                    spec.get("rho", 2329.0), T, Z, fmap)
+            elif k in (7, 8):
+                L.orc_model_add_hot_froehlich(self.h, k, mech["valley"], mech["hw"], mech["mrel"],
+                                              mech["eps_hi"], mech["eps_lo"], mech["bath"])
             else:
                 L.orc_model_add_froehlich(self.h, k, mech["valley"], mech["hw"], mech["mrel"],
                                           mech["eps_hi"], mech["eps_lo"], T)
@@ -232,14 +259,49 @@ class OracleModel:
         return (None if o is None else o.reshape(nsteps, 3, nv)), fin
 
     def run_shared(self, seed, max_pos, spacing, doping, T, field_dir, field_strength, dt, nsteps,
-                   max_particles=100000):
+                   max_particles=100000, bath_history=False):
         nv = self.n_valleys
         obs = np.zeros((nsteps + 1) * 3 * nv)
+        hist = np.zeros(max(nsteps * self.n_baths, 1)) if bath_history else None
         n = self.L.orc_run_shared(self.h, seed, np.array(max_pos, dtype=np.float64),
                                   np.array(spacing, dtype=np.float64), doping, T,
                                   np.array(field_dir, dtype=np.float64), field_strength, dt, nsteps,
-                                  max_particles, obs, None, None, None, None)
+                                  max_particles, obs, None, None, None, None, _ptr(hist))
+        if bath_history:
+            return n, obs.reshape(nsteps + 1, 3, nv), hist[:nsteps * self.n_baths].reshape(nsteps, -1)
         return n, obs.reshape(nsteps + 1, 3, nv)
+
+    def init_philox_energy(self, np_, E0, sigma, seed, instance=0):
+        k = np.zeros(3 * np_)
+        E = np.zeros(np_)
+        tau = np.zeros(np_)
+        v = np.zeros(np_, dtype=np.int32)
+        s = np.zeros(np_, dtype=np.int32)
+        self.L.orc_init_particles_philox_energy(self.h, np_, E0, sigma, seed, instance, k, E, tau, v, s)
+        return dict(k=k, E=E, tau=tau, valley=v, sub=s)
+
+    # -- phonon baths -----------------------------------------------------
+    def bath(self, b=0):
+        n = self.spec["baths"][b]["n_bins"]
+        nq, em, ab, sc = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(6)
+        self.L.orc_model_get_bath(self.h, b, nq, em, ab, sc)
+        return dict(nq=nq, n_em=em, n_abs=ab, mean=sc[0], n_ac=sc[1], n_to=sc[2], n0=sc[3],
+                    t_ac=sc[4], t_to=sc[5])
+
+    def set_bath(self, b, nq, n_ac=0.0, n_to=0.0):
+        self.L.orc_model_set_bath(self.h, b, np.ascontiguousarray(nq, dtype=np.float64), n_ac, n_to)
+
+    def bath_record(self, b, q, emission):
+        self.L.orc_model_bath_record_one(self.h, b, q, int(emission))
+
+    def bath_update(self, b, dt):
+        self.L.orc_model_bath_update_one(self.h, b, dt)
+
+    def bath_step(self, dt, rebuild=True):
+        self.L.orc_model_bath_step(self.h, dt, int(rebuild))
+
+    def set_bath_mode(self, update=True, rebuild_every=1, step_count=0):
+        self.L.orc_model_set_bath_mode(self.h, int(update), rebuild_every, step_count)
 
 
 def mt64_fill(seed, n):

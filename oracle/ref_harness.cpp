@@ -23,6 +23,7 @@
 #include <ScatterMechanisms/emcAcousticScatterMechanism.hpp>
 #include <ScatterMechanisms/emcFirstOrderInterValleyScatterMechanism.hpp>
 #include <ScatterMechanisms/emcFroehlichInteraction.hpp>
+#include <ScatterMechanisms/emcHotPhononFroehlichMechanism.hpp>
 #include <ScatterMechanisms/emcZeroOrderInterValleyScatterMechanism.hpp>
 #include <ValleyTypes/emcNonParabolicAnistropValley.hpp>
 #include <ValleyTypes/emcNonParabolicIsotropValley.hpp>
@@ -31,6 +32,7 @@
 #include <emcConstants.hpp>
 #include <emcDevice.hpp>
 #include <emcParticleDrift.hpp>
+#include <emcPhononBath.hpp>
 
 #include "../examples/bulkSimulation/basicBulkParticleHandler.hpp"
 
@@ -86,6 +88,16 @@ struct Model {
   std::array<double, 3> fieldDir;
 };
 
+using Bath = emcPhononBath<T>;
+using BathPtr = std::shared_ptr<Bath>;
+
+// hot_polar: one Kane Gamma valley (m* 0.28, alpha 0.3) with hot-phonon
+// Froehlich absorption + emission (60 meV, eps 3.5 / 10) on a 256-bin bath
+static const double kHotDq = 2e7, kHotTauLO = 1e-12, kHotVsim = 2.7e-23, kHotE0 = 0.5;
+static BathPtr makeHotBath(const Model &m) {
+  return std::make_shared<Bath>(256, kHotDq, kHotTauLO, 0.06, m.T, kHotVsim);
+}
+
 static const double kSubAxes[3][3][3] = {
     {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}},
     {{0, 1, 0}, {1, 0, 0}, {0, 0, 1}},
@@ -123,7 +135,7 @@ static void addValleys(const Model &m, Electron &tp) {
           {kSubAxes[s][1][0], kSubAxes[s][1][1], kSubAxes[s][1][2]},
           {kSubAxes[s][2][0], kSubAxes[s][2][1], kSubAxes[s][2][2]});
     tp.addValley(std::move(v));
-  } else if (m.name == "polar_gamma") {
+  } else if (m.name == "polar_gamma" || m.name == "hot_polar") {
     tp.addValley(std::make_unique<emcNonParabolicIsotropValley<T>>(0.28, tp.getMass(), 1, 0.3));
   } else if (m.name == "para_iso") {
     tp.addValley(std::make_unique<emcParabolicIsotropValley<T>>(0.26, tp.getMass(), 1));
@@ -140,7 +152,8 @@ static void addValleys(const Model &m, Electron &tp) {
 }
 
 // Mechanisms of a model in registration order.
-static std::vector<MechPtr> makeMechs(const Model &m, const Device &device) {
+static std::vector<MechPtr> makeMechs(const Model &m, const Device &device,
+                                      BathPtr bath = nullptr) {
   std::vector<MechPtr> v;
   using ZAb = emcZeroOrderInterValleyAbsorptionScatterMechanism<T>;
   using ZEm = emcZeroOrderInterValleyEmissionScatterMechanism<T>;
@@ -174,6 +187,9 @@ static std::vector<MechPtr> makeMechs(const Model &m, const Device &device) {
   } else if (m.name == "polar_gamma") {
     v.push_back(std::make_unique<emcFroehlichAbsorption3D<T>>(0, 0.06, 0.28, 3.5, 10., m.T));
     v.push_back(std::make_unique<emcFroehlichEmission3D<T>>(0, 0.06, 0.28, 3.5, 10., m.T));
+  } else if (m.name == "hot_polar") {
+    v.push_back(std::make_unique<emcHotPhononFroehlichAbsorption3D<T>>(0, 0.06, 0.28, 3.5, 10., bath));
+    v.push_back(std::make_unique<emcHotPhononFroehlichEmission3D<T>>(0, 0.06, 0.28, 3.5, 10., bath));
   } else if (m.name == "para_iso") {
     std::map<SizeType, std::vector<SizeType>> self = {{0, {0}}};
     v.push_back(std::make_unique<emcAcousticScatterMechanism<T>>(0, 9., device));
@@ -196,10 +212,13 @@ static std::vector<MechPtr> makeMechs(const Model &m, const Device &device) {
 }
 
 // Builds the particle type for a model (valleys + mechanisms in order).
-static std::unique_ptr<Electron> makeType(const Model &m, Device &device) {
-  auto type = std::make_unique<Electron>(m.nLevels, m.maxE, false);
+// (hot_polar starts every particle at kHotE0 through emcElectron's
+// fixed-energy initialisation)
+static std::unique_ptr<Electron> makeType(const Model &m, Device &device, BathPtr bath = nullptr) {
+  auto type = std::make_unique<Electron>(m.nLevels, m.maxE, false,
+                                         m.name == "hot_polar" ? kHotE0 : 0.0);
   addValleys(m, *type);
-  for (auto &mech : makeMechs(m, device))
+  for (auto &mech : makeMechs(m, device, bath))
     type->addScatterMechanism({0}, std::move(mech));
   return type;
 }
@@ -210,8 +229,9 @@ static std::unique_ptr<Electron> makeType(const Model &m, Device &device) {
 
 // rates of every mechanism (standalone copies bound to the type's valleys)
 // on a sample grid and on the table grid (l+1)*dE, plus tau0 per valley
-static void dumpRates(const Model &m, const Device &device, Electron &type) {
-  auto mechs = makeMechs(m, device);
+static void dumpRates(const Model &m, const Device &device, Electron &type,
+                      BathPtr bath = nullptr) {
+  auto mechs = makeMechs(m, device, bath);
   std::vector<double> energies;
   for (int i = 0; i <= 400; i++)
     energies.push_back(m.maxE * 1.1 * i / 400.0);
@@ -249,7 +269,9 @@ struct RngCounter {
 };
 
 static void dumpTrajectories(const Model &m, Device &device, Electron &type,
-                             uint64_t seed, int np, int nsteps, double dt) {
+                             uint64_t seed, int np, int nsteps, double dt,
+                             BathPtr bath = nullptr) {
+  std::vector<double> bathMean, bathNq;
   std::array<T, 3> dir = m.fieldDir;
   normalize(dir);
   std::array<T, 3> force = scale(scale(dir, m.fieldKVcm * 1e5), type.getCharge());
@@ -320,8 +342,20 @@ static void dumpTrajectories(const Model &m, Device &device, Electron &type,
       nevOut.push_back(nev);
       drawsOut.push_back(RngCounter::between(atStart, rng));
     }
+    if (bath) {
+      // end of a hot-phonon step (hotPhononGa2O3.cpp:274-281)
+      bath->update(dt);
+      type.reinitScatterTables();
+      bathMean.push_back(bath->getMeanNq());
+      if (s % 5 == 4)
+        bathNq.insert(bathNq.end(), bath->Nq.begin(), bath->Nq.end());
+    }
   }
   std::string b = m.name + ".traj";
+  if (bath) {
+    sec((b + ".bath_mean").c_str(), bathMean);
+    sec((b + ".bath_nq").c_str(), bathNq);
+  }
   seci((b + ".meta").c_str(), {(long long)np, (long long)nsteps, (long long)seed});
   sec((b + ".dt").c_str(), {dt});
   sec((b + ".force").c_str(), {force[0], force[1], force[2]});
@@ -382,9 +416,11 @@ static void dumpDrift(const Model &m, Electron &type) {
 
 // final-state KATs: each mechanism's scatterParticle on many states.
 // Rows: mech, valley, sub, k0,k1,k2, E, seed | draws, valley', sub', k', E'
-static void dumpFinalStates(const Model &m, const Device &device, Electron &type) {
-  auto mechs = makeMechs(m, device);
+static void dumpFinalStates(const Model &m, const Device &device, Electron &type,
+                            BathPtr bath = nullptr) {
+  auto mechs = makeMechs(m, device, bath);
   std::vector<double> rows;
+  std::vector<long long> recorded;  // hot mechanisms: bin, emission per row
   emcRNG rng(777);
   std::uniform_real_distribution<T> u(0., 1.);
   for (SizeType j = 0; j < mechs.size(); j++) {
@@ -416,13 +452,35 @@ static void dumpFinalStates(const Model &m, const Device &device, Electron &type
       emcRNG before = r;
       rows.insert(rows.end(), {(double)j, (double)v, (double)part.subValley, part.k[0],
                                part.k[1], part.k[2], part.energy});
+      std::vector<T> em0, ab0;
+      if (bath) {
+        em0 = bath->nEm;
+        ab0 = bath->nAbs;
+      }
       mech->scatterParticle(part, r);
       rows.insert(rows.end(), {(double)RngCounter::between(before, r), (double)part.valley,
                                (double)part.subValley, part.k[0], part.k[1], part.k[2],
                                part.energy});
+      if (bath) {
+        long long bin = -1, emis = -1;
+        for (SizeType i = 0; i < bath->nrBins; i++) {
+          if (bath->nEm[i] != em0[i]) {
+            bin = (long long)i;
+            emis = 1;
+          }
+          if (bath->nAbs[i] != ab0[i]) {
+            bin = (long long)i;
+            emis = 0;
+          }
+        }
+        recorded.push_back(bin);
+        recorded.push_back(emis);
+      }
     }
   }
   sec(nm(m.name, "final").c_str(), rows);
+  if (bath)
+    seci(nm(m.name, "final_bins").c_str(), recorded);
   // seeds are reproducible: splitmix64(1000 + t + 7919 * j)
 }
 
@@ -432,8 +490,10 @@ static void dumpHandler(const Model &m, uint64_t seed, int nsteps, double dt,
   double mp = spacing * (nodes - 1);
   Device device{silicon(), {mp, mp, mp}, {spacing, spacing, spacing}, m.T};
   device.addConstantDopingRegion({0, 0, 0}, {mp, mp, mp}, doping);
+  BathPtr bath = (m.name == "hot_polar") ? makeHotBath(m) : nullptr;
+  std::vector<double> bathMean;
   TypeMap types;
-  types[0] = makeType(m, device);
+  types[0] = makeType(m, device, bath);
   Handler handler(device, types, m.fieldDir, m.fieldKVcm * 1e5, seed);
   handler.generateInitialParticles();
   std::vector<double> obs;
@@ -449,12 +509,82 @@ static void dumpHandler(const Model &m, uint64_t seed, int nsteps, double dt,
   for (int s = 0; s < nsteps; s++) {
     handler.moveParticles(dt);
     push();
+    if (bath) {
+      // hotPhononGa2O3.cpp:274-281 with reinitEvery = 1
+      bath->update(dt);
+      types[0]->reinitScatterTables();
+      bathMean.push_back(bath->getMeanNq());
+    }
   }
   std::string b = m.name + ".handler";
+  if (bath)
+    sec((b + ".bath_mean").c_str(), bathMean);
   seci((b + ".meta").c_str(), {(long long)handler.getNrParticles(0), (long long)nsteps,
                                 (long long)seed, (long long)nodes});
   sec((b + ".geom").c_str(), {dt, doping, spacing, m.T});
   sec((b + ".obs").c_str(), obs);
+}
+
+// emcPhononBath KATs: rounds of recorded events (q from a SplitMix64
+// stream, partly beyond the grid) followed by update(dt), for the plain
+// bath, the acoustic reservoir, acoustic + Ridley, and lifetime profiles
+static void dumpBathKATs() {
+  struct Cfg {
+    const char *name;
+    bool ac;
+    double wR;
+    bool profile;
+  };
+  const Cfg cfgs[] = {{"plain", false, 0, false},
+                      {"acoustic", true, 0, false},
+                      {"ridley", true, 0.3, false},
+                      {"profile", false, 0, true},
+                      {"acoustic_profile", true, 0.25, true}};
+  const int nb = 64;
+  const double dq = 5e7, dt = 2e-14;
+  for (const Cfg &c : cfgs) {
+    Bath bath(nb, dq, 1e-12, 0.06, 300., 2e-22, c.ac, 0.03, 1e-11, c.wR, 0.02, 5e-12);
+    if (c.profile) {
+      std::vector<T> prof(nb);
+      for (int i = 0; i < nb; i++)
+        prof[i] = 1e-12 * (0.5 + 1.5 * ((i + 0.5) * dq) / (nb * dq));
+      bath.setTauLOProfile(prof);
+    }
+    std::vector<double> evq, nq, scal;
+    std::vector<long long> evem, evn;
+    uint64_t st = 0x5EEDBA7Full;
+    for (int r = 0; r < 6; r++) {
+      int n = 50 + 37 * r;
+      // round 3 is absorption-heavy: drives low bins below zero (clamp)
+      double pEm = (r == 3) ? 0.1 : 0.75;
+      evn.push_back(n);
+      for (int e = 0; e < n; e++) {
+        st = splitmix64(st);
+        double u1 = (double)(st >> 11) * 0x1p-53;
+        st = splitmix64(st);
+        double u2 = (double)(st >> 11) * 0x1p-53;
+        double q = u1 * u1 * nb * dq * 1.1;
+        bool em = u2 < pEm;
+        evq.push_back(q);
+        evem.push_back(em ? 1 : 0);
+        if (em)
+          bath.recordEmission(q);
+        else
+          bath.recordAbsorption(q);
+      }
+      bath.update(dt);
+      nq.insert(nq.end(), bath.Nq.begin(), bath.Nq.end());
+      scal.insert(scal.end(), {bath.getMeanNq(), bath.getMeanNac(), bath.getMeanNTO(),
+                               bath.getN0(), bath.getAcousticTemp(), bath.getTOTemp()});
+    }
+    std::string b = std::string("bath.") + c.name;
+    sec((b + ".q").c_str(), evq);
+    seci((b + ".emission").c_str(), evem);
+    seci((b + ".n").c_str(), evn);
+    sec((b + ".nq").c_str(), nq);
+    sec((b + ".scal").c_str(), scal);
+  }
+  sec("bath.params", {64, 5e7, 1e-12, 0.06, 300., 2e-22, 0.03, 1e-11, 0.02, 5e-12, 2e-14});
 }
 
 int main(int argc, char **argv) {
@@ -490,18 +620,23 @@ int main(int argc, char **argv) {
       {"polar_gamma", 2000, 2.0, 300., 20., {0, 0, 1}},
       {"para_iso", 1000, 1.5, 77., 5., {1, 1, 0}},
       {"two_valley", 1500, 1.5, 300., 30., {1, 1, 1}},
+      {"hot_polar", 2000, 2.0, 300., 10., {0, 0, 1}},
   };
+  dumpBathKATs();
   const uint64_t seed = 20261202ULL;
   for (const auto &m : models) {
     double mp = 4e-8;
     Device device{silicon(), {mp, mp, mp}, {1e-8, 1e-8, 1e-8}, m.T};
     device.addConstantDopingRegion({0, 0, 0}, {mp, mp, mp}, 1e23);
-    auto type = makeType(m, device);
+    const bool hot = m.name == "hot_polar";
+    BathPtr bath = hot ? makeHotBath(m) : nullptr;
+    auto type = makeType(m, device, bath);
     type->initScatterTables();
-    dumpRates(m, device, *type);
+    dumpRates(m, device, *type, bath);
     dumpDrift(m, *type);
-    dumpFinalStates(m, device, *type);
-    dumpTrajectories(m, device, *type, seed, 64, 300, 1e-15);
+    // the final-state KATs record into a bath of their own
+    dumpFinalStates(m, device, *type, hot ? makeHotBath(m) : nullptr);
+    dumpTrajectories(m, device, *type, seed, 64, 300, 1e-15, bath);
     dumpHandler(m, seed + 1, 120, 1e-15, 5e24, 1e-8, 4);
   }
   std::fclose(out);

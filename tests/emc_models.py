@@ -16,7 +16,8 @@ Q = 1.60219e-19   # include/emcConstants.hpp:12
 ME = 9.11e-31     # include/emcConstants.hpp:24
 
 PARA_ISO, NONPARA_ISO, PARA_ANISO, NONPARA_ANISO = 0, 1, 2, 3
-ACOUSTIC, ZERO_IV_AB, ZERO_IV_EM, FIRST_IV_AB, FIRST_IV_EM, FROEHLICH_AB, FROEHLICH_EM = range(7)
+(ACOUSTIC, ZERO_IV_AB, ZERO_IV_EM, FIRST_IV_AB, FIRST_IV_EM, FROEHLICH_AB, FROEHLICH_EM,
+ HOT_FROEHLICH_AB, HOT_FROEHLICH_EM) = range(9)
 
 SI_RHO = 2329.0   # examples/SiliconFunctions.hpp:22
 SI_VSOUND = 9040.0  # examples/SiliconFunctions.hpp:24
@@ -126,7 +127,30 @@ MODELS = {
         mechs=_two_valley_mechs()),
 }
 
+# hot-phonon model: one Kane Gamma valley whose Froehlich absorption and
+# emission read and feed a 256-bin LO phonon bath (emcPhononBath), every
+# particle starting at 0.5 eV (emcElectron's fixed-energy initialisation)
+MODELS["hot_polar"] = dict(
+    n_levels=2000, max_energy=2.0, T=300.0, field_kvcm=10.0, field_dir=(0, 0, 1), init_energy=0.5,
+    valleys=[dict(kind=NONPARA_ISO, mass=(0.28, 0.28, 0.28), deg=1, alpha=0.3, bottom=0.0)],
+    baths=[dict(n_bins=256, dq=2e7, tau_lo=1e-12, hw=0.06, volume=2.7e-23)],
+    mechs=[dict(kind=HOT_FROEHLICH_AB, valley=0, hw=0.06, mrel=0.28, eps_hi=3.5, eps_lo=10.0, bath=0),
+           dict(kind=HOT_FROEHLICH_EM, valley=0, hw=0.06, mrel=0.28, eps_hi=3.5, eps_lo=10.0, bath=0)])
+
 MODEL_NAMES = list(MODELS)
+
+
+def photoexcited_cooling(n_particles=4_000_000):
+    """The hot-phonon cooling workload: 4M electrons in a Kane Gamma valley
+    (m* 0.28, alpha 0.3/eV), energies drawn from N(0.5 eV, 0.05 eV), hot-
+    phonon Froehlich absorption + emission (60 meV, eps 3.5 / 10) on a
+    256-bin LO bath (dq 2e7 /m, tau_LO 1 ps) in the volume that holds them
+    at 1e24 /m^3, no field, dt 0.5 fs."""
+    import copy
+    m = copy.deepcopy(MODELS["hot_polar"])
+    m.update(field_kvcm=0.0, init_energy=0.5, init_sigma=0.05)
+    m["baths"][0]["volume"] = n_particles / 1e24
+    return m
 
 
 def field_force(model):

@@ -122,25 +122,39 @@ def test_drift_velocity_bit_exact(name, golden, oracle_models):
 
 @pytest.mark.parametrize("name", NAMES)
 def test_final_states_bit_exact(name, golden, oracle_models):
-    """Every mechanism's scatterParticle: draws consumed, valley, subvalley, k, E."""
-    M = oracle_models[name]
+    """Every mechanism's scatterParticle: draws consumed, valley, subvalley, k,
+    E; hot-phonon mechanisms also the bath bin they record |k' - k| in."""
+    M = O.OracleModel(MODELS[name]) if MODELS[name].get("baths") else oracle_models[name]
     rows = golden[name + ".final"].reshape(-1, 14)
+    bins = golden[name + ".final_bins"].reshape(-1, 2) if M.n_baths else None
     for idx, r in enumerate(rows):
         j, t = int(r[0]), idx % 48
         draws = O.mt64_fill(_sm64(1000 + t + 7919 * j), 16)
+        b0 = M.bath(0) if M.n_baths else None
         k, E, vv, ss, used = M.mech_scatter(j, r[3:6], r[6], int(r[1]), int(r[2]), draws)
         assert used == int(r[7]), (name, idx)
         assert (vv, ss) == (int(r[8]), int(r[9])), (name, idx)
         assert (k == r[10:13]).all() and E == r[13], (name, idx)
+        if bins is not None:
+            b1 = M.bath(0)
+            dem = np.nonzero(b1["n_em"] != b0["n_em"])[0]
+            dab = np.nonzero(b1["n_abs"] != b0["n_abs"])[0]
+            if bins[idx, 1] == 1:
+                assert list(dem) == [bins[idx, 0]] and len(dab) == 0, (name, idx)
+            else:
+                assert list(dab) == [bins[idx, 0]] and len(dem) == 0, (name, idx)
 
 
 # ---- trajectories -----------------------------------------------------------
 
 @pytest.mark.parametrize("name", NAMES)
 def test_replay_trajectories_bit_exact(name, golden, oracle_models):
-    """Per-particle reference streams: init, 300 steps, every event and state."""
+    """Per-particle reference streams: init, 300 steps, every event and state.
+    Hot-phonon models also update the bath and rebuild the tables after
+    every step; the bath's mean occupation is compared step by step and its
+    N_q every fifth step."""
     spec = MODELS[name]
-    M = oracle_models[name]
+    M = O.OracleModel(spec)  # fresh: a bath evolves with the run
     b = name + ".traj."
     np_, ns, seed = [int(x) for x in golden[b + "meta"]]
     dt = golden[b + "dt"][0]
@@ -153,8 +167,24 @@ def test_replay_trajectories_bit_exact(name, golden, oracle_models):
                    ("valley", "init_valley"), ("sub", "init_sub")):
         assert (st[key] == golden[b + g]).all(), key
     assert (cur == golden[b + "init_draws"]).all()
-    nover, tr = M.run_replay(st, draws, cur, ns, dt, force, trace=True)
-    assert nover == 0
+    if M.n_baths:
+        tr = {key: [] for key in ("valley", "sub", "nev", "k", "E")}
+        mean, nq = [], []
+        for s in range(ns):
+            nover, t1 = M.run_replay(st, draws, cur, 1, dt, force, trace=True)
+            assert nover == 0
+            for key in tr:
+                tr[key].append(t1[key])
+            bt = M.bath(0)
+            mean.append(bt["mean"])
+            if s % 5 == 4:
+                nq.append(bt["nq"])
+        tr = {key: np.concatenate(v) for key, v in tr.items()}
+        assert (np.array(mean) == golden[b + "bath_mean"]).all()
+        assert (np.concatenate(nq) == golden[b + "bath_nq"]).all()
+    else:
+        nover, tr = M.run_replay(st, draws, cur, ns, dt, force, trace=True)
+        assert nover == 0
     shape = (ns, np_)
     assert (tr["valley"].reshape(shape) == golden[b + "valley"].reshape(shape)).all()
     assert (tr["sub"].reshape(shape) == golden[b + "sub"].reshape(shape)).all()
@@ -171,13 +201,16 @@ def test_bulk_handler_loop_bit_exact(name, golden, oracle_models):
     """The real basicBulkParticleHandler (shared stream): particle creation,
     moveParticles and the per-step observables of bulkSimulation.cpp."""
     spec = MODELS[name]
-    M = oracle_models[name]
+    M = O.OracleModel(spec)  # fresh: a bath evolves with the run
     hb = name + ".handler."
     n, nsteps, seed, nodes = [int(x) for x in golden[hb + "meta"]]
     dt, doping, spacing, T = golden[hb + "geom"]
     mp = spacing * (nodes - 1)
-    n2, obs = M.run_shared(seed, [mp] * 3, [spacing] * 3, doping, T, spec["field_dir"],
-                           spec["field_kvcm"] * 1e5, dt, nsteps)
+    n2, obs, hist = M.run_shared(seed, [mp] * 3, [spacing] * 3, doping, T, spec["field_dir"],
+                                 spec["field_kvcm"] * 1e5, dt, nsteps, bath_history=True)
     assert n2 == n
+    if M.n_baths:
+        # hotPhononGa2O3's loop: moveParticles, bath update, table rebuild
+        assert (hist[:, 0] == golden[hb + "bath_mean"]).all()
     ref = golden[hb + "obs"].reshape(nsteps + 1, 3, M.n_valleys)
     assert (obs == ref).all()
