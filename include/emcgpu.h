@@ -30,6 +30,8 @@ extern "C" {
 #define EMCGPU_MAX_Z 8
 #define EMCGPU_MAX_MECH 32
 #define EMCGPU_MAX_LEVELS 8192
+#define EMCGPU_MAX_BATHS 4
+#define EMCGPU_MAX_BINS 1024
 
 typedef struct emcgpu_ctx emcgpu_ctx;
 
@@ -59,7 +61,10 @@ typedef enum {
  *   ZERO_IV_*     emcZeroOrderInterValleyScatterMechanism.hpp:11-21,103-129,246-272
  *   FIRST_IV_*    emcFirstOrderInterValleyScatterMechanism.hpp:11-20,102-131,250-279
  *   FROEHLICH_*   emcFroehlichInteraction.hpp:45-57,95-128,167-201
- *   CUSTOM_*      rate supplied by the caller (emcgpu_set_rate_tables) with
+ *   HOT_FROEHLICH_* emcHotPhononFroehlichMechanism.hpp:39-200: the Froehlich
+ *                 rate with the occupation of a phonon bath (getMeanNq) and
+ *                 the Froehlich final state, recording |k' - k| in the bath
+ *   custom        rate supplied by the caller (emcgpu_set_rate_tables) with
  *                 one of the built-in final states */
 typedef enum {
   EMCGPU_MECH_ACOUSTIC = 0,
@@ -68,7 +73,9 @@ typedef enum {
   EMCGPU_MECH_FIRST_IV_AB = 3,
   EMCGPU_MECH_FIRST_IV_EM = 4,
   EMCGPU_MECH_FROEHLICH_AB = 5,
-  EMCGPU_MECH_FROEHLICH_EM = 6
+  EMCGPU_MECH_FROEHLICH_EM = 6,
+  EMCGPU_MECH_HOT_FROEHLICH_AB = 7,
+  EMCGPU_MECH_HOT_FROEHLICH_EM = 8
 } emcgpu_mech_kind;
 
 typedef struct {
@@ -103,7 +110,27 @@ typedef struct {
   double eps_hi, eps_lo; /* Froehlich: relative permittivities */
   int z;              /* IV: number of final subvalleys per initial one */
   int final_sub[EMCGPU_MAX_DEG][EMCGPU_MAX_Z]; /* IV: finalSubValleys map */
+  int bath;           /* HOT_FROEHLICH_*: index of the instance's phonon bath */
 } emcgpu_mech_desc;
+
+/* emcPhononBath constructor + setTauLOProfile (include/emcPhononBath.hpp:
+ * 160-192, 214-216): LO phonon occupation N_q on n_bins wavevector bins,
+ * optionally coupled to an acoustic (Klemens) and a TO (Ridley) reservoir */
+typedef struct {
+  int n_bins;              /* <= EMCGPU_MAX_BINS */
+  double dq;               /* bin width [1/m] */
+  double tau_lo;           /* LO lifetime [s] */
+  double phonon_energy;    /* hbar w_LO [eV] */
+  double lattice_temp;     /* [K] */
+  double volume;           /* simulated volume V_sim [m^3] */
+  int acoustic;            /* non-zero: enable the acoustic reservoir */
+  double ac_phonon_energy; /* hbar w_ac [eV] */
+  double tau_acoustic;     /* [s] */
+  double w_ridley;         /* Ridley branching ratio (0 = off) */
+  double to_phonon_energy; /* hbar w_TO [eV] */
+  double tau_to;           /* [s] */
+  const double *tau_profile; /* per-bin LO lifetime [s] (n_bins), or NULL */
+} emcgpu_bath_desc;
 
 typedef struct {
   double eps_r, rho, ni, v_sound; /* emcMaterial.hpp:23-25 (rho kg/m^3, v m/s) */
@@ -124,6 +151,15 @@ typedef struct {
                            reduction kernels) in a HIP graph and launch it;
                            the executable graph is updated in place on the
                            next call.  Ignored with timing or trace. */
+  /* instances with phonon baths: after every step each bath is updated
+   * (emcPhononBath::update) and the tables of instances whose mechanisms
+   * read a bath are rebuilt from it on the GPU, as the reference's
+   * hot-phonon drivers do (examples/hotPhononGa2O3/hotPhononGa2O3.cpp:
+   * 274-281).  Steps are then launched one at a time. */
+  int bath_record_only;  /* non-zero: only tally the events (no update) */
+  int rebuild_every;     /* rebuild after steps whose 1-based global index
+                            step0 + s + 1 is a multiple of this (reinitEvery
+                            of hotPhononGa2O3.cpp:104-109); 0 or 1 = every */
   /* replay (FP64 contexts only): per-particle recorded raw 64-bit draws of
    * the reference engine, host memory, particle order of instance 0 */
   const uint64_t *draws; /* [n_particles][draw_stride] */
@@ -193,6 +229,40 @@ emcgpu_status emcgpu_get_particles(const emcgpu_ctx *ctx, int instance,
  * emcElectron.hpp:75-90 + emcParticleInitialization.hpp:36-51), Philox */
 emcgpu_status emcgpu_init_particles(emcgpu_ctx *ctx, int instance,
                                     uint64_t seed, double temperature);
+/* initial state at a prescribed energy: sigma = 0 is emcElectron's
+ * fixed-energy photoexcitation (initEnergyEV > 0: emcElectron.hpp:84-85,
+ * initParticleKSpaceFixed, emcParticleInitialization.hpp:57-71); sigma > 0
+ * draws E from N(mean, sigma) by Box-Muller on two Philox uniforms
+ * (|E - mean| <= 5.26 sigma, |E| taken for a negative draw). */
+emcgpu_status emcgpu_init_particles_energy(emcgpu_ctx *ctx, int instance,
+                                           uint64_t seed, double mean_ev,
+                                           double sigma_ev);
+
+/* ---- phonon baths --------------------------------------------------------
+ * Baths of an instance (replacing the shared_ptr<emcPhononBath> given to
+ * the hot-phonon mechanisms, emcHotPhononFroehlichMechanism.hpp:45); set
+ * them before emcgpu_build_tables.  State starts at N_q = N0. */
+emcgpu_status emcgpu_set_baths(emcgpu_ctx *ctx, int instance,
+                               const emcgpu_bath_desc *baths, int n_baths);
+/* current state: nq[n_bins] (may be NULL) and scalars[6] = getMeanNq,
+ * getMeanNac, getMeanNTO, getN0, getAcousticTemp, getTOTemp
+ * (emcPhononBath.hpp:461-501) (may be NULL) */
+emcgpu_status emcgpu_get_bath(const emcgpu_ctx *ctx, int instance, int bath,
+                              double *nq, double *scalars);
+/* overwrite N_q and the reservoir occupations (tables follow on the next
+ * emcgpu_step) */
+emcgpu_status emcgpu_set_bath_state(emcgpu_ctx *ctx, int instance, int bath,
+                                    const double *nq, double n_ac,
+                                    double n_to);
+/* event tallies not yet consumed by an update (recordEmission /
+ * recordAbsorption, emcPhononBath.hpp:237-253), exact integers */
+emcgpu_status emcgpu_get_bath_tallies(const emcgpu_ctx *ctx, int instance,
+                                      int bath, uint64_t *n_em,
+                                      uint64_t *n_abs);
+/* getMeanNq after every step of the last emcgpu_step: mean_nq[n_steps] */
+emcgpu_status emcgpu_get_bath_history(const emcgpu_ctx *ctx, int instance,
+                                      int bath, double *mean_nq,
+                                      int *n_steps);
 
 /* ---- time stepping ------------------------------------------------------
  * moveParticles(dt) repeated n_steps times

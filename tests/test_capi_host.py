@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 from emc_models import MODELS
+from oracle import oracle_py as O
 from viennaemc_amd import emcgpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -46,8 +47,10 @@ def test_struct_layouts_match_header():
     # sizes computed by the C compiler for the descriptors of include/emcgpu.h
     assert C.sizeof(emcgpu.ValleyDesc) == 4 + 4 + 24 + 8 + 4 + 4 + 8 + 8 + 8 * 9 * 8
     assert C.sizeof(emcgpu.MechDesc) == 3 * 4 + 4 + 5 * 8 + 4 + 8 * 8 * 4 + 4
-    assert C.sizeof(emcgpu.StepArgs) == 8 + 4 * 3 + 4 + 8 + 4 + 4 + 4 + 4 + 8 * 9
-    assert emcgpu.StepArgs.draws.offset == 48
+    assert emcgpu.MechDesc.bath.offset == 316
+    assert C.sizeof(emcgpu.BathDesc) == 4 + 4 + 5 * 8 + 4 + 4 + 5 * 8 + 8
+    assert C.sizeof(emcgpu.StepArgs) == 8 + 4 * 3 + 4 + 8 + 4 + 4 + 4 + 4 + 4 + 4 + 8 * 9
+    assert emcgpu.StepArgs.draws.offset == 56
 
 
 def host_ctx(n_instances=1):
@@ -59,8 +62,10 @@ def test_host_tables_match_reference(name, golden, oracle_models):
     spec = MODELS[name]
     ctx = host_ctx()
     ctx.set_model(0, spec)
+    if spec.get("baths"):
+        ctx.set_baths(0, spec["baths"], spec["T"])
     ctx.build_tables(0, spec["n_levels"], spec["max_energy"], spec["T"])
-    M = oracle_models[name]
+    M = O.OracleModel(spec) if spec.get("baths") else oracle_models[name]
     for v in range(len(spec["valleys"])):
         cum, tau0 = ctx.get_tables(0, v, spec["n_levels"])
         # tau0 is the reference's getTau; the tables are the reference's
@@ -143,3 +148,66 @@ def test_rejects_negative_user_rates(golden):
     with pytest.raises(emcgpu.EmcGpuError) as e:
         ctx.set_rate_tables(0, spec["max_energy"], raw)
     assert "non-negative" in str(e.value)
+
+
+# ---- phonon baths (host-only context) ----------------------------------------
+
+def _hot_ctx(spec=None):
+    spec = spec or MODELS["hot_polar"]
+    ctx = host_ctx()
+    ctx.set_model(0, spec)
+    ctx.set_baths(0, spec["baths"], spec["T"])
+    return ctx, spec
+
+
+def test_bath_initial_state_matches_oracle():
+    ctx, spec = _hot_ctx()
+    M = O.OracleModel(spec)
+    got, ref = ctx.get_bath(0, 0), M.bath(0)
+    assert (got["nq"] == ref["nq"]).all()
+    for key in ("mean", "n_ac", "n_to", "n0", "t_ac", "t_to"):
+        assert got[key] == ref[key], key
+
+
+def test_bath_state_drives_host_tables():
+    """A hot bath state set before the table build gives the oracle's tables
+    for the same state (rates read getMeanNq)."""
+    ctx, spec = _hot_ctx()
+    nb = spec["baths"][0]["n_bins"]
+    rng = np.random.default_rng(5)
+    nq = rng.uniform(0.0, 3.0, nb)
+    ctx.set_bath_state(0, 0, nq, 0.0, 0.0)
+    M2 = O.OracleModel(spec)
+    M2.set_bath(0, nq)
+    M2.L.orc_model_build_tables(M2.h)
+    ctx.build_tables(0, spec["n_levels"], spec["max_energy"], spec["T"])
+    cum, tau0 = ctx.get_tables(0, 0, spec["n_levels"])
+    assert tau0 == M2.tau0(0)
+    assert (cum == M2.cum(0)).all()
+    assert ctx.get_bath(0, 0)["mean"] == M2.bath(0)["mean"]
+
+
+def test_bath_validation():
+    spec = MODELS["hot_polar"]
+    ctx = host_ctx()
+    ctx.set_model(0, spec)
+    with pytest.raises(emcgpu.EmcGpuError, match="set the baths first"):
+        ctx.build_tables(0, spec["n_levels"], spec["max_energy"], spec["T"])
+    bad = dict(spec["baths"][0], n_bins=0)
+    with pytest.raises(emcgpu.EmcGpuError, match="n_bins"):
+        ctx.set_baths(0, [bad], 300.0)
+    bad = dict(spec["baths"][0], volume=0.0)
+    with pytest.raises(emcgpu.EmcGpuError, match="must be > 0"):
+        ctx.set_baths(0, [bad], 300.0)
+    bad = dict(spec["baths"][0], acoustic=True, ac_hw=0.03, tau_ac=0.0)
+    with pytest.raises(emcgpu.EmcGpuError, match="acoustic"):
+        ctx.set_baths(0, [bad], 300.0)
+    ctx.set_baths(0, spec["baths"], 300.0)
+    with pytest.raises(emcgpu.EmcGpuError, match="follow their bath"):
+        ctx.set_rate_tables(0, spec["max_energy"], np.ones((2, spec["n_levels"])))
+    with pytest.raises(emcgpu.EmcGpuError, match="occupations"):
+        ctx.set_bath_state(0, 0, -np.ones(spec["baths"][0]["n_bins"]))
+    # a mechanism pointing past the supported baths
+    m2 = dict(spec, mechs=[dict(spec["mechs"][0], bath=7)])
+    with pytest.raises(emcgpu.EmcGpuError, match="bath index"):
+        ctx.set_model(0, m2)

@@ -24,7 +24,15 @@ namespace emcgpu {
 template <typename R>
 hipError_t launch_steps(const StepArgs<R> &a, int mode, int ppt, hipStream_t st);
 template <typename R>
-hipError_t launch_init(const StepArgs<R> &a, hipStream_t st);
+hipError_t launch_init(const StepArgs<R> &a, double init_mean, double init_sigma, hipStream_t st);
+hipError_t launch_bath_update(const BathDev *baths, int n_baths, double *nq,
+                              const double *tau_prof, unsigned long long *tally, BathVar *var,
+                              double dt, double *hist, int hist_steps, int hist_step,
+                              hipStream_t st);
+template <typename R>
+hipError_t launch_rebuild(const RebuildJob *jobs, int n_jobs, const double *arena,
+                          const BathVar *var, R *cum, double *cum64, InstDev<R> *inst,
+                          double *tau0_64, hipStream_t st);
 int reduce_chunks(long long max_count, int tile_size);
 hipError_t launch_reduce(const double *partial, int n_steps, int n_tiles, int n_obs, int tile_size,
                          const long long *inst_start, const long long *inst_count, int n_inst,
@@ -37,6 +45,15 @@ using namespace emcgpu;
 namespace {
 
 thread_local std::string g_create_error;
+
+// host copy of an LO phonon bath (emcPhononBath<double>, emcPhononBath.hpp:56-192)
+struct BathHost {
+  emcgpu_bath_desc d{};  // tau_profile pointer not kept
+  std::vector<double> profile;
+  std::vector<double> nq;
+  double n0 = 0, n_ac = 0, n_ac_eq = 0, n_to = 0, n_to_eq = 0, w_r = 0;
+  bool has_ac = false, has_ridley = false;
+};
 
 struct Instance {
   std::vector<emcgpu_valley_desc> valleys;
@@ -55,6 +72,18 @@ struct Instance {
   bool has_field = false;
   size_t count = 0;
   long long start = 0;
+  // phonon baths and the bath-dependent table data: per mechanism its FP64
+  // rates per level, or for a hot-phonon mechanism kI[level] then ln[level]
+  std::vector<BathHost> baths;
+  std::vector<std::vector<double>> rate_data;
+  std::vector<double> hot_C;
+  bool has_hot = false;
+  long long bath_index0 = 0;    // global index of the first bath
+  std::vector<long long> bath_off;  // global first bin of each bath
+  long long gbin_off = 0;
+  int total_bins = 0;
+  bool dev_bath = false;        // the device holds the current bath state
+  std::vector<long long> cum_off;  // per valley: element offset of the device tables
 };
 
 }  // namespace
@@ -92,8 +121,20 @@ struct emcgpu_ctx {
   long long launches = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipGraphExec_t gexec = nullptr;  // last captured step sequence
-  // autotuned launch shape of the production kernel (0 = not tuned)
-  int tuned_ppt = 0, tuned_spl = 0;
+  // phonon baths of all instances (layout built by upload_baths)
+  int n_baths = 0, lds_bins = 0, n_jobs = 0;
+  bool bath_layout_dirty = true;
+  BathDev *d_baths = nullptr;
+  double *d_nq = nullptr, *d_tau_prof = nullptr;
+  unsigned long long *d_tally = nullptr;
+  BathVar *d_bath_var = nullptr;
+  double *d_bath_hist = nullptr;
+  size_t bath_hist_bytes = 0;
+  int bath_hist_steps = 0;
+  RebuildJob *d_jobs = nullptr;
+  double *d_rate_arena = nullptr;
+  double *d_cum64 = nullptr, *d_tau0_64 = nullptr;
+  size_t cum64_bytes = 0;
 };
 
 namespace {
@@ -235,14 +276,205 @@ void lower_instance(const emcgpu_ctx *c, const Instance &in, double dt_s, InstDe
             mm.final_sub[s][z] = (uint8_t)md.final_sub[s][z];
         break;
       }
-      default:
+      default: {
+        const bool ab = md.kind == EMCGPU_MECH_FROEHLICH_AB || md.kind == EMCGPU_MECH_HOT_FROEHLICH_AB;
         mm.final_kind = kFinalFroehlich;
-        mm.shift = (md.kind == EMCGPU_MECH_FROEHLICH_AB) ? (R)md.phonon_energy
-                                                          : (R)(-md.phonon_energy);
+        mm.shift = ab ? (R)md.phonon_energy : (R)(-md.phonon_energy);
+        if (md.kind == EMCGPU_MECH_HOT_FROEHLICH_AB || md.kind == EMCGPU_MECH_HOT_FROEHLICH_EM)
+          mm.bath1 = md.bath + 1;
         break;
+      }
       }
     }
   }
+  P.n_baths = (int)in.baths.size();
+  P.total_bins = in.total_bins;
+  P.gbin_off = in.gbin_off;
+  int loff = 0;
+  for (int b = 0; b < P.n_baths; b++) {
+    P.bath_nb[b] = in.baths[b].d.n_bins;
+    P.bath_loff[b] = loff;
+    P.bath_dq[b] = (R)(in.baths[b].d.dq * kToNm);
+    loff += in.baths[b].d.n_bins;
+  }
+}
+
+// ---- phonon baths -----------------------------------------------------
+// planck / planckTemp / getMeanNq (emcPhononBath.hpp:127-137, 461-470)
+double planck(double energy_ev, double temp_k) {
+  const double x = phys::q * energy_ev / (phys::kB * temp_k);
+  return 1.0 / (std::exp(x) - 1.0);
+}
+double planck_temp(double energy_ev, double n) {
+  if (n <= 0.0)
+    return 0.0;
+  return phys::q * energy_ev / (phys::kB * std::log(1.0 + 1.0 / n));
+}
+double bath_mean(const BathHost &b) {
+  double sum_w = 0.0, sum_wn = 0.0;
+  for (int i = 0; i < b.d.n_bins; i++) {
+    const double q = ((double)(std::size_t)i + 0.5) * b.d.dq;
+    const double w = q * q;
+    sum_w += w;
+    sum_wn += w * b.nq[i];
+  }
+  return (sum_w > 0.0) ? sum_wn / sum_w : b.n0;
+}
+
+// copies the evolving device state of an instance's baths to the host copy
+emcgpu_status pull_baths(emcgpu_ctx *c, Instance &in) {
+  if (!in.dev_bath || c->device < 0)
+    return EMCGPU_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (size_t b = 0; b < in.baths.size(); b++) {
+    BathHost &B = in.baths[b];
+    HIPCHK(c, hipMemcpy(B.nq.data(), c->d_nq + in.bath_off[b], sizeof(double) * B.d.n_bins,
+                        hipMemcpyDeviceToHost));
+    BathVar v;
+    HIPCHK(c, hipMemcpy(&v, c->d_bath_var + in.bath_index0 + (long long)b, sizeof(BathVar),
+                        hipMemcpyDeviceToHost));
+    B.n_ac = v.n_ac;
+    B.n_to = v.n_to;
+  }
+  return EMCGPU_OK;
+}
+
+// (re)creates the device bath arrays of all instances from their host state
+emcgpu_status upload_bath_state(emcgpu_ctx *c) {
+  for (auto &in : c->inst) {
+    emcgpu_status st = pull_baths(c, in);
+    if (st)
+      return st;
+  }
+  std::vector<BathDev> bd;
+  std::vector<BathVar> bv;
+  std::vector<double> nq, prof;
+  int lds = 0;
+  for (auto &in : c->inst) {
+    in.bath_index0 = (long long)bd.size();
+    in.gbin_off = (long long)nq.size();
+    in.bath_off.clear();
+    in.total_bins = 0;
+    for (const BathHost &B : in.baths) {
+      BathDev d{};
+      d.n_bins = B.d.n_bins;
+      d.has_ac = B.has_ac;
+      d.has_ridley = B.has_ridley;
+      d.has_profile = !B.profile.empty();
+      d.off = (long long)nq.size();
+      d.dq = B.d.dq;
+      d.tau_lo = B.d.tau_lo;
+      d.n0 = B.n0;
+      d.vsim = B.d.volume;
+      d.lo_e = B.d.phonon_energy;
+      d.lat_t = B.d.lattice_temp;
+      d.ac_e = B.d.ac_phonon_energy;
+      d.tau_ac = B.d.tau_acoustic;
+      d.n_ac_eq = B.n_ac_eq;
+      d.w_r = B.w_r;
+      d.to_e = B.d.to_phonon_energy;
+      d.tau_to = B.d.tau_to;
+      d.n_to_eq = B.n_to_eq;
+      in.bath_off.push_back(d.off);
+      bd.push_back(d);
+      bv.push_back(BathVar{B.n_ac, B.n_to, bath_mean(B), 0.0});
+      nq.insert(nq.end(), B.nq.begin(), B.nq.end());
+      if (B.profile.empty())
+        prof.insert(prof.end(), B.d.n_bins, 0.0);
+      else
+        prof.insert(prof.end(), B.profile.begin(), B.profile.end());
+      in.total_bins += B.d.n_bins;
+    }
+    lds = std::max(lds, in.total_bins);
+  }
+  dfreet(c->d_baths);
+  dfreet(c->d_nq);
+  dfreet(c->d_tau_prof);
+  dfreet(c->d_tally);
+  dfreet(c->d_bath_var);
+  c->n_baths = (int)bd.size();
+  c->lds_bins = lds;
+  if (c->n_baths) {
+    HIPCHK(c, hipMalloc((void **)&c->d_baths, sizeof(BathDev) * bd.size()));
+    HIPCHK(c, hipMalloc((void **)&c->d_nq, sizeof(double) * nq.size()));
+    HIPCHK(c, hipMalloc((void **)&c->d_tau_prof, sizeof(double) * prof.size()));
+    HIPCHK(c, hipMalloc((void **)&c->d_tally, sizeof(unsigned long long) * 2 * nq.size()));
+    HIPCHK(c, hipMalloc((void **)&c->d_bath_var, sizeof(BathVar) * bv.size()));
+    HIPCHK(c, hipMemcpy(c->d_baths, bd.data(), sizeof(BathDev) * bd.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_nq, nq.data(), sizeof(double) * nq.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_tau_prof, prof.data(), sizeof(double) * prof.size(),
+                        hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(c->d_tally, 0, sizeof(unsigned long long) * 2 * nq.size()));
+    HIPCHK(c, hipMemcpy(c->d_bath_var, bv.data(), sizeof(BathVar) * bv.size(),
+                        hipMemcpyHostToDevice));
+  }
+  for (auto &in : c->inst)
+    in.dev_bath = !in.baths.empty();
+  c->bath_layout_dirty = false;
+  return EMCGPU_OK;
+}
+
+// rebuild jobs, their FP64 rate data and the FP64 table copies, then one
+// rebuild so the device tables follow the current bath state
+template <typename R>
+emcgpu_status upload_rebuild(emcgpu_ctx *c, size_t cum_elems) {
+  std::vector<RebuildJob> jobs;
+  std::vector<double> arena;
+  for (int i = 0; i < c->n_inst; i++) {
+    const Instance &in = c->inst[i];
+    if (!in.has_hot || !in.has_tables)
+      continue;
+    for (size_t v = 0; v < in.valleys.size(); v++) {
+      const auto &tm = in.table_mech[v];
+      bool hot = false;
+      for (int j : tm)
+        hot |= in.mechs[j].kind == EMCGPU_MECH_HOT_FROEHLICH_AB ||
+               in.mechs[j].kind == EMCGPU_MECH_HOT_FROEHLICH_EM;
+      if (!hot)
+        continue;
+      RebuildJob J{};
+      J.inst = i;
+      J.valley = (int)v;
+      J.n_tables = (int)tm.size();
+      J.n_levels = in.n_levels;
+      J.cum_off = in.cum_off[v];
+      for (int t = 0; t < J.n_tables; t++) {
+        const emcgpu_mech_desc &md = in.mechs[tm[t]];
+        J.rate_off[t] = (long long)arena.size();
+        arena.insert(arena.end(), in.rate_data[tm[t]].begin(), in.rate_data[tm[t]].end());
+        if (md.kind == EMCGPU_MECH_HOT_FROEHLICH_AB || md.kind == EMCGPU_MECH_HOT_FROEHLICH_EM) {
+          J.kind[t] = md.kind == EMCGPU_MECH_HOT_FROEHLICH_AB ? kRateHotAb : kRateHotEm;
+          J.bath[t] = (int)(in.bath_index0 + md.bath);
+          J.C[t] = in.hot_C[tm[t]];
+        } else {
+          J.kind[t] = kRateConst;
+        }
+      }
+      jobs.push_back(J);
+    }
+  }
+  dfreet(c->d_jobs);
+  dfreet(c->d_rate_arena);
+  c->n_jobs = (int)jobs.size();
+  if (jobs.empty())
+    return EMCGPU_OK;
+  HIPCHK(c, hipMalloc((void **)&c->d_jobs, sizeof(RebuildJob) * jobs.size()));
+  HIPCHK(c, hipMalloc((void **)&c->d_rate_arena, sizeof(double) * std::max<size_t>(arena.size(), 1)));
+  HIPCHK(c, hipMemcpy(c->d_jobs, jobs.data(), sizeof(RebuildJob) * jobs.size(),
+                      hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->d_rate_arena, arena.data(), sizeof(double) * arena.size(),
+                      hipMemcpyHostToDevice));
+  if (cum_elems * sizeof(double) > c->cum64_bytes) {
+    dfreet(c->d_cum64);
+    HIPCHK(c, hipMalloc((void **)&c->d_cum64, cum_elems * sizeof(double)));
+    c->cum64_bytes = cum_elems * sizeof(double);
+  }
+  if (!c->d_tau0_64)
+    HIPCHK(c, hipMalloc((void **)&c->d_tau0_64, sizeof(double) * kMaxV * c->n_inst));
+  HIPCHK(c, launch_rebuild<R>(c->d_jobs, c->n_jobs, c->d_rate_arena, c->d_bath_var, (R *)c->d_cum,
+                              c->d_cum64, (InstDev<R> *)c->d_params, c->d_tau0_64, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return EMCGPU_OK;
 }
 
 // builds the parameter blocks and the table arena of every instance
@@ -250,6 +482,11 @@ template <typename R>
 emcgpu_status upload_params(emcgpu_ctx *c, double dt_s) {
   if (!c->params_dirty && dt_s == c->params_dt)
     return EMCGPU_OK;
+  if (c->bath_layout_dirty) {
+    emcgpu_status st = upload_bath_state(c);
+    if (st)
+      return st;
+  }
   std::vector<InstDev<R>> P(c->n_inst);
   std::vector<R> cum;
   for (int i = 0; i < c->n_inst; i++) {
@@ -265,6 +502,7 @@ emcgpu_status upload_params(emcgpu_ctx *c, double dt_s) {
         for (int j = 0; j < nt; j++)
           cum.push_back((R)in.cum[v][(size_t)j * in.n_levels + l]);
     }
+    c->inst[i].cum_off = off;
     lower_instance<R>(c, in, dt_s, P[i], off);
   }
   if (cum.empty())
@@ -282,6 +520,9 @@ emcgpu_status upload_params(emcgpu_ctx *c, double dt_s) {
                            hipMemcpyHostToDevice, c->stream));
   // the host vectors die here: wait for the copies
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  emcgpu_status st = upload_rebuild<R>(c, cum.size());
+  if (st)
+    return st;
   c->params_dirty = false;
   c->params_dt = dt_s;
   return EMCGPU_OK;
@@ -356,6 +597,23 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
     return st;
   int K = s->steps_per_launch > 0 ? s->steps_per_launch : std::min(s->n_steps, 64);
   K = std::min(K, s->n_steps);
+  // phonon baths couple the particles between steps: one step per launch,
+  // then the bath update and the table rebuild
+  const bool baths = c->n_baths > 0;
+  if (baths)
+    K = 1;
+  const int every = s->rebuild_every > 1 ? s->rebuild_every : 1;
+  if (baths && !s->bath_record_only) {
+    const size_t hb = sizeof(double) * (size_t)c->n_baths * (size_t)s->n_steps;
+    if (hb > c->bath_hist_bytes) {
+      dfreet(c->d_bath_hist);
+      HIPCHK(c, hipMalloc((void **)&c->d_bath_hist, hb));
+      c->bath_hist_bytes = hb;
+    }
+    c->bath_hist_steps = s->n_steps;
+  } else {
+    c->bath_hist_steps = 0;
+  }
   // particles per lane of the production kernel (replay: one per lane)
   const int ppt = replay ? 1 : (ppl ? ppl : default_ppt(c->precision));
   const int tile_size = kTile * ppt;
@@ -382,6 +640,10 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
   a.key1 = (uint32_t)(s->seed >> 32);
   a.partial = c->d_partial;
   a.dt_s = s->dt;
+  if (baths) {
+    a.tally = c->d_tally;
+    a.lds_bins = c->lds_bins;
+  }
   const size_t n0 = c->inst[0].count;
 
   // replay and trace buffers (instance 0)
@@ -479,6 +741,17 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
       HIPCHK_CLEAN(hipEventElapsedTime(&ms, c->ev0, c->ev1));
       c->ms_total += ms;
       c->launches += 1;
+    }
+    if (baths && !s->bath_record_only) {
+      // end of the step of the hot-phonon drivers (hotPhononGa2O3.cpp:274-281)
+      HIPCHK_CLEAN(launch_bath_update(c->d_baths, c->n_baths, c->d_nq, c->d_tau_prof, c->d_tally,
+                                      c->d_bath_var, s->dt, c->d_bath_hist, s->n_steps, done,
+                                      c->stream));
+      const long long gstep = (long long)s->step0 + done + 1;  // 1-based global step
+      if (c->n_jobs && gstep % every == 0)
+        HIPCHK_CLEAN(launch_rebuild<R>(c->d_jobs, c->n_jobs, c->d_rate_arena, c->d_bath_var,
+                                       (R *)c->d_cum, c->d_cum64, (InstDev<R> *)c->d_params,
+                                       c->d_tau0_64, c->stream));
     }
     HIPCHK_CLEAN(launch_reduce(c->d_partial, k, n_tiles, c->n_obs, tile_size, c->d_inst_start,
                                c->d_inst_count, c->n_inst, n_chunks, c->d_chunks, c->d_obs, done,
@@ -583,7 +856,8 @@ emcgpu_status finish_tables(emcgpu_ctx *c, Instance &in, int n_levels, double ma
 }
 
 template <typename R>
-emcgpu_status init_thermal(emcgpu_ctx *c, int instance, uint64_t seed) {
+emcgpu_status init_thermal(emcgpu_ctx *c, int instance, uint64_t seed, double mean = 0.0,
+                           double sigma = 0.0) {
   const Instance &in = c->inst[instance];
   if (!in.has_tables)
     return fail(c, EMCGPU_ERR_STATE, "instance %d: build tables before initialising", instance);
@@ -597,7 +871,7 @@ emcgpu_status init_thermal(emcgpu_ctx *c, int instance, uint64_t seed) {
   // one instance: its tiles only
   a.tile_base = (int)(in.start / kTile);
   a.n_tiles = (int)((in.count + kTile - 1) / kTile);
-  HIPCHK(c, launch_init<R>(a, c->stream));
+  HIPCHK(c, launch_init<R>(a, mean, sigma, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return EMCGPU_OK;
 }
@@ -742,6 +1016,16 @@ void emcgpu_destroy(emcgpu_ctx *c) {
   dfreet(c->d_partial);
   dfreet(c->d_chunks);
   dfreet(c->d_obs);
+  dfreet(c->d_baths);
+  dfreet(c->d_nq);
+  dfreet(c->d_tau_prof);
+  dfreet(c->d_tally);
+  dfreet(c->d_bath_var);
+  dfreet(c->d_bath_hist);
+  dfreet(c->d_jobs);
+  dfreet(c->d_rate_arena);
+  dfreet(c->d_cum64);
+  dfreet(c->d_tau0_64);
   if (c->gexec)
     (void)hipGraphExecDestroy(c->gexec);
   if (c->ev0)
@@ -795,8 +1079,11 @@ emcgpu_status emcgpu_set_model(emcgpu_ctx *c, int instance, const emcgpu_valley_
   }
   for (int j = 0; j < n_mechs; j++) {
     const emcgpu_mech_desc &m = mechs[j];
-    if (m.kind < 0 || m.kind > 6)
+    if (m.kind < 0 || m.kind > EMCGPU_MECH_HOT_FROEHLICH_EM)
       return fail(c, EMCGPU_ERR_ARG, "mechanism %d: unknown kind %d", j, m.kind);
+    if ((m.kind == EMCGPU_MECH_HOT_FROEHLICH_AB || m.kind == EMCGPU_MECH_HOT_FROEHLICH_EM) &&
+        (m.bath < 0 || m.bath >= EMCGPU_MAX_BATHS))
+      return fail(c, EMCGPU_ERR_ARG, "mechanism %d: bath index %d out of range", j, m.bath);
     if (m.valley < 0 || m.valley >= n_valleys)
       return fail(c, EMCGPU_ERR_ARG, "mechanism %d: Used index for Valley is invalid", j);
     if (m.kind >= EMCGPU_MECH_ZERO_IV_AB && m.kind <= EMCGPU_MECH_FIRST_IV_EM) {
@@ -813,7 +1100,7 @@ emcgpu_status emcgpu_set_model(emcgpu_ctx *c, int instance, const emcgpu_valley_
                         "mechanism %d: Used idx %d for valley of degeneracy %d is not valid", j,
                         m.final_sub[s][z], fdeg);
     }
-    if ((m.kind >= EMCGPU_MECH_ZERO_IV_AB && m.kind <= EMCGPU_MECH_FROEHLICH_EM) &&
+    if ((m.kind >= EMCGPU_MECH_ZERO_IV_AB && m.kind <= EMCGPU_MECH_HOT_FROEHLICH_EM) &&
         !(m.phonon_energy > 0))
       return fail(c, EMCGPU_ERR_ARG, "mechanism %d: phonon energy must be > 0", j);
   }
@@ -824,6 +1111,9 @@ emcgpu_status emcgpu_set_model(emcgpu_ctx *c, int instance, const emcgpu_valley_
   for (const auto &d : in.valleys)
     in.vm.emplace_back(d.kind, d.rel_mass, d.particle_mass, d.alpha, d.bottom_energy);
   in.has_tables = false;
+  in.has_hot = false;
+  for (const auto &m : in.mechs)
+    in.has_hot |= m.kind == EMCGPU_MECH_HOT_FROEHLICH_AB || m.kind == EMCGPU_MECH_HOT_FROEHLICH_EM;
   // observable layout covers the largest model of the context
   int mv = 1, md = 1;
   for (const auto &i2 : c->inst) {
@@ -849,9 +1139,13 @@ emcgpu_status emcgpu_set_rate_tables(emcgpu_ctx *c, int instance, int n_levels,
   if (n_levels < 1 || n_levels > EMCGPU_MAX_LEVELS || !(max_energy > 0) || !rates)
     return fail(c, EMCGPU_ERR_ARG, "n_levels in [1, %d], max_energy > 0 and rates required",
                 EMCGPU_MAX_LEVELS);
+  if (in.has_hot)
+    return fail(c, EMCGPU_ERR_ARG,
+                "hot-phonon rates follow their bath: build them with emcgpu_build_tables");
   std::vector<std::vector<double>> raw(in.mechs.size());
   for (size_t j = 0; j < in.mechs.size(); j++)
     raw[j].assign(rates + j * n_levels, rates + (j + 1) * n_levels);
+  in.rate_data = raw;
   // selection relies on non-decreasing cumulative rows
   for (size_t j = 0; j < raw.size(); j++)
     for (int l = 0; l < n_levels; l++)
@@ -877,13 +1171,55 @@ emcgpu_status emcgpu_build_tables(emcgpu_ctx *c, int instance, int n_levels, dou
                 EMCGPU_MAX_LEVELS);
   in.temperature = temperature;
   const double dE = max_energy / (double)(std::size_t)n_levels;
+  for (size_t j = 0; j < in.mechs.size(); j++) {
+    const emcgpu_mech_desc &m = in.mechs[j];
+    if ((m.kind == EMCGPU_MECH_HOT_FROEHLICH_AB || m.kind == EMCGPU_MECH_HOT_FROEHLICH_EM) &&
+        m.bath >= (int)in.baths.size())
+      return fail(c, EMCGPU_ERR_STATE, "mechanism %zu reads bath %d: set the baths first", j,
+                  m.bath);
+  }
+  if ((st = pull_baths(c, in)))
+    return st;
   std::vector<std::vector<double>> raw(in.mechs.size(), std::vector<double>(n_levels));
+  in.rate_data.assign(in.mechs.size(), {});
+  in.hot_C.assign(in.mechs.size(), 0.0);
   for (size_t j = 0; j < in.mechs.size(); j++) {
     const emcgpu_mech_desc &m = in.mechs[j];
     const phys::ValleyMass &iv = in.vm[m.valley];
     double C = 0, nq = 0, meff = 0;
+    const bool hot =
+        m.kind == EMCGPU_MECH_HOT_FROEHLICH_AB || m.kind == EMCGPU_MECH_HOT_FROEHLICH_EM;
     const bool ab = m.kind == EMCGPU_MECH_ZERO_IV_AB || m.kind == EMCGPU_MECH_FIRST_IV_AB ||
-                    m.kind == EMCGPU_MECH_FROEHLICH_AB;
+                    m.kind == EMCGPU_MECH_FROEHLICH_AB || m.kind == EMCGPU_MECH_HOT_FROEHLICH_AB;
+    if (hot) {
+      // emcHotPhononFroehlichMechanism.hpp:79-91 / :159-173: the Froehlich
+      // rate with the bath's mean occupation; kI and ln kept per level for
+      // the rebuilds on the GPU (kI = 0: zero rate)
+      meff = m.rel_eff_mass * phys::me;
+      C = phys::froehlichConst(m.phonon_energy, meff, m.eps_hi, m.eps_lo);
+      const double n = bath_mean(in.baths[m.bath]);
+      std::vector<double> data(2 * (size_t)n_levels, 0.0);
+      for (int l = 0; l < n_levels; l++) {
+        const double E = (double)(std::size_t)(l + 1) * dE;
+        double r = 0.0;
+        if (ab || E > m.phonon_energy) {
+          const double gI = iv.gamma(E);
+          const double gF = iv.gamma(ab ? E + m.phonon_energy : E - m.phonon_energy);
+          if (gI > 0.0 && gF > 0.0) {
+            const double kI = std::sqrt(2.0 * meff * gI * phys::q) / phys::hbar;
+            const double kF = std::sqrt(2.0 * meff * gF * phys::q) / phys::hbar;
+            const double ln = ab ? std::log((kI + kF) / (kF - kI)) : std::log((kI + kF) / (kI - kF));
+            data[l] = kI;
+            data[n_levels + l] = ln;
+            r = ab ? C * n / kI * ln : C * (n + 1.0) / kI * ln;
+          }
+        }
+        raw[j][l] = r;
+      }
+      in.rate_data[j] = std::move(data);
+      in.hot_C[j] = C;
+      continue;
+    }
     switch (m.kind) {
     case EMCGPU_MECH_ACOUSTIC:
       C = phys::acousticConst(m.coupling, mat->rho, mat->v_sound, temperature);
@@ -927,6 +1263,7 @@ emcgpu_status emcgpu_build_tables(emcgpu_ctx *c, int instance, int n_levels, dou
       }
       raw[j][l] = r;
     }
+    in.rate_data[j] = raw[j];
   }
   return finish_tables(c, in, n_levels, max_energy, raw);
 }
@@ -941,8 +1278,39 @@ emcgpu_status emcgpu_get_tables(const emcgpu_ctx *c, int instance, int valley, d
     return fail(const_cast<emcgpu_ctx *>(c), EMCGPU_ERR_STATE, "no tables");
   if (valley < 0 || valley >= (int)in.valleys.size())
     return fail(const_cast<emcgpu_ctx *>(c), EMCGPU_ERR_ARG, "valley out of range");
+  const int nt = (int)in.table_mech[valley].size();
   if (n_tables)
-    *n_tables = (int)in.table_mech[valley].size();
+    *n_tables = nt;
+  bool hot_valley = false;
+  for (int j : in.table_mech[valley])
+    hot_valley |= in.mechs[j].kind == EMCGPU_MECH_HOT_FROEHLICH_AB ||
+                  in.mechs[j].kind == EMCGPU_MECH_HOT_FROEHLICH_EM;
+  if (hot_valley && in.dev_bath && c->device >= 0) {
+    // the tables rebuilt on the GPU from the evolving baths
+    emcgpu_ctx *cm = const_cast<emcgpu_ctx *>(c);
+    (void)hipSetDevice(c->device);
+    if (c->params_dirty) {
+      const double dt = c->params_dt > 0 ? c->params_dt : 1e-15;
+      emcgpu_status s2 = c->precision == EMCGPU_FP32 ? upload_params<float>(cm, dt)
+                                                     : upload_params<double>(cm, dt);
+      if (s2)
+        return s2;
+    }
+    HIPCHK(cm, hipStreamSynchronize(c->stream));
+    std::vector<double> dev((size_t)nt * in.n_levels);
+    HIPCHK(cm, hipMemcpy(dev.data(), c->d_cum64 + in.cum_off[valley], sizeof(double) * dev.size(),
+                         hipMemcpyDeviceToHost));
+    double t0 = 0;
+    HIPCHK(cm, hipMemcpy(&t0, c->d_tau0_64 + instance * kMaxV + valley, sizeof(double),
+                         hipMemcpyDeviceToHost));
+    if (tau0)
+      *tau0 = t0;
+    if (cum)
+      for (int l = 0; l < in.n_levels; l++)
+        for (int j = 0; j < nt; j++)
+          cum[(size_t)j * in.n_levels + l] = dev[(size_t)l * nt + j];
+    return EMCGPU_OK;
+  }
   if (tau0)
     *tau0 = in.tau0[valley];
   if (cum)
@@ -1088,6 +1456,184 @@ emcgpu_status emcgpu_init_particles(emcgpu_ctx *c, int instance, uint64_t seed,
   c->params_dirty = true;
   return c->precision == EMCGPU_FP32 ? init_thermal<float>(c, instance, seed)
                                      : init_thermal<double>(c, instance, seed);
+}
+
+emcgpu_status emcgpu_init_particles_energy(emcgpu_ctx *c, int instance, uint64_t seed,
+                                           double mean_ev, double sigma_ev) {
+  emcgpu_status st = check_instance(c, instance);
+  if (st)
+    return st;
+  if (!(mean_ev > 0) || !(sigma_ev >= 0) || !std::isfinite(mean_ev) || !std::isfinite(sigma_ev))
+    return fail(c, EMCGPU_ERR_ARG, "mean energy must be > 0 and sigma >= 0");
+  if (c->device < 0)
+    return fail(c, EMCGPU_ERR_STATE, "host-only context has no device memory");
+  (void)hipSetDevice(c->device);
+  return c->precision == EMCGPU_FP32 ? init_thermal<float>(c, instance, seed, mean_ev, sigma_ev)
+                                     : init_thermal<double>(c, instance, seed, mean_ev, sigma_ev);
+}
+
+emcgpu_status emcgpu_set_baths(emcgpu_ctx *c, int instance, const emcgpu_bath_desc *baths,
+                               int n_baths) {
+  emcgpu_status st = check_instance(c, instance);
+  if (st)
+    return st;
+  if (n_baths < 0 || n_baths > EMCGPU_MAX_BATHS || (n_baths && !baths))
+    return fail(c, EMCGPU_ERR_ARG, "n_baths must be in [0, %d]", EMCGPU_MAX_BATHS);
+  std::vector<BathHost> hb;
+  for (int b = 0; b < n_baths; b++) {
+    const emcgpu_bath_desc &d = baths[b];
+    if (d.n_bins < 1 || d.n_bins > EMCGPU_MAX_BINS)
+      return fail(c, EMCGPU_ERR_ARG, "bath %d: n_bins must be in [1, %d]", b, EMCGPU_MAX_BINS);
+    if (!(d.dq > 0) || !(d.tau_lo > 0) || !(d.phonon_energy > 0) || !(d.lattice_temp > 0) ||
+        !(d.volume > 0))
+      return fail(c, EMCGPU_ERR_ARG,
+                  "bath %d: dq, tau_lo, phonon_energy, lattice_temp and volume must be > 0", b);
+    if (d.acoustic && (!(d.ac_phonon_energy > 0) || !(d.tau_acoustic > 0)))
+      return fail(c, EMCGPU_ERR_ARG, "bath %d: acoustic reservoir needs energy and time > 0", b);
+    if (d.acoustic && d.w_ridley > 0 && d.to_phonon_energy > 0 && !(d.tau_to > 0))
+      return fail(c, EMCGPU_ERR_ARG, "bath %d: Ridley reservoir needs tau_to > 0", b);
+    BathHost B;
+    B.d = d;
+    B.d.tau_profile = nullptr;
+    if (d.tau_profile) {
+      B.profile.assign(d.tau_profile, d.tau_profile + d.n_bins);
+      for (double t : B.profile)
+        if (!(t > 0) || !std::isfinite(t))
+          return fail(c, EMCGPU_ERR_ARG, "bath %d: lifetime profile values must be > 0", b);
+    }
+    // constructor (emcPhononBath.hpp:160-192)
+    B.n0 = planck(d.phonon_energy, d.lattice_temp);
+    B.nq.assign(d.n_bins, B.n0);
+    if (d.acoustic) {
+      B.has_ac = true;
+      B.n_ac_eq = planck(d.ac_phonon_energy, d.lattice_temp);
+      B.n_ac = B.n_ac_eq;
+    }
+    if (d.acoustic && d.w_ridley > 0 && d.to_phonon_energy > 0) {
+      B.has_ridley = true;
+      B.w_r = (d.w_ridley > 1.0) ? 1.0 : d.w_ridley;
+      B.n_to_eq = planck(d.to_phonon_energy, d.lattice_temp);
+      B.n_to = B.n_to_eq;
+    }
+    hb.push_back(std::move(B));
+  }
+  Instance &in = c->inst[instance];
+  in.baths = std::move(hb);
+  in.dev_bath = false;
+  if (in.has_hot)
+    in.has_tables = false;  // the hot-phonon rates read the new baths
+  c->bath_layout_dirty = true;
+  c->params_dirty = true;
+  return EMCGPU_OK;
+}
+
+emcgpu_status emcgpu_get_bath(const emcgpu_ctx *c0, int instance, int bath, double *nq,
+                              double *scalars) {
+  emcgpu_ctx *c = const_cast<emcgpu_ctx *>(c0);
+  emcgpu_status st = check_instance(c, instance);
+  if (st)
+    return st;
+  Instance &in = c->inst[instance];
+  if (bath < 0 || bath >= (int)in.baths.size())
+    return fail(c, EMCGPU_ERR_ARG, "instance %d has no bath %d", instance, bath);
+  if (c->device >= 0)
+    (void)hipSetDevice(c->device);
+  if ((st = pull_baths(c, in)))
+    return st;
+  const BathHost &B = in.baths[bath];
+  if (nq)
+    std::copy(B.nq.begin(), B.nq.end(), nq);
+  if (scalars) {
+    scalars[0] = bath_mean(B);
+    scalars[1] = B.n_ac;
+    scalars[2] = B.n_to;
+    scalars[3] = B.n0;
+    // getAcousticTemp / getTOTemp (emcPhononBath.hpp:480-494)
+    scalars[4] = (!B.has_ac || B.n_ac <= B.n_ac_eq) ? B.d.lattice_temp
+                                                    : planck_temp(B.d.ac_phonon_energy, B.n_ac);
+    scalars[5] = (!B.has_ridley || B.n_to <= B.n_to_eq)
+                     ? B.d.lattice_temp
+                     : planck_temp(B.d.to_phonon_energy, B.n_to);
+  }
+  return EMCGPU_OK;
+}
+
+emcgpu_status emcgpu_set_bath_state(emcgpu_ctx *c, int instance, int bath, const double *nq,
+                                    double n_ac, double n_to) {
+  emcgpu_status st = check_instance(c, instance);
+  if (st)
+    return st;
+  Instance &in = c->inst[instance];
+  if (bath < 0 || bath >= (int)in.baths.size() || !nq)
+    return fail(c, EMCGPU_ERR_ARG, "instance %d has no bath %d (or nq is NULL)", instance, bath);
+  if (c->device >= 0)
+    (void)hipSetDevice(c->device);
+  if ((st = pull_baths(c, in)))
+    return st;
+  BathHost &B = in.baths[bath];
+  for (int i = 0; i < B.d.n_bins; i++)
+    if (!(nq[i] >= 0) || !std::isfinite(nq[i]))
+      return fail(c, EMCGPU_ERR_ARG, "bin %d: occupations must be finite and >= 0", i);
+  B.nq.assign(nq, nq + B.d.n_bins);
+  B.n_ac = n_ac;
+  B.n_to = n_to;
+  if (in.dev_bath && c->device >= 0) {
+    const BathVar v{B.n_ac, B.n_to, bath_mean(B), 0.0};
+    HIPCHK(c, hipMemcpy(c->d_nq + in.bath_off[bath], B.nq.data(), sizeof(double) * B.d.n_bins,
+                        hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_bath_var + in.bath_index0 + bath, &v, sizeof(BathVar),
+                        hipMemcpyHostToDevice));
+  }
+  // the tables follow the new state on the next step
+  c->params_dirty = true;
+  return EMCGPU_OK;
+}
+
+emcgpu_status emcgpu_get_bath_tallies(const emcgpu_ctx *c0, int instance, int bath,
+                                      uint64_t *n_em, uint64_t *n_abs) {
+  emcgpu_ctx *c = const_cast<emcgpu_ctx *>(c0);
+  emcgpu_status st = check_instance(c, instance);
+  if (st)
+    return st;
+  Instance &in = c->inst[instance];
+  if (bath < 0 || bath >= (int)in.baths.size())
+    return fail(c, EMCGPU_ERR_ARG, "instance %d has no bath %d", instance, bath);
+  const int nb = in.baths[bath].d.n_bins;
+  std::vector<unsigned long long> t(2 * (size_t)nb, 0ull);
+  if (in.dev_bath && c->device >= 0) {
+    (void)hipSetDevice(c->device);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(t.data(), c->d_tally + 2 * in.bath_off[bath],
+                        sizeof(unsigned long long) * 2 * nb, hipMemcpyDeviceToHost));
+  }
+  for (int i = 0; i < nb; i++) {
+    if (n_em)
+      n_em[i] = t[2 * i];
+    if (n_abs)
+      n_abs[i] = t[2 * i + 1];
+  }
+  return EMCGPU_OK;
+}
+
+emcgpu_status emcgpu_get_bath_history(const emcgpu_ctx *c0, int instance, int bath,
+                                      double *mean_nq, int *n_steps) {
+  emcgpu_ctx *c = const_cast<emcgpu_ctx *>(c0);
+  emcgpu_status st = check_instance(c, instance);
+  if (st)
+    return st;
+  Instance &in = c->inst[instance];
+  if (bath < 0 || bath >= (int)in.baths.size())
+    return fail(c, EMCGPU_ERR_ARG, "instance %d has no bath %d", instance, bath);
+  const int ns = in.dev_bath ? c->bath_hist_steps : 0;
+  if (n_steps)
+    *n_steps = ns;
+  if (mean_nq && ns > 0) {
+    (void)hipSetDevice(c->device);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(mean_nq, c->d_bath_hist + (in.bath_index0 + bath) * (long long)ns,
+                        sizeof(double) * ns, hipMemcpyDeviceToHost));
+  }
+  return EMCGPU_OK;
 }
 
 emcgpu_status emcgpu_step(emcgpu_ctx *c, const emcgpu_step_args *s) {

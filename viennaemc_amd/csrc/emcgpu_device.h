@@ -26,6 +26,8 @@ constexpr int kTile = 256;  // threads per block
 // two instances
 constexpr int kUnit = 2048;
 constexpr int kMaxPPT = 8;
+constexpr int kMaxBaths = EMCGPU_MAX_BATHS;
+constexpr int kMaxBins = EMCGPU_MAX_BINS;
 
 // final-state sampler of a mechanism
 enum FinalKind : int {
@@ -50,7 +52,7 @@ struct MechDev {
   int final_kind;
   int final_valley;
   int z;
-  int pad;
+  int bath1;  // hot-phonon Froehlich: instance-local bath index + 1 (0: none)
   R shift;  // energy change of the final state [eV] (IV, Froehlich)
   uint8_t final_sub[kMaxDeg][kMaxZ];
 };
@@ -68,6 +70,39 @@ struct InstDev {
   long long table_off[kMaxV];  // element offset of the valley's [level][table] block
   ValleyDev<R> valley[kMaxV];
   MechDev<R> mech[kMaxV][kMaxMech];  // per valley, in table order
+  // phonon baths of the instance: their bins are contiguous, bath b at
+  // local bin offset bath_loff[b] of the instance's block that starts at
+  // global bin gbin_off
+  int n_baths;
+  int total_bins;
+  int bath_nb[kMaxBaths];
+  int bath_loff[kMaxBaths];
+  R bath_dq[kMaxBaths];  // bin width [1/nm]
+  long long gbin_off;
+};
+
+// constants of one LO phonon bath (emcPhononBath<double>)
+struct BathDev {
+  int n_bins, has_ac, has_ridley, has_profile;
+  long long off;  // first bin in the global bin arrays
+  double dq, tau_lo, n0, vsim, lo_e, lat_t;
+  double ac_e, tau_ac, n_ac_eq;
+  double w_r, to_e, tau_to, n_to_eq;
+};
+// evolving scalars of a bath
+struct BathVar {
+  double n_ac, n_to, mean, pad;
+};
+
+// one (instance, valley) table set rebuilt on the GPU from the baths
+constexpr int kRateConst = 0, kRateHotAb = 1, kRateHotEm = 2;
+struct RebuildJob {
+  int inst, valley, n_tables, n_levels;
+  long long cum_off;                // element offset in the [level][table] tables
+  int kind[kMaxMech];               // kRate*
+  int bath[kMaxMech];               // global bath index (hot kinds)
+  double C[kMaxMech];               // rate prefactor (hot kinds)
+  long long rate_off[kMaxMech];     // const: rates[level]; hot: kI[level], ln[level]
 };
 
 template <typename R>
@@ -99,6 +134,9 @@ struct StepArgs {
   int32_t *tr_valley, *tr_sub, *tr_events;
   double *tr_k, *tr_E;
   long long tr_n;
+  // phonon-bath event tallies [global bin][2] (emission, absorption)
+  unsigned long long *tally;
+  int lds_bins;  // bins held in LDS per block (max over instances)
 };
 
 }  // namespace emcgpu

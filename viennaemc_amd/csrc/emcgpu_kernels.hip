@@ -255,12 +255,24 @@ __device__ __forceinline__ void direction_wrt_k(R &kx, R &ky, R &kz, R cosT, R u
   kz = -kxp * st0 + kzp * ct0;
 }
 
+// bin of a phonon wavevector magnitude q (binOf, emcPhononBath.hpp:222-225)
+// and its tally in the block's LDS histogram [local bin][emission, absorption]
+template <typename R>
+__device__ __forceinline__ void hot_record(const InstDev<R> *P, const MechDev<R> &m, R q,
+                                           uint32_t *lds_tally) {
+  const int b = m.bath1 - 1;
+  const int nb = P->bath_nb[b];
+  const R r = q / P->bath_dq[b];
+  const int bin = (r < (R)nb) ? (int)r : nb - 1;
+  atomicAdd(&lds_tally[2 * (P->bath_loff[b] + bin) + (m.shift < R(0) ? 0 : 1)], 1u);
+}
+
 // final state of one mechanism: ua/ub are the two final-state uniforms
 // (phi / cos-theta for isotropic kinds, exponent / phi for Froehlich)
-template <typename R>
+template <typename R, bool HOT = false>
 __device__ __forceinline__ void final_state(const InstDev<R> *P, const MechDev<R> &m, R &kx,
                                             R &ky, R &kz, R &E, int &v, int &sub, R ua, R ub,
-                                            int zidx) {
+                                            int zidx, uint32_t *lds_tally = nullptr) {
   if (m.final_kind == kFinalIsotropicElastic) {
     // emcAcousticScatterMechanism.hpp:70-72
     const R n = Math<R>::sqrt(kx * kx + ky * ky + kz * kz);
@@ -273,7 +285,10 @@ __device__ __forceinline__ void final_state(const InstDev<R> *P, const MechDev<R
     const R n = normk_of(P->valley[v], E);
     random_direction(n, ua, ub, kx, ky, kz);
   } else {
-    // emcFroehlichInteraction.hpp:109-128 / :183-200
+    // emcFroehlichInteraction.hpp:109-128 / :183-200; the hot-phonon kinds
+    // (emcHotPhononFroehlichMechanism.hpp:94-117 / :175-199) add the record
+    // of |k' - k| in their bath
+    const R kx0 = kx, ky0 = ky, kz0 = kz;
     const R ei = E;
     E += m.shift;
     const R ef = E;
@@ -290,6 +305,13 @@ __device__ __forceinline__ void final_state(const InstDev<R> *P, const MechDev<R
       kx = kx * s;
       ky = ky * s;
       kz = kz * s;
+    }
+    if constexpr (HOT) {
+      if (m.bath1 != 0) {
+        // subtract(k, kOld) then norm (emcUtil.hpp:25-35, 72-80)
+        const R dx = kx - kx0, dy = ky - ky0, dz = kz - kz0;
+        hot_record(P, m, Math<R>::sqrt(dx * dx + dy * dy + dz * dz), lds_tally);
+      }
     }
   }
 }
@@ -339,10 +361,11 @@ __device__ __forceinline__ void drift(const InstDev<R> *P, R t, R &kx, R &ky, R 
 // one pass of the scatter loop of moveParticles: scatter (selection +
 // final state), draw tau', drift min(rem, tau'), rem -= tau'
 // (basicBulkParticleHandler.hpp:203-212); e counts the events of the step
-template <typename R, int MODE>
+template <typename R, int MODE, bool HOT = false>
 __device__ __forceinline__ void event_once(const InstDev<R> *P, const R *__restrict__ cum, R &kx,
                                            R &ky, R &kz, R &E, R &tau, R &rem, int &v, int &sub,
-                                           uint32_t &e, StepRng<R, MODE> &rng, int &nreal) {
+                                           uint32_t &e, StepRng<R, MODE> &rng, int &nreal,
+                                           uint32_t *lds_tally = nullptr) {
   const int nt = P->n_tables[v];
   R u[4];
   if constexpr (MODE == 0)
@@ -401,7 +424,7 @@ __device__ __forceinline__ void event_once(const InstDev<R> *P, const R *__restr
             ub = (R)rng.rs.uniform(0., 1.);
           }
         }
-        final_state(P, m, kx, ky, kz, E, v, sub, ua, ub, zidx);
+        final_state<R, HOT>(P, m, kx, ky, kz, E, v, sub, ua, ub, zidx, lds_tally);
         ++nreal;
       }
     }
@@ -422,10 +445,11 @@ __device__ __forceinline__ void event_once(const InstDev<R> *P, const R *__restr
 
 // one particle, one step: drift min(tau, dt), then the scatter loop
 // (basicBulkParticleHandler.hpp:195-214)
-template <typename R, int MODE>
+template <typename R, int MODE, bool HOT = false>
 __device__ __forceinline__ void move_one(const InstDev<R> *P, const R *__restrict__ cum, R &kx,
                                          R &ky, R &kz, R &E, R &tau, int &v, int &sub,
-                                         StepRng<R, MODE> &rng, int &nev, int &nreal) {
+                                         StepRng<R, MODE> &rng, int &nev, int &nreal,
+                                         uint32_t *lds_tally = nullptr) {
   const R dt = P->dt;
   drift(P, (dt < tau) ? dt : tau, kx, ky, kz, E, v, sub);  // std::min(tau, dt)
   R rem = dt - tau;
@@ -433,7 +457,7 @@ __device__ __forceinline__ void move_one(const InstDev<R> *P, const R *__restric
   // every pass draws tau' > 0 (the uniform is clamped below 1), so rem
   // strictly decreases; the event cap bounds the loop regardless
   while (rem > R(0) && e < kMaxEventsPerStep)
-    event_once<R, MODE>(P, cum, kx, ky, kz, E, tau, rem, v, sub, e, rng, nreal);
+    event_once<R, MODE, HOT>(P, cum, kx, ky, kz, E, tau, rem, v, sub, e, rng, nreal, lds_tally);
   nev += (int)e;
   tau -= dt;
 }
@@ -525,6 +549,30 @@ __device__ __forceinline__ int tile_instance(const int *unit_inst, long long til
 }
 
 // ---------------------------------------------------------------------------
+// phonon-bath tallies: each block counts its events in an LDS histogram of
+// its instance's bins and adds the non-zero counts to the global 64-bit
+// tallies once at the end
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ void tally_clear(const InstDev<R> *P, uint32_t *lds_tally) {
+  for (int i = threadIdx.x; i < 2 * P->total_bins; i += kTile)
+    lds_tally[i] = 0u;
+  __syncthreads();
+}
+
+template <typename R>
+__device__ __forceinline__ void tally_flush(const InstDev<R> *P, const uint32_t *lds_tally,
+                                            unsigned long long *tally) {
+  __syncthreads();
+  unsigned long long *g = tally + 2 * P->gbin_off;
+  for (int i = threadIdx.x; i < 2 * P->total_bins; i += kTile) {
+    const uint32_t c = lds_tally[i];
+    if (c)
+      atomicAdd(&g[i], (unsigned long long)c);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // production step kernel: PPT particles per lane
 //
 // Each lane owns PPT particles of a tile of 256*PPT (particle j of lane t is
@@ -537,13 +585,16 @@ __device__ __forceinline__ int tile_instance(const int *unit_inst, long long til
 // busy in the body instead of ~20 % (one particle per lane at ~0.24 events
 // per step).  No barriers are involved in the event loop.
 // ---------------------------------------------------------------------------
-template <typename R, int PPT>
+template <typename R, int PPT, bool HOT>
 __global__ __launch_bounds__(kTile) void k_bulk_multi(StepArgs<R> a) {
   __shared__ double wsum[kTile / 64][kObsMax];
+  extern __shared__ uint32_t lds_tally[];
   constexpr int TS = kTile * PPT;
   const int tile = blockIdx.x;
   const int inst = tile_instance(a.unit_inst, tile, TS);
   const InstDev<R> *P = a.inst + inst;
+  if constexpr (HOT)
+    tally_clear(P, lds_tally);
   const long long base = a.inst_start[inst];
   const long long count = a.inst_count[inst];
   const long long first = (long long)tile * TS - base + threadIdx.x;  // local index, j = 0
@@ -615,7 +666,7 @@ __global__ __launch_bounds__(kTile) void k_bulk_multi(StepArgs<R> a) {
           ee = ev[j];
         }
       rng.pid = (uint32_t)(first + (long long)sel * kTile);
-      event_once<R, 0>(P, a.cum, x, y, z, en, ta, rm, vv, ss, ee, rng, nreal);
+      event_once<R, 0, HOT>(P, a.cum, x, y, z, en, ta, rm, vv, ss, ee, rng, nreal, lds_tally);
 #pragma unroll
       for (int j = 0; j < PPT; ++j)
         if (sel == j) {
@@ -639,6 +690,8 @@ __global__ __launch_bounds__(kTile) void k_bulk_multi(StepArgs<R> a) {
     }
     tile_observables<R, PPT>(a, P, wsum, tile, s, valid, v, sub, kx, ky, kz, E, nev, nreal);
   }
+  if constexpr (HOT)
+    tally_flush(P, lds_tally, a.tally);
 
 #pragma unroll
   for (int j = 0; j < PPT; ++j) {
@@ -658,12 +711,15 @@ __global__ __launch_bounds__(kTile) void k_bulk_multi(StepArgs<R> a) {
 // replay step kernel (FP64): one particle per lane, the reference engine's
 // recorded draws consumed in the reference's order, optional per-step trace
 // ---------------------------------------------------------------------------
-template <typename R>
+template <typename R, bool HOT>
 __global__ __launch_bounds__(kTile) void k_bulk_replay(StepArgs<R> a) {
   __shared__ double wsum[kTile / 64][kObsMax];
+  extern __shared__ uint32_t lds_tally[];
   const int tile = blockIdx.x;
   const int inst = tile_instance(a.unit_inst, tile, kTile);
   const InstDev<R> *P = a.inst + inst;
+  if constexpr (HOT)
+    tally_clear(P, lds_tally);
   const long long base = a.inst_start[inst];
   const long long local = (long long)tile * kTile + threadIdx.x - base;
   bool valid[1] = {local < a.inst_count[inst]};
@@ -691,7 +747,8 @@ __global__ __launch_bounds__(kTile) void k_bulk_replay(StepArgs<R> a) {
   for (int s = 0; s < a.n_steps; ++s) {
     int nev = 0, nreal = 0;
     if (valid[0]) {
-      move_one<R, 1>(P, a.cum, kx[0], ky[0], kz[0], E[0], tau, v[0], sub[0], rng, nev, nreal);
+      move_one<R, 1, HOT>(P, a.cum, kx[0], ky[0], kz[0], E[0], tau, v[0], sub[0], rng, nev,
+                          nreal, lds_tally);
       // grain bookkeeping (basicBulkParticleHandler.hpp:213-220), no grain
       // mechanism: a new grainTau of -ln U[1e-6,1) * 1 s
       grain -= a.dt_s;
@@ -710,6 +767,8 @@ __global__ __launch_bounds__(kTile) void k_bulk_replay(StepArgs<R> a) {
     }
     tile_observables<R, 1>(a, P, wsum, tile, s, valid, v, sub, kx, ky, kz, E, nev, nreal);
   }
+  if constexpr (HOT)
+    tally_flush(P, lds_tally, a.tally);
 
   if (valid[0]) {
     a.kx[gi] = kx[0];
@@ -806,10 +865,218 @@ __global__ __launch_bounds__(256) void k_reduce_final(const double *__restrict__
 }
 
 // ---------------------------------------------------------------------------
+// LO phonon bath update (emcPhononBath::update, emcPhononBath.hpp:264-358)
+// and the mean occupation that the hot-phonon rates read (getMeanNq,
+// :461-470).  One block per bath.  The per-bin update is independent across
+// bins and runs on all lanes; the two DOS-weighted sums are sequential over
+// the bins in the reference and are taken in that order by lane 0, so the
+// FP64 result matches the reference operation for operation.
+// ---------------------------------------------------------------------------
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kQ = 1.60219e-19, kKB = 1.38066e-23;  // include/emcConstants.hpp
+
+__device__ double bath_planck(double energy_ev, double temp_k) {
+#pragma clang fp contract(off)
+  const double x = kQ * energy_ev / (kKB * temp_k);
+  return 1.0 / (::exp(x) - 1.0);
+}
+__device__ double bath_planck_temp(double energy_ev, double n) {
+#pragma clang fp contract(off)
+  if (n <= 0.0)
+    return 0.0;
+  return kQ * energy_ev / (kKB * ::log(1.0 + 1.0 / n));
+}
+
+__global__ __launch_bounds__(256) void k_bath_update(const BathDev *__restrict__ baths,
+                                                     double *__restrict__ nq,
+                                                     const double *__restrict__ tau_prof,
+                                                     unsigned long long *__restrict__ tally,
+                                                     BathVar *__restrict__ var, double dt,
+                                                     double *__restrict__ hist, int hist_steps,
+                                                     int hist_step) {
+#pragma clang fp contract(off)
+  __shared__ double s_old[kMaxBins];
+  __shared__ double s_target;
+  const BathDev &B = baths[blockIdx.x];
+  BathVar &V = var[blockIdx.x];
+  const int nb = B.n_bins;
+  if (threadIdx.x == 0) {
+    // decay target: N0 unless a daughter reservoir is hot (:269-286)
+    double target = B.n0;
+    if (B.has_ac) {
+      double n_klemens = B.n0;
+      if (V.n_ac > B.n_ac_eq)
+        n_klemens = bath_planck(B.lo_e, bath_planck_temp(B.ac_e, V.n_ac));
+      if (B.has_ridley) {
+        double n_ridley = B.n0;
+        if (V.n_to > B.n_to_eq)
+          n_ridley = bath_planck(B.lo_e, bath_planck_temp(B.to_e, V.n_to));
+        target = (1.0 - B.w_r) * n_klemens + B.w_r * n_ridley;
+      } else {
+        target = n_klemens;
+      }
+    }
+    s_target = target;
+  }
+  __syncthreads();
+  const double target = s_target;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    const long long gi = B.off + i;
+    const double q = ((double)i + 0.5) * B.dq;
+    const double dph = q * q * B.dq * B.vsim / (2.0 * kPi * kPi);
+    const double n_em = (double)tally[2 * gi], n_abs = (double)tally[2 * gi + 1];
+    const double g = (dph > 0.0) ? (n_em - n_abs) / (dph * dt) : 0.0;
+    const double tau_i = B.has_profile ? tau_prof[gi] : B.tau_lo;
+    double n = nq[gi];
+    s_old[i] = n;
+    n += g * dt - (dt / tau_i) * (n - target);
+    if (n < 0.0)
+      n = 0.0;
+    nq[gi] = n;
+    tally[2 * gi] = 0ull;
+    tally[2 * gi + 1] = 0ull;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (B.has_ac) {
+      // reservoirs from the start-of-step occupations (:291-355)
+      double sum_w = 0.0, sum_wdn = 0.0, sum_wdn_tau = 0.0;
+      for (int i = 0; i < nb; ++i) {
+        const double q = ((double)i + 0.5) * B.dq;
+        const double w = q * q;
+        sum_w += w;
+        if (B.has_profile)
+          sum_wdn_tau += w * (s_old[i] - target) / tau_prof[B.off + i];
+        else
+          sum_wdn += w * (s_old[i] - target);
+      }
+      const double f_klemens = B.has_ridley ? (1.0 - B.w_r) : 1.0;
+      const double src = B.has_profile ? ((sum_w > 0.0) ? sum_wdn_tau / sum_w : 0.0)
+                                       : ((sum_w > 0.0) ? sum_wdn / sum_w : 0.0);
+      double n_ac = V.n_ac, n_to = V.n_to;
+      if (!B.has_profile) {
+        n_ac += dt * f_klemens * src / B.tau_lo - dt * (n_ac - B.n_ac_eq) / B.tau_ac;
+        if (n_ac < B.n_ac_eq)
+          n_ac = B.n_ac_eq;
+        if (B.has_ridley) {
+          n_to += dt * B.w_r * src / B.tau_lo - dt * (n_to - B.n_to_eq) / B.tau_to;
+          if (n_to < B.n_to_eq)
+            n_to = B.n_to_eq;
+        }
+      } else {
+        n_ac += dt * f_klemens * src - dt * (n_ac - B.n_ac_eq) / B.tau_ac;
+        if (n_ac < B.n_ac_eq)
+          n_ac = B.n_ac_eq;
+        if (B.has_ridley) {
+          n_to += dt * B.w_r * src - dt * (n_to - B.n_to_eq) / B.tau_to;
+          if (n_to < B.n_to_eq)
+            n_to = B.n_to_eq;
+        }
+      }
+      V.n_ac = n_ac;
+      V.n_to = n_to;
+    }
+    // getMeanNq over the updated occupations
+    double sum_w = 0.0, sum_wn = 0.0;
+    for (int i = 0; i < nb; ++i) {
+      const double q = ((double)i + 0.5) * B.dq;
+      const double w = q * q;
+      sum_w += w;
+      sum_wn += w * nq[B.off + i];
+    }
+    const double mean = (sum_w > 0.0) ? sum_wn / sum_w : B.n0;
+    V.mean = mean;
+    if (hist)
+      hist[(long long)blockIdx.x * hist_steps + hist_step] = mean;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// scatter-table rebuild from the baths (reinitScatterTables:
+// fillScatterTables + renormalizeTables, emcScatterHandler.hpp:100-108,
+// 220-272).  One block per (instance, valley) whose mechanisms read a bath.
+// Hot-phonon rates are scatterConst * N / kI * ln (absorption) and
+// scatterConst * (N + 1) / kI * ln (emission) with the per-level kI and ln
+// of emcHotPhononFroehlichMechanism.hpp:79-91 / :159-173 precomputed in FP64
+// on the host (kI = 0 marks a zero rate); every other mechanism keeps its
+// FP64 rates.  Cumulative sums run over the tables in registration order,
+// normalised by the largest total rate; the FP64 result is stored in the
+// precision of the kinematics and in an FP64 copy.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double rebuild_rate(const RebuildJob &J, int j, int l,
+                                               const double *__restrict__ arena,
+                                               const BathVar *__restrict__ var) {
+#pragma clang fp contract(off)
+  const long long o = J.rate_off[j];
+  if (J.kind[j] == kRateConst)
+    return arena[o + l];
+  const double ki = arena[o + l];
+  if (!(ki > 0.0))
+    return 0.0;
+  const double ln = arena[o + J.n_levels + l];
+  const double n = var[J.bath[j]].mean;
+  return (J.kind[j] == kRateHotAb) ? J.C[j] * n / ki * ln : J.C[j] * (n + 1.0) / ki * ln;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_rebuild_tables(const RebuildJob *__restrict__ jobs,
+                                                        const double *__restrict__ arena,
+                                                        const BathVar *__restrict__ var,
+                                                        R *__restrict__ cum,
+                                                        double *__restrict__ cum64,
+                                                        InstDev<R> *__restrict__ inst,
+                                                        double *__restrict__ tau0_64) {
+#pragma clang fp contract(off)
+  __shared__ double s_max[256];
+  const RebuildJob &J = jobs[blockIdx.x];
+  const int nt = J.n_tables, nl = J.n_levels;
+  double mx = 0.0;
+  bool first = true;
+  for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+    double c = rebuild_rate(J, 0, l, arena, var);
+    for (int j = 1; j < nt; ++j)
+      c = rebuild_rate(J, j, l, arena, var) + c;
+    if (first || mx < c)
+      mx = c;
+    first = false;
+  }
+  s_max[threadIdx.x] = first ? -1.0 : mx;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w && s_max[threadIdx.x] < s_max[threadIdx.x + w])
+      s_max[threadIdx.x] = s_max[threadIdx.x + w];
+    __syncthreads();
+  }
+  mx = s_max[0];
+  if (!(mx > 0.0))
+    return;  // no rate at any level: keep the previous tables
+  for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+    double c = 0.0;
+    for (int j = 0; j < nt; ++j) {
+      const double r = rebuild_rate(J, j, l, arena, var);
+      c = (j == 0) ? r : r + c;
+      const double x = c / mx;
+      cum[J.cum_off + (long long)l * nt + j] = (R)x;
+      cum64[J.cum_off + (long long)l * nt + j] = x;
+    }
+  }
+  if (threadIdx.x == 0) {
+    const double tau = 1. / mx;
+    inst[J.inst].tau0[J.valley] = (R)(tau * 1e15);
+    tau0_64[J.inst * kMaxV + J.valley] = tau;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // thermal initialisation (emcElectron.hpp:75-90, emcParticleInitialization.hpp:36-51)
 // ---------------------------------------------------------------------------
+// init_mean > 0 selects the prescribed-energy initialiser: E = mean +
+// sigma * sqrt(-2 ln u2) cos(2 pi u6) (u2 in [1e-6, 1)), |E| if negative,
+// the mean if zero; sigma = 0 is the fixed-energy photoexcitation
+// (emcParticleInitialization.hpp:57-71)
 template <typename R>
-__global__ __launch_bounds__(kTile) void k_init_thermal(StepArgs<R> a) {
+__global__ __launch_bounds__(kTile) void k_init_thermal(StepArgs<R> a, double init_mean,
+                                                        double init_sigma) {
   const long long tile = (long long)a.tile_base + blockIdx.x;
   const int inst = tile_instance(a.unit_inst, tile, kTile);
   const InstDev<R> *P = a.inst + inst;
@@ -831,7 +1098,20 @@ __global__ __launch_bounds__(kTile) void k_init_thermal(StepArgs<R> a) {
   const ValleyDev<R> &V = P->valley[v];
   int sub = (int)Math<R>::floor((R)V.deg * us);
   sub = sub < V.deg ? sub : V.deg - 1;
-  const R E = R(-1.5) * P->vt * Math<R>::log(ue);
+  R E;
+  if (init_mean > 0.0) {
+    E = (R)init_mean;
+    if (init_sigma > 0.0) {
+      const R u6 = stream_uniform<R>(6, st, pid, in, a.key0, a.key1);
+      R sn, cs;
+      Math<R>::sincos2pi(u6, &sn, &cs);
+      E = (R)init_mean + (R)init_sigma * (Math<R>::sqrt(R(-2) * Math<R>::log(ue)) * cs);
+      E = (E < R(0)) ? -E : E;
+      E = (E == R(0)) ? (R)init_mean : E;
+    }
+  } else {
+    E = R(-1.5) * P->vt * Math<R>::log(ue);
+  }
   R kx, ky, kz;
   random_direction(normk_of(V, E), uphi, ucos, kx, ky, kz);
   a.kx[gi] = kx;
@@ -848,26 +1128,25 @@ __global__ __launch_bounds__(kTile) void k_init_thermal(StepArgs<R> a) {
 // particles per tile of a step-kernel variant (1 = replay / one per lane)
 int tile_size_of(int ppt) { return kTile * ppt; }
 
-template <typename R>
-hipError_t launch_steps(const StepArgs<R> &a, int mode, int ppt, hipStream_t st) {
-  if (a.n_tiles <= 0)
-    return hipSuccess;
+template <typename R, bool HOT>
+hipError_t launch_steps_t(const StepArgs<R> &a, int mode, int ppt, hipStream_t st) {
   const dim3 g(a.n_tiles), b(kTile);
+  const size_t lds = HOT ? sizeof(uint32_t) * 2 * (size_t)a.lds_bins : 0;
   if (mode == 1) {
-    hipLaunchKernelGGL((k_bulk_replay<R>), g, b, 0, st, a);
+    hipLaunchKernelGGL((k_bulk_replay<R, HOT>), g, b, lds, st, a);
   } else {
     switch (ppt) {
     case 1:
-      hipLaunchKernelGGL((k_bulk_multi<R, 1>), g, b, 0, st, a);
+      hipLaunchKernelGGL((k_bulk_multi<R, 1, HOT>), g, b, lds, st, a);
       break;
     case 2:
-      hipLaunchKernelGGL((k_bulk_multi<R, 2>), g, b, 0, st, a);
+      hipLaunchKernelGGL((k_bulk_multi<R, 2, HOT>), g, b, lds, st, a);
       break;
     case 4:
-      hipLaunchKernelGGL((k_bulk_multi<R, 4>), g, b, 0, st, a);
+      hipLaunchKernelGGL((k_bulk_multi<R, 4, HOT>), g, b, lds, st, a);
       break;
     case 8:
-      hipLaunchKernelGGL((k_bulk_multi<R, 8>), g, b, 0, st, a);
+      hipLaunchKernelGGL((k_bulk_multi<R, 8, HOT>), g, b, lds, st, a);
       break;
     default:
       return hipErrorInvalidValue;
@@ -877,10 +1156,43 @@ hipError_t launch_steps(const StepArgs<R> &a, int mode, int ppt, hipStream_t st)
 }
 
 template <typename R>
-hipError_t launch_init(const StepArgs<R> &a, hipStream_t st) {
+hipError_t launch_steps(const StepArgs<R> &a, int mode, int ppt, hipStream_t st) {
   if (a.n_tiles <= 0)
     return hipSuccess;
-  hipLaunchKernelGGL((k_init_thermal<R>), dim3(a.n_tiles), dim3(kTile), 0, st, a);
+  if (a.tally && a.lds_bins > 0)
+    return launch_steps_t<R, true>(a, mode, ppt, st);
+  return launch_steps_t<R, false>(a, mode, ppt, st);
+}
+
+template <typename R>
+hipError_t launch_init(const StepArgs<R> &a, double init_mean, double init_sigma,
+                       hipStream_t st) {
+  if (a.n_tiles <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL((k_init_thermal<R>), dim3(a.n_tiles), dim3(kTile), 0, st, a, init_mean,
+                     init_sigma);
+  return hipGetLastError();
+}
+
+hipError_t launch_bath_update(const BathDev *baths, int n_baths, double *nq,
+                              const double *tau_prof, unsigned long long *tally, BathVar *var,
+                              double dt, double *hist, int hist_steps, int hist_step,
+                              hipStream_t st) {
+  if (n_baths <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_bath_update, dim3(n_baths), dim3(256), 0, st, baths, nq, tau_prof, tally,
+                     var, dt, hist, hist_steps, hist_step);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_rebuild(const RebuildJob *jobs, int n_jobs, const double *arena,
+                          const BathVar *var, R *cum, double *cum64, InstDev<R> *inst,
+                          double *tau0_64, hipStream_t st) {
+  if (n_jobs <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL((k_rebuild_tables<R>), dim3(n_jobs), dim3(256), 0, st, jobs, arena, var, cum,
+                     cum64, inst, tau0_64);
   return hipGetLastError();
 }
 
@@ -905,7 +1217,13 @@ hipError_t launch_reduce(const double *partial, int n_steps, int n_tiles, int n_
 
 template hipError_t launch_steps<float>(const StepArgs<float> &, int, int, hipStream_t);
 template hipError_t launch_steps<double>(const StepArgs<double> &, int, int, hipStream_t);
-template hipError_t launch_init<float>(const StepArgs<float> &, hipStream_t);
-template hipError_t launch_init<double>(const StepArgs<double> &, hipStream_t);
+template hipError_t launch_init<float>(const StepArgs<float> &, double, double, hipStream_t);
+template hipError_t launch_init<double>(const StepArgs<double> &, double, double, hipStream_t);
+template hipError_t launch_rebuild<float>(const RebuildJob *, int, const double *, const BathVar *,
+                                          float *, double *, InstDev<float> *, double *,
+                                          hipStream_t);
+template hipError_t launch_rebuild<double>(const RebuildJob *, int, const double *,
+                                           const BathVar *, double *, double *,
+                                           InstDev<double> *, double *, hipStream_t);
 
 }  // namespace emcgpu

@@ -14,11 +14,12 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libemcgpu.so")
 
 MAX_VALLEYS, MAX_DEG, MAX_Z, MAX_MECH, MAX_LEVELS = 4, 8, 8, 32, 8192
+MAX_BATHS, MAX_BINS = 4, 1024
 FP64, FP32 = 0, 1
 PRODUCTION, REPLAY = 0, 1
 PARA_ISO, NONPARA_ISO, PARA_ANISO, NONPARA_ANISO = 0, 1, 2, 3
 (ACOUSTIC, ZERO_IV_AB, ZERO_IV_EM, FIRST_IV_AB, FIRST_IV_EM, FROEHLICH_AB,
- FROEHLICH_EM) = range(7)
+ FROEHLICH_EM, HOT_FROEHLICH_AB, HOT_FROEHLICH_EM) = range(9)
 
 STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_STATE", 4: "ERR_DRAWS", 5: "ERR_NOMEM"}
 
@@ -48,7 +49,15 @@ class MechDesc(C.Structure):
     _fields_ = [("kind", C.c_int), ("valley", C.c_int), ("final_valley", C.c_int),
                 ("phonon_energy", C.c_double), ("coupling", C.c_double),
                 ("rel_eff_mass", C.c_double), ("eps_hi", C.c_double), ("eps_lo", C.c_double),
-                ("z", C.c_int), ("final_sub", (C.c_int * MAX_Z) * MAX_DEG)]
+                ("z", C.c_int), ("final_sub", (C.c_int * MAX_Z) * MAX_DEG), ("bath", C.c_int)]
+
+
+class BathDesc(C.Structure):
+    _fields_ = [("n_bins", C.c_int), ("dq", C.c_double), ("tau_lo", C.c_double),
+                ("phonon_energy", C.c_double), ("lattice_temp", C.c_double), ("volume", C.c_double),
+                ("acoustic", C.c_int), ("ac_phonon_energy", C.c_double), ("tau_acoustic", C.c_double),
+                ("w_ridley", C.c_double), ("to_phonon_energy", C.c_double), ("tau_to", C.c_double),
+                ("tau_profile", C.c_void_p)]
 
 
 class Material(C.Structure):
@@ -60,7 +69,7 @@ class StepArgs(C.Structure):
     _fields_ = [("dt", C.c_double), ("n_steps", C.c_int), ("steps_per_launch", C.c_int),
                 ("mode", C.c_int), ("seed", C.c_uint64), ("step0", C.c_uint32),
                 ("timing", C.c_int), ("particles_per_lane", C.c_int), ("use_graph", C.c_int),
-                ("draws", C.c_void_p),
+                ("bath_record_only", C.c_int), ("rebuild_every", C.c_int), ("draws", C.c_void_p),
                 ("draw_stride", C.c_size_t),
                 ("cursor", C.c_void_p), ("grain_tau", C.c_void_p),
                 ("trace_valley", C.c_void_p), ("trace_sub", C.c_void_p),
@@ -74,7 +83,8 @@ EXPORTS = [
     "emcgpu_build_tables", "emcgpu_set_rate_tables", "emcgpu_get_tables", "emcgpu_set_field",
     "emcgpu_resize", "emcgpu_get_count", "emcgpu_set_particles", "emcgpu_get_particles",
     "emcgpu_init_particles", "emcgpu_step", "emcgpu_get_observables", "emcgpu_obs_layout",
-    "emcgpu_get_timing", "emcgpu_synchronize",
+    "emcgpu_get_timing", "emcgpu_synchronize", "emcgpu_init_particles_energy", "emcgpu_set_baths",
+    "emcgpu_get_bath", "emcgpu_set_bath_state", "emcgpu_get_bath_tallies", "emcgpu_get_bath_history",
 ]
 
 
@@ -106,6 +116,12 @@ def load(path=LIB_PATH):
         "emcgpu_obs_layout": (i, [vp, p(i), p(i), p(i)]),
         "emcgpu_get_timing": (i, [vp, p(d), p(C.c_longlong), i]),
         "emcgpu_synchronize": (i, [vp]),
+        "emcgpu_init_particles_energy": (i, [vp, i, C.c_uint64, d, d]),
+        "emcgpu_set_baths": (i, [vp, i, p(BathDesc), i]),
+        "emcgpu_get_bath": (i, [vp, i, i, vp, vp]),
+        "emcgpu_set_bath_state": (i, [vp, i, i, vp, d, d]),
+        "emcgpu_get_bath_tallies": (i, [vp, i, i, vp, vp]),
+        "emcgpu_get_bath_history": (i, [vp, i, i, vp, p(i)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -157,7 +173,31 @@ def mech_desc(m):
         d.rel_eff_mass = m["mrel"]
         d.eps_hi = m["eps_hi"]
         d.eps_lo = m["eps_lo"]
+        d.bath = m.get("bath", 0)
     return d
+
+
+def bath_desc(b, T):
+    """Lower a bath dictionary (tests/emc_models.py format) to the C descriptor;
+    the returned profile array must outlive the call that uses the descriptor."""
+    d = BathDesc()
+    d.n_bins = b["n_bins"]
+    d.dq = b["dq"]
+    d.tau_lo = b["tau_lo"]
+    d.phonon_energy = b["hw"]
+    d.lattice_temp = b.get("T", T)
+    d.volume = b["volume"]
+    d.acoustic = int(b.get("acoustic", False))
+    d.ac_phonon_energy = b.get("ac_hw", 0.0)
+    d.tau_acoustic = b.get("tau_ac", 0.0)
+    d.w_ridley = b.get("w_ridley", 0.0)
+    d.to_phonon_energy = b.get("to_hw", 0.0)
+    d.tau_to = b.get("tau_to", 0.0)
+    prof = None
+    if b.get("tau_profile") is not None:
+        prof = np.ascontiguousarray(b["tau_profile"], dtype=np.float64)
+        d.tau_profile = _ptr(prof)
+    return d, prof
 
 
 class Context:
@@ -224,10 +264,43 @@ class Context:
         d = (C.c_double * 3)(*direction)
         self._chk(self.L.emcgpu_set_field(self.h, instance, d, strength, charge))
 
+    def set_baths(self, instance, baths, T):
+        lowered = [bath_desc(b, T) for b in baths]
+        arr = (BathDesc * max(len(lowered), 1))(*[d for d, _ in lowered])
+        self._chk(self.L.emcgpu_set_baths(self.h, instance, arr, len(lowered)))
+        self.n_bins = getattr(self, "n_bins", {})
+        for b, bd in enumerate(baths):
+            self.n_bins[(instance, b)] = bd["n_bins"]
+
+    def get_bath(self, instance=0, bath=0):
+        nb = self.n_bins[(instance, bath)]
+        nq, sc = np.zeros(nb), np.zeros(6)
+        self._chk(self.L.emcgpu_get_bath(self.h, instance, bath, _ptr(nq), _ptr(sc)))
+        return dict(nq=nq, mean=sc[0], n_ac=sc[1], n_to=sc[2], n0=sc[3], t_ac=sc[4], t_to=sc[5])
+
+    def set_bath_state(self, instance, bath, nq, n_ac=0.0, n_to=0.0):
+        nq = np.ascontiguousarray(nq, dtype=np.float64)
+        self._chk(self.L.emcgpu_set_bath_state(self.h, instance, bath, _ptr(nq), n_ac, n_to))
+
+    def bath_tallies(self, instance=0, bath=0):
+        nb = self.n_bins[(instance, bath)]
+        em, ab = np.zeros(nb, dtype=np.uint64), np.zeros(nb, dtype=np.uint64)
+        self._chk(self.L.emcgpu_get_bath_tallies(self.h, instance, bath, _ptr(em), _ptr(ab)))
+        return em, ab
+
+    def bath_history(self, instance=0, bath=0):
+        n = C.c_int()
+        self._chk(self.L.emcgpu_get_bath_history(self.h, instance, bath, None, C.byref(n)))
+        out = np.zeros(n.value)
+        self._chk(self.L.emcgpu_get_bath_history(self.h, instance, bath, _ptr(out), C.byref(n)))
+        return out
+
     def setup(self, instance, spec, T=None):
-        """set_model + build_tables + set_field from a model dictionary."""
+        """set_model (+ set_baths) + build_tables + set_field from a model dictionary."""
         T = spec["T"] if T is None else T
         self.set_model(instance, spec)
+        if spec.get("baths"):
+            self.set_baths(instance, spec["baths"], T)
         self.build_tables(instance, spec["n_levels"], spec["max_energy"], T,
                           rho=spec.get("rho", 2329.0), v_sound=spec.get("v_sound", 9040.0))
         self.set_field(instance, spec["field_dir"], spec["field_kvcm"] * 1e5)
@@ -264,10 +337,13 @@ class Context:
     def init_particles(self, instance, seed, T):
         self._chk(self.L.emcgpu_init_particles(self.h, instance, seed, T))
 
+    def init_particles_energy(self, instance, seed, mean, sigma=0.0):
+        self._chk(self.L.emcgpu_init_particles_energy(self.h, instance, seed, mean, sigma))
+
     # -- stepping --------------------------------------------------------
     def step(self, dt, n_steps, mode=PRODUCTION, seed=0, step0=0, steps_per_launch=0,
              timing=False, draws=None, cursor=None, grain_tau=None, trace=False, ppl=0,
-             graph=False):
+             graph=False, bath_record_only=False, rebuild_every=0):
         a = StepArgs()
         a.dt = dt
         a.n_steps = n_steps
@@ -278,6 +354,8 @@ class Context:
         a.timing = 1 if timing else 0
         a.particles_per_lane = ppl
         a.use_graph = 1 if graph else 0
+        a.bath_record_only = 1 if bath_record_only else 0
+        a.rebuild_every = rebuild_every
         keep = []
         if mode == REPLAY:
             draws = np.ascontiguousarray(draws, dtype=np.uint64)
