@@ -191,13 +191,19 @@ def test_fp32_cooling_matches_fp64():
         ctx = _ctx(spec, n, p)
         ctx.init_particles_energy(0, 2024, 0.5, 0.05)
         ctx.step(0.5e-15, ns, seed=2024)
-        res[p] = (ctx.averages(0)["avgE"][:, 0], ctx.bath_history(0, 0))
+        res[p] = (ctx.averages(0)["avgE"][:, 0], ctx.bath_history(0, 0), ctx.get_bath(0, 0))
         ctx.close()
-    e64, n64 = res[emcgpu.FP64]
-    e32, n32 = res[emcgpu.FP32]
-    print("cooling: <E> %.4f -> %.4f eV, <N> %.4f -> %.4f" % (e64[0], e64[-1], n64[0], n64[-1]),
-          flush=True)
+    e64, n64, b64 = res[emcgpu.FP64]
+    e32, n32, b32 = res[emcgpu.FP32]
+    print("cooling: <E> %.4f -> %.4f eV, <N> %.4f -> %.4f, max N_q %.3f" % (
+        e64[0], e64[-1], n64[0], n64[-1], b64["nq"].max()), flush=True)
     assert e64[-1] < e64[0] - 0.05  # carriers cool by LO emission
-    assert n64[-1] > n64[0] * 1.5    # and heat the LO bath
+    # ... into small-q phonons (q ~ k_i - k_f): those bins heat far above N0
+    # while the q^2-weighted mean moves little
+    assert b64["nq"].max() > 3 * b64["n0"]
+    assert n64[-1] > n64[0]
     assert np.abs(e32 - e64).max() < 3e-3 * e64.max()
     assert np.abs(n32 - n64).max() < 1e-2 * n64.max()
+    hot = b64["nq"] > b64["n0"] + 1.0
+    assert hot.sum() >= 5
+    assert np.allclose(b32["nq"][hot], b64["nq"][hot], rtol=0.1)
