@@ -49,6 +49,9 @@ def parse():
                     help="particles per GPU lane of the production kernel (0 = library default)")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-instance sweep leg")
     ap.add_argument("--batched-steps", type=int, default=2000)
+    ap.add_argument("--no-hot", action="store_true", help="skip the hot-phonon cooling leg")
+    ap.add_argument("--hot-steps", type=int, default=2000)
+    ap.add_argument("--hot-particles", type=int, default=4_000_000)
     ap.add_argument("--cpu-particles", type=int, default=1_000_000)
     ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -128,6 +131,40 @@ def run_batched(emcgpu, warmup, steps, seed=20261202):
                         "T U[77,500] K, N log-U[1e4,2e6], FP32, HIP-graph launch"}
 
 
+def run_hot_phonon(emcgpu, n, warmup, steps, precision, seed=20261202):
+    """Photoexcited cooling with a hot-phonon bath: every step is the
+    particle step with LDS bath tallies, the bath update and the table
+    rebuild (one HIP graph per call); whole-job particle-updates/s."""
+    from emc_models import photoexcited_cooling
+    spec = photoexcited_cooling(n)
+    ctx = emcgpu.Context(precision=precision)
+    ctx.setup(0, spec)
+    ctx.resize([n])
+    ctx.init_particles_energy(0, seed, 0.5, 0.05)
+    dt = 0.5e-15
+    if warmup:
+        ctx.step(dt, warmup, seed=seed, step0=0, graph=True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(dt, steps, seed=seed, step0=warmup, graph=True)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    av = ctx.averages(0)
+    hist = ctx.bath_history(0, 0)
+    bath = ctx.get_bath(0, 0)
+    ctx.timing(reset=True)
+    ctx.step(dt, 64, seed=seed, step0=warmup + steps, timing=True)
+    ms, launches = ctx.timing()
+    ctx.close()
+    return {"updates_per_s": n * steps / wall, "particles": n, "steps": steps,
+            "ms_per_step": wall / steps * 1e3, "step_kernel_ms": ms / max(launches, 1),
+            "meanE_first": float(av["avgE"][0, 0]), "meanE_last": float(av["avgE"][-1, 0]),
+            "meanNq_last": float(hist[-1]), "maxNq_last": float(bath["nq"].max()),
+            "workload": "photoexcited cooling, %d e- N(0.5, 0.05) eV, Kane Gamma valley, hot-phonon "
+                        "Froehlich on a 256-bin LO bath, dt 0.5 fs, bath update + table rebuild "
+                        "every step, %s" % (n, "FP32" if precision else "FP64")}
+
+
 PROFILE = os.path.join(ROOT, "profiles", "emc_step_si_bulk_10M.md")
 
 
@@ -198,6 +235,12 @@ def main():
         out["fp32_plain_roofline_frac"] = BYTES_FP32 * n / (r1["kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
     if not args.no_batched:
         out["batched_sweep"] = run_batched(emcgpu, args.warmup, args.batched_steps)
+    if not args.no_hot:
+        out["hot_phonon"] = run_hot_phonon(emcgpu, args.hot_particles, args.warmup,
+                                           args.hot_steps, emcgpu.FP32)
+        if not args.no_fp64:
+            out["hot_phonon_fp64"] = run_hot_phonon(emcgpu, args.hot_particles, args.warmup,
+                                                    args.hot_steps, emcgpu.FP64)
     spl = r32["steps_per_launch"]
     # algorithmic bytes per launch: 48 B per particle-update (the SoA state
     # read and written once per update) x the updates one launch processes;

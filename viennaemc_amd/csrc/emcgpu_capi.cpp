@@ -25,14 +25,10 @@ template <typename R>
 hipError_t launch_steps(const StepArgs<R> &a, int mode, int ppt, hipStream_t st);
 template <typename R>
 hipError_t launch_init(const StepArgs<R> &a, double init_mean, double init_sigma, hipStream_t st);
-hipError_t launch_bath_update(const BathDev *baths, int n_baths, double *nq,
-                              const double *tau_prof, unsigned long long *tally, BathVar *var,
-                              double dt, double *hist, int hist_steps, int hist_step,
-                              hipStream_t st);
 template <typename R>
-hipError_t launch_rebuild(const RebuildJob *jobs, int n_jobs, const double *arena,
-                          const BathVar *var, R *cum, double *cum64, InstDev<R> *inst,
-                          double *tau0_64, hipStream_t st);
+hipError_t launch_bath_step(const BathStepArgs<R> &a, int n_groups, hipStream_t st);
+template <typename R>
+hipError_t launch_rebuild(const BathStepArgs<R> &a, int n_jobs, hipStream_t st);
 int reduce_chunks(long long max_count, int tile_size);
 hipError_t launch_reduce(const double *partial, int n_steps, int n_tiles, int n_obs, int tile_size,
                          const long long *inst_start, const long long *inst_count, int n_inst,
@@ -123,6 +119,7 @@ struct emcgpu_ctx {
   hipGraphExec_t gexec = nullptr;  // last captured step sequence
   // phonon baths of all instances (layout built by upload_baths)
   int n_baths = 0, lds_bins = 0, n_jobs = 0;
+  long long tally_bins = 0;  // bins of all baths (one tally copy)
   bool bath_layout_dirty = true;
   BathDev *d_baths = nullptr;
   double *d_nq = nullptr, *d_tau_prof = nullptr;
@@ -132,6 +129,8 @@ struct emcgpu_ctx {
   size_t bath_hist_bytes = 0;
   int bath_hist_steps = 0;
   RebuildJob *d_jobs = nullptr;
+  BathInstDev *d_groups = nullptr;
+  int n_groups = 0;
   double *d_rate_arena = nullptr;
   double *d_cum64 = nullptr, *d_tau0_64 = nullptr;
   size_t cum64_bytes = 0;
@@ -375,6 +374,11 @@ emcgpu_status upload_bath_state(emcgpu_ctx *c) {
       d.to_e = B.d.to_phonon_energy;
       d.tau_to = B.d.tau_to;
       d.n_to_eq = B.n_to_eq;
+      d.sum_w = 0.0;
+      for (int i = 0; i < B.d.n_bins; i++) {
+        const double q = ((double)(std::size_t)i + 0.5) * B.d.dq;
+        d.sum_w += q * q;
+      }
       in.bath_off.push_back(d.off);
       bd.push_back(d);
       bv.push_back(BathVar{B.n_ac, B.n_to, bath_mean(B), 0.0});
@@ -394,17 +398,19 @@ emcgpu_status upload_bath_state(emcgpu_ctx *c) {
   dfreet(c->d_bath_var);
   c->n_baths = (int)bd.size();
   c->lds_bins = lds;
+  c->tally_bins = (long long)nq.size();
   if (c->n_baths) {
     HIPCHK(c, hipMalloc((void **)&c->d_baths, sizeof(BathDev) * bd.size()));
     HIPCHK(c, hipMalloc((void **)&c->d_nq, sizeof(double) * nq.size()));
     HIPCHK(c, hipMalloc((void **)&c->d_tau_prof, sizeof(double) * prof.size()));
-    HIPCHK(c, hipMalloc((void **)&c->d_tally, sizeof(unsigned long long) * 2 * nq.size()));
+    HIPCHK(c, hipMalloc((void **)&c->d_tally,
+                        sizeof(unsigned long long) * 2 * kTallyCopies * nq.size()));
     HIPCHK(c, hipMalloc((void **)&c->d_bath_var, sizeof(BathVar) * bv.size()));
     HIPCHK(c, hipMemcpy(c->d_baths, bd.data(), sizeof(BathDev) * bd.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_nq, nq.data(), sizeof(double) * nq.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_tau_prof, prof.data(), sizeof(double) * prof.size(),
                         hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemset(c->d_tally, 0, sizeof(unsigned long long) * 2 * nq.size()));
+    HIPCHK(c, hipMemset(c->d_tally, 0, sizeof(unsigned long long) * 2 * kTallyCopies * nq.size()));
     HIPCHK(c, hipMemcpy(c->d_bath_var, bv.data(), sizeof(BathVar) * bv.size(),
                         hipMemcpyHostToDevice));
   }
@@ -414,17 +420,45 @@ emcgpu_status upload_bath_state(emcgpu_ctx *c) {
   return EMCGPU_OK;
 }
 
+template <typename R>
+BathStepArgs<R> bath_args(const emcgpu_ctx *c, double dt, double *hist, int hist_steps,
+                          int hist_step, int rebuild) {
+  BathStepArgs<R> a;
+  std::memset(&a, 0, sizeof(a));
+  a.groups = c->d_groups;
+  a.baths = c->d_baths;
+  a.nq = c->d_nq;
+  a.tau_prof = c->d_tau_prof;
+  a.tally = c->d_tally;
+  a.tally_bins = c->tally_bins;
+  a.var = c->d_bath_var;
+  a.dt = dt;
+  a.hist = hist;
+  a.hist_steps = hist_steps;
+  a.hist_step = hist_step;
+  a.rebuild = rebuild;
+  a.jobs = c->d_jobs;
+  a.arena = c->d_rate_arena;
+  a.cum = (R *)c->d_cum;
+  a.cum64 = c->d_cum64;
+  a.inst = (InstDev<R> *)c->d_params;
+  a.tau0_64 = c->d_tau0_64;
+  return a;
+}
+
 // rebuild jobs, their FP64 rate data and the FP64 table copies, then one
 // rebuild so the device tables follow the current bath state
 template <typename R>
 emcgpu_status upload_rebuild(emcgpu_ctx *c, size_t cum_elems) {
   std::vector<RebuildJob> jobs;
+  std::vector<BathInstDev> groups;
   std::vector<double> arena;
   for (int i = 0; i < c->n_inst; i++) {
     const Instance &in = c->inst[i];
-    if (!in.has_hot || !in.has_tables)
+    if (in.baths.empty())
       continue;
-    for (size_t v = 0; v < in.valleys.size(); v++) {
+    BathInstDev g{(int)in.bath_index0, (int)in.baths.size(), (int)jobs.size(), 0};
+    for (size_t v = 0; in.has_hot && in.has_tables && v < in.valleys.size(); v++) {
       const auto &tm = in.table_mech[v];
       bool hot = false;
       for (int j : tm)
@@ -452,10 +486,19 @@ emcgpu_status upload_rebuild(emcgpu_ctx *c, size_t cum_elems) {
       }
       jobs.push_back(J);
     }
+    g.n_jobs = (int)jobs.size() - g.job0;
+    groups.push_back(g);
   }
   dfreet(c->d_jobs);
+  dfreet(c->d_groups);
   dfreet(c->d_rate_arena);
   c->n_jobs = (int)jobs.size();
+  c->n_groups = (int)groups.size();
+  if (!groups.empty()) {
+    HIPCHK(c, hipMalloc((void **)&c->d_groups, sizeof(BathInstDev) * groups.size()));
+    HIPCHK(c, hipMemcpy(c->d_groups, groups.data(), sizeof(BathInstDev) * groups.size(),
+                        hipMemcpyHostToDevice));
+  }
   if (jobs.empty())
     return EMCGPU_OK;
   HIPCHK(c, hipMalloc((void **)&c->d_jobs, sizeof(RebuildJob) * jobs.size()));
@@ -471,8 +514,7 @@ emcgpu_status upload_rebuild(emcgpu_ctx *c, size_t cum_elems) {
   }
   if (!c->d_tau0_64)
     HIPCHK(c, hipMalloc((void **)&c->d_tau0_64, sizeof(double) * kMaxV * c->n_inst));
-  HIPCHK(c, launch_rebuild<R>(c->d_jobs, c->n_jobs, c->d_rate_arena, c->d_bath_var, (R *)c->d_cum,
-                              c->d_cum64, (InstDev<R> *)c->d_params, c->d_tau0_64, c->stream));
+  HIPCHK(c, launch_rebuild<R>(bath_args<R>(c, 0.0, nullptr, 0, 0, 1), c->n_jobs, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return EMCGPU_OK;
 }
@@ -603,6 +645,9 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
   if (baths)
     K = 1;
   const int every = s->rebuild_every > 1 ? s->rebuild_every : 1;
+  // steps whose per-step partials are reduced by one pair of reduction
+  // launches (a launch's steps, or up to 64 one-step launches with baths)
+  const int Kr = baths ? std::min(s->n_steps, 64) : K;
   if (baths && !s->bath_record_only) {
     const size_t hb = sizeof(double) * (size_t)c->n_baths * (size_t)s->n_steps;
     if (hb > c->bath_hist_bytes) {
@@ -618,14 +663,14 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
   const int ppt = replay ? 1 : (ppl ? ppl : default_ppt(c->precision));
   const int tile_size = kTile * ppt;
   const int n_tiles = (int)(c->capacity / tile_size);
-  if ((st = ensure_partial(c, K, n_tiles)) || (st = ensure_obs(c, s->n_steps)))
+  if ((st = ensure_partial(c, Kr, n_tiles)) || (st = ensure_obs(c, s->n_steps)))
     return st;
   long long max_count = 0;
   for (const auto &in : c->inst)
     max_count = std::max<long long>(max_count, (long long)in.count);
   const int n_chunks = reduce_chunks(max_count, tile_size);
   {
-    const size_t bytes = (size_t)c->n_inst * K * c->n_obs * n_chunks * 2 * sizeof(double);
+    const size_t bytes = (size_t)c->n_inst * Kr * c->n_obs * n_chunks * 2 * sizeof(double);
     if (bytes > c->chunk_bytes) {
       dfreet(c->d_chunks);
       HIPCHK(c, hipMalloc((void **)&c->d_chunks, bytes));
@@ -642,6 +687,7 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
   a.dt_s = s->dt;
   if (baths) {
     a.tally = c->d_tally;
+    a.tally_bins = c->tally_bins;
     a.lds_bins = c->lds_bins;
   }
   const size_t n0 = c->inst[0].count;
@@ -722,6 +768,7 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
   for (int done = 0; done < s->n_steps; done += K) {
     const int k = std::min(K, s->n_steps - done);
     a.n_steps = k;
+    a.partial = c->d_partial + (size_t)(done % Kr) * c->n_obs * (size_t)n_tiles;
     a.step0 = s->step0 + (uint32_t)done;
     if (trace) {
       const long long o = (long long)done * (long long)n0;
@@ -744,18 +791,18 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
     }
     if (baths && !s->bath_record_only) {
       // end of the step of the hot-phonon drivers (hotPhononGa2O3.cpp:274-281)
-      HIPCHK_CLEAN(launch_bath_update(c->d_baths, c->n_baths, c->d_nq, c->d_tau_prof, c->d_tally,
-                                      c->d_bath_var, s->dt, c->d_bath_hist, s->n_steps, done,
-                                      c->stream));
       const long long gstep = (long long)s->step0 + done + 1;  // 1-based global step
-      if (c->n_jobs && gstep % every == 0)
-        HIPCHK_CLEAN(launch_rebuild<R>(c->d_jobs, c->n_jobs, c->d_rate_arena, c->d_bath_var,
-                                       (R *)c->d_cum, c->d_cum64, (InstDev<R> *)c->d_params,
-                                       c->d_tau0_64, c->stream));
+      HIPCHK_CLEAN(launch_bath_step<R>(
+          bath_args<R>(c, s->dt, c->d_bath_hist, s->n_steps, done, gstep % every == 0),
+          c->n_groups, c->stream));
     }
-    HIPCHK_CLEAN(launch_reduce(c->d_partial, k, n_tiles, c->n_obs, tile_size, c->d_inst_start,
-                               c->d_inst_count, c->n_inst, n_chunks, c->d_chunks, c->d_obs, done,
-                               s->n_steps, c->stream));
+    // per-step partials of a batch of Kr steps are reduced together
+    const int in_batch = (done % Kr) + k;
+    if (in_batch == Kr || done + k == s->n_steps)
+      HIPCHK_CLEAN(launch_reduce(c->d_partial, in_batch, n_tiles, c->n_obs, tile_size,
+                                 c->d_inst_start, c->d_inst_count, c->n_inst, n_chunks,
+                                 c->d_chunks, c->d_obs, done + k - in_batch, s->n_steps,
+                                 c->stream));
   }
   if (graph) {
     hipGraph_t g = nullptr;
@@ -1599,12 +1646,16 @@ emcgpu_status emcgpu_get_bath_tallies(const emcgpu_ctx *c0, int instance, int ba
   if (bath < 0 || bath >= (int)in.baths.size())
     return fail(c, EMCGPU_ERR_ARG, "instance %d has no bath %d", instance, bath);
   const int nb = in.baths[bath].d.n_bins;
-  std::vector<unsigned long long> t(2 * (size_t)nb, 0ull);
+  std::vector<unsigned long long> t(2 * (size_t)nb, 0ull), part(2 * (size_t)nb);
   if (in.dev_bath && c->device >= 0) {
     (void)hipSetDevice(c->device);
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(t.data(), c->d_tally + 2 * in.bath_off[bath],
-                        sizeof(unsigned long long) * 2 * nb, hipMemcpyDeviceToHost));
+    for (int r = 0; r < kTallyCopies; r++) {
+      HIPCHK(c, hipMemcpy(part.data(), c->d_tally + 2 * (r * c->tally_bins + in.bath_off[bath]),
+                          sizeof(unsigned long long) * 2 * nb, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < part.size(); i++)
+        t[i] += part[i];
+    }
   }
   for (int i = 0; i < nb; i++) {
     if (n_em)

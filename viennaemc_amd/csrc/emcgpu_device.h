@@ -28,6 +28,9 @@ constexpr int kUnit = 2048;
 constexpr int kMaxPPT = 8;
 constexpr int kMaxBaths = EMCGPU_MAX_BATHS;
 constexpr int kMaxBins = EMCGPU_MAX_BINS;
+// copies of the global bath tallies: block b adds its counts to copy
+// b % kTallyCopies, so the few hot bins are not hammered by every block
+constexpr int kTallyCopies = 64;
 
 // final-state sampler of a mechanism
 enum FinalKind : int {
@@ -86,6 +89,7 @@ struct BathDev {
   int n_bins, has_ac, has_ridley, has_profile;
   long long off;  // first bin in the global bin arrays
   double dq, tau_lo, n0, vsim, lo_e, lat_t;
+  double sum_w;   // sum of q_i^2 over the bins in bin order (getMeanNq weight)
   double ac_e, tau_ac, n_ac_eq;
   double w_r, to_e, tau_to, n_to_eq;
 };
@@ -103,6 +107,32 @@ struct RebuildJob {
   int bath[kMaxMech];               // global bath index (hot kinds)
   double C[kMaxMech];               // rate prefactor (hot kinds)
   long long rate_off[kMaxMech];     // const: rates[level]; hot: kI[level], ln[level]
+};
+
+// the baths and rebuild jobs of one instance (one block of k_bath_step)
+struct BathInstDev {
+  int bath0, n_baths, job0, n_jobs;
+};
+
+template <typename R>
+struct BathStepArgs {
+  const BathInstDev *groups;
+  const BathDev *baths;
+  double *nq;
+  const double *tau_prof;
+  unsigned long long *tally;  // [kTallyCopies][tally_bins][2]
+  long long tally_bins;
+  BathVar *var;
+  double dt;
+  double *hist;  // [bath][hist_steps] mean occupation after each step, or null
+  int hist_steps, hist_step;
+  int rebuild;
+  const RebuildJob *jobs;
+  const double *arena;
+  R *cum;
+  double *cum64;
+  InstDev<R> *inst;
+  double *tau0_64;
 };
 
 template <typename R>
@@ -134,8 +164,10 @@ struct StepArgs {
   int32_t *tr_valley, *tr_sub, *tr_events;
   double *tr_k, *tr_E;
   long long tr_n;
-  // phonon-bath event tallies [global bin][2] (emission, absorption)
+  // phonon-bath event tallies [kTallyCopies][global bin][2] (emission,
+  // absorption), tally_bins bins per copy
   unsigned long long *tally;
+  long long tally_bins;
   int lds_bins;  // bins held in LDS per block (max over instances)
 };
 

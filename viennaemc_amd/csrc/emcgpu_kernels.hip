@@ -562,9 +562,10 @@ __device__ __forceinline__ void tally_clear(const InstDev<R> *P, uint32_t *lds_t
 
 template <typename R>
 __device__ __forceinline__ void tally_flush(const InstDev<R> *P, const uint32_t *lds_tally,
-                                            unsigned long long *tally) {
+                                            unsigned long long *tally, long long tally_bins) {
   __syncthreads();
-  unsigned long long *g = tally + 2 * P->gbin_off;
+  unsigned long long *g =
+      tally + 2 * ((long long)(blockIdx.x % kTallyCopies) * tally_bins + P->gbin_off);
   for (int i = threadIdx.x; i < 2 * P->total_bins; i += kTile) {
     const uint32_t c = lds_tally[i];
     if (c)
@@ -691,7 +692,7 @@ __global__ __launch_bounds__(kTile) void k_bulk_multi(StepArgs<R> a) {
     tile_observables<R, PPT>(a, P, wsum, tile, s, valid, v, sub, kx, ky, kz, E, nev, nreal);
   }
   if constexpr (HOT)
-    tally_flush(P, lds_tally, a.tally);
+    tally_flush(P, lds_tally, a.tally, a.tally_bins);
 
 #pragma unroll
   for (int j = 0; j < PPT; ++j) {
@@ -768,7 +769,7 @@ __global__ __launch_bounds__(kTile) void k_bulk_replay(StepArgs<R> a) {
     tile_observables<R, 1>(a, P, wsum, tile, s, valid, v, sub, kx, ky, kz, E, nev, nreal);
   }
   if constexpr (HOT)
-    tally_flush(P, lds_tally, a.tally);
+    tally_flush(P, lds_tally, a.tally, a.tally_bins);
 
   if (valid[0]) {
     a.kx[gi] = kx[0];
@@ -865,15 +866,26 @@ __global__ __launch_bounds__(256) void k_reduce_final(const double *__restrict__
 }
 
 // ---------------------------------------------------------------------------
-// LO phonon bath update (emcPhononBath::update, emcPhononBath.hpp:264-358)
-// and the mean occupation that the hot-phonon rates read (getMeanNq,
-// :461-470).  One block per bath.  The per-bin update is independent across
-// bins and runs on all lanes; the two DOS-weighted sums are sequential over
-// the bins in the reference and are taken in that order by lane 0, so the
-// FP64 result matches the reference operation for operation.
+// End of a hot-phonon step, one block per instance with baths:
+//   every bath of the instance is updated (emcPhononBath::update,
+//   emcPhononBath.hpp:264-358) and its mean occupation taken (getMeanNq,
+//   :461-470); then, on rebuild steps, the instance's scatter tables are
+//   rebuilt from the baths (reinitScatterTables: fillScatterTables +
+//   renormalizeTables, emcScatterHandler.hpp:100-108, 220-272).
+// The per-bin update runs on all lanes; the DOS-weighted sums that the
+// reference takes sequentially over the bins are taken in that order by
+// lane 0 from LDS, so the FP64 results match the reference operation for
+// operation.  Hot-phonon rates are scatterConst * N / kI * ln (absorption)
+// and scatterConst * (N + 1) / kI * ln (emission) with the per-level kI and
+// ln of emcHotPhononFroehlichMechanism.hpp:79-91 / :159-173 precomputed in
+// FP64 on the host (kI = 0 marks a zero rate); other mechanisms keep their
+// FP64 rates.  Cumulative sums run over the tables in registration order and
+// are normalised by the largest total rate; the FP64 result is stored in the
+// precision of the kinematics and in an FP64 copy.
 // ---------------------------------------------------------------------------
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kQ = 1.60219e-19, kKB = 1.38066e-23;  // include/emcConstants.hpp
+constexpr int kBathThreads = 1024;
 
 __device__ double bath_planck(double energy_ev, double temp_k) {
 #pragma clang fp contract(off)
@@ -887,18 +899,19 @@ __device__ double bath_planck_temp(double energy_ev, double n) {
   return kQ * energy_ev / (kKB * ::log(1.0 + 1.0 / n));
 }
 
-__global__ __launch_bounds__(256) void k_bath_update(const BathDev *__restrict__ baths,
-                                                     double *__restrict__ nq,
-                                                     const double *__restrict__ tau_prof,
-                                                     unsigned long long *__restrict__ tally,
-                                                     BathVar *__restrict__ var, double dt,
-                                                     double *__restrict__ hist, int hist_steps,
-                                                     int hist_step) {
+struct BathShared {
+  double old_n[kMaxBins];
+  double new_n[kMaxBins];
+  double w[kMaxBins];
+  double vmax[kBathThreads];
+  double target;
+};
+
+__device__ void bath_update_one(const BathDev &B, BathVar &V, double *__restrict__ nq,
+                                const double *__restrict__ tau_prof,
+                                unsigned long long *__restrict__ tally, long long tally_bins,
+                                double dt, double *__restrict__ hist, BathShared &sh) {
 #pragma clang fp contract(off)
-  __shared__ double s_old[kMaxBins];
-  __shared__ double s_target;
-  const BathDev &B = baths[blockIdx.x];
-  BathVar &V = var[blockIdx.x];
   const int nb = B.n_bins;
   if (threadIdx.x == 0) {
     // decay target: N0 unless a daughter reservoir is hot (:269-286)
@@ -916,25 +929,35 @@ __global__ __launch_bounds__(256) void k_bath_update(const BathDev *__restrict__
         target = n_klemens;
       }
     }
-    s_target = target;
+    sh.target = target;
   }
   __syncthreads();
-  const double target = s_target;
+  const double target = sh.target;
   for (int i = threadIdx.x; i < nb; i += blockDim.x) {
     const long long gi = B.off + i;
     const double q = ((double)i + 0.5) * B.dq;
     const double dph = q * q * B.dq * B.vsim / (2.0 * kPi * kPi);
-    const double n_em = (double)tally[2 * gi], n_abs = (double)tally[2 * gi + 1];
+    // the event counts of this step: sum of the copies, which are cleared
+    unsigned long long c_em = 0ull, c_abs = 0ull;
+#pragma unroll 8
+    for (int r = 0; r < kTallyCopies; ++r) {
+      unsigned long long *t = tally + 2 * ((long long)r * tally_bins + gi);
+      c_em += t[0];
+      c_abs += t[1];
+      t[0] = 0ull;
+      t[1] = 0ull;
+    }
+    const double n_em = (double)c_em, n_abs = (double)c_abs;
     const double g = (dph > 0.0) ? (n_em - n_abs) / (dph * dt) : 0.0;
     const double tau_i = B.has_profile ? tau_prof[gi] : B.tau_lo;
     double n = nq[gi];
-    s_old[i] = n;
+    sh.old_n[i] = n;
+    sh.w[i] = q * q;
     n += g * dt - (dt / tau_i) * (n - target);
     if (n < 0.0)
       n = 0.0;
     nq[gi] = n;
-    tally[2 * gi] = 0ull;
-    tally[2 * gi + 1] = 0ull;
+    sh.new_n[i] = n;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -942,13 +965,12 @@ __global__ __launch_bounds__(256) void k_bath_update(const BathDev *__restrict__
       // reservoirs from the start-of-step occupations (:291-355)
       double sum_w = 0.0, sum_wdn = 0.0, sum_wdn_tau = 0.0;
       for (int i = 0; i < nb; ++i) {
-        const double q = ((double)i + 0.5) * B.dq;
-        const double w = q * q;
+        const double w = sh.w[i];
         sum_w += w;
         if (B.has_profile)
-          sum_wdn_tau += w * (s_old[i] - target) / tau_prof[B.off + i];
+          sum_wdn_tau += w * (sh.old_n[i] - target) / tau_prof[B.off + i];
         else
-          sum_wdn += w * (s_old[i] - target);
+          sum_wdn += w * (sh.old_n[i] - target);
       }
       const double f_klemens = B.has_ridley ? (1.0 - B.w_r) : 1.0;
       const double src = B.has_profile ? ((sum_w > 0.0) ? sum_wdn_tau / sum_w : 0.0)
@@ -976,33 +998,19 @@ __global__ __launch_bounds__(256) void k_bath_update(const BathDev *__restrict__
       V.n_ac = n_ac;
       V.n_to = n_to;
     }
-    // getMeanNq over the updated occupations
-    double sum_w = 0.0, sum_wn = 0.0;
-    for (int i = 0; i < nb; ++i) {
-      const double q = ((double)i + 0.5) * B.dq;
-      const double w = q * q;
-      sum_w += w;
-      sum_wn += w * nq[B.off + i];
-    }
-    const double mean = (sum_w > 0.0) ? sum_wn / sum_w : B.n0;
+    // getMeanNq over the updated occupations (sum of the weights is a
+    // constant of the bath, summed in the same order on the host)
+    double sum_wn = 0.0;
+    for (int i = 0; i < nb; ++i)
+      sum_wn += sh.w[i] * sh.new_n[i];
+    const double mean = (B.sum_w > 0.0) ? sum_wn / B.sum_w : B.n0;
     V.mean = mean;
     if (hist)
-      hist[(long long)blockIdx.x * hist_steps + hist_step] = mean;
+      *hist = mean;
   }
+  __syncthreads();
 }
 
-// ---------------------------------------------------------------------------
-// scatter-table rebuild from the baths (reinitScatterTables:
-// fillScatterTables + renormalizeTables, emcScatterHandler.hpp:100-108,
-// 220-272).  One block per (instance, valley) whose mechanisms read a bath.
-// Hot-phonon rates are scatterConst * N / kI * ln (absorption) and
-// scatterConst * (N + 1) / kI * ln (emission) with the per-level kI and ln
-// of emcHotPhononFroehlichMechanism.hpp:79-91 / :159-173 precomputed in FP64
-// on the host (kI = 0 marks a zero rate); every other mechanism keeps its
-// FP64 rates.  Cumulative sums run over the tables in registration order,
-// normalised by the largest total rate; the FP64 result is stored in the
-// precision of the kinematics and in an FP64 copy.
-// ---------------------------------------------------------------------------
 __device__ __forceinline__ double rebuild_rate(const RebuildJob &J, int j, int l,
                                                const double *__restrict__ arena,
                                                const BathVar *__restrict__ var) {
@@ -1019,35 +1027,28 @@ __device__ __forceinline__ double rebuild_rate(const RebuildJob &J, int j, int l
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_rebuild_tables(const RebuildJob *__restrict__ jobs,
-                                                        const double *__restrict__ arena,
-                                                        const BathVar *__restrict__ var,
-                                                        R *__restrict__ cum,
-                                                        double *__restrict__ cum64,
-                                                        InstDev<R> *__restrict__ inst,
-                                                        double *__restrict__ tau0_64) {
+__device__ void rebuild_one(const RebuildJob &J, const double *__restrict__ arena,
+                            const BathVar *__restrict__ var, R *__restrict__ cum,
+                            double *__restrict__ cum64, InstDev<R> *__restrict__ inst,
+                            double *__restrict__ tau0_64, BathShared &sh) {
 #pragma clang fp contract(off)
-  __shared__ double s_max[256];
-  const RebuildJob &J = jobs[blockIdx.x];
   const int nt = J.n_tables, nl = J.n_levels;
-  double mx = 0.0;
-  bool first = true;
+  double mx = -1.0;
   for (int l = threadIdx.x; l < nl; l += blockDim.x) {
     double c = rebuild_rate(J, 0, l, arena, var);
     for (int j = 1; j < nt; ++j)
       c = rebuild_rate(J, j, l, arena, var) + c;
-    if (first || mx < c)
-      mx = c;
-    first = false;
+    mx = (mx < c) ? c : mx;
   }
-  s_max[threadIdx.x] = first ? -1.0 : mx;
+  sh.vmax[threadIdx.x] = mx;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w && s_max[threadIdx.x] < s_max[threadIdx.x + w])
-      s_max[threadIdx.x] = s_max[threadIdx.x + w];
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w && sh.vmax[threadIdx.x] < sh.vmax[threadIdx.x + w])
+      sh.vmax[threadIdx.x] = sh.vmax[threadIdx.x + w];
     __syncthreads();
   }
-  mx = s_max[0];
+  mx = sh.vmax[0];
+  __syncthreads();
   if (!(mx > 0.0))
     return;  // no rate at any level: keep the previous tables
   for (int l = threadIdx.x; l < nl; l += blockDim.x) {
@@ -1065,6 +1066,26 @@ __global__ __launch_bounds__(256) void k_rebuild_tables(const RebuildJob *__rest
     inst[J.inst].tau0[J.valley] = (R)(tau * 1e15);
     tau0_64[J.inst * kMaxV + J.valley] = tau;
   }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBathThreads) void k_bath_step(BathStepArgs<R> a) {
+  __shared__ BathShared sh;
+  const BathInstDev &I = a.groups[blockIdx.x];
+  for (int b = I.bath0; b < I.bath0 + I.n_baths; ++b)
+    bath_update_one(a.baths[b], a.var[b], a.nq, a.tau_prof, a.tally, a.tally_bins, a.dt,
+                    a.hist ? a.hist + (long long)b * a.hist_steps + a.hist_step : nullptr, sh);
+  if (!a.rebuild)
+    return;
+  for (int j = I.job0; j < I.job0 + I.n_jobs; ++j)
+    rebuild_one<R>(a.jobs[j], a.arena, a.var, a.cum, a.cum64, a.inst, a.tau0_64, sh);
+}
+
+// table rebuild alone (after the model, the field or a bath state changed)
+template <typename R>
+__global__ __launch_bounds__(kBathThreads) void k_rebuild_only(BathStepArgs<R> a) {
+  __shared__ BathShared sh;
+  rebuild_one<R>(a.jobs[blockIdx.x], a.arena, a.var, a.cum, a.cum64, a.inst, a.tau0_64, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -1174,25 +1195,19 @@ hipError_t launch_init(const StepArgs<R> &a, double init_mean, double init_sigma
   return hipGetLastError();
 }
 
-hipError_t launch_bath_update(const BathDev *baths, int n_baths, double *nq,
-                              const double *tau_prof, unsigned long long *tally, BathVar *var,
-                              double dt, double *hist, int hist_steps, int hist_step,
-                              hipStream_t st) {
-  if (n_baths <= 0)
+template <typename R>
+hipError_t launch_bath_step(const BathStepArgs<R> &a, int n_groups, hipStream_t st) {
+  if (n_groups <= 0)
     return hipSuccess;
-  hipLaunchKernelGGL(k_bath_update, dim3(n_baths), dim3(256), 0, st, baths, nq, tau_prof, tally,
-                     var, dt, hist, hist_steps, hist_step);
+  hipLaunchKernelGGL((k_bath_step<R>), dim3(n_groups), dim3(kBathThreads), 0, st, a);
   return hipGetLastError();
 }
 
 template <typename R>
-hipError_t launch_rebuild(const RebuildJob *jobs, int n_jobs, const double *arena,
-                          const BathVar *var, R *cum, double *cum64, InstDev<R> *inst,
-                          double *tau0_64, hipStream_t st) {
+hipError_t launch_rebuild(const BathStepArgs<R> &a, int n_jobs, hipStream_t st) {
   if (n_jobs <= 0)
     return hipSuccess;
-  hipLaunchKernelGGL((k_rebuild_tables<R>), dim3(n_jobs), dim3(256), 0, st, jobs, arena, var, cum,
-                     cum64, inst, tau0_64);
+  hipLaunchKernelGGL((k_rebuild_only<R>), dim3(n_jobs), dim3(kBathThreads), 0, st, a);
   return hipGetLastError();
 }
 
@@ -1219,11 +1234,9 @@ template hipError_t launch_steps<float>(const StepArgs<float> &, int, int, hipSt
 template hipError_t launch_steps<double>(const StepArgs<double> &, int, int, hipStream_t);
 template hipError_t launch_init<float>(const StepArgs<float> &, double, double, hipStream_t);
 template hipError_t launch_init<double>(const StepArgs<double> &, double, double, hipStream_t);
-template hipError_t launch_rebuild<float>(const RebuildJob *, int, const double *, const BathVar *,
-                                          float *, double *, InstDev<float> *, double *,
-                                          hipStream_t);
-template hipError_t launch_rebuild<double>(const RebuildJob *, int, const double *,
-                                           const BathVar *, double *, double *,
-                                           InstDev<double> *, double *, hipStream_t);
+template hipError_t launch_bath_step<float>(const BathStepArgs<float> &, int, hipStream_t);
+template hipError_t launch_bath_step<double>(const BathStepArgs<double> &, int, hipStream_t);
+template hipError_t launch_rebuild<float>(const BathStepArgs<float> &, int, hipStream_t);
+template hipError_t launch_rebuild<double>(const BathStepArgs<double> &, int, hipStream_t);
 
 }  // namespace emcgpu
