@@ -172,8 +172,8 @@ static double orc_square3(const double v[3]) {
 static double orc_norm3(const double v[3]) { return sqrt(orc_square3(v)); }
 
 /* initRandomDirection (emcUtil.hpp:131-139) */
-static void orc_init_random_direction(double norm, double rand1,
-                                      double rand2, double res[3]) {
+void orc_init_random_direction(double norm, double rand1, double rand2,
+                               double res[3]) {
   double phi = 2 * ORC_PI * rand1;
   double cosTheta = 1 - 2 * rand2;
   res[0] = norm * sqrt(1 - cosTheta * cosTheta) * cos(phi);

@@ -129,6 +129,10 @@ typedef struct {
   int region;
 } orc_particle;
 
+/* initRandomDirection (emcUtil.hpp:131-139) */
+void orc_init_random_direction(double norm, double rand1, double rand2,
+                               double res[3]);
+
 /* emcParticleDrift.hpp:11-35 */
 void orc_drift(double dt, orc_particle *p, const orc_valley *v, double *pos,
                int dim, const double force[3]);

@@ -80,6 +80,26 @@ def load():
         "orc_model_bath_step": (None, [vp, _D, _I]),
         "orc_model_set_bath_mode": (None, [vp, _I, _I, C.c_longlong]),
         "orc_model_set_init_energy": (None, [vp, _D]),
+        "orc_device_new": (vp, [_D, _D, _D, _D, _D, _D, _D]),
+        "orc_device_delete": (None, [vp]),
+        "orc_device_set_width": (None, [vp, _D]),
+        "orc_device_add_doping": (None, [vp, _PD, _PD, _D]),
+        "orc_device_add_ohmic": (_I, [vp, _I, _D, _D, _D]),
+        "orc_device_info": (None, [vp, _PI32, _PD]),
+        "orc_device_get_grids": (None, [vp, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "orc_sor_solve": (C.c_longlong, [vp, _PD, C.c_void_p, _D, _D, _I]),
+        "orc_pm_assign_h": (None, [vp, _I, _PD, _I, _D, _PD]),
+        "orc_pm_forces_h": (None, [vp, _I, _PD, _PD, _PD, _I, _D, _PD]),
+        "orc_pm_efield": (None, [_I, vp, _PD, _PD, _PD]),
+        "orc_device_init_potential": (None, [vp, _PD]),
+        "orc_device_concentration": (None, [vp, _PD, _PD]),
+        "orc_device_contact_rule": (None, [vp, _PD, _I, _D, _PI32, _PI32, _PD]),
+        "orc_device_init_replay": (None, [vp, vp, _I, _PI32, _PU64, C.c_size_t, _PSZ, _PD]),
+        "orc_device_run_replay": (None, [vp, vp, _I, _PD, _PD, _I, _PD, _PU64, C.c_size_t, _PSZ, _I, _D,
+                                         _PI32, _PI32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "orc_device_run_h": (_I, [vp, vp, _I, _D, _D, _D, _I, C.c_uint64, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_void_p, _I]),
         "orc_mt64_fill": (None, [C.c_uint64, _PU64, C.c_size_t]),
         "orc_mt64_fill_particles": (None, [C.c_uint64, C.c_uint64, C.c_size_t, _PU64, C.c_size_t, C.c_size_t]),
         "orc_particle_seed": (C.c_uint64, [C.c_uint64, C.c_uint64]),
@@ -333,3 +353,126 @@ def philox(ctr, key):
 def thermal_voltage(T):
     """kB / q * T (emcDevice.hpp:88)."""
     return KB / Q * T
+
+
+PM_NGP, PM_CIC, PM_NEC = 0, 1, 2
+
+
+class OracleDevice:
+    """A reference 2-D device (emcDevice + doping + ohmic contacts) in the oracle."""
+
+    def __init__(self, spec):
+        L = load()
+        self.L = L
+        self.spec = spec
+        mx, my = spec["max_pos"]
+        hx, hy = spec["spacing"]
+        self.h = L.orc_device_new(mx, my, hx, hy, spec["T"], spec["eps_r"], spec["ni"])
+        L.orc_device_set_width(self.h, spec["width"])
+        for lo, hi, dop in spec["doping"]:
+            L.orc_device_add_doping(self.h, np.array(lo, dtype=np.float64), np.array(hi, dtype=np.float64),
+                                    dop)
+        for b, v, lo, hi in spec["contacts"]:
+            if L.orc_device_add_ohmic(self.h, b, v, lo, hi) < 0:
+                raise ValueError("invalid contact")
+        ext = np.zeros(2, dtype=np.int32)
+        sc = np.zeros(4)
+        L.orc_device_info(self.h, ext, sc)
+        self.ext = (int(ext[0]), int(ext[1]))
+        self.n_nodes = self.ext[0] * self.ext[1]
+        self.Vt, self.debye, self.cell_volume, self.norm_cell_volume = sc
+
+    def __del__(self):
+        try:
+            self.L.orc_device_delete(self.h)
+        except Exception:
+            pass
+
+    def grids(self):
+        d = np.zeros(self.n_nodes)
+        r = np.zeros(self.n_nodes, dtype=np.int32)
+        o = np.zeros(self.n_nodes, dtype=np.int32)
+        self.L.orc_device_get_grids(self.h, _ptr(d), _ptr(r), _ptr(o))
+        return dict(doping=d, region=r, contact=o)
+
+    def init_potential(self):
+        pot = np.zeros(self.n_nodes)
+        self.L.orc_device_init_potential(self.h, pot)
+        return pot
+
+    def sor(self, pot, conc=None, accuracy=1e-12, omega=1.8, reset_bc=True):
+        pot = np.array(pot, dtype=np.float64)
+        c = None if conc is None else np.ascontiguousarray(conc, dtype=np.float64)
+        sweeps = self.L.orc_sor_solve(self.h, pot, _ptr(c), accuracy, omega, int(reset_bc))
+        return pot, sweeps
+
+    def assign(self, scheme, pos, carriers=1.0):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+        g = np.zeros(self.n_nodes)
+        self.L.orc_pm_assign_h(self.h, scheme, pos, pos.size // 2, carriers, g)
+        return g
+
+    def forces(self, scheme, ex, ey, pos, charge=-Q):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+        f = np.zeros(3 * (pos.size // 2))
+        self.L.orc_pm_forces_h(self.h, scheme, np.ascontiguousarray(ex, dtype=np.float64),
+                               np.ascontiguousarray(ey, dtype=np.float64), pos, pos.size // 2, charge, f)
+        return f.reshape(-1, 3)
+
+    def efield(self, scheme, pot):
+        ex, ey = np.zeros(self.n_nodes), np.zeros(self.n_nodes)
+        self.L.orc_pm_efield(scheme, self.h, np.ascontiguousarray(pot, dtype=np.float64), ex, ey)
+        return ex, ey
+
+    def concentration(self, nr_part):
+        c = np.zeros(self.n_nodes)
+        self.L.orc_device_concentration(self.h, np.ascontiguousarray(nr_part, dtype=np.float64), c)
+        return c
+
+    def contact_rule(self, pos, carriers=1.0):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+        n = pos.size // 2
+        keep = np.zeros(max(n, 1), dtype=np.int32)
+        inj = np.zeros(self.n_nodes, dtype=np.int32)
+        exp = np.zeros(self.n_nodes)
+        self.L.orc_device_contact_rule(self.h, pos, n, carriers, keep, inj, exp)
+        return keep[:n], inj, exp
+
+    def init_replay(self, model, coords, draws, cursor):
+        st = np.zeros((len(coords), 10))
+        self.L.orc_device_init_replay(model.h, self.h, len(coords),
+                                      np.ascontiguousarray(coords, dtype=np.int32).reshape(-1),
+                                      draws, draws.shape[1], cursor, st)
+        return st
+
+    def run_replay(self, model, scheme, ex, ey, state, draws, cursor, nsteps, dt, trace=True):
+        np_ = state.shape[0]
+        rem = np.full(np_, -1, dtype=np.int32)
+        con = np.full(np_, -1, dtype=np.int32)
+        tr = {}
+        if trace:
+            tr = dict(pos=np.zeros(nsteps * np_ * 2), k=np.zeros(nsteps * np_ * 3), E=np.zeros(nsteps * np_),
+                      nev=np.zeros(nsteps * np_, dtype=np.int32))
+        self.L.orc_device_run_replay(model.h, self.h, scheme, np.ascontiguousarray(ex, dtype=np.float64),
+                                     np.ascontiguousarray(ey, dtype=np.float64), np_, state, draws,
+                                     draws.shape[1], cursor, nsteps, dt, rem, con, _ptr(tr.get("pos")),
+                                     _ptr(tr.get("k")), _ptr(tr.get("E")), _ptr(tr.get("nev")))
+        return rem, con, tr
+
+    def run(self, model, scheme, accuracy, omega, dt, nsteps, seed, max_state=0):
+        nn, nc = self.n_nodes, len(self.spec["contacts"])
+        out = dict(eq_pot=np.zeros(nn), n_init=np.zeros(1, dtype=np.int32),
+                   n_part=np.zeros(nsteps, dtype=np.int32), pot=np.zeros(nsteps * nn),
+                   exits=np.zeros(nsteps * nc, dtype=np.int32), dels=np.zeros(nsteps * nc, dtype=np.int32),
+                   injs=np.zeros(nsteps * nc, dtype=np.int32), nr_part=np.zeros(nn),
+                   state=np.zeros(max(max_state, 1) * 9))
+        n = self.L.orc_device_run_h(self.h, model.h, scheme, accuracy, omega, dt, nsteps, seed,
+                                    _ptr(out["eq_pot"]), _ptr(out["n_init"]), _ptr(out["n_part"]),
+                                    _ptr(out["pot"]), _ptr(out["exits"]), _ptr(out["dels"]),
+                                    _ptr(out["injs"]), _ptr(out["nr_part"]), _ptr(out["state"]), max_state)
+        out["n_final"] = n
+        out["pot"] = out["pot"].reshape(nsteps, nn)
+        for k in ("exits", "dels", "injs"):
+            out[k] = out[k].reshape(nsteps, nc)
+        out["state"] = out["state"][:min(n, max_state) * 9].reshape(-1, 9)
+        return out

@@ -35,6 +35,11 @@
 #include <emcPhononBath.hpp>
 
 #include "../examples/bulkSimulation/basicBulkParticleHandler.hpp"
+#include <PMSchemes/emcCICScheme.hpp>
+#include <PMSchemes/emcNECScheme.hpp>
+#include <PMSchemes/emcNGPScheme.hpp>
+#include <ParticleHandler/emcBasicParticleHandler.hpp>
+#include <PoissonSolver/emcSORSolver.hpp>
 
 using T = double;
 using Device = emcDevice<T, 3>;
@@ -587,6 +592,294 @@ static void dumpBathKATs() {
   sec("bath.params", {64, 5e7, 1e-12, 0.06, 300., 2e-22, 0.03, 1e-11, 0.02, 5e-12, 2e-14});
 }
 
+// ---------------------------------------------------------------------------
+// 2-D device path: a 21 x 7 node n+ n n+ resistor with ohmic contacts at
+// XMIN (0.2 V) and XMAX (0 V), Si material, one parabolic valley (m* 0.26)
+// with acoustic + zero-order intervalley phonons for every doping region.
+// ---------------------------------------------------------------------------
+using Device2 = emcDevice<T, 2>;
+using Electron2 = emcElectron<T, Device2>;
+using Grid2 = emcGrid<T, 2>;
+using TypeMap2 = std::map<SizeType, std::unique_ptr<emcParticleType<T, Device2>>>;
+
+static const double kDevMax[2] = {20e-9, 6e-9};
+static const double kDevH = 1e-9, kDevWidth = 1e-6, kDevVBias = 0.2;
+
+static std::unique_ptr<Device2> makeDevice2() {
+  auto dev = std::make_unique<Device2>(silicon(), std::array<T, 2>{kDevMax[0], kDevMax[1]},
+                                       std::array<T, 2>{kDevH, kDevH}, 300.);
+  dev->setDeviceWidth(kDevWidth);
+  dev->addConstantDopingRegion({0, 0}, {kDevMax[0], kDevMax[1]}, 1e24);
+  dev->addConstantDopingRegion({0, 0}, {3e-9, kDevMax[1]}, 1e25);
+  dev->addConstantDopingRegion({17e-9, 0}, {kDevMax[0], kDevMax[1]}, 1e25);
+  dev->addOhmicContact(emcBoundaryPos::XMIN, kDevVBias, {0}, {kDevMax[1]});
+  dev->addOhmicContact(emcBoundaryPos::XMAX, 0., {0}, {kDevMax[1]});
+  return dev;
+}
+
+static std::unique_ptr<Electron2> makeDeviceType(Device2 &dev) {
+  auto type = std::make_unique<Electron2>(1000, 1.5, false);
+  type->addValley(std::make_unique<emcParabolicIsotropValley<T>>(0.26, type->getMass(), 1));
+  std::map<SizeType, std::vector<SizeType>> self = {{0, {0}}};
+  using ZAb = emcZeroOrderInterValleyAbsorptionScatterMechanism<T>;
+  using ZEm = emcZeroOrderInterValleyEmissionScatterMechanism<T>;
+  type->addScatterMechanism({0, 1, 2}, std::make_unique<emcAcousticScatterMechanism<T>>(0, 9., dev));
+  type->addScatterMechanism({0, 1, 2}, std::make_unique<ZAb>("P", 0, self, 5e10, 0.05, dev));
+  type->addScatterMechanism({0, 1, 2}, std::make_unique<ZEm>("P", 0, self, 5e10, 0.05, dev));
+  return type;
+}
+
+template <class T2> static std::vector<double> gridVec(const emcGrid<T2, 2> &g) {
+  return std::vector<double>(g.begin(), g.end());
+}
+
+// the emcSimulation step order (emcSimulation.hpp:106-193) with the
+// reference's solver, PM scheme and particle handler, one shared stream
+template <class PM>
+static void dumpDeviceRun(const std::string &tag, uint64_t seed, int nSteps, double dt,
+                          double accV, double omega) {
+  auto dev = makeDevice2();
+  PM pm;
+  TypeMap2 types;
+  types[0] = makeDeviceType(*dev);
+  using Handler2 = emcBasicParticleHandler<T, Device2, PM>;
+  using Solver2 = emcSORSolver<T, Device2, Handler2>;
+  Handler2 handler(*dev, pm, types, 1, seed);
+  Solver2 solver(*dev, accV, omega);
+  auto ext = dev->getGridExtent();
+  Grid2 pot(ext);
+  std::array<SizeType, 2> c;
+  for (c.fill(0); !pot.isEndCoord(c); pot.advanceCoord(c))
+    pot[c] = std::asinh(0.5 * dev->getDopingProfile().getDoping(c, true));
+  solver.calcEquilibriumPotential(pot, *dev);
+  std::vector<Grid2> eField(2, Grid2(ext));
+  pm.calcEField(eField, pot, *dev);
+  std::string b = "device." + tag + ".";
+  sec((b + "eqpot").c_str(), gridVec(pot));
+  handler.generateInitialParticles(pot);
+  Grid2 nrPart(ext, 0), conc(ext, 0);
+  auto concentration = [&]() {
+    // updateCurrentParticleConcentrations (emcSimulationResults.hpp:98-116)
+    std::array<SizeType, 2> cc;
+    T factor = 1. / dev->getCellVolume();
+    for (cc.fill(0); !conc.isEndCoord(cc); conc.advanceCoord(cc)) {
+      T v = dev->normalizeDoping(nrPart[cc]) * factor;
+      for (int d = 0; d < 2; d++)
+        if (cc[d] == 0 || cc[d] == ext[d] - 1)
+          v *= 2;
+      conc[cc] = v;
+    }
+  };
+  nrPart.fill(0);
+  handler.assignParticlesToMesh(0, nrPart);
+  concentration();
+  seci((b + "ninit").c_str(), {(long long)handler.getNrParticles(0)});
+  std::vector<double> pots;
+  std::vector<long long> npart, exits, netto;
+  bool resetBC = true;
+  for (int s = 0; s < nSteps; s++) {
+    solver.calcNonEquilibriumPotential(pot, *dev, conc, resetBC);
+    resetBC = false;
+    auto pv = gridVec(pot);
+    pots.insert(pots.end(), pv.begin(), pv.end());
+    pm.calcEField(eField, pot, *dev);
+    auto rem = handler.driftScatterParticles(dt, eField);
+    auto inj = handler.handleOhmicContacts();
+    for (int k = 0; k < 2; k++) {
+      exits.push_back(rem[0][k]);
+      netto.push_back(inj[0][k]);
+    }
+    nrPart.fill(0);
+    handler.assignParticlesToMesh(0, nrPart);
+    concentration();
+    npart.push_back((long long)handler.getNrParticles(0));
+  }
+  sec((b + "pot").c_str(), pots);
+  seci((b + "npart").c_str(), npart);
+  seci((b + "exits").c_str(), exits);
+  seci((b + "ohmic").c_str(), netto);
+  sec((b + "nrpart_last").c_str(), gridVec(nrPart));
+  sec((b + "cfg").c_str(), {(double)seed, (double)nSteps, dt, accV, omega});
+}
+
+// PM schemes, E-field and SOR known answers on the same device
+static void dumpDeviceKATs() {
+  auto dev = makeDevice2();
+  auto ext = dev->getGridExtent();
+  uint64_t st = 0xDE71CEull;
+  auto u = [&]() {
+    st = splitmix64(st);
+    return (double)(st >> 11) * 0x1p-53;
+  };
+  std::vector<Grid2> eField(2, Grid2(ext));
+  std::array<SizeType, 2> c;
+  for (int d = 0; d < 2; d++)
+    for (c.fill(0); !eField[d].isEndCoord(c); eField[d].advanceCoord(c))
+      eField[d][c] = (u() - 0.5) * 2e7;
+  sec("device.kat.ex", gridVec(eField[0]));
+  sec("device.kat.ey", gridVec(eField[1]));
+  std::vector<std::array<T, 2>> pos;
+  std::vector<double> posFlat;
+  for (int i = 0; i < 64; i++) {
+    std::array<T, 2> p = {u() * kDevMax[0] * 0.999, u() * kDevMax[1] * 0.999};
+    if (i == 0)
+      p = {0, 0};
+    if (i == 1)
+      p = {kDevH * 0.5, kDevH * 2.5};  // exactly between nodes
+    pos.push_back(p);
+    posFlat.insert(posFlat.end(), p.begin(), p.end());
+  }
+  sec("device.kat.pos", posFlat);
+  auto schemes = [&](auto pm, const char *name) {
+    Grid2 g(ext, 0);
+    pm.assignToMesh(pos, 1, dev->getSpacing(), g);
+    sec((std::string("device.kat.assign_") + name).c_str(), gridVec(g));
+    std::vector<double> f;
+    for (auto &p : pos) {
+      auto fr = pm.interpolateForce(eField, p, dev->getSpacing(), -constants::q);
+      f.insert(f.end(), fr.begin(), fr.end());
+    }
+    sec((std::string("device.kat.force_") + name).c_str(), f);
+  };
+  schemes(emcNGPScheme<T, Device2>(), "ngp");
+  schemes(emcCICScheme<T, Device2>(), "cic");
+  schemes(emcNECScheme<T, Device2>(), "nec");
+  Grid2 pot(ext);
+  for (c.fill(0); !pot.isEndCoord(c); pot.advanceCoord(c))
+    pot[c] = (u() - 0.5) * 10;
+  sec("device.kat.pot", gridVec(pot));
+  emcCICScheme<T, Device2>().calcEField(eField, pot, *dev);
+  sec("device.kat.efield_grid_x", gridVec(eField[0]));
+  sec("device.kat.efield_grid_y", gridVec(eField[1]));
+  emcNECScheme<T, Device2>().calcEField(eField, pot, *dev);
+  sec("device.kat.efield_edge_x", gridVec(eField[0]));
+  sec("device.kat.efield_edge_y", gridVec(eField[1]));
+  // SOR: equilibrium and non-equilibrium solutions at 1e-12 V
+  using Solver2 = emcSORSolver<T, Device2, emcBasicParticleHandler<T, Device2, emcCICScheme<T, Device2>>>;
+  Solver2 solver(*dev, 1e-12, 1.8);
+  for (c.fill(0); !pot.isEndCoord(c); pot.advanceCoord(c))
+    pot[c] = std::asinh(0.5 * dev->getDopingProfile().getDoping(c, true));
+  solver.calcEquilibriumPotential(pot, *dev);
+  sec("device.kat.sor_eq", gridVec(pot));
+  Grid2 conc(ext);
+  for (c.fill(0); !conc.isEndCoord(c); conc.advanceCoord(c))
+    conc[c] = dev->getDopingProfile().getDoping(c, true) * (0.8 + 0.1 * u());
+  sec("device.kat.sor_conc", gridVec(conc));
+  solver.calcNonEquilibriumPotential(pot, *dev, conc, true);
+  sec("device.kat.sor_neq", gridVec(pot));
+}
+
+// device particle trajectories under a frozen field with per-particle
+// streams: the loop body of emcBasicParticleHandler::driftScatterParticles
+// (:93-139) with driftParticle (emcAbstractParticleHandler.hpp:238-249)
+static void dumpDeviceTrajectories(uint64_t seed, int np, int nsteps, double dt) {
+  auto dev = makeDevice2();
+  auto ext = dev->getGridExtent();
+  auto type = makeDeviceType(*dev);
+  type->initScatterTables();
+  emcCICScheme<T, Device2> pm;
+  // field: the non-equilibrium potential with conc = 0.9 * doping
+  using Solver2 = emcSORSolver<T, Device2, emcBasicParticleHandler<T, Device2, emcCICScheme<T, Device2>>>;
+  Solver2 solver(*dev, 1e-10, 1.8);
+  Grid2 pot(ext), conc(ext);
+  std::array<SizeType, 2> c;
+  for (c.fill(0); !pot.isEndCoord(c); pot.advanceCoord(c)) {
+    pot[c] = std::asinh(0.5 * dev->getDopingProfile().getDoping(c, true));
+    conc[c] = 0.9 * dev->getDopingProfile().getDoping(c, true);
+  }
+  solver.calcEquilibriumPotential(pot, *dev);
+  solver.calcNonEquilibriumPotential(pot, *dev, conc, true);
+  std::vector<Grid2> eField(2, Grid2(ext));
+  pm.calcEField(eField, pot, *dev);
+  sec("device.traj.ex", gridVec(eField[0]));
+  sec("device.traj.ey", gridVec(eField[1]));
+  std::vector<emcParticle<T>> parts(np);
+  std::vector<std::array<T, 2>> pos(np);
+  std::vector<emcRNG> rngs;
+  std::vector<double> initState;
+  std::vector<long long> initDraws;
+  for (int p = 0; p < np; p++) {
+    rngs.emplace_back(splitmix64(seed ^ (uint64_t)p));
+    std::array<SizeType, 2> cc = {(SizeType)((p * 5) % ext[0]), (SizeType)(p % ext[1])};
+    pos[p] = initParticlePos(cc, ext, dev->getSpacing(), rngs[p]);
+    parts[p] = type->generateInitialParticle(cc, *dev, rngs[p]);
+    initState.insert(initState.end(), {pos[p][0], pos[p][1], parts[p].k[0], parts[p].k[1],
+                                       parts[p].k[2], parts[p].energy, parts[p].tau,
+                                       parts[p].grainTau});
+    initDraws.push_back(RngCounter::between(emcRNG(splitmix64(seed ^ (uint64_t)p)), rngs[p]));
+  }
+  std::vector<double> posOut, kOut, eOut;
+  std::vector<long long> nevOut, removedAt, contactOut, drawsOut;
+  std::vector<int> removedStep(np, -1);
+  for (int s = 0; s < nsteps; s++) {
+    for (int p = 0; p < np; p++) {
+      auto &part = parts[p];
+      auto &ps = pos[p];
+      auto &rng = rngs[p];
+      long long nev = 0;
+      if (removedStep[p] < 0) {
+        emcRNG atStart = rng;
+        auto dp = [&](T t, const std::array<T, 3> &force) {
+          drift(t, part, type->getValley(part.valley), ps, force);
+          bool removed = handleParticleAtBoundary(part, ps, type.get(), *dev, rng);
+          if (!removed)
+            part.region = dev->getDopingProfile().getDopingRegionIdx(dev->posToCoord(ps));
+          return removed;
+        };
+        auto force = pm.interpolateForce(eField, ps, dev->getSpacing(), type->getCharge());
+        bool removed = false;
+        if (part.tau >= dt) {
+          removed = dp(dt, force);
+        } else {
+          removed = dp(part.tau, force);
+          T tRem = dt - part.tau;
+          while (tRem > 0 && !removed) {
+            type->scatterParticle(part, rng);
+            nev++;
+            T newTau = type->getNewTau(part.valley, part.region, rng);
+            part.tau += newTau;
+            force = pm.interpolateForce(eField, ps, dev->getSpacing(), type->getCharge());
+            removed = dp(std::min(tRem, newTau), force);
+            tRem -= newTau;
+          }
+        }
+        part.tau -= dt;
+        part.grainTau -= dt;
+        if (part.grainTau <= 0 && !removed) {
+          type->scatterParticleAtGrain(part, rng);
+          part.grainTau = type->getNewGrainTau(rng);
+        }
+        if (removed) {
+          removedStep[p] = s;
+          contactOut.push_back(dev->getSurface().getOhmicContactIdx(dev->posToCoord(ps)));
+          removedAt.push_back(p);
+        }
+        drawsOut.push_back(RngCounter::between(atStart, rng));
+      } else {
+        drawsOut.push_back(0);
+      }
+      nevOut.push_back(nev);
+      posOut.insert(posOut.end(), ps.begin(), ps.end());
+      kOut.insert(kOut.end(), part.k.begin(), part.k.end());
+      eOut.push_back(part.energy);
+    }
+  }
+  std::string b = "device.traj.";
+  seci((b + "meta").c_str(), {(long long)np, (long long)nsteps, (long long)seed});
+  sec((b + "dt").c_str(), {dt});
+  sec((b + "init").c_str(), initState);
+  seci((b + "init_draws").c_str(), initDraws);
+  sec((b + "pos").c_str(), posOut);
+  sec((b + "k").c_str(), kOut);
+  sec((b + "E").c_str(), eOut);
+  seci((b + "nevents").c_str(), nevOut);
+  seci((b + "draws").c_str(), drawsOut);
+  std::vector<long long> rs(removedStep.begin(), removedStep.end());
+  seci((b + "removed_step").c_str(), rs);
+  seci((b + "removed_who").c_str(), removedAt);
+  seci((b + "removed_contact").c_str(), contactOut);
+}
+
 int main(int argc, char **argv) {
   if (argc < 2) {
     std::fprintf(stderr, "usage: ref_harness <outfile>\n");
@@ -622,8 +915,12 @@ int main(int argc, char **argv) {
       {"two_valley", 1500, 1.5, 300., 30., {1, 1, 1}},
       {"hot_polar", 2000, 2.0, 300., 10., {0, 0, 1}},
   };
-  dumpBathKATs();
   const uint64_t seed = 20261202ULL;
+  dumpBathKATs();
+  dumpDeviceKATs();
+  dumpDeviceTrajectories(seed + 5, 64, 150, 1e-15);
+  dumpDeviceRun<emcCICScheme<T, Device2>>("cic", seed + 11, 40, 1e-15, 1e-8, 1.8);
+  dumpDeviceRun<emcNGPScheme<T, Device2>>("ngp", seed + 13, 40, 1e-15, 1e-8, 1.8);
   for (const auto &m : models) {
     double mp = 4e-8;
     Device device{silicon(), {mp, mp, mp}, {1e-8, 1e-8, 1e-8}, m.T};

@@ -199,3 +199,52 @@ def batched_instances(n_instances=64, seed=20261202, n_min=10_000, n_max=2_000_0
         m.update(T=temperature, field_kvcm=field, field_dir=direction)
         out.append((m, count))
     return out
+
+
+# ---- 2-D device (resistor2D-style) -------------------------------------------
+SI_EPS_R = 11.8    # examples/SiliconFunctions.hpp:21
+SI_NI = 1.45e16    # examples/SiliconFunctions.hpp:23
+
+# one parabolic valley (m* 0.26) with acoustic + zero-order intervalley
+# phonons at 300 K, registered for every doping region of the device
+MODELS_DEVICE = {
+    "device_para": dict(
+        n_levels=1000, max_energy=1.5, T=300.0, field_kvcm=0.0, field_dir=(1, 0, 0),
+        valleys=[dict(kind=PARA_ISO, mass=(0.26, 0.26, 0.26), deg=1, alpha=0.0, bottom=0.0)],
+        mechs=[dict(kind=ACOUSTIC, valley=0, sigma=9.0),
+               dict(kind=ZERO_IV_AB, valley=0, final=0, defpot=5e10, hw=0.05, fmap=[[0]]),
+               dict(kind=ZERO_IV_EM, valley=0, final=0, defpot=5e10, hw=0.05, fmap=[[0]])]),
+}
+
+# the 21 x 7 node n+ n n+ resistor of the golden fixtures: doping regions
+# (min corner, max corner, doping [1/m^3]) in order, ohmic contacts
+# (boundary 0=XMIN 1=XMAX 2=YMIN 3=YMAX, voltage, min, max along the boundary)
+DEVICE_SMALL = dict(
+    max_pos=(20e-9, 6e-9), spacing=(1e-9, 1e-9), width=1e-6, T=300.0, eps_r=SI_EPS_R, ni=SI_NI,
+    doping=[((0, 0), (20e-9, 6e-9), 1e24), ((0, 0), (3e-9, 6e-9), 1e25),
+            ((17e-9, 0), (20e-9, 6e-9), 1e25)],
+    contacts=[(0, 0.2, 0.0, 6e-9), (1, 0.0, 0.0, 6e-9)])
+
+
+def resistor_device(nx=200, ny=50, h=1e-9, n_particles=2_000_000, contact_nodes=10, bias=0.5):
+    """The 2-D resistor workload: nx x ny nodes at spacing h, N_D = 1e23 /m^3
+    with 1e25 /m^3 ohmic contact regions of contact_nodes columns at both
+    ends, bias on the XMIN contact, and the device width that makes the
+    doping-weighted particle count about n_particles."""
+    mx, my = (nx - 1) * h, (ny - 1) * h
+    xc = (contact_nodes - 1) * h
+    dev = dict(max_pos=(mx, my), spacing=(h, h), width=1e-6, T=300.0, eps_r=SI_EPS_R, ni=SI_NI,
+               doping=[((0, 0), (mx, my), 1e23), ((0, 0), (xc, my), 1e25),
+                       ((mx - xc, 0), (mx, my), 1e25)],
+               contacts=[(0, bias, 0.0, my), (1, 0.0, 0.0, my)])
+    # sum of doping * (1/2 per boundary dimension) over the nodes, per m of width
+    w = 0.0
+    for iy in range(ny):
+        for ix in range(nx):
+            d = 1e25 if (ix < contact_nodes or ix >= nx - contact_nodes) else 1e23
+            for c, n in ((ix, nx), (iy, ny)):
+                if c == 0 or c == n - 1:
+                    d *= 0.5
+            w += d * h * h
+    dev["width"] = n_particles / w
+    return dev

@@ -1,0 +1,169 @@
+"""The oracle's 2-D device path against the reference's own device classes.
+
+Golden fixtures recorded by the harness through emcDevice, the NGP / CIC /
+NEC particle-mesh schemes, calcEField, emcSORSolver and
+emcBasicParticleHandler (a 21 x 7 node n+ n n+ resistor, ohmic contacts at
+both ends):
+* charge assignment, force interpolation and E-field: bit-exact;
+* SOR equilibrium and non-equilibrium solutions: bit-exact;
+* 64 particles x 150 steps of the device particle loop under a frozen field
+  with per-particle streams (drift with positions, boundary reflection,
+  removal at contacts): bit-exact;
+* full simulations in the emcSimulation step order with one shared stream
+  (equilibrium, initial particles, Poisson every step, drift/scatter, ohmic
+  contact equilibrium, charge assignment), CIC and NGP: bit-exact.
+The mathematics pins what the fixtures leave open: CIC weights of every
+particle sum to one, NGP mass lands on the nearest node, the SOR fixed point
+solves the discrete Poisson equation.
+"""
+import numpy as np
+import pytest
+
+from emc_models import DEVICE_SMALL, MODELS_DEVICE
+from oracle import oracle_py as O
+
+
+@pytest.fixture(scope="module")
+def dev():
+    return O.OracleDevice(DEVICE_SMALL)
+
+
+def _model():
+    return O.OracleModel(MODELS_DEVICE["device_para"])
+
+
+def test_device_geometry(dev):
+    assert dev.ext == (21, 7)
+    g = dev.grids()
+    d = g["doping"].reshape(7, 21)
+    assert (d[:, :4] == 1e25).all() and (d[:, 17:] == 1e25).all() and (d[:, 4:17] == 1e24).all()
+    c = g["contact"].reshape(7, 21)
+    assert (c[:, 0] == 0).all() and (c[:, -1] == 1).all() and (c[1:-1, 1:-1] == -1).all()
+    assert abs(dev.Vt - 1.38066e-23 / 1.60219e-19 * 300) < 1e-18
+    assert abs(dev.cell_volume - 1e-9 * 1e-9 * 1e-6) < 1e-40
+
+
+@pytest.mark.parametrize("scheme,name", [(O.PM_NGP, "ngp"), (O.PM_CIC, "cic"), (O.PM_NEC, "nec")])
+def test_pm_schemes_bit_exact(dev, golden, scheme, name):
+    pos = golden["device.kat.pos"]
+    g = dev.assign(scheme, pos)
+    assert (g == golden["device.kat.assign_" + name]).all()
+    f = dev.forces(scheme, golden["device.kat.ex"], golden["device.kat.ey"], pos)
+    assert (f.reshape(-1) == golden["device.kat.force_" + name]).all()
+    # every particle deposits exactly its carrier count
+    assert abs(g.sum() - pos.size // 2) < 1e-12
+
+
+def test_cic_weights_are_the_reference_ones(dev):
+    """The reference gives node (floor x, floor y) the weight wX * wY (the
+    mirror of the textbook CIC weight); a particle one quarter into a cell
+    puts 1/16 on the lower node and 9/16 on the upper one."""
+    pos = np.array([[0.25e-9 + 5e-9, 0.25e-9 + 2e-9]])
+    g = dev.assign(O.PM_CIC, pos).reshape(7, 21)
+    assert abs(g[2, 5] - 0.0625) < 1e-15 and abs(g[3, 6] - 0.5625) < 1e-15
+    assert abs(g[2, 6] - 0.1875) < 1e-15 and abs(g[3, 5] - 0.1875) < 1e-15
+    ngp = dev.assign(O.PM_NGP, np.array([[5.4e-9, 2.6e-9]])).reshape(7, 21)
+    assert ngp[3, 5] == 1.0 and ngp.sum() == 1.0
+
+
+def test_efield_bit_exact(dev, golden):
+    pot = golden["device.kat.pot"]
+    ex, ey = dev.efield(O.PM_CIC, pot)
+    assert (ex == golden["device.kat.efield_grid_x"]).all()
+    assert (ey == golden["device.kat.efield_grid_y"]).all()
+    ex, ey = dev.efield(O.PM_NEC, pot)
+    assert (ex == golden["device.kat.efield_edge_x"]).all()
+    assert (ey == golden["device.kat.efield_edge_y"]).all()
+    # central difference in the interior
+    ex, _ = dev.efield(O.PM_CIC, pot)
+    p = pot.reshape(7, 21)
+    want = (p[3, 4] - p[3, 6]) * dev.Vt / (2 * 1e-9)
+    assert abs(ex.reshape(7, 21)[3, 5] - want) <= 1e-15 * abs(want)
+
+
+def test_sor_bit_exact(dev, golden):
+    pot0 = dev.init_potential()
+    eq, sweeps = dev.sor(pot0, accuracy=1e-12, omega=1.8)
+    assert sweeps > 10
+    assert (eq == golden["device.kat.sor_eq"]).all()
+    neq, _ = dev.sor(eq, conc=golden["device.kat.sor_conc"], accuracy=1e-12, omega=1.8)
+    assert (neq == golden["device.kat.sor_neq"]).all()
+
+
+def test_sor_fixed_point_solves_poisson(dev, golden):
+    """At the converged point the discrete equation
+    sum_d hF_d (phi_prev + phi_next - 2 phi) + hprod (exp(-phi) - n + C) = 0
+    holds at every non-contact node (Neumann by mirrored neighbours)."""
+    phi = golden["device.kat.sor_neq"].reshape(7, 21)
+    n = golden["device.kat.sor_conc"].reshape(7, 21)
+    C = dev.grids()["doping"].reshape(7, 21) / DEVICE_SMALL["ni"]
+    h = 1e-9 / dev.debye
+    worst = 0.0
+    for iy in range(7):
+        for ix in range(1, 20):
+            xm = phi[iy, ix - 1]
+            xp = phi[iy, ix + 1]
+            ym = phi[iy - 1, ix] if iy > 0 else phi[iy + 1, ix]
+            yp = phi[iy + 1, ix] if iy < 6 else phi[iy - 1, ix]
+            r = (xm + xp + ym + yp - 4 * phi[iy, ix]) + h * h * (np.exp(-phi[iy, ix]) - n[iy, ix] + C[iy, ix])
+            worst = max(worst, abs(r) / (h * h * abs(C[iy, ix])))
+    assert worst < 1e-9
+
+
+def test_device_trajectories_bit_exact(dev, golden):
+    M = _model()
+    b = "device.traj."
+    np_, ns, seed = [int(x) for x in golden[b + "meta"]]
+    dt = golden[b + "dt"][0]
+    draws = O.mt64_particle_draws(seed, np_, 512)
+    cursor = np.zeros(np_, dtype=np.uintp)
+    coords = [((p * 5) % 21, p % 7) for p in range(np_)]
+    st = dev.init_replay(M, coords, draws, cursor)
+    init = golden[b + "init"].reshape(np_, 8)
+    assert (st[:, :8] == init).all()
+    assert (cursor == golden[b + "init_draws"]).all()
+    rem, con, tr = dev.run_replay(M, O.PM_CIC, golden[b + "ex"], golden[b + "ey"], st, draws, cursor,
+                                  ns, dt)
+    assert (rem == golden[b + "removed_step"]).all()
+    who = golden[b + "removed_who"]
+    assert (con[who] == golden[b + "removed_contact"]).all()
+    assert (tr["nev"] == golden[b + "nevents"]).all()
+    assert (tr["pos"] == golden[b + "pos"]).all()
+    snaps = golden[b + "snap_steps"]  # k and E are kept every fifth step
+    assert (tr["k"].reshape(ns, np_, 3)[snaps].reshape(-1) == golden[b + "k"]).all()
+    assert (tr["E"].reshape(ns, np_)[snaps].reshape(-1) == golden[b + "E"]).all()
+    total = golden[b + "init_draws"] + golden[b + "draws"].reshape(ns, np_).sum(axis=0)
+    assert (cursor == total).all()
+
+
+@pytest.mark.parametrize("pm,tag", [(O.PM_CIC, "cic"), (O.PM_NGP, "ngp")])
+def test_device_simulation_bit_exact(dev, golden, pm, tag):
+    b = "device." + tag + "."
+    seed, nsteps, dt, acc, omega = golden[b + "cfg"]
+    out = dev.run(_model(), pm, acc, omega, dt, int(nsteps), int(seed))
+    assert (out["eq_pot"] == golden[b + "eqpot"]).all()
+    assert out["n_init"][0] == golden[b + "ninit"][0]
+    assert (out["pot"].reshape(-1) == golden[b + "pot"]).all()
+    assert (out["n_part"] == golden[b + "npart"]).all()
+    assert (out["exits"].reshape(-1) == golden[b + "exits"]).all()
+    # handleOhmicContacts returns injected - deleted per contact
+    assert ((out["injs"] - out["dels"]).reshape(-1) == golden[b + "ohmic"]).all()
+    assert (out["nr_part"] == golden[b + "nrpart_last"]).all()
+    assert out["exits"].sum() > 20 and out["injs"].sum() > 20
+
+
+def test_contact_rule_keeps_the_first_in_array_order(dev):
+    """Excess particles at a contact node are the last ones in array order;
+    the deficit is injected (ceil of expected - kept)."""
+    exp_node = dev.cell_volume * 1e25 * 0.5  # XMIN node (0, 3): about 5
+    pos = np.array([[0.1e-9, 3.1e-9]] * 8 + [[10e-9, 3e-9]] + [[0.2e-9, 2.9e-9]] * 2)
+    keep, inj, exp = dev.contact_rule(pos)
+    node = 3 * 21 + 0
+    assert abs(exp[node] - exp_node) < 1e-12
+    # count while nrPart < expected: ceil(expected) kept (1e-24 * 1e25 * 0.5
+    # rounds to just above 5, so six are kept)
+    kept = int(np.ceil(exp[node]))
+    assert list(keep) == [1] * kept + [0] * (8 - kept) + [1] + [0] * 2
+    assert inj[node] == 0
+    keep, inj, exp = dev.contact_rule(pos[:2])
+    assert inj[node] == int(np.ceil(exp[node] - 2)) and list(keep) == [1, 1]
