@@ -284,6 +284,169 @@ emcgpu_status emcgpu_get_observables(const emcgpu_ctx *ctx, int instance,
 emcgpu_status emcgpu_obs_layout(const emcgpu_ctx *ctx, int *n_obs,
                                 int *max_valleys, int *max_deg);
 
+/* ---- 2-D device simulation ------------------------------------------------
+ * The device path of emcSimulation (emcSimulation.hpp:106-193) on instance 0
+ * of a context, electrons only (charge -q, one carrier per particle):
+ *   geometry        emcDevice (emcDevice.hpp:83-343), emcDopingProfile,
+ *                   emcSurface ohmic contacts (emcSurface.hpp:205-408)
+ *   charge / force  PMSchemes/emcNGPScheme.hpp:24-66, emcCICScheme.hpp:32-173,
+ *                   emcNECScheme.hpp:31-113 (the reference's weights, quirks
+ *                   included)
+ *   field           PMSchemes/emcEFieldCalculation.hpp:13-81
+ *   Poisson         the discrete equations of PoissonSolver/emcSORSolver.hpp:
+ *                   49-197, solved by Newton's method with an FP32 multigrid
+ *                   inner solve and FP64 residuals (iterative refinement)
+ *   particles       emcBasicParticleHandler (ParticleHandler/
+ *                   emcBasicParticleHandler.hpp:76-254): drift with positions,
+ *                   boundaries, ohmic-contact equilibrium, injection
+ * Positions are in m at the boundary; grids are x-fastest (iy * nx + ix),
+ * potentials normalised by V_T as in the reference. */
+#define EMCGPU_MAX_CONTACTS 8
+#define EMCGPU_MAX_REGIONS 16
+
+typedef enum { EMCGPU_PM_NGP = 0, EMCGPU_PM_CIC = 1, EMCGPU_PM_NEC = 2 } emcgpu_pm_scheme;
+typedef enum {
+  EMCGPU_XMIN = 0, EMCGPU_XMAX = 1, EMCGPU_YMIN = 2, EMCGPU_YMAX = 3
+} emcgpu_boundary;
+
+/* addConstantDopingRegion (emcDevice.hpp:162-167), applied in order */
+typedef struct {
+  double min_pos[2], max_pos[2]; /* [m] */
+  double doping;                 /* [1/m^3] */
+} emcgpu_doping_region;
+
+/* addOhmicContact (emcDevice.hpp:177-182) */
+typedef struct {
+  int boundary;     /* emcgpu_boundary */
+  double voltage;   /* [V] */
+  double min_pos, max_pos; /* extent along the boundary [m] */
+} emcgpu_contact;
+
+typedef struct {
+  double max_pos[2];   /* device extent [m] */
+  double spacing[2];   /* grid spacing [m] */
+  double width;        /* device width [m] (setDeviceWidth) */
+  double temperature;  /* [K] */
+  double eps_r;        /* relative permittivity */
+  double ni;           /* intrinsic carrier density [1/m^3] */
+  int n_doping;
+  const emcgpu_doping_region *doping;
+  int n_contacts;
+  const emcgpu_contact *contacts;
+  int pm_scheme;        /* emcgpu_pm_scheme */
+  double poisson_tol;   /* relative residual: max |F| / max h^2 (|C| + n + p) */
+  int poisson_max_iter; /* Newton / refinement iterations per solve */
+} emcgpu_device_desc;
+
+/* geometry (host) and device grids; call after emcgpu_set_model and
+ * emcgpu_build_tables of instance 0.  Needs a GPU context except for the
+ * geometry getters below. */
+emcgpu_status emcgpu_device_setup(emcgpu_ctx *ctx, const emcgpu_device_desc *desc);
+/* ext[2] node counts; scal[4] = V_T [V], intrinsic Debye length [m], cell
+ * volume [m^3], normalised cell volume (emcDevice.hpp:236-343) */
+emcgpu_status emcgpu_device_info(const emcgpu_ctx *ctx, int *ext, double *scal);
+/* per node: doping [1/m^3], doping region (-1: none), ohmic contact index
+ * (-1: none); each may be NULL */
+emcgpu_status emcgpu_device_grids(const emcgpu_ctx *ctx, double *doping, int *region,
+                                  int *contact);
+
+/* emcgpu_poisson -- calcEquilibriumPotential (equilibrium != 0, holes and
+ * electrons Boltzmann) or calcNonEquilibriumPotential (electrons = conc,
+ * normalised by n_i) in place on pot[nodes] (normalised).  reset_bc sets the
+ * ohmic nodes first (V/V_T + asinh(C/2), no V in equilibrium).  iters and
+ * resid (relative residual reached) may be NULL. */
+emcgpu_status emcgpu_poisson(emcgpu_ctx *ctx, double *pot, const double *conc,
+                             int equilibrium, int reset_bc, int *iters, double *resid);
+/* emcgpu_efield -- calcEField of the context's scheme: E [V/m] at nodes
+ * (NGP, CIC) or edge midpoints (NEC) from pot (normalised) */
+emcgpu_status emcgpu_efield(emcgpu_ctx *ctx, const double *pot, double *ex, double *ey);
+/* emcgpu_deposit -- assignToMesh of n positions pos[n][2] [m]: carriers per
+ * node, accumulated into grid[nodes] */
+emcgpu_status emcgpu_deposit(emcgpu_ctx *ctx, const double *pos, size_t n, double *grid);
+/* interpolateForce of the context's scheme: force[n][2] [N] on a particle
+ * of the given charge from ex, ey (V/m, as emcgpu_efield lays them out) */
+emcgpu_status emcgpu_interpolate(emcgpu_ctx *ctx, const double *ex, const double *ey,
+                                 const double *pos, size_t n, double charge, double *force);
+/* emcgpu_contacts -- the ohmic-contact equilibrium of handleOhmicContacts
+ * (emcBasicParticleHandler.hpp:158-192) on positions in array order:
+ * keep[n] = 0 for particles deleted as excess, inject[nodes] = particles to
+ * inject (generateInjectedParticles, emcAbstractParticleHandler.hpp:200-216) */
+emcgpu_status emcgpu_contacts(emcgpu_ctx *ctx, const double *pos, size_t n, int *keep,
+                              int *inject);
+
+/* ---- device particles ------------------------------------------------------
+ * set / get the ensemble in array order: pos[n][2] [m], k[n][3] [1/m], E [eV],
+ * tau [s], valley, sub (any output may be NULL; *n in: capacity of the
+ * outputs, out: particles held) */
+emcgpu_status emcgpu_device_set_particles(emcgpu_ctx *ctx, size_t n, const double *pos,
+                                          const double *k, const double *energy,
+                                          const double *tau, const int32_t *valley,
+                                          const int32_t *sub);
+emcgpu_status emcgpu_device_get_particles(const emcgpu_ctx *ctx, size_t *n, double *pos,
+                                          double *k, double *energy, double *tau,
+                                          int32_t *valley, int32_t *sub);
+/* frozen field for emcgpu_device_drift: ex, ey [V/m] as emcgpu_efield lays
+ * them out */
+emcgpu_status emcgpu_device_set_efield(emcgpu_ctx *ctx, const double *ex, const double *ey);
+
+typedef struct {
+  double dt;            /* [s] */
+  int n_steps;
+  int mode;             /* emcgpu_mode */
+  uint64_t seed;        /* production */
+  uint32_t step0;       /* production: first step index of the Philox counter */
+  /* replay: per-particle raw draws (stride draw_stride) and cursors, the
+   * grain timers [s] (in/out, may be NULL) */
+  const uint64_t *draws;
+  long long draw_stride;
+  uint64_t *cursor;
+  double *grain_tau;
+  /* outputs (may be NULL): exit_step[n] (-1: still inside), exit_contact[n];
+   * per step s and particle i: pos[s][i][2], k[s][i][3], E[s][i], events[s][i] */
+  int32_t *exit_step, *exit_contact;
+  double *tr_pos, *tr_k, *tr_E;
+  int32_t *tr_events;
+} emcgpu_device_drift_args;
+
+/* driftScatterParticles (emcBasicParticleHandler.hpp:93-139) for n_steps
+ * under the frozen field: a particle that leaves through an ohmic contact
+ * stops there and is reported; no contact equilibrium, no deposit */
+emcgpu_status emcgpu_device_drift(emcgpu_ctx *ctx, const emcgpu_device_drift_args *args);
+
+/* calcEquilibriumCharacteristics + generateInitialParticles + the first
+ * charge assignment (emcSimulation.hpp:139-160, emcElectron.hpp:48-104):
+ * equilibrium potential, E-field, particles per node (the reference's
+ * count with one Philox Bernoulli draw for the remainder), initial states
+ * from Philox; *n_particles gets the ensemble size */
+emcgpu_status emcgpu_device_init(emcgpu_ctx *ctx, uint64_t seed, size_t *n_particles);
+
+typedef struct {
+  double dt;          /* [s] */
+  int n_steps;
+  uint64_t seed;
+  int use_graph;      /* capture the per-step kernel sequence in a HIP graph */
+  /* per-step records (may be NULL): [n_steps][n_contacts] particles that
+   * left through / were deleted at / were injected at each contact, the
+   * ensemble size after the step, Newton iterations and relative residual
+   * of the step's Poisson solve, and |sum(nodes) - N| / N of the charge
+   * assignment the step's Poisson solve used (FP64 check sum) */
+  int32_t *exits, *deleted, *injected;
+  int64_t *n_particles;
+  int32_t *poisson_iters;
+  double *poisson_resid, *checksum;
+} emcgpu_device_step_args;
+
+/* performEMCStep repeated n_steps times (emcSimulation.hpp:164-193):
+ * Poisson, E-field, driftScatterParticles, handleOhmicContacts, injection,
+ * charge assignment; the step counter carries over between calls */
+emcgpu_status emcgpu_device_step(emcgpu_ctx *ctx, const emcgpu_device_step_args *args);
+
+/* current potential (normalised), E-field [V/m], electron concentration
+ * (normalised) and carriers per node of the last assignment (each may be
+ * NULL) */
+emcgpu_status emcgpu_device_fields(const emcgpu_ctx *ctx, double *pot, double *ex,
+                                   double *ey, double *conc, double *nr_part);
+
 /* accumulated kernel time of timed emcgpu_step calls (HIP events on the
  * context's stream): total ms and number of particle-step launches */
 emcgpu_status emcgpu_get_timing(const emcgpu_ctx *ctx, double *ms_total,

@@ -1,0 +1,256 @@
+"""2-D device path on the GPU against the oracle and the reference's fixtures.
+
+* charge assignment, force interpolation (NGP / CIC / NEC) and the E-field
+  on the reference's known-answer positions and potentials: bit-exact (FP64);
+* Poisson: the GPU's Newton / FP32-multigrid solution of the SOR equations
+  against the oracle's SOR fixed point (both converged far below the
+  tolerance) and against the discrete equation itself;
+* the device particle loop under the reference's frozen field with the
+  reference's recorded per-particle draws (replay): exit step and contact
+  and event counts exact, positions / k / E within 1e-12 relative;
+* the ohmic-contact rule on positions in array order: kept flags and
+  injection counts exact;
+* a self-consistent run (Poisson every step, contacts, injection, charge
+  assignment): conservation of the ensemble counters, the FP64 check sum,
+  Poisson convergence, graph launch equal to eager launch.
+"""
+import numpy as np
+import pytest
+
+from emc_models import DEVICE_SMALL, MODELS_DEVICE, resistor_device
+from oracle import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-12
+
+
+def _emc():
+    from viennaemc_amd import emcgpu
+    return emcgpu
+
+
+def _ctx(dev=DEVICE_SMALL, pm=None, precision=0, tol=1e-13, max_iter=60):
+    emcgpu = _emc()
+    ctx = emcgpu.Context(precision=precision)
+    ctx.setup(0, MODELS_DEVICE["device_para"])
+    ctx.device_setup(dev, pm=emcgpu.PM_CIC if pm is None else pm, poisson_tol=tol,
+                     poisson_max_iter=max_iter)
+    return ctx
+
+
+@pytest.fixture(scope="module")
+def odev():
+    return O.OracleDevice(DEVICE_SMALL)
+
+
+@pytest.mark.parametrize("name", ["ngp", "cic", "nec"])
+def test_pm_known_answers_bit_exact(golden, name):
+    emcgpu = _emc()
+    pm = {"ngp": emcgpu.PM_NGP, "cic": emcgpu.PM_CIC, "nec": emcgpu.PM_NEC}[name]
+    print("PM known answers", name, flush=True)
+    ctx = _ctx(pm=pm)
+    pos = golden["device.kat.pos"].reshape(-1, 2)
+    # every particle's weights are the reference's bit for bit ...
+    od = O.OracleDevice(DEVICE_SMALL)
+    opm = {"ngp": O.PM_NGP, "cic": O.PM_CIC, "nec": O.PM_NEC}[name]
+    for p in pos[:40]:
+        assert (ctx.deposit(p[None, :]) == od.assign(opm, p[None, :])).all()
+    # ... and the grid sums them in atomic (unordered) order
+    g = ctx.deposit(pos)
+    ref = golden["device.kat.assign_" + name]
+    assert np.allclose(g, ref, rtol=1e-15, atol=1e-15)
+    assert abs(g.sum() - pos.shape[0]) < 1e-12
+    f = ctx.interpolate(golden["device.kat.ex"], golden["device.kat.ey"], pos)
+    ref = golden["device.kat.force_" + name].reshape(-1, 3)
+    assert (f == ref[:, :2]).all()
+    ctx.close()
+
+
+def test_efield_bit_exact(golden):
+    emcgpu = _emc()
+    print("E-field known answers", flush=True)
+    pot = golden["device.kat.pot"]
+    for pm, tag in ((emcgpu.PM_CIC, "grid"), (emcgpu.PM_NEC, "edge")):
+        ctx = _ctx(pm=pm)
+        ex, ey = ctx.efield(pot)
+        assert (ex == golden["device.kat.efield_%s_x" % tag]).all()
+        assert (ey == golden["device.kat.efield_%s_y" % tag]).all()
+        ctx.close()
+
+
+def _residual(dev, phi, n, C, eq):
+    """max |F| of the discrete SOR equations at the non-ohmic nodes"""
+    nx, ny = dev.ext
+    phi = phi.reshape(ny, nx)
+    h = 1e-9 / dev.debye
+    ohm = dev.grids()["contact"].reshape(ny, nx) >= 0
+    worst = 0.0
+    for iy in range(ny):
+        for ix in range(nx):
+            if ohm[iy, ix]:
+                continue
+            xm = phi[iy, ix - 1] if ix > 0 else phi[iy, ix + 1]
+            xp = phi[iy, ix + 1] if ix < nx - 1 else phi[iy, ix - 1]
+            ym = phi[iy - 1, ix] if iy > 0 else phi[iy + 1, ix]
+            yp = phi[iy + 1, ix] if iy < ny - 1 else phi[iy - 1, ix]
+            nn = np.exp(phi[iy, ix]) if eq else n.reshape(ny, nx)[iy, ix]
+            r = (xm + xp + ym + yp - 4 * phi[iy, ix]) + h * h * (
+                np.exp(-phi[iy, ix]) - nn + C.reshape(ny, nx)[iy, ix])
+            worst = max(worst, abs(r))
+    return worst
+
+
+def test_poisson_matches_sor_fixed_point(odev, golden):
+    print("Poisson vs SOR", flush=True)
+    ctx = _ctx(tol=1e-14)
+    pot0 = odev.init_potential()
+    ref_eq, _ = odev.sor(pot0, accuracy=1e-14, omega=1.8)
+    eq, it, res = ctx.poisson(pot0, equilibrium=True, reset_bc=True)
+    print("equilibrium: %d Newton iterations, residual %.2e, max |d| %.2e" % (
+        it, res, np.abs(eq - ref_eq).max()), flush=True)
+    assert res < 1e-13
+    assert np.abs(eq - ref_eq).max() < 1e-9
+    conc = golden["device.kat.sor_conc"]
+    ref_neq, _ = odev.sor(ref_eq, conc=conc, accuracy=1e-14, omega=1.8)
+    neq, it2, res2 = ctx.poisson(eq, conc=conc, equilibrium=False, reset_bc=True)
+    print("non-equilibrium: %d iterations, residual %.2e, max |d| %.2e" % (
+        it2, res2, np.abs(neq - ref_neq).max()), flush=True)
+    assert res2 < 1e-13
+    assert np.abs(neq - ref_neq).max() < 1e-9
+    C = odev.grids()["doping"] / DEVICE_SMALL["ni"]
+    assert _residual(odev, neq, conc, C, eq=False) < 1e-12
+    ctx.close()
+
+
+def test_poisson_resistor_grid_converges():
+    """200 x 50 nodes: the multigrid Newton solve reaches 1e-10 in a few
+    iterations from the charge-neutral guess and from a warm start."""
+    print("Poisson 200x50", flush=True)
+    dev = resistor_device()
+    ctx = _ctx(dev=dev, tol=1e-10)
+    od = O.OracleDevice(dev)
+    pot0 = od.init_potential()
+    eq, it, res = ctx.poisson(pot0, equilibrium=True, reset_bc=True)
+    print("equilibrium: %d iterations, residual %.2e" % (it, res), flush=True)
+    assert res <= 1e-10 and it < 60
+    n = np.exp(eq)  # electrons in equilibrium: a bias then moves the potential
+    neq, it2, res2 = ctx.poisson(eq, conc=n, equilibrium=False, reset_bc=True)
+    print("biased: %d iterations, residual %.2e" % (it2, res2), flush=True)
+    assert res2 <= 1e-10 and it2 < 30
+    ref, _ = od.sor(eq, conc=n, accuracy=1e-11, omega=1.9)
+    print("max |phi - phi_SOR| = %.2e" % np.abs(neq - ref).max(), flush=True)
+    assert np.abs(neq - ref).max() < 1e-5
+    again, it3, res3 = ctx.poisson(neq, conc=n, equilibrium=False, reset_bc=False)
+    assert it3 <= 1 and res3 <= 1e-10
+    ctx.close()
+
+
+def test_device_trajectories_replay(golden):
+    emcgpu = _emc()
+    b = "device.traj."
+    np_, ns, seed = [int(x) for x in golden[b + "meta"]]
+    dt = golden[b + "dt"][0]
+    print("device replay:", np_, "particles", ns, "steps", flush=True)
+    init = golden[b + "init"].reshape(np_, 8)  # x, y, k, E, tau, grainTau
+    ctx = _ctx()
+    zero = np.zeros(np_, dtype=np.int32)
+    ctx.device_set_particles(init[:, 0:2], init[:, 2:5], init[:, 5], init[:, 6], zero, zero)
+    ctx.device_set_efield(golden[b + "ex"], golden[b + "ey"])
+    draws = O.mt64_particle_draws(seed, np_, 512)
+    cursor = golden[b + "init_draws"].astype(np.uint64).copy()
+    grain = init[:, 7].copy()
+    tr = ctx.device_drift(dt, ns, np_, mode=emcgpu.REPLAY, draws=draws, cursor=cursor,
+                          grain_tau=grain, trace=True)
+    rem = tr["exit_step"]
+    assert (rem == golden[b + "removed_step"]).all()
+    who = golden[b + "removed_who"]
+    assert (tr["exit_contact"][who] == golden[b + "removed_contact"]).all()
+    assert (tr["events"] == golden[b + "nevents"].reshape(ns, np_)).all()
+    pos_ref = golden[b + "pos"].reshape(ns, np_, 2)
+    rel = np.abs(tr["pos"] - pos_ref).max() / np.abs(pos_ref).max()
+    print("positions: max rel %.2e" % rel, flush=True)
+    assert rel < REL_TOL
+    snaps = golden[b + "snap_steps"]
+    k = tr["k"][snaps].reshape(-1, 3)
+    kr = golden[b + "k"].reshape(-1, 3)
+    assert (np.linalg.norm(k - kr, axis=1) / np.linalg.norm(kr, axis=1)).max() < REL_TOL
+    E = tr["E"][snaps].reshape(-1)
+    assert (np.abs(E - golden[b + "E"]) / golden[b + "E"]).max() < REL_TOL
+    total = golden[b + "init_draws"] + golden[b + "draws"].reshape(ns, np_).sum(axis=0)
+    assert (cursor == total).all()
+    ctx.close()
+
+
+def test_contact_rule_exact(odev):
+    print("contact rule", flush=True)
+    rng = np.random.default_rng(7)
+    # 400 particles, half of them inside the ohmic columns (x < 0.5 nm or
+    # x > 19.5 nm), in random array order
+    n = 400
+    x = np.where(rng.random(n) < 0.5, rng.random(n) * 0.5e-9,
+                 rng.random(n) * 20e-9)
+    x = np.where(rng.random(n) < 0.3, 20e-9 - rng.random(n) * 0.5e-9, x)
+    pos = np.stack([x, rng.random(n) * 6e-9], axis=1)
+    ctx = _ctx()
+    keep, inj = ctx.contacts(pos)
+    okeep, oinj, _ = odev.contact_rule(pos)
+    assert (keep == okeep).all()
+    assert (inj == oinj).all()
+    assert (keep == 0).sum() > 10
+    # few particles: everything kept, deficits injected
+    keep, inj = ctx.contacts(pos[:20])
+    okeep, oinj, _ = odev.contact_rule(pos[:20])
+    assert (keep == okeep).all() and (inj == oinj).all()
+    assert keep.all() and inj.sum() > 0
+    ctx.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_self_consistent_run(precision):
+    """Resistor at 0.5 V: counters conserve the ensemble, the charge check
+    sum stays at rounding level, every step's Poisson solve converges."""
+    dev = resistor_device(n_particles=200_000)
+    print("self-consistent run, precision", precision, flush=True)
+    ctx = _ctx(dev=dev, precision=precision, tol=1e-10)
+    n0 = ctx.device_init(1234)
+    assert abs(n0 - 200_000) < 2000
+    out = ctx.device_step(0.1e-15, 200, seed=1234)
+    n = out["n_particles"]
+    d = (out["injected"] - out["exits"] - out["deleted"]).sum(axis=1)
+    assert n[0] == n0 + d[0] and (np.diff(n) == d[1:]).all()
+    assert (out["poisson_resid"] <= 1e-10).all()
+    cs = out["checksum"].max()
+    print("n %d -> %d, exits %d, injected %d, deleted %d, Newton its %.2f, check sum %.2e" % (
+        n0, n[-1], out["exits"].sum(), out["injected"].sum(), out["deleted"].sum(),
+        out["poisson_iters"].mean(), cs), flush=True)
+    assert cs < (1e-12 if precision == 0 else 1e-5)
+    assert out["exits"].sum() > 0 and out["injected"].sum() > 0
+    p = ctx.device_get_particles()
+    assert p["E"].size == n[-1]
+    assert (p["pos"][:, 0] >= 0).all() and (p["pos"][:, 0] <= dev["max_pos"][0]).all()
+    f = ctx.device_fields()
+    assert abs(f["nr_part"].sum() - n[-1]) < (1e-6 if precision == 0 else 1.0)
+    ctx.close()
+
+
+def test_graph_launch_equals_eager():
+    dev = resistor_device(n_particles=50_000)
+    print("device graph vs eager", flush=True)
+    res = {}
+    for graph in (False, True):
+        ctx = _ctx(dev=dev, precision=0, tol=1e-10)
+        ctx.device_init(99)
+        a = ctx.device_step(0.1e-15, 70, seed=99, graph=graph)
+        b = ctx.device_step(0.1e-15, 64, seed=99, graph=graph)
+        p = ctx.device_get_particles()
+        res[graph] = (a, b, p)
+        ctx.close()
+    for x, y in zip(res[False][:2], res[True][:2]):
+        for key in ("exits", "deleted", "injected", "n_particles"):
+            assert (x[key] == y[key]).all(), key
+    pa, pb = res[False][2], res[True][2]
+    assert pa["E"].size == pb["E"].size
+    # the deposit sums in a fixed order except for the few FP64 atomic
+    # corrections; trajectories agree to rounding
+    assert np.abs(pa["pos"] - pb["pos"]).max() < 1e-15

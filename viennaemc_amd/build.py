@@ -15,8 +15,9 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libemcgpu.so")
 
-SOURCES = ["emcgpu_kernels.hip", "emcgpu_capi.cpp"]
-HEADERS = ["emcgpu_device.h", "../include/emcgpu/emcgpuPhysics.hpp", "../../include/emcgpu.h"]
+SOURCES = ["emcgpu_kernels.hip", "emcgpu_capi.cpp", "emcgpu_device_kernels.hip",
+           "emcgpu_device_capi.cpp"]
+HEADERS = ["emcgpu_device.h", "emcgpu_particle.h", "emcgpu_host.h", "../include/emcgpu/emcgpuPhysics.hpp", "../../include/emcgpu.h"]
 
 ARCH = os.environ.get("EMCGPU_ARCH", "gfx950")
 

@@ -38,151 +38,11 @@ hipError_t launch_reduce(const double *partial, int n_steps, int n_tiles, int n_
 
 using namespace emcgpu;
 
-namespace {
+#include "emcgpu_host.h"
 
-thread_local std::string g_create_error;
-
-// host copy of an LO phonon bath (emcPhononBath<double>, emcPhononBath.hpp:56-192)
-struct BathHost {
-  emcgpu_bath_desc d{};  // tau_profile pointer not kept
-  std::vector<double> profile;
-  std::vector<double> nq;
-  double n0 = 0, n_ac = 0, n_ac_eq = 0, n_to = 0, n_to_eq = 0, w_r = 0;
-  bool has_ac = false, has_ridley = false;
-};
-
-struct Instance {
-  std::vector<emcgpu_valley_desc> valleys;
-  std::vector<emcgpu_mech_desc> mechs;
-  std::vector<phys::ValleyMass> vm;
-  // normalised tables per valley: [table][level] (FP64 master) and tau0 [s]
-  int n_levels = 0;
-  double max_energy = 0, dE = 0;
-  std::vector<std::vector<double>> cum;     // per valley
-  std::vector<std::vector<int>> table_mech;  // per valley: mechanism ids
-  std::vector<double> tau0;
-  bool has_tables = false;
-  double temperature = 300;
-  double dir[3] = {1, 0, 0};
-  double force[3] = {0, 0, 0};  // N
-  bool has_field = false;
-  size_t count = 0;
-  long long start = 0;
-  // phonon baths and the bath-dependent table data: per mechanism its FP64
-  // rates per level, or for a hot-phonon mechanism kI[level] then ln[level]
-  std::vector<BathHost> baths;
-  std::vector<std::vector<double>> rate_data;
-  std::vector<double> hot_C;
-  bool has_hot = false;
-  long long bath_index0 = 0;    // global index of the first bath
-  std::vector<long long> bath_off;  // global first bin of each bath
-  long long gbin_off = 0;
-  int total_bins = 0;
-  bool dev_bath = false;        // the device holds the current bath state
-  std::vector<long long> cum_off;  // per valley: element offset of the device tables
-};
-
-}  // namespace
-
-struct emcgpu_ctx {
-  int device = 0;
-  int precision = EMCGPU_FP64;
-  int n_inst = 1;
-  hipStream_t stream = nullptr;
-  std::string err;
-  std::vector<Instance> inst;
-  // arena
-  long long capacity = 0;  // padded particles (multiple of kUnit)
-  void *kx = nullptr, *ky = nullptr, *kz = nullptr, *E = nullptr, *tau = nullptr;
-  uint32_t *vs = nullptr;
-  int *d_unit_inst = nullptr;
-  long long *d_inst_start = nullptr, *d_inst_count = nullptr;
-  // parameters and tables
-  void *d_params = nullptr;
-  void *d_cum = nullptr;
-  size_t cum_bytes = 0;
-  bool params_dirty = true;
-  double params_dt = 0;
-  // observables
-  int max_v = 1, max_deg = 1, n_obs = 0;
-  double *d_partial = nullptr;
-  size_t partial_bytes = 0;
-  double *d_chunks = nullptr;  // (sum, compensation) pairs of the first reduction pass
-  size_t chunk_bytes = 0;
-  double *d_obs = nullptr;
-  size_t obs_bytes = 0;
-  int obs_steps = 0;
-  // timing
-  double ms_total = 0;
-  long long launches = 0;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipGraphExec_t gexec = nullptr;  // last captured step sequence
-  // phonon baths of all instances (layout built by upload_baths)
-  int n_baths = 0, lds_bins = 0, n_jobs = 0;
-  long long tally_bins = 0;  // bins of all baths (one tally copy)
-  bool bath_layout_dirty = true;
-  BathDev *d_baths = nullptr;
-  double *d_nq = nullptr, *d_tau_prof = nullptr;
-  unsigned long long *d_tally = nullptr;
-  BathVar *d_bath_var = nullptr;
-  double *d_bath_hist = nullptr;
-  size_t bath_hist_bytes = 0;
-  int bath_hist_steps = 0;
-  RebuildJob *d_jobs = nullptr;
-  BathInstDev *d_groups = nullptr;
-  int n_groups = 0;
-  double *d_rate_arena = nullptr;
-  double *d_cum64 = nullptr, *d_tau0_64 = nullptr;
-  size_t cum64_bytes = 0;
-};
+using namespace emcgpu::host;
 
 namespace {
-
-emcgpu_status fail(emcgpu_ctx *c, emcgpu_status st, const char *fmt, ...) {
-  char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof buf, fmt, ap);
-  va_end(ap);
-  if (c)
-    c->err = buf;
-  else
-    g_create_error = buf;
-  return st;
-}
-
-#define HIPCHK(c, x)                                                                     \
-  do {                                                                                   \
-    hipError_t e_ = (x);                                                                 \
-    if (e_ != hipSuccess)                                                                \
-      return fail((c), EMCGPU_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_));      \
-  } while (0)
-
-size_t real_size(const emcgpu_ctx *c) {
-  return c->precision == EMCGPU_FP32 ? sizeof(float) : sizeof(double);
-}
-
-void dfree(void *&p) {
-  if (p) {
-    (void)hipFree(p);
-    p = nullptr;
-  }
-}
-template <typename T>
-void dfreet(T *&p) {
-  if (p) {
-    (void)hipFree(p);
-    p = nullptr;
-  }
-}
-
-emcgpu_status check_instance(emcgpu_ctx *c, int instance) {
-  if (!c)
-    return EMCGPU_ERR_ARG;
-  if (instance < 0 || instance >= c->n_inst)
-    return fail(c, EMCGPU_ERR_ARG, "instance %d out of range [0, %d)", instance, c->n_inst);
-  return EMCGPU_OK;
-}
 
 // ---- scaled-unit lowering of an instance -------------------------------
 // k [1/nm] = k [1/m] * 1e-9, t [fs] = t [s] * 1e15
@@ -250,6 +110,9 @@ void lower_instance(const emcgpu_ctx *c, const Instance &in, double dt_s, InstDe
         V.accel[s][i] = (R)(a / phys::hbar * kToNm / kToFs);
         V.wdir[s][i] = (R)w;
       }
+      const double hv[6] = {M[0][0], M[0][1], M[0][2], M[1][1], M[1][2], M[2][2]};
+      for (int i = 0; i < 6; i++)
+        V.hv[s][i] = (R)hv[i];
     }
     for (int j = 0; j < P.n_tables[v]; j++) {
       const emcgpu_mech_desc &md = in.mechs[in.table_mech[v][j]];
@@ -1043,12 +906,15 @@ void emcgpu_destroy(emcgpu_ctx *c) {
   if (!c)
     return;
   if (c->device < 0) {
+    device_sim_destroy(c->dsim);
     delete c;
     return;
   }
   (void)hipSetDevice(c->device);
   if (c->stream)
     (void)hipStreamSynchronize(c->stream);
+  device_sim_destroy(c->dsim);
+  c->dsim = nullptr;
   dfree(c->kx);
   dfree(c->ky);
   dfree(c->kz);
@@ -1758,3 +1624,16 @@ emcgpu_status emcgpu_synchronize(emcgpu_ctx *c) {
 }
 
 }  // extern "C"
+
+namespace emcgpu {
+namespace host {
+
+emcgpu_status upload_model(emcgpu_ctx *c, double dt_s) {
+  return c->precision == EMCGPU_FP32 ? upload_params<float>(c, dt_s)
+                                     : upload_params<double>(c, dt_s);
+}
+const void *device_params(const emcgpu_ctx *c) { return c->d_params; }
+const void *device_tables(const emcgpu_ctx *c) { return c->d_cum; }
+
+}  // namespace host
+}  // namespace emcgpu

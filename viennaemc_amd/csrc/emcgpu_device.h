@@ -48,6 +48,9 @@ struct ValleyDev {
   int pad;
   R accel[kMaxDeg][3];  // dk/dt in device frame [1/nm per fs]
   R wdir[kMaxDeg][3];   // v.dir = cv / npf * dot(k, wdir[sub])
+  // Herring-Vogt metric of each subvalley in the device frame,
+  // M = R^T diag(vogt) R, symmetric: M00 M01 M02 M11 M12 M22
+  R hv[kMaxDeg][6];
 };
 
 template <typename R>
@@ -169,6 +172,169 @@ struct StepArgs {
   unsigned long long *tally;
   long long tally_bins;
   int lds_bins;  // bins held in LDS per block (max over instances)
+};
+
+// ===========================================================================
+// 2-D device simulation
+// ===========================================================================
+constexpr int kDevThreads = 1024;      // field solver: one workgroup
+constexpr int kDevStepThreads = 512;   // particle step: 8 waves per block
+constexpr int kDevStepWaves = kDevStepThreads / 64;
+constexpr uint32_t kHole = 0xFFFFFFFFu;  // valley/subvalley word of a deleted particle
+constexpr int kMaxLevels = 12;
+constexpr int kMaxContacts = EMCGPU_MAX_CONTACTS;
+constexpr int kMaxSlots = 1024;        // ohmic nodes
+// dynamic LDS of one workgroup (160 KB per CU on gfx950, less the kernels'
+// static arrays)
+constexpr int kDevMaxLds = 159 * 1024;
+// Philox domains of the device streams (the instance word of the counter)
+constexpr uint32_t kDomMove = 0x40000000u;
+constexpr uint32_t kDomInit = 0x40000001u;
+constexpr uint32_t kDomCount = 0x40000002u;
+
+// committed state of the device ensemble (device memory)
+struct DevState {
+  long long n_total;    // array length: particles + holes
+  long long n_alive;
+  long long next_pid;   // creation index of the next particle
+  long long step;       // step counter (Philox step word)
+  long long hist_base;  // step of history row 0
+  long long inj_total;  // injections of the current step (contact scan)
+  long long capacity;
+  int overflow;
+  unsigned int ticket;  // contact-scan completion counter
+};
+
+// per-step records [row][...] (row = step - hist_base)
+struct DevHist {
+  int *exits, *dels, *injs;  // [row][n_contacts]
+  long long *n_part;         // [row]
+  int *iters;                // [row]
+  double *resid, *checksum;  // [row]
+  int rows;
+};
+
+// particle arrays of the device ensemble (SoA, arena of capacity entries)
+template <typename R>
+struct DevParticles {
+  R *x, *y;  // [nm]
+  R *kx, *ky, *kz, *E, *tau;
+  uint32_t *vs;   // valley << 16 | subvalley, kHole for a deleted particle
+  uint32_t *pid;  // creation index (Philox stream of the particle)
+};
+
+// geometry for the particle kernels (positions in nm)
+template <typename R>
+struct DevGeom {
+  int nx, ny, nn, pm;
+  R hx, hy;        // [nm]
+  R maxx, maxy;    // [nm]
+  const int *cnode;  // ohmic contact index per node, -1 elsewhere
+  const int *cslot;  // contact slot per node, -1 elsewhere
+  const R *acc;      // [node][2] interpolated force / hbar [1/nm per fs]
+};
+
+// one multigrid level: node counts, offset into the level-stacked arrays,
+// first entries of its x / y tables
+struct MGLevel {
+  int nx, ny, off, tx, ty;
+};
+// per level and dimension, per index: FV coefficients and transfer data
+struct DimTab {
+  float inv_l, inv_r, w;  // 1/left spacing, 1/right spacing, dual width (fine units)
+  float a;                // prolongation: weight of coarse node cI (1: coincident)
+  int cI;                 // prolongation: left coarse index
+  int f;                  // restriction: fine index of this (coarse) node
+  float wl, wr;           // restriction: weights of fine nodes f-1, f+1 (mirror folded)
+};
+
+// Poisson + E-field on one workgroup
+template <typename R>
+struct DevFieldArgs {
+  int nx, ny, nn, pm;
+  double hfx, hfy, hp;  // h-factors and product of the normalised spacings
+  double vt, hxm, hym;  // V_T [V], spacing [m]
+  double tol;
+  int max_iter, equilibrium, reset_bc, do_poisson, do_efield;
+  double *pot;                 // normalised potential
+  const double *dopn;          // doping / n_i
+  const double *conc;          // electron concentration / n_i (non-equilibrium)
+  const double *bc_eq, *bc_neq;  // ohmic node values (eq / with the applied voltage)
+  const uint8_t *dmask;        // ohmic node (Dirichlet) of the fine grid
+  const signed char *bcx, *bcy;  // E-field boundary: -2 off the boundary, -1 no contact
+  double *ex, *ey;             // [V/m]
+  R *acc;                      // [node][2] scaled force / hbar
+  double acc_scale;            // charge * 1e-24 / hbar
+  float *sig;                  // Newton diagonal shift of every level (level-stacked)
+  int n_levels;
+  MGLevel lev[kMaxLevels];
+  const DimTab *tab;
+  const uint8_t *lmask;        // Dirichlet mask of all levels (level-stacked)
+  int nu_pre, nu_post, nu_coarse;
+  // records
+  int *iters_out;
+  double *resid_out;
+  const double *nrpart;        // check sum input (may be null)
+  DevState *state;
+  DevHist hist;
+  int record;                  // write iters / resid / check sum to the history
+};
+
+// particle step of a device ensemble
+template <typename R>
+struct DevStepArgs {
+  DevParticles<R> p;
+  DevGeom<R> g;
+  const InstDev<R> *inst;
+  const R *cum;
+  DevState *state;
+  DevHist hist;
+  uint32_t key0, key1;
+  int n_contacts, n_slots;
+  R *dep;                 // [block][node] deposits of the step kernel
+  uint32_t *seg_cnt;      // [slot][segment] contact particles per segment
+  uint32_t *seg_nst;      // [segment] staged entries
+  uint2 *stage;           // (particle, slot << 22 | rank in segment) at segment start
+  int lds_dep;            // deposit grid in LDS (else straight to dep row 0 with atomics)
+  int move;              // 0: classify positions only (contact rule of given positions)
+  // frozen-field drift (emcgpu_device_drift)
+  long long n;            // particles (drift)
+  uint32_t step;          // first Philox step (drift)
+  int n_steps;
+  int *exit_step, *exit_contact;
+  double *tr_pos, *tr_k, *tr_E;
+  int *tr_events;
+  const uint64_t *draws;
+  long long draw_stride;
+  uint64_t *cursor;
+  double *grain_tau;
+  double dt_s;
+  int *draw_overflow;
+};
+
+// contact scan, resolve (deletions + injections), reduce
+template <typename R>
+struct DevContactArgs {
+  DevParticles<R> p;
+  DevGeom<R> g;
+  const InstDev<R> *inst;
+  DevState *state;
+  DevHist hist;
+  uint32_t key0, key1;
+  int n_contacts, n_slots, n_segments, n_blocks_dep;
+  const uint32_t *seg_cnt;
+  uint32_t *seg_off;       // [slot][segment] exclusive offsets
+  const uint32_t *seg_nst;
+  const uint2 *stage;
+  const int *slot_node;    // grid node of each slot
+  const int *slot_contact;
+  const double *slot_exp;  // expected carriers at the node
+  int *slot_kept, *slot_inj;
+  double *corr;            // [node] deposit corrections (FP64 atomics)
+  R *dep;
+  double *nrpart, *conc;
+  const double *cfac;      // conc = nrpart * cfac
+  R vt;                    // thermal voltage [V]
 };
 
 }  // namespace emcgpu

@@ -1,0 +1,1269 @@
+// emcgpu_device_kernels.hip -- gfx950 kernels of the 2-D device simulation.
+//
+// One step of emcSimulation::performEMCStep (emcSimulation.hpp:164-193) is
+// five launches on the context's stream:
+//   k_dev_field         one workgroup: Poisson (Newton, FP64 residuals,
+//                       FP32 red-black multigrid in LDS) and the E-field
+//   k_dev_step          persistent blocks: drift/scatter of every particle
+//                       (emcBasicParticleHandler.hpp:93-139), charge
+//                       assignment into a block-private LDS grid, ranks of
+//                       the particles at ohmic nodes in array order
+//   k_dev_contact_scan  one block per ohmic node: array-order offsets of the
+//                       segments, particles kept / deleted / to inject
+//                       (handleOhmicContacts, emcBasicParticleHandler.hpp:
+//                       158-192)
+//   k_dev_resolve       deletions (holes) and injections (appended), with
+//                       their charge corrections
+//   k_dev_reduce        FP64 sum of the block grids -> carriers per node,
+//                       concentration, commit of the ensemble counters
+// Array order is the reference's: deleted particles leave holes (kHole),
+// injected ones are appended, and a stable compaction between step chunks
+// removes the holes.  Positions are in nm on the device.
+#include <hip/hip_runtime.h>
+
+#include "emcgpu_particle.h"
+
+namespace emcgpu {
+
+// ---------------------------------------------------------------------------
+// geometry
+// ---------------------------------------------------------------------------
+// posToCoord (emcDevice.hpp:290-297): round(pos / h) per dimension
+template <typename R>
+__device__ __forceinline__ int pos_coord(R p, R h) {
+  return (int)Math<R>::round(Math<R>::div(p, h));
+}
+
+template <typename R>
+__device__ __forceinline__ void acc_at(const DevGeom<R> &g, int ix, int iy, R &ax, R &ay) {
+  if ((unsigned)ix < (unsigned)g.nx && (unsigned)iy < (unsigned)g.ny) {
+    const R *q = g.acc + 2 * ((long long)iy * g.nx + ix);
+    ax = q[0];
+    ay = q[1];
+  } else {
+    ax = R(0);
+    ay = R(0);
+  }
+}
+
+// interpolateForce of the schemes on the grid g.acc (nodes outside the grid
+// count zero): NGP emcNGPScheme.hpp:55-68, CIC emcCICScheme.hpp:122-141
+// with the (1 - wY)(1 - wY) weight of its last node, NEC emcNECScheme.hpp:
+// 99-113
+template <typename R>
+__device__ __forceinline__ void interp(const DevGeom<R> &g, R x, R y, R &fx, R &fy) {
+#pragma clang fp contract(off)
+  if (g.pm == EMCGPU_PM_NGP) {
+    acc_at(g, pos_coord(x, g.hx), pos_coord(y, g.hy), fx, fy);
+    return;
+  }
+  const R sx = Math<R>::div(x, g.hx), sy = Math<R>::div(y, g.hy);
+  const R flx = Math<R>::floor(sx), fly = Math<R>::floor(sy);
+  const int mx = (int)flx, my = (int)fly;
+  if (g.pm == EMCGPU_PM_CIC) {
+    const R wx = sx - flx, wy = sy - fly;
+    R a0x, a0y, a1x, a1y, a2x, a2y, a3x, a3y;
+    acc_at(g, mx, my, a0x, a0y);
+    acc_at(g, mx + 1, my, a1x, a1y);
+    acc_at(g, mx, my + 1, a2x, a2y);
+    acc_at(g, mx + 1, my + 1, a3x, a3y);
+    fx = a0x * wx * wy + a1x * (R(1) - wx) * wy + a2x * wx * (R(1) - wy) +
+         a3x * (R(1) - wy) * (R(1) - wy);
+    fy = a0y * wx * wy + a1y * (R(1) - wx) * wy + a2y * wx * (R(1) - wy) +
+         a3y * (R(1) - wy) * (R(1) - wy);
+  } else {
+    R e0x, e0y, e1x, e1y, e2x, e2y;
+    acc_at(g, mx, my, e0x, e0y);
+    acc_at(g, mx, my + 1, e1x, e1y);
+    acc_at(g, mx + 1, my, e2x, e2y);
+    fx = (e0x + e1x) / R(2);
+    fy = (e0y + e2y) / R(2);
+  }
+}
+
+// assignToMesh of one particle (one carrier): add(ix, iy, weight) for the
+// scheme's nodes inside the grid (NGP emcNGPScheme.hpp:42-52, CIC
+// emcCICScheme.hpp:71-86 with node (floor x, floor y) taking wX * wY, NEC
+// emcNECScheme.hpp:62-75)
+template <typename R, typename F>
+__device__ __forceinline__ void assign(const DevGeom<R> &g, R x, R y, F &&add) {
+#pragma clang fp contract(off)
+  auto put = [&](int ix, int iy, R w) {
+    if ((unsigned)ix < (unsigned)g.nx && (unsigned)iy < (unsigned)g.ny)
+      add(iy * g.nx + ix, w);
+  };
+  if (g.pm == EMCGPU_PM_NGP) {
+    put(pos_coord(x, g.hx), pos_coord(y, g.hy), R(1));
+    return;
+  }
+  const R sx = Math<R>::div(x, g.hx), sy = Math<R>::div(y, g.hy);
+  const R flx = Math<R>::floor(sx), fly = Math<R>::floor(sy);
+  const int mx = (int)flx, my = (int)fly;
+  if (g.pm == EMCGPU_PM_CIC) {
+    const R wx = sx - flx, wy = sy - fly;
+    put(mx, my, wx * wy);
+    put(mx + 1, my, (R(1) - wx) * wy);
+    put(mx, my + 1, wx * (R(1) - wy));
+    put(mx + 1, my + 1, (R(1) - wx) * (R(1) - wy));
+  } else {
+    put(mx, my, R(0.25));
+    put(mx + 1, my, R(0.25));
+    put(mx, my + 1, R(0.25));
+    put(mx + 1, my + 1, R(0.25));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// free flight with position (emcParticleDrift.hpp:11-35) and the boundary
+// (handleParticleAtBoundary, emcParticleDrift.hpp:41-66): returns true if
+// the particle left through an ohmic contact (position clamped onto it);
+// other boundaries reflect specularly (emcScatterHandler.hpp:177-189)
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ bool drift_dev(const InstDev<R> *P, const DevGeom<R> &g, R t, R ax,
+                                          R ay, R &x, R &y, R &kx, R &ky, R &kz, R &E, int v,
+                                          int sub) {
+  const ValleyDev<R> &V = P->valley[v];
+  const R *M = V.hv[sub];
+  const R kx0 = kx, ky0 = ky, kz0 = kz;
+  // k += M F t / hbar (force has no z component)
+  kx += (M[0] * ax + M[1] * ay) * t;
+  ky += (M[1] * ax + M[3] * ay) * t;
+  kz += (M[2] * ax + M[4] * ay) * t;
+  E = energy_of(V, kx, ky, kz);
+  // dx = hbar M (k + k0) / 2 t / m_c(E)  [nm]: cv [m/s per 1/nm] * t [fs] * 1e-6
+  const R mx = (kx + kx0) * R(0.5), my = (ky + ky0) * R(0.5), mz = (kz + kz0) * R(0.5);
+  const R f = Math<R>::div(V.cv * t * R(1e-6), R(1) + R(2) * V.alpha * E);
+  x += (M[0] * mx + M[1] * my + M[2] * mz) * f;
+  y += (M[1] * mx + M[3] * my + M[4] * mz) * f;
+  if (x < R(0) || x > g.maxx || y < R(0) || y > g.maxy) {
+    // std::max(0, std::min(pos, maxPos))
+    R bx = (g.maxx < x) ? g.maxx : x;
+    bx = (R(0) < bx) ? bx : R(0);
+    R by = (g.maxy < y) ? g.maxy : y;
+    by = (R(0) < by) ? by : R(0);
+    const int cx = pos_coord(bx, g.hx), cy = pos_coord(by, g.hy);
+    if ((unsigned)cx < (unsigned)g.nx && (unsigned)cy < (unsigned)g.ny &&
+        g.cnode[cy * g.nx + cx] >= 0) {
+      x = bx;
+      y = by;
+      return true;
+    }
+    if (x < R(0)) {
+      x = -x;
+      kx = -kx;
+    } else if (x > g.maxx) {
+      x = R(2) * g.maxx - x;
+      kx = -kx;
+    }
+    if (y < R(0)) {
+      y = -y;
+      ky = -ky;
+    } else if (y > g.maxy) {
+      y = R(2) * g.maxy - y;
+      ky = -ky;
+    }
+  }
+  return false;
+}
+
+// one particle, one step of driftScatterParticles
+// (emcBasicParticleHandler.hpp:100-131): force at the current position,
+// drift min(tau, dt); while time remains and the particle is inside:
+// scatter, tau += tau', force again, drift min(rem, tau'), rem -= tau';
+// finally tau -= dt
+template <typename R, int MODE>
+__device__ __forceinline__ bool move_dev(const InstDev<R> *P, const R *__restrict__ cum,
+                                         const DevGeom<R> &g, R &x, R &y, R &kx, R &ky, R &kz,
+                                         R &E, R &tau, int &v, int &sub, StepRng<R, MODE> &rng,
+                                         int &nev) {
+  const R dt = P->dt;
+  R ax, ay;
+  interp(g, x, y, ax, ay);
+  bool removed;
+  uint32_t e = 0;
+  int nreal = 0;
+  if (tau >= dt) {
+    removed = drift_dev(P, g, dt, ax, ay, x, y, kx, ky, kz, E, v, sub);
+  } else {
+    removed = drift_dev(P, g, tau, ax, ay, x, y, kx, ky, kz, E, v, sub);
+    R rem = dt - tau;
+    // every pass draws tau' > 0, so rem strictly decreases; the event cap
+    // bounds the loop regardless
+    while (rem > R(0) && !removed && e < kMaxEventsPerStep) {
+      const R newTau = scatter_event<R, MODE>(P, cum, kx, ky, kz, E, v, sub, e, rng, nreal);
+      tau += newTau;
+      interp(g, x, y, ax, ay);
+      removed = drift_dev(P, g, (newTau < rem) ? newTau : rem, ax, ay, x, y, kx, ky, kz, E, v,
+                          sub);
+      rem -= newTau;
+    }
+  }
+  tau -= dt;
+  nev = (int)e;
+  return removed;
+}
+
+// a new particle at grid node (cx, cy): initParticlePos
+// (emcParticleInitialization.hpp:13-29), the thermal k-space state of
+// emcElectron::generateInitialParticle (emcElectron.hpp:75-90) with the
+// boundary flips of emcParticleInitialization.hpp:36-51, on the Philox
+// stream (step word, pid) of the init domain
+template <typename R>
+__device__ __forceinline__ void new_particle(const InstDev<R> *P, const DevGeom<R> &g, int cx,
+                                             int cy, uint32_t stw, uint32_t pid, uint32_t k0,
+                                             uint32_t k1, R &x, R &y, R &kx, R &ky, R &kz, R &E,
+                                             R &tau, uint32_t &vs) {
+  R u[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+    u[j] = stream_uniform<R>((uint32_t)j, stw, pid, kDomInit, k0, k1);
+  const R lo = R(1e-6), sc = R(1 - 1e-6);
+  auto place = [&](int c, int n, R h, R uu) -> R {
+    if (c == n - 1)
+      return ((R)c - uu * R(0.5)) * h;
+    if (c == 0)
+      return uu * R(0.5) * h;
+    return ((R)c + uu - R(0.5)) * h;
+  };
+  x = place(cx, g.nx, g.hx, u[0]);
+  y = place(cy, g.ny, g.hy, u[1]);
+  int v = (int)Math<R>::floor((R)P->n_valleys * (u[2] * sc + lo));
+  v = v < P->n_valleys ? v : P->n_valleys - 1;
+  const ValleyDev<R> &V = P->valley[v];
+  int sub = (int)Math<R>::floor((R)V.deg * (u[3] * sc + lo));
+  sub = sub < V.deg ? sub : V.deg - 1;
+  E = R(-1.5) * P->vt * Math<R>::log(u[4] * sc + lo);
+  random_direction(normk_of(V, E), u[5], u[6], kx, ky, kz);
+  if ((cx == 0 && kx < R(0)) || (cx == g.nx - 1 && kx > R(0)))
+    kx = -kx;
+  if ((cy == 0 && ky < R(0)) || (cy == g.ny - 1 && ky > R(0)))
+    ky = -ky;
+  tau = -Math<R>::log(u[7] * sc + lo) * P->tau0[v];
+  vs = ((uint32_t)v << 16) | (uint32_t)sub;
+}
+
+// ---------------------------------------------------------------------------
+// frozen-field drift (emcgpu_device_drift): one thread per particle, all
+// steps fused; a particle that leaves stops and is reported
+// ---------------------------------------------------------------------------
+template <typename R, int MODE>
+__global__ __launch_bounds__(256) void k_dev_drift(DevStepArgs<R> a) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.n)
+    return;
+  const InstDev<R> *P = a.inst;
+  R x = a.p.x[i], y = a.p.y[i], kx = a.p.kx[i], ky = a.p.ky[i], kz = a.p.kz[i];
+  R E = a.p.E[i], tau = a.p.tau[i];
+  const uint32_t vs = a.p.vs[i];
+  int v = (int)(vs >> 16), sub = (int)(vs & 0xffffu);
+  bool out = a.exit_step[i] >= 0 || vs == kHole;
+  StepRng<R, MODE> rng;
+  if constexpr (MODE == 0) {
+    rng.pid = a.p.pid[i];
+    rng.inst = kDomMove;
+    rng.k0 = a.key0;
+    rng.k1 = a.key1;
+  } else {
+    rng.rs.p = a.draws + i * a.draw_stride;
+    rng.rs.len = a.draw_stride;
+    rng.rs.pos = (long long)a.cursor[i];
+    rng.rs.over = false;
+  }
+  double grain = (MODE == 1 && a.grain_tau) ? a.grain_tau[i] : 1.0;
+  for (int s = 0; s < a.n_steps; ++s) {
+    int nev = 0;
+    if (!out) {
+      if constexpr (MODE == 0)
+        rng.step = a.step + (uint32_t)s;
+      const bool removed = move_dev<R, MODE>(P, a.cum, a.g, x, y, kx, ky, kz, E, tau, v, sub,
+                                             rng, nev);
+      if constexpr (MODE == 1) {
+        // grain bookkeeping (emcBasicParticleHandler.hpp:134-138), no grain
+        // mechanism: a new grainTau of -ln U[1e-6,1) * 1 s while inside
+        grain -= a.dt_s;
+        if (grain <= 0 && !removed)
+          grain = -::log(rng.rs.uniform(1e-6, 1.)) * 1.0;
+      }
+      if (removed) {
+        out = true;
+        a.exit_step[i] = s;
+        a.exit_contact[i] = a.g.cnode[pos_coord(y, a.g.hy) * a.g.nx + pos_coord(x, a.g.hx)];
+      }
+    }
+    if (a.tr_pos) {  // SI: m, 1/m
+      const long long o = (long long)s * a.n + i;
+      a.tr_pos[2 * o] = (double)x * 1e-9;
+      a.tr_pos[2 * o + 1] = (double)y * 1e-9;
+      a.tr_k[3 * o] = (double)kx * 1e9;
+      a.tr_k[3 * o + 1] = (double)ky * 1e9;
+      a.tr_k[3 * o + 2] = (double)kz * 1e9;
+      a.tr_E[o] = (double)E;
+      a.tr_events[o] = nev;
+    }
+  }
+  a.p.x[i] = x;
+  a.p.y[i] = y;
+  a.p.kx[i] = kx;
+  a.p.ky[i] = ky;
+  a.p.kz[i] = kz;
+  a.p.E[i] = E;
+  a.p.tau[i] = tau;
+  if (vs != kHole)
+    a.p.vs[i] = ((uint32_t)v << 16) | (uint32_t)sub;
+  if constexpr (MODE == 1) {
+    a.cursor[i] = (uint64_t)rng.rs.pos;
+    if (a.grain_tau)
+      a.grain_tau[i] = grain;
+    if (rng.rs.over)
+      atomicAdd(a.draw_overflow, 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// the particle step of a self-consistent run
+// ---------------------------------------------------------------------------
+// particle range of segment seg (one wave each) of an array of n entries
+__device__ __forceinline__ void segment_range(long long n, long long n_seg, long long seg,
+                                              long long &lo, long long &hi) {
+  long long len = (n + n_seg - 1) / n_seg;
+  len = (len + 63) & ~63LL;
+  lo = seg * len < n ? seg * len : n;
+  hi = lo + len < n ? lo + len : n;
+}
+
+template <typename R>
+__global__ __launch_bounds__(kDevStepThreads) void k_dev_step(DevStepArgs<R> a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  __shared__ int ex_cnt[kMaxContacts];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nn = a.g.nn;
+  const size_t grid_bytes = a.lds_dep ? (((size_t)nn * sizeof(R) + 15) & ~(size_t)15) : 0;
+  R *grid = (R *)lds_raw;
+  uint32_t *mycnt = (uint32_t *)(lds_raw + grid_bytes) + wave * a.n_slots;
+  if (a.lds_dep)
+    for (int k = tid; k < nn; k += kDevStepThreads)
+      grid[k] = R(0);
+  for (int k = lane; k < a.n_slots; k += 64)
+    mycnt[k] = 0u;
+  if (tid < kMaxContacts)
+    ex_cnt[tid] = 0;
+  __syncthreads();
+
+  const long long n_total = a.state->n_total;
+  const long long row = a.state->step - a.state->hist_base;
+  const uint32_t stw = (uint32_t)a.state->step;
+  const long long n_seg = (long long)gridDim.x * kDevStepWaves;
+  const long long seg = (long long)blockIdx.x * kDevStepWaves + wave;
+  long long lo, hi;
+  segment_range(n_total, n_seg, seg, lo, hi);
+  const InstDev<R> *P = a.inst;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  uint32_t nst = 0;
+  for (long long base = lo; base < hi; base += 64) {
+    const long long i = base + lane;
+    const uint32_t vs = (i < hi) ? a.p.vs[i] : kHole;
+    int slot = -1;
+    if (vs != kHole) {
+      R x = a.p.x[i], y = a.p.y[i];
+      bool removed = false;
+      if (a.move) {
+        R kx = a.p.kx[i], ky = a.p.ky[i], kz = a.p.kz[i], E = a.p.E[i], tau = a.p.tau[i];
+        int v = (int)(vs >> 16), sub = (int)(vs & 0xffffu);
+        StepRng<R, 0> rng;
+        rng.step = stw;
+        rng.pid = a.p.pid[i];
+        rng.inst = kDomMove;
+        rng.k0 = a.key0;
+        rng.k1 = a.key1;
+        int nev;
+        removed = move_dev<R, 0>(P, a.cum, a.g, x, y, kx, ky, kz, E, tau, v, sub, rng, nev);
+        if (removed) {
+          a.p.vs[i] = kHole;
+          const int c = a.g.cnode[pos_coord(y, a.g.hy) * a.g.nx + pos_coord(x, a.g.hx)];
+          atomicAdd(&ex_cnt[c], 1);
+        } else {
+          a.p.x[i] = x;
+          a.p.y[i] = y;
+          a.p.kx[i] = kx;
+          a.p.ky[i] = ky;
+          a.p.kz[i] = kz;
+          a.p.E[i] = E;
+          a.p.tau[i] = tau;
+          a.p.vs[i] = ((uint32_t)v << 16) | (uint32_t)sub;
+        }
+      }
+      if (!removed) {
+        if (a.lds_dep)
+          assign(a.g, x, y, [&](int k, R w) { atomicAdd(&grid[k], w); });
+        else
+          assign(a.g, x, y, [&](int k, R w) { atomicAdd(&a.dep[k], w); });
+        const int cx = pos_coord(x, a.g.hx), cy = pos_coord(y, a.g.hy);
+        if ((unsigned)cx < (unsigned)a.g.nx && (unsigned)cy < (unsigned)a.g.ny)
+          slot = a.g.cslot[cy * a.g.nx + cx];
+      }
+    }
+    // rank of each particle at an ohmic node among the segment's particles
+    // at that node, in array order
+    const uint64_t at = __ballot(slot >= 0);
+    if (at) {
+      uint64_t pend = at;
+      uint32_t rank = 0;
+      while (pend) {
+        const int leader = __ffsll((unsigned long long)pend) - 1;
+        const int s = __shfl(slot, leader, 64);
+        const uint64_t m = __ballot(slot == s);
+        if (slot == s)
+          rank = mycnt[s] + (uint32_t)__popcll(m & lt);
+        if (lane == leader)
+          mycnt[s] += (uint32_t)__popcll(m);
+        pend &= ~m;
+      }
+      if (slot >= 0)
+        a.stage[lo + nst + (uint32_t)__popcll(at & lt)] =
+            make_uint2((uint32_t)i, ((uint32_t)slot << 22) | rank);
+      nst += (uint32_t)__popcll(at);
+    }
+  }
+  for (int k = lane; k < a.n_slots; k += 64)
+    a.seg_cnt[(long long)k * n_seg + seg] = mycnt[k];
+  if (lane == 0)
+    a.seg_nst[seg] = nst;
+  __syncthreads();
+  if (a.lds_dep)
+    for (int k = tid; k < nn; k += kDevStepThreads)
+      a.dep[(long long)blockIdx.x * nn + k] = grid[k];
+  if (tid < a.n_contacts && ex_cnt[tid] && a.hist.exits)
+    atomicAdd(&a.hist.exits[row * a.n_contacts + tid], ex_cnt[tid]);
+}
+
+// ---------------------------------------------------------------------------
+// ohmic contacts: per node, array-order offsets of the segments' particles,
+// then keep while nrPart < expected, inject ceil(expected - kept)
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void k_dev_contact_scan(DevContactArgs<R> a) {
+  __shared__ uint32_t sc[256];
+  const int slot = blockIdx.x, tid = threadIdx.x;
+  const int S = a.n_segments;
+  const uint32_t *cnt = a.seg_cnt + (long long)slot * S;
+  uint32_t *off = a.seg_off + (long long)slot * S;
+  const int per = (S + 255) / 256;
+  const int b0 = tid * per < S ? tid * per : S;
+  const int b1 = b0 + per < S ? b0 + per : S;
+  uint32_t tot = 0;
+  for (int b = b0; b < b1; ++b)
+    tot += cnt[b];
+  sc[tid] = tot;
+  __syncthreads();
+  for (int d = 1; d < 256; d <<= 1) {
+    const uint32_t t = tid >= d ? sc[tid - d] : 0u;
+    __syncthreads();
+    sc[tid] += t;
+    __syncthreads();
+  }
+  uint32_t run = sc[tid] - tot;
+  for (int b = b0; b < b1; ++b) {
+    off[b] = run;
+    run += cnt[b];
+  }
+  if (tid == 0) {
+    const long long total = sc[255];
+    const long long row = a.state->step - a.state->hist_base;
+    const double e = a.slot_exp[slot];
+    const long long ce = (long long)ceil(e);
+    const long long kept = total < ce ? total : ce;
+    const double diff = e - (double)kept;
+    const long long inj = diff > 0.0 ? (long long)ceil(diff) : 0;
+    a.slot_kept[slot] = (int)kept;
+    a.slot_inj[slot] = (int)inj;
+    const int c = a.slot_contact[slot];
+    if (a.hist.dels && total > kept)
+      atomicAdd(&a.hist.dels[row * a.n_contacts + c], (int)(total - kept));
+    if (a.hist.injs && inj > 0)
+      atomicAdd(&a.hist.injs[row * a.n_contacts + c], (int)inj);
+    __threadfence();
+    const unsigned int t = atomicAdd(&a.state->ticket, 1u);
+    if (t == (unsigned int)a.n_slots - 1) {
+      __threadfence();
+      long long sum = 0;
+      for (int s = 0; s < a.n_slots; ++s)
+        sum += __hip_atomic_load(&a.slot_inj[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (a.state->n_total + sum > a.state->capacity) {
+        a.state->overflow = 1;
+        sum = a.state->capacity - a.state->n_total;
+      }
+      a.state->inj_total = sum;
+      a.state->ticket = 0u;
+    }
+  }
+}
+
+// deletions of the excess at ohmic nodes (one wave per segment) and
+// injections (one thread per new particle, after the deletion blocks)
+template <typename R>
+__global__ __launch_bounds__(256) void k_dev_resolve(DevContactArgs<R> a) {
+  __shared__ int pre[kMaxSlots + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long long n_total = a.state->n_total;
+  const int nA = (a.n_segments + 3) / 4;
+  if ((int)blockIdx.x < nA) {
+    const long long seg = (long long)blockIdx.x * 4 + wave;
+    if (seg >= a.n_segments)
+      return;
+    long long lo, hi;
+    segment_range(n_total, a.n_segments, seg, lo, hi);
+    const uint32_t nst = a.seg_nst[seg];
+    for (uint32_t j = lane; j < nst; j += 64) {
+      const uint2 e = a.stage[lo + j];
+      const int slot = (int)(e.y >> 22);
+      const uint32_t rank = a.seg_off[(long long)slot * a.n_segments + seg] + (e.y & 0x3fffffu);
+      if (rank >= (uint32_t)a.slot_kept[slot]) {
+        const long long i = e.x;
+        a.p.vs[i] = kHole;
+        assign(a.g, a.p.x[i], a.p.y[i], [&](int k, R w) { atomicAdd(&a.corr[k], -(double)w); });
+      }
+    }
+    return;
+  }
+  // injections
+  const long long T = a.state->inj_total;
+  const long long t = ((long long)blockIdx.x - nA) * 256 + tid;
+  if (((long long)blockIdx.x - nA) * 256 >= T)
+    return;
+  if (tid == 0) {
+    int s = 0;
+    pre[0] = 0;
+    for (int q = 0; q < a.n_slots; ++q) {
+      s += a.slot_inj[q];
+      pre[q + 1] = s;
+    }
+  }
+  __syncthreads();
+  if (t >= T)
+    return;
+  int lo = 0, hi = a.n_slots;  // pre[lo] <= t < pre[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pre[mid] <= t)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const int node = a.slot_node[lo];
+  const int cx = node % a.g.nx, cy = node / a.g.nx;
+  const long long idx = n_total + t;
+  const uint32_t pid = (uint32_t)(a.state->next_pid + t);
+  R x, y, kx, ky, kz, E, tau;
+  uint32_t vs;
+  new_particle(a.inst, a.g, cx, cy, (uint32_t)a.state->step, pid, a.key0, a.key1, x, y, kx, ky,
+               kz, E, tau, vs);
+  a.p.x[idx] = x;
+  a.p.y[idx] = y;
+  a.p.kx[idx] = kx;
+  a.p.ky[idx] = ky;
+  a.p.kz[idx] = kz;
+  a.p.E[idx] = E;
+  a.p.tau[idx] = tau;
+  a.p.vs[idx] = vs;
+  a.p.pid[idx] = pid;
+  assign(a.g, x, y, [&](int k, R w) { atomicAdd(&a.corr[k], (double)w); });
+}
+
+// carriers per node = FP64 sum of the block grids (fixed order) + the
+// corrections; concentration (updateCurrentParticleConcentrations,
+// emcSimulationResults.hpp:98-116); commit of the step's counters
+template <typename R>
+__global__ __launch_bounds__(256) void k_dev_reduce(DevContactArgs<R> a, int commit) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  const int nn = a.g.nn;
+  if (k < nn) {
+    double s = 0.0;
+    for (int b = 0; b < a.n_blocks_dep; ++b)
+      s += (double)a.dep[(long long)b * nn + k];
+    s += a.corr[k];
+    a.corr[k] = 0.0;
+    if (a.n_blocks_dep == 1)  // the atomic deposit accumulates in row 0
+      a.dep[k] = R(0);
+    a.nrpart[k] = s;
+    a.conc[k] = s * a.cfac[k];
+  }
+  if (commit && blockIdx.x == 0 && threadIdx.x == 0) {
+    DevState *st = a.state;
+    const long long row = st->step - st->hist_base;
+    long long d = 0;
+    for (int c = 0; c < a.n_contacts; ++c) {
+      const long long o = row * a.n_contacts + c;
+      d += (long long)a.hist.injs[o] - a.hist.exits[o] - a.hist.dels[o];
+    }
+    st->n_alive += d;
+    st->n_total += st->inj_total;
+    st->next_pid += st->inj_total;
+    st->inj_total = 0;
+    if (a.hist.n_part)
+      a.hist.n_part[row] = st->n_alive;
+    st->step += 1;
+  }
+}
+
+// zeroes a global grid row (deposits of the atomic path, corrections)
+__global__ void k_dev_zero(double *p, long long n) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n)
+    p[i] = 0.0;
+}
+template <typename R>
+__global__ void k_dev_zero_r(R *p, long long n) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n)
+    p[i] = R(0);
+}
+__global__ void k_dev_zero_i(int *p, long long n) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n)
+    p[i] = 0;
+}
+
+// initial particles: particle t at the node whose prefix range holds t
+template <typename R>
+__global__ __launch_bounds__(256) void k_dev_init(DevContactArgs<R> a, const long long *prefix,
+                                                  long long n) {
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n)
+    return;
+  int lo = 0, hi = a.g.nn;  // prefix[lo] <= t < prefix[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= t)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  R x, y, kx, ky, kz, E, tau;
+  uint32_t vs;
+  new_particle(a.inst, a.g, lo % a.g.nx, lo / a.g.nx, 0xFFFFFFFFu, (uint32_t)t, a.key0, a.key1, x,
+               y, kx, ky, kz, E, tau, vs);
+  a.p.x[t] = x;
+  a.p.y[t] = y;
+  a.p.kx[t] = kx;
+  a.p.ky[t] = ky;
+  a.p.kz[t] = kz;
+  a.p.E[t] = E;
+  a.p.tau[t] = tau;
+  a.p.vs[t] = vs;
+  a.p.pid[t] = (uint32_t)t;
+}
+
+// ---------------------------------------------------------------------------
+// stable compaction of the ensemble (holes out, array order kept)
+// ---------------------------------------------------------------------------
+constexpr int kCompactTile = 1024;
+
+__global__ __launch_bounds__(256) void k_dev_count_alive(const uint32_t *vs, const DevState *st,
+                                                         int *blk) {
+  __shared__ int ws[4];
+  const long long n = st->n_total;
+  const long long i0 = (long long)blockIdx.x * kCompactTile;
+  int c = 0;
+  for (int j = threadIdx.x; j < kCompactTile; j += 256) {
+    const long long i = i0 + j;
+    c += (i < n && vs[i] != kHole) ? 1 : 0;
+  }
+  for (int off = 32; off > 0; off >>= 1)
+    c += __shfl_xor(c, off, 64);
+  if ((threadIdx.x & 63) == 0)
+    ws[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    blk[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+// exclusive scan of the block counts in one workgroup
+__global__ __launch_bounds__(1024) void k_dev_scan_blocks(int *blk, int nb, DevState *st) {
+  __shared__ long long sc[1024];
+  const int tid = threadIdx.x;
+  const int per = (nb + 1023) / 1024;
+  const int b0 = tid * per < nb ? tid * per : nb;
+  const int b1 = b0 + per < nb ? b0 + per : nb;
+  long long tot = 0;
+  for (int b = b0; b < b1; ++b)
+    tot += blk[b];
+  sc[tid] = tot;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    const long long t = tid >= d ? sc[tid - d] : 0;
+    __syncthreads();
+    sc[tid] += t;
+    __syncthreads();
+  }
+  long long run = sc[tid] - tot;
+  for (int b = b0; b < b1; ++b) {
+    const int c = blk[b];
+    blk[b] = (int)run;
+    run += c;
+  }
+  if (tid == 1023)
+    st->n_alive = sc[1023];
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_dev_compact(DevParticles<R> src, DevParticles<R> dst,
+                                                     const int *blk, DevState *st) {
+  __shared__ int ws[4];
+  const long long n = st->n_total;
+  const long long i0 = (long long)blockIdx.x * kCompactTile;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int base = blk[blockIdx.x];
+  for (int j0 = 0; j0 < kCompactTile; j0 += 256) {
+    const long long i = i0 + j0 + threadIdx.x;
+    const bool keep = i < n && src.vs[i] != kHole;
+    const uint64_t m = __ballot(keep);
+    if (lane == 0)
+      ws[wave] = __popcll(m);
+    __syncthreads();
+    int wo = 0;
+    for (int w = 0; w < wave; ++w)
+      wo += ws[w];
+    const int tot = ws[0] + ws[1] + ws[2] + ws[3];
+    if (keep) {
+      const long long o = base + wo + __popcll(m & ((1ull << lane) - 1ull));
+      dst.x[o] = src.x[i];
+      dst.y[o] = src.y[i];
+      dst.kx[o] = src.kx[i];
+      dst.ky[o] = src.ky[i];
+      dst.kz[o] = src.kz[i];
+      dst.E[o] = src.E[i];
+      dst.tau[o] = src.tau[i];
+      dst.vs[o] = src.vs[i];
+      dst.pid[o] = src.pid[i];
+    }
+    base += tot;
+    __syncthreads();
+  }
+}
+
+__global__ void k_dev_compact_commit(DevState *st) { st->n_total = st->n_alive; }
+
+// history rows of the next steps start at the current step
+__global__ void k_dev_mark(DevState *st) { st->hist_base = st->step; }
+
+// ---------------------------------------------------------------------------
+// known-answer entry points on caller positions (metres, FP64)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_dev_assign_pos(DevGeom<double> g, const double *pos,
+                                                        long long n, double *grid) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n)
+    return;
+  assign(g, pos[2 * i], pos[2 * i + 1], [&](int k, double w) { atomicAdd(&grid[k], w); });
+}
+
+__global__ __launch_bounds__(256) void k_dev_interp_pos(DevGeom<double> g, const double *pos,
+                                                        long long n, double charge,
+                                                        double *force) {
+#pragma clang fp contract(off)
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n)
+    return;
+  double fx, fy;
+  interp(g, pos[2 * i], pos[2 * i + 1], fx, fy);
+  force[2 * i] = charge * fx;
+  force[2 * i + 1] = charge * fy;
+}
+
+// ---------------------------------------------------------------------------
+// Poisson (Newton + FP32 multigrid) and E-field on one workgroup
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double block_max(double v, double *red) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0)
+    red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double m = red[0];
+  for (int w = 1; w < kDevThreads / 64; ++w)
+    m = red[w] > m ? red[w] : m;
+  return m;
+}
+
+__device__ __forceinline__ double block_sum(double v, double *red) {
+  for (int off = 32; off > 0; off >>= 1)
+    v += __shfl_xor(v, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0)
+    red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < kDevThreads / 64; ++w)
+    s += red[w];
+  return s;
+}
+
+struct MGView {
+  float *d, *r, *res;
+  float *sig;  // level-stacked diagonal shift (global)
+  float hfx, hfy;
+};
+
+// red-black Gauss-Seidel sweep of colour c on level l of A d = r
+// (A: 5-point finite-volume operator, mirror neighbours at boundaries)
+template <typename R>
+__device__ __forceinline__ void mg_sweep(const DevFieldArgs<R> &a, const MGView &m, int l, int c) {
+  const MGLevel L = a.lev[l];
+  float *d = m.d + L.off;
+  const float *r = m.r + L.off;
+  const uint8_t *mask = a.lmask + L.off;
+  const DimTab *TX = a.tab + L.tx, *TY = a.tab + L.ty;
+  const int hr = (L.nx + 1) >> 1;
+  for (int t = threadIdx.x; t < hr * L.ny; t += kDevThreads) {
+    const int j = t / hr;
+    const int i = 2 * (t - j * hr) + ((j + c) & 1);
+    if (i >= L.nx)
+      continue;
+    const int k = j * L.nx + i;
+    if (mask[k])
+      continue;
+    const DimTab tx = TX[i], ty = TY[j];
+    const float cl = m.hfx * ty.w * tx.inv_l, cr = m.hfx * ty.w * tx.inv_r;
+    const float cd = m.hfy * tx.w * ty.inv_l, cu = m.hfy * tx.w * ty.inv_r;
+    const int il = i > 0 ? k - 1 : k + 1, ir = i < L.nx - 1 ? k + 1 : k - 1;
+    const int jd = j > 0 ? k - L.nx : k + L.nx, ju = j < L.ny - 1 ? k + L.nx : k - L.nx;
+    const float diag = cl + cr + cd + cu + m.sig[L.off + k];
+    d[k] = (r[k] + cl * d[il] + cr * d[ir] + cd * d[jd] + cu * d[ju]) / diag;
+  }
+  __syncthreads();
+}
+
+// residual of level l into m.res, then full-weighting restriction (the
+// transpose of bilinear prolongation, mirror images folded in) to level l+1
+template <typename R>
+__device__ __forceinline__ void mg_restrict(const DevFieldArgs<R> &a, const MGView &m, int l) {
+  const MGLevel L = a.lev[l], C = a.lev[l + 1];
+  const float *d = m.d + L.off;
+  const float *r = m.r + L.off;
+  const uint8_t *mask = a.lmask + L.off;
+  const DimTab *TX = a.tab + L.tx, *TY = a.tab + L.ty;
+  for (int k = threadIdx.x; k < L.nx * L.ny; k += kDevThreads) {
+    const int j = k / L.nx, i = k - j * L.nx;
+    if (mask[k]) {
+      m.res[k] = 0.0f;
+      continue;
+    }
+    const DimTab tx = TX[i], ty = TY[j];
+    const float cl = m.hfx * ty.w * tx.inv_l, cr = m.hfx * ty.w * tx.inv_r;
+    const float cd = m.hfy * tx.w * ty.inv_l, cu = m.hfy * tx.w * ty.inv_r;
+    const int il = i > 0 ? k - 1 : k + 1, ir = i < L.nx - 1 ? k + 1 : k - 1;
+    const int jd = j > 0 ? k - L.nx : k + L.nx, ju = j < L.ny - 1 ? k + L.nx : k - L.nx;
+    const float diag = cl + cr + cd + cu + m.sig[L.off + k];
+    m.res[k] = r[k] - (diag * d[k] - (cl * d[il] + cr * d[ir] + cd * d[jd] + cu * d[ju]));
+  }
+  __syncthreads();
+  float *rc = m.r + C.off;
+  float *dc = m.d + C.off;
+  const uint8_t *cmask = a.lmask + C.off;
+  const DimTab *CX = a.tab + C.tx, *CY = a.tab + C.ty;
+  for (int K = threadIdx.x; K < C.nx * C.ny; K += kDevThreads) {
+    dc[K] = 0.0f;
+    const int J = K / C.nx, I = K - J * C.nx;
+    if (cmask[K]) {
+      rc[K] = 0.0f;
+      continue;
+    }
+    const DimTab cx = CX[I], cy = CY[J];
+    const float wx[3] = {cx.wl, 1.0f, cx.wr}, wy[3] = {cy.wl, 1.0f, cy.wr};
+    float s = 0.0f;
+    for (int b = 0; b < 3; ++b) {
+      if (wy[b] == 0.0f)
+        continue;
+      const int jj = cy.f + b - 1;
+      for (int q = 0; q < 3; ++q) {
+        if (wx[q] == 0.0f)
+          continue;
+        s += wx[q] * wy[b] * m.res[jj * L.nx + cx.f + q - 1];
+      }
+    }
+    rc[K] = s;
+  }
+  __syncthreads();
+}
+
+// the Newton diagonal shift of every coarse level: full weighting of the
+// finer level's (the lumped Galerkin product P^T diag(sigma) P)
+template <typename R>
+__device__ __forceinline__ void mg_restrict_sig(const DevFieldArgs<R> &a, const MGView &m) {
+  for (int l = 0; l + 1 < a.n_levels; ++l) {
+    const MGLevel L = a.lev[l], C = a.lev[l + 1];
+    const float *sf = m.sig + L.off;
+    float *sc = m.sig + C.off;
+    const uint8_t *cmask = a.lmask + C.off;
+    const DimTab *CX = a.tab + C.tx, *CY = a.tab + C.ty;
+    for (int K = threadIdx.x; K < C.nx * C.ny; K += kDevThreads) {
+      const int J = K / C.nx, I = K - J * C.nx;
+      if (cmask[K]) {
+        sc[K] = 0.0f;
+        continue;
+      }
+      const DimTab cx = CX[I], cy = CY[J];
+      const float wx[3] = {cx.wl, 1.0f, cx.wr}, wy[3] = {cy.wl, 1.0f, cy.wr};
+      float s = 0.0f;
+      for (int b = 0; b < 3; ++b) {
+        if (wy[b] == 0.0f)
+          continue;
+        const int jj = cy.f + b - 1;
+        for (int q = 0; q < 3; ++q) {
+          if (wx[q] == 0.0f)
+            continue;
+          s += wx[q] * wy[b] * sf[jj * L.nx + cx.f + q - 1];
+        }
+      }
+      sc[K] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// d_l += P d_{l+1} (bilinear in the fine index positions)
+template <typename R>
+__device__ __forceinline__ void mg_prolong(const DevFieldArgs<R> &a, const MGView &m, int l) {
+  const MGLevel L = a.lev[l], C = a.lev[l + 1];
+  float *d = m.d + L.off;
+  const float *dc = m.d + C.off;
+  const uint8_t *mask = a.lmask + L.off;
+  const DimTab *TX = a.tab + L.tx, *TY = a.tab + L.ty;
+  for (int k = threadIdx.x; k < L.nx * L.ny; k += kDevThreads) {
+    if (mask[k])
+      continue;
+    const int j = k / L.nx, i = k - j * L.nx;
+    const DimTab tx = TX[i], ty = TY[j];
+    const float ax = tx.a, ay = ty.a;
+    const int b0 = ty.cI * C.nx + tx.cI;
+    float v = ax * ay * dc[b0];
+    if (ax < 1.0f)
+      v += (1.0f - ax) * ay * dc[b0 + 1];
+    if (ay < 1.0f) {
+      v += ax * (1.0f - ay) * dc[b0 + C.nx];
+      if (ax < 1.0f)
+        v += (1.0f - ax) * (1.0f - ay) * dc[b0 + C.nx + 1];
+    }
+    d[k] += v;
+  }
+  __syncthreads();
+}
+
+template <typename R>
+__device__ void mg_vcycle(const DevFieldArgs<R> &a, const MGView &m) {
+  const int nl = a.n_levels;
+  for (int k = threadIdx.x; k < a.lev[0].nx * a.lev[0].ny; k += kDevThreads)
+    m.d[k] = 0.0f;
+  __syncthreads();
+  for (int l = 0; l < nl - 1; ++l) {
+    for (int s = 0; s < a.nu_pre; ++s) {
+      mg_sweep(a, m, l, 0);
+      mg_sweep(a, m, l, 1);
+    }
+    mg_restrict(a, m, l);
+  }
+  for (int s = 0; s < a.nu_coarse; ++s) {
+    mg_sweep(a, m, nl - 1, 0);
+    mg_sweep(a, m, nl - 1, 1);
+  }
+  for (int l = nl - 2; l >= 0; --l) {
+    mg_prolong(a, m, l);
+    for (int s = 0; s < a.nu_post; ++s) {
+      mg_sweep(a, m, l, 1);
+      mg_sweep(a, m, l, 0);
+    }
+  }
+}
+
+// Newton's method on the discrete equations of emcSORSolver
+// (PoissonSolver/emcSORSolver.hpp:49-197):
+//   F_k = sum_d hF_d (phi_prev + phi_next - 2 phi_k) + hP (p - n + C) = 0
+// at every non-ohmic node (mirror neighbours at the boundary), with
+// p = exp(-phi) and n = conc (non-equilibrium) or exp(phi) (equilibrium);
+// ohmic nodes hold V/V_T + asinh(C/2).  The residual and the update are
+// FP64; the Newton correction comes from one FP32 multigrid V-cycle.
+// Then calcEField (PMSchemes/emcEFieldCalculation.hpp:13-81).
+template <typename R>
+__global__ __launch_bounds__(kDevThreads) void k_dev_field(DevFieldArgs<R> a) {
+  extern __shared__ __align__(16) float mg[];
+  __shared__ double red[kDevThreads / 64];
+  const int tid = threadIdx.x;
+  const int nx = a.nx, ny = a.ny, nn = a.nn;
+  const MGLevel LL = a.lev[a.n_levels - 1];
+  const int ntot = LL.off + LL.nx * LL.ny;
+  MGView m;
+  m.d = mg;
+  m.r = mg + ntot;
+  m.res = mg + 2 * ntot;
+  m.sig = a.sig;
+  m.hfx = (float)a.hfx;
+  m.hfy = (float)a.hfy;
+  double *pot = a.pot;
+  long long row = 0;
+  if (a.record)
+    row = a.state->step - a.state->hist_base;
+  if (a.record && a.nrpart) {
+    double s = 0.0;
+    for (int k = tid; k < nn; k += kDevThreads)
+      s += a.nrpart[k];
+    s = block_sum(s, red);
+    if (tid == 0) {
+      const double na = (double)a.state->n_alive;
+      a.hist.checksum[row] = fabs(s - na) / (na > 1.0 ? na : 1.0);
+    }
+  }
+  if (a.do_poisson) {
+    if (a.reset_bc) {
+      for (int k = tid; k < nn; k += kDevThreads)
+        if (a.dmask[k])
+          pot[k] = a.equilibrium ? a.bc_eq[k] : a.bc_neq[k];
+      __syncthreads();
+    }
+    const double hfx = a.hfx, hfy = a.hfy, hp = a.hp, hfs2 = 2.0 * (a.hfx + a.hfy);
+    double scale = 1.0, fmax = 0.0;
+    int it = 0;
+    for (;; ++it) {
+      double lmax = 0.0, lsc = 0.0;
+      for (int k = tid; k < nn; k += kDevThreads) {
+        if (a.dmask[k]) {
+          m.r[k] = 0.0f;
+          a.sig[k] = 0.0f;
+          continue;
+        }
+        const int j = k / nx, i = k - j * nx;
+        const int il = i > 0 ? k - 1 : k + 1, ir = i < nx - 1 ? k + 1 : k - 1;
+        const int jd = j > 0 ? k - nx : k + nx, ju = j < ny - 1 ? k + nx : k - nx;
+        const double phi = pot[k];
+        const double p = exp(-phi);
+        const double n = a.equilibrium ? exp(phi) : a.conc[k];
+        const double C = a.dopn[k];
+        const double F = hfx * (pot[il] + pot[ir]) + hfy * (pot[jd] + pot[ju]) - hfs2 * phi +
+                         hp * (p - n + C);
+        m.r[k] = (float)F;
+        a.sig[k] = (float)(hp * (a.equilibrium ? p + n : p));
+        const double af = fabs(F);
+        lmax = af > lmax ? af : lmax;
+        const double sc = hp * (fabs(C) + n + p);
+        lsc = sc > lsc ? sc : lsc;
+      }
+      fmax = block_max(lmax, red);
+      if (it == 0) {
+        scale = block_max(lsc, red);
+        scale = scale > 0.0 ? scale : 1.0;
+      }
+      if (fmax <= a.tol * scale || it >= a.max_iter)
+        break;
+      mg_restrict_sig(a, m);
+      mg_vcycle(a, m);
+      for (int k = tid; k < nn; k += kDevThreads) {
+        if (a.dmask[k])
+          continue;
+        double d = (double)m.d[k];
+        if (a.equilibrium) {  // damped Newton from the charge-neutral guess
+          d = d > 2.0 ? 2.0 : d;
+          d = d < -2.0 ? -2.0 : d;
+        }
+        pot[k] += d;
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      if (a.iters_out)
+        *a.iters_out = it;
+      if (a.resid_out)
+        *a.resid_out = fmax / scale;
+      if (a.record) {
+        a.hist.iters[row] = it;
+        a.hist.resid[row] = fmax / scale;
+      }
+    }
+    __syncthreads();
+  }
+  if (a.do_efield) {
+#pragma clang fp contract(off)
+    const bool nec = a.pm == EMCGPU_PM_NEC;
+    for (int k = tid; k < nn; k += kDevThreads) {
+      const int j = k / nx, i = k - j * nx;
+      if (i != 0 && i != nx - 1)
+        a.ex[k] = nec ? (pot[k] - pot[k + 1]) * a.vt / a.hxm
+                      : (pot[k - 1] - pot[k + 1]) * a.vt / (2 * a.hxm);
+      if (j != 0 && j != ny - 1)
+        a.ey[k] = nec ? (pot[k] - pot[k + nx]) * a.vt / a.hym
+                      : (pot[k - nx] - pot[k + nx]) * a.vt / (2 * a.hym);
+    }
+    __syncthreads();
+    // setEFieldBoundaryValues: a contact copies the inner neighbour, any
+    // other boundary node is zero
+    for (int k = tid; k < nn; k += kDevThreads) {
+      const int j = k / nx, i = k - j * nx;
+      const int bx = a.bcx[k], by = a.bcy[k];
+      if (bx != -2)
+        a.ex[k] = bx == -1 ? 0.0 : a.ex[i == 0 ? k + 1 : k - 1];
+      if (by != -2)
+        a.ey[k] = by == -1 ? 0.0 : a.ey[j == 0 ? k + nx : k - nx];
+    }
+    __syncthreads();
+    if (a.acc)
+      for (int k = tid; k < nn; k += kDevThreads) {
+        a.acc[2 * k] = (R)(a.ex[k] * a.acc_scale);
+        a.acc[2 * k + 1] = (R)(a.ey[k] * a.acc_scale);
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launch wrappers
+// ---------------------------------------------------------------------------
+template <typename R>
+hipError_t dev_launch_field(const DevFieldArgs<R> &a, size_t lds, hipStream_t st) {
+  if (lds > (size_t)kDevMaxLds)
+    return hipErrorInvalidValue;
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_dev_field<R>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return e;
+    }
+  }
+  hipLaunchKernelGGL((k_dev_field<R>), dim3(1), dim3(kDevThreads), lds, st, a);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t dev_launch_step(const DevStepArgs<R> &a, int blocks, size_t lds, hipStream_t st) {
+  if (lds > (size_t)kDevMaxLds)
+    return hipErrorInvalidValue;
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_dev_step<R>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return e;
+    }
+  }
+  hipLaunchKernelGGL((k_dev_step<R>), dim3(blocks), dim3(kDevStepThreads), lds, st, a);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t dev_launch_drift(const DevStepArgs<R> &a, int mode, hipStream_t st) {
+  const dim3 g((unsigned)((a.n + 255) / 256)), b(256);
+  if (mode == 1)
+    hipLaunchKernelGGL((k_dev_drift<R, 1>), g, b, 0, st, a);
+  else
+    hipLaunchKernelGGL((k_dev_drift<R, 0>), g, b, 0, st, a);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t dev_launch_contacts(const DevContactArgs<R> &a, long long max_inj, hipStream_t st) {
+  hipLaunchKernelGGL((k_dev_contact_scan<R>), dim3(a.n_slots), dim3(256), 0, st, a);
+  const long long nA = (a.n_segments + 3) / 4, nB = (max_inj + 255) / 256;
+  hipLaunchKernelGGL((k_dev_resolve<R>), dim3((unsigned)(nA + nB)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t dev_launch_reduce(const DevContactArgs<R> &a, int commit, hipStream_t st) {
+  hipLaunchKernelGGL((k_dev_reduce<R>), dim3((a.g.nn + 255) / 256), dim3(256), 0, st, a, commit);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t dev_launch_init(const DevContactArgs<R> &a, const long long *prefix, long long n,
+                           hipStream_t st) {
+  if (n <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL((k_dev_init<R>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a,
+                     prefix, n);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t dev_launch_compact(const DevParticles<R> &src, const DevParticles<R> &dst, int *blk,
+                              long long n_total, DevState *state, hipStream_t st) {
+  const long long nb = (n_total + kCompactTile - 1) / kCompactTile;
+  if (nb <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_dev_count_alive, dim3((unsigned)nb), dim3(256), 0, st, src.vs, state, blk);
+  hipLaunchKernelGGL(k_dev_scan_blocks, dim3(1), dim3(1024), 0, st, blk, (int)nb, state);
+  hipLaunchKernelGGL((k_dev_compact<R>), dim3((unsigned)nb), dim3(256), 0, st, src, dst, blk,
+                     state);
+  hipLaunchKernelGGL(k_dev_compact_commit, dim3(1), dim3(1), 0, st, state);
+  return hipGetLastError();
+}
+
+hipError_t dev_launch_mark(DevState *state, hipStream_t st) {
+  hipLaunchKernelGGL(k_dev_mark, dim3(1), dim3(1), 0, st, state);
+  return hipGetLastError();
+}
+
+// resident step blocks per CU at the given dynamic LDS
+template <typename R>
+int dev_step_blocks_per_cu(size_t lds) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_dev_step<R>, kDevStepThreads, lds) !=
+      hipSuccess)
+    return 1;
+  return n > 0 ? n : 1;
+}
+
+hipError_t dev_launch_zero(double *p, long long n, hipStream_t st) {
+  if (n <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_dev_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+template <typename R>
+hipError_t dev_launch_zero_r(R *p, long long n, hipStream_t st) {
+  if (n <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL((k_dev_zero_r<R>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+hipError_t dev_launch_zero_i(int *p, long long n, hipStream_t st) {
+  if (n <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_dev_zero_i, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+hipError_t dev_launch_assign_pos(const DevGeom<double> &g, const double *pos, long long n,
+                                 double *grid, hipStream_t st) {
+  if (n <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_dev_assign_pos, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, pos,
+                     n, grid);
+  return hipGetLastError();
+}
+hipError_t dev_launch_interp_pos(const DevGeom<double> &g, const double *pos, long long n,
+                                 double charge, double *force, hipStream_t st) {
+  if (n <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_dev_interp_pos, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, pos,
+                     n, charge, force);
+  return hipGetLastError();
+}
+
+#define EMCGPU_DEV_INST(R)                                                                   \
+  template hipError_t dev_launch_field<R>(const DevFieldArgs<R> &, size_t, hipStream_t);      \
+  template hipError_t dev_launch_step<R>(const DevStepArgs<R> &, int, size_t, hipStream_t);   \
+  template hipError_t dev_launch_drift<R>(const DevStepArgs<R> &, int, hipStream_t);          \
+  template hipError_t dev_launch_contacts<R>(const DevContactArgs<R> &, long long,           \
+                                             hipStream_t);                                    \
+  template hipError_t dev_launch_reduce<R>(const DevContactArgs<R> &, int, hipStream_t);      \
+  template hipError_t dev_launch_init<R>(const DevContactArgs<R> &, const long long *,        \
+                                         long long, hipStream_t);                             \
+  template hipError_t dev_launch_compact<R>(const DevParticles<R> &, const DevParticles<R> &, \
+                                            int *, long long, DevState *, hipStream_t);       \
+  template hipError_t dev_launch_zero_r<R>(R *, long long, hipStream_t);                      \
+  template int dev_step_blocks_per_cu<R>(size_t);
+EMCGPU_DEV_INST(float)
+EMCGPU_DEV_INST(double)
+
+}  // namespace emcgpu

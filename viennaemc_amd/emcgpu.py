@@ -21,6 +21,10 @@ PARA_ISO, NONPARA_ISO, PARA_ANISO, NONPARA_ANISO = 0, 1, 2, 3
 (ACOUSTIC, ZERO_IV_AB, ZERO_IV_EM, FIRST_IV_AB, FIRST_IV_EM, FROEHLICH_AB,
  FROEHLICH_EM, HOT_FROEHLICH_AB, HOT_FROEHLICH_EM) = range(9)
 
+PM_NGP, PM_CIC, PM_NEC = 0, 1, 2
+XMIN, XMAX, YMIN, YMAX = 0, 1, 2, 3
+MAX_CONTACTS, MAX_REGIONS = 8, 16
+
 STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_STATE", 4: "ERR_DRAWS", 5: "ERR_NOMEM"}
 
 # include/emcConstants.hpp:9-25
@@ -77,6 +81,39 @@ class StepArgs(C.Structure):
                 ("trace_energy", C.c_void_p)]
 
 
+class DopingRegion(C.Structure):
+    _fields_ = [("min_pos", C.c_double * 2), ("max_pos", C.c_double * 2), ("doping", C.c_double)]
+
+
+class ContactDesc(C.Structure):
+    _fields_ = [("boundary", C.c_int), ("voltage", C.c_double), ("min_pos", C.c_double),
+                ("max_pos", C.c_double)]
+
+
+class DeviceDesc(C.Structure):
+    _fields_ = [("max_pos", C.c_double * 2), ("spacing", C.c_double * 2), ("width", C.c_double),
+                ("temperature", C.c_double), ("eps_r", C.c_double), ("ni", C.c_double),
+                ("n_doping", C.c_int), ("doping", C.c_void_p), ("n_contacts", C.c_int),
+                ("contacts", C.c_void_p), ("pm_scheme", C.c_int), ("poisson_tol", C.c_double),
+                ("poisson_max_iter", C.c_int)]
+
+
+class DeviceDriftArgs(C.Structure):
+    _fields_ = [("dt", C.c_double), ("n_steps", C.c_int), ("mode", C.c_int),
+                ("seed", C.c_uint64), ("step0", C.c_uint32), ("draws", C.c_void_p),
+                ("draw_stride", C.c_longlong), ("cursor", C.c_void_p), ("grain_tau", C.c_void_p),
+                ("exit_step", C.c_void_p), ("exit_contact", C.c_void_p), ("tr_pos", C.c_void_p),
+                ("tr_k", C.c_void_p), ("tr_E", C.c_void_p), ("tr_events", C.c_void_p)]
+
+
+class DeviceStepArgs(C.Structure):
+    _fields_ = [("dt", C.c_double), ("n_steps", C.c_int), ("seed", C.c_uint64),
+                ("use_graph", C.c_int), ("exits", C.c_void_p), ("deleted", C.c_void_p),
+                ("injected", C.c_void_p), ("n_particles", C.c_void_p),
+                ("poisson_iters", C.c_void_p), ("poisson_resid", C.c_void_p),
+                ("checksum", C.c_void_p)]
+
+
 _LIB = None
 EXPORTS = [
     "emcgpu_create", "emcgpu_destroy", "emcgpu_last_error", "emcgpu_set_model",
@@ -85,6 +122,10 @@ EXPORTS = [
     "emcgpu_init_particles", "emcgpu_step", "emcgpu_get_observables", "emcgpu_obs_layout",
     "emcgpu_get_timing", "emcgpu_synchronize", "emcgpu_init_particles_energy", "emcgpu_set_baths",
     "emcgpu_get_bath", "emcgpu_set_bath_state", "emcgpu_get_bath_tallies", "emcgpu_get_bath_history",
+    "emcgpu_device_setup", "emcgpu_device_info", "emcgpu_device_grids", "emcgpu_poisson",
+    "emcgpu_efield", "emcgpu_deposit", "emcgpu_interpolate", "emcgpu_contacts",
+    "emcgpu_device_set_particles", "emcgpu_device_get_particles", "emcgpu_device_set_efield",
+    "emcgpu_device_drift", "emcgpu_device_init", "emcgpu_device_step", "emcgpu_device_fields",
 ]
 
 
@@ -122,6 +163,21 @@ def load(path=LIB_PATH):
         "emcgpu_set_bath_state": (i, [vp, i, i, vp, d, d]),
         "emcgpu_get_bath_tallies": (i, [vp, i, i, vp, vp]),
         "emcgpu_get_bath_history": (i, [vp, i, i, vp, p(i)]),
+        "emcgpu_device_setup": (i, [vp, p(DeviceDesc)]),
+        "emcgpu_device_info": (i, [vp, vp, vp]),
+        "emcgpu_device_grids": (i, [vp, vp, vp, vp]),
+        "emcgpu_poisson": (i, [vp, vp, vp, i, i, p(i), p(d)]),
+        "emcgpu_efield": (i, [vp, vp, vp, vp]),
+        "emcgpu_deposit": (i, [vp, vp, C.c_size_t, vp]),
+        "emcgpu_interpolate": (i, [vp, vp, vp, vp, C.c_size_t, d, vp]),
+        "emcgpu_contacts": (i, [vp, vp, C.c_size_t, vp, vp]),
+        "emcgpu_device_set_particles": (i, [vp, C.c_size_t, vp, vp, vp, vp, vp, vp]),
+        "emcgpu_device_get_particles": (i, [vp, p(C.c_size_t), vp, vp, vp, vp, vp, vp]),
+        "emcgpu_device_set_efield": (i, [vp, vp, vp]),
+        "emcgpu_device_drift": (i, [vp, p(DeviceDriftArgs)]),
+        "emcgpu_device_init": (i, [vp, C.c_uint64, p(C.c_size_t)]),
+        "emcgpu_device_step": (i, [vp, p(DeviceStepArgs)]),
+        "emcgpu_device_fields": (i, [vp, vp, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -424,3 +480,174 @@ class Context:
 
     def synchronize(self):
         self._chk(self.L.emcgpu_synchronize(self.h))
+
+    # -- 2-D device (instance 0) ------------------------------------------
+    def device_setup(self, dev, pm=PM_CIC, poisson_tol=1e-10, poisson_max_iter=50):
+        """Geometry from a device dictionary (tests/emc_models.py format):
+        max_pos, spacing, width, T, eps_r, ni, doping [(min, max, N)],
+        contacts [(boundary, voltage, min, max)]."""
+        regs = (DopingRegion * max(len(dev["doping"]), 1))()
+        for r, (lo, hi, n) in enumerate(dev["doping"]):
+            regs[r].min_pos[0], regs[r].min_pos[1] = lo
+            regs[r].max_pos[0], regs[r].max_pos[1] = hi
+            regs[r].doping = n
+        cons = (ContactDesc * max(len(dev["contacts"]), 1))()
+        for k, (b, v, lo, hi) in enumerate(dev["contacts"]):
+            cons[k].boundary, cons[k].voltage, cons[k].min_pos, cons[k].max_pos = b, v, lo, hi
+        d = DeviceDesc()
+        d.max_pos[0], d.max_pos[1] = dev["max_pos"]
+        d.spacing[0], d.spacing[1] = dev["spacing"]
+        d.width = dev["width"]
+        d.temperature = dev["T"]
+        d.eps_r = dev["eps_r"]
+        d.ni = dev["ni"]
+        d.n_doping = len(dev["doping"])
+        d.doping = C.cast(regs, C.c_void_p)
+        d.n_contacts = len(dev["contacts"])
+        d.contacts = C.cast(cons, C.c_void_p)
+        d.pm_scheme = pm
+        d.poisson_tol = poisson_tol
+        d.poisson_max_iter = poisson_max_iter
+        self._chk(self.L.emcgpu_device_setup(self.h, C.byref(d)))
+        self.n_contacts = len(dev["contacts"])
+        ext = np.zeros(2, dtype=np.int32)
+        sc = np.zeros(4)
+        self._chk(self.L.emcgpu_device_info(self.h, _ptr(ext), _ptr(sc)))
+        self.ext = (int(ext[0]), int(ext[1]))
+        self.nn = self.ext[0] * self.ext[1]
+        self.Vt, self.debye, self.cell_volume, self.norm_cell_volume = [float(x) for x in sc]
+
+    def device_grids(self):
+        dop = np.zeros(self.nn)
+        reg = np.zeros(self.nn, dtype=np.int32)
+        con = np.zeros(self.nn, dtype=np.int32)
+        self._chk(self.L.emcgpu_device_grids(self.h, _ptr(dop), _ptr(reg), _ptr(con)))
+        return dict(doping=dop, region=reg, contact=con)
+
+    def poisson(self, pot, conc=None, equilibrium=False, reset_bc=True):
+        pot = np.array(pot, dtype=np.float64).reshape(-1)
+        c = None if conc is None else np.ascontiguousarray(conc, dtype=np.float64).reshape(-1)
+        it = C.c_int()
+        res = C.c_double()
+        self._chk(self.L.emcgpu_poisson(self.h, _ptr(pot), _ptr(c), 1 if equilibrium else 0,
+                                        1 if reset_bc else 0, C.byref(it), C.byref(res)))
+        return pot, it.value, res.value
+
+    def efield(self, pot):
+        pot = np.ascontiguousarray(pot, dtype=np.float64).reshape(-1)
+        ex, ey = np.zeros(self.nn), np.zeros(self.nn)
+        self._chk(self.L.emcgpu_efield(self.h, _ptr(pot), _ptr(ex), _ptr(ey)))
+        return ex, ey
+
+    def deposit(self, pos, grid=None):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 2)
+        g = np.zeros(self.nn) if grid is None else np.array(grid, dtype=np.float64)
+        self._chk(self.L.emcgpu_deposit(self.h, _ptr(pos), pos.shape[0], _ptr(g)))
+        return g
+
+    def interpolate(self, ex, ey, pos, charge=-Q):
+        ex = np.ascontiguousarray(ex, dtype=np.float64).reshape(-1)
+        ey = np.ascontiguousarray(ey, dtype=np.float64).reshape(-1)
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 2)
+        f = np.zeros((pos.shape[0], 2))
+        self._chk(self.L.emcgpu_interpolate(self.h, _ptr(ex), _ptr(ey), _ptr(pos), pos.shape[0],
+                                            charge, _ptr(f)))
+        return f
+
+    def contacts(self, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 2)
+        keep = np.zeros(pos.shape[0], dtype=np.int32)
+        inj = np.zeros(self.nn, dtype=np.int32)
+        self._chk(self.L.emcgpu_contacts(self.h, _ptr(pos), pos.shape[0], _ptr(keep), _ptr(inj)))
+        return keep, inj
+
+    def device_set_particles(self, pos, k, E, tau, valley, sub):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+        k = np.ascontiguousarray(k, dtype=np.float64).reshape(-1)
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        tau = np.ascontiguousarray(tau, dtype=np.float64)
+        valley = np.ascontiguousarray(valley, dtype=np.int32)
+        sub = np.ascontiguousarray(sub, dtype=np.int32)
+        n = E.size
+        self._chk(self.L.emcgpu_device_set_particles(self.h, n, _ptr(pos), _ptr(k), _ptr(E),
+                                                     _ptr(tau), _ptr(valley), _ptr(sub)))
+
+    def device_get_particles(self):
+        n = C.c_size_t(0)
+        self._chk(self.L.emcgpu_device_get_particles(self.h, C.byref(n), None, None, None, None,
+                                                     None, None))
+        m = n.value
+        out = dict(pos=np.zeros((m, 2)), k=np.zeros((m, 3)), E=np.zeros(m), tau=np.zeros(m),
+                   valley=np.zeros(m, dtype=np.int32), sub=np.zeros(m, dtype=np.int32))
+        self._chk(self.L.emcgpu_device_get_particles(
+            self.h, C.byref(n), _ptr(out["pos"]), _ptr(out["k"]), _ptr(out["E"]),
+            _ptr(out["tau"]), _ptr(out["valley"]), _ptr(out["sub"])))
+        return out
+
+    def device_set_efield(self, ex, ey):
+        ex = np.ascontiguousarray(ex, dtype=np.float64).reshape(-1)
+        ey = np.ascontiguousarray(ey, dtype=np.float64).reshape(-1)
+        self._chk(self.L.emcgpu_device_set_efield(self.h, _ptr(ex), _ptr(ey)))
+
+    def device_drift(self, dt, n_steps, n, mode=PRODUCTION, seed=0, step0=0, draws=None,
+                     cursor=None, grain_tau=None, trace=False):
+        a = DeviceDriftArgs()
+        a.dt = dt
+        a.n_steps = n_steps
+        a.mode = mode
+        a.seed = seed
+        a.step0 = step0
+        keep = []
+        if mode == REPLAY:
+            draws = np.ascontiguousarray(draws, dtype=np.uint64)
+            if cursor.dtype != np.uint64 or not cursor.flags["C_CONTIGUOUS"]:
+                raise ValueError("cursor must be a contiguous uint64 array")
+            a.draws = _ptr(draws)
+            a.draw_stride = draws.shape[1]
+            a.cursor = _ptr(cursor)
+            if grain_tau is not None:
+                a.grain_tau = _ptr(grain_tau)
+            keep.append(draws)
+        out = dict(exit_step=np.zeros(n, dtype=np.int32), exit_contact=np.zeros(n, dtype=np.int32))
+        a.exit_step = _ptr(out["exit_step"])
+        a.exit_contact = _ptr(out["exit_contact"])
+        if trace:
+            out.update(pos=np.zeros((n_steps, n, 2)), k=np.zeros((n_steps, n, 3)),
+                       E=np.zeros((n_steps, n)), events=np.zeros((n_steps, n), dtype=np.int32))
+            a.tr_pos = _ptr(out["pos"])
+            a.tr_k = _ptr(out["k"])
+            a.tr_E = _ptr(out["E"])
+            a.tr_events = _ptr(out["events"])
+        self._chk(self.L.emcgpu_device_drift(self.h, C.byref(a)))
+        return out
+
+    def device_init(self, seed):
+        n = C.c_size_t(0)
+        self._chk(self.L.emcgpu_device_init(self.h, seed, C.byref(n)))
+        return n.value
+
+    def device_step(self, dt, n_steps, seed=0, graph=False):
+        nc = max(self.n_contacts, 1)
+        out = dict(exits=np.zeros((n_steps, nc), dtype=np.int32),
+                   deleted=np.zeros((n_steps, nc), dtype=np.int32),
+                   injected=np.zeros((n_steps, nc), dtype=np.int32),
+                   n_particles=np.zeros(n_steps, dtype=np.int64),
+                   poisson_iters=np.zeros(n_steps, dtype=np.int32),
+                   poisson_resid=np.zeros(n_steps), checksum=np.zeros(n_steps))
+        a = DeviceStepArgs()
+        a.dt = dt
+        a.n_steps = n_steps
+        a.seed = seed
+        a.use_graph = 1 if graph else 0
+        for key in ("exits", "deleted", "injected", "n_particles", "poisson_iters",
+                    "poisson_resid", "checksum"):
+            setattr(a, key, _ptr(out[key]))
+        self._chk(self.L.emcgpu_device_step(self.h, C.byref(a)))
+        return out
+
+    def device_fields(self):
+        out = {k: np.zeros(self.nn) for k in ("pot", "ex", "ey", "conc", "nr_part")}
+        self._chk(self.L.emcgpu_device_fields(self.h, _ptr(out["pot"]), _ptr(out["ex"]),
+                                              _ptr(out["ey"]), _ptr(out["conc"]),
+                                              _ptr(out["nr_part"])))
+        return out
