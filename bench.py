@@ -52,6 +52,12 @@ def parse():
     ap.add_argument("--no-hot", action="store_true", help="skip the hot-phonon cooling leg")
     ap.add_argument("--hot-steps", type=int, default=2000)
     ap.add_argument("--hot-particles", type=int, default=4_000_000)
+    ap.add_argument("--no-device", action="store_true", help="skip the 2-D resistor leg")
+    ap.add_argument("--device-only", action="store_true",
+                    help="run only the 2-D resistor leg and print its record")
+    ap.add_argument("--device-steps", type=int, default=20000)
+    ap.add_argument("--device-particles", type=int, default=2_000_000)
+    ap.add_argument("--device-fp64-steps", type=int, default=2000)
     ap.add_argument("--cpu-particles", type=int, default=1_000_000)
     ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -165,6 +171,50 @@ def run_hot_phonon(emcgpu, n, warmup, steps, precision, seed=20261202):
                         "every step, %s" % (n, "FP32" if precision else "FP64")}
 
 
+def run_device(emcgpu, n, warmup, steps, precision, seed=20261202, graph=True):
+    """Self-consistent 2-D resistor: 200 x 50 nodes at 1 nm, N_D 1e17 cm^-3
+    with 1e19 cm^-3 ohmic regions, 0.5 V, parabolic m* = 0.26 valley with
+    acoustic + intervalley phonons, CIC, Poisson (Newton + FP32 multigrid,
+    FP64 residual to 1e-10) every 0.1 fs step; whole-job particle-updates/s
+    (particles moved per step x steps / wall time, HIP-graph launch)."""
+    from emc_models import MODELS_DEVICE, resistor_device
+    dev = resistor_device(n_particles=n)
+    ctx = emcgpu.Context(precision=precision)
+    ctx.setup(0, MODELS_DEVICE["device_para"])
+    ctx.device_setup(dev, pm=emcgpu.PM_CIC, poisson_tol=1e-10, poisson_max_iter=50)
+    t0 = time.perf_counter()
+    n0 = ctx.device_init(seed)
+    t_init = time.perf_counter() - t0
+    dt = 0.1e-15
+    if warmup:
+        ctx.device_step(dt, warmup, seed=seed, graph=graph)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    out = ctx.device_step(dt, steps, seed=seed, graph=graph)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    ctx.close()
+    npart = out["n_particles"].astype(np.float64)
+    before = np.concatenate([[npart[0] - (out["injected"][0] - out["exits"][0]
+                                          - out["deleted"][0]).sum()], npart[:-1]])
+    # electrons leaving through each contact per step, net of the contact
+    # equilibrium (handleOhmicContacts returns injected - deleted)
+    net = (out["exits"] - (out["injected"] - out["deleted"])).astype(np.float64)
+    tail = slice(steps // 2, None)
+    q = 1.60219e-19
+    return {"updates_per_s": before.sum() / wall, "particles_init": int(n0), "steps": steps,
+            "ms_per_step": wall / steps * 1e3, "init_s": t_init,
+            "particles_last": int(npart[-1]),
+            "poisson_newton_iters_mean": float(out["poisson_iters"].mean()),
+            "poisson_resid_max": float(out["poisson_resid"].max()),
+            "charge_checksum_max": float(out["checksum"].max()),
+            "contact_current_A": [float(x) for x in q * net[tail].mean(axis=0) / dt],
+            "exits_per_step": float(out["exits"].sum(axis=1).mean()),
+            "workload": "2-D n+ n n+ resistor, 200 x 50 nodes at 1 nm, N_D 1e17 / 1e19 cm^-3, "
+                        "0.5 V, %d e- (parabolic 0.26 m0, acoustic + IV), CIC, Poisson every "
+                        "0.1 fs step, %s" % (n, "FP32" if precision else "FP64")}
+
+
 PROFILE = os.path.join(ROOT, "profiles", "emc_step_si_bulk_10M.md")
 
 
@@ -207,6 +257,11 @@ def main():
     from viennaemc_amd import emcgpu
     spec = MODELS["si_jacoboni"]
     n = args.particles
+    if args.device_only:
+        rec = run_device(emcgpu, args.device_particles, args.warmup, args.device_steps,
+                         emcgpu.FP32)
+        print(json.dumps({"device_resistor": rec}), flush=True)
+        return 0
 
     r32 = run_gpu(emcgpu, spec, emcgpu.FP32, n, args.warmup, args.steps, args.spl,
                   ppl=args.ppl)
@@ -241,6 +296,12 @@ def main():
         if not args.no_fp64:
             out["hot_phonon_fp64"] = run_hot_phonon(emcgpu, args.hot_particles, args.warmup,
                                                     args.hot_steps, emcgpu.FP64)
+    if not args.no_device:
+        out["device_resistor"] = run_device(emcgpu, args.device_particles, args.warmup,
+                                            args.device_steps, emcgpu.FP32)
+        if not args.no_fp64:
+            out["device_resistor_fp64"] = run_device(emcgpu, args.device_particles, args.warmup,
+                                                     args.device_fp64_steps, emcgpu.FP64)
     spl = r32["steps_per_launch"]
     # algorithmic bytes per launch: 48 B per particle-update (the SoA state
     # read and written once per update) x the updates one launch processes;

@@ -1414,6 +1414,41 @@ static void orc_mech_scatter_u(const orc_model *md, const orc_mech *m,
   }
 }
 
+/* one scatter event of the production mode on the (step, particle) stream:
+ * selection on uniform 4e, final state on 4e+1 and 4e+2, the next free
+ * flight on 4e+3 (the loop body of orc_move_particle_philox); returns tau' */
+double orc_philox_event(const orc_model *m, orc_particle *p, uint64_t seed,
+                        uint32_t inst, uint32_t pid, uint32_t step, uint32_t e) {
+  double usel = orc_prod_u(seed, inst, pid, step, 4 * e + 0);
+  double ua = orc_prod_u(seed, inst, pid, step, 4 * e + 1);
+  double ub = orc_prod_u(seed, inst, pid, step, 4 * e + 2);
+  double ut = orc_prod_log_u(orc_prod_u(seed, inst, pid, step, 4 * e + 3));
+  int vv = p->valley;
+  int nt = m->nTables[vv];
+  if (nt > 0) {
+    int level = orc_energy_level(m, p->energy);
+    const double *cum = m->cum[vv];
+    if (!(usel > cum[(size_t)(nt - 1) * m->nLevels + level])) {
+      double lower = 0, upper;
+      for (int j = 0; j < nt; j++) {
+        upper = cum[(size_t)j * m->nLevels + level];
+        if (usel >= lower && usel < upper) {
+          double uc = (usel - lower) / (upper - lower);
+          orc_mech_scatter_u(m, &m->mech[m->tableMech[vv][j]], p, ua, ub, uc);
+          break;
+        }
+        lower = upper;
+      }
+    }
+  }
+  return -log(ut) * m->tau0[p->valley];
+}
+
+double orc_prod_uniform(uint64_t seed, uint32_t inst, uint32_t pid, uint32_t step,
+                        uint32_t j) {
+  return orc_prod_u(seed, inst, pid, step, j);
+}
+
 void orc_move_particle_philox(const orc_model *m, orc_particle *p,
                               double tStep, const double force[3],
                               uint64_t seed, uint32_t inst, uint32_t pid,

@@ -336,6 +336,13 @@ void orc_move_particle_philox(const orc_model *m, orc_particle *p,
                               int32_t *evCount);
 void orc_init_particle_philox(const orc_model *m, orc_particle *p, double Vt,
                               uint64_t seed, uint32_t instance, uint32_t pid);
+/* one production-mode scatter event e of the (step, particle) stream:
+ * selection + final state, returns the next free-flight time [s] */
+double orc_philox_event(const orc_model *m, orc_particle *p, uint64_t seed,
+                        uint32_t inst, uint32_t pid, uint32_t step, uint32_t e);
+/* uniform j of the (step, particle) stream of an instance word */
+double orc_prod_uniform(uint64_t seed, uint32_t inst, uint32_t pid, uint32_t step,
+                        uint32_t j);
 /* production uniform in (0,1]-safe form used by the GPU kernels */
 double orc_philox_u01_f64(uint32_t hi, uint32_t lo);
 

@@ -795,6 +795,287 @@ int orc_device_run(const orc_device_run_cfg *cfg, double *eqPot, int *nInit,
 }
 
 /* ======================================================================
+ * Production twin of the GPU device run (Philox streams)
+ *
+ * The GPU device path keeps the reference's algorithm and array order but
+ * draws its random numbers from Philox-4x32-10 streams: particle pid in
+ * step s uses the counter (event, s, pid, ORC_DOM_MOVE); a particle created
+ * in step s (the initial ones: s = 0xFFFFFFFF) draws its eight initial
+ * uniforms from (j, s, pid, ORC_DOM_INIT); the remainder of a node's initial
+ * carrier count is one Bernoulli draw (0, 0xFFFFFFFE, node, ORC_DOM_COUNT).
+ * Creation indices (pid) run in creation order.  Everything else follows
+ * orc_device_run.
+ * ====================================================================== */
+#define ORC_DOM_MOVE 0x40000000u
+#define ORC_DOM_INIT 0x40000001u
+#define ORC_DOM_COUNT 0x40000002u
+
+int orc_device_move_particle_philox(const orc_model *m, const orc_device *d, int pm,
+                                    const double *ex, const double *ey, double charge,
+                                    orc_particle *p, double pos[2], double tStep,
+                                    uint64_t seed, uint32_t pid, uint32_t step,
+                                    int *contact) {
+  double force[3];
+  orc_pm_force(pm, ex, ey, pos, d->h, d->ext, charge, force);
+  int removed = 0;
+  uint32_t e = 0;
+  if (p->tau >= tStep) {
+    removed = orc_device_drift(m, d, tStep, p, pos, force);
+  } else {
+    removed = orc_device_drift(m, d, p->tau, p, pos, force);
+    double tRem = tStep - p->tau;
+    while (tRem > 0 && !removed) {
+      double newTau = orc_philox_event(m, p, seed, ORC_DOM_MOVE, pid, step, e++);
+      p->tau += newTau;
+      orc_pm_force(pm, ex, ey, pos, d->h, d->ext, charge, force);
+      removed = orc_device_drift(m, d, orc_min_d(tRem, newTau), p, pos, force);
+      tRem -= newTau;
+    }
+  }
+  p->tau -= tStep;
+  if (removed && contact) {
+    size_t c[2];
+    orc_device_pos_to_coord(d, pos, c);
+    *contact = orc_device_contact_idx(d, c);
+  }
+  return removed;
+}
+
+/* a particle created at node c: position (initParticlePos), valley,
+ * subvalley, thermal energy, isotropic direction with the boundary flips,
+ * free-flight time, from the eight initial uniforms of its stream */
+static void orc_device_new_philox(const orc_model *m, const orc_device *d, const size_t c[2],
+                                  uint64_t seed, uint32_t pid, uint32_t sw, double Vt,
+                                  orc_particle *p, double pos[2]) {
+  double u[8];
+  for (int j = 0; j < 8; j++)
+    u[j] = orc_prod_uniform(seed, ORC_DOM_INIT, pid, sw, (uint32_t)j);
+  const double lo = 1e-6, sc = 1. - 1e-6;
+  for (int i = 0; i < 2; i++) {
+    if (c[i] == (size_t)(d->ext[i] - 1))
+      pos[i] = ((double)c[i] - u[i] * 0.5) * d->h[i];
+    else if (c[i] == 0)
+      pos[i] = u[i] * 0.5 * d->h[i];
+    else
+      pos[i] = ((double)c[i] + u[i] - 0.5) * d->h[i];
+  }
+  memset(p, 0, sizeof(*p));
+  p->region = d->region[orc_device_idx(d, c[0], c[1])];
+  p->valley = (int)floor((double)m->nValleys * (u[2] * sc + lo));
+  if (p->valley >= m->nValleys)
+    p->valley = m->nValleys - 1;
+  const orc_valley *v = &m->valley[p->valley];
+  p->subValley = (int)floor((double)v->deg * (u[3] * sc + lo));
+  if (p->subValley >= v->deg)
+    p->subValley = v->deg - 1;
+  p->energy = -1.5 * Vt * log(u[4] * sc + lo);
+  orc_init_random_direction(orc_valley_normk(v, p->energy), u[5], u[6], p->k);
+  for (int i = 0; i < 2; i++)
+    if ((c[i] == 0 && p->k[i] < 0) || (c[i] == (size_t)(d->ext[i] - 1) && p->k[i] > 0))
+      p->k[i] *= -1;
+  p->tau = -log(u[7] * sc + lo) * m->tau0[p->valley];
+  p->grainTau = 1.0;
+}
+
+typedef struct {
+  orc_ensemble e;
+  uint32_t *pid;
+  int pcap;
+} orc_pens;
+
+static void orc_pens_push(orc_pens *q, const orc_particle *p, const double pos[2], uint32_t pid) {
+  orc_ens_push(&q->e, p, pos);
+  if (q->e.n > q->pcap) {
+    q->pcap = q->e.cap;
+    q->pid = (uint32_t *)realloc(q->pid, sizeof(uint32_t) * (size_t)q->pcap);
+  }
+  q->pid[q->e.n - 1] = pid;
+}
+
+static void orc_pens_erase(orc_pens *q, int i) {
+  memmove(q->pid + i, q->pid + i + 1, sizeof(uint32_t) * (size_t)(q->e.n - i - 1));
+  orc_ens_erase(&q->e, i);
+}
+
+int orc_device_run_philox(const orc_device_run_cfg *cfg, double Vt, double *eqPot,
+                          int *nInit, int *nPartOut, double *potOut, int *exitOut,
+                          int *delOut, int *injOut, double *nrPartLast, double *stateOut,
+                          int maxState) {
+  const orc_device *d = cfg->dev;
+  orc_model *m = cfg->model;
+  const int nc = d->nContacts;
+  const size_t nn = (size_t)d->ext[0] * d->ext[1];
+  const double charge = -ORC_Q;
+  orc_sor sor;
+  orc_sor_init(&sor, d, cfg->accuracyV, cfg->omega);
+  double *pot = (double *)malloc(sizeof(double) * nn);
+  double *ex = (double *)calloc(nn, sizeof(double));
+  double *ey = (double *)calloc(nn, sizeof(double));
+  double *nrPart = (double *)calloc(nn, sizeof(double));
+  double *conc = (double *)calloc(nn, sizeof(double));
+  double *expNr = (double *)calloc(nn, sizeof(double));
+  orc_pens q;
+  memset(&q, 0, sizeof q);
+  for (size_t iy = 0; iy < (size_t)d->ext[1]; iy++)
+    for (size_t ix = 0; ix < (size_t)d->ext[0]; ix++) {
+      size_t c[2] = {ix, iy};
+      if (orc_device_is_ohmic(d, c))
+        expNr[orc_device_idx(d, ix, iy)] = orc_expected_at_contact(d, ix, iy);
+    }
+  orc_device_init_potential(d, pot);
+  orc_sor_equilibrium(&sor, d, pot, 1);
+  if (eqPot)
+    memcpy(eqPot, pot, sizeof(double) * nn);
+  orc_pm_efield(cfg->pm, d, pot, ex, ey);
+  uint32_t nextPid = 0;
+  for (size_t iy = 0; iy < (size_t)d->ext[1]; iy++)
+    for (size_t ix = 0; ix < (size_t)d->ext[0]; ix++) {
+      size_t c[2] = {ix, iy};
+      double dens = d->doping[orc_device_idx(d, ix, iy)];
+      for (int i = 0; i < 2; i++)
+        if (c[i] == 0 || c[i] == (size_t)(d->ext[i] - 1))
+          dens *= 0.5;
+      double nr = dens * d->cellVolume;
+      int cnt = 0;
+      while (nr >= 1) {
+        cnt++;
+        nr -= cfg->nrCarriers;
+      }
+      if (orc_prod_uniform(cfg->seed, ORC_DOM_COUNT, (uint32_t)orc_device_idx(d, ix, iy),
+                           0xFFFFFFFEu, 0) < nr)
+        cnt++;
+      for (int k = 0; k < cnt; k++) {
+        orc_particle p;
+        double pos[2];
+        orc_device_new_philox(m, d, c, cfg->seed, nextPid, 0xFFFFFFFFu, Vt, &p, pos);
+        orc_pens_push(&q, &p, pos, nextPid++);
+      }
+    }
+  if (nInit)
+    *nInit = q.e.n;
+  orc_pm_assign(cfg->pm, q.e.pos, q.e.n, cfg->nrCarriers, d->h, d->ext, nrPart);
+  orc_device_concentration(d, nrPart, conc);
+  for (int s = 0; s < cfg->nSteps; s++) {
+    orc_sor_nonequilibrium(&sor, d, pot, conc, s == 0);
+    if (potOut)
+      memcpy(potOut + (size_t)s * nn, pot, sizeof(double) * nn);
+    orc_pm_efield(cfg->pm, d, pot, ex, ey);
+    int *exits = exitOut ? exitOut + (size_t)s * nc : NULL;
+    if (exits)
+      memset(exits, 0, sizeof(int) * (size_t)nc);
+    for (int i = 0; i < q.e.n; i++) {
+      int contact = -1;
+      if (orc_device_move_particle_philox(m, d, cfg->pm, ex, ey, charge, &q.e.p[i],
+                                          q.e.pos + 2 * i, cfg->tStep, cfg->seed, q.pid[i],
+                                          (uint32_t)s, &contact)) {
+        if (exits && contact >= 0)
+          exits[contact]++;
+        q.e.p[i].region = -9; /* marked: erased below in array order */
+      }
+    }
+    for (int i = 0; i < q.e.n; i++)
+      if (q.e.p[i].region == -9) {
+        orc_pens_erase(&q, i);
+        i--;
+      }
+    int *dels = delOut ? delOut + (size_t)s * nc : NULL;
+    int *injs = injOut ? injOut + (size_t)s * nc : NULL;
+    if (dels)
+      memset(dels, 0, sizeof(int) * (size_t)nc);
+    if (injs)
+      memset(injs, 0, sizeof(int) * (size_t)nc);
+    memset(nrPart, 0, sizeof(double) * nn);
+    for (int i = 0; i < q.e.n; i++) {
+      size_t c[2];
+      orc_device_pos_to_coord(d, q.e.pos + 2 * i, c);
+      if (orc_device_is_ohmic(d, c)) {
+        size_t gi = orc_device_idx(d, c[0], c[1]);
+        if (nrPart[gi] < expNr[gi]) {
+          nrPart[gi] += cfg->nrCarriers;
+        } else {
+          if (dels)
+            dels[orc_device_contact_idx(d, c)]++;
+          orc_pens_erase(&q, i);
+          i--;
+        }
+      }
+    }
+    for (size_t iy = 0; iy < (size_t)d->ext[1]; iy++)
+      for (size_t ix = 0; ix < (size_t)d->ext[0]; ix++) {
+        size_t c[2] = {ix, iy};
+        if (!orc_device_is_ohmic(d, c))
+          continue;
+        size_t gi = orc_device_idx(d, ix, iy);
+        double diff = expNr[gi] - nrPart[gi];
+        while (diff > 0) {
+          orc_particle p;
+          double pos[2];
+          orc_device_new_philox(m, d, c, cfg->seed, nextPid, (uint32_t)s, Vt, &p, pos);
+          orc_pens_push(&q, &p, pos, nextPid++);
+          if (injs)
+            injs[orc_device_contact_idx(d, c)]++;
+          diff -= cfg->nrCarriers;
+        }
+      }
+    memset(nrPart, 0, sizeof(double) * nn);
+    orc_pm_assign(cfg->pm, q.e.pos, q.e.n, cfg->nrCarriers, d->h, d->ext, nrPart);
+    orc_device_concentration(d, nrPart, conc);
+    if (nPartOut)
+      nPartOut[s] = q.e.n;
+  }
+  if (nrPartLast)
+    memcpy(nrPartLast, nrPart, sizeof(double) * nn);
+  if (stateOut) {
+    /* per particle: x, y, kx, ky, kz, E, tau, valley, sub, pid */
+    int ns = q.e.n < maxState ? q.e.n : maxState;
+    for (int i = 0; i < ns; i++) {
+      double *o = stateOut + (size_t)i * 10;
+      o[0] = q.e.pos[2 * i];
+      o[1] = q.e.pos[2 * i + 1];
+      o[2] = q.e.p[i].k[0];
+      o[3] = q.e.p[i].k[1];
+      o[4] = q.e.p[i].k[2];
+      o[5] = q.e.p[i].energy;
+      o[6] = q.e.p[i].tau;
+      o[7] = q.e.p[i].valley;
+      o[8] = q.e.p[i].subValley;
+      o[9] = q.pid[i];
+    }
+  }
+  int nFinal = q.e.n;
+  free(q.e.p);
+  free(q.e.pos);
+  free(q.pid);
+  free(pot);
+  free(ex);
+  free(ey);
+  free(nrPart);
+  free(conc);
+  free(expNr);
+  orc_sor_free(&sor);
+  return nFinal;
+}
+
+int orc_device_run_philox_h(orc_device *d, orc_model *m, int pm, double accuracyV,
+                            double omega, double tStep, int nSteps, uint64_t seed, double Vt,
+                            double *eqPot, int *nInit, int *nPartOut, double *potOut,
+                            int *exitOut, int *delOut, int *injOut, double *nrPartLast,
+                            double *stateOut, int maxState) {
+  orc_device_run_cfg cfg;
+  cfg.dev = d;
+  cfg.model = m;
+  cfg.pm = pm;
+  cfg.accuracyV = accuracyV;
+  cfg.omega = omega;
+  cfg.tStep = tStep;
+  cfg.nSteps = nSteps;
+  cfg.seed = seed;
+  cfg.nrCarriers = 1.0;
+  return orc_device_run_philox(&cfg, Vt, eqPot, nInit, nPartOut, potOut, exitOut, delOut,
+                               injOut, nrPartLast, stateOut, maxState);
+}
+
+/* ======================================================================
  * Handle API for language bindings (tests)
  * ====================================================================== */
 

@@ -156,6 +156,26 @@ void orc_device_contact_rule(const orc_device *d, const double *pos, int n,
                              double nrCarriers, int *keepOut, int *injOut,
                              double *expOut);
 
+/* ---- production twin of the GPU device path (Philox streams) -----------
+ * orc_device_move_particle with the scatter events of the (step, pid)
+ * stream of the GPU kernels, and orc_device_run with Philox creation,
+ * initial-count and move streams (stateOut rows: x, y, k, E, tau, valley,
+ * sub, pid).  Vt: thermal voltage of the particle initialisation. */
+int orc_device_move_particle_philox(const orc_model *m, const orc_device *d, int pm,
+                                    const double *ex, const double *ey, double charge,
+                                    orc_particle *p, double pos[2], double tStep,
+                                    uint64_t seed, uint32_t pid, uint32_t step,
+                                    int *contact);
+int orc_device_run_philox(const orc_device_run_cfg *cfg, double Vt, double *eqPot,
+                          int *nInit, int *nPartOut, double *potOut, int *exitOut,
+                          int *delOut, int *injOut, double *nrPartLast, double *stateOut,
+                          int maxState);
+int orc_device_run_philox_h(orc_device *d, orc_model *m, int pm, double accuracyV,
+                            double omega, double tStep, int nSteps, uint64_t seed, double Vt,
+                            double *eqPot, int *nInit, int *nPartOut, double *potOut,
+                            int *exitOut, int *delOut, int *injOut, double *nrPartLast,
+                            double *stateOut, int maxState);
+
 /* ---- handle API for language bindings (tests) ---------------------------- */
 orc_device *orc_device_new(double maxX, double maxY, double hx, double hy,
                            double T, double epsR, double ni);

@@ -100,6 +100,9 @@ def load():
         "orc_device_run_h": (_I, [vp, vp, _I, _D, _D, _D, _I, C.c_uint64, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p, _I]),
+        "orc_device_run_philox_h": (_I, [vp, vp, _I, _D, _D, _D, _I, C.c_uint64, _D, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _I]),
         "orc_mt64_fill": (None, [C.c_uint64, _PU64, C.c_size_t]),
         "orc_mt64_fill_particles": (None, [C.c_uint64, C.c_uint64, C.c_size_t, _PU64, C.c_size_t, C.c_size_t]),
         "orc_particle_seed": (C.c_uint64, [C.c_uint64, C.c_uint64]),
@@ -475,4 +478,25 @@ class OracleDevice:
         for k in ("exits", "dels", "injs"):
             out[k] = out[k].reshape(nsteps, nc)
         out["state"] = out["state"][:min(n, max_state) * 9].reshape(-1, 9)
+        return out
+
+    def run_philox(self, model, scheme, accuracy, omega, dt, nsteps, seed, Vt, max_state=0):
+        """the production twin of the GPU device run (Philox streams); state
+        rows: x, y, kx, ky, kz, E, tau, valley, sub, pid"""
+        nn, nc = self.n_nodes, len(self.spec["contacts"])
+        out = dict(eq_pot=np.zeros(nn), n_init=np.zeros(1, dtype=np.int32),
+                   n_part=np.zeros(nsteps, dtype=np.int32), pot=np.zeros(nsteps * nn),
+                   exits=np.zeros(nsteps * nc, dtype=np.int32), dels=np.zeros(nsteps * nc, dtype=np.int32),
+                   injs=np.zeros(nsteps * nc, dtype=np.int32), nr_part=np.zeros(nn),
+                   state=np.zeros(max(max_state, 1) * 10))
+        n = self.L.orc_device_run_philox_h(
+            self.h, model.h, scheme, accuracy, omega, dt, nsteps, seed, Vt, _ptr(out["eq_pot"]),
+            _ptr(out["n_init"]), _ptr(out["n_part"]), _ptr(out["pot"]), _ptr(out["exits"]),
+            _ptr(out["dels"]), _ptr(out["injs"]), _ptr(out["nr_part"]), _ptr(out["state"]),
+            max_state)
+        out["n_final"] = n
+        out["pot"] = out["pot"].reshape(nsteps, nn)
+        for k in ("exits", "dels", "injs"):
+            out[k] = out[k].reshape(nsteps, nc)
+        out["state"] = out["state"][:min(n, max_state) * 10].reshape(-1, 10)
         return out

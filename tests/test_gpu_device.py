@@ -254,3 +254,42 @@ def test_graph_launch_equals_eager():
     # the deposit sums in a fixed order except for the few FP64 atomic
     # corrections; trajectories agree to rounding
     assert np.abs(pa["pos"] - pb["pos"]).max() < 1e-15
+
+
+def test_self_consistent_run_matches_oracle_twin():
+    """The whole self-consistent loop (equilibrium, initial particles,
+    Poisson every step, drift/scatter with the reference's boundaries,
+    contact equilibrium in array order, injection, CIC assignment) against
+    the oracle's Philox twin: ensemble counts per step and per contact
+    exact, final ensemble in array order within 1e-9 (the two Poisson
+    solvers agree to their tolerance, not bit for bit)."""
+    emcgpu = _emc()
+    dev = dict(DEVICE_SMALL, width=1e-5)
+    seed, ns, dt = 4242, 60, 1e-15
+    od = O.OracleDevice(dev)
+    M = O.OracleModel(MODELS_DEVICE["device_para"])
+    ref = od.run_philox(M, O.PM_CIC, 1e-14, 1.8, dt, ns, seed, O.thermal_voltage(300.0),
+                        max_state=100000)
+    print("twin: %d particles, %d steps" % (ref["n_init"][0], ns), flush=True)
+    ctx = _ctx(dev=dev, tol=1e-14, max_iter=80)
+    n0 = ctx.device_init(seed)
+    assert n0 == ref["n_init"][0]
+    eq = ctx.device_fields()["pot"]
+    assert np.abs(eq - ref["eq_pot"]).max() < 1e-9
+    out = ctx.device_step(dt, ns, seed=seed)
+    assert (out["n_particles"] == ref["n_part"]).all()
+    assert (out["exits"] == ref["exits"]).all()
+    assert (out["deleted"] == ref["dels"]).all()
+    assert (out["injected"] == ref["injs"]).all()
+    p = ctx.device_get_particles()
+    st = ref["state"]
+    assert p["E"].size == st.shape[0]
+    assert (p["valley"] == st[:, 7]).all() and (p["sub"] == st[:, 8]).all()
+    dpos = np.abs(p["pos"] - st[:, 0:2]).max() / 20e-9
+    dk = (np.linalg.norm(p["k"] - st[:, 2:5], axis=1) / np.linalg.norm(st[:, 2:5], axis=1)).max()
+    dE = (np.abs(p["E"] - st[:, 5]) / st[:, 5]).max()
+    print("final ensemble: |dpos| %.2e, |dk| %.2e, |dE| %.2e" % (dpos, dk, dE), flush=True)
+    assert dpos < 1e-9 and dk < 1e-9 and dE < 1e-9
+    f = ctx.device_fields()
+    assert np.allclose(f["nr_part"], ref["nr_part"], rtol=1e-9, atol=1e-9)
+    ctx.close()
