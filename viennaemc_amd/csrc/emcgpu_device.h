@@ -182,6 +182,8 @@ constexpr int kDevStepThreads = 512;   // particle step: 8 waves per block
 constexpr int kDevStepWaves = kDevStepThreads / 64;
 constexpr uint32_t kHole = 0xFFFFFFFFu;  // valley/subvalley word of a deleted particle
 constexpr int kMaxLevels = 12;
+constexpr int kOwn = 8;  // fine-grid nodes per colour and thread of the Poisson workgroup
+constexpr int kPad = 4 * kDevThreads;  // node padding of the Poisson vectors
 constexpr int kMaxContacts = EMCGPU_MAX_CONTACTS;
 constexpr int kMaxSlots = 1024;        // ohmic nodes
 // dynamic LDS of one workgroup (160 KB per CU on gfx950, less the kernels'
@@ -265,12 +267,27 @@ struct DevFieldArgs {
   double *ex, *ey;             // [V/m]
   R *acc;                      // [node][2] scaled force / hbar
   double acc_scale;            // charge * 1e-24 / hbar
-  float *sig;                  // Newton diagonal shift of every level (level-stacked)
+  // FP64 Newton / conjugate-gradient vectors [nn], FP32 copies of the
+  // residual and diagonal shift for the multigrid sweeps
+  double *r64, *x64, *p64, *q64, *s64;  // [n_pad]
+  float2 *rs0;     // [n_pad] FP32 residual and diagonal shift (0, big at Dirichlet nodes)
+  const float *w0; // [n_pad] 1 at active nodes, 0 at Dirichlet nodes and padding
+  const float *hw0; // [n_pad] control-volume weights of the CG inner product
+  int n_pad;       // nn rounded up to kPad
+  const float *cinv;    // [n][n] inverse of the direct_level operator (row-major)
+  int direct_level;     // coarse level solved exactly (0: none)
+  int max_vcycles;             // V-cycles per solve
+  int max_pcg;                 // conjugate-gradient iterations per Newton step
   int n_levels;
   MGLevel lev[kMaxLevels];
   const DimTab *tab;
+  int n_tab;
   const uint8_t *lmask;        // Dirichlet mask of all levels (level-stacked)
+  int n_coarse;                // nodes of the levels below the fine grid
+  int wave_level;              // first level run by one wave
   int nu_pre, nu_post, nu_coarse;
+  // dynamic LDS layout (float offsets; tab and cmask in bytes)
+  int lds_res, lds_rc, lds_sc, lds_tab, lds_cmask;
   // records
   int *iters_out;
   double *resid_out;
@@ -332,6 +349,8 @@ struct DevContactArgs {
   int *slot_kept, *slot_inj;
   double *corr;            // [node] deposit corrections (FP64 atomics)
   R *dep;
+  double *dpart;           // [n_chunks][node] partial sums of the block grids
+  int rows_per_chunk, n_chunks;
   double *nrpart, *conc;
   const double *cfac;      // conc = nrpart * cfac
   R vt;                    // thermal voltage [V]

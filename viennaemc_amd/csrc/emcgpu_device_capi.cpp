@@ -19,7 +19,7 @@
 
 namespace emcgpu {
 template <typename R>
-hipError_t dev_launch_field(const DevFieldArgs<R> &a, size_t lds, hipStream_t st);
+hipError_t dev_launch_field(const DevFieldArgs<R> *args_dev, size_t lds, hipStream_t st);
 template <typename R>
 hipError_t dev_launch_step(const DevStepArgs<R> &a, int blocks, size_t lds, hipStream_t st);
 template <typename R>
@@ -82,6 +82,10 @@ struct DeviceSim {
   std::vector<uint8_t> lmask;
   size_t mg_lds = 0;
   int mg_nodes = 0;  // nodes of all levels
+  int lds_res = 0, lds_rc = 0, lds_sc = 0, lds_tab = 0, lds_cmask = 0, wave_level = 0;
+  int direct_level = 0;
+  std::vector<float> cinv_t;
+  float *d_cinv_t = nullptr;
   // ---- device memory ------------------------------------------------------
   bool on_gpu = false;
   int *d_cnode = nullptr, *d_cslot = nullptr;
@@ -91,7 +95,12 @@ struct DeviceSim {
   double *d_pot = nullptr, *d_conc = nullptr, *d_nrpart = nullptr, *d_ex = nullptr;
   double *d_ey = nullptr, *d_corr = nullptr;
   void *d_acc = nullptr;
-  float *d_sig = nullptr;
+  double *d_r64 = nullptr, *d_x64 = nullptr, *d_p64 = nullptr, *d_q64 = nullptr;
+  double *d_s64 = nullptr;
+  float2 *d_rs0 = nullptr;
+  float *d_w0 = nullptr, *d_hw0 = nullptr;
+  void *d_fargs = nullptr;  // device copies of the field-kernel arguments [slot]
+  int n_pad = 0;
   int *d_slot_node = nullptr, *d_slot_contact = nullptr, *d_slot_kept = nullptr;
   int *d_slot_inj = nullptr;
   double *d_slot_exp = nullptr;
@@ -115,6 +124,8 @@ struct DeviceSim {
   size_t step_lds = 0;
   bool lds_dep = true;
   void *d_dep = nullptr;
+  double *d_dpart = nullptr;
+  int rows_per_chunk = 1, n_chunks = 1;
   uint32_t *d_seg_cnt = nullptr, *d_seg_off = nullptr, *d_seg_nst = nullptr;
   // per-step history of a call
   int hist_rows = 0;
@@ -167,7 +178,16 @@ void device_sim_destroy(DeviceSim *d) {
     dfreet(d->d_ey);
     dfreet(d->d_corr);
     dfree(d->d_acc);
-    dfreet(d->d_sig);
+    dfreet(d->d_r64);
+    dfreet(d->d_x64);
+    dfreet(d->d_p64);
+    dfreet(d->d_q64);
+    dfreet(d->d_s64);
+    dfreet(d->d_rs0);
+    dfreet(d->d_w0);
+    dfree(d->d_fargs);
+    dfreet(d->d_hw0);
+    dfreet(d->d_cinv_t);
     dfreet(d->d_slot_node);
     dfreet(d->d_slot_contact);
     dfreet(d->d_slot_kept);
@@ -183,6 +203,7 @@ void device_sim_destroy(DeviceSim *d) {
     dfreet(d->d_ex_t);
     dfreet(d->d_ey_t);
     dfree(d->d_dep);
+    dfreet(d->d_dpart);
     dfreet(d->d_seg_cnt);
     dfreet(d->d_seg_off);
     dfreet(d->d_seg_nst);
@@ -508,7 +529,97 @@ emcgpu_status build_multigrid(emcgpu_ctx *c, DeviceSim &d) {
     off += L.nx * L.ny;
   }
   d.mg_nodes = off;
-  d.mg_lds = sizeof(float) * (size_t)(2 * off + d.lev[0].nx * d.lev[0].ny);
+  const int n0 = d.lev[0].nx * d.lev[0].ny, nc = off - n0;
+  d.lds_res = off;
+  d.lds_rc = d.lds_res + n0;
+  d.lds_sc = d.lds_rc + nc;
+  d.lds_tab = (int)(((size_t)(d.lds_sc + nc) * sizeof(float) + 15) & ~(size_t)15);
+  d.lds_cmask = d.lds_tab + (int)(d.tab.size() * sizeof(DimTab));
+  d.mg_lds = ((size_t)d.lds_cmask + nc + 15) & ~(size_t)15;
+  d.wave_level = nl;
+  for (int l = 1; l < nl; l++)
+    if (d.lev[l].nx * d.lev[l].ny <= 256) {
+      d.wave_level = l;
+      break;
+    }
+  // exact inverse of the operator of the first coarse level with at most 256
+  // nodes (rows / columns of Dirichlet nodes zero), row-major, FP32
+  d.direct_level = 0;
+  d.cinv_t.clear();
+  if (d.wave_level < nl) {
+    const int lc = d.wave_level;
+    const MGLevel &C = d.lev[lc];
+    const int n = C.nx * C.ny;
+    const uint8_t *msk = d.lmask.data() + C.off;
+    std::vector<int> act;
+    std::vector<int> pos(n, -1);
+    for (int k = 0; k < n; k++)
+      if (!msk[k]) {
+        pos[k] = (int)act.size();
+        act.push_back(k);
+      }
+    const int m = (int)act.size();
+    std::vector<double> A((size_t)m * m, 0.0), Inv((size_t)m * m, 0.0);
+    for (int r = 0; r < m; r++) {
+      const int k = act[r], j = k / C.nx, i = k - j * C.nx;
+      const DimTab &tx = d.tab[C.tx + i], &ty = d.tab[C.ty + j];
+      const double cl = d.hfx * ty.w * tx.inv_l, cr = d.hfx * ty.w * tx.inv_r;
+      const double cd = d.hfy * tx.w * ty.inv_l, cu = d.hfy * tx.w * ty.inv_r;
+      const int il = i > 0 ? k - 1 : k + 1, ir = i < C.nx - 1 ? k + 1 : k - 1;
+      const int jd = j > 0 ? k - C.nx : k + C.nx, ju = j < C.ny - 1 ? k + C.nx : k - C.nx;
+      A[(size_t)r * m + r] += cl + cr + cd + cu;
+      const int nbk[4] = {il, ir, jd, ju};
+      const double cf[4] = {cl, cr, cd, cu};
+      for (int q = 0; q < 4; q++)
+        if (pos[nbk[q]] >= 0)
+          A[(size_t)r * m + pos[nbk[q]]] -= cf[q];
+      Inv[(size_t)r * m + r] = 1.0;
+    }
+    // Gauss-Jordan elimination with partial pivoting
+    bool ok = m > 0;
+    for (int col = 0; col < m && ok; col++) {
+      int piv = col;
+      for (int r = col + 1; r < m; r++)
+        if (std::fabs(A[(size_t)r * m + col]) > std::fabs(A[(size_t)piv * m + col]))
+          piv = r;
+      if (std::fabs(A[(size_t)piv * m + col]) < 1e-300) {
+        ok = false;
+        break;
+      }
+      if (piv != col)
+        for (int q = 0; q < m; q++) {
+          std::swap(A[(size_t)piv * m + q], A[(size_t)col * m + q]);
+          std::swap(Inv[(size_t)piv * m + q], Inv[(size_t)col * m + q]);
+        }
+      const double inv = 1.0 / A[(size_t)col * m + col];
+      for (int q = 0; q < m; q++) {
+        A[(size_t)col * m + q] *= inv;
+        Inv[(size_t)col * m + q] *= inv;
+      }
+      for (int r = 0; r < m; r++) {
+        if (r == col)
+          continue;
+        const double f = A[(size_t)r * m + col];
+        if (f == 0.0)
+          continue;
+        for (int q = 0; q < m; q++) {
+          A[(size_t)r * m + q] -= f * A[(size_t)col * m + q];
+          Inv[(size_t)r * m + q] -= f * Inv[(size_t)col * m + q];
+        }
+      }
+    }
+    if (ok) {
+      d.cinv_t.assign((size_t)n * n, 0.0f);
+      for (int r = 0; r < m; r++)
+        for (int q = 0; q < m; q++)
+          d.cinv_t[(size_t)act[r] * n + act[q]] = (float)Inv[(size_t)r * m + q];
+      d.direct_level = lc;
+    }
+  }
+  const int per_colour = ((d.ext[0] + 1) / 2) * d.ext[1];
+  if (per_colour > kOwn * kDevThreads)
+    return fail(c, EMCGPU_ERR_ARG, "a %d x %d grid exceeds the single-workgroup Poisson solver",
+                d.ext[0], d.ext[1]);
   if (d.mg_lds > (size_t)kDevMaxLds)
     return fail(c, EMCGPU_ERR_ARG,
                 "a %d x %d grid exceeds the single-workgroup Poisson solver (%zu bytes of LDS > "
@@ -578,8 +689,33 @@ emcgpu_status upload_device(emcgpu_ctx *c, DeviceSim &d) {
   ST(dalloc(c, d.d_ex_t, 2 * nn));
   ST(dalloc(c, d.d_ey_t, nn));
   ST(dallocv(c, d.d_acc, 2 * nn * rs));
-  ST(dalloc(c, d.d_sig, d.mg_nodes));
+  d.n_pad = (int)((nn + kPad - 1) / kPad * kPad);
+  const size_t np = d.n_pad;
+  ST(dalloc(c, d.d_r64, np));
+  ST(dalloc(c, d.d_x64, np));
+  ST(dalloc(c, d.d_p64, np));
+  ST(dalloc(c, d.d_q64, np));
+  ST(dalloc(c, d.d_s64, np));
+  ST(dalloc(c, d.d_rs0, np));
+  {
+    std::vector<float> w(np, 0.0f), hw(np, 0.0f);
+    std::vector<float2> rs(np, make_float2(0.0f, 1e30f));
+    for (size_t k = 0; k < nn; k++) {
+      w[k] = d.dmask[k] ? 0.0f : 1.0f;
+      const int ix = (int)(k % d.ext[0]), iy = (int)(k / d.ext[0]);
+      hw[k] = (ix == 0 || ix == d.ext[0] - 1 ? 0.5f : 1.0f) *
+              (iy == 0 || iy == d.ext[1] - 1 ? 0.5f : 1.0f);
+    }
+    ST(dput(c, d.d_w0, w));
+    ST(dput(c, d.d_hw0, hw));
+    ST(dput(c, d.d_cinv_t, d.cinv_t));
+    HIPCHK(c, hipMemcpy(d.d_rs0, rs.data(), np * sizeof(float2), hipMemcpyHostToDevice));
+    std::vector<double> z(np, 0.0);
+    for (double *v : {d.d_r64, d.d_x64, d.d_p64, d.d_q64, d.d_s64})
+      HIPCHK(c, hipMemcpy(v, z.data(), np * sizeof(double), hipMemcpyHostToDevice));
+  }
   ST(dalloc(c, d.d_state, 1));
+  ST(dallocv(c, d.d_fargs, 4 * std::max(sizeof(DevFieldArgs<float>), sizeof(DevFieldArgs<double>))));
   ST(dalloc(c, d.d_iters, 1));
   ST(dalloc(c, d.d_resid, 1));
   HIPCHK(c, hipMemset(d.d_pot, 0, sizeof(double) * nn));
@@ -604,6 +740,9 @@ emcgpu_status upload_device(emcgpu_ctx *c, DeviceSim &d) {
   d.n_seg = d.blocks * kDevStepWaves;
   const size_t dep_rows = d.lds_dep ? (size_t)d.blocks : 1;
   ST(dallocv(c, d.d_dep, dep_rows * nn * rs));
+  d.rows_per_chunk = 16;
+  d.n_chunks = (int)((dep_rows + d.rows_per_chunk - 1) / d.rows_per_chunk);
+  ST(dalloc(c, d.d_dpart, (size_t)d.n_chunks * nn));
   HIPCHK(c, hipMemset(d.d_dep, 0, dep_rows * nn * rs));
   ST(dalloc(c, d.d_seg_cnt, ns * d.n_seg));
   ST(dalloc(c, d.d_seg_off, ns * d.n_seg));
@@ -778,15 +917,35 @@ DevFieldArgs<R> field_args(const emcgpu_ctx *c, const DeviceSim &d) {
   a.ey = d.d_ey;
   a.acc = (R *)d.d_acc;
   a.acc_scale = kAccScale;
-  a.sig = d.d_sig;
+  a.r64 = d.d_r64;
+  a.x64 = d.d_x64;
+  a.p64 = d.d_p64;
+  a.q64 = d.d_q64;
+  a.s64 = d.d_s64;
+  a.rs0 = d.d_rs0;
+  a.w0 = d.d_w0;
+  a.hw0 = d.d_hw0;
+  a.cinv = d.d_cinv_t;
+  a.direct_level = d.direct_level;
+  a.n_pad = d.n_pad;
+  a.max_vcycles = 400;
+  a.max_pcg = 60;
   a.n_levels = (int)d.lev.size();
   for (size_t l = 0; l < d.lev.size(); l++)
     a.lev[l] = d.lev[l];
   a.tab = d.d_tab;
+  a.n_tab = (int)d.tab.size();
   a.lmask = d.d_lmask;
+  a.n_coarse = d.mg_nodes - d.nn;
+  a.wave_level = d.wave_level;
   a.nu_pre = 2;
   a.nu_post = 2;
-  a.nu_coarse = 16;
+  a.nu_coarse = 4;
+  a.lds_res = d.lds_res;
+  a.lds_rc = d.lds_rc;
+  a.lds_sc = d.lds_sc;
+  a.lds_tab = d.lds_tab;
+  a.lds_cmask = d.lds_cmask;
   a.state = d.d_state;
   a.hist = hist_of(d);
   return a;
@@ -841,6 +1000,9 @@ DevContactArgs<R> contact_args(const emcgpu_ctx *c, const DeviceSim &d) {
   a.slot_inj = d.d_slot_inj;
   a.corr = d.d_corr;
   a.dep = (R *)d.d_dep;
+  a.dpart = d.d_dpart;
+  a.rows_per_chunk = d.rows_per_chunk;
+  a.n_chunks = d.n_chunks;
   a.nrpart = d.d_nrpart;
   a.conc = d.d_conc;
   a.cfac = d.d_cfac;
@@ -868,16 +1030,34 @@ emcgpu_status need_model(emcgpu_ctx *c, double dt_s) {
   return upload_model(c, dt_s);
 }
 
-// one Poisson / E-field launch on the given potential and outputs
+// device slot of the field-kernel arguments: 0 steps, 1 the first step
+// (contacts reset), 2 the other calls
 template <typename R>
-emcgpu_status run_field(emcgpu_ctx *c, DeviceSim &d, DevFieldArgs<R> a) {
-  HIPCHK(c, dev_launch_field<R>(a, d.mg_lds, c->stream));
+DevFieldArgs<R> *fargs_slot(const DeviceSim &d, int slot) {
+  return (DevFieldArgs<R> *)d.d_fargs + slot;
+}
+
+// writes the arguments of a field-kernel slot (synchronous: never inside a
+// captured graph, whose launches only reference the slot)
+template <typename R>
+emcgpu_status put_fargs(emcgpu_ctx *c, DeviceSim &d, int slot, const DevFieldArgs<R> &a) {
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(fargs_slot<R>(d, slot), &a, sizeof(a), hipMemcpyHostToDevice));
   return EMCGPU_OK;
 }
 
-// the kernel sequence of one step (emcSimulation.hpp:164-193)
+// one Poisson / E-field launch on the given potential and outputs
 template <typename R>
-emcgpu_status launch_step(emcgpu_ctx *c, DeviceSim &d, bool reset_bc) {
+emcgpu_status run_field(emcgpu_ctx *c, DeviceSim &d, DevFieldArgs<R> a) {
+  ST(put_fargs<R>(c, d, 2, a));
+  HIPCHK(c, dev_launch_field<R>(fargs_slot<R>(d, 2), d.mg_lds, c->stream));
+  return EMCGPU_OK;
+}
+
+// arguments of the per-step Poisson / E-field launch (slot 1: the first
+// step, which sets the contacts' applied voltages)
+template <typename R>
+DevFieldArgs<R> step_field_args(const emcgpu_ctx *c, const DeviceSim &d, bool reset_bc) {
   DevFieldArgs<R> f = field_args<R>(c, d);
   f.equilibrium = 0;
   f.reset_bc = reset_bc ? 1 : 0;
@@ -885,7 +1065,13 @@ emcgpu_status launch_step(emcgpu_ctx *c, DeviceSim &d, bool reset_bc) {
   f.do_efield = 1;
   f.record = 1;
   f.nrpart = d.d_nrpart;
-  HIPCHK(c, dev_launch_field<R>(f, d.mg_lds, c->stream));
+  return f;
+}
+
+// the kernel sequence of one step (emcSimulation.hpp:164-193)
+template <typename R>
+emcgpu_status launch_step(emcgpu_ctx *c, DeviceSim &d, bool reset_bc) {
+  HIPCHK(c, dev_launch_field<R>(fargs_slot<R>(d, reset_bc ? 1 : 0), d.mg_lds, c->stream));
   HIPCHK(c, dev_launch_step<R>(step_args<R>(c, d), d.blocks, d.step_lds, c->stream));
   const DevContactArgs<R> ca = contact_args<R>(c, d);
   if (ca.n_slots > 0)
@@ -912,6 +1098,8 @@ emcgpu_status run_device_steps(emcgpu_ctx *c, DeviceSim &d, const emcgpu_device_
   HIPCHK(c, dev_launch_zero_i(d.d_hdels, (long long)s->n_steps * nc, c->stream));
   HIPCHK(c, dev_launch_zero_i(d.d_hinjs, (long long)s->n_steps * nc, c->stream));
   HIPCHK(c, dev_launch_mark(d.d_state, c->stream));
+  ST(put_fargs<R>(c, d, 0, step_field_args<R>(c, d, false)));
+  ST(put_fargs<R>(c, d, 1, step_field_args<R>(c, d, true)));
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
   int done = 0;
   while (done < s->n_steps) {

@@ -360,19 +360,45 @@ __global__ __launch_bounds__(kDevStepThreads) void k_dev_step(DevStepArgs<R> a) 
   const InstDev<R> *P = a.inst;
   const uint64_t lt = (1ull << lane) - 1ull;
   uint32_t nst = 0;
+  // the state of the wave's next chunk is loaded while the current one moves
+  struct Chunk {
+    R x, y, kx, ky, kz, E, tau;
+    uint32_t vs, pid;
+  };
+  auto load = [&](long long i, Chunk &q) {
+    if (i < hi) {
+      q.vs = a.p.vs[i];
+      q.x = a.p.x[i];
+      q.y = a.p.y[i];
+      q.kx = a.p.kx[i];
+      q.ky = a.p.ky[i];
+      q.kz = a.p.kz[i];
+      q.E = a.p.E[i];
+      q.tau = a.p.tau[i];
+      q.pid = a.p.pid[i];
+    } else {
+      q.vs = kHole;
+    }
+  };
+  Chunk nxt;
+  if (lo < hi)
+    load(lo + lane, nxt);
   for (long long base = lo; base < hi; base += 64) {
     const long long i = base + lane;
-    const uint32_t vs = (i < hi) ? a.p.vs[i] : kHole;
+    const Chunk cur = nxt;
+    if (base + 64 < hi)
+      load(base + 64 + lane, nxt);
+    const uint32_t vs = cur.vs;
     int slot = -1;
     if (vs != kHole) {
-      R x = a.p.x[i], y = a.p.y[i];
+      R x = cur.x, y = cur.y;
       bool removed = false;
       if (a.move) {
-        R kx = a.p.kx[i], ky = a.p.ky[i], kz = a.p.kz[i], E = a.p.E[i], tau = a.p.tau[i];
+        R kx = cur.kx, ky = cur.ky, kz = cur.kz, E = cur.E, tau = cur.tau;
         int v = (int)(vs >> 16), sub = (int)(vs & 0xffffu);
         StepRng<R, 0> rng;
         rng.step = stw;
-        rng.pid = a.p.pid[i];
+        rng.pid = cur.pid;
         rng.inst = kDomMove;
         rng.k0 = a.key0;
         rng.k1 = a.key1;
@@ -467,6 +493,8 @@ __global__ __launch_bounds__(256) void k_dev_contact_scan(DevContactArgs<R> a) {
     off[b] = run;
     run += cnt[b];
   }
+  __shared__ int last;
+  __shared__ long long part[4];
   if (tid == 0) {
     const long long total = sc[255];
     const long long row = a.state->step - a.state->hist_base;
@@ -483,19 +511,29 @@ __global__ __launch_bounds__(256) void k_dev_contact_scan(DevContactArgs<R> a) {
     if (a.hist.injs && inj > 0)
       atomicAdd(&a.hist.injs[row * a.n_contacts + c], (int)inj);
     __threadfence();
-    const unsigned int t = atomicAdd(&a.state->ticket, 1u);
-    if (t == (unsigned int)a.n_slots - 1) {
-      __threadfence();
-      long long sum = 0;
-      for (int s = 0; s < a.n_slots; ++s)
-        sum += __hip_atomic_load(&a.slot_inj[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (a.state->n_total + sum > a.state->capacity) {
-        a.state->overflow = 1;
-        sum = a.state->capacity - a.state->n_total;
-      }
-      a.state->inj_total = sum;
-      a.state->ticket = 0u;
+    last = atomicAdd(&a.state->ticket, 1u) == (unsigned int)a.n_slots - 1;
+  }
+  __syncthreads();
+  if (!last)
+    return;
+  // the last block: the step's injections over all ohmic nodes
+  __threadfence();
+  long long v = 0;
+  for (int q = tid; q < a.n_slots; q += 256)
+    v += __hip_atomic_load(&a.slot_inj[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int off = 32; off > 0; off >>= 1)
+    v += __shfl_xor(v, off, 64);
+  if ((tid & 63) == 0)
+    part[tid >> 6] = v;
+  __syncthreads();
+  if (tid == 0) {
+    long long sum = part[0] + part[1] + part[2] + part[3];
+    if (a.state->n_total + sum > a.state->capacity) {
+      a.state->overflow = 1;
+      sum = a.state->capacity - a.state->n_total;
     }
+    a.state->inj_total = sum;
+    a.state->ticket = 0u;
   }
 }
 
@@ -531,12 +569,32 @@ __global__ __launch_bounds__(256) void k_dev_resolve(DevContactArgs<R> a) {
   const long long t = ((long long)blockIdx.x - nA) * 256 + tid;
   if (((long long)blockIdx.x - nA) * 256 >= T)
     return;
-  if (tid == 0) {
-    int s = 0;
-    pre[0] = 0;
-    for (int q = 0; q < a.n_slots; ++q) {
-      s += a.slot_inj[q];
-      pre[q + 1] = s;
+  {  // prefix of the injections over the ohmic nodes (4 nodes per thread)
+    __shared__ int ws[256];
+    int v[4], run = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = 4 * tid + u;
+      v[u] = q < a.n_slots ? a.slot_inj[q] : 0;
+      run += v[u];
+    }
+    ws[tid] = run;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+      const int o = tid >= d ? ws[tid - d] : 0;
+      __syncthreads();
+      ws[tid] += o;
+      __syncthreads();
+    }
+    int acc = ws[tid] - run;
+    if (tid == 0)
+      pre[0] = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = 4 * tid + u;
+      acc += v[u];
+      if (q < a.n_slots)
+        pre[q + 1] = acc;
     }
   }
   __syncthreads();
@@ -573,18 +631,33 @@ __global__ __launch_bounds__(256) void k_dev_resolve(DevContactArgs<R> a) {
 // carriers per node = FP64 sum of the block grids (fixed order) + the
 // corrections; concentration (updateCurrentParticleConcentrations,
 // emcSimulationResults.hpp:98-116); commit of the step's counters
+// first pass: chunk c of the block grids' rows summed per node (FP64)
+template <typename R>
+__global__ __launch_bounds__(256) void k_dev_reduce_rows(DevContactArgs<R> a) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  const int nn = a.g.nn;
+  if (k >= nn)
+    return;
+  const int b0 = blockIdx.y * a.rows_per_chunk;
+  const int b1 = b0 + a.rows_per_chunk < a.n_blocks_dep ? b0 + a.rows_per_chunk : a.n_blocks_dep;
+  double s = 0.0;
+  for (int b = b0; b < b1; ++b)
+    s += (double)a.dep[(long long)b * nn + k];
+  a.dpart[(long long)blockIdx.y * nn + k] = s;
+  if (a.n_blocks_dep == 1)  // the atomic deposit accumulates in row 0
+    a.dep[k] = R(0);
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_dev_reduce(DevContactArgs<R> a, int commit) {
   const int k = blockIdx.x * 256 + threadIdx.x;
   const int nn = a.g.nn;
   if (k < nn) {
     double s = 0.0;
-    for (int b = 0; b < a.n_blocks_dep; ++b)
-      s += (double)a.dep[(long long)b * nn + k];
+    for (int c = 0; c < a.n_chunks; ++c)
+      s += a.dpart[(long long)c * nn + k];
     s += a.corr[k];
     a.corr[k] = 0.0;
-    if (a.n_blocks_dep == 1)  // the atomic deposit accumulates in row 0
-      a.dep[k] = R(0);
     a.nrpart[k] = s;
     a.conc[k] = s * a.cfac[k];
   }
@@ -802,138 +875,152 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
   return s;
 }
 
-struct MGView {
-  float *d, *r, *res;
-  float *sig;  // level-stacked diagonal shift (global)
+// ---------------------------------------------------------------------------
+// multigrid V-cycle (FP32, LDS) as the preconditioner of an FP64 conjugate
+// gradient solve of the Newton system
+// ---------------------------------------------------------------------------
+// level-0 nodes a thread owns: colour c, slot m -> node index (-1: none,
+// -2 - k: Dirichlet node k); kOwn per colour (emcgpu_device.h)
+
+// wave-synchronous step: the wave's LDS writes are visible to its lanes
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct BlockTeam {
+  static constexpr int size = kDevThreads;
+  __device__ __forceinline__ int rank() const { return (int)threadIdx.x; }
+  __device__ __forceinline__ void sync() const { __syncthreads(); }
+};
+struct WaveTeam {  // wave 0 alone
+  static constexpr int size = 64;
+  __device__ __forceinline__ int rank() const { return (int)threadIdx.x; }
+  __device__ __forceinline__ void sync() const { wave_sync(); }
+};
+
+// the coarse levels' data in LDS (level l >= 1 at offset lev[l].off - n0)
+struct MGShared {
+  float *d;        // corrections of every level (level-stacked)
+  float *res;      // level-0 residual (restriction input), level-0 shift scratch
+  float *rc, *sc;  // right-hand sides and diagonal shifts of the coarse levels
+  const DimTab *tab;
+  const uint8_t *cmask;
+  int n0;
   float hfx, hfy;
 };
 
-// red-black Gauss-Seidel sweep of colour c on level l of A d = r
-// (A: 5-point finite-volume operator, mirror neighbours at boundaries)
-template <typename R>
-__device__ __forceinline__ void mg_sweep(const DevFieldArgs<R> &a, const MGView &m, int l, int c) {
+// diagonal shift of a Dirichlet node: its correction vanishes in a sweep
+// without a test
+constexpr float kBigShift = 1e30f;
+
+// one red-black Gauss-Seidel colour on coarse level l of A d = r
+template <typename R, typename T>
+__device__ __forceinline__ void mg_sweep_c(const DevFieldArgs<R> &a, const MGShared &m, int l,
+                                           int c, T team) {
   const MGLevel L = a.lev[l];
   float *d = m.d + L.off;
-  const float *r = m.r + L.off;
-  const uint8_t *mask = a.lmask + L.off;
-  const DimTab *TX = a.tab + L.tx, *TY = a.tab + L.ty;
+  const float *r = m.rc + (L.off - m.n0);
+  const float *sg = m.sc + (L.off - m.n0);  // kBigShift at Dirichlet nodes
+  const DimTab *TX = m.tab + L.tx, *TY = m.tab + L.ty;
   const int hr = (L.nx + 1) >> 1;
-  for (int t = threadIdx.x; t < hr * L.ny; t += kDevThreads) {
+  for (int t = team.rank(); t < hr * L.ny; t += T::size) {
     const int j = t / hr;
     const int i = 2 * (t - j * hr) + ((j + c) & 1);
     if (i >= L.nx)
       continue;
     const int k = j * L.nx + i;
-    if (mask[k])
-      continue;
     const DimTab tx = TX[i], ty = TY[j];
     const float cl = m.hfx * ty.w * tx.inv_l, cr = m.hfx * ty.w * tx.inv_r;
     const float cd = m.hfy * tx.w * ty.inv_l, cu = m.hfy * tx.w * ty.inv_r;
     const int il = i > 0 ? k - 1 : k + 1, ir = i < L.nx - 1 ? k + 1 : k - 1;
     const int jd = j > 0 ? k - L.nx : k + L.nx, ju = j < L.ny - 1 ? k + L.nx : k - L.nx;
-    const float diag = cl + cr + cd + cu + m.sig[L.off + k];
-    d[k] = (r[k] + cl * d[il] + cr * d[ir] + cd * d[jd] + cu * d[ju]) / diag;
+    d[k] = (r[k] + cl * d[il] + cr * d[ir] + cd * d[jd] + cu * d[ju]) / (cl + cr + cd + cu + sg[k]);
   }
-  __syncthreads();
+  team.sync();
 }
 
-// residual of level l into m.res, then full-weighting restriction (the
-// transpose of bilinear prolongation, mirror images folded in) to level l+1
-template <typename R>
-__device__ __forceinline__ void mg_restrict(const DevFieldArgs<R> &a, const MGView &m, int l) {
+// full weighting (the transpose of bilinear prolongation, mirror images
+// folded in) of a fine-level field src (level l, dense nx * ny) into the
+// coarse level l+1 array dst; masked coarse nodes get 0; zero_d clears the
+// coarse correction
+template <typename R, typename T>
+__device__ __forceinline__ void mg_restrict_to(const DevFieldArgs<R> &a, const MGShared &m, int l,
+                                               const float *src, float *dst, bool zero_d,
+                                               T team, bool keep_masked = false) {
   const MGLevel L = a.lev[l], C = a.lev[l + 1];
-  const float *d = m.d + L.off;
-  const float *r = m.r + L.off;
-  const uint8_t *mask = a.lmask + L.off;
-  const DimTab *TX = a.tab + L.tx, *TY = a.tab + L.ty;
-  for (int k = threadIdx.x; k < L.nx * L.ny; k += kDevThreads) {
-    const int j = k / L.nx, i = k - j * L.nx;
-    if (mask[k]) {
-      m.res[k] = 0.0f;
-      continue;
-    }
-    const DimTab tx = TX[i], ty = TY[j];
-    const float cl = m.hfx * ty.w * tx.inv_l, cr = m.hfx * ty.w * tx.inv_r;
-    const float cd = m.hfy * tx.w * ty.inv_l, cu = m.hfy * tx.w * ty.inv_r;
-    const int il = i > 0 ? k - 1 : k + 1, ir = i < L.nx - 1 ? k + 1 : k - 1;
-    const int jd = j > 0 ? k - L.nx : k + L.nx, ju = j < L.ny - 1 ? k + L.nx : k - L.nx;
-    const float diag = cl + cr + cd + cu + m.sig[L.off + k];
-    m.res[k] = r[k] - (diag * d[k] - (cl * d[il] + cr * d[ir] + cd * d[jd] + cu * d[ju]));
-  }
-  __syncthreads();
-  float *rc = m.r + C.off;
+  const uint8_t *cmask = m.cmask + (C.off - m.n0);
+  const DimTab *CX = m.tab + C.tx, *CY = m.tab + C.ty;
   float *dc = m.d + C.off;
-  const uint8_t *cmask = a.lmask + C.off;
-  const DimTab *CX = a.tab + C.tx, *CY = a.tab + C.ty;
-  for (int K = threadIdx.x; K < C.nx * C.ny; K += kDevThreads) {
-    dc[K] = 0.0f;
-    const int J = K / C.nx, I = K - J * C.nx;
+  for (int K = team.rank(); K < C.nx * C.ny; K += T::size) {
+    if (zero_d)
+      dc[K] = 0.0f;
     if (cmask[K]) {
-      rc[K] = 0.0f;
+      if (!keep_masked)
+        dst[K] = 0.0f;
       continue;
     }
+    const int J = K / C.nx, I = K - J * C.nx;
     const DimTab cx = CX[I], cy = CY[J];
     const float wx[3] = {cx.wl, 1.0f, cx.wr}, wy[3] = {cy.wl, 1.0f, cy.wr};
     float s = 0.0f;
+#pragma unroll
     for (int b = 0; b < 3; ++b) {
       if (wy[b] == 0.0f)
         continue;
       const int jj = cy.f + b - 1;
+#pragma unroll
       for (int q = 0; q < 3; ++q) {
         if (wx[q] == 0.0f)
           continue;
-        s += wx[q] * wy[b] * m.res[jj * L.nx + cx.f + q - 1];
+        s += wx[q] * wy[b] * src[jj * L.nx + cx.f + q - 1];
       }
     }
-    rc[K] = s;
+    dst[K] = s;
   }
-  __syncthreads();
+  team.sync();
 }
 
-// the Newton diagonal shift of every coarse level: full weighting of the
-// finer level's (the lumped Galerkin product P^T diag(sigma) P)
-template <typename R>
-__device__ __forceinline__ void mg_restrict_sig(const DevFieldArgs<R> &a, const MGView &m) {
-  for (int l = 0; l + 1 < a.n_levels; ++l) {
-    const MGLevel L = a.lev[l], C = a.lev[l + 1];
-    const float *sf = m.sig + L.off;
-    float *sc = m.sig + C.off;
-    const uint8_t *cmask = a.lmask + C.off;
-    const DimTab *CX = a.tab + C.tx, *CY = a.tab + C.ty;
-    for (int K = threadIdx.x; K < C.nx * C.ny; K += kDevThreads) {
-      const int J = K / C.nx, I = K - J * C.nx;
-      if (cmask[K]) {
-        sc[K] = 0.0f;
-        continue;
-      }
-      const DimTab cx = CX[I], cy = CY[J];
-      const float wx[3] = {cx.wl, 1.0f, cx.wr}, wy[3] = {cy.wl, 1.0f, cy.wr};
-      float s = 0.0f;
-      for (int b = 0; b < 3; ++b) {
-        if (wy[b] == 0.0f)
-          continue;
-        const int jj = cy.f + b - 1;
-        for (int q = 0; q < 3; ++q) {
-          if (wx[q] == 0.0f)
-            continue;
-          s += wx[q] * wy[b] * sf[jj * L.nx + cx.f + q - 1];
-        }
-      }
-      sc[K] = s;
+// residual of coarse level l into res (dense), then its restriction
+template <typename R, typename T>
+__device__ __forceinline__ void mg_residual_c(const DevFieldArgs<R> &a, const MGShared &m, int l,
+                                              float *res, T team) {
+  const MGLevel L = a.lev[l];
+  const float *d = m.d + L.off;
+  const float *r = m.rc + (L.off - m.n0);
+  const float *sg = m.sc + (L.off - m.n0);
+  const uint8_t *mask = m.cmask + (L.off - m.n0);
+  const DimTab *TX = m.tab + L.tx, *TY = m.tab + L.ty;
+  for (int k = team.rank(); k < L.nx * L.ny; k += T::size) {
+    if (mask[k]) {
+      res[k] = 0.0f;
+      continue;
     }
-    __syncthreads();
+    const int j = k / L.nx, i = k - j * L.nx;
+    const DimTab tx = TX[i], ty = TY[j];
+    const float cl = m.hfx * ty.w * tx.inv_l, cr = m.hfx * ty.w * tx.inv_r;
+    const float cd = m.hfy * tx.w * ty.inv_l, cu = m.hfy * tx.w * ty.inv_r;
+    const int il = i > 0 ? k - 1 : k + 1, ir = i < L.nx - 1 ? k + 1 : k - 1;
+    const int jd = j > 0 ? k - L.nx : k + L.nx, ju = j < L.ny - 1 ? k + L.nx : k - L.nx;
+    res[k] = r[k] - ((cl + cr + cd + cu + sg[k]) * d[k] -
+                     (cl * d[il] + cr * d[ir] + cd * d[jd] + cu * d[ju]));
   }
+  team.sync();
+  mg_restrict_to(a, m, l, res, m.rc + (a.lev[l + 1].off - m.n0), true, team);
 }
 
-// d_l += P d_{l+1} (bilinear in the fine index positions)
-template <typename R>
-__device__ __forceinline__ void mg_prolong(const DevFieldArgs<R> &a, const MGView &m, int l) {
+// d_l += P d_{l+1} on coarse level l >= 0 (bilinear in the fine positions);
+// level 0 skips its Dirichlet nodes through dmask
+template <typename R, typename T>
+__device__ __forceinline__ void mg_prolong(const DevFieldArgs<R> &a, const MGShared &m, int l,
+                                           T team) {
   const MGLevel L = a.lev[l], C = a.lev[l + 1];
   float *d = m.d + L.off;
   const float *dc = m.d + C.off;
-  const uint8_t *mask = a.lmask + L.off;
-  const DimTab *TX = a.tab + L.tx, *TY = a.tab + L.ty;
-  for (int k = threadIdx.x; k < L.nx * L.ny; k += kDevThreads) {
+  const uint8_t *mask = l == 0 ? a.dmask : m.cmask + (L.off - m.n0);
+  const DimTab *TX = m.tab + L.tx, *TY = m.tab + L.ty;
+  for (int k = team.rank(); k < L.nx * L.ny; k += T::size) {
     if (mask[k])
       continue;
     const int j = k / L.nx, i = k - j * L.nx;
@@ -950,32 +1037,207 @@ __device__ __forceinline__ void mg_prolong(const DevFieldArgs<R> &a, const MGVie
     }
     d[k] += v;
   }
+  team.sync();
+}
+
+// the levels from lo down to the coarsest and back (V-cycle below level lo)
+template <typename R, typename T>
+__device__ __forceinline__ void mg_lower(const DevFieldArgs<R> &a, const MGShared &m, int lo,
+                                         T team) {
+  const int nl = a.n_levels;
+  for (int l = lo; l < nl - 1; ++l) {
+    for (int s = 0; s < a.nu_pre; ++s) {
+      mg_sweep_c(a, m, l, 0, team);
+      mg_sweep_c(a, m, l, 1, team);
+    }
+    mg_residual_c(a, m, l, m.res, team);
+  }
+  for (int s = 0; s < a.nu_coarse; ++s) {
+    mg_sweep_c(a, m, nl - 1, 0, team);
+    mg_sweep_c(a, m, nl - 1, 1, team);
+  }
+  for (int l = nl - 2; l >= lo; --l) {
+    mg_prolong(a, m, l, team);
+    for (int s = 0; s < a.nu_post; ++s) {
+      mg_sweep_c(a, m, l, 1, team);
+      mg_sweep_c(a, m, l, 0, team);
+    }
+  }
+}
+
+__device__ __forceinline__ void nb0(int k, int nx, int ny, int &il, int &ir, int &jd, int &ju) {
+  const int j = k / nx, i = k - j * nx;
+  il = i > 0 ? k - 1 : k + 1;
+  ir = i < nx - 1 ? k + 1 : k - 1;
+  jd = j > 0 ? k - nx : k + nx;
+  ju = j < ny - 1 ? k + nx : k - nx;
+}
+
+// fine-grid node of sweep colour c, slot q of this thread (sweep index
+// tid + q * kDevThreads), -1 if none
+__device__ __forceinline__ int fine_node(int c, int q, int nx, int ny, int hr) {
+  const int t = (int)threadIdx.x + q * kDevThreads;
+  const int j = t / hr;
+  const int i = 2 * (t - j * hr) + ((j + c) & 1);
+  return (j < ny && i < nx) ? j * nx + i : -1;
+}
+
+// one colour of red-black Gauss-Seidel on the uniform fine grid; the
+// right-hand side and diagonal shift of the thread's nodes (interleaved in
+// global memory, Dirichlet nodes carrying (0, kBigShift)) are all loaded
+// before the updates
+template <int c, typename R>
+__device__ __forceinline__ void mg_sweep0(const DevFieldArgs<R> &a, const MGShared &m) {
+  float *d = m.d;
+  const float hfx = m.hfx, hfy = m.hfy, diag = 2.0f * (m.hfx + m.hfy);
+  const int nx = a.nx, ny = a.ny, hr = (nx + 1) >> 1;
+  float2 v[kOwn];
+#pragma unroll
+  for (int q = 0; q < kOwn; ++q) {
+    const int k = fine_node(c, q, nx, ny, hr);
+    v[q] = a.rs0[k >= 0 ? k : 0];
+  }
+#pragma unroll
+  for (int q = 0; q < kOwn; ++q) {
+    const int k = fine_node(c, q, nx, ny, hr);
+    if (k < 0)
+      continue;
+    int il, ir, jd, ju;
+    nb0(k, nx, ny, il, ir, jd, ju);
+    d[k] = (v[q].x + hfx * (d[il] + d[ir]) + hfy * (d[jd] + d[ju])) / (diag + v[q].y);
+  }
   __syncthreads();
 }
 
+// exact solve on the coarse level lc: d = A_lc^-1 r with the inverse built
+// on the host (zero rows and columns at Dirichlet nodes, row-major); four
+// threads per row, each summing a contiguous quarter
 template <typename R>
-__device__ void mg_vcycle(const DevFieldArgs<R> &a, const MGView &m) {
+__device__ __forceinline__ void mg_direct(const DevFieldArgs<R> &a, const MGShared &m, int lc) {
+  const MGLevel C = a.lev[lc];
+  const int n = C.nx * C.ny;
+  const float *r = m.rc + (C.off - m.n0);
+  float *d = m.d + C.off;
+  const int row = threadIdx.x >> 2, part = threadIdx.x & 3;
+  const int seg = (n + 3) >> 2;
+  float s = 0.0f;
+  if (row < n) {
+    const float *A = a.cinv + (long long)row * n;
+    const int j1 = (part + 1) * seg < n ? (part + 1) * seg : n;
+#pragma unroll 8
+    for (int j = part * seg; j < j1; ++j)
+      s += A[j] * r[j];
+  }
+  s += __shfl_xor(s, 1, 64);
+  s += __shfl_xor(s, 2, 64);
+  if (row < n && part == 0)
+    d[row] = s;
+  __syncthreads();
+}
+
+// z = V-cycle(r) into m.d[0 .. n0)
+template <typename R>
+__device__ __forceinline__ void mg_vcycle(const DevFieldArgs<R> &a, const MGShared &m) {
   const int nl = a.n_levels;
-  for (int k = threadIdx.x; k < a.lev[0].nx * a.lev[0].ny; k += kDevThreads)
+  for (int k = threadIdx.x; k < a.nn; k += kDevThreads)
     m.d[k] = 0.0f;
   __syncthreads();
-  for (int l = 0; l < nl - 1; ++l) {
+  if (nl == 1) {
+    for (int s = 0; s < a.nu_coarse; ++s) {
+      mg_sweep0<0>(a, m);
+      mg_sweep0<1>(a, m);
+    }
+    return;
+  }
+  for (int s = 0; s < a.nu_pre; ++s) {
+    mg_sweep0<0>(a, m);
+    mg_sweep0<1>(a, m);
+  }
+  {  // level-0 residual, restricted to level 1
+    const float hfx = m.hfx, hfy = m.hfy, diag = 2.0f * (m.hfx + m.hfy);
+    const int nx = a.nx, ny = a.ny, hr = (nx + 1) >> 1;
+    (void)hr;
+    for (int k0 = threadIdx.x; k0 < a.n_pad; k0 += kPad) {
+      float2 v[kPad / kDevThreads];
+#pragma unroll
+      for (int u = 0; u < kPad / kDevThreads; ++u)
+        v[u] = a.rs0[k0 + u * kDevThreads];
+#pragma unroll
+      for (int u = 0; u < kPad / kDevThreads; ++u) {
+        const int k = k0 + u * kDevThreads;
+        if (k >= a.nn)
+          continue;
+        int il, ir, jd, ju;
+        nb0(k, nx, ny, il, ir, jd, ju);
+        const float rk = v[u].x - ((diag + v[u].y) * m.d[k] -
+                                   (hfx * (m.d[il] + m.d[ir]) + hfy * (m.d[jd] + m.d[ju])));
+        m.res[k] = v[u].y >= kBigShift ? 0.0f : rk;
+      }
+    }
+    __syncthreads();
+    mg_restrict_to(a, m, 0, m.res, m.rc + (a.lev[1].off - m.n0), true, BlockTeam());
+  }
+  // the coarse level below which the hierarchy is replaced by its exact
+  // inverse (when the Jacobian shift is negligible there), or handed to one
+  // wave
+  const int lc = a.direct_level;
+  const bool direct = lc > 0 && lc < nl && !a.equilibrium;
+  const int lw = direct ? lc : (a.wave_level < nl ? a.wave_level : nl);
+  for (int l = 1; l < lw && l < nl - 1; ++l) {
     for (int s = 0; s < a.nu_pre; ++s) {
-      mg_sweep(a, m, l, 0);
-      mg_sweep(a, m, l, 1);
+      mg_sweep_c(a, m, l, 0, BlockTeam());
+      mg_sweep_c(a, m, l, 1, BlockTeam());
     }
-    mg_restrict(a, m, l);
+    mg_residual_c(a, m, l, m.res, BlockTeam());
   }
-  for (int s = 0; s < a.nu_coarse; ++s) {
-    mg_sweep(a, m, nl - 1, 0);
-    mg_sweep(a, m, nl - 1, 1);
+  if (direct) {
+    mg_direct(a, m, lc);
+  } else if (lw < nl) {
+    if (threadIdx.x < 64)
+      mg_lower(a, m, lw, WaveTeam());
+    __syncthreads();
+  } else {
+    for (int s = 0; s < a.nu_coarse; ++s) {
+      mg_sweep_c(a, m, nl - 1, 0, BlockTeam());
+      mg_sweep_c(a, m, nl - 1, 1, BlockTeam());
+    }
   }
-  for (int l = nl - 2; l >= 0; --l) {
-    mg_prolong(a, m, l);
+  for (int l = (lw < nl ? lw : nl - 1) - 1; l >= 1; --l) {
+    mg_prolong(a, m, l, BlockTeam());
     for (int s = 0; s < a.nu_post; ++s) {
-      mg_sweep(a, m, l, 1);
-      mg_sweep(a, m, l, 0);
+      mg_sweep_c(a, m, l, 1, BlockTeam());
+      mg_sweep_c(a, m, l, 0, BlockTeam());
     }
+  }
+  mg_prolong(a, m, 0, BlockTeam());
+  for (int s = 0; s < a.nu_post; ++s) {
+    mg_sweep0<1>(a, m);
+    mg_sweep0<0>(a, m);
+  }
+}
+
+// diagonal shifts of the coarse levels from the level-0 one (lumped
+// Galerkin P^T diag(sigma) P)
+template <typename R>
+__device__ __forceinline__ void mg_shift_down(const DevFieldArgs<R> &a, const MGShared &m) {
+  const int nl = a.n_levels;
+  if (nl == 1)
+    return;
+  for (int k = threadIdx.x; k < a.nn; k += kDevThreads) {
+    const float sk = a.rs0[k].y;
+    m.res[k] = sk >= kBigShift ? 0.0f : sk;
+  }
+  __syncthreads();
+  mg_restrict_to(a, m, 0, m.res, m.sc + (a.lev[1].off - m.n0), false, BlockTeam(), true);
+  for (int l = 1; l < nl - 1; ++l) {
+    // the restriction input: the finer level's shift without its Dirichlet
+    // values
+    const MGLevel L = a.lev[l];
+    const float *sf = m.sc + (L.off - m.n0);
+    for (int k = threadIdx.x; k < L.nx * L.ny; k += kDevThreads)
+      m.res[k] = sf[k] >= kBigShift ? 0.0f : sf[k];
+    __syncthreads();
+    mg_restrict_to(a, m, l, m.res, m.sc + (a.lev[l + 1].off - m.n0), false, BlockTeam(), true);
   }
 }
 
@@ -984,24 +1246,17 @@ __device__ void mg_vcycle(const DevFieldArgs<R> &a, const MGView &m) {
 //   F_k = sum_d hF_d (phi_prev + phi_next - 2 phi_k) + hP (p - n + C) = 0
 // at every non-ohmic node (mirror neighbours at the boundary), with
 // p = exp(-phi) and n = conc (non-equilibrium) or exp(phi) (equilibrium);
-// ohmic nodes hold V/V_T + asinh(C/2).  The residual and the update are
-// FP64; the Newton correction comes from one FP32 multigrid V-cycle.
-// Then calcEField (PMSchemes/emcEFieldCalculation.hpp:13-81).
+// ohmic nodes hold V/V_T + asinh(C/2).  Residuals, the conjugate-gradient
+// vectors and the update are FP64; the CG preconditioner is one FP32
+// multigrid V-cycle.  Then calcEField (PMSchemes/emcEFieldCalculation.hpp:
+// 13-81).
 template <typename R>
-__global__ __launch_bounds__(kDevThreads) void k_dev_field(DevFieldArgs<R> a) {
-  extern __shared__ __align__(16) float mg[];
+__global__ __launch_bounds__(kDevThreads) void k_dev_field(const DevFieldArgs<R> *__restrict__ args) {
+  const DevFieldArgs<R> &a = *args;
+  extern __shared__ __align__(16) float lds[];
   __shared__ double red[kDevThreads / 64];
   const int tid = threadIdx.x;
   const int nx = a.nx, ny = a.ny, nn = a.nn;
-  const MGLevel LL = a.lev[a.n_levels - 1];
-  const int ntot = LL.off + LL.nx * LL.ny;
-  MGView m;
-  m.d = mg;
-  m.r = mg + ntot;
-  m.res = mg + 2 * ntot;
-  m.sig = a.sig;
-  m.hfx = (float)a.hfx;
-  m.hfy = (float)a.hfy;
   double *pot = a.pot;
   long long row = 0;
   if (a.record)
@@ -1017,52 +1272,143 @@ __global__ __launch_bounds__(kDevThreads) void k_dev_field(DevFieldArgs<R> a) {
     }
   }
   if (a.do_poisson) {
-    if (a.reset_bc) {
+    MGShared m;
+    m.d = lds;
+    m.res = lds + a.lds_res;
+    m.rc = lds + a.lds_rc;
+    m.sc = lds + a.lds_sc;
+    DimTab *tab = (DimTab *)((unsigned char *)lds + a.lds_tab);
+    uint8_t *cmask = (uint8_t *)lds + a.lds_cmask;
+    m.tab = tab;
+    m.cmask = cmask;
+    m.n0 = nn;
+    m.hfx = (float)a.hfx;
+    m.hfy = (float)a.hfy;
+    for (int e = tid; e < a.n_tab; e += kDevThreads)
+      tab[e] = a.tab[e];
+    for (int e = tid; e < a.n_coarse; e += kDevThreads) {
+      const uint8_t mk = a.lmask[nn + e];
+      cmask[e] = mk;
+      m.sc[e] = mk ? kBigShift : 0.0f;
+    }
+    if (a.reset_bc)
       for (int k = tid; k < nn; k += kDevThreads)
         if (a.dmask[k])
           pot[k] = a.equilibrium ? a.bc_eq[k] : a.bc_neq[k];
-      __syncthreads();
-    }
+    __syncthreads();
     const double hfx = a.hfx, hfy = a.hfy, hp = a.hp, hfs2 = 2.0 * (a.hfx + a.hfy);
-    double scale = 1.0, fmax = 0.0;
-    int it = 0;
+    double *r64 = a.r64, *x64 = a.x64, *p64 = a.p64, *q64 = a.q64, *s64 = a.s64;
+    const float *w = a.w0;  // 1 at active nodes, 0 at Dirichlet and padding
+    // control-volume weights (1/2 per boundary dimension): the mirror
+    // boundary operator is symmetric in this inner product
+    const float *hw = a.hw0;
+    const int np = a.n_pad;  // nn rounded up to kPad nodes
+    double scale = 1.0, fm = 0.0;
+    int it = 0, vcycles = 0;
     for (;; ++it) {
+      // Newton residual F and the Jacobian's diagonal shift
       double lmax = 0.0, lsc = 0.0;
-      for (int k = tid; k < nn; k += kDevThreads) {
-        if (a.dmask[k]) {
-          m.r[k] = 0.0f;
-          a.sig[k] = 0.0f;
-          continue;
+      for (int k0 = tid; k0 < np; k0 += kPad) {
+#pragma unroll
+        for (int u = 0; u < kPad / kDevThreads; ++u) {
+          const int k = k0 + u * kDevThreads;
+          const float wk = w[k];
+          const int kc = k < nn ? k : nn - 1;
+          int il, ir, jd, ju;
+          nb0(kc, nx, ny, il, ir, jd, ju);
+          const double phi = pot[kc];
+          const double p = exp(-phi);
+          const double n = a.equilibrium ? exp(phi) : a.conc[kc];
+          const double C = a.dopn[kc];
+          const double F = wk * (hfx * (pot[il] + pot[ir]) + hfy * (pot[jd] + pot[ju]) -
+                                 hfs2 * phi + hp * (p - n + C));
+          const double sg = hp * (a.equilibrium ? p + n : p);
+          r64[k] = F;
+          s64[k] = sg;
+          a.rs0[k] = make_float2((float)F, wk != 0.0f ? (float)sg : kBigShift);
+          const double af = fabs(F);
+          lmax = af > lmax ? af : lmax;
+          const double sc = wk * hp * (fabs(C) + n + p);
+          lsc = sc > lsc ? sc : lsc;
         }
-        const int j = k / nx, i = k - j * nx;
-        const int il = i > 0 ? k - 1 : k + 1, ir = i < nx - 1 ? k + 1 : k - 1;
-        const int jd = j > 0 ? k - nx : k + nx, ju = j < ny - 1 ? k + nx : k - nx;
-        const double phi = pot[k];
-        const double p = exp(-phi);
-        const double n = a.equilibrium ? exp(phi) : a.conc[k];
-        const double C = a.dopn[k];
-        const double F = hfx * (pot[il] + pot[ir]) + hfy * (pot[jd] + pot[ju]) - hfs2 * phi +
-                         hp * (p - n + C);
-        m.r[k] = (float)F;
-        a.sig[k] = (float)(hp * (a.equilibrium ? p + n : p));
-        const double af = fabs(F);
-        lmax = af > lmax ? af : lmax;
-        const double sc = hp * (fabs(C) + n + p);
-        lsc = sc > lsc ? sc : lsc;
       }
-      fmax = block_max(lmax, red);
+      fm = block_max(lmax, red);
       if (it == 0) {
         scale = block_max(lsc, red);
         scale = scale > 0.0 ? scale : 1.0;
       }
-      if (fmax <= a.tol * scale || it >= a.max_iter)
+      if (fm <= a.tol * scale || it >= a.max_iter || vcycles >= a.max_vcycles)
         break;
-      mg_restrict_sig(a, m);
-      mg_vcycle(a, m);
+      if (a.equilibrium || it == 0)
+        mg_shift_down(a, m);
+      // conjugate gradients on A x = F (A = -Jacobian, symmetric in the
+      // control-volume inner product), x0 = 0
+      double target = 0.5 * a.tol * scale;
+      if (a.equilibrium && 0.1 * fm > target)  // inexact Newton away from the root
+        target = 0.1 * fm;
+      double rz_old = 1.0;
+      for (int k = tid; k < np; k += kDevThreads) {
+        x64[k] = 0.0;
+        p64[k] = 0.0;
+      }
+      for (int pit = 0; pit < a.max_pcg; ++pit) {
+        mg_vcycle(a, m);  // z = M^-1 r in m.d[0 .. nn)
+        ++vcycles;
+        double rz = 0.0;
+        for (int k0 = tid; k0 < np; k0 += kPad) {
+#pragma unroll
+          for (int u = 0; u < kPad / kDevThreads; ++u) {
+            const int k = k0 + u * kDevThreads;
+            rz += hw[k] * r64[k] * (double)(w[k] * m.d[k < nn ? k : nn - 1]);
+          }
+        }
+        rz = block_sum(rz, red);
+        const double beta = pit == 0 ? 0.0 : rz / rz_old;
+        rz_old = rz;
+        for (int k0 = tid; k0 < np; k0 += kPad) {
+#pragma unroll
+          for (int u = 0; u < kPad / kDevThreads; ++u) {
+            const int k = k0 + u * kDevThreads;
+            p64[k] = (double)(w[k] * m.d[k < nn ? k : nn - 1]) + beta * p64[k];
+          }
+        }
+        __syncthreads();
+        double pq = 0.0;
+        for (int k0 = tid; k0 < np; k0 += kPad) {
+#pragma unroll
+          for (int u = 0; u < kPad / kDevThreads; ++u) {
+            const int k = k0 + u * kDevThreads;
+            const int kc = k < nn ? k : nn - 1;
+            int il, ir, jd, ju;
+            nb0(kc, nx, ny, il, ir, jd, ju);
+            const double pk = p64[k];
+            const double qq = w[k] * ((hfs2 + s64[k]) * pk -
+                                      (hfx * (p64[il] + p64[ir]) + hfy * (p64[jd] + p64[ju])));
+            q64[k] = qq;
+            pq += hw[k] * pk * qq;
+          }
+        }
+        pq = block_sum(pq, red);
+        const double alpha = pq != 0.0 ? rz / pq : 0.0;
+        double rmax = 0.0;
+        for (int k0 = tid; k0 < np; k0 += kPad) {
+#pragma unroll
+          for (int u = 0; u < kPad / kDevThreads; ++u) {
+            const int k = k0 + u * kDevThreads;
+            x64[k] += alpha * p64[k];
+            const double rn = r64[k] - alpha * q64[k];
+            r64[k] = rn;
+            a.rs0[k].x = (float)rn;
+            const double ar = fabs(rn);
+            rmax = ar > rmax ? ar : rmax;
+          }
+        }
+        rmax = block_max(rmax, red);
+        if (rmax <= target || vcycles >= a.max_vcycles)
+          break;
+      }
       for (int k = tid; k < nn; k += kDevThreads) {
-        if (a.dmask[k])
-          continue;
-        double d = (double)m.d[k];
+        double d = x64[k];  // zero at Dirichlet nodes
         if (a.equilibrium) {  // damped Newton from the charge-neutral guess
           d = d > 2.0 ? 2.0 : d;
           d = d < -2.0 ? -2.0 : d;
@@ -1073,12 +1419,12 @@ __global__ __launch_bounds__(kDevThreads) void k_dev_field(DevFieldArgs<R> a) {
     }
     if (tid == 0) {
       if (a.iters_out)
-        *a.iters_out = it;
+        *a.iters_out = vcycles;
       if (a.resid_out)
-        *a.resid_out = fmax / scale;
+        *a.resid_out = fm / scale;
       if (a.record) {
-        a.hist.iters[row] = it;
-        a.hist.resid[row] = fmax / scale;
+        a.hist.iters[row] = vcycles;
+        a.hist.resid[row] = fm / scale;
       }
     }
     __syncthreads();
@@ -1118,8 +1464,11 @@ __global__ __launch_bounds__(kDevThreads) void k_dev_field(DevFieldArgs<R> a) {
 // ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
+// the arguments live in device memory (args_dev, written beforehand by
+// the caller): dynamic indexing of the level table would otherwise copy the
+// whole block to scratch
 template <typename R>
-hipError_t dev_launch_field(const DevFieldArgs<R> &a, size_t lds, hipStream_t st) {
+hipError_t dev_launch_field(const DevFieldArgs<R> *args_dev, size_t lds, hipStream_t st) {
   if (lds > (size_t)kDevMaxLds)
     return hipErrorInvalidValue;
   if (lds > 65536) {
@@ -1130,7 +1479,7 @@ hipError_t dev_launch_field(const DevFieldArgs<R> &a, size_t lds, hipStream_t st
       return e;
     }
   }
-  hipLaunchKernelGGL((k_dev_field<R>), dim3(1), dim3(kDevThreads), lds, st, a);
+  hipLaunchKernelGGL((k_dev_field<R>), dim3(1), dim3(kDevThreads), lds, st, args_dev);
   return hipGetLastError();
 }
 
@@ -1170,6 +1519,8 @@ hipError_t dev_launch_contacts(const DevContactArgs<R> &a, long long max_inj, hi
 
 template <typename R>
 hipError_t dev_launch_reduce(const DevContactArgs<R> &a, int commit, hipStream_t st) {
+  hipLaunchKernelGGL((k_dev_reduce_rows<R>), dim3((a.g.nn + 255) / 256, a.n_chunks), dim3(256), 0,
+                     st, a);
   hipLaunchKernelGGL((k_dev_reduce<R>), dim3((a.g.nn + 255) / 256), dim3(256), 0, st, a, commit);
   return hipGetLastError();
 }
@@ -1251,7 +1602,7 @@ hipError_t dev_launch_interp_pos(const DevGeom<double> &g, const double *pos, lo
 }
 
 #define EMCGPU_DEV_INST(R)                                                                   \
-  template hipError_t dev_launch_field<R>(const DevFieldArgs<R> &, size_t, hipStream_t);      \
+  template hipError_t dev_launch_field<R>(const DevFieldArgs<R> *, size_t, hipStream_t);      \
   template hipError_t dev_launch_step<R>(const DevStepArgs<R> &, int, size_t, hipStream_t);   \
   template hipError_t dev_launch_drift<R>(const DevStepArgs<R> &, int, hipStream_t);          \
   template hipError_t dev_launch_contacts<R>(const DevContactArgs<R> &, long long,           \
