@@ -106,7 +106,9 @@ def run_gpu(emcgpu, spec, precision, n, warmup, steps, spl, seed=20261202, ppl=0
 def run_batched(emcgpu, warmup, steps, seed=20261202):
     """The 64-instance Si bias/temperature sweep in one FP32 arena (tables
     built per instance in FP64, stored FP32), every call launched as one HIP
-    graph; whole-sweep particle-updates/s."""
+    graph with the launch shape (particles per lane = tile size / 256, steps
+    fused per launch) chosen by emcgpu_autotune on this ensemble;
+    whole-sweep particle-updates/s."""
     from emc_models import batched_instances
     inst = batched_instances(64, seed)
     counts = [n for _, n in inst]
@@ -119,6 +121,9 @@ def run_batched(emcgpu, warmup, steps, seed=20261202):
         ctx.init_particles(i, seed, m["T"])
     ctx.synchronize()
     setup_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tune = ctx.autotune(1e-15, seed=seed, step0=0)
+    tune_s = time.perf_counter() - t0
     if warmup:
         ctx.step(1e-15, warmup, seed=seed, step0=0, graph=True)
     ctx.synchronize()
@@ -133,8 +138,11 @@ def run_batched(emcgpu, warmup, steps, seed=20261202):
     return {"updates_per_s": total * steps / wall, "instances": len(inst), "particles": total,
             "steps": steps, "ms_per_step": wall / steps * 1e3, "setup_s": setup_s,
             "vd_min": min(vd), "vd_max": max(vd),
+            "autotune": {"particles_per_lane": tune["best"][0], "steps_per_launch": tune["best"][1],
+                         "tune_s": tune_s,
+                         "trials_ms_per_step": [[p, k, round(ms, 5)] for p, k, ms in tune["trials"]]},
             "workload": "64 Si bulk instances, field log-U[0.1,200] kV/cm along [100]/[111], "
-                        "T U[77,500] K, N log-U[1e4,2e6], FP32, HIP-graph launch"}
+                        "T U[77,500] K, N log-U[1e4,2e6], FP32, HIP-graph launch, autotuned shape"}
 
 
 def run_hot_phonon(emcgpu, n, warmup, steps, precision, seed=20261202):

@@ -139,14 +139,15 @@ typedef struct {
 typedef struct {
   double dt;            /* time step [s] */
   int n_steps;          /* steps to advance */
-  int steps_per_launch; /* steps fused per kernel launch (0 = automatic) */
+  int steps_per_launch; /* steps fused per kernel launch (0 = the value of
+                           emcgpu_autotune, else min(n_steps, 64)) */
   int mode;             /* emcgpu_mode */
   uint64_t seed;        /* production: Philox key */
   uint32_t step0;       /* production: global index of the first step */
   int timing;           /* non-zero: time every launch with HIP events */
   int particles_per_lane; /* production kernel: particles owned by each
-                             GPU lane, 1, 2, 4 or 8 (0 = library default,
-                             or the autotuned value) */
+                             GPU lane, 1, 2, 4 or 8 (0 = the value of
+                             emcgpu_autotune, else the library default) */
   int use_graph;        /* production: capture the call's launches (step and
                            reduction kernels) in a HIP graph and launch it;
                            the executable graph is updated in place on the
@@ -452,6 +453,40 @@ emcgpu_status emcgpu_device_fields(const emcgpu_ctx *ctx, double *pot, double *e
 emcgpu_status emcgpu_get_timing(const emcgpu_ctx *ctx, double *ms_total,
                                 long long *launches, int reset);
 emcgpu_status emcgpu_synchronize(emcgpu_ctx *ctx);
+
+/* ---- launch-shape autotuner (no counterpart in the reference, whose
+ * OpenMP loop has no launch shape) ------------------------------------------
+ * Times production runs of the context's current ensemble (all instances)
+ * for every pair of candidate particles-per-lane (tile = 256 x ppl
+ * particles) and steps fused per launch, restoring the particle state
+ * before every trial and at the end, and keeps the fastest pair as the
+ * context's default for emcgpu_step arguments left at 0.  Results of
+ * emcgpu_step do not depend on the launch shape (random numbers are
+ * indexed by particle and step), so tuning changes only speed.  The
+ * observables of the last emcgpu_step are discarded.  Not available with
+ * phonon baths (their state evolves with every step); emcgpu_resize clears
+ * the tuned values. */
+#define EMCGPU_TUNE_MAX 32
+typedef struct {
+  double dt;            /* step of the trial runs [s] */
+  int n_steps;          /* steps per trial (0 = twice the largest
+                           steps-per-launch candidate) */
+  uint64_t seed;        /* Philox key of the trial runs */
+  uint32_t step0;
+  int repeats;          /* timed trials per candidate after one untimed
+                           warm-up; the fastest counts (0 = 1) */
+  int n_ppl;            /* candidates (0: 1, 2, 4, 8) */
+  int ppl[4];
+  int n_spl;            /* candidates (0: 16, 32, 64, 128) */
+  int spl[8];
+  /* results */
+  int n_results;
+  int res_ppl[EMCGPU_TUNE_MAX], res_spl[EMCGPU_TUNE_MAX];
+  double res_ms_per_step[EMCGPU_TUNE_MAX];
+  int best_ppl, best_spl;
+  double best_ms_per_step;
+} emcgpu_autotune_args;
+emcgpu_status emcgpu_autotune(emcgpu_ctx *ctx, emcgpu_autotune_args *args);
 
 #ifdef __cplusplus
 }

@@ -6,7 +6,10 @@ oracle's tables for that instance's model.
 GPU: heterogeneous instances ([100] and [111] fields, ragged counts,
 different temperatures) each evolve exactly as the oracle's Philox twin of
 that instance; a step sequence launched through a HIP graph reproduces the
-eagerly launched one bit for bit, over repeated calls.
+eagerly launched one bit for bit, over repeated calls.  The launch-shape
+autotuner leaves the ensemble as it found it, reports every candidate, and
+its choice becomes the default shape: a run with the tuned defaults equals
+a run with that shape given explicitly.
 """
 import numpy as np
 import pytest
@@ -111,3 +114,49 @@ def test_graph_launch_matches_eager(precision):
             for key in ("k", "E", "tau", "valley", "sub"):
                 assert np.array_equal(pa[i][key], pb[i][key]), (i, key)
             assert np.array_equal(oa[i], ob[i]), i
+
+
+def test_autotune_needs_a_gpu_context():
+    ctx = emcgpu.Context(precision=emcgpu.FP32, device=-1)
+    with pytest.raises(emcgpu.EmcGpuError, match="host-only"):
+        ctx.autotune(1e-15)
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_autotune_restores_state_and_sets_the_default_shape():
+    inst = batched_instances(n_instances=3, seed=5, n_min=2000, n_max=40000)
+    counts = [n for _, n in inst]
+    print("autotune, counts", counts, flush=True)
+
+    def make():
+        ctx = emcgpu.Context(precision=emcgpu.FP32, n_instances=len(inst))
+        for i, (m, _) in enumerate(inst):
+            ctx.setup(i, m)
+        ctx.resize(counts)
+        for i, (m, _) in enumerate(inst):
+            ctx.init_particles(i, 77, m["T"])
+        return ctx
+
+    a = make()
+    p0 = [a.get_particles(i) for i in range(len(inst))]
+    res = a.autotune(1e-15, seed=77, n_steps=64, ppl=(1, 4), spl=(8, 32))
+    trials = res["trials"]
+    assert sorted((p, k) for p, k, _ in trials) == [(1, 8), (1, 32), (4, 8), (4, 32)]
+    assert all(ms > 0 for _, _, ms in trials)
+    fastest = min(trials, key=lambda t: t[2])
+    assert res["best"] == fastest
+    for i in range(len(inst)):
+        p = a.get_particles(i)
+        for key in ("k", "E", "tau", "valley", "sub"):
+            assert np.array_equal(p[key], p0[i][key]), (i, key)
+    a.step(1e-15, 100, seed=77, step0=0, graph=True)
+    b = make()
+    b.step(1e-15, 100, seed=77, step0=0, graph=True, ppl=fastest[0], steps_per_launch=fastest[1])
+    for i in range(len(inst)):
+        pa, pb = a.get_particles(i), b.get_particles(i)
+        for key in ("k", "E", "tau", "valley", "sub"):
+            assert np.array_equal(pa[key], pb[key]), (i, key)
+        assert np.array_equal(a.observables(i), b.observables(i)), i
+    a.close()
+    b.close()

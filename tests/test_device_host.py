@@ -87,10 +87,12 @@ C_LAYOUT = r"""
 int main(void) {
   P(emcgpu_config); P(emcgpu_valley_desc); P(emcgpu_mech_desc); P(emcgpu_bath_desc);
   P(emcgpu_step_args); P(emcgpu_doping_region); P(emcgpu_contact); P(emcgpu_device_desc);
-  P(emcgpu_device_drift_args); P(emcgpu_device_step_args);
+  P(emcgpu_device_drift_args); P(emcgpu_device_step_args); P(emcgpu_autotune_args);
   O(emcgpu_mech_desc, bath); O(emcgpu_step_args, draws); O(emcgpu_device_desc, contacts);
   O(emcgpu_device_desc, poisson_tol); O(emcgpu_device_drift_args, draws);
   O(emcgpu_device_drift_args, tr_events); O(emcgpu_device_step_args, checksum);
+  O(emcgpu_autotune_args, spl); O(emcgpu_autotune_args, res_ms_per_step);
+  O(emcgpu_autotune_args, best_ms_per_step);
   return 0;
 }
 """
@@ -108,7 +110,8 @@ def test_ctypes_layouts_match_the_c_compiler():
           "emcgpu_step_args": emcgpu.StepArgs, "emcgpu_doping_region": emcgpu.DopingRegion,
           "emcgpu_contact": emcgpu.ContactDesc, "emcgpu_device_desc": emcgpu.DeviceDesc,
           "emcgpu_device_drift_args": emcgpu.DeviceDriftArgs,
-          "emcgpu_device_step_args": emcgpu.DeviceStepArgs}
+          "emcgpu_device_step_args": emcgpu.DeviceStepArgs,
+          "emcgpu_autotune_args": emcgpu.AutotuneArgs}
     for name, cls in py.items():
         assert C.sizeof(cls) == int(out[name]), name
     for key, val in out.items():

@@ -500,7 +500,11 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
   emcgpu_status st = upload_params<R>(c, s->dt);
   if (st)
     return st;
-  int K = s->steps_per_launch > 0 ? s->steps_per_launch : std::min(s->n_steps, 64);
+  if (s->steps_per_launch < 0)
+    return fail(c, EMCGPU_ERR_ARG, "steps_per_launch must be >= 0");
+  int K = s->steps_per_launch > 0 ? s->steps_per_launch
+          : c->tuned_spl > 0      ? c->tuned_spl
+                                  : std::min(s->n_steps, 64);
   K = std::min(K, s->n_steps);
   // phonon baths couple the particles between steps: one step per launch,
   // then the bath update and the table rebuild
@@ -523,7 +527,10 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
     c->bath_hist_steps = 0;
   }
   // particles per lane of the production kernel (replay: one per lane)
-  const int ppt = replay ? 1 : (ppl ? ppl : default_ppt(c->precision));
+  const int ppt = replay ? 1
+                  : ppl  ? ppl
+                  : c->tuned_ppl ? c->tuned_ppl
+                                 : default_ppt(c->precision);
   const int tile_size = kTile * ppt;
   const int n_tiles = (int)(c->capacity / tile_size);
   if ((st = ensure_partial(c, Kr, n_tiles)) || (st = ensure_obs(c, s->n_steps)))
@@ -1272,6 +1279,7 @@ emcgpu_status emcgpu_resize(emcgpu_ctx *c, const size_t *counts) {
   if (total / kTile > (1LL << 30) || total > (long long)UINT32_MAX)
     return fail(c, EMCGPU_ERR_ARG, "too many particles");
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->tuned_ppl = c->tuned_spl = 0;
   const size_t rs = real_size(c);
   dfree(c->kx);
   dfree(c->ky);
@@ -1563,6 +1571,140 @@ emcgpu_status emcgpu_step(emcgpu_ctx *c, const emcgpu_step_args *s) {
     if (c->inst[i].count && !c->inst[i].has_field)
       return fail(c, EMCGPU_ERR_STATE, "instance %d: set the field first", i);
   return c->precision == EMCGPU_FP32 ? run_steps<float>(c, s) : run_steps<double>(c, s);
+}
+
+// Launch-shape autotuner: production trials of the current ensemble for
+// each (particles per lane, steps per launch) pair, timed with HIP events on
+// the context's stream; the particle state is copied aside once and put back
+// before every trial and at the end, so every trial runs the same work and
+// the ensemble is left as it was.
+emcgpu_status emcgpu_autotune(emcgpu_ctx *c, emcgpu_autotune_args *t) {
+  if (!c || !t)
+    return fail(c, EMCGPU_ERR_ARG, "ctx and args must be non-NULL");
+  if (c->device < 0)
+    return fail(c, EMCGPU_ERR_STATE, "host-only context cannot tune");
+  if (c->n_baths > 0)
+    return fail(c, EMCGPU_ERR_STATE, "autotune is not available with phonon baths");
+  if (c->capacity == 0)
+    return fail(c, EMCGPU_ERR_STATE, "no particles (call emcgpu_resize first)");
+  for (int i = 0; i < c->n_inst; i++)
+    if (c->inst[i].count && !c->inst[i].has_field)
+      return fail(c, EMCGPU_ERR_STATE, "instance %d: set the field first", i);
+  if (t->n_ppl < 0 || t->n_ppl > 4 || t->n_spl < 0 || t->n_spl > 8 || t->repeats < 0 ||
+      t->n_steps < 0 || !(t->dt > 0))
+    return fail(c, EMCGPU_ERR_ARG, "autotune: dt > 0, n_steps >= 0, repeats >= 0, "
+                                   "at most 4 ppl and 8 steps-per-launch candidates");
+  std::vector<int> P, S;
+  if (t->n_ppl == 0)
+    P = {1, 2, 4, 8};
+  else
+    P.assign(t->ppl, t->ppl + t->n_ppl);
+  if (t->n_spl == 0)
+    S = {16, 32, 64, 128};
+  else
+    S.assign(t->spl, t->spl + t->n_spl);
+  for (int p : P)
+    if (p != 1 && p != 2 && p != 4 && p != 8)
+      return fail(c, EMCGPU_ERR_ARG, "autotune: particles per lane must be 1, 2, 4 or 8 (got %d)",
+                  p);
+  int max_spl = 1;
+  for (int k : S) {
+    if (k < 1 || k > 4096)
+      return fail(c, EMCGPU_ERR_ARG, "autotune: steps per launch must be in [1, 4096] (got %d)", k);
+    max_spl = std::max(max_spl, k);
+  }
+  const int n_steps = t->n_steps > 0 ? t->n_steps : 2 * max_spl;
+  const int repeats = t->repeats > 0 ? t->repeats : 1;
+  (void)hipSetDevice(c->device);
+
+  // particle state aside: k (3), E, tau in the context precision, vs
+  const size_t n = (size_t)c->capacity, rs = real_size(c);
+  void *arr[5] = {c->kx, c->ky, c->kz, c->E, c->tau};
+  char *snap = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto release = [&]() {
+    if (snap)
+      (void)hipFree(snap);
+    if (e0)
+      (void)hipEventDestroy(e0);
+    if (e1)
+      (void)hipEventDestroy(e1);
+  };
+  auto copy_state = [&](bool save) -> hipError_t {
+    for (int i = 0; i < 5; i++) {
+      char *s0 = snap + (size_t)i * n * rs;
+      hipError_t e = save ? hipMemcpyAsync(s0, arr[i], n * rs, hipMemcpyDeviceToDevice, c->stream)
+                          : hipMemcpyAsync(arr[i], s0, n * rs, hipMemcpyDeviceToDevice, c->stream);
+      if (e != hipSuccess)
+        return e;
+    }
+    char *sv = snap + 5 * n * rs;
+    return save ? hipMemcpyAsync(sv, c->vs, n * 4, hipMemcpyDeviceToDevice, c->stream)
+                : hipMemcpyAsync(c->vs, sv, n * 4, hipMemcpyDeviceToDevice, c->stream);
+  };
+#define TUNECHK(x)                                                                         do {                                                                                       hipError_t e_ = (x);                                                                     if (e_ != hipSuccess) {                                                                    release();                                                                               return fail(c, EMCGPU_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_));            }                                                                                      } while (0)
+  TUNECHK(hipMalloc((void **)&snap, n * (5 * rs + 4)));
+  TUNECHK(hipEventCreate(&e0));
+  TUNECHK(hipEventCreate(&e1));
+  TUNECHK(copy_state(true));
+
+  emcgpu_step_args a;
+  std::memset(&a, 0, sizeof(a));
+  a.dt = t->dt;
+  a.n_steps = n_steps;
+  a.mode = EMCGPU_MODE_PRODUCTION;
+  a.seed = t->seed;
+  a.step0 = t->step0;
+  t->n_results = 0;
+  t->best_ppl = t->best_spl = 0;
+  t->best_ms_per_step = 0;
+  for (int p : P) {
+    for (int k : S) {
+      a.particles_per_lane = p;
+      a.steps_per_launch = k;
+      double best = -1;
+      for (int r = 0; r <= repeats; r++) {  // r = 0: untimed warm-up
+        TUNECHK(copy_state(false));
+        TUNECHK(hipEventRecord(e0, c->stream));
+        emcgpu_status st = c->precision == EMCGPU_FP32 ? run_steps<float>(c, &a)
+                                                       : run_steps<double>(c, &a);
+        if (st) {  // put the ensemble back, keep the trial's error
+          const std::string msg = c->err;
+          (void)copy_state(false);
+          (void)hipStreamSynchronize(c->stream);
+          release();
+          c->err = msg;
+          return st;
+        }
+        TUNECHK(hipEventRecord(e1, c->stream));
+        TUNECHK(hipEventSynchronize(e1));
+        float ms = 0;
+        TUNECHK(hipEventElapsedTime(&ms, e0, e1));
+        if (r > 0 && (best < 0 || ms < best))
+          best = ms;
+      }
+      const double per_step = best / n_steps;
+      if (t->n_results < EMCGPU_TUNE_MAX) {
+        t->res_ppl[t->n_results] = p;
+        t->res_spl[t->n_results] = k;
+        t->res_ms_per_step[t->n_results] = per_step;
+        t->n_results++;
+      }
+      if (t->best_ppl == 0 || per_step < t->best_ms_per_step) {
+        t->best_ppl = p;
+        t->best_spl = k;
+        t->best_ms_per_step = per_step;
+      }
+    }
+  }
+  TUNECHK(copy_state(false));
+  TUNECHK(hipStreamSynchronize(c->stream));
+#undef TUNECHK
+  release();
+  c->tuned_ppl = t->best_ppl;
+  c->tuned_spl = t->best_spl;
+  c->obs_steps = 0;
+  return EMCGPU_OK;
 }
 
 emcgpu_status emcgpu_get_observables(const emcgpu_ctx *c0, int instance, double *out, int *n_obs,

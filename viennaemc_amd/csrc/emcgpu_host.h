@@ -100,6 +100,8 @@ struct emcgpu_ctx {
   long long launches = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipGraphExec_t gexec = nullptr;  // last captured step sequence
+  // launch shape chosen by emcgpu_autotune (0: not tuned)
+  int tuned_ppl = 0, tuned_spl = 0;
   // phonon baths of all instances (layout built by upload_baths)
   int n_baths = 0, lds_bins = 0, n_jobs = 0;
   long long tally_bins = 0;  // bins of all baths (one tally copy)

@@ -114,6 +114,18 @@ class DeviceStepArgs(C.Structure):
                 ("checksum", C.c_void_p)]
 
 
+TUNE_MAX = 32
+
+
+class AutotuneArgs(C.Structure):
+    _fields_ = [("dt", C.c_double), ("n_steps", C.c_int), ("seed", C.c_uint64),
+                ("step0", C.c_uint32), ("repeats", C.c_int), ("n_ppl", C.c_int),
+                ("ppl", C.c_int * 4), ("n_spl", C.c_int), ("spl", C.c_int * 8),
+                ("n_results", C.c_int), ("res_ppl", C.c_int * TUNE_MAX),
+                ("res_spl", C.c_int * TUNE_MAX), ("res_ms_per_step", C.c_double * TUNE_MAX),
+                ("best_ppl", C.c_int), ("best_spl", C.c_int), ("best_ms_per_step", C.c_double)]
+
+
 _LIB = None
 EXPORTS = [
     "emcgpu_create", "emcgpu_destroy", "emcgpu_last_error", "emcgpu_set_model",
@@ -126,6 +138,7 @@ EXPORTS = [
     "emcgpu_efield", "emcgpu_deposit", "emcgpu_interpolate", "emcgpu_contacts",
     "emcgpu_device_set_particles", "emcgpu_device_get_particles", "emcgpu_device_set_efield",
     "emcgpu_device_drift", "emcgpu_device_init", "emcgpu_device_step", "emcgpu_device_fields",
+    "emcgpu_autotune",
 ]
 
 
@@ -178,6 +191,7 @@ def load(path=LIB_PATH):
         "emcgpu_device_init": (i, [vp, C.c_uint64, p(C.c_size_t)]),
         "emcgpu_device_step": (i, [vp, p(DeviceStepArgs)]),
         "emcgpu_device_fields": (i, [vp, vp, vp, vp, vp, vp]),
+        "emcgpu_autotune": (i, [vp, p(AutotuneArgs)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -480,6 +494,22 @@ class Context:
 
     def synchronize(self):
         self._chk(self.L.emcgpu_synchronize(self.h))
+
+    def autotune(self, dt, seed=0, step0=0, n_steps=0, repeats=0, ppl=(), spl=()):
+        """emcgpu_autotune: time every (particles per lane, steps per launch)
+        pair on the current ensemble and keep the fastest as the default of
+        step(); returns {"best": (ppl, spl, ms_per_step), "trials": [...]}."""
+        a = AutotuneArgs()
+        a.dt, a.seed, a.step0, a.n_steps, a.repeats = dt, seed, step0, n_steps, repeats
+        a.n_ppl = len(ppl)
+        for j, v in enumerate(ppl):
+            a.ppl[j] = v
+        a.n_spl = len(spl)
+        for j, v in enumerate(spl):
+            a.spl[j] = v
+        self._chk(self.L.emcgpu_autotune(self.h, C.byref(a)))
+        trials = [(a.res_ppl[j], a.res_spl[j], a.res_ms_per_step[j]) for j in range(a.n_results)]
+        return {"best": (a.best_ppl, a.best_spl, a.best_ms_per_step), "trials": trials}
 
     # -- 2-D device (instance 0) ------------------------------------------
     def device_setup(self, dev, pm=PM_CIC, poisson_tol=1e-10, poisson_max_iter=50):
