@@ -435,6 +435,22 @@ typedef struct {
   int64_t *n_particles;
   int32_t *poisson_iters;
   double *poisson_resid, *checksum;
+  /* setPoissonInterval (emcSimulation.hpp:84-86): Poisson and E-field on
+   * the steps whose index (counted from emcgpu_device_init, 0-based) is a
+   * multiple of this, the field kept in between; 0 or 1 = every step.
+   * Steps without a solve record 0 iterations and residual 0. */
+  int poisson_interval;
+  /* setChannelCurrentRegion (emcSimulation.hpp:64-70): with
+   * channel_length > 0, channel_current[n_steps] gets after each step
+   * q * sum v_x / channel_length [A] over the particles with
+   * channel_x0 <= x <= channel_x1 [m] (getChannelDriftCurrent,
+   * emcBasicParticleHandler.hpp:224-238) */
+  double channel_x0, channel_x1, channel_length;
+  double *channel_current;
+  /* non-zero: add each step's potential and concentration to the running
+   * sums read by emcgpu_device_averages (updateAverageCharacteristics,
+   * emcSimulationResults.hpp:85-93) */
+  int accumulate;
 } emcgpu_device_step_args;
 
 /* performEMCStep repeated n_steps times (emcSimulation.hpp:164-193):
@@ -447,6 +463,12 @@ emcgpu_status emcgpu_device_step(emcgpu_ctx *ctx, const emcgpu_device_step_args 
  * NULL) */
 emcgpu_status emcgpu_device_fields(const emcgpu_ctx *ctx, double *pot, double *ex,
                                    double *ey, double *conc, double *nr_part);
+/* averages of the potential and concentration (normalised) over the steps
+ * run with accumulate != 0 since emcgpu_device_init or the last reset
+ * (avgPot / avgConc of writeFinalResults, emcSimulationResults.hpp:175-183);
+ * *n_steps gets the number of steps; reset != 0 clears the sums afterwards */
+emcgpu_status emcgpu_device_averages(emcgpu_ctx *ctx, double *pot, double *conc,
+                                     long long *n_steps, int reset);
 
 /* accumulated kernel time of timed emcgpu_step calls (HIP events on the
  * context's stream): total ms and number of particle-step launches */

@@ -635,6 +635,22 @@ void orc_device_contact_rule(const orc_device *d, const double *pos, int n,
 
 /* ---- the device simulation ------------------------------------------------ */
 
+/* getChannelDriftCurrent (emcBasicParticleHandler.hpp:224-238): q * carriers
+ * per particle * sum of v_x over the particles with x0 <= x <= x1, in array
+ * order, / length */
+static double orc_channel_current(const orc_model *m, const orc_particle *p, const double *pos,
+                                  int n, const double chan[3], double nrCarriers) {
+  double sumVx = 0;
+  for (int i = 0; i < n; i++) {
+    if (pos[2 * i] < chan[0] || pos[2 * i] > chan[1])
+      continue;
+    double vel[3];
+    orc_valley_velocity(&m->valley[p[i].valley], p[i].k, p[i].energy, p[i].subValley, vel);
+    sumVx += vel[0];
+  }
+  return ORC_Q * nrCarriers * sumVx / chan[2];
+}
+
 int orc_device_run(const orc_device_run_cfg *cfg, double *eqPot, int *nInit,
                    int *nPartOut, double *potOut, int *exitOut, int *delOut,
                    int *injOut, double *nrPartLast, double *stateOut,
@@ -692,14 +708,18 @@ int orc_device_run(const orc_device_run_cfg *cfg, double *eqPot, int *nInit,
   orc_device_concentration(d, nrPart, conc);
 
   int resetBC = 1;
+  const int every = cfg->poissonInterval > 1 ? cfg->poissonInterval : 1;
   int *rem = (int *)malloc(sizeof(int) * (size_t)(e.n + 1));
   for (int s = 0; s < cfg->nSteps; s++) {
-    /* performEMCStep (emcSimulation.hpp:164-193) */
-    orc_sor_nonequilibrium(&sor, d, pot, conc, resetBC);
+    /* performEMCStep (emcSimulation.hpp:164-193); the field is kept between
+     * solves (solvePoisson, :116) */
+    if (s % every == 0) {
+      orc_sor_nonequilibrium(&sor, d, pot, conc, resetBC);
+      orc_pm_efield(cfg->pm, d, pot, ex, ey);
+    }
     resetBC = 0; /* showProgress at step 0 clears it (:118-121) */
     if (potOut)
       memcpy(potOut + (size_t)s * nn, pot, sizeof(double) * nn);
-    orc_pm_efield(cfg->pm, d, pot, ex, ey);
     /* driftScatterParticles (emcBasicParticleHandler.hpp:76-145) */
     int *exits = exitOut ? exitOut + (size_t)s * nc : NULL;
     if (exits)
@@ -761,6 +781,9 @@ int orc_device_run(const orc_device_run_cfg *cfg, double *eqPot, int *nInit,
     orc_device_concentration(d, nrPart, conc);
     if (nPartOut)
       nPartOut[s] = e.n;
+    /* accumulateDriftCurrent (emcSimulation.hpp:196-203) */
+    if (cfg->chanOut)
+      cfg->chanOut[s] = orc_channel_current(m, e.p, e.pos, e.n, cfg->chan, cfg->nrCarriers);
   }
   if (nrPartLast)
     memcpy(nrPartLast, nrPart, sizeof(double) * nn);
@@ -955,11 +978,14 @@ int orc_device_run_philox(const orc_device_run_cfg *cfg, double Vt, double *eqPo
     *nInit = q.e.n;
   orc_pm_assign(cfg->pm, q.e.pos, q.e.n, cfg->nrCarriers, d->h, d->ext, nrPart);
   orc_device_concentration(d, nrPart, conc);
+  const int every = cfg->poissonInterval > 1 ? cfg->poissonInterval : 1;
   for (int s = 0; s < cfg->nSteps; s++) {
-    orc_sor_nonequilibrium(&sor, d, pot, conc, s == 0);
+    if (s % every == 0) {
+      orc_sor_nonequilibrium(&sor, d, pot, conc, s == 0);
+      orc_pm_efield(cfg->pm, d, pot, ex, ey);
+    }
     if (potOut)
       memcpy(potOut + (size_t)s * nn, pot, sizeof(double) * nn);
-    orc_pm_efield(cfg->pm, d, pot, ex, ey);
     int *exits = exitOut ? exitOut + (size_t)s * nc : NULL;
     if (exits)
       memset(exits, 0, sizeof(int) * (size_t)nc);
@@ -1022,6 +1048,8 @@ int orc_device_run_philox(const orc_device_run_cfg *cfg, double Vt, double *eqPo
     orc_device_concentration(d, nrPart, conc);
     if (nPartOut)
       nPartOut[s] = q.e.n;
+    if (cfg->chanOut)
+      cfg->chanOut[s] = orc_channel_current(m, q.e.p, q.e.pos, q.e.n, cfg->chan, cfg->nrCarriers);
   }
   if (nrPartLast)
     memcpy(nrPartLast, nrPart, sizeof(double) * nn);
@@ -1060,8 +1088,10 @@ int orc_device_run_philox_h(orc_device *d, orc_model *m, int pm, double accuracy
                             double omega, double tStep, int nSteps, uint64_t seed, double Vt,
                             double *eqPot, int *nInit, int *nPartOut, double *potOut,
                             int *exitOut, int *delOut, int *injOut, double *nrPartLast,
-                            double *stateOut, int maxState) {
+                            double *stateOut, int maxState, int poissonInterval,
+                            double chanX0, double chanX1, double chanLen, double *chanOut) {
   orc_device_run_cfg cfg;
+  memset(&cfg, 0, sizeof cfg);
   cfg.dev = d;
   cfg.model = m;
   cfg.pm = pm;
@@ -1071,6 +1101,11 @@ int orc_device_run_philox_h(orc_device *d, orc_model *m, int pm, double accuracy
   cfg.nSteps = nSteps;
   cfg.seed = seed;
   cfg.nrCarriers = 1.0;
+  cfg.poissonInterval = poissonInterval;
+  cfg.chan[0] = chanX0;
+  cfg.chan[1] = chanX1;
+  cfg.chan[2] = chanLen;
+  cfg.chanOut = chanOut;
   return orc_device_run_philox(&cfg, Vt, eqPot, nInit, nPartOut, potOut, exitOut, delOut,
                                injOut, nrPartLast, stateOut, maxState);
 }
@@ -1242,8 +1277,10 @@ int orc_device_run_h(orc_device *d, orc_model *m, int pm, double accuracyV,
                      double omega, double tStep, int nSteps, uint64_t seed,
                      double *eqPot, int *nInit, int *nPartOut, double *potOut,
                      int *exitOut, int *delOut, int *injOut, double *nrPartLast,
-                     double *stateOut, int maxState) {
+                     double *stateOut, int maxState, int poissonInterval, double chanX0,
+                     double chanX1, double chanLen, double *chanOut) {
   orc_device_run_cfg cfg;
+  memset(&cfg, 0, sizeof cfg);
   cfg.dev = d;
   cfg.model = m;
   cfg.pm = pm;
@@ -1253,6 +1290,11 @@ int orc_device_run_h(orc_device *d, orc_model *m, int pm, double accuracyV,
   cfg.nSteps = nSteps;
   cfg.seed = seed;
   cfg.nrCarriers = 1.0;
+  cfg.poissonInterval = poissonInterval;
+  cfg.chan[0] = chanX0;
+  cfg.chan[1] = chanX1;
+  cfg.chan[2] = chanLen;
+  cfg.chanOut = chanOut;
   return orc_device_run(&cfg, eqPot, nInit, nPartOut, potOut, exitOut, delOut, injOut,
                         nrPartLast, stateOut, maxState);
 }

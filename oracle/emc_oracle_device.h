@@ -139,6 +139,14 @@ typedef struct {
   int nSteps;
   uint64_t seed;
   double nrCarriers;
+  /* setPoissonInterval (emcSimulation.hpp:84-86): Poisson + E-field on
+   * steps s with s % poissonInterval == 0 (<= 1: every step) */
+  int poissonInterval;
+  /* getChannelDriftCurrent (emcBasicParticleHandler.hpp:224-238) after
+   * every step into chanOut[s] when chanOut != NULL: region chan[0] <= x <=
+   * chan[1] [m], length chan[2] [m] */
+  double chan[3];
+  double *chanOut;
 } orc_device_run_cfg;
 
 int orc_device_run(const orc_device_run_cfg *cfg, double *eqPot, int *nInit,
@@ -174,7 +182,8 @@ int orc_device_run_philox_h(orc_device *d, orc_model *m, int pm, double accuracy
                             double omega, double tStep, int nSteps, uint64_t seed, double Vt,
                             double *eqPot, int *nInit, int *nPartOut, double *potOut,
                             int *exitOut, int *delOut, int *injOut, double *nrPartLast,
-                            double *stateOut, int maxState);
+                            double *stateOut, int maxState, int poissonInterval,
+                            double chanX0, double chanX1, double chanLen, double *chanOut);
 
 /* ---- handle API for language bindings (tests) ---------------------------- */
 orc_device *orc_device_new(double maxX, double maxY, double hx, double hy,
@@ -205,7 +214,8 @@ int orc_device_run_h(orc_device *d, orc_model *m, int pm, double accuracyV,
                      double omega, double tStep, int nSteps, uint64_t seed,
                      double *eqPot, int *nInit, int *nPartOut, double *potOut,
                      int *exitOut, int *delOut, int *injOut, double *nrPartLast,
-                     double *stateOut, int maxState);
+                     double *stateOut, int maxState, int poissonInterval, double chanX0,
+                     double chanX1, double chanLen, double *chanOut);
 
 #ifdef __cplusplus
 }

@@ -99,10 +99,11 @@ def load():
                                          _PI32, _PI32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
         "orc_device_run_h": (_I, [vp, vp, _I, _D, _D, _D, _I, C.c_uint64, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                  C.c_void_p, C.c_void_p, _I]),
+                                  C.c_void_p, C.c_void_p, _I, _I, _D, _D, _D, C.c_void_p]),
         "orc_device_run_philox_h": (_I, [vp, vp, _I, _D, _D, _D, _I, C.c_uint64, _D, C.c_void_p,
                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _I]),
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _I,
+                                         _I, _D, _D, _D, C.c_void_p]),
         "orc_mt64_fill": (None, [C.c_uint64, _PU64, C.c_size_t]),
         "orc_mt64_fill_particles": (None, [C.c_uint64, C.c_uint64, C.c_size_t, _PU64, C.c_size_t, C.c_size_t]),
         "orc_particle_seed": (C.c_uint64, [C.c_uint64, C.c_uint64]),
@@ -462,17 +463,24 @@ class OracleDevice:
                                      _ptr(tr.get("k")), _ptr(tr.get("E")), _ptr(tr.get("nev")))
         return rem, con, tr
 
-    def run(self, model, scheme, accuracy, omega, dt, nsteps, seed, max_state=0):
+    def run(self, model, scheme, accuracy, omega, dt, nsteps, seed, max_state=0, interval=1,
+            channel=None):
+        """the emcSimulation step order with the reference engine's streams;
+        interval: setPoissonInterval, channel = (x0, x1, length) [m]: the
+        drift-velocity channel current of every step (key "chan")"""
         nn, nc = self.n_nodes, len(self.spec["contacts"])
         out = dict(eq_pot=np.zeros(nn), n_init=np.zeros(1, dtype=np.int32),
                    n_part=np.zeros(nsteps, dtype=np.int32), pot=np.zeros(nsteps * nn),
                    exits=np.zeros(nsteps * nc, dtype=np.int32), dels=np.zeros(nsteps * nc, dtype=np.int32),
                    injs=np.zeros(nsteps * nc, dtype=np.int32), nr_part=np.zeros(nn),
                    state=np.zeros(max(max_state, 1) * 9))
+        ch = channel or (0.0, 0.0, 0.0)
+        out["chan"] = np.zeros(nsteps) if channel else None
         n = self.L.orc_device_run_h(self.h, model.h, scheme, accuracy, omega, dt, nsteps, seed,
                                     _ptr(out["eq_pot"]), _ptr(out["n_init"]), _ptr(out["n_part"]),
                                     _ptr(out["pot"]), _ptr(out["exits"]), _ptr(out["dels"]),
-                                    _ptr(out["injs"]), _ptr(out["nr_part"]), _ptr(out["state"]), max_state)
+                                    _ptr(out["injs"]), _ptr(out["nr_part"]), _ptr(out["state"]), max_state,
+                                    interval, ch[0], ch[1], ch[2], _ptr(out["chan"]))
         out["n_final"] = n
         out["pot"] = out["pot"].reshape(nsteps, nn)
         for k in ("exits", "dels", "injs"):
@@ -480,20 +488,24 @@ class OracleDevice:
         out["state"] = out["state"][:min(n, max_state) * 9].reshape(-1, 9)
         return out
 
-    def run_philox(self, model, scheme, accuracy, omega, dt, nsteps, seed, Vt, max_state=0):
+    def run_philox(self, model, scheme, accuracy, omega, dt, nsteps, seed, Vt, max_state=0,
+                   interval=1, channel=None):
         """the production twin of the GPU device run (Philox streams); state
-        rows: x, y, kx, ky, kz, E, tau, valley, sub, pid"""
+        rows: x, y, kx, ky, kz, E, tau, valley, sub, pid; interval and
+        channel as in run"""
         nn, nc = self.n_nodes, len(self.spec["contacts"])
         out = dict(eq_pot=np.zeros(nn), n_init=np.zeros(1, dtype=np.int32),
                    n_part=np.zeros(nsteps, dtype=np.int32), pot=np.zeros(nsteps * nn),
                    exits=np.zeros(nsteps * nc, dtype=np.int32), dels=np.zeros(nsteps * nc, dtype=np.int32),
                    injs=np.zeros(nsteps * nc, dtype=np.int32), nr_part=np.zeros(nn),
                    state=np.zeros(max(max_state, 1) * 10))
+        ch = channel or (0.0, 0.0, 0.0)
+        out["chan"] = np.zeros(nsteps) if channel else None
         n = self.L.orc_device_run_philox_h(
             self.h, model.h, scheme, accuracy, omega, dt, nsteps, seed, Vt, _ptr(out["eq_pot"]),
             _ptr(out["n_init"]), _ptr(out["n_part"]), _ptr(out["pot"]), _ptr(out["exits"]),
             _ptr(out["dels"]), _ptr(out["injs"]), _ptr(out["nr_part"]), _ptr(out["state"]),
-            max_state)
+            max_state, interval, ch[0], ch[1], ch[2], _ptr(out["chan"]))
         out["n_final"] = n
         out["pot"] = out["pot"].reshape(nsteps, nn)
         for k in ("exits", "dels", "injs"):

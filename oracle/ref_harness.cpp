@@ -40,6 +40,7 @@
 #include <PMSchemes/emcNGPScheme.hpp>
 #include <ParticleHandler/emcBasicParticleHandler.hpp>
 #include <PoissonSolver/emcSORSolver.hpp>
+#include <emcSimulation.hpp>
 
 using T = double;
 using Device = emcDevice<T, 3>;
@@ -635,9 +636,12 @@ template <class T2> static std::vector<double> gridVec(const emcGrid<T2, 2> &g) 
 
 // the emcSimulation step order (emcSimulation.hpp:106-193) with the
 // reference's solver, PM scheme and particle handler, one shared stream
+// channel-current region of the device runs [m]: x0, x1, length
+static const double kChan[3] = {5e-9, 15e-9, 10e-9};
+
 template <class PM>
 static void dumpDeviceRun(const std::string &tag, uint64_t seed, int nSteps, double dt,
-                          double accV, double omega) {
+                          double accV, double omega, int interval = 1) {
   auto dev = makeDevice2();
   PM pm;
   TypeMap2 types;
@@ -674,15 +678,17 @@ static void dumpDeviceRun(const std::string &tag, uint64_t seed, int nSteps, dou
   handler.assignParticlesToMesh(0, nrPart);
   concentration();
   seci((b + "ninit").c_str(), {(long long)handler.getNrParticles(0)});
-  std::vector<double> pots;
+  std::vector<double> pots, chan;
   std::vector<long long> npart, exits, netto;
   bool resetBC = true;
   for (int s = 0; s < nSteps; s++) {
-    solver.calcNonEquilibriumPotential(pot, *dev, conc, resetBC);
+    if (s % interval == 0) {  // setPoissonInterval: the field is kept in between
+      solver.calcNonEquilibriumPotential(pot, *dev, conc, resetBC);
+      pm.calcEField(eField, pot, *dev);
+    }
     resetBC = false;
     auto pv = gridVec(pot);
     pots.insert(pots.end(), pv.begin(), pv.end());
-    pm.calcEField(eField, pot, *dev);
     auto rem = handler.driftScatterParticles(dt, eField);
     auto inj = handler.handleOhmicContacts();
     for (int k = 0; k < 2; k++) {
@@ -693,13 +699,41 @@ static void dumpDeviceRun(const std::string &tag, uint64_t seed, int nSteps, dou
     handler.assignParticlesToMesh(0, nrPart);
     concentration();
     npart.push_back((long long)handler.getNrParticles(0));
+    chan.push_back(handler.getChannelDriftCurrent(0, kChan[0], kChan[1], kChan[2]));
   }
   sec((b + "pot").c_str(), pots);
   seci((b + "npart").c_str(), npart);
   seci((b + "exits").c_str(), exits);
   seci((b + "ohmic").c_str(), netto);
   sec((b + "nrpart_last").c_str(), gridVec(nrPart));
-  sec((b + "cfg").c_str(), {(double)seed, (double)nSteps, dt, accV, omega});
+  sec((b + "cfg").c_str(), {(double)seed, (double)nSteps, dt, accV, omega, (double)interval});
+  sec((b + "chan").c_str(), chan);
+}
+
+// the reference's own emcSimulation::execute on the same device: its result
+// files (Eq / Final potential, concentration, E-field, particles, current,
+// averages) are written into the working directory; the harness records
+// the averaged drift current
+static void dumpSimulationFiles(uint64_t seed) {
+  using PM = emcCICScheme<T, Device2>;
+  using Handler2 = emcBasicParticleHandler<T, Device2, PM>;
+  using Solver2 = emcSORSolver<T, Device2, Handler2>;
+  using Sim = emcSimulation<T, Device2, Solver2, Handler2, PM>;
+  auto dev = makeDevice2();
+  Solver2 solver(*dev, 1e-8, 1.8);
+  emcSimulationParameter<T, Device2> param;
+  param.setTimes(30e-15, 1e-15, 10e-15);
+  param.setSeed(seed);
+  param.setNamePrefix("sim_");
+  param.setNrStepsBetweenShowProgress(1000);
+  param.setNrStepsForFinalAvg(8);
+  param.addParticleType(makeDeviceType(*dev));
+  Sim sim(param, *dev, solver, PM());
+  sim.setPoissonInterval(2);
+  sim.setChannelCurrentRegion(kChan[0], kChan[1], kChan[2]);
+  sim.execute();
+  sec("sim.avg_drift_current", {sim.getAvgDriftCurrent()});
+  sec("sim.cfg", {(double)seed, 30e-15, 1e-15, 10e-15, 8, 2});
 }
 
 // PM schemes, E-field and SOR known answers on the same device
@@ -921,6 +955,8 @@ int main(int argc, char **argv) {
   dumpDeviceTrajectories(seed + 5, 64, 150, 1e-15);
   dumpDeviceRun<emcCICScheme<T, Device2>>("cic", seed + 11, 40, 1e-15, 1e-8, 1.8);
   dumpDeviceRun<emcNGPScheme<T, Device2>>("ngp", seed + 13, 40, 1e-15, 1e-8, 1.8);
+  dumpDeviceRun<emcCICScheme<T, Device2>>("cic_sub3", seed + 17, 40, 1e-15, 1e-8, 1.8, 3);
+  dumpSimulationFiles(seed + 19);
   for (const auto &m : models) {
     double mp = 4e-8;
     Device device{silicon(), {mp, mp, mp}, {1e-8, 1e-8, 1e-8}, m.T};

@@ -3,7 +3,8 @@
 
 Builds oracle/_ref/ref_harness (our driver compiled against the ViennaEMC
 headers where they lie, see oracle/Makefile), runs it in a scratch folder
-under oracle/_ref/, and packs what it recorded into
+under oracle/_ref/, and packs what it recorded, together with the result
+files the reference's emcSimulation wrote there (text, as bytes), into
 tests/golden/reference_golden.npz.  Only runs where the reference sources
 are present; the committed .npz is what the tests read.
 
@@ -81,6 +82,11 @@ def main():
     with open(os.path.join(ORACLE, "_ref", "harness_stdout.txt"), "w") as log:
         subprocess.check_call([os.path.join(ORACLE, "_ref", "ref_harness"), raw], cwd=work, stdout=log)
     d = shrink(parse(raw))
+    # result files written by the reference's emcSimulation::execute (text)
+    for name in sorted(os.listdir(work)):
+        if name.startswith("sim_") and name.endswith(".txt"):
+            with open(os.path.join(work, name), "rb") as f:
+                d["simfile." + name[:-4]] = np.frombuffer(f.read(), dtype=np.uint8)
     d["_meta.compiler"] = np.frombuffer(
         subprocess.check_output(["g++", "--version"]).splitlines()[0], dtype=np.uint8)
     np.savez_compressed(args.out, **d)

@@ -2,7 +2,10 @@
 
 CPU: the reference's bulk-silicon particle type written against the host API
 lowers to the same scatter rates and tau0 as the reference (golden values,
-bit-exact) and keeps the reference's subvalley-frame validation.
+bit-exact) and keeps the reference's subvalley-frame validation.  The
+result-file writers of device simulations reproduce, byte for byte, the
+files the reference's emcSimulation wrote (grids, contact currents,
+particle dumps) from the numbers in them.
 GPU: the C++ example (bulkSimulationGpu, the reference's bulkSimulation
 written against the host API) produces exactly the observables of the
 Python binding on the same seed.
@@ -59,3 +62,27 @@ def test_cpp_example_matches_python_binding():
         assert float(r[2]) == av["avgE"][s, 0]
         assert float(r[3]) == av["avgV"][s, 0]
         assert float(r[4]) == av["occ"][s, 0]
+
+
+@pytest.fixture(scope="module")
+def output_binary(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("cpp") / "output_format")
+    subprocess.check_call(["g++", "-std=c++17", "-O2", "-I", os.path.join(PKG, "include"),
+                           os.path.join(ROOT, "tests", "cpp", "output_format.cpp"), "-o", out])
+    return out
+
+
+@pytest.mark.parametrize("name,mode", [
+    ("sim_PotentialEq", "grid"), ("sim_PotentialAvg", "grid"), ("sim_EFieldXEq", "grid"),
+    ("sim_EFieldYEq", "grid"), ("sim_ElectronsConcEq", "grid"), ("sim_ElectronsConcAvg", "grid"),
+    ("sim_ElectronsCurrent", "current"), ("sim_ElectronsEq", "particles"),
+    ("sim_ElectronsFinal", "particles")])
+def test_result_files_match_the_reference_format(output_binary, golden, tmp_path, name, mode):
+    text = bytes(golden["simfile." + name])
+    src = tmp_path / (name + ".txt")
+    src.write_bytes(text)
+    args = [output_binary, mode, str(src)]
+    if mode == "current":
+        _, t_sim, dt, t_tr, _, _ = golden["sim.cfg"]
+        args += ["2", repr(float(dt)), repr(float(t_tr))]
+    assert subprocess.check_output(args) == text

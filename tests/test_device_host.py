@@ -92,7 +92,8 @@ int main(void) {
   O(emcgpu_device_desc, poisson_tol); O(emcgpu_device_drift_args, draws);
   O(emcgpu_device_drift_args, tr_events); O(emcgpu_device_step_args, checksum);
   O(emcgpu_autotune_args, spl); O(emcgpu_autotune_args, res_ms_per_step);
-  O(emcgpu_autotune_args, best_ms_per_step);
+  O(emcgpu_autotune_args, best_ms_per_step); O(emcgpu_device_step_args, poisson_interval);
+  O(emcgpu_device_step_args, channel_current); O(emcgpu_device_step_args, accumulate);
   return 0;
 }
 """

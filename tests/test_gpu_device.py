@@ -12,7 +12,14 @@
   injection counts exact;
 * a self-consistent run (Poisson every step, contacts, injection, charge
   assignment): conservation of the ensemble counters, the FP64 check sum,
-  Poisson convergence, graph launch equal to eager launch.
+  Poisson convergence, graph launch equal to eager launch;
+* Poisson every third step and the drift-velocity channel current (over the
+  middle of the device, and over all of it, where deleted and injected
+  particles count) against the oracle's Philox twin; the running averages
+  of potential and concentration against the per-step fields;
+* the C++ host API's emcSimulation (resistor2DGpu example): the result
+  files of the reference's layout, its equilibrium potential against the
+  reference's, its contact currents against the per-step tallies.
 """
 import numpy as np
 import pytest
@@ -292,4 +299,134 @@ def test_self_consistent_run_matches_oracle_twin():
     assert dpos < 1e-9 and dk < 1e-9 and dE < 1e-9
     f = ctx.device_fields()
     assert np.allclose(f["nr_part"], ref["nr_part"], rtol=1e-9, atol=1e-9)
+    ctx.close()
+
+
+@pytest.mark.parametrize("region", ["middle", "whole"])
+def test_poisson_interval_and_channel_current_match_oracle_twin(region):
+    """setPoissonInterval(3) + setChannelCurrentRegion against the oracle's
+    twin: counts per step and contact exact, solves only on every third
+    step, channel current per step within 1e-6 of the twin's (the particles
+    agree to ~1e-10, the sums cancel in part).  Over the whole device the
+    particles deleted or injected at the contacts are inside the region, so
+    the current includes the step kernel's corrections for them."""
+    emcgpu = _emc()
+    dev = dict(DEVICE_SMALL, width=1e-5)
+    seed, ns, dt, every = 5150, 45, 1e-15, 3
+    chan = (5e-9, 15e-9, 10e-9) if region == "middle" else (0.0, 20e-9, 20e-9)
+    od = O.OracleDevice(dev)
+    M = O.OracleModel(MODELS_DEVICE["device_para"])
+    ref = od.run_philox(M, O.PM_CIC, 1e-14, 1.8, dt, ns, seed, O.thermal_voltage(300.0),
+                        interval=every, channel=chan)
+    print("interval / channel twin (%s): %d particles" % (region, ref["n_init"][0]), flush=True)
+    ctx = _ctx(dev=dev, tol=1e-14, max_iter=80)
+    assert ctx.device_init(seed) == ref["n_init"][0]
+    out = ctx.device_step(dt, ns, seed=seed, poisson_interval=every, channel=chan)
+    assert (out["n_particles"] == ref["n_part"]).all()
+    assert (out["exits"] == ref["exits"]).all()
+    assert (out["deleted"] == ref["dels"]).all()
+    assert (out["injected"] == ref["injs"]).all()
+    solved = out["poisson_iters"] > 0
+    assert (solved == (np.arange(ns) % every == 0)).all()
+    if region == "whole":
+        assert out["deleted"].sum() + out["injected"].sum() > 50
+    c, r = out["channel_current"], ref["chan"]
+    print("channel current: max |gpu - twin| / max |twin| = %.2e"
+          % (np.abs(c - r).max() / np.abs(r).max()), flush=True)
+    assert np.allclose(c, r, rtol=1e-6, atol=1e-6 * np.abs(r).max())
+    ctx.close()
+
+
+def test_running_averages_equal_the_mean_of_the_step_fields():
+    """updateAverageCharacteristics on the GPU: after steps run with
+    accumulate, the averages are the mean of the potential and the
+    concentration each of those steps left (same summation order:
+    bit-exact); a reset clears them."""
+    emcgpu = _emc()
+    ctx = _ctx(tol=1e-12)
+    ctx.device_init(99)
+    ctx.device_step(1e-15, 20, seed=99, poisson_interval=2)
+    pot_sum, conc_sum = np.zeros(ctx.nn), np.zeros(ctx.nn)
+    n = 12
+    for _ in range(n):
+        ctx.device_step(1e-15, 1, seed=99, poisson_interval=2, accumulate=True)
+        f = ctx.device_fields()
+        pot_sum += f["pot"]
+        conc_sum += f["conc"]
+    pot, conc, k = ctx.device_averages(reset=True)
+    assert k == n
+    assert (pot == pot_sum / n).all() and (conc == conc_sum / n).all()
+    assert ctx.device_averages()[2] == 0
+    ctx.close()
+
+
+def _grid(text):
+    lines = text.strip().split("\n")
+    nx, ny = (int(x) for x in lines[0].split())
+    g = np.array([[float(x) for x in l.split()] for l in lines[1:]])
+    assert g.shape == (ny, nx)
+    return g
+
+
+def test_cpp_simulation_writes_the_reference_results(golden, tmp_path):
+    """resistor2DGpu (small) is the harness's emcSimulation run (same device,
+    times, Poisson interval, channel, averaging) on the GPU: it writes the
+    same nine files with the same layout; its equilibrium potential and
+    field match the reference's files to their printed digits; its contact
+    current file is the running mean of its own per-step tallies, which the
+    Python binding reproduces on the same seed."""
+    import os
+    import subprocess
+    emcgpu = _emc()
+    exe = os.path.join(os.path.dirname(emcgpu.__file__), "examples", "resistor2DGpu")
+    seed = 3131
+    prefix = str(tmp_path / "sim_")
+    print("cpp device example", flush=True)
+    txt = subprocess.check_output([exe, prefix, "small", "fp64", str(seed)],
+                                  env=dict(os.environ, EMCGPU_THROW="1"), timeout=120).decode()
+    names = sorted(k[len("simfile."):] for k in golden.files if k.startswith("simfile."))
+    for name in names:
+        ours = (tmp_path / (name + ".txt")).read_text()
+        theirs = bytes(golden["simfile." + name]).decode()
+        lo, lt = ours.split("\n"), theirs.split("\n")
+        if name.endswith(("Eq", "Final")) and name.startswith("sim_Electrons") \
+                and "Conc" not in name:
+            # particle dumps: the ensemble sizes differ with the random
+            # numbers; header, columns and numbering as the reference's
+            assert lo[0] == lt[0], name
+            assert abs(len(lo) - len(lt)) < 0.05 * len(lt), name
+            for i, line in enumerate(lo[1:]):
+                t = line.split()
+                assert len(t) == 10 and int(t[0]) == i, (name, i)
+            assert not ours.endswith("\n"), name
+            continue
+        assert len(lo) == len(lt), name
+        # header (extent), or the step time of the first current line
+        first = lo[0], lt[0]
+        if name.endswith("Current"):
+            first = first[0].split()[0], first[1].split()[0]
+        assert first[0] == first[1], name
+    for name in ("sim_PotentialEq",):
+        ours = _grid((tmp_path / (name + ".txt")).read_text())
+        theirs = _grid(bytes(golden["simfile." + name]).decode())
+        assert np.allclose(ours, theirs, rtol=2e-5, atol=1e-9), name
+    # E-field: printed digits and the SOR's accuracy (1e-8 V over 1 nm)
+    e_scale = np.abs(_grid(bytes(golden["simfile.sim_EFieldXEq"]).decode())).max()
+    for name in ("sim_EFieldXEq", "sim_EFieldYEq"):
+        ours = _grid((tmp_path / (name + ".txt")).read_text())
+        theirs = _grid(bytes(golden["simfile." + name]).decode())
+        assert np.allclose(ours, theirs, rtol=1e-4, atol=1e-4 * e_scale), name
+    # the same run through the Python binding: contact tallies -> currents
+    ctx = _ctx(tol=1e-10, max_iter=50)
+    ctx.device_init(seed)
+    out = ctx.device_step(1e-15, 30, seed=seed, poisson_interval=2)
+    netto = (out["exits"] - (out["injected"] - out["deleted"]))[10:]
+    cur = np.cumsum(netto, axis=0) / np.arange(1, 21)[:, None] * (-O.Q / 1e-15)
+    rows = [[float(x) for x in l.split()]
+            for l in (tmp_path / "sim_ElectronsCurrent.txt").read_text().strip().split("\n")]
+    assert len(rows) == 20
+    for s, r in enumerate(rows):
+        assert r[1:3] == netto[s].tolist(), s
+        assert np.allclose(r[3:5], cur[s], rtol=1e-5, atol=0), s
+    assert "drift current" in txt
     ctx.close()

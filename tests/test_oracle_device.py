@@ -10,8 +10,12 @@ both ends):
   with per-particle streams (drift with positions, boundary reflection,
   removal at contacts): bit-exact;
 * full simulations in the emcSimulation step order with one shared stream
-  (equilibrium, initial particles, Poisson every step, drift/scatter, ohmic
-  contact equilibrium, charge assignment), CIC and NGP: bit-exact.
+  (equilibrium, initial particles, Poisson every step or every third step,
+  drift/scatter, ohmic contact equilibrium, charge assignment, the
+  drift-velocity channel current), CIC and NGP: bit-exact;
+* the reference's own emcSimulation::execute (Poisson every second step,
+  transient steps, final averages): its averaged channel current, contact
+  current file and averaged potential file from the oracle's run.
 The mathematics pins what the fixtures leave open: CIC weights of every
 particle sum to one, NGP mass lands on the nearest node, the SOR fixed point
 solves the discrete Poisson equation.
@@ -136,11 +140,22 @@ def test_device_trajectories_bit_exact(dev, golden):
     assert (cursor == total).all()
 
 
-@pytest.mark.parametrize("pm,tag", [(O.PM_CIC, "cic"), (O.PM_NGP, "ngp")])
+# drift-velocity channel current region of the harness runs [m]
+CHANNEL = (5e-9, 15e-9, 10e-9)
+
+
+@pytest.mark.parametrize("pm,tag", [(O.PM_CIC, "cic"), (O.PM_NGP, "ngp"), (O.PM_CIC, "cic_sub3")])
 def test_device_simulation_bit_exact(dev, golden, pm, tag):
     b = "device." + tag + "."
-    seed, nsteps, dt, acc, omega = golden[b + "cfg"]
-    out = dev.run(_model(), pm, acc, omega, dt, int(nsteps), int(seed))
+    seed, nsteps, dt, acc, omega, interval = golden[b + "cfg"]
+    out = dev.run(_model(), pm, acc, omega, dt, int(nsteps), int(seed), interval=int(interval),
+                  channel=CHANNEL)
+    # getChannelDriftCurrent after every step
+    assert (out["chan"] == golden[b + "chan"]).all()
+    if interval > 1:  # the potential changes only on solve steps
+        pot = out["pot"]
+        for s in range(1, int(nsteps)):
+            assert (s % int(interval) == 0) != (pot[s] == pot[s - 1]).all(), s
     assert (out["eq_pot"] == golden[b + "eqpot"]).all()
     assert out["n_init"][0] == golden[b + "ninit"][0]
     assert (out["pot"].reshape(-1) == golden[b + "pot"]).all()
@@ -167,3 +182,35 @@ def test_contact_rule_keeps_the_first_in_array_order(dev):
     assert inj[node] == 0
     keep, inj, exp = dev.contact_rule(pos[:2])
     assert inj[node] == int(np.ceil(exp[node] - 2)) and list(keep) == [1, 1]
+
+
+def _cols(text):
+    return [[float(x) for x in line.split()] for line in text.strip().split("\n")]
+
+
+def test_emc_simulation_results_from_the_oracle_run(dev, golden):
+    """emcSimulation::execute with setPoissonInterval(2) and a channel
+    region: transient steps are excluded from the currents, the contact
+    current is the running mean of (left - (injected - deleted)) * charge /
+    dt, the drift current is the mean over the non-transient steps, and the
+    final potential average covers the last nrStepsForFinalAvg steps."""
+    seed, t_sim, dt, t_tr, n_avg, interval = golden["sim.cfg"]
+    n_steps, n_tr = int(round(t_sim / dt)), int(round(t_tr / dt))
+    out = dev.run(_model(), O.PM_CIC, 1e-8, 1.8, dt, n_steps, int(seed), interval=int(interval),
+                  channel=CHANNEL)
+    chan = out["chan"][n_tr:]
+    assert np.isclose(chan.mean(), golden["sim.avg_drift_current"][0], rtol=1e-12, atol=0)
+    netto = (out["exits"] - (out["injs"] - out["dels"]))[n_tr:]
+    current = np.cumsum(netto, axis=0) / np.arange(1, len(netto) + 1)[:, None] * (-O.Q / dt)
+    rows = _cols(bytes(golden["simfile.sim_ElectronsCurrent"]).decode())
+    assert len(rows) == n_steps - n_tr
+    for s, r in enumerate(rows):
+        assert np.isclose(r[0], t_tr + (s + 1) * dt, rtol=1e-5)
+        assert r[1:3] == netto[s].tolist()
+        assert np.allclose(r[3:5], current[s], rtol=1e-5, atol=0)
+    lines = bytes(golden["simfile.sim_PotentialAvg"]).decode().strip().split("\n")
+    nx, ny = (int(x) for x in lines[0].split())
+    avg = np.array([[float(x) for x in l.split()] for l in lines[1:]])
+    expect = out["pot"][n_steps - int(n_avg):].mean(axis=0).reshape(ny, nx) * dev.Vt
+    assert avg.shape == (ny, nx)
+    assert np.allclose(avg, expect, rtol=1e-5, atol=1e-9)

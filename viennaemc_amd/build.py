@@ -62,7 +62,7 @@ def build(verbose=True):
     return LIB
 
 
-EXAMPLES = ["bulkSimulationGpu"]
+EXAMPLES = ["bulkSimulationGpu", "resistor2DGpu"]
 
 
 def build_examples(verbose=True):

@@ -205,6 +205,7 @@ struct DevState {
   long long capacity;
   int overflow;
   unsigned int ticket;  // contact-scan completion counter
+  long long avg_n;      // steps added to the running potential / concentration sums
 };
 
 // per-step records [row][...] (row = step - hist_base)
@@ -213,6 +214,7 @@ struct DevHist {
   long long *n_part;         // [row]
   int *iters;                // [row]
   double *resid, *checksum;  // [row]
+  double *chan;              // [row] drift-velocity (Ramo) channel current [A]
   int rows;
 };
 
@@ -295,6 +297,7 @@ struct DevFieldArgs {
   DevState *state;
   DevHist hist;
   int record;                  // write iters / resid / check sum to the history
+  int interval;                // recorded steps: solve only when step % interval == 0
 };
 
 // particle step of a device ensemble
@@ -314,6 +317,11 @@ struct DevStepArgs {
   uint2 *stage;           // (particle, slot << 22 | rank in segment) at segment start
   int lds_dep;            // deposit grid in LDS (else straight to dep row 0 with atomics)
   int move;              // 0: classify positions only (contact rule of given positions)
+  // drift-velocity channel current: sum of v_x over the moved particles
+  // with chan_x0 <= x <= chan_x1 [nm], one FP64 partial per block
+  int chan;
+  R chan_x0, chan_x1;
+  double *chan_part;      // [block]
   // frozen-field drift (emcgpu_device_drift)
   long long n;            // particles (drift)
   uint32_t step;          // first Philox step (drift)
@@ -354,6 +362,18 @@ struct DevContactArgs {
   double *nrpart, *conc;
   const double *cfac;      // conc = nrpart * cfac
   R vt;                    // thermal voltage [V]
+  // channel current: block partials of the step kernel, FP64 corrections of
+  // the deleted / injected particles, scale q * carriers / length [A s/m]
+  int chan;
+  R chan_x0, chan_x1;
+  const double *chan_part;
+  int n_chan_blocks;
+  double *chan_corr;
+  double chan_scale;
+  // running sums of the potential and concentration (final averages)
+  int accumulate;
+  const double *pot;
+  double *avg_pot, *avg_conc;
 };
 
 }  // namespace emcgpu

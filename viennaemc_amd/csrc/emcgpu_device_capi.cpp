@@ -131,13 +131,24 @@ struct DeviceSim {
   int hist_rows = 0;
   int *d_hexits = nullptr, *d_hdels = nullptr, *d_hinjs = nullptr, *d_hiters = nullptr;
   long long *d_hnpart = nullptr;
-  double *d_hresid = nullptr, *d_hcheck = nullptr;
+  double *d_hresid = nullptr, *d_hcheck = nullptr, *d_hchan = nullptr;
+  // options of the current emcgpu_device_step call
+  int interval = 1;                 // Poisson solve every interval-th step
+  int chan = 0;                     // drift-velocity channel current
+  double chan_x0 = 0, chan_x1 = 0;  // [nm]
+  double chan_scale = 0;            // q * carriers / length [A s/m]
+  int accumulate = 0;               // add potential / concentration to the sums
+  double *d_chan_part = nullptr, *d_chan_corr = nullptr;
+  double *d_avg_pot = nullptr, *d_avg_conc = nullptr;
   // run state
   bool initialized = false, bc_applied = false;
   uint64_t seed = 0;
   hipGraphExec_t gexec = nullptr;
   int graph_steps = 0, graph_cur = -1;
   long long graph_cap = 0;
+  // kernel arguments the captured graph was recorded with
+  int graph_chan = -1, graph_acc = -1;
+  double graph_x0 = 0, graph_x1 = 0, graph_scale = 0;
 };
 
 void dev_free_particles(DeviceSim *d) {
@@ -214,6 +225,11 @@ void device_sim_destroy(DeviceSim *d) {
     dfreet(d->d_hnpart);
     dfreet(d->d_hresid);
     dfreet(d->d_hcheck);
+    dfreet(d->d_hchan);
+    dfreet(d->d_chan_part);
+    dfreet(d->d_chan_corr);
+    dfreet(d->d_avg_pot);
+    dfreet(d->d_avg_conc);
     if (d->gexec)
       (void)hipGraphExecDestroy(d->gexec);
   }
@@ -747,6 +763,13 @@ emcgpu_status upload_device(emcgpu_ctx *c, DeviceSim &d) {
   ST(dalloc(c, d.d_seg_cnt, ns * d.n_seg));
   ST(dalloc(c, d.d_seg_off, ns * d.n_seg));
   ST(dalloc(c, d.d_seg_nst, d.n_seg));
+  ST(dalloc(c, d.d_chan_part, (size_t)d.blocks));
+  ST(dalloc(c, d.d_chan_corr, 1));
+  ST(dalloc(c, d.d_avg_pot, nn));
+  ST(dalloc(c, d.d_avg_conc, nn));
+  HIPCHK(c, hipMemset(d.d_chan_corr, 0, sizeof(double)));
+  HIPCHK(c, hipMemset(d.d_avg_pot, 0, sizeof(double) * nn));
+  HIPCHK(c, hipMemset(d.d_avg_conc, 0, sizeof(double) * nn));
   d.on_gpu = true;
   return EMCGPU_OK;
 }
@@ -761,6 +784,7 @@ emcgpu_status ensure_hist(emcgpu_ctx *c, DeviceSim &d, int rows) {
     ST(dalloc(c, d.d_hnpart, rows));
     ST(dalloc(c, d.d_hresid, rows));
     ST(dalloc(c, d.d_hcheck, rows));
+    ST(dalloc(c, d.d_hchan, rows));
     d.hist_rows = rows;
     if (d.gexec) {
       (void)hipGraphExecDestroy(d.gexec);
@@ -779,6 +803,7 @@ DevHist hist_of(const DeviceSim &d) {
   h.iters = d.d_hiters;
   h.resid = d.d_hresid;
   h.checksum = d.d_hcheck;
+  h.chan = d.d_hchan;
   h.rows = d.hist_rows;
   return h;
 }
@@ -971,6 +996,10 @@ DevStepArgs<R> step_args(const emcgpu_ctx *c, const DeviceSim &d) {
   a.stage = d.d_stage;
   a.lds_dep = d.lds_dep ? 1 : 0;
   a.move = 1;
+  a.chan = d.chan;
+  a.chan_x0 = (R)d.chan_x0;
+  a.chan_x1 = (R)d.chan_x1;
+  a.chan_part = d.d_chan_part;
   return a;
 }
 
@@ -1006,6 +1035,17 @@ DevContactArgs<R> contact_args(const emcgpu_ctx *c, const DeviceSim &d) {
   a.nrpart = d.d_nrpart;
   a.conc = d.d_conc;
   a.cfac = d.d_cfac;
+  a.chan = d.chan;
+  a.chan_x0 = (R)d.chan_x0;
+  a.chan_x1 = (R)d.chan_x1;
+  a.chan_part = d.d_chan_part;
+  a.n_chan_blocks = d.blocks;
+  a.chan_corr = d.d_chan_corr;
+  a.chan_scale = d.chan_scale;
+  a.accumulate = d.accumulate;
+  a.pot = d.d_pot;
+  a.avg_pot = d.d_avg_pot;
+  a.avg_conc = d.d_avg_conc;
   return a;
 }
 
@@ -1065,6 +1105,7 @@ DevFieldArgs<R> step_field_args(const emcgpu_ctx *c, const DeviceSim &d, bool re
   f.do_efield = 1;
   f.record = 1;
   f.nrpart = d.d_nrpart;
+  f.interval = d.interval;
   return f;
 }
 
@@ -1092,6 +1133,23 @@ emcgpu_status compact(emcgpu_ctx *c, DeviceSim &d, long long n_total) {
 template <typename R>
 emcgpu_status run_device_steps(emcgpu_ctx *c, DeviceSim &d, const emcgpu_device_step_args *s) {
   ST(need_model(c, s->dt));
+  if (s->poisson_interval < 0)
+    return fail(c, EMCGPU_ERR_ARG, "poisson_interval must be >= 0");
+  d.interval = s->poisson_interval > 1 ? s->poisson_interval : 1;
+  d.chan = s->channel_length > 0 ? 1 : 0;
+  if (d.chan) {
+    if (!(s->channel_x1 >= s->channel_x0))
+      return fail(c, EMCGPU_ERR_ARG, "channel region: x1 must be >= x0");
+    d.chan_x0 = s->channel_x0 * kNm;
+    d.chan_x1 = s->channel_x1 * kNm;
+    // getChannelDriftCurrent (emcBasicParticleHandler.hpp:224-238): q *
+    // carriers per particle * sum v_x / length (electrons: charge -q is not
+    // applied, as in the reference)
+    d.chan_scale = phys::q / s->channel_length;
+  } else {
+    d.chan_x0 = d.chan_x1 = d.chan_scale = 0;
+  }
+  d.accumulate = s->accumulate ? 1 : 0;
   ST(ensure_hist(c, d, s->n_steps));
   const size_t nc = std::max(d.nContacts, 1);
   HIPCHK(c, dev_launch_zero_i(d.d_hexits, (long long)s->n_steps * nc, c->stream));
@@ -1122,7 +1180,9 @@ emcgpu_status run_device_steps(emcgpu_ctx *c, DeviceSim &d, const emcgpu_device_
       continue;
     }
     if (s->use_graph && k == kChunk) {
-      if (!d.gexec || d.graph_cur != d.cur || d.graph_cap != d.capacity) {
+      if (!d.gexec || d.graph_cur != d.cur || d.graph_cap != d.capacity ||
+          d.graph_chan != d.chan || d.graph_acc != d.accumulate || d.graph_x0 != d.chan_x0 ||
+          d.graph_x1 != d.chan_x1 || d.graph_scale != d.chan_scale) {
         if (d.gexec) {
           (void)hipGraphExecDestroy(d.gexec);
           d.gexec = nullptr;
@@ -1143,6 +1203,11 @@ emcgpu_status run_device_steps(emcgpu_ctx *c, DeviceSim &d, const emcgpu_device_
           return fail(c, EMCGPU_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
         d.graph_cur = d.cur;
         d.graph_cap = d.capacity;
+        d.graph_chan = d.chan;
+        d.graph_acc = d.accumulate;
+        d.graph_x0 = d.chan_x0;
+        d.graph_x1 = d.chan_x1;
+        d.graph_scale = d.chan_scale;
       }
       HIPCHK(c, hipGraphLaunch(d.gexec, c->stream));
     } else {
@@ -1175,6 +1240,8 @@ emcgpu_status run_device_steps(emcgpu_ctx *c, DeviceSim &d, const emcgpu_device_
   ST(get(s->poisson_iters, d.d_hiters, ns * sizeof(int)));
   ST(get(s->poisson_resid, d.d_hresid, ns * sizeof(double)));
   ST(get(s->checksum, d.d_hcheck, ns * sizeof(double)));
+  if (d.chan)
+    ST(get(s->channel_current, d.d_hchan, ns * sizeof(double)));
   return EMCGPU_OK;
 }
 
@@ -1242,6 +1309,11 @@ emcgpu_status device_init(emcgpu_ctx *c, DeviceSim &d, uint64_t seed, size_t *n_
   s.next_pid = n0;
   s.capacity = d.capacity;
   ST(write_state(c, d, s));
+  d.chan = 0;
+  d.accumulate = 0;
+  HIPCHK(c, hipMemset(d.d_avg_pot, 0, sizeof(double) * d.nn));
+  HIPCHK(c, hipMemset(d.d_avg_conc, 0, sizeof(double) * d.nn));
+  HIPCHK(c, hipMemset(d.d_chan_corr, 0, sizeof(double)));
   const DevContactArgs<R> ca = contact_args<R>(c, d);
   hipError_t e = dev_launch_init<R>(ca, d_prefix, n0, c->stream);
   if (e == hipSuccess)
@@ -1805,6 +1877,37 @@ emcgpu_status emcgpu_device_fields(const emcgpu_ctx *c0, double *pot, double *ex
     HIPCHK(c, hipMemcpy(conc, d.d_conc, b, hipMemcpyDeviceToHost));
   if (nr_part)
     HIPCHK(c, hipMemcpy(nr_part, d.d_nrpart, b, hipMemcpyDeviceToHost));
+  return EMCGPU_OK;
+}
+
+emcgpu_status emcgpu_device_averages(emcgpu_ctx *c, double *pot, double *conc,
+                                     long long *n_steps, int reset) {
+  emcgpu_status st = need_device(c);
+  if (st)
+    return st;
+  DeviceSim &d = *c->dsim;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  DevState s{};
+  ST(read_state(c, d, s));
+  if (n_steps)
+    *n_steps = s.avg_n;
+  std::vector<double> buf(d.nn);
+  for (int q = 0; q < 2; q++) {
+    double *out = q == 0 ? pot : conc;
+    if (!out)
+      continue;
+    HIPCHK(c, hipMemcpy(buf.data(), q == 0 ? d.d_avg_pot : d.d_avg_conc, sizeof(double) * d.nn,
+                        hipMemcpyDeviceToHost));
+    // writeFinalResults (emcSimulationResults.hpp:181-183): sum / steps
+    for (int k = 0; k < d.nn; k++)
+      out[k] = s.avg_n > 0 ? buf[k] / (double)s.avg_n : 0.0;
+  }
+  if (reset) {
+    HIPCHK(c, hipMemset(d.d_avg_pot, 0, sizeof(double) * d.nn));
+    HIPCHK(c, hipMemset(d.d_avg_conc, 0, sizeof(double) * d.nn));
+    s.avg_n = 0;
+    ST(write_state(c, d, s));
+  }
   return EMCGPU_OK;
 }
 

@@ -167,6 +167,19 @@ __device__ __forceinline__ bool drift_dev(const InstDev<R> *P, const DevGeom<R> 
   return false;
 }
 
+// x component of getVelocity in the device frame [m/s]
+// (emcNonParabolicAnistropValley.hpp:126-136; the parabolic kinds have
+// alpha = 0): hbar M k / (m_c sqrt(1 + 4 alpha gamma(E)))
+template <typename R>
+__device__ __forceinline__ double vx_dev(const InstDev<R> *P, int v, int sub, R kx, R ky, R kz,
+                                         R E) {
+  const ValleyDev<R> &V = P->valley[v];
+  const R *M = V.hv[sub];
+  const R g = E * (R(1) + V.alpha * E);
+  const R npf = Math<R>::sqrt(R(1) + R(4) * V.alpha * g);
+  return (double)Math<R>::div(V.cv * (M[0] * kx + M[1] * ky + M[2] * kz), npf);
+}
+
 // one particle, one step of driftScatterParticles
 // (emcBasicParticleHandler.hpp:100-131): force at the current position,
 // drift min(tau, dt); while time remains and the particle is inside:
@@ -360,6 +373,7 @@ __global__ __launch_bounds__(kDevStepThreads) void k_dev_step(DevStepArgs<R> a) 
   const InstDev<R> *P = a.inst;
   const uint64_t lt = (1ull << lane) - 1ull;
   uint32_t nst = 0;
+  double vxs = 0.0;  // channel current: this lane's sum of v_x
   // the state of the wave's next chunk is loaded while the current one moves
   struct Chunk {
     R x, y, kx, ky, kz, E, tau;
@@ -417,6 +431,8 @@ __global__ __launch_bounds__(kDevStepThreads) void k_dev_step(DevStepArgs<R> a) 
           a.p.E[i] = E;
           a.p.tau[i] = tau;
           a.p.vs[i] = ((uint32_t)v << 16) | (uint32_t)sub;
+          if (a.chan && x >= a.chan_x0 && x <= a.chan_x1)
+            vxs += vx_dev(P, v, sub, kx, ky, kz, E);
         }
       }
       if (!removed) {
@@ -455,7 +471,21 @@ __global__ __launch_bounds__(kDevStepThreads) void k_dev_step(DevStepArgs<R> a) 
     a.seg_cnt[(long long)k * n_seg + seg] = mycnt[k];
   if (lane == 0)
     a.seg_nst[seg] = nst;
+  __shared__ double vx_w[kDevStepWaves];
+  if (a.chan) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+      vxs += __shfl_xor(vxs, off, 64);
+    if (lane == 0)
+      vx_w[wave] = vxs;
+  }
   __syncthreads();
+  if (a.chan && tid == 0) {
+    double b = 0.0;
+    for (int w = 0; w < kDevStepWaves; ++w)
+      b += vx_w[w];
+    a.chan_part[blockIdx.x] = b;
+  }
   if (a.lds_dep)
     for (int k = tid; k < nn; k += kDevStepThreads)
       a.dep[(long long)blockIdx.x * nn + k] = grid[k];
@@ -558,8 +588,14 @@ __global__ __launch_bounds__(256) void k_dev_resolve(DevContactArgs<R> a) {
       const uint32_t rank = a.seg_off[(long long)slot * a.n_segments + seg] + (e.y & 0x3fffffu);
       if (rank >= (uint32_t)a.slot_kept[slot]) {
         const long long i = e.x;
+        const R x = a.p.x[i];
+        if (a.chan && x >= a.chan_x0 && x <= a.chan_x1) {
+          const uint32_t vs = a.p.vs[i];
+          atomicAdd(a.chan_corr, -vx_dev(a.inst, (int)(vs >> 16), (int)(vs & 0xffffu), a.p.kx[i],
+                                         a.p.ky[i], a.p.kz[i], a.p.E[i]));
+        }
         a.p.vs[i] = kHole;
-        assign(a.g, a.p.x[i], a.p.y[i], [&](int k, R w) { atomicAdd(&a.corr[k], -(double)w); });
+        assign(a.g, x, a.p.y[i], [&](int k, R w) { atomicAdd(&a.corr[k], -(double)w); });
       }
     }
     return;
@@ -626,6 +662,8 @@ __global__ __launch_bounds__(256) void k_dev_resolve(DevContactArgs<R> a) {
   a.p.vs[idx] = vs;
   a.p.pid[idx] = pid;
   assign(a.g, x, y, [&](int k, R w) { atomicAdd(&a.corr[k], (double)w); });
+  if (a.chan && x >= a.chan_x0 && x <= a.chan_x1)
+    atomicAdd(a.chan_corr, vx_dev(a.inst, (int)(vs >> 16), (int)(vs & 0xffffu), kx, ky, kz, E));
 }
 
 // carriers per node = FP64 sum of the block grids (fixed order) + the
@@ -659,11 +697,33 @@ __global__ __launch_bounds__(256) void k_dev_reduce(DevContactArgs<R> a, int com
     s += a.corr[k];
     a.corr[k] = 0.0;
     a.nrpart[k] = s;
-    a.conc[k] = s * a.cfac[k];
+    const double cn = s * a.cfac[k];
+    a.conc[k] = cn;
+    if (a.accumulate) {  // updateAverageCharacteristics (emcSimulationResults.hpp:85-93)
+      a.avg_pot[k] += a.pot[k];
+      a.avg_conc[k] += cn;
+    }
+  }
+  if (commit && a.chan && blockIdx.x == 0 && threadIdx.x < 64) {
+    // the step's channel current: block partials in a fixed order, then the
+    // corrections of the deleted / injected particles
+    const long long row = a.state->step - a.state->hist_base;
+    double v = 0.0;
+    for (int b = threadIdx.x; b < a.n_chan_blocks; b += 64)
+      v += a.chan_part[b];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+      v += __shfl_xor(v, off, 64);
+    if (threadIdx.x == 0) {
+      a.hist.chan[row] = (v + *a.chan_corr) * a.chan_scale;
+      *a.chan_corr = 0.0;
+    }
   }
   if (commit && blockIdx.x == 0 && threadIdx.x == 0) {
     DevState *st = a.state;
     const long long row = st->step - st->hist_base;
+    if (a.accumulate)
+      st->avg_n += 1;
     long long d = 0;
     for (int c = 0; c < a.n_contacts; ++c) {
       const long long o = row * a.n_contacts + c;
@@ -1270,6 +1330,15 @@ __global__ __launch_bounds__(kDevThreads) void k_dev_field(const DevFieldArgs<R>
       const double na = (double)a.state->n_alive;
       a.hist.checksum[row] = fabs(s - na) / (na > 1.0 ? na : 1.0);
     }
+  }
+  // setPoissonInterval (emcSimulation.hpp:84-86): between solves the field
+  // of the last solve is kept
+  if (a.record && a.interval > 1 && a.state->step % a.interval != 0) {
+    if (tid == 0) {
+      a.hist.iters[row] = 0;
+      a.hist.resid[row] = 0.0;
+    }
+    return;
   }
   if (a.do_poisson) {
     MGShared m;

@@ -111,7 +111,10 @@ class DeviceStepArgs(C.Structure):
                 ("use_graph", C.c_int), ("exits", C.c_void_p), ("deleted", C.c_void_p),
                 ("injected", C.c_void_p), ("n_particles", C.c_void_p),
                 ("poisson_iters", C.c_void_p), ("poisson_resid", C.c_void_p),
-                ("checksum", C.c_void_p)]
+                ("checksum", C.c_void_p), ("poisson_interval", C.c_int),
+                ("channel_x0", C.c_double), ("channel_x1", C.c_double),
+                ("channel_length", C.c_double), ("channel_current", C.c_void_p),
+                ("accumulate", C.c_int)]
 
 
 TUNE_MAX = 32
@@ -138,7 +141,7 @@ EXPORTS = [
     "emcgpu_efield", "emcgpu_deposit", "emcgpu_interpolate", "emcgpu_contacts",
     "emcgpu_device_set_particles", "emcgpu_device_get_particles", "emcgpu_device_set_efield",
     "emcgpu_device_drift", "emcgpu_device_init", "emcgpu_device_step", "emcgpu_device_fields",
-    "emcgpu_autotune",
+    "emcgpu_autotune", "emcgpu_device_averages",
 ]
 
 
@@ -192,6 +195,7 @@ def load(path=LIB_PATH):
         "emcgpu_device_step": (i, [vp, p(DeviceStepArgs)]),
         "emcgpu_device_fields": (i, [vp, vp, vp, vp, vp, vp]),
         "emcgpu_autotune": (i, [vp, p(AutotuneArgs)]),
+        "emcgpu_device_averages": (i, [vp, vp, vp, p(C.c_longlong), i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -656,7 +660,10 @@ class Context:
         self._chk(self.L.emcgpu_device_init(self.h, seed, C.byref(n)))
         return n.value
 
-    def device_step(self, dt, n_steps, seed=0, graph=False):
+    def device_step(self, dt, n_steps, seed=0, graph=False, poisson_interval=1, channel=None,
+                    accumulate=False):
+        """emcgpu_device_step; channel = (x0, x1, length) [m] records the
+        drift-velocity channel current of every step (key "channel_current")."""
         nc = max(self.n_contacts, 1)
         out = dict(exits=np.zeros((n_steps, nc), dtype=np.int32),
                    deleted=np.zeros((n_steps, nc), dtype=np.int32),
@@ -669,11 +676,25 @@ class Context:
         a.n_steps = n_steps
         a.seed = seed
         a.use_graph = 1 if graph else 0
+        a.poisson_interval = poisson_interval
+        a.accumulate = 1 if accumulate else 0
+        if channel is not None:
+            a.channel_x0, a.channel_x1, a.channel_length = channel
+            out["channel_current"] = np.zeros(n_steps)
+            a.channel_current = _ptr(out["channel_current"])
         for key in ("exits", "deleted", "injected", "n_particles", "poisson_iters",
                     "poisson_resid", "checksum"):
             setattr(a, key, _ptr(out[key]))
         self._chk(self.L.emcgpu_device_step(self.h, C.byref(a)))
         return out
+
+    def device_averages(self, reset=False):
+        """emcgpu_device_averages: (potential, concentration, steps) averaged
+        over the steps run with accumulate=True."""
+        pot, conc, n = np.zeros(self.nn), np.zeros(self.nn), C.c_longlong(0)
+        self._chk(self.L.emcgpu_device_averages(self.h, _ptr(pot), _ptr(conc), C.byref(n),
+                                                1 if reset else 0))
+        return pot, conc, n.value
 
     def device_fields(self):
         out = {k: np.zeros(self.nn) for k in ("pot", "ex", "ey", "conc", "nr_part")}

@@ -84,6 +84,10 @@ public:
 
   T getMass() const { return particleMass; }
   T getCharge() const { return particleCharge; }
+  /// name used in result file names (emcElectron::getName,
+  /// include/ParticleType/emcElectron.hpp:38)
+  std::string getName() const { return typeName; }
+  void setName(const std::string &n) { typeName = n; }
   SizeType getNrValleys() const { return valleys.size(); }
   const AbstractValley *getValley(SizeType i) const { return valleys.at(i).get(); }
   SizeType getNrEnergyLevels() const { return nLevels; }
@@ -137,6 +141,7 @@ public:
 private:
   SizeType nLevels;
   T maxE, particleMass, particleCharge;
+  std::string typeName = "Electrons";
   std::vector<std::unique_ptr<AbstractValley>> valleys;
   std::vector<std::unique_ptr<AbstractScatterMechanism>> mechs;
 };
