@@ -182,7 +182,6 @@ constexpr int kDevStepThreads = 512;   // particle step: 8 waves per block
 constexpr int kDevStepWaves = kDevStepThreads / 64;
 constexpr uint32_t kHole = 0xFFFFFFFFu;  // valley/subvalley word of a deleted particle
 constexpr int kMaxLevels = 12;
-constexpr int kOwn = 8;  // fine-grid nodes per colour and thread of the Poisson workgroup
 constexpr int kPad = 4 * kDevThreads;  // node padding of the Poisson vectors
 constexpr int kMaxContacts = EMCGPU_MAX_CONTACTS;
 constexpr int kMaxSlots = 1024;        // ohmic nodes
@@ -290,6 +289,11 @@ struct DevFieldArgs {
   int nu_pre, nu_post, nu_coarse;
   // dynamic LDS layout (float offsets; tab and cmask in bytes)
   int lds_res, lds_rc, lds_sc, lds_tab, lds_cmask;
+  int lds_dummy;  // float offset of one scratch word per thread
+  // fine-grid thread map: a thread owns column (tid mod 2^cw_log2) of a row
+  // block; half-row map (red-black sweeps, node pairs) and full-row map,
+  // with the number of row blocks (passes) covering the grid
+  int hcw_log2, nph, fcw_log2, npf;
   // records
   int *iters_out;
   double *resid_out;

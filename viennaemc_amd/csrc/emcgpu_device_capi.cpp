@@ -83,7 +83,9 @@ struct DeviceSim {
   size_t mg_lds = 0;
   int mg_nodes = 0;  // nodes of all levels
   int lds_res = 0, lds_rc = 0, lds_sc = 0, lds_tab = 0, lds_cmask = 0, wave_level = 0;
+  int lds_dummy = 0;
   int direct_level = 0;
+  int hcw_log2 = 0, nph = 0, fcw_log2 = 0, npf = 0;
   std::vector<float> cinv_t;
   float *d_cinv_t = nullptr;
   // ---- device memory ------------------------------------------------------
@@ -552,6 +554,9 @@ emcgpu_status build_multigrid(emcgpu_ctx *c, DeviceSim &d) {
   d.lds_tab = (int)(((size_t)(d.lds_sc + nc) * sizeof(float) + 15) & ~(size_t)15);
   d.lds_cmask = d.lds_tab + (int)(d.tab.size() * sizeof(DimTab));
   d.mg_lds = ((size_t)d.lds_cmask + nc + 15) & ~(size_t)15;
+  // one word per thread for the fine-grid passes' off-grid slots
+  d.lds_dummy = (int)(d.mg_lds / sizeof(float));
+  d.mg_lds += kDevThreads * sizeof(float);
   d.wave_level = nl;
   for (int l = 1; l < nl; l++)
     if (d.lev[l].nx * d.lev[l].ny <= 256) {
@@ -632,10 +637,22 @@ emcgpu_status build_multigrid(emcgpu_ctx *c, DeviceSim &d) {
       d.direct_level = lc;
     }
   }
-  const int per_colour = ((d.ext[0] + 1) / 2) * d.ext[1];
-  if (per_colour > kOwn * kDevThreads)
-    return fail(c, EMCGPU_ERR_ARG, "a %d x %d grid exceeds the single-workgroup Poisson solver",
-                d.ext[0], d.ext[1]);
+  if (d.ext[0] > kDevThreads)
+    return fail(c, EMCGPU_ERR_ARG,
+                "a %d x %d grid exceeds the single-workgroup Poisson solver (at most %d nodes "
+                "along x)",
+                d.ext[0], d.ext[1], kDevThreads);
+  auto clog2 = [](int v) {
+    int l = 0;
+    while ((1 << l) < v)
+      ++l;
+    return l;
+  };
+  d.hcw_log2 = clog2((d.ext[0] + 1) / 2);
+  d.fcw_log2 = clog2(d.ext[0]);
+  const int rh = kDevThreads >> d.hcw_log2, rf = kDevThreads >> d.fcw_log2;
+  d.nph = (d.ext[1] + rh - 1) / rh;
+  d.npf = (d.ext[1] + rf - 1) / rf;
   if (d.mg_lds > (size_t)kDevMaxLds)
     return fail(c, EMCGPU_ERR_ARG,
                 "a %d x %d grid exceeds the single-workgroup Poisson solver (%zu bytes of LDS > "
@@ -971,6 +988,11 @@ DevFieldArgs<R> field_args(const emcgpu_ctx *c, const DeviceSim &d) {
   a.lds_sc = d.lds_sc;
   a.lds_tab = d.lds_tab;
   a.lds_cmask = d.lds_cmask;
+  a.lds_dummy = d.lds_dummy;
+  a.hcw_log2 = d.hcw_log2;
+  a.nph = d.nph;
+  a.fcw_log2 = d.fcw_log2;
+  a.npf = d.npf;
   a.state = d.d_state;
   a.hist = hist_of(d);
   return a;

@@ -939,8 +939,6 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
 // multigrid V-cycle (FP32, LDS) as the preconditioner of an FP64 conjugate
 // gradient solve of the Newton system
 // ---------------------------------------------------------------------------
-// level-0 nodes a thread owns: colour c, slot m -> node index (-1: none,
-// -2 - k: Dirichlet node k); kOwn per colour (emcgpu_device.h)
 
 // wave-synchronous step: the wave's LDS writes are visible to its lanes
 __device__ __forceinline__ void wave_sync() {
@@ -959,6 +957,32 @@ struct WaveTeam {  // wave 0 alone
   __device__ __forceinline__ int rank() const { return (int)threadIdx.x; }
   __device__ __forceinline__ void sync() const { wave_sync(); }
 };
+
+// Pointers of the Poisson kernel come from its argument block in memory, so
+// the compiler cannot tell their address space and emits flat accesses,
+// which also count against the LDS wait counter; these views are global.
+// (the host compilation pass of this file sees no address spaces)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define EMC_GLOBAL __attribute__((address_space(1)))
+#else
+#define EMC_GLOBAL
+#endif
+template <typename T>
+__device__ __forceinline__ EMC_GLOBAL T *glb(T *p) {
+  return (EMC_GLOBAL T *)p;
+}
+
+// k / n for 0 <= k < 2^22, n >= 1, exact: a reciprocal estimate (error below
+// one) and one correction.  The index maths of the Poisson passes is the
+// bulk of their vector-ALU work; a run-time integer division costs ~30
+// instructions.
+__device__ __forceinline__ int idiv(int k, int n) {
+  int q = (int)((float)k * __builtin_amdgcn_rcpf((float)n));
+  const int r = k - q * n;
+  q += r >= n ? 1 : 0;
+  q -= r < 0 ? 1 : 0;
+  return q;
+}
 
 // the coarse levels' data in LDS (level l >= 1 at offset lev[l].off - n0)
 struct MGShared {
@@ -986,7 +1010,7 @@ __device__ __forceinline__ void mg_sweep_c(const DevFieldArgs<R> &a, const MGSha
   const DimTab *TX = m.tab + L.tx, *TY = m.tab + L.ty;
   const int hr = (L.nx + 1) >> 1;
   for (int t = team.rank(); t < hr * L.ny; t += T::size) {
-    const int j = t / hr;
+    const int j = idiv(t, hr);
     const int i = 2 * (t - j * hr) + ((j + c) & 1);
     if (i >= L.nx)
       continue;
@@ -996,7 +1020,9 @@ __device__ __forceinline__ void mg_sweep_c(const DevFieldArgs<R> &a, const MGSha
     const float cd = m.hfy * tx.w * ty.inv_l, cu = m.hfy * tx.w * ty.inv_r;
     const int il = i > 0 ? k - 1 : k + 1, ir = i < L.nx - 1 ? k + 1 : k - 1;
     const int jd = j > 0 ? k - L.nx : k + L.nx, ju = j < L.ny - 1 ? k + L.nx : k - L.nx;
-    d[k] = (r[k] + cl * d[il] + cr * d[ir] + cd * d[jd] + cu * d[ju]) / (cl + cr + cd + cu + sg[k]);
+    // the smoother is a preconditioner: the hardware reciprocal suffices
+    d[k] = (r[k] + cl * d[il] + cr * d[ir] + cd * d[jd] + cu * d[ju]) *
+           __builtin_amdgcn_rcpf(cl + cr + cd + cu + sg[k]);
   }
   team.sync();
 }
@@ -1021,7 +1047,7 @@ __device__ __forceinline__ void mg_restrict_to(const DevFieldArgs<R> &a, const M
         dst[K] = 0.0f;
       continue;
     }
-    const int J = K / C.nx, I = K - J * C.nx;
+    const int J = idiv(K, C.nx), I = K - J * C.nx;
     const DimTab cx = CX[I], cy = CY[J];
     const float wx[3] = {cx.wl, 1.0f, cx.wr}, wy[3] = {cy.wl, 1.0f, cy.wr};
     float s = 0.0f;
@@ -1057,7 +1083,7 @@ __device__ __forceinline__ void mg_residual_c(const DevFieldArgs<R> &a, const MG
       res[k] = 0.0f;
       continue;
     }
-    const int j = k / L.nx, i = k - j * L.nx;
+    const int j = idiv(k, L.nx), i = k - j * L.nx;
     const DimTab tx = TX[i], ty = TY[j];
     const float cl = m.hfx * ty.w * tx.inv_l, cr = m.hfx * ty.w * tx.inv_r;
     const float cd = m.hfy * tx.w * ty.inv_l, cu = m.hfy * tx.w * ty.inv_r;
@@ -1083,7 +1109,7 @@ __device__ __forceinline__ void mg_prolong(const DevFieldArgs<R> &a, const MGSha
   for (int k = team.rank(); k < L.nx * L.ny; k += T::size) {
     if (mask[k])
       continue;
-    const int j = k / L.nx, i = k - j * L.nx;
+    const int j = idiv(k, L.nx), i = k - j * L.nx;
     const DimTab tx = TX[i], ty = TY[j];
     const float ax = tx.a, ay = ty.a;
     const int b0 = ty.cI * C.nx + tx.cI;
@@ -1126,45 +1152,107 @@ __device__ __forceinline__ void mg_lower(const DevFieldArgs<R> &a, const MGShare
 }
 
 __device__ __forceinline__ void nb0(int k, int nx, int ny, int &il, int &ir, int &jd, int &ju) {
-  const int j = k / nx, i = k - j * nx;
+  const int j = idiv(k, nx), i = k - j * nx;
   il = i > 0 ? k - 1 : k + 1;
   ir = i < nx - 1 ? k + 1 : k - 1;
   jd = j > 0 ? k - nx : k + nx;
   ju = j < ny - 1 ? k + nx : k - nx;
 }
 
-// fine-grid node of sweep colour c, slot q of this thread (sweep index
-// tid + q * kDevThreads), -1 if none
-__device__ __forceinline__ int fine_node(int c, int q, int nx, int ny, int hr) {
-  const int t = (int)threadIdx.x + q * kDevThreads;
-  const int j = t / hr;
-  const int i = 2 * (t - j * hr) + ((j + c) & 1);
-  return (j < ny && i < nx) ? j * nx + i : -1;
+// Fine-grid passes: thread t owns column (t mod 2^cw) of a block of
+// 1024 / 2^cw rows; pass p covers the block's rows shifted by p blocks.  The
+// index maths is shifts and masks.  A slot off the grid reads and writes
+// only the thread's own LDS scratch word (k and the neighbours all point
+// there; kg, its global index, is node 0), so no branch separates the loads
+// of a pass from its stores.
+struct FineSlot {
+  int kg, k, il, ir, jd, ju;
+  bool ok;
+};
+__device__ __forceinline__ FineSlot fine_slot(int i, int j, int nx, int ny, int dummy) {
+  FineSlot f;
+  f.ok = (i < nx) & (j < ny);
+  // bitwise selects: the compiler turns nested conditionals into branches
+  const int on = -(int)f.ok, off = ~on;
+  const int k = j * nx + i;
+  f.kg = k & on;
+  f.k = (k & on) | (dummy & off);
+  f.il = ((k - 1 + 2 * (int)(i == 0)) & on) | (dummy & off);
+  f.ir = ((k + 1 - 2 * (int)(i == nx - 1)) & on) | (dummy & off);
+  f.jd = ((k - nx + 2 * nx * (int)(j == 0)) & on) | (dummy & off);
+  f.ju = ((k + nx - 2 * nx * (int)(j == ny - 1)) & on) | (dummy & off);
+  return f;
+}
+// slot of pass p in the half-row map of colour c (node pairs)
+template <int c>
+__device__ __forceinline__ FineSlot half_slot(int p, int cw_log2, int nx, int ny, int dummy) {
+  const int j = ((int)threadIdx.x >> cw_log2) + p * (kDevThreads >> cw_log2);
+  const int i = 2 * ((int)threadIdx.x & ((1 << cw_log2) - 1)) + ((j + c) & 1);
+  return fine_slot(i, j, nx, ny, dummy);
+}
+// slot of pass p in the full-row map
+__device__ __forceinline__ FineSlot full_slot(int p, int cw_log2, int nx, int ny, int dummy) {
+  const int j = ((int)threadIdx.x >> cw_log2) + p * (kDevThreads >> cw_log2);
+  const int i = (int)threadIdx.x & ((1 << cw_log2) - 1);
+  return fine_slot(i, j, nx, ny, dummy);
 }
 
 // one colour of red-black Gauss-Seidel on the uniform fine grid; the
-// right-hand side and diagonal shift of the thread's nodes (interleaved in
-// global memory, Dirichlet nodes carrying (0, kBigShift)) are all loaded
-// before the updates
+// right-hand side and diagonal shift (interleaved in global memory,
+// Dirichlet nodes carrying (0, kBigShift)) of four passes are loaded before
+// their updates
 template <int c, typename R>
 __device__ __forceinline__ void mg_sweep0(const DevFieldArgs<R> &a, const MGShared &m) {
   float *d = m.d;
   const float hfx = m.hfx, hfy = m.hfy, diag = 2.0f * (m.hfx + m.hfy);
-  const int nx = a.nx, ny = a.ny, hr = (nx + 1) >> 1;
-  float2 v[kOwn];
+  const int nx = a.nx, ny = a.ny, cw = a.hcw_log2, dummy = a.lds_dummy + (int)threadIdx.x;
+  const EMC_GLOBAL float2 *rs0 = glb(a.rs0);
+  for (int p0 = 0; p0 < a.nph; p0 += 4) {
+    FineSlot f[4];
+    float2 v[4];
+    float nv[4];
 #pragma unroll
-  for (int q = 0; q < kOwn; ++q) {
-    const int k = fine_node(c, q, nx, ny, hr);
-    v[q] = a.rs0[k >= 0 ? k : 0];
+    for (int u = 0; u < 4; ++u) {
+      f[u] = half_slot<c>(p0 + u, cw, nx, ny, dummy);
+      v[u] = rs0[f[u].kg];
+    }
+    // all reads before the stores (the compiler cannot see that the stores
+    // never alias the other colour's nodes)
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      nv[u] = (v[u].x + hfx * (d[f[u].il] + d[f[u].ir]) + hfy * (d[f[u].jd] + d[f[u].ju])) *
+              __builtin_amdgcn_rcpf(diag + v[u].y);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (f[u].ok)
+        d[f[u].k] = nv[u];
   }
-#pragma unroll
-  for (int q = 0; q < kOwn; ++q) {
-    const int k = fine_node(c, q, nx, ny, hr);
-    if (k < 0)
-      continue;
-    int il, ir, jd, ju;
-    nb0(k, nx, ny, il, ir, jd, ju);
-    d[k] = (v[q].x + hfx * (d[il] + d[ir]) + hfy * (d[jd] + d[ju])) / (diag + v[q].y);
+  __syncthreads();
+}
+
+// d_0 += P d_1 on the fine grid in the full-row map: the thread's column
+// fixes its x interpolation; Dirichlet nodes are skipped
+template <typename R>
+__device__ __forceinline__ void mg_prolong0(const DevFieldArgs<R> &a, const MGShared &m) {
+  const MGLevel C = a.lev[1];
+  float *d = m.d;
+  const float *dc = m.d + C.off;
+  const DimTab *TX = m.tab + a.lev[0].tx, *TY = m.tab + a.lev[0].ty;
+  const int nx = a.nx, ny = a.ny, cw = a.fcw_log2, dummy = a.lds_dummy + (int)threadIdx.x;
+  const int i = (int)threadIdx.x & ((1 << cw) - 1);
+  const DimTab tx = TX[i < nx ? i : 0];
+  const float ax = tx.a;
+  const int dx = ax < 1.0f ? 1 : 0;
+  for (int p = 0; p < a.npf; ++p) {
+    const FineSlot f = full_slot(p, cw, nx, ny, dummy);
+    const int j = ((int)threadIdx.x >> cw) + p * (kDevThreads >> cw);
+    const DimTab ty = TY[j < ny ? j : 0];
+    const float ay = ty.a;
+    const int b0 = ty.cI * C.nx + tx.cI;
+    const int dy = ay < 1.0f ? C.nx : 0;
+    const float v = ax * ay * dc[b0] + (1.0f - ax) * ay * dc[b0 + dx] +
+                    ax * (1.0f - ay) * dc[b0 + dy] + (1.0f - ax) * (1.0f - ay) * dc[b0 + dx + dy];
+    d[f.k] += glb(a.dmask)[f.kg] ? 0.0f : v;
   }
   __syncthreads();
 }
@@ -1182,7 +1270,7 @@ __device__ __forceinline__ void mg_direct(const DevFieldArgs<R> &a, const MGShar
   const int seg = (n + 3) >> 2;
   float s = 0.0f;
   if (row < n) {
-    const float *A = a.cinv + (long long)row * n;
+    const EMC_GLOBAL float *A = glb(a.cinv) + (long long)row * n;
     const int j1 = (part + 1) * seg < n ? (part + 1) * seg : n;
 #pragma unroll 8
     for (int j = part * seg; j < j1; ++j)
@@ -1215,23 +1303,23 @@ __device__ __forceinline__ void mg_vcycle(const DevFieldArgs<R> &a, const MGShar
   }
   {  // level-0 residual, restricted to level 1
     const float hfx = m.hfx, hfy = m.hfy, diag = 2.0f * (m.hfx + m.hfy);
-    const int nx = a.nx, ny = a.ny, hr = (nx + 1) >> 1;
-    (void)hr;
-    for (int k0 = threadIdx.x; k0 < a.n_pad; k0 += kPad) {
-      float2 v[kPad / kDevThreads];
+    const int nx = a.nx, ny = a.ny, cw = a.fcw_log2, dummy = a.lds_dummy + (int)threadIdx.x;
+    const EMC_GLOBAL float2 *rs0 = glb(a.rs0);
+    for (int p0 = 0; p0 < a.npf; p0 += 4) {
+      FineSlot f[4];
+      float2 v[4];
 #pragma unroll
-      for (int u = 0; u < kPad / kDevThreads; ++u)
-        v[u] = a.rs0[k0 + u * kDevThreads];
+      for (int u = 0; u < 4; ++u) {
+        f[u] = full_slot(p0 + u, cw, nx, ny, dummy);
+        v[u] = rs0[f[u].kg];
+      }
 #pragma unroll
-      for (int u = 0; u < kPad / kDevThreads; ++u) {
-        const int k = k0 + u * kDevThreads;
-        if (k >= a.nn)
-          continue;
-        int il, ir, jd, ju;
-        nb0(k, nx, ny, il, ir, jd, ju);
-        const float rk = v[u].x - ((diag + v[u].y) * m.d[k] -
-                                   (hfx * (m.d[il] + m.d[ir]) + hfy * (m.d[jd] + m.d[ju])));
-        m.res[k] = v[u].y >= kBigShift ? 0.0f : rk;
+      for (int u = 0; u < 4; ++u) {
+        const float rk =
+            v[u].x - ((diag + v[u].y) * m.d[f[u].k] -
+                      (hfx * (m.d[f[u].il] + m.d[f[u].ir]) + hfy * (m.d[f[u].jd] + m.d[f[u].ju])));
+        // m.res[k] (m.res = m.d + lds_res), the scratch word off the grid
+        m.d[f[u].ok ? a.lds_res + f[u].k : f[u].k] = v[u].y >= kBigShift ? 0.0f : rk;
       }
     }
     __syncthreads();
@@ -1269,7 +1357,7 @@ __device__ __forceinline__ void mg_vcycle(const DevFieldArgs<R> &a, const MGShar
       mg_sweep_c(a, m, l, 0, BlockTeam());
     }
   }
-  mg_prolong(a, m, 0, BlockTeam());
+  mg_prolong0(a, m);
   for (int s = 0; s < a.nu_post; ++s) {
     mg_sweep0<1>(a, m);
     mg_sweep0<0>(a, m);
@@ -1366,11 +1454,15 @@ __global__ __launch_bounds__(kDevThreads) void k_dev_field(const DevFieldArgs<R>
           pot[k] = a.equilibrium ? a.bc_eq[k] : a.bc_neq[k];
     __syncthreads();
     const double hfx = a.hfx, hfy = a.hfy, hp = a.hp, hfs2 = 2.0 * (a.hfx + a.hfy);
-    double *r64 = a.r64, *x64 = a.x64, *p64 = a.p64, *q64 = a.q64, *s64 = a.s64;
-    const float *w = a.w0;  // 1 at active nodes, 0 at Dirichlet and padding
+    EMC_GLOBAL double *r64 = glb(a.r64), *x64 = glb(a.x64), *p64 = glb(a.p64),
+                      *q64 = glb(a.q64), *s64 = glb(a.s64);
+    EMC_GLOBAL double *potg = glb(pot);
+    EMC_GLOBAL float2 *rs0 = glb(a.rs0);
+    const EMC_GLOBAL double *conc = glb(a.conc), *dopn = glb(a.dopn);
+    const EMC_GLOBAL float *w = glb(a.w0);  // 1 at active nodes, 0 at Dirichlet and padding
     // control-volume weights (1/2 per boundary dimension): the mirror
     // boundary operator is symmetric in this inner product
-    const float *hw = a.hw0;
+    const EMC_GLOBAL float *hw = glb(a.hw0);
     const int np = a.n_pad;  // nn rounded up to kPad nodes
     double scale = 1.0, fm = 0.0;
     int it = 0, vcycles = 0;
@@ -1385,16 +1477,16 @@ __global__ __launch_bounds__(kDevThreads) void k_dev_field(const DevFieldArgs<R>
           const int kc = k < nn ? k : nn - 1;
           int il, ir, jd, ju;
           nb0(kc, nx, ny, il, ir, jd, ju);
-          const double phi = pot[kc];
+          const double phi = potg[kc];
           const double p = exp(-phi);
-          const double n = a.equilibrium ? exp(phi) : a.conc[kc];
-          const double C = a.dopn[kc];
-          const double F = wk * (hfx * (pot[il] + pot[ir]) + hfy * (pot[jd] + pot[ju]) -
+          const double n = a.equilibrium ? exp(phi) : conc[kc];
+          const double C = dopn[kc];
+          const double F = wk * (hfx * (potg[il] + potg[ir]) + hfy * (potg[jd] + potg[ju]) -
                                  hfs2 * phi + hp * (p - n + C));
           const double sg = hp * (a.equilibrium ? p + n : p);
           r64[k] = F;
           s64[k] = sg;
-          a.rs0[k] = make_float2((float)F, wk != 0.0f ? (float)sg : kBigShift);
+          rs0[k] = make_float2((float)F, wk != 0.0f ? (float)sg : kBigShift);
           const double af = fabs(F);
           lmax = af > lmax ? af : lmax;
           const double sc = wk * hp * (fabs(C) + n + p);
@@ -1467,7 +1559,7 @@ __global__ __launch_bounds__(kDevThreads) void k_dev_field(const DevFieldArgs<R>
             x64[k] += alpha * p64[k];
             const double rn = r64[k] - alpha * q64[k];
             r64[k] = rn;
-            a.rs0[k].x = (float)rn;
+            rs0[k].x = (float)rn;
             const double ar = fabs(rn);
             rmax = ar > rmax ? ar : rmax;
           }
@@ -1482,7 +1574,7 @@ __global__ __launch_bounds__(kDevThreads) void k_dev_field(const DevFieldArgs<R>
           d = d > 2.0 ? 2.0 : d;
           d = d < -2.0 ? -2.0 : d;
         }
-        pot[k] += d;
+        potg[k] += d;
       }
       __syncthreads();
     }
@@ -1502,7 +1594,7 @@ __global__ __launch_bounds__(kDevThreads) void k_dev_field(const DevFieldArgs<R>
 #pragma clang fp contract(off)
     const bool nec = a.pm == EMCGPU_PM_NEC;
     for (int k = tid; k < nn; k += kDevThreads) {
-      const int j = k / nx, i = k - j * nx;
+      const int j = idiv(k, nx), i = k - j * nx;
       if (i != 0 && i != nx - 1)
         a.ex[k] = nec ? (pot[k] - pot[k + 1]) * a.vt / a.hxm
                       : (pot[k - 1] - pot[k + 1]) * a.vt / (2 * a.hxm);
@@ -1514,7 +1606,7 @@ __global__ __launch_bounds__(kDevThreads) void k_dev_field(const DevFieldArgs<R>
     // setEFieldBoundaryValues: a contact copies the inner neighbour, any
     // other boundary node is zero
     for (int k = tid; k < nn; k += kDevThreads) {
-      const int j = k / nx, i = k - j * nx;
+      const int j = idiv(k, nx), i = k - j * nx;
       const int bx = a.bcx[k], by = a.bcy[k];
       if (bx != -2)
         a.ex[k] = bx == -1 ? 0.0 : a.ex[i == 0 ? k + 1 : k - 1];
