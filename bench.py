@@ -175,8 +175,9 @@ def run_hot_phonon(emcgpu, n, warmup, steps, precision, seed=20261202):
             "meanE_first": float(av["avgE"][0, 0]), "meanE_last": float(av["avgE"][-1, 0]),
             "meanNq_last": float(hist[-1]), "maxNq_last": float(bath["nq"].max()),
             "workload": "photoexcited cooling, %d e- N(0.5, 0.05) eV, Kane Gamma valley, hot-phonon "
-                        "Froehlich on a 256-bin LO bath, dt 0.5 fs, bath update + table rebuild "
-                        "every step, %s" % (n, "FP32" if precision else "FP64")}
+                        "Froehlich on a 256-bin LO bath (tau_LO 1 ps into the acoustic reservoir, "
+                        "30 ps to the lattice), dt 0.5 fs, bath update + table rebuild every step, "
+                        "%s" % (n, "FP32" if precision else "FP64")}
 
 
 def run_device(emcgpu, n, warmup, steps, precision, seed=20261202, graph=True):

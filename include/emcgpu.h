@@ -337,6 +337,10 @@ typedef struct {
   int pm_scheme;        /* emcgpu_pm_scheme */
   double poisson_tol;   /* relative residual: max |F| / max h^2 (|C| + n + p) */
   int poisson_max_iter; /* Newton / refinement iterations per solve */
+  /* emcElectron's usePotentialForInit (emcElectron.hpp:48-61): initial
+   * carriers per node from exp(equilibrium potential) * n_i instead of the
+   * doping */
+  int init_from_potential;
 } emcgpu_device_desc;
 
 /* geometry (host) and device grids; call after emcgpu_set_model and

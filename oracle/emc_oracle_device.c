@@ -687,8 +687,10 @@ int orc_device_run(const orc_device_run_cfg *cfg, double *eqPot, int *nInit,
   orc_pm_efield(cfg->pm, d, pot, ex, ey);
   for (size_t iy = 0; iy < (size_t)d->ext[1]; iy++)
     for (size_t ix = 0; ix < (size_t)d->ext[0]; ix++) {
-      /* getInitialNrParticles, doping branch (emcElectron.hpp:48-61) */
-      double dens = d->doping[orc_device_idx(d, ix, iy)];
+      /* getInitialNrParticles (emcElectron.hpp:48-61): doping, or
+       * exp(potential) * ni with usePotentialForInit */
+      double dens = cfg->initFromPotential ? exp(pot[orc_device_idx(d, ix, iy)]) * d->ni
+                                           : d->doping[orc_device_idx(d, ix, iy)];
       size_t c[2] = {ix, iy};
       for (int i = 0; i < 2; i++)
         if (c[i] == 0 || c[i] == (size_t)(d->ext[i] - 1))
@@ -954,7 +956,8 @@ int orc_device_run_philox(const orc_device_run_cfg *cfg, double Vt, double *eqPo
   for (size_t iy = 0; iy < (size_t)d->ext[1]; iy++)
     for (size_t ix = 0; ix < (size_t)d->ext[0]; ix++) {
       size_t c[2] = {ix, iy};
-      double dens = d->doping[orc_device_idx(d, ix, iy)];
+      double dens = cfg->initFromPotential ? exp(pot[orc_device_idx(d, ix, iy)]) * d->ni
+                                           : d->doping[orc_device_idx(d, ix, iy)];
       for (int i = 0; i < 2; i++)
         if (c[i] == 0 || c[i] == (size_t)(d->ext[i] - 1))
           dens *= 0.5;
@@ -1089,7 +1092,8 @@ int orc_device_run_philox_h(orc_device *d, orc_model *m, int pm, double accuracy
                             double *eqPot, int *nInit, int *nPartOut, double *potOut,
                             int *exitOut, int *delOut, int *injOut, double *nrPartLast,
                             double *stateOut, int maxState, int poissonInterval,
-                            double chanX0, double chanX1, double chanLen, double *chanOut) {
+                            double chanX0, double chanX1, double chanLen, double *chanOut,
+                            int initFromPotential) {
   orc_device_run_cfg cfg;
   memset(&cfg, 0, sizeof cfg);
   cfg.dev = d;
@@ -1106,6 +1110,7 @@ int orc_device_run_philox_h(orc_device *d, orc_model *m, int pm, double accuracy
   cfg.chan[1] = chanX1;
   cfg.chan[2] = chanLen;
   cfg.chanOut = chanOut;
+  cfg.initFromPotential = initFromPotential;
   return orc_device_run_philox(&cfg, Vt, eqPot, nInit, nPartOut, potOut, exitOut, delOut,
                                injOut, nrPartLast, stateOut, maxState);
 }
@@ -1278,7 +1283,7 @@ int orc_device_run_h(orc_device *d, orc_model *m, int pm, double accuracyV,
                      double *eqPot, int *nInit, int *nPartOut, double *potOut,
                      int *exitOut, int *delOut, int *injOut, double *nrPartLast,
                      double *stateOut, int maxState, int poissonInterval, double chanX0,
-                     double chanX1, double chanLen, double *chanOut) {
+                     double chanX1, double chanLen, double *chanOut, int initFromPotential) {
   orc_device_run_cfg cfg;
   memset(&cfg, 0, sizeof cfg);
   cfg.dev = d;
@@ -1295,6 +1300,7 @@ int orc_device_run_h(orc_device *d, orc_model *m, int pm, double accuracyV,
   cfg.chan[1] = chanX1;
   cfg.chan[2] = chanLen;
   cfg.chanOut = chanOut;
+  cfg.initFromPotential = initFromPotential;
   return orc_device_run(&cfg, eqPot, nInit, nPartOut, potOut, exitOut, delOut, injOut,
                         nrPartLast, stateOut, maxState);
 }

@@ -147,6 +147,9 @@ typedef struct {
    * chan[1] [m], length chan[2] [m] */
   double chan[3];
   double *chanOut;
+  /* usePotentialForInit (emcElectron.hpp:48-61): initial carriers from
+   * exp(equilibrium potential) * ni instead of the doping */
+  int initFromPotential;
 } orc_device_run_cfg;
 
 int orc_device_run(const orc_device_run_cfg *cfg, double *eqPot, int *nInit,
@@ -183,7 +186,8 @@ int orc_device_run_philox_h(orc_device *d, orc_model *m, int pm, double accuracy
                             double *eqPot, int *nInit, int *nPartOut, double *potOut,
                             int *exitOut, int *delOut, int *injOut, double *nrPartLast,
                             double *stateOut, int maxState, int poissonInterval,
-                            double chanX0, double chanX1, double chanLen, double *chanOut);
+                            double chanX0, double chanX1, double chanLen, double *chanOut,
+                            int initFromPotential);
 
 /* ---- handle API for language bindings (tests) ---------------------------- */
 orc_device *orc_device_new(double maxX, double maxY, double hx, double hy,
@@ -215,7 +219,7 @@ int orc_device_run_h(orc_device *d, orc_model *m, int pm, double accuracyV,
                      double *eqPot, int *nInit, int *nPartOut, double *potOut,
                      int *exitOut, int *delOut, int *injOut, double *nrPartLast,
                      double *stateOut, int maxState, int poissonInterval, double chanX0,
-                     double chanX1, double chanLen, double *chanOut);
+                     double chanX1, double chanLen, double *chanOut, int initFromPotential);
 
 #ifdef __cplusplus
 }

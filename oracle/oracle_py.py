@@ -99,11 +99,11 @@ def load():
                                          _PI32, _PI32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
         "orc_device_run_h": (_I, [vp, vp, _I, _D, _D, _D, _I, C.c_uint64, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                  C.c_void_p, C.c_void_p, _I, _I, _D, _D, _D, C.c_void_p]),
+                                  C.c_void_p, C.c_void_p, _I, _I, _D, _D, _D, C.c_void_p, _I]),
         "orc_device_run_philox_h": (_I, [vp, vp, _I, _D, _D, _D, _I, C.c_uint64, _D, C.c_void_p,
                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _I,
-                                         _I, _D, _D, _D, C.c_void_p]),
+                                         _I, _D, _D, _D, C.c_void_p, _I]),
         "orc_mt64_fill": (None, [C.c_uint64, _PU64, C.c_size_t]),
         "orc_mt64_fill_particles": (None, [C.c_uint64, C.c_uint64, C.c_size_t, _PU64, C.c_size_t, C.c_size_t]),
         "orc_particle_seed": (C.c_uint64, [C.c_uint64, C.c_uint64]),
@@ -464,7 +464,7 @@ class OracleDevice:
         return rem, con, tr
 
     def run(self, model, scheme, accuracy, omega, dt, nsteps, seed, max_state=0, interval=1,
-            channel=None):
+            channel=None, init_from_potential=False):
         """the emcSimulation step order with the reference engine's streams;
         interval: setPoissonInterval, channel = (x0, x1, length) [m]: the
         drift-velocity channel current of every step (key "chan")"""
@@ -480,7 +480,8 @@ class OracleDevice:
                                     _ptr(out["eq_pot"]), _ptr(out["n_init"]), _ptr(out["n_part"]),
                                     _ptr(out["pot"]), _ptr(out["exits"]), _ptr(out["dels"]),
                                     _ptr(out["injs"]), _ptr(out["nr_part"]), _ptr(out["state"]), max_state,
-                                    interval, ch[0], ch[1], ch[2], _ptr(out["chan"]))
+                                    interval, ch[0], ch[1], ch[2], _ptr(out["chan"]),
+                                    1 if init_from_potential else 0)
         out["n_final"] = n
         out["pot"] = out["pot"].reshape(nsteps, nn)
         for k in ("exits", "dels", "injs"):
@@ -489,7 +490,7 @@ class OracleDevice:
         return out
 
     def run_philox(self, model, scheme, accuracy, omega, dt, nsteps, seed, Vt, max_state=0,
-                   interval=1, channel=None):
+                   interval=1, channel=None, init_from_potential=False):
         """the production twin of the GPU device run (Philox streams); state
         rows: x, y, kx, ky, kz, E, tau, valley, sub, pid; interval and
         channel as in run"""
@@ -505,7 +506,8 @@ class OracleDevice:
             self.h, model.h, scheme, accuracy, omega, dt, nsteps, seed, Vt, _ptr(out["eq_pot"]),
             _ptr(out["n_init"]), _ptr(out["n_part"]), _ptr(out["pot"]), _ptr(out["exits"]),
             _ptr(out["dels"]), _ptr(out["injs"]), _ptr(out["nr_part"]), _ptr(out["state"]),
-            max_state, interval, ch[0], ch[1], ch[2], _ptr(out["chan"]))
+            max_state, interval, ch[0], ch[1], ch[2], _ptr(out["chan"]),
+            1 if init_from_potential else 0)
         out["n_final"] = n
         out["pot"] = out["pot"].reshape(nsteps, nn)
         for k in ("exits", "dels", "injs"):

@@ -14,6 +14,8 @@
 // reference's table initialisation writes *ScatterMechanism.txt files).
 
 #include <cstdio>
+#include <sys/stat.h>
+#include <unistd.h>
 #include <cstdint>
 #include <memory>
 #include <string>
@@ -603,6 +605,23 @@ using Electron2 = emcElectron<T, Device2>;
 using Grid2 = emcGrid<T, 2>;
 using TypeMap2 = std::map<SizeType, std::unique_ptr<emcParticleType<T, Device2>>>;
 
+// runs a block with the working directory switched to a subfolder (the
+// reference writes its scatter-table dumps into the working directory)
+struct InDir {
+  std::string prev;
+  explicit InDir(const char *d) {
+    mkdir(d, 0755);
+    char buf[4096];
+    prev = getcwd(buf, sizeof buf) ? buf : ".";
+    if (chdir(d) != 0)
+      std::perror("chdir");
+  }
+  ~InDir() {
+    if (chdir(prev.c_str()) != 0)
+      std::perror("chdir");
+  }
+};
+
 static const double kDevMax[2] = {20e-9, 6e-9};
 static const double kDevH = 1e-9, kDevWidth = 1e-6, kDevVBias = 0.2;
 
@@ -618,8 +637,8 @@ static std::unique_ptr<Device2> makeDevice2() {
   return dev;
 }
 
-static std::unique_ptr<Electron2> makeDeviceType(Device2 &dev) {
-  auto type = std::make_unique<Electron2>(1000, 1.5, false);
+static std::unique_ptr<Electron2> makeDeviceType(Device2 &dev, bool usePotential = false) {
+  auto type = std::make_unique<Electron2>(1000, 1.5, usePotential);
   type->addValley(std::make_unique<emcParabolicIsotropValley<T>>(0.26, type->getMass(), 1));
   std::map<SizeType, std::vector<SizeType>> self = {{0, {0}}};
   using ZAb = emcZeroOrderInterValleyAbsorptionScatterMechanism<T>;
@@ -641,11 +660,11 @@ static const double kChan[3] = {5e-9, 15e-9, 10e-9};
 
 template <class PM>
 static void dumpDeviceRun(const std::string &tag, uint64_t seed, int nSteps, double dt,
-                          double accV, double omega, int interval = 1) {
+                          double accV, double omega, int interval = 1, bool usePotential = false) {
   auto dev = makeDevice2();
   PM pm;
   TypeMap2 types;
-  types[0] = makeDeviceType(*dev);
+  types[0] = makeDeviceType(*dev, usePotential);
   using Handler2 = emcBasicParticleHandler<T, Device2, PM>;
   using Solver2 = emcSORSolver<T, Device2, Handler2>;
   Handler2 handler(*dev, pm, types, 1, seed);
@@ -706,7 +725,8 @@ static void dumpDeviceRun(const std::string &tag, uint64_t seed, int nSteps, dou
   seci((b + "exits").c_str(), exits);
   seci((b + "ohmic").c_str(), netto);
   sec((b + "nrpart_last").c_str(), gridVec(nrPart));
-  sec((b + "cfg").c_str(), {(double)seed, (double)nSteps, dt, accV, omega, (double)interval});
+  sec((b + "cfg").c_str(),
+      {(double)seed, (double)nSteps, dt, accV, omega, (double)interval, (double)usePotential});
   sec((b + "chan").c_str(), chan);
 }
 
@@ -810,7 +830,10 @@ static void dumpDeviceTrajectories(uint64_t seed, int np, int nsteps, double dt)
   auto dev = makeDevice2();
   auto ext = dev->getGridExtent();
   auto type = makeDeviceType(*dev);
-  type->initScatterTables();
+  {
+    InDir in("tab_device");  // *ScatterMechanism.txt of the device particle type
+    type->initScatterTables();
+  }
   emcCICScheme<T, Device2> pm;
   // field: the non-equilibrium potential with conc = 0.9 * doping
   using Solver2 = emcSORSolver<T, Device2, emcBasicParticleHandler<T, Device2, emcCICScheme<T, Device2>>>;
@@ -956,6 +979,7 @@ int main(int argc, char **argv) {
   dumpDeviceRun<emcCICScheme<T, Device2>>("cic", seed + 11, 40, 1e-15, 1e-8, 1.8);
   dumpDeviceRun<emcNGPScheme<T, Device2>>("ngp", seed + 13, 40, 1e-15, 1e-8, 1.8);
   dumpDeviceRun<emcCICScheme<T, Device2>>("cic_sub3", seed + 17, 40, 1e-15, 1e-8, 1.8, 3);
+  dumpDeviceRun<emcCICScheme<T, Device2>>("cic_potinit", seed + 23, 20, 1e-15, 1e-8, 1.8, 1, true);
   dumpSimulationFiles(seed + 19);
   for (const auto &m : models) {
     double mp = 4e-8;
@@ -964,7 +988,12 @@ int main(int argc, char **argv) {
     const bool hot = m.name == "hot_polar";
     BathPtr bath = hot ? makeHotBath(m) : nullptr;
     auto type = makeType(m, device, bath);
-    type->initScatterTables();
+    {
+      std::unique_ptr<InDir> in;
+      if (m.name == "si_vasileska")  // its *ScatterMechanism.txt
+        in = std::make_unique<InDir>("tab_si_vasileska");
+      type->initScatterTables();
+    }
     dumpRates(m, device, *type, bath);
     dumpDrift(m, *type);
     // the final-state KATs record into a bath of their own

@@ -1,10 +1,15 @@
 // Builds the reference's bulk-silicon particle type (examples/SiliconFunctions.hpp,
 // Vasileska parameters) through the GPU host API, lowers it to a host-only
 // emcgpu context (no GPU) and prints tau0 and the normalised tables, which the
-// test compares with the reference's own (golden) values.
+// test compares with the reference's own (golden) values.  With a folder
+// argument it also writes the scatter-table dumps of that type and of the
+// device particle type (one parabolic valley, acoustic + zero-order
+// intervalley phonons in doping regions 0, 1 and 2) into <folder>/si_ and
+// <folder>/device_.
 #include <cstdio>
 #include <map>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "emcgpu/emcgpuBulkParticleHandler.hpp"
@@ -12,7 +17,7 @@
 using T = double;
 using namespace emcgpu;
 
-int main() {
+int main(int argc, char **argv) {
   setErrorPolicy(ErrorPolicy::Throw);
   emcgpuBulkDevice<T> device({11.8, 2329., 1.45e16, 9040}, 300.);
   emcgpuParticleType<T> e(1000, 1.);
@@ -58,5 +63,16 @@ int main() {
     std::printf("rejected %s\n", ex.what());
   }
   emcgpu_destroy(ctx);
+  if (argc > 1) {
+    const std::string dir = argv[1];
+    e.writeTablesToFiles(dir + "/si_");
+    emcgpuParticleType<T> d(1000, 1.5);
+    d.addValley(std::make_unique<emcParabolicIsotropValley<T>>(0.26, d.getMass(), 1));
+    const std::map<SizeType, std::vector<SizeType>> self = {{0, {0}}};
+    d.addScatterMechanism({0, 1, 2}, std::make_unique<emcAcousticScatterMechanism<T>>(0, 9., device));
+    d.addScatterMechanism({0, 1, 2}, std::make_unique<emcZeroOrderInterValleyAbsorptionScatterMechanism<T>>("P", 0, self, 5e10, 0.05, device));
+    d.addScatterMechanism({0, 1, 2}, std::make_unique<emcZeroOrderInterValleyEmissionScatterMechanism<T>>("P", 0, self, 5e10, 0.05, device));
+    d.writeTablesToFiles(dir + "/device_");
+  }
   return 0;
 }

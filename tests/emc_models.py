@@ -144,12 +144,17 @@ def photoexcited_cooling(n_particles=4_000_000):
     """The hot-phonon cooling workload: 4M electrons in a Kane Gamma valley
     (m* 0.28, alpha 0.3/eV), energies drawn from N(0.5 eV, 0.05 eV), hot-
     phonon Froehlich absorption + emission (60 meV, eps 3.5 / 10) on a
-    256-bin LO bath (dq 2e7 /m, tau_LO 1 ps) in the volume that holds them
-    at 1e24 /m^3, no field, dt 0.5 fs."""
+    256-bin LO bath (dq 2e7 /m) whose LO phonons decay with tau_LO = 1 ps
+    into the acoustic (Klemens) reservoir, hbar w_ac = hbar w_LO / 2, coupled
+    to the lattice in 30 ps (the reference hot-carrier driver's defaults,
+    examples/hotCarrierMHP/hotCarrierMHP.cpp:155, 209, 377), in the volume
+    that holds them at 1e24 /m^3, no field, dt 0.5 fs."""
     import copy
     m = copy.deepcopy(MODELS["hot_polar"])
     m.update(field_kvcm=0.0, init_energy=0.5, init_sigma=0.05)
-    m["baths"][0]["volume"] = n_particles / 1e24
+    b = m["baths"][0]
+    b["volume"] = n_particles / 1e24
+    b.update(acoustic=True, ac_hw=b["hw"] / 2, tau_ac=30e-12)
     return m
 
 

@@ -87,6 +87,13 @@ def main():
         if name.startswith("sim_") and name.endswith(".txt"):
             with open(os.path.join(work, name), "rb") as f:
                 d["simfile." + name[:-4]] = np.frombuffer(f.read(), dtype=np.uint8)
+    # scatter-table dumps (initScatterTables -> writeTablesToFiles) of two
+    # particle types, written into their own folders
+    for sub in ("tab_si_vasileska", "tab_device"):
+        folder = os.path.join(work, sub)
+        for name in sorted(os.listdir(folder)):
+            with open(os.path.join(folder, name), "rb") as f:
+                d["tabfile.%s.%s" % (sub[4:], name[:-4])] = np.frombuffer(f.read(), dtype=np.uint8)
     d["_meta.compiler"] = np.frombuffer(
         subprocess.check_output(["g++", "--version"]).splitlines()[0], dtype=np.uint8)
     np.savez_compressed(args.out, **d)

@@ -5,7 +5,9 @@ lowers to the same scatter rates and tau0 as the reference (golden values,
 bit-exact) and keeps the reference's subvalley-frame validation.  The
 result-file writers of device simulations reproduce, byte for byte, the
 files the reference's emcSimulation wrote (grids, contact currents,
-particle dumps) from the numbers in them.
+particle dumps) from the numbers in them, and the particle type's
+scatter-table dumps equal the reference's *ScatterMechanism.txt files (bulk
+silicon and the device type in three doping regions).
 GPU: the C++ example (bulkSimulationGpu, the reference's bulkSimulation
 written against the host API) produces exactly the observables of the
 Python binding on the same seed.
@@ -29,6 +31,16 @@ def host_api_binary(tmp_path_factory):
                            os.path.join(ROOT, "tests", "cpp", "host_api_tables.cpp"), "-o", out,
                            "-L", PKG, "-lemcgpu", "-Wl,-rpath," + PKG])
     return out
+
+
+def test_scatter_table_dumps_match_the_reference(host_api_binary, golden, tmp_path):
+    subprocess.check_call([host_api_binary, str(tmp_path)], stdout=subprocess.DEVNULL)
+    keys = [k for k in golden.files if k.startswith("tabfile.")]
+    assert len(keys) == 18
+    for k in keys:
+        _, kind, name = k.split(".")
+        ours = (tmp_path / ("%s_%s.txt" % ("si" if kind == "si_vasileska" else kind, name))).read_bytes()
+        assert ours == bytes(golden[k]), k
 
 
 def test_host_api_tables_match_reference(host_api_binary, golden):

@@ -90,6 +90,7 @@ int main(void) {
   P(emcgpu_device_drift_args); P(emcgpu_device_step_args); P(emcgpu_autotune_args);
   O(emcgpu_mech_desc, bath); O(emcgpu_step_args, draws); O(emcgpu_device_desc, contacts);
   O(emcgpu_device_desc, poisson_tol); O(emcgpu_device_drift_args, draws);
+  O(emcgpu_device_desc, init_from_potential);
   O(emcgpu_device_drift_args, tr_events); O(emcgpu_device_step_args, checksum);
   O(emcgpu_autotune_args, spl); O(emcgpu_autotune_args, res_ms_per_step);
   O(emcgpu_autotune_args, best_ms_per_step); O(emcgpu_device_step_args, poisson_interval);

@@ -430,3 +430,31 @@ def test_cpp_simulation_writes_the_reference_results(golden, tmp_path):
         assert np.allclose(r[3:5], cur[s], rtol=1e-5, atol=0), s
     assert "drift current" in txt
     ctx.close()
+
+
+def test_initial_particles_from_the_equilibrium_potential_match_oracle_twin():
+    """usePotentialForInit: the initial count per node is exp(phi_eq) * n_i
+    times the cell volume (halved on boundaries), then the reference's
+    remainder rule; against the oracle's twin with the same option, the
+    initial ensemble and 20 steps of counts per contact are exact."""
+    emcgpu = _emc()
+    dev = dict(DEVICE_SMALL, width=1e-5)
+    seed, ns, dt = 8080, 20, 1e-15
+    od = O.OracleDevice(dev)
+    M = O.OracleModel(MODELS_DEVICE["device_para"])
+    ref = od.run_philox(M, O.PM_CIC, 1e-14, 1.8, dt, ns, seed, O.thermal_voltage(300.0),
+                        init_from_potential=True)
+    ref_dop = od.run_philox(M, O.PM_CIC, 1e-14, 1.8, dt, 1, seed, O.thermal_voltage(300.0))
+    print("potential init twin: %d particles (doping init: %d)"
+          % (ref["n_init"][0], ref_dop["n_init"][0]), flush=True)
+    assert ref["n_init"][0] != ref_dop["n_init"][0]
+    ctx = emcgpu.Context(precision=emcgpu.FP64)
+    ctx.setup(0, MODELS_DEVICE["device_para"])
+    ctx.device_setup(dev, pm=emcgpu.PM_CIC, poisson_tol=1e-14, poisson_max_iter=80,
+                     init_from_potential=True)
+    assert ctx.device_init(seed) == ref["n_init"][0]
+    out = ctx.device_step(dt, ns, seed=seed)
+    assert (out["n_particles"] == ref["n_part"]).all()
+    assert (out["exits"] == ref["exits"]).all()
+    assert (out["injected"] - out["deleted"] == ref["injs"] - ref["dels"]).all()
+    ctx.close()

@@ -10,7 +10,9 @@ both ends):
   with per-particle streams (drift with positions, boundary reflection,
   removal at contacts): bit-exact;
 * full simulations in the emcSimulation step order with one shared stream
-  (equilibrium, initial particles, Poisson every step or every third step,
+  (equilibrium, initial particles from the doping or, with
+  usePotentialForInit, from the equilibrium potential, Poisson every step
+  or every third step,
   drift/scatter, ohmic contact equilibrium, charge assignment, the
   drift-velocity channel current), CIC and NGP: bit-exact;
 * the reference's own emcSimulation::execute (Poisson every second step,
@@ -144,12 +146,13 @@ def test_device_trajectories_bit_exact(dev, golden):
 CHANNEL = (5e-9, 15e-9, 10e-9)
 
 
-@pytest.mark.parametrize("pm,tag", [(O.PM_CIC, "cic"), (O.PM_NGP, "ngp"), (O.PM_CIC, "cic_sub3")])
+@pytest.mark.parametrize("pm,tag", [(O.PM_CIC, "cic"), (O.PM_NGP, "ngp"), (O.PM_CIC, "cic_sub3"),
+                                    (O.PM_CIC, "cic_potinit")])
 def test_device_simulation_bit_exact(dev, golden, pm, tag):
     b = "device." + tag + "."
-    seed, nsteps, dt, acc, omega, interval = golden[b + "cfg"]
+    seed, nsteps, dt, acc, omega, interval, pot_init = golden[b + "cfg"]
     out = dev.run(_model(), pm, acc, omega, dt, int(nsteps), int(seed), interval=int(interval),
-                  channel=CHANNEL)
+                  channel=CHANNEL, init_from_potential=bool(pot_init))
     # getChannelDriftCurrent after every step
     assert (out["chan"] == golden[b + "chan"]).all()
     if interval > 1:  # the potential changes only on solve steps

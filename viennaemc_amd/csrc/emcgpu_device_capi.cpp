@@ -65,6 +65,7 @@ struct DeviceSim {
   int nContacts = 0;
   double contactV[kMaxContacts] = {0};
   int pm = EMCGPU_PM_CIC;
+  bool init_pot = false;  // initial carriers from the equilibrium potential
   double tol = 1e-10;
   int max_iter = 50;
   // ---- per node / per ohmic node ----------------------------------------
@@ -295,6 +296,7 @@ emcgpu_status build_geometry(emcgpu_ctx *c, DeviceSim &d, const emcgpu_device_de
   d.pm = g.pm_scheme;
   d.tol = g.poisson_tol > 0 ? g.poisson_tol : 1e-10;
   d.max_iter = g.poisson_max_iter > 0 ? g.poisson_max_iter : 50;
+  d.init_pot = g.init_from_potential != 0;
   // emcDevice ctor (emcDevice.hpp:83-99): thermal voltage, intrinsic Debye
   // length; calcCellVolume (:333-343)
   d.Vt = phys::kB / phys::q * d.T;
@@ -1298,12 +1300,20 @@ emcgpu_status device_init(emcgpu_ctx *c, DeviceSim &d, uint64_t seed, size_t *n_
   // particles per node (getInitialNrParticles, emcElectron.hpp:48-61, and the
   // remainder rule of generateInitialParticles,
   // emcAbstractParticleHandler.hpp:133-148): the whole carriers, then one
-  // more with probability equal to the remainder (a Philox draw per node)
+  // more with probability equal to the remainder (a Philox draw per node);
+  // the density is the doping, or exp(equilibrium potential) * n_i
+  std::vector<double> eq;
+  if (d.init_pot) {
+    eq.resize(d.nn);
+    // the context stream is non-blocking: wait for the solve before copying
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(eq.data(), d.d_pot, sizeof(double) * d.nn, hipMemcpyDeviceToHost));
+  }
   std::vector<long long> prefix(d.nn + 1, 0);
   for (int iy = 0; iy < d.ext[1]; iy++)
     for (int ix = 0; ix < d.ext[0]; ix++) {
       const int k = idx(d, ix, iy);
-      double dens = d.doping[k];
+      double dens = d.init_pot ? std::exp(eq[k]) * d.ni : d.doping[k];
       if (ix == 0 || ix == d.ext[0] - 1)
         dens *= 0.5;
       if (iy == 0 || iy == d.ext[1] - 1)

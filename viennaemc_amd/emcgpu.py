@@ -95,7 +95,7 @@ class DeviceDesc(C.Structure):
                 ("temperature", C.c_double), ("eps_r", C.c_double), ("ni", C.c_double),
                 ("n_doping", C.c_int), ("doping", C.c_void_p), ("n_contacts", C.c_int),
                 ("contacts", C.c_void_p), ("pm_scheme", C.c_int), ("poisson_tol", C.c_double),
-                ("poisson_max_iter", C.c_int)]
+                ("poisson_max_iter", C.c_int), ("init_from_potential", C.c_int)]
 
 
 class DeviceDriftArgs(C.Structure):
@@ -516,7 +516,8 @@ class Context:
         return {"best": (a.best_ppl, a.best_spl, a.best_ms_per_step), "trials": trials}
 
     # -- 2-D device (instance 0) ------------------------------------------
-    def device_setup(self, dev, pm=PM_CIC, poisson_tol=1e-10, poisson_max_iter=50):
+    def device_setup(self, dev, pm=PM_CIC, poisson_tol=1e-10, poisson_max_iter=50,
+                     init_from_potential=False):
         """Geometry from a device dictionary (tests/emc_models.py format):
         max_pos, spacing, width, T, eps_r, ni, doping [(min, max, N)],
         contacts [(boundary, voltage, min, max)]."""
@@ -542,6 +543,7 @@ class Context:
         d.pm_scheme = pm
         d.poisson_tol = poisson_tol
         d.poisson_max_iter = poisson_max_iter
+        d.init_from_potential = 1 if init_from_potential else 0
         self._chk(self.L.emcgpu_device_setup(self.h, C.byref(d)))
         self.n_contacts = len(dev["contacts"])
         ext = np.zeros(2, dtype=np.int32)

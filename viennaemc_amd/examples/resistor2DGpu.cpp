@@ -90,7 +90,9 @@ int main(int argc, char **argv) {
   param.setPrecision(fp64 ? emcgpuSimulationParameter<NumType>::FP64
                           : emcgpuSimulationParameter<NumType>::FP32);
 
+  // emcElectron(1000, 1.5, false): initial carriers from the doping
   auto electrons = std::make_unique<emcgpuParticleType<NumType>>(1000, 1.5);
+  electrons->setUsePotentialForInit(false);
   electrons->addValley(
       std::make_unique<emcParabolicIsotropValley<NumType>>(0.26, electrons->getMass(), 1));
   const std::map<SizeType, std::vector<SizeType>> self = {{0, {0}}};

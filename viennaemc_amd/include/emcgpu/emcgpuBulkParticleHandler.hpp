@@ -21,6 +21,7 @@
 #include <array>
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -88,6 +89,11 @@ public:
   /// include/ParticleType/emcElectron.hpp:38)
   std::string getName() const { return typeName; }
   void setName(const std::string &n) { typeName = n; }
+  /// device runs: initial carriers per node from the equilibrium potential
+  /// (emcElectron's usePotentialForInit, on by default there too,
+  /// include/ParticleType/emcElectron.hpp:30-61) or from the doping
+  bool usesPotentialForInit() const { return potentialInit; }
+  void setUsePotentialForInit(bool on) { potentialInit = on; }
   SizeType getNrValleys() const { return valleys.size(); }
   const AbstractValley *getValley(SizeType i) const { return valleys.at(i).get(); }
   SizeType getNrEnergyLevels() const { return nLevels; }
@@ -101,15 +107,39 @@ public:
       m->setPtrValley(valleys);
   }
 
-  /// regions are accepted for interface compatibility; bulk runs use region 0
+  /// the tables use region 0 (the GPU path has one table set per valley);
+  /// the regions name the table dumps of writeTablesToFiles
   template <class DerivedMechanism>
-  void addScatterMechanism(const std::vector<int> & /*regions*/,
+  void addScatterMechanism(const std::vector<int> &regions,
                            std::unique_ptr<DerivedMechanism> &&m) {
     if (m->getIdxValley() >= valleys.size())
       reportError("Used index for Valley is invalid.");
     m->setPtrValley(valleys);
     m->check();
     mechs.push_back(std::move(m));
+    mechRegions.push_back(regions.empty() ? std::vector<int>{0} : regions);
+  }
+
+  /// the scatter-table dumps of initScatterTables (writeTablesToFiles,
+  /// include/emcScatterHandler.hpp:198-217): per mechanism and region a file
+  /// <prefix><name><region><valley>ScatterMechanism.txt with one line
+  /// "energy rate" per level, the energy accumulated in steps of dE
+  void writeTablesToFiles(const std::string &prefix = "") const {
+    const T dE = maxE / (T)nLevels;
+    for (SizeType j = 0; j < mechs.size(); j++)
+      for (int region : mechRegions[j]) {
+        const std::string name = prefix + mechs[j]->getName() + std::to_string(region) +
+                                 std::to_string(mechs[j]->getIdxValley()) +
+                                 "ScatterMechanism.txt";
+        std::ofstream os(name);
+        if (!os)
+          reportError("cannot write " + name);
+        T e = dE;
+        for (SizeType l = 0; l < nLevels; l++) {
+          os << e << " " << mechs[j]->getScatterRate((T)(l + 1) * dE, (SizeType)region) << "\n";
+          e += dE;
+        }
+      }
   }
 
   /// raw rates rates[mech][level] at energies (level+1) * maxEnergy / nLevels
@@ -142,8 +172,10 @@ private:
   SizeType nLevels;
   T maxE, particleMass, particleCharge;
   std::string typeName = "Electrons";
+  bool potentialInit = true;
   std::vector<std::unique_ptr<AbstractValley>> valleys;
   std::vector<std::unique_ptr<AbstractScatterMechanism>> mechs;
+  std::vector<std::vector<int>> mechRegions;
 };
 
 template <class T>

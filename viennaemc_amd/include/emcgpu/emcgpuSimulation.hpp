@@ -298,8 +298,9 @@ private:
     check(emcgpu_set_rate_tables(ctx, 0, (int)type.getNrEnergyLevels(), type.getMaxEnergy(),
                                  rates.data()),
           ctx, "emcgpu_set_rate_tables");
-    const emcgpu_device_desc d =
+    emcgpu_device_desc d =
         device.descriptor(PMScheme::id, solver.getTolerance(), solver.getMaxIter());
+    d.init_from_potential = type.usesPotentialForInit() ? 1 : 0;
     check(emcgpu_device_setup(ctx, &d), ctx, "emcgpu_device_setup");
     check(emcgpu_device_info(ctx, ext, scal), ctx, "emcgpu_device_info");
     nContacts = (int)device.getNrContacts();
