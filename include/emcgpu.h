@@ -64,6 +64,10 @@ typedef enum {
  *   HOT_FROEHLICH_* emcHotPhononFroehlichMechanism.hpp:39-200: the Froehlich
  *                 rate with the occupation of a phonon bath (getMeanNq) and
  *                 the Froehlich final state, recording |k' - k| in the bath
+ *   SCREENED_FROEHLICH_* emcScreenedFroehlichInteraction.hpp:52-77,101-213:
+ *                 the Froehlich rate with the vertex 1/q^2 screened to
+ *                 1/(q^2 + q_s^2) and the angle drawn from the screened
+ *                 distribution; q_s^2 from emcgpu_set_screening
  *   custom        rate supplied by the caller (emcgpu_set_rate_tables) with
  *                 one of the built-in final states */
 typedef enum {
@@ -75,7 +79,9 @@ typedef enum {
   EMCGPU_MECH_FROEHLICH_AB = 5,
   EMCGPU_MECH_FROEHLICH_EM = 6,
   EMCGPU_MECH_HOT_FROEHLICH_AB = 7,
-  EMCGPU_MECH_HOT_FROEHLICH_EM = 8
+  EMCGPU_MECH_HOT_FROEHLICH_EM = 8,
+  EMCGPU_MECH_SCREENED_FROEHLICH_AB = 9,
+  EMCGPU_MECH_SCREENED_FROEHLICH_EM = 10
 } emcgpu_mech_kind;
 
 typedef struct {
@@ -106,7 +112,7 @@ typedef struct {
   double phonon_energy; /* eV (IV, Froehlich) */
   double coupling;    /* acoustic: deformation potential [eV]; zero-order IV:
                          [eV/m]; first-order IV: [eV]; unused otherwise */
-  double rel_eff_mass; /* Froehlich: m* / m0 of the rate prefactor */
+  double rel_eff_mass; /* Froehlich kinds: m* / m0 of the rate prefactor */
   double eps_hi, eps_lo; /* Froehlich: relative permittivities */
   int z;              /* IV: number of final subvalleys per initial one */
   int final_sub[EMCGPU_MAX_DEG][EMCGPU_MAX_Z]; /* IV: finalSubValleys map */
@@ -135,6 +141,21 @@ typedef struct {
 typedef struct {
   double eps_r, rho, ni, v_sound; /* emcMaterial.hpp:23-25 (rho kg/m^3, v m/s) */
 } emcgpu_material;
+
+/* emcPlasmonScreening (include/emcPlasmonScreening.hpp:52-90): the q_s^2
+ * that an instance's SCREENED_FROEHLICH_* mechanisms share.
+ *   FIXED   q_s^2 as given (setQs2).  Like the reference's setQs2 it acts on
+ *           the final-state angle at once; the rates follow at the next
+ *           emcgpu_build_tables (reinitScatterTables). */
+typedef enum { EMCGPU_SCREENING_FIXED = 0 } emcgpu_screening_mode;
+
+typedef struct {
+  int mode;          /* emcgpu_screening_mode */
+  int enabled;       /* 0: q_s^2 stays 0 (emcPlasmonScreening(eps, false)),
+                        the screened kinds reduce to the bare Froehlich ones */
+  double eps_static; /* static relative permittivity (Debye form) */
+  double qs2;        /* q_s^2 [1/m^2], >= 0 */
+} emcgpu_screening_desc;
 
 typedef struct {
   double dt;            /* time step [s] */
@@ -238,6 +259,15 @@ emcgpu_status emcgpu_init_particles(emcgpu_ctx *ctx, int instance,
 emcgpu_status emcgpu_init_particles_energy(emcgpu_ctx *ctx, int instance,
                                            uint64_t seed, double mean_ev,
                                            double sigma_ev);
+
+/* ---- plasmon screening ---------------------------------------------------
+ * Replaces the shared_ptr<emcPlasmonScreening> of the screened mechanisms
+ * (emcScreenedFroehlichInteraction.hpp:105, 114) and its setQs2/getQs2
+ * (emcPlasmonScreening.hpp:79-82).  Per instance; kept across
+ * emcgpu_set_model; q_s^2 = 0 until set. */
+emcgpu_status emcgpu_set_screening(emcgpu_ctx *ctx, int instance,
+                                   const emcgpu_screening_desc *desc);
+emcgpu_status emcgpu_get_screening(const emcgpu_ctx *ctx, int instance, double *qs2);
 
 /* ---- phonon baths --------------------------------------------------------
  * Baths of an instance (replacing the shared_ptr<emcPhononBath> given to

@@ -662,6 +662,38 @@ double orc_bath_to_temp(const orc_bath *b) {
   return orc_planck_temp(b->toE, b->Nto);
 }
 
+/* screenedFroehlichLogFactor (emcScreenedFroehlichInteraction.hpp:52-62):
+ * 1/2 ln((q+^2 + qs2) / (q-^2 + qs2)), q+- = kI +- kF */
+double orc_scr_log_factor(double kI, double kF, double qs2) {
+  const double qPlus2 = (kI + kF) * (kI + kF);
+  const double qMinus2 = (kI - kF) * (kI - kF);
+  if (qs2 <= 0.0) {
+    if (qMinus2 <= 0.0)
+      return 0.0;
+    return 0.5 * log(qPlus2 / qMinus2);
+  }
+  return 0.5 * log((qPlus2 + qs2) / (qMinus2 + qs2));
+}
+
+static double orc_stdmin(double a, double b);
+static double orc_stdmax(double a, double b);
+
+/* sampleScreenedFroehlichCosTheta (:65-77): the inverse of the cumulative
+ * distribution of P(cos) ~ 1 / (A' - B cos), A' = kI^2 + kF^2 + qs2,
+ * B = 2 kI kF; isotropic where the form degenerates */
+double orc_scr_cos_theta(double kI, double kF, double qs2, double r) {
+  const double B = 2.0 * kI * kF;
+  if (B <= 0.0)
+    return 1.0 - 2.0 * r;
+  const double Ap = kI * kI + kF * kF + qs2;
+  const double num = Ap - B;
+  const double den = Ap + B;
+  if (num <= 0.0 || den <= 0.0)
+    return 1.0 - 2.0 * r;
+  const double cosTheta = (Ap - den * pow(num / den, r)) / B;
+  return orc_stdmax(-1.0, orc_stdmin(1.0, cosTheta));
+}
+
 /* getScatterRate of each mechanism kind */
 double orc_mech_rate(const orc_model *md, orc_mech *m, double energy) {
   const orc_valley *iv = &md->valley[m->valley];
@@ -741,6 +773,20 @@ double orc_mech_rate(const orc_model *md, orc_mech *m, double energy) {
     double Nq = orc_bath_mean_nq(&md->bath[m->bath]);
     return m->scatterConst * Nq / kI * ln;
   }
+  case ORC_MECH_SCR_FROEHLICH_AB:
+  case ORC_MECH_SCR_FROEHLICH_EM: { /* emcScreenedFroehlichInteraction.hpp:128-138 / :184-196 */
+    const int ab = m->kind == ORC_MECH_SCR_FROEHLICH_AB;
+    if (!ab && energy <= m->phononEnergy)
+      return 0.0;
+    double gammaI = orc_valley_gamma(iv, energy);
+    double gammaF = orc_valley_gamma(iv, ab ? energy + m->phononEnergy : energy - m->phononEnergy);
+    if (gammaI <= 0.0 || gammaF <= 0.0)
+      return 0.0;
+    double kI = sqrt(2.0 * m->effMass * gammaI * ORC_Q) / ORC_HBAR;
+    double kF = sqrt(2.0 * m->effMass * gammaF * ORC_Q) / ORC_HBAR;
+    return (ab ? m->scatterConst * m->nBose : m->scatterConst * (m->nBose + 1.0)) / kI *
+           orc_scr_log_factor(kI, kF, md->qs2);
+  }
   case ORC_MECH_HOT_FROEHLICH_EM: { /* emcHotPhononFroehlichMechanism.hpp:159-173 */
     if (energy <= m->phononEnergy)
       return 0.0;
@@ -778,12 +824,46 @@ static void orc_hot_record(const orc_model *md, const orc_mech *m,
 static double orc_stdmin(double a, double b) { return (b < a) ? b : a; }
 static double orc_stdmax(double a, double b) { return (a < b) ? b : a; }
 
+/* scatterParticle of the screened Froehlich kinds
+ * (emcScreenedFroehlichInteraction.hpp:140-152 / :198-211): kI and kF from
+ * the valley's |k|(E) before and after the energy change, the screened
+ * angle from uniform ua, the azimuth from ub, |k| rescaled to kF */
+static void orc_scr_final(const orc_model *md, const orc_mech *m, orc_particle *p,
+                          double ua, double ub) {
+  const orc_valley *v = &md->valley[m->valley];
+  double kI = orc_valley_normk(v, p->energy);
+  if (m->kind == ORC_MECH_SCR_FROEHLICH_AB)
+    p->energy += m->phononEnergy;
+  else
+    p->energy -= m->phononEnergy;
+  double kF = orc_valley_normk(v, p->energy);
+  double cosTheta = orc_scr_cos_theta(kI, kF, md->qs2, ua);
+  double nk[3];
+  orc_init_random_direction_wrt_k(p->k, cosTheta, ub, nk);
+  memcpy(p->k, nk, sizeof nk);
+  double kCurr = orc_norm3(p->k);
+  if (kCurr > 0.0) {
+    double s = kF / kCurr;
+    for (int i = 0; i < 3; i++)
+      p->k[i] = p->k[i] * s;
+  }
+}
+
 /* scatterParticle of each mechanism kind.  Two uniforms fed to
  * initRandomDirection are drawn right-to-left (g++ evaluates the call
  * arguments in that order): cos(theta) first, then phi. */
 void orc_mech_scatter(const orc_model *md, const orc_mech *m,
                       orc_particle *p, orc_rng *rng) {
   switch (m->kind) {
+  case ORC_MECH_SCR_FROEHLICH_AB:
+  case ORC_MECH_SCR_FROEHLICH_EM: {
+    /* the angle's uniform is drawn before the azimuth's (separate
+     * statements) */
+    double ua = orc_rng_uniform(rng, 0., 1.);
+    double ub = orc_rng_uniform(rng, 0., 1.);
+    orc_scr_final(md, m, p, ua, ub);
+    return;
+  }
   case ORC_MECH_ACOUSTIC: { /* emcAcousticScatterMechanism.hpp:70-72 */
     double n = orc_norm3(p->k);
     double r2 = orc_rng_uniform(rng, 0., 1.);
@@ -1382,6 +1462,10 @@ static void orc_mech_scatter_u(const orc_model *md, const orc_mech *m,
     orc_init_random_direction(knew, ua, ub, p->k);
     return;
   }
+  case ORC_MECH_SCR_FROEHLICH_AB:
+  case ORC_MECH_SCR_FROEHLICH_EM:
+    orc_scr_final(md, m, p, ua, ub);
+    return;
   case ORC_MECH_FROEHLICH_AB:
   case ORC_MECH_FROEHLICH_EM:
   case ORC_MECH_HOT_FROEHLICH_AB:
@@ -1555,6 +1639,18 @@ int orc_model_add_froehlich(orc_model *m, int kind, int valley,
   orc_mech_froehlich(&mech, kind, valley, phononEnergy, relEffMass, epsHi, epsLo, T);
   return orc_model_add_mech(m, &mech);
 }
+
+int orc_model_add_screened_froehlich(orc_model *m, int kind, int valley,
+                                     double phononEnergy, double relEffMass,
+                                     double epsHi, double epsLo, double T) {
+  orc_mech mech;
+  orc_mech_froehlich(&mech, ORC_MECH_FROEHLICH_AB, valley, phononEnergy, relEffMass, epsHi,
+                     epsLo, T);
+  mech.kind = kind;
+  return orc_model_add_mech(m, &mech);
+}
+
+void orc_model_set_qs2(orc_model *m, double qs2) { m->qs2 = qs2; }
 
 int orc_model_n_tables(const orc_model *m, int valley) { return m->nTables[valley]; }
 int orc_model_table_mech(const orc_model *m, int valley, int j) {

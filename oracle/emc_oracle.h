@@ -148,7 +148,13 @@ enum {
   ORC_MECH_FROEHLICH_EM = 6,
   /* emcHotPhononFroehlichMechanism.hpp:39-118 / :122-200 */
   ORC_MECH_HOT_FROEHLICH_AB = 7,
-  ORC_MECH_HOT_FROEHLICH_EM = 8
+  ORC_MECH_HOT_FROEHLICH_EM = 8,
+  /* emcScreenedFroehlich{Absorption,Emission}3D
+   * (emcScreenedFroehlichInteraction.hpp:96-204): the equilibrium Froehlich
+   * rate and angle with the vertex 1/q^2 screened to 1/(q^2 + q_s^2), q_s^2
+   * the model's (a shared emcPlasmonScreening, emcPlasmonScreening.hpp) */
+  ORC_MECH_SCR_FROEHLICH_AB = 9,
+  ORC_MECH_SCR_FROEHLICH_EM = 10
 };
 
 typedef struct {
@@ -241,6 +247,8 @@ typedef struct {
   /* emcElectron's initEnergyEV (emcElectron.hpp:27,84-85): > 0 selects the
    * fixed-energy initialisation in orc_run_shared */
   double initEnergy;
+  /* q_s^2 [1/m^2] of the screened mechanisms (emcPlasmonScreening::getQs2) */
+  double qs2;
 } orc_model;
 
 void orc_model_init(orc_model *m, int nLevels, double maxEnergy);
@@ -363,6 +371,17 @@ int orc_model_add_iv(orc_model *m, int kind, int valley, int finalValley,
 int orc_model_add_froehlich(orc_model *m, int kind, int valley,
                             double phononEnergy, double relEffMass,
                             double epsHi, double epsLo, double T);
+/* kind = ORC_MECH_SCR_FROEHLICH_{AB,EM} (constructor :107-122 / :163-178) */
+int orc_model_add_screened_froehlich(orc_model *m, int kind, int valley,
+                                     double phononEnergy, double relEffMass,
+                                     double epsHi, double epsLo, double T);
+/* emcPlasmonScreening::setQs2 (enabled): the q_s^2 the screened rates and
+ * angles read (rebuild the tables afterwards) */
+void orc_model_set_qs2(orc_model *m, double qs2);
+/* screenedFroehlichLogFactor / sampleScreenedFroehlichCosTheta
+ * (emcScreenedFroehlichInteraction.hpp:52-77) */
+double orc_scr_log_factor(double kI, double kF, double qs2);
+double orc_scr_cos_theta(double kI, double kF, double qs2, double r);
 int orc_model_n_tables(const orc_model *m, int valley);
 int orc_model_table_mech(const orc_model *m, int valley, int j);
 double orc_model_tau0(const orc_model *m, int valley);

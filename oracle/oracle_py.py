@@ -45,6 +45,10 @@ def load():
         "orc_model_add_acoustic": (_I, [vp, _I, _D, _D, _D, _D]),
         "orc_model_add_iv": (_I, [vp, _I, _I, _I, _D, _D, _D, _D, _I, _PI32]),
         "orc_model_add_froehlich": (_I, [vp, _I, _I, _D, _D, _D, _D, _D]),
+        "orc_model_add_screened_froehlich": (_I, [vp, _I, _I, _D, _D, _D, _D, _D]),
+        "orc_model_set_qs2": (None, [vp, _D]),
+        "orc_scr_log_factor": (_D, [_D, _D, _D]),
+        "orc_scr_cos_theta": (_D, [_D, _D, _D, _D]),
         "orc_model_build_tables": (None, [vp]),
         "orc_model_n_tables": (_I, [vp, _I]),
         "orc_model_table_mech": (_I, [vp, _I, _I]),
@@ -146,6 +150,8 @@ class OracleModel:
         self.n_baths = len(spec.get("baths", []))
         if spec.get("init_energy"):
             L.orc_model_set_init_energy(self.h, spec["init_energy"])
+        if spec.get("screening"):
+            L.orc_model_set_qs2(self.h, spec["screening"]["qs2"])
         for v in spec["valleys"]:
             axes = None
             if "axes" in v:
@@ -168,6 +174,9 @@ class OracleModel:
             elif k in (7, 8):
                 L.orc_model_add_hot_froehlich(self.h, k, mech["valley"], mech["hw"], mech["mrel"],
                                               mech["eps_hi"], mech["eps_lo"], mech["bath"])
+            elif k in (9, 10):
+                L.orc_model_add_screened_froehlich(self.h, k, mech["valley"], mech["hw"], mech["mrel"],
+                                                   mech["eps_hi"], mech["eps_lo"], T)
             else:
                 L.orc_model_add_froehlich(self.h, k, mech["valley"], mech["hw"], mech["mrel"],
                                           mech["eps_hi"], mech["eps_lo"], T)

@@ -26,6 +26,8 @@
 #include <ScatterMechanisms/emcFirstOrderInterValleyScatterMechanism.hpp>
 #include <ScatterMechanisms/emcFroehlichInteraction.hpp>
 #include <ScatterMechanisms/emcHotPhononFroehlichMechanism.hpp>
+#include <ScatterMechanisms/emcScreenedFroehlichInteraction.hpp>
+#include <emcPlasmonScreening.hpp>
 #include <ScatterMechanisms/emcZeroOrderInterValleyScatterMechanism.hpp>
 #include <ValleyTypes/emcNonParabolicAnistropValley.hpp>
 #include <ValleyTypes/emcNonParabolicIsotropValley.hpp>
@@ -143,7 +145,7 @@ static void addValleys(const Model &m, Electron &tp) {
           {kSubAxes[s][1][0], kSubAxes[s][1][1], kSubAxes[s][1][2]},
           {kSubAxes[s][2][0], kSubAxes[s][2][1], kSubAxes[s][2][2]});
     tp.addValley(std::move(v));
-  } else if (m.name == "polar_gamma" || m.name == "hot_polar") {
+  } else if (m.name == "polar_gamma" || m.name == "hot_polar" || m.name == "screened_polar") {
     tp.addValley(std::make_unique<emcNonParabolicIsotropValley<T>>(0.28, tp.getMass(), 1, 0.3));
   } else if (m.name == "para_iso") {
     tp.addValley(std::make_unique<emcParabolicIsotropValley<T>>(0.26, tp.getMass(), 1));
@@ -195,6 +197,12 @@ static std::vector<MechPtr> makeMechs(const Model &m, const Device &device,
   } else if (m.name == "polar_gamma") {
     v.push_back(std::make_unique<emcFroehlichAbsorption3D<T>>(0, 0.06, 0.28, 3.5, 10., m.T));
     v.push_back(std::make_unique<emcFroehlichEmission3D<T>>(0, 0.06, 0.28, 3.5, 10., m.T));
+  } else if (m.name == "screened_polar") {
+    // one emcPlasmonScreening shared by both mechanisms, q_s^2 = 1e17 / m^2
+    auto scr = std::make_shared<emcPlasmonScreening<T>>(10., true);
+    scr->setQs2(1e17);
+    v.push_back(std::make_unique<emcScreenedFroehlichAbsorption3D<T>>(0, 0.06, 0.28, 3.5, 10., m.T, scr));
+    v.push_back(std::make_unique<emcScreenedFroehlichEmission3D<T>>(0, 0.06, 0.28, 3.5, 10., m.T, scr));
   } else if (m.name == "hot_polar") {
     v.push_back(std::make_unique<emcHotPhononFroehlichAbsorption3D<T>>(0, 0.06, 0.28, 3.5, 10., bath));
     v.push_back(std::make_unique<emcHotPhononFroehlichEmission3D<T>>(0, 0.06, 0.28, 3.5, 10., bath));
@@ -971,6 +979,7 @@ int main(int argc, char **argv) {
       {"para_iso", 1000, 1.5, 77., 5., {1, 1, 0}},
       {"two_valley", 1500, 1.5, 300., 30., {1, 1, 1}},
       {"hot_polar", 2000, 2.0, 300., 10., {0, 0, 1}},
+      {"screened_polar", 2000, 2.0, 300., 20., {0, 0, 1}},
   };
   const uint64_t seed = 20261202ULL;
   dumpBathKATs();

@@ -17,7 +17,7 @@ ME = 9.11e-31     # include/emcConstants.hpp:24
 
 PARA_ISO, NONPARA_ISO, PARA_ANISO, NONPARA_ANISO = 0, 1, 2, 3
 (ACOUSTIC, ZERO_IV_AB, ZERO_IV_EM, FIRST_IV_AB, FIRST_IV_EM, FROEHLICH_AB, FROEHLICH_EM,
- HOT_FROEHLICH_AB, HOT_FROEHLICH_EM) = range(9)
+ HOT_FROEHLICH_AB, HOT_FROEHLICH_EM, SCR_FROEHLICH_AB, SCR_FROEHLICH_EM) = range(11)
 
 SI_RHO = 2329.0   # examples/SiliconFunctions.hpp:22
 SI_VSOUND = 9040.0  # examples/SiliconFunctions.hpp:24
@@ -136,6 +136,16 @@ MODELS["hot_polar"] = dict(
     baths=[dict(n_bins=256, dq=2e7, tau_lo=1e-12, hw=0.06, volume=2.7e-23)],
     mechs=[dict(kind=HOT_FROEHLICH_AB, valley=0, hw=0.06, mrel=0.28, eps_hi=3.5, eps_lo=10.0, bath=0),
            dict(kind=HOT_FROEHLICH_EM, valley=0, hw=0.06, mrel=0.28, eps_hi=3.5, eps_lo=10.0, bath=0)])
+
+# screened polar model: the Kane Gamma valley of polar_gamma with
+# emcScreenedFroehlich{Absorption,Emission}3D sharing one
+# emcPlasmonScreening at q_s^2 = 1e17 / m^2 (screening length 3.2 nm)
+MODELS["screened_polar"] = dict(
+    n_levels=2000, max_energy=2.0, T=300.0, field_kvcm=20.0, field_dir=(0, 0, 1),
+    valleys=[dict(kind=NONPARA_ISO, mass=(0.28, 0.28, 0.28), deg=1, alpha=0.3, bottom=0.0)],
+    screening=dict(qs2=1e17),
+    mechs=[dict(kind=SCR_FROEHLICH_AB, valley=0, hw=0.06, mrel=0.28, eps_hi=3.5, eps_lo=10.0),
+           dict(kind=SCR_FROEHLICH_EM, valley=0, hw=0.06, mrel=0.28, eps_hi=3.5, eps_lo=10.0)])
 
 MODEL_NAMES = list(MODELS)
 

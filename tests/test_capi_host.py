@@ -64,6 +64,8 @@ def test_host_tables_match_reference(name, golden, oracle_models):
     ctx.set_model(0, spec)
     if spec.get("baths"):
         ctx.set_baths(0, spec["baths"], spec["T"])
+    if spec.get("screening"):
+        ctx.set_screening(0, spec["screening"]["qs2"])
     ctx.build_tables(0, spec["n_levels"], spec["max_energy"], spec["T"])
     M = O.OracleModel(spec) if spec.get("baths") else oracle_models[name]
     for v in range(len(spec["valleys"])):

@@ -38,7 +38,8 @@ def _rel_k(a, b):
 
 
 @pytest.mark.parametrize("ppl", [1, 2, 4, 8])
-@pytest.mark.parametrize("name", ["si_jacoboni", "polar_gamma", "two_valley", "para_iso"])
+@pytest.mark.parametrize("name", ["si_jacoboni", "polar_gamma", "two_valley", "para_iso",
+                                  "screened_polar"])
 def test_fp64_production_matches_oracle_twin(name, ppl):
     emcgpu = _emc()
     spec = MODELS[name]
@@ -135,6 +136,34 @@ def test_ensemble_matches_reference_rng(precision):
     assert np.all(np.abs(occ - occ_cpu) < 4 * sig), (occ, occ_cpu)
     assert abs(occ[1] - occ[2]) < 4 * np.sqrt(2 * occ[1] * (1 - occ[1]) / n_gpu)
     assert occ[0] > occ[1] + 10 * sig[0]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_screened_ensemble_matches_reference_rng(precision):
+    """Screened polar optical scattering (q_s^2 = 1e17 / m^2, 20 kV/cm): GPU
+    (Philox) vs the oracle's ensemble on the reference engine."""
+    emcgpu = _emc()
+    name = "screened_polar"
+    spec = MODELS[name]
+    nsteps, dt = 400, 1e-15
+    n_gpu, n_cpu = 200_000, 20_000
+    print("screened ensemble vs reference rng, precision", precision, flush=True)
+    M = O.OracleModel(spec)
+    force, d = field_force(spec)
+    obs_cpu, fin = M.run_mt(n_cpu, 20261203, O.thermal_voltage(spec["T"]), nsteps, dt, force, d,
+                            threads=8, final=True)
+    vel = np.array([M.velocity(int(v), int(s), k, e) @ np.array(d) for v, s, k, e in
+                    zip(fin["valley"], fin["sub"], fin["k"].reshape(-1, 3), fin["E"])])
+    sig_v, sig_e = vel.std(), fin["E"].std()
+    ctx = _ctx(name, n_gpu, precision)
+    ctx.init_particles(0, 99, spec["T"])
+    ctx.step(dt, nsteps, seed=99)
+    av = ctx.averages(0)
+    for s in (49, 149, 299, 399):
+        assert _stat_compare(av["avgV"][s, 0], sig_v / np.sqrt(n_gpu),
+                             obs_cpu[s, 1, 0], sig_v / np.sqrt(n_cpu)), (s, av["avgV"][s, 0], obs_cpu[s, 1, 0])
+        assert _stat_compare(av["avgE"][s, 0], sig_e / np.sqrt(n_gpu),
+                             obs_cpu[s, 0, 0], sig_e / np.sqrt(n_cpu)), (s, av["avgE"][s, 0], obs_cpu[s, 0, 0])
 
 
 def test_fp32_matches_fp64():

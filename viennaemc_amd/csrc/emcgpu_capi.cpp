@@ -138,6 +138,12 @@ void lower_instance(const emcgpu_ctx *c, const Instance &in, double dt_s, InstDe
             mm.final_sub[s][z] = (uint8_t)md.final_sub[s][z];
         break;
       }
+      case EMCGPU_MECH_SCREENED_FROEHLICH_AB:
+      case EMCGPU_MECH_SCREENED_FROEHLICH_EM:
+        mm.final_kind = kFinalScreenedFroehlich;
+        mm.shift = md.kind == EMCGPU_MECH_SCREENED_FROEHLICH_AB ? (R)md.phonon_energy
+                                                                 : (R)(-md.phonon_energy);
+        break;
       default: {
         const bool ab = md.kind == EMCGPU_MECH_FROEHLICH_AB || md.kind == EMCGPU_MECH_HOT_FROEHLICH_AB;
         mm.final_kind = kFinalFroehlich;
@@ -149,6 +155,7 @@ void lower_instance(const emcgpu_ctx *c, const Instance &in, double dt_s, InstDe
       }
     }
   }
+  P.qs2 = (R)(in.qs2 * kToNm * kToNm);
   P.n_baths = (int)in.baths.size();
   P.total_bins = in.total_bins;
   P.gbin_off = in.gbin_off;
@@ -999,7 +1006,7 @@ emcgpu_status emcgpu_set_model(emcgpu_ctx *c, int instance, const emcgpu_valley_
   }
   for (int j = 0; j < n_mechs; j++) {
     const emcgpu_mech_desc &m = mechs[j];
-    if (m.kind < 0 || m.kind > EMCGPU_MECH_HOT_FROEHLICH_EM)
+    if (m.kind < 0 || m.kind > EMCGPU_MECH_SCREENED_FROEHLICH_EM)
       return fail(c, EMCGPU_ERR_ARG, "mechanism %d: unknown kind %d", j, m.kind);
     if ((m.kind == EMCGPU_MECH_HOT_FROEHLICH_AB || m.kind == EMCGPU_MECH_HOT_FROEHLICH_EM) &&
         (m.bath < 0 || m.bath >= EMCGPU_MAX_BATHS))
@@ -1020,8 +1027,7 @@ emcgpu_status emcgpu_set_model(emcgpu_ctx *c, int instance, const emcgpu_valley_
                         "mechanism %d: Used idx %d for valley of degeneracy %d is not valid", j,
                         m.final_sub[s][z], fdeg);
     }
-    if ((m.kind >= EMCGPU_MECH_ZERO_IV_AB && m.kind <= EMCGPU_MECH_HOT_FROEHLICH_EM) &&
-        !(m.phonon_energy > 0))
+    if (m.kind != EMCGPU_MECH_ACOUSTIC && !(m.phonon_energy > 0))
       return fail(c, EMCGPU_ERR_ARG, "mechanism %d: phonon energy must be > 0", j);
   }
   Instance &in = c->inst[instance];
@@ -1110,7 +1116,8 @@ emcgpu_status emcgpu_build_tables(emcgpu_ctx *c, int instance, int n_levels, dou
     const bool hot =
         m.kind == EMCGPU_MECH_HOT_FROEHLICH_AB || m.kind == EMCGPU_MECH_HOT_FROEHLICH_EM;
     const bool ab = m.kind == EMCGPU_MECH_ZERO_IV_AB || m.kind == EMCGPU_MECH_FIRST_IV_AB ||
-                    m.kind == EMCGPU_MECH_FROEHLICH_AB || m.kind == EMCGPU_MECH_HOT_FROEHLICH_AB;
+                    m.kind == EMCGPU_MECH_FROEHLICH_AB || m.kind == EMCGPU_MECH_HOT_FROEHLICH_AB ||
+                    m.kind == EMCGPU_MECH_SCREENED_FROEHLICH_AB;
     if (hot) {
       // emcHotPhononFroehlichMechanism.hpp:79-91 / :159-173: the Froehlich
       // rate with the bath's mean occupation; kI and ln kept per level for
@@ -1177,6 +1184,10 @@ emcgpu_status emcgpu_build_tables(emcgpu_ctx *c, int instance, int n_levels, dou
         r = phys::firstOrderRate(C, iv, fv, E, m.phonon_energy, fv.bottom - iv.bottom, ab);
         break;
       }
+      case EMCGPU_MECH_SCREENED_FROEHLICH_AB:
+      case EMCGPU_MECH_SCREENED_FROEHLICH_EM:
+        r = phys::screenedFroehlichRate(C, nq, meff, iv, E, m.phonon_energy, in.qs2, ab);
+        break;
       default:
         r = phys::froehlichRate(C, nq, meff, iv, E, m.phonon_energy, ab);
         break;
@@ -1391,6 +1402,39 @@ emcgpu_status emcgpu_init_particles_energy(emcgpu_ctx *c, int instance, uint64_t
   (void)hipSetDevice(c->device);
   return c->precision == EMCGPU_FP32 ? init_thermal<float>(c, instance, seed, mean_ev, sigma_ev)
                                      : init_thermal<double>(c, instance, seed, mean_ev, sigma_ev);
+}
+
+// emcPlasmonScreening::setQs2 (emcPlasmonScreening.hpp:79): q_s^2 of the
+// instance's screened mechanisms, 0 when disabled.  The final-state angle
+// reads it from the next step on; the rates at the next table build.
+emcgpu_status emcgpu_set_screening(emcgpu_ctx *c, int instance, const emcgpu_screening_desc *d) {
+  emcgpu_status st = check_instance(c, instance);
+  if (st)
+    return st;
+  if (!d)
+    return fail(c, EMCGPU_ERR_ARG, "screening descriptor required");
+  if (d->mode != EMCGPU_SCREENING_FIXED)
+    return fail(c, EMCGPU_ERR_ARG, "unknown screening mode %d", d->mode);
+  if (!(d->qs2 >= 0.0) || !std::isfinite(d->qs2))
+    return fail(c, EMCGPU_ERR_ARG, "q_s^2 must be finite and >= 0 (%g)", d->qs2);
+  if (!(d->eps_static > 0.0))
+    return fail(c, EMCGPU_ERR_ARG, "eps_static must be > 0");
+  Instance &in = c->inst[instance];
+  in.scr = *d;
+  in.qs2 = d->enabled ? d->qs2 : 0.0;
+  c->params_dirty = true;
+  return EMCGPU_OK;
+}
+
+emcgpu_status emcgpu_get_screening(const emcgpu_ctx *c0, int instance, double *qs2) {
+  emcgpu_ctx *c = const_cast<emcgpu_ctx *>(c0);
+  emcgpu_status st = check_instance(c, instance);
+  if (st)
+    return st;
+  if (!qs2)
+    return fail(c, EMCGPU_ERR_ARG, "qs2 pointer required");
+  *qs2 = c->inst[instance].qs2;
+  return EMCGPU_OK;
 }
 
 emcgpu_status emcgpu_set_baths(emcgpu_ctx *c, int instance, const emcgpu_bath_desc *baths,

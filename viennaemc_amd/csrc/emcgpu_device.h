@@ -36,7 +36,8 @@ constexpr int kTallyCopies = 64;
 enum FinalKind : int {
   kFinalIsotropicElastic = 0,  // acoustic: |k| kept, random direction
   kFinalIsotropicInelastic = 1,  // IV: E += shift, new valley/subvalley
-  kFinalFroehlich = 2            // polar optical: angle wrt old k
+  kFinalFroehlich = 2,           // polar optical: angle wrt old k
+  kFinalScreenedFroehlich = 3    // screened polar optical: angle wrt old k
 };
 
 template <typename R>
@@ -85,6 +86,7 @@ struct InstDev {
   int bath_loff[kMaxBaths];
   R bath_dq[kMaxBaths];  // bin width [1/nm]
   long long gbin_off;
+  R qs2;  // plasmon screening q_s^2 [1/nm^2]
 };
 
 // constants of one LO phonon bath (emcPhononBath<double>)

@@ -61,6 +61,9 @@ struct Instance {
   long long gbin_off = 0;
   int total_bins = 0;
   bool dev_bath = false;        // the device holds the current bath state
+  // plasmon screening of the screened Froehlich kinds: q_s^2 [1/m^2]
+  emcgpu_screening_desc scr{EMCGPU_SCREENING_FIXED, 0, 1.0, 0.0};
+  double qs2 = 0;
   std::vector<long long> cum_off;  // per valley: element offset of the device tables
 };
 

@@ -256,8 +256,28 @@ __device__ __forceinline__ void hot_record(const InstDev<R> *P, const MechDev<R>
   atomicAdd(&lds_tally[2 * (P->bath_loff[b] + bin) + (m.shift < R(0) ? 0 : 1)], 1u);
 }
 
+// sampleScreenedFroehlichCosTheta (emcScreenedFroehlichInteraction.hpp:
+// 65-77): inverse of the cumulative distribution of P(c) ~ 1 / (A' - B c),
+// A' = kI^2 + kF^2 + qs2, B = 2 kI kF, with A' - B formed as
+// (kI - kF)^2 + qs2 (the same value without the cancellation); isotropic
+// 1 - 2u where the form degenerates
+template <typename R>
+__device__ __forceinline__ R screened_cos_theta(R ki, R kf, R qs2, R u) {
+  const R B = R(2) * ki * kf;
+  const R dk = ki - kf;
+  const R ap = ki * ki + kf * kf + qs2;
+  const R num = dk * dk + qs2;
+  const R den = ap + B;
+  if (!(B > R(0)) || !(num > R(0)) || !(den > R(0)))
+    return R(1) - R(2) * u;
+  R c = Math<R>::div(ap - den * Math<R>::pow(Math<R>::div(num, den), u), B);
+  c = (c < R(1)) ? c : R(1);   // std::min(1, c)
+  return (R(-1) < c) ? c : R(-1); // std::max(-1, c)
+}
+
 // final state of one mechanism: ua/ub are the two final-state uniforms
-// (phi / cos-theta for isotropic kinds, exponent / phi for Froehlich)
+// (phi / cos-theta for isotropic kinds, exponent / phi for the Froehlich
+// kinds)
 template <typename R, bool HOT = false>
 __device__ __forceinline__ void final_state(const InstDev<R> *P, const MechDev<R> &m, R &kx,
                                             R &ky, R &kz, R &E, int &v, int &sub, R ua, R ub,
@@ -278,15 +298,25 @@ __device__ __forceinline__ void final_state(const InstDev<R> *P, const MechDev<R
     // (emcHotPhononFroehlichMechanism.hpp:94-117 / :175-199) add the record
     // of |k' - k| in their bath
     const R kx0 = kx, ky0 = ky, kz0 = kz;
-    const R ei = E;
-    E += m.shift;
-    const R ef = E;
-    const R d = Math<R>::sqrt(ei) - Math<R>::sqrt(ef);
-    const R f = Math<R>::div(Math<R>::div(R(2) * Math<R>::sqrt(ei * ef), d), d);
-    R ct = Math<R>::div(R(1) + f - Math<R>::pow(R(1) + R(2) * f, ua), f);
-    ct = (ct < R(1)) ? ct : R(1);   // std::min(1, ct)
-    ct = (R(-1) < ct) ? ct : R(-1); // std::max(-1, ct)
-    const R kn = normk_of(P->valley[v], E);
+    R ct, kn;
+    if (m.final_kind == kFinalScreenedFroehlich) {
+      // emcScreenedFroehlichInteraction.hpp:140-152 / :198-211: kI and kF
+      // are the valley's |k| before and after the energy change
+      const R ki = normk_of(P->valley[v], E);
+      E += m.shift;
+      kn = normk_of(P->valley[v], E);
+      ct = screened_cos_theta(ki, kn, P->qs2, ua);
+    } else {
+      const R ei = E;
+      E += m.shift;
+      const R ef = E;
+      const R d = Math<R>::sqrt(ei) - Math<R>::sqrt(ef);
+      const R f = Math<R>::div(Math<R>::div(R(2) * Math<R>::sqrt(ei * ef), d), d);
+      ct = Math<R>::div(R(1) + f - Math<R>::pow(R(1) + R(2) * f, ua), f);
+      ct = (ct < R(1)) ? ct : R(1);   // std::min(1, ct)
+      ct = (R(-1) < ct) ? ct : R(-1); // std::max(-1, ct)
+      kn = normk_of(P->valley[v], E);
+    }
     direction_wrt_k(kx, ky, kz, ct, ub);
     const R kc = Math<R>::sqrt(kx * kx + ky * ky + kz * kz);
     if (kc > R(0)) {

@@ -19,7 +19,9 @@ FP64, FP32 = 0, 1
 PRODUCTION, REPLAY = 0, 1
 PARA_ISO, NONPARA_ISO, PARA_ANISO, NONPARA_ANISO = 0, 1, 2, 3
 (ACOUSTIC, ZERO_IV_AB, ZERO_IV_EM, FIRST_IV_AB, FIRST_IV_EM, FROEHLICH_AB,
- FROEHLICH_EM, HOT_FROEHLICH_AB, HOT_FROEHLICH_EM) = range(9)
+ FROEHLICH_EM, HOT_FROEHLICH_AB, HOT_FROEHLICH_EM, SCREENED_FROEHLICH_AB,
+ SCREENED_FROEHLICH_EM) = range(11)
+SCREENING_FIXED = 0
 
 PM_NGP, PM_CIC, PM_NEC = 0, 1, 2
 XMIN, XMAX, YMIN, YMAX = 0, 1, 2, 3
@@ -62,6 +64,11 @@ class BathDesc(C.Structure):
                 ("acoustic", C.c_int), ("ac_phonon_energy", C.c_double), ("tau_acoustic", C.c_double),
                 ("w_ridley", C.c_double), ("to_phonon_energy", C.c_double), ("tau_to", C.c_double),
                 ("tau_profile", C.c_void_p)]
+
+
+class ScreeningDesc(C.Structure):
+    _fields_ = [("mode", C.c_int), ("enabled", C.c_int), ("eps_static", C.c_double),
+                ("qs2", C.c_double)]
 
 
 class Material(C.Structure):
@@ -141,7 +148,7 @@ EXPORTS = [
     "emcgpu_efield", "emcgpu_deposit", "emcgpu_interpolate", "emcgpu_contacts",
     "emcgpu_device_set_particles", "emcgpu_device_get_particles", "emcgpu_device_set_efield",
     "emcgpu_device_drift", "emcgpu_device_init", "emcgpu_device_step", "emcgpu_device_fields",
-    "emcgpu_autotune", "emcgpu_device_averages",
+    "emcgpu_autotune", "emcgpu_device_averages", "emcgpu_set_screening", "emcgpu_get_screening",
 ]
 
 
@@ -175,6 +182,8 @@ def load(path=LIB_PATH):
         "emcgpu_synchronize": (i, [vp]),
         "emcgpu_init_particles_energy": (i, [vp, i, C.c_uint64, d, d]),
         "emcgpu_set_baths": (i, [vp, i, p(BathDesc), i]),
+        "emcgpu_set_screening": (i, [vp, i, p(ScreeningDesc)]),
+        "emcgpu_get_screening": (i, [vp, i, p(C.c_double)]),
         "emcgpu_get_bath": (i, [vp, i, i, vp, vp]),
         "emcgpu_set_bath_state": (i, [vp, i, i, vp, d, d]),
         "emcgpu_get_bath_tallies": (i, [vp, i, i, vp, vp]),
@@ -369,12 +378,24 @@ class Context:
         self._chk(self.L.emcgpu_get_bath_history(self.h, instance, bath, _ptr(out), C.byref(n)))
         return out
 
+    def set_screening(self, instance, qs2, enabled=True, eps_static=1.0, mode=SCREENING_FIXED):
+        d = ScreeningDesc(mode, 1 if enabled else 0, eps_static, qs2)
+        self._chk(self.L.emcgpu_set_screening(self.h, instance, C.byref(d)))
+
+    def get_screening(self, instance=0):
+        v = C.c_double()
+        self._chk(self.L.emcgpu_get_screening(self.h, instance, C.byref(v)))
+        return v.value
+
     def setup(self, instance, spec, T=None):
-        """set_model (+ set_baths) + build_tables + set_field from a model dictionary."""
+        """set_model (+ set_baths, set_screening) + build_tables + set_field from a
+        model dictionary."""
         T = spec["T"] if T is None else T
         self.set_model(instance, spec)
         if spec.get("baths"):
             self.set_baths(instance, spec["baths"], T)
+        if spec.get("screening"):
+            self.set_screening(instance, spec["screening"]["qs2"])
         self.build_tables(instance, spec["n_levels"], spec["max_energy"], T,
                           rho=spec.get("rho", 2329.0), v_sound=spec.get("v_sound", 9040.0))
         self.set_field(instance, spec["field_dir"], spec["field_kvcm"] * 1e5)

@@ -145,6 +145,30 @@ inline double froehlichRate(double C, double nq, double effMass, const ValleyMas
   return absorption ? C * nq / kI * ln : C * (nq + 1.0) / kI * ln;
 }
 
+// screenedFroehlichLogFactor (emcScreenedFroehlichInteraction.hpp:52-62):
+// 1/2 ln((q+^2 + qs2) / (q-^2 + qs2)), q+- = kI +- kF; qs2 <= 0 is the bare
+// ln(q+ / q-), 0 where q- vanishes
+inline double screenedLogFactor(double kI, double kF, double qs2) {
+  const double p2 = (kI + kF) * (kI + kF), m2 = (kI - kF) * (kI - kF);
+  if (qs2 <= 0.0)
+    return m2 <= 0.0 ? 0.0 : 0.5 * std::log(p2 / m2);
+  return 0.5 * std::log((p2 + qs2) / (m2 + qs2));
+}
+// screened Froehlich rates (emcScreenedFroehlichInteraction.hpp:128-138,
+// :184-196): the Froehlich rate with the screened log factor
+inline double screenedFroehlichRate(double C, double nq, double effMass, const ValleyMass &v,
+                                    double E, double hw, double qs2, bool absorption) {
+  if (!absorption && E <= hw)
+    return 0.0;
+  const double gammaI = v.gamma(E);
+  const double gammaF = v.gamma(absorption ? E + hw : E - hw);
+  if (gammaI <= 0.0 || gammaF <= 0.0)
+    return 0.0;
+  const double kI = std::sqrt(2.0 * effMass * gammaI * q) / hbar;
+  const double kF = std::sqrt(2.0 * effMass * gammaF * q) / hbar;
+  return (absorption ? C * nq : C * (nq + 1.0)) / kI * screenedLogFactor(kI, kF, qs2);
+}
+
 }  // namespace phys
 }  // namespace emcgpu
 
