@@ -68,6 +68,14 @@ typedef enum {
  *                 the Froehlich rate with the vertex 1/q^2 screened to
  *                 1/(q^2 + q_s^2) and the angle drawn from the screened
  *                 distribution; q_s^2 from emcgpu_set_screening
+ *   SCREENED_HOT_FROEHLICH_* emcScreenedFroehlichInteraction.hpp:87-97,
+ *                 218-375: the screened rate with the occupation of a phonon
+ *                 bath, either its mean (getMeanNq) or, with q_resolved, its
+ *                 coupling-weighted mean over the reachable window
+ *                 [|kI - kF|, kI + kF] (getNqInWindow, emcPhononBath.hpp:
+ *                 378-394); the angle screened, or with q_resolved and
+ *                 q_resolved_angle drawn through the bath's q distribution
+ *                 (sampleQ, :423-458); |k' - k| recorded in the bath
  *   custom        rate supplied by the caller (emcgpu_set_rate_tables) with
  *                 one of the built-in final states */
 typedef enum {
@@ -81,7 +89,9 @@ typedef enum {
   EMCGPU_MECH_HOT_FROEHLICH_AB = 7,
   EMCGPU_MECH_HOT_FROEHLICH_EM = 8,
   EMCGPU_MECH_SCREENED_FROEHLICH_AB = 9,
-  EMCGPU_MECH_SCREENED_FROEHLICH_EM = 10
+  EMCGPU_MECH_SCREENED_FROEHLICH_EM = 10,
+  EMCGPU_MECH_SCREENED_HOT_FROEHLICH_AB = 11,
+  EMCGPU_MECH_SCREENED_HOT_FROEHLICH_EM = 12
 } emcgpu_mech_kind;
 
 typedef struct {
@@ -116,7 +126,12 @@ typedef struct {
   double eps_hi, eps_lo; /* Froehlich: relative permittivities */
   int z;              /* IV: number of final subvalleys per initial one */
   int final_sub[EMCGPU_MAX_DEG][EMCGPU_MAX_Z]; /* IV: finalSubValleys map */
-  int bath;           /* HOT_FROEHLICH_*: index of the instance's phonon bath */
+  int bath;           /* HOT_FROEHLICH_*, SCREENED_HOT_FROEHLICH_*: index of the
+                         instance's phonon bath */
+  int q_resolved;     /* SCREENED_HOT_FROEHLICH_*: window occupation in the rate */
+  int q_resolved_angle; /* SCREENED_HOT_FROEHLICH_* with q_resolved: angle from
+                           the bath's q distribution (the reference's default,
+                           inQResolvedAngle = true) */
 } emcgpu_mech_desc;
 
 /* emcPhononBath constructor + setTauLOProfile (include/emcPhononBath.hpp:
@@ -146,15 +161,26 @@ typedef struct {
  * that an instance's SCREENED_FROEHLICH_* mechanisms share.
  *   FIXED   q_s^2 as given (setQs2).  Like the reference's setQs2 it acts on
  *           the final-state angle at once; the rates follow at the next
- *           emcgpu_build_tables (reinitScatterTables). */
-typedef enum { EMCGPU_SCREENING_FIXED = 0 } emcgpu_screening_mode;
+ *           emcgpu_build_tables (reinitScatterTables).
+ *   PLASMA  refreshed on the GPU after every step from the instance's
+ *           carriers, q_s^2 = n q^2 / (eps_static eps0 kB T) with
+ *           n = N / volume and T = 2 <E>_valley0 q / (3 kB) (getMBTemp), 0
+ *           without carriers; then every bath's window weights and the
+ *           screened tables are rebuilt (the updateScreening and
+ *           reinitScatterTables of examples/hotCarrierMHP/hotCarrierMHP.cpp:
+ *           563-581, 693-703).  emcgpu_update_screening seeds it before the
+ *           first step (:630-637).
+ * Every bath of the instance weights its window sums with the same q_s^2
+ * (setScreeningQ2). */
+typedef enum { EMCGPU_SCREENING_FIXED = 0, EMCGPU_SCREENING_PLASMA = 1 } emcgpu_screening_mode;
 
 typedef struct {
   int mode;          /* emcgpu_screening_mode */
   int enabled;       /* 0: q_s^2 stays 0 (emcPlasmonScreening(eps, false)),
                         the screened kinds reduce to the bare Froehlich ones */
   double eps_static; /* static relative permittivity (Debye form) */
-  double qs2;        /* q_s^2 [1/m^2], >= 0 */
+  double qs2;        /* q_s^2 [1/m^2], >= 0 (PLASMA: until the first update) */
+  double volume;     /* PLASMA: volume of the carriers [m^3] */
 } emcgpu_screening_desc;
 
 typedef struct {
@@ -268,6 +294,10 @@ emcgpu_status emcgpu_init_particles_energy(emcgpu_ctx *ctx, int instance,
 emcgpu_status emcgpu_set_screening(emcgpu_ctx *ctx, int instance,
                                    const emcgpu_screening_desc *desc);
 emcgpu_status emcgpu_get_screening(const emcgpu_ctx *ctx, int instance, double *qs2);
+/* PLASMA: q_s^2 from the instance's current particles (sum of valley-0
+ * energies in particle order, like getAvgEnergy) and the screened tables
+ * rebuilt: the seeding before the first step */
+emcgpu_status emcgpu_update_screening(emcgpu_ctx *ctx, int instance);
 
 /* ---- phonon baths --------------------------------------------------------
  * Baths of an instance (replacing the shared_ptr<emcPhononBath> given to

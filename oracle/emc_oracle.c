@@ -524,6 +524,8 @@ void orc_bath_init(orc_bath *b, int nBins, double dq, double tauLO,
   b->Nq = (double *)calloc((size_t)nBins, sizeof(double));
   b->nEm = (double *)calloc((size_t)nBins, sizeof(double));
   b->nAbs = (double *)calloc((size_t)nBins, sizeof(double));
+  b->cumW = (double *)calloc((size_t)nBins + 1, sizeof(double));
+  b->cumWN = (double *)calloc((size_t)nBins + 1, sizeof(double));
   b->loE = phononEnergy;
   b->latT = latticeTemp;
   b->N0 = orc_planck(phononEnergy, latticeTemp);
@@ -544,9 +546,103 @@ void orc_bath_init(orc_bath *b, int nBins, double dq, double tauLO,
     b->NtoEq = orc_planck(toPhononEnergy, latticeTemp);
     b->Nto = b->NtoEq;
   }
+  orc_bath_rebuild_window_sums(b);
+}
+
+/* rebuildWindowSums (:109-124): weight q/(q^2 + qs2Screen) at the bin
+ * centre, prefix sums in bin order */
+void orc_bath_rebuild_window_sums(orc_bath *b) {
+  b->cumW[0] = 0.0;
+  b->cumWN[0] = 0.0;
+  for (int i = 0; i < b->nBins; i++) {
+    const double q = ((double)(size_t)i + 0.5) * b->dq;
+    const double denom = q * q + b->qs2Screen;
+    const double w = (denom > 0.0) ? q / denom : 0.0;
+    b->cumW[i + 1] = b->cumW[i] + w;
+    b->cumWN[i + 1] = b->cumWN[i] + w * b->Nq[i];
+  }
+}
+
+/* setScreeningQ2 (:203-208): rebuild only on a change */
+void orc_bath_set_screening_q2(orc_bath *b, double qs2) {
+  if (qs2 == b->qs2Screen)
+    return;
+  b->qs2Screen = qs2;
+  orc_bath_rebuild_window_sums(b);
+}
+
+/* bin range [lo, hi) covering [qMin, qMax] (:381-389 / :426-434); returns
+ * 0 when it is empty */
+static int orc_bath_window(const orc_bath *b, double qMin, double qMax, long *lo, long *hi) {
+  long l = (long)floor(qMin / b->dq);
+  long h = (long)ceil(qMax / b->dq);
+  if (l < 0)
+    l = 0;
+  if (h > (long)b->nBins)
+    h = (long)b->nBins;
+  *lo = l;
+  *hi = h;
+  return h - l >= 1;
+}
+
+/* getNqInWindow (:378-394) */
+double orc_bath_nq_in_window(const orc_bath *b, double qMin, double qMax) {
+  long lo, hi;
+  if (b->nBins < 2 || qMax <= qMin)
+    return orc_bath_mean_nq(b);
+  if (!orc_bath_window(b, qMin, qMax, &lo, &hi))
+    return orc_bath_mean_nq(b);
+  const double wSum = b->cumW[hi] - b->cumW[lo];
+  if (wSum <= 0.0)
+    return orc_bath_mean_nq(b);
+  return (b->cumWN[hi] - b->cumWN[lo]) / wSum;
+}
+
+static double orc_bath_s(const orc_bath *b, long i, int emission) {
+  return emission ? (b->cumWN[i] + b->cumW[i]) : b->cumWN[i];
+}
+
+/* sampleQ (:423-458): inverse of the piecewise-linear cumulative sum of
+ * f(q) q/(q^2 + qs2) over the window, f = N (absorption) or N + 1 */
+double orc_bath_sample_q(const orc_bath *b, double qMin, double qMax, int emission, double r) {
+  long lo, hi;
+  if (b->nBins < 2 || qMax <= qMin)
+    return qMin;
+  if (!orc_bath_window(b, qMin, qMax, &lo, &hi))
+    return qMin;
+  const double sLo = orc_bath_s(b, lo, emission), sHi = orc_bath_s(b, hi, emission);
+  const double span = sHi - sLo;
+  if (!(span > 0.0))
+    return 0.5 * (qMin + qMax);
+  const double target = sLo + r * span;
+  long a = lo, c = hi;
+  while (c - a > 1) {
+    const long mid = (a + c) / 2;
+    if (orc_bath_s(b, mid, emission) <= target)
+      a = mid;
+    else
+      c = mid;
+  }
+  const double sA = orc_bath_s(b, a, emission), sB = orc_bath_s(b, a + 1, emission);
+  const double frac = (sB > sA) ? (target - sA) / (sB - sA) : 0.5;
+  const double q = ((double)a + frac) * b->dq;
+  return (qMin < ((q < qMax) ? q : qMax)) ? ((q < qMax) ? q : qMax) : qMin;
+}
+
+/* sampleQResolvedCosTheta (emcScreenedFroehlichInteraction.hpp:87-97) */
+double orc_qres_cos_theta(const orc_bath *b, double kI, double kF, int emission, double r) {
+  const double B = 2.0 * kI * kF;
+  if (B <= 0.0)
+    return 1.0 - 2.0 * r;
+  const double q = orc_bath_sample_q(b, fabs(kI - kF), kI + kF, emission, r);
+  const double cosTheta = (kI * kI + kF * kF - q * q) / B;
+  const double c1 = (cosTheta < 1.0) ? cosTheta : 1.0;
+  return (-1.0 < c1) ? c1 : -1.0;
 }
 
 void orc_bath_free(orc_bath *b) {
+  free(b->cumW);
+  free(b->cumWN);
   free(b->Nq);
   free(b->nEm);
   free(b->nAbs);
@@ -636,6 +732,7 @@ void orc_bath_update(orc_bath *b, double dt) {
       }
     }
   }
+  orc_bath_rebuild_window_sums(b);
 }
 
 /* getMeanNq (:461-470) */
@@ -773,6 +870,23 @@ double orc_mech_rate(const orc_model *md, orc_mech *m, double energy) {
     double Nq = orc_bath_mean_nq(&md->bath[m->bath]);
     return m->scatterConst * Nq / kI * ln;
   }
+  case ORC_MECH_SCR_HOT_FROEHLICH_AB:
+  case ORC_MECH_SCR_HOT_FROEHLICH_EM: { /* emcScreenedFroehlichInteraction.hpp:257-269 / :333-348 */
+    const int ab = m->kind == ORC_MECH_SCR_HOT_FROEHLICH_AB;
+    if (!ab && energy <= m->phononEnergy)
+      return 0.0;
+    double gammaI = orc_valley_gamma(iv, energy);
+    double gammaF = orc_valley_gamma(iv, ab ? energy + m->phononEnergy : energy - m->phononEnergy);
+    if (gammaI <= 0.0 || gammaF <= 0.0)
+      return 0.0;
+    double kI = sqrt(2.0 * m->effMass * gammaI * ORC_Q) / ORC_HBAR;
+    double kF = sqrt(2.0 * m->effMass * gammaF * ORC_Q) / ORC_HBAR;
+    const orc_bath *bt = &md->bath[m->bath];
+    const double Nq = m->qResolved ? orc_bath_nq_in_window(bt, fabs(kI - kF), kI + kF)
+                                   : orc_bath_mean_nq(bt);
+    return (ab ? m->scatterConst * Nq : m->scatterConst * (Nq + 1.0)) / kI *
+           orc_scr_log_factor(kI, kF, md->qs2);
+  }
   case ORC_MECH_SCR_FROEHLICH_AB:
   case ORC_MECH_SCR_FROEHLICH_EM: { /* emcScreenedFroehlichInteraction.hpp:128-138 / :184-196 */
     const int ab = m->kind == ORC_MECH_SCR_FROEHLICH_AB;
@@ -813,11 +927,12 @@ static int orc_is_froehlich_ab(int kind) {
  * (emcHotPhononFroehlichMechanism.hpp:113-116, :195-198) */
 static void orc_hot_record(const orc_model *md, const orc_mech *m,
                            const double kOld[3], const double kNew[3]) {
-  if (m->kind != ORC_MECH_HOT_FROEHLICH_AB && m->kind != ORC_MECH_HOT_FROEHLICH_EM)
+  if (m->kind != ORC_MECH_HOT_FROEHLICH_AB && m->kind != ORC_MECH_HOT_FROEHLICH_EM &&
+      m->kind != ORC_MECH_SCR_HOT_FROEHLICH_AB && m->kind != ORC_MECH_SCR_HOT_FROEHLICH_EM)
     return;
   double qv[3] = {kNew[0] - kOld[0], kNew[1] - kOld[1], kNew[2] - kOld[2]};
   orc_bath_record(&md->bath[m->bath], orc_norm3(qv),
-                  m->kind == ORC_MECH_HOT_FROEHLICH_EM);
+                  m->kind == ORC_MECH_HOT_FROEHLICH_EM || m->kind == ORC_MECH_SCR_HOT_FROEHLICH_EM);
 }
 
 /* std::min / std::max semantics used by the Froehlich clamp */
@@ -828,16 +943,26 @@ static double orc_stdmax(double a, double b) { return (a < b) ? b : a; }
  * (emcScreenedFroehlichInteraction.hpp:140-152 / :198-211): kI and kF from
  * the valley's |k|(E) before and after the energy change, the screened
  * angle from uniform ua, the azimuth from ub, |k| rescaled to kF */
+static void orc_hot_record(const orc_model *md, const orc_mech *m,
+                           const double kOld[3], const double kNew[3]);
+
 static void orc_scr_final(const orc_model *md, const orc_mech *m, orc_particle *p,
                           double ua, double ub) {
   const orc_valley *v = &md->valley[m->valley];
+  const int hot = m->kind == ORC_MECH_SCR_HOT_FROEHLICH_AB || m->kind == ORC_MECH_SCR_HOT_FROEHLICH_EM;
+  const int ab = m->kind == ORC_MECH_SCR_FROEHLICH_AB || m->kind == ORC_MECH_SCR_HOT_FROEHLICH_AB;
+  double kOld[3] = {p->k[0], p->k[1], p->k[2]};
   double kI = orc_valley_normk(v, p->energy);
-  if (m->kind == ORC_MECH_SCR_FROEHLICH_AB)
+  if (ab)
     p->energy += m->phononEnergy;
   else
     p->energy -= m->phononEnergy;
   double kF = orc_valley_normk(v, p->energy);
-  double cosTheta = orc_scr_cos_theta(kI, kF, md->qs2, ua);
+  /* the hot kinds (:283-303 / :358-375): q-resolved angle when both
+   * switches are on, else the screened one; then the bath record */
+  double cosTheta = (hot && m->qResolved && m->qResolvedAngle)
+                        ? orc_qres_cos_theta(&md->bath[m->bath], kI, kF, !ab, ua)
+                        : orc_scr_cos_theta(kI, kF, md->qs2, ua);
   double nk[3];
   orc_init_random_direction_wrt_k(p->k, cosTheta, ub, nk);
   memcpy(p->k, nk, sizeof nk);
@@ -847,6 +972,8 @@ static void orc_scr_final(const orc_model *md, const orc_mech *m, orc_particle *
     for (int i = 0; i < 3; i++)
       p->k[i] = p->k[i] * s;
   }
+  if (hot)
+    orc_hot_record(md, m, kOld, p->k);
 }
 
 /* scatterParticle of each mechanism kind.  Two uniforms fed to
@@ -856,7 +983,9 @@ void orc_mech_scatter(const orc_model *md, const orc_mech *m,
                       orc_particle *p, orc_rng *rng) {
   switch (m->kind) {
   case ORC_MECH_SCR_FROEHLICH_AB:
-  case ORC_MECH_SCR_FROEHLICH_EM: {
+  case ORC_MECH_SCR_FROEHLICH_EM:
+  case ORC_MECH_SCR_HOT_FROEHLICH_AB:
+  case ORC_MECH_SCR_HOT_FROEHLICH_EM: {
     /* the angle's uniform is drawn before the azimuth's (separate
      * statements) */
     double ua = orc_rng_uniform(rng, 0., 1.);
@@ -954,7 +1083,9 @@ void orc_model_free(orc_model *m) {
 int orc_model_has_hot(const orc_model *m) {
   for (int j = 0; j < m->nMech; j++)
     if (m->mech[j].kind == ORC_MECH_HOT_FROEHLICH_AB ||
-        m->mech[j].kind == ORC_MECH_HOT_FROEHLICH_EM)
+        m->mech[j].kind == ORC_MECH_HOT_FROEHLICH_EM ||
+        m->mech[j].kind == ORC_MECH_SCR_HOT_FROEHLICH_AB ||
+        m->mech[j].kind == ORC_MECH_SCR_HOT_FROEHLICH_EM)
       return 1;
   return 0;
 }
@@ -966,12 +1097,63 @@ void orc_model_bath_step(orc_model *m, double dt, int rebuild) {
     orc_model_build_tables(m);
 }
 
-void orc_model_end_step(orc_model *m, double dt) {
+void orc_model_end_step(orc_model *m, double dt, long long nPart, double sumE0,
+                        long long cnt0) {
+  if (m->scrPlasma) {
+    /* hotCarrierMHP.cpp:693-703: every bath's update, updateScreening,
+     * then reinitScatterTables every step */
+    m->stepCount++;
+    if (m->bathUpdate)
+      for (int b = 0; b < m->nBaths; b++)
+        orc_bath_update(&m->bath[b], dt);
+    orc_model_update_screening(m, nPart, sumE0, cnt0);
+    orc_model_build_tables(m);
+    return;
+  }
   if (m->nBaths == 0 || !m->bathUpdate)
     return;
   m->stepCount++;
   int every = m->rebuildEvery > 1 ? m->rebuildEvery : 1;
   orc_model_bath_step(m, dt, m->stepCount % every == 0);
+}
+
+void orc_model_update_screening(orc_model *m, long long nPart, double sumE0, long long cnt0) {
+  double qs2 = 0.0;
+  if (nPart != 0) {
+    const double nS = (double)nPart / m->scrVol;
+    const double avgE = (cnt0 != 0) ? sumE0 / (double)(size_t)cnt0 : 0.0;
+    const double tS = 2.0 * avgE * ORC_Q / (3.0 * ORC_KB);
+    if (tS > 0.0)
+      qs2 += nS * ORC_Q * ORC_Q / (m->scrEps * ORC_EPS0 * ORC_KB * tS);
+  }
+  m->qs2 = qs2;
+  for (int b = 0; b < m->nBaths; b++)
+    orc_bath_set_screening_q2(&m->bath[b], qs2);
+}
+
+void orc_valley0_sums(int np, const double *E, const int32_t *valley, double *sumE0,
+                      long long *cnt0) {
+  double s = 0.0;
+  long long c = 0;
+  for (int i = 0; i < np; i++)
+    if (valley[i] == 0) {
+      s += E[i];
+      c++;
+    }
+  *sumE0 = s;
+  *cnt0 = c;
+}
+
+void orc_valley0_sums_aos(int np, const orc_particle *p, double *sumE0, long long *cnt0) {
+  double s = 0.0;
+  long long c = 0;
+  for (int i = 0; i < np; i++)
+    if (p[i].valley == 0) {
+      s += p[i].energy;
+      c++;
+    }
+  *sumE0 = s;
+  *cnt0 = c;
 }
 
 int orc_model_add_valley(orc_model *m, const orc_valley *v) {
@@ -1163,7 +1345,12 @@ int orc_run_replay(orc_model *m, int np, double *k, double *E,
       valley[ip] = p.valley;
       sub[ip] = p.subValley;
     }
-    orc_model_end_step(m, tStep);
+    {
+      double se0;
+      long long c0;
+      orc_valley0_sums(np, E, valley, &se0, &c0);
+      orc_model_end_step(m, tStep, np, se0, c0);
+    }
   }
   int nOverflow = 0;
   for (int ip = 0; ip < np; ip++)
@@ -1331,13 +1518,27 @@ int orc_run_shared(orc_model *m, uint64_t seed, const double maxPos[3],
     return -needed;
   }
   int nv = m->nValleys;
+  if (m->scrPlasma) {
+    /* the screening seeded from the initial carriers (hotCarrierMHP.cpp:
+     * 630-637) */
+    double se0;
+    long long c0;
+    orc_valley0_sums_aos(np, parts, &se0, &c0);
+    orc_model_update_screening(m, np, se0, c0);
+    orc_model_build_tables(m);
+  }
   orc_observables(m, np, parts, dir, obs, obs + nv, obs + 2 * nv);
   for (int s = 1; s <= nsteps; s++) {
     for (int i = 0; i < np; i++)
       orc_move_particle(m, &parts[i], tStep, force, &rng, NULL, NULL);
     double *o = obs + (size_t)s * 3 * nv;
     orc_observables(m, np, parts, dir, o, o + nv, o + 2 * nv);
-    orc_model_end_step(m, tStep);
+    {
+      double se0;
+      long long c0;
+      orc_valley0_sums_aos(np, parts, &se0, &c0);
+      orc_model_end_step(m, tStep, np, se0, c0);
+    }
     if (bathMeanOut)
       for (int b = 0; b < m->nBaths; b++)
         bathMeanOut[(size_t)(s - 1) * m->nBaths + b] = orc_bath_mean_nq(&m->bath[b]);
@@ -1464,6 +1665,8 @@ static void orc_mech_scatter_u(const orc_model *md, const orc_mech *m,
   }
   case ORC_MECH_SCR_FROEHLICH_AB:
   case ORC_MECH_SCR_FROEHLICH_EM:
+  case ORC_MECH_SCR_HOT_FROEHLICH_AB:
+  case ORC_MECH_SCR_HOT_FROEHLICH_EM:
     orc_scr_final(md, m, p, ua, ub);
     return;
   case ORC_MECH_FROEHLICH_AB:
@@ -1650,7 +1853,44 @@ int orc_model_add_screened_froehlich(orc_model *m, int kind, int valley,
   return orc_model_add_mech(m, &mech);
 }
 
-void orc_model_set_qs2(orc_model *m, double qs2) { m->qs2 = qs2; }
+void orc_model_set_qs2(orc_model *m, double qs2) {
+  m->qs2 = qs2;
+  for (int b = 0; b < m->nBaths; b++)
+    orc_bath_set_screening_q2(&m->bath[b], qs2);
+}
+
+int orc_model_add_screened_hot_froehlich(orc_model *m, int kind, int valley,
+                                         double phononEnergy, double relEffMass,
+                                         double epsHi, double epsLo, int bath,
+                                         int qResolved, int qResolvedAngle) {
+  orc_mech mech;
+  orc_mech_hot_froehlich(&mech, ORC_MECH_HOT_FROEHLICH_AB, valley, phononEnergy, relEffMass,
+                         epsHi, epsLo, bath);
+  mech.kind = kind;
+  mech.qResolved = qResolved;
+  mech.qResolvedAngle = qResolvedAngle;
+  return orc_model_add_mech(m, &mech);
+}
+
+void orc_model_set_screening_plasma(orc_model *m, double eps, double volume) {
+  m->scrPlasma = eps > 0.0;
+  m->scrEps = eps;
+  m->scrVol = volume;
+}
+
+void orc_model_seed_screening(orc_model *m, int np, const double *E, const int32_t *valley) {
+  double se0;
+  long long c0;
+  orc_valley0_sums(np, E, valley, &se0, &c0);
+  orc_model_update_screening(m, np, se0, c0);
+  orc_model_build_tables(m);
+}
+
+void orc_model_get_bath_window(const orc_model *m, int b, double *cumW, double *cumWN) {
+  const orc_bath *bt = &m->bath[b];
+  memcpy(cumW, bt->cumW, sizeof(double) * (size_t)(bt->nBins + 1));
+  memcpy(cumWN, bt->cumWN, sizeof(double) * (size_t)(bt->nBins + 1));
+}
 
 int orc_model_n_tables(const orc_model *m, int valley) { return m->nTables[valley]; }
 int orc_model_table_mech(const orc_model *m, int valley, int j) {
@@ -1801,7 +2041,12 @@ void orc_run_philox(orc_model *m, int np, double *k, double *E,
     }
     if (obs)
       orc_obs_soa(m, np, k, E, valley, sub, fieldDir, obs + (size_t)s * 3 * nv);
-    orc_model_end_step(m, tStep);
+    {
+      double se0;
+      long long c0;
+      orc_valley0_sums(np, E, valley, &se0, &c0);
+      orc_model_end_step(m, tStep, np, se0, c0);
+    }
   }
 }
 
@@ -1893,6 +2138,7 @@ void orc_model_set_bath(orc_model *m, int b, const double *Nq, double Nac,
   memcpy(bt->Nq, Nq, sizeof(double) * (size_t)bt->nBins);
   bt->Nac = Nac;
   bt->Nto = Nto;
+  orc_bath_rebuild_window_sums(bt);
 }
 
 void orc_model_bath_record_one(orc_model *m, int b, double q, int emission) {
@@ -1957,7 +2203,7 @@ void orc_run_mt(orc_model *m, int np, uint64_t seed, double Vt,
     nThreads = 1;
   /* observables need every step: run the threads one step at a time when
    * obs is requested, else let each thread run all its steps */
-  int chunkSteps = (obs || m->nBaths) ? 1 : nsteps;
+  int chunkSteps = (obs || m->nBaths || m->scrPlasma) ? 1 : nsteps;
   /* bath tallies: one private copy of the model per thread whose tally
    * pointers are thread-local, summed in thread order after each step
    * (integer-valued doubles: the sum is exact in any order) */
@@ -2006,7 +2252,12 @@ void orc_run_mt(orc_model *m, int np, uint64_t seed, double Vt,
           free(locals[t].bath[b].nEm);
           free(locals[t].bath[b].nAbs);
         }
-      orc_model_end_step(m, tStep);
+    }
+    if (locals || m->scrPlasma) {
+      double se0;
+      long long c0;
+      orc_valley0_sums_aos(np, parts, &se0, &c0);
+      orc_model_end_step(m, tStep, np, se0, c0);
     }
   }
   free(locals);
@@ -2018,4 +2269,18 @@ void orc_run_mt(orc_model *m, int np, uint64_t seed, double Vt,
   }
   free(parts);
   free(rngs);
+}
+
+/* single-call wrappers over a model's bath and screening state */
+double orc_model_get_qs2(const orc_model *m) { return m->qs2; }
+double orc_bath_nq_window_one(const orc_model *m, int b, double qMin, double qMax) {
+  return orc_bath_nq_in_window(&m->bath[b], qMin, qMax);
+}
+double orc_bath_sample_q_one(const orc_model *m, int b, double qMin, double qMax, int emission,
+                             double r) {
+  return orc_bath_sample_q(&m->bath[b], qMin, qMax, emission, r);
+}
+double orc_qres_cos_theta_one(const orc_model *m, int b, double kI, double kF, int emission,
+                              double r) {
+  return orc_qres_cos_theta(&m->bath[b], kI, kF, emission, r);
 }

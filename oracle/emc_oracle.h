@@ -154,7 +154,9 @@ enum {
    * rate and angle with the vertex 1/q^2 screened to 1/(q^2 + q_s^2), q_s^2
    * the model's (a shared emcPlasmonScreening, emcPlasmonScreening.hpp) */
   ORC_MECH_SCR_FROEHLICH_AB = 9,
-  ORC_MECH_SCR_FROEHLICH_EM = 10
+  ORC_MECH_SCR_FROEHLICH_EM = 10,
+  ORC_MECH_SCR_HOT_FROEHLICH_AB = 11,
+  ORC_MECH_SCR_HOT_FROEHLICH_EM = 12
 };
 
 typedef struct {
@@ -169,6 +171,9 @@ typedef struct {
   double effMass; /* Froehlich: relEffMass * me */
   double nBose;   /* Froehlich */
   int bath;       /* hot-phonon Froehlich: index of the model's phonon bath */
+  /* screened hot-phonon Froehlich: q-resolved occupation in the rate and
+   * q-resolved angle (emcScreenedFroehlichInteraction.hpp:225-231) */
+  int qResolved, qResolvedAngle;
 } orc_mech;
 
 /* ---- LO phonon bath: include/emcPhononBath.hpp ------------------------
@@ -186,6 +191,10 @@ typedef struct {
   double acE, tauAc, Nac, NacEq;
   int hasRidley;
   double wR, toE, tauTO, Nto, NtoEq;
+  /* prefix sums of the transition weight q/(q^2 + qs2Screen) and of
+   * weight * N_q, nBins + 1 entries each (:73-82, 109-124) */
+  double qs2Screen;
+  double *cumW, *cumWN;
 } orc_bath;
 
 /* constructor (:160-192) */
@@ -203,6 +212,14 @@ void orc_bath_record(const orc_bath *b, double q, int emission);
 void orc_bath_update(orc_bath *b, double dt);
 /* getMeanNq (:461-470) */
 double orc_bath_mean_nq(const orc_bath *b);
+/* rebuildWindowSums (:109-124), setScreeningQ2 (:203-208) */
+void orc_bath_rebuild_window_sums(orc_bath *b);
+void orc_bath_set_screening_q2(orc_bath *b, double qs2);
+/* getNqInWindow (:378-394), sampleQ (:423-458) */
+double orc_bath_nq_in_window(const orc_bath *b, double qMin, double qMax);
+double orc_bath_sample_q(const orc_bath *b, double qMin, double qMax, int emission, double r);
+/* sampleQResolvedCosTheta (emcScreenedFroehlichInteraction.hpp:87-97) */
+double orc_qres_cos_theta(const orc_bath *b, double kI, double kF, int emission, double r);
 /* getAcousticTemp / getTOTemp (:480-494) */
 double orc_bath_acoustic_temp(const orc_bath *b);
 double orc_bath_to_temp(const orc_bath *b);
@@ -249,6 +266,11 @@ typedef struct {
   double initEnergy;
   /* q_s^2 [1/m^2] of the screened mechanisms (emcPlasmonScreening::getQs2) */
   double qs2;
+  /* plasma screening: q_s^2 refreshed from the carriers after every step
+   * (updateScreening of examples/hotCarrierMHP/hotCarrierMHP.cpp:563-581)
+   * with static permittivity scrEps in the volume scrVol */
+  int scrPlasma;
+  double scrEps, scrVol;
 } orc_model;
 
 void orc_model_init(orc_model *m, int nLevels, double maxEnergy);
@@ -274,8 +296,22 @@ int orc_model_has_hot(const orc_model *m);
  * update(dt), then reinitScatterTables (emcScatterHandler.hpp:100-108)
  * when `rebuild` is set and a mechanism reads a bath */
 void orc_model_bath_step(orc_model *m, double dt, int rebuild);
-/* the runners' end-of-step hook described at orc_model.bathUpdate */
-void orc_model_end_step(orc_model *m, double dt);
+/* the runners' end-of-step hook described at orc_model.bathUpdate; with
+ * plasma screening (scrPlasma) it refreshes q_s^2 from the carriers after
+ * the bath update and rebuilds the tables every step: nPart carriers, of
+ * which cnt0 in valley 0 with energies summing to sumE0 (particle order) */
+void orc_model_end_step(orc_model *m, double dt, long long nPart, double sumE0,
+                        long long cnt0);
+/* the hotCarrierMHP updateScreening (:563-581) for one carrier species:
+ * qs2 = n q^2 / (eps eps0 kB T), n = nPart / scrVol, T = getMBTemp of the
+ * valley-0 mean energy (emcHotCarrierOutput.hpp:89-92); 0 without carriers
+ * or temperature.  Sets the model's qs2 and every bath's setScreeningQ2. */
+void orc_model_update_screening(orc_model *m, long long nPart, double sumE0, long long cnt0);
+/* valley-0 energy sum and count in particle order (getAvgEnergy,
+ * basicBulkParticleHandler.hpp:304-321) */
+void orc_valley0_sums(int np, const double *E, const int32_t *valley, double *sumE0,
+                      long long *cnt0);
+void orc_valley0_sums_aos(int np, const orc_particle *p, double *sumE0, long long *cnt0);
 
 /* ---- bulk loop: examples/bulkSimulation/basicBulkParticleHandler.hpp --- */
 typedef struct {
@@ -378,6 +414,24 @@ int orc_model_add_screened_froehlich(orc_model *m, int kind, int valley,
 /* emcPlasmonScreening::setQs2 (enabled): the q_s^2 the screened rates and
  * angles read (rebuild the tables afterwards) */
 void orc_model_set_qs2(orc_model *m, double qs2);
+/* kind = ORC_MECH_SCR_HOT_FROEHLICH_{AB,EM} (constructors :238-250 /
+ * :313-325) bound to bath index `bath` */
+int orc_model_add_screened_hot_froehlich(orc_model *m, int kind, int valley,
+                                         double phononEnergy, double relEffMass,
+                                         double epsHi, double epsLo, int bath,
+                                         int qResolved, int qResolvedAngle);
+/* plasma screening on (eps, volume) or off (eps <= 0) */
+void orc_model_set_screening_plasma(orc_model *m, double eps, double volume);
+/* refresh q_s^2 from the SoA state and rebuild the tables: the seeding
+ * before the first step (hotCarrierMHP.cpp:630-637) */
+void orc_model_seed_screening(orc_model *m, int np, const double *E, const int32_t *valley);
+void orc_model_get_bath_window(const orc_model *m, int b, double *cumW, double *cumWN);
+double orc_model_get_qs2(const orc_model *m);
+double orc_bath_nq_window_one(const orc_model *m, int b, double qMin, double qMax);
+double orc_bath_sample_q_one(const orc_model *m, int b, double qMin, double qMax, int emission,
+                             double r);
+double orc_qres_cos_theta_one(const orc_model *m, int b, double kI, double kF, int emission,
+                              double r);
 /* screenedFroehlichLogFactor / sampleScreenedFroehlichCosTheta
  * (emcScreenedFroehlichInteraction.hpp:52-77) */
 double orc_scr_log_factor(double kI, double kF, double qs2);

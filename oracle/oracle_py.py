@@ -47,6 +47,15 @@ def load():
         "orc_model_add_froehlich": (_I, [vp, _I, _I, _D, _D, _D, _D, _D]),
         "orc_model_add_screened_froehlich": (_I, [vp, _I, _I, _D, _D, _D, _D, _D]),
         "orc_model_set_qs2": (None, [vp, _D]),
+        "orc_model_add_screened_hot_froehlich": (_I, [vp, _I, _I, _D, _D, _D, _D, _I, _I, _I]),
+        "orc_model_set_screening_plasma": (None, [vp, _D, _D]),
+        "orc_model_seed_screening": (None, [vp, _I, vp, vp]),
+        "orc_model_update_screening": (None, [vp, C.c_longlong, _D, C.c_longlong]),
+        "orc_model_get_bath_window": (None, [vp, _I, vp, vp]),
+        "orc_model_get_qs2": (_D, [vp]),
+        "orc_bath_nq_window_one": (_D, [vp, _I, _D, _D]),
+        "orc_bath_sample_q_one": (_D, [vp, _I, _D, _D, _I, _D]),
+        "orc_qres_cos_theta_one": (_D, [vp, _I, _D, _D, _I, _D]),
         "orc_scr_log_factor": (_D, [_D, _D, _D]),
         "orc_scr_cos_theta": (_D, [_D, _D, _D, _D]),
         "orc_model_build_tables": (None, [vp]),
@@ -150,8 +159,11 @@ class OracleModel:
         self.n_baths = len(spec.get("baths", []))
         if spec.get("init_energy"):
             L.orc_model_set_init_energy(self.h, spec["init_energy"])
-        if spec.get("screening"):
-            L.orc_model_set_qs2(self.h, spec["screening"]["qs2"])
+        scr = spec.get("screening")
+        if scr:
+            L.orc_model_set_qs2(self.h, scr.get("qs2", 0.0))
+            if scr.get("plasma"):
+                L.orc_model_set_screening_plasma(self.h, scr["eps"], scr["volume"])
         for v in spec["valleys"]:
             axes = None
             if "axes" in v:
@@ -174,6 +186,11 @@ class OracleModel:
             elif k in (7, 8):
                 L.orc_model_add_hot_froehlich(self.h, k, mech["valley"], mech["hw"], mech["mrel"],
                                               mech["eps_hi"], mech["eps_lo"], mech["bath"])
+            elif k in (11, 12):
+                L.orc_model_add_screened_hot_froehlich(
+                    self.h, k, mech["valley"], mech["hw"], mech["mrel"], mech["eps_hi"],
+                    mech["eps_lo"], mech["bath"], int(mech.get("q_resolved", False)),
+                    int(mech.get("q_resolved_angle", True)))
             elif k in (9, 10):
                 L.orc_model_add_screened_froehlich(self.h, k, mech["valley"], mech["hw"], mech["mrel"],
                                                    mech["eps_hi"], mech["eps_lo"], T)
@@ -329,6 +346,34 @@ class OracleModel:
 
     def bath_update(self, b, dt):
         self.L.orc_model_bath_update_one(self.h, b, dt)
+
+    def qs2(self):
+        return self.L.orc_model_get_qs2(self.h)
+
+    def set_qs2(self, qs2):
+        """setQs2 on the screening and setScreeningQ2 on every bath."""
+        self.L.orc_model_set_qs2(self.h, qs2)
+
+    def seed_screening(self, E, valley):
+        """updateScreening + reinitScatterTables before the first step."""
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        v = np.ascontiguousarray(valley, dtype=np.int32)
+        self.L.orc_model_seed_screening(self.h, E.size, _ptr(E), _ptr(v))
+
+    def bath_window(self, b=0):
+        nb = self.spec["baths"][b]["n_bins"]
+        w, wn = np.zeros(nb + 1), np.zeros(nb + 1)
+        self.L.orc_model_get_bath_window(self.h, b, _ptr(w), _ptr(wn))
+        return w, wn
+
+    def nq_in_window(self, b, qmin, qmax):
+        return self.L.orc_bath_nq_window_one(self.h, b, qmin, qmax)
+
+    def sample_q(self, b, qmin, qmax, emission, r):
+        return self.L.orc_bath_sample_q_one(self.h, b, qmin, qmax, int(emission), r)
+
+    def qres_cos_theta(self, b, kI, kF, emission, r):
+        return self.L.orc_qres_cos_theta_one(self.h, b, kI, kF, int(emission), r)
 
     def bath_step(self, dt, rebuild=True):
         self.L.orc_model_bath_step(self.h, dt, int(rebuild))

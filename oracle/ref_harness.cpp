@@ -28,6 +28,8 @@
 #include <ScatterMechanisms/emcHotPhononFroehlichMechanism.hpp>
 #include <ScatterMechanisms/emcScreenedFroehlichInteraction.hpp>
 #include <emcPlasmonScreening.hpp>
+#include <emcHotCarrierOutput.hpp>
+#include <map>
 #include <ScatterMechanisms/emcZeroOrderInterValleyScatterMechanism.hpp>
 #include <ValleyTypes/emcNonParabolicAnistropValley.hpp>
 #include <ValleyTypes/emcNonParabolicIsotropValley.hpp>
@@ -104,8 +106,44 @@ using BathPtr = std::shared_ptr<Bath>;
 // hot_polar: one Kane Gamma valley (m* 0.28, alpha 0.3) with hot-phonon
 // Froehlich absorption + emission (60 meV, eps 3.5 / 10) on a 256-bin bath
 static const double kHotDq = 2e7, kHotTauLO = 1e-12, kHotVsim = 2.7e-23, kHotE0 = 0.5;
+// screened_hot_polar: the same bath read through
+// emcScreenedHotPhononFroehlich{Absorption,Emission}3D with the q-resolved
+// rate and angle, screened by an emcPlasmonScreening (eps 3.5) that starts
+// at q_s^2 = kScrQs2 (rates and final-state KATs) and in the loops follows
+// the carriers after every step like hotCarrierMHP's updateScreening
+using Screening = emcPlasmonScreening<T>;
+using ScrPtr = std::shared_ptr<Screening>;
+static const double kScrEps = 3.5, kScrQs2 = 3e16;
+static std::map<const Bath *, ScrPtr> gScreenOf;  // the screening paired with a bath
+
+static bool isHot(const Model &m) { return m.name == "hot_polar" || m.name == "screened_hot_polar"; }
+
 static BathPtr makeHotBath(const Model &m) {
-  return std::make_shared<Bath>(256, kHotDq, kHotTauLO, 0.06, m.T, kHotVsim);
+  auto bath = std::make_shared<Bath>(256, kHotDq, kHotTauLO, 0.06, m.T, kHotVsim);
+  if (m.name == "screened_hot_polar") {
+    auto scr = std::make_shared<Screening>(kScrEps, true);
+    scr->setQs2(kScrQs2);
+    bath->setScreeningQ2(kScrQs2);
+    gScreenOf[bath.get()] = scr;
+  }
+  return bath;
+}
+
+// updateScreening of hotCarrierMHP.cpp:563-581 for one species: nrPart
+// carriers in kHotVsim at the temperature of the valley-0 mean energy
+static void updateScreening(const BathPtr &bath, SizeType nrPart, T avgE0) {
+  auto it = gScreenOf.find(bath.get());
+  if (it == gScreenOf.end())
+    return;
+  T qs2 = 0;
+  if (nrPart != 0) {
+    const T nS = static_cast<T>(nrPart) / kHotVsim;
+    const T tS = emcHotCarrierOutput::getMBTemp(avgE0);
+    if (tS > T(0))
+      qs2 += nS * constants::q * constants::q / (kScrEps * constants::eps0 * constants::kB * tS);
+  }
+  it->second->setQs2(qs2);
+  bath->setScreeningQ2(qs2);
 }
 
 static const double kSubAxes[3][3][3] = {
@@ -145,7 +183,8 @@ static void addValleys(const Model &m, Electron &tp) {
           {kSubAxes[s][1][0], kSubAxes[s][1][1], kSubAxes[s][1][2]},
           {kSubAxes[s][2][0], kSubAxes[s][2][1], kSubAxes[s][2][2]});
     tp.addValley(std::move(v));
-  } else if (m.name == "polar_gamma" || m.name == "hot_polar" || m.name == "screened_polar") {
+  } else if (m.name == "polar_gamma" || m.name == "hot_polar" || m.name == "screened_polar" ||
+             m.name == "screened_hot_polar") {
     tp.addValley(std::make_unique<emcNonParabolicIsotropValley<T>>(0.28, tp.getMass(), 1, 0.3));
   } else if (m.name == "para_iso") {
     tp.addValley(std::make_unique<emcParabolicIsotropValley<T>>(0.26, tp.getMass(), 1));
@@ -203,6 +242,12 @@ static std::vector<MechPtr> makeMechs(const Model &m, const Device &device,
     scr->setQs2(1e17);
     v.push_back(std::make_unique<emcScreenedFroehlichAbsorption3D<T>>(0, 0.06, 0.28, 3.5, 10., m.T, scr));
     v.push_back(std::make_unique<emcScreenedFroehlichEmission3D<T>>(0, 0.06, 0.28, 3.5, 10., m.T, scr));
+  } else if (m.name == "screened_hot_polar") {
+    ScrPtr scr = gScreenOf.at(bath.get());
+    v.push_back(std::make_unique<emcScreenedHotPhononFroehlichAbsorption3D<T>>(
+        0, 0.06, 0.28, 3.5, 10., bath, scr, true));
+    v.push_back(std::make_unique<emcScreenedHotPhononFroehlichEmission3D<T>>(
+        0, 0.06, 0.28, 3.5, 10., bath, scr, true));
   } else if (m.name == "hot_polar") {
     v.push_back(std::make_unique<emcHotPhononFroehlichAbsorption3D<T>>(0, 0.06, 0.28, 3.5, 10., bath));
     v.push_back(std::make_unique<emcHotPhononFroehlichEmission3D<T>>(0, 0.06, 0.28, 3.5, 10., bath));
@@ -231,8 +276,7 @@ static std::vector<MechPtr> makeMechs(const Model &m, const Device &device,
 // (hot_polar starts every particle at kHotE0 through emcElectron's
 // fixed-energy initialisation)
 static std::unique_ptr<Electron> makeType(const Model &m, Device &device, BathPtr bath = nullptr) {
-  auto type = std::make_unique<Electron>(m.nLevels, m.maxE, false,
-                                         m.name == "hot_polar" ? kHotE0 : 0.0);
+  auto type = std::make_unique<Electron>(m.nLevels, m.maxE, false, isHot(m) ? kHotE0 : 0.0);
   addValleys(m, *type);
   for (auto &mech : makeMechs(m, device, bath))
     type->addScatterMechanism({0}, std::move(mech));
@@ -315,6 +359,24 @@ static void dumpTrajectories(const Model &m, Device &device, Electron &type,
   std::vector<long long> drawsInit;
   for (int p = 0; p < np; p++)
     drawsInit.push_back(RngCounter::between(emcRNG(splitmix64(seed ^ (uint64_t)p)), rngs[p]));
+  std::vector<double> qs2Hist;
+  // mean energy of valley 0 (getAvgEnergy(0)[0]) and the screening update
+  auto screen = [&]() {
+    T sum = 0;
+    SizeType cnt = 0;
+    for (auto &part : parts)
+      if (part.valley == 0) {
+        sum += part.energy;
+        cnt++;
+      }
+    updateScreening(bath, parts.size(), cnt != 0 ? sum / cnt : 0.);
+  };
+  const bool scr = bath && gScreenOf.count(bath.get());
+  if (scr) {
+    // seeded from the initial distribution (hotCarrierMHP.cpp:630-637)
+    screen();
+    type.reinitScatterTables();
+  }
   for (int s = 0; s < nsteps; s++) {
     for (int p = 0; p < np; p++) {
       auto &part = parts[p];
@@ -359,8 +421,13 @@ static void dumpTrajectories(const Model &m, Device &device, Electron &type,
       drawsOut.push_back(RngCounter::between(atStart, rng));
     }
     if (bath) {
-      // end of a hot-phonon step (hotPhononGa2O3.cpp:274-281)
+      // end of a hot-phonon step (hotPhononGa2O3.cpp:274-281; with
+      // screening hotCarrierMHP.cpp:693-703)
       bath->update(dt);
+      if (scr) {
+        screen();
+        qs2Hist.push_back(gScreenOf.at(bath.get())->getQs2());
+      }
       type.reinitScatterTables();
       bathMean.push_back(bath->getMeanNq());
       if (s % 5 == 4)
@@ -372,6 +439,8 @@ static void dumpTrajectories(const Model &m, Device &device, Electron &type,
     sec((b + ".bath_mean").c_str(), bathMean);
     sec((b + ".bath_nq").c_str(), bathNq);
   }
+  if (!qs2Hist.empty())
+    sec((b + ".qs2").c_str(), qs2Hist);
   seci((b + ".meta").c_str(), {(long long)np, (long long)nsteps, (long long)seed});
   sec((b + ".dt").c_str(), {dt});
   sec((b + ".force").c_str(), {force[0], force[1], force[2]});
@@ -506,12 +575,20 @@ static void dumpHandler(const Model &m, uint64_t seed, int nsteps, double dt,
   double mp = spacing * (nodes - 1);
   Device device{silicon(), {mp, mp, mp}, {spacing, spacing, spacing}, m.T};
   device.addConstantDopingRegion({0, 0, 0}, {mp, mp, mp}, doping);
-  BathPtr bath = (m.name == "hot_polar") ? makeHotBath(m) : nullptr;
-  std::vector<double> bathMean;
+  BathPtr bath = isHot(m) ? makeHotBath(m) : nullptr;
+  std::vector<double> bathMean, qs2Hist;
   TypeMap types;
   types[0] = makeType(m, device, bath);
   Handler handler(device, types, m.fieldDir, m.fieldKVcm * 1e5, seed);
   handler.generateInitialParticles();
+  const bool scr = bath && gScreenOf.count(bath.get());
+  auto screen = [&]() {
+    updateScreening(bath, handler.getNrParticles(0), handler.getAvgEnergy(0)[0]);
+  };
+  if (scr) {
+    screen();
+    types[0]->reinitScatterTables();
+  }
   std::vector<double> obs;
   auto push = [&]() {
     auto e = handler.getAvgEnergy(0);
@@ -526,8 +603,13 @@ static void dumpHandler(const Model &m, uint64_t seed, int nsteps, double dt,
     handler.moveParticles(dt);
     push();
     if (bath) {
-      // hotPhononGa2O3.cpp:274-281 with reinitEvery = 1
+      // hotPhononGa2O3.cpp:274-281 with reinitEvery = 1 (with screening
+      // hotCarrierMHP.cpp:693-703)
       bath->update(dt);
+      if (scr) {
+        screen();
+        qs2Hist.push_back(gScreenOf.at(bath.get())->getQs2());
+      }
       types[0]->reinitScatterTables();
       bathMean.push_back(bath->getMeanNq());
     }
@@ -535,6 +617,8 @@ static void dumpHandler(const Model &m, uint64_t seed, int nsteps, double dt,
   std::string b = m.name + ".handler";
   if (bath)
     sec((b + ".bath_mean").c_str(), bathMean);
+  if (!qs2Hist.empty())
+    sec((b + ".qs2").c_str(), qs2Hist);
   seci((b + ".meta").c_str(), {(long long)handler.getNrParticles(0), (long long)nsteps,
                                 (long long)seed, (long long)nodes});
   sec((b + ".geom").c_str(), {dt, doping, spacing, m.T});
@@ -566,9 +650,36 @@ static void dumpBathKATs() {
         prof[i] = 1e-12 * (0.5 + 1.5 * ((i + 0.5) * dq) / (nb * dq));
       bath.setTauLOProfile(prof);
     }
-    std::vector<double> evq, nq, scal;
+    std::vector<double> evq, nq, scal, win, nqwin, sampq;
     std::vector<long long> evem, evn;
     uint64_t st = 0x5EEDBA7Full;
+    // window queries (getNqInWindow, sampleQ): windows from their own
+    // SplitMix64 stream, partly beyond the grid, plus degenerate ones
+    std::vector<double> qw;
+    {
+      uint64_t s2 = 0x3A11D0Cull;
+      for (int i = 0; i < 40; i++) {
+        s2 = splitmix64(s2);
+        double u1 = (double)(s2 >> 11) * 0x1p-53;
+        s2 = splitmix64(s2);
+        double u2 = (double)(s2 >> 11) * 0x1p-53;
+        s2 = splitmix64(s2);
+        double u3 = (double)(s2 >> 11) * 0x1p-53;
+        double qMin = u1 * nb * dq * 1.2;
+        qw.insert(qw.end(), {qMin, qMin + u2 * nb * dq * 0.6, u3});
+      }
+      qw.insert(qw.end(), {3 * dq, 3 * dq, 0.4, 5 * dq, 2 * dq, 0.6, 0.2 * dq, 0.7 * dq, 0.5,
+                           1.3 * nb * dq, 1.5 * nb * dq, 0.5, 0.0, nb * dq, 0.999});
+    }
+    auto queries = [&]() {
+      win.insert(win.end(), bath.cumW.begin(), bath.cumW.end());
+      win.insert(win.end(), bath.cumWN.begin(), bath.cumWN.end());
+      for (size_t i = 0; i < qw.size(); i += 3) {
+        nqwin.push_back(bath.getNqInWindow(qw[i], qw[i + 1]));
+        sampq.push_back(bath.sampleQ(qw[i], qw[i + 1], false, qw[i + 2]));
+        sampq.push_back(bath.sampleQ(qw[i], qw[i + 1], true, qw[i + 2]));
+      }
+    };
     for (int r = 0; r < 6; r++) {
       int n = 50 + 37 * r;
       // round 3 is absorption-heavy: drives low bins below zero (clamp)
@@ -592,8 +703,21 @@ static void dumpBathKATs() {
       nq.insert(nq.end(), bath.Nq.begin(), bath.Nq.end());
       scal.insert(scal.end(), {bath.getMeanNq(), bath.getMeanNac(), bath.getMeanNTO(),
                                bath.getN0(), bath.getAcousticTemp(), bath.getTOTemp()});
+      if (r == 1 || r == 5)
+        queries();
     }
+    // screened weight q / (q^2 + q_s^2), then the same value again (no
+    // rebuild), then back to unscreened
+    bath.setScreeningQ2(kScrQs2);
+    queries();
+    bath.setScreeningQ2(kScrQs2);
+    bath.setScreeningQ2(0.);
+    queries();
     std::string b = std::string("bath.") + c.name;
+    sec((b + ".qwin").c_str(), qw);
+    sec((b + ".win").c_str(), win);
+    sec((b + ".nqwin").c_str(), nqwin);
+    sec((b + ".sampq").c_str(), sampq);
     sec((b + ".q").c_str(), evq);
     seci((b + ".emission").c_str(), evem);
     seci((b + ".n").c_str(), evn);
@@ -980,6 +1104,7 @@ int main(int argc, char **argv) {
       {"two_valley", 1500, 1.5, 300., 30., {1, 1, 1}},
       {"hot_polar", 2000, 2.0, 300., 10., {0, 0, 1}},
       {"screened_polar", 2000, 2.0, 300., 20., {0, 0, 1}},
+      {"screened_hot_polar", 2000, 2.0, 300., 10., {0, 0, 1}},
   };
   const uint64_t seed = 20261202ULL;
   dumpBathKATs();
@@ -994,7 +1119,7 @@ int main(int argc, char **argv) {
     double mp = 4e-8;
     Device device{silicon(), {mp, mp, mp}, {1e-8, 1e-8, 1e-8}, m.T};
     device.addConstantDopingRegion({0, 0, 0}, {mp, mp, mp}, 1e23);
-    const bool hot = m.name == "hot_polar";
+    const bool hot = isHot(m);
     BathPtr bath = hot ? makeHotBath(m) : nullptr;
     auto type = makeType(m, device, bath);
     {

@@ -17,7 +17,8 @@ ME = 9.11e-31     # include/emcConstants.hpp:24
 
 PARA_ISO, NONPARA_ISO, PARA_ANISO, NONPARA_ANISO = 0, 1, 2, 3
 (ACOUSTIC, ZERO_IV_AB, ZERO_IV_EM, FIRST_IV_AB, FIRST_IV_EM, FROEHLICH_AB, FROEHLICH_EM,
- HOT_FROEHLICH_AB, HOT_FROEHLICH_EM, SCR_FROEHLICH_AB, SCR_FROEHLICH_EM) = range(11)
+ HOT_FROEHLICH_AB, HOT_FROEHLICH_EM, SCR_FROEHLICH_AB, SCR_FROEHLICH_EM, SCR_HOT_FROEHLICH_AB,
+ SCR_HOT_FROEHLICH_EM) = range(13)
 
 SI_RHO = 2329.0   # examples/SiliconFunctions.hpp:22
 SI_VSOUND = 9040.0  # examples/SiliconFunctions.hpp:24
@@ -147,6 +148,21 @@ MODELS["screened_polar"] = dict(
     mechs=[dict(kind=SCR_FROEHLICH_AB, valley=0, hw=0.06, mrel=0.28, eps_hi=3.5, eps_lo=10.0),
            dict(kind=SCR_FROEHLICH_EM, valley=0, hw=0.06, mrel=0.28, eps_hi=3.5, eps_lo=10.0)])
 
+# screened hot-phonon model (the hotCarrierMHP default): the hot_polar bath
+# read through emcScreenedHotPhononFroehlich{Absorption,Emission}3D with the
+# q-resolved rate and angle; the screening (eps 3.5) starts at q_s^2 = 3e16
+# / m^2 and in a run follows the carriers (n = N / 2.7e-23 m^3, T from the
+# mean energy) after every step
+MODELS["screened_hot_polar"] = dict(
+    n_levels=2000, max_energy=2.0, T=300.0, field_kvcm=10.0, field_dir=(0, 0, 1), init_energy=0.5,
+    valleys=[dict(kind=NONPARA_ISO, mass=(0.28, 0.28, 0.28), deg=1, alpha=0.3, bottom=0.0)],
+    baths=[dict(n_bins=256, dq=2e7, tau_lo=1e-12, hw=0.06, volume=2.7e-23)],
+    screening=dict(qs2=3e16, plasma=True, eps=3.5, volume=2.7e-23),
+    mechs=[dict(kind=SCR_HOT_FROEHLICH_AB, valley=0, hw=0.06, mrel=0.28, eps_hi=3.5, eps_lo=10.0,
+                bath=0, q_resolved=True, q_resolved_angle=True),
+           dict(kind=SCR_HOT_FROEHLICH_EM, valley=0, hw=0.06, mrel=0.28, eps_hi=3.5, eps_lo=10.0,
+                bath=0, q_resolved=True, q_resolved_angle=True)])
+
 MODEL_NAMES = list(MODELS)
 
 
@@ -165,6 +181,19 @@ def photoexcited_cooling(n_particles=4_000_000):
     b = m["baths"][0]
     b["volume"] = n_particles / 1e24
     b.update(acoustic=True, ac_hw=b["hw"] / 2, tau_ac=30e-12)
+    return m
+
+
+def screened_cooling(n_particles=4_000_000):
+    """The cooling workload of photoexcited_cooling with the reference's
+    hot-carrier default (examples/hotCarrierMHP/hotCarrierMHP.cpp:221, 415-435):
+    screened hot-phonon Froehlich with the q-resolved rate and angle, and the
+    plasma screening (eps 3.5) following the carriers after every step."""
+    import copy
+    m = photoexcited_cooling(n_particles)
+    src = MODELS["screened_hot_polar"]
+    m["mechs"] = copy.deepcopy(src["mechs"])
+    m["screening"] = dict(src["screening"], volume=m["baths"][0]["volume"])
     return m
 
 

@@ -46,8 +46,10 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_header():
     # sizes computed by the C compiler for the descriptors of include/emcgpu.h
     assert C.sizeof(emcgpu.ValleyDesc) == 4 + 4 + 24 + 8 + 4 + 4 + 8 + 8 + 8 * 9 * 8
-    assert C.sizeof(emcgpu.MechDesc) == 3 * 4 + 4 + 5 * 8 + 4 + 8 * 8 * 4 + 4
+    assert C.sizeof(emcgpu.MechDesc) == 3 * 4 + 4 + 5 * 8 + 4 + 8 * 8 * 4 + 4 + 2 * 4
     assert emcgpu.MechDesc.bath.offset == 316
+    assert emcgpu.MechDesc.q_resolved_angle.offset == 324
+    assert C.sizeof(emcgpu.ScreeningDesc) == 4 + 4 + 3 * 8
     assert C.sizeof(emcgpu.BathDesc) == 4 + 4 + 5 * 8 + 4 + 4 + 5 * 8 + 8
     assert C.sizeof(emcgpu.StepArgs) == 8 + 4 * 3 + 4 + 8 + 4 + 4 + 4 + 4 + 4 + 4 + 8 * 9
     assert emcgpu.StepArgs.draws.offset == 56

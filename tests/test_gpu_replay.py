@@ -75,6 +75,8 @@ def test_replay_matches_reference_trajectories(name, golden):
     ctx = _ctx(name, np_)
     ctx.set_particles(0, golden[b + "init_k"], golden[b + "init_E"], golden[b + "init_tau"],
                       golden[b + "init_valley"], golden[b + "init_sub"])
+    if spec.get("screening", {}).get("plasma"):
+        ctx.update_screening(0)  # seeded from the initial carriers, as the reference does
     tr = ctx.step(dt, ns, mode=emcgpu.REPLAY, draws=draws, cursor=cursor, grain_tau=grain,
                   trace=True, steps_per_launch=64)
     shape = (ns, np_)
@@ -87,12 +89,58 @@ def test_replay_matches_reference_trajectories(name, golden):
     kref = golden[b + "k"].reshape(len(snaps), np_, 3)
     Eref = golden[b + "E"].reshape(len(snaps), np_)
     kf, ef = _floors(spec)
-    assert _rel_k(tr["k"][snaps], kref, kf).max() < REL_TOL
-    assert _rel_e(tr["E"][snaps], Eref, ef).max() < REL_TOL
+    tol_k = tol_e = REL_TOL
+    if _q_resolved_angle(spec):
+        sk, se = ulp_sensitivity(spec, golden, b, snaps, kf, ef)
+        tol_k, tol_e = max(REL_TOL, 10 * sk), max(REL_TOL, 10 * se)
+        print("q-resolved angle: 1-ulp sensitivity of the reference algorithm k %.2e E %.2e"
+              % (sk, se), flush=True)
+    dk, de = _rel_k(tr["k"][snaps], kref, kf).max(), _rel_e(tr["E"][snaps], Eref, ef).max()
+    print("max rel diff k %.2e E %.2e (tolerance %.1e / %.1e)" % (dk, de, tol_k, tol_e), flush=True)
+    assert dk < tol_k
+    assert de < tol_e
     # the stored state after the last step agrees with the last trace row
     st = ctx.get_particles(0)
     assert (st["valley"] == tr["valley"][-1]).all()
-    assert _rel_k(st["k"].reshape(-1, 3), kref[-1], kf).max() < REL_TOL
+    assert _rel_k(st["k"].reshape(-1, 3), kref[-1], kf).max() < tol_k
+
+
+def _q_resolved_angle(spec):
+    return any(m.get("q_resolved") and m.get("q_resolved_angle", True) for m in spec["mechs"])
+
+
+def ulp_sensitivity(spec, golden, b, snaps, kf, ef):
+    """How far the reference algorithm itself (the oracle, bit-exact against
+    the reference's trajectories) moves when every initial k is changed by
+    one ulp: max relative k and E change at the snapshot steps.
+
+    With the q-resolved angle (sampleQ, emcPhononBath.hpp:423-458) an event
+    whose q lands on the window edge |kI - kF| has cos(theta) = 1 up to
+    rounding, so the new direction carries rounding noise of sqrt(ulp) ~
+    1e-8: no implementation whose arithmetic is not the reference's bit for
+    bit (the GPU's transcendentals are not glibc's) can follow such a
+    trajectory to 1e-12.  The bar for these models is a multiple of this
+    measured sensitivity instead."""
+    np_, ns, seed = [int(x) for x in golden[b + "meta"]]
+    dt, force = golden[b + "dt"][0], golden[b + "force"]
+    out = []
+    for pert in (0.0, 2.0 ** -52):
+        M = O.OracleModel(spec)
+        draws = O.mt64_particle_draws(seed, np_, 4096)
+        cur = np.zeros(np_, dtype=np.uintp)
+        st = M.init_replay(draws, cur, O.thermal_voltage(spec["T"]))
+        st["k"] *= 1.0 + pert
+        if spec.get("screening", {}).get("plasma"):
+            M.seed_screening(st["E"], st["valley"])
+        ks, es = [], []
+        for s in range(ns):
+            M.run_replay(st, draws, cur, 1, dt, force)
+            if s in set(int(x) for x in snaps):
+                ks.append(st["k"].reshape(-1, 3).copy())
+                es.append(st["E"].copy())
+        out.append((np.array(ks), np.array(es)))
+    (k0, e0), (k1, e1) = out
+    return _rel_k(k1, k0, kf).max(), _rel_e(e1, e0, ef).max()
 
 
 @pytest.mark.parametrize("name", ["si_jacoboni", "two_valley"])

@@ -185,3 +185,112 @@ def test_energy_initialiser_statistics():
     assert np.all(np.abs((k / kn[:, None]).mean(axis=0)) < 5 / np.sqrt(E.size))
     fixed = M.init_philox_energy(1000, 0.5, 0.0, 12345)
     assert (fixed["E"] == 0.5).all()
+
+
+# ---- window sums (q-resolved coupling) -------------------------------------
+
+@pytest.mark.parametrize("name", ["plain", "acoustic", "ridley", "profile", "acoustic_profile"])
+def test_window_sums_and_queries_match_reference(name, golden):
+    """rebuildWindowSums after update and setScreeningQ2, getNqInWindow and
+    sampleQ (emcPhononBath.hpp:109-124, 203-208, 378-458), bit-exact against
+    emcPhononBath on the KAT rounds' occupations, unscreened and screened."""
+    bath, dt = _kat_bath(golden, name)
+    M = _bath_model(bath)
+    g = "bath." + name + "."
+    q, em, counts = golden[g + "q"], golden[g + "emission"], golden[g + "n"]
+    qw = golden[g + "qwin"].reshape(-1, 3)
+    nb = bath["n_bins"]
+    win, nqwin, sampq = [], [], []
+
+    def queries():
+        w, wn = M.bath_window(0)
+        win.extend([w, wn])
+        for qmin, qmax, r in qw:
+            nqwin.append(M.nq_in_window(0, qmin, qmax))
+            sampq.append(M.sample_q(0, qmin, qmax, False, r))
+            sampq.append(M.sample_q(0, qmin, qmax, True, r))
+
+    pos = 0
+    for r, n in enumerate(counts):
+        for i in range(pos, pos + int(n)):
+            M.bath_record(0, q[i], em[i])
+        pos += int(n)
+        M.bath_update(0, dt)
+        if r in (1, 5):
+            queries()
+    M.set_qs2(3e16)
+    queries()
+    M.set_qs2(3e16)
+    M.set_qs2(0.0)
+    queries()
+    assert (np.concatenate(win) == golden[g + "win"]).all()
+    assert (np.array(nqwin) == golden[g + "nqwin"]).all()
+    assert (np.array(sampq) == golden[g + "sampq"]).all()
+    assert len(win) == 8 and win[0].size == nb + 1
+
+
+def _uneven_bath(qs2):
+    """A 32-bin bath whose occupation falls off with q (events recorded at
+    small q, one update), with screening q_s^2."""
+    b = dict(n_bins=32, dq=5e7, tau_lo=1e-12, hw=0.06, volume=1e-22)
+    M = _bath_model(b)
+    rng = np.random.default_rng(5)
+    for x in rng.random(3000) ** 2 * 32 * 5e7:
+        M.bath_record(0, x, True)
+    M.bath_update(0, 1e-14)
+    M.set_qs2(qs2)
+    return M, b
+
+
+@pytest.mark.parametrize("qs2", [0.0, 3e16, 1e18])
+def test_window_mean_is_the_coupling_weighted_mean(qs2):
+    """getNqInWindow = sum w N / sum w over the bins the window touches, with
+    w = q/(q^2 + q_s^2) at the bin centres (the Froehlich dq/q weight)."""
+    M, b = _uneven_bath(qs2)
+    nq = M.bath(0)["nq"]
+    dq = b["dq"]
+    qc = (np.arange(32) + 0.5) * dq
+    w = qc / (qc ** 2 + qs2)
+    for qmin, qmax in ((0.2 * dq, 3.7 * dq), (5 * dq, 20 * dq), (10.5 * dq, 31.2 * dq),
+                       (2 * dq, 40 * dq)):
+        lo, hi = int(np.floor(qmin / dq)), min(int(np.ceil(qmax / dq)), 32)
+        want = (w[lo:hi] * nq[lo:hi]).sum() / w[lo:hi].sum()
+        # differences of prefix sums: a window high in q loses the digits of
+        # the sum below it
+        assert abs(M.nq_in_window(0, qmin, qmax) - want) <= 1e-10 * want
+    # outside the grid or degenerate: the DOS-weighted mean
+    mean = M.bath(0)["mean"]
+    assert M.nq_in_window(0, 3 * dq, 3 * dq) == mean
+    assert M.nq_in_window(0, 40 * dq, 50 * dq) == mean
+
+
+@pytest.mark.parametrize("emission", [False, True])
+@pytest.mark.parametrize("qs2", [0.0, 1e18])
+def test_sample_q_follows_the_transition_weight(emission, qs2):
+    """Sampled q fall into each bin of a bin-aligned window with probability
+    w f / sum w f, f = N (absorption) or N + 1 (emission); q stays in the
+    window and grows with r."""
+    M, b = _uneven_bath(qs2)
+    nq = M.bath(0)["nq"]
+    dq = b["dq"]
+    qc = (np.arange(32) + 0.5) * dq
+    w = qc / (qc ** 2 + qs2) * (nq + 1.0 if emission else nq)
+    lo, hi = 2, 27
+    n = 20000
+    qs = np.array([M.sample_q(0, lo * dq, hi * dq, emission, (j + 0.5) / n) for j in range(n)])
+    assert (np.diff(qs) >= 0).all() and qs.min() >= lo * dq and qs.max() <= hi * dq
+    frac = np.bincount((qs / dq).astype(int), minlength=32)[lo:hi] / n
+    want = w[lo:hi] / w[lo:hi].sum()
+    assert np.abs(frac - want).max() < 2.0 / n + 1e-12
+
+
+def test_qres_angle_maps_q_to_cos_theta():
+    """sampleQResolvedCosTheta: cos = (kI^2 + kF^2 - q^2) / (2 kI kF) of the
+    sampled q, so the momentum transfer of the new direction is q."""
+    M, b = _uneven_bath(3e16)
+    kI, kF = 7.0e8, 5.5e8
+    for r in (0.05, 0.5, 0.95):
+        c = M.qres_cos_theta(0, kI, kF, True, r)
+        qq = M.sample_q(0, abs(kI - kF), kI + kF, True, r)
+        assert abs(np.sqrt(kI ** 2 + kF ** 2 - 2 * kI * kF * c) - qq) < 1e-6 * qq
+    assert M.qres_cos_theta(0, 0.0, kF, False, 0.3) == 1.0 - 2.0 * 0.3

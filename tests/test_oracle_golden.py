@@ -167,9 +167,13 @@ def test_replay_trajectories_bit_exact(name, golden, oracle_models):
                    ("valley", "init_valley"), ("sub", "init_sub")):
         assert (st[key] == golden[b + g]).all(), key
     assert (cur == golden[b + "init_draws"]).all()
+    plasma = spec.get("screening", {}).get("plasma", False)
+    if plasma:
+        # hotCarrierMHP seeds the screening from the initial carriers
+        M.seed_screening(st["E"], st["valley"])
     if M.n_baths:
         tr = {key: [] for key in ("valley", "sub", "nev", "k", "E")}
-        mean, nq = [], []
+        mean, nq, qs2 = [], [], []
         for s in range(ns):
             nover, t1 = M.run_replay(st, draws, cur, 1, dt, force, trace=True)
             assert nover == 0
@@ -177,11 +181,15 @@ def test_replay_trajectories_bit_exact(name, golden, oracle_models):
                 tr[key].append(t1[key])
             bt = M.bath(0)
             mean.append(bt["mean"])
+            qs2.append(M.qs2())
             if s % 5 == 4:
                 nq.append(bt["nq"])
         tr = {key: np.concatenate(v) for key, v in tr.items()}
         assert (np.array(mean) == golden[b + "bath_mean"]).all()
         assert (np.concatenate(nq) == golden[b + "bath_nq"]).all()
+        if plasma:
+            # q_s^2 after every step's updateScreening
+            assert (np.array(qs2) == golden[b + "qs2"]).all()
     else:
         nover, tr = M.run_replay(st, draws, cur, ns, dt, force, trace=True)
         assert nover == 0

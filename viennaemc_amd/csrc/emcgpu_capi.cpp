@@ -57,6 +57,59 @@ void normalize3(double d[3]) {
     d[i] /= n;
 }
 
+// ---- mechanism kinds ---------------------------------------------------
+inline bool is_hot_kind(int k) {
+  return k == EMCGPU_MECH_HOT_FROEHLICH_AB || k == EMCGPU_MECH_HOT_FROEHLICH_EM;
+}
+inline bool is_scr_kind(int k) {
+  return k == EMCGPU_MECH_SCREENED_FROEHLICH_AB || k == EMCGPU_MECH_SCREENED_FROEHLICH_EM;
+}
+inline bool is_scr_hot_kind(int k) {
+  return k == EMCGPU_MECH_SCREENED_HOT_FROEHLICH_AB || k == EMCGPU_MECH_SCREENED_HOT_FROEHLICH_EM;
+}
+inline bool reads_bath(int k) { return is_hot_kind(k) || is_scr_hot_kind(k); }
+// a mechanism whose rate the GPU rebuilds during a run
+inline bool dyn_mech(const Instance &in, const emcgpu_mech_desc &m) {
+  return reads_bath(m.kind) || (in.plasma() && is_scr_kind(m.kind));
+}
+bool dyn_valley(const Instance &in, size_t v) {
+  for (int j : in.table_mech[v])
+    if (dyn_mech(in, in.mechs[j]))
+      return true;
+  return false;
+}
+
+// rebuildWindowSums + getNqInWindow (emcPhononBath.hpp:109-124, 378-394) on
+// the host copy of a bath
+double window_nq_host(const BathHost &b, double qs2, double mean, double qmin, double qmax) {
+  const int nb = b.d.n_bins;
+  if (nb < 2 || qmax <= qmin)
+    return mean;
+  long lo = (long)std::floor(qmin / b.d.dq), hi = (long)std::ceil(qmax / b.d.dq);
+  if (lo < 0)
+    lo = 0;
+  if (hi > (long)nb)
+    hi = (long)nb;
+  if (hi - lo < 1)
+    return mean;
+  double cw = 0, cwn = 0, wlo = 0, wnlo = 0;
+  for (long i = 0; i < hi; i++) {
+    if (i == lo) {
+      wlo = cw;
+      wnlo = cwn;
+    }
+    const double q = ((double)(std::size_t)i + 0.5) * b.d.dq;
+    const double denom = q * q + qs2;
+    const double w = (denom > 0.0) ? q / denom : 0.0;
+    cw = cw + w;
+    cwn = cwn + w * b.nq[i];
+  }
+  const double wsum = cw - wlo;
+  if (wsum <= 0.0)
+    return mean;
+  return (cwn - wnlo) / wsum;
+}
+
 template <typename R>
 void lower_instance(const emcgpu_ctx *c, const Instance &in, double dt_s, InstDev<R> &P,
                     const std::vector<long long> &table_off) {
@@ -140,10 +193,18 @@ void lower_instance(const emcgpu_ctx *c, const Instance &in, double dt_s, InstDe
       }
       case EMCGPU_MECH_SCREENED_FROEHLICH_AB:
       case EMCGPU_MECH_SCREENED_FROEHLICH_EM:
+      case EMCGPU_MECH_SCREENED_HOT_FROEHLICH_AB:
+      case EMCGPU_MECH_SCREENED_HOT_FROEHLICH_EM: {
+        const bool ab = md.kind == EMCGPU_MECH_SCREENED_FROEHLICH_AB ||
+                        md.kind == EMCGPU_MECH_SCREENED_HOT_FROEHLICH_AB;
         mm.final_kind = kFinalScreenedFroehlich;
-        mm.shift = md.kind == EMCGPU_MECH_SCREENED_FROEHLICH_AB ? (R)md.phonon_energy
-                                                                 : (R)(-md.phonon_energy);
+        mm.shift = ab ? (R)md.phonon_energy : (R)(-md.phonon_energy);
+        if (is_scr_hot_kind(md.kind)) {
+          mm.bath1 = md.bath + 1;
+          mm.qres = (md.q_resolved && md.q_resolved_angle) ? 1 : 0;
+        }
         break;
+      }
       default: {
         const bool ab = md.kind == EMCGPU_MECH_FROEHLICH_AB || md.kind == EMCGPU_MECH_HOT_FROEHLICH_AB;
         mm.final_kind = kFinalFroehlich;
@@ -156,6 +217,8 @@ void lower_instance(const emcgpu_ctx *c, const Instance &in, double dt_s, InstDe
     }
   }
   P.qs2 = (R)(in.qs2 * kToNm * kToNm);
+  for (size_t b = 0; b < in.baths.size() && c->d_win; b++)
+    P.bath_win[b] = (const R *)c->d_win + 2 * (in.bath_off[b] + in.bath_index0 + (long long)b);
   P.n_baths = (int)in.baths.size();
   P.total_bins = in.total_bins;
   P.gbin_off = in.gbin_off;
@@ -208,6 +271,22 @@ emcgpu_status pull_baths(emcgpu_ctx *c, Instance &in) {
   return EMCGPU_OK;
 }
 
+// copies the q_s^2 that plasma screening evolves on the GPU to the host
+emcgpu_status pull_screening(emcgpu_ctx *c) {
+  bool any = false;
+  for (const auto &in : c->inst)
+    any |= in.dev_scr;
+  if (!any || c->device < 0 || !c->d_scr)
+    return EMCGPU_OK;
+  std::vector<double> v(c->n_inst);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(v.data(), c->d_scr, sizeof(double) * v.size(), hipMemcpyDeviceToHost));
+  for (int i = 0; i < c->n_inst; i++)
+    if (c->inst[i].dev_scr)
+      c->inst[i].qs2 = v[i];
+  return EMCGPU_OK;
+}
+
 // (re)creates the device bath arrays of all instances from their host state
 emcgpu_status upload_bath_state(emcgpu_ctx *c) {
   for (auto &in : c->inst) {
@@ -244,6 +323,7 @@ emcgpu_status upload_bath_state(emcgpu_ctx *c) {
       d.to_e = B.d.to_phonon_energy;
       d.tau_to = B.d.tau_to;
       d.n_to_eq = B.n_to_eq;
+      d.woff = 2 * (d.off + (long long)bd.size());
       d.sum_w = 0.0;
       for (int i = 0; i < B.d.n_bins; i++) {
         const double q = ((double)(std::size_t)i + 0.5) * B.d.dq;
@@ -266,6 +346,9 @@ emcgpu_status upload_bath_state(emcgpu_ctx *c) {
   dfreet(c->d_tau_prof);
   dfreet(c->d_tally);
   dfreet(c->d_bath_var);
+  dfreet(c->d_win64);
+  dfree(c->d_win);
+  c->win_elems = 2 * ((long long)nq.size() + (long long)bd.size());
   c->n_baths = (int)bd.size();
   c->lds_bins = lds;
   c->tally_bins = (long long)nq.size();
@@ -276,6 +359,9 @@ emcgpu_status upload_bath_state(emcgpu_ctx *c) {
     HIPCHK(c, hipMalloc((void **)&c->d_tally,
                         sizeof(unsigned long long) * 2 * kTallyCopies * nq.size()));
     HIPCHK(c, hipMalloc((void **)&c->d_bath_var, sizeof(BathVar) * bv.size()));
+    HIPCHK(c, hipMalloc((void **)&c->d_win64, sizeof(double) * c->win_elems));
+    HIPCHK(c, hipMalloc(&c->d_win, (c->precision == EMCGPU_FP32 ? sizeof(float) : sizeof(double)) *
+                                       c->win_elems));
     HIPCHK(c, hipMemcpy(c->d_baths, bd.data(), sizeof(BathDev) * bd.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_nq, nq.data(), sizeof(double) * nq.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_tau_prof, prof.data(), sizeof(double) * prof.size(),
@@ -313,6 +399,10 @@ BathStepArgs<R> bath_args(const emcgpu_ctx *c, double dt, double *hist, int hist
   a.cum64 = c->d_cum64;
   a.inst = (InstDev<R> *)c->d_params;
   a.tau0_64 = c->d_tau0_64;
+  a.scr = c->d_scr;
+  a.win64 = c->d_win64;
+  a.win = (R *)c->d_win;
+  a.n_obs = c->n_obs;
   return a;
 }
 
@@ -323,18 +413,24 @@ emcgpu_status upload_rebuild(emcgpu_ctx *c, size_t cum_elems) {
   std::vector<RebuildJob> jobs;
   std::vector<BathInstDev> groups;
   std::vector<double> arena;
+  std::vector<double> scr(c->n_inst, 0.0);
   for (int i = 0; i < c->n_inst; i++) {
     const Instance &in = c->inst[i];
-    if (in.baths.empty())
+    scr[i] = in.qs2;
+    if (in.baths.empty() && !in.plasma())
       continue;
-    BathInstDev g{(int)in.bath_index0, (int)in.baths.size(), (int)jobs.size(), 0};
-    for (size_t v = 0; in.has_hot && in.has_tables && v < in.valleys.size(); v++) {
+    BathInstDev g{};
+    g.bath0 = (int)in.bath_index0;
+    g.n_baths = (int)in.baths.size();
+    g.job0 = (int)jobs.size();
+    g.inst = i;
+    g.plasma = in.plasma() ? 1 : 0;
+    g.n_valleys = (int)in.valleys.size();
+    g.scr_eps = in.scr.eps_static;
+    g.scr_vol = in.scr.volume;
+    for (size_t v = 0; in.has_tables && v < in.valleys.size(); v++) {
       const auto &tm = in.table_mech[v];
-      bool hot = false;
-      for (int j : tm)
-        hot |= in.mechs[j].kind == EMCGPU_MECH_HOT_FROEHLICH_AB ||
-               in.mechs[j].kind == EMCGPU_MECH_HOT_FROEHLICH_EM;
-      if (!hot)
+      if (!dyn_valley(in, v))
         continue;
       RebuildJob J{};
       J.inst = i;
@@ -346,10 +442,24 @@ emcgpu_status upload_rebuild(emcgpu_ctx *c, size_t cum_elems) {
         const emcgpu_mech_desc &md = in.mechs[tm[t]];
         J.rate_off[t] = (long long)arena.size();
         arena.insert(arena.end(), in.rate_data[tm[t]].begin(), in.rate_data[tm[t]].end());
-        if (md.kind == EMCGPU_MECH_HOT_FROEHLICH_AB || md.kind == EMCGPU_MECH_HOT_FROEHLICH_EM) {
+        if (is_hot_kind(md.kind)) {
           J.kind[t] = md.kind == EMCGPU_MECH_HOT_FROEHLICH_AB ? kRateHotAb : kRateHotEm;
           J.bath[t] = (int)(in.bath_index0 + md.bath);
           J.C[t] = in.hot_C[tm[t]];
+        } else if (dyn_mech(in, md)) {
+          // screened kinds: kI, kF per level; the occupation a constant, the
+          // bath mean or the bath window mean
+          const bool ab = md.kind == EMCGPU_MECH_SCREENED_FROEHLICH_AB ||
+                          md.kind == EMCGPU_MECH_SCREENED_HOT_FROEHLICH_AB;
+          J.kind[t] = ab ? kRateScrAb : kRateScrEm;
+          J.C[t] = in.hot_C[tm[t]];
+          if (is_scr_hot_kind(md.kind)) {
+            J.bath[t] = (int)(in.bath_index0 + md.bath);
+            J.nsrc[t] = md.q_resolved ? kNWindow : kNMean;
+          } else {
+            J.nsrc[t] = kNConst;
+            J.nconst[t] = in.mech_n0[tm[t]];
+          }
         } else {
           J.kind[t] = kRateConst;
         }
@@ -362,15 +472,25 @@ emcgpu_status upload_rebuild(emcgpu_ctx *c, size_t cum_elems) {
   dfreet(c->d_jobs);
   dfreet(c->d_groups);
   dfreet(c->d_rate_arena);
+  dfreet(c->d_scr);
   c->n_jobs = (int)jobs.size();
   c->n_groups = (int)groups.size();
+  HIPCHK(c, hipMalloc((void **)&c->d_scr, sizeof(double) * scr.size()));
+  HIPCHK(c, hipMemcpy(c->d_scr, scr.data(), sizeof(double) * scr.size(), hipMemcpyHostToDevice));
+  for (auto &in : c->inst)
+    in.dev_scr = in.plasma();
   if (!groups.empty()) {
     HIPCHK(c, hipMalloc((void **)&c->d_groups, sizeof(BathInstDev) * groups.size()));
     HIPCHK(c, hipMemcpy(c->d_groups, groups.data(), sizeof(BathInstDev) * groups.size(),
                         hipMemcpyHostToDevice));
   }
-  if (jobs.empty())
+  if (jobs.empty()) {
+    // window sums of the baths still follow their state and q_s^2
+    if (!groups.empty())
+      HIPCHK(c, launch_rebuild<R>(bath_args<R>(c, 0.0, nullptr, 0, 0, 1), c->n_groups, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return EMCGPU_OK;
+  }
   HIPCHK(c, hipMalloc((void **)&c->d_jobs, sizeof(RebuildJob) * jobs.size()));
   HIPCHK(c, hipMalloc((void **)&c->d_rate_arena, sizeof(double) * std::max<size_t>(arena.size(), 1)));
   HIPCHK(c, hipMemcpy(c->d_jobs, jobs.data(), sizeof(RebuildJob) * jobs.size(),
@@ -384,7 +504,7 @@ emcgpu_status upload_rebuild(emcgpu_ctx *c, size_t cum_elems) {
   }
   if (!c->d_tau0_64)
     HIPCHK(c, hipMalloc((void **)&c->d_tau0_64, sizeof(double) * kMaxV * c->n_inst));
-  HIPCHK(c, launch_rebuild<R>(bath_args<R>(c, 0.0, nullptr, 0, 0, 1), c->n_jobs, c->stream));
+  HIPCHK(c, launch_rebuild<R>(bath_args<R>(c, 0.0, nullptr, 0, 0, 1), c->n_groups, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return EMCGPU_OK;
 }
@@ -394,6 +514,11 @@ template <typename R>
 emcgpu_status upload_params(emcgpu_ctx *c, double dt_s) {
   if (!c->params_dirty && dt_s == c->params_dt)
     return EMCGPU_OK;
+  {
+    emcgpu_status st = pull_screening(c);
+    if (st)
+      return st;
+  }
   if (c->bath_layout_dirty) {
     emcgpu_status st = upload_bath_state(c);
     if (st)
@@ -515,13 +640,19 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
   K = std::min(K, s->n_steps);
   // phonon baths couple the particles between steps: one step per launch,
   // then the bath update and the table rebuild
+  // plasma screening likewise: q_s^2 and the screened tables follow the
+  // carriers after every step
   const bool baths = c->n_baths > 0;
-  if (baths)
+  bool plasma = false;
+  for (const auto &in : c->inst)
+    plasma |= in.plasma();
+  const bool dyn = baths || plasma;
+  if (dyn)
     K = 1;
   const int every = s->rebuild_every > 1 ? s->rebuild_every : 1;
   // steps whose per-step partials are reduced by one pair of reduction
   // launches (a launch's steps, or up to 64 one-step launches with baths)
-  const int Kr = baths ? std::min(s->n_steps, 64) : K;
+  const int Kr = dyn ? std::min(s->n_steps, 64) : K;
   if (baths && !s->bath_record_only) {
     const size_t hb = sizeof(double) * (size_t)c->n_baths * (size_t)s->n_steps;
     if (hb > c->bath_hist_bytes) {
@@ -666,12 +797,18 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
       c->ms_total += ms;
       c->launches += 1;
     }
-    if (baths && !s->bath_record_only) {
-      // end of the step of the hot-phonon drivers (hotPhononGa2O3.cpp:274-281)
+    if (dyn && !s->bath_record_only) {
+      // end of the step of the hot-phonon drivers (hotPhononGa2O3.cpp:274-281;
+      // with screening hotCarrierMHP.cpp:693-703)
       const long long gstep = (long long)s->step0 + done + 1;  // 1-based global step
-      HIPCHK_CLEAN(launch_bath_step<R>(
-          bath_args<R>(c, s->dt, c->d_bath_hist, s->n_steps, done, gstep % every == 0),
-          c->n_groups, c->stream));
+      BathStepArgs<R> ba =
+          bath_args<R>(c, s->dt, c->d_bath_hist, s->n_steps, done, gstep % every == 0);
+      ba.partial = a.partial;
+      ba.n_tiles = n_tiles;
+      ba.tile_size = tile_size;
+      ba.inst_start = c->d_inst_start;
+      ba.inst_count = c->d_inst_count;
+      HIPCHK_CLEAN(launch_bath_step<R>(ba, c->n_groups, c->stream));
     }
     // per-step partials of a batch of Kr steps are reduced together
     const int in_batch = (done % Kr) + k;
@@ -953,6 +1090,10 @@ void emcgpu_destroy(emcgpu_ctx *c) {
   dfreet(c->d_rate_arena);
   dfreet(c->d_cum64);
   dfreet(c->d_tau0_64);
+  dfreet(c->d_groups);
+  dfreet(c->d_scr);
+  dfreet(c->d_win64);
+  dfree(c->d_win);
   if (c->gexec)
     (void)hipGraphExecDestroy(c->gexec);
   if (c->ev0)
@@ -1006,10 +1147,9 @@ emcgpu_status emcgpu_set_model(emcgpu_ctx *c, int instance, const emcgpu_valley_
   }
   for (int j = 0; j < n_mechs; j++) {
     const emcgpu_mech_desc &m = mechs[j];
-    if (m.kind < 0 || m.kind > EMCGPU_MECH_SCREENED_FROEHLICH_EM)
+    if (m.kind < 0 || m.kind > EMCGPU_MECH_SCREENED_HOT_FROEHLICH_EM)
       return fail(c, EMCGPU_ERR_ARG, "mechanism %d: unknown kind %d", j, m.kind);
-    if ((m.kind == EMCGPU_MECH_HOT_FROEHLICH_AB || m.kind == EMCGPU_MECH_HOT_FROEHLICH_EM) &&
-        (m.bath < 0 || m.bath >= EMCGPU_MAX_BATHS))
+    if (reads_bath(m.kind) && (m.bath < 0 || m.bath >= EMCGPU_MAX_BATHS))
       return fail(c, EMCGPU_ERR_ARG, "mechanism %d: bath index %d out of range", j, m.bath);
     if (m.valley < 0 || m.valley >= n_valleys)
       return fail(c, EMCGPU_ERR_ARG, "mechanism %d: Used index for Valley is invalid", j);
@@ -1039,7 +1179,7 @@ emcgpu_status emcgpu_set_model(emcgpu_ctx *c, int instance, const emcgpu_valley_
   in.has_tables = false;
   in.has_hot = false;
   for (const auto &m : in.mechs)
-    in.has_hot |= m.kind == EMCGPU_MECH_HOT_FROEHLICH_AB || m.kind == EMCGPU_MECH_HOT_FROEHLICH_EM;
+    in.has_hot |= reads_bath(m.kind);
   // observable layout covers the largest model of the context
   int mv = 1, md = 1;
   for (const auto &i2 : c->inst) {
@@ -1099,8 +1239,7 @@ emcgpu_status emcgpu_build_tables(emcgpu_ctx *c, int instance, int n_levels, dou
   const double dE = max_energy / (double)(std::size_t)n_levels;
   for (size_t j = 0; j < in.mechs.size(); j++) {
     const emcgpu_mech_desc &m = in.mechs[j];
-    if ((m.kind == EMCGPU_MECH_HOT_FROEHLICH_AB || m.kind == EMCGPU_MECH_HOT_FROEHLICH_EM) &&
-        m.bath >= (int)in.baths.size())
+    if (reads_bath(m.kind) && m.bath >= (int)in.baths.size())
       return fail(c, EMCGPU_ERR_STATE, "mechanism %zu reads bath %d: set the baths first", j,
                   m.bath);
   }
@@ -1109,6 +1248,7 @@ emcgpu_status emcgpu_build_tables(emcgpu_ctx *c, int instance, int n_levels, dou
   std::vector<std::vector<double>> raw(in.mechs.size(), std::vector<double>(n_levels));
   in.rate_data.assign(in.mechs.size(), {});
   in.hot_C.assign(in.mechs.size(), 0.0);
+  in.mech_n0.assign(in.mechs.size(), 0.0);
   for (size_t j = 0; j < in.mechs.size(); j++) {
     const emcgpu_mech_desc &m = in.mechs[j];
     const phys::ValleyMass &iv = in.vm[m.valley];
@@ -1117,7 +1257,45 @@ emcgpu_status emcgpu_build_tables(emcgpu_ctx *c, int instance, int n_levels, dou
         m.kind == EMCGPU_MECH_HOT_FROEHLICH_AB || m.kind == EMCGPU_MECH_HOT_FROEHLICH_EM;
     const bool ab = m.kind == EMCGPU_MECH_ZERO_IV_AB || m.kind == EMCGPU_MECH_FIRST_IV_AB ||
                     m.kind == EMCGPU_MECH_FROEHLICH_AB || m.kind == EMCGPU_MECH_HOT_FROEHLICH_AB ||
-                    m.kind == EMCGPU_MECH_SCREENED_FROEHLICH_AB;
+                    m.kind == EMCGPU_MECH_SCREENED_FROEHLICH_AB ||
+                    m.kind == EMCGPU_MECH_SCREENED_HOT_FROEHLICH_AB;
+    if (is_scr_hot_kind(m.kind) || (in.plasma() && is_scr_kind(m.kind))) {
+      // emcScreenedFroehlichInteraction.hpp:128-138 / :184-196 and
+      // :257-269 / :333-348: kI and kF kept per level for the rebuilds on
+      // the GPU (kI = 0: zero rate); the occupation the Bose one, the bath
+      // mean or the bath's window mean
+      meff = m.rel_eff_mass * phys::me;
+      C = phys::froehlichConst(m.phonon_energy, meff, m.eps_hi, m.eps_lo);
+      const bool hotk = is_scr_hot_kind(m.kind);
+      const double n0 = hotk ? 0.0 : phys::boseEinstein(m.phonon_energy, temperature);
+      const double mean = hotk ? bath_mean(in.baths[m.bath]) : 0.0;
+      std::vector<double> data(2 * (size_t)n_levels, 0.0);
+      for (int l = 0; l < n_levels; l++) {
+        const double E = (double)(std::size_t)(l + 1) * dE;
+        double r = 0.0;
+        if (ab || E > m.phonon_energy) {
+          const double gI = iv.gamma(E);
+          const double gF = iv.gamma(ab ? E + m.phonon_energy : E - m.phonon_energy);
+          if (gI > 0.0 && gF > 0.0) {
+            const double kI = std::sqrt(2.0 * meff * gI * phys::q) / phys::hbar;
+            const double kF = std::sqrt(2.0 * meff * gF * phys::q) / phys::hbar;
+            data[l] = kI;
+            data[n_levels + l] = kF;
+            double n = n0;
+            if (hotk)
+              n = m.q_resolved ? window_nq_host(in.baths[m.bath], in.qs2, mean, std::fabs(kI - kF),
+                                                kI + kF)
+                               : mean;
+            r = (ab ? C * n : C * (n + 1.0)) / kI * phys::screenedLogFactor(kI, kF, in.qs2);
+          }
+        }
+        raw[j][l] = r;
+      }
+      in.rate_data[j] = std::move(data);
+      in.hot_C[j] = C;
+      in.mech_n0[j] = n0;
+      continue;
+    }
     if (hot) {
       // emcHotPhononFroehlichMechanism.hpp:79-91 / :159-173: the Froehlich
       // rate with the bath's mean occupation; kI and ln kept per level for
@@ -1212,11 +1390,8 @@ emcgpu_status emcgpu_get_tables(const emcgpu_ctx *c, int instance, int valley, d
   const int nt = (int)in.table_mech[valley].size();
   if (n_tables)
     *n_tables = nt;
-  bool hot_valley = false;
-  for (int j : in.table_mech[valley])
-    hot_valley |= in.mechs[j].kind == EMCGPU_MECH_HOT_FROEHLICH_AB ||
-                  in.mechs[j].kind == EMCGPU_MECH_HOT_FROEHLICH_EM;
-  if (hot_valley && in.dev_bath && c->device >= 0) {
+  const bool hot_valley = dyn_valley(in, (size_t)valley);
+  if (hot_valley && (in.dev_bath || in.dev_scr) && c->device >= 0) {
     // the tables rebuilt on the GPU from the evolving baths
     emcgpu_ctx *cm = const_cast<emcgpu_ctx *>(c);
     (void)hipSetDevice(c->device);
@@ -1413,17 +1588,85 @@ emcgpu_status emcgpu_set_screening(emcgpu_ctx *c, int instance, const emcgpu_scr
     return st;
   if (!d)
     return fail(c, EMCGPU_ERR_ARG, "screening descriptor required");
-  if (d->mode != EMCGPU_SCREENING_FIXED)
+  if (d->mode != EMCGPU_SCREENING_FIXED && d->mode != EMCGPU_SCREENING_PLASMA)
     return fail(c, EMCGPU_ERR_ARG, "unknown screening mode %d", d->mode);
   if (!(d->qs2 >= 0.0) || !std::isfinite(d->qs2))
     return fail(c, EMCGPU_ERR_ARG, "q_s^2 must be finite and >= 0 (%g)", d->qs2);
   if (!(d->eps_static > 0.0))
     return fail(c, EMCGPU_ERR_ARG, "eps_static must be > 0");
+  if (d->mode == EMCGPU_SCREENING_PLASMA && !(d->volume > 0.0 && std::isfinite(d->volume)))
+    return fail(c, EMCGPU_ERR_ARG, "plasma screening needs a volume > 0");
+  if (c->device >= 0)
+    (void)hipSetDevice(c->device);
+  emcgpu_status st2 = pull_screening(c);
+  if (st2)
+    return st2;
   Instance &in = c->inst[instance];
+  const bool was = in.plasma();
   in.scr = *d;
   in.qs2 = d->enabled ? d->qs2 : 0.0;
+  // the screened rates keep kI, kF per level in plasma mode: a change of
+  // mode needs a new table build
+  if (was != in.plasma())
+    in.has_tables = false;
+  in.dev_scr = false;
   c->params_dirty = true;
   return EMCGPU_OK;
+}
+
+// updateScreening + reinitScatterTables before the first step
+// (hotCarrierMHP.cpp:630-637): the valley-0 energies summed on the host in
+// particle order, as getAvgEnergy does
+emcgpu_status emcgpu_update_screening(emcgpu_ctx *c, int instance) {
+  emcgpu_status st = check_instance(c, instance);
+  if (st)
+    return st;
+  Instance &in = c->inst[instance];
+  if (!in.plasma())
+    return fail(c, EMCGPU_ERR_STATE, "instance %d has no plasma screening", instance);
+  if (c->device < 0)
+    return fail(c, EMCGPU_ERR_STATE, "no device");
+  if (!in.has_tables)
+    return fail(c, EMCGPU_ERR_STATE, "instance %d has no scatter tables", instance);
+  (void)hipSetDevice(c->device);
+  const size_t n = in.count;
+  std::vector<double> E(n);
+  std::vector<uint32_t> vs(n);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (n) {
+    const size_t es = c->precision == EMCGPU_FP32 ? sizeof(float) : sizeof(double);
+    std::vector<unsigned char> raw(n * es);
+    HIPCHK(c, hipMemcpy(raw.data(), (unsigned char *)c->E + (size_t)in.start * es, n * es,
+                        hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(vs.data(), c->vs + in.start, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++)
+      E[i] = c->precision == EMCGPU_FP32 ? (double)((const float *)raw.data())[i]
+                                         : ((const double *)raw.data())[i];
+  }
+  double sum = 0.0;
+  size_t cnt = 0;
+  for (size_t i = 0; i < n; i++)
+    if ((vs[i] >> 16) == 0) {
+      sum += E[i];
+      cnt++;
+    }
+  double qs2 = 0.0;
+  if (n != 0) {
+    const double ns = (double)n / in.scr.volume;
+    const double avg = cnt != 0 ? sum / (double)cnt : 0.0;
+    const double ts = 2.0 * avg * phys::q / (3.0 * phys::kB);
+    if (ts > 0.0)
+      qs2 += ns * phys::q * phys::q / (in.scr.eps_static * phys::eps0 * phys::kB * ts);
+  }
+  in.qs2 = qs2;
+  in.dev_scr = false;
+  if ((st = pull_baths(c, in)))
+    return st;
+  // the next upload lowers the new q_s^2 and rebuilds the tables and the
+  // window sums on the GPU from it
+  c->params_dirty = true;
+  const double dt = c->params_dt > 0 ? c->params_dt : 1e-15;
+  return c->precision == EMCGPU_FP32 ? upload_params<float>(c, dt) : upload_params<double>(c, dt);
 }
 
 emcgpu_status emcgpu_get_screening(const emcgpu_ctx *c0, int instance, double *qs2) {
@@ -1433,6 +1676,10 @@ emcgpu_status emcgpu_get_screening(const emcgpu_ctx *c0, int instance, double *q
     return st;
   if (!qs2)
     return fail(c, EMCGPU_ERR_ARG, "qs2 pointer required");
+  if (c->device >= 0)
+    (void)hipSetDevice(c->device);
+  if ((st = pull_screening(c)))
+    return st;
   *qs2 = c->inst[instance].qs2;
   return EMCGPU_OK;
 }

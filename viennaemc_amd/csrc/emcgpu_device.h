@@ -60,6 +60,7 @@ struct MechDev {
   int final_valley;
   int z;
   int bath1;  // hot-phonon Froehlich: instance-local bath index + 1 (0: none)
+  int qres;   // screened hot-phonon: angle through the bath's q distribution
   R shift;  // energy change of the final state [eV] (IV, Froehlich)
   uint8_t final_sub[kMaxDeg][kMaxZ];
 };
@@ -87,6 +88,8 @@ struct InstDev {
   R bath_dq[kMaxBaths];  // bin width [1/nm]
   long long gbin_off;
   R qs2;  // plasmon screening q_s^2 [1/nm^2]
+  // window sums of each bath (cumW[nb + 1] then cumWN[nb + 1], q in 1/m)
+  const R *bath_win[kMaxBaths];
 };
 
 // constants of one LO phonon bath (emcPhononBath<double>)
@@ -97,6 +100,7 @@ struct BathDev {
   double sum_w;   // sum of q_i^2 over the bins in bin order (getMeanNq weight)
   double ac_e, tau_ac, n_ac_eq;
   double w_r, to_e, tau_to, n_to_eq;
+  long long woff;  // first element of the bath's window sums
 };
 // evolving scalars of a bath
 struct BathVar {
@@ -104,19 +108,26 @@ struct BathVar {
 };
 
 // one (instance, valley) table set rebuilt on the GPU from the baths
-constexpr int kRateConst = 0, kRateHotAb = 1, kRateHotEm = 2;
+constexpr int kRateConst = 0, kRateHotAb = 1, kRateHotEm = 2, kRateScrAb = 3, kRateScrEm = 4;
+// occupation of a screened rate: constant, bath mean, bath window mean
+constexpr int kNConst = 0, kNMean = 1, kNWindow = 2;
 struct RebuildJob {
   int inst, valley, n_tables, n_levels;
   long long cum_off;                // element offset in the [level][table] tables
   int kind[kMaxMech];               // kRate*
   int bath[kMaxMech];               // global bath index (hot kinds)
-  double C[kMaxMech];               // rate prefactor (hot kinds)
-  long long rate_off[kMaxMech];     // const: rates[level]; hot: kI[level], ln[level]
+  double C[kMaxMech];               // rate prefactor (hot and screened kinds)
+  long long rate_off[kMaxMech];     // const: rates[level]; hot: kI[level], ln[level];
+                                    // screened: kI[level], kF[level] (1/m)
+  int nsrc[kMaxMech];               // screened: kN*
+  double nconst[kMaxMech];          // screened, kNConst: Bose occupation
 };
 
 // the baths and rebuild jobs of one instance (one block of k_bath_step)
 struct BathInstDev {
   int bath0, n_baths, job0, n_jobs;
+  int inst, plasma, n_valleys, pad;
+  double scr_eps, scr_vol; // plasma screening
 };
 
 template <typename R>
@@ -138,6 +149,13 @@ struct BathStepArgs {
   double *cum64;
   InstDev<R> *inst;
   double *tau0_64;
+  // screening and window sums
+  double *scr;     // q_s^2 per instance [1/m^2]
+  double *win64;   // window sums, FP64
+  R *win;          // the same in the precision of the kinematics
+  const double *partial;  // step partials [n_obs][n_tiles] of the step just taken
+  int n_obs, n_tiles, tile_size;
+  const long long *inst_start, *inst_count;  // particles of each instance
 };
 
 template <typename R>

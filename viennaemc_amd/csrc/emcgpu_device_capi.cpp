@@ -1089,8 +1089,12 @@ emcgpu_status need_model(emcgpu_ctx *c, double dt_s) {
   if (!c->inst[0].has_tables)
     return fail(c, EMCGPU_ERR_STATE, "instance 0 has no scatter tables");
   for (const auto &m : c->inst[0].mechs)
-    if (m.kind == EMCGPU_MECH_HOT_FROEHLICH_AB || m.kind == EMCGPU_MECH_HOT_FROEHLICH_EM)
+    if (m.kind == EMCGPU_MECH_HOT_FROEHLICH_AB || m.kind == EMCGPU_MECH_HOT_FROEHLICH_EM ||
+        m.kind == EMCGPU_MECH_SCREENED_HOT_FROEHLICH_AB ||
+        m.kind == EMCGPU_MECH_SCREENED_HOT_FROEHLICH_EM)
       return fail(c, EMCGPU_ERR_ARG, "hot-phonon mechanisms are not supported in device runs");
+  if (c->inst[0].plasma())
+    return fail(c, EMCGPU_ERR_ARG, "plasma screening is not supported in device runs");
   return upload_model(c, dt_s);
 }
 

@@ -55,6 +55,7 @@ struct Instance {
   std::vector<BathHost> baths;
   std::vector<std::vector<double>> rate_data;
   std::vector<double> hot_C;
+  std::vector<double> mech_n0;  // screened kinds: Bose occupation of the mode
   bool has_hot = false;
   long long bath_index0 = 0;    // global index of the first bath
   std::vector<long long> bath_off;  // global first bin of each bath
@@ -62,8 +63,10 @@ struct Instance {
   int total_bins = 0;
   bool dev_bath = false;        // the device holds the current bath state
   // plasmon screening of the screened Froehlich kinds: q_s^2 [1/m^2]
-  emcgpu_screening_desc scr{EMCGPU_SCREENING_FIXED, 0, 1.0, 0.0};
+  emcgpu_screening_desc scr{EMCGPU_SCREENING_FIXED, 0, 1.0, 0.0, 0.0};
   double qs2 = 0;
+  bool dev_scr = false;  // the device holds the current q_s^2 (plasma mode)
+  bool plasma() const { return scr.mode == EMCGPU_SCREENING_PLASMA && scr.enabled; }
   std::vector<long long> cum_off;  // per valley: element offset of the device tables
 };
 
@@ -121,6 +124,12 @@ struct emcgpu_ctx {
   int n_groups = 0;
   double *d_rate_arena = nullptr;
   double *d_cum64 = nullptr, *d_tau0_64 = nullptr;
+  // q_s^2 per instance [1/m^2] and the baths' window sums (FP64 and in the
+  // precision of the kinematics): per bath cumW[n_bins + 1], cumWN[n_bins + 1]
+  double *d_scr = nullptr;
+  double *d_win64 = nullptr;
+  void *d_win = nullptr;
+  long long win_elems = 0;
   size_t cum64_bytes = 0;
   // 2-D device simulation on instance 0 (null until emcgpu_device_setup)
   emcgpu::host::DeviceSim *dsim = nullptr;

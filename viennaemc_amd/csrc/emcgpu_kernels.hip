@@ -448,6 +448,7 @@ __global__ __launch_bounds__(256) void k_reduce_final(const double *__restrict__
 // ---------------------------------------------------------------------------
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kQ = 1.60219e-19, kKB = 1.38066e-23;  // include/emcConstants.hpp
+constexpr double kEps0 = 8.85419e-12;
 constexpr int kBathThreads = 1024;
 
 __device__ double bath_planck(double energy_ev, double temp_k) {
@@ -574,9 +575,41 @@ __device__ void bath_update_one(const BathDev &B, BathVar &V, double *__restrict
   __syncthreads();
 }
 
+// getNqInWindow (emcPhononBath.hpp:378-394) over the FP64 window sums
+__device__ __forceinline__ double window_nq(const BathDev &B, const double *__restrict__ win64,
+                                           double mean, double qmin, double qmax) {
+#pragma clang fp contract(off)
+  if (B.n_bins < 2 || qmax <= qmin)
+    return mean;
+  long lo = (long)::floor(qmin / B.dq);
+  long hi = (long)::ceil(qmax / B.dq);
+  if (lo < 0)
+    lo = 0;
+  if (hi > (long)B.n_bins)
+    hi = (long)B.n_bins;
+  if (hi - lo < 1)
+    return mean;
+  const double *cw = win64 + B.woff, *cwn = cw + B.n_bins + 1;
+  const double wsum = cw[hi] - cw[lo];
+  if (wsum <= 0.0)
+    return mean;
+  return (cwn[hi] - cwn[lo]) / wsum;
+}
+
+// screenedFroehlichLogFactor (emcScreenedFroehlichInteraction.hpp:52-62)
+__device__ __forceinline__ double screened_log(double ki, double kf, double qs2) {
+#pragma clang fp contract(off)
+  const double p2 = (ki + kf) * (ki + kf), m2 = (ki - kf) * (ki - kf);
+  if (qs2 <= 0.0)
+    return (m2 <= 0.0) ? 0.0 : 0.5 * ::log(p2 / m2);
+  return 0.5 * ::log((p2 + qs2) / (m2 + qs2));
+}
+
 __device__ __forceinline__ double rebuild_rate(const RebuildJob &J, int j, int l,
                                                const double *__restrict__ arena,
-                                               const BathVar *__restrict__ var) {
+                                               const BathVar *__restrict__ var,
+                                               const BathDev *__restrict__ baths,
+                                               const double *__restrict__ win64, double qs2) {
 #pragma clang fp contract(off)
   const long long o = J.rate_off[j];
   if (J.kind[j] == kRateConst)
@@ -584,23 +617,41 @@ __device__ __forceinline__ double rebuild_rate(const RebuildJob &J, int j, int l
   const double ki = arena[o + l];
   if (!(ki > 0.0))
     return 0.0;
+  if (J.kind[j] == kRateScrAb || J.kind[j] == kRateScrEm) {
+    // emcScreenedFroehlichInteraction.hpp:128-138 / :184-196 (constant N)
+    // and :257-269 / :333-348 (bath mean or window mean)
+    const double kf = arena[o + J.n_levels + l];
+    double n = J.nconst[j];
+    if (J.nsrc[j] == kNMean) {
+      n = var[J.bath[j]].mean;
+    } else if (J.nsrc[j] == kNWindow) {
+      const double d = ki - kf;
+      n = window_nq(baths[J.bath[j]], win64, var[J.bath[j]].mean, (d < 0.0) ? -d : d, ki + kf);
+    }
+    const double lf = screened_log(ki, kf, qs2);
+    return (J.kind[j] == kRateScrAb) ? J.C[j] * n / ki * lf : J.C[j] * (n + 1.0) / ki * lf;
+  }
   const double ln = arena[o + J.n_levels + l];
   const double n = var[J.bath[j]].mean;
   return (J.kind[j] == kRateHotAb) ? J.C[j] * n / ki * ln : J.C[j] * (n + 1.0) / ki * ln;
 }
 
 template <typename R>
-__device__ void rebuild_one(const RebuildJob &J, const double *__restrict__ arena,
-                            const BathVar *__restrict__ var, R *__restrict__ cum,
-                            double *__restrict__ cum64, InstDev<R> *__restrict__ inst,
-                            double *__restrict__ tau0_64, BathShared &sh) {
+__device__ void rebuild_one(const RebuildJob &J, const BathStepArgs<R> &a, BathShared &sh) {
 #pragma clang fp contract(off)
+  const double *__restrict__ arena = a.arena;
+  const BathVar *__restrict__ var = a.var;
+  const BathDev *__restrict__ baths = a.baths;
+  const double *__restrict__ win64 = a.win64;
+  const double qs2 = a.scr ? a.scr[J.inst] : 0.0;
+  R *__restrict__ cum = a.cum;
+  double *__restrict__ cum64 = a.cum64;
   const int nt = J.n_tables, nl = J.n_levels;
   double mx = -1.0;
   for (int l = threadIdx.x; l < nl; l += blockDim.x) {
-    double c = rebuild_rate(J, 0, l, arena, var);
+    double c = rebuild_rate(J, 0, l, arena, var, baths, win64, qs2);
     for (int j = 1; j < nt; ++j)
-      c = rebuild_rate(J, j, l, arena, var) + c;
+      c = rebuild_rate(J, j, l, arena, var, baths, win64, qs2) + c;
     mx = (mx < c) ? c : mx;
   }
   sh.vmax[threadIdx.x] = mx;
@@ -617,7 +668,7 @@ __device__ void rebuild_one(const RebuildJob &J, const double *__restrict__ aren
   for (int l = threadIdx.x; l < nl; l += blockDim.x) {
     double c = 0.0;
     for (int j = 0; j < nt; ++j) {
-      const double r = rebuild_rate(J, j, l, arena, var);
+      const double r = rebuild_rate(J, j, l, arena, var, baths, win64, qs2);
       c = (j == 0) ? r : r + c;
       const double x = c / mx;
       cum[J.cum_off + (long long)l * nt + j] = (R)x;
@@ -626,11 +677,115 @@ __device__ void rebuild_one(const RebuildJob &J, const double *__restrict__ aren
   }
   if (threadIdx.x == 0) {
     const double tau = 1. / mx;
-    inst[J.inst].tau0[J.valley] = (R)(tau * 1e15);
-    tau0_64[J.inst * kMaxV + J.valley] = tau;
+    a.inst[J.inst].tau0[J.valley] = (R)(tau * 1e15);
+    a.tau0_64[J.inst * kMaxV + J.valley] = tau;
   }
 }
 
+// updateScreening (examples/hotCarrierMHP/hotCarrierMHP.cpp:563-581) of one
+// instance from the partials of the step just taken: N over all valleys,
+// the valley-0 energy sum and count, the Debye q_s^2 at the temperature of
+// the valley-0 mean energy (getMBTemp, emcHotCarrierOutput.hpp:89-92)
+template <typename R>
+__device__ void screening_update(const BathInstDev &I, const BathStepArgs<R> &a, BathShared &sh) {
+#pragma clang fp contract(off)
+  double vals[3] = {0.0, 0.0, 0.0};  // N, sum E valley 0, count valley 0
+  // the instance's tiles: it starts on a tile boundary, tiles past its
+  // count hold zeros
+  const long long ts = a.tile_size;
+  const long long tile0 = a.inst_start[I.inst] / ts;
+  const long long tile1 = (a.inst_start[I.inst] + a.inst_count[I.inst] + ts - 1) / ts;
+  for (long long t = tile0 + threadIdx.x; t < tile1 && t < a.n_tiles; t += blockDim.x) {
+    for (int v = 0; v < I.n_valleys; ++v)
+      vals[0] += a.partial[(long long)(2 + 3 * v) * a.n_tiles + t];
+    vals[1] += a.partial[3LL * a.n_tiles + t];
+    vals[2] += a.partial[2LL * a.n_tiles + t];
+  }
+  double tot[3];
+  for (int k = 0; k < 3; ++k) {
+    sh.vmax[threadIdx.x] = vals[k];
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w)
+        sh.vmax[threadIdx.x] = sh.vmax[threadIdx.x] + sh.vmax[threadIdx.x + w];
+      __syncthreads();
+    }
+    tot[k] = sh.vmax[0];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double qs2 = 0.0;
+    if (tot[0] != 0.0) {
+      const double ns = tot[0] / I.scr_vol;
+      const double avg = (tot[2] != 0.0) ? tot[1] / tot[2] : 0.0;
+      const double ts = 2.0 * avg * kQ / (3.0 * kKB);
+      if (ts > 0.0)
+        qs2 += ns * kQ * kQ / (I.scr_eps * kEps0 * kKB * ts);
+    }
+    a.scr[I.inst] = qs2;
+    a.inst[I.inst].qs2 = (R)(qs2 * 1e-18);
+  }
+  __syncthreads();
+}
+
+// rebuildWindowSums (emcPhononBath.hpp:109-124) of one bath with the
+// instance's q_s^2: weights on all lanes, the prefix sums by lane 0 in bin
+// order (FP64, as the reference adds them) and a copy in the precision of
+// the kinematics
+template <typename R>
+__device__ void window_sums_one(const BathDev &B, double qs2, const BathStepArgs<R> &a,
+                                BathShared &sh) {
+#pragma clang fp contract(off)
+  const int nb = B.n_bins;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    const double q = ((double)i + 0.5) * B.dq;
+    const double denom = q * q + qs2;
+    const double w = (denom > 0.0) ? q / denom : 0.0;
+    sh.old_n[i] = w;
+    sh.new_n[i] = w * a.nq[B.off + i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double *cw = a.win64 + B.woff, *cwn = cw + nb + 1;
+    R *rw = a.win + B.woff, *rwn = rw + nb + 1;
+    double s = 0.0, sn = 0.0;
+    cw[0] = 0.0;
+    cwn[0] = 0.0;
+    rw[0] = R(0);
+    rwn[0] = R(0);
+    for (int i = 0; i < nb; ++i) {
+      s = s + sh.old_n[i];
+      sn = sn + sh.new_n[i];
+      cw[i + 1] = s;
+      cwn[i + 1] = sn;
+      rw[i + 1] = (R)s;
+      rwn[i + 1] = (R)sn;
+    }
+  }
+  __syncthreads();
+}
+
+// the instance's screening, its baths' window sums, then its tables
+template <typename R>
+__device__ void refresh_group(const BathInstDev &I, const BathStepArgs<R> &a, bool plasma,
+                              bool rebuild, BathShared &sh) {
+  if (plasma && I.plasma)
+    screening_update(I, a, sh);
+  if (a.win64) {
+    const double qs2 = a.scr ? a.scr[I.inst] : 0.0;
+    for (int b = I.bath0; b < I.bath0 + I.n_baths; ++b)
+      window_sums_one(a.baths[b], qs2, a, sh);
+  }
+  if (!rebuild)
+    return;
+  for (int j = I.job0; j < I.job0 + I.n_jobs; ++j)
+    rebuild_one<R>(a.jobs[j], a, sh);
+}
+
+// One block per instance with baths or plasma screening: the baths' update,
+// the screening from the step's partials, the window sums, the tables (the
+// order of hotCarrierMHP.cpp:693-703).  Window sums and screening follow
+// the values the reference's setScreeningQ2 / update leave behind.
 template <typename R>
 __global__ __launch_bounds__(kBathThreads) void k_bath_step(BathStepArgs<R> a) {
   __shared__ BathShared sh;
@@ -638,17 +793,15 @@ __global__ __launch_bounds__(kBathThreads) void k_bath_step(BathStepArgs<R> a) {
   for (int b = I.bath0; b < I.bath0 + I.n_baths; ++b)
     bath_update_one(a.baths[b], a.var[b], a.nq, a.tau_prof, a.tally, a.tally_bins, a.dt,
                     a.hist ? a.hist + (long long)b * a.hist_steps + a.hist_step : nullptr, sh);
-  if (!a.rebuild)
-    return;
-  for (int j = I.job0; j < I.job0 + I.n_jobs; ++j)
-    rebuild_one<R>(a.jobs[j], a.arena, a.var, a.cum, a.cum64, a.inst, a.tau0_64, sh);
+  refresh_group<R>(I, a, a.partial != nullptr, a.rebuild != 0, sh);
 }
 
-// table rebuild alone (after the model, the field or a bath state changed)
+// window sums and tables alone (after the model, the field, a bath state or
+// the screening changed)
 template <typename R>
-__global__ __launch_bounds__(kBathThreads) void k_rebuild_only(BathStepArgs<R> a) {
+__global__ __launch_bounds__(kBathThreads) void k_bath_refresh(BathStepArgs<R> a) {
   __shared__ BathShared sh;
-  rebuild_one<R>(a.jobs[blockIdx.x], a.arena, a.var, a.cum, a.cum64, a.inst, a.tau0_64, sh);
+  refresh_group<R>(a.groups[blockIdx.x], a, false, true, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -770,7 +923,7 @@ template <typename R>
 hipError_t launch_rebuild(const BathStepArgs<R> &a, int n_jobs, hipStream_t st) {
   if (n_jobs <= 0)
     return hipSuccess;
-  hipLaunchKernelGGL((k_rebuild_only<R>), dim3(n_jobs), dim3(kBathThreads), 0, st, a);
+  hipLaunchKernelGGL((k_bath_refresh<R>), dim3(n_jobs), dim3(kBathThreads), 0, st, a);
   return hipGetLastError();
 }
 

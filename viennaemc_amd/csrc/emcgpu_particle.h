@@ -25,6 +25,7 @@ struct Math<double> {
   static __device__ __forceinline__ double div(double a, double b) { return a / b; }
   static __device__ __forceinline__ double pow(double a, double b) { return ::pow(a, b); }
   static __device__ __forceinline__ double floor(double x) { return ::floor(x); }
+  static __device__ __forceinline__ double ceil(double x) { return ::ceil(x); }
   static __device__ __forceinline__ double round(double x) { return ::round(x); }
   // sin / cos of 2*pi*u, evaluated as the reference does (phi = 2*pi*u)
   static __device__ __forceinline__ void sincos2pi(double u, double *s, double *c) {
@@ -47,6 +48,7 @@ struct Math<float> {
     return __builtin_amdgcn_exp2f(b * __builtin_amdgcn_logf(a));
   }
   static __device__ __forceinline__ float floor(float x) { return __builtin_floorf(x); }
+  static __device__ __forceinline__ float ceil(float x) { return __builtin_ceilf(x); }
   static __device__ __forceinline__ float round(float x) { return __builtin_roundf(x); }
   // v_sin_f32 / v_cos_f32 take their argument in revolutions
   static __device__ __forceinline__ void sincos2pi(float u, float *s, float *c) {
@@ -275,6 +277,59 @@ __device__ __forceinline__ R screened_cos_theta(R ki, R kf, R qs2, R u) {
   return (R(-1) < c) ? c : R(-1); // std::max(-1, c)
 }
 
+// sampleQ (emcPhononBath.hpp:423-458) over a bath's window sums w (cumW at
+// w[0..nb], cumWN at w[nb+1..2nb+1]; their unit cancels): the bins
+// [floor(qmin/dq), ceil(qmax/dq)) clamped to the grid, the bin whose prefix
+// sum brackets sLo + u (sHi - sLo) found by bisection, then linear
+// interpolation inside it; f = N + 1 for emission, N for absorption.  Every
+// index stays in [0, nb].
+template <typename R>
+__device__ __forceinline__ R sample_q(const R *__restrict__ w, int nb, R dq, R qmin, R qmax,
+                                      bool emission, R u) {
+  if (nb < 2 || !(qmax > qmin))
+    return qmin;
+  const R flo = Math<R>::floor(Math<R>::div(qmin, dq));
+  const R fhi = Math<R>::ceil(Math<R>::div(qmax, dq));
+  const int lo = (flo > R(0)) ? ((flo < (R)nb) ? (int)flo : nb) : 0;
+  const int hi = (fhi < (R)nb) ? ((fhi > R(0)) ? (int)fhi : 0) : nb;
+  if (hi - lo < 1)
+    return qmin;
+  const R *cw = w, *cwn = w + nb + 1;
+  auto S = [&](int i) { return emission ? cwn[i] + cw[i] : cwn[i]; };
+  const R slo = S(lo), shi = S(hi);
+  const R span = shi - slo;
+  if (!(span > R(0)))
+    return R(0.5) * (qmin + qmax);
+  const R target = slo + u * span;
+  int a = lo, b = hi;
+  while (b - a > 1) {
+    const int mid = (a + b) / 2;
+    if (S(mid) <= target)
+      a = mid;
+    else
+      b = mid;
+  }
+  const R sa = S(a), sb = S(a + 1);
+  const R frac = (sb > sa) ? Math<R>::div(target - sa, sb - sa) : R(0.5);
+  const R q = ((R)a + frac) * dq;
+  const R q1 = (q < qmax) ? q : qmax;  // std::min(qMax, q)
+  return (qmin < q1) ? q1 : qmin;      // std::max(qMin, .)
+}
+
+// sampleQResolvedCosTheta (emcScreenedFroehlichInteraction.hpp:87-97)
+template <typename R>
+__device__ __forceinline__ R qres_cos_theta(const R *__restrict__ w, int nb, R dq, R ki, R kf,
+                                            bool emission, R u) {
+  const R B = R(2) * ki * kf;
+  if (!(B > R(0)))
+    return R(1) - R(2) * u;
+  const R d = ki - kf;
+  const R q = sample_q(w, nb, dq, (d < R(0)) ? -d : d, ki + kf, emission, u);
+  R c = Math<R>::div(ki * ki + kf * kf - q * q, B);
+  c = (c < R(1)) ? c : R(1);
+  return (R(-1) < c) ? c : R(-1);
+}
+
 // final state of one mechanism: ua/ub are the two final-state uniforms
 // (phi / cos-theta for isotropic kinds, exponent / phi for the Froehlich
 // kinds)
@@ -305,7 +360,15 @@ __device__ __forceinline__ void final_state(const InstDev<R> *P, const MechDev<R
       const R ki = normk_of(P->valley[v], E);
       E += m.shift;
       kn = normk_of(P->valley[v], E);
-      ct = screened_cos_theta(ki, kn, P->qs2, ua);
+      if (m.qres) {
+        // the screened hot-phonon kinds with q-resolved angle (:283-290 /
+        // :358-365): q through the bath's distribution
+        const int b = m.bath1 - 1;
+        ct = qres_cos_theta(P->bath_win[b], P->bath_nb[b], P->bath_dq[b], ki, kn,
+                            m.shift < R(0), ua);
+      } else {
+        ct = screened_cos_theta(ki, kn, P->qs2, ua);
+      }
     } else {
       const R ei = E;
       E += m.shift;

@@ -20,8 +20,8 @@ PRODUCTION, REPLAY = 0, 1
 PARA_ISO, NONPARA_ISO, PARA_ANISO, NONPARA_ANISO = 0, 1, 2, 3
 (ACOUSTIC, ZERO_IV_AB, ZERO_IV_EM, FIRST_IV_AB, FIRST_IV_EM, FROEHLICH_AB,
  FROEHLICH_EM, HOT_FROEHLICH_AB, HOT_FROEHLICH_EM, SCREENED_FROEHLICH_AB,
- SCREENED_FROEHLICH_EM) = range(11)
-SCREENING_FIXED = 0
+ SCREENED_FROEHLICH_EM, SCREENED_HOT_FROEHLICH_AB, SCREENED_HOT_FROEHLICH_EM) = range(13)
+SCREENING_FIXED, SCREENING_PLASMA = 0, 1
 
 PM_NGP, PM_CIC, PM_NEC = 0, 1, 2
 XMIN, XMAX, YMIN, YMAX = 0, 1, 2, 3
@@ -55,7 +55,8 @@ class MechDesc(C.Structure):
     _fields_ = [("kind", C.c_int), ("valley", C.c_int), ("final_valley", C.c_int),
                 ("phonon_energy", C.c_double), ("coupling", C.c_double),
                 ("rel_eff_mass", C.c_double), ("eps_hi", C.c_double), ("eps_lo", C.c_double),
-                ("z", C.c_int), ("final_sub", (C.c_int * MAX_Z) * MAX_DEG), ("bath", C.c_int)]
+                ("z", C.c_int), ("final_sub", (C.c_int * MAX_Z) * MAX_DEG), ("bath", C.c_int),
+                ("q_resolved", C.c_int), ("q_resolved_angle", C.c_int)]
 
 
 class BathDesc(C.Structure):
@@ -68,7 +69,7 @@ class BathDesc(C.Structure):
 
 class ScreeningDesc(C.Structure):
     _fields_ = [("mode", C.c_int), ("enabled", C.c_int), ("eps_static", C.c_double),
-                ("qs2", C.c_double)]
+                ("qs2", C.c_double), ("volume", C.c_double)]
 
 
 class Material(C.Structure):
@@ -149,6 +150,7 @@ EXPORTS = [
     "emcgpu_device_set_particles", "emcgpu_device_get_particles", "emcgpu_device_set_efield",
     "emcgpu_device_drift", "emcgpu_device_init", "emcgpu_device_step", "emcgpu_device_fields",
     "emcgpu_autotune", "emcgpu_device_averages", "emcgpu_set_screening", "emcgpu_get_screening",
+    "emcgpu_update_screening",
 ]
 
 
@@ -184,6 +186,7 @@ def load(path=LIB_PATH):
         "emcgpu_set_baths": (i, [vp, i, p(BathDesc), i]),
         "emcgpu_set_screening": (i, [vp, i, p(ScreeningDesc)]),
         "emcgpu_get_screening": (i, [vp, i, p(C.c_double)]),
+        "emcgpu_update_screening": (i, [vp, i]),
         "emcgpu_get_bath": (i, [vp, i, i, vp, vp]),
         "emcgpu_set_bath_state": (i, [vp, i, i, vp, d, d]),
         "emcgpu_get_bath_tallies": (i, [vp, i, i, vp, vp]),
@@ -257,6 +260,8 @@ def mech_desc(m):
         d.eps_hi = m["eps_hi"]
         d.eps_lo = m["eps_lo"]
         d.bath = m.get("bath", 0)
+        d.q_resolved = int(bool(m.get("q_resolved", False)))
+        d.q_resolved_angle = int(bool(m.get("q_resolved_angle", True)))
     return d
 
 
@@ -378,9 +383,14 @@ class Context:
         self._chk(self.L.emcgpu_get_bath_history(self.h, instance, bath, _ptr(out), C.byref(n)))
         return out
 
-    def set_screening(self, instance, qs2, enabled=True, eps_static=1.0, mode=SCREENING_FIXED):
-        d = ScreeningDesc(mode, 1 if enabled else 0, eps_static, qs2)
+    def set_screening(self, instance, qs2, enabled=True, eps_static=1.0, mode=SCREENING_FIXED,
+                      volume=0.0):
+        d = ScreeningDesc(mode, 1 if enabled else 0, eps_static, qs2, volume)
         self._chk(self.L.emcgpu_set_screening(self.h, instance, C.byref(d)))
+
+    def update_screening(self, instance=0):
+        """Plasma screening: q_s^2 from the current particles, tables rebuilt."""
+        self._chk(self.L.emcgpu_update_screening(self.h, instance))
 
     def get_screening(self, instance=0):
         v = C.c_double()
@@ -394,8 +404,13 @@ class Context:
         self.set_model(instance, spec)
         if spec.get("baths"):
             self.set_baths(instance, spec["baths"], T)
-        if spec.get("screening"):
-            self.set_screening(instance, spec["screening"]["qs2"])
+        scr = spec.get("screening")
+        if scr:
+            if scr.get("plasma"):
+                self.set_screening(instance, scr.get("qs2", 0.0), eps_static=scr["eps"],
+                                   mode=SCREENING_PLASMA, volume=scr["volume"])
+            else:
+                self.set_screening(instance, scr["qs2"])
         self.build_tables(instance, spec["n_levels"], spec["max_energy"], T,
                           rho=spec.get("rho", 2329.0), v_sound=spec.get("v_sound", 9040.0))
         self.set_field(instance, spec["field_dir"], spec["field_kvcm"] * 1e5)
