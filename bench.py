@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--batched-steps", type=int, default=2000)
     ap.add_argument("--no-hot", action="store_true", help="skip the hot-phonon cooling leg")
     ap.add_argument("--hot-steps", type=int, default=2000)
+    ap.add_argument("--no-hot-plain", action="store_true",
+                    help="run only the screened (q-resolved, plasma-screened) hot-phonon leg")
     ap.add_argument("--hot-particles", type=int, default=4_000_000)
     ap.add_argument("--no-device", action="store_true", help="skip the 2-D resistor leg")
     ap.add_argument("--device-only", action="store_true",
@@ -145,16 +147,21 @@ def run_batched(emcgpu, warmup, steps, seed=20261202):
                         "T U[77,500] K, N log-U[1e4,2e6], FP32, HIP-graph launch, autotuned shape"}
 
 
-def run_hot_phonon(emcgpu, n, warmup, steps, precision, seed=20261202):
+def run_hot_phonon(emcgpu, n, warmup, steps, precision, seed=20261202, screened=False):
     """Photoexcited cooling with a hot-phonon bath: every step is the
     particle step with LDS bath tallies, the bath update and the table
-    rebuild (one HIP graph per call); whole-job particle-updates/s."""
-    from emc_models import photoexcited_cooling
-    spec = photoexcited_cooling(n)
+    rebuild (one HIP graph per call); whole-job particle-updates/s.
+    screened: the hot-carrier default of the reference (screened hot-phonon
+    Froehlich, q-resolved rate and angle, plasma screening refreshed after
+    every step from the step's tile partials)."""
+    from emc_models import photoexcited_cooling, screened_cooling
+    spec = screened_cooling(n) if screened else photoexcited_cooling(n)
     ctx = emcgpu.Context(precision=precision)
     ctx.setup(0, spec)
     ctx.resize([n])
     ctx.init_particles_energy(0, seed, 0.5, 0.05)
+    if screened:
+        ctx.update_screening(0)
     dt = 0.5e-15
     if warmup:
         ctx.step(dt, warmup, seed=seed, step0=0, graph=True)
@@ -166,10 +173,20 @@ def run_hot_phonon(emcgpu, n, warmup, steps, precision, seed=20261202):
     av = ctx.averages(0)
     hist = ctx.bath_history(0, 0)
     bath = ctx.get_bath(0, 0)
+    qs2 = ctx.get_screening(0)
     ctx.timing(reset=True)
     ctx.step(dt, 64, seed=seed, step0=warmup + steps, timing=True)
     ms, launches = ctx.timing()
     ctx.close()
+    if screened:
+        return {"updates_per_s": n * steps / wall, "particles": n, "steps": steps,
+                "ms_per_step": wall / steps * 1e3, "step_kernel_ms": ms / max(launches, 1),
+                "meanE_first": float(av["avgE"][0, 0]), "meanE_last": float(av["avgE"][-1, 0]),
+                "meanNq_last": float(hist[-1]), "qs2_last": float(qs2),
+                "workload": "screened photoexcited cooling, %d e- N(0.5, 0.05) eV, screened hot-phonon "
+                            "Froehlich with q-resolved rate and angle on the 256-bin bath, plasma "
+                            "screening (eps 3.5) and window sums refreshed every step, dt 0.5 fs, "
+                            "%s" % (n, "FP32" if precision else "FP64")}
     return {"updates_per_s": n * steps / wall, "particles": n, "steps": steps,
             "ms_per_step": wall / steps * 1e3, "step_kernel_ms": ms / max(launches, 1),
             "meanE_first": float(av["avgE"][0, 0]), "meanE_last": float(av["avgE"][-1, 0]),
@@ -300,11 +317,14 @@ def main():
     if not args.no_batched:
         out["batched_sweep"] = run_batched(emcgpu, args.warmup, args.batched_steps)
     if not args.no_hot:
-        out["hot_phonon"] = run_hot_phonon(emcgpu, args.hot_particles, args.warmup,
-                                           args.hot_steps, emcgpu.FP32)
-        if not args.no_fp64:
-            out["hot_phonon_fp64"] = run_hot_phonon(emcgpu, args.hot_particles, args.warmup,
-                                                    args.hot_steps, emcgpu.FP64)
+        if not args.no_hot_plain:
+            out["hot_phonon"] = run_hot_phonon(emcgpu, args.hot_particles, args.warmup,
+                                               args.hot_steps, emcgpu.FP32)
+            if not args.no_fp64:
+                out["hot_phonon_fp64"] = run_hot_phonon(emcgpu, args.hot_particles, args.warmup,
+                                                        args.hot_steps, emcgpu.FP64)
+        out["hot_phonon_screened"] = run_hot_phonon(emcgpu, args.hot_particles, args.warmup,
+                                                    args.hot_steps, emcgpu.FP32, screened=True)
     if not args.no_device:
         out["device_resistor"] = run_device(emcgpu, args.device_particles, args.warmup,
                                             args.device_steps, emcgpu.FP32)

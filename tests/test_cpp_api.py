@@ -98,3 +98,26 @@ def test_result_files_match_the_reference_format(output_binary, golden, tmp_path
         _, t_sim, dt, t_tr, _, _ = golden["sim.cfg"]
         args += ["2", repr(float(dt)), repr(float(t_tr))]
     assert subprocess.check_output(args) == text
+
+
+def test_screened_mechanisms_match_reference(tmp_path, golden):
+    """emcScreenedFroehlich{Absorption,Emission}3D and emcPlasmonScreening in
+    the host API: rates bit-exact against the reference's screened_polar
+    tables, q_s^2 handed to the C ABI, the Debye update and the disabled
+    screening of emcPlasmonScreening.hpp:69-82."""
+    exe = str(tmp_path / "host_api_screened")
+    subprocess.check_call(["g++", "-std=c++17", "-O2", "-I", os.path.join(PKG, "include"),
+                           os.path.join(ROOT, "tests", "cpp", "host_api_screened.cpp"), "-o", exe,
+                           "-L", PKG, "-lemcgpu", "-Wl,-rpath," + PKG])
+    txt = subprocess.check_output([exe]).decode().splitlines()
+    rates = np.array([float(l.split()[1]) for l in txt if l.startswith("rate ")])
+    ref = golden["screened_polar.rate_table"]
+    assert rates.shape == ref.shape and (rates == ref).all()
+    val = {l.split()[0]: l.split()[1:] for l in txt if not l.startswith("rate ")}
+    assert float(val["ctx_qs2"][0]) == 1e17
+    assert [int(x) for x in val["kinds"]] == [9, 10]
+    Q, EPS0, KB = 1.60219e-19, 8.85419e-12, 1.38066e-23
+    qs2 = 1e24 * Q * Q / (10.0 * EPS0 * KB * 600.0)
+    assert abs(float(val["debye"][0]) - qs2) <= 1e-15 * qs2
+    assert abs(float(val["debye"][1]) - qs2 ** -0.5) <= 1e-15 * qs2 ** -0.5
+    assert float(val["off"][0]) == 0.0

@@ -428,6 +428,8 @@ emcgpu_status upload_rebuild(emcgpu_ctx *c, size_t cum_elems) {
     g.n_valleys = (int)in.valleys.size();
     g.scr_eps = in.scr.eps_static;
     g.scr_vol = in.scr.volume;
+    for (const auto &m : in.mechs)
+      g.need_win |= (is_scr_hot_kind(m.kind) && m.q_resolved) ? 1 : 0;
     for (size_t v = 0; in.has_tables && v < in.valleys.size(); v++) {
       const auto &tm = in.table_mech[v];
       if (!dyn_valley(in, v))

@@ -126,7 +126,8 @@ struct RebuildJob {
 // the baths and rebuild jobs of one instance (one block of k_bath_step)
 struct BathInstDev {
   int bath0, n_baths, job0, n_jobs;
-  int inst, plasma, n_valleys, pad;
+  int inst, plasma, n_valleys;
+  int need_win;  // a q-resolved mechanism reads the baths' window sums
   double scr_eps, scr_vol; // plasma screening
 };
 

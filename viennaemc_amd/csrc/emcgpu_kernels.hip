@@ -463,6 +463,8 @@ __device__ double bath_planck_temp(double energy_ev, double n) {
   return kQ * energy_ev / (kKB * ::log(1.0 + 1.0 / n));
 }
 
+static_assert(kMaxBins <= kBathThreads, "one lane per bin in the bath update");
+
 struct BathShared {
   double old_n[kMaxBins];
   double new_n[kMaxBins];
@@ -495,23 +497,48 @@ __device__ void bath_update_one(const BathDev &B, BathVar &V, double *__restrict
     }
     sh.target = target;
   }
-  __syncthreads();
-  const double target = sh.target;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+  // the event counts of this step: the sum of the tally copies, which are
+  // cleared.  G = blockDim / nb lane groups share the copies of a bin
+  // (copies g, g + G, ...), their integer partial sums meet in LDS (exact as
+  // doubles: far below 2^53)
+  const int G = (nb > 0 && (int)blockDim.x >= nb) ? (int)blockDim.x / nb : 1;
+  if ((int)threadIdx.x < G * nb) {
+    const int i = (int)threadIdx.x % nb, g = (int)threadIdx.x / nb;
     const long long gi = B.off + i;
-    const double q = ((double)i + 0.5) * B.dq;
-    const double dph = q * q * B.dq * B.vsim / (2.0 * kPi * kPi);
-    // the event counts of this step: sum of the copies, which are cleared
     unsigned long long c_em = 0ull, c_abs = 0ull;
 #pragma unroll 8
-    for (int r = 0; r < kTallyCopies; ++r) {
+    for (int r = g; r < kTallyCopies; r += G) {
       unsigned long long *t = tally + 2 * ((long long)r * tally_bins + gi);
       c_em += t[0];
       c_abs += t[1];
       t[0] = 0ull;
       t[1] = 0ull;
     }
-    const double n_em = (double)c_em, n_abs = (double)c_abs;
+    sh.vmax[threadIdx.x] = (double)c_em;
+    sh.old_n[threadIdx.x] = (double)c_abs;
+  }
+  __syncthreads();
+  double cnt_em[kMaxBins / kBathThreads + 1], cnt_abs[kMaxBins / kBathThreads + 1];
+  {
+    int u = 0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x, ++u) {
+      double e = 0.0, b = 0.0;
+      for (int g = 0; g < G; ++g) {
+        e = e + sh.vmax[g * nb + i];
+        b = b + sh.old_n[g * nb + i];
+      }
+      cnt_em[u] = e;
+      cnt_abs[u] = b;
+    }
+  }
+  __syncthreads();
+  const double target = sh.target;
+  int u = 0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x, ++u) {
+    const long long gi = B.off + i;
+    const double q = ((double)i + 0.5) * B.dq;
+    const double dph = q * q * B.dq * B.vsim / (2.0 * kPi * kPi);
+    const double n_em = cnt_em[u], n_abs = cnt_abs[u];
     const double g = (dph > 0.0) ? (n_em - n_abs) / (dph * dt) : 0.0;
     const double tau_i = B.has_profile ? tau_prof[gi] : B.tau_lo;
     double n = nq[gi];
@@ -527,14 +554,23 @@ __device__ void bath_update_one(const BathDev &B, BathVar &V, double *__restrict
   if (threadIdx.x == 0) {
     if (B.has_ac) {
       // reservoirs from the start-of-step occupations (:291-355)
+      // sums in bin order (the reference's), the LDS loads batched ahead of
+      // the dependent adds
       double sum_w = 0.0, sum_wdn = 0.0, sum_wdn_tau = 0.0;
-      for (int i = 0; i < nb; ++i) {
-        const double w = sh.w[i];
-        sum_w += w;
-        if (B.has_profile)
+      if (B.has_profile) {
+#pragma unroll 16
+        for (int i = 0; i < nb; ++i) {
+          const double w = sh.w[i];
+          sum_w += w;
           sum_wdn_tau += w * (sh.old_n[i] - target) / tau_prof[B.off + i];
-        else
+        }
+      } else {
+#pragma unroll 16
+        for (int i = 0; i < nb; ++i) {
+          const double w = sh.w[i];
+          sum_w += w;
           sum_wdn += w * (sh.old_n[i] - target);
+        }
       }
       const double f_klemens = B.has_ridley ? (1.0 - B.w_r) : 1.0;
       const double src = B.has_profile ? ((sum_w > 0.0) ? sum_wdn_tau / sum_w : 0.0)
@@ -565,6 +601,7 @@ __device__ void bath_update_one(const BathDev &B, BathVar &V, double *__restrict
     // getMeanNq over the updated occupations (sum of the weights is a
     // constant of the bath, summed in the same order on the host)
     double sum_wn = 0.0;
+#pragma unroll 16
     for (int i = 0; i < nb; ++i)
       sum_wn += sh.w[i] * sh.new_n[i];
     const double mean = (B.sum_w > 0.0) ? sum_wn / B.sum_w : B.n0;
@@ -647,11 +684,16 @@ __device__ void rebuild_one(const RebuildJob &J, const BathStepArgs<R> &a, BathS
   R *__restrict__ cum = a.cum;
   double *__restrict__ cum64 = a.cum64;
   const int nt = J.n_tables, nl = J.n_levels;
+  // pass 1: the cumulative rates of each level, kept unnormalised in the
+  // FP64 copy by the lane that normalises them in pass 2
   double mx = -1.0;
   for (int l = threadIdx.x; l < nl; l += blockDim.x) {
-    double c = rebuild_rate(J, 0, l, arena, var, baths, win64, qs2);
-    for (int j = 1; j < nt; ++j)
-      c = rebuild_rate(J, j, l, arena, var, baths, win64, qs2) + c;
+    double c = 0.0;
+    for (int j = 0; j < nt; ++j) {
+      const double r = rebuild_rate(J, j, l, arena, var, baths, win64, qs2);
+      c = (j == 0) ? r : r + c;
+      cum64[J.cum_off + (long long)l * nt + j] = c;
+    }
     mx = (mx < c) ? c : mx;
   }
   sh.vmax[threadIdx.x] = mx;
@@ -664,15 +706,13 @@ __device__ void rebuild_one(const RebuildJob &J, const BathStepArgs<R> &a, BathS
   mx = sh.vmax[0];
   __syncthreads();
   if (!(mx > 0.0))
-    return;  // no rate at any level: keep the previous tables
+    return;  // no rate at any level: the kinematics keep the previous tables
   for (int l = threadIdx.x; l < nl; l += blockDim.x) {
-    double c = 0.0;
     for (int j = 0; j < nt; ++j) {
-      const double r = rebuild_rate(J, j, l, arena, var, baths, win64, qs2);
-      c = (j == 0) ? r : r + c;
-      const double x = c / mx;
-      cum[J.cum_off + (long long)l * nt + j] = (R)x;
-      cum64[J.cum_off + (long long)l * nt + j] = x;
+      const long long e = J.cum_off + (long long)l * nt + j;
+      const double x = cum64[e] / mx;
+      cum[e] = (R)x;
+      cum64[e] = x;
     }
   }
   if (threadIdx.x == 0) {
@@ -753,6 +793,7 @@ __device__ void window_sums_one(const BathDev &B, double qs2, const BathStepArgs
     cwn[0] = 0.0;
     rw[0] = R(0);
     rwn[0] = R(0);
+#pragma unroll 16
     for (int i = 0; i < nb; ++i) {
       s = s + sh.old_n[i];
       sn = sn + sh.new_n[i];
@@ -771,7 +812,7 @@ __device__ void refresh_group(const BathInstDev &I, const BathStepArgs<R> &a, bo
                               bool rebuild, BathShared &sh) {
   if (plasma && I.plasma)
     screening_update(I, a, sh);
-  if (a.win64) {
+  if (a.win64 && I.need_win) {
     const double qs2 = a.scr ? a.scr[I.inst] : 0.0;
     for (int b = I.bath0; b < I.bath0 + I.n_baths; ++b)
       window_sums_one(a.baths[b], qs2, a, sh);

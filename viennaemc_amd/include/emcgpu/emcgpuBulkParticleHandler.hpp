@@ -167,6 +167,26 @@ public:
       d.push_back(m->finalState());
     return d;
   }
+  /// the screening the screened mechanisms read (the first one found; they
+  /// share one emcPlasmonScreening as in the reference), or null
+  const emcPlasmonScreening<T> *screening() const {
+    for (auto &m : mechs)
+      if (auto *s = dynamic_cast<const emcScreenedMechanism<T> *>(m.get()))
+        return s->getScreening();
+    return nullptr;
+  }
+  /// its state lowered to the C ABI (a fixed q_s^2)
+  bool screeningDescriptor(emcgpu_screening_desc &d) const {
+    const emcPlasmonScreening<T> *s = screening();
+    if (!s)
+      return false;
+    d = emcgpu_screening_desc{};
+    d.mode = EMCGPU_SCREENING_FIXED;
+    d.enabled = s->isEnabled() ? 1 : 0;
+    d.eps_static = s->getEpsStatic();
+    d.qs2 = s->getQs2();
+    return true;
+  }
 
 private:
   SizeType nLevels;
@@ -198,10 +218,7 @@ public:
     auto md = type.mechanismDescriptors();
     check(emcgpu_set_model(ctx, 0, vd.data(), (int)vd.size(), md.data(), (int)md.size()), ctx,
           "emcgpu_set_model");
-    auto rates = type.rawRates();
-    check(emcgpu_set_rate_tables(ctx, 0, (int)type.getNrEnergyLevels(), type.getMaxEnergy(),
-                                 rates.data()),
-          ctx, "emcgpu_set_rate_tables");
+    reinitScatterTables();
     resetAppliedFieldStrength(fieldStrength);
     size_t n = nrParticles;
     check(emcgpu_resize(ctx, &n), ctx, "emcgpu_resize");
@@ -210,6 +227,18 @@ public:
   emcgpuBulkParticleHandler(const emcgpuBulkParticleHandler &) = delete;
   emcgpuBulkParticleHandler &operator=(const emcgpuBulkParticleHandler &) = delete;
 
+  /// reinitScatterTables (emcParticleType.hpp / emcScatterHandler.hpp:
+  /// 100-108): the tables from the mechanisms' current rates, with the
+  /// current q_s^2 of a shared emcPlasmonScreening for the GPU final state
+  void reinitScatterTables() {
+    emcgpu_screening_desc sd;
+    if (ptype.screeningDescriptor(sd))
+      check(emcgpu_set_screening(ctx, 0, &sd), ctx, "emcgpu_set_screening");
+    auto rates = ptype.rawRates();
+    check(emcgpu_set_rate_tables(ctx, 0, (int)ptype.getNrEnergyLevels(), ptype.getMaxEnergy(),
+                                 rates.data()),
+          ctx, "emcgpu_set_rate_tables");
+  }
   void setSeed(SizeType inSeed) { seed = inSeed; }
   void resetAppliedFieldStrength(T strength) {
     const double d[3] = {dir[0], dir[1], dir[2]};

@@ -14,6 +14,8 @@
 #ifndef EMCGPU_MECHANISMS_HPP
 #define EMCGPU_MECHANISMS_HPP
 
+#include <cmath>
+#include <limits>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -200,6 +202,89 @@ template <class T>
 using emcFroehlichAbsorption3D = emcFroehlich3D<T, true>;
 template <class T>
 using emcFroehlichEmission3D = emcFroehlich3D<T, false>;
+
+/// emcPlasmonScreening.hpp:52-90: the q_s^2 that screened mechanisms share
+/// (Debye form q_s^2 = n q^2 / (eps_s eps0 kB T_e); 0 when disabled)
+template <class T>
+class emcPlasmonScreening {
+  T qs2 = T(0);
+  T epsStatic;
+  bool enabled;
+
+public:
+  emcPlasmonScreening() = delete;
+  emcPlasmonScreening(T inEpsStatic, bool inEnabled = true)
+      : epsStatic(inEpsStatic), enabled(inEnabled) {}
+  /// refresh from the carrier density [1/m^3] and temperature [K]
+  void update(T density, T carrierTemp) {
+    if (!enabled || density <= T(0) || carrierTemp <= T(0)) {
+      qs2 = T(0);
+      return;
+    }
+    qs2 = density * phys::q * phys::q / (epsStatic * phys::eps0 * phys::kB * carrierTemp);
+  }
+  void setQs2(T inQs2) { qs2 = enabled ? inQs2 : T(0); }
+  T getQs2() const { return qs2; }
+  T getQs() const { return std::sqrt(qs2); }
+  T getScreeningLength() const {
+    return (qs2 > T(0)) ? T(1) / std::sqrt(qs2) : std::numeric_limits<T>::infinity();
+  }
+  bool isEnabled() const { return enabled; }
+  T getEpsStatic() const { return epsStatic; }
+};
+
+/// a mechanism that reads a shared emcPlasmonScreening; the particle type
+/// hands its q_s^2 to the GPU final state (emcgpu_set_screening)
+template <class T>
+struct emcScreenedMechanism {
+  virtual ~emcScreenedMechanism() = default;
+  virtual const emcPlasmonScreening<T> *getScreening() const = 0;
+};
+
+/// emcScreenedFroehlichInteraction.hpp:101-213 (equilibrium Bose
+/// occupation): the Froehlich rate with the screened log factor
+/// 1/2 ln((q+^2 + q_s^2) / (q-^2 + q_s^2)) and the screened angle on the GPU
+template <class T, bool Absorption>
+class emcScreenedFroehlich3D : public emcScatterMechanism<T>, public emcScreenedMechanism<T> {
+  T phononEnergy, effMass, scatterConst, nBose;
+  std::shared_ptr<emcPlasmonScreening<T>> screening;
+  std::string nameSuffix;
+
+public:
+  emcScreenedFroehlich3D(SizeType inValley, T inPhononEnergy, T relEffMass, T epsHi, T epsLo,
+                         T temperature, std::shared_ptr<emcPlasmonScreening<T>> inScreening,
+                         std::string inNameSuffix = "")
+      : emcScatterMechanism<T>(inValley), phononEnergy(inPhononEnergy),
+        effMass(relEffMass * phys::me), screening(std::move(inScreening)),
+        nameSuffix(std::move(inNameSuffix)) {
+    if (!screening)
+      throw std::invalid_argument("screened Froehlich mechanism without a screening object");
+    scatterConst = phys::froehlichConst(phononEnergy, effMass, epsHi, epsLo);
+    nBose = phys::boseEinstein(phononEnergy, temperature);
+    this->desc_.kind = Absorption ? EMCGPU_MECH_SCREENED_FROEHLICH_AB
+                                  : EMCGPU_MECH_SCREENED_FROEHLICH_EM;
+    this->desc_.phonon_energy = inPhononEnergy;
+    this->desc_.rel_eff_mass = relEffMass;
+    this->desc_.eps_hi = epsHi;
+    this->desc_.eps_lo = epsLo;
+  }
+  std::string getName() const override {
+    return std::string(Absorption ? "ScreenedFroehlichAbsorption3D"
+                                  : "ScreenedFroehlichEmission3D") +
+           nameSuffix;
+  }
+  T getScatterRate(T energy, SizeType) const override {
+    return phys::screenedFroehlichRate(scatterConst, nBose, effMass,
+                                       this->ptrValley[this->idxValley]->mass(), energy,
+                                       phononEnergy, screening->getQs2(), Absorption);
+  }
+  const emcPlasmonScreening<T> *getScreening() const override { return screening.get(); }
+};
+
+template <class T>
+using emcScreenedFroehlichAbsorption3D = emcScreenedFroehlich3D<T, true>;
+template <class T>
+using emcScreenedFroehlichEmission3D = emcScreenedFroehlich3D<T, false>;
 
 }  // namespace emcgpu
 

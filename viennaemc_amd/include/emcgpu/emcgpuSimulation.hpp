@@ -294,6 +294,9 @@ private:
     auto md = type.mechanismDescriptors();
     check(emcgpu_set_model(ctx, 0, vd.data(), (int)vd.size(), md.data(), (int)md.size()), ctx,
           "emcgpu_set_model");
+    emcgpu_screening_desc sd;
+    if (type.screeningDescriptor(sd))
+      check(emcgpu_set_screening(ctx, 0, &sd), ctx, "emcgpu_set_screening");
     auto rates = type.rawRates();
     check(emcgpu_set_rate_tables(ctx, 0, (int)type.getNrEnergyLevels(), type.getMaxEnergy(),
                                  rates.data()),
