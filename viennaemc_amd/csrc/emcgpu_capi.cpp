@@ -403,6 +403,8 @@ BathStepArgs<R> bath_args(const emcgpu_ctx *c, double dt, double *hist, int hist
   a.win64 = c->d_win64;
   a.win = (R *)c->d_win;
   a.n_obs = c->n_obs;
+  a.n_jobs = c->n_jobs;
+  a.rb_max = c->d_rb_max;
   return a;
 }
 
@@ -436,6 +438,7 @@ emcgpu_status upload_rebuild(emcgpu_ctx *c, size_t cum_elems) {
         continue;
       RebuildJob J{};
       J.inst = i;
+      J.every_step = in.plasma() ? 1 : 0;
       J.valley = (int)v;
       J.n_tables = (int)tm.size();
       J.n_levels = in.n_levels;
@@ -475,6 +478,7 @@ emcgpu_status upload_rebuild(emcgpu_ctx *c, size_t cum_elems) {
   dfreet(c->d_groups);
   dfreet(c->d_rate_arena);
   dfreet(c->d_scr);
+  dfreet(c->d_rb_max);
   c->n_jobs = (int)jobs.size();
   c->n_groups = (int)groups.size();
   HIPCHK(c, hipMalloc((void **)&c->d_scr, sizeof(double) * scr.size()));
@@ -494,6 +498,7 @@ emcgpu_status upload_rebuild(emcgpu_ctx *c, size_t cum_elems) {
     return EMCGPU_OK;
   }
   HIPCHK(c, hipMalloc((void **)&c->d_jobs, sizeof(RebuildJob) * jobs.size()));
+  HIPCHK(c, hipMalloc((void **)&c->d_rb_max, sizeof(double) * kRebuildChunks * jobs.size()));
   HIPCHK(c, hipMalloc((void **)&c->d_rate_arena, sizeof(double) * std::max<size_t>(arena.size(), 1)));
   HIPCHK(c, hipMemcpy(c->d_jobs, jobs.data(), sizeof(RebuildJob) * jobs.size(),
                       hipMemcpyHostToDevice));
@@ -1094,6 +1099,7 @@ void emcgpu_destroy(emcgpu_ctx *c) {
   dfreet(c->d_tau0_64);
   dfreet(c->d_groups);
   dfreet(c->d_scr);
+  dfreet(c->d_rb_max);
   dfreet(c->d_win64);
   dfree(c->d_win);
   if (c->gexec)

@@ -121,7 +121,11 @@ struct RebuildJob {
                                     // screened: kI[level], kF[level] (1/m)
   int nsrc[kMaxMech];               // screened: kN*
   double nconst[kMaxMech];          // screened, kNConst: Bose occupation
+  int every_step;                   // rebuilt after every step (plasma screening)
 };
+// the per-step rebuild over many blocks: kRebuildChunk levels per block
+constexpr int kRebuildChunk = 256;
+constexpr int kRebuildChunks = (EMCGPU_MAX_LEVELS + kRebuildChunk - 1) / kRebuildChunk;
 
 // the baths and rebuild jobs of one instance (one block of k_bath_step)
 struct BathInstDev {
@@ -156,6 +160,8 @@ struct BathStepArgs {
   R *win;          // the same in the precision of the kinematics
   const double *partial;  // step partials [n_obs][n_tiles] of the step just taken
   int n_obs, n_tiles, tile_size;
+  int n_jobs;
+  double *rb_max;  // [job][kRebuildChunks] block maxima of the cumulative rates
   const long long *inst_start, *inst_count;  // particles of each instance
 };
 

@@ -123,6 +123,7 @@ struct emcgpu_ctx {
   emcgpu::BathInstDev *d_groups = nullptr;
   int n_groups = 0;
   double *d_rate_arena = nullptr;
+  double *d_rb_max = nullptr;  // [job][kRebuildChunks] of the multi-block rebuild
   double *d_cum64 = nullptr, *d_tau0_64 = nullptr;
   // q_s^2 per instance [1/m^2] and the baths' window sums (FP64 and in the
   // precision of the kinematics): per bath cumW[n_bins + 1], cumWN[n_bins + 1]

@@ -463,7 +463,9 @@ __device__ double bath_planck_temp(double energy_ev, double n) {
   return kQ * energy_ev / (kKB * ::log(1.0 + 1.0 / n));
 }
 
-static_assert(kMaxBins <= kBathThreads, "one lane per bin in the bath update");
+// one lane per bin in the bath update; the per-bin LDS arrays double as
+// per-lane reduction buffers
+static_assert(kMaxBins == kBathThreads, "bath block: one lane per bin");
 
 struct BathShared {
   double old_n[kMaxBins];
@@ -471,12 +473,15 @@ struct BathShared {
   double w[kMaxBins];
   double vmax[kBathThreads];
   double target;
+  double qs2;  // the instance's q_s^2, broadcast to the block
 };
 
+template <typename R>
 __device__ void bath_update_one(const BathDev &B, BathVar &V, double *__restrict__ nq,
                                 const double *__restrict__ tau_prof,
                                 unsigned long long *__restrict__ tally, long long tally_bins,
-                                double dt, double *__restrict__ hist, BathShared &sh) {
+                                double dt, double *__restrict__ hist, BathShared &sh,
+                                double qs2, double *__restrict__ win64, R *__restrict__ win) {
 #pragma clang fp contract(off)
   const int nb = B.n_bins;
   if (threadIdx.x == 0) {
@@ -506,7 +511,7 @@ __device__ void bath_update_one(const BathDev &B, BathVar &V, double *__restrict
     const int i = (int)threadIdx.x % nb, g = (int)threadIdx.x / nb;
     const long long gi = B.off + i;
     unsigned long long c_em = 0ull, c_abs = 0ull;
-#pragma unroll 8
+#pragma unroll 16
     for (int r = g; r < kTallyCopies; r += G) {
       unsigned long long *t = tally + 2 * ((long long)r * tally_bins + gi);
       c_em += t[0];
@@ -541,40 +546,40 @@ __device__ void bath_update_one(const BathDev &B, BathVar &V, double *__restrict
     const double n_em = cnt_em[u], n_abs = cnt_abs[u];
     const double g = (dph > 0.0) ? (n_em - n_abs) / (dph * dt) : 0.0;
     const double tau_i = B.has_profile ? tau_prof[gi] : B.tau_lo;
-    double n = nq[gi];
-    sh.old_n[i] = n;
-    sh.w[i] = q * q;
+    const double n_old = nq[gi];
+    const double w = q * q;
+    double n = n_old;
     n += g * dt - (dt / tau_i) * (n - target);
     if (n < 0.0)
       n = 0.0;
     nq[gi] = n;
-    sh.new_n[i] = n;
+    // the terms of the bin-order sums, formed on all lanes: the reservoir
+    // source from the start-of-step occupations (:291-355), getMeanNq
+    // (:461-470) and, when read, the window sums (:109-124)
+    sh.old_n[i] = B.has_profile ? w * (n_old - target) / tau_i : w * (n_old - target);
+    sh.new_n[i] = w * n;
+    if (win64) {
+      const double denom = q * q + qs2;
+      const double ww = (denom > 0.0) ? q / denom : 0.0;
+      sh.w[i] = ww;
+      sh.vmax[i] = ww * n;
+    }
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    if (B.has_ac) {
-      // reservoirs from the start-of-step occupations (:291-355)
-      // sums in bin order (the reference's), the LDS loads batched ahead of
-      // the dependent adds
-      double sum_w = 0.0, sum_wdn = 0.0, sum_wdn_tau = 0.0;
-      if (B.has_profile) {
+  // the sums themselves in bin order (the reference's), one lane each of
+  // four waves side by side
+  const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+  if (lane == 0 && wave == 0 && B.has_ac) {
+    {
+      // sum of the weights q^2: a constant of the bath, summed on the host
+      // in the same order
+      const double sum_w = B.sum_w;
+      double sum_t = 0.0;
 #pragma unroll 16
-        for (int i = 0; i < nb; ++i) {
-          const double w = sh.w[i];
-          sum_w += w;
-          sum_wdn_tau += w * (sh.old_n[i] - target) / tau_prof[B.off + i];
-        }
-      } else {
-#pragma unroll 16
-        for (int i = 0; i < nb; ++i) {
-          const double w = sh.w[i];
-          sum_w += w;
-          sum_wdn += w * (sh.old_n[i] - target);
-        }
-      }
+      for (int i = 0; i < nb; ++i)
+        sum_t += sh.old_n[i];
       const double f_klemens = B.has_ridley ? (1.0 - B.w_r) : 1.0;
-      const double src = B.has_profile ? ((sum_w > 0.0) ? sum_wdn_tau / sum_w : 0.0)
-                                       : ((sum_w > 0.0) ? sum_wdn / sum_w : 0.0);
+      const double src = (sum_w > 0.0) ? sum_t / sum_w : 0.0;
       double n_ac = V.n_ac, n_to = V.n_to;
       if (!B.has_profile) {
         n_ac += dt * f_klemens * src / B.tau_lo - dt * (n_ac - B.n_ac_eq) / B.tau_ac;
@@ -598,16 +603,40 @@ __device__ void bath_update_one(const BathDev &B, BathVar &V, double *__restrict
       V.n_ac = n_ac;
       V.n_to = n_to;
     }
-    // getMeanNq over the updated occupations (sum of the weights is a
-    // constant of the bath, summed in the same order on the host)
+  } else if (lane == 0 && wave == 1) {
+    // getMeanNq over the updated occupations
     double sum_wn = 0.0;
 #pragma unroll 16
     for (int i = 0; i < nb; ++i)
-      sum_wn += sh.w[i] * sh.new_n[i];
+      sum_wn += sh.new_n[i];
     const double mean = (B.sum_w > 0.0) ? sum_wn / B.sum_w : B.n0;
     V.mean = mean;
     if (hist)
       *hist = mean;
+  } else if (lane == 0 && wave == 2 && win64) {
+    double *cw = win64 + B.woff;
+    R *rw = win + B.woff;
+    double acc = 0.0;
+    cw[0] = 0.0;
+    rw[0] = R(0);
+#pragma unroll 16
+    for (int i = 0; i < nb; ++i) {
+      acc = acc + sh.w[i];
+      cw[i + 1] = acc;
+      rw[i + 1] = (R)acc;
+    }
+  } else if (lane == 0 && wave == 3 && win64) {
+    double *cwn = win64 + B.woff + nb + 1;
+    R *rwn = win + B.woff + nb + 1;
+    double acc = 0.0;
+    cwn[0] = 0.0;
+    rwn[0] = R(0);
+#pragma unroll 16
+    for (int i = 0; i < nb; ++i) {
+      acc = acc + sh.vmax[i];
+      cwn[i + 1] = acc;
+      rwn[i + 1] = (R)acc;
+    }
   }
   __syncthreads();
 }
@@ -674,13 +703,13 @@ __device__ __forceinline__ double rebuild_rate(const RebuildJob &J, int j, int l
 }
 
 template <typename R>
-__device__ void rebuild_one(const RebuildJob &J, const BathStepArgs<R> &a, BathShared &sh) {
+__device__ void rebuild_one(const RebuildJob &J, const BathStepArgs<R> &a, double qs2,
+                            BathShared &sh) {
 #pragma clang fp contract(off)
   const double *__restrict__ arena = a.arena;
   const BathVar *__restrict__ var = a.var;
   const BathDev *__restrict__ baths = a.baths;
   const double *__restrict__ win64 = a.win64;
-  const double qs2 = a.scr ? a.scr[J.inst] : 0.0;
   R *__restrict__ cum = a.cum;
   double *__restrict__ cum64 = a.cum64;
   const int nt = J.n_tables, nl = J.n_levels;
@@ -735,24 +764,30 @@ __device__ void screening_update(const BathInstDev &I, const BathStepArgs<R> &a,
   const long long ts = a.tile_size;
   const long long tile0 = a.inst_start[I.inst] / ts;
   const long long tile1 = (a.inst_start[I.inst] + a.inst_count[I.inst] + ts - 1) / ts;
-  for (long long t = tile0 + threadIdx.x; t < tile1 && t < a.n_tiles; t += blockDim.x) {
+  const long long tend = (tile1 < a.n_tiles) ? tile1 : a.n_tiles;
+#pragma unroll 4
+  for (long long t = tile0 + threadIdx.x; t < tend; t += blockDim.x) {
     for (int v = 0; v < I.n_valleys; ++v)
       vals[0] += a.partial[(long long)(2 + 3 * v) * a.n_tiles + t];
     vals[1] += a.partial[3LL * a.n_tiles + t];
     vals[2] += a.partial[2LL * a.n_tiles + t];
   }
-  double tot[3];
-  for (int k = 0; k < 3; ++k) {
-    sh.vmax[threadIdx.x] = vals[k];
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w)
-        sh.vmax[threadIdx.x] = sh.vmax[threadIdx.x] + sh.vmax[threadIdx.x + w];
-      __syncthreads();
+  // the three sums in one tree (fixed order, deterministic)
+  double *r0 = sh.vmax, *r1 = sh.old_n, *r2 = sh.new_n;
+  r0[threadIdx.x] = vals[0];
+  r1[threadIdx.x] = vals[1];
+  r2[threadIdx.x] = vals[2];
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      r0[threadIdx.x] = r0[threadIdx.x] + r0[threadIdx.x + w];
+      r1[threadIdx.x] = r1[threadIdx.x] + r1[threadIdx.x + w];
+      r2[threadIdx.x] = r2[threadIdx.x] + r2[threadIdx.x + w];
     }
-    tot[k] = sh.vmax[0];
     __syncthreads();
   }
+  const double tot[3] = {r0[0], r1[0], r2[0]};
+  __syncthreads();
   if (threadIdx.x == 0) {
     double qs2 = 0.0;
     if (tot[0] != 0.0) {
@@ -764,6 +799,7 @@ __device__ void screening_update(const BathInstDev &I, const BathStepArgs<R> &a,
     }
     a.scr[I.inst] = qs2;
     a.inst[I.inst].qs2 = (R)(qs2 * 1e-18);
+    sh.qs2 = qs2;
   }
   __syncthreads();
 }
@@ -806,35 +842,118 @@ __device__ void window_sums_one(const BathDev &B, double qs2, const BathStepArgs
   __syncthreads();
 }
 
-// the instance's screening, its baths' window sums, then its tables
+// the instance's q_s^2 into LDS: from the step's partials (plasma
+// screening) or as held
 template <typename R>
-__device__ void refresh_group(const BathInstDev &I, const BathStepArgs<R> &a, bool plasma,
-                              bool rebuild, BathShared &sh) {
-  if (plasma && I.plasma)
+__device__ void group_screening(const BathInstDev &I, const BathStepArgs<R> &a, bool plasma,
+                                BathShared &sh) {
+  if (plasma && I.plasma) {
     screening_update(I, a, sh);
-  if (a.win64 && I.need_win) {
-    const double qs2 = a.scr ? a.scr[I.inst] : 0.0;
-    for (int b = I.bath0; b < I.bath0 + I.n_baths; ++b)
-      window_sums_one(a.baths[b], qs2, a, sh);
-  }
-  if (!rebuild)
     return;
+  }
+  if (threadIdx.x == 0)
+    sh.qs2 = a.scr ? a.scr[I.inst] : 0.0;
+  __syncthreads();
+}
+
+template <typename R>
+__device__ void rebuild_group(const BathInstDev &I, const BathStepArgs<R> &a, BathShared &sh) {
+  const double qs2 = sh.qs2;
   for (int j = I.job0; j < I.job0 + I.n_jobs; ++j)
-    rebuild_one<R>(a.jobs[j], a, sh);
+    rebuild_one<R>(a.jobs[j], a, qs2, sh);
 }
 
 // One block per instance with baths or plasma screening: the baths' update,
 // the screening from the step's partials, the window sums, the tables (the
 // order of hotCarrierMHP.cpp:693-703).  Window sums and screening follow
 // the values the reference's setScreeningQ2 / update leave behind.
+// The screening depends on the carriers only, so it is taken first; the
+// window sums then see the updated occupations and the new q_s^2, as after
+// the reference's update + setScreeningQ2.
 template <typename R>
 __global__ __launch_bounds__(kBathThreads) void k_bath_step(BathStepArgs<R> a) {
   __shared__ BathShared sh;
   const BathInstDev &I = a.groups[blockIdx.x];
+  group_screening<R>(I, a, a.partial != nullptr, sh);
+  const double qs2 = sh.qs2;
+  double *win64 = (a.win64 && I.need_win) ? a.win64 : nullptr;
   for (int b = I.bath0; b < I.bath0 + I.n_baths; ++b)
-    bath_update_one(a.baths[b], a.var[b], a.nq, a.tau_prof, a.tally, a.tally_bins, a.dt,
-                    a.hist ? a.hist + (long long)b * a.hist_steps + a.hist_step : nullptr, sh);
-  refresh_group<R>(I, a, a.partial != nullptr, a.rebuild != 0, sh);
+    bath_update_one<R>(a.baths[b], a.var[b], a.nq, a.tau_prof, a.tally, a.tally_bins, a.dt,
+                       a.hist ? a.hist + (long long)b * a.hist_steps + a.hist_step : nullptr, sh,
+                       qs2, win64, a.win);
+  // the tables follow in k_rebuild_rates / k_rebuild_norm
+}
+
+// The per-step table rebuild over (level chunk, job) blocks, one level per
+// lane: the cumulative FP64 rates of the level (kept unnormalised in the
+// FP64 copy) and the block's maximum ...
+template <typename R>
+__global__ __launch_bounds__(kRebuildChunk) void k_rebuild_rates(BathStepArgs<R> a) {
+#pragma clang fp contract(off)
+  __shared__ double red[kRebuildChunk];
+  const RebuildJob &J = a.jobs[blockIdx.y];
+  if (!(a.rebuild || J.every_step))
+    return;
+  const int nl = J.n_levels, nt = J.n_tables;
+  const int l0 = (int)blockIdx.x * kRebuildChunk;
+  if (l0 >= nl)
+    return;  // the whole block leaves: no barrier below is reached
+  const double qs2 = a.scr ? a.scr[J.inst] : 0.0;
+  const int l = l0 + (int)threadIdx.x;
+  double c = -1.0;
+  if (l < nl) {
+    for (int j = 0; j < nt; ++j) {
+      const double r = rebuild_rate(J, j, l, a.arena, a.var, a.baths, a.win64, qs2);
+      c = (j == 0) ? r : r + c;
+      a.cum64[J.cum_off + (long long)l * nt + j] = c;
+    }
+  }
+  red[threadIdx.x] = c;
+  __syncthreads();
+  for (int w = kRebuildChunk / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w && red[threadIdx.x] < red[threadIdx.x + w])
+      red[threadIdx.x] = red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    a.rb_max[(long long)blockIdx.y * kRebuildChunks + blockIdx.x] = red[0];
+}
+
+// ... then the largest total rate of the job over its chunks, the
+// normalised tables and tau0 (renormalizeTables, emcScatterHandler.hpp:
+// 248-272); with no rate at any level the kinematics keep the previous tables
+template <typename R>
+__global__ __launch_bounds__(kRebuildChunk) void k_rebuild_norm(BathStepArgs<R> a) {
+#pragma clang fp contract(off)
+  const RebuildJob &J = a.jobs[blockIdx.y];
+  if (!(a.rebuild || J.every_step))
+    return;
+  const int nl = J.n_levels, nt = J.n_tables;
+  const int l0 = (int)blockIdx.x * kRebuildChunk;
+  if (l0 >= nl)
+    return;
+  const int nc = (nl + kRebuildChunk - 1) / kRebuildChunk;
+  double mx = -1.0;
+  for (int k = 0; k < nc; ++k) {
+    const double m = a.rb_max[(long long)blockIdx.y * kRebuildChunks + k];
+    mx = (mx < m) ? m : mx;
+  }
+  if (!(mx > 0.0))
+    return;
+  const int l = l0 + (int)threadIdx.x;
+  if (l < nl) {
+    for (int j = 0; j < nt; ++j) {
+      const long long e = J.cum_off + (long long)l * nt + j;
+      const double x = a.cum64[e] / mx;
+      a.cum[e] = (R)x;
+      a.cum64[e] = x;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const double tau = 1. / mx;
+    a.inst[J.inst].tau0[J.valley] = (R)(tau * 1e15);
+    a.tau0_64[J.inst * kMaxV + J.valley] = tau;
+  }
 }
 
 // window sums and tables alone (after the model, the field, a bath state or
@@ -842,7 +961,12 @@ __global__ __launch_bounds__(kBathThreads) void k_bath_step(BathStepArgs<R> a) {
 template <typename R>
 __global__ __launch_bounds__(kBathThreads) void k_bath_refresh(BathStepArgs<R> a) {
   __shared__ BathShared sh;
-  refresh_group<R>(a.groups[blockIdx.x], a, false, true, sh);
+  const BathInstDev &I = a.groups[blockIdx.x];
+  group_screening<R>(I, a, false, sh);
+  if (a.win64 && I.need_win)
+    for (int b = I.bath0; b < I.bath0 + I.n_baths; ++b)
+      window_sums_one(a.baths[b], sh.qs2, a, sh);
+  rebuild_group<R>(I, a, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -957,6 +1081,11 @@ hipError_t launch_bath_step(const BathStepArgs<R> &a, int n_groups, hipStream_t 
   if (n_groups <= 0)
     return hipSuccess;
   hipLaunchKernelGGL((k_bath_step<R>), dim3(n_groups), dim3(kBathThreads), 0, st, a);
+  if (a.n_jobs > 0) {
+    const dim3 g(kRebuildChunks, a.n_jobs);
+    hipLaunchKernelGGL((k_rebuild_rates<R>), g, dim3(kRebuildChunk), 0, st, a);
+    hipLaunchKernelGGL((k_rebuild_norm<R>), g, dim3(kRebuildChunk), 0, st, a);
+  }
   return hipGetLastError();
 }
 
