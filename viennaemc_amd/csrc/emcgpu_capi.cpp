@@ -614,6 +614,9 @@ void fill_common(const emcgpu_ctx *c, StepArgs<R> &a) {
   a.n_obs = c->n_obs;
   a.max_v = c->max_v;
   a.max_deg = c->max_deg;
+  for (const auto &in : c->inst)
+    for (const auto &m : in.mechs)
+      a.ext_kinds |= (is_scr_kind(m.kind) || is_scr_hot_kind(m.kind)) ? 1 : 0;
 }
 
 template <typename R>

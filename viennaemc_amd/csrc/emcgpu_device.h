@@ -199,6 +199,7 @@ struct StepArgs {
   unsigned long long *tally;
   long long tally_bins;
   int lds_bins;  // bins held in LDS per block (max over instances)
+  int ext_kinds; // some instance uses a screened Froehlich kind
 };
 
 // ===========================================================================

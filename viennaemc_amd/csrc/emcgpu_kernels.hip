@@ -230,7 +230,7 @@ __global__ __launch_bounds__(kTile) void k_bulk_multi(StepArgs<R> a) {
           ee = ev[j];
         }
       rng.pid = (uint32_t)(first + (long long)sel * kTile);
-      event_once<R, 0, HOT>(P, a.cum, x, y, z, en, ta, rm, vv, ss, ee, rng, nreal, lds_tally);
+      event_once<R, 0, HOT, HOT>(P, a.cum, x, y, z, en, ta, rm, vv, ss, ee, rng, nreal, lds_tally);
 #pragma unroll
       for (int j = 0; j < PPT; ++j)
         if (sel == j) {
@@ -311,7 +311,7 @@ __global__ __launch_bounds__(kTile) void k_bulk_replay(StepArgs<R> a) {
   for (int s = 0; s < a.n_steps; ++s) {
     int nev = 0, nreal = 0;
     if (valid[0]) {
-      move_one<R, 1, HOT>(P, a.cum, kx[0], ky[0], kz[0], E[0], tau, v[0], sub[0], rng, nev,
+      move_one<R, 1, HOT, HOT>(P, a.cum, kx[0], ky[0], kz[0], E[0], tau, v[0], sub[0], rng, nev,
                           nreal, lds_tally);
       // grain bookkeeping (basicBulkParticleHandler.hpp:213-220), no grain
       // mechanism: a new grainTau of -ln U[1e-6,1) * 1 s
@@ -1061,7 +1061,8 @@ template <typename R>
 hipError_t launch_steps(const StepArgs<R> &a, int mode, int ppt, hipStream_t st) {
   if (a.n_tiles <= 0)
     return hipSuccess;
-  if (a.tally && a.lds_bins > 0)
+  // the extended kernels: bath tallies and the screened final states
+  if ((a.tally && a.lds_bins > 0) || a.ext_kinds)
     return launch_steps_t<R, true>(a, mode, ppt, st);
   return launch_steps_t<R, false>(a, mode, ppt, st);
 }

@@ -333,7 +333,7 @@ __device__ __forceinline__ R qres_cos_theta(const R *__restrict__ w, int nb, R d
 // final state of one mechanism: ua/ub are the two final-state uniforms
 // (phi / cos-theta for isotropic kinds, exponent / phi for the Froehlich
 // kinds)
-template <typename R, bool HOT = false>
+template <typename R, bool HOT = false, bool SCR = true>
 __device__ __forceinline__ void final_state(const InstDev<R> *P, const MechDev<R> &m, R &kx,
                                             R &ky, R &kz, R &E, int &v, int &sub, R ua, R ub,
                                             int zidx, uint32_t *lds_tally = nullptr) {
@@ -354,7 +354,11 @@ __device__ __forceinline__ void final_state(const InstDev<R> *P, const MechDev<R
     // of |k' - k| in their bath
     const R kx0 = kx, ky0 = ky, kz0 = kz;
     R ct, kn;
-    if (m.final_kind == kFinalScreenedFroehlich) {
+    // the screened kinds are compiled into the kernels that may meet them
+    bool screened = false;
+    if constexpr (SCR)
+      screened = m.final_kind == kFinalScreenedFroehlich;
+    if (screened) {
       // emcScreenedFroehlichInteraction.hpp:140-152 / :198-211: kI and kF
       // are the valley's |k| before and after the energy change
       const R ki = normk_of(P->valley[v], E);
@@ -444,7 +448,7 @@ __device__ __forceinline__ void drift(const InstDev<R> *P, R t, R &kx, R &ky, R 
 // (emcScatterHandler::scatterParticle, emcScatterHandler.hpp:148-170), then
 // the next free-flight time (getNewTau, emcParticleType.hpp:187-189), which
 // is returned; e counts the events of the step
-template <typename R, int MODE, bool HOT = false>
+template <typename R, int MODE, bool HOT = false, bool SCR = true>
 __device__ __forceinline__ R scatter_event(const InstDev<R> *P, const R *__restrict__ cum, R &kx,
                                            R &ky, R &kz, R &E, int &v, int &sub, uint32_t &e,
                                            StepRng<R, MODE> &rng, int &nreal,
@@ -507,7 +511,7 @@ __device__ __forceinline__ R scatter_event(const InstDev<R> *P, const R *__restr
             ub = (R)rng.rs.uniform(0., 1.);
           }
         }
-        final_state<R, HOT>(P, m, kx, ky, kz, E, v, sub, ua, ub, zidx, lds_tally);
+        final_state<R, HOT, SCR>(P, m, kx, ky, kz, E, v, sub, ua, ub, zidx, lds_tally);
         ++nreal;
       }
     }
@@ -526,13 +530,13 @@ __device__ __forceinline__ R scatter_event(const InstDev<R> *P, const R *__restr
 // one pass of the scatter loop of moveParticles: scatter (selection +
 // final state), draw tau', drift min(rem, tau'), rem -= tau'
 // (basicBulkParticleHandler.hpp:203-212)
-template <typename R, int MODE, bool HOT = false>
+template <typename R, int MODE, bool HOT = false, bool SCR = true>
 __device__ __forceinline__ void event_once(const InstDev<R> *P, const R *__restrict__ cum, R &kx,
                                            R &ky, R &kz, R &E, R &tau, R &rem, int &v, int &sub,
                                            uint32_t &e, StepRng<R, MODE> &rng, int &nreal,
                                            uint32_t *lds_tally = nullptr) {
   const R newTau =
-      scatter_event<R, MODE, HOT>(P, cum, kx, ky, kz, E, v, sub, e, rng, nreal, lds_tally);
+      scatter_event<R, MODE, HOT, SCR>(P, cum, kx, ky, kz, E, v, sub, e, rng, nreal, lds_tally);
   tau += newTau;
   drift(P, (newTau < rem) ? newTau : rem, kx, ky, kz, E, v, sub);  // std::min(rem, newTau)
   rem -= newTau;
@@ -540,7 +544,7 @@ __device__ __forceinline__ void event_once(const InstDev<R> *P, const R *__restr
 
 // one particle, one step: drift min(tau, dt), then the scatter loop
 // (basicBulkParticleHandler.hpp:195-214)
-template <typename R, int MODE, bool HOT = false>
+template <typename R, int MODE, bool HOT = false, bool SCR = true>
 __device__ __forceinline__ void move_one(const InstDev<R> *P, const R *__restrict__ cum, R &kx,
                                          R &ky, R &kz, R &E, R &tau, int &v, int &sub,
                                          StepRng<R, MODE> &rng, int &nev, int &nreal,
@@ -552,7 +556,7 @@ __device__ __forceinline__ void move_one(const InstDev<R> *P, const R *__restric
   // every pass draws tau' > 0 (the uniform is clamped below 1), so rem
   // strictly decreases; the event cap bounds the loop regardless
   while (rem > R(0) && e < kMaxEventsPerStep)
-    event_once<R, MODE, HOT>(P, cum, kx, ky, kz, E, tau, rem, v, sub, e, rng, nreal, lds_tally);
+    event_once<R, MODE, HOT, SCR>(P, cum, kx, ky, kz, E, tau, rem, v, sub, e, rng, nreal, lds_tally);
   nev += (int)e;
   tau -= dt;
 }
