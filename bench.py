@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--hot-steps", type=int, default=2000)
     ap.add_argument("--no-hot-plain", action="store_true",
                     help="run only the screened (q-resolved, plasma-screened) hot-phonon leg")
+    ap.add_argument("--no-hot-screened", action="store_true",
+                    help="skip the screened hot-phonon leg")
     ap.add_argument("--hot-particles", type=int, default=4_000_000)
     ap.add_argument("--no-device", action="store_true", help="skip the 2-D resistor leg")
     ap.add_argument("--device-only", action="store_true",
@@ -323,8 +325,9 @@ def main():
             if not args.no_fp64:
                 out["hot_phonon_fp64"] = run_hot_phonon(emcgpu, args.hot_particles, args.warmup,
                                                         args.hot_steps, emcgpu.FP64)
-        out["hot_phonon_screened"] = run_hot_phonon(emcgpu, args.hot_particles, args.warmup,
-                                                    args.hot_steps, emcgpu.FP32, screened=True)
+        if not args.no_hot_screened:
+            out["hot_phonon_screened"] = run_hot_phonon(emcgpu, args.hot_particles, args.warmup,
+                                                        args.hot_steps, emcgpu.FP32, screened=True)
     if not args.no_device:
         out["device_resistor"] = run_device(emcgpu, args.device_particles, args.warmup,
                                             args.device_steps, emcgpu.FP32)

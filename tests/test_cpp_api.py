@@ -121,3 +121,41 @@ def test_screened_mechanisms_match_reference(tmp_path, golden):
     assert abs(float(val["debye"][0]) - qs2) <= 1e-15 * qs2
     assert abs(float(val["debye"][1]) - qs2 ** -0.5) <= 1e-15 * qs2 ** -0.5
     assert float(val["off"][0]) == 0.0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant,name", [("plain", "hot_polar"), ("screened", "screened_hot_polar")])
+def test_cpp_hot_phonon_example_matches_python_binding(variant, name):
+    """hotPhononGpu (the hot-carrier drivers' loop written against the host
+    API: emcPhononBath, emcHotPhononFroehlich* / emcScreenedHotPhononFroehlich*,
+    emcPlasmonScreening) equals the Python binding on the same seed: mean
+    energy and mean N_q after every step, and the final q_s^2."""
+    from viennaemc_amd import emcgpu
+    exe = os.path.join(PKG, "examples", "hotPhononGpu")
+    n, steps, seed = 20000, 60, 97
+    print("cpp hot-phonon example", variant, flush=True)
+    out = subprocess.check_output([exe, str(n), str(steps), "fp64", str(seed), variant],
+                                  env=dict(os.environ, EMCGPU_THROW="1"), timeout=120).decode()
+    rows = [l.split() for l in out.splitlines() if l.startswith("step ")]
+    assert len(rows) == steps
+    spec = dict(MODELS[name])
+    if variant == "screened":
+        # the example's emcPlasmonScreening starts at q_s^2 = 0, and the initial
+        # free flights follow the tables of that moment
+        spec["screening"] = dict(spec["screening"], qs2=0.0)
+    ctx = emcgpu.Context(precision=emcgpu.FP64)
+    ctx.setup(0, spec)
+    ctx.resize([n])
+    ctx.init_particles(0, seed, spec["T"])
+    if variant == "screened":
+        ctx.update_screening(0)
+    ctx.step(1e-15, steps, seed=seed)
+    av = ctx.averages(0)
+    hist = ctx.bath_history(0, 0)
+    for r in rows:
+        s = int(r[1]) - 1
+        assert float(r[2]) == av["avgE"][s, 0]
+        assert float(r[3]) == hist[s]
+    if variant == "screened":
+        assert float(rows[-1][4]) == ctx.get_screening(0) and ctx.get_screening(0) > 0
+    assert hist[-1] != hist[0]

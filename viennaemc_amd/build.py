@@ -62,7 +62,7 @@ def build(verbose=True):
     return LIB
 
 
-EXAMPLES = ["bulkSimulationGpu", "resistor2DGpu"]
+EXAMPLES = ["bulkSimulationGpu", "resistor2DGpu", "hotPhononGpu"]
 
 
 def build_examples(verbose=True):

@@ -1219,6 +1219,11 @@ emcgpu_status emcgpu_set_rate_tables(emcgpu_ctx *c, int instance, int n_levels,
   if (in.has_hot)
     return fail(c, EMCGPU_ERR_ARG,
                 "hot-phonon rates follow their bath: build them with emcgpu_build_tables");
+  if (in.plasma())
+    for (const auto &m : in.mechs)
+      if (is_scr_kind(m.kind))
+        return fail(c, EMCGPU_ERR_ARG,
+                    "plasma-screened rates follow the carriers: build them with emcgpu_build_tables");
   std::vector<std::vector<double>> raw(in.mechs.size());
   for (size_t j = 0; j < in.mechs.size(); j++)
     raw[j].assign(rates + j * n_levels, rates + (j + 1) * n_levels);

@@ -163,9 +163,35 @@ public:
   }
   std::vector<emcgpu_mech_desc> mechanismDescriptors() const {
     std::vector<emcgpu_mech_desc> d;
-    for (auto &m : mechs)
+    const auto bl = baths();
+    for (auto &m : mechs) {
       d.push_back(m->finalState());
+      if (auto *bm = dynamic_cast<const emcBathMechanism<T> *>(m.get()))
+        for (size_t b = 0; b < bl.size(); b++)
+          if (bl[b] == bm->getBath())
+            d.back().bath = (int)b;
+    }
     return d;
+  }
+  /// the phonon baths the mechanisms read, in registration order
+  std::vector<std::shared_ptr<emcPhononBath<T>>> baths() const {
+    std::vector<std::shared_ptr<emcPhononBath<T>>> r;
+    for (auto &m : mechs)
+      if (auto *bm = dynamic_cast<const emcBathMechanism<T> *>(m.get())) {
+        bool seen = false;
+        for (auto &b : r)
+          seen |= b == bm->getBath();
+        if (!seen)
+          r.push_back(bm->getBath());
+      }
+    return r;
+  }
+  /// every mechanism a built-in class (their descriptors carry the rates)
+  bool allBuiltIn() const {
+    for (auto &m : mechs)
+      if (!m->isBuiltIn())
+        return false;
+    return true;
   }
   /// the screening the screened mechanisms read (the first one found; they
   /// share one emcPlasmonScreening as in the reference), or null
@@ -212,12 +238,25 @@ public:
                             unsigned long inSeed = 1, Precision precision = FP32, int gpu = 0)
       : ptype(type), dir(fieldDirection), seed(inSeed), temperature(device.getTemperature()),
         nPart(nrParticles) {
+    material = emcgpu_material{0.0, (double)device.getMaterial().getRho(), 0.0,
+                               (double)device.getMaterial().getVelSound()};
     emcgpu_config cfg{gpu, (int)precision, 1};
     check(emcgpu_create(&cfg, &ctx), nullptr, "emcgpu_create");
     auto vd = type.valleyDescriptors();
     auto md = type.mechanismDescriptors();
     check(emcgpu_set_model(ctx, 0, vd.data(), (int)vd.size(), md.data(), (int)md.size()), ctx,
           "emcgpu_set_model");
+    // phonon baths: their state moves to the GPU, which updates them after
+    // every step and rebuilds the tables from them
+    baths = type.baths();
+    if (!baths.empty()) {
+      std::vector<emcgpu_bath_desc> bd;
+      for (auto &b : baths)
+        bd.push_back(b->descriptor());
+      check(emcgpu_set_baths(ctx, 0, bd.data(), (int)bd.size()), ctx, "emcgpu_set_baths");
+      for (size_t b = 0; b < baths.size(); b++)
+        baths[b]->bind(ctx, 0, (int)b);
+    }
     reinitScatterTables();
     resetAppliedFieldStrength(fieldStrength);
     size_t n = nrParticles;
@@ -232,12 +271,47 @@ public:
   /// current q_s^2 of a shared emcPlasmonScreening for the GPU final state
   void reinitScatterTables() {
     emcgpu_screening_desc sd;
-    if (ptype.screeningDescriptor(sd))
+    if (ptype.screeningDescriptor(sd)) {
+      if (plasmaVolume > 0) {
+        sd.mode = EMCGPU_SCREENING_PLASMA;
+        sd.volume = plasmaVolume;
+      }
       check(emcgpu_set_screening(ctx, 0, &sd), ctx, "emcgpu_set_screening");
+    }
+    if (!baths.empty() || plasmaVolume > 0) {
+      // rates that follow a bath or the carriers are rebuilt on the GPU from
+      // the descriptors (material and temperature of the device)
+      if (!ptype.allBuiltIn())
+        reportError("custom mechanisms cannot be combined with phonon baths or plasma screening");
+      check(emcgpu_build_tables(ctx, 0, (int)ptype.getNrEnergyLevels(), ptype.getMaxEnergy(),
+                                temperature, &material),
+            ctx, "emcgpu_build_tables");
+      return;
+    }
     auto rates = ptype.rawRates();
     check(emcgpu_set_rate_tables(ctx, 0, (int)ptype.getNrEnergyLevels(), ptype.getMaxEnergy(),
                                  rates.data()),
           ctx, "emcgpu_set_rate_tables");
+  }
+  /// the updateScreening loop of the hot-carrier driver
+  /// (examples/hotCarrierMHP/hotCarrierMHP.cpp:563-581, 693-703) on the GPU:
+  /// after every step q_s^2 follows the carriers of the given volume [m^3]
+  /// (static permittivity of the shared emcPlasmonScreening), the baths'
+  /// window weights and the tables follow
+  void enablePlasmaScreening(T volume) {
+    if (!ptype.screening())
+      reportError("plasma screening needs a screened mechanism");
+    plasmaVolume = volume;
+    reinitScatterTables();
+  }
+  /// updateScreening + reinitScatterTables from the current particles (the
+  /// seeding before the first step, hotCarrierMHP.cpp:630-637)
+  void updateScreening() { check(emcgpu_update_screening(ctx, 0), ctx, "emcgpu_update_screening"); }
+  /// the q_s^2 the GPU holds [1/m^2]
+  T getScreeningQ2() const {
+    double q2 = 0;
+    check(emcgpu_get_screening(ctx, 0, &q2), ctx, "emcgpu_get_screening");
+    return (T)q2;
   }
   void setSeed(SizeType inSeed) { seed = inSeed; }
   void resetAppliedFieldStrength(T strength) {
@@ -312,6 +386,9 @@ private:
   uint32_t step = 0;
   std::vector<double> obs, lastObs;
   int nObsL = 0, maxV = 1, maxDeg = 1;
+  emcgpu_material material{};
+  std::vector<std::shared_ptr<emcPhononBath<T>>> baths;
+  T plasmaVolume = 0;
 };
 
 }  // namespace emcgpu
