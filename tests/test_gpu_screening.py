@@ -206,3 +206,43 @@ def test_fp32_screened_cooling_matches_fp64():
     assert np.abs(e32 - e64).max() < 3e-3 * e64.max()
     assert np.abs(n32 - n64).max() < 1e-2 * n64.max()
     assert abs(q32 - q64) < 1e-2 * q64
+
+
+def test_batched_screened_instances_are_independent():
+    """Two plasma-screened hot-phonon instances with ragged counts in one
+    arena (each with its own bath, screening and tables, the screening taken
+    from its own tiles of the step partials) evolve exactly as each does
+    alone."""
+    emcgpu = _emc()
+    import copy
+    spec_a = MODELS["screened_hot_polar"]
+    spec_b = copy.deepcopy(spec_a)
+    spec_b["field_kvcm"] = 30.0
+    spec_b["screening"]["volume"] = 1.0e-23
+    counts = [3001, 1500]
+    print("batched screened instances", counts, flush=True)
+    ctx = emcgpu.Context(precision=emcgpu.FP64, n_instances=2)
+    ctx.setup(0, spec_a)
+    ctx.setup(1, spec_b)
+    ctx.resize(counts)
+    for i in range(2):
+        ctx.init_particles_energy(i, 17, 0.5, 0.05)
+        ctx.update_screening(i)
+    ctx.step(0.5e-15, 40, seed=17)
+    both = [(ctx.get_particles(i), ctx.bath_history(i, 0), ctx.get_screening(i)) for i in range(2)]
+    ctx.close()
+    for i, spec in enumerate((spec_a, spec_b)):
+        # alone, at the same instance index (the Philox key includes it)
+        solo = emcgpu.Context(precision=emcgpu.FP64, n_instances=2)
+        solo.setup(0, spec_a)
+        solo.setup(1, spec_b)
+        solo.resize([counts[0] if i == 0 else 0, counts[1] if i == 1 else 0])
+        solo.init_particles_energy(i, 17, 0.5, 0.05)
+        solo.update_screening(i)
+        solo.step(0.5e-15, 40, seed=17)
+        p, h, q = solo.get_particles(i), solo.bath_history(i, 0), solo.get_screening(i)
+        solo.close()
+        assert (p["valley"] == both[i][0]["valley"]).all()
+        assert np.array_equal(p["k"], both[i][0]["k"]) and np.array_equal(p["E"], both[i][0]["E"])
+        assert np.array_equal(h, both[i][1]) and q == both[i][2]
+    assert both[0][2] != both[1][2]
