@@ -278,7 +278,7 @@ def main():
         copy_calibration(args.copy_calibration)
         return 0
     if args.gpus != 1:
-        print("bench.py measures one GPU: the particle step has no multi-GPU path in this build "
+        print("bench.py measures one GPU: the particle step has no multi-GPU path in this library "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         return 2
     from emc_models import MODELS

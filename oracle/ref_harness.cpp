@@ -682,7 +682,7 @@ static void dumpBathKATs() {
     };
     for (int r = 0; r < 6; r++) {
       int n = 50 + 37 * r;
-      // round 3 is absorption-heavy: drives low bins below zero (clamp)
+      // event batch r = 3 is absorption-heavy: drives low bins below zero (clamp)
       double pEm = (r == 3) ? 0.1 : 0.75;
       evn.push_back(n);
       for (int e = 0; e < n; e++) {

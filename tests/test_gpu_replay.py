@@ -1,7 +1,7 @@
 """Replay-mode parity on the GPU (FP64): the HIP kernel consumes the
 reference engine's recorded draws and must reproduce each particle's valley,
 subvalley and scatter-event counts bit-exactly, and k and E within 1e-12
-relative, against the reference's own trajectories
+relative (the FP64 replay tolerance), against the reference's own trajectories
 (golden fixtures) and, at larger sizes, against the oracle.
 """
 import numpy as np
