@@ -577,7 +577,7 @@ emcgpu_status upload_params(emcgpu_ctx *c, double dt_s) {
 int default_ppt(int) { return 4; }
 
 emcgpu_status ensure_partial(emcgpu_ctx *c, int k_steps, int n_tiles) {
-  size_t bytes = (size_t)k_steps * (size_t)std::max(n_tiles, 1) * c->n_obs * sizeof(double);
+  size_t bytes = (size_t)k_steps * (size_t)std::max(n_tiles, 1) * kSubs * c->n_obs * sizeof(double);
   if (bytes > c->partial_bytes) {
     dfreet(c->d_partial);
     HIPCHK(c, hipMalloc((void **)&c->d_partial, bytes));
@@ -786,7 +786,7 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
   for (int done = 0; done < s->n_steps; done += K) {
     const int k = std::min(K, s->n_steps - done);
     a.n_steps = k;
-    a.partial = c->d_partial + (size_t)(done % Kr) * c->n_obs * (size_t)n_tiles;
+    a.partial = c->d_partial + (size_t)(done % Kr) * c->n_obs * (size_t)n_tiles * kSubs;
     a.step0 = s->step0 + (uint32_t)done;
     if (trace) {
       const long long o = (long long)done * (long long)n0;

@@ -21,6 +21,7 @@ constexpr int kMaxDeg = EMCGPU_MAX_DEG;
 constexpr int kMaxZ = EMCGPU_MAX_Z;
 constexpr int kMaxMech = EMCGPU_MAX_MECH;
 constexpr int kTile = 256;  // threads per block
+constexpr int kSubs = kTile / 64;  // waves per block: one step partial each
 // arena unit: every instance starts on a multiple of kUnit particles, so a
 // kernel tile of 256 * PPT particles (PPT in {1, 2, 4, 8}) never straddles
 // two instances
@@ -179,7 +180,7 @@ struct StepArgs {
   int n_steps;   // steps fused in this launch
   uint32_t step0;
   uint32_t key0, key1;
-  double *partial;  // [n_steps][n_obs][n_tiles]
+  double *partial;  // [n_steps][n_obs][n_tiles * kSubs] (one per wave)
   int n_obs;
   int max_v;
   int max_deg;

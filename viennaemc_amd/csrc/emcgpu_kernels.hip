@@ -42,29 +42,29 @@ __device__ __forceinline__ int wave_sum_i(int x) {
 }
 
 constexpr int kObsMax = 2 + kMaxV * (3 + kMaxDeg);
+static_assert(kObsMax <= 64, "one lane per observable of a wave");
 
-// Per-step ensemble sums of one tile (each lane holds PPT particles): event
+// Per-step ensemble sums of one wave (each lane holds PPT particles): event
 // counts, per-valley count / sum E / sum v.dir, per-(valley, subvalley)
-// counts.  Lanes sum their own particles in FP64, waves reduce with
-// shuffles and ballots, the block's waves are added in fixed order, and the
-// tile writes one FP64 partial per value: partial[step][obs][tile].
+// counts.  Lanes sum their own particles in FP64, the wave reduces with
+// shuffles and ballots, and lane o writes value o of the wave's partial:
+// partial[step][obs][tile * kSubs + wave].  No block barrier: the waves of a
+// block run their steps independently.
 template <typename R, int PPT>
 __device__ __forceinline__ void tile_observables(const StepArgs<R> &a, const InstDev<R> *P,
-                                                 double (*wsum)[kObsMax], int tile, int s,
-                                                 const bool (&valid)[PPT], const int (&v)[PPT],
-                                                 const int (&sub)[PPT], const R (&kx)[PPT],
-                                                 const R (&ky)[PPT], const R (&kz)[PPT],
-                                                 const R (&E)[PPT], int nev, int nreal) {
+                                                 int tile, int s, const bool (&valid)[PPT],
+                                                 const int (&v)[PPT], const int (&sub)[PPT],
+                                                 const R (&kx)[PPT], const R (&ky)[PPT],
+                                                 const R (&kz)[PPT], const R (&E)[PPT], int nev,
+                                                 int nreal) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int nv = P->n_valleys;
+  double mine = 0.0;  // the value this lane writes
   const int ev = wave_sum_i(nev);
   const int re = wave_sum_i(nreal);
-  double *ws = wsum[wave];
-  if (lane == 0) {
-    ws[0] = (double)ev;
-    ws[1] = (double)re;
-  }
+  mine = (lane == 0) ? (double)ev : mine;
+  mine = (lane == 1) ? (double)re : mine;
   for (int vv = 0; vv < a.max_v; ++vv) {
     double se = 0.0, sv = 0.0;
     int cnt = 0;
@@ -80,29 +80,20 @@ __device__ __forceinline__ void tile_observables(const StepArgs<R> &a, const Ins
       se = wave_sum(de);
       sv = wave_sum(dv);
     }
-    if (lane == 0) {
-      ws[2 + 3 * vv] = (double)cnt;
-      ws[3 + 3 * vv] = se;
-      ws[4 + 3 * vv] = sv;
-    }
+    mine = (lane == 2 + 3 * vv) ? (double)cnt : mine;
+    mine = (lane == 3 + 3 * vv) ? se : mine;
+    mine = (lane == 4 + 3 * vv) ? sv : mine;
     for (int ss = 0; ss < a.max_deg; ++ss) {
       int c2 = 0;
 #pragma unroll
       for (int j = 0; j < PPT; ++j)
         c2 += (int)__popcll(__ballot(valid[j] && v[j] == vv && sub[j] == ss));
-      if (lane == 0)
-        ws[2 + 3 * a.max_v + vv * a.max_deg + ss] = (double)c2;
+      mine = (lane == 2 + 3 * a.max_v + vv * a.max_deg + ss) ? (double)c2 : mine;
     }
   }
-  __syncthreads();
-  if ((int)threadIdx.x < a.n_obs) {
-    double t = 0.0;
-#pragma unroll
-    for (int w = 0; w < kTile / 64; ++w)
-      t += wsum[w][threadIdx.x];
-    a.partial[((long long)s * a.n_obs + threadIdx.x) * a.n_tiles + tile] = t;
-  }
-  __syncthreads();
+  if (lane < a.n_obs)
+    a.partial[((long long)s * a.n_obs + lane) * ((long long)a.n_tiles * kSubs) +
+              (long long)tile * kSubs + wave] = mine;
 }
 
 // instance of a kernel tile of tile_size particles (tiles never straddle
@@ -151,7 +142,6 @@ __device__ __forceinline__ void tally_flush(const InstDev<R> *P, const uint32_t 
 // ---------------------------------------------------------------------------
 template <typename R, int PPT, bool HOT>
 __global__ __launch_bounds__(kTile) void k_bulk_multi(StepArgs<R> a) {
-  __shared__ double wsum[kTile / 64][kObsMax];
   extern __shared__ uint32_t lds_tally[];
   constexpr int TS = kTile * PPT;
   const int tile = blockIdx.x;
@@ -252,7 +242,7 @@ __global__ __launch_bounds__(kTile) void k_bulk_multi(StepArgs<R> a) {
       if (valid[j])
         tau[j] -= dt;
     }
-    tile_observables<R, PPT>(a, P, wsum, tile, s, valid, v, sub, kx, ky, kz, E, nev, nreal);
+    tile_observables<R, PPT>(a, P, tile, s, valid, v, sub, kx, ky, kz, E, nev, nreal);
   }
   if constexpr (HOT)
     tally_flush(P, lds_tally, a.tally, a.tally_bins);
@@ -277,7 +267,6 @@ __global__ __launch_bounds__(kTile) void k_bulk_multi(StepArgs<R> a) {
 // ---------------------------------------------------------------------------
 template <typename R, bool HOT>
 __global__ __launch_bounds__(kTile) void k_bulk_replay(StepArgs<R> a) {
-  __shared__ double wsum[kTile / 64][kObsMax];
   extern __shared__ uint32_t lds_tally[];
   const int tile = blockIdx.x;
   const int inst = tile_instance(a.unit_inst, tile, kTile);
@@ -329,7 +318,7 @@ __global__ __launch_bounds__(kTile) void k_bulk_replay(StepArgs<R> a) {
         a.tr_E[o] = (double)E[0];
       }
     }
-    tile_observables<R, 1>(a, P, wsum, tile, s, valid, v, sub, kx, ky, kz, E, nev, nreal);
+    tile_observables<R, 1>(a, P, tile, s, valid, v, sub, kx, ky, kz, E, nev, nreal);
   }
   if constexpr (HOT)
     tally_flush(P, lds_tally, a.tally, a.tally_bins);
@@ -390,11 +379,13 @@ __global__ __launch_bounds__(256) void k_reduce_chunks(const double *__restrict_
                                                        int n_chunks, double *__restrict__ chunk_out) {
   __shared__ double ss[256], sc[256];
   const int chunk = blockIdx.x, so = blockIdx.y, inst = blockIdx.z;
-  const long long t0 = inst_start[inst] / tile_size;
-  const long long nt = (inst_count[inst] + tile_size - 1) / tile_size;
+  // the wave partials of the instance's tiles (tiles start on instance
+  // boundaries; every wave of a tile with particles wrote one)
+  const long long t0 = inst_start[inst] / tile_size * kSubs;
+  const long long nt = (inst_count[inst] + tile_size - 1) / tile_size * kSubs;
   const long long lo = (long long)chunk * kChunk;
   const long long hi = (lo + kChunk < nt) ? lo + kChunk : nt;
-  const double *src = partial + (long long)so * n_tiles + t0;
+  const double *src = partial + (long long)so * ((long long)n_tiles * kSubs) + t0;
   double sum = 0.0, comp = 0.0;
   for (long long t = lo + threadIdx.x; t < hi; t += 256)
     neumaier_add(sum, comp, src[t]);
@@ -761,16 +752,16 @@ __device__ void screening_update(const BathInstDev &I, const BathStepArgs<R> &a,
   double vals[3] = {0.0, 0.0, 0.0};  // N, sum E valley 0, count valley 0
   // the instance's tiles: it starts on a tile boundary, tiles past its
   // count hold zeros
-  const long long ts = a.tile_size;
-  const long long tile0 = a.inst_start[I.inst] / ts;
-  const long long tile1 = (a.inst_start[I.inst] + a.inst_count[I.inst] + ts - 1) / ts;
-  const long long tend = (tile1 < a.n_tiles) ? tile1 : a.n_tiles;
+  const long long ts = a.tile_size, nsub = (long long)a.n_tiles * kSubs;
+  const long long tile0 = a.inst_start[I.inst] / ts * kSubs;
+  const long long tile1 = (a.inst_start[I.inst] + a.inst_count[I.inst] + ts - 1) / ts * kSubs;
+  const long long tend = (tile1 < nsub) ? tile1 : nsub;
 #pragma unroll 4
   for (long long t = tile0 + threadIdx.x; t < tend; t += blockDim.x) {
     for (int v = 0; v < I.n_valleys; ++v)
-      vals[0] += a.partial[(long long)(2 + 3 * v) * a.n_tiles + t];
-    vals[1] += a.partial[3LL * a.n_tiles + t];
-    vals[2] += a.partial[2LL * a.n_tiles + t];
+      vals[0] += a.partial[(long long)(2 + 3 * v) * nsub + t];
+    vals[1] += a.partial[3LL * nsub + t];
+    vals[2] += a.partial[2LL * nsub + t];
   }
   // the three sums in one tree (fixed order, deterministic)
   double *r0 = sh.vmax, *r1 = sh.old_n, *r2 = sh.new_n;
@@ -1100,7 +1091,7 @@ hipError_t launch_rebuild(const BathStepArgs<R> &a, int n_jobs, hipStream_t st) 
 
 // number of chunk pairs per output value for the largest instance
 int reduce_chunks(long long max_count, int tile_size) {
-  const long long nt = (max_count + tile_size - 1) / tile_size;
+  const long long nt = (max_count + tile_size - 1) / tile_size * kSubs;
   return (int)((nt + kChunk - 1) / kChunk > 0 ? (nt + kChunk - 1) / kChunk : 1);
 }
 
