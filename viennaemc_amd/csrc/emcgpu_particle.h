@@ -182,8 +182,8 @@ __device__ __forceinline__ R normk_of(const ValleyDev<R> &V, R E) {
 
 template <typename R>
 __device__ __forceinline__ R vdir_of(const ValleyDev<R> &V, int sub, R kx, R ky, R kz, R E) {
-  const R g = E * (R(1) + V.alpha * E);
-  const R npf = Math<R>::sqrt(R(1) + R(4) * V.alpha * g);
+  // sqrt(1 + 4 alpha gamma(E)) = 1 + 2 alpha E for gamma = E (1 + alpha E)
+  const R npf = R(1) + R(2) * V.alpha * E;
   const R d = kx * V.wdir[sub][0] + ky * V.wdir[sub][1] + kz * V.wdir[sub][2];
   return Math<R>::div(V.cv * d, npf);
 }
