@@ -28,17 +28,42 @@ namespace emcgpu {
 // ---------------------------------------------------------------------------
 // wave-level reductions (wave64)
 // ---------------------------------------------------------------------------
+// DPP moves (GCN/CDNA data-parallel primitives): quad_perm [1,0,3,2] and
+// [2,3,0,1], row_shr:4 and :8 within rows of 16 lanes, row_bcast:15 (rows 1
+// and 3) and row_bcast:31 (rows 2 and 3); lanes without a source read 0, so
+// after the six steps lane 63 holds the wave's sum, which readlane makes
+// uniform.  VALU moves instead of LDS permutes.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_i(int x) {
+  return __builtin_amdgcn_update_dpp(0, x, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_d(double x) {
+  const long long b = __builtin_bit_cast(long long, x);
+  const int lo = dpp_i<CTRL, ROWS>((int)(b & 0xffffffffll));
+  const int hi = dpp_i<CTRL, ROWS>((int)(b >> 32));
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
 __device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1)
-    x += __shfl_xor(x, off, 64);
-  return x;
+  x = x + dpp_d<0xb1, 0xf>(x);
+  x = x + dpp_d<0x4e, 0xf>(x);
+  x = x + dpp_d<0x114, 0xf>(x);
+  x = x + dpp_d<0x118, 0xf>(x);
+  x = x + dpp_d<0x142, 0xa>(x);
+  x = x + dpp_d<0x143, 0xc>(x);
+  const long long b = __builtin_bit_cast(long long, x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), 63);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
 }
 __device__ __forceinline__ int wave_sum_i(int x) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1)
-    x += __shfl_xor(x, off, 64);
-  return x;
+  x = x + dpp_i<0xb1, 0xf>(x);
+  x = x + dpp_i<0x4e, 0xf>(x);
+  x = x + dpp_i<0x114, 0xf>(x);
+  x = x + dpp_i<0x118, 0xf>(x);
+  x = x + dpp_i<0x142, 0xa>(x);
+  x = x + dpp_i<0x143, 0xc>(x);
+  return __builtin_amdgcn_readlane(x, 63);
 }
 
 constexpr int kObsMax = 2 + kMaxV * (3 + kMaxDeg);
