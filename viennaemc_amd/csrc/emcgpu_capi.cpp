@@ -217,6 +217,10 @@ void lower_instance(const emcgpu_ctx *c, const Instance &in, double dt_s, InstDe
     }
   }
   P.qs2 = (R)(in.qs2 * kToNm * kToNm);
+  for (const auto &m : in.mechs)
+    P.win_lds |= (is_scr_hot_kind(m.kind) && m.q_resolved && m.q_resolved_angle) ? 1 : 0;
+  if (!c->d_win)
+    P.win_lds = 0;
   for (size_t b = 0; b < in.baths.size() && c->d_win; b++)
     P.bath_win[b] = (const R *)c->d_win + 2 * (in.bath_off[b] + in.bath_index0 + (long long)b);
   P.n_baths = (int)in.baths.size();
@@ -707,6 +711,10 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
     a.tally = c->d_tally;
     a.tally_bins = c->tally_bins;
     a.lds_bins = c->lds_bins;
+    for (const auto &in : c->inst)
+      for (const auto &m : in.mechs)
+        if (is_scr_hot_kind(m.kind) && m.q_resolved && m.q_resolved_angle)
+          a.lds_win = std::max(a.lds_win, 2 * (in.total_bins + (int)in.baths.size()));
   }
   const size_t n0 = c->inst[0].count;
 

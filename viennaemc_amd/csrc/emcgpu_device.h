@@ -89,8 +89,11 @@ struct InstDev {
   R bath_dq[kMaxBaths];  // bin width [1/nm]
   long long gbin_off;
   R qs2;  // plasmon screening q_s^2 [1/nm^2]
-  // window sums of each bath (cumW[nb + 1] then cumWN[nb + 1], q in 1/m)
+  // window sums of each bath (cumW[nb + 1] then cumWN[nb + 1], q in 1/m);
+  // the instance's baths are contiguous from bath_win[0], 2 (total_bins +
+  // n_baths) values
   const R *bath_win[kMaxBaths];
+  int win_lds;  // a q-resolved angle reads them: the step kernel stages them in LDS
 };
 
 // constants of one LO phonon bath (emcPhononBath<double>)
@@ -200,6 +203,7 @@ struct StepArgs {
   unsigned long long *tally;
   long long tally_bins;
   int lds_bins;  // bins held in LDS per block (max over instances)
+  int lds_win;   // window-sum values staged in LDS per block (max over instances)
   int ext_kinds; // some instance uses a screened Froehlich kind
 };
 

@@ -136,6 +136,14 @@ template <typename R>
 __device__ __forceinline__ void tally_clear(const InstDev<R> *P, uint32_t *lds_tally) {
   for (int i = threadIdx.x; i < 2 * P->total_bins; i += kTile)
     lds_tally[i] = 0u;
+  // the baths' window sums, read by the bisection of the q-resolved angle,
+  // from LDS: they change only between steps (one step per launch here)
+  if (P->win_lds) {
+    R *dst = lds_window<R>(P, lds_tally);
+    const R *src = P->bath_win[0];
+    for (int i = threadIdx.x; i < 2 * (P->total_bins + P->n_baths); i += kTile)
+      dst[i] = src[i];
+  }
   __syncthreads();
 }
 
@@ -167,7 +175,7 @@ __device__ __forceinline__ void tally_flush(const InstDev<R> *P, const uint32_t 
 // ---------------------------------------------------------------------------
 template <typename R, int PPT, bool HOT>
 __global__ __launch_bounds__(kTile) void k_bulk_multi(StepArgs<R> a) {
-  extern __shared__ uint32_t lds_tally[];
+  extern __shared__ __attribute__((aligned(16))) uint32_t lds_tally[];
   constexpr int TS = kTile * PPT;
   const int tile = blockIdx.x;
   const int inst = tile_instance(a.unit_inst, tile, TS);
@@ -292,7 +300,7 @@ __global__ __launch_bounds__(kTile) void k_bulk_multi(StepArgs<R> a) {
 // ---------------------------------------------------------------------------
 template <typename R, bool HOT>
 __global__ __launch_bounds__(kTile) void k_bulk_replay(StepArgs<R> a) {
-  extern __shared__ uint32_t lds_tally[];
+  extern __shared__ __attribute__((aligned(16))) uint32_t lds_tally[];
   const int tile = blockIdx.x;
   const int inst = tile_instance(a.unit_inst, tile, kTile);
   const InstDev<R> *P = a.inst + inst;
@@ -1049,7 +1057,10 @@ int tile_size_of(int ppt) { return kTile * ppt; }
 template <typename R, bool HOT>
 hipError_t launch_steps_t(const StepArgs<R> &a, int mode, int ppt, hipStream_t st) {
   const dim3 g(a.n_tiles), b(kTile);
-  const size_t lds = HOT ? sizeof(uint32_t) * 2 * (size_t)a.lds_bins : 0;
+  // tallies, then (8-byte aligned) the staged window sums
+  const size_t lds = HOT ? sizeof(uint32_t) * 2 * (size_t)a.lds_bins + 8 +
+                               sizeof(R) * (size_t)a.lds_win
+                         : 0;
   if (mode == 1) {
     hipLaunchKernelGGL((k_bulk_replay<R, HOT>), g, b, lds, st, a);
   } else {

@@ -277,6 +277,13 @@ __device__ __forceinline__ R screened_cos_theta(R ki, R kf, R qs2, R u) {
   return (R(-1) < c) ? c : R(-1); // std::max(-1, c)
 }
 
+// the staged window sums of the block's instance: after its 2 total_bins
+// tally words (8-byte aligned: the dynamic LDS base is 16-byte aligned)
+template <typename R>
+__device__ __forceinline__ R *lds_window(const InstDev<R> *P, uint32_t *lds_tally) {
+  return (R *)(lds_tally + 2 * P->total_bins);  // an even number of words
+}
+
 // sampleQ (emcPhononBath.hpp:423-458) over a bath's window sums w (cumW at
 // w[0..nb], cumWN at w[nb+1..2nb+1]; their unit cancels): the bins
 // [floor(qmin/dq), ceil(qmax/dq)) clamped to the grid, the bin whose prefix
@@ -366,10 +373,13 @@ __device__ __forceinline__ void final_state(const InstDev<R> *P, const MechDev<R
       kn = normk_of(P->valley[v], E);
       if (m.qres) {
         // the screened hot-phonon kinds with q-resolved angle (:283-290 /
-        // :358-365): q through the bath's distribution
+        // :358-365): q through the bath's distribution, its window sums
+        // staged in LDS by the step kernels
         const int b = m.bath1 - 1;
-        ct = qres_cos_theta(P->bath_win[b], P->bath_nb[b], P->bath_dq[b], ki, kn,
-                            m.shift < R(0), ua);
+        const R *w = (HOT && lds_tally)
+                         ? lds_window<R>(P, lds_tally) + 2 * (P->bath_loff[b] + b)
+                         : P->bath_win[b];
+        ct = qres_cos_theta(w, P->bath_nb[b], P->bath_dq[b], ki, kn, m.shift < R(0), ua);
       } else {
         ct = screened_cos_theta(ki, kn, P->qs2, ua);
       }
