@@ -29,6 +29,8 @@ template <typename R>
 hipError_t launch_bath_step(const BathStepArgs<R> &a, int n_groups, hipStream_t st);
 template <typename R>
 hipError_t launch_rebuild(const BathStepArgs<R> &a, int n_jobs, hipStream_t st);
+hipError_t launch_rates_desc(const RateDescJob *jobs, int n_jobs, int n_levels, double dE,
+                             double *out, hipStream_t st);
 int reduce_chunks(long long max_count, int tile_size);
 hipError_t launch_reduce(const double *partial, int n_steps, int n_tiles, int n_obs, int tile_size,
                          const long long *inst_start, const long long *inst_count, int n_inst,
@@ -1399,6 +1401,116 @@ emcgpu_status emcgpu_build_tables(emcgpu_ctx *c, int instance, int n_levels, dou
     in.rate_data[j] = raw[j];
   }
   return finish_tables(c, in, n_levels, max_energy, raw);
+}
+
+// initScatterTables of every instance with the rates evaluated on the GPU
+// from the descriptors (one lane per level and mechanism), then normalised
+// on the host as emcgpu_build_tables does
+emcgpu_status emcgpu_build_tables_all(emcgpu_ctx *c, int n_levels, double max_energy,
+                                      const double *temperature, const emcgpu_material *mat) {
+  if (!c)
+    return EMCGPU_ERR_ARG;
+  if (c->device < 0)
+    return fail(c, EMCGPU_ERR_STATE, "emcgpu_build_tables_all needs a device");
+  if (n_levels < 1 || n_levels > EMCGPU_MAX_LEVELS || !(max_energy > 0) || !temperature || !mat)
+    return fail(c, EMCGPU_ERR_ARG,
+                "n_levels in [1, %d], max_energy > 0, temperatures and materials required",
+                EMCGPU_MAX_LEVELS);
+  const double dE = max_energy / (double)(std::size_t)n_levels;
+  std::vector<RateDescJob> jobs;
+  std::vector<std::pair<int, int>> owner;  // (instance, mechanism) of each job
+  for (int i = 0; i < c->n_inst; i++) {
+    Instance &in = c->inst[i];
+    if (in.valleys.empty())
+      return fail(c, EMCGPU_ERR_STATE, "instance %d: set the model first", i);
+    if (!(temperature[i] > 0))
+      return fail(c, EMCGPU_ERR_ARG, "instance %d: temperature must be > 0", i);
+    bool dynamic = in.plasma();
+    for (const emcgpu_mech_desc &m : in.mechs)
+      dynamic |= reads_bath(m.kind) || is_scr_hot_kind(m.kind);
+    if (dynamic)
+      return fail(c, EMCGPU_ERR_ARG,
+                  "instance %d: rates that follow a bath or the carriers: use emcgpu_build_tables", i);
+    for (size_t j = 0; j < in.mechs.size(); j++) {
+      const emcgpu_mech_desc &m = in.mechs[j];
+      RateDescJob J{};
+      J.kind = m.kind;
+      J.absorption = (m.kind == EMCGPU_MECH_ZERO_IV_AB || m.kind == EMCGPU_MECH_FIRST_IV_AB ||
+                      m.kind == EMCGPU_MECH_FROEHLICH_AB ||
+                      m.kind == EMCGPU_MECH_SCREENED_FROEHLICH_AB) ? 1 : 0;
+      J.iv = in.vm[m.valley];
+      J.fv = in.vm[(m.kind >= EMCGPU_MECH_ZERO_IV_AB && m.kind <= EMCGPU_MECH_FIRST_IV_EM)
+                       ? m.final_valley : m.valley];
+      J.hw = m.phonon_energy;
+      J.delta = J.fv.bottom - J.iv.bottom;
+      J.qs2 = in.qs2;
+      const double T = temperature[i];
+      switch (m.kind) {
+      case EMCGPU_MECH_ACOUSTIC:
+        J.C = phys::acousticConst(m.coupling, mat[i].rho, mat[i].v_sound, T);
+        break;
+      case EMCGPU_MECH_ZERO_IV_AB:
+      case EMCGPU_MECH_ZERO_IV_EM:
+        J.C = phys::zeroOrderConst(m.coupling, m.phonon_energy, mat[i].rho, T, m.z, J.absorption);
+        break;
+      case EMCGPU_MECH_FIRST_IV_AB:
+      case EMCGPU_MECH_FIRST_IV_EM:
+        J.C = phys::firstOrderConst(m.coupling, m.phonon_energy, mat[i].rho, T, m.z, J.absorption);
+        break;
+      default:
+        J.meff = m.rel_eff_mass * phys::me;
+        J.C = phys::froehlichConst(m.phonon_energy, J.meff, m.eps_hi, m.eps_lo);
+        J.nq = phys::boseEinstein(m.phonon_energy, T);
+        break;
+      }
+      J.out_off = (long long)jobs.size() * n_levels;
+      jobs.push_back(J);
+      owner.emplace_back(i, (int)j);
+    }
+  }
+  if (jobs.size() > 65535)
+    return fail(c, EMCGPU_ERR_ARG, "at most 65535 mechanisms over all instances");
+  (void)hipSetDevice(c->device);
+  std::vector<double> out(jobs.size() * (size_t)n_levels);
+  RateDescJob *d_jobs = nullptr;
+  double *d_out = nullptr;
+  auto cleanup = [&]() {
+    dfreet(d_jobs);
+    dfreet(d_out);
+  };
+  hipError_t e = hipMalloc((void **)&d_jobs, sizeof(RateDescJob) * std::max<size_t>(jobs.size(), 1));
+  if (e == hipSuccess)
+    e = hipMalloc((void **)&d_out, sizeof(double) * std::max<size_t>(out.size(), 1));
+  if (e == hipSuccess && !jobs.empty())
+    e = hipMemcpyAsync(d_jobs, jobs.data(), sizeof(RateDescJob) * jobs.size(),
+                       hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    e = launch_rates_desc(d_jobs, (int)jobs.size(), n_levels, dE, d_out, c->stream);
+  if (e == hipSuccess && !out.empty())
+    e = hipMemcpyAsync(out.data(), d_out, sizeof(double) * out.size(), hipMemcpyDeviceToHost,
+                       c->stream);
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(c->stream);
+  cleanup();
+  if (e != hipSuccess)
+    return fail(c, EMCGPU_ERR_HIP, "table construction on the GPU: %s", hipGetErrorString(e));
+  std::vector<std::vector<std::vector<double>>> raw(c->n_inst);
+  for (int i = 0; i < c->n_inst; i++)
+    raw[i].assign(c->inst[i].mechs.size(), {});
+  for (size_t k = 0; k < jobs.size(); k++)
+    raw[owner[k].first][owner[k].second].assign(out.begin() + (long long)k * n_levels,
+                                                  out.begin() + (long long)(k + 1) * n_levels);
+  for (int i = 0; i < c->n_inst; i++) {
+    Instance &in = c->inst[i];
+    in.temperature = temperature[i];
+    in.rate_data = raw[i];
+    in.hot_C.assign(in.mechs.size(), 0.0);
+    in.mech_n0.assign(in.mechs.size(), 0.0);
+    emcgpu_status st = finish_tables(c, in, n_levels, max_energy, raw[i]);
+    if (st)
+      return st;
+  }
+  return EMCGPU_OK;
 }
 
 emcgpu_status emcgpu_get_tables(const emcgpu_ctx *c, int instance, int valley, double *cum,

@@ -13,6 +13,7 @@
 #include <stdint.h>
 
 #include "../../include/emcgpu.h"
+#include "../include/emcgpu/emcgpuPhysics.hpp"
 
 namespace emcgpu {
 
@@ -127,6 +128,15 @@ struct RebuildJob {
   double nconst[kMaxMech];          // screened, kNConst: Bose occupation
   int every_step;                   // rebuilt after every step (plasma screening)
 };
+// one mechanism's raw rates on the table grid, evaluated on the GPU from its
+// descriptor (kind: emcgpu_mech_kind of a static rate)
+struct RateDescJob {
+  int kind, absorption;
+  double C, nq, hw, delta, meff, qs2;
+  phys::ValleyMass iv, fv;
+  long long out_off;  // first level of the mechanism in the output
+};
+
 // the per-step rebuild over many blocks: kRebuildChunk levels per block
 constexpr int kRebuildChunk = 256;
 constexpr int kRebuildChunks = (EMCGPU_MAX_LEVELS + kRebuildChunk - 1) / kRebuildChunk;

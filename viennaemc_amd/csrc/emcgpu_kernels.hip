@@ -994,6 +994,54 @@ __global__ __launch_bounds__(kBathThreads) void k_bath_refresh(BathStepArgs<R> a
 }
 
 // ---------------------------------------------------------------------------
+// scatter rates from descriptors: one lane per (level, mechanism), the
+// FP64 formulas of emcgpuPhysics.hpp (the reference's operation order; the
+// device's pow / sqrt / log are within an ulp of the host's)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_rates_desc(const RateDescJob *__restrict__ jobs,
+                                                    int n_levels, double dE,
+                                                    double *__restrict__ out) {
+#pragma clang fp contract(off)
+  const RateDescJob &J = jobs[blockIdx.y];
+  const int l = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  if (l >= n_levels)
+    return;
+  const double E = (double)(unsigned long long)(l + 1) * dE;
+  const bool ab = J.absorption != 0;
+  double r = 0.0;
+  switch (J.kind) {
+  case EMCGPU_MECH_ACOUSTIC:
+    r = phys::acousticRate(J.C, J.iv, E);
+    break;
+  case EMCGPU_MECH_ZERO_IV_AB:
+  case EMCGPU_MECH_ZERO_IV_EM:
+    r = phys::zeroOrderRate(J.C, J.fv, E, J.hw, J.delta, ab);
+    break;
+  case EMCGPU_MECH_FIRST_IV_AB:
+  case EMCGPU_MECH_FIRST_IV_EM:
+    r = phys::firstOrderRate(J.C, J.iv, J.fv, E, J.hw, J.delta, ab);
+    break;
+  case EMCGPU_MECH_SCREENED_FROEHLICH_AB:
+  case EMCGPU_MECH_SCREENED_FROEHLICH_EM:
+    r = phys::screenedFroehlichRate(J.C, J.nq, J.meff, J.iv, E, J.hw, J.qs2, ab);
+    break;
+  default:
+    r = phys::froehlichRate(J.C, J.nq, J.meff, J.iv, E, J.hw, ab);
+    break;
+  }
+  out[J.out_off + l] = r;
+}
+
+hipError_t launch_rates_desc(const RateDescJob *jobs, int n_jobs, int n_levels, double dE,
+                             double *out, hipStream_t st) {
+  if (n_jobs <= 0 || n_levels <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_rates_desc, dim3((n_levels + 255) / 256, n_jobs), dim3(256), 0, st, jobs,
+                     n_levels, dE, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // thermal initialisation (emcElectron.hpp:75-90, emcParticleInitialization.hpp:36-51)
 // ---------------------------------------------------------------------------
 // init_mean > 0 selects the prescribed-energy initialiser: E = mean +

@@ -150,7 +150,7 @@ EXPORTS = [
     "emcgpu_device_set_particles", "emcgpu_device_get_particles", "emcgpu_device_set_efield",
     "emcgpu_device_drift", "emcgpu_device_init", "emcgpu_device_step", "emcgpu_device_fields",
     "emcgpu_autotune", "emcgpu_device_averages", "emcgpu_set_screening", "emcgpu_get_screening",
-    "emcgpu_update_screening",
+    "emcgpu_update_screening", "emcgpu_build_tables_all",
 ]
 
 
@@ -170,6 +170,7 @@ def load(path=LIB_PATH):
         "emcgpu_set_model": (i, [vp, i, p(ValleyDesc), i, p(MechDesc), i]),
         "emcgpu_build_tables": (i, [vp, i, i, d, d, p(Material)]),
         "emcgpu_set_rate_tables": (i, [vp, i, i, d, vp]),
+        "emcgpu_build_tables_all": (i, [vp, i, d, p(d), vp]),
         "emcgpu_get_tables": (i, [vp, i, i, vp, p(d), p(i)]),
         "emcgpu_set_field": (i, [vp, i, p(d), d, d]),
         "emcgpu_resize": (i, [vp, p(C.c_size_t)]),
@@ -333,6 +334,21 @@ class Context:
         mat = Material(eps_r, rho, ni, v_sound)
         self._chk(self.L.emcgpu_build_tables(self.h, instance, n_levels, max_energy, T,
                                              C.byref(mat)))
+
+    def build_tables_all(self, n_levels, max_energy, T, materials=None):
+        """Tables of every instance with the rates evaluated on the GPU.
+        T: one temperature or one per instance; materials: None (silicon's
+        constants) or a list of dicts with rho / v_sound / eps_r / ni."""
+        n = self.n_instances
+        temps = np.ascontiguousarray(np.broadcast_to(np.asarray(T, dtype=np.float64), (n,)))
+        mats = (Material * n)()
+        for k in range(n):
+            m = (materials or [{}] * n)[k]
+            mats[k] = Material(m.get("eps_r", 11.8), m.get("rho", 2329.0), m.get("ni", 1.45e16),
+                               m.get("v_sound", 9040.0))
+        self._chk(self.L.emcgpu_build_tables_all(self.h, n_levels, max_energy,
+                                                 temps.ctypes.data_as(C.POINTER(C.c_double)),
+                                                 C.cast(mats, C.c_void_p)))
 
     def set_rate_tables(self, instance, max_energy, rates):
         r = np.ascontiguousarray(rates, dtype=np.float64)

@@ -11,6 +11,14 @@
 #include <cmath>
 #include <cstddef>
 
+// the rate formulas also run on the GPU (table construction from
+// descriptors) when this header is compiled as HIP
+#if defined(__HIPCC__)
+#define EMCGPU_HD __host__ __device__
+#else
+#define EMCGPU_HD
+#endif
+
 namespace emcgpu {
 namespace phys {
 
@@ -24,8 +32,8 @@ constexpr double me = 9.11e-31;
 
 enum ValleyKind { PARA_ISO = 0, NONPARA_ISO = 1, PARA_ANISO = 2, NONPARA_ANISO = 3 };
 
-inline bool isAniso(int kind) { return kind == PARA_ANISO || kind == NONPARA_ANISO; }
-inline bool isNonPara(int kind) { return kind == NONPARA_ISO || kind == NONPARA_ANISO; }
+EMCGPU_HD inline bool isAniso(int kind) { return kind == PARA_ANISO || kind == NONPARA_ANISO; }
+EMCGPU_HD inline bool isNonPara(int kind) { return kind == NONPARA_ISO || kind == NONPARA_ANISO; }
 
 // Derived quantities of a valley (ValleyTypes/*: initEffMasses, initVogtFactor)
 struct ValleyMass {
@@ -56,9 +64,9 @@ struct ValleyMass {
     }
   }
   // getGamma (emcNonParabolicAnistropValley.hpp:123)
-  double gamma(double E) const { return isNonPara(kind) ? E * (1 + alpha * E) : E; }
+  EMCGPU_HD double gamma(double E) const { return isNonPara(kind) ? E * (1 + alpha * E) : E; }
   // getEffMassDOS at the default E = 0 used by every rate (:84-86)
-  double dosMass0() const {
+  EMCGPU_HD double dosMass0() const {
     return isNonPara(kind) ? effMassDOS * std::pow(1 + 2 * alpha * 0.0, 3.) : effMassDOS;
   }
 };
@@ -69,7 +77,7 @@ inline double acousticConst(double sigma, double rho, double vSound, double T) {
   return std::sqrt(2.0 * q) * std::pow(sigma * q, 2) * kB * T / (pi * cL * std::pow(hbar, 4));
 }
 // emcAcousticScatterMechanism.hpp:49-56
-inline double acousticRate(double C, const ValleyMass &v, double E) {
+EMCGPU_HD inline double acousticRate(double C, const ValleyMass &v, double E) {
   double md = v.dosMass0();
   double alpha = v.alpha;
   double gamma = v.gamma(E);
@@ -94,7 +102,7 @@ inline double firstOrderConst(double defPot, double phEnergy, double rho, double
 }
 
 // zero-order rate (:103-117 absorption, :246-260 emission)
-inline double zeroOrderRate(double C, const ValleyMass &fin, double E, double hw, double delta,
+EMCGPU_HD inline double zeroOrderRate(double C, const ValleyMass &fin, double E, double hw, double delta,
                             bool absorption) {
   double Ef = absorption ? E + hw - delta : E - hw - delta;
   if (Ef > 0) {
@@ -106,7 +114,7 @@ inline double zeroOrderRate(double C, const ValleyMass &fin, double E, double hw
   return 0;
 }
 // first-order rate (emcFirstOrderInterValleyScatterMechanism.hpp:102-119, :250-267)
-inline double firstOrderRate(double C, const ValleyMass &ini, const ValleyMass &fin, double E,
+EMCGPU_HD inline double firstOrderRate(double C, const ValleyMass &ini, const ValleyMass &fin, double E,
                              double hw, double delta, bool absorption) {
   double Ef = absorption ? E + hw - delta : E - hw - delta;
   if (Ef > 0) {
@@ -130,7 +138,7 @@ inline double boseEinstein(double hw, double T) {
   return 1.0 / (std::exp(x) - 1.0);
 }
 // Froehlich rates (emcFroehlichInteraction.hpp:95-107, :167-181)
-inline double froehlichRate(double C, double nq, double effMass, const ValleyMass &v, double E,
+EMCGPU_HD inline double froehlichRate(double C, double nq, double effMass, const ValleyMass &v, double E,
                             double hw, bool absorption) {
   if (!absorption && E <= hw)
     return 0.0;
@@ -148,7 +156,7 @@ inline double froehlichRate(double C, double nq, double effMass, const ValleyMas
 // screenedFroehlichLogFactor (emcScreenedFroehlichInteraction.hpp:52-62):
 // 1/2 ln((q+^2 + qs2) / (q-^2 + qs2)), q+- = kI +- kF; qs2 <= 0 is the bare
 // ln(q+ / q-), 0 where q- vanishes
-inline double screenedLogFactor(double kI, double kF, double qs2) {
+EMCGPU_HD inline double screenedLogFactor(double kI, double kF, double qs2) {
   const double p2 = (kI + kF) * (kI + kF), m2 = (kI - kF) * (kI - kF);
   if (qs2 <= 0.0)
     return m2 <= 0.0 ? 0.0 : 0.5 * std::log(p2 / m2);
@@ -156,7 +164,7 @@ inline double screenedLogFactor(double kI, double kF, double qs2) {
 }
 // screened Froehlich rates (emcScreenedFroehlichInteraction.hpp:128-138,
 // :184-196): the Froehlich rate with the screened log factor
-inline double screenedFroehlichRate(double C, double nq, double effMass, const ValleyMass &v,
+EMCGPU_HD inline double screenedFroehlichRate(double C, double nq, double effMass, const ValleyMass &v,
                                     double E, double hw, double qs2, bool absorption) {
   if (!absorption && E <= hw)
     return 0.0;
