@@ -215,3 +215,17 @@ def test_bath_validation():
     m2 = dict(spec, mechs=[dict(spec["mechs"][0], bath=7)])
     with pytest.raises(emcgpu.EmcGpuError, match="bath index"):
         ctx.set_model(0, m2)
+
+
+def test_build_tables_all_needs_a_device_and_valid_arguments():
+    """emcgpu_build_tables_all evaluates the rates on the GPU: a host-only
+    context, a bad level count or energy, and missing arrays are refused
+    before any work (no compute call without a GPU)."""
+    from viennaemc_amd import emcgpu
+    ctx = emcgpu.Context(precision=emcgpu.FP64, n_instances=2, device=-1)
+    ctx.set_model(0, MODELS["si_jacoboni"])
+    ctx.set_model(1, MODELS["polar_gamma"])
+    with pytest.raises(emcgpu.EmcGpuError, match="needs a device"):
+        ctx.build_tables_all(1000, 2.0, 300.0)
+    L = ctx.L
+    assert L.emcgpu_build_tables_all(None, 10, 1.0, None, None) != 0
