@@ -239,14 +239,6 @@ emcgpu_status emcgpu_set_model(emcgpu_ctx *ctx, int instance,
 
 /* initScatterTables (emcScatterHandler.hpp:91-95, 220-288) with the built-in
  * rate formulas, FP64 on the host; region 0. */
-/* initScatterTables of every instance at once (emcScatterHandler.hpp:220-288)
- * with the rates of the built-in kinds evaluated on the GPU from the
- * descriptors, one lane per (level, mechanism); temperature[n_instances],
- * mat[n_instances].  The FP64 formulas are the host's, the device's pow /
- * sqrt / log within an ulp of them.  Instances whose rates follow a bath or
- * the carriers use emcgpu_build_tables. */
-emcgpu_status emcgpu_build_tables_all(emcgpu_ctx *ctx, int n_levels, double max_energy,
-                                      const double *temperature, const emcgpu_material *mat);
 emcgpu_status emcgpu_build_tables(emcgpu_ctx *ctx, int instance, int n_levels,
                                   double max_energy, double temperature,
                                   const emcgpu_material *material);
@@ -256,6 +248,15 @@ emcgpu_status emcgpu_build_tables(emcgpu_ctx *ctx, int instance, int n_levels,
 emcgpu_status emcgpu_set_rate_tables(emcgpu_ctx *ctx, int instance,
                                      int n_levels, double max_energy,
                                      const double *rates);
+
+/* initScatterTables of every instance at once (emcScatterHandler.hpp:220-288)
+ * with the rates of the built-in kinds evaluated on the GPU from the
+ * descriptors, one lane per (level, mechanism); temperature[n_instances],
+ * mat[n_instances].  The FP64 formulas are the host's, the device's pow /
+ * sqrt / log within an ulp of them.  Instances whose rates follow a bath or
+ * the carriers use emcgpu_build_tables. */
+emcgpu_status emcgpu_build_tables_all(emcgpu_ctx *ctx, int n_levels, double max_energy,
+                                      const double *temperature, const emcgpu_material *mat);
 
 /* normalised cumulative tables cum[table][level] and tau0 [s] of a valley
  * (emcParticleType::getTau, emcParticleType.hpp:181-183) */

@@ -2012,6 +2012,12 @@ emcgpu_status emcgpu_autotune(emcgpu_ctx *c, emcgpu_autotune_args *t) {
     return fail(c, EMCGPU_ERR_STATE, "host-only context cannot tune");
   if (c->n_baths > 0)
     return fail(c, EMCGPU_ERR_STATE, "autotune is not available with phonon baths");
+  // plasma screening updates q_s^2 and the screened tables after every
+  // step; the trials would leave them at the state of the last trial step
+  for (int i = 0; i < c->n_inst; i++)
+    if (c->inst[i].plasma())
+      return fail(c, EMCGPU_ERR_STATE,
+                  "autotune is not available with plasma screening (instance %d)", i);
   if (c->capacity == 0)
     return fail(c, EMCGPU_ERR_STATE, "no particles (call emcgpu_resize first)");
   for (int i = 0; i < c->n_inst; i++)
