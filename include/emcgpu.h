@@ -340,6 +340,25 @@ emcgpu_status emcgpu_get_bath_history(const emcgpu_ctx *ctx, int instance,
  * examples/bulkSimulation/bulkSimulation.cpp:150-156 reduced on the GPU. */
 emcgpu_status emcgpu_step(emcgpu_ctx *ctx, const emcgpu_step_args *args);
 
+/* emcgpu_step without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * per-step partials and reduction pairs use the library's static scratch
+ * (64 MiB per GPU, held until the process ends); the per-step observables
+ * (layout of emcgpu_get_observables, all instances: obs[instance][step][j])
+ * go to the caller's device buffer obs of obs_len doubles, which must hold
+ * n_instances * n_steps * n_obs values.  Production mode only: no replay,
+ * trace or use_graph, no phonon baths or plasma screening, and the model
+ * must already be on the device for args->dt (an earlier emcgpu_step or
+ * emcgpu_init_particles with that dt after the last model, table or field
+ * change).  Steps per launch are capped so a launch's partials fit the
+ * scratch; an ensemble whose single step does not fit is refused
+ * (EMCGPU_ERR_ARG).  Results equal emcgpu_step's.  The scratch is shared by
+ * every context on a GPU and keeps the last call's contents: calls that
+ * overlap on one GPU (other threads or streams) race. */
+emcgpu_status emcgpu_step_noalloc(emcgpu_ctx *ctx, const emcgpu_step_args *args,
+                                  double *obs, size_t obs_len);
+
 /* Per-step ensemble sums of the last emcgpu_step, FP64:
  * out[step][j], j < n_obs, with
  *   j = 0: scatter calls (incl. self scattering); j = 1: real events;

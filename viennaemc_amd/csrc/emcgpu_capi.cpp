@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -45,6 +46,59 @@ using namespace emcgpu;
 using namespace emcgpu::host;
 
 namespace {
+
+// ---- static scratch of emcgpu_step_noalloc ---------------------------------
+// The per-step partials and the reduction pairs of the allocation-free step
+// live in this file-scope array: each GPU that loads the library holds its
+// own copy (64 MiB) until the process ends.  In host code the name is only a
+// stand-in; the device address of each GPU's copy is looked up once per
+// device (scratch_setup, from emcgpu_create with that device current) and
+// only that address is handed to kernels.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __attribute__((used)) __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+struct ScratchTable {
+  std::mutex mu;
+  std::vector<double *> ptr;       // per device ordinal
+  std::vector<std::string> error;  // per device: why the lookup failed
+};
+ScratchTable &scratch_table() {
+  static ScratchTable t;
+  return t;
+}
+
+// the calling thread's current device must be `device`
+void scratch_setup(int device) {
+  ScratchTable &t = scratch_table();
+  std::lock_guard<std::mutex> lock(t.mu);
+  if ((int)t.ptr.size() <= device) {
+    t.ptr.resize(device + 1, nullptr);
+    t.error.resize(device + 1);
+  }
+  if (t.ptr[device])
+    return;
+  double *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    (void)hipGetLastError();
+    t.error[device] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    return;
+  }
+  t.ptr[device] = p;
+}
+
+// the device address of `device`'s scratch, or null with the reason in *why
+double *scratch_ptr(int device, std::string *why) {
+  ScratchTable &t = scratch_table();
+  std::lock_guard<std::mutex> lock(t.mu);
+  if (device < 0 || device >= (int)t.ptr.size()) {
+    *why = "no scratch lookup for this device";
+    return nullptr;
+  }
+  if (!t.ptr[device])
+    *why = t.error[device];
+  return t.ptr[device];
+}
 
 // ---- scaled-unit lowering of an instance -------------------------------
 // k [1/nm] = k [1/m] * 1e-9, t [fs] = t [s] * 1e15
@@ -582,8 +636,17 @@ emcgpu_status upload_params(emcgpu_ctx *c, double dt_s) {
 // the library (measured best on MI355X for the Si bulk workload)
 int default_ppt(int) { return 4; }
 
+// doubles of the per-step partials of one step (one per wave and value)
+size_t partial_per_step(const emcgpu_ctx *c, int n_tiles) {
+  return (size_t)std::max(n_tiles, 1) * kSubs * (size_t)c->n_obs;
+}
+// doubles of the first reduction pass's (sum, compensation) pairs per step
+size_t chunks_per_step(const emcgpu_ctx *c, int n_chunks) {
+  return (size_t)c->n_inst * (size_t)c->n_obs * (size_t)n_chunks * 2;
+}
+
 emcgpu_status ensure_partial(emcgpu_ctx *c, int k_steps, int n_tiles) {
-  size_t bytes = (size_t)k_steps * (size_t)std::max(n_tiles, 1) * kSubs * c->n_obs * sizeof(double);
+  size_t bytes = (size_t)k_steps * partial_per_step(c, n_tiles) * sizeof(double);
   if (bytes > c->partial_bytes) {
     dfreet(c->d_partial);
     HIPCHK(c, hipMalloc((void **)&c->d_partial, bytes));
@@ -625,8 +688,19 @@ void fill_common(const emcgpu_ctx *c, StepArgs<R> &a) {
       a.ext_kinds |= (is_scr_kind(m.kind) || is_scr_hot_kind(m.kind)) ? 1 : 0;
 }
 
+// buffers of emcgpu_step_noalloc: the per-step partials and the reduction
+// pairs in the library's static scratch, the observables in the caller's
+// device buffer
+struct NoAllocBuffers {
+  double *scratch;     // device address of the static scratch array
+  size_t scratch_len;  // its length in doubles
+  double *obs;         // caller's device buffer [instance][step][value]
+  size_t obs_len;      // its length in doubles
+};
+
 template <typename R>
-emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
+emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s,
+                        const NoAllocBuffers *na = nullptr) {
   if (c->capacity == 0)
     return fail(c, EMCGPU_ERR_STATE, "no particles (call emcgpu_resize first)");
   for (int i = 0; i < c->n_inst; i++) {
@@ -645,6 +719,25 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
     return fail(c, EMCGPU_ERR_ARG, "dt must be > 0 and n_steps >= 0");
   if (s->n_steps == 0)
     return EMCGPU_OK;
+  if (na) {
+    // everything that would allocate is refused: the model must be on the
+    // device for this dt already, no replay / trace / graph, no baths or
+    // plasma screening (their per-step buffers)
+    if (replay || s->trace_valley || s->trace_sub || s->trace_events || s->trace_k ||
+        s->trace_energy || s->use_graph)
+      return fail(c, EMCGPU_ERR_ARG,
+                  "emcgpu_step_noalloc: production mode only (no replay, trace or graph)");
+    if (c->n_baths > 0)
+      return fail(c, EMCGPU_ERR_STATE, "emcgpu_step_noalloc: phonon baths are not supported");
+    for (const auto &in : c->inst)
+      if (in.plasma())
+        return fail(c, EMCGPU_ERR_STATE, "emcgpu_step_noalloc: plasma screening is not supported");
+    if (c->params_dirty || !c->d_params || s->dt != c->params_dt)
+      return fail(c, EMCGPU_ERR_STATE,
+                  "emcgpu_step_noalloc: the model is not on the device for dt = %g s (run "
+                  "emcgpu_step once with this dt after the last model, table or field change)",
+                  s->dt);
+  }
   emcgpu_status st = upload_params<R>(c, s->dt);
   if (st)
     return st;
@@ -668,7 +761,7 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
   const int every = s->rebuild_every > 1 ? s->rebuild_every : 1;
   // steps whose per-step partials are reduced by one pair of reduction
   // launches (a launch's steps, or up to 64 one-step launches with baths)
-  const int Kr = dyn ? std::min(s->n_steps, 64) : K;
+  int Kr = dyn ? std::min(s->n_steps, 64) : K;
   if (baths && !s->bath_record_only) {
     const size_t hb = sizeof(double) * (size_t)c->n_baths * (size_t)s->n_steps;
     if (hb > c->bath_hist_bytes) {
@@ -687,19 +780,44 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
                                  : default_ppt(c->precision);
   const int tile_size = kTile * ppt;
   const int n_tiles = (int)(c->capacity / tile_size);
-  if ((st = ensure_partial(c, Kr, n_tiles)) || (st = ensure_obs(c, s->n_steps)))
-    return st;
   long long max_count = 0;
   for (const auto &in : c->inst)
     max_count = std::max<long long>(max_count, (long long)in.count);
   const int n_chunks = reduce_chunks(max_count, tile_size);
-  {
-    const size_t bytes = (size_t)c->n_inst * Kr * c->n_obs * n_chunks * 2 * sizeof(double);
+  double *d_partial = nullptr, *d_chunks = nullptr, *d_obs = nullptr;
+  if (na) {
+    // as many steps per launch as the scratch holds partials for (results
+    // do not depend on the launch shape)
+    const size_t per_step = partial_per_step(c, n_tiles) + chunks_per_step(c, n_chunks);
+    const size_t kmax = na->scratch_len / per_step;
+    if (kmax < 1)
+      return fail(c, EMCGPU_ERR_ARG,
+                  "emcgpu_step_noalloc: %zu bytes of scratch per step needed, the library's "
+                  "scratch holds %zu bytes (fewer particles or instances per context)",
+                  per_step * sizeof(double), na->scratch_len * sizeof(double));
+    if ((size_t)K > kmax)
+      K = (int)kmax;
+    Kr = K;
+    const size_t need = (size_t)c->n_inst * (size_t)s->n_steps * (size_t)c->n_obs;
+    if (!na->obs || na->obs_len < need)
+      return fail(c, EMCGPU_ERR_ARG,
+                  "emcgpu_step_noalloc: the observable buffer needs %zu doubles "
+                  "(instances x steps x values), got %zu", need, na->obs ? na->obs_len : 0);
+    d_partial = na->scratch;
+    d_chunks = na->scratch + (size_t)Kr * partial_per_step(c, n_tiles);
+    d_obs = na->obs;
+  } else {
+    if ((st = ensure_partial(c, Kr, n_tiles)) || (st = ensure_obs(c, s->n_steps)))
+      return st;
+    const size_t bytes = (size_t)Kr * chunks_per_step(c, n_chunks) * sizeof(double);
     if (bytes > c->chunk_bytes) {
       dfreet(c->d_chunks);
       HIPCHK(c, hipMalloc((void **)&c->d_chunks, bytes));
       c->chunk_bytes = bytes;
     }
+    d_partial = c->d_partial;
+    d_chunks = c->d_chunks;
+    d_obs = c->d_obs;
   }
 
   StepArgs<R> a;
@@ -707,7 +825,7 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
   a.n_tiles = n_tiles;
   a.key0 = (uint32_t)s->seed;
   a.key1 = (uint32_t)(s->seed >> 32);
-  a.partial = c->d_partial;
+  a.partial = d_partial;
   a.dt_s = s->dt;
   if (baths) {
     a.tally = c->d_tally;
@@ -796,7 +914,7 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
   for (int done = 0; done < s->n_steps; done += K) {
     const int k = std::min(K, s->n_steps - done);
     a.n_steps = k;
-    a.partial = c->d_partial + (size_t)(done % Kr) * c->n_obs * (size_t)n_tiles * kSubs;
+    a.partial = d_partial + (size_t)(done % Kr) * partial_per_step(c, n_tiles);
     a.step0 = s->step0 + (uint32_t)done;
     if (trace) {
       const long long o = (long long)done * (long long)n0;
@@ -833,9 +951,9 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s) {
     // per-step partials of a batch of Kr steps are reduced together
     const int in_batch = (done % Kr) + k;
     if (in_batch == Kr || done + k == s->n_steps)
-      HIPCHK_CLEAN(launch_reduce(c->d_partial, in_batch, n_tiles, c->n_obs, tile_size,
+      HIPCHK_CLEAN(launch_reduce(d_partial, in_batch, n_tiles, c->n_obs, tile_size,
                                  c->d_inst_start, c->d_inst_count, c->n_inst, n_chunks,
-                                 c->d_chunks, c->d_obs, done + k - in_batch, s->n_steps,
+                                 d_chunks, d_obs, done + k - in_batch, s->n_steps,
                                  c->stream));
   }
   if (graph) {
@@ -1059,6 +1177,9 @@ emcgpu_status emcgpu_create(const emcgpu_config *cfg, emcgpu_ctx **out) {
     return fail(nullptr, EMCGPU_ERR_ARG, "device %d out of range [0, %d)", cfg->device, ndev);
   if ((e = hipSetDevice(cfg->device)) != hipSuccess)
     return fail(nullptr, EMCGPU_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+  // the allocation-free step's scratch address on this device (once per
+  // device; a failure is reported by emcgpu_step_noalloc)
+  scratch_setup(cfg->device);
   emcgpu_ctx *c = new emcgpu_ctx();
   c->device = cfg->device;
   c->precision = cfg->precision;
@@ -1998,6 +2119,26 @@ emcgpu_status emcgpu_step(emcgpu_ctx *c, const emcgpu_step_args *s) {
     if (c->inst[i].count && !c->inst[i].has_field)
       return fail(c, EMCGPU_ERR_STATE, "instance %d: set the field first", i);
   return c->precision == EMCGPU_FP32 ? run_steps<float>(c, s) : run_steps<double>(c, s);
+}
+
+emcgpu_status emcgpu_step_noalloc(emcgpu_ctx *c, const emcgpu_step_args *s, double *obs,
+                                  size_t obs_len) {
+  if (!c || !s)
+    return fail(c, EMCGPU_ERR_ARG, "ctx and args must be non-NULL");
+  if (c->device < 0)
+    return fail(c, EMCGPU_ERR_STATE, "host-only context cannot step");
+  (void)hipSetDevice(c->device);
+  for (int i = 0; i < c->n_inst; i++)
+    if (c->inst[i].count && !c->inst[i].has_field)
+      return fail(c, EMCGPU_ERR_STATE, "instance %d: set the field first", i);
+  std::string why;
+  double *p = scratch_ptr(c->device, &why);
+  if (!p)
+    return fail(c, EMCGPU_ERR_HIP, "emcgpu_step_noalloc: no scratch on device %d (%s)",
+                c->device, why.c_str());
+  const NoAllocBuffers na{p, kNoAllocScratchBytes / sizeof(double), obs, obs_len};
+  return c->precision == EMCGPU_FP32 ? run_steps<float>(c, s, &na)
+                                     : run_steps<double>(c, s, &na);
 }
 
 // Launch-shape autotuner: production trials of the current ensemble for

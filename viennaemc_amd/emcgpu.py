@@ -150,7 +150,7 @@ EXPORTS = [
     "emcgpu_device_set_particles", "emcgpu_device_get_particles", "emcgpu_device_set_efield",
     "emcgpu_device_drift", "emcgpu_device_init", "emcgpu_device_step", "emcgpu_device_fields",
     "emcgpu_autotune", "emcgpu_device_averages", "emcgpu_set_screening", "emcgpu_get_screening",
-    "emcgpu_update_screening", "emcgpu_build_tables_all",
+    "emcgpu_update_screening", "emcgpu_build_tables_all", "emcgpu_step_noalloc",
 ]
 
 
@@ -179,6 +179,7 @@ def load(path=LIB_PATH):
         "emcgpu_get_particles": (i, [vp, i, vp, vp, vp, vp, vp]),
         "emcgpu_init_particles": (i, [vp, i, C.c_uint64, d]),
         "emcgpu_step": (i, [vp, p(StepArgs)]),
+        "emcgpu_step_noalloc": (i, [vp, p(StepArgs), vp, C.c_size_t]),
         "emcgpu_get_observables": (i, [vp, i, vp, p(i), p(i)]),
         "emcgpu_obs_layout": (i, [vp, p(i), p(i), p(i)]),
         "emcgpu_get_timing": (i, [vp, p(d), p(C.c_longlong), i]),
@@ -507,6 +508,22 @@ class Context:
             a.trace_energy = _ptr(tr["E"])
         self._chk(self.L.emcgpu_step(self.h, C.byref(a)))
         return tr
+
+    def step_noalloc(self, dt, n_steps, obs_ptr, obs_len, seed=0, step0=0,
+                     steps_per_launch=0, ppl=0, timing=False):
+        """emcgpu_step_noalloc: production steps with no device allocation;
+        the per-step sums go to the caller's device buffer obs_ptr of
+        obs_len doubles, laid out [instance][step][value]."""
+        a = StepArgs()
+        a.dt = dt
+        a.n_steps = n_steps
+        a.steps_per_launch = steps_per_launch
+        a.mode = PRODUCTION
+        a.seed = seed
+        a.step0 = step0
+        a.timing = 1 if timing else 0
+        a.particles_per_lane = ppl
+        self._chk(self.L.emcgpu_step_noalloc(self.h, C.byref(a), C.c_void_p(obs_ptr), obs_len))
 
     def observables(self, instance=0):
         """Raw per-step sums, shape (n_steps, n_obs)."""
