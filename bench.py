@@ -8,13 +8,24 @@ levels on (0, 2] eV), 50 kV/cm along [100], dt = 1 fs.  A "step" advances
 all particles by one dt including every scatter event inside it and the
 per-step ensemble sums (drift velocity, mean energy, subvalley occupancy).
 
-value      = FP32 (batched kernel, fused steps) particle-updates/s with the
-             state resident in HBM; the FP64 plain kernel (one step per
-             launch) is measured beside it.
-roofline   = the FP32 step kernel: algorithmic bytes per launch / average
-             launch time (HIP events on the library's stream).
+value      = FP32 (fused steps) particle-updates/s with the state resident
+             in HBM, buffers sized before the timed call; the FP64 plain
+             kernel (one step per launch) is measured beside it.
+roofline   = the FP32 step kernel with the timed call's launch shape: it is
+             bound by vector-ALU issue, so `achieved` is its VALU lane-
+             instruction rate (instructions per particle-update from the
+             committed rocprof counter summary x updates per launch / the
+             launch time measured here with HIP events) against the issue
+             peak; the HBM view (measured traffic and the 48 B per update
+             equivalent) is reported beside it.
 cpu_baseline = the oracle's FP64 restatement of the reference loop with the
-             reference engine (one mt19937_64 per particle), all host cores.
+             reference engine (one mt19937_64 per particle) on the host
+             threads this process may use.
+Other legs (each a key of the JSON line): the pathwise FP32-vs-FP64
+precision statement, the plain 10k-electron run of the CPU reference path
+(C1) with the CPU timed beside it, the 64-instance sweep, hot-phonon
+cooling (plain and screened) with the bath energy balance, the 2-D
+resistor, and CPU legs of those workloads.
 """
 import argparse
 import json
@@ -25,13 +36,20 @@ import time
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
-for p in (ROOT, os.path.join(ROOT, "tests")):
-    if p not in sys.path:
-        sys.path.insert(0, p)
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_FP32 = 48  # k(3) + E + tau + valley|sub, 4 B each, read + write
 BYTES_FP64 = 88  # k(3) + E + tau 8 B each + valley|sub 4 B, read + write
+# vector-ALU issue peak: 256 CUs, one wave64 VALU instruction per CU and
+# cycle (4 SIMDs, 4 cycles each), 2.4 GHz peak engine clock -> lane-
+# instructions per second
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+DT = 1e-15
+SEED = 20261202
+LEGS = ["headline", "fp64", "fp32_plain", "precision", "c1", "batched", "hot", "hot_screened",
+        "device", "cpu"]
 
 
 def parse():
@@ -41,32 +59,29 @@ def parse():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--particles", type=int, default=10_000_000)
     ap.add_argument("--spl", type=int, default=0, help="FP32 steps fused per launch (0 = auto)")
-    ap.add_argument("--fp64-steps", type=int, default=0, help="FP64 timed steps (0 = --steps)")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--no-plain", action="store_true", help="skip the FP32 one-step-per-launch leg")
     ap.add_argument("--ppl", type=int, default=0,
                     help="particles per GPU lane of the production kernel (0 = library default)")
-    ap.add_argument("--no-batched", action="store_true", help="skip the 64-instance sweep leg")
+    ap.add_argument("--fp64-steps", type=int, default=0, help="FP64 timed steps (0 = --steps)")
+    ap.add_argument("--legs", default=",".join(LEGS),
+                    help="comma-separated subset of: " + ", ".join(LEGS))
+    ap.add_argument("--precision-steps", type=int, default=1000)
     ap.add_argument("--batched-steps", type=int, default=2000)
-    ap.add_argument("--no-hot", action="store_true", help="skip the hot-phonon cooling leg")
     ap.add_argument("--hot-steps", type=int, default=2000)
-    ap.add_argument("--no-hot-plain", action="store_true",
-                    help="run only the screened (q-resolved, plasma-screened) hot-phonon leg")
-    ap.add_argument("--no-hot-screened", action="store_true",
-                    help="skip the screened hot-phonon leg")
     ap.add_argument("--hot-particles", type=int, default=4_000_000)
-    ap.add_argument("--no-device", action="store_true", help="skip the 2-D resistor leg")
-    ap.add_argument("--device-only", action="store_true",
-                    help="run only the 2-D resistor leg and print its record")
     ap.add_argument("--device-steps", type=int, default=20000)
     ap.add_argument("--device-particles", type=int, default=2_000_000)
-    ap.add_argument("--device-fp64-steps", type=int, default=2000)
+    ap.add_argument("--device-pair-steps", type=int, default=2000,
+                    help="steps of the FP32 / FP64 resistor pair")
     ap.add_argument("--cpu-particles", type=int, default=1_000_000)
     ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (rocprof byte-counter calibration)")
-    return ap.parse_args()
+    a = ap.parse_args()
+    a.legs = set(x.strip() for x in a.legs.split(",") if x.strip())
+    bad = a.legs - set(LEGS)
+    if bad:
+        ap.error("unknown legs: %s" % ", ".join(sorted(bad)))
+    return a
 
 
 def copy_calibration(mib):
@@ -83,141 +98,294 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}))
 
 
-def run_gpu(emcgpu, spec, precision, n, warmup, steps, spl, seed=20261202, ppl=0):
+def host_threads():
+    """Threads this process may run on: the CPU affinity, capped by
+    OMP_NUM_THREADS when set (the GPU host allots 16 per GPU)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(n, 1)
+
+
+def launch_shape(steps, spl):
+    """steps fused per launch by the library for a call of `steps` steps"""
+    return min(spl if spl > 0 else 64, steps)
+
+
+def run_gpu(emcgpu, spec, precision, n, warmup, steps, spl, seed=SEED, ppl=0):
+    """Thermal init, warm-up, buffers sized for the timed call, then the
+    timed call; a second pass of 4 launches of the same shape timed with HIP
+    events on the library stream gives the kernel's launch duration."""
     ctx = emcgpu.Context(precision=precision)
     ctx.setup(0, spec)
     ctx.resize([n])
     ctx.init_particles(0, seed, spec["T"])
     if warmup:
-        ctx.step(1e-15, warmup, seed=seed, step0=0, steps_per_launch=spl, ppl=ppl)
+        ctx.step(DT, warmup, seed=seed, step0=0, steps_per_launch=spl, ppl=ppl)
+    ctx.reserve(DT, steps, steps_per_launch=spl, ppl=ppl)
     ctx.synchronize()
     t0 = time.perf_counter()
-    ctx.step(1e-15, steps, seed=seed, step0=warmup, steps_per_launch=spl, ppl=ppl)
+    ctx.step(DT, steps, seed=seed, step0=warmup, steps_per_launch=spl, ppl=ppl)
     ctx.synchronize()
     wall = time.perf_counter() - t0
     av = ctx.averages(0)
-    # timed pass for the kernel duration (HIP events around every launch)
-    nt = max(spl, 1) * 4 if spl else 64
+    K = launch_shape(steps, spl)
     ctx.timing(reset=True)
-    ctx.step(1e-15, nt, seed=seed, step0=warmup + steps, steps_per_launch=spl, timing=True,
-             ppl=ppl)
+    ctx.step(DT, 4 * K, seed=seed, step0=warmup + steps, steps_per_launch=K, timing=True, ppl=ppl)
     ms, launches = ctx.timing()
     ctx.close()
-    return dict(wall=wall, av=av, kernel_ms=ms / max(launches, 1), launches=launches,
-                steps_per_launch=(nt / max(launches, 1)))
+    return dict(wall=wall, av=av, kernel_ms=ms / max(launches, 1), launches=launches, K=K)
 
 
-def run_batched(emcgpu, warmup, steps, seed=20261202):
-    """The 64-instance Si bias/temperature sweep in one FP32 arena (tables
-    built per instance in FP64, stored FP32), every call launched as one HIP
-    graph with the launch shape (particles per lane = tile size / 256, steps
-    fused per launch) chosen by emcgpu_autotune on this ensemble;
-    whole-sweep particle-updates/s."""
-    from emc_models import batched_instances
-    inst = batched_instances(64, seed)
-    counts = [n for _, n in inst]
-    ctx = emcgpu.Context(precision=emcgpu.FP32, n_instances=len(inst))
-    t0 = time.perf_counter()
-    for i, (m, _) in enumerate(inst):
-        ctx.setup(i, m)
-    ctx.resize(counts)
-    for i, (m, _) in enumerate(inst):
-        ctx.init_particles(i, seed, m["T"])
-    ctx.synchronize()
-    setup_s = time.perf_counter() - t0
-    # the tables of all instances again: per instance on the host, then in
-    # one launch from the descriptors (the run uses the latter)
-    nl, emax = inst[0][0]["n_levels"], inst[0][0]["max_energy"]
-    mats = [dict(rho=m.get("rho", 2329.0), v_sound=m.get("v_sound", 9040.0)) for m, _ in inst]
-    t0 = time.perf_counter()
-    for i, (m, _) in enumerate(inst):
-        ctx.build_tables(i, nl, emax, m["T"], **mats[i])
-    tables_host_s = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    ctx.build_tables_all(nl, emax, [m["T"] for m, _ in inst], mats)
-    tables_gpu_s = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    tune = ctx.autotune(1e-15, seed=seed, step0=0)
-    tune_s = time.perf_counter() - t0
-    if warmup:
-        ctx.step(1e-15, warmup, seed=seed, step0=0, graph=True)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    ctx.step(1e-15, steps, seed=seed, step0=warmup, graph=True)
-    ctx.synchronize()
-    wall = time.perf_counter() - t0
-    av = [ctx.averages(i) for i in range(len(inst))]
-    ctx.close()
-    total = sum(counts)
-    vd = [float(a["avgV"][-1, 0]) for a in av]
-    return {"updates_per_s": total * steps / wall, "instances": len(inst), "particles": total,
-            "steps": steps, "ms_per_step": wall / steps * 1e3, "setup_s": setup_s,
-            "tables_host_s": tables_host_s, "tables_gpu_s": tables_gpu_s,
-            "vd_min": min(vd), "vd_max": max(vd),
-            "autotune": {"particles_per_lane": tune["best"][0], "steps_per_launch": tune["best"][1],
-                         "tune_s": tune_s,
-                         "trials_ms_per_step": [[p, k, round(ms, 5)] for p, k, ms in tune["trials"]]},
-            "workload": "64 Si bulk instances, field log-U[0.1,200] kV/cm along [100]/[111], "
-                        "T U[77,500] K, N log-U[1e4,2e6], FP32, HIP-graph launch, autotuned shape"}
+PROFILE_MAIN = os.path.join(ROOT, "profiles", "emc_step_si_bulk_10M.md")
 
 
-def run_hot_phonon(emcgpu, n, warmup, steps, precision, seed=20261202, screened=False):
+def profile_figures(path=PROFILE_MAIN):
+    """Figures of a committed rocprof summary: table rows `| figure: name |
+    value ... |` (VALU lane-instructions per particle-update, calibrated HBM
+    bytes per launch of a profiled shape, ...)."""
+    out = {}
+    if not os.path.exists(path):
+        return out
+    for line in open(path):
+        parts = [p.strip() for p in line.split("|")]
+        if len(parts) >= 4 and parts[1].startswith("figure:"):
+            try:
+                out[parts[1][len("figure:"):].strip()] = float(parts[2].split()[0])
+            except ValueError:
+                pass
+    return out
+
+
+# ---------------------------------------------------------------------------
+# legs
+# ---------------------------------------------------------------------------
+def leg_precision(emcgpu, spec, n, steps):
+    """FP32 against FP64 on the same Philox stream (pathwise): 16 equal
+    groups of the 10M electrons, `steps` steps, time averages over the second
+    half; the relative difference of v_d and <E>, its paired standard error
+    and the ensemble's own standard error."""
+    from viennaemc_amd.workloads import group_stats, run_groups_gpu
+    groups = 16
+    per = n // groups
+    r = {p: run_groups_gpu(emcgpu, spec, p, groups, per, steps, DT, SEED, steps // 2)
+         for p in (emcgpu.FP64, emcgpu.FP32)}
+    out = {"groups": groups, "particles": per * groups, "steps": steps,
+           "window": "steps %d-%d" % (steps // 2, steps - 1)}
+    for key, name in (("v", "vd"), ("E", "E")):
+        m64, s64 = group_stats(r[0][key])
+        md, sd = group_stats(r[1][key] - r[0][key])
+        out["delta_%s_rel" % name] = md / abs(m64)
+        out["delta_%s_rel_sigma_paired" % name] = sd / abs(m64)
+        out["%s_rel_sigma_ensemble" % name] = s64 / abs(m64)
+        out["%s_fp64" % name] = m64
+    return out
+
+
+def leg_c1(emcgpu, cpu_threads):
+    """The plain bulk Si run of the CPU reference path (10k e-, 10 kV/cm
+    [100], 300 K, 3000 steps of 1 fs): GPU FP64 plain and FP32, each in 20
+    equal groups for the statistical error, and the oracle's reference-engine
+    loop timed on the same workload (step loop only, all host threads);
+    observables compared at 3 sigma."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import oracle_py as O
+    from viennaemc_amd.workloads import (C1_AVG_FROM, C1_DT, C1_GROUPS, C1_PARTICLES, C1_STEPS,
+                                         field_force, group_stats, run_groups_gpu, si_bulk_c1)
+    spec = si_bulk_c1()
+    per = C1_PARTICLES // C1_GROUPS
+    out = {"workload": "Si bulk, 10k e-, 10 kV/cm [100], 300 K, 3000 x 1 fs, %d groups"
+                       % C1_GROUPS}
+    gpu = {}
+    for p, name, spl, ppl in ((emcgpu.FP64, "fp64_plain", 1, 1), (emcgpu.FP32, "fp32", 0, 0)):
+        r = run_groups_gpu(emcgpu, spec, p, C1_GROUPS, per, C1_STEPS, C1_DT, SEED, C1_AVG_FROM,
+                           steps_per_launch=spl, ppl=ppl)
+        gpu[name] = r
+        out["gpu_%s_updates_per_s" % name] = C1_PARTICLES * C1_STEPS / r["wall"]
+    # CPU: the reference engine's loop, timed on one 10k ensemble
+    O.load()
+    force, d = field_force(spec)
+    Vt = O.thermal_voltage(spec["T"])
+    t = np.zeros(1)
+    O.OracleModel(spec).run_mt(C1_PARTICLES, SEED, Vt, C1_STEPS, C1_DT, force, d, obs=False,
+                               threads=cpu_threads, timing=t)
+    out["cpu_updates_per_s"] = C1_PARTICLES * C1_STEPS / t[0]
+    out["cpu_threads"] = cpu_threads
+    out["cpu_kind"] = "port"
+
+    # CPU observables in equal groups (untimed), streams far apart
+    def one(g):
+        obs, _, sub = O.OracleModel(spec).run_mt(per, SEED + (g << 32), Vt, C1_STEPS, C1_DT,
+                                                  force, d, threads=1, sub_occ=True)
+        return np.concatenate([[obs[C1_AVG_FROM:, 1, 0].mean(), obs[C1_AVG_FROM:, 0, 0].mean()],
+                               sub[C1_AVG_FROM:, 0, :3].mean(axis=0)])
+
+    with ThreadPoolExecutor(max_workers=cpu_threads) as ex:
+        cpu = np.array(list(ex.map(one, range(C1_GROUPS))))
+    mc, sc = group_stats(cpu)
+    names = ["vd", "E", "occ0", "occ1", "occ2"]
+    out["cpu"] = {k: [float(mc[i]), float(sc[i])] for i, k in enumerate(names)}
+    for gname, r in gpu.items():
+        g = np.column_stack([r["v"], r["E"], r["sub"]])
+        mg, sg = group_stats(g)
+        z = (mg - mc) / np.sqrt(sg ** 2 + sc ** 2)
+        out["gpu_" + gname] = {k: [float(mg[i]), float(sg[i])] for i, k in enumerate(names)}
+        out["z_" + gname] = {k: float(z[i]) for i, k in enumerate(names)}
+        out["within_3sigma_" + gname] = bool(np.all(np.abs(z) <= 3.0))
+    return out
+
+
+def leg_batched(emcgpu, warmup, steps, with_fp64=True):
+    """The 64-instance Si bias/temperature sweep in one arena (tables built
+    per instance in FP64, stored in the kinematics' precision), every call
+    launched as one HIP graph with the launch shape chosen by emcgpu_autotune
+    on this ensemble; whole-sweep particle-updates/s.  With FP64: the same
+    sweep in FP64 on the same streams, and per instance the relative
+    difference of v_d and <E> over the second half."""
+    from viennaemc_amd.workloads import batched_instances
+
+    def run(precision, tune):
+        inst = batched_instances(64, SEED)
+        counts = [n for _, n in inst]
+        ctx = emcgpu.Context(precision=precision, n_instances=len(inst))
+        t0 = time.perf_counter()
+        for i, (m, _) in enumerate(inst):
+            ctx.setup(i, m)
+        ctx.resize(counts)
+        for i, (m, _) in enumerate(inst):
+            ctx.init_particles(i, SEED, m["T"])
+        ctx.synchronize()
+        res = {"setup_s": time.perf_counter() - t0}
+        nl, emax = inst[0][0]["n_levels"], inst[0][0]["max_energy"]
+        mats = [dict(rho=m.get("rho", 2329.0), v_sound=m.get("v_sound", 9040.0)) for m, _ in inst]
+        if tune:
+            t0 = time.perf_counter()
+            for i, (m, _) in enumerate(inst):
+                ctx.build_tables(i, nl, emax, m["T"], **mats[i])
+            res["tables_host_s"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ctx.build_tables_all(nl, emax, [m["T"] for m, _ in inst], mats)
+        res["tables_gpu_s"] = time.perf_counter() - t0
+        if tune:
+            t0 = time.perf_counter()
+            res["tune"] = ctx.autotune(DT, seed=SEED, step0=0)
+            res["tune_s"] = time.perf_counter() - t0
+        if warmup:
+            ctx.step(DT, warmup, seed=SEED, step0=0, graph=True)
+        ctx.reserve(DT, steps)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.step(DT, steps, seed=SEED, step0=warmup, graph=True)
+        ctx.synchronize()
+        res["wall"] = time.perf_counter() - t0
+        half = steps // 2
+        res["vd"] = np.array([ctx.averages(i)["avgV"][half:, 0].mean() for i in range(len(inst))])
+        res["E"] = np.array([ctx.averages(i)["avgE"][half:, 0].mean() for i in range(len(inst))])
+        res["total"] = sum(counts)
+        res["fields"] = np.array([m["field_kvcm"] for m, _ in inst])
+        ctx.close()
+        return res
+
+    r32 = run(emcgpu.FP32, True)
+    tune = r32["tune"]
+    out = {"updates_per_s": r32["total"] * steps / r32["wall"], "instances": 64,
+           "particles": r32["total"], "steps": steps, "ms_per_step": r32["wall"] / steps * 1e3,
+           "setup_s": r32["setup_s"], "tables_host_s": r32["tables_host_s"],
+           "tables_gpu_s": r32["tables_gpu_s"],
+           "vd_min": float(r32["vd"].min()), "vd_max": float(r32["vd"].max()),
+           "autotune": {"particles_per_lane": tune["best"][0], "steps_per_launch": tune["best"][1],
+                        "tune_s": r32["tune_s"],
+                        "trials_ms_per_step": [[p, k, round(ms, 5)] for p, k, ms in tune["trials"]]},
+           "workload": "64 Si bulk instances, field log-U[0.1,200] kV/cm along [100]/[111], "
+                       "T U[77,500] K, N log-U[1e4,2e6], FP32, HIP-graph launch, autotuned shape"}
+    if with_fp64:
+        r64 = run(emcgpu.FP64, False)
+        dE = np.abs(r32["E"] - r64["E"]) / np.abs(r64["E"])
+        strong = r32["fields"] >= 1.0  # v_d well above the thermal noise
+        dv = np.abs(r32["vd"] - r64["vd"]) / np.abs(r64["vd"])
+        out["fp64_updates_per_s"] = r64["total"] * steps / r64["wall"]
+        out["fp32_vs_fp64"] = {
+            "delta_E_rel_max": float(dE.max()), "delta_E_rel_median": float(np.median(dE)),
+            "delta_vd_rel_max_fields_ge_1kVcm": float(dv[strong].max()),
+            "delta_vd_rel_median_fields_ge_1kVcm": float(np.median(dv[strong])),
+            "instances_ge_1kVcm": int(strong.sum()),
+            "window": "steps %d-%d, same Philox streams" % (steps // 2, steps - 1)}
+    return out
+
+
+def leg_hot(emcgpu, n, warmup, steps, precision, screened=False):
     """Photoexcited cooling with a hot-phonon bath: every step is the
     particle step with LDS bath tallies, the bath update and the table
-    rebuild (one HIP graph per call); whole-job particle-updates/s.
+    rebuild (one HIP graph per call); whole-job particle-updates/s, and the
+    FP64 energy balance of the timed steps: carrier energy lost = net emitted
+    LO energy = LO energy gained + LO energy passed to the acoustic reservoir.
     screened: the hot-carrier default of the reference (screened hot-phonon
     Froehlich, q-resolved rate and angle, plasma screening refreshed after
     every step from the step's tile partials)."""
-    from emc_models import photoexcited_cooling, screened_cooling
+    from viennaemc_amd.workloads import photoexcited_cooling, screened_cooling
     spec = screened_cooling(n) if screened else photoexcited_cooling(n)
     ctx = emcgpu.Context(precision=precision)
     ctx.setup(0, spec)
     ctx.resize([n])
-    ctx.init_particles_energy(0, seed, 0.5, 0.05)
+    ctx.init_particles_energy(0, SEED, 0.5, 0.05)
     if screened:
         ctx.update_screening(0)
     dt = 0.5e-15
+    e0 = float(ctx.get_particles(0)["E"].sum())
     if warmup:
-        ctx.step(dt, warmup, seed=seed, step0=0, graph=True)
+        ctx.step(dt, warmup, seed=SEED, step0=0, graph=True)
+        e0 = float(ctx.averages(0)["sumE"][-1, 0])
+    ctx.reserve(dt, steps)
     ctx.synchronize()
     t0 = time.perf_counter()
-    ctx.step(dt, steps, seed=seed, step0=warmup, graph=True)
+    ctx.step(dt, steps, seed=SEED, step0=warmup, graph=True)
     ctx.synchronize()
     wall = time.perf_counter() - t0
     av = ctx.averages(0)
     hist = ctx.bath_history(0, 0)
+    en = ctx.bath_energy(0, 0)
     bath = ctx.get_bath(0, 0)
     qs2 = ctx.get_screening(0)
     ctx.timing(reset=True)
-    ctx.step(dt, 64, seed=seed, step0=warmup + steps, timing=True)
+    ctx.step(dt, 64, seed=SEED, step0=warmup + steps, timing=True)
     ms, launches = ctx.timing()
     ctx.close()
+    out = {"updates_per_s": n * steps / wall, "particles": n, "steps": steps,
+           "ms_per_step": wall / steps * 1e3, "step_kernel_ms": ms / max(launches, 1),
+           "meanE_first": float(av["avgE"][0, 0]), "meanE_last": float(av["avgE"][-1, 0]),
+           "meanNq_last": float(hist[-1])}
+    # the energy balance of the timed steps (eV in the bath volume)
+    loss = e0 - float(av["sumE"][-1, 0])
+    em, lo, dec = (float(x) for x in en.sum(axis=0))
+    out["energy_balance_eV"] = {
+        "carrier_loss": loss, "net_emitted": em, "lo_gained": lo, "passed_to_reservoir": dec,
+        "rel_resid_carrier_vs_emitted": (loss - em) / em,
+        "rel_resid_emitted_vs_lo_plus_reservoir": (em - lo - dec) / em}
     if screened:
-        return {"updates_per_s": n * steps / wall, "particles": n, "steps": steps,
-                "ms_per_step": wall / steps * 1e3, "step_kernel_ms": ms / max(launches, 1),
-                "meanE_first": float(av["avgE"][0, 0]), "meanE_last": float(av["avgE"][-1, 0]),
-                "meanNq_last": float(hist[-1]), "qs2_last": float(qs2),
-                "workload": "screened photoexcited cooling, %d e- N(0.5, 0.05) eV, screened hot-phonon "
-                            "Froehlich with q-resolved rate and angle on the 256-bin bath, plasma "
-                            "screening (eps 3.5) and window sums refreshed every step, dt 0.5 fs, "
-                            "%s" % (n, "FP32" if precision else "FP64")}
-    return {"updates_per_s": n * steps / wall, "particles": n, "steps": steps,
-            "ms_per_step": wall / steps * 1e3, "step_kernel_ms": ms / max(launches, 1),
-            "meanE_first": float(av["avgE"][0, 0]), "meanE_last": float(av["avgE"][-1, 0]),
-            "meanNq_last": float(hist[-1]), "maxNq_last": float(bath["nq"].max()),
-            "workload": "photoexcited cooling, %d e- N(0.5, 0.05) eV, Kane Gamma valley, hot-phonon "
-                        "Froehlich on a 256-bin LO bath (tau_LO 1 ps into the acoustic reservoir, "
-                        "30 ps to the lattice), dt 0.5 fs, bath update + table rebuild every step, "
-                        "%s" % (n, "FP32" if precision else "FP64")}
+        out["qs2_last"] = float(qs2)
+        out["workload"] = ("screened photoexcited cooling, %d e- N(0.5, 0.05) eV, screened hot-phonon "
+                           "Froehlich with q-resolved rate and angle on the 256-bin bath, plasma "
+                           "screening (eps 3.5) and window sums refreshed every step, dt 0.5 fs, %s"
+                           % (n, "FP32" if precision else "FP64"))
+    else:
+        out["maxNq_last"] = float(bath["nq"].max())
+        out["workload"] = ("photoexcited cooling, %d e- N(0.5, 0.05) eV, Kane Gamma valley, hot-phonon "
+                           "Froehlich on a 256-bin LO bath (tau_LO 1 ps into the acoustic reservoir, "
+                           "30 ps to the lattice), dt 0.5 fs, bath update + table rebuild every step, "
+                           "%s" % (n, "FP32" if precision else "FP64"))
+    out["meanE_curve"] = av["avgE"][:, 0]
+    return out
 
 
-def run_device(emcgpu, n, warmup, steps, precision, seed=20261202, graph=True):
+def leg_device(emcgpu, n, warmup, steps, precision, seed=SEED):
     """Self-consistent 2-D resistor: 200 x 50 nodes at 1 nm, N_D 1e17 cm^-3
     with 1e19 cm^-3 ohmic regions, 0.5 V, parabolic m* = 0.26 valley with
     acoustic + intervalley phonons, CIC, Poisson (Newton + FP32 multigrid,
     FP64 residual to 1e-10) every 0.1 fs step; whole-job particle-updates/s
     (particles moved per step x steps / wall time, HIP-graph launch)."""
-    from emc_models import MODELS_DEVICE, resistor_device
+    from viennaemc_amd.workloads import MODELS_DEVICE, resistor_device
     dev = resistor_device(n_particles=n)
     ctx = emcgpu.Context(precision=precision)
     ctx.setup(0, MODELS_DEVICE["device_para"])
@@ -227,12 +395,13 @@ def run_device(emcgpu, n, warmup, steps, precision, seed=20261202, graph=True):
     t_init = time.perf_counter() - t0
     dt = 0.1e-15
     if warmup:
-        ctx.device_step(dt, warmup, seed=seed, graph=graph)
+        ctx.device_step(dt, warmup, seed=seed, graph=True)
     ctx.synchronize()
     t0 = time.perf_counter()
-    out = ctx.device_step(dt, steps, seed=seed, graph=graph)
+    out = ctx.device_step(dt, steps, seed=seed, graph=True)
     ctx.synchronize()
     wall = time.perf_counter() - t0
+    pot = ctx.device_fields()["pot"]
     ctx.close()
     npart = out["n_particles"].astype(np.float64)
     before = np.concatenate([[npart[0] - (out["injected"][0] - out["exits"][0]
@@ -250,38 +419,80 @@ def run_device(emcgpu, n, warmup, steps, precision, seed=20261202, graph=True):
             "charge_checksum_max": float(out["checksum"].max()),
             "contact_current_A": [float(x) for x in q * net[tail].mean(axis=0) / dt],
             "exits_per_step": float(out["exits"].sum(axis=1).mean()),
+            "pot": pot,
             "workload": "2-D n+ n n+ resistor, 200 x 50 nodes at 1 nm, N_D 1e17 / 1e19 cm^-3, "
                         "0.5 V, %d e- (parabolic 0.26 m0, acoustic + IV), CIC, Poisson every "
                         "0.1 fs step, %s" % (n, "FP32" if precision else "FP64")}
 
 
-PROFILE = os.path.join(ROOT, "profiles", "emc_step_si_bulk_10M.md")
-
-
-def measured_traffic(n, spl):
-    """Calibrated HBM bytes per FP32 launch from the committed rocprof counter
-    summary (FETCH_SIZE / WRITE_SIZE scaled by a copy kernel), when this run
-    has the profiled shape (10M particles, 64 steps per launch); else None."""
-    if n != 10_000_000 or spl != 64 or not os.path.exists(PROFILE):
-        return None
-    for line in open(PROFILE):
-        if line.startswith("| FP32 launch HBM traffic, calibrated"):
-            return float(line.split("|")[2].split()[0]) * 1e6
-    return None
-
-
-def cpu_baseline(spec, n, steps):
+def cpu_baseline(spec, n, steps, threads):
     from oracle import oracle_py as O
-    from emc_models import field_force
+    from viennaemc_amd.workloads import field_force
     M = O.OracleModel(spec)
     force, d = field_force(spec)
-    threads = min(os.cpu_count() or 1, 16)
     t = np.zeros(1)
-    M.run_mt(n, 20261202, O.thermal_voltage(spec["T"]), steps, 1e-15, force, d, obs=False,
+    M.run_mt(n, SEED, O.thermal_voltage(spec["T"]), steps, DT, force, d, obs=False,
              threads=threads, timing=t)
-    return dict(value=n * steps / t[0], unit="particle-updates/s", cores=threads, kind="port",
-                sample="%d particles x %d steps, FP64, mt19937_64 per particle (step loop only, %.1f s)"
-                       % (n, steps, t[0]))
+    res = dict(value=n * steps / t[0], unit="particle-updates/s", cores=threads, kind="port",
+               sample="%d particles x %d steps of the 10M-electron workload, FP64, mt19937_64 per "
+                      "particle, step loop only (%.1f s wall on %d threads)"
+                      % (n, steps, t[0], threads))
+    cal = profile_figures(os.path.join(ROOT, "profiles", "cpu_calibration_c1.md"))
+    if cal:
+        res["calibration"] = cal
+    return res
+
+
+def cpu_legs(threads):
+    """The CPU side of the other workloads, bounded samples on the host:
+    C3 the 64 instances run one after the other (1/100 of each instance's
+    particles, 100 steps), C4 the oracle's reference device loop with SOR
+    (the resistor with 200k electrons, one thread: the reference's deposit,
+    particle removal and Poisson solve are serial), C5 the hot-phonon loop with the bath update
+    and table rebuild after every step (400k electrons, 20 steps)."""
+    from oracle import oracle_py as O
+    from viennaemc_amd.workloads import (MODELS_DEVICE, batched_instances, field_force,
+                                         photoexcited_cooling, resistor_device)
+    out = {}
+    # C3
+    tot, t_all = 0, 0.0
+    for m, cnt in batched_instances(64, SEED):
+        n = max(cnt // 100, 256)
+        force, d = field_force(m)
+        t = np.zeros(1)
+        O.OracleModel(m).run_mt(n, SEED, O.thermal_voltage(m["T"]), 100, DT, force, d, obs=False,
+                                threads=threads, timing=t)
+        tot += n * 100
+        t_all += t[0]
+    out["batched_sweep"] = {"updates_per_s": tot / t_all, "threads": threads,
+                            "sample": "64 sequential runs, 1/100 of each instance's particles "
+                                      "(>= 256) x 100 steps, step loops only"}
+    # C4: a 6-step run minus a 1-step run (initialisation and the
+    # equilibrium solve cancel)
+    dev = resistor_device(n_particles=200_000)
+    od = O.OracleDevice(dev)
+    t0 = time.perf_counter()
+    r1 = od.run(O.OracleModel(MODELS_DEVICE["device_para"]), O.PM_CIC, 1e-8, 1.8, 0.1e-15, 1, SEED)
+    t1 = time.perf_counter()
+    r6 = od.run(O.OracleModel(MODELS_DEVICE["device_para"]), O.PM_CIC, 1e-8, 1.8, 0.1e-15, 6, SEED)
+    t2 = time.perf_counter()
+    moved = float(r6["n_part"][:5].sum())
+    out["device_resistor"] = {"updates_per_s": moved / ((t2 - t1) - (t1 - t0)), "threads": 1,
+                              "particles": int(r1["n_init"][0]),
+                              "sample": "oracle device loop with SOR (1e-8 V, omega 1.8) on the "
+                                        "200 x 50 resistor with 200k electrons, steps 2-6 (a "
+                                        "6-step run minus a 1-step run)"}
+    # C5
+    n = 400_000
+    spec = photoexcited_cooling(n)
+    force, d = field_force(spec)
+    t = np.zeros(1)
+    O.OracleModel(spec).run_mt(n, SEED, O.thermal_voltage(spec["T"]), 20, 0.5e-15, force, d,
+                               obs=False, threads=threads, timing=t)
+    out["hot_phonon"] = {"updates_per_s": n * 20 / t[0], "threads": threads,
+                         "sample": "400k electrons x 20 steps of 0.5 fs with the bath update and "
+                                   "table rebuild after every step"}
+    return out
 
 
 def main():
@@ -293,87 +504,112 @@ def main():
         print("bench.py measures one GPU: the particle step has no multi-GPU path in this library "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         return 2
-    from emc_models import MODELS
     from viennaemc_amd import emcgpu
+    from viennaemc_amd.workloads import MODELS
     spec = MODELS["si_jacoboni"]
     n = args.particles
-    if args.device_only:
-        rec = run_device(emcgpu, args.device_particles, args.warmup, args.device_steps,
-                         emcgpu.FP32)
-        print(json.dumps({"device_resistor": rec}), flush=True)
-        return 0
-
-    r32 = run_gpu(emcgpu, spec, emcgpu.FP32, n, args.warmup, args.steps, args.spl,
-                  ppl=args.ppl)
-    ups32 = n * args.steps / r32["wall"]
+    legs = args.legs
+    threads = host_threads()
     out = {}
-    if not args.no_fp64:
-        s64 = args.fp64_steps or args.steps
-        # the plain FP64 path: one particle per lane, one step per launch
-        r64 = run_gpu(emcgpu, spec, emcgpu.FP64, n, args.warmup, s64, 1, ppl=1)
-        ups64 = n * s64 / r64["wall"]
-        # FP64 with the batched kernel's launch shape, for reference
-        r64b = run_gpu(emcgpu, spec, emcgpu.FP64, n, args.warmup, s64, args.spl, ppl=args.ppl)
-        out["fp64_batched_updates_per_s"] = n * s64 / r64b["wall"]
-        tail = max(1, min(args.steps, s64) // 2)
-        v32, v64 = r32["av"]["avgV"][-tail:, 0].mean(), r64["av"]["avgV"][-tail:, 0].mean()
-        e32, e64 = r32["av"]["avgE"][-tail:, 0].mean(), r64["av"]["avgE"][-tail:, 0].mean()
-        out.update(fp64_plain_updates_per_s=ups64, fp32_over_fp64_plain=ups32 / ups64,
-                   fp64_kernel_ms=r64["kernel_ms"],
-                   fp64_roofline_frac=(BYTES_FP64 * n / (r64["kernel_ms"] * 1e-3)) / 1e9 / HBM_PEAK_GBS,
-                   delta_vd_rel=(v32 - v64) / v64, delta_E_rel=(e32 - e64) / e64,
-                   vd_fp32=v32, vd_fp64=v64, meanE_fp32=e32, meanE_fp64=e64)
-    if not args.no_plain:
-        r1 = run_gpu(emcgpu, spec, emcgpu.FP32, n, 0, min(args.steps, 200), 1, ppl=args.ppl)
-        out["fp32_plain_updates_per_s"] = n * min(args.steps, 200) / r1["wall"]
-        out["fp32_plain_kernel_ms"] = r1["kernel_ms"]
-        out["fp32_plain_roofline_frac"] = BYTES_FP32 * n / (r1["kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
-    if not args.no_batched:
-        out["batched_sweep"] = run_batched(emcgpu, args.warmup, args.batched_steps)
-    if not args.no_hot:
-        if not args.no_hot_plain:
-            out["hot_phonon"] = run_hot_phonon(emcgpu, args.hot_particles, args.warmup,
-                                               args.hot_steps, emcgpu.FP32)
-            if not args.no_fp64:
-                out["hot_phonon_fp64"] = run_hot_phonon(emcgpu, args.hot_particles, args.warmup,
-                                                        args.hot_steps, emcgpu.FP64)
-        if not args.no_hot_screened:
-            out["hot_phonon_screened"] = run_hot_phonon(emcgpu, args.hot_particles, args.warmup,
-                                                        args.hot_steps, emcgpu.FP32, screened=True)
-    if not args.no_device:
-        out["device_resistor"] = run_device(emcgpu, args.device_particles, args.warmup,
-                                            args.device_steps, emcgpu.FP32)
-        if not args.no_fp64:
-            out["device_resistor_fp64"] = run_device(emcgpu, args.device_particles, args.warmup,
-                                                     args.device_fp64_steps, emcgpu.FP64)
-    spl = r32["steps_per_launch"]
-    # algorithmic bytes per launch: 48 B per particle-update (the SoA state
-    # read and written once per update) x the updates one launch processes;
-    # with fused steps the HBM traffic itself is 1/spl of this
-    achieved = BYTES_FP32 * n * spl / (r32["kernel_ms"] * 1e-3) / 1e9
     res = {
         "metric": "particle-updates/s (Si bulk, 10M e-) FP32-batched vs FP64 plain; % HBM peak; dv_d",
-        "value": ups32,
-        "unit": "particle-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": r32["wall"] / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
+        "value": None, "unit": "particle-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (thermal Maxwellian init, Philox-4x32-10)",
-        "config": {"workload": "Si bulk X-valley EMC, 10M e-, 50 kV/cm [100], dt 1 fs, 300 K, "
-                               "acoustic + 6 zero-order Jacoboni phonons, 2000-level tables",
-                   "particles": n, "steps_per_launch": spl, "particles_per_lane": args.ppl},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(n, spl)},
     }
+    if "headline" in legs:
+        r32 = run_gpu(emcgpu, spec, emcgpu.FP32, n, args.warmup, args.steps, args.spl,
+                      ppl=args.ppl)
+        K = r32["K"]
+        t_launch = r32["kernel_ms"] * 1e-3
+        res["value"] = n * args.steps / r32["wall"]
+        res["ms_per_step"] = r32["wall"] / args.steps * 1e3
+        res["config"] = {"workload": "Si bulk X-valley EMC, 10M e-, 50 kV/cm [100], dt 1 fs, 300 K, "
+                                     "acoustic + 6 zero-order Jacoboni phonons, 2000-level tables",
+                         "particles": n, "steps_per_launch": K,
+                         "particles_per_lane": args.ppl or 4,
+                         "kernel_ms_per_launch": r32["kernel_ms"]}
+        fig = profile_figures()
+        upd = n * K  # particle-updates per launch
+        hbm_equiv = BYTES_FP32 * upd / t_launch / 1e9
+        roof = {"bound": "valu", "achieved": None, "peak": VALU_PEAK_TOPS,
+                "unit": "TOP/s (VALU lane-instructions)", "frac": None, "traffic": None,
+                "hbm_equiv": {"achieved": hbm_equiv, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": hbm_equiv / HBM_PEAK_GBS,
+                              "note": "48 B per particle-update x updates per launch / launch time"}}
+        ipu = fig.get("valu_lane_instr_per_update")
+        if ipu:
+            roof["achieved"] = ipu * upd / t_launch / 1e12
+            roof["frac"] = roof["achieved"] / VALU_PEAK_TOPS
+            roof["valu_lane_instr_per_update"] = ipu
+        tb = fig.get("hbm_bytes_per_launch_K%d" % K)
+        if tb:
+            roof["traffic"] = tb
+            roof["hbm_measured"] = {"achieved": tb / t_launch / 1e9, "peak": HBM_PEAK_GBS,
+                                    "unit": "GB/s", "frac": tb / t_launch / 1e9 / HBM_PEAK_GBS}
+        res["roofline"] = roof
+    if "fp64" in legs:
+        s64 = args.fp64_steps or args.steps
+        r64 = run_gpu(emcgpu, spec, emcgpu.FP64, n, args.warmup, s64, 1, ppl=1)
+        ups64 = n * s64 / r64["wall"]
+        out.update(fp64_plain_updates_per_s=ups64, fp64_kernel_ms=r64["kernel_ms"],
+                   fp64_plain_hbm_frac=(BYTES_FP64 * n / (r64["kernel_ms"] * 1e-3)) / 1e9
+                   / HBM_PEAK_GBS)
+        if res["value"]:
+            out["fp32_over_fp64_plain"] = res["value"] / ups64
+        r64b = run_gpu(emcgpu, spec, emcgpu.FP64, n, args.warmup, s64, args.spl, ppl=args.ppl)
+        out["fp64_batched_updates_per_s"] = n * s64 / r64b["wall"]
+    if "fp32_plain" in legs:
+        s1 = min(args.steps, 200)
+        r1 = run_gpu(emcgpu, spec, emcgpu.FP32, n, 0, s1, 1, ppl=args.ppl)
+        out["fp32_plain_updates_per_s"] = n * s1 / r1["wall"]
+        out["fp32_plain_kernel_ms"] = r1["kernel_ms"]
+        out["fp32_plain_hbm_frac"] = BYTES_FP32 * n / (r1["kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+    if "precision" in legs:
+        out["precision_fp32_vs_fp64"] = leg_precision(emcgpu, spec, n, args.precision_steps)
+    if "c1" in legs:
+        out["c1"] = leg_c1(emcgpu, threads)
+    if "batched" in legs:
+        out["batched_sweep"] = leg_batched(emcgpu, args.warmup, args.batched_steps)
+    if "hot" in legs:
+        h32 = leg_hot(emcgpu, args.hot_particles, args.warmup, args.hot_steps, emcgpu.FP32)
+        h64 = leg_hot(emcgpu, args.hot_particles, args.warmup, args.hot_steps, emcgpu.FP64)
+        c32, c64 = h32.pop("meanE_curve"), h64.pop("meanE_curve")
+        h32["fp32_vs_fp64_meanE_rel_max"] = float(np.abs(c32 - c64).max() / np.abs(c64).max())
+        out["hot_phonon"], out["hot_phonon_fp64"] = h32, h64
+    if "hot_screened" in legs:
+        s32 = leg_hot(emcgpu, args.hot_particles, args.warmup, args.hot_steps, emcgpu.FP32,
+                      screened=True)
+        s64 = leg_hot(emcgpu, args.hot_particles, args.warmup, args.hot_steps, emcgpu.FP64,
+                      screened=True)
+        c32, c64 = s32.pop("meanE_curve"), s64.pop("meanE_curve")
+        s32["fp32_vs_fp64_meanE_rel_max"] = float(np.abs(c32 - c64).max() / np.abs(c64).max())
+        s32["fp32_vs_fp64_qs2_rel"] = abs(s32["qs2_last"] - s64["qs2_last"]) / s64["qs2_last"]
+        out["hot_phonon_screened"], out["hot_phonon_screened_fp64"] = s32, s64
+    if "device" in legs:
+        d = leg_device(emcgpu, args.device_particles, args.warmup, args.device_steps, emcgpu.FP32)
+        d.pop("pot")
+        out["device_resistor"] = d
+        # FP32 and FP64 on equal step counts from the same seed
+        p32 = leg_device(emcgpu, args.device_particles, args.warmup, args.device_pair_steps,
+                         emcgpu.FP32)
+        p64 = leg_device(emcgpu, args.device_particles, args.warmup, args.device_pair_steps,
+                         emcgpu.FP64)
+        pot32, pot64 = p32.pop("pot"), p64.pop("pot")
+        out["device_resistor_fp64"] = p64
+        out["device_resistor_fp32_vs_fp64"] = {
+            "steps": args.device_pair_steps,
+            "fp32_updates_per_s": p32["updates_per_s"],
+            "contact_current_A_fp32": p32["contact_current_A"],
+            "contact_current_A_fp64": p64["contact_current_A"],
+            "potential_max_abs_diff_rel": float(np.abs(pot32 - pot64).max() / np.abs(pot64).max()),
+            "particles_last_fp32": p32["particles_last"], "particles_last_fp64": p64["particles_last"]}
+    if "cpu" in legs:
+        res["cpu_baseline"] = cpu_baseline(spec, args.cpu_particles, args.cpu_steps, threads)
+        out["cpu_legs"] = cpu_legs(threads)
     res.update(out)
-    if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline(spec, args.cpu_particles, args.cpu_steps)
-    print(json.dumps(res), flush=True)
+    print(json.dumps(res, default=float), flush=True)
     return 0
 
 

@@ -333,12 +333,27 @@ emcgpu_status emcgpu_get_bath_tallies(const emcgpu_ctx *ctx, int instance,
 emcgpu_status emcgpu_get_bath_history(const emcgpu_ctx *ctx, int instance,
                                       int bath, double *mean_nq,
                                       int *n_steps);
+/* energy record of every bath update of the last emcgpu_step,
+ * energy[step][3] in eV within the bath volume, FP64 sums in bin order over
+ * the update of emcPhononBath.hpp:264-358: [0] hbar w_LO x (emissions -
+ * absorptions) of the step, [1] hbar w_LO x sum_q D_ph(q) (N_q after - N_q
+ * before) (LO energy gained), [2] hbar w_LO x sum_q D_ph(q) dt / tau(q)
+ * (N_q before - N*) (LO energy passed to the reservoirs); [0] = [1] + [2]
+ * unless the clamp N_q >= 0 acted.  No counterpart in the reference. */
+emcgpu_status emcgpu_get_bath_energy(const emcgpu_ctx *ctx, int instance,
+                                     int bath, double *energy, int *n_steps);
 
 /* ---- time stepping ------------------------------------------------------
  * moveParticles(dt) repeated n_steps times
  * (basicBulkParticleHandler.hpp:181-225) with the per-step observables of
  * examples/bulkSimulation/bulkSimulation.cpp:150-156 reduced on the GPU. */
 emcgpu_status emcgpu_step(emcgpu_ctx *ctx, const emcgpu_step_args *args);
+
+/* Sizes the step buffers (per-step partials, observables, reduction pairs,
+ * bath history) and uploads the model for a later emcgpu_step with the
+ * same arguments (or fewer steps), which then allocates nothing; launches
+ * nothing.  No counterpart in the reference (std::vector::reserve). */
+emcgpu_status emcgpu_reserve(emcgpu_ctx *ctx, const emcgpu_step_args *args);
 
 /* emcgpu_step without any device allocation.  No counterpart in the
  * reference: for callers that cannot allocate device memory (code under HIP

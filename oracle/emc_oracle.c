@@ -148,13 +148,6 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2],
   out[3] = c3;
 }
 
-/* production-mode FP64 uniform from two 32-bit words: 53 bits, forced odd,
- * so the value lies strictly inside (0, 1). */
-double orc_philox_u01_f64(uint32_t hi, uint32_t lo) {
-  uint64_t m = ((((uint64_t)hi) << 32) | lo) >> 11;
-  m |= 1ULL;
-  return (double)m * 0x1p-53;
-}
 
 /* ======================================================================
  * Vector helpers: include/emcUtil.hpp:25-175
@@ -689,11 +682,16 @@ void orc_bath_update(orc_bath *b, double dt) {
   }
   const int hasProfile = b->tauProfile != NULL;
   double sumW = 0.0, sumWdN = 0.0, sumWdNtau = 0.0;
+  /* energy balance of the step (not in the reference), bin order: the net
+   * phonons emitted by the carriers, the change of the LO population and
+   * the LO decay towards the target, as phonon counts in the bath volume */
+  double cEm = 0.0, cLo = 0.0, cDec = 0.0;
   for (int i = 0; i < b->nBins; i++) {
     double q = ((double)(size_t)i + 0.5) * b->dq;
     double Dph = q * q * b->dq * b->Vsim / (2.0 * ORC_PI * ORC_PI);
     double g = (Dph > 0.0) ? (b->nEm[i] - b->nAbs[i]) / (Dph * dt) : 0.0;
     const double tau_i = hasProfile ? b->tauProfile[i] : b->tauLO;
+    const double nOld = b->Nq[i];
     if (b->hasAc) {
       double w = q * q;
       sumW += w;
@@ -705,9 +703,15 @@ void orc_bath_update(orc_bath *b, double dt) {
     b->Nq[i] += g * dt - (dt / tau_i) * (b->Nq[i] - target);
     if (b->Nq[i] < 0.0)
       b->Nq[i] = 0.0;
+    cEm += b->nEm[i] - b->nAbs[i];
+    cLo += Dph * (b->Nq[i] - nOld);
+    cDec += Dph * (dt / tau_i) * (nOld - target);
     b->nEm[i] = 0.0;
     b->nAbs[i] = 0.0;
   }
+  b->energy[0] = b->loE * cEm;
+  b->energy[1] = b->loE * cLo;
+  b->energy[2] = b->loE * cDec;
   if (b->hasAc) {
     double fKlemens = b->hasRidley ? (1.0 - b->wR) : 1.0;
     if (!hasProfile) {
@@ -1564,16 +1568,18 @@ int orc_run_shared(orc_model *m, uint64_t seed, const double maxPos[3],
  * Production mode: Philox draws (CPU twin of the GPU kernel, FP64)
  * ====================================================================== */
 
-/* uniform #j of the (step, pid, instance) stream, FP64 layout: block j/2,
- * words (2*(j%2), 2*(j%2)+1) */
+/* uniform #j of the (step, pid, instance) stream: word j%4 of counter block
+ * j/4, its top 24 bits forced odd, (w >> 8 | 1) * 2^-24 -- the GPU's layout
+ * in both precisions (FP32 holds the value exactly) */
+double orc_philox_u01_24(uint32_t w) { return (double)((w >> 8) | 1u) * 0x1p-24; }
+
 static double orc_prod_u(uint64_t seed, uint32_t inst, uint32_t pid,
                          uint32_t step, uint32_t j) {
-  uint32_t ctr[4] = {j >> 1, step, pid, inst};
+  uint32_t ctr[4] = {j >> 2, step, pid, inst};
   uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
   uint32_t w[4];
   orc_philox4x32_10(ctr, key, w);
-  int h = (int)(j & 1u) * 2;
-  return orc_philox_u01_f64(w[h], w[h + 1]);
+  return orc_philox_u01_24(w[j & 3u]);
 }
 
 /* the reference draws free-flight times from U[1e-6, 1) */
@@ -2144,6 +2150,11 @@ void orc_model_set_bath(orc_model *m, int b, const double *Nq, double Nac,
 void orc_model_bath_record_one(orc_model *m, int b, double q, int emission) {
   orc_bath_record(&m->bath[b], q, emission);
 }
+void orc_model_get_bath_energy(const orc_model *m, int b, double out[3]) {
+  for (int j = 0; j < 3; j++)
+    out[j] = m->bath[b].energy[j];
+}
+
 void orc_model_bath_update_one(orc_model *m, int b, double dt) {
   orc_bath_update(&m->bath[b], dt);
 }
@@ -2185,11 +2196,22 @@ static double orc_now(void) {
   return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
+/* fraction of the particles in each (valley, subvalley): occ[v * ORC_MAX_DEG
+ * + s] (the subValley column of basicBulkParticleHandler::print, :240) */
+static void orc_sub_occupancy(const orc_model *m, int np, const orc_particle *parts,
+                              double *occ) {
+  size_t cnt[ORC_MAX_VALLEYS * ORC_MAX_DEG] = {0};
+  for (int i = 0; i < np; i++)
+    cnt[parts[i].valley * ORC_MAX_DEG + parts[i].subValley]++;
+  for (int j = 0; j < m->nValleys * ORC_MAX_DEG; j++)
+    occ[j] = np ? (double)cnt[j] / (double)(size_t)np : 0.;
+}
+
 void orc_run_mt(orc_model *m, int np, uint64_t seed, double Vt,
                 int nsteps, double tStep, const double force[3],
                 const double fieldDir[3], double *obs, int nThreads,
                 double *kOut, double *EOut, int32_t *valleyOut,
-                int32_t *subOut, double *stepSeconds) {
+                int32_t *subOut, double *stepSeconds, double *subOcc) {
   orc_particle *parts = (orc_particle *)malloc(sizeof(orc_particle) * (size_t)np);
   orc_rng *rngs = (orc_rng *)malloc(sizeof(orc_rng) * (size_t)np);
   for (int i = 0; i < np; i++) {
@@ -2203,7 +2225,7 @@ void orc_run_mt(orc_model *m, int np, uint64_t seed, double Vt,
     nThreads = 1;
   /* observables need every step: run the threads one step at a time when
    * obs is requested, else let each thread run all its steps */
-  int chunkSteps = (obs || m->nBaths || m->scrPlasma) ? 1 : nsteps;
+  int chunkSteps = (obs || subOcc || m->nBaths || m->scrPlasma) ? 1 : nsteps;
   /* bath tallies: one private copy of the model per thread whose tally
    * pointers are thread-local, summed in thread order after each step
    * (integer-valued doubles: the sum is exact in any order) */
@@ -2242,6 +2264,8 @@ void orc_run_mt(orc_model *m, int np, uint64_t seed, double Vt,
                       obs + (size_t)s0 * 3 * nv + nv,
                       obs + (size_t)s0 * 3 * nv + 2 * nv);
     }
+    if (subOcc)
+      orc_sub_occupancy(m, np, parts, subOcc + (size_t)s0 * nv * ORC_MAX_DEG);
     if (locals) {
       for (int t = 0; t < nt; t++)
         for (int b = 0; b < m->nBaths; b++) {

@@ -195,6 +195,10 @@ typedef struct {
    * weight * N_q, nBins + 1 entries each (:73-82, 109-124) */
   double qs2Screen;
   double *cumW, *cumWN;
+  /* energy record of the last update [eV in the bath volume]: loE x the
+   * net emissions, loE x sum_q Dph (N_new - N_old), loE x sum_q Dph
+   * (dt / tau_q) (N_old - N*) (the LO energy passed to the reservoirs) */
+  double energy[3];
 } orc_bath;
 
 /* constructor (:160-192) */
@@ -387,8 +391,9 @@ double orc_philox_event(const orc_model *m, orc_particle *p, uint64_t seed,
 /* uniform j of the (step, particle) stream of an instance word */
 double orc_prod_uniform(uint64_t seed, uint32_t inst, uint32_t pid, uint32_t step,
                         uint32_t j);
-/* production uniform in (0,1]-safe form used by the GPU kernels */
-double orc_philox_u01_f64(uint32_t hi, uint32_t lo);
+/* production uniform of one Philox word: (w >> 8 | 1) * 2^-24, strictly
+ * inside (0, 1), the GPU kernels' value in both precisions */
+double orc_philox_u01_24(uint32_t w);
 
 
 /* ---- handle API for language bindings (tests) -------------------------- */
@@ -499,18 +504,22 @@ void orc_model_set_bath(orc_model *m, int b, const double *Nq, double Nac,
                         double Nto);
 void orc_model_bath_record_one(orc_model *m, int b, double q, int emission);
 void orc_model_bath_update_one(orc_model *m, int b, double dt);
+/* the energy record of bath b's last update (orc_bath.energy) */
+void orc_model_get_bath_energy(const orc_model *m, int b, double out[3]);
 void orc_model_set_bath_mode(orc_model *m, int update, int rebuildEvery,
                              long long stepCount);
 void orc_model_set_init_energy(orc_model *m, double E0);
 /* mt19937_64 ensemble (reference RNG, one stream per particle) with
  * per-step observables; the statistical baseline for production mode.
  * nThreads > 1 runs particles in parallel (pthreads); the wall time of
- * the step loop alone (no initialisation) goes to *stepSeconds. */
+ * the step loop alone (no initialisation) goes to *stepSeconds; subOcc
+ * (may be NULL): per step the fraction in each (valley, subvalley),
+ * [step][valley][ORC_MAX_DEG]. */
 void orc_run_mt(orc_model *m, int np, uint64_t seed, double Vt,
                 int nsteps, double tStep, const double force[3],
                 const double fieldDir[3], double *obs, int nThreads,
                 double *kOut, double *EOut, int32_t *valleyOut,
-                int32_t *subOut, double *stepSeconds);
+                int32_t *subOut, double *stepSeconds, double *subOcc);
 
 #ifdef __cplusplus
 }

@@ -12,6 +12,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
+MAX_DEG = 8  # ORC_MAX_DEG
 Q = 1.60219e-19
 KB = 1.38066e-23
 ME = 9.11e-31
@@ -80,7 +81,8 @@ def load():
                                   C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
         "orc_init_particles_philox": (None, [vp, _I, _D, C.c_uint64, C.c_uint32, _PD, _PD, _PD, _PI32, _PI32]),
         "orc_run_mt": (None, [vp, _I, C.c_uint64, _D, _I, _D, _PD, _PD, C.c_void_p, _I,
-                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                              C.c_void_p]),
         "orc_init_particles_philox_energy": (None, [vp, _I, _D, _D, C.c_uint64, C.c_uint32, _PD, _PD, _PD,
                                                     _PI32, _PI32]),
         "orc_model_add_bath": (_I, [vp, _I, _D, _D, _D, _D, _D, _I, _D, _D, _D, _D, _D]),
@@ -90,6 +92,7 @@ def load():
         "orc_model_set_bath": (None, [vp, _I, _PD, _D, _D]),
         "orc_model_bath_record_one": (None, [vp, _I, _D, _I]),
         "orc_model_bath_update_one": (None, [vp, _I, _D]),
+        "orc_model_get_bath_energy": (None, [vp, _I, _PD]),
         "orc_model_bath_step": (None, [vp, _D, _I]),
         "orc_model_set_bath_mode": (None, [vp, _I, _I, C.c_longlong]),
         "orc_model_set_init_energy": (None, [vp, _D]),
@@ -124,7 +127,8 @@ def load():
         "orc_philox4x32_10": (None, [np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"),
                                      np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"),
                                      np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")]),
-        "orc_philox_u01_f64": (_D, [C.c_uint32, C.c_uint32]),
+        "orc_philox_u01_24": (_D, [C.c_uint32]),
+        "orc_prod_uniform": (_D, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -295,9 +299,14 @@ class OracleModel:
         return dict(k=k, E=E, tau=tau, valley=v, sub=s)
 
     def run_mt(self, np_, seed, Vt, nsteps, dt, force, field_dir, obs=True, threads=1, final=False,
-               timing=None):
+               timing=None, sub_occ=False):
+        """mt19937_64 ensemble: per-step (avgE, avgV, occ) per valley, shape
+        (nsteps, 3, n_valleys); with final the last state; with sub_occ a
+        third result, the per-step (valley, subvalley) fractions, shape
+        (nsteps, n_valleys, MAX_DEG)."""
         nv = self.n_valleys
         o = np.zeros(nsteps * 3 * nv) if obs else None
+        so = np.zeros(nsteps * nv * MAX_DEG) if sub_occ else None
         fin = None
         if final:
             fin = dict(k=np.zeros(3 * np_), E=np.zeros(np_), valley=np.zeros(np_, dtype=np.int32),
@@ -305,8 +314,11 @@ class OracleModel:
         self.L.orc_run_mt(self.h, np_, seed, Vt, nsteps, dt, np.array(force, dtype=np.float64),
                           np.array(field_dir, dtype=np.float64), _ptr(o), threads,
                           _ptr(fin and fin["k"]), _ptr(fin and fin["E"]), _ptr(fin and fin["valley"]),
-                          _ptr(fin and fin["sub"]), _ptr(timing))
-        return (None if o is None else o.reshape(nsteps, 3, nv)), fin
+                          _ptr(fin and fin["sub"]), _ptr(timing), _ptr(so))
+        res = ((None if o is None else o.reshape(nsteps, 3, nv)), fin)
+        if sub_occ:
+            res = res + (so.reshape(nsteps, nv, MAX_DEG),)
+        return res
 
     def run_shared(self, seed, max_pos, spacing, doping, T, field_dir, field_strength, dt, nsteps,
                    max_particles=100000, bath_history=False):
@@ -337,6 +349,13 @@ class OracleModel:
         self.L.orc_model_get_bath(self.h, b, nq, em, ab, sc)
         return dict(nq=nq, n_em=em, n_abs=ab, mean=sc[0], n_ac=sc[1], n_to=sc[2], n0=sc[3],
                     t_ac=sc[4], t_to=sc[5])
+
+    def bath_energy(self, b=0):
+        """energy record of the last update [eV]: net emitted phonon energy,
+        LO energy gained, LO energy passed to the reservoirs"""
+        out = np.zeros(3)
+        self.L.orc_model_get_bath_energy(self.h, b, out)
+        return out
 
     def set_bath(self, b, nq, n_ac=0.0, n_to=0.0):
         self.L.orc_model_set_bath(self.h, b, np.ascontiguousarray(nq, dtype=np.float64), n_ac, n_to)
@@ -406,6 +425,11 @@ def philox(ctr, key):
     out = np.zeros(4, dtype=np.uint32)
     load().orc_philox4x32_10(np.array(ctr, dtype=np.uint32), np.array(key, dtype=np.uint32), out)
     return out
+
+
+def prod_uniform(seed, inst, pid, step, j):
+    """uniform j of the production (step, particle, instance) stream"""
+    return load().orc_prod_uniform(seed, inst, pid, step, j)
 
 
 def thermal_voltage(T):

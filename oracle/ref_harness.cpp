@@ -89,7 +89,7 @@ static uint64_t splitmix64(uint64_t x) {
 static emcMaterial<T> silicon() { return emcMaterial<T>{11.8, 2329., 1.45e16, 9040, 1.15}; }
 
 // ---------------------------------------------------------------------------
-// Model descriptions (mirrored by tests/emc_models.py)
+// Model descriptions (mirrored by viennaemc_amd/workloads.py)
 // ---------------------------------------------------------------------------
 struct Model {
   std::string name;

@@ -207,3 +207,40 @@ def test_fp32_cooling_matches_fp64():
     hot = b64["nq"] > b64["n0"] + 1.0
     assert hot.sum() >= 5
     assert np.allclose(b32["nq"][hot], b64["nq"][hot], rtol=0.1)
+
+
+def test_bath_energy_record_and_balance():
+    """The FP64 energy record of every bath update (emcPhononBath.hpp:264-358)
+    against the oracle twin's, and the energy balance of photoexcited cooling
+    (no field): per step the carriers' energy loss equals the net emitted
+    LO phonon energy, which equals the LO energy gained plus the LO energy
+    passed to the acoustic reservoir."""
+    emcgpu = _emc()
+    n, ns, seed = 4096, 50, 0xE4E
+    spec = photoexcited_cooling(n)
+    print("bath energy record vs twin", flush=True)
+    M = O.OracleModel(spec)
+    ref = M.init_philox_energy(n, spec["init_energy"], 0.05, seed)
+    ctx = _ctx(spec, n, emcgpu.FP64)
+    ctx.set_particles(0, ref["k"], ref["E"], ref["tau"], ref["valley"], ref["sub"])
+    e_before = ref["E"].sum()
+    ctx.step(0.5e-15, ns, seed=seed)
+    force, d = field_force(spec)
+    rec = []
+    for s in range(ns):
+        M.run_philox(ref, 1, 0.5e-15, force, d, seed, step0=s, obs=False)
+        rec.append(M.bath_energy(0))
+    rec = np.array(rec)
+    got = ctx.bath_energy(0, 0)
+    _check_state(ctx, ref)
+    assert got.shape == (ns, 3)
+    assert np.allclose(got, rec, rtol=1e-12, atol=1e-12 * np.abs(rec).max())
+    # balance: net emission = LO gained + passed on (the clamp N_q >= 0 idle)
+    assert np.allclose(got[:, 0], got[:, 1] + got[:, 2], rtol=0, atol=1e-9 * np.abs(got).max())
+    # carriers: the energy lost in each step is the net emitted phonon energy
+    sums = np.concatenate([[e_before], ctx.averages(0)["sumE"][:, 0]])
+    loss = sums[:-1] - sums[1:]
+    print("carrier loss %.6g eV, net emission %.6g eV, LO gained %.6g eV, passed on %.6g eV"
+          % (loss.sum(), got[:, 0].sum(), got[:, 1].sum(), got[:, 2].sum()), flush=True)
+    assert got[:, 0].sum() > 0
+    assert np.allclose(loss, got[:, 0], rtol=0, atol=1e-10 * e_before)

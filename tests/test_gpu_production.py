@@ -4,9 +4,9 @@
   identical discrete history, k and E within 1e-12 relative.
 * GPU-side ensemble reductions vs the same sums taken on the CPU.
 * FP32 and FP64 kernels vs the reference-RNG (mt19937_64) CPU ensemble:
-  drift velocity and mean energy within 4 sigma of the combined statistical
-  error (3 sigma per-step bars widened for the number of compared steps).
-* FP32 vs FP64 GPU runs: observables within 0.5 % or 4 sigma.
+  drift velocity, mean energy and subvalley occupancy within 3 sigma of the
+  combined statistical error.
+(FP32 against FP64 on the same stream: test_gpu_precision.py.)
 """
 import numpy as np
 import pytest
@@ -98,7 +98,7 @@ def test_reductions_match_cpu_sums():
             assert av["sub_occ"][-1, v, s] * n == pytest.approx(((g["valley"] == v) & (g["sub"] == s)).sum())
 
 
-def _stat_compare(a_mean, a_sig, b_mean, b_sig, nsig=4.0):
+def _stat_compare(a_mean, a_sig, b_mean, b_sig, nsig=3.0):
     return np.abs(a_mean - b_mean) <= nsig * np.sqrt(a_sig ** 2 + b_sig ** 2)
 
 
@@ -133,8 +133,8 @@ def test_ensemble_matches_reference_rng(precision):
     occ = av["sub_occ"][-1, 0, :3]
     occ_cpu = np.array([(fin["sub"] == s).mean() for s in range(3)])
     sig = np.sqrt(occ * (1 - occ) / n_gpu + occ_cpu * (1 - occ_cpu) / n_cpu)
-    assert np.all(np.abs(occ - occ_cpu) < 4 * sig), (occ, occ_cpu)
-    assert abs(occ[1] - occ[2]) < 4 * np.sqrt(2 * occ[1] * (1 - occ[1]) / n_gpu)
+    assert np.all(np.abs(occ - occ_cpu) < 3 * sig), (occ, occ_cpu)
+    assert abs(occ[1] - occ[2]) < 3 * np.sqrt(2 * occ[1] * (1 - occ[1]) / n_gpu)
     assert occ[0] > occ[1] + 10 * sig[0]
 
 
@@ -164,26 +164,6 @@ def test_screened_ensemble_matches_reference_rng(precision):
                              obs_cpu[s, 1, 0], sig_v / np.sqrt(n_cpu)), (s, av["avgV"][s, 0], obs_cpu[s, 1, 0])
         assert _stat_compare(av["avgE"][s, 0], sig_e / np.sqrt(n_gpu),
                              obs_cpu[s, 0, 0], sig_e / np.sqrt(n_cpu)), (s, av["avgE"][s, 0], obs_cpu[s, 0, 0])
-
-
-def test_fp32_matches_fp64():
-    """FP32 kinematics vs the FP64 kernel on the same seed: v_d and <E>
-    averaged over the last 200 steps agree within 0.5 %."""
-    emcgpu = _emc()
-    name = "si_jacoboni"
-    spec = MODELS[name]
-    n, nsteps = 1_000_000, 600
-    out = {}
-    for p in (emcgpu.FP64, emcgpu.FP32):
-        ctx = _ctx(name, n, p)
-        ctx.init_particles(0, 777, spec["T"])
-        ctx.step(1e-15, nsteps, seed=777)
-        av = ctx.averages(0)
-        out[p] = (av["avgV"][-200:, 0].mean(), av["avgE"][-200:, 0].mean())
-    dv = abs(out[1][0] - out[0][0]) / abs(out[0][0])
-    de = abs(out[1][1] - out[0][1]) / abs(out[0][1])
-    print("fp32 vs fp64: dv/v = %.2e, dE/E = %.2e" % (dv, de), flush=True)
-    assert dv < 5e-3 and de < 5e-3
 
 
 def test_batched_instances_are_independent():

@@ -63,6 +63,25 @@ def test_philox_known_answers():
                          [0xA4093822, 0x299F31D0])) == [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
 
 
+def test_production_uniform_layout():
+    """Uniform j of a production stream is word j % 4 of Philox block
+    (j / 4, step, particle, instance) under the seed's two halves, its top 24
+    bits forced odd: strictly inside (0, 1) and exact in FP32, which is what
+    lets the FP32 and FP64 kernels draw the same numbers."""
+    import numpy as np
+    seed = 0x0123456789ABCDEF
+    key = [seed & 0xFFFFFFFF, seed >> 32]
+    for j, step, pid, inst in [(0, 0, 0, 0), (3, 7, 11, 2), (4, 0xFFFFFFFF, 5, 0),
+                               (9, 123456, 987654, 0x40000001)]:
+        w = O.philox([j >> 2, step, pid, inst], key)[j & 3]
+        u = O.prod_uniform(seed, inst, pid, step, j)
+        assert u == ((int(w) >> 8) | 1) / 2.0 ** 24
+        assert 0.0 < u < 1.0 and float(np.float32(u)) == u
+    # the extremes of the 24-bit grid
+    assert O.load().orc_philox_u01_24(0) == 2.0 ** -24
+    assert O.load().orc_philox_u01_24(0xFFFFFFFF) == 1.0 - 2.0 ** -24
+
+
 # ---- tables --------------------------------------------------------------
 
 @pytest.mark.parametrize("name", NAMES)

@@ -662,7 +662,6 @@ emcgpu_status ensure_obs(emcgpu_ctx *c, int n_steps) {
     HIPCHK(c, hipMalloc((void **)&c->d_obs, bytes));
     c->obs_bytes = bytes;
   }
-  c->obs_steps = n_steps;
   return EMCGPU_OK;
 }
 
@@ -698,9 +697,11 @@ struct NoAllocBuffers {
   size_t obs_len;      // its length in doubles
 };
 
+// reserve_only: size (and upload) everything a call with these arguments
+// needs, then return without launching
 template <typename R>
 emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s,
-                        const NoAllocBuffers *na = nullptr) {
+                        const NoAllocBuffers *na = nullptr, bool reserve_only = false) {
   if (c->capacity == 0)
     return fail(c, EMCGPU_ERR_STATE, "no particles (call emcgpu_resize first)");
   for (int i = 0; i < c->n_inst; i++) {
@@ -763,14 +764,16 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s,
   // launches (a launch's steps, or up to 64 one-step launches with baths)
   int Kr = dyn ? std::min(s->n_steps, 64) : K;
   if (baths && !s->bath_record_only) {
-    const size_t hb = sizeof(double) * (size_t)c->n_baths * (size_t)s->n_steps;
+    // per bath and step: the mean occupation, then the 3-value energy record
+    const size_t hb = 4 * sizeof(double) * (size_t)c->n_baths * (size_t)s->n_steps;
     if (hb > c->bath_hist_bytes) {
       dfreet(c->d_bath_hist);
       HIPCHK(c, hipMalloc((void **)&c->d_bath_hist, hb));
       c->bath_hist_bytes = hb;
     }
-    c->bath_hist_steps = s->n_steps;
-  } else {
+    if (!reserve_only)
+      c->bath_hist_steps = s->n_steps;
+  } else if (!reserve_only) {
     c->bath_hist_steps = 0;
   }
   // particles per lane of the production kernel (replay: one per lane)
@@ -818,6 +821,9 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s,
     d_partial = c->d_partial;
     d_chunks = c->d_chunks;
     d_obs = c->d_obs;
+    if (reserve_only)
+      return EMCGPU_OK;
+    c->obs_steps = s->n_steps;
   }
 
   StepArgs<R> a;
@@ -941,6 +947,7 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s,
       const long long gstep = (long long)s->step0 + done + 1;  // 1-based global step
       BathStepArgs<R> ba =
           bath_args<R>(c, s->dt, c->d_bath_hist, s->n_steps, done, gstep % every == 0);
+      ba.energy = c->d_bath_hist + (size_t)c->n_baths * (size_t)s->n_steps;
       ba.partial = a.partial;
       ba.n_tiles = n_tiles;
       ba.tile_size = tile_size;
@@ -2109,6 +2116,28 @@ emcgpu_status emcgpu_get_bath_history(const emcgpu_ctx *c0, int instance, int ba
   return EMCGPU_OK;
 }
 
+emcgpu_status emcgpu_get_bath_energy(const emcgpu_ctx *c0, int instance, int bath,
+                                     double *energy, int *n_steps) {
+  emcgpu_ctx *c = const_cast<emcgpu_ctx *>(c0);
+  emcgpu_status st = check_instance(c, instance);
+  if (st)
+    return st;
+  Instance &in = c->inst[instance];
+  if (bath < 0 || bath >= (int)in.baths.size())
+    return fail(c, EMCGPU_ERR_ARG, "instance %d has no bath %d", instance, bath);
+  const int ns = in.dev_bath ? c->bath_hist_steps : 0;
+  if (n_steps)
+    *n_steps = ns;
+  if (energy && ns > 0) {
+    (void)hipSetDevice(c->device);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double *src = c->d_bath_hist + (size_t)c->n_baths * (size_t)ns +
+                        3 * (in.bath_index0 + bath) * (long long)ns;
+    HIPCHK(c, hipMemcpy(energy, src, 3 * sizeof(double) * ns, hipMemcpyDeviceToHost));
+  }
+  return EMCGPU_OK;
+}
+
 emcgpu_status emcgpu_step(emcgpu_ctx *c, const emcgpu_step_args *s) {
   if (!c || !s)
     return fail(c, EMCGPU_ERR_ARG, "ctx and args must be non-NULL");
@@ -2119,6 +2148,16 @@ emcgpu_status emcgpu_step(emcgpu_ctx *c, const emcgpu_step_args *s) {
     if (c->inst[i].count && !c->inst[i].has_field)
       return fail(c, EMCGPU_ERR_STATE, "instance %d: set the field first", i);
   return c->precision == EMCGPU_FP32 ? run_steps<float>(c, s) : run_steps<double>(c, s);
+}
+
+emcgpu_status emcgpu_reserve(emcgpu_ctx *c, const emcgpu_step_args *s) {
+  if (!c || !s)
+    return fail(c, EMCGPU_ERR_ARG, "ctx and args must be non-NULL");
+  if (c->device < 0)
+    return fail(c, EMCGPU_ERR_STATE, "host-only context cannot step");
+  (void)hipSetDevice(c->device);
+  return c->precision == EMCGPU_FP32 ? run_steps<float>(c, s, nullptr, true)
+                                     : run_steps<double>(c, s, nullptr, true);
 }
 
 emcgpu_status emcgpu_step_noalloc(emcgpu_ctx *c, const emcgpu_step_args *s, double *obs,

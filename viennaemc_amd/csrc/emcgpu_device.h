@@ -160,6 +160,7 @@ struct BathStepArgs {
   BathVar *var;
   double dt;
   double *hist;  // [bath][hist_steps] mean occupation after each step, or null
+  double *energy;  // [bath][hist_steps][3] energy record of each update, or null
   int hist_steps, hist_step;
   int rebuild;
   const RebuildJob *jobs;

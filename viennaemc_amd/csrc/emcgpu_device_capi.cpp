@@ -1273,12 +1273,9 @@ emcgpu_status run_device_steps(emcgpu_ctx *c, DeviceSim &d, const emcgpu_device_
   return EMCGPU_OK;
 }
 
-// host Philox uniform of the device streams (the device's u01_f64 pair)
+// host Philox uniform of the device streams (the kernels' stream_uniform)
 double host_uniform(uint32_t j, uint32_t step, uint32_t pid, uint32_t dom, uint64_t seed) {
-  uint32_t ctr[4] = {j >> 1, step, pid, dom};
-  philox4x32_10(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
-  const int hh = (int)(j & 1u) * 2;
-  return u01_f64(ctr[hh], ctr[hh + 1]);
+  return stream_uniform<double>(j, step, pid, dom, (uint32_t)seed, (uint32_t)(seed >> 32));
 }
 
 template <typename R>

@@ -14,8 +14,9 @@
 // Random numbers:
 //   production: Philox-4x32-10 keyed by the seed, counter
 //               (block, step, particle, instance); one counter block per
-//               scatter event in FP32 (4 words -> selection, two final-state
-//               uniforms, free-flight time) and two in FP64 (53-bit pairs).
+//               scatter event (4 words -> selection, two final-state
+//               uniforms, free-flight time), 24-bit uniforms that FP64
+//               widens, so both precisions follow the same random path.
 //   replay:     the reference engine's recorded raw 64-bit draws, consumed
 //               in the reference's order and mapped like libstdc++'s
 //               generate_canonical (FP64 only).
@@ -496,6 +497,8 @@ struct BathShared {
   double new_n[kMaxBins];
   double w[kMaxBins];
   double vmax[kBathThreads];
+  // terms of the energy record: net emissions, LO population change, decay
+  double e_em[kMaxBins], e_lo[kMaxBins], e_dec[kMaxBins];
   double target;
   double qs2;  // the instance's q_s^2, broadcast to the block
 };
@@ -504,7 +507,8 @@ template <typename R>
 __device__ void bath_update_one(const BathDev &B, BathVar &V, double *__restrict__ nq,
                                 const double *__restrict__ tau_prof,
                                 unsigned long long *__restrict__ tally, long long tally_bins,
-                                double dt, double *__restrict__ hist, BathShared &sh,
+                                double dt, double *__restrict__ hist,
+                                double *__restrict__ energy, BathShared &sh,
                                 double qs2, double *__restrict__ win64, R *__restrict__ win) {
 #pragma clang fp contract(off)
   const int nb = B.n_bins;
@@ -577,6 +581,11 @@ __device__ void bath_update_one(const BathDev &B, BathVar &V, double *__restrict
     if (n < 0.0)
       n = 0.0;
     nq[gi] = n;
+    if (energy) {
+      sh.e_em[i] = n_em - n_abs;
+      sh.e_lo[i] = dph * (n - n_old);
+      sh.e_dec[i] = dph * (dt / tau_i) * (n_old - target);
+    }
     // the terms of the bin-order sums, formed on all lanes: the reservoir
     // source from the start-of-step occupations (:291-355), getMeanNq
     // (:461-470) and, when read, the window sums (:109-124)
@@ -661,6 +670,14 @@ __device__ void bath_update_one(const BathDev &B, BathVar &V, double *__restrict
       cwn[i + 1] = acc;
       rwn[i + 1] = (R)acc;
     }
+  } else if (lane == 0 && wave >= 4 && wave <= 6 && energy) {
+    // the energy record in bin order, x hbar w_LO [eV]
+    const double *t = (wave == 4) ? sh.e_em : (wave == 5) ? sh.e_lo : sh.e_dec;
+    double acc = 0.0;
+#pragma unroll 16
+    for (int i = 0; i < nb; ++i)
+      acc += t[i];
+    energy[wave - 4] = B.lo_e * acc;
   }
   __syncthreads();
 }
@@ -903,8 +920,10 @@ __global__ __launch_bounds__(kBathThreads) void k_bath_step(BathStepArgs<R> a) {
   double *win64 = (a.win64 && I.need_win) ? a.win64 : nullptr;
   for (int b = I.bath0; b < I.bath0 + I.n_baths; ++b)
     bath_update_one<R>(a.baths[b], a.var[b], a.nq, a.tau_prof, a.tally, a.tally_bins, a.dt,
-                       a.hist ? a.hist + (long long)b * a.hist_steps + a.hist_step : nullptr, sh,
-                       qs2, win64, a.win);
+                       a.hist ? a.hist + (long long)b * a.hist_steps + a.hist_step : nullptr,
+                       a.energy ? a.energy + 3 * ((long long)b * a.hist_steps + a.hist_step)
+                                : nullptr,
+                       sh, qs2, win64, a.win);
   // the tables follow in k_rebuild_rates / k_rebuild_norm
 }
 

@@ -78,68 +78,37 @@ __host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
   }
 }
 
-__device__ __forceinline__ float u01_f32(uint32_t w) {
+// a uniform in (0, 1) from the top 24 bits of a word, forced odd: exact in
+// FP32, so both precisions see the same value (the FP64 kernels widen it)
+__host__ __device__ inline float u01_f32(uint32_t w) {
   return (float)((w >> 8) | 1u) * 0x1p-24f;
-}
-__host__ __device__ inline double u01_f64(uint32_t hi, uint32_t lo) {
-  uint64_t m = ((((uint64_t)hi) << 32) | lo) >> 11;
-  return (double)(m | 1ull) * 0x1p-53;
 }
 
 // the four uniforms of scatter event e: selection, two final-state
-// uniforms, free-flight time
+// uniforms, free-flight time -- one counter block, one word each, in both
+// precisions, so an FP32 and an FP64 run on the same key follow the same
+// random path
 template <typename R>
-struct EventDraws;
-
-template <>
-struct EventDraws<float> {
+struct EventDraws {
   static __device__ __forceinline__ void get(uint32_t e, uint32_t step, uint32_t pid,
                                              uint32_t inst, uint32_t k0, uint32_t k1,
-                                             float u[4]) {
+                                             R u[4]) {
     uint32_t c[4] = {e, step, pid, inst};
     philox4x32_10(c, k0, k1);
-    u[0] = u01_f32(c[0]);
-    u[1] = u01_f32(c[1]);
-    u[2] = u01_f32(c[2]);
-    u[3] = u01_f32(c[3]);
+    u[0] = (R)u01_f32(c[0]);
+    u[1] = (R)u01_f32(c[1]);
+    u[2] = (R)u01_f32(c[2]);
+    u[3] = (R)u01_f32(c[3]);
   }
 };
 
-template <>
-struct EventDraws<double> {
-  static __device__ __forceinline__ void get(uint32_t e, uint32_t step, uint32_t pid,
-                                             uint32_t inst, uint32_t k0, uint32_t k1,
-                                             double u[4]) {
-    uint32_t a[4] = {2u * e, step, pid, inst};
-    uint32_t b[4] = {2u * e + 1u, step, pid, inst};
-    philox4x32_10(a, k0, k1);
-    philox4x32_10(b, k0, k1);
-    u[0] = u01_f64(a[0], a[1]);
-    u[1] = u01_f64(a[2], a[3]);
-    u[2] = u01_f64(b[0], b[1]);
-    u[3] = u01_f64(b[2], b[3]);
-  }
-};
-
-// uniform #j of a (step, particle) stream: FP32 uses one word per uniform,
-// FP64 a pair of words
+// uniform #j of a (step, particle) stream: word j % 4 of counter block j / 4
 template <typename R>
-__device__ __forceinline__ R stream_uniform(uint32_t j, uint32_t step, uint32_t pid,
-                                            uint32_t inst, uint32_t k0, uint32_t k1);
-template <>
-__device__ __forceinline__ float stream_uniform<float>(uint32_t j, uint32_t step, uint32_t pid,
-                                                       uint32_t inst, uint32_t k0, uint32_t k1) {
+__host__ __device__ inline R stream_uniform(uint32_t j, uint32_t step, uint32_t pid,
+                                            uint32_t inst, uint32_t k0, uint32_t k1) {
   uint32_t c[4] = {j >> 2, step, pid, inst};
   philox4x32_10(c, k0, k1);
-  return u01_f32(c[j & 3u]);
-}
-template <>
-__device__ __forceinline__ double stream_uniform<double>(uint32_t j, uint32_t step, uint32_t pid,
-                                                         uint32_t inst, uint32_t k0, uint32_t k1) {
-  uint32_t c[4] = {j >> 1, step, pid, inst};
-  philox4x32_10(c, k0, k1);
-  const int h = (int)(j & 1u) * 2;
-  return u01_f64(c[h], c[h + 1]);
+  return (R)u01_f32(c[j & 3u]);
 }
 
 // recorded raw draws of the reference engine (replay mode)

@@ -151,6 +151,7 @@ EXPORTS = [
     "emcgpu_device_drift", "emcgpu_device_init", "emcgpu_device_step", "emcgpu_device_fields",
     "emcgpu_autotune", "emcgpu_device_averages", "emcgpu_set_screening", "emcgpu_get_screening",
     "emcgpu_update_screening", "emcgpu_build_tables_all", "emcgpu_step_noalloc",
+    "emcgpu_reserve", "emcgpu_get_bath_energy",
 ]
 
 
@@ -180,6 +181,7 @@ def load(path=LIB_PATH):
         "emcgpu_init_particles": (i, [vp, i, C.c_uint64, d]),
         "emcgpu_step": (i, [vp, p(StepArgs)]),
         "emcgpu_step_noalloc": (i, [vp, p(StepArgs), vp, C.c_size_t]),
+        "emcgpu_reserve": (i, [vp, p(StepArgs)]),
         "emcgpu_get_observables": (i, [vp, i, vp, p(i), p(i)]),
         "emcgpu_obs_layout": (i, [vp, p(i), p(i), p(i)]),
         "emcgpu_get_timing": (i, [vp, p(d), p(C.c_longlong), i]),
@@ -193,6 +195,7 @@ def load(path=LIB_PATH):
         "emcgpu_set_bath_state": (i, [vp, i, i, vp, d, d]),
         "emcgpu_get_bath_tallies": (i, [vp, i, i, vp, vp]),
         "emcgpu_get_bath_history": (i, [vp, i, i, vp, p(i)]),
+        "emcgpu_get_bath_energy": (i, [vp, i, i, vp, p(i)]),
         "emcgpu_device_setup": (i, [vp, p(DeviceDesc)]),
         "emcgpu_device_info": (i, [vp, vp, vp]),
         "emcgpu_device_grids": (i, [vp, vp, vp, vp]),
@@ -224,7 +227,7 @@ def _ptr(a):
 
 
 def valley_desc(v, particle_mass=ME):
-    """Lower a valley dictionary (tests/emc_models.py format) to the C descriptor."""
+    """Lower a valley dictionary (viennaemc_amd/workloads.py format) to the C descriptor."""
     d = ValleyDesc()
     d.kind = v["kind"]
     for j in range(3):
@@ -241,7 +244,7 @@ def valley_desc(v, particle_mass=ME):
 
 
 def mech_desc(m):
-    """Lower a mechanism dictionary (tests/emc_models.py format) to the C descriptor."""
+    """Lower a mechanism dictionary (viennaemc_amd/workloads.py format) to the C descriptor."""
     d = MechDesc()
     d.kind = m["kind"]
     d.valley = m["valley"]
@@ -268,7 +271,7 @@ def mech_desc(m):
 
 
 def bath_desc(b, T):
-    """Lower a bath dictionary (tests/emc_models.py format) to the C descriptor;
+    """Lower a bath dictionary (viennaemc_amd/workloads.py format) to the C descriptor;
     the returned profile array must outlive the call that uses the descriptor."""
     d = BathDesc()
     d.n_bins = b["n_bins"]
@@ -400,6 +403,16 @@ class Context:
         self._chk(self.L.emcgpu_get_bath_history(self.h, instance, bath, _ptr(out), C.byref(n)))
         return out
 
+    def bath_energy(self, instance=0, bath=0):
+        """Energy record of every bath update of the last step() call, shape
+        (n_steps, 3) in eV: net emitted phonon energy, LO energy gained, LO
+        energy passed to the reservoirs."""
+        n = C.c_int()
+        self._chk(self.L.emcgpu_get_bath_energy(self.h, instance, bath, None, C.byref(n)))
+        out = np.zeros(3 * n.value)
+        self._chk(self.L.emcgpu_get_bath_energy(self.h, instance, bath, _ptr(out), C.byref(n)))
+        return out.reshape(n.value, 3)
+
     def set_screening(self, instance, qs2, enabled=True, eps_static=1.0, mode=SCREENING_FIXED,
                       volume=0.0):
         d = ScreeningDesc(mode, 1 if enabled else 0, eps_static, qs2, volume)
@@ -509,6 +522,17 @@ class Context:
         self._chk(self.L.emcgpu_step(self.h, C.byref(a)))
         return tr
 
+    def reserve(self, dt, n_steps, steps_per_launch=0, ppl=0):
+        """emcgpu_reserve: size the buffers of a later step() call with these
+        arguments (or fewer steps) so that call allocates nothing."""
+        a = StepArgs()
+        a.dt = dt
+        a.n_steps = n_steps
+        a.steps_per_launch = steps_per_launch
+        a.mode = PRODUCTION
+        a.particles_per_lane = ppl
+        self._chk(self.L.emcgpu_reserve(self.h, C.byref(a)))
+
     def step_noalloc(self, dt, n_steps, obs_ptr, obs_len, seed=0, step0=0,
                      steps_per_launch=0, ppl=0, timing=False):
         """emcgpu_step_noalloc: production steps with no device allocation;
@@ -587,7 +611,7 @@ class Context:
     # -- 2-D device (instance 0) ------------------------------------------
     def device_setup(self, dev, pm=PM_CIC, poisson_tol=1e-10, poisson_max_iter=50,
                      init_from_potential=False):
-        """Geometry from a device dictionary (tests/emc_models.py format):
+        """Geometry from a device dictionary (viennaemc_amd/workloads.py format):
         max_pos, spacing, width, T, eps_r, ni, doping [(min, max, N)],
         contacts [(boundary, voltage, min, max)]."""
         regs = (DopingRegion * max(len(dev["doping"]), 1))()
