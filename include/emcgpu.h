@@ -220,6 +220,7 @@ typedef struct {
   int32_t *trace_events; /* [n_steps][n] scatter calls incl. self scattering */
   double *trace_k;       /* [n_steps][n][3] 1/m */
   double *trace_energy;  /* [n_steps][n] eV */
+  double *trace_pos;     /* [n_steps][n][3] m, with tracked positions (may be NULL) */
 } emcgpu_step_args;
 
 /* ---- lifecycle ---------------------------------------------------------
@@ -294,6 +295,18 @@ emcgpu_status emcgpu_init_particles(emcgpu_ctx *ctx, int instance,
 emcgpu_status emcgpu_init_particles_energy(emcgpu_ctx *ctx, int instance,
                                            uint64_t seed, double mean_ev,
                                            double sigma_ev);
+
+/* ---- positions of bulk particles ------------------------------------------
+ * The bulk handler keeps a position per particle, drifted with the free
+ * flight (include/emcParticleDrift.hpp:26-34) and wrapped into the device
+ * box [0, maxPos] (basicBulkParticleHandler.hpp:600-613); it is written by
+ * print (:227-248).  emcgpu_set_box turns this on for the context (the
+ * step kernels then run one particle per lane) with the instance's box
+ * max_pos[3] [m]; emcgpu_init_particles then also draws positions uniform
+ * in the box.  Positions are [n][3] in m. */
+emcgpu_status emcgpu_set_box(emcgpu_ctx *ctx, int instance, const double max_pos[3]);
+emcgpu_status emcgpu_set_positions(emcgpu_ctx *ctx, int instance, const double *pos);
+emcgpu_status emcgpu_get_positions(const emcgpu_ctx *ctx, int instance, double *pos);
 
 /* ---- plasmon screening ---------------------------------------------------
  * Replaces the shared_ptr<emcPlasmonScreening> of the screened mechanisms
@@ -387,6 +400,11 @@ emcgpu_status emcgpu_get_observables(const emcgpu_ctx *ctx, int instance,
                                      double *out, int *n_obs, int *n_steps);
 emcgpu_status emcgpu_obs_layout(const emcgpu_ctx *ctx, int *n_obs,
                                 int *max_valleys, int *max_deg);
+/* the ensemble sums of the current state of every instance without a step,
+ * out[instance][j] in the layout above (scatter counts 0): the getAvg*
+ * values right after generateInitialParticles
+ * (bulkSimulation.cpp:143-146) */
+emcgpu_status emcgpu_observe(emcgpu_ctx *ctx, double *out);
 
 /* ---- 2-D device simulation ------------------------------------------------
  * The device path of emcSimulation (emcSimulation.hpp:106-193) on instance 0

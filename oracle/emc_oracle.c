@@ -1249,6 +1249,22 @@ int orc_scatter(const orc_model *m, orc_particle *p, orc_rng *rng) {
   return -2;
 }
 
+/* driftParticle of the bulk handler (basicBulkParticleHandler.hpp:600-613):
+ * drift with the position, then each coordinate wrapped into [0, maxPos]
+ * once; pos == NULL drifts k only */
+static void orc_drift_wrap(double dt, orc_particle *p, const orc_valley *v, double *pos,
+                           const double box[3], const double force[3]) {
+  orc_drift(dt, p, v, pos, 3, force);
+  if (!pos)
+    return;
+  for (int d = 0; d < 3; d++) {
+    if (pos[d] < 0)
+      pos[d] = pos[d] + box[d];
+    else if (pos[d] > box[d])
+      pos[d] = pos[d] - box[d];
+  }
+}
+
 /* getNewTau (ParticleType/emcParticleType.hpp:187-189) */
 double orc_new_tau(const orc_model *m, int valley, orc_rng *rng) {
   return -log(orc_rng_uniform(rng, 1e-6, 1.)) * m->tau0[valley];
@@ -1259,9 +1275,9 @@ double orc_new_tau(const orc_model *m, int valley, orc_rng *rng) {
  * ====================================================================== */
 void orc_move_particle(const orc_model *m, orc_particle *p, double tStep,
                        const double force[3], orc_rng *rng,
-                       orc_step_info *info, int32_t *evCount) {
+                       orc_step_info *info, int32_t *evCount, double *pos) {
   const orc_valley *v = &m->valley[p->valley];
-  orc_drift(orc_stdmin(p->tau, tStep), p, v, NULL, 3, force);
+  orc_drift_wrap(orc_stdmin(p->tau, tStep), p, v, pos, m->box, force);
   double tRemaining = tStep - p->tau;
   int32_t nEv = 0, nReal = 0, last = -1;
   while (tRemaining > 0) {
@@ -1276,7 +1292,7 @@ void orc_move_particle(const orc_model *m, orc_particle *p, double tStep,
     double newTau = orc_new_tau(m, p->valley, rng);
     p->tau += newTau;
     v = &m->valley[p->valley];
-    orc_drift(orc_stdmin(tRemaining, newTau), p, v, NULL, 3, force);
+    orc_drift_wrap(orc_stdmin(tRemaining, newTau), p, v, pos, m->box, force);
     tRemaining -= newTau;
   }
   p->tau -= tStep;
@@ -1297,7 +1313,7 @@ int orc_run_replay(orc_model *m, int np, double *k, double *E,
                    size_t *cursor, int nsteps, double tStep,
                    const double force[3], int32_t *valleyOut,
                    int32_t *subOut, int32_t *nEventsOut, double *kOut,
-                   double *EOut, int32_t *evCount) {
+                   double *EOut, int32_t *evCount, double *pos, double *posOut) {
   int ntMax = m->nTables[0];
   for (int v = 1; v < m->nValleys; v++)
     if (m->nTables[v] > ntMax)
@@ -1321,8 +1337,12 @@ int orc_run_replay(orc_model *m, int np, double *k, double *E,
       rng.pos = cursor[ip];
       orc_step_info info;
       orc_move_particle(m, &p, tStep, force, &rng, &info,
-                        evCount ? evCount + (size_t)ip * ntMax : NULL);
+                        evCount ? evCount + (size_t)ip * ntMax : NULL,
+                        pos ? pos + 3 * (size_t)ip : NULL);
       size_t o = (size_t)s * np + ip;
+      if (pos && posOut)
+        for (int d = 0; d < 3; d++)
+          posOut[3 * o + d] = pos[3 * (size_t)ip + d];
       if (valleyOut)
         valleyOut[o] = p.valley;
       if (subOut)
@@ -1430,11 +1450,20 @@ void orc_observables(const orc_model *m, int np, const orc_particle *parts,
  * generateInitialParticle (emcElectron.hpp:75-90) with the Maxwellian
  * k-space init and its boundary flips (emcParticleInitialization.hpp:36-51) */
 static void orc_shared_add(const orc_model *m, orc_rng *rng, const size_t c[3],
-                           const size_t ext[3], double Vt, orc_particle *parts,
-                           int *np, int maxParticles) {
+                           const size_t ext[3], const double spacing[3], double Vt,
+                           orc_particle *parts, double *pos, int *np, int maxParticles) {
   orc_particle p;
-  for (int d = 0; d < 3; d++)
-    (void)orc_rng_uniform(rng, 0., 1.);
+  /* initParticlePos (emcParticleInitialization.hpp:13-29) */
+  double x[3];
+  for (int d = 0; d < 3; d++) {
+    double u = orc_rng_uniform(rng, 0., 1.);
+    if (c[d] == ext[d] - 1)
+      x[d] = ((double)c[d] - u * 0.5) * spacing[d];
+    else if (c[d] == 0)
+      x[d] = u * 0.5 * spacing[d];
+    else
+      x[d] = ((double)c[d] + u - 0.5) * spacing[d];
+  }
   memset(&p, 0, sizeof p);
   p.valley = (int)floor((double)(size_t)m->nValleys * orc_rng_uniform(rng, 1e-6, 1.));
   const orc_valley *v = &m->valley[p.valley];
@@ -1453,8 +1482,11 @@ static void orc_shared_add(const orc_model *m, orc_rng *rng, const size_t c[3],
       p.k[d] *= -1;
   p.tau = orc_new_tau(m, p.valley, rng);
   p.grainTau = -log(orc_rng_uniform(rng, 1e-6, 1.)) * 1.0;
-  if (*np < maxParticles)
+  if (*np < maxParticles) {
+    for (int d = 0; d < 3; d++)
+      pos[3 * (size_t)*np + d] = x[d];
     parts[(*np)++] = p;
+  }
 }
 
 /* ---- full reference bulk driver with one shared stream ---------------- */
@@ -1463,9 +1495,13 @@ int orc_run_shared(orc_model *m, uint64_t seed, const double maxPos[3],
                    const double fieldDir[3], double fieldStrength,
                    double tStep, int nsteps, int maxParticles, double *obs,
                    double *kOut, double *EOut, int32_t *valleyOut,
-                   int32_t *subOut, double *bathMeanOut) {
+                   int32_t *subOut, double *bathMeanOut, double *posInitOut,
+                   double *posOut) {
   orc_rng rng;
   orc_rng_init_mt(&rng, seed);
+  /* positions drift with the particles in the device's periodic box */
+  for (int d = 0; d < 3; d++)
+    m->box[d] = maxPos[d];
   /* emcDevice: extent = round(maxPos/spacing)+1 (emcUtil.hpp:115-124),
    * cellVolume = prod(spacing) (emcDevice.hpp:333-343),
    * Vt = kB/q*T (emcDevice.hpp:88) */
@@ -1487,6 +1523,7 @@ int orc_run_shared(orc_model *m, uint64_t seed, const double maxPos[3],
 
   orc_particle *parts =
       (orc_particle *)malloc(sizeof(orc_particle) * (size_t)maxParticles);
+  double *pos = (double *)malloc(sizeof(double) * 3 * (size_t)maxParticles);
   int np = 0, needed = 0;
   /* generateInitialParticles (:143-159), x-fastest coordinate order
    * (emcGrid.hpp:122-137) */
@@ -1499,12 +1536,12 @@ int orc_run_shared(orc_model *m, uint64_t seed, const double maxPos[3],
         dens *= 0.5;
     double nr = dens * cellVolume;
     while (nr >= 1) {
-      orc_shared_add(m, &rng, c, ext, Vt, parts, &np, maxParticles);
+      orc_shared_add(m, &rng, c, ext, spacing, Vt, parts, pos, &np, maxParticles);
       needed++;
       nr--;
     }
     if (orc_rng_uniform(&rng, 0., 1.) < nr) {
-      orc_shared_add(m, &rng, c, ext, Vt, parts, &np, maxParticles);
+      orc_shared_add(m, &rng, c, ext, spacing, Vt, parts, pos, &np, maxParticles);
       needed++;
     }
     /* advanceCoord */
@@ -1519,8 +1556,11 @@ int orc_run_shared(orc_model *m, uint64_t seed, const double maxPos[3],
   }
   if (needed > maxParticles) {
     free(parts);
+    free(pos);
     return -needed;
   }
+  if (posInitOut)
+    memcpy(posInitOut, pos, sizeof(double) * 3 * (size_t)np);
   int nv = m->nValleys;
   if (m->scrPlasma) {
     /* the screening seeded from the initial carriers (hotCarrierMHP.cpp:
@@ -1534,7 +1574,7 @@ int orc_run_shared(orc_model *m, uint64_t seed, const double maxPos[3],
   orc_observables(m, np, parts, dir, obs, obs + nv, obs + 2 * nv);
   for (int s = 1; s <= nsteps; s++) {
     for (int i = 0; i < np; i++)
-      orc_move_particle(m, &parts[i], tStep, force, &rng, NULL, NULL);
+      orc_move_particle(m, &parts[i], tStep, force, &rng, NULL, NULL, pos + 3 * (size_t)i);
     double *o = obs + (size_t)s * 3 * nv;
     orc_observables(m, np, parts, dir, o, o + nv, o + 2 * nv);
     {
@@ -1560,7 +1600,10 @@ int orc_run_shared(orc_model *m, uint64_t seed, const double maxPos[3],
     if (subOut)
       subOut[i] = parts[i].subValley;
   }
+  if (posOut)
+    memcpy(posOut, pos, sizeof(double) * 3 * (size_t)np);
   free(parts);
+  free(pos);
   return np;
 }
 
@@ -1746,9 +1789,9 @@ void orc_move_particle_philox(const orc_model *m, orc_particle *p,
                               double tStep, const double force[3],
                               uint64_t seed, uint32_t inst, uint32_t pid,
                               uint32_t step, orc_step_info *info,
-                              int32_t *evCount) {
+                              int32_t *evCount, double *pos) {
   const orc_valley *v = &m->valley[p->valley];
-  orc_drift(orc_stdmin(p->tau, tStep), p, v, NULL, 3, force);
+  orc_drift_wrap(orc_stdmin(p->tau, tStep), p, v, pos, m->box, force);
   double tRemaining = tStep - p->tau;
   int32_t nEv = 0, nReal = 0, last = -1;
   uint32_t e = 0;
@@ -1784,7 +1827,7 @@ void orc_move_particle_philox(const orc_model *m, orc_particle *p,
     double newTau = -log(ut) * m->tau0[p->valley];
     p->tau += newTau;
     v = &m->valley[p->valley];
-    orc_drift(orc_stdmin(tRemaining, newTau), p, v, NULL, 3, force);
+    orc_drift_wrap(orc_stdmin(tRemaining, newTau), p, v, pos, m->box, force);
     tRemaining -= newTau;
   }
   p->tau -= tStep;
@@ -2030,7 +2073,7 @@ void orc_run_philox(orc_model *m, int np, double *k, double *E,
                     double tStep, const double force[3],
                     const double fieldDir[3], uint64_t seed,
                     uint32_t instance, uint32_t step0, double *obs,
-                    int32_t *nEventsOut) {
+                    int32_t *nEventsOut, double *pos) {
   int nv = m->nValleys;
   for (int s = 0; s < nsteps; s++) {
     for (int i = 0; i < np; i++) {
@@ -2039,7 +2082,8 @@ void orc_run_philox(orc_model *m, int np, double *k, double *E,
       p.tau = tau[i];
       orc_step_info info;
       orc_move_particle_philox(m, &p, tStep, force, seed, instance, (uint32_t)i,
-                               step0 + (uint32_t)s, &info, NULL);
+                               step0 + (uint32_t)s, &info, NULL,
+                               pos ? pos + 3 * (size_t)i : NULL);
       orc_unpack(&p, k + 3 * i, E + i, valley + i, sub + i);
       tau[i] = p.tau;
       if (nEventsOut)
@@ -2059,13 +2103,23 @@ void orc_run_philox(orc_model *m, int np, double *k, double *E,
 void orc_init_particles_philox(const orc_model *m, int np, double Vt,
                                uint64_t seed, uint32_t instance, double *k,
                                double *E, double *tau, int32_t *valley,
-                               int32_t *sub) {
+                               int32_t *sub, double *pos) {
   for (int i = 0; i < np; i++) {
     orc_particle p;
     orc_init_particle_philox(m, &p, Vt, seed, instance, (uint32_t)i);
     orc_unpack(&p, k + 3 * i, E + i, valley + i, sub + i);
     tau[i] = p.tau;
+    /* positions uniform in the box: uniforms 8, 9, 10 of the stream */
+    if (pos)
+      for (int d = 0; d < 3; d++)
+        pos[3 * i + d] =
+            orc_prod_u(seed, instance, (uint32_t)i, 0xFFFFFFFFu, 8 + (uint32_t)d) * m->box[d];
   }
+}
+
+void orc_model_set_box(orc_model *m, const double box[3]) {
+  for (int d = 0; d < 3; d++)
+    m->box[d] = box[d];
 }
 
 void orc_init_particles_philox_energy(const orc_model *m, int np, double E0,
@@ -2184,7 +2238,7 @@ static void *orc_mt_worker(void *arg) {
   for (int s = 0; s < j->nsteps; s++)
     for (int i = j->i0; i < j->i1; i++)
       orc_move_particle(j->m, &j->parts[i], j->tStep, j->force, &j->rngs[i], NULL,
-                        NULL);
+                        NULL, NULL);
   return NULL;
 }
 

@@ -275,6 +275,9 @@ typedef struct {
    * with static permittivity scrEps in the volume scrVol */
   int scrPlasma;
   double scrEps, scrVol;
+  /* periodic box [m] of tracked positions: the device's maxPos of the bulk
+   * handler (basicBulkParticleHandler.hpp:600-613) */
+  double box[3];
 } orc_model;
 
 void orc_model_init(orc_model *m, int nLevels, double maxEnergy);
@@ -327,7 +330,7 @@ typedef struct {
 /* one particle, one step (:195-221); evCount[nTables] optional */
 void orc_move_particle(const orc_model *m, orc_particle *p, double tStep,
                        const double force[3], orc_rng *rng,
-                       orc_step_info *info, int32_t *evCount);
+                       orc_step_info *info, int32_t *evCount, double *pos);
 
 /* Replay runner: np particles with per-particle raw streams
  * draws[p*stride ...].  Arrays are SoA, k[3*np] (x,y,z per particle).
@@ -340,7 +343,7 @@ int orc_run_replay(orc_model *m, int np, double *k, double *E,
                    size_t *cursor, int nsteps, double tStep,
                    const double force[3], int32_t *valleyOut,
                    int32_t *subOut, int32_t *nEventsOut, double *kOut,
-                   double *EOut, int32_t *evCount);
+                   double *EOut, int32_t *evCount, double *pos, double *posOut);
 
 /* per-particle initial state from a per-particle stream
  * (emcElectron.hpp:75-90, emcParticleInitialization.hpp:36-51), with the
@@ -370,7 +373,8 @@ int orc_run_shared(orc_model *m, uint64_t seed, const double maxPos[3],
                    const double fieldDir[3], double fieldStrength,
                    double tStep, int nsteps, int maxParticles, double *obs,
                    double *kOut, double *EOut, int32_t *valleyOut,
-                   int32_t *subOut, double *bathMeanOut);
+                   int32_t *subOut, double *bathMeanOut, double *posInitOut,
+                   double *posOut);
 
 /* ---- production mode (Philox draws, FP64) ------------------------------
  * The CPU twin of the GPU production kernel: same algorithm as
@@ -381,7 +385,7 @@ void orc_move_particle_philox(const orc_model *m, orc_particle *p,
                               double tStep, const double force[3],
                               uint64_t seed, uint32_t instance, uint32_t pid,
                               uint32_t step, orc_step_info *info,
-                              int32_t *evCount);
+                              int32_t *evCount, double *pos);
 void orc_init_particle_philox(const orc_model *m, orc_particle *p, double Vt,
                               uint64_t seed, uint32_t instance, uint32_t pid);
 /* one production-mode scatter event e of the (step, particle) stream:
@@ -475,11 +479,13 @@ void orc_run_philox(orc_model *m, int np, double *k, double *E,
                     double tStep, const double force[3],
                     const double fieldDir[3], uint64_t seed,
                     uint32_t instance, uint32_t step0, double *obs,
-                    int32_t *nEventsOut);
+                    int32_t *nEventsOut, double *pos);
 void orc_init_particles_philox(const orc_model *m, int np, double Vt,
                                uint64_t seed, uint32_t instance, double *k,
                                double *E, double *tau, int32_t *valley,
-                               int32_t *sub);
+                               int32_t *sub, double *pos);
+/* the periodic box [m] of tracked positions */
+void orc_model_set_box(orc_model *m, const double box[3]);
 void orc_init_particle_philox_energy(const orc_model *m, orc_particle *p,
                                      double E0, double sigma, uint64_t seed,
                                      uint32_t inst, uint32_t pid);

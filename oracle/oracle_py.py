@@ -74,12 +74,16 @@ def load():
         "orc_normk_one": (_D, [vp, _I, _D]),
         "orc_init_particles_replay": (None, [vp, _I, _D, _PU64, C.c_size_t, _PSZ, _PD, _PD, _PD, _PD, _PI32, _PI32]),
         "orc_run_replay": (_I, [vp, _I, _PD, _PD, _PD, _PD, _PI32, _PI32, _PU64, C.c_size_t, _PSZ, _I, _D, _PD,
-                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_void_p, C.c_void_p]),
         "orc_run_shared": (_I, [vp, C.c_uint64, _PD, _PD, _D, _D, _PD, _D, _D, _I, _I, _PD,
-                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_void_p, C.c_void_p]),
         "orc_run_philox": (None, [vp, _I, _PD, _PD, _PD, _PI32, _PI32, _I, _D, _PD, _PD, C.c_uint64,
-                                  C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
-        "orc_init_particles_philox": (None, [vp, _I, _D, C.c_uint64, C.c_uint32, _PD, _PD, _PD, _PI32, _PI32]),
+                                  C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "orc_model_set_box": (None, [vp, _PD]),
+        "orc_init_particles_philox": (None, [vp, _I, _D, C.c_uint64, C.c_uint32, _PD, _PD, _PD, _PI32, _PI32,
+                                             C.c_void_p]),
         "orc_run_mt": (None, [vp, _I, C.c_uint64, _D, _I, _D, _PD, _PD, C.c_void_p, _I,
                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                               C.c_void_p]),
@@ -267,18 +271,26 @@ class OracleModel:
         return dict(k=k, E=E, tau=tau, grain=gt, valley=v, sub=s)
 
     def run_replay(self, st, draws, cursor, nsteps, dt, force, trace=False):
+        """st may hold "pos" [np*3] (m): positions drift and wrap in the box
+        (set_box); the trace then holds "pos" per step"""
         np_ = st["E"].size
         outs = {}
+        pos = st.get("pos")
         if trace:
             outs = dict(valley=np.zeros(nsteps * np_, dtype=np.int32), sub=np.zeros(nsteps * np_, dtype=np.int32),
                         nev=np.zeros(nsteps * np_, dtype=np.int32), k=np.zeros(nsteps * np_ * 3),
                         E=np.zeros(nsteps * np_))
+            if pos is not None:
+                outs["pos"] = np.zeros(nsteps * np_ * 3)
         nover = self.L.orc_run_replay(
             self.h, np_, st["k"], st["E"], st["tau"], st["grain"], st["valley"], st["sub"], draws,
             draws.shape[1], cursor, nsteps, dt, np.array(force, dtype=np.float64),
             _ptr(outs.get("valley")), _ptr(outs.get("sub")), _ptr(outs.get("nev")), _ptr(outs.get("k")),
-            _ptr(outs.get("E")), None)
+            _ptr(outs.get("E")), None, _ptr(pos), _ptr(outs.get("pos")))
         return nover, outs
+
+    def set_box(self, box):
+        self.L.orc_model_set_box(self.h, np.ascontiguousarray(box, dtype=np.float64))
 
     def run_philox(self, st, nsteps, dt, force, field_dir, seed, instance=0, step0=0, obs=True):
         np_ = st["E"].size
@@ -286,17 +298,21 @@ class OracleModel:
         o = np.zeros(nsteps * 3 * nv) if obs else None
         self.L.orc_run_philox(self.h, np_, st["k"], st["E"], st["tau"], st["valley"], st["sub"], nsteps, dt,
                               np.array(force, dtype=np.float64), np.array(field_dir, dtype=np.float64),
-                              seed, instance, step0, _ptr(o), None)
+                              seed, instance, step0, _ptr(o), None, _ptr(st.get("pos")))
         return None if o is None else o.reshape(nsteps, 3, nv)
 
-    def init_philox(self, np_, Vt, seed, instance=0):
+    def init_philox(self, np_, Vt, seed, instance=0, positions=False):
         k = np.zeros(3 * np_)
         E = np.zeros(np_)
         tau = np.zeros(np_)
         v = np.zeros(np_, dtype=np.int32)
         s = np.zeros(np_, dtype=np.int32)
-        self.L.orc_init_particles_philox(self.h, np_, Vt, seed, instance, k, E, tau, v, s)
-        return dict(k=k, E=E, tau=tau, valley=v, sub=s)
+        pos = np.zeros(3 * np_) if positions else None
+        self.L.orc_init_particles_philox(self.h, np_, Vt, seed, instance, k, E, tau, v, s, _ptr(pos))
+        out = dict(k=k, E=E, tau=tau, valley=v, sub=s)
+        if positions:
+            out["pos"] = pos
+        return out
 
     def run_mt(self, np_, seed, Vt, nsteps, dt, force, field_dir, obs=True, threads=1, final=False,
                timing=None, sub_occ=False):
@@ -321,17 +337,32 @@ class OracleModel:
         return res
 
     def run_shared(self, seed, max_pos, spacing, doping, T, field_dir, field_strength, dt, nsteps,
-                   max_particles=100000, bath_history=False):
+                   max_particles=100000, bath_history=False, state=False):
+        """the real handler's loop on one shared stream; with state, a third
+        result: the initial positions and the final particles with their
+        positions (keys pos0, pos, k, E, valley, sub)"""
         nv = self.n_valleys
         obs = np.zeros((nsteps + 1) * 3 * nv)
         hist = np.zeros(max(nsteps * self.n_baths, 1)) if bath_history else None
+        st = None
+        if state:
+            m = max_particles
+            st = dict(pos0=np.zeros(3 * m), pos=np.zeros(3 * m), k=np.zeros(3 * m), E=np.zeros(m),
+                      valley=np.zeros(m, dtype=np.int32), sub=np.zeros(m, dtype=np.int32))
+        g = (lambda key: _ptr(st[key])) if state else (lambda key: None)
         n = self.L.orc_run_shared(self.h, seed, np.array(max_pos, dtype=np.float64),
                                   np.array(spacing, dtype=np.float64), doping, T,
                                   np.array(field_dir, dtype=np.float64), field_strength, dt, nsteps,
-                                  max_particles, obs, None, None, None, None, _ptr(hist))
+                                  max_particles, obs, g("k"), g("E"), g("valley"), g("sub"),
+                                  _ptr(hist), g("pos0"), g("pos"))
+        res = (n, obs.reshape(nsteps + 1, 3, nv))
         if bath_history:
-            return n, obs.reshape(nsteps + 1, 3, nv), hist[:nsteps * self.n_baths].reshape(nsteps, -1)
-        return n, obs.reshape(nsteps + 1, 3, nv)
+            res = res + (hist[:nsteps * self.n_baths].reshape(nsteps, -1),)
+        if state:
+            for key, per in (("pos0", 3), ("pos", 3), ("k", 3), ("E", 1), ("valley", 1), ("sub", 1)):
+                st[key] = st[key][:per * n]
+            res = res + (st,)
+        return res
 
     def init_philox_energy(self, np_, E0, sigma, seed, instance=0):
         k = np.zeros(3 * np_)

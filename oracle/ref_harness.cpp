@@ -287,6 +287,22 @@ static std::unique_ptr<Electron> makeType(const Model &m, Device &device, BathPt
 // Recorded pieces
 // ---------------------------------------------------------------------------
 
+// run a scope inside a folder (created if missing)
+struct InDir {
+  std::string prev;
+  explicit InDir(const char *d) {
+    mkdir(d, 0755);
+    char buf[4096];
+    prev = getcwd(buf, sizeof buf) ? buf : ".";
+    if (chdir(d) != 0)
+      std::perror("chdir");
+  }
+  ~InDir() {
+    if (chdir(prev.c_str()) != 0)
+      std::perror("chdir");
+  }
+};
+
 // rates of every mechanism (standalone copies bound to the type's valleys)
 // on a sample grid and on the table grid (l+1)*dE, plus tau0 per valley
 static void dumpRates(const Model &m, const Device &device, Electron &type,
@@ -346,6 +362,26 @@ static void dumpTrajectories(const Model &m, Device &device, Electron &type,
   std::array<SizeType, 3> coord = {1, 1, 1};
   std::vector<emcParticle<T>> parts(np);
   std::vector<emcRNG> rngs;
+  // positions in the device's periodic box (driftParticle of the bulk
+  // handler, basicBulkParticleHandler.hpp:600-613), started on a fixed
+  // low-discrepancy set so the particles' streams are untouched
+  const auto maxPos = device.getMaxPos();
+  std::vector<std::array<T, 3>> posP(np);
+  std::vector<double> initPos, posOut;
+  for (int p = 0; p < np; p++)
+    for (int d = 0; d < 3; d++) {
+      const double f = std::fmod(0.5 + 0.6180339887498949 * (p + 1) + 0.3819660112501051 * d, 1.0);
+      posP[p][d] = f * maxPos[d];
+      initPos.push_back(posP[p][d]);
+    }
+  auto wrap = [&](std::array<T, 3> &pos) {
+    for (int d = 0; d < 3; d++) {
+      if (pos[d] < 0)
+        pos[d] = pos[d] + maxPos[d];
+      else if (pos[d] > maxPos[d])
+        pos[d] = pos[d] - maxPos[d];
+    }
+  };
   for (int p = 0; p < np; p++) {
     rngs.emplace_back(splitmix64(seed ^ (uint64_t)p));
     parts[p] = type.generateInitialParticle(coord, device, rngs[p]);
@@ -381,11 +417,12 @@ static void dumpTrajectories(const Model &m, Device &device, Electron &type,
     for (int p = 0; p < np; p++) {
       auto &part = parts[p];
       auto &rng = rngs[p];
-      std::array<T, 3> pos = {0, 0, 0};
+      std::array<T, 3> &pos = posP[p];
       emcRNG atStart = rng;
       // the loop body of basicBulkParticleHandler::moveParticles (:195-221)
       auto valley = type.getValley(part.valley);
       drift(std::min(part.tau, dt), part, valley, pos, force);
+      wrap(pos);
       T tRemaining = dt - part.tau;
       long long nev = 0;
       while (tRemaining > 0) {
@@ -404,6 +441,7 @@ static void dumpTrajectories(const Model &m, Device &device, Electron &type,
         part.tau += newTau;
         valley = type.getValley(part.valley);
         drift(std::min(tRemaining, newTau), part, valley, pos, force);
+        wrap(pos);
         tRemaining -= newTau;
       }
       part.tau -= dt;
@@ -413,6 +451,7 @@ static void dumpTrajectories(const Model &m, Device &device, Electron &type,
         part.grainTau = type.getNewGrainTau(rng);
       }
       kOut.insert(kOut.end(), part.k.begin(), part.k.end());
+      posOut.insert(posOut.end(), pos.begin(), pos.end());
       eOut.push_back(part.energy);
       tauOut.push_back(part.tau);
       vOut.push_back((long long)part.valley);
@@ -452,6 +491,9 @@ static void dumpTrajectories(const Model &m, Device &device, Electron &type,
   seci((b + ".init_sub").c_str(), initSub);
   seci((b + ".init_draws").c_str(), drawsInit);
   sec((b + ".k").c_str(), kOut);
+  sec((b + ".init_pos").c_str(), initPos);
+  sec((b + ".pos").c_str(), posOut);
+  sec((b + ".box").c_str(), {maxPos[0], maxPos[1], maxPos[2]});
   sec((b + ".E").c_str(), eOut);
   sec((b + ".tau").c_str(), tauOut);
   seci((b + ".valley").c_str(), vOut);
@@ -581,6 +623,15 @@ static void dumpHandler(const Model &m, uint64_t seed, int nsteps, double dt,
   types[0] = makeType(m, device, bath);
   Handler handler(device, types, m.fieldDir, m.fieldKVcm * 1e5, seed);
   handler.generateInitialParticles();
+  // the bulk example's output files (bulkSimulation.cpp:127-128, 171-183)
+  // for the models without baths, in a folder of their own
+  const bool files = !bath;
+  const std::string folder = "bulk_" + m.name;
+  if (files) {
+    InDir in(folder.c_str());
+    handler.print("bulk_", "Eq");
+  }
+  std::vector<std::vector<T>> avgEnergy, avgDriftVel, valleyOcc;
   const bool scr = bath && gScreenOf.count(bath.get());
   auto screen = [&]() {
     updateScreening(bath, handler.getNrParticles(0), handler.getAvgEnergy(0)[0]);
@@ -597,6 +648,9 @@ static void dumpHandler(const Model &m, uint64_t seed, int nsteps, double dt,
     obs.insert(obs.end(), e.begin(), e.end());
     obs.insert(obs.end(), v.begin(), v.end());
     obs.insert(obs.end(), o.begin(), o.end());
+    avgEnergy.push_back(e);
+    avgDriftVel.push_back(v);
+    valleyOcc.push_back(o);
   };
   push();
   for (int s = 0; s < nsteps; s++) {
@@ -612,6 +666,17 @@ static void dumpHandler(const Model &m, uint64_t seed, int nsteps, double dt,
       }
       types[0]->reinitScatterTables();
       bathMean.push_back(bath->getMeanNq());
+    }
+  }
+  if (files) {
+    InDir in(folder.c_str());
+    handler.print("bulk_", "End");
+    std::ofstream energyFile("bulk_AvgEnergy.txt"), driftVelocityFile("bulk_AvgDriftVelocity.txt"),
+        occFile("bulk_valleyOccupation.txt");
+    for (size_t idxStep = 0; idxStep < avgEnergy.size(); idxStep++) {
+      energyFile << idxStep * dt << " " << avgEnergy[idxStep] << "\n";
+      driftVelocityFile << idxStep * dt << " " << avgDriftVel[idxStep] << "\n";
+      occFile << idxStep * dt << " " << valleyOcc[idxStep] << "\n";
     }
   }
   std::string b = m.name + ".handler";
@@ -739,20 +804,6 @@ using TypeMap2 = std::map<SizeType, std::unique_ptr<emcParticleType<T, Device2>>
 
 // runs a block with the working directory switched to a subfolder (the
 // reference writes its scatter-table dumps into the working directory)
-struct InDir {
-  std::string prev;
-  explicit InDir(const char *d) {
-    mkdir(d, 0755);
-    char buf[4096];
-    prev = getcwd(buf, sizeof buf) ? buf : ".";
-    if (chdir(d) != 0)
-      std::perror("chdir");
-  }
-  ~InDir() {
-    if (chdir(prev.c_str()) != 0)
-      std::perror("chdir");
-  }
-};
 
 static const double kDevMax[2] = {20e-9, 6e-9};
 static const double kDevH = 1e-9, kDevWidth = 1e-6, kDevVBias = 0.2;

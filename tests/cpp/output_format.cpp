@@ -5,6 +5,8 @@
 // usage: output_format grid <file>
 //        output_format current <file> <contacts> <stepTime> <startTime>
 //        output_format particles <file>
+//        output_format bulkparticles <file>
+//        output_format timeseries <file> <dt>
 #include <cstdlib>
 #include <fstream>
 #include <iostream>
@@ -85,6 +87,37 @@ int main(int argc, char **argv) {
     }
     output::printParticles(std::cout, maxPos, n, pos.data(), k.data(), E.data(), sub.data(),
                            valley.data(), tau.data());
+  } else if (mode == "bulkparticles") {
+    double maxPos[3];
+    for (int d = 0; d < 3; d++)
+      maxPos[d] = std::strtod(r[0][d].c_str(), nullptr);
+    const size_t n = r.size() - 1;
+    std::vector<double> pos(3 * n), k(3 * n), E(n);
+    std::vector<int32_t> sub(n), valley(n);
+    for (size_t i = 0; i < n; i++) {
+      const auto &t = r[i + 1];
+      if (std::stoul(t[0]) != i)
+        return 4;
+      for (int d = 0; d < 3; d++) {
+        pos[3 * i + d] = std::strtod(t[1 + d].c_str(), nullptr);
+        k[3 * i + d] = std::strtod(t[4 + d].c_str(), nullptr);
+      }
+      E[i] = std::strtod(t[7].c_str(), nullptr);
+      sub[i] = std::atoi(t[8].c_str());
+      valley[i] = std::atoi(t[9].c_str());
+    }
+    output::printBulkParticles(std::cout, maxPos, n, pos.data(), k.data(), E.data(), sub.data(),
+                               valley.data());
+  } else if (mode == "timeseries") {
+    const double dt = std::strtod(argv[3], nullptr);
+    std::vector<std::vector<double>> rowsV;
+    for (size_t s = 0; s < r.size(); s++) {
+      std::vector<double> v;
+      for (size_t j = 1; j < r[s].size(); j++)
+        v.push_back(std::strtod(r[s][j].c_str(), nullptr));
+      rowsV.push_back(v);
+    }
+    output::printTimeSeries(std::cout, dt, rowsV);
   } else {
     return 2;
   }

@@ -49,10 +49,11 @@ def shrink(d):
     res = {}
     for name, a in d.items():
         parts = name.split(".")
-        if len(parts) == 3 and parts[1] == "traj" and parts[2] in ("k", "E", "tau"):
+        if (len(parts) == 3 and parts[1] == "traj" and parts[2] in ("k", "E", "tau", "pos")
+                and not (parts[0] == "device" and parts[2] == "pos")):
             meta = d[parts[0] + ".traj.meta"]
             np_, ns = int(meta[0]), int(meta[1])
-            per = 3 if parts[2] == "k" else 1
+            per = 3 if parts[2] in ("k", "pos") else 1
             a = a.reshape(ns, np_, per)
             keep = sorted(set(list(range(SNAP - 1, ns, SNAP)) + [ns - 1]))
             res[name] = a[keep].reshape(-1)
@@ -94,6 +95,14 @@ def main():
         for name in sorted(os.listdir(folder)):
             with open(os.path.join(folder, name), "rb") as f:
                 d["tabfile.%s.%s" % (sub[4:], name[:-4])] = np.frombuffer(f.read(), dtype=np.uint8)
+    # the bulk example's files of the real handler (particle dumps after
+    # initialisation and at the end, the three time series)
+    for sub in sorted(os.listdir(work)):
+        if sub.startswith("bulk_") and os.path.isdir(os.path.join(work, sub)):
+            for name in sorted(os.listdir(os.path.join(work, sub))):
+                with open(os.path.join(work, sub, name), "rb") as f:
+                    d["bulkfile.%s.%s" % (sub[5:], name[5:-4])] = np.frombuffer(f.read(),
+                                                                           dtype=np.uint8)
     d["_meta.compiler"] = np.frombuffer(
         subprocess.check_output(["g++", "--version"]).splitlines()[0], dtype=np.uint8)
     np.savez_compressed(args.out, **d)

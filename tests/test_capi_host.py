@@ -51,7 +51,7 @@ def test_struct_layouts_match_header():
     assert emcgpu.MechDesc.q_resolved_angle.offset == 324
     assert C.sizeof(emcgpu.ScreeningDesc) == 4 + 4 + 3 * 8
     assert C.sizeof(emcgpu.BathDesc) == 4 + 4 + 5 * 8 + 4 + 4 + 5 * 8 + 8
-    assert C.sizeof(emcgpu.StepArgs) == 8 + 4 * 3 + 4 + 8 + 4 + 4 + 4 + 4 + 4 + 4 + 8 * 9
+    assert C.sizeof(emcgpu.StepArgs) == 8 + 4 * 3 + 4 + 8 + 4 + 4 + 4 + 4 + 4 + 4 + 8 * 10
     assert emcgpu.StepArgs.draws.offset == 56
 
 

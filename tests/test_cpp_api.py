@@ -76,6 +76,54 @@ def test_cpp_example_matches_python_binding():
         assert float(r[4]) == av["occ"][s, 0]
 
 
+@pytest.mark.gpu
+def test_cpp_example_files_match_python_binding(tmp_path):
+    """bulkSimulationGpu with a file prefix: the bulk example's five files
+    (particle dumps with positions after initialisation and at the end, and
+    the per-step time series from step 0) hold exactly the state and the
+    averages of the Python binding on the same seed, in the reference's
+    text format."""
+    from viennaemc_amd import emcgpu
+    exe = os.path.join(PKG, "examples", "bulkSimulationGpu")
+    n, steps, seed, every, dt = 5000, 60, 777, 25, 1e-15
+    box = [5e-7] * 3
+    print("cpp example files", n, steps, flush=True)
+    prefix = str(tmp_path / "bulkSimulation")
+    subprocess.check_call([exe, str(n), str(steps), "fp64", str(seed), str(every), prefix],
+                          env=dict(os.environ, EMCGPU_THROW="1"), timeout=120,
+                          stdout=subprocess.DEVNULL)
+    spec = MODELS["si_vasileska"]
+    ctx = emcgpu.Context(precision=emcgpu.FP64)
+    ctx.setup(0, spec)
+    ctx.resize([n])
+    ctx.set_box(0, box)
+    ctx.init_particles(0, seed, spec["T"])
+
+    def dump():
+        g, pos = ctx.get_particles(0), ctx.get_positions(0)
+        k = g["k"].reshape(-1, 3)
+        lines = [" ".join("%g" % x for x in box)]
+        lines += ["%d %s %s %g %d %d" % (i, " ".join("%g" % x for x in pos[i]),
+                                         " ".join("%g" % x for x in k[i]), g["E"][i],
+                                         g["sub"][i], g["valley"][i]) for i in range(n)]
+        return "\n".join(lines).encode()
+
+    def read(part):
+        return (tmp_path / ("bulkSimulation%s.txt" % part)).read_bytes()
+
+    assert read("ElectronsEq") == dump()
+    o0 = ctx.observe()[0]
+    ctx.step(dt, steps, seed=seed)
+    assert read("ElectronsEnd") == dump()
+    av = ctx.averages(0)
+    row0 = dict(avgE=o0[3] / o0[2], avgV=o0[4] / o0[2], occ=o0[2] / n)
+    for part, key in (("AvgEnergy", "avgE"), ("AvgDriftVelocity", "avgV"),
+                      ("valleyOccupation", "occ")):
+        vals = [row0[key]] + list(av[key][:, 0])
+        text = "".join("%g %g\n" % (s * dt, v) for s, v in enumerate(vals))
+        assert read(part) == text.encode(), part
+
+
 @pytest.fixture(scope="module")
 def output_binary(tmp_path_factory):
     out = str(tmp_path_factory.mktemp("cpp") / "output_format")
@@ -97,6 +145,28 @@ def test_result_files_match_the_reference_format(output_binary, golden, tmp_path
     if mode == "current":
         _, t_sim, dt, t_tr, _, _ = golden["sim.cfg"]
         args += ["2", repr(float(dt)), repr(float(t_tr))]
+    assert subprocess.check_output(args) == text
+
+
+BULK_FILES = ["si_jacoboni", "si_vasileska", "polar_gamma", "two_valley", "para_iso",
+              "screened_polar"]
+
+
+@pytest.mark.parametrize("name", BULK_FILES)
+@pytest.mark.parametrize("part,mode", [
+    ("ElectronsEq", "bulkparticles"), ("ElectronsEnd", "bulkparticles"),
+    ("AvgEnergy", "timeseries"), ("AvgDriftVelocity", "timeseries"),
+    ("valleyOccupation", "timeseries")])
+def test_bulk_files_match_the_reference_format(output_binary, golden, tmp_path, name, part, mode):
+    """The bulk example's files (print of the bulk handler and the three
+    time series of bulkSimulation.cpp) re-printed through the host API's
+    writers equal the reference's files byte for byte."""
+    text = bytes(golden["bulkfile.%s.%s" % (name, part)])
+    src = tmp_path / (part + ".txt")
+    src.write_bytes(text)
+    args = [output_binary, mode, str(src)]
+    if mode == "timeseries":
+        args.append(repr(float(golden[name + ".handler.geom"][0])))
     assert subprocess.check_output(args) == text
 
 

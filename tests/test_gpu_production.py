@@ -74,6 +74,90 @@ def test_fp64_production_matches_oracle_twin(name, ppl):
     assert (av["occ"] == obs_ref[:, 2, :nv]).all()
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("name", ["si_jacoboni", "two_valley"])
+def test_positions_match_oracle_twin(name, precision):
+    """Positions in the periodic box on the Philox stream: initial positions
+    uniform in the box (draws 8-10 of the particle's initial stream) and
+    their drift with the free flights, against the oracle's twin; FP64
+    within 1e-12 of the box, FP32 draws bit-identical initially and within
+    1e-5 of the box after 50 steps on the FP32 trajectory (the FP32 kernel
+    follows the same history as long as no draw lands within rounding of a
+    decision boundary, so the FP32 comparison takes the particles whose
+    final valley, subvalley and k (1e-4) agree with the twin, at least 90 %
+    of them)."""
+    emcgpu = _emc()
+    spec = MODELS[name]
+    n, ns, seed = 4096, 50, 0xB0C5
+    box = (3e-7, 2e-7, 4e-7)
+    print("positions vs oracle twin:", name, precision, flush=True)
+    M = O.OracleModel(spec)
+    M.set_box(box)
+    ref = M.init_philox(n, O.thermal_voltage(spec["T"]), seed, positions=True)
+    ctx = _ctx(name, n, precision)
+    ctx.set_box(0, box)
+    ctx.init_particles(0, seed, spec["T"])
+    p0 = ctx.get_positions(0)
+    rp = ref["pos"].reshape(-1, 3)
+    if precision == emcgpu.FP64:
+        assert (np.abs(p0 - rp) <= 1e-12 * np.array(box)).all()
+    else:
+        # 24-bit uniforms times the box, rounded to FP32
+        assert (np.abs(p0 - rp) <= 2.0 ** -23 * np.array(box)).all()
+    assert (p0 >= 0).all() and (p0 <= np.array(box)).all()
+    # the step from the oracle's state
+    ctx.set_particles(0, ref["k"], ref["E"], ref["tau"], ref["valley"], ref["sub"])
+    ctx.set_positions(0, rp)
+    force, d = field_force(spec)
+    ctx.step(1e-15, ns, seed=seed, step0=3)
+    M.run_philox(ref, ns, 1e-15, force, d, seed, instance=0, step0=3, obs=False)
+    g = ctx.get_particles(0)
+    p1 = ctx.get_positions(0)
+    rp1 = ref["pos"].reshape(-1, 3)
+    same = (g["valley"] == ref["valley"]) & (g["sub"] == ref["sub"])
+    same &= _rel_k(g["k"].reshape(-1, 3), ref["k"].reshape(-1, 3)) < 1e-4
+    dp = np.abs(p1 - rp1)
+    dp = np.minimum(dp, np.array(box) - dp)
+    tol = 1e-12 if precision == emcgpu.FP64 else 1e-5
+    print("same history %.4f, max position diff / box %.2e"
+          % (same.mean(), (dp[same] / np.array(box)).max()), flush=True)
+    assert same.mean() > (0.999999 if precision == emcgpu.FP64 else 0.9)
+    assert (dp[same] <= tol * np.array(box)).all(), dp[same].max(axis=0)
+    assert (p1 >= 0).all() and (p1 <= np.array(box)).all()
+    # the particles moved: mean displacement against the field
+    assert np.abs(p1 - rp).mean() > 0
+
+
+def test_observe_matches_cpu_sums():
+    """emcgpu_observe: the ensemble sums of the current state without a step
+    (the step-0 row of the bulk example) equal the sums of the returned state."""
+    emcgpu = _emc()
+    spec = MODELS["two_valley"]
+    counts = [50_001, 777]
+    ctx = emcgpu.Context(precision=emcgpu.FP64, n_instances=2)
+    for i in range(2):
+        ctx.setup(i, spec)
+    ctx.resize(counts)
+    for i in range(2):
+        ctx.init_particles(i, 5 + i, spec["T"])
+    o = ctx.observe()
+    _, mv, md = ctx.obs_layout()
+    M = O.OracleModel(spec)
+    _, d = field_force(spec)
+    for i in range(2):
+        g = ctx.get_particles(i)
+        vdir = np.array([M.velocity(int(v), int(s), k, e) @ np.array(d) for v, s, k, e in
+                         zip(g["valley"], g["sub"], g["k"].reshape(-1, 3), g["E"])])
+        assert o[i, 0] == 0 and o[i, 1] == 0
+        for v in range(len(spec["valleys"])):
+            m = g["valley"] == v
+            assert o[i, 2 + 3 * v] == m.sum()
+            assert np.isclose(o[i, 3 + 3 * v], g["E"][m].sum(), rtol=1e-13)
+            assert np.isclose(o[i, 4 + 3 * v], vdir[m].sum(), rtol=1e-9, atol=1e-3)
+            for s in range(spec["valleys"][v]["deg"]):
+                assert o[i, 2 + 3 * mv + v * md + s] == ((g["valley"] == v) & (g["sub"] == s)).sum()
+
+
 def test_reductions_match_cpu_sums():
     """Per-step sums of the last step equal the sums of the returned state."""
     emcgpu = _emc()

@@ -186,12 +186,15 @@ def test_replay_trajectories_bit_exact(name, golden, oracle_models):
                    ("valley", "init_valley"), ("sub", "init_sub")):
         assert (st[key] == golden[b + g]).all(), key
     assert (cur == golden[b + "init_draws"]).all()
+    # positions in the periodic device box, started on the harness's fixed set
+    M.set_box(golden[b + "box"])
+    st["pos"] = golden[b + "init_pos"].copy()
     plasma = spec.get("screening", {}).get("plasma", False)
     if plasma:
         # hotCarrierMHP seeds the screening from the initial carriers
         M.seed_screening(st["E"], st["valley"])
     if M.n_baths:
-        tr = {key: [] for key in ("valley", "sub", "nev", "k", "E")}
+        tr = {key: [] for key in ("valley", "sub", "nev", "k", "E", "pos")}
         mean, nq, qs2 = [], [], []
         for s in range(ns):
             nover, t1 = M.run_replay(st, draws, cur, 1, dt, force, trace=True)
@@ -219,6 +222,11 @@ def test_replay_trajectories_bit_exact(name, golden, oracle_models):
     snaps = golden[b + "snap_steps"]
     assert (tr["k"].reshape(ns, np_, 3)[snaps].reshape(-1) == golden[b + "k"]).all()
     assert (tr["E"].reshape(ns, np_)[snaps].reshape(-1) == golden[b + "E"]).all()
+    # driftParticle of the bulk handler: position += dt * velocity, wrapped
+    pos = tr["pos"].reshape(ns, np_, 3)
+    assert (pos[snaps].reshape(-1) == golden[b + "pos"]).all()
+    box = golden[b + "box"]
+    assert (pos >= 0).all() and (pos <= box).all()
     total = golden[b + "init_draws"] + golden[b + "draws"].reshape(shape).sum(axis=0)
     assert (cur == total).all()
 
@@ -241,3 +249,42 @@ def test_bulk_handler_loop_bit_exact(name, golden, oracle_models):
         assert (hist[:, 0] == golden[hb + "bath_mean"]).all()
     ref = golden[hb + "obs"].reshape(nsteps + 1, 3, M.n_valleys)
     assert (obs == ref).all()
+
+
+def _bulk_dump(box, st, pos):
+    """basicBulkParticleHandler::print (basicBulkParticleHandler.hpp:227-248)
+    in the reference's default stream format (%g)"""
+    lines = [" ".join("%g" % x for x in box)]
+    k = st["k"].reshape(-1, 3)
+    pos = pos.reshape(-1, 3)
+    for i in range(st["E"].size):
+        lines.append("%d %s %s %g %d %d" % (i, " ".join("%g" % x for x in pos[i]),
+                                            " ".join("%g" % x for x in k[i]), st["E"][i],
+                                            st["sub"][i], st["valley"][i]))
+    return "\n".join(lines).encode()
+
+
+@pytest.mark.parametrize("name", [n for n in NAMES if not MODELS[n].get("baths")])
+def test_bulk_handler_files(name, golden):
+    """The bulk example's files of the real handler (bulkSimulation.cpp:
+    127-128, 171-183): the oracle's particles with their positions in the
+    periodic box after generateInitialParticles and after the last step, and
+    the three time series, printed in the reference's format, equal the
+    reference's files byte for byte."""
+    spec = MODELS[name]
+    hb = name + ".handler."
+    n, nsteps, seed, nodes = [int(x) for x in golden[hb + "meta"]]
+    dt, doping, spacing, T = golden[hb + "geom"]
+    box = [spacing * (nodes - 1)] * 3
+    args = ([spacing] * 3, doping, T, spec["field_dir"], spec["field_kvcm"] * 1e5, dt)
+    _, _, eq = O.OracleModel(spec).run_shared(seed, box, *args, 0, state=True)
+    _, obs, end = O.OracleModel(spec).run_shared(seed, box, *args, nsteps, state=True)
+    assert (eq["pos"] == end["pos0"]).all() and (eq["pos0"] == eq["pos"]).all()
+    pos = end["pos"].reshape(-1, 3)
+    assert (pos >= 0).all() and (pos <= box[0]).all()
+    assert _bulk_dump(box, eq, eq["pos"]) == bytes(golden["bulkfile.%s.ElectronsEq" % name])
+    assert _bulk_dump(box, end, end["pos"]) == bytes(golden["bulkfile.%s.ElectronsEnd" % name])
+    for j, part in enumerate(("AvgEnergy", "AvgDriftVelocity", "valleyOccupation")):
+        text = "".join("%g %s\n" % (s * dt, " ".join("%g" % x for x in obs[s, j]))
+                       for s in range(nsteps + 1))
+        assert text.encode() == bytes(golden["bulkfile.%s.%s" % (name, part)]), part

@@ -27,6 +27,8 @@ hipError_t launch_steps(const StepArgs<R> &a, int mode, int ppt, hipStream_t st)
 template <typename R>
 hipError_t launch_init(const StepArgs<R> &a, double init_mean, double init_sigma, hipStream_t st);
 template <typename R>
+hipError_t launch_observe(const StepArgs<R> &a, hipStream_t st);
+template <typename R>
 hipError_t launch_bath_step(const BathStepArgs<R> &a, int n_groups, hipStream_t st);
 template <typename R>
 hipError_t launch_rebuild(const BathStepArgs<R> &a, int n_jobs, hipStream_t st);
@@ -176,6 +178,8 @@ void lower_instance(const emcgpu_ctx *c, const Instance &in, double dt_s, InstDe
   P.inv_dE = (R)(1.0 / in.dE);
   P.dt = (R)(dt_s * kToFs);
   P.vt = (R)(phys::kB / phys::q * in.temperature);
+  for (int d = 0; d < 3; d++)
+    P.box[d] = (R)(in.box[d] * 1e9);
   for (int v = 0; v < P.n_valleys; v++) {
     const emcgpu_valley_desc &vd = in.valleys[v];
     const phys::ValleyMass &m = in.vm[v];
@@ -674,6 +678,11 @@ void fill_common(const emcgpu_ctx *c, StepArgs<R> &a) {
   a.E = (R *)c->E;
   a.tau = (R *)c->tau;
   a.vs = c->vs;
+  if (c->track_pos) {
+    a.px = (R *)c->px;
+    a.py = (R *)c->py;
+    a.pz = (R *)c->pz;
+  }
   a.inst = (const InstDev<R> *)c->d_params;
   a.cum = (const R *)c->d_cum;
   a.unit_inst = c->d_unit_inst;
@@ -776,11 +785,14 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s,
   } else if (!reserve_only) {
     c->bath_hist_steps = 0;
   }
-  // particles per lane of the production kernel (replay: one per lane)
-  const int ppt = replay ? 1
-                  : ppl  ? ppl
-                  : c->tuned_ppl ? c->tuned_ppl
-                                 : default_ppt(c->precision);
+  // particles per lane of the production kernel (replay and tracked
+  // positions: one per lane)
+  if (c->track_pos && ppl > 1)
+    return fail(c, EMCGPU_ERR_ARG, "tracked positions run one particle per lane (ppl %d)", ppl);
+  const int ppt = (replay || c->track_pos) ? 1
+                  : ppl                    ? ppl
+                  : c->tuned_ppl           ? c->tuned_ppl
+                                           : default_ppt(c->precision);
   const int tile_size = kTile * ppt;
   const int n_tiles = (int)(c->capacity / tile_size);
   long long max_count = 0;
@@ -849,7 +861,7 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s,
   double *d_grain = nullptr;
   int *d_over = nullptr;
   int32_t *d_tv = nullptr, *d_ts = nullptr, *d_te = nullptr;
-  double *d_tk = nullptr, *d_tE = nullptr;
+  double *d_tk = nullptr, *d_tE = nullptr, *d_tp = nullptr;
   bool capturing = false;
   auto cleanup = [&]() {
     if (capturing) {  // abandon a partial capture so the stream is usable
@@ -869,6 +881,7 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s,
     dfreet(d_te);
     dfreet(d_tk);
     dfreet(d_tE);
+    dfreet(d_tp);
   };
 #define HIPCHK_CLEAN(x)                                                                  \
   do {                                                                                   \
@@ -902,6 +915,7 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s,
   }
   const bool trace = s->trace_valley && s->trace_sub && s->trace_events && s->trace_k &&
                      s->trace_energy;
+  const bool trace_pos = trace && s->trace_pos && c->track_pos;
   if (trace) {
     const size_t m = n0 * (size_t)s->n_steps + 1;
     HIPCHK_CLEAN(hipMalloc((void **)&d_tv, sizeof(int32_t) * m));
@@ -909,6 +923,8 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s,
     HIPCHK_CLEAN(hipMalloc((void **)&d_te, sizeof(int32_t) * m));
     HIPCHK_CLEAN(hipMalloc((void **)&d_tk, sizeof(double) * 3 * m));
     HIPCHK_CLEAN(hipMalloc((void **)&d_tE, sizeof(double) * m));
+    if (trace_pos)
+      HIPCHK_CLEAN(hipMalloc((void **)&d_tp, sizeof(double) * 3 * m));
     a.tr_n = (long long)n0;
   }
 
@@ -929,6 +945,7 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s,
       a.tr_events = d_te + o;
       a.tr_k = d_tk + 3 * o;
       a.tr_E = d_tE + o;
+      a.tr_pos = trace_pos ? d_tp + 3 * o : nullptr;
     }
     if (s->timing)
       HIPCHK_CLEAN(hipEventRecord(c->ev0, c->stream));
@@ -1008,14 +1025,20 @@ emcgpu_status run_steps(emcgpu_ctx *c, const emcgpu_step_args *s,
     HIPCHK_CLEAN(hipMemcpyAsync(s->trace_events, d_te, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->stream));
     HIPCHK_CLEAN(hipMemcpyAsync(s->trace_k, d_tk, sizeof(double) * 3 * m, hipMemcpyDeviceToHost, c->stream));
     HIPCHK_CLEAN(hipMemcpyAsync(s->trace_energy, d_tE, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+    if (trace_pos)
+      HIPCHK_CLEAN(hipMemcpyAsync(s->trace_pos, d_tp, sizeof(double) * 3 * m,
+                                  hipMemcpyDeviceToHost, c->stream));
   }
   if (replay || trace)
     HIPCHK_CLEAN(hipStreamSynchronize(c->stream));
   if (trace) {
-    // the kernels trace k in 1/nm
+    // the kernels trace k in 1/nm and positions in nm
     const size_t m = 3 * n0 * (size_t)s->n_steps;
     for (size_t i = 0; i < m; i++)
       s->trace_k[i] /= kToNm;
+    if (trace_pos)
+      for (size_t i = 0; i < m; i++)
+        s->trace_pos[i] *= 1e-9;
   }
   cleanup();
 #undef HIPCHK_CLEAN
@@ -1153,6 +1176,86 @@ emcgpu_status get_particles_t(const emcgpu_ctx *c0, int instance, double *k, dou
 }
 }  // namespace
 
+namespace {
+// the position arrays of the arena (zeroed); capacity may be 0
+emcgpu_status alloc_positions(emcgpu_ctx *c) {
+  dfree(c->px);
+  dfree(c->py);
+  dfree(c->pz);
+  const size_t n = (size_t)std::max<long long>(c->capacity, 1), rs = real_size(c);
+  HIPCHK(c, hipMalloc(&c->px, n * rs));
+  HIPCHK(c, hipMalloc(&c->py, n * rs));
+  HIPCHK(c, hipMalloc(&c->pz, n * rs));
+  HIPCHK(c, hipMemsetAsync(c->px, 0, n * rs, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->py, 0, n * rs, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->pz, 0, n * rs, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return EMCGPU_OK;
+}
+
+template <typename R>
+emcgpu_status positions_t(emcgpu_ctx *c, int instance, double *pos, bool get) {
+  const Instance &in = c->inst[instance];
+  const size_t n = in.count, off = (size_t)in.start;
+  if (n == 0)
+    return EMCGPU_OK;
+  std::vector<R> h[3];
+  void *dev[3] = {c->px, c->py, c->pz};
+  for (int d = 0; d < 3; d++) {
+    h[d].resize(n);
+    if (!get)
+      for (size_t i = 0; i < n; i++)
+        h[d][i] = (R)(pos[3 * i + d] * 1e9);
+    if (get)
+      HIPCHK(c, hipMemcpyAsync(h[d].data(), (R *)dev[d] + off, n * sizeof(R),
+                               hipMemcpyDeviceToHost, c->stream));
+    else
+      HIPCHK(c, hipMemcpyAsync((R *)dev[d] + off, h[d].data(), n * sizeof(R),
+                               hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (get)
+    for (size_t i = 0; i < n; i++)
+      for (int d = 0; d < 3; d++)
+        pos[3 * i + d] = (double)h[d][i] * 1e-9;
+  return EMCGPU_OK;
+}
+
+template <typename R>
+emcgpu_status observe_t(emcgpu_ctx *c, double *out) {
+  emcgpu_status st = upload_params<R>(c, c->params_dt > 0 ? c->params_dt : 1e-15);
+  if (st)
+    return st;
+  const int n_tiles = (int)(c->capacity / kTile);
+  if ((st = ensure_partial(c, 1, n_tiles)))
+    return st;
+  long long max_count = 0;
+  for (const auto &in : c->inst)
+    max_count = std::max<long long>(max_count, (long long)in.count);
+  const int n_chunks = reduce_chunks(max_count, kTile);
+  const size_t cb = chunks_per_step(c, n_chunks) * sizeof(double);
+  if (cb > c->chunk_bytes) {
+    dfreet(c->d_chunks);
+    HIPCHK(c, hipMalloc((void **)&c->d_chunks, cb));
+    c->chunk_bytes = cb;
+  }
+  if (!c->d_obs0)
+    HIPCHK(c, hipMalloc((void **)&c->d_obs0, sizeof(double) * c->n_inst * (2 + kMaxV * (3 + kMaxDeg))));
+  StepArgs<R> a;
+  fill_common<R>(c, a);
+  a.n_tiles = n_tiles;
+  a.partial = c->d_partial;
+  HIPCHK(c, launch_observe<R>(a, c->stream));
+  HIPCHK(c, launch_reduce(c->d_partial, 1, n_tiles, c->n_obs, kTile, c->d_inst_start,
+                          c->d_inst_count, c->n_inst, n_chunks, c->d_chunks, c->d_obs0, 0, 1,
+                          c->stream));
+  HIPCHK(c, hipMemcpyAsync(out, c->d_obs0, sizeof(double) * c->n_inst * c->n_obs,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return EMCGPU_OK;
+}
+}  // namespace
+
 // ===========================================================================
 // exported C ABI
 // ===========================================================================
@@ -1228,6 +1331,10 @@ void emcgpu_destroy(emcgpu_ctx *c) {
   dfreet(c->d_partial);
   dfreet(c->d_chunks);
   dfreet(c->d_obs);
+  dfreet(c->d_obs0);
+  dfree(c->px);
+  dfree(c->py);
+  dfree(c->pz);
   dfreet(c->d_baths);
   dfreet(c->d_nq);
   dfreet(c->d_tau_prof);
@@ -1754,6 +1861,11 @@ emcgpu_status emcgpu_resize(emcgpu_ctx *c, const size_t *counts) {
   HIPCHK(c, hipMemsetAsync(c->E, 0, n * rs, c->stream));
   HIPCHK(c, hipMemsetAsync(c->tau, 0, n * rs, c->stream));
   HIPCHK(c, hipMemsetAsync(c->vs, 0, n * sizeof(uint32_t), c->stream));
+  if (c->track_pos) {
+    emcgpu_status st = alloc_positions(c);
+    if (st)
+      return st;
+  }
   std::vector<int> unit_inst((size_t)std::max<long long>(total / kUnit, 1), 0);
   for (int i = 0; i < c->n_inst; i++)
     for (long long u = start[i] / kUnit; u < (start[i] + cnt[i] + kUnit - 1) / kUnit; u++)
@@ -1784,6 +1896,70 @@ emcgpu_status emcgpu_get_count(const emcgpu_ctx *c, int instance, size_t *count)
   return EMCGPU_OK;
 }
 
+
+
+emcgpu_status emcgpu_set_box(emcgpu_ctx *c, int instance, const double max_pos[3]) {
+  emcgpu_status st = check_instance(c, instance);
+  if (st)
+    return st;
+  if (!max_pos || !(max_pos[0] > 0) || !(max_pos[1] > 0) || !(max_pos[2] > 0))
+    return fail(c, EMCGPU_ERR_ARG, "the box needs three positive extents");
+  if (c->dsim)
+    return fail(c, EMCGPU_ERR_STATE, "device simulations keep their own positions");
+  for (int d = 0; d < 3; d++)
+    c->inst[instance].box[d] = max_pos[d];
+  c->params_dirty = true;
+  if (c->device >= 0 && !c->track_pos) {
+    (void)hipSetDevice(c->device);
+    c->track_pos = true;
+    c->tuned_ppl = c->tuned_spl = 0;
+    if ((st = alloc_positions(c)))
+      return st;
+  }
+  c->track_pos = true;
+  return EMCGPU_OK;
+}
+
+emcgpu_status emcgpu_set_positions(emcgpu_ctx *c, int instance, const double *pos) {
+  emcgpu_status st = check_instance(c, instance);
+  if (st)
+    return st;
+  if (!pos)
+    return fail(c, EMCGPU_ERR_ARG, "pos must be non-NULL");
+  if (c->device < 0 || !c->track_pos)
+    return fail(c, EMCGPU_ERR_STATE, "positions are not tracked (emcgpu_set_box first)");
+  (void)hipSetDevice(c->device);
+  return c->precision == EMCGPU_FP32 ? positions_t<float>(c, instance, const_cast<double *>(pos), false)
+                                     : positions_t<double>(c, instance, const_cast<double *>(pos), false);
+}
+
+emcgpu_status emcgpu_get_positions(const emcgpu_ctx *c0, int instance, double *pos) {
+  emcgpu_ctx *c = const_cast<emcgpu_ctx *>(c0);
+  emcgpu_status st = check_instance(c, instance);
+  if (st)
+    return st;
+  if (!pos)
+    return fail(c, EMCGPU_ERR_ARG, "pos must be non-NULL");
+  if (c->device < 0 || !c->track_pos)
+    return fail(c, EMCGPU_ERR_STATE, "positions are not tracked (emcgpu_set_box first)");
+  (void)hipSetDevice(c->device);
+  return c->precision == EMCGPU_FP32 ? positions_t<float>(c, instance, pos, true)
+                                     : positions_t<double>(c, instance, pos, true);
+}
+
+emcgpu_status emcgpu_observe(emcgpu_ctx *c, double *out) {
+  if (!c || !out)
+    return fail(c, EMCGPU_ERR_ARG, "ctx and out must be non-NULL");
+  if (c->device < 0)
+    return fail(c, EMCGPU_ERR_STATE, "host-only context has no particles");
+  if (c->capacity == 0)
+    return fail(c, EMCGPU_ERR_STATE, "no particles (call emcgpu_resize first)");
+  for (int i = 0; i < c->n_inst; i++)
+    if (c->inst[i].count && !c->inst[i].has_tables)
+      return fail(c, EMCGPU_ERR_STATE, "instance %d has particles but no scatter tables", i);
+  (void)hipSetDevice(c->device);
+  return c->precision == EMCGPU_FP32 ? observe_t<float>(c, out) : observe_t<double>(c, out);
+}
 
 emcgpu_status emcgpu_set_particles(emcgpu_ctx *c, int instance, const double *k, const double *E,
                                    const double *tau, const int32_t *valley, const int32_t *sub) {
@@ -2192,6 +2368,9 @@ emcgpu_status emcgpu_autotune(emcgpu_ctx *c, emcgpu_autotune_args *t) {
     return fail(c, EMCGPU_ERR_STATE, "host-only context cannot tune");
   if (c->n_baths > 0)
     return fail(c, EMCGPU_ERR_STATE, "autotune is not available with phonon baths");
+  if (c->track_pos)
+    return fail(c, EMCGPU_ERR_STATE, "autotune is not available with tracked positions "
+                                     "(one particle per lane)");
   // plasma screening updates q_s^2 and the screened tables after every
   // step; the trials would leave them at the state of the last trial step
   for (int i = 0; i < c->n_inst; i++)

@@ -95,6 +95,9 @@ struct InstDev {
   // n_baths) values
   const R *bath_win[kMaxBaths];
   int win_lds;  // a q-resolved angle reads them: the step kernel stages them in LDS
+  // bulk runs with positions: the periodic box [0, box] per dimension [nm]
+  // (the device's maxPos, basicBulkParticleHandler.hpp:600-613)
+  R box[3];
 };
 
 // constants of one LO phonon bath (emcPhononBath<double>)
@@ -184,6 +187,7 @@ template <typename R>
 struct StepArgs {
   R *kx, *ky, *kz, *E, *tau;
   uint32_t *vs;  // valley << 16 | subvalley
+  R *px, *py, *pz;  // bulk positions [nm], or null (not tracked)
   const InstDev<R> *inst;
   const R *cum;
   const int *unit_inst;  // instance of each kUnit-particle arena unit
@@ -208,6 +212,7 @@ struct StepArgs {
   // trace (instance 0)
   int32_t *tr_valley, *tr_sub, *tr_events;
   double *tr_k, *tr_E;
+  double *tr_pos;  // positions [nm] of the trace, or null
   long long tr_n;
   // phonon-bath event tallies [kTallyCopies][global bin][2] (emission,
   // absorption), tally_bins bins per copy

@@ -1636,6 +1636,8 @@ extern "C" {
 emcgpu_status emcgpu_device_setup(emcgpu_ctx *c, const emcgpu_device_desc *desc) {
   if (!c || !desc)
     return c ? fail(c, EMCGPU_ERR_ARG, "desc must be non-NULL") : EMCGPU_ERR_ARG;
+  if (c->track_pos)
+    return fail(c, EMCGPU_ERR_STATE, "a bulk context with a periodic box cannot hold a device");
   DeviceSim *d = new DeviceSim();
   emcgpu_status st = build_geometry(c, *d, *desc);
   if (st == EMCGPU_OK) {

@@ -67,6 +67,8 @@ struct Instance {
   double qs2 = 0;
   bool dev_scr = false;  // the device holds the current q_s^2 (plasma mode)
   bool plasma() const { return scr.mode == EMCGPU_SCREENING_PLASMA && scr.enabled; }
+  // periodic bulk box [m] of tracked positions (0: none)
+  double box[3] = {0, 0, 0};
   std::vector<long long> cum_off;  // per valley: element offset of the device tables
 };
 
@@ -84,6 +86,10 @@ struct emcgpu_ctx {
   long long capacity = 0;  // padded particles (multiple of kUnit)
   void *kx = nullptr, *ky = nullptr, *kz = nullptr, *E = nullptr, *tau = nullptr;
   uint32_t *vs = nullptr;
+  // positions of the bulk particles [nm] (allocated once a box is set)
+  bool track_pos = false;
+  void *px = nullptr, *py = nullptr, *pz = nullptr;
+  double *d_obs0 = nullptr;  // ensemble sums of the current state [instance][value]
   int *d_unit_inst = nullptr;
   long long *d_inst_start = nullptr, *d_inst_count = nullptr;
   // parameters and tables

@@ -174,13 +174,29 @@ __device__ __forceinline__ void tally_flush(const InstDev<R> *P, const uint32_t 
 // busy in the body instead of ~20 % (one particle per lane at ~0.24 events
 // per step).  No barriers are involved in the event loop.
 // ---------------------------------------------------------------------------
-template <typename R, int PPT, bool HOT>
+// the block's instance parameters copied into LDS: every per-lane indexed
+// read of the step (subvalley accelerations, table offsets, mechanism
+// descriptors, velocity weights) then takes a 32-bit LDS address instead of
+// a 64-bit global address computed on the vector ALU
+template <typename R>
+__device__ __forceinline__ void stage_params(InstDev<R> &dst, const InstDev<R> *src) {
+  static_assert(sizeof(InstDev<R>) % 8 == 0, "8-byte copy");
+  const unsigned long long *s = reinterpret_cast<const unsigned long long *>(src);
+  unsigned long long *d = reinterpret_cast<unsigned long long *>(&dst);
+  for (int i = threadIdx.x; i < (int)(sizeof(InstDev<R>) / 8); i += kTile)
+    d[i] = s[i];
+  __syncthreads();
+}
+
+template <typename R, int PPT, bool HOT, bool POS>
 __global__ __launch_bounds__(kTile) void k_bulk_multi(StepArgs<R> a) {
   extern __shared__ __attribute__((aligned(16))) uint32_t lds_tally[];
+  __shared__ InstDev<R> lds_params;
   constexpr int TS = kTile * PPT;
   const int tile = blockIdx.x;
   const int inst = tile_instance(a.unit_inst, tile, TS);
-  const InstDev<R> *P = a.inst + inst;
+  stage_params(lds_params, a.inst + inst);
+  const InstDev<R> *P = &lds_params;
   if constexpr (HOT)
     tally_clear(P, lds_tally);
   const long long base = a.inst_start[inst];
@@ -188,6 +204,7 @@ __global__ __launch_bounds__(kTile) void k_bulk_multi(StepArgs<R> a) {
   const long long first = (long long)tile * TS - base + threadIdx.x;  // local index, j = 0
 
   R kx[PPT], ky[PPT], kz[PPT], E[PPT], tau[PPT], rem[PPT];
+  R pos[PPT][3];  // positions (POS only; folded away otherwise)
   int v[PPT], sub[PPT];
   uint32_t ev[PPT];
   bool valid[PPT];
@@ -196,6 +213,7 @@ __global__ __launch_bounds__(kTile) void k_bulk_multi(StepArgs<R> a) {
     const long long local = first + (long long)j * kTile;
     valid[j] = local < count;
     kx[j] = ky[j] = kz[j] = E[j] = tau[j] = rem[j] = R(0);
+    pos[j][0] = pos[j][1] = pos[j][2] = R(0);
     v[j] = sub[j] = 0;
     ev[j] = 0;
     if (valid[j]) {
@@ -208,6 +226,11 @@ __global__ __launch_bounds__(kTile) void k_bulk_multi(StepArgs<R> a) {
       const uint32_t vs = a.vs[gi];
       v[j] = (int)(vs >> 16);
       sub[j] = (int)(vs & 0xffffu);
+      if constexpr (POS) {
+        pos[j][0] = a.px[gi];
+        pos[j][1] = a.py[gi];
+        pos[j][2] = a.pz[gi];
+      }
     }
   }
   StepRng<R, 0> rng;
@@ -223,7 +246,8 @@ __global__ __launch_bounds__(kTile) void k_bulk_multi(StepArgs<R> a) {
       ev[j] = 0;
       rem[j] = R(0);
       if (valid[j]) {
-        drift(P, (dt < tau[j]) ? dt : tau[j], kx[j], ky[j], kz[j], E[j], v[j], sub[j]);
+        drift<R, POS>(P, (dt < tau[j]) ? dt : tau[j], kx[j], ky[j], kz[j], E[j], v[j], sub[j],
+                      pos[j]);
         rem[j] = dt - tau[j];  // std::min(tau, dt) above, then tRemaining
       }
     }
@@ -238,6 +262,7 @@ __global__ __launch_bounds__(kTile) void k_bulk_multi(StepArgs<R> a) {
       if (sel < 0)
         break;
       R x = kx[0], y = ky[0], z = kz[0], en = E[0], ta = tau[0], rm = rem[0];
+      R wp[3] = {pos[0][0], pos[0][1], pos[0][2]};
       int vv = v[0], ss = sub[0];
       uint32_t ee = ev[0];
 #pragma unroll
@@ -252,9 +277,15 @@ __global__ __launch_bounds__(kTile) void k_bulk_multi(StepArgs<R> a) {
           vv = v[j];
           ss = sub[j];
           ee = ev[j];
+          if constexpr (POS) {
+            wp[0] = pos[j][0];
+            wp[1] = pos[j][1];
+            wp[2] = pos[j][2];
+          }
         }
       rng.pid = (uint32_t)(first + (long long)sel * kTile);
-      event_once<R, 0, HOT, HOT>(P, a.cum, x, y, z, en, ta, rm, vv, ss, ee, rng, nreal, lds_tally);
+      event_once<R, 0, HOT, HOT, POS>(P, a.cum, x, y, z, en, ta, rm, vv, ss, ee, rng, nreal,
+                                      lds_tally, wp);
 #pragma unroll
       for (int j = 0; j < PPT; ++j)
         if (sel == j) {
@@ -267,6 +298,11 @@ __global__ __launch_bounds__(kTile) void k_bulk_multi(StepArgs<R> a) {
           v[j] = vv;
           sub[j] = ss;
           ev[j] = ee;
+          if constexpr (POS) {
+            pos[j][0] = wp[0];
+            pos[j][1] = wp[1];
+            pos[j][2] = wp[2];
+          }
         }
     }
     int nev = 0;
@@ -291,6 +327,11 @@ __global__ __launch_bounds__(kTile) void k_bulk_multi(StepArgs<R> a) {
       a.E[gi] = E[j];
       a.tau[gi] = tau[j];
       a.vs[gi] = ((uint32_t)v[j] << 16) | (uint32_t)sub[j];
+      if constexpr (POS) {
+        a.px[gi] = pos[j][0];
+        a.py[gi] = pos[j][1];
+        a.pz[gi] = pos[j][2];
+      }
     }
   }
 }
@@ -299,7 +340,7 @@ __global__ __launch_bounds__(kTile) void k_bulk_multi(StepArgs<R> a) {
 // replay step kernel (FP64): one particle per lane, the reference engine's
 // recorded draws consumed in the reference's order, optional per-step trace
 // ---------------------------------------------------------------------------
-template <typename R, bool HOT>
+template <typename R, bool HOT, bool POS>
 __global__ __launch_bounds__(kTile) void k_bulk_replay(StepArgs<R> a) {
   extern __shared__ __attribute__((aligned(16))) uint32_t lds_tally[];
   const int tile = blockIdx.x;
@@ -313,6 +354,7 @@ __global__ __launch_bounds__(kTile) void k_bulk_replay(StepArgs<R> a) {
   const long long gi = base + local;
 
   R kx[1] = {0}, ky[1] = {0}, kz[1] = {0}, E[1] = {0}, tau = 0;
+  R pos[3] = {0, 0, 0};
   int v[1] = {0}, sub[1] = {0};
   if (valid[0]) {
     kx[0] = a.kx[gi];
@@ -323,6 +365,11 @@ __global__ __launch_bounds__(kTile) void k_bulk_replay(StepArgs<R> a) {
     const uint32_t vs = a.vs[gi];
     v[0] = (int)(vs >> 16);
     sub[0] = (int)(vs & 0xffffu);
+    if constexpr (POS) {
+      pos[0] = a.px[gi];
+      pos[1] = a.py[gi];
+      pos[2] = a.pz[gi];
+    }
   }
   StepRng<R, 1> rng;
   rng.rs.p = a.draws + (valid[0] ? local : 0) * a.draw_stride;
@@ -334,8 +381,8 @@ __global__ __launch_bounds__(kTile) void k_bulk_replay(StepArgs<R> a) {
   for (int s = 0; s < a.n_steps; ++s) {
     int nev = 0, nreal = 0;
     if (valid[0]) {
-      move_one<R, 1, HOT, HOT>(P, a.cum, kx[0], ky[0], kz[0], E[0], tau, v[0], sub[0], rng, nev,
-                          nreal, lds_tally);
+      move_one<R, 1, HOT, HOT, POS>(P, a.cum, kx[0], ky[0], kz[0], E[0], tau, v[0], sub[0], rng,
+                                    nev, nreal, lds_tally, pos);
       // grain bookkeeping (basicBulkParticleHandler.hpp:213-220), no grain
       // mechanism: a new grainTau of -ln U[1e-6,1) * 1 s
       grain -= a.dt_s;
@@ -350,6 +397,11 @@ __global__ __launch_bounds__(kTile) void k_bulk_replay(StepArgs<R> a) {
         a.tr_k[3 * o + 1] = (double)ky[0];
         a.tr_k[3 * o + 2] = (double)kz[0];
         a.tr_E[o] = (double)E[0];
+        if (a.tr_pos) {
+          a.tr_pos[3 * o] = (double)pos[0];
+          a.tr_pos[3 * o + 1] = (double)pos[1];
+          a.tr_pos[3 * o + 2] = (double)pos[2];
+        }
       }
     }
     tile_observables<R, 1>(a, P, tile, s, valid, v, sub, kx, ky, kz, E, nev, nreal);
@@ -364,12 +416,43 @@ __global__ __launch_bounds__(kTile) void k_bulk_replay(StepArgs<R> a) {
     a.E[gi] = E[0];
     a.tau[gi] = tau;
     a.vs[gi] = ((uint32_t)v[0] << 16) | (uint32_t)sub[0];
+    if constexpr (POS) {
+      a.px[gi] = pos[0];
+      a.py[gi] = pos[1];
+      a.pz[gi] = pos[2];
+    }
     a.cursor[local] = (uint64_t)rng.rs.pos;
     if (a.grain_tau)
       a.grain_tau[local] = grain;
     if (rng.rs.over)
       atomicAdd(a.overflow, 1);
   }
+}
+
+// the ensemble sums of the current state, without a step: one particle per
+// lane, the partials of "step 0" (the getAvg* calls right after
+// generateInitialParticles, bulkSimulation.cpp:143-146)
+template <typename R>
+__global__ __launch_bounds__(kTile) void k_observe(StepArgs<R> a) {
+  const int tile = blockIdx.x;
+  const int inst = tile_instance(a.unit_inst, tile, kTile);
+  const InstDev<R> *P = a.inst + inst;
+  const long long base = a.inst_start[inst];
+  const long long local = (long long)tile * kTile + threadIdx.x - base;
+  bool valid[1] = {local < a.inst_count[inst]};
+  const long long gi = base + local;
+  R kx[1] = {0}, ky[1] = {0}, kz[1] = {0}, E[1] = {0};
+  int v[1] = {0}, sub[1] = {0};
+  if (valid[0]) {
+    kx[0] = a.kx[gi];
+    ky[0] = a.ky[gi];
+    kz[0] = a.kz[gi];
+    E[0] = a.E[gi];
+    const uint32_t vs = a.vs[gi];
+    v[0] = (int)(vs >> 16);
+    sub[0] = (int)(vs & 0xffffu);
+  }
+  tile_observables<R, 1>(a, P, tile, 0, valid, v, sub, kx, ky, kz, E, 0, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1113,6 +1196,14 @@ __global__ __launch_bounds__(kTile) void k_init_thermal(StepArgs<R> a, double in
   a.E[gi] = E;
   a.tau[gi] = -Math<R>::log(ut) * P->tau0[v];
   a.vs[gi] = ((uint32_t)v << 16) | (uint32_t)sub;
+  if (a.px) {
+    // positions uniform in the periodic box: the density of the bulk
+    // handler's initParticlePos per node (emcParticleInitialization.hpp:
+    // 13-29) over nodes weighted by their cells
+    a.px[gi] = stream_uniform<R>(8, st, pid, in, a.key0, a.key1) * P->box[0];
+    a.py[gi] = stream_uniform<R>(9, st, pid, in, a.key0, a.key1) * P->box[1];
+    a.pz[gi] = stream_uniform<R>(10, st, pid, in, a.key0, a.key1) * P->box[2];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1128,21 +1219,31 @@ hipError_t launch_steps_t(const StepArgs<R> &a, int mode, int ppt, hipStream_t s
   const size_t lds = HOT ? sizeof(uint32_t) * 2 * (size_t)a.lds_bins + 8 +
                                sizeof(R) * (size_t)a.lds_win
                          : 0;
+  if (a.px) {
+    // positions tracked: one particle per lane
+    if (mode == 1)
+      hipLaunchKernelGGL((k_bulk_replay<R, HOT, true>), g, b, lds, st, a);
+    else if (ppt == 1)
+      hipLaunchKernelGGL((k_bulk_multi<R, 1, HOT, true>), g, b, lds, st, a);
+    else
+      return hipErrorInvalidValue;
+    return hipGetLastError();
+  }
   if (mode == 1) {
-    hipLaunchKernelGGL((k_bulk_replay<R, HOT>), g, b, lds, st, a);
+    hipLaunchKernelGGL((k_bulk_replay<R, HOT, false>), g, b, lds, st, a);
   } else {
     switch (ppt) {
     case 1:
-      hipLaunchKernelGGL((k_bulk_multi<R, 1, HOT>), g, b, lds, st, a);
+      hipLaunchKernelGGL((k_bulk_multi<R, 1, HOT, false>), g, b, lds, st, a);
       break;
     case 2:
-      hipLaunchKernelGGL((k_bulk_multi<R, 2, HOT>), g, b, lds, st, a);
+      hipLaunchKernelGGL((k_bulk_multi<R, 2, HOT, false>), g, b, lds, st, a);
       break;
     case 4:
-      hipLaunchKernelGGL((k_bulk_multi<R, 4, HOT>), g, b, lds, st, a);
+      hipLaunchKernelGGL((k_bulk_multi<R, 4, HOT, false>), g, b, lds, st, a);
       break;
     case 8:
-      hipLaunchKernelGGL((k_bulk_multi<R, 8, HOT>), g, b, lds, st, a);
+      hipLaunchKernelGGL((k_bulk_multi<R, 8, HOT, false>), g, b, lds, st, a);
       break;
     default:
       return hipErrorInvalidValue;
@@ -1159,6 +1260,14 @@ hipError_t launch_steps(const StepArgs<R> &a, int mode, int ppt, hipStream_t st)
   if ((a.tally && a.lds_bins > 0) || a.ext_kinds)
     return launch_steps_t<R, true>(a, mode, ppt, st);
   return launch_steps_t<R, false>(a, mode, ppt, st);
+}
+
+template <typename R>
+hipError_t launch_observe(const StepArgs<R> &a, hipStream_t st) {
+  if (a.n_tiles <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL((k_observe<R>), dim3(a.n_tiles), dim3(kTile), 0, st, a);
+  return hipGetLastError();
 }
 
 template <typename R>
@@ -1213,6 +1322,8 @@ hipError_t launch_reduce(const double *partial, int n_steps, int n_tiles, int n_
 
 template hipError_t launch_steps<float>(const StepArgs<float> &, int, int, hipStream_t);
 template hipError_t launch_steps<double>(const StepArgs<double> &, int, int, hipStream_t);
+template hipError_t launch_observe<float>(const StepArgs<float> &, hipStream_t);
+template hipError_t launch_observe<double>(const StepArgs<double> &, hipStream_t);
 template hipError_t launch_init<float>(const StepArgs<float> &, double, double, hipStream_t);
 template hipError_t launch_init<double>(const StepArgs<double> &, double, double, hipStream_t);
 template hipError_t launch_bath_step<float>(const BathStepArgs<float> &, int, hipStream_t);

@@ -412,15 +412,34 @@ struct StepRng<R, 1> {  // replay
 
 // free flight of duration t in the current valley/subvalley
 // (emcParticleDrift.hpp:11-35: k += F t vogt / hbar in the ellipse frame,
-// precomputed here as one device-frame acceleration per subvalley)
-template <typename R>
+// precomputed here as one device-frame acceleration per subvalley).  With
+// POS the position moves too (the leapfrog step of :26-34, in the device
+// frame: dx = hbar M (k_old + k_new) / 2 t / m_c(E_new), M = R^T diag(vogt)
+// R) and wraps into the periodic box like the bulk handler's driftParticle
+// (basicBulkParticleHandler.hpp:600-613); pos[3] in nm.
+template <typename R, bool POS = false>
 __device__ __forceinline__ void drift(const InstDev<R> *P, R t, R &kx, R &ky, R &kz, R &E, int v,
-                                      int sub) {
+                                      int sub, R *pos = nullptr) {
   const ValleyDev<R> &V = P->valley[v];
+  const R kx0 = kx, ky0 = ky, kz0 = kz;
   kx += V.accel[sub][0] * t;
   ky += V.accel[sub][1] * t;
   kz += V.accel[sub][2] * t;
   E = energy_of(V, kx, ky, kz);
+  if constexpr (POS) {
+    const R *M = V.hv[sub];
+    const R mx = (kx + kx0) * R(0.5), my = (ky + ky0) * R(0.5), mz = (kz + kz0) * R(0.5);
+    // cv [m/s per 1/nm] * t [fs] * 1e-6 -> nm
+    const R f = Math<R>::div(V.cv * t * R(1e-6), R(1) + R(2) * V.alpha * E);
+    R x[3] = {pos[0] + (M[0] * mx + M[1] * my + M[2] * mz) * f,
+              pos[1] + (M[1] * mx + M[3] * my + M[4] * mz) * f,
+              pos[2] + (M[2] * mx + M[4] * my + M[5] * mz) * f};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const R b = P->box[d];
+      pos[d] = (x[d] < R(0)) ? x[d] + b : ((x[d] > b) ? x[d] - b : x[d]);
+    }
+  }
 }
 
 // one scatter event: selection against the tables and the final state
@@ -509,33 +528,35 @@ __device__ __forceinline__ R scatter_event(const InstDev<R> *P, const R *__restr
 // one pass of the scatter loop of moveParticles: scatter (selection +
 // final state), draw tau', drift min(rem, tau'), rem -= tau'
 // (basicBulkParticleHandler.hpp:203-212)
-template <typename R, int MODE, bool HOT = false, bool SCR = true>
+template <typename R, int MODE, bool HOT = false, bool SCR = true, bool POS = false>
 __device__ __forceinline__ void event_once(const InstDev<R> *P, const R *__restrict__ cum, R &kx,
                                            R &ky, R &kz, R &E, R &tau, R &rem, int &v, int &sub,
                                            uint32_t &e, StepRng<R, MODE> &rng, int &nreal,
-                                           uint32_t *lds_tally = nullptr) {
+                                           uint32_t *lds_tally = nullptr, R *pos = nullptr) {
   const R newTau =
       scatter_event<R, MODE, HOT, SCR>(P, cum, kx, ky, kz, E, v, sub, e, rng, nreal, lds_tally);
   tau += newTau;
-  drift(P, (newTau < rem) ? newTau : rem, kx, ky, kz, E, v, sub);  // std::min(rem, newTau)
+  // std::min(rem, newTau)
+  drift<R, POS>(P, (newTau < rem) ? newTau : rem, kx, ky, kz, E, v, sub, pos);
   rem -= newTau;
 }
 
 // one particle, one step: drift min(tau, dt), then the scatter loop
 // (basicBulkParticleHandler.hpp:195-214)
-template <typename R, int MODE, bool HOT = false, bool SCR = true>
+template <typename R, int MODE, bool HOT = false, bool SCR = true, bool POS = false>
 __device__ __forceinline__ void move_one(const InstDev<R> *P, const R *__restrict__ cum, R &kx,
                                          R &ky, R &kz, R &E, R &tau, int &v, int &sub,
                                          StepRng<R, MODE> &rng, int &nev, int &nreal,
-                                         uint32_t *lds_tally = nullptr) {
+                                         uint32_t *lds_tally = nullptr, R *pos = nullptr) {
   const R dt = P->dt;
-  drift(P, (dt < tau) ? dt : tau, kx, ky, kz, E, v, sub);  // std::min(tau, dt)
+  drift<R, POS>(P, (dt < tau) ? dt : tau, kx, ky, kz, E, v, sub, pos);  // std::min(tau, dt)
   R rem = dt - tau;
   uint32_t e = 0;
   // every pass draws tau' > 0 (the uniform is clamped below 1), so rem
   // strictly decreases; the event cap bounds the loop regardless
   while (rem > R(0) && e < kMaxEventsPerStep)
-    event_once<R, MODE, HOT, SCR>(P, cum, kx, ky, kz, E, tau, rem, v, sub, e, rng, nreal, lds_tally);
+    event_once<R, MODE, HOT, SCR, POS>(P, cum, kx, ky, kz, E, tau, rem, v, sub, e, rng, nreal,
+                                       lds_tally, pos);
   nev += (int)e;
   tau -= dt;
 }

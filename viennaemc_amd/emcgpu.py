@@ -86,7 +86,7 @@ class StepArgs(C.Structure):
                 ("cursor", C.c_void_p), ("grain_tau", C.c_void_p),
                 ("trace_valley", C.c_void_p), ("trace_sub", C.c_void_p),
                 ("trace_events", C.c_void_p), ("trace_k", C.c_void_p),
-                ("trace_energy", C.c_void_p)]
+                ("trace_energy", C.c_void_p), ("trace_pos", C.c_void_p)]
 
 
 class DopingRegion(C.Structure):
@@ -151,7 +151,8 @@ EXPORTS = [
     "emcgpu_device_drift", "emcgpu_device_init", "emcgpu_device_step", "emcgpu_device_fields",
     "emcgpu_autotune", "emcgpu_device_averages", "emcgpu_set_screening", "emcgpu_get_screening",
     "emcgpu_update_screening", "emcgpu_build_tables_all", "emcgpu_step_noalloc",
-    "emcgpu_reserve", "emcgpu_get_bath_energy",
+    "emcgpu_reserve", "emcgpu_get_bath_energy", "emcgpu_set_box", "emcgpu_set_positions",
+    "emcgpu_get_positions", "emcgpu_observe",
 ]
 
 
@@ -182,6 +183,10 @@ def load(path=LIB_PATH):
         "emcgpu_step": (i, [vp, p(StepArgs)]),
         "emcgpu_step_noalloc": (i, [vp, p(StepArgs), vp, C.c_size_t]),
         "emcgpu_reserve": (i, [vp, p(StepArgs)]),
+        "emcgpu_set_box": (i, [vp, i, p(d)]),
+        "emcgpu_set_positions": (i, [vp, i, vp]),
+        "emcgpu_get_positions": (i, [vp, i, vp]),
+        "emcgpu_observe": (i, [vp, vp]),
         "emcgpu_get_observables": (i, [vp, i, vp, p(i), p(i)]),
         "emcgpu_obs_layout": (i, [vp, p(i), p(i), p(i)]),
         "emcgpu_get_timing": (i, [vp, p(d), p(C.c_longlong), i]),
@@ -308,6 +313,7 @@ class Context:
         self.n_instances = n_instances
         self.counts = [0] * n_instances
         self.n_valleys = [0] * n_instances
+        self.tracks_positions = False
 
     def close(self):
         if getattr(self, "h", None):
@@ -519,8 +525,36 @@ class Context:
             a.trace_events = _ptr(tr["events"])
             a.trace_k = _ptr(tr["k"])
             a.trace_energy = _ptr(tr["E"])
+            if self.tracks_positions:
+                tr["pos"] = np.zeros((n_steps, n, 3))
+                a.trace_pos = _ptr(tr["pos"])
         self._chk(self.L.emcgpu_step(self.h, C.byref(a)))
         return tr
+
+    # -- positions of bulk particles ---------------------------------------
+    def set_box(self, instance, max_pos):
+        """emcgpu_set_box: positions tracked in the periodic box [0, max_pos] (m)"""
+        self._chk(self.L.emcgpu_set_box(self.h, instance,
+                                        (C.c_double * 3)(*[float(x) for x in max_pos])))
+        self.tracks_positions = True
+
+    def set_positions(self, instance, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+        if pos.size != 3 * self.counts[instance]:
+            raise ValueError("positions must be [n][3]")
+        self._chk(self.L.emcgpu_set_positions(self.h, instance, _ptr(pos)))
+
+    def get_positions(self, instance=0):
+        out = np.zeros((self.counts[instance], 3))
+        self._chk(self.L.emcgpu_get_positions(self.h, instance, _ptr(out)))
+        return out
+
+    def observe(self):
+        """emcgpu_observe: the ensemble sums of the current state, (n_instances, n_obs)"""
+        n_obs = self.obs_layout()[0]
+        out = np.zeros((self.n_instances, n_obs))
+        self._chk(self.L.emcgpu_observe(self.h, _ptr(out)))
+        return out
 
     def reserve(self, dt, n_steps, steps_per_launch=0, ppl=0):
         """emcgpu_reserve: size the buffers of a later step() call with these

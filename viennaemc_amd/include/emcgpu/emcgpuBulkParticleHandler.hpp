@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "emcgpuMechanisms.hpp"
+#include "emcgpuOutput.hpp"
 
 namespace emcgpu {
 
@@ -64,13 +65,21 @@ public:
     T getVelSound() const { return velSound; }
   };
   emcgpuBulkDevice(Material m, T temperature) : mat(m), temp(temperature) {}
+  /// with the extent of the periodic box [0, maxPos] [m]: the handler then
+  /// keeps a position per particle (emcDevice::getMaxPos of the reference's
+  /// bulk device, bulkSimulation.cpp:64-70)
+  emcgpuBulkDevice(Material m, T temperature, std::array<T, 3> maxPos)
+      : mat(m), temp(temperature), box(maxPos) {}
   const Material &getMaterial() const { return mat; }
   T getTemperature() const { return temp; }
   T getThermalVoltage() const { return phys::kB / phys::q * temp; }
+  std::array<T, 3> getMaxPos() const { return box; }
+  bool hasExtent() const { return box[0] > 0 && box[1] > 0 && box[2] > 0; }
 
 private:
   Material mat;
   T temp;
+  std::array<T, 3> box{0, 0, 0};
 };
 
 template <class T>
@@ -261,6 +270,14 @@ public:
     resetAppliedFieldStrength(fieldStrength);
     size_t n = nrParticles;
     check(emcgpu_resize(ctx, &n), ctx, "emcgpu_resize");
+    if (device.hasExtent()) {
+      // positions drift with the particles and wrap in the box
+      // (basicBulkParticleHandler.hpp:600-613)
+      box = device.getMaxPos();
+      const double b[3] = {box[0], box[1], box[2]};
+      check(emcgpu_set_box(ctx, 0, b), ctx, "emcgpu_set_box");
+    }
+    check(emcgpu_obs_layout(ctx, &nObsL, &maxV, &maxDeg), ctx, "emcgpu_obs_layout");
   }
   ~emcgpuBulkParticleHandler() { emcgpu_destroy(ctx); }
   emcgpuBulkParticleHandler(const emcgpuBulkParticleHandler &) = delete;
@@ -319,9 +336,15 @@ public:
     check(emcgpu_set_field(ctx, 0, d, strength, ptype.getCharge()), ctx, "emcgpu_set_field");
   }
   /// thermal initial ensemble at the device temperature (generateInitialParticles)
+  /// (positions uniform in the box when the device has an extent); the
+  /// getAvg* calls then return the initial ensemble's values
   void generateInitialParticles() {
     check(emcgpu_init_particles(ctx, 0, seed, temperature), ctx, "emcgpu_init_particles");
     step = 0;
+    obs.clear();
+    nObsSteps = 0;
+    lastObs.assign((size_t)nObsL, 0.0);
+    check(emcgpu_observe(ctx, lastObs.data()), ctx, "emcgpu_observe");
   }
   SizeType getNrParticles(SizeType = 0) const { return nPart; }
 
@@ -340,8 +363,8 @@ public:
     check(emcgpu_get_observables(ctx, 0, nullptr, &nObs, &nS), ctx, "emcgpu_get_observables");
     obs.resize((size_t)nObs * nS);
     check(emcgpu_get_observables(ctx, 0, obs.data(), &nObs, &nS), ctx, "emcgpu_get_observables");
+    nObsSteps = nS;
     lastObs.assign(obs.end() - nObs, obs.end());
-    check(emcgpu_obs_layout(ctx, &nObsL, &maxV, &maxDeg), ctx, "emcgpu_obs_layout");
   }
   /// per-step sums of the last moveParticles call: [step][value]
   const std::vector<double> &stepObservables() const { return obs; }
@@ -349,14 +372,18 @@ public:
 
   /// getAvgEnergy / getAvgDriftVelocity / getValleyOccupationProbability
   /// (basicBulkParticleHandler.hpp:289-347) of the state after the last step
-  std::vector<T> getAvgEnergy(SizeType = 0) const { return perValley(1); }
-  std::vector<T> getAvgDriftVelocity(SizeType = 0) const { return perValley(2); }
+  std::vector<T> getAvgEnergy(SizeType = 0) const { return perValley(lastObs.data(), 1); }
+  std::vector<T> getAvgDriftVelocity(SizeType = 0) const { return perValley(lastObs.data(), 2); }
   std::vector<T> getValleyOccupationProbability(SizeType = 0) const {
-    std::vector<T> r(ptype.getNrValleys());
-    for (SizeType v = 0; v < r.size(); v++)
-      r[v] = lastObs[2 + 3 * v] / (T)nPart;
-    return r;
+    return occupation(lastObs.data());
   }
+  /// the same values after step s (0 <= s < nSteps) of the last
+  /// moveParticles(dt, nSteps) call: the per-step records of the bulk
+  /// example (bulkSimulation.cpp:148-162) without a launch per step
+  std::vector<T> getAvgEnergyAt(int s) const { return perValley(row(s), 1); }
+  std::vector<T> getAvgDriftVelocityAt(int s) const { return perValley(row(s), 2); }
+  std::vector<T> getValleyOccupationProbabilityAt(int s) const { return occupation(row(s)); }
+  int stepsOfLastMove() const { return nObsSteps; }
   /// occupation of each (valley, subvalley)
   std::vector<T> getSubValleyOccupationProbability(SizeType valley) const {
     std::vector<T> r(ptype.getValley(valley)->getDegeneracyFactor());
@@ -365,15 +392,45 @@ public:
     return r;
   }
 
+  /// print (basicBulkParticleHandler.hpp:227-248): the file
+  /// <namePrefix><type name><nameSuffix>.txt with the box extent and
+  /// "index x y z kx ky kz energy subvalley valley" per particle; needs a
+  /// device with an extent
+  void print(const std::string &namePrefix, const std::string &nameSuffix) const {
+    if (!(box[0] > 0))
+      reportError("print needs particle positions: construct the device with its extent");
+    const size_t n = nPart;
+    std::vector<double> k(3 * n), E(n), tau(n), pos(3 * n);
+    std::vector<int32_t> valley(n), sub(n);
+    check(emcgpu_get_particles(ctx, 0, k.data(), E.data(), tau.data(), valley.data(), sub.data()),
+          ctx, "emcgpu_get_particles");
+    check(emcgpu_get_positions(ctx, 0, pos.data()), ctx, "emcgpu_get_positions");
+    const double b[3] = {box[0], box[1], box[2]};
+    auto f = output::openResultFile(namePrefix + ptype.getName() + nameSuffix);
+    output::printBulkParticles(f, b, n, pos.data(), k.data(), E.data(), sub.data(),
+                               valley.data());
+  }
+
   emcgpu_ctx *context() { return ctx; }
 
 private:
-  std::vector<T> perValley(int which) const {
+  const double *row(int s) const {
+    if (s < 0 || s >= nObsSteps)
+      reportError("step index beyond the last moveParticles call");
+    return obs.data() + (size_t)s * nObsL;
+  }
+  std::vector<T> perValley(const double *o, int which) const {
     std::vector<T> r(ptype.getNrValleys());
     for (SizeType v = 0; v < r.size(); v++) {
-      const double cnt = lastObs[2 + 3 * v];
-      r[v] = cnt > 0 ? lastObs[2 + 3 * v + which] / cnt : 0;
+      const double cnt = o[2 + 3 * v];
+      r[v] = cnt > 0 ? o[2 + 3 * v + which] / cnt : 0;
     }
+    return r;
+  }
+  std::vector<T> occupation(const double *o) const {
+    std::vector<T> r(ptype.getNrValleys());
+    for (SizeType v = 0; v < r.size(); v++)
+      r[v] = o[2 + 3 * v] / (T)nPart;
     return r;
   }
 
@@ -385,7 +442,8 @@ private:
   SizeType nPart;
   uint32_t step = 0;
   std::vector<double> obs, lastObs;
-  int nObsL = 0, maxV = 1, maxDeg = 1;
+  int nObsL = 0, maxV = 1, maxDeg = 1, nObsSteps = 0;
+  std::array<T, 3> box{0, 0, 0};
   emcgpu_material material{};
   std::vector<std::shared_ptr<emcPhononBath<T>>> baths;
   T plasmaVolume = 0;

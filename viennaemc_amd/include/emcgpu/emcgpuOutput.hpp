@@ -11,7 +11,14 @@
 //   * particle dumps (emcBasicParticleHandler::print,
 //     include/ParticleHandler/emcBasicParticleHandler.hpp:191-215): the
 //     device extent, then "index x y kx ky kz energy subvalley valley tau"
-//     per particle, lines separated (not terminated) by newlines.
+//     per particle, lines separated (not terminated) by newlines;
+//   * bulk particle dumps (basicBulkParticleHandler::print,
+//     examples/bulkSimulation/basicBulkParticleHandler.hpp:227-248): the
+//     box extent (three numbers), then "index x y z kx ky kz energy
+//     subvalley valley" per particle, lines separated by newlines;
+//   * per-step time series of the bulk example
+//     (examples/bulkSimulation/bulkSimulation.cpp:171-183): "time v_0 ..
+//     v_{V-1}" per step, time = step * dt, every line terminated.
 // Numbers use the default std::ostream formatting (6 significant digits),
 // as the reference writes them.
 #ifndef EMCGPU_OUTPUT_HPP
@@ -72,6 +79,34 @@ inline void printParticles(std::ostream &os, const double maxPos[2], std::size_t
        << valley[i] << " " << tau[i];
     if (i + 1 < n)
       os << "\n";
+  }
+}
+
+/// bulk particle dump: pos[n][3] [m], k[n][3] [1/m], energy [eV]
+inline void printBulkParticles(std::ostream &os, const double maxPos[3], std::size_t n,
+                               const double *pos, const double *k, const double *energy,
+                               const int32_t *sub, const int32_t *valley) {
+  os << maxPos[0] << " " << maxPos[1] << " " << maxPos[2] << "\n";
+  for (std::size_t i = 0; i < n; i++) {
+    os << i << " " << pos[3 * i] << " " << pos[3 * i + 1] << " " << pos[3 * i + 2] << " "
+       << k[3 * i] << " " << k[3 * i + 1] << " " << k[3 * i + 2] << " " << energy[i] << " "
+       << sub[i] << " " << valley[i];
+    if (i + 1 < n)
+      os << "\n";
+  }
+}
+
+/// one line per step s of rows: "s * dt rows[s][0] rows[s][1] .."
+template <class V>
+inline void printTimeSeries(std::ostream &os, double dt, const std::vector<std::vector<V>> &rows) {
+  for (std::size_t s = 0; s < rows.size(); s++) {
+    os << (double)s * dt << " ";
+    for (std::size_t j = 0; j < rows[s].size(); j++) {
+      if (j > 0)
+        os << " ";
+      os << rows[s][j];
+    }
+    os << "\n";
   }
 }
 
