@@ -12,12 +12,15 @@ value      = FP32 (fused steps) particle-updates/s with the state resident
              in HBM, buffers sized before the timed call; the FP64 plain
              kernel (one step per launch) is measured beside it.
 roofline   = the FP32 step kernel with the timed call's launch shape: it is
-             bound by vector-ALU issue, so `achieved` is its VALU lane-
-             instruction rate (instructions per particle-update from the
-             committed rocprof counter summary x updates per launch / the
-             launch time measured here with HIP events) against the issue
-             peak; the HBM view (measured traffic and the 48 B per update
-             equivalent) is reported beside it.
+             bound by vector-ALU issue, so `achieved` is the rate of VALU
+             lane-instructions on active lanes (SQ_THREAD_CYCLES_VALU per
+             particle-update from the committed rocprof counter summary of
+             that shape x updates per launch / the launch time measured here
+             with HIP events) against the full-rate VALU lane throughput
+             (one wave64 instruction per SIMD every 4 cycles, 1024 SIMDs,
+             2.4 GHz); the issue rate, the lane utilisation and the HBM view
+             (measured traffic, and the 48 B per update equivalent) are
+             reported beside it.
 cpu_baseline = the oracle's FP64 restatement of the reference loop with the
              reference engine (one mt19937_64 per particle) on the host
              threads this process may use.
@@ -42,10 +45,13 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 BYTES_FP32 = 48  # k(3) + E + tau + valley|sub, 4 B each, read + write
 BYTES_FP64 = 88  # k(3) + E + tau 8 B each + valley|sub 4 B, read + write
-# vector-ALU issue peak: 256 CUs, one wave64 VALU instruction per CU and
-# cycle (4 SIMDs, 4 cycles each), 2.4 GHz peak engine clock -> lane-
-# instructions per second
-VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+# full-rate vector-ALU lane throughput: 1024 SIMDs (256 CUs x 4), one wave64
+# instruction per SIMD every 4 cycles, 2.4 GHz peak engine clock -> lane-
+# instructions per second (the rate of a stream of v_fma_f32; 32-bit integer
+# add/xor streams issue faster, see calib/valu_ceiling.hip)
+N_SIMD = 256 * 4
+CLOCK_HZ = 2.4e9
+VALU_PEAK_TOPS = N_SIMD * 64 * CLOCK_HZ / 4 / 1e12
 DT = 1e-15
 SEED = 20261202
 LEGS = ["headline", "fp64", "fp32_plain", "precision", "c1", "batched", "hot", "hot_screened",
@@ -76,6 +82,8 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (rocprof byte-counter calibration)")
+    ap.add_argument("--valu-calibration", action="store_true",
+                    help="only run the vector-ALU issue-rate streams (viennaemc_amd/calib)")
     a = ap.parse_args()
     a.legs = set(x.strip() for x in a.legs.split(",") if x.strip())
     bad = a.legs - set(LEGS)
@@ -439,7 +447,11 @@ def cpu_baseline(spec, n, steps, threads):
                       % (n, steps, t[0], threads))
     cal = profile_figures(os.path.join(ROOT, "profiles", "cpu_calibration_c1.md"))
     if cal:
+        # the reference's own handler loop against this port, one thread each on
+        # the plain bulk Si run (timed where the reference sources are)
         res["calibration"] = cal
+        if cal.get("reference_over_port"):
+            res["reference_equivalent_value"] = res["value"] * cal["reference_over_port"]
     return res
 
 
@@ -500,6 +512,9 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return 0
+    if args.valu_calibration:
+        import subprocess
+        return subprocess.call([os.path.join(ROOT, "viennaemc_amd", "calib", "valu_ceiling")])
     if args.gpus != 1:
         print("bench.py measures one GPU: the particle step has no multi-GPU path in this library "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
@@ -534,15 +549,23 @@ def main():
         upd = n * K  # particle-updates per launch
         hbm_equiv = BYTES_FP32 * upd / t_launch / 1e9
         roof = {"bound": "valu", "achieved": None, "peak": VALU_PEAK_TOPS,
-                "unit": "TOP/s (VALU lane-instructions)", "frac": None, "traffic": None,
+                "unit": "T lane-instr/s (VALU, active lanes)", "frac": None, "traffic": None,
                 "hbm_equiv": {"achieved": hbm_equiv, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": hbm_equiv / HBM_PEAK_GBS,
                               "note": "48 B per particle-update x updates per launch / launch time"}}
-        ipu = fig.get("valu_lane_instr_per_update")
-        if ipu:
-            roof["achieved"] = ipu * upd / t_launch / 1e12
+        lpu = fig.get("valu_lane_instr_per_update_K%d" % K)
+        wpu = fig.get("valu_wave_instr_per_update_K%d" % K)
+        if lpu:
+            roof["achieved"] = lpu * upd / t_launch / 1e12
             roof["frac"] = roof["achieved"] / VALU_PEAK_TOPS
-            roof["valu_lane_instr_per_update"] = ipu
+            roof["valu_lane_instr_per_update"] = lpu
+        if wpu:
+            # wave64 instructions issued per SIMD and cycle at the peak clock
+            # (0.25 = one full-rate instruction every 4 cycles)
+            roof["valu_wave_instr_per_update"] = wpu
+            roof["issue_per_simd_cycle"] = wpu * upd / t_launch / N_SIMD / CLOCK_HZ
+            if lpu:
+                roof["lane_utilisation"] = lpu / (64 * wpu)
         tb = fig.get("hbm_bytes_per_launch_K%d" % K)
         if tb:
             roof["traffic"] = tb
@@ -556,6 +579,9 @@ def main():
         out.update(fp64_plain_updates_per_s=ups64, fp64_kernel_ms=r64["kernel_ms"],
                    fp64_plain_hbm_frac=(BYTES_FP64 * n / (r64["kernel_ms"] * 1e-3)) / 1e9
                    / HBM_PEAK_GBS)
+        tb = profile_figures().get("hbm_bytes_per_launch_fp64_plain")
+        if tb:
+            out["fp64_plain_hbm_frac_measured"] = tb / (r64["kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
         if res["value"]:
             out["fp32_over_fp64_plain"] = res["value"] / ups64
         r64b = run_gpu(emcgpu, spec, emcgpu.FP64, n, args.warmup, s64, args.spl, ppl=args.ppl)
@@ -566,6 +592,11 @@ def main():
         out["fp32_plain_updates_per_s"] = n * s1 / r1["wall"]
         out["fp32_plain_kernel_ms"] = r1["kernel_ms"]
         out["fp32_plain_hbm_frac"] = BYTES_FP32 * n / (r1["kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+        tb = profile_figures().get("hbm_bytes_per_launch_K1")
+        if tb:
+            # measured traffic (state + partial sums) of the same launch shape
+            out["fp32_plain_hbm_frac_measured"] = (tb / (r1["kernel_ms"] * 1e-3) / 1e9
+                                                   / HBM_PEAK_GBS)
     if "precision" in legs:
         out["precision_fp32_vs_fp64"] = leg_precision(emcgpu, spec, n, args.precision_steps)
     if "c1" in legs:

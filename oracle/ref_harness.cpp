@@ -13,6 +13,7 @@
 // Usage: ref_harness <outfile>      (run with cwd = a scratch folder: the
 // reference's table initialisation writes *ScatterMechanism.txt files).
 
+#include <chrono>
 #include <cstdio>
 #include <sys/stat.h>
 #include <unistd.h>
@@ -1120,9 +1121,47 @@ static void dumpDeviceTrajectories(uint64_t seed, int np, int nsteps, double dt)
   seci((b + "removed_contact").c_str(), contactOut);
 }
 
+// Timing of the real handler's loop on the plain bulk Si run of the bulk
+// example (Si X valleys, Jacoboni phonons, m_t 0.19, 10 kV/cm along [100],
+// 300 K, dt 1 fs, 2000 levels to 2 eV): particles from a constant doping in
+// a periodic cube, then moveParticles and the three per-step observables of
+// bulkSimulation.cpp:148-157, timed with a steady clock (initialisation
+// excluded).  One thread: the harness is built without OpenMP.
+static int timeC1(long long nWanted, int nsteps, uint64_t seed) {
+  const Model m{"si_jacoboni", 2000, 2.0, 300., 10., {1, 0, 0}};
+  const double mp = 1e-7, spacing = mp / 3;
+  Device device{silicon(), {mp, mp, mp}, {spacing, spacing, spacing}, m.T};
+  device.addConstantDopingRegion({0, 0, 0}, {mp, mp, mp}, (double)nWanted / (mp * mp * mp));
+  TypeMap types;
+  types[0] = makeType(m, device, nullptr);
+  Handler handler(device, types, m.fieldDir, m.fieldKVcm * 1e5, seed);
+  handler.generateInitialParticles();
+  const double dt = 1e-15;
+  double sink = 0;
+  auto t0 = std::chrono::steady_clock::now();
+  for (int s = 0; s < nsteps; s++) {
+    handler.moveParticles(dt);
+    sink += handler.getAvgEnergy(0)[0] + handler.getAvgDriftVelocity(0)[0] +
+            handler.getValleyOccupationProbability(0)[0];
+  }
+  const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  const long long n = (long long)handler.getNrParticles(0);
+  std::printf("{\"particles\": %lld, \"steps\": %d, \"seconds\": %.6f, "
+              "\"updates_per_s\": %.6g, \"check\": %.6g}\n",
+              n, nsteps, sec, (double)n * nsteps / sec, sink / nsteps);
+  return 0;
+}
+
 int main(int argc, char **argv) {
+  if (argc >= 2 && std::string(argv[1]) == "--time-c1") {
+    if (argc < 5) {
+      std::fprintf(stderr, "usage: ref_harness --time-c1 <particles> <steps> <seed>\n");
+      return 2;
+    }
+    return timeC1(std::atoll(argv[2]), std::atoi(argv[3]), std::strtoull(argv[4], nullptr, 10));
+  }
   if (argc < 2) {
-    std::fprintf(stderr, "usage: ref_harness <outfile>\n");
+    std::fprintf(stderr, "usage: ref_harness <outfile> | --time-c1 <particles> <steps> <seed>\n");
     return 2;
   }
   out = std::fopen(argv[1], "w");

@@ -59,7 +59,21 @@ def build(verbose=True):
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
     build_examples(verbose)
+    build_calibration(verbose)
     return LIB
+
+
+CALIB = os.path.join(HERE, "calib", "valu_ceiling")
+
+
+def build_calibration(verbose=True):
+    """The vector-ALU issue-rate calibration program (calib/valu_ceiling.hip),
+    run by bench.py --valu-calibration."""
+    cmd = [hipcc()] + [f for f in flags() if f not in ("-fPIC", "-munsafe-fp-atomics")] + [
+        CALIB + ".hip", "-o", CALIB]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
 
 
 EXAMPLES = ["bulkSimulationGpu", "resistor2DGpu", "hotPhononGpu"]
