@@ -553,9 +553,10 @@ typedef struct {
   int use_graph;      /* capture the per-step kernel sequence in a HIP graph */
   /* per-step records (may be NULL): [n_steps][n_contacts] particles that
    * left through / were deleted at / were injected at each contact, the
-   * ensemble size after the step, Newton iterations and relative residual
-   * of the step's Poisson solve, and |sum(nodes) - N| / N of the charge
-   * assignment the step's Poisson solve used (FP64 check sum) */
+   * ensemble size after the step, the work of the step's Poisson solve
+   * (dense Newton corrections applied + multigrid V-cycles) and its relative
+   * residual, and |sum(nodes) - N| / N of the charge assignment the step's
+   * Poisson solve used (FP64 check sum) */
   int32_t *exits, *deleted, *injected;
   int64_t *n_particles;
   int32_t *poisson_iters;
@@ -576,7 +577,21 @@ typedef struct {
    * sums read by emcgpu_device_averages (updateAverageCharacteristics,
    * emcSimulationResults.hpp:85-93) */
   int accumulate;
+  /* how the per-step Poisson solves reach the stopping rule of
+   * emcgpu_poisson (the solution is the same to that tolerance):
+   * EMCGPU_POISSON_AUTO: two Newton corrections from the inverse of the
+   * discrete operator (a dense FP32 matrix of the active nodes, built once
+   * per geometry with rocSOLVER, applied with FP64 sums over all CUs), then
+   * the multigrid-preconditioned Newton solve checks the residual and
+   * finishes the solve if needed.  Used for grids of up to 24576 active
+   * nodes, from the second step on, while the hole term h^2 p dropped from
+   * the inverse stays below 1e-9 of the diagonal; otherwise, and with
+   * EMCGPU_POISSON_MULTIGRID, the multigrid path alone. */
+  int poisson_mode;
 } emcgpu_device_step_args;
+
+#define EMCGPU_POISSON_AUTO 0
+#define EMCGPU_POISSON_MULTIGRID 1
 
 /* performEMCStep repeated n_steps times (emcSimulation.hpp:164-193):
  * Poisson, E-field, driftScatterParticles, handleOhmicContacts, injection,

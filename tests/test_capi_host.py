@@ -55,6 +55,31 @@ def test_struct_layouts_match_header():
     assert emcgpu.StepArgs.draws.offset == 56
 
 
+def test_step_arg_layouts_match_the_c_compiler(tmp_path):
+    """sizeof / offsetof of the step-argument structs from gcc on the header
+    itself, against the ctypes mirrors of the binding"""
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "emcgpu.h"\n'
+        'int main(void) {\n'
+        '  printf("%zu %zu %zu\\n", sizeof(emcgpu_step_args), sizeof(emcgpu_device_step_args),\n'
+        '         offsetof(emcgpu_device_step_args, poisson_mode));\n'
+        '  printf("%zu %zu\\n", offsetof(emcgpu_step_args, trace_pos),\n'
+        '         offsetof(emcgpu_device_step_args, accumulate));\n'
+        '  return 0;\n}\n')
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.check_call(["gcc", "-std=c11", "-I", inc, str(src), "-o", str(exe)])
+    a, b = subprocess.check_output([str(exe)]).decode().split("\n")[:2]
+    s_step, s_dev, o_mode = (int(x) for x in a.split())
+    o_trace, o_acc = (int(x) for x in b.split())
+    assert C.sizeof(emcgpu.StepArgs) == s_step
+    assert emcgpu.StepArgs.trace_pos.offset == o_trace
+    assert C.sizeof(emcgpu.DeviceStepArgs) == s_dev
+    assert emcgpu.DeviceStepArgs.poisson_mode.offset == o_mode
+    assert emcgpu.DeviceStepArgs.accumulate.offset == o_acc
+
+
 def host_ctx(n_instances=1):
     return emcgpu.Context(precision=emcgpu.FP64, n_instances=n_instances, device=-1)
 

@@ -241,6 +241,39 @@ def test_self_consistent_run(precision):
     ctx.close()
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_dense_poisson_corrections_match_multigrid(precision):
+    """The per-step solves with the dense Newton corrections (inverse of the
+    discrete operator, POISSON_AUTO) against the multigrid path alone
+    (POISSON_MULTIGRID) on the 0.5 V resistor from the same seed: both meet
+    the stopping rule at every step, the dense path leaves the multigrid
+    solve (almost) nothing to do, and the potentials and ensembles agree to
+    the solver tolerance."""
+    emcgpu = _emc()
+    dev = resistor_device(n_particles=200_000)
+    print("dense corrections vs multigrid, precision", precision, flush=True)
+    res = {}
+    for mode in (emcgpu.POISSON_MULTIGRID, emcgpu.POISSON_AUTO):
+        ctx = _ctx(dev=dev, precision=precision, tol=1e-10)
+        ctx.device_init(777)
+        out = ctx.device_step(0.1e-15, 40, seed=777, poisson_mode=mode)
+        res[mode] = (out, ctx.device_fields()["pot"], ctx.device_get_particles())
+        ctx.close()
+    mg, auto = res[emcgpu.POISSON_MULTIGRID], res[emcgpu.POISSON_AUTO]
+    for out, _, _ in (mg, auto):
+        assert (out["poisson_resid"] <= 1e-10).all()
+    # the first step sets the contacts' voltages and solves by multigrid;
+    # afterwards the recorded work is dense corrections + V-cycles
+    work_mg = mg[0]["poisson_iters"][1:]
+    work_auto = auto[0]["poisson_iters"][1:]
+    print("work per step: multigrid %.2f V-cycles, dense %.2f (max %d)"
+          % (work_mg.mean(), work_auto.mean(), work_auto.max()), flush=True)
+    assert (work_auto <= 2).mean() >= 0.9 and work_auto.mean() < 0.5 * work_mg.mean()
+    scale = np.abs(mg[1]).max()
+    assert np.abs(auto[1] - mg[1]).max() < 1e-8 * scale
+    assert abs(int(auto[0]["n_particles"][-1]) - int(mg[0]["n_particles"][-1])) <= 2
+
+
 def test_graph_launch_equals_eager():
     dev = resistor_device(n_particles=50_000)
     print("device graph vs eager", flush=True)

@@ -53,7 +53,8 @@ def build(verbose=True):
         subprocess.check_call(cmd)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    cmd = [cc, "--offload-arch=%s" % ARCH, "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [cc, "--offload-arch=%s" % ARCH, "-shared", "-fPIC", "-o", tmp] + objs + [
+        "-lrocsolver", "-lrocblas"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)

@@ -192,6 +192,6 @@ int main() {
                 names[kind], best, instr);
   }
   std::printf("}}\n");
-  hipFree(out);
+  (void)hipFree(out);
   return 0;
 }

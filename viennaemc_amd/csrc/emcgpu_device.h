@@ -351,6 +351,40 @@ struct DevFieldArgs {
   DevHist hist;
   int record;                  // write iters / resid / check sum to the history
   int interval;                // recorded steps: solve only when step % interval == 0
+  // dense Newton corrections before this launch: green_iters of them at
+  // most, their residual checks in green_acc (see GreenArgs); the launch
+  // keeps the solve when the last check met the stopping rule, and clears
+  // the checks
+  int green_iters;
+  const unsigned long long *green_flag;
+  unsigned long long *green_acc;
+};
+
+// dense Newton corrections of a per-step Poisson solve: the inverse of the
+// control-volume weighted operator over the active nodes, G = (H A0)^-1
+// (A0: the discrete operator of the Newton step without the hole term
+// h^2 p, H: the control-volume weights), applied as x = G H F
+constexpr int kGreenThreads = 1024;  // apply kernel: 16 waves, one row per wave at a time
+constexpr int kGreenMaxActive = 24576;
+struct GreenArgs {
+  int nx, ny, nn, ld;          // grid; row stride of G (multiple of 256)
+  double hfx, hfy, hp;
+  double *pot;                 // normalised potential (updated at active nodes)
+  const double *conc, *dopn;   // electron concentration, doping / n_i
+  const float *w0, *hw0;       // active-node mask, control-volume weights
+  const float *green;          // [nn][ld], rows and columns of Dirichlet nodes zero
+  double *g;                   // [nn] H F of the current potential
+  // step + 1 of a step whose hole term is too large for the inverse: its
+  // corrections are skipped (the multigrid path then solves alone)
+  unsigned long long *flag;
+  double shift_max;            // largest h^2 p the inverse may ignore
+  // residual check c (before correction c; c = iters: after the last) as
+  // bit patterns of non-negative doubles, [iters + 1][2]: max |F| and the
+  // scale max h^2 (|C| + n + p) of the stopping rule |F| <= tol * scale
+  unsigned long long *acc;
+  double tol;
+  const DevState *state;
+  int interval;                // solve only when step % interval == 0
 };
 
 // particle step of a device ensemble

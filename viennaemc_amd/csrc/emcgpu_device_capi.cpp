@@ -8,10 +8,13 @@
 // compaction) and drives the per-step kernel sequence of
 // emcgpu_device_kernels.hip in chunks, captured in a HIP graph on request.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "emcgpu_host.h"
@@ -28,6 +31,12 @@ template <typename R>
 hipError_t dev_launch_contacts(const DevContactArgs<R> &a, long long max_inj, hipStream_t st);
 template <typename R>
 hipError_t dev_launch_reduce(const DevContactArgs<R> &a, int commit, hipStream_t st);
+hipError_t dev_launch_green(const GreenArgs &a, int iters, int blocks, hipStream_t st);
+hipError_t dev_launch_green_scatter(const int *rows, const int *cols, const double *vals, int n,
+                                    double *A, long long lda, hipStream_t st);
+hipError_t dev_launch_green_expand(const double *Ainv, long long lda, const int *idx, int nn,
+                                   int ld, float *G, hipStream_t st);
+hipError_t dev_green_lds(size_t lds);
 template <typename R>
 hipError_t dev_launch_init(const DevContactArgs<R> &a, const long long *prefix, long long n,
                            hipStream_t st);
@@ -150,8 +159,17 @@ struct DeviceSim {
   int graph_steps = 0, graph_cur = -1;
   long long graph_cap = 0;
   // kernel arguments the captured graph was recorded with
-  int graph_chan = -1, graph_acc = -1;
+  int graph_chan = -1, graph_acc = -1, graph_green = -1, graph_interval = -1;
   double graph_x0 = 0, graph_x1 = 0, graph_scale = 0;
+  // dense Newton corrections of the per-step solves: G = (H A0)^-1 over the
+  // active nodes as [nn][green_ld] FP32 (0: not built yet, 1: built,
+  // -1: not available for this geometry)
+  int green_state = 0;
+  int green_ld = 0, green_blocks = 0;
+  int green_use = 0;  // the current call applies them
+  float *d_green = nullptr;
+  unsigned long long *d_green_flag = nullptr;
+  unsigned long long *d_green_acc = nullptr;  // residual checks of a step
 };
 
 void dev_free_particles(DeviceSim *d) {
@@ -202,6 +220,9 @@ void device_sim_destroy(DeviceSim *d) {
     dfree(d->d_fargs);
     dfreet(d->d_hw0);
     dfreet(d->d_cinv_t);
+    dfreet(d->d_green);
+    dfreet(d->d_green_flag);
+    dfreet(d->d_green_acc);
     dfreet(d->d_slot_node);
     dfreet(d->d_slot_contact);
     dfreet(d->d_slot_kept);
@@ -1122,6 +1143,153 @@ emcgpu_status run_field(emcgpu_ctx *c, DeviceSim &d, DevFieldArgs<R> a) {
   return EMCGPU_OK;
 }
 
+// dense corrections per step solve (each one shrinks the residual by the
+// FP32 rounding of the inverse, ~1e-5 or better; the multigrid solve after
+// them checks the stopping rule)
+constexpr int kGreenIters = 2;
+// the hole term h^2 p the inverse leaves out, relative to the diagonal
+constexpr double kGreenShiftRel = 1e-9;
+
+// G = (H A0)^-1 of the active nodes for the dense Newton corrections: the
+// operator of the Newton step without the hole term, weighted by the
+// control volumes (symmetric positive definite), assembled on the host,
+// inverted in FP64 with rocSOLVER (Cholesky, potrf + potri), stored FP32
+// as [nn][ld] with zero rows and columns at the Dirichlet nodes.  Any
+// failure leaves the multigrid path alone in charge (green_state -1).
+emcgpu_status green_build(emcgpu_ctx *c, DeviceSim &d) {
+  if (d.green_state != 0)
+    return EMCGPU_OK;
+  d.green_state = -1;
+  const int nx = d.ext[0], ny = d.ext[1], nn = (int)d.nn;
+  std::vector<int> idx(nn, -1);
+  int na = 0;
+  for (int k = 0; k < nn; k++)
+    if (!d.dmask[k])
+      idx[k] = na++;
+  const int ld = (nn + 255) / 256 * 256;
+  if (na == 0 || na > kGreenMaxActive || (size_t)ld * sizeof(float) > 160 * 1024)
+    return EMCGPU_OK;
+  // entries of H A0, row by row (the stencil of k_dev_field: mirrored
+  // neighbours at the boundaries, Dirichlet neighbours dropped)
+  std::map<std::pair<int, int>, double> ent;
+  for (int k = 0; k < nn; k++) {
+    if (idx[k] < 0)
+      continue;
+    const int i = k % nx, j = k / nx;
+    const double hw = (i == 0 || i == nx - 1 ? 0.5 : 1.0) * (j == 0 || j == ny - 1 ? 0.5 : 1.0);
+    ent[{idx[k], idx[k]}] += hw * 2.0 * (d.hfx + d.hfy);
+    const int nb[4] = {i > 0 ? k - 1 : k + 1, i < nx - 1 ? k + 1 : k - 1,
+                       j > 0 ? k - nx : k + nx, j < ny - 1 ? k + nx : k - nx};
+    for (int q = 0; q < 4; q++)
+      if (idx[nb[q]] >= 0)
+        ent[{idx[k], idx[nb[q]]}] -= hw * (q < 2 ? d.hfx : d.hfy);
+  }
+  std::vector<int> rows, cols;
+  std::vector<double> vals;
+  for (const auto &e : ent) {
+    const auto t = ent.find({e.first.second, e.first.first});
+    if (t == ent.end() || std::fabs(t->second - e.second) > 1e-12 * std::fabs(e.second))
+      return EMCGPU_OK;  // not symmetric: no Cholesky
+    rows.push_back(e.first.first);
+    cols.push_back(e.first.second);
+    vals.push_back(e.second);
+  }
+  double *dA = nullptr, *dvals = nullptr;
+  int *drows = nullptr, *dcols = nullptr, *didx = nullptr;
+  rocblas_int *dinfo = nullptr;
+  float *dG = nullptr;
+  rocblas_handle h = nullptr;
+  auto release = [&]() {
+    dfreet(dA);
+    dfreet(dvals);
+    dfreet(drows);
+    dfreet(dcols);
+    dfreet(didx);
+    dfreet(dinfo);
+    dfreet(dG);
+    if (h)
+      (void)rocblas_destroy_handle(h);
+    (void)hipGetLastError();  // an allocation failure here only disables the path
+  };
+  const size_t n2 = (size_t)na * na;
+  bool ok = hipMalloc((void **)&dA, n2 * sizeof(double)) == hipSuccess &&
+            hipMalloc((void **)&dvals, vals.size() * sizeof(double)) == hipSuccess &&
+            hipMalloc((void **)&drows, rows.size() * sizeof(int)) == hipSuccess &&
+            hipMalloc((void **)&dcols, cols.size() * sizeof(int)) == hipSuccess &&
+            hipMalloc((void **)&didx, (size_t)nn * sizeof(int)) == hipSuccess &&
+            hipMalloc((void **)&dinfo, sizeof(rocblas_int)) == hipSuccess &&
+            hipMalloc((void **)&dG, (size_t)nn * ld * sizeof(float)) == hipSuccess;
+  if (!ok) {
+    release();
+    return EMCGPU_OK;
+  }
+  HIPCHK(c, hipMemsetAsync(dA, 0, n2 * sizeof(double), c->stream));
+  HIPCHK(c, hipMemcpy(dvals, vals.data(), vals.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(drows, rows.data(), rows.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(dcols, cols.data(), cols.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(didx, idx.data(), (size_t)nn * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(c, dev_launch_green_scatter(drows, dcols, dvals, (int)vals.size(), dA, na, c->stream));
+  rocblas_int info = 0;
+  ok = rocblas_create_handle(&h) == rocblas_status_success &&
+       rocblas_set_stream(h, c->stream) == rocblas_status_success &&
+       rocsolver_dpotrf(h, rocblas_fill_lower, na, dA, na, dinfo) == rocblas_status_success &&
+       hipMemcpy(&info, dinfo, sizeof info, hipMemcpyDeviceToHost) == hipSuccess && info == 0 &&
+       rocsolver_dpotri(h, rocblas_fill_lower, na, dA, na, dinfo) == rocblas_status_success &&
+       hipMemcpy(&info, dinfo, sizeof info, hipMemcpyDeviceToHost) == hipSuccess && info == 0;
+  if (!ok) {
+    release();
+    return EMCGPU_OK;
+  }
+  HIPCHK(c, dev_launch_green_expand(dA, na, didx, nn, ld, dG, c->stream));
+  HIPCHK(c, dev_green_lds((size_t)ld * sizeof(float)));
+  if (!d.d_green_flag) {
+    HIPCHK(c, hipMalloc((void **)&d.d_green_flag, sizeof(unsigned long long)));
+    HIPCHK(c, hipMemset(d.d_green_flag, 0, sizeof(unsigned long long)));
+  }
+  if (!d.d_green_acc) {
+    const size_t acc_bytes = 2 * (kGreenIters + 1) * sizeof(unsigned long long);
+    HIPCHK(c, hipMalloc((void **)&d.d_green_acc, acc_bytes));
+    HIPCHK(c, hipMemset(d.d_green_acc, 0, acc_bytes));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  d.d_green = dG;
+  dG = nullptr;
+  release();
+  int cus = 0;
+  HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+  const int rows_per_block = kGreenThreads / 64;
+  d.green_blocks = std::max(1, std::min(2 * cus, (nn + rows_per_block - 1) / rows_per_block));
+  d.green_ld = ld;
+  d.green_state = 1;
+  return EMCGPU_OK;
+}
+
+GreenArgs green_args(const DeviceSim &d) {
+  GreenArgs g;
+  std::memset(&g, 0, sizeof g);
+  g.nx = d.ext[0];
+  g.ny = d.ext[1];
+  g.nn = (int)d.nn;
+  g.ld = d.green_ld;
+  g.hfx = d.hfx;
+  g.hfy = d.hfy;
+  g.hp = d.hp;
+  g.pot = d.d_pot;
+  g.conc = d.d_conc;
+  g.dopn = d.d_dopn;
+  g.w0 = d.d_w0;
+  g.hw0 = d.d_hw0;
+  g.green = d.d_green;
+  g.g = d.d_r64;  // scratch: the field kernel after the corrections rewrites it
+  g.flag = d.d_green_flag;
+  g.shift_max = kGreenShiftRel * 2.0 * (d.hfx + d.hfy);
+  g.acc = d.d_green_acc;
+  g.tol = d.tol;
+  g.state = d.d_state;
+  g.interval = d.interval;
+  return g;
+}
+
 // arguments of the per-step Poisson / E-field launch (slot 1: the first
 // step, which sets the contacts' applied voltages)
 template <typename R>
@@ -1134,12 +1302,19 @@ DevFieldArgs<R> step_field_args(const emcgpu_ctx *c, const DeviceSim &d, bool re
   f.record = 1;
   f.nrpart = d.d_nrpart;
   f.interval = d.interval;
+  if (d.green_use && !reset_bc) {
+    f.green_iters = kGreenIters;
+    f.green_flag = d.d_green_flag;
+    f.green_acc = d.d_green_acc;
+  }
   return f;
 }
 
 // the kernel sequence of one step (emcSimulation.hpp:164-193)
 template <typename R>
 emcgpu_status launch_step(emcgpu_ctx *c, DeviceSim &d, bool reset_bc) {
+  if (d.green_use && !reset_bc)
+    HIPCHK(c, dev_launch_green(green_args(d), kGreenIters, d.green_blocks, c->stream));
   HIPCHK(c, dev_launch_field<R>(fargs_slot<R>(d, reset_bc ? 1 : 0), d.mg_lds, c->stream));
   HIPCHK(c, dev_launch_step<R>(step_args<R>(c, d), d.blocks, d.step_lds, c->stream));
   const DevContactArgs<R> ca = contact_args<R>(c, d);
@@ -1178,6 +1353,11 @@ emcgpu_status run_device_steps(emcgpu_ctx *c, DeviceSim &d, const emcgpu_device_
     d.chan_x0 = d.chan_x1 = d.chan_scale = 0;
   }
   d.accumulate = s->accumulate ? 1 : 0;
+  if (s->poisson_mode != EMCGPU_POISSON_AUTO && s->poisson_mode != EMCGPU_POISSON_MULTIGRID)
+    return fail(c, EMCGPU_ERR_ARG, "poisson_mode must be EMCGPU_POISSON_AUTO or _MULTIGRID");
+  if (s->poisson_mode == EMCGPU_POISSON_AUTO)
+    ST(green_build(c, d));
+  d.green_use = (s->poisson_mode == EMCGPU_POISSON_AUTO && d.green_state == 1) ? 1 : 0;
   ST(ensure_hist(c, d, s->n_steps));
   const size_t nc = std::max(d.nContacts, 1);
   HIPCHK(c, dev_launch_zero_i(d.d_hexits, (long long)s->n_steps * nc, c->stream));
@@ -1209,7 +1389,9 @@ emcgpu_status run_device_steps(emcgpu_ctx *c, DeviceSim &d, const emcgpu_device_
     }
     if (s->use_graph && k == kChunk) {
       if (!d.gexec || d.graph_cur != d.cur || d.graph_cap != d.capacity ||
-          d.graph_chan != d.chan || d.graph_acc != d.accumulate || d.graph_x0 != d.chan_x0 ||
+          d.graph_chan != d.chan || d.graph_acc != d.accumulate ||
+          d.graph_green != d.green_use || d.graph_interval != d.interval ||
+          d.graph_x0 != d.chan_x0 ||
           d.graph_x1 != d.chan_x1 || d.graph_scale != d.chan_scale) {
         if (d.gexec) {
           (void)hipGraphExecDestroy(d.gexec);
@@ -1233,6 +1415,8 @@ emcgpu_status run_device_steps(emcgpu_ctx *c, DeviceSim &d, const emcgpu_device_
         d.graph_cap = d.capacity;
         d.graph_chan = d.chan;
         d.graph_acc = d.accumulate;
+        d.graph_green = d.green_use;
+        d.graph_interval = d.interval;
         d.graph_x0 = d.chan_x0;
         d.graph_x1 = d.chan_x1;
         d.graph_scale = d.chan_scale;

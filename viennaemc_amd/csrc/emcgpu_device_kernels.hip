@@ -1428,7 +1428,41 @@ __global__ __launch_bounds__(kDevThreads) void k_dev_field(const DevFieldArgs<R>
     }
     return;
   }
-  if (a.do_poisson) {
+  // the dense corrections before this launch: their checks say how many
+  // ran and whether one already meets the stopping rule
+  int green_done = 0;
+  bool green_ok = false;
+  double green_fm = 0.0, green_scale = 1.0;
+  if (a.do_poisson && a.green_iters > 0 && a.green_acc &&
+      *a.green_flag != (unsigned long long)a.state->step + 1ull) {
+    for (int c = 0; c <= a.green_iters && !green_ok; ++c) {
+      const double fm_c = __longlong_as_double((long long)a.green_acc[2 * c]);
+      double sc_c = __longlong_as_double((long long)a.green_acc[2 * c + 1]);
+      sc_c = sc_c > 0.0 ? sc_c : 1.0;
+      green_fm = fm_c;
+      green_scale = sc_c;
+      if (fm_c <= a.tol * sc_c)
+        green_ok = true;  // the corrections after this check were skipped
+      else if (c < a.green_iters)
+        ++green_done;
+    }
+  }
+  if (a.green_iters > 0 && a.green_acc) {
+    __syncthreads();  // every lane has read the checks
+    if (tid < 2 * (a.green_iters + 1))
+      a.green_acc[tid] = 0ull;
+  }
+  if (a.do_poisson && green_ok && tid == 0) {
+    if (a.iters_out)
+      *a.iters_out = green_done;
+    if (a.resid_out)
+      *a.resid_out = green_fm / green_scale;
+    if (a.record) {
+      a.hist.iters[row] = green_done;
+      a.hist.resid[row] = green_fm / green_scale;
+    }
+  }
+  if (a.do_poisson && !green_ok) {
     MGShared m;
     m.d = lds;
     m.res = lds + a.lds_res;
@@ -1584,7 +1618,7 @@ __global__ __launch_bounds__(kDevThreads) void k_dev_field(const DevFieldArgs<R>
       if (a.resid_out)
         *a.resid_out = fm / scale;
       if (a.record) {
-        a.hist.iters[row] = vcycles;
+        a.hist.iters[row] = vcycles + green_done;
         a.hist.resid[row] = fm / scale;
       }
     }
@@ -1759,6 +1793,194 @@ hipError_t dev_launch_interp_pos(const DevGeom<double> &g, const double *pos, lo
     return hipSuccess;
   hipLaunchKernelGGL(k_dev_interp_pos, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, pos,
                      n, charge, force);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// dense Newton corrections of the per-step Poisson solve
+// ---------------------------------------------------------------------------
+// Newton residual F of the non-equilibrium equation at every node (the
+// stencil of k_dev_field), g = H F, and residual check c: max |F| and the
+// stopping rule's scale into acc[c].  A node whose hole term h^2 p is not
+// negligible against the diagonal marks the step, so its corrections are
+// skipped and the multigrid solve works alone.
+__device__ __forceinline__ bool green_converged(const GreenArgs &a, int c) {
+  const double fm = __longlong_as_double((long long)a.acc[2 * c]);
+  const double sc = __longlong_as_double((long long)a.acc[2 * c + 1]);
+  return fm <= a.tol * (sc > 0.0 ? sc : 1.0);
+}
+
+__global__ __launch_bounds__(256) void k_green_res(GreenArgs a, int c) {
+#pragma clang fp contract(off)
+  __shared__ double red[2][4];
+  if (a.interval > 1 && a.state->step % a.interval != 0)
+    return;
+  const int k = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  double af = 0.0, sc = 0.0;
+  if (k < a.nn) {
+    const double wk = a.w0[k];
+    int il, ir, jd, ju;
+    nb0(k, a.nx, a.ny, il, ir, jd, ju);
+    const double *pot = a.pot;
+    const double phi = pot[k];
+    const double p = exp(-phi);
+    const double n = a.conc[k], C = a.dopn[k];
+    const double F = wk * (a.hfx * (pot[il] + pot[ir]) + a.hfy * (pot[jd] + pot[ju]) -
+                           2.0 * (a.hfx + a.hfy) * phi + a.hp * (p - n + C));
+    a.g[k] = (double)a.hw0[k] * F;
+    af = fabs(F);
+    sc = wk * a.hp * (fabs(C) + n + p);
+    if (wk != 0.0 && a.hp * p > a.shift_max)
+      atomicMax(a.flag, (unsigned long long)a.state->step + 1ull);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double x = __shfl_xor(af, o, 64), y = __shfl_xor(sc, o, 64);
+    af = x > af ? x : af;
+    sc = y > sc ? y : sc;
+  }
+  const int wave = (int)threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][wave] = af;
+    red[1][wave] = sc;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) {
+      af = red[0][w] > af ? red[0][w] : af;
+      sc = red[1][w] > sc ? red[1][w] : sc;
+    }
+    // non-negative doubles order like their bit patterns
+    atomicMax(a.acc + 2 * c, (unsigned long long)__double_as_longlong(af));
+    atomicMax(a.acc + 2 * c + 1, (unsigned long long)__double_as_longlong(sc));
+  }
+}
+
+// x = G g with FP64 sums, pot += x at the active nodes.  g is staged in LDS
+// (FP32, the precision of G); each wave takes one row of G at a time and
+// reads it as coalesced float4 (the kernel streams G from HBM once).
+typedef float green_f4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(kGreenThreads) void k_green_apply(GreenArgs a, int c) {
+  extern __shared__ __align__(16) float gs[];  // [ld]
+  if (a.interval > 1 && a.state->step % a.interval != 0)
+    return;
+  // the same values for every block: the whole grid leaves together
+  if (*a.flag == (unsigned long long)a.state->step + 1ull || green_converged(a, c))
+    return;
+  for (int j = threadIdx.x; j < a.ld; j += kGreenThreads)
+    gs[j] = j < a.nn ? (float)a.g[j] : 0.0f;
+  __syncthreads();
+  const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+  const int rows_per_block = kGreenThreads / 64;
+  const int q4 = a.ld / 4;  // float4 per row, a multiple of 64
+  const green_f4 *g4 = reinterpret_cast<const green_f4 *>(gs);
+  for (int row = (int)blockIdx.x * rows_per_block + wave; row < a.nn;
+       row += (int)gridDim.x * rows_per_block) {
+    if (a.w0[row] == 0.0f)
+      continue;  // the same for the whole wave
+    const green_f4 *G = reinterpret_cast<const green_f4 *>(a.green + (size_t)row * a.ld);
+    // four row segments in flight per lane
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    int q = lane;
+    for (; q + 192 < q4; q += 256) {
+      green_f4 m[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        m[u] = __builtin_nontemporal_load(G + q + 64 * u);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const green_f4 v = g4[q + 64 * u];
+        acc[u] = fma((double)m[u].x, (double)v.x, acc[u]);
+        acc[u] = fma((double)m[u].y, (double)v.y, acc[u]);
+        acc[u] = fma((double)m[u].z, (double)v.z, acc[u]);
+        acc[u] = fma((double)m[u].w, (double)v.w, acc[u]);
+      }
+    }
+    for (; q < q4; q += 64) {
+      const green_f4 m0 = G[q];
+      const green_f4 v = g4[q];
+      acc[0] = fma((double)m0.x, (double)v.x, acc[0]);
+      acc[0] = fma((double)m0.y, (double)v.y, acc[0]);
+      acc[0] = fma((double)m0.z, (double)v.z, acc[0]);
+      acc[0] = fma((double)m0.w, (double)v.w, acc[0]);
+    }
+    double acc_row = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+      acc_row += __shfl_xor(acc_row, o, 64);
+    if (lane == 0)
+      a.pot[row] += acc_row;
+  }
+}
+
+// dynamic LDS of the apply kernel beyond the default 64 KB
+hipError_t dev_green_lds(size_t lds) {
+  if (lds <= 64 * 1024)
+    return hipSuccess;
+  return hipFuncSetAttribute((const void *)k_green_apply,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+// residual check, correction (skipped once a check meets the stopping
+// rule), ..., and a last check after the last correction
+hipError_t dev_launch_green(const GreenArgs &a, int iters, int blocks, hipStream_t st) {
+  const unsigned rb = (unsigned)((a.nn + 255) / 256);
+  for (int it = 0; it < iters; ++it) {
+    hipLaunchKernelGGL(k_green_res, dim3(rb), dim3(256), 0, st, a, it);
+    hipLaunchKernelGGL(k_green_apply, dim3((unsigned)blocks), dim3(kGreenThreads),
+                       (size_t)a.ld * sizeof(float), st, a, it);
+  }
+  hipLaunchKernelGGL(k_green_res, dim3(rb), dim3(256), 0, st, a, iters);
+  return hipGetLastError();
+}
+
+// the operator's entries (row, column, value) into the column-major dense
+// matrix of the active nodes
+__global__ void k_green_scatter(const int *__restrict__ rows, const int *__restrict__ cols,
+                                const double *__restrict__ vals, int n, double *__restrict__ A,
+                                long long lda) {
+  const int e = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  if (e < n)
+    A[(long long)cols[e] * lda + rows[e]] = vals[e];
+}
+
+// G[k][j] (row stride ld, FP32) from the lower triangle of the FP64
+// inverse of the active nodes; Dirichlet rows and columns and the padding
+// are zero
+__global__ void k_green_expand(const double *__restrict__ Ainv, long long lda,
+                               const int *__restrict__ idx, int nn, int ld,
+                               float *__restrict__ G) {
+  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long total = (long long)nn * ld;
+  if (e >= total)
+    return;
+  const int k = (int)(e / ld), j = (int)(e - (long long)k * ld);
+  float v = 0.0f;
+  if (j < nn) {
+    const int ak = idx[k], aj = idx[j];
+    if (ak >= 0 && aj >= 0) {
+      const int hi = ak > aj ? ak : aj, lo = ak > aj ? aj : ak;
+      v = (float)Ainv[(long long)lo * lda + hi];
+    }
+  }
+  G[e] = v;
+}
+
+hipError_t dev_launch_green_scatter(const int *rows, const int *cols, const double *vals, int n,
+                                    double *A, long long lda, hipStream_t st) {
+  if (n <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_green_scatter, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, rows,
+                     cols, vals, n, A, lda);
+  return hipGetLastError();
+}
+
+hipError_t dev_launch_green_expand(const double *Ainv, long long lda, const int *idx, int nn,
+                                   int ld, float *G, hipStream_t st) {
+  const long long total = (long long)nn * ld;
+  hipLaunchKernelGGL(k_green_expand, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+                     Ainv, lda, idx, nn, ld, G);
   return hipGetLastError();
 }
 

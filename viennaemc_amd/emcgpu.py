@@ -122,7 +122,10 @@ class DeviceStepArgs(C.Structure):
                 ("checksum", C.c_void_p), ("poisson_interval", C.c_int),
                 ("channel_x0", C.c_double), ("channel_x1", C.c_double),
                 ("channel_length", C.c_double), ("channel_current", C.c_void_p),
-                ("accumulate", C.c_int)]
+                ("accumulate", C.c_int), ("poisson_mode", C.c_int)]
+
+
+POISSON_AUTO, POISSON_MULTIGRID = 0, 1
 
 
 TUNE_MAX = 32
@@ -790,9 +793,10 @@ class Context:
         return n.value
 
     def device_step(self, dt, n_steps, seed=0, graph=False, poisson_interval=1, channel=None,
-                    accumulate=False):
+                    accumulate=False, poisson_mode=POISSON_AUTO):
         """emcgpu_device_step; channel = (x0, x1, length) [m] records the
-        drift-velocity channel current of every step (key "channel_current")."""
+        drift-velocity channel current of every step (key "channel_current");
+        poisson_mode POISSON_MULTIGRID solves without the dense corrections."""
         nc = max(self.n_contacts, 1)
         out = dict(exits=np.zeros((n_steps, nc), dtype=np.int32),
                    deleted=np.zeros((n_steps, nc), dtype=np.int32),
@@ -807,6 +811,7 @@ class Context:
         a.use_graph = 1 if graph else 0
         a.poisson_interval = poisson_interval
         a.accumulate = 1 if accumulate else 0
+        a.poisson_mode = poisson_mode
         if channel is not None:
             a.channel_x0, a.channel_x1, a.channel_length = channel
             out["channel_current"] = np.zeros(n_steps)
