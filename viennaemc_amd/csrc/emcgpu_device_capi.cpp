@@ -24,6 +24,8 @@ namespace emcgpu {
 template <typename R>
 hipError_t dev_launch_field(const DevFieldArgs<R> *args_dev, size_t lds, hipStream_t st);
 template <typename R>
+hipError_t dev_launch_efield(const DevFieldArgs<R> *args_dev, int nn, hipStream_t st);
+template <typename R>
 hipError_t dev_launch_step(const DevStepArgs<R> &a, int blocks, size_t lds, hipStream_t st);
 template <typename R>
 hipError_t dev_launch_drift(const DevStepArgs<R> &a, int mode, hipStream_t st);
@@ -1290,6 +1292,10 @@ GreenArgs green_args(const DeviceSim &d) {
   return g;
 }
 
+// grids whose contact nodes copy an inner node's field (the E-field kernel
+// over all CUs forms it directly)
+inline bool split_efield(const DeviceSim &d) { return d.ext[0] >= 3 && d.ext[1] >= 3; }
+
 // arguments of the per-step Poisson / E-field launch (slot 1: the first
 // step, which sets the contacts' applied voltages)
 template <typename R>
@@ -1302,6 +1308,9 @@ DevFieldArgs<R> step_field_args(const emcgpu_ctx *c, const DeviceSim &d, bool re
   f.record = 1;
   f.nrpart = d.d_nrpart;
   f.interval = d.interval;
+  // the E-field and force grid of a step over all CUs (k_dev_efield)
+  if (split_efield(d))
+    f.do_efield = 0;
   if (d.green_use && !reset_bc) {
     f.green_iters = kGreenIters;
     f.green_flag = d.d_green_flag;
@@ -1316,6 +1325,8 @@ emcgpu_status launch_step(emcgpu_ctx *c, DeviceSim &d, bool reset_bc) {
   if (d.green_use && !reset_bc)
     HIPCHK(c, dev_launch_green(green_args(d), kGreenIters, d.green_blocks, c->stream));
   HIPCHK(c, dev_launch_field<R>(fargs_slot<R>(d, reset_bc ? 1 : 0), d.mg_lds, c->stream));
+  if (split_efield(d))
+    HIPCHK(c, dev_launch_efield<R>(fargs_slot<R>(d, reset_bc ? 1 : 0), (int)d.nn, c->stream));
   HIPCHK(c, dev_launch_step<R>(step_args<R>(c, d), d.blocks, d.step_lds, c->stream));
   const DevContactArgs<R> ca = contact_args<R>(c, d);
   if (ca.n_slots > 0)

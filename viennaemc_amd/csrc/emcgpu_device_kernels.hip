@@ -1656,9 +1656,59 @@ __global__ __launch_bounds__(kDevThreads) void k_dev_field(const DevFieldArgs<R>
   }
 }
 
+// The E-field and force grid of k_dev_field over all CUs, one node per
+// lane, for grids of at least 3 x 3 nodes (a contact node copies its inner
+// neighbour, which is then an inner node whose value is formed here
+// directly; the formulas and their rounding are those of k_dev_field).
+template <typename R>
+__global__ __launch_bounds__(256) void k_dev_efield(const DevFieldArgs<R> *__restrict__ args) {
+#pragma clang fp contract(off)
+  const DevFieldArgs<R> &a = *args;
+  if (a.record && a.interval > 1 && a.state->step % a.interval != 0)
+    return;  // the field of the last solve is kept
+  const int nx = a.nx, nn = a.nn;
+  const int k = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  if (k >= nn)
+    return;
+  const double *pot = a.pot;
+  const bool nec = a.pm == EMCGPU_PM_NEC;
+  const int j = idiv(k, nx), i = k - j * nx;
+  auto fx = [&](int q) {
+    return nec ? (pot[q] - pot[q + 1]) * a.vt / a.hxm
+               : (pot[q - 1] - pot[q + 1]) * a.vt / (2 * a.hxm);
+  };
+  auto fy = [&](int q) {
+    return nec ? (pot[q] - pot[q + nx]) * a.vt / a.hym
+               : (pot[q - nx] - pot[q + nx]) * a.vt / (2 * a.hym);
+  };
+  const int bx = a.bcx[k], by = a.bcy[k];
+  double ex = a.ex[k], ey = a.ey[k];
+  if (bx == -2)
+    ex = fx(k);
+  else
+    ex = bx == -1 ? 0.0 : fx(i == 0 ? k + 1 : k - 1);
+  if (by == -2)
+    ey = fy(k);
+  else
+    ey = by == -1 ? 0.0 : fy(j == 0 ? k + nx : k - nx);
+  a.ex[k] = ex;
+  a.ey[k] = ey;
+  if (a.acc) {
+    a.acc[2 * k] = (R)(ex * a.acc_scale);
+    a.acc[2 * k + 1] = (R)(ey * a.acc_scale);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
+template <typename R>
+hipError_t dev_launch_efield(const DevFieldArgs<R> *args_dev, int nn, hipStream_t st) {
+  hipLaunchKernelGGL((k_dev_efield<R>), dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st,
+                     args_dev);
+  return hipGetLastError();
+}
+
 // the arguments live in device memory (args_dev, written beforehand by
 // the caller): dynamic indexing of the level table would otherwise copy the
 // whole block to scratch
@@ -1986,6 +2036,7 @@ hipError_t dev_launch_green_expand(const double *Ainv, long long lda, const int 
 
 #define EMCGPU_DEV_INST(R)                                                                   \
   template hipError_t dev_launch_field<R>(const DevFieldArgs<R> *, size_t, hipStream_t);      \
+  template hipError_t dev_launch_efield<R>(const DevFieldArgs<R> *, int, hipStream_t);        \
   template hipError_t dev_launch_step<R>(const DevStepArgs<R> &, int, size_t, hipStream_t);   \
   template hipError_t dev_launch_drift<R>(const DevStepArgs<R> &, int, hipStream_t);          \
   template hipError_t dev_launch_contacts<R>(const DevContactArgs<R> &, long long,           \
