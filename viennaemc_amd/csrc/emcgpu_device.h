@@ -31,8 +31,10 @@ constexpr int kMaxPPT = 8;
 constexpr int kMaxBaths = EMCGPU_MAX_BATHS;
 constexpr int kMaxBins = EMCGPU_MAX_BINS;
 // copies of the global bath tallies: block b adds its counts to copy
-// b % kTallyCopies, so the few hot bins are not hammered by every block
-constexpr int kTallyCopies = 64;
+// b % kTallyCopies.  Workgroups are dealt to the 8 XCDs round-robin, so each
+// copy is written from one XCD and its atomics stay in that XCD's L2, and
+// the few hot bins are not hammered by every block
+constexpr int kTallyCopies = 8;
 
 // final-state sampler of a mechanism
 enum FinalKind : int {
@@ -180,6 +182,7 @@ struct BathStepArgs {
   int n_obs, n_tiles, tile_size;
   int n_jobs;
   double *rb_max;  // [job][kRebuildChunks] block maxima of the cumulative rates
+  unsigned int *rb_ticket;  // [job] chunks of the job's rebuild finished (0 between rebuilds)
   const long long *inst_start, *inst_count;  // particles of each instance
 };
 
