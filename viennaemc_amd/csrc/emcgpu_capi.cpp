@@ -469,7 +469,6 @@ BathStepArgs<R> bath_args(const emcgpu_ctx *c, double dt, double *hist, int hist
   a.n_obs = c->n_obs;
   a.n_jobs = c->n_jobs;
   a.rb_max = c->d_rb_max;
-  a.rb_ticket = c->d_rb_ticket;
   return a;
 }
 
@@ -544,7 +543,6 @@ emcgpu_status upload_rebuild(emcgpu_ctx *c, size_t cum_elems) {
   dfreet(c->d_rate_arena);
   dfreet(c->d_scr);
   dfreet(c->d_rb_max);
-  dfreet(c->d_rb_ticket);
   c->n_jobs = (int)jobs.size();
   c->n_groups = (int)groups.size();
   HIPCHK(c, hipMalloc((void **)&c->d_scr, sizeof(double) * scr.size()));
@@ -565,8 +563,6 @@ emcgpu_status upload_rebuild(emcgpu_ctx *c, size_t cum_elems) {
   }
   HIPCHK(c, hipMalloc((void **)&c->d_jobs, sizeof(RebuildJob) * jobs.size()));
   HIPCHK(c, hipMalloc((void **)&c->d_rb_max, sizeof(double) * kRebuildChunks * jobs.size()));
-  HIPCHK(c, hipMalloc((void **)&c->d_rb_ticket, sizeof(unsigned int) * jobs.size()));
-  HIPCHK(c, hipMemset(c->d_rb_ticket, 0, sizeof(unsigned int) * jobs.size()));
   HIPCHK(c, hipMalloc((void **)&c->d_rate_arena, sizeof(double) * std::max<size_t>(arena.size(), 1)));
   HIPCHK(c, hipMemcpy(c->d_jobs, jobs.data(), sizeof(RebuildJob) * jobs.size(),
                       hipMemcpyHostToDevice));
@@ -1352,7 +1348,6 @@ void emcgpu_destroy(emcgpu_ctx *c) {
   dfreet(c->d_groups);
   dfreet(c->d_scr);
   dfreet(c->d_rb_max);
-  dfreet(c->d_rb_ticket);
   dfreet(c->d_win64);
   dfree(c->d_win);
   if (c->gexec)

@@ -182,7 +182,6 @@ struct BathStepArgs {
   int n_obs, n_tiles, tile_size;
   int n_jobs;
   double *rb_max;  // [job][kRebuildChunks] block maxima of the cumulative rates
-  unsigned int *rb_ticket;  // [job] chunks of the job's rebuild finished (0 between rebuilds)
   const long long *inst_start, *inst_count;  // particles of each instance
 };
 

@@ -130,7 +130,6 @@ struct emcgpu_ctx {
   int n_groups = 0;
   double *d_rate_arena = nullptr;
   double *d_rb_max = nullptr;  // [job][kRebuildChunks] of the multi-block rebuild
-  unsigned int *d_rb_ticket = nullptr;  // [job] completion tickets of the rebuild
   double *d_cum64 = nullptr, *d_tau0_64 = nullptr;
   // q_s^2 per instance [1/m^2] and the baths' window sums (FP64 and in the
   // precision of the kinematics): per bath cumW[n_bins + 1], cumWN[n_bins + 1]

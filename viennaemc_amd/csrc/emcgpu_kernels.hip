@@ -1041,44 +1041,45 @@ __global__ __launch_bounds__(kRebuildChunk) void k_rebuild_rates(BathStepArgs<R>
       red[threadIdx.x] = red[threadIdx.x + w];
     __syncthreads();
   }
-  // ... then the block that finishes the job's chunks last (a ticket, no
-  // waiting) takes the largest total rate over the chunks and writes the
-  // normalised tables and tau0 (renormalizeTables, emcScatterHandler.hpp:
-  // 248-272); with no rate at any level the kinematics keep the previous
-  // tables
-  __shared__ bool last;
-  const int nc = (nl + kRebuildChunk - 1) / kRebuildChunk;
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0)
     a.rb_max[(long long)blockIdx.y * kRebuildChunks + blockIdx.x] = red[0];
-    __threadfence();
-    last = atomicAdd(&a.rb_ticket[blockIdx.y], 1u) == (unsigned int)nc - 1;
-  }
-  __syncthreads();
-  if (!last)
+}
+
+// ... then the largest total rate of the job over its chunks, the
+// normalised tables and tau0 (renormalizeTables, emcScatterHandler.hpp:
+// 248-272); with no rate at any level the kinematics keep the previous tables
+template <typename R>
+__global__ __launch_bounds__(kRebuildChunk) void k_rebuild_norm(BathStepArgs<R> a) {
+#pragma clang fp contract(off)
+  const RebuildJob &J = a.jobs[blockIdx.y];
+  if (!(a.rebuild || J.every_step))
     return;
-  __threadfence();
+  const int nl = J.n_levels, nt = J.n_tables;
+  const int l0 = (int)blockIdx.x * kRebuildChunk;
+  if (l0 >= nl)
+    return;
+  const int nc = (nl + kRebuildChunk - 1) / kRebuildChunk;
   double mx = -1.0;
   for (int k = 0; k < nc; ++k) {
-    const double m = __hip_atomic_load(&a.rb_max[(long long)blockIdx.y * kRebuildChunks + k],
-                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const double m = a.rb_max[(long long)blockIdx.y * kRebuildChunks + k];
     mx = (mx < m) ? m : mx;
   }
-  if (mx > 0.0) {
-    for (int e0 = (int)threadIdx.x; e0 < nl * nt; e0 += kRebuildChunk) {
-      const long long e = J.cum_off + e0;
-      const double x =
-          __hip_atomic_load(&a.cum64[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) / mx;
+  if (!(mx > 0.0))
+    return;
+  const int l = l0 + (int)threadIdx.x;
+  if (l < nl) {
+    for (int j = 0; j < nt; ++j) {
+      const long long e = J.cum_off + (long long)l * nt + j;
+      const double x = a.cum64[e] / mx;
       a.cum[e] = (R)x;
       a.cum64[e] = x;
     }
-    if (threadIdx.x == 0) {
-      const double tau = 1. / mx;
-      a.inst[J.inst].tau0[J.valley] = (R)(tau * 1e15);
-      a.tau0_64[J.inst * kMaxV + J.valley] = tau;
-    }
   }
-  if (threadIdx.x == 0)
-    a.rb_ticket[blockIdx.y] = 0u;  // for the next rebuild
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const double tau = 1. / mx;
+    a.inst[J.inst].tau0[J.valley] = (R)(tau * 1e15);
+    a.tau0_64[J.inst * kMaxV + J.valley] = tau;
+  }
 }
 
 // window sums and tables alone (after the model, the field, a bath state or
@@ -1287,6 +1288,7 @@ hipError_t launch_bath_step(const BathStepArgs<R> &a, int n_groups, hipStream_t 
   if (a.n_jobs > 0) {
     const dim3 g(kRebuildChunks, a.n_jobs);
     hipLaunchKernelGGL((k_rebuild_rates<R>), g, dim3(kRebuildChunk), 0, st, a);
+    hipLaunchKernelGGL((k_rebuild_norm<R>), g, dim3(kRebuildChunk), 0, st, a);
   }
   return hipGetLastError();
 }
