@@ -145,6 +145,30 @@ int orc_device_add_ohmic(orc_device *d, int b, double voltage, double minPos,
         d->contactGrid[bb][orc_coord_boundary(c, bb)] = idx;
   }
   d->contactV[idx] = voltage;
+  d->contactType[idx] = 0;
+  d->nContacts++;
+  return idx;
+}
+
+int orc_device_add_gate(orc_device *d, int b, double voltage, double minPos, double maxPos,
+                        double epsOxide, double thickness, double barrierHeight) {
+  if (b < 0 || b > 3 || d->nContacts >= ORC_MAX_CONTACTS)
+    return -1;
+  double hs = (b / 2 == 0) ? d->h[1] : d->h[0];
+  size_t mn = (size_t)round(minPos / hs), mx = (size_t)round(maxPos / hs);
+  size_t len = (size_t)d->ext[b < 2 ? 1 : 0];
+  if (mn >= len || mx >= len || mx < mn)
+    return -1;
+  int idx = d->nContacts;
+  /* idxContactGrid[boundary].fill only: a gate is not carried to the other
+   * boundaries its corner nodes lie on */
+  for (size_t i = mn; i <= mx; i++)
+    d->contactGrid[b][i] = idx;
+  d->contactV[idx] = voltage;
+  d->contactType[idx] = 2;
+  d->gateEps[idx] = epsOxide;
+  d->gateThick[idx] = thickness;
+  d->gateBarrier[idx] = barrierHeight;
   d->nContacts++;
   return idx;
 }
@@ -177,7 +201,8 @@ int orc_device_contact_idx(const orc_device *d, const size_t c[2]) {
 }
 
 int orc_device_is_ohmic(const orc_device *d, const size_t c[2]) {
-  return orc_device_contact_idx(d, c) != -1;
+  int ci = orc_device_contact_idx(d, c);
+  return ci != -1 && d->contactType[ci] == 0;
 }
 
 /* ---- particle-mesh schemes ----------------------------------------------- */
@@ -327,6 +352,16 @@ void orc_sor_init(orc_sor *s, const orc_device *d, double accuracyV,
   s->dopingNorm = (double *)malloc(sizeof(double) * n);
   for (size_t i = 0; i < n; i++)
     s->dopingNorm[i] = d->doping[i] / d->ni;
+  /* gate characteristics (emcSORSolver.hpp:355-366) */
+  for (int k = 0; k < d->nContacts; k++) {
+    s->gateVoltage[k] = d->contactV[k] / d->Vt;
+    if (d->contactType[k] == 2) {
+      double gamma = d->gateEps[k] / d->epsR;
+      double thicknessOx = d->gateThick[k] / d->debye;
+      s->gateFactor[k] = 2 * gamma / thicknessOx;
+      s->gateBarrier[k] = d->gateBarrier[k] / d->Vt;
+    }
+  }
 }
 
 void orc_sor_free(orc_sor *s) {
@@ -334,22 +369,42 @@ void orc_sor_free(orc_sor *s) {
   s->dopingNorm = NULL;
 }
 
+/* checkForGateContact (emcSORSolver.hpp:399-412): a gate on boundary b at
+ * this node adds its Robin term (barrier height, plus the applied voltage
+ * in the non-equilibrium solve) */
+static void orc_sor_gate(const orc_sor *s, const orc_device *d, const size_t c[2], int dim,
+                         int isMax, int useVApp, double *nom, double *den) {
+  int b = 2 * dim + isMax;
+  int ci = d->contactGrid[b][orc_coord_boundary(c, b)];
+  if (ci == -1 || d->contactType[ci] != 2)
+    return;
+  double Vg = s->gateBarrier[ci];
+  if (useVApp)
+    Vg += s->gateVoltage[ci];
+  *nom += s->gateFactor[ci] * Vg * s->hFactor[dim] * s->hNorm[dim];
+  *den += s->gateFactor[ci] * s->hFactor[dim] * s->hNorm[dim];
+}
+
 static void orc_sor_neighbours(const orc_sor *s, const orc_device *d,
                                const double *pot, size_t ix, size_t iy,
-                               double *nom) {
+                               double *nom, double *den, int useVApp) {
   size_t c[2] = {ix, iy};
   for (int dim = 0; dim < 2; dim++) {
     size_t pv[2] = {ix, iy}, nx[2] = {ix, iy};
     pv[dim]--;
     nx[dim]++;
-    if (c[dim] != 0)
+    if (c[dim] != 0) {
       *nom += pot[orc_device_idx(d, pv[0], pv[1])] * s->hFactor[dim];
-    else /* at the minimal boundary: mirrored neighbour */
+    } else { /* at the minimal boundary: mirrored neighbour */
       *nom += pot[orc_device_idx(d, nx[0], nx[1])] * s->hFactor[dim];
-    if (c[dim] != s->maxCoord[dim])
+      orc_sor_gate(s, d, c, dim, 0, useVApp, nom, den);
+    }
+    if (c[dim] != s->maxCoord[dim]) {
       *nom += pot[orc_device_idx(d, nx[0], nx[1])] * s->hFactor[dim];
-    else
+    } else {
       *nom += pot[orc_device_idx(d, pv[0], pv[1])] * s->hFactor[dim];
+      orc_sor_gate(s, d, c, dim, 1, useVApp, nom, den);
+    }
   }
 }
 
@@ -359,7 +414,7 @@ static void orc_sor_set_contacts(const orc_sor *s, const orc_device *d,
     size_t len = (size_t)d->ext[b < 2 ? 1 : 0];
     for (size_t cs = 0; cs < len; cs++) {
       int ci = d->contactGrid[b][cs];
-      if (ci == -1)
+      if (ci == -1 || d->contactType[ci] != 0)
         continue;
       size_t c[2];
       orc_coord_device(d, cs, b, c);
@@ -389,7 +444,7 @@ void orc_sor_equilibrium(orc_sor *s, const orc_device *d, double *pot,
         double p = 1. / n;
         double nom = s->hProduct * (p - n + s->dopingNorm[i] + cur * (p + n));
         double den = 2 * s->hFactorSum + s->hProduct * (n + p);
-        orc_sor_neighbours(s, d, pot, ix, iy, &nom);
+        orc_sor_neighbours(s, d, pot, ix, iy, &nom, &den, 0);
         double delta = s->omega * (nom / den - cur);
         pot[i] = cur + delta;
         if (fabs(delta) > error)
@@ -418,7 +473,7 @@ void orc_sor_nonequilibrium(orc_sor *s, const orc_device *d, double *pot,
         double n = eConc[i];
         double nom = s->hProduct * (p - n + s->dopingNorm[i] + cur * (p + n));
         double den = 2 * s->hFactorSum + s->hProduct * (n + p);
-        orc_sor_neighbours(s, d, pot, ix, iy, &nom);
+        orc_sor_neighbours(s, d, pot, ix, iy, &nom, &den, 1);
         double delta = s->omega * (nom / den - cur);
         pot[i] = cur + delta;
         if (fabs(delta) > error)
@@ -1154,7 +1209,7 @@ void orc_device_get_grids(const orc_device *d, double *doping, int *region,
       if (region)
         region[i] = d->region[i];
       if (ohmic)
-        ohmic[i] = orc_device_contact_idx(d, c);
+        ohmic[i] = orc_device_is_ohmic(d, c) ? orc_device_contact_idx(d, c) : -1;
     }
 }
 

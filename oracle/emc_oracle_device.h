@@ -40,6 +40,11 @@ typedef struct {
   int *contactGrid[4]; /* per boundary XMIN, XMAX (ext[1] nodes), YMIN, YMAX (ext[0]) */
   int nContacts;
   double contactV[ORC_MAX_CONTACTS];
+  /* emcContactType (emcContact.hpp:8): 0 ohmic, 2 gate; gate parameters
+   * (emcGateContact, emcContact.hpp:80-100): oxide permittivity, oxide
+   * thickness [m], barrier height [V] */
+  int contactType[ORC_MAX_CONTACTS];
+  double gateEps[ORC_MAX_CONTACTS], gateThick[ORC_MAX_CONTACTS], gateBarrier[ORC_MAX_CONTACTS];
 } orc_device;
 
 /* emcDevice ctor (emcDevice.hpp:83-99), maxPosToExtent (emcUtil.hpp:112-121) */
@@ -55,6 +60,11 @@ void orc_device_add_doping(orc_device *d, const double minPos[2],
 /* addOhmicContact (emcDevice.hpp:177-182, emcSurface.hpp:205-233, 381-408) */
 int orc_device_add_ohmic(orc_device *d, int boundary, double voltage,
                          double minPos, double maxPos);
+/* addGateContact (emcDevice.hpp:195-202, emcSurface.hpp:198-214): marked on
+ * its own boundary only */
+int orc_device_add_gate(orc_device *d, int boundary, double voltage, double minPos,
+                        double maxPos, double epsOxide, double thickness,
+                        double barrierHeight);
 /* posToCoord (emcDevice.hpp:290-297) */
 void orc_device_pos_to_coord(const orc_device *d, const double pos[2],
                              size_t coord[2]);
@@ -87,6 +97,11 @@ typedef struct {
   size_t maxCoord[2];
   double *dopingNorm;
   long long sweeps; /* sweeps of the last solve */
+  /* gate characteristics per contact (initDeviceCharacteristics,
+   * emcSORSolver.hpp:355-366): 2 (eps_ox / eps_sc) / t_ox, the normalised
+   * barrier height and applied voltage */
+  double gateFactor[ORC_MAX_CONTACTS], gateBarrier[ORC_MAX_CONTACTS],
+      gateVoltage[ORC_MAX_CONTACTS];
 } orc_sor;
 
 void orc_sor_init(orc_sor *s, const orc_device *d, double accuracyV,

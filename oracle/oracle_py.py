@@ -105,6 +105,7 @@ def load():
         "orc_device_set_width": (None, [vp, _D]),
         "orc_device_add_doping": (None, [vp, _PD, _PD, _D]),
         "orc_device_add_ohmic": (_I, [vp, _I, _D, _D, _D]),
+        "orc_device_add_gate": (_I, [vp, _I, _D, _D, _D, _D, _D, _D]),
         "orc_device_info": (None, [vp, _PI32, _PD]),
         "orc_device_get_grids": (None, [vp, C.c_void_p, C.c_void_p, C.c_void_p]),
         "orc_sor_solve": (C.c_longlong, [vp, _PD, C.c_void_p, _D, _D, _I]),
@@ -472,7 +473,10 @@ PM_NGP, PM_CIC, PM_NEC = 0, 1, 2
 
 
 class OracleDevice:
-    """A reference 2-D device (emcDevice + doping + ohmic contacts) in the oracle."""
+    """A reference 2-D device (emcDevice + doping + contacts) in the oracle.
+    A contact is (boundary, voltage, min, max) for an ohmic contact or
+    (boundary, voltage, min, max, eps_oxide, thickness, barrier_height) for a
+    gate (addGateContact, emcDevice.hpp:195-202)."""
 
     def __init__(self, spec):
         L = load()
@@ -485,8 +489,12 @@ class OracleDevice:
         for lo, hi, dop in spec["doping"]:
             L.orc_device_add_doping(self.h, np.array(lo, dtype=np.float64), np.array(hi, dtype=np.float64),
                                     dop)
-        for b, v, lo, hi in spec["contacts"]:
-            if L.orc_device_add_ohmic(self.h, b, v, lo, hi) < 0:
+        for ct in spec["contacts"]:
+            if len(ct) == 4:
+                ok = L.orc_device_add_ohmic(self.h, *ct) >= 0
+            else:
+                ok = L.orc_device_add_gate(self.h, *ct) >= 0
+            if not ok:
                 raise ValueError("invalid contact")
         ext = np.zeros(2, dtype=np.int32)
         sc = np.zeros(4)

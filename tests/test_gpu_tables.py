@@ -1,7 +1,8 @@
 """Scatter tables built on the GPU from the mechanism descriptors
 (initScatterTables of every instance at once, emcScatterHandler.hpp:220-288)
 against the host build of the same C-ABI, whose tables are bit-exact with
-the reference's (test_capi_host.py).
+the reference's (test_capi_host.py), and against the reference's own rate
+tables (golden fixtures).
 
 The rates come from the same FP64 formulas; the device's sqrt is correctly
 rounded and its pow / log / exp within an ulp of the host's, and the
@@ -66,6 +67,40 @@ def test_gpu_tables_equal_the_host_build(n_levels, max_energy):
             nz = th != 0
             assert np.max(np.abs(tg[nz] - th[nz]) / np.abs(th[nz])) <= RTOL, (NAMES[i], v)
             assert abs(tau_g - tau_h) <= RTOL * tau_h, (NAMES[i], v)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_gpu_tables_match_the_reference_rates(name, golden):
+    """The GPU build against the reference's own rate tables (golden, written
+    by the reference's mechanisms at every level, E = (l + 1) dE): each
+    valley's cumulative rates over its mechanisms in registration order,
+    normalised by the largest total (renormalizeTables,
+    emcScatterHandler.hpp:248-272), and tau0 = 1 / that total, within the
+    device's few-ulp transcendental differences (rtol 1e-13)."""
+    emcgpu = _emc()
+    spec = MODELS[name]
+    nl, emax, T = spec["n_levels"], spec["max_energy"], spec["T"]
+    print("GPU tables vs reference rates:", name, flush=True)
+    gpu = emcgpu.Context(precision=emcgpu.FP64, n_instances=1, device=0)
+    _prep(gpu, 0, spec)
+    gpu.build_tables_all(nl, emax, [T], [dict(rho=spec.get("rho", 2329.0),
+                                              v_sound=spec.get("v_sound", 9040.0))])
+    rates = golden[name + ".rate_table"].reshape(len(spec["mechs"]), nl)
+    mv = golden[name + ".mech_valley"]
+    for v in range(len(spec["valleys"])):
+        mine = rates[mv == v]
+        tg, tau_g = gpu.get_tables(0, v, nl)
+        if mine.shape[0] == 0:
+            assert tg.size == 0
+            continue
+        cum = np.cumsum(mine, axis=0)
+        top = cum[-1].max()
+        ref = cum / top
+        nz = ref != 0
+        assert np.array_equal(tg == 0, ~nz), (name, v)
+        assert np.max(np.abs(tg[nz] - ref[nz]) / ref[nz]) <= RTOL, (name, v)
+        assert abs(tau_g - golden[name + ".tau0"][v]) <= RTOL * golden[name + ".tau0"][v]
+    gpu.close()
 
 
 def test_gpu_tables_drive_the_same_step():
