@@ -1007,6 +1007,36 @@ static void dumpDeviceKATs() {
   sec("device.kat.sor_neq", gridVec(pot));
 }
 
+// SOR with gate (Robin) contacts: the device above plus a gate on each
+// y boundary (the YMAX one over the channel, a narrower one on YMIN with
+// other oxide constants); the ohmic contacts stay Dirichlet
+static void dumpDeviceGateKATs() {
+  auto dev = makeDevice2();
+  dev->addGateContact(emcBoundaryPos::YMAX, 0.4, {5e-9}, {15e-9}, 3.9, 1e-9, 0.3);
+  dev->addGateContact(emcBoundaryPos::YMIN, -0.2, {8e-9}, {12e-9}, 25., 2e-9, 0.1);
+  auto ext = dev->getGridExtent();
+  uint64_t st = 0x6A7Eull;
+  auto u = [&]() {
+    st = splitmix64(st);
+    return (double)(st >> 11) * 0x1p-53;
+  };
+  using Solver2 = emcSORSolver<T, Device2, emcBasicParticleHandler<T, Device2, emcCICScheme<T, Device2>>>;
+  Solver2 solver(*dev, 1e-12, 1.8);
+  Grid2 pot(ext);
+  std::array<SizeType, 2> c;
+  for (c.fill(0); !pot.isEndCoord(c); pot.advanceCoord(c))
+    pot[c] = std::asinh(0.5 * dev->getDopingProfile().getDoping(c, true));
+  solver.calcEquilibriumPotential(pot, *dev);
+  sec("device.gate.sor_eq", gridVec(pot));
+  Grid2 conc(ext);
+  for (c.fill(0); !conc.isEndCoord(c); conc.advanceCoord(c))
+    conc[c] = dev->getDopingProfile().getDoping(c, true) * (0.8 + 0.1 * u());
+  sec("device.gate.sor_conc", gridVec(conc));
+  solver.calcNonEquilibriumPotential(pot, *dev, conc, true);
+  sec("device.gate.sor_neq", gridVec(pot));
+  sec("device.gate.cfg", {0.4, 5e-9, 15e-9, 3.9, 1e-9, 0.3, -0.2, 8e-9, 12e-9, 25., 2e-9, 0.1});
+}
+
 // device particle trajectories under a frozen field with per-particle
 // streams: the loop body of emcBasicParticleHandler::driftScatterParticles
 // (:93-139) with driftParticle (emcAbstractParticleHandler.hpp:238-249)
@@ -1199,6 +1229,7 @@ int main(int argc, char **argv) {
   const uint64_t seed = 20261202ULL;
   dumpBathKATs();
   dumpDeviceKATs();
+  dumpDeviceGateKATs();
   dumpDeviceTrajectories(seed + 5, 64, 150, 1e-15);
   dumpDeviceRun<emcCICScheme<T, Device2>>("cic", seed + 11, 40, 1e-15, 1e-8, 1.8);
   dumpDeviceRun<emcNGPScheme<T, Device2>>("ngp", seed + 13, 40, 1e-15, 1e-8, 1.8);

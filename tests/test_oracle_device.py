@@ -25,7 +25,7 @@ solves the discrete Poisson equation.
 import numpy as np
 import pytest
 
-from emc_models import DEVICE_SMALL, MODELS_DEVICE
+from emc_models import DEVICE_GATED, DEVICE_SMALL, MODELS_DEVICE
 from oracle import oracle_py as O
 
 
@@ -114,6 +114,57 @@ def test_sor_fixed_point_solves_poisson(dev, golden):
             r = (xm + xp + ym + yp - 4 * phi[iy, ix]) + h * h * (np.exp(-phi[iy, ix]) - n[iy, ix] + C[iy, ix])
             worst = max(worst, abs(r) / (h * h * abs(C[iy, ix])))
     assert worst < 1e-9
+
+
+def test_sor_with_gates_bit_exact(golden):
+    """Two gate contacts (YMAX over x = 5..15 nm, YMIN over 8..12 nm) add the
+    Robin terms of checkForGateContact: barrier height only in the
+    equilibrium solve, barrier plus gate voltage in the non-equilibrium one."""
+    cfg = golden["device.gate.cfg"]
+    gates = [c for c in DEVICE_GATED["contacts"] if len(c) == 7]
+    assert np.array_equal(cfg, np.array([x for g in gates for x in (g[1],) + tuple(g[2:])]))
+    dev = O.OracleDevice(DEVICE_GATED)
+    c = dev.grids()["contact"].reshape(7, 21)
+    # a gate is no Dirichlet contact
+    assert (c[:, 0] == 0).all() and (c[:, -1] == 1).all() and (c[:, 1:-1] == -1).all()
+    eq, _ = dev.sor(dev.init_potential(), accuracy=1e-12, omega=1.8)
+    assert (eq == golden["device.gate.sor_eq"]).all()
+    neq, _ = dev.sor(eq, conc=golden["device.gate.sor_conc"], accuracy=1e-12, omega=1.8)
+    assert (neq == golden["device.gate.sor_neq"]).all()
+    # without the gates the solution is another one
+    assert np.abs(eq - golden["device.kat.sor_eq"]).max() > 1.0
+
+
+def test_sor_gate_fixed_point_is_the_robin_equation(golden):
+    """At a gate node of boundary YMAX the converged non-equilibrium point
+    solves the Poisson equation whose missing y-neighbour is the mirrored one
+    plus the oxide flux: hF_y (2 phi_prev - 2 phi) + g h hF_y (V_g - phi),
+    g = 2 (eps_ox / eps_sc) / (t_ox / L_D), V_g = (barrier + voltage) / V_t;
+    a node of the boundary outside the gate keeps the plain Neumann form."""
+    dev = O.OracleDevice(DEVICE_GATED)
+    phi = golden["device.gate.sor_neq"].reshape(7, 21)
+    n = golden["device.gate.sor_conc"].reshape(7, 21)
+    C = dev.grids()["doping"].reshape(7, 21) / DEVICE_GATED["ni"]
+    h = 1e-9 / dev.debye
+    worst = {}
+    for g in [c for c in DEVICE_GATED["contacts"] if len(c) == 7]:
+        b, v, lo, hi, eps_ox, t_ox, barrier = g
+        iy, iy_in = (6, 5) if b == 3 else (0, 1)
+        gf = 2 * (eps_ox / DEVICE_GATED["eps_r"]) / (t_ox / dev.debye)
+        vg = (barrier + v) / dev.Vt
+        for ix in range(1, 20):
+            on_gate = round(lo / 1e-9) <= ix <= round(hi / 1e-9)
+            r = (phi[iy, ix - 1] + phi[iy, ix + 1] - 2 * phi[iy, ix]
+                 + 2 * phi[iy_in, ix] - 2 * phi[iy, ix]
+                 + h * h * (np.exp(-phi[iy, ix]) - n[iy, ix] + C[iy, ix]))
+            diag = 4 + h * h * (np.exp(-phi[iy, ix]) + n[iy, ix])
+            if on_gate:
+                r += gf * h * (vg - phi[iy, ix])
+                diag += gf * h
+            # the residual as a potential change: within 100 x the 1e-12
+            # stopping rule (a dropped or mis-signed gate term is ~1)
+            worst[on_gate] = max(worst.get(on_gate, 0.0), abs(r) / diag)
+    assert worst[True] < 1e-10 and worst[False] < 1e-10
 
 
 def test_device_trajectories_bit_exact(dev, golden):

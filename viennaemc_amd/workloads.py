@@ -296,6 +296,13 @@ DEVICE_SMALL = dict(
     contacts=[(0, 0.2, 0.0, 6e-9), (1, 0.0, 0.0, 6e-9)])
 
 
+# the same resistor with a gate (Robin) contact on each y boundary: (boundary,
+# voltage, min, max, relative permittivity of the oxide, oxide thickness [m],
+# barrier height [V]) as in addGateContact (emcDevice.hpp:195-202)
+DEVICE_GATED = dict(DEVICE_SMALL, contacts=DEVICE_SMALL["contacts"] + [
+    (3, 0.4, 5e-9, 15e-9, 3.9, 1e-9, 0.3),
+    (2, -0.2, 8e-9, 12e-9, 25.0, 2e-9, 0.1)])
+
 def resistor_device(nx=200, ny=50, h=1e-9, n_particles=2_000_000, contact_nodes=10, bias=0.5):
     """The 2-D resistor workload: nx x ny nodes at spacing h, N_D = 1e23 /m^3
     with 1e25 /m^3 ohmic contact regions of contact_nodes columns at both
