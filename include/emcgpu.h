@@ -410,7 +410,7 @@ emcgpu_status emcgpu_observe(emcgpu_ctx *ctx, double *out);
  * The device path of emcSimulation (emcSimulation.hpp:106-193) on instance 0
  * of a context, electrons only (charge -q, one carrier per particle):
  *   geometry        emcDevice (emcDevice.hpp:83-343), emcDopingProfile,
- *                   emcSurface ohmic contacts (emcSurface.hpp:205-408)
+ *                   emcSurface ohmic and gate contacts (emcSurface.hpp:199-408)
  *   charge / force  PMSchemes/emcNGPScheme.hpp:24-66, emcCICScheme.hpp:32-173,
  *                   emcNECScheme.hpp:31-113 (the reference's weights, quirks
  *                   included)
@@ -437,11 +437,26 @@ typedef struct {
   double doping;                 /* [1/m^3] */
 } emcgpu_doping_region;
 
-/* addOhmicContact (emcDevice.hpp:177-182) */
+/* emcContactType (emcContact.hpp:8); Schottky contacts need the degenerate
+ * 2-D carrier model of the reference and are rejected */
+typedef enum {
+  EMCGPU_CONTACT_OHMIC = 0, EMCGPU_CONTACT_SCHOTTKY = 1, EMCGPU_CONTACT_GATE = 2
+} emcgpu_contact_type;
+
+/* addOhmicContact (emcDevice.hpp:177-182) and addGateContact (:195-202).
+ * An ohmic contact is a Dirichlet node set marked on every boundary its nodes
+ * lie on; a gate adds the Robin term of checkForGateContact
+ * (PoissonSolver/emcSORSolver.hpp:399-412) on its own boundary, reflects
+ * particles like an artificial boundary and keeps the contact rule of the
+ * E-field boundary.  Zero-initialised fields give an ohmic contact. */
 typedef struct {
   int boundary;     /* emcgpu_boundary */
   double voltage;   /* [V] */
   double min_pos, max_pos; /* extent along the boundary [m] */
+  int type;         /* emcgpu_contact_type */
+  double eps_oxide;      /* gate: relative permittivity of the oxide */
+  double thickness;      /* gate: oxide thickness [m] */
+  double barrier_height; /* gate: barrier height [V] */
 } emcgpu_contact;
 
 typedef struct {

@@ -56,8 +56,9 @@ def test_struct_layouts_match_header():
 
 
 def test_step_arg_layouts_match_the_c_compiler(tmp_path):
-    """sizeof / offsetof of the step-argument structs from gcc on the header
-    itself, against the ctypes mirrors of the binding"""
+    """sizeof / offsetof of the step-argument, contact and device structs
+    from gcc on the header itself, against the ctypes mirrors of the
+    binding"""
     src = tmp_path / "layout.c"
     src.write_text(
         '#include <stdio.h>\n#include <stddef.h>\n#include "emcgpu.h"\n'
@@ -66,11 +67,15 @@ def test_step_arg_layouts_match_the_c_compiler(tmp_path):
         '         offsetof(emcgpu_device_step_args, poisson_mode));\n'
         '  printf("%zu %zu\\n", offsetof(emcgpu_step_args, trace_pos),\n'
         '         offsetof(emcgpu_device_step_args, accumulate));\n'
+        '  printf("%zu %zu %zu\\n", sizeof(emcgpu_contact), offsetof(emcgpu_contact, type),\n'
+        '         offsetof(emcgpu_contact, barrier_height));\n'
+        '  printf("%zu %zu\\n", sizeof(emcgpu_device_desc),\n'
+        '         offsetof(emcgpu_device_desc, init_from_potential));\n'
         '  return 0;\n}\n')
     exe = tmp_path / "layout"
     inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
     subprocess.check_call(["gcc", "-std=c11", "-I", inc, str(src), "-o", str(exe)])
-    a, b = subprocess.check_output([str(exe)]).decode().split("\n")[:2]
+    a, b, c, d = subprocess.check_output([str(exe)]).decode().split("\n")[:4]
     s_step, s_dev, o_mode = (int(x) for x in a.split())
     o_trace, o_acc = (int(x) for x in b.split())
     assert C.sizeof(emcgpu.StepArgs) == s_step
@@ -78,6 +83,13 @@ def test_step_arg_layouts_match_the_c_compiler(tmp_path):
     assert C.sizeof(emcgpu.DeviceStepArgs) == s_dev
     assert emcgpu.DeviceStepArgs.poisson_mode.offset == o_mode
     assert emcgpu.DeviceStepArgs.accumulate.offset == o_acc
+    s_con, o_type, o_bar = (int(x) for x in c.split())
+    assert C.sizeof(emcgpu.ContactDesc) == s_con
+    assert emcgpu.ContactDesc.type.offset == o_type
+    assert emcgpu.ContactDesc.barrier_height.offset == o_bar
+    s_desc, o_init = (int(x) for x in d.split())
+    assert C.sizeof(emcgpu.DeviceDesc) == s_desc
+    assert emcgpu.DeviceDesc.init_from_potential.offset == o_init
 
 
 def host_ctx(n_instances=1):

@@ -4,7 +4,8 @@
   on the reference's known-answer positions and potentials: bit-exact (FP64);
 * Poisson: the GPU's Newton / FP32-multigrid solution of the SOR equations
   against the oracle's SOR fixed point (both converged far below the
-  tolerance) and against the discrete equation itself;
+  tolerance) and against the discrete equation itself, also with two gate
+  (Robin) contacts against the reference's recorded SOR solutions;
 * the device particle loop under the reference's frozen field with the
   reference's recorded per-particle draws (replay): exit step and contact
   and event counts exact, positions / k / E within 1e-12 relative;
@@ -13,6 +14,8 @@
 * a self-consistent run (Poisson every step, contacts, injection, charge
   assignment): conservation of the ensemble counters, the FP64 check sum,
   Poisson convergence, graph launch equal to eager launch;
+* the same loop with two gates (reflecting, Robin terms in every solve), by
+  dense corrections and by multigrid;
 * Poisson every third step and the drift-velocity channel current (over the
   middle of the device, and over all of it, where deleted and injected
   particles count) against the oracle's Philox twin; the running averages
@@ -24,7 +27,7 @@
 import numpy as np
 import pytest
 
-from emc_models import DEVICE_SMALL, MODELS_DEVICE, resistor_device
+from emc_models import DEVICE_GATED, DEVICE_SMALL, MODELS_DEVICE, resistor_device
 from oracle import oracle_py as O
 
 pytestmark = pytest.mark.gpu
@@ -86,9 +89,23 @@ def test_efield_bit_exact(golden):
         ctx.close()
 
 
-def _residual(dev, phi, n, C, eq):
-    """max |F| of the discrete SOR equations at the non-ohmic nodes"""
+def _residual(dev, phi, n, C, eq, spec=DEVICE_SMALL):
+    """max |F| of the discrete SOR equations at the non-ohmic nodes (with
+    the Robin term g (V_g - phi) at the nodes of a gate in spec)"""
     nx, ny = dev.ext
+    gterm = np.zeros((ny, nx))
+    gdiag = np.zeros((ny, nx))
+    for ct in spec["contacts"]:
+        if len(ct) != 7:
+            continue
+        b, v, lo, hi, eps_ox, t_ox, barrier = ct
+        gf = 2 * (eps_ox / spec["eps_r"]) / (t_ox / dev.debye) * (1e-9 / dev.debye)
+        vg = (barrier + (0.0 if eq else v)) / dev.Vt
+        iy = ny - 1 if b == 3 else 0
+        assert b in (2, 3)
+        for ix in range(int(round(lo / 1e-9)), int(round(hi / 1e-9)) + 1):
+            gterm[iy, ix] += gf * vg
+            gdiag[iy, ix] += gf
     phi = phi.reshape(ny, nx)
     h = 1e-9 / dev.debye
     ohm = dev.grids()["contact"].reshape(ny, nx) >= 0
@@ -104,6 +121,7 @@ def _residual(dev, phi, n, C, eq):
             nn = np.exp(phi[iy, ix]) if eq else n.reshape(ny, nx)[iy, ix]
             r = (xm + xp + ym + yp - 4 * phi[iy, ix]) + h * h * (
                 np.exp(-phi[iy, ix]) - nn + C.reshape(ny, nx)[iy, ix])
+            r += gterm[iy, ix] - gdiag[iy, ix] * phi[iy, ix]
             worst = max(worst, abs(r))
     return worst
 
@@ -127,6 +145,42 @@ def test_poisson_matches_sor_fixed_point(odev, golden):
     assert np.abs(neq - ref_neq).max() < 1e-9
     C = odev.grids()["doping"] / DEVICE_SMALL["ni"]
     assert _residual(odev, neq, conc, C, eq=False) < 1e-12
+    ctx.close()
+
+
+def test_poisson_with_gates_matches_sor(golden):
+    """Two gate contacts (addGateContact on YMAX and YMIN): the GPU solution
+    against the oracle's SOR at 1e-14 and the reference's own SOR solutions
+    (1e-12 stopping rule), equilibrium (barrier height only) and
+    non-equilibrium (barrier plus gate voltage)."""
+    print("Poisson with gates vs SOR", flush=True)
+    od = O.OracleDevice(DEVICE_GATED)
+    ctx = _ctx(dev=DEVICE_GATED, tol=1e-14)
+    pot0 = od.init_potential()
+    ref_eq, _ = od.sor(pot0, accuracy=1e-14, omega=1.8)
+    eq, it, res = ctx.poisson(pot0, equilibrium=True, reset_bc=True)
+    print("equilibrium: %d iterations, residual %.2e, max |d| %.2e, vs reference %.2e" % (
+        it, res, np.abs(eq - ref_eq).max(), np.abs(eq - golden["device.gate.sor_eq"]).max()),
+        flush=True)
+    assert res < 1e-13
+    assert np.abs(eq - ref_eq).max() < 1e-9
+    assert np.abs(eq - golden["device.gate.sor_eq"]).max() < 1e-8
+    conc = golden["device.gate.sor_conc"]
+    ref_neq, _ = od.sor(golden["device.gate.sor_eq"], conc=conc, accuracy=1e-14, omega=1.8)
+    neq, it2, res2 = ctx.poisson(golden["device.gate.sor_eq"], conc=conc, equilibrium=False,
+                                 reset_bc=True)
+    print("non-equilibrium: %d iterations, residual %.2e, max |d| %.2e, vs reference %.2e" % (
+        it2, res2, np.abs(neq - ref_neq).max(), np.abs(neq - golden["device.gate.sor_neq"]).max()),
+        flush=True)
+    assert res2 < 1e-13
+    assert np.abs(neq - ref_neq).max() < 1e-9
+    assert np.abs(neq - golden["device.gate.sor_neq"]).max() < 1e-8
+    C = od.grids()["doping"] / DEVICE_GATED["ni"]
+    assert _residual(od, neq, conc, C, eq=False, spec=DEVICE_GATED) < 1e-12
+    assert _residual(od, eq, None, C, eq=True, spec=DEVICE_GATED) < 1e-12
+    # the gate nodes are no Dirichlet nodes: the gate pulls them far from
+    # the ungated solution
+    assert np.abs(eq - golden["device.kat.sor_eq"]).max() > 1.0
     ctx.close()
 
 
@@ -332,6 +386,45 @@ def test_self_consistent_run_matches_oracle_twin():
     assert dpos < 1e-9 and dk < 1e-9 and dE < 1e-9
     f = ctx.device_fields()
     assert np.allclose(f["nr_part"], ref["nr_part"], rtol=1e-9, atol=1e-9)
+    ctx.close()
+
+
+@pytest.mark.parametrize("mode", ["auto", "multigrid"])
+def test_gated_run_matches_oracle_twin(mode):
+    """The self-consistent loop with two gates: particles reaching a gate
+    are reflected (only the ohmic contacts delete and inject), every solve
+    carries the gates' Robin terms.  Against the oracle's Philox twin as
+    above, with the dense corrections (auto) and with multigrid alone."""
+    emcgpu = _emc()
+    dev = dict(DEVICE_GATED, width=1e-5)
+    seed, ns, dt = 777, 50, 1e-15
+    od = O.OracleDevice(dev)
+    M = O.OracleModel(MODELS_DEVICE["device_para"])
+    ref = od.run_philox(M, O.PM_CIC, 1e-14, 1.8, dt, ns, seed, O.thermal_voltage(300.0),
+                        max_state=100000)
+    print("gated twin (%s): %d particles, %d steps" % (mode, ref["n_init"][0], ns), flush=True)
+    ctx = _ctx(dev=dev, tol=1e-14, max_iter=80)
+    assert ctx.device_init(seed) == ref["n_init"][0]
+    assert np.abs(ctx.device_fields()["pot"] - ref["eq_pot"]).max() < 1e-9
+    pmode = emcgpu.POISSON_AUTO if mode == "auto" else emcgpu.POISSON_MULTIGRID
+    out = ctx.device_step(dt, ns, seed=seed, poisson_mode=pmode)
+    assert (out["n_particles"] == ref["n_part"]).all()
+    assert (out["exits"] == ref["exits"]).all()
+    assert (out["deleted"] == ref["dels"]).all()
+    assert (out["injected"] == ref["injs"]).all()
+    # 1e-14 is at the FP64 rounding floor of the residual: every solve ends
+    # within a few units of it
+    assert (out["poisson_resid"] <= 1e-13).all()
+    p = ctx.device_get_particles()
+    st = ref["state"]
+    assert p["E"].size == st.shape[0]
+    dpos = np.abs(p["pos"] - st[:, 0:2]).max() / 20e-9
+    dE = (np.abs(p["E"] - st[:, 5]) / st[:, 5]).max()
+    print("final ensemble: |dpos| %.2e, |dE| %.2e, Poisson work %s" % (
+        dpos, dE, np.bincount(out["poisson_iters"])), flush=True)
+    assert dpos < 1e-9 and dE < 1e-9
+    # the potential of the last step's solve
+    assert np.abs(ctx.device_fields()["pot"] - ref["pot"][-1]).max() < 1e-9
     ctx.close()
 
 

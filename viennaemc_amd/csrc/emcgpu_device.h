@@ -314,6 +314,9 @@ struct DevFieldArgs {
   const double *dopn;          // doping / n_i
   const double *conc;          // electron concentration / n_i (non-equilibrium)
   const double *bc_eq, *bc_neq;  // ohmic node values (eq / with the applied voltage)
+  // gate (Robin) terms per node, null without gates: [node][3] = g, g V_eq,
+  // g V_neq; the residual gains g V - g phi and the Jacobian's shift g
+  const double *gate;
   const uint8_t *dmask;        // ohmic node (Dirichlet) of the fine grid
   const signed char *bcx, *bcy;  // E-field boundary: -2 off the boundary, -1 no contact
   double *ex, *ey;             // [V/m]
@@ -365,7 +368,8 @@ struct DevFieldArgs {
 // dense Newton corrections of a per-step Poisson solve: the inverse of the
 // control-volume weighted operator over the active nodes, G = (H A0)^-1
 // (A0: the discrete operator of the Newton step without the hole term
-// h^2 p, H: the control-volume weights), applied as x = G H F
+// h^2 p, gate shifts included, H: the control-volume weights), applied as
+// x = G H F
 constexpr int kGreenThreads = 1024;  // apply kernel: 16 waves, one row per wave at a time
 constexpr int kGreenMaxActive = 24576;
 struct GreenArgs {
@@ -373,6 +377,7 @@ struct GreenArgs {
   double hfx, hfy, hp;
   double *pot;                 // normalised potential (updated at active nodes)
   const double *conc, *dopn;   // electron concentration, doping / n_i
+  const double *gate;          // gate terms [node][3] (DevFieldArgs::gate), or null
   const float *w0, *hw0;       // active-node mask, control-volume weights
   const float *green;          // [nn][ld], rows and columns of Dirichlet nodes zero
   double *g;                   // [nn] H F of the current potential

@@ -75,6 +75,10 @@ struct DeviceSim {
   std::vector<int> contactGrid[4];  // per boundary XMIN, XMAX (ext[1]), YMIN, YMAX (ext[0])
   int nContacts = 0;
   double contactV[kMaxContacts] = {0};
+  int contactType[kMaxContacts] = {0};
+  double gateEps[kMaxContacts] = {0}, gateThick[kMaxContacts] = {0},
+         gateBarrier[kMaxContacts] = {0};
+  bool has_gate = false;
   int pm = EMCGPU_PM_CIC;
   bool init_pot = false;  // initial carriers from the equilibrium potential
   double tol = 1e-10;
@@ -84,6 +88,7 @@ struct DeviceSim {
   std::vector<signed char> bcx, bcy;
   std::vector<uint8_t> dmask;
   std::vector<double> dopn, bc_eq, bc_neq, cfac;
+  std::vector<double> gate;  // [node][3] Robin terms (empty without gates)
   std::vector<int> slot_node, slot_contact;
   std::vector<double> slot_exp;
   long long max_inj = 0;  // bound of the injections of one step
@@ -106,6 +111,7 @@ struct DeviceSim {
   signed char *d_bcx = nullptr, *d_bcy = nullptr;
   uint8_t *d_dmask = nullptr;
   double *d_dopn = nullptr, *d_bc_eq = nullptr, *d_bc_neq = nullptr, *d_cfac = nullptr;
+  double *d_gate = nullptr;
   double *d_pot = nullptr, *d_conc = nullptr, *d_nrpart = nullptr, *d_ex = nullptr;
   double *d_ey = nullptr, *d_corr = nullptr;
   void *d_acc = nullptr;
@@ -201,6 +207,7 @@ void device_sim_destroy(DeviceSim *d) {
     dfreet(d->d_bcx);
     dfreet(d->d_bcy);
     dfreet(d->d_dmask);
+    dfreet(d->d_gate);
     dfreet(d->d_dopn);
     dfreet(d->d_bc_eq);
     dfreet(d->d_bc_neq);
@@ -359,8 +366,9 @@ emcgpu_status build_geometry(emcgpu_ctx *c, DeviceSim &d, const emcgpu_device_de
           d.region[idx(d, ix, iy)] = r;
         }
   }
-  // ohmic contacts (emcSurface.hpp:205-233) marked on every boundary their
-  // nodes lie on (updateAllOccurences, :381-408)
+  // contacts (addContact, emcSurface.hpp:199-222): an ohmic contact is
+  // marked on every boundary its nodes lie on (updateAllOccurences,
+  // :381-408), a gate on its own boundary only
   for (int b = 0; b < 4; b++)
     d.contactGrid[b].assign(d.ext[b < 2 ? 1 : 0], -1);
   if (g.n_contacts < 0 || g.n_contacts > EMCGPU_MAX_CONTACTS ||
@@ -377,22 +385,46 @@ emcgpu_status build_geometry(emcgpu_ctx *c, DeviceSim &d, const emcgpu_device_de
     const long long len = (long long)d.contactGrid[b].size();
     if (mn < 0 || mx < mn || mx >= len)
       return fail(c, EMCGPU_ERR_ARG, "contact %d lies outside its boundary", k);
+    if (ct.type != EMCGPU_CONTACT_OHMIC && ct.type != EMCGPU_CONTACT_GATE)
+      return fail(c, EMCGPU_ERR_ARG,
+                  ct.type == EMCGPU_CONTACT_SCHOTTKY
+                      ? "contact %d: Schottky contacts are not supported"
+                      : "contact %d: unknown contact type",
+                  k);
+    const bool gate = ct.type == EMCGPU_CONTACT_GATE;
+    if (gate && !(ct.eps_oxide > 0 && ct.thickness > 0))
+      return fail(c, EMCGPU_ERR_ARG, "gate contact %d needs a positive eps_oxide and thickness",
+                  k);
     for (long long s = mn; s <= mx; s++) {
       const int ix = b == 0 ? 0 : b == 1 ? d.ext[0] - 1 : (int)s;
       const int iy = b == 2 ? 0 : b == 3 ? d.ext[1] - 1 : (int)s;
       for (int bb = 0; bb < 4; bb++)
-        if (on_boundary(d, ix, iy, bb))
+        if (bb == b || (!gate && on_boundary(d, ix, iy, bb)))
           d.contactGrid[bb][along(ix, iy, bb)] = k;
     }
     d.contactV[k] = ct.voltage;
+    d.contactType[k] = ct.type;
+    d.gateEps[k] = gate ? ct.eps_oxide : 0.0;
+    d.gateThick[k] = gate ? ct.thickness : 0.0;
+    d.gateBarrier[k] = gate ? ct.barrier_height : 0.0;
+    d.has_gate = d.has_gate || gate;
   }
   return EMCGPU_OK;
 }
 
 // per-node arrays of the kernels, ohmic-node slots, SOR constants
-// (initDeviceCharacteristics, emcSORSolver.hpp:330-353)
+// (initDeviceCharacteristics, emcSORSolver.hpp:330-366)
 void derive_nodes(DeviceSim &d) {
   const int nx = d.ext[0], ny = d.ext[1];
+  const double hn0 = d.h[0] / d.debye, hn1 = d.h[1] / d.debye;
+  d.hfx = hn1 / hn0;
+  d.hfy = hn0 / hn1;
+  d.hp = 1.0;
+  d.hp = d.hp * hn0;
+  d.hp = d.hp * hn1;
+  d.gate.clear();
+  if (d.has_gate)
+    d.gate.assign((size_t)3 * d.nn, 0.0);
   d.cnode.assign(d.nn, -1);
   d.cslot.assign(d.nn, -1);
   d.bcx.assign(d.nn, -2);
@@ -410,8 +442,30 @@ void derive_nodes(DeviceSim &d) {
     for (int ix = 0; ix < nx; ix++) {
       const int k = idx(d, ix, iy);
       d.dopn[k] = d.doping[k] / d.ni;
-      const int ci = contact_of(d, ix, iy);
+      // isOhmicContact (emcSurface.hpp:240-246): the contact of the node's
+      // first boundary, if ohmic
+      int ci = contact_of(d, ix, iy);
+      if (ci >= 0 && d.contactType[ci] != EMCGPU_CONTACT_OHMIC)
+        ci = -1;
       d.cnode[k] = ci;
+      if (d.has_gate)
+        for (int b = 0; b < 4; b++) {
+          // checkForGateContact (emcSORSolver.hpp:399-412): a gate of each
+          // boundary the node lies on adds g hF_d h_d (V_g - phi) to the
+          // residual, g = 2 (eps_ox / eps_r) / (t_ox / L_D), V_g the barrier
+          // height (equilibrium) plus the applied voltage (non-equilibrium)
+          if (!on_boundary(d, ix, iy, b))
+            continue;
+          const int gi = d.contactGrid[b][along(ix, iy, b)];
+          if (gi < 0 || d.contactType[gi] != EMCGPU_CONTACT_GATE)
+            continue;
+          const double gf = 2.0 * (d.gateEps[gi] / d.epsR) / (d.gateThick[gi] / d.debye);
+          const double hh = b / 2 == 0 ? d.hfx * hn0 : d.hfy * hn1;
+          const double vb = d.gateBarrier[gi] / d.Vt, va = d.contactV[gi] / d.Vt;
+          d.gate[3 * (size_t)k] += gf * hh;
+          d.gate[3 * (size_t)k + 1] += gf * vb * hh;
+          d.gate[3 * (size_t)k + 2] += gf * (vb + va) * hh;
+        }
       if (ix == 0)
         d.bcx[k] = (signed char)d.contactGrid[0][iy];
       else if (ix == nx - 1)
@@ -446,12 +500,6 @@ void derive_nodes(DeviceSim &d) {
         d.max_inj += (long long)std::ceil(e);
       }
     }
-  const double hn0 = d.h[0] / d.debye, hn1 = d.h[1] / d.debye;
-  d.hfx = hn1 / hn0;
-  d.hfy = hn0 / hn1;
-  d.hp = 1.0;
-  d.hp = d.hp * hn0;
-  d.hp = d.hp * hn1;
 }
 
 // coarse positions: every other node; when that misses the last node, the
@@ -592,7 +640,9 @@ emcgpu_status build_multigrid(emcgpu_ctx *c, DeviceSim &d) {
   // nodes (rows / columns of Dirichlet nodes zero), row-major, FP32
   d.direct_level = 0;
   d.cinv_t.clear();
-  if (d.wave_level < nl) {
+  // (a gate's Robin shift is not in this operator: with gates the coarse
+  // levels are swept instead)
+  if (d.wave_level < nl && !d.has_gate) {
     const int lc = d.wave_level;
     const MGLevel &C = d.lev[lc];
     const int n = C.nx * C.ny;
@@ -728,6 +778,8 @@ emcgpu_status upload_device(emcgpu_ctx *c, DeviceSim &d) {
   ST(dput(c, d.d_bc_eq, d.bc_eq));
   ST(dput(c, d.d_bc_neq, d.bc_neq));
   ST(dput(c, d.d_cfac, d.cfac));
+  if (d.has_gate)
+    ST(dput(c, d.d_gate, d.gate));
   ST(dput(c, d.d_slot_node, d.slot_node));
   ST(dput(c, d.d_slot_contact, d.slot_contact));
   ST(dput(c, d.d_slot_exp, d.slot_exp));
@@ -977,6 +1029,7 @@ DevFieldArgs<R> field_args(const emcgpu_ctx *c, const DeviceSim &d) {
   a.conc = d.d_conc;
   a.bc_eq = d.d_bc_eq;
   a.bc_neq = d.d_bc_neq;
+  a.gate = d.d_gate;
   a.dmask = d.d_dmask;
   a.bcx = d.d_bcx;
   a.bcy = d.d_bcy;
@@ -1180,6 +1233,8 @@ emcgpu_status green_build(emcgpu_ctx *c, DeviceSim &d) {
     const int i = k % nx, j = k / nx;
     const double hw = (i == 0 || i == nx - 1 ? 0.5 : 1.0) * (j == 0 || j == ny - 1 ? 0.5 : 1.0);
     ent[{idx[k], idx[k]}] += hw * 2.0 * (d.hfx + d.hfy);
+    if (d.has_gate)  // the gate's Robin shift
+      ent[{idx[k], idx[k]}] += hw * d.gate[3 * (size_t)k];
     const int nb[4] = {i > 0 ? k - 1 : k + 1, i < nx - 1 ? k + 1 : k - 1,
                        j > 0 ? k - nx : k + nx, j < ny - 1 ? k + nx : k - nx};
     for (int q = 0; q < 4; q++)
@@ -1279,6 +1334,7 @@ GreenArgs green_args(const DeviceSim &d) {
   g.pot = d.d_pot;
   g.conc = d.d_conc;
   g.dopn = d.d_dopn;
+  g.gate = d.d_gate;
   g.w0 = d.d_w0;
   g.hw0 = d.d_hw0;
   g.green = d.d_green;
@@ -1843,7 +1899,7 @@ emcgpu_status emcgpu_device_setup(emcgpu_ctx *c, const emcgpu_device_desc *desc)
   if (st == EMCGPU_OK)
     st = build_multigrid(c, *d);
   if (st == EMCGPU_OK && c->device >= 0) {
-    if (d->nContacts == 0)
+    if (d->slot_node.empty())  // the contact and injection kernels need ohmic nodes
       st = fail(c, EMCGPU_ERR_ARG, "the device needs at least one ohmic contact");
     else
       st = upload_device(c, *d);

@@ -1391,7 +1391,8 @@ __device__ __forceinline__ void mg_shift_down(const DevFieldArgs<R> &a, const MG
 
 // Newton's method on the discrete equations of emcSORSolver
 // (PoissonSolver/emcSORSolver.hpp:49-197):
-//   F_k = sum_d hF_d (phi_prev + phi_next - 2 phi_k) + hP (p - n + C) = 0
+//   F_k = sum_d hF_d (phi_prev + phi_next - 2 phi_k) + hP (p - n + C)
+//         + sum_gates g hF_d h_d (V_g - phi_k) = 0
 // at every non-ohmic node (mirror neighbours at the boundary), with
 // p = exp(-phi) and n = conc (non-equilibrium) or exp(phi) (equilibrium);
 // ohmic nodes hold V/V_T + asinh(C/2).  Residuals, the conjugate-gradient
@@ -1493,6 +1494,7 @@ __global__ __launch_bounds__(kDevThreads) void k_dev_field(const DevFieldArgs<R>
     EMC_GLOBAL double *potg = glb(pot);
     EMC_GLOBAL float2 *rs0 = glb(a.rs0);
     const EMC_GLOBAL double *conc = glb(a.conc), *dopn = glb(a.dopn);
+    const EMC_GLOBAL double *gate = a.gate ? glb(a.gate) : nullptr;
     const EMC_GLOBAL float *w = glb(a.w0);  // 1 at active nodes, 0 at Dirichlet and padding
     // control-volume weights (1/2 per boundary dimension): the mirror
     // boundary operator is symmetric in this inner product
@@ -1515,9 +1517,14 @@ __global__ __launch_bounds__(kDevThreads) void k_dev_field(const DevFieldArgs<R>
           const double p = exp(-phi);
           const double n = a.equilibrium ? exp(phi) : conc[kc];
           const double C = dopn[kc];
+          double gt = 0.0, sg = hp * (a.equilibrium ? p + n : p);
+          if (gate) {  // gate (Robin) term and its Jacobian shift
+            const double gg = gate[3 * kc];
+            gt = gate[3 * kc + (a.equilibrium ? 1 : 2)] - gg * phi;
+            sg += gg;
+          }
           const double F = wk * (hfx * (potg[il] + potg[ir]) + hfy * (potg[jd] + potg[ju]) -
-                                 hfs2 * phi + hp * (p - n + C));
-          const double sg = hp * (a.equilibrium ? p + n : p);
+                                 hfs2 * phi + hp * (p - n + C) + gt);
           r64[k] = F;
           s64[k] = sg;
           rs0[k] = make_float2((float)F, wk != 0.0f ? (float)sg : kBigShift);
@@ -1875,8 +1882,9 @@ __global__ __launch_bounds__(256) void k_green_res(GreenArgs a, int c) {
     const double phi = pot[k];
     const double p = exp(-phi);
     const double n = a.conc[k], C = a.dopn[k];
+    const double gt = a.gate ? a.gate[3 * k + 2] - a.gate[3 * k] * phi : 0.0;
     const double F = wk * (a.hfx * (pot[il] + pot[ir]) + a.hfy * (pot[jd] + pot[ju]) -
-                           2.0 * (a.hfx + a.hfy) * phi + a.hp * (p - n + C));
+                           2.0 * (a.hfx + a.hfy) * phi + a.hp * (p - n + C) + gt);
     a.g[k] = (double)a.hw0[k] * F;
     af = fabs(F);
     sc = wk * a.hp * (fabs(C) + n + p);

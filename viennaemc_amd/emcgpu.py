@@ -93,9 +93,13 @@ class DopingRegion(C.Structure):
     _fields_ = [("min_pos", C.c_double * 2), ("max_pos", C.c_double * 2), ("doping", C.c_double)]
 
 
+CONTACT_OHMIC, CONTACT_SCHOTTKY, CONTACT_GATE = 0, 1, 2
+
+
 class ContactDesc(C.Structure):
     _fields_ = [("boundary", C.c_int), ("voltage", C.c_double), ("min_pos", C.c_double),
-                ("max_pos", C.c_double)]
+                ("max_pos", C.c_double), ("type", C.c_int), ("eps_oxide", C.c_double),
+                ("thickness", C.c_double), ("barrier_height", C.c_double)]
 
 
 class DeviceDesc(C.Structure):
@@ -650,15 +654,24 @@ class Context:
                      init_from_potential=False):
         """Geometry from a device dictionary (viennaemc_amd/workloads.py format):
         max_pos, spacing, width, T, eps_r, ni, doping [(min, max, N)],
-        contacts [(boundary, voltage, min, max)]."""
+        contacts [(boundary, voltage, min, max)] for ohmic contacts and
+        [(boundary, voltage, min, max, eps_oxide, thickness, barrier_height)]
+        for gates."""
         regs = (DopingRegion * max(len(dev["doping"]), 1))()
         for r, (lo, hi, n) in enumerate(dev["doping"]):
             regs[r].min_pos[0], regs[r].min_pos[1] = lo
             regs[r].max_pos[0], regs[r].max_pos[1] = hi
             regs[r].doping = n
         cons = (ContactDesc * max(len(dev["contacts"]), 1))()
-        for k, (b, v, lo, hi) in enumerate(dev["contacts"]):
+        for k, ct in enumerate(dev["contacts"]):
+            if len(ct) not in (4, 7):
+                raise ValueError("a contact is (boundary, voltage, min, max) or a gate "
+                                 "(boundary, voltage, min, max, eps_oxide, thickness, barrier)")
+            b, v, lo, hi = ct[:4]
             cons[k].boundary, cons[k].voltage, cons[k].min_pos, cons[k].max_pos = b, v, lo, hi
+            if len(ct) == 7:
+                cons[k].type = CONTACT_GATE
+                cons[k].eps_oxide, cons[k].thickness, cons[k].barrier_height = ct[4:]
         d = DeviceDesc()
         d.max_pos[0], d.max_pos[1] = dev["max_pos"]
         d.spacing[0], d.spacing[1] = dev["spacing"]

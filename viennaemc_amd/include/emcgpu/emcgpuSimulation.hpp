@@ -77,6 +77,22 @@ public:
     c.max_pos = maxPos[0];
     contacts.push_back(c);
   }
+  /// gate over [minPos, maxPos] of a boundary (emcDevice.hpp:195-202): oxide
+  /// relative permittivity, oxide thickness [m], barrier height [V]
+  void addGateContact(emcBoundaryPos pos, T voltage, const std::array<T, 1> &minPos,
+                      const std::array<T, 1> &maxPos, T epsRoxide, T thickness,
+                      T barrierHeight) {
+    emcgpu_contact c{};
+    c.boundary = (int)pos;
+    c.voltage = voltage;
+    c.min_pos = minPos[0];
+    c.max_pos = maxPos[0];
+    c.type = EMCGPU_CONTACT_GATE;
+    c.eps_oxide = epsRoxide;
+    c.thickness = thickness;
+    c.barrier_height = barrierHeight;
+    contacts.push_back(c);
+  }
 
   const Material &getMaterial() const { return mat; }
   T getTemperature() const { return temp; }
