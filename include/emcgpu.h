@@ -593,20 +593,29 @@ typedef struct {
    * emcSimulationResults.hpp:85-93) */
   int accumulate;
   /* how the per-step Poisson solves reach the stopping rule of
-   * emcgpu_poisson (the solution is the same to that tolerance):
-   * EMCGPU_POISSON_AUTO: two Newton corrections from the inverse of the
-   * discrete operator (a dense FP32 matrix of the active nodes, built once
-   * per geometry with rocSOLVER, applied with FP64 sums over all CUs), then
-   * the multigrid-preconditioned Newton solve checks the residual and
-   * finishes the solve if needed.  Used for grids of up to 24576 active
-   * nodes, from the second step on, while the hole term h^2 p dropped from
-   * the inverse stays below 1e-9 of the diagonal; otherwise, and with
-   * EMCGPU_POISSON_MULTIGRID, the multigrid path alone. */
+   * emcgpu_poisson (the solution is the same to that tolerance): up to two
+   * Newton corrections from an inverse of the discrete operator without the
+   * hole term, each after a residual check, then the multigrid-
+   * preconditioned Newton solve checks the residual and finishes the solve
+   * if needed.  The inverse is
+   *   separable: for devices whose Dirichlet nodes are whole columns and
+   *     rows and whose gates span whole boundary lines, the operator is a
+   *     Kronecker sum of two 1-D operators; their FP64 eigenbases (rocSOLVER,
+   *     once per geometry) apply the inverse exactly in O(n (nx + ny)),
+   *   dense: otherwise a dense FP32 matrix of the active nodes (rocSOLVER
+   *     Cholesky, once per geometry), streamed with FP64 sums over all CUs,
+   *     for up to 24576 active nodes.
+   * Corrections run from the second step on, while the hole term h^2 p
+   * stays below 1e-9 of the diagonal.
+   * EMCGPU_POISSON_AUTO: separable, else dense, else multigrid alone;
+   * EMCGPU_POISSON_DENSE: dense, else multigrid alone;
+   * EMCGPU_POISSON_MULTIGRID: multigrid alone. */
   int poisson_mode;
 } emcgpu_device_step_args;
 
 #define EMCGPU_POISSON_AUTO 0
 #define EMCGPU_POISSON_MULTIGRID 1
+#define EMCGPU_POISSON_DENSE 2
 
 /* performEMCStep repeated n_steps times (emcSimulation.hpp:164-193):
  * Poisson, E-field, driftScatterParticles, handleOhmicContacts, injection,

@@ -297,35 +297,41 @@ def test_self_consistent_run(precision):
 
 @pytest.mark.parametrize("precision", [0, 1])
 def test_dense_poisson_corrections_match_multigrid(precision):
-    """The per-step solves with the dense Newton corrections (inverse of the
-    discrete operator, POISSON_AUTO) against the multigrid path alone
-    (POISSON_MULTIGRID) on the 0.5 V resistor from the same seed: both meet
-    the stopping rule at every step, the dense path leaves the multigrid
-    solve (almost) nothing to do, and the potentials and ensembles agree to
-    the solver tolerance."""
+    """The per-step solves with Newton corrections from an inverse of the
+    discrete operator against the multigrid path alone (POISSON_MULTIGRID)
+    on the 0.5 V resistor from the same seed.  The resistor's Dirichlet
+    nodes are whole columns, so POISSON_AUTO takes the separable inverse;
+    POISSON_DENSE takes the dense one.  All meet the stopping rule at every
+    step, the corrections leave the multigrid solve (almost) nothing to do
+    (the exact separable inverse needs one correction per step), and the
+    potentials and ensembles agree to the solver tolerance."""
     emcgpu = _emc()
     dev = resistor_device(n_particles=200_000)
-    print("dense corrections vs multigrid, precision", precision, flush=True)
+    print("inverse corrections vs multigrid, precision", precision, flush=True)
     res = {}
-    for mode in (emcgpu.POISSON_MULTIGRID, emcgpu.POISSON_AUTO):
+    for mode in (emcgpu.POISSON_MULTIGRID, emcgpu.POISSON_AUTO, emcgpu.POISSON_DENSE):
         ctx = _ctx(dev=dev, precision=precision, tol=1e-10)
         ctx.device_init(777)
         out = ctx.device_step(0.1e-15, 40, seed=777, poisson_mode=mode)
         res[mode] = (out, ctx.device_fields()["pot"], ctx.device_get_particles())
         ctx.close()
-    mg, auto = res[emcgpu.POISSON_MULTIGRID], res[emcgpu.POISSON_AUTO]
-    for out, _, _ in (mg, auto):
+    mg = res[emcgpu.POISSON_MULTIGRID]
+    for out, _, _ in res.values():
         assert (out["poisson_resid"] <= 1e-10).all()
     # the first step sets the contacts' voltages and solves by multigrid;
-    # afterwards the recorded work is dense corrections + V-cycles
+    # afterwards the recorded work is corrections + V-cycles
     work_mg = mg[0]["poisson_iters"][1:]
-    work_auto = auto[0]["poisson_iters"][1:]
-    print("work per step: multigrid %.2f V-cycles, dense %.2f (max %d)"
-          % (work_mg.mean(), work_auto.mean(), work_auto.max()), flush=True)
-    assert (work_auto <= 2).mean() >= 0.9 and work_auto.mean() < 0.5 * work_mg.mean()
     scale = np.abs(mg[1]).max()
-    assert np.abs(auto[1] - mg[1]).max() < 1e-8 * scale
-    assert abs(int(auto[0]["n_particles"][-1]) - int(mg[0]["n_particles"][-1])) <= 2
+    for mode, name, most in ((emcgpu.POISSON_AUTO, "separable", 1),
+                             (emcgpu.POISSON_DENSE, "dense", 2)):
+        out, pot, _ = res[mode]
+        work = out["poisson_iters"][1:]
+        print("work per step: multigrid %.2f V-cycles, %s %.2f (max %d), residual <= %.1e"
+              % (work_mg.mean(), name, work.mean(), work.max(), out["poisson_resid"][1:].max()),
+              flush=True)
+        assert (work <= most).mean() >= 0.9 and work.mean() < 0.5 * work_mg.mean()
+        assert np.abs(pot - mg[1]).max() < 1e-8 * scale
+        assert abs(int(out["n_particles"][-1]) - int(mg[0]["n_particles"][-1])) <= 2
 
 
 def test_graph_launch_equals_eager():
@@ -389,20 +395,29 @@ def test_self_consistent_run_matches_oracle_twin():
     ctx.close()
 
 
-@pytest.mark.parametrize("mode", ["auto", "multigrid"])
-def test_gated_run_matches_oracle_twin(mode):
-    """The self-consistent loop with two gates: particles reaching a gate
-    are reflected (only the ohmic contacts delete and inject), every solve
+# a gate over the whole YMAX boundary keeps the device separable
+DEVICE_GATED_FULL = dict(DEVICE_SMALL, contacts=DEVICE_SMALL["contacts"] + [
+    (3, 0.3, 0.0, 20e-9, 3.9, 1e-9, 0.3)])
+
+
+@pytest.mark.parametrize("gates,mode", [("partial", "auto"), ("partial", "multigrid"),
+                                        ("full", "auto")])
+def test_gated_run_matches_oracle_twin(gates, mode):
+    """The self-consistent loop with gates: particles reaching a gate are
+    reflected (only the ohmic contacts delete and inject), every solve
     carries the gates' Robin terms.  Against the oracle's Philox twin as
-    above, with the dense corrections (auto) and with multigrid alone."""
+    above: two partial gates by the dense corrections (auto) and by
+    multigrid alone, a gate over a whole boundary by the separable
+    corrections (auto)."""
     emcgpu = _emc()
-    dev = dict(DEVICE_GATED, width=1e-5)
+    dev = dict(DEVICE_GATED if gates == "partial" else DEVICE_GATED_FULL, width=1e-5)
     seed, ns, dt = 777, 50, 1e-15
     od = O.OracleDevice(dev)
     M = O.OracleModel(MODELS_DEVICE["device_para"])
     ref = od.run_philox(M, O.PM_CIC, 1e-14, 1.8, dt, ns, seed, O.thermal_voltage(300.0),
                         max_state=100000)
-    print("gated twin (%s): %d particles, %d steps" % (mode, ref["n_init"][0], ns), flush=True)
+    print("gated twin (%s gates, %s): %d particles, %d steps" % (gates, mode, ref["n_init"][0], ns),
+          flush=True)
     ctx = _ctx(dev=dev, tol=1e-14, max_iter=80)
     assert ctx.device_init(seed) == ref["n_init"][0]
     assert np.abs(ctx.device_fields()["pot"] - ref["eq_pot"]).max() < 1e-9

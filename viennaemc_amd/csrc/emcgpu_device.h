@@ -372,6 +372,26 @@ struct DevFieldArgs {
 // x = G H F
 constexpr int kGreenThreads = 1024;  // apply kernel: 16 waves, one row per wave at a time
 constexpr int kGreenMaxActive = 24576;
+
+// separable devices: the active nodes are the tensor product of mx columns
+// and my rows (Dirichlet nodes fill whole columns or rows) and the gate
+// shifts split into a column and a row part, so A0 = Ax (+) Ay (Kronecker
+// sum of the 1-D operators over the active columns and rows) and
+// A0^-1 = (Bx (x) By) (Lx (+) Ly)^-1 (Px (x) Py), with Ax = Bx Lx Px the
+// eigen-decomposition of each 1-D operator (Px = Bx^-1).  FP64 matrices;
+// the x ones stored transposed so a thread per column reads them coalesced.
+constexpr int kFdThreads = 256;
+struct FdOps {
+  int mx, my;
+  const int *ix, *iy;        // node column / row of each active column / row
+  const double *py;          // [my][my] Py[j][b]
+  const double *pxt;         // [mx][mx] PxT[a][k] = Px[k][a]
+  const double *by;          // [my][my] By[j][l]
+  const double *bxt;         // [mx][mx] BxT[k][a] = Bx[a][k]
+  const double *inv_den;     // [my][mx] 1 / (lx[k] + ly[j])
+  double *t;                 // [my][mx] spectral coefficients between the two launches
+};
+
 struct GreenArgs {
   int nx, ny, nn, ld;          // grid; row stride of G (multiple of 256)
   double hfx, hfy, hp;
@@ -392,6 +412,9 @@ struct GreenArgs {
   double tol;
   const DevState *state;
   int interval;                // solve only when step % interval == 0
+  // 1: corrections by the separable inverse (fd; g holds F), 0: by G (g = H F)
+  int use_fd;
+  FdOps fd;
 };
 
 // particle step of a device ensemble

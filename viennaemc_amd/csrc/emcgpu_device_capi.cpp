@@ -89,6 +89,7 @@ struct DeviceSim {
   std::vector<uint8_t> dmask;
   std::vector<double> dopn, bc_eq, bc_neq, cfac;
   std::vector<double> gate;  // [node][3] Robin terms (empty without gates)
+  std::vector<double> gate_xy;  // [node][2] shift of the x-boundary and y-boundary gates
   std::vector<int> slot_node, slot_contact;
   std::vector<double> slot_exp;
   long long max_inj = 0;  // bound of the injections of one step
@@ -178,6 +179,12 @@ struct DeviceSim {
   float *d_green = nullptr;
   unsigned long long *d_green_flag = nullptr;
   unsigned long long *d_green_acc = nullptr;  // residual checks of a step
+  // separable inverse of the same operator (0: not tried, 1: built, -1: the
+  // device is not separable); fd_use: the current call applies it
+  int fd_state = 0, fd_use = 0;
+  FdOps fd{};
+  int *d_fd_ix = nullptr, *d_fd_iy = nullptr;
+  double *d_fd_mats = nullptr;  // py, pxt, by, bxt, inv_den, t
 };
 
 void dev_free_particles(DeviceSim *d) {
@@ -232,6 +239,9 @@ void device_sim_destroy(DeviceSim *d) {
     dfreet(d->d_green);
     dfreet(d->d_green_flag);
     dfreet(d->d_green_acc);
+    dfreet(d->d_fd_ix);
+    dfreet(d->d_fd_iy);
+    dfreet(d->d_fd_mats);
     dfreet(d->d_slot_node);
     dfreet(d->d_slot_contact);
     dfreet(d->d_slot_kept);
@@ -423,8 +433,11 @@ void derive_nodes(DeviceSim &d) {
   d.hp = d.hp * hn0;
   d.hp = d.hp * hn1;
   d.gate.clear();
-  if (d.has_gate)
+  d.gate_xy.clear();
+  if (d.has_gate) {
     d.gate.assign((size_t)3 * d.nn, 0.0);
+    d.gate_xy.assign((size_t)2 * d.nn, 0.0);
+  }
   d.cnode.assign(d.nn, -1);
   d.cslot.assign(d.nn, -1);
   d.bcx.assign(d.nn, -2);
@@ -463,6 +476,7 @@ void derive_nodes(DeviceSim &d) {
           const double hh = b / 2 == 0 ? d.hfx * hn0 : d.hfy * hn1;
           const double vb = d.gateBarrier[gi] / d.Vt, va = d.contactV[gi] / d.Vt;
           d.gate[3 * (size_t)k] += gf * hh;
+          d.gate_xy[2 * (size_t)k + b / 2] += gf * hh;
           d.gate[3 * (size_t)k + 1] += gf * vb * hh;
           d.gate[3 * (size_t)k + 2] += gf * (vb + va) * hh;
         }
@@ -1211,6 +1225,8 @@ constexpr double kGreenShiftRel = 1e-9;
 // inverted in FP64 with rocSOLVER (Cholesky, potrf + potri), stored FP32
 // as [nn][ld] with zero rows and columns at the Dirichlet nodes.  Any
 // failure leaves the multigrid path alone in charge (green_state -1).
+emcgpu_status green_checks_alloc(emcgpu_ctx *c, DeviceSim &d);
+
 emcgpu_status green_build(emcgpu_ctx *c, DeviceSim &d) {
   if (d.green_state != 0)
     return EMCGPU_OK;
@@ -1290,8 +1306,10 @@ emcgpu_status green_build(emcgpu_ctx *c, DeviceSim &d) {
   ok = rocblas_create_handle(&h) == rocblas_status_success &&
        rocblas_set_stream(h, c->stream) == rocblas_status_success &&
        rocsolver_dpotrf(h, rocblas_fill_lower, na, dA, na, dinfo) == rocblas_status_success &&
+       hipStreamSynchronize(c->stream) == hipSuccess &&
        hipMemcpy(&info, dinfo, sizeof info, hipMemcpyDeviceToHost) == hipSuccess && info == 0 &&
        rocsolver_dpotri(h, rocblas_fill_lower, na, dA, na, dinfo) == rocblas_status_success &&
+       hipStreamSynchronize(c->stream) == hipSuccess &&
        hipMemcpy(&info, dinfo, sizeof info, hipMemcpyDeviceToHost) == hipSuccess && info == 0;
   if (!ok) {
     release();
@@ -1299,15 +1317,7 @@ emcgpu_status green_build(emcgpu_ctx *c, DeviceSim &d) {
   }
   HIPCHK(c, dev_launch_green_expand(dA, na, didx, nn, ld, dG, c->stream));
   HIPCHK(c, dev_green_lds((size_t)ld * sizeof(float)));
-  if (!d.d_green_flag) {
-    HIPCHK(c, hipMalloc((void **)&d.d_green_flag, sizeof(unsigned long long)));
-    HIPCHK(c, hipMemset(d.d_green_flag, 0, sizeof(unsigned long long)));
-  }
-  if (!d.d_green_acc) {
-    const size_t acc_bytes = 2 * (kGreenIters + 1) * sizeof(unsigned long long);
-    HIPCHK(c, hipMalloc((void **)&d.d_green_acc, acc_bytes));
-    HIPCHK(c, hipMemset(d.d_green_acc, 0, acc_bytes));
-  }
+  ST(green_checks_alloc(c, d));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   d.d_green = dG;
   dG = nullptr;
@@ -1345,8 +1355,217 @@ GreenArgs green_args(const DeviceSim &d) {
   g.tol = d.tol;
   g.state = d.d_state;
   g.interval = d.interval;
+  g.use_fd = d.fd_use;
+  if (d.fd_use)
+    g.fd = d.fd;
   return g;
 }
+
+// the residual checks of the corrections (shared by both inverses)
+emcgpu_status green_checks_alloc(emcgpu_ctx *c, DeviceSim &d) {
+  if (!d.d_green_flag) {
+    HIPCHK(c, hipMalloc((void **)&d.d_green_flag, sizeof(unsigned long long)));
+    HIPCHK(c, hipMemset(d.d_green_flag, 0, sizeof(unsigned long long)));
+  }
+  if (!d.d_green_acc) {
+    const size_t acc_bytes = 2 * (kGreenIters + 1) * sizeof(unsigned long long);
+    HIPCHK(c, hipMalloc((void **)&d.d_green_acc, acc_bytes));
+    HIPCHK(c, hipMemset(d.d_green_acc, 0, acc_bytes));
+  }
+  return EMCGPU_OK;
+}
+
+// The separable inverse.  The Newton operator without the hole term,
+// A0 = -J0, acts per dimension: hF_d (2 x_k - x_prev - x_next) with the
+// mirrored neighbour at a Neumann boundary and Dirichlet neighbours dropped,
+// plus the gates' shifts.  When the Dirichlet nodes are exactly the union of
+// whole columns and whole rows, and the shift of the x-boundary gates
+// depends only on the row and that of the y-boundary gates only on the
+// column, A0 is the Kronecker sum Ax (+) Ay of two 1-D operators over the
+// active columns and rows.  Each 1-D operator is symmetric under the
+// control-volume weights w (1/2 at a boundary node): S = W^1/2 A W^-1/2 =
+// Q L Q^T (rocSOLVER dsyevd, FP64), so A = B L P with B = W^-1/2 Q and
+// P = Q^T W^1/2 = B^-1.  Not separable, not positive definite, or any
+// failure: fd_state -1 (the dense inverse or multigrid then takes over).
+emcgpu_status fd_build(emcgpu_ctx *c, DeviceSim &d) {
+  if (d.fd_state != 0)
+    return EMCGPU_OK;
+  d.fd_state = -1;
+  const int nx = d.ext[0], ny = d.ext[1];
+  std::vector<uint8_t> colD(nx, 1), rowD(ny, 1);
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++)
+      if (!d.dmask[idx(d, i, j)]) {
+        colD[i] = 0;
+        rowD[j] = 0;
+      }
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++)
+      if ((d.dmask[idx(d, i, j)] != 0) != (colD[i] || rowD[j]))
+        return EMCGPU_OK;  // a Dirichlet node set that is no union of lines
+  std::vector<int> ix, iy;
+  for (int i = 0; i < nx; i++)
+    if (!colD[i])
+      ix.push_back(i);
+  for (int j = 0; j < ny; j++)
+    if (!rowD[j])
+      iy.push_back(j);
+  const int mx = (int)ix.size(), my = (int)iy.size();
+  if (mx == 0 || my == 0)
+    return EMCGPU_OK;
+  // gate shifts: the x part must not vary along y, the y part not along x
+  std::vector<double> gx(nx, 0.0), gy(ny, 0.0);
+  if (d.has_gate) {
+    for (int i : ix)
+      gx[i] = d.gate_xy[2 * (size_t)idx(d, i, iy[0])];
+    for (int j : iy)
+      gy[j] = d.gate_xy[2 * (size_t)idx(d, ix[0], j) + 1];
+    for (int j : iy)
+      for (int i : ix) {
+        const size_t k = (size_t)idx(d, i, j);
+        if (d.gate_xy[2 * k] != gx[i] || d.gate_xy[2 * k + 1] != gy[j])
+          return EMCGPU_OK;
+      }
+  }
+  // S of one dimension (n nodes, active list act, Dirichlet lines dl, h-factor
+  // hf, shifts g), column-major, with the weights' square roots in sw
+  auto build_1d = [](int n, const std::vector<int> &act, const std::vector<uint8_t> &dl,
+                     double hf, const std::vector<double> &g, std::vector<double> &S,
+                     std::vector<double> &sw) {
+    const int m = (int)act.size();
+    std::vector<int> pos(n, -1);
+    for (int a = 0; a < m; a++)
+      pos[act[a]] = a;
+    std::vector<double> A((size_t)m * m, 0.0);
+    sw.assign(m, 1.0);
+    for (int a = 0; a < m; a++) {
+      const int i = act[a];
+      A[(size_t)a * m + a] += 2.0 * hf + g[i];
+      const int nb[2] = {i > 0 ? i - 1 : i + 1, i < n - 1 ? i + 1 : i - 1};
+      for (int q = 0; q < 2; q++)
+        if (!dl[nb[q]])
+          A[(size_t)a * m + pos[nb[q]]] -= hf;
+      sw[a] = (i == 0 || i == n - 1) ? std::sqrt(0.5) : 1.0;
+    }
+    S.assign((size_t)m * m, 0.0);
+    for (int a = 0; a < m; a++)
+      for (int b = 0; b < m; b++)  // row a, column b: stored at [b * m + a]
+        S[(size_t)b * m + a] = sw[a] * A[(size_t)a * m + b] / sw[b];
+    for (int a = 0; a < m; a++)  // symmetric up to rounding: take the mean
+      for (int b = a + 1; b < m; b++) {
+        const double v = 0.5 * (S[(size_t)b * m + a] + S[(size_t)a * m + b]);
+        S[(size_t)b * m + a] = S[(size_t)a * m + b] = v;
+      }
+  };
+  std::vector<double> Sx, Sy, swx, swy;
+  build_1d(nx, ix, colD, d.hfx, gx, Sx, swx);
+  build_1d(ny, iy, rowD, d.hfy, gy, Sy, swy);
+  const std::vector<double> Sx0 = Sx, Sy0 = Sy;
+  // the eigenpairs must solve S q = l q (S is tridiagonal: O(m^2) check)
+  auto eig_ok = [](const std::vector<double> &S, const std::vector<double> &Q,
+                   const std::vector<double> &lam, int m) {
+    double smax = 0.0;
+    for (double v : lam)
+      smax = std::max(smax, std::fabs(v));
+    for (int k = 0; k < m; k++)
+      for (int a = 0; a < m; a++) {
+        double r = -lam[k] * Q[(size_t)k * m + a];
+        for (int b = std::max(0, a - 1); b <= std::min(m - 1, a + 1); b++)
+          r += S[(size_t)b * m + a] * Q[(size_t)k * m + b];
+        if (!(std::fabs(r) <= 1e-10 * smax))
+          return false;
+      }
+    return true;
+  };
+  // eigen-decompositions on the device
+  std::vector<double> lx(mx), ly(my);
+  {
+    double *dA = nullptr, *dD = nullptr, *dE = nullptr;
+    rocblas_int *dinfo = nullptr;
+    rocblas_handle h = nullptr;
+    const int mm = std::max(mx, my);
+    bool ok = hipMalloc((void **)&dA, (size_t)mm * mm * sizeof(double)) == hipSuccess &&
+              hipMalloc((void **)&dD, (size_t)mm * sizeof(double)) == hipSuccess &&
+              hipMalloc((void **)&dE, (size_t)mm * sizeof(double)) == hipSuccess &&
+              hipMalloc((void **)&dinfo, sizeof(rocblas_int)) == hipSuccess &&
+              rocblas_create_handle(&h) == rocblas_status_success &&
+              rocblas_set_stream(h, c->stream) == rocblas_status_success;
+    auto eig = [&](std::vector<double> &S, int m, std::vector<double> &lam) {
+      rocblas_int info = 1;
+      return hipMemcpy(dA, S.data(), (size_t)m * m * sizeof(double), hipMemcpyHostToDevice) ==
+                 hipSuccess &&
+             rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, m, dA, m, dD, dE,
+                              dinfo) == rocblas_status_success &&
+             // the context's stream does not order against the null-stream copies
+             hipStreamSynchronize(c->stream) == hipSuccess &&
+             hipMemcpy(&info, dinfo, sizeof info, hipMemcpyDeviceToHost) == hipSuccess &&
+             info == 0 &&
+             hipMemcpy(S.data(), dA, (size_t)m * m * sizeof(double), hipMemcpyDeviceToHost) ==
+                 hipSuccess &&
+             hipMemcpy(lam.data(), dD, (size_t)m * sizeof(double), hipMemcpyDeviceToHost) ==
+                 hipSuccess;
+    };
+    ok = ok && eig(Sx, mx, lx) && eig(Sy, my, ly);
+    dfreet(dA);
+    dfreet(dD);
+    dfreet(dE);
+    dfreet(dinfo);
+    if (h)
+      (void)rocblas_destroy_handle(h);
+    (void)hipGetLastError();  // a failure here only disables the path
+    if (!ok)
+      return EMCGPU_OK;
+  }
+  if (!eig_ok(Sx0, Sx, lx, mx) || !eig_ok(Sy0, Sy, ly, my))
+    return EMCGPU_OK;
+  // positive definite (the smallest eigenvalue of A0 is lx[0] + ly[0])
+  if (!(lx[0] + ly[0] > 1e-12 * (lx[mx - 1] + ly[my - 1])))
+    return EMCGPU_OK;
+  // Q (column-major: Q[a][k] at [k * m + a]) -> the kernels' matrices
+  const size_t nxx = (size_t)mx * mx, nyy = (size_t)my * my, nyx = (size_t)my * mx;
+  std::vector<double> mats(2 * nyy + 2 * nxx + 2 * nyx, 0.0);
+  double *py = mats.data(), *by = py + nyy, *pxt = by + nyy, *bxt = pxt + nxx,
+         *inv_den = bxt + nxx;
+  for (int j = 0; j < my; j++)
+    for (int b = 0; b < my; b++) {
+      const double q = Sy[(size_t)j * my + b];  // Q[b][j]
+      py[(size_t)j * my + b] = q * swy[b];      // Py[j][b] = Q[b][j] sqrt(w_b)
+      by[(size_t)b * my + j] = q / swy[b];      // By[b][j] = Q[b][j] / sqrt(w_b)
+    }
+  for (int k = 0; k < mx; k++)
+    for (int a = 0; a < mx; a++) {
+      const double q = Sx[(size_t)k * mx + a];  // Q[a][k]
+      pxt[(size_t)a * mx + k] = q * swx[a];     // Px[k][a]
+      bxt[(size_t)k * mx + a] = q / swx[a];     // Bx[a][k]
+    }
+  for (int j = 0; j < my; j++)
+    for (int k = 0; k < mx; k++)
+      inv_den[(size_t)j * mx + k] = 1.0 / (lx[k] + ly[j]);
+  ST(green_checks_alloc(c, d));
+  HIPCHK(c, hipMalloc((void **)&d.d_fd_mats, mats.size() * sizeof(double)));
+  HIPCHK(c, hipMemcpy(d.d_fd_mats, mats.data(), mats.size() * sizeof(double),
+                      hipMemcpyHostToDevice));
+  HIPCHK(c, hipMalloc((void **)&d.d_fd_ix, (size_t)mx * sizeof(int)));
+  HIPCHK(c, hipMalloc((void **)&d.d_fd_iy, (size_t)my * sizeof(int)));
+  HIPCHK(c, hipMemcpy(d.d_fd_ix, ix.data(), (size_t)mx * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(d.d_fd_iy, iy.data(), (size_t)my * sizeof(int), hipMemcpyHostToDevice));
+  d.fd.mx = mx;
+  d.fd.my = my;
+  d.fd.ix = d.d_fd_ix;
+  d.fd.iy = d.d_fd_iy;
+  d.fd.py = d.d_fd_mats;
+  d.fd.by = d.fd.py + nyy;
+  d.fd.pxt = d.fd.by + nyy;
+  d.fd.bxt = d.fd.pxt + nxx;
+  d.fd.inv_den = d.fd.bxt + nxx;
+  d.fd.t = d.d_fd_mats + (mats.size() - nyx);
+  d.fd_state = 1;
+  return EMCGPU_OK;
+}
+
+// corrections per step: the separable inverse is exact up to the hole term
+// (one correction meets the stopping rule), the FP32 dense one takes two
+inline int green_iters(const DeviceSim &d) { return d.fd_use ? 1 : kGreenIters; }
 
 // grids whose contact nodes copy an inner node's field (the E-field kernel
 // over all CUs forms it directly)
@@ -1368,7 +1587,7 @@ DevFieldArgs<R> step_field_args(const emcgpu_ctx *c, const DeviceSim &d, bool re
   if (split_efield(d))
     f.do_efield = 0;
   if (d.green_use && !reset_bc) {
-    f.green_iters = kGreenIters;
+    f.green_iters = green_iters(d);
     f.green_flag = d.d_green_flag;
     f.green_acc = d.d_green_acc;
   }
@@ -1379,7 +1598,7 @@ DevFieldArgs<R> step_field_args(const emcgpu_ctx *c, const DeviceSim &d, bool re
 template <typename R>
 emcgpu_status launch_step(emcgpu_ctx *c, DeviceSim &d, bool reset_bc) {
   if (d.green_use && !reset_bc)
-    HIPCHK(c, dev_launch_green(green_args(d), kGreenIters, d.green_blocks, c->stream));
+    HIPCHK(c, dev_launch_green(green_args(d), green_iters(d), d.green_blocks, c->stream));
   HIPCHK(c, dev_launch_field<R>(fargs_slot<R>(d, reset_bc ? 1 : 0), d.mg_lds, c->stream));
   if (split_efield(d))
     HIPCHK(c, dev_launch_efield<R>(fargs_slot<R>(d, reset_bc ? 1 : 0), (int)d.nn, c->stream));
@@ -1420,11 +1639,17 @@ emcgpu_status run_device_steps(emcgpu_ctx *c, DeviceSim &d, const emcgpu_device_
     d.chan_x0 = d.chan_x1 = d.chan_scale = 0;
   }
   d.accumulate = s->accumulate ? 1 : 0;
-  if (s->poisson_mode != EMCGPU_POISSON_AUTO && s->poisson_mode != EMCGPU_POISSON_MULTIGRID)
-    return fail(c, EMCGPU_ERR_ARG, "poisson_mode must be EMCGPU_POISSON_AUTO or _MULTIGRID");
+  if (s->poisson_mode != EMCGPU_POISSON_AUTO && s->poisson_mode != EMCGPU_POISSON_MULTIGRID &&
+      s->poisson_mode != EMCGPU_POISSON_DENSE)
+    return fail(c, EMCGPU_ERR_ARG,
+                "poisson_mode must be EMCGPU_POISSON_AUTO, _MULTIGRID or _DENSE");
   if (s->poisson_mode == EMCGPU_POISSON_AUTO)
+    ST(fd_build(c, d));
+  d.fd_use = (s->poisson_mode == EMCGPU_POISSON_AUTO && d.fd_state == 1) ? 1 : 0;
+  if (s->poisson_mode == EMCGPU_POISSON_DENSE ||
+      (s->poisson_mode == EMCGPU_POISSON_AUTO && !d.fd_use))
     ST(green_build(c, d));
-  d.green_use = (s->poisson_mode == EMCGPU_POISSON_AUTO && d.green_state == 1) ? 1 : 0;
+  d.green_use = d.fd_use || (s->poisson_mode != EMCGPU_POISSON_MULTIGRID && d.green_state == 1);
   ST(ensure_hist(c, d, s->n_steps));
   const size_t nc = std::max(d.nContacts, 1);
   HIPCHK(c, dev_launch_zero_i(d.d_hexits, (long long)s->n_steps * nc, c->stream));
@@ -1457,7 +1682,7 @@ emcgpu_status run_device_steps(emcgpu_ctx *c, DeviceSim &d, const emcgpu_device_
     if (s->use_graph && k == kChunk) {
       if (!d.gexec || d.graph_cur != d.cur || d.graph_cap != d.capacity ||
           d.graph_chan != d.chan || d.graph_acc != d.accumulate ||
-          d.graph_green != d.green_use || d.graph_interval != d.interval ||
+          d.graph_green != d.green_use + d.fd_use || d.graph_interval != d.interval ||
           d.graph_x0 != d.chan_x0 ||
           d.graph_x1 != d.chan_x1 || d.graph_scale != d.chan_scale) {
         if (d.gexec) {
@@ -1482,7 +1707,7 @@ emcgpu_status run_device_steps(emcgpu_ctx *c, DeviceSim &d, const emcgpu_device_
         d.graph_cap = d.capacity;
         d.graph_chan = d.chan;
         d.graph_acc = d.accumulate;
-        d.graph_green = d.green_use;
+        d.graph_green = d.green_use + d.fd_use;
         d.graph_interval = d.interval;
         d.graph_x0 = d.chan_x0;
         d.graph_x1 = d.chan_x1;

@@ -1885,7 +1885,7 @@ __global__ __launch_bounds__(256) void k_green_res(GreenArgs a, int c) {
     const double gt = a.gate ? a.gate[3 * k + 2] - a.gate[3 * k] * phi : 0.0;
     const double F = wk * (a.hfx * (pot[il] + pot[ir]) + a.hfy * (pot[jd] + pot[ju]) -
                            2.0 * (a.hfx + a.hfy) * phi + a.hp * (p - n + C) + gt);
-    a.g[k] = (double)a.hw0[k] * F;
+    a.g[k] = a.use_fd ? F : (double)a.hw0[k] * F;
     af = fabs(F);
     sc = wk * a.hp * (fabs(C) + n + p);
     if (wk != 0.0 && a.hp * p > a.shift_max)
@@ -1982,12 +1982,131 @@ hipError_t dev_green_lds(size_t lds) {
 
 // residual check, correction (skipped once a check meets the stopping
 // rule), ..., and a last check after the last correction
+// Newton correction by the separable inverse.  Both launches run a block per
+// (active row j, chunk of kFdChunk columns): the block first forms a whole
+// row of the y transform in LDS (each thread one column, four independent
+// partial sums), then each lane owns one output column of the chunk and the
+// four waves split the x sum into quarters (four partial sums each), so a
+// thread waits on few dependent L2 round trips.
+constexpr int kFdChunk = 64;
+
+// t[j][k] = sum_q Px[k][q] (sum_b Py[j][b] F[b][q]) / (lx[k] + ly[j])
+__global__ __launch_bounds__(kFdThreads) void k_fd_fwd(GreenArgs a, int c) {
+  extern __shared__ double fd_lds[];  // [mx] row, [waves][kFdChunk] partial sums
+  if (a.interval > 1 && a.state->step % a.interval != 0)
+    return;
+  if (*a.flag == (unsigned long long)a.state->step + 1ull || green_converged(a, c))
+    return;
+  const FdOps &f = a.fd;
+  const int j = blockIdx.y, mx = f.mx, my = f.my, nx = a.nx;
+  double *row = fd_lds, *part = fd_lds + mx;
+  const double *py = f.py + (size_t)j * my;
+  for (int q = threadIdx.x; q < mx; q += kFdThreads) {
+    const double *gc = a.g + f.ix[q];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int b = 0;
+    for (; b + 4 <= my; b += 4) {
+      s0 = fma(py[b], gc[(size_t)f.iy[b] * nx], s0);
+      s1 = fma(py[b + 1], gc[(size_t)f.iy[b + 1] * nx], s1);
+      s2 = fma(py[b + 2], gc[(size_t)f.iy[b + 2] * nx], s2);
+      s3 = fma(py[b + 3], gc[(size_t)f.iy[b + 3] * nx], s3);
+    }
+    for (; b < my; ++b)
+      s0 = fma(py[b], gc[(size_t)f.iy[b] * nx], s0);
+    row[q] = (s0 + s1) + (s2 + s3);
+  }
+  __syncthreads();
+  const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+  const int k = (int)blockIdx.x * kFdChunk + lane;
+  constexpr int kWaves = kFdThreads / 64;
+  const int span = (mx + kWaves - 1) / kWaves, q0 = w * span, q1 = min(mx, q0 + span);
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  if (k < mx) {
+    const double *pc = f.pxt + k;
+    int q = q0;
+    for (; q + 4 <= q1; q += 4) {
+      s0 = fma(pc[(size_t)q * mx], row[q], s0);
+      s1 = fma(pc[(size_t)(q + 1) * mx], row[q + 1], s1);
+      s2 = fma(pc[(size_t)(q + 2) * mx], row[q + 2], s2);
+      s3 = fma(pc[(size_t)(q + 3) * mx], row[q + 3], s3);
+    }
+    for (; q < q1; ++q)
+      s0 = fma(pc[(size_t)q * mx], row[q], s0);
+  }
+  part[w * kFdChunk + lane] = (s0 + s1) + (s2 + s3);
+  __syncthreads();
+  if (w == 0 && k < mx) {
+    const double t = (part[lane] + part[kFdChunk + lane]) +
+                     (part[2 * kFdChunk + lane] + part[3 * kFdChunk + lane]);
+    f.t[(size_t)j * mx + k] = t * f.inv_den[(size_t)j * mx + k];
+  }
+}
+
+// pot[j][q] += sum_k Bx[q][k] (sum_l By[j][l] t[l][k])
+__global__ __launch_bounds__(kFdThreads) void k_fd_bwd(GreenArgs a, int c) {
+  extern __shared__ double fd_lds[];  // [mx] row, [waves][kFdChunk] partial sums
+  if (a.interval > 1 && a.state->step % a.interval != 0)
+    return;
+  if (*a.flag == (unsigned long long)a.state->step + 1ull || green_converged(a, c))
+    return;
+  const FdOps &f = a.fd;
+  const int j = blockIdx.y, mx = f.mx, my = f.my;
+  double *row = fd_lds, *part = fd_lds + mx;
+  const double *by = f.by + (size_t)j * my;
+  for (int k = threadIdx.x; k < mx; k += kFdThreads) {
+    const double *tc = f.t + k;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int l = 0;
+    for (; l + 4 <= my; l += 4) {
+      s0 = fma(by[l], tc[(size_t)l * mx], s0);
+      s1 = fma(by[l + 1], tc[(size_t)(l + 1) * mx], s1);
+      s2 = fma(by[l + 2], tc[(size_t)(l + 2) * mx], s2);
+      s3 = fma(by[l + 3], tc[(size_t)(l + 3) * mx], s3);
+    }
+    for (; l < my; ++l)
+      s0 = fma(by[l], tc[(size_t)l * mx], s0);
+    row[k] = (s0 + s1) + (s2 + s3);
+  }
+  __syncthreads();
+  const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+  const int q = (int)blockIdx.x * kFdChunk + lane;
+  constexpr int kWaves = kFdThreads / 64;
+  const int span = (mx + kWaves - 1) / kWaves, k0 = w * span, k1 = min(mx, k0 + span);
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  if (q < mx) {
+    const double *bc = f.bxt + q;
+    int k = k0;
+    for (; k + 4 <= k1; k += 4) {
+      s0 = fma(bc[(size_t)k * mx], row[k], s0);
+      s1 = fma(bc[(size_t)(k + 1) * mx], row[k + 1], s1);
+      s2 = fma(bc[(size_t)(k + 2) * mx], row[k + 2], s2);
+      s3 = fma(bc[(size_t)(k + 3) * mx], row[k + 3], s3);
+    }
+    for (; k < k1; ++k)
+      s0 = fma(bc[(size_t)k * mx], row[k], s0);
+  }
+  part[w * kFdChunk + lane] = (s0 + s1) + (s2 + s3);
+  __syncthreads();
+  if (w == 0 && q < mx) {
+    const double x = (part[lane] + part[kFdChunk + lane]) +
+                     (part[2 * kFdChunk + lane] + part[3 * kFdChunk + lane]);
+    a.pot[(size_t)f.iy[j] * a.nx + f.ix[q]] += x;
+  }
+}
+
 hipError_t dev_launch_green(const GreenArgs &a, int iters, int blocks, hipStream_t st) {
   const unsigned rb = (unsigned)((a.nn + 255) / 256);
   for (int it = 0; it < iters; ++it) {
     hipLaunchKernelGGL(k_green_res, dim3(rb), dim3(256), 0, st, a, it);
-    hipLaunchKernelGGL(k_green_apply, dim3((unsigned)blocks), dim3(kGreenThreads),
-                       (size_t)a.ld * sizeof(float), st, a, it);
+    if (a.use_fd) {
+      const size_t lds = (size_t)(a.fd.mx + kFdThreads) * sizeof(double);
+      const dim3 grid((unsigned)((a.fd.mx + kFdChunk - 1) / kFdChunk), (unsigned)a.fd.my);
+      hipLaunchKernelGGL(k_fd_fwd, grid, dim3(kFdThreads), lds, st, a, it);
+      hipLaunchKernelGGL(k_fd_bwd, grid, dim3(kFdThreads), lds, st, a, it);
+    } else {
+      hipLaunchKernelGGL(k_green_apply, dim3((unsigned)blocks), dim3(kGreenThreads),
+                         (size_t)a.ld * sizeof(float), st, a, it);
+    }
   }
   hipLaunchKernelGGL(k_green_res, dim3(rb), dim3(256), 0, st, a, iters);
   return hipGetLastError();

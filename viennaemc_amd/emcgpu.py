@@ -129,7 +129,7 @@ class DeviceStepArgs(C.Structure):
                 ("accumulate", C.c_int), ("poisson_mode", C.c_int)]
 
 
-POISSON_AUTO, POISSON_MULTIGRID = 0, 1
+POISSON_AUTO, POISSON_MULTIGRID, POISSON_DENSE = 0, 1, 2
 
 
 TUNE_MAX = 32
@@ -809,7 +809,9 @@ class Context:
                     accumulate=False, poisson_mode=POISSON_AUTO):
         """emcgpu_device_step; channel = (x0, x1, length) [m] records the
         drift-velocity channel current of every step (key "channel_current");
-        poisson_mode POISSON_MULTIGRID solves without the dense corrections."""
+        poisson_mode POISSON_MULTIGRID solves without the Newton corrections from an
+        inverse, POISSON_DENSE takes them from the dense inverse even where the
+        separable one applies."""
         nc = max(self.n_contacts, 1)
         out = dict(exits=np.zeros((n_steps, nc), dtype=np.int32),
                    deleted=np.zeros((n_steps, nc), dtype=np.int32),
