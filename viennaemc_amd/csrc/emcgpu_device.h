@@ -229,8 +229,18 @@ struct StepArgs {
 // 2-D device simulation
 // ===========================================================================
 constexpr int kDevThreads = 1024;      // field solver: one workgroup
-constexpr int kDevStepThreads = 512;   // particle step: 8 waves per block
+constexpr int kDevStepThreads = 512;   // contact ranks: 8 waves (segments) per block
 constexpr int kDevStepWaves = kDevStepThreads / 64;
+constexpr int kDevSegPerCU = 16;       // contact-rank segments per CU
+// particle step: one block per CU whose waves share the CU's chunks through
+// an LDS counter; FP32 runs 16 waves of 4 particles per lane, FP64 (twice
+// the registers) 8 waves of 2
+template <typename R>
+struct DevStepShape {
+  static constexpr int ppt = sizeof(R) == 4 ? 4 : 2;
+  static constexpr int threads = sizeof(R) == 4 ? 1024 : 512;
+  static constexpr int waves = threads / 64;
+};
 constexpr uint32_t kHole = 0xFFFFFFFFu;  // valley/subvalley word of a deleted particle
 constexpr int kMaxLevels = 12;
 constexpr int kPad = 4 * kDevThreads;  // node padding of the Poisson vectors
@@ -254,7 +264,6 @@ struct DevState {
   long long inj_total;  // injections of the current step (contact scan)
   long long capacity;
   int overflow;
-  unsigned int ticket;  // contact-scan completion counter
   long long avg_n;      // steps added to the running potential / concentration sums
 };
 
@@ -380,7 +389,7 @@ constexpr int kGreenMaxActive = 24576;
 // A0^-1 = (Bx (x) By) (Lx (+) Ly)^-1 (Px (x) Py), with Ax = Bx Lx Px the
 // eigen-decomposition of each 1-D operator (Px = Bx^-1).  FP64 matrices;
 // the x ones stored transposed so a thread per column reads them coalesced.
-constexpr int kFdThreads = 256;
+constexpr int kFdThreads = 512;
 struct FdOps {
   int mx, my;
   const int *ix, *iy;        // node column / row of each active column / row
@@ -429,9 +438,11 @@ struct DevStepArgs {
   uint32_t key0, key1;
   int n_contacts, n_slots;
   R *dep;                 // [block][node] deposits of the step kernel
-  uint32_t *seg_cnt;      // [slot][segment] contact particles per segment
+  uint32_t *seg_cnt;      // [segment][slot] contact particles per segment
   uint32_t *seg_nst;      // [segment] staged entries
   uint2 *stage;           // (particle, slot << 22 | rank in segment) at segment start
+  int16_t *pslot;         // [particle] ohmic slot of the position after the move, -1 if none
+  int *exit_part;         // [block][contact] particles that left through each contact
   int lds_dep;            // deposit grid in LDS (else straight to dep row 0 with atomics)
   int move;              // 0: classify positions only (contact rule of given positions)
   // drift-velocity channel current: sum of v_x over the moved particles
@@ -471,7 +482,9 @@ struct DevContactArgs {
   const int *slot_node;    // grid node of each slot
   const int *slot_contact;
   const double *slot_exp;  // expected carriers at the node
-  int *slot_kept, *slot_inj;
+  int *slot_kept, *slot_inj, *slot_total;
+  const int *exit_part;    // [step block][contact] exits of the step kernel's blocks
+  int n_step_blocks;
   double *corr;            // [node] deposit corrections (FP64 atomics)
   R *dep;
   double *dpart;           // [n_chunks][node] partial sums of the block grids

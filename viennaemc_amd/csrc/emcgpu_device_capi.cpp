@@ -26,7 +26,8 @@ hipError_t dev_launch_field(const DevFieldArgs<R> *args_dev, size_t lds, hipStre
 template <typename R>
 hipError_t dev_launch_efield(const DevFieldArgs<R> *args_dev, int nn, hipStream_t st);
 template <typename R>
-hipError_t dev_launch_step(const DevStepArgs<R> &a, int blocks, size_t lds, hipStream_t st);
+hipError_t dev_launch_step(const DevStepArgs<R> &a, int blocks, int n_seg, size_t lds,
+                           hipStream_t st);
 template <typename R>
 hipError_t dev_launch_drift(const DevStepArgs<R> &a, int mode, hipStream_t st);
 template <typename R>
@@ -123,7 +124,8 @@ struct DeviceSim {
   void *d_fargs = nullptr;  // device copies of the field-kernel arguments [slot]
   int n_pad = 0;
   int *d_slot_node = nullptr, *d_slot_contact = nullptr, *d_slot_kept = nullptr;
-  int *d_slot_inj = nullptr;
+  int *d_slot_inj = nullptr, *d_slot_total = nullptr;
+  int *d_exit_part = nullptr;  // [step block][contact]
   double *d_slot_exp = nullptr;
   DimTab *d_tab = nullptr;
   uint8_t *d_lmask = nullptr;
@@ -139,6 +141,7 @@ struct DeviceSim {
        *ptau[2] = {};
   uint32_t *pvs[2] = {}, *ppid[2] = {};
   uint2 *d_stage = nullptr;
+  int16_t *d_pslot = nullptr;
   int *d_blk = nullptr;
   // step kernel
   int blocks = 0, n_seg = 0;
@@ -200,6 +203,7 @@ void dev_free_particles(DeviceSim *d) {
     dfreet(d->ppid[b]);
   }
   dfreet(d->d_stage);
+  dfreet(d->d_pslot);
   dfreet(d->d_blk);
   d->capacity = 0;
 }
@@ -246,6 +250,8 @@ void device_sim_destroy(DeviceSim *d) {
     dfreet(d->d_slot_contact);
     dfreet(d->d_slot_kept);
     dfreet(d->d_slot_inj);
+    dfreet(d->d_slot_total);
+    dfreet(d->d_exit_part);
     dfreet(d->d_slot_exp);
     dfreet(d->d_tab);
     dfreet(d->d_lmask);
@@ -802,6 +808,7 @@ emcgpu_status upload_device(emcgpu_ctx *c, DeviceSim &d) {
   const size_t ns = std::max<size_t>(d.slot_node.size(), 1);
   ST(dalloc(c, d.d_slot_kept, ns));
   ST(dalloc(c, d.d_slot_inj, ns));
+  ST(dalloc(c, d.d_slot_total, ns));
   ST(dalloc(c, d.d_pot, nn));
   ST(dalloc(c, d.d_conc, nn));
   ST(dalloc(c, d.d_nrpart, nn));
@@ -849,19 +856,20 @@ emcgpu_status upload_device(emcgpu_ctx *c, DeviceSim &d) {
   HIPCHK(c, hipMemset(d.d_acc, 0, 2 * nn * rs));
   DevState s{};
   HIPCHK(c, hipMemcpy(d.d_state, &s, sizeof s, hipMemcpyHostToDevice));
-  // step kernel: a block-private deposit grid when it fits the LDS
-  const size_t cnt_bytes = sizeof(uint32_t) * kDevStepWaves * ns;
+  // step kernel: a block-private deposit grid when it fits the LDS (the
+  // rank kernel holds the per-wave slot counts, at most 32 KB)
   const size_t grid_bytes = ((nn * rs) + 15) & ~(size_t)15;
-  d.lds_dep = grid_bytes + cnt_bytes <= (size_t)kDevMaxLds;
-  d.step_lds = (d.lds_dep ? grid_bytes : 0) + cnt_bytes;
-  if (d.step_lds > (size_t)kDevMaxLds)
-    return fail(c, EMCGPU_ERR_ARG, "too many ohmic nodes (%zu) for the step kernel", ns);
+  d.lds_dep = grid_bytes <= (size_t)kDevMaxLds;
+  d.step_lds = d.lds_dep ? grid_bytes : 0;
   hipDeviceProp_t prop;
   HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
+  // one step block per CU (its waves share the CU's chunks)
   const int per_cu = c->precision == EMCGPU_FP32 ? dev_step_blocks_per_cu<float>(d.step_lds)
                                                  : dev_step_blocks_per_cu<double>(d.step_lds);
-  d.blocks = prop.multiProcessorCount * per_cu;
-  d.n_seg = d.blocks * kDevStepWaves;
+  if (per_cu < 1)
+    return fail(c, EMCGPU_ERR_ARG, "the step kernel does not fit a CU");
+  d.blocks = prop.multiProcessorCount;
+  d.n_seg = prop.multiProcessorCount * kDevSegPerCU;
   const size_t dep_rows = d.lds_dep ? (size_t)d.blocks : 1;
   ST(dallocv(c, d.d_dep, dep_rows * nn * rs));
   d.rows_per_chunk = 16;
@@ -872,6 +880,7 @@ emcgpu_status upload_device(emcgpu_ctx *c, DeviceSim &d) {
   ST(dalloc(c, d.d_seg_off, ns * d.n_seg));
   ST(dalloc(c, d.d_seg_nst, d.n_seg));
   ST(dalloc(c, d.d_chan_part, (size_t)d.blocks));
+  ST(dalloc(c, d.d_exit_part, (size_t)d.blocks * std::max(d.nContacts, 1)));
   ST(dalloc(c, d.d_chan_corr, 1));
   ST(dalloc(c, d.d_avg_pot, nn));
   ST(dalloc(c, d.d_avg_conc, nn));
@@ -971,6 +980,7 @@ emcgpu_status ensure_capacity(emcgpu_ctx *c, DeviceSim &d, long long need, long 
     d.ppid[b] = np;
   }
   ST(dalloc(c, d.d_stage, cap));
+  ST(dalloc(c, d.d_pslot, cap));
   ST(dalloc(c, d.d_blk, cap / 1024 + 2));
   d.capacity = cap;
   DevState s{};
@@ -1108,6 +1118,8 @@ DevStepArgs<R> step_args(const emcgpu_ctx *c, const DeviceSim &d) {
   a.seg_cnt = d.d_seg_cnt;
   a.seg_nst = d.d_seg_nst;
   a.stage = d.d_stage;
+  a.pslot = d.d_pslot;
+  a.exit_part = d.d_exit_part;
   a.lds_dep = d.lds_dep ? 1 : 0;
   a.move = 1;
   a.chan = d.chan;
@@ -1141,6 +1153,9 @@ DevContactArgs<R> contact_args(const emcgpu_ctx *c, const DeviceSim &d) {
   a.slot_exp = d.d_slot_exp;
   a.slot_kept = d.d_slot_kept;
   a.slot_inj = d.d_slot_inj;
+  a.slot_total = d.d_slot_total;
+  a.exit_part = d.d_exit_part;
+  a.n_step_blocks = d.blocks;
   a.corr = d.d_corr;
   a.dep = (R *)d.d_dep;
   a.dpart = d.d_dpart;
@@ -1602,7 +1617,7 @@ emcgpu_status launch_step(emcgpu_ctx *c, DeviceSim &d, bool reset_bc) {
   HIPCHK(c, dev_launch_field<R>(fargs_slot<R>(d, reset_bc ? 1 : 0), d.mg_lds, c->stream));
   if (split_efield(d))
     HIPCHK(c, dev_launch_efield<R>(fargs_slot<R>(d, reset_bc ? 1 : 0), (int)d.nn, c->stream));
-  HIPCHK(c, dev_launch_step<R>(step_args<R>(c, d), d.blocks, d.step_lds, c->stream));
+  HIPCHK(c, dev_launch_step<R>(step_args<R>(c, d), d.blocks, d.n_seg, d.step_lds, c->stream));
   const DevContactArgs<R> ca = contact_args<R>(c, d);
   if (ca.n_slots > 0)
     HIPCHK(c, dev_launch_contacts<R>(ca, d.max_inj, c->stream));
@@ -1837,7 +1852,7 @@ emcgpu_status device_init(emcgpu_ctx *c, DeviceSim &d, uint64_t seed, size_t *n_
   DevStepArgs<R> sa = step_args<R>(c, d);
   sa.move = 0;
   sa.hist = DevHist{};
-  HIPCHK(c, dev_launch_step<R>(sa, d.blocks, d.step_lds, c->stream));
+  HIPCHK(c, dev_launch_step<R>(sa, d.blocks, d.n_seg, d.step_lds, c->stream));
   HIPCHK(c, dev_launch_reduce<R>(ca, 0, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   d.initialized = true;
@@ -2073,7 +2088,7 @@ emcgpu_status contacts_t(emcgpu_ctx *c, DeviceSim &d, const double *pos, size_t 
   HIPCHK(c, dev_launch_mark(d.d_state, c->stream));
   DevStepArgs<R> sa = step_args<R>(c, d);
   sa.move = 0;
-  HIPCHK(c, dev_launch_step<R>(sa, d.blocks, d.step_lds, c->stream));
+  HIPCHK(c, dev_launch_step<R>(sa, d.blocks, d.n_seg, d.step_lds, c->stream));
   const DevContactArgs<R> ca = contact_args<R>(c, d);
   if (ca.n_slots > 0)
     HIPCHK(c, dev_launch_contacts<R>(ca, 0, c->stream));

@@ -21,6 +21,8 @@
 // removes the holes.  Positions are in nm on the device.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include "emcgpu_particle.h"
 
 namespace emcgpu {
@@ -345,133 +347,199 @@ __device__ __forceinline__ void segment_range(long long n, long long n_seg, long
   hi = lo + len < n ? lo + len : n;
 }
 
+// The step kernel runs one block per CU on a contiguous range of chunks of
+// 64 * PPT consecutive particles (particle j of lane t of a chunk is
+// chunk_first + 64 j + t); the block's waves take the chunks one at a time
+// from an LDS counter, so waves that progress faster take more of them.
+// The PPT particles of a lane move together: their force gathers are in
+// flight at once, then their first flights, then the lane works through its
+// particles' scatter events one at a time (lowest j first).
 template <typename R>
-__global__ __launch_bounds__(kDevStepThreads) void k_dev_step(DevStepArgs<R> a) {
+__global__ __launch_bounds__(DevStepShape<R>::threads) void k_dev_step(DevStepArgs<R> a) {
+  constexpr int PPT = DevStepShape<R>::ppt, NT = DevStepShape<R>::threads;
+  constexpr int NW = DevStepShape<R>::waves;
+  constexpr long long CH = 64LL * PPT;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   __shared__ int ex_cnt[kMaxContacts];
+  __shared__ double vx_w[NW];
+  __shared__ int q_next;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nn = a.g.nn;
-  const size_t grid_bytes = a.lds_dep ? (((size_t)nn * sizeof(R) + 15) & ~(size_t)15) : 0;
   R *grid = (R *)lds_raw;
-  uint32_t *mycnt = (uint32_t *)(lds_raw + grid_bytes) + wave * a.n_slots;
   if (a.lds_dep)
-    for (int k = tid; k < nn; k += kDevStepThreads)
+    for (int k = tid; k < nn; k += NT)
       grid[k] = R(0);
-  for (int k = lane; k < a.n_slots; k += 64)
-    mycnt[k] = 0u;
   if (tid < kMaxContacts)
     ex_cnt[tid] = 0;
+  if (tid == 0)
+    q_next = 0;
   __syncthreads();
 
   const long long n_total = a.state->n_total;
-  const long long row = a.state->step - a.state->hist_base;
   const uint32_t stw = (uint32_t)a.state->step;
-  const long long n_seg = (long long)gridDim.x * kDevStepWaves;
-  const long long seg = (long long)blockIdx.x * kDevStepWaves + wave;
-  long long lo, hi;
-  segment_range(n_total, n_seg, seg, lo, hi);
+  const long long n_chunks = (n_total + CH - 1) / CH;
+  const long long c_lo = n_chunks * blockIdx.x / gridDim.x;
+  const long long c_hi = n_chunks * (blockIdx.x + 1) / gridDim.x;
   const InstDev<R> *P = a.inst;
-  const uint64_t lt = (1ull << lane) - 1ull;
-  uint32_t nst = 0;
+  const R dt = P->dt;
   double vxs = 0.0;  // channel current: this lane's sum of v_x
-  // the state of the wave's next chunk is loaded while the current one moves
-  struct Chunk {
-    R x, y, kx, ky, kz, E, tau;
-    uint32_t vs, pid;
+  auto take = [&]() -> long long {
+    int t = 0;
+    if (lane == 0)
+      t = atomicAdd(&q_next, 1);
+    return c_lo + __shfl(t, 0, 64);
   };
-  auto load = [&](long long i, Chunk &q) {
-    if (i < hi) {
-      q.vs = a.p.vs[i];
-      q.x = a.p.x[i];
-      q.y = a.p.y[i];
-      q.kx = a.p.kx[i];
-      q.ky = a.p.ky[i];
-      q.kz = a.p.kz[i];
-      q.E = a.p.E[i];
-      q.tau = a.p.tau[i];
-      q.pid = a.p.pid[i];
-    } else {
-      q.vs = kHole;
-    }
-  };
-  Chunk nxt;
-  if (lo < hi)
-    load(lo + lane, nxt);
-  for (long long base = lo; base < hi; base += 64) {
-    const long long i = base + lane;
-    const Chunk cur = nxt;
-    if (base + 64 < hi)
-      load(base + 64 + lane, nxt);
-    const uint32_t vs = cur.vs;
-    int slot = -1;
-    if (vs != kHole) {
-      R x = cur.x, y = cur.y;
-      bool removed = false;
-      if (a.move) {
-        R kx = cur.kx, ky = cur.ky, kz = cur.kz, E = cur.E, tau = cur.tau;
-        int v = (int)(vs >> 16), sub = (int)(vs & 0xffffu);
-        StepRng<R, 0> rng;
-        rng.step = stw;
-        rng.pid = cur.pid;
-        rng.inst = kDomMove;
-        rng.k0 = a.key0;
-        rng.k1 = a.key1;
-        int nev;
-        removed = move_dev<R, 0>(P, a.cum, a.g, x, y, kx, ky, kz, E, tau, v, sub, rng, nev);
-        if (removed) {
-          a.p.vs[i] = kHole;
-          const int c = a.g.cnode[pos_coord(y, a.g.hy) * a.g.nx + pos_coord(x, a.g.hx)];
-          atomicAdd(&ex_cnt[c], 1);
-        } else {
-          a.p.x[i] = x;
-          a.p.y[i] = y;
-          a.p.kx[i] = kx;
-          a.p.ky[i] = ky;
-          a.p.kz[i] = kz;
-          a.p.E[i] = E;
-          a.p.tau[i] = tau;
-          a.p.vs[i] = ((uint32_t)v << 16) | (uint32_t)sub;
-          if (a.chan && x >= a.chan_x0 && x <= a.chan_x1)
-            vxs += vx_dev(P, v, sub, kx, ky, kz, E);
+  for (long long c = take(); c < c_hi; c = take()) {
+    const long long i0 = c * CH + lane;
+    R x[PPT], y[PPT], kx[PPT], ky[PPT], kz[PPT], E[PPT], tau[PPT], rem[PPT];
+    int v[PPT], sub[PPT];
+    uint32_t pid[PPT], ev[PPT];
+    bool live[PPT], gone[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      const long long i = i0 + 64LL * j;
+      const uint32_t vs = i < n_total ? a.p.vs[i] : kHole;
+      live[j] = vs != kHole;
+      gone[j] = false;
+      v[j] = (int)(vs >> 16);
+      sub[j] = (int)(vs & 0xffffu);
+      x[j] = y[j] = kx[j] = ky[j] = kz[j] = E[j] = tau[j] = rem[j] = R(0);
+      pid[j] = 0u;
+      ev[j] = 0u;
+      if (live[j]) {
+        x[j] = a.p.x[i];
+        y[j] = a.p.y[i];
+        if (a.move) {
+          kx[j] = a.p.kx[i];
+          ky[j] = a.p.ky[i];
+          kz[j] = a.p.kz[i];
+          E[j] = a.p.E[i];
+          tau[j] = a.p.tau[i];
+          pid[j] = a.p.pid[i];
         }
       }
-      if (!removed) {
-        if (a.lds_dep)
-          assign(a.g, x, y, [&](int k, R w) { atomicAdd(&grid[k], w); });
-        else
-          assign(a.g, x, y, [&](int k, R w) { atomicAdd(&a.dep[k], w); });
-        const int cx = pos_coord(x, a.g.hx), cy = pos_coord(y, a.g.hy);
-        if ((unsigned)cx < (unsigned)a.g.nx && (unsigned)cy < (unsigned)a.g.ny)
-          slot = a.g.cslot[cy * a.g.nx + cx];
-      }
     }
-    // rank of each particle at an ohmic node among the segment's particles
-    // at that node, in array order
-    const uint64_t at = __ballot(slot >= 0);
-    if (at) {
-      uint64_t pend = at;
-      uint32_t rank = 0;
-      while (pend) {
-        const int leader = __ffsll((unsigned long long)pend) - 1;
-        const int s = __shfl(slot, leader, 64);
-        const uint64_t m = __ballot(slot == s);
-        if (slot == s)
-          rank = mycnt[s] + (uint32_t)__popcll(m & lt);
-        if (lane == leader)
-          mycnt[s] += (uint32_t)__popcll(m);
-        pend &= ~m;
+    if (a.move) {
+      // driftScatterParticles (emcBasicParticleHandler.hpp:100-131): force at
+      // the current position, drift min(tau, dt); while time remains and the
+      // particle is inside: scatter, tau += tau', force again, drift
+      // min(rem, tau'), rem -= tau'; finally tau -= dt
+      R ax[PPT], ay[PPT];
+#pragma unroll
+      for (int j = 0; j < PPT; ++j) {
+        ax[j] = ay[j] = R(0);
+        if (live[j])
+          interp(a.g, x[j], y[j], ax[j], ay[j]);
       }
-      if (slot >= 0)
-        a.stage[lo + nst + (uint32_t)__popcll(at & lt)] =
-            make_uint2((uint32_t)i, ((uint32_t)slot << 22) | rank);
-      nst += (uint32_t)__popcll(at);
+#pragma unroll
+      for (int j = 0; j < PPT; ++j)
+        if (live[j]) {
+          const bool whole = tau[j] >= dt;
+          gone[j] = drift_dev(P, a.g, whole ? dt : tau[j], ax[j], ay[j], x[j], y[j], kx[j], ky[j],
+                              kz[j], E[j], v[j], sub[j]);
+          rem[j] = whole ? R(0) : dt - tau[j];
+        }
+      StepRng<R, 0> rng;
+      rng.step = stw;
+      rng.inst = kDomMove;
+      rng.k0 = a.key0;
+      rng.k1 = a.key1;
+      int nreal = 0;
+      for (;;) {
+        // every pass draws tau' > 0, so rem strictly decreases; the event
+        // cap bounds the loop regardless
+        int sel = -1;
+#pragma unroll
+        for (int j = PPT - 1; j >= 0; --j)
+          if (live[j] && !gone[j] && rem[j] > R(0) && ev[j] < kMaxEventsPerStep)
+            sel = j;
+        if (sel < 0)
+          break;
+        R px = x[0], py = y[0], qx = kx[0], qy = ky[0], qz = kz[0], en = E[0], ta = tau[0],
+          rm = rem[0];
+        int vv = v[0], ss = sub[0];
+        uint32_t ee = ev[0], pp = pid[0];
+#pragma unroll
+        for (int j = 1; j < PPT; ++j)
+          if (sel == j) {
+            px = x[j];
+            py = y[j];
+            qx = kx[j];
+            qy = ky[j];
+            qz = kz[j];
+            en = E[j];
+            ta = tau[j];
+            rm = rem[j];
+            vv = v[j];
+            ss = sub[j];
+            ee = ev[j];
+            pp = pid[j];
+          }
+        rng.pid = pp;
+        const R newTau = scatter_event<R, 0>(P, a.cum, qx, qy, qz, en, vv, ss, ee, rng, nreal);
+        ta += newTau;
+        R fx, fy;
+        interp(a.g, px, py, fx, fy);
+        const bool out = drift_dev(P, a.g, (newTau < rm) ? newTau : rm, fx, fy, px, py, qx, qy, qz,
+                                   en, vv, ss);
+        rm -= newTau;
+#pragma unroll
+        for (int j = 0; j < PPT; ++j)
+          if (sel == j) {
+            x[j] = px;
+            y[j] = py;
+            kx[j] = qx;
+            ky[j] = qy;
+            kz[j] = qz;
+            E[j] = en;
+            tau[j] = ta;
+            rem[j] = rm;
+            v[j] = vv;
+            sub[j] = ss;
+            ev[j] = ee;
+            gone[j] = out;
+          }
+      }
+#pragma unroll
+      for (int j = 0; j < PPT; ++j)
+        tau[j] -= dt;
+    }
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      const long long i = i0 + 64LL * j;
+      if (i >= n_total)
+        continue;
+      int slot = -1;
+      if (live[j]) {
+        if (gone[j]) {
+          a.p.vs[i] = kHole;
+          const int cidx = a.g.cnode[pos_coord(y[j], a.g.hy) * a.g.nx + pos_coord(x[j], a.g.hx)];
+          atomicAdd(&ex_cnt[cidx], 1);
+        } else {
+          if (a.move) {
+            a.p.x[i] = x[j];
+            a.p.y[i] = y[j];
+            a.p.kx[i] = kx[j];
+            a.p.ky[i] = ky[j];
+            a.p.kz[i] = kz[j];
+            a.p.E[i] = E[j];
+            a.p.tau[i] = tau[j];
+            a.p.vs[i] = ((uint32_t)v[j] << 16) | (uint32_t)sub[j];
+            if (a.chan && x[j] >= a.chan_x0 && x[j] <= a.chan_x1)
+              vxs += vx_dev(P, v[j], sub[j], kx[j], ky[j], kz[j], E[j]);
+          }
+          if (a.lds_dep)
+            assign(a.g, x[j], y[j], [&](int k, R w) { atomicAdd(&grid[k], w); });
+          else
+            assign(a.g, x[j], y[j], [&](int k, R w) { atomicAdd(&a.dep[k], w); });
+          const int cx = pos_coord(x[j], a.g.hx), cy = pos_coord(y[j], a.g.hy);
+          if ((unsigned)cx < (unsigned)a.g.nx && (unsigned)cy < (unsigned)a.g.ny)
+            slot = a.g.cslot[cy * a.g.nx + cx];
+        }
+      }
+      a.pslot[i] = (int16_t)slot;
     }
   }
-  for (int k = lane; k < a.n_slots; k += 64)
-    a.seg_cnt[(long long)k * n_seg + seg] = mycnt[k];
-  if (lane == 0)
-    a.seg_nst[seg] = nst;
-  __shared__ double vx_w[kDevStepWaves];
   if (a.chan) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1)
@@ -482,15 +550,78 @@ __global__ __launch_bounds__(kDevStepThreads) void k_dev_step(DevStepArgs<R> a) 
   __syncthreads();
   if (a.chan && tid == 0) {
     double b = 0.0;
-    for (int w = 0; w < kDevStepWaves; ++w)
+    for (int w = 0; w < NW; ++w)
       b += vx_w[w];
     a.chan_part[blockIdx.x] = b;
   }
   if (a.lds_dep)
-    for (int k = tid; k < nn; k += kDevStepThreads)
+    for (int k = tid; k < nn; k += NT)
       a.dep[(long long)blockIdx.x * nn + k] = grid[k];
-  if (tid < a.n_contacts && ex_cnt[tid] && a.hist.exits)
-    atomicAdd(&a.hist.exits[row * a.n_contacts + tid], ex_cnt[tid]);
+  if (tid < a.n_contacts && a.exit_part)
+    a.exit_part[blockIdx.x * a.n_contacts + tid] = ex_cnt[tid];
+}
+
+// rank of each particle at an ohmic node among the particles of its segment
+// at that node, in array order (one wave per segment), from the slots the
+// step kernel recorded; the staged entries and per-slot counts feed the
+// contact scan
+__global__ __launch_bounds__(kDevStepThreads) void k_dev_rank(DevStepArgs<float> a) {
+  extern __shared__ uint32_t rank_cnt[];  // [wave][slot]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  uint32_t *mycnt = rank_cnt + wave * a.n_slots;
+  for (int k = lane; k < a.n_slots; k += 64)
+    mycnt[k] = 0u;
+  const long long n_total = a.state->n_total;
+  const long long n_seg = (long long)gridDim.x * kDevStepWaves;
+  const long long seg = (long long)blockIdx.x * kDevStepWaves + wave;
+  long long lo, hi;
+  segment_range(n_total, n_seg, seg, lo, hi);
+  const uint64_t lt = (1ull << lane) - 1ull;
+  uint32_t nst = 0;
+  for (long long base = lo; base < hi; base += 256) {
+    int sl[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long long i = base + 64 * u + lane;
+      sl[u] = i < hi ? (int)a.pslot[i] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long long i = base + 64 * u + lane;
+      const int slot = sl[u];
+      const uint64_t at = __ballot(slot >= 0);
+      if (!at)
+        continue;
+      // one pass per distinct slot of the 64 entries, registers only: the
+      // lane's rank among the lanes with its slot, and the group size at
+      // the group's lowest lane
+      uint64_t pend = at;
+      uint32_t lrank = 0, gsize = 0;
+      while (pend) {
+        const int leader = __ffsll((unsigned long long)pend) - 1;
+        const int s = __builtin_amdgcn_readlane(slot, leader);
+        const uint64_t m = __ballot(slot == s);
+        if (slot == s)
+          lrank = (uint32_t)__popcll(m & lt);
+        if (lane == leader)
+          gsize = (uint32_t)__popcll(m);
+        pend &= ~m;
+      }
+      // then the segment's counts: every lane reads its slot's count, the
+      // groups' lowest lanes advance them
+      const uint32_t prior = slot >= 0 ? mycnt[slot] : 0u;
+      if (gsize)
+        mycnt[slot] = prior + gsize;
+      if (slot >= 0)
+        a.stage[lo + nst + (uint32_t)__popcll(at & lt)] =
+            make_uint2((uint32_t)i, ((uint32_t)slot << 22) | (prior + lrank));
+      nst += (uint32_t)__popcll(at);
+    }
+  }
+  for (int k = lane; k < a.n_slots; k += 64)
+    a.seg_cnt[seg * a.n_slots + k] = mycnt[k];
+  if (lane == 0)
+    a.seg_nst[seg] = nst;
 }
 
 // ---------------------------------------------------------------------------
@@ -502,14 +633,23 @@ __global__ __launch_bounds__(256) void k_dev_contact_scan(DevContactArgs<R> a) {
   __shared__ uint32_t sc[256];
   const int slot = blockIdx.x, tid = threadIdx.x;
   const int S = a.n_segments;
-  const uint32_t *cnt = a.seg_cnt + (long long)slot * S;
+  const uint32_t *cnt = a.seg_cnt + slot;  // column of the [segment][slot] counts
+  const long long cs = a.n_slots;
   uint32_t *off = a.seg_off + (long long)slot * S;
   const int per = (S + 255) / 256;
   const int b0 = tid * per < S ? tid * per : S;
   const int b1 = b0 + per < S ? b0 + per : S;
+  // the thread's counts, sixteen loads in flight at a time
   uint32_t tot = 0;
-  for (int b = b0; b < b1; ++b)
-    tot += cnt[b];
+  for (int b = b0; b < b1; b += 16) {
+    uint32_t v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      v[u] = b + u < b1 ? cnt[(long long)(b + u) * cs] : 0u;
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      tot += v[u];
+  }
   sc[tid] = tot;
   __syncthreads();
   for (int d = 1; d < 256; d <<= 1) {
@@ -519,51 +659,30 @@ __global__ __launch_bounds__(256) void k_dev_contact_scan(DevContactArgs<R> a) {
     __syncthreads();
   }
   uint32_t run = sc[tid] - tot;
-  for (int b = b0; b < b1; ++b) {
-    off[b] = run;
-    run += cnt[b];
+  for (int b = b0; b < b1; b += 16) {
+    uint32_t v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      v[u] = b + u < b1 ? cnt[(long long)(b + u) * cs] : 0u;
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      if (b + u < b1) {
+        off[b + u] = run;
+        run += v[u];
+      }
   }
-  __shared__ int last;
-  __shared__ long long part[4];
+  // the node's counts (no atomics: the resolve kernel totals them per
+  // contact)
   if (tid == 0) {
     const long long total = sc[255];
-    const long long row = a.state->step - a.state->hist_base;
     const double e = a.slot_exp[slot];
     const long long ce = (long long)ceil(e);
     const long long kept = total < ce ? total : ce;
     const double diff = e - (double)kept;
     const long long inj = diff > 0.0 ? (long long)ceil(diff) : 0;
+    a.slot_total[slot] = (int)total;
     a.slot_kept[slot] = (int)kept;
     a.slot_inj[slot] = (int)inj;
-    const int c = a.slot_contact[slot];
-    if (a.hist.dels && total > kept)
-      atomicAdd(&a.hist.dels[row * a.n_contacts + c], (int)(total - kept));
-    if (a.hist.injs && inj > 0)
-      atomicAdd(&a.hist.injs[row * a.n_contacts + c], (int)inj);
-    __threadfence();
-    last = atomicAdd(&a.state->ticket, 1u) == (unsigned int)a.n_slots - 1;
-  }
-  __syncthreads();
-  if (!last)
-    return;
-  // the last block: the step's injections over all ohmic nodes
-  __threadfence();
-  long long v = 0;
-  for (int q = tid; q < a.n_slots; q += 256)
-    v += __hip_atomic_load(&a.slot_inj[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  for (int off = 32; off > 0; off >>= 1)
-    v += __shfl_xor(v, off, 64);
-  if ((tid & 63) == 0)
-    part[tid >> 6] = v;
-  __syncthreads();
-  if (tid == 0) {
-    long long sum = part[0] + part[1] + part[2] + part[3];
-    if (a.state->n_total + sum > a.state->capacity) {
-      a.state->overflow = 1;
-      sum = a.state->capacity - a.state->n_total;
-    }
-    a.state->inj_total = sum;
-    a.state->ticket = 0u;
   }
 }
 
@@ -601,10 +720,8 @@ __global__ __launch_bounds__(256) void k_dev_resolve(DevContactArgs<R> a) {
     return;
   }
   // injections
-  const long long T = a.state->inj_total;
-  const long long t = ((long long)blockIdx.x - nA) * 256 + tid;
-  if (((long long)blockIdx.x - nA) * 256 >= T)
-    return;
+  const long long bi = (long long)blockIdx.x - nA;
+  const long long t = bi * 256 + tid;
   {  // prefix of the injections over the ohmic nodes (4 nodes per thread)
     __shared__ int ws[256];
     int v[4], run = 0;
@@ -633,7 +750,42 @@ __global__ __launch_bounds__(256) void k_dev_resolve(DevContactArgs<R> a) {
         pre[q + 1] = acc;
     }
   }
+  __shared__ int per_contact[3][kMaxContacts];
+  if (bi == 0 && tid < 3 * kMaxContacts)
+    per_contact[tid / kMaxContacts][tid % kMaxContacts] = 0;
   __syncthreads();
+  long long T = pre[a.n_slots];
+  const long long room = a.state->capacity - n_total;
+  if (T > room)
+    T = room;
+  if (bi == 0) {
+    // the step's records: deletions and injections per contact from the
+    // nodes' counts, exits from the step blocks' counts
+    for (int q = tid; q < a.n_slots; q += 256) {
+      const int c = a.slot_contact[q];
+      atomicAdd(&per_contact[0][c], a.slot_total[q] - a.slot_kept[q]);
+      atomicAdd(&per_contact[1][c], a.slot_inj[q]);
+    }
+    if (a.exit_part)
+      for (int e = tid; e < a.n_step_blocks * a.n_contacts; e += 256)
+        atomicAdd(&per_contact[2][e % a.n_contacts], a.exit_part[e]);
+    __syncthreads();
+    if (tid == 0) {
+      if (T < pre[a.n_slots])
+        a.state->overflow = 1;
+      a.state->inj_total = T;
+    }
+    const long long row = a.state->step - a.state->hist_base;
+    if (tid < a.n_contacts) {
+      const long long o = row * a.n_contacts + tid;
+      if (a.hist.dels)
+        a.hist.dels[o] += per_contact[0][tid];
+      if (a.hist.injs)
+        a.hist.injs[o] += per_contact[1][tid];
+      if (a.hist.exits && a.exit_part)
+        a.hist.exits[o] += per_contact[2][tid];
+    }
+  }
   if (t >= T)
     return;
   int lo = 0, hi = a.n_slots;  // pre[lo] <= t < pre[hi]
@@ -1736,7 +1888,8 @@ hipError_t dev_launch_field(const DevFieldArgs<R> *args_dev, size_t lds, hipStre
 }
 
 template <typename R>
-hipError_t dev_launch_step(const DevStepArgs<R> &a, int blocks, size_t lds, hipStream_t st) {
+hipError_t dev_launch_step(const DevStepArgs<R> &a, int blocks, int n_seg, size_t lds,
+                           hipStream_t st) {
   if (lds > (size_t)kDevMaxLds)
     return hipErrorInvalidValue;
   if (lds > 65536) {
@@ -1747,7 +1900,19 @@ hipError_t dev_launch_step(const DevStepArgs<R> &a, int blocks, size_t lds, hipS
       return e;
     }
   }
-  hipLaunchKernelGGL((k_dev_step<R>), dim3(blocks), dim3(kDevStepThreads), lds, st, a);
+  hipLaunchKernelGGL((k_dev_step<R>), dim3(blocks), dim3(DevStepShape<R>::threads), lds, st, a);
+  // the ranks read only the slots, segments and stage (no precision-typed
+  // field): one instantiation serves both precisions
+  DevStepArgs<float> ra;
+  std::memset(&ra, 0, sizeof ra);
+  ra.state = a.state;
+  ra.n_slots = a.n_slots;
+  ra.seg_cnt = a.seg_cnt;
+  ra.seg_nst = a.seg_nst;
+  ra.stage = a.stage;
+  ra.pslot = a.pslot;
+  hipLaunchKernelGGL(k_dev_rank, dim3((unsigned)(n_seg / kDevStepWaves)), dim3(kDevStepThreads),
+                     sizeof(uint32_t) * kDevStepWaves * (size_t)a.n_slots, st, ra);
   return hipGetLastError();
 }
 
@@ -1810,10 +1975,11 @@ hipError_t dev_launch_mark(DevState *state, hipStream_t st) {
 template <typename R>
 int dev_step_blocks_per_cu(size_t lds) {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_dev_step<R>, kDevStepThreads, lds) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_dev_step<R>, DevStepShape<R>::threads,
+                                                   lds) !=
       hipSuccess)
     return 1;
-  return n > 0 ? n : 1;
+  return n;
 }
 
 hipError_t dev_launch_zero(double *p, long long n, hipStream_t st) {
@@ -1980,19 +2146,89 @@ hipError_t dev_green_lds(size_t lds) {
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
-// residual check, correction (skipped once a check meets the stopping
-// rule), ..., and a last check after the last correction
 // Newton correction by the separable inverse.  Both launches run a block per
-// (active row j, chunk of kFdChunk columns): the block first forms a whole
-// row of the y transform in LDS (each thread one column, four independent
-// partial sums), then each lane owns one output column of the chunk and the
-// four waves split the x sum into quarters (four partial sums each), so a
-// thread waits on few dependent L2 round trips.
+// (active row j, chunk of kFdChunk columns) and are bound by the latency of
+// dependent L2 round trips, so every sum keeps eight loads in flight per
+// lane and is split over as many threads as the block has: the row of the
+// y transform (all mx columns, in LDS) takes P = kFdThreads / mx threads
+// per column, each summing a P-th of the rows; the chunk's x sums take one
+// lane per output column and the block's waves each a slice of the rows.
 constexpr int kFdChunk = 64;
+constexpr int kFdWaves = kFdThreads / 64;
+
+// sum over r in [r0, r1) of m[r * ld] * v(r), eight loads in flight
+template <typename V>
+__device__ __forceinline__ double fd_dot8(const double *m, size_t ld, int r0, int r1, V v) {
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  int r = r0;
+  for (; r + 8 <= r1; r += 8) {
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      x[u] = m[(size_t)(r + u) * ld];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      s[u & 3] = fma(x[u], v(r + u), s[u & 3]);
+  }
+  for (; r < r1; ++r)
+    s[0] = fma(m[(size_t)r * ld], v(r), s[0]);
+  return (s[0] + s[1]) + (s[2] + s[3]);
+}
+
+// row[q] = sum_b w[b] in(b, q) for the mx columns q, P threads per column
+// (part: max(mx, kFdThreads) doubles of LDS)
+template <typename IN>
+__device__ __forceinline__ void fd_row(const double *w, int my, int mx, double *row, double *part,
+                                       IN in) {
+  const int P = max(1, min(kFdThreads / mx, 8));
+  const int span = (my + P - 1) / P;
+  for (int e = threadIdx.x; e < P * mx; e += kFdThreads) {
+    const int p = e / mx, q = e - p * mx;
+    const int b0 = p * span, b1 = min(my, b0 + span);
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    int b = b0;
+    for (; b + 8 <= b1; b += 8) {
+      double x[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        x[u] = in(b + u, q);
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        s[u & 3] = fma(w[b + u], x[u], s[u & 3]);
+    }
+    for (; b < b1; ++b)
+      s[0] = fma(w[b], in(b, q), s[0]);
+    part[e] = (s[0] + s[1]) + (s[2] + s[3]);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < mx; q += kFdThreads) {
+    double t = 0.0;
+    for (int p = 0; p < P; ++p)
+      t += part[p * mx + q];
+    row[q] = t;
+  }
+  __syncthreads();
+}
+
+// the chunk's x sums: out = sum_r M[r][k] row[r] for lane k of the chunk,
+// each wave a slice of the rows; the result is valid in wave 0
+__device__ __forceinline__ double fd_chunk(const double *M, const double *row, double *part,
+                                           int mx, int k) {
+  const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+  const int span = (mx + kFdWaves - 1) / kFdWaves, r0 = w * span, r1 = min(mx, r0 + span);
+  part[w * kFdChunk + lane] =
+      k < mx ? fd_dot8(M + k, (size_t)mx, r0, r1, [&](int r) { return row[r]; }) : 0.0;
+  __syncthreads();
+  double t = 0.0;
+  if (w == 0)
+    for (int v = 0; v < kFdWaves; ++v)
+      t += part[v * kFdChunk + lane];
+  return t;
+}
 
 // t[j][k] = sum_q Px[k][q] (sum_b Py[j][b] F[b][q]) / (lx[k] + ly[j])
 __global__ __launch_bounds__(kFdThreads) void k_fd_fwd(GreenArgs a, int c) {
-  extern __shared__ double fd_lds[];  // [mx] row, [waves][kFdChunk] partial sums
+  extern __shared__ double fd_lds[];  // [mx] row, [max(mx, kFdThreads)] partial sums
   if (a.interval > 1 && a.state->step % a.interval != 0)
     return;
   if (*a.flag == (unsigned long long)a.state->step + 1ull || green_converged(a, c))
@@ -2000,51 +2236,17 @@ __global__ __launch_bounds__(kFdThreads) void k_fd_fwd(GreenArgs a, int c) {
   const FdOps &f = a.fd;
   const int j = blockIdx.y, mx = f.mx, my = f.my, nx = a.nx;
   double *row = fd_lds, *part = fd_lds + mx;
-  const double *py = f.py + (size_t)j * my;
-  for (int q = threadIdx.x; q < mx; q += kFdThreads) {
-    const double *gc = a.g + f.ix[q];
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int b = 0;
-    for (; b + 4 <= my; b += 4) {
-      s0 = fma(py[b], gc[(size_t)f.iy[b] * nx], s0);
-      s1 = fma(py[b + 1], gc[(size_t)f.iy[b + 1] * nx], s1);
-      s2 = fma(py[b + 2], gc[(size_t)f.iy[b + 2] * nx], s2);
-      s3 = fma(py[b + 3], gc[(size_t)f.iy[b + 3] * nx], s3);
-    }
-    for (; b < my; ++b)
-      s0 = fma(py[b], gc[(size_t)f.iy[b] * nx], s0);
-    row[q] = (s0 + s1) + (s2 + s3);
-  }
-  __syncthreads();
-  const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
-  const int k = (int)blockIdx.x * kFdChunk + lane;
-  constexpr int kWaves = kFdThreads / 64;
-  const int span = (mx + kWaves - 1) / kWaves, q0 = w * span, q1 = min(mx, q0 + span);
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  if (k < mx) {
-    const double *pc = f.pxt + k;
-    int q = q0;
-    for (; q + 4 <= q1; q += 4) {
-      s0 = fma(pc[(size_t)q * mx], row[q], s0);
-      s1 = fma(pc[(size_t)(q + 1) * mx], row[q + 1], s1);
-      s2 = fma(pc[(size_t)(q + 2) * mx], row[q + 2], s2);
-      s3 = fma(pc[(size_t)(q + 3) * mx], row[q + 3], s3);
-    }
-    for (; q < q1; ++q)
-      s0 = fma(pc[(size_t)q * mx], row[q], s0);
-  }
-  part[w * kFdChunk + lane] = (s0 + s1) + (s2 + s3);
-  __syncthreads();
-  if (w == 0 && k < mx) {
-    const double t = (part[lane] + part[kFdChunk + lane]) +
-                     (part[2 * kFdChunk + lane] + part[3 * kFdChunk + lane]);
+  fd_row(f.py + (size_t)j * my, my, mx, row, part,
+         [&](int b, int q) { return a.g[(size_t)f.iy[b] * nx + f.ix[q]]; });
+  const int k = (int)blockIdx.x * kFdChunk + ((int)threadIdx.x & 63);
+  const double t = fd_chunk(f.pxt, row, part, mx, k);
+  if (threadIdx.x < 64 && k < mx)
     f.t[(size_t)j * mx + k] = t * f.inv_den[(size_t)j * mx + k];
-  }
 }
 
 // pot[j][q] += sum_k Bx[q][k] (sum_l By[j][l] t[l][k])
 __global__ __launch_bounds__(kFdThreads) void k_fd_bwd(GreenArgs a, int c) {
-  extern __shared__ double fd_lds[];  // [mx] row, [waves][kFdChunk] partial sums
+  extern __shared__ double fd_lds[];  // [mx] row, [max(mx, kFdThreads)] partial sums
   if (a.interval > 1 && a.state->step % a.interval != 0)
     return;
   if (*a.flag == (unsigned long long)a.state->step + 1ull || green_converged(a, c))
@@ -2052,54 +2254,22 @@ __global__ __launch_bounds__(kFdThreads) void k_fd_bwd(GreenArgs a, int c) {
   const FdOps &f = a.fd;
   const int j = blockIdx.y, mx = f.mx, my = f.my;
   double *row = fd_lds, *part = fd_lds + mx;
-  const double *by = f.by + (size_t)j * my;
-  for (int k = threadIdx.x; k < mx; k += kFdThreads) {
-    const double *tc = f.t + k;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int l = 0;
-    for (; l + 4 <= my; l += 4) {
-      s0 = fma(by[l], tc[(size_t)l * mx], s0);
-      s1 = fma(by[l + 1], tc[(size_t)(l + 1) * mx], s1);
-      s2 = fma(by[l + 2], tc[(size_t)(l + 2) * mx], s2);
-      s3 = fma(by[l + 3], tc[(size_t)(l + 3) * mx], s3);
-    }
-    for (; l < my; ++l)
-      s0 = fma(by[l], tc[(size_t)l * mx], s0);
-    row[k] = (s0 + s1) + (s2 + s3);
-  }
-  __syncthreads();
-  const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
-  const int q = (int)blockIdx.x * kFdChunk + lane;
-  constexpr int kWaves = kFdThreads / 64;
-  const int span = (mx + kWaves - 1) / kWaves, k0 = w * span, k1 = min(mx, k0 + span);
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  if (q < mx) {
-    const double *bc = f.bxt + q;
-    int k = k0;
-    for (; k + 4 <= k1; k += 4) {
-      s0 = fma(bc[(size_t)k * mx], row[k], s0);
-      s1 = fma(bc[(size_t)(k + 1) * mx], row[k + 1], s1);
-      s2 = fma(bc[(size_t)(k + 2) * mx], row[k + 2], s2);
-      s3 = fma(bc[(size_t)(k + 3) * mx], row[k + 3], s3);
-    }
-    for (; k < k1; ++k)
-      s0 = fma(bc[(size_t)k * mx], row[k], s0);
-  }
-  part[w * kFdChunk + lane] = (s0 + s1) + (s2 + s3);
-  __syncthreads();
-  if (w == 0 && q < mx) {
-    const double x = (part[lane] + part[kFdChunk + lane]) +
-                     (part[2 * kFdChunk + lane] + part[3 * kFdChunk + lane]);
+  fd_row(f.by + (size_t)j * my, my, mx, row, part,
+         [&](int l, int k) { return f.t[(size_t)l * mx + k]; });
+  const int q = (int)blockIdx.x * kFdChunk + ((int)threadIdx.x & 63);
+  const double x = fd_chunk(f.bxt, row, part, mx, q);
+  if (threadIdx.x < 64 && q < mx)
     a.pot[(size_t)f.iy[j] * a.nx + f.ix[q]] += x;
-  }
 }
 
+// residual check, correction (skipped once a check meets the stopping
+// rule), ..., and a last check after the last correction
 hipError_t dev_launch_green(const GreenArgs &a, int iters, int blocks, hipStream_t st) {
   const unsigned rb = (unsigned)((a.nn + 255) / 256);
   for (int it = 0; it < iters; ++it) {
     hipLaunchKernelGGL(k_green_res, dim3(rb), dim3(256), 0, st, a, it);
     if (a.use_fd) {
-      const size_t lds = (size_t)(a.fd.mx + kFdThreads) * sizeof(double);
+      const size_t lds = (size_t)(a.fd.mx + max(a.fd.mx, kFdThreads)) * sizeof(double);
       const dim3 grid((unsigned)((a.fd.mx + kFdChunk - 1) / kFdChunk), (unsigned)a.fd.my);
       hipLaunchKernelGGL(k_fd_fwd, grid, dim3(kFdThreads), lds, st, a, it);
       hipLaunchKernelGGL(k_fd_bwd, grid, dim3(kFdThreads), lds, st, a, it);
@@ -2164,7 +2334,7 @@ hipError_t dev_launch_green_expand(const double *Ainv, long long lda, const int 
 #define EMCGPU_DEV_INST(R)                                                                   \
   template hipError_t dev_launch_field<R>(const DevFieldArgs<R> *, size_t, hipStream_t);      \
   template hipError_t dev_launch_efield<R>(const DevFieldArgs<R> *, int, hipStream_t);        \
-  template hipError_t dev_launch_step<R>(const DevStepArgs<R> &, int, size_t, hipStream_t);   \
+  template hipError_t dev_launch_step<R>(const DevStepArgs<R> &, int, int, size_t, hipStream_t); \
   template hipError_t dev_launch_drift<R>(const DevStepArgs<R> &, int, hipStream_t);          \
   template hipError_t dev_launch_contacts<R>(const DevContactArgs<R> &, long long,           \
                                              hipStream_t);                                    \
