@@ -330,7 +330,11 @@ def test_dense_poisson_corrections_match_multigrid(precision):
               % (work_mg.mean(), name, work.mean(), work.max(), out["poisson_resid"][1:].max()),
               flush=True)
         assert (work <= most).mean() >= 0.9 and work.mean() < 0.5 * work_mg.mean()
-        assert np.abs(pot - mg[1]).max() < 1e-8 * scale
+        # the solves agree to their tolerance; over 40 steps FP32 positions
+        # turn that into differences of ~1e-8 of the scale
+        dpot = np.abs(pot - mg[1]).max() / scale
+        print("  max |pot - multigrid pot| = %.2e of the scale" % dpot, flush=True)
+        assert dpot < (1e-8 if precision == 0 else 1e-7)
         assert abs(int(out["n_particles"][-1]) - int(mg[0]["n_particles"][-1])) <= 2
 
 

@@ -231,13 +231,14 @@ struct StepArgs {
 constexpr int kDevThreads = 1024;      // field solver: one workgroup
 constexpr int kDevStepThreads = 512;   // contact ranks: 8 waves (segments) per block
 constexpr int kDevStepWaves = kDevStepThreads / 64;
-constexpr int kDevSegPerCU = 16;       // contact-rank segments per CU
+constexpr int kDevSegPerCU = 16;       // contact-rank segments per CU (4 measured slower)
 // particle step: one block per CU whose waves share the CU's chunks through
-// an LDS counter; FP32 runs 16 waves of 4 particles per lane, FP64 (twice
-// the registers) 8 waves of 2
+// an LDS counter (more blocks per CU, or more particles per lane, measured
+// slower: the blocks of one CU then finish unevenly); FP32 16 waves, FP64
+// 8 (its registers would spill at 16)
 template <typename R>
 struct DevStepShape {
-  static constexpr int ppt = sizeof(R) == 4 ? 4 : 2;
+  static constexpr int ppt = 1;
   static constexpr int threads = sizeof(R) == 4 ? 1024 : 512;
   static constexpr int waves = threads / 64;
 };

@@ -26,8 +26,8 @@ hipError_t dev_launch_field(const DevFieldArgs<R> *args_dev, size_t lds, hipStre
 template <typename R>
 hipError_t dev_launch_efield(const DevFieldArgs<R> *args_dev, int nn, hipStream_t st);
 template <typename R>
-hipError_t dev_launch_step(const DevStepArgs<R> &a, int blocks, int n_seg, size_t lds,
-                           hipStream_t st);
+hipError_t dev_launch_step(const DevStepArgs<R> &a, const DevContactArgs<R> &ca, int blocks,
+                           int n_seg, size_t lds, hipStream_t st);
 template <typename R>
 hipError_t dev_launch_drift(const DevStepArgs<R> &a, int mode, hipStream_t st);
 template <typename R>
@@ -1617,8 +1617,8 @@ emcgpu_status launch_step(emcgpu_ctx *c, DeviceSim &d, bool reset_bc) {
   HIPCHK(c, dev_launch_field<R>(fargs_slot<R>(d, reset_bc ? 1 : 0), d.mg_lds, c->stream));
   if (split_efield(d))
     HIPCHK(c, dev_launch_efield<R>(fargs_slot<R>(d, reset_bc ? 1 : 0), (int)d.nn, c->stream));
-  HIPCHK(c, dev_launch_step<R>(step_args<R>(c, d), d.blocks, d.n_seg, d.step_lds, c->stream));
   const DevContactArgs<R> ca = contact_args<R>(c, d);
+  HIPCHK(c, dev_launch_step<R>(step_args<R>(c, d), ca, d.blocks, d.n_seg, d.step_lds, c->stream));
   if (ca.n_slots > 0)
     HIPCHK(c, dev_launch_contacts<R>(ca, d.max_inj, c->stream));
   HIPCHK(c, dev_launch_reduce<R>(ca, 1, c->stream));
@@ -1852,7 +1852,7 @@ emcgpu_status device_init(emcgpu_ctx *c, DeviceSim &d, uint64_t seed, size_t *n_
   DevStepArgs<R> sa = step_args<R>(c, d);
   sa.move = 0;
   sa.hist = DevHist{};
-  HIPCHK(c, dev_launch_step<R>(sa, d.blocks, d.n_seg, d.step_lds, c->stream));
+  HIPCHK(c, dev_launch_step<R>(sa, ca, d.blocks, d.n_seg, d.step_lds, c->stream));
   HIPCHK(c, dev_launch_reduce<R>(ca, 0, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   d.initialized = true;
@@ -2088,8 +2088,8 @@ emcgpu_status contacts_t(emcgpu_ctx *c, DeviceSim &d, const double *pos, size_t 
   HIPCHK(c, dev_launch_mark(d.d_state, c->stream));
   DevStepArgs<R> sa = step_args<R>(c, d);
   sa.move = 0;
-  HIPCHK(c, dev_launch_step<R>(sa, d.blocks, d.n_seg, d.step_lds, c->stream));
   const DevContactArgs<R> ca = contact_args<R>(c, d);
+  HIPCHK(c, dev_launch_step<R>(sa, ca, d.blocks, d.n_seg, d.step_lds, c->stream));
   if (ca.n_slots > 0)
     HIPCHK(c, dev_launch_contacts<R>(ca, 0, c->stream));
   HIPCHK(c, dev_launch_zero(d.d_corr, d.nn, c->stream));

@@ -565,15 +565,13 @@ __global__ __launch_bounds__(DevStepShape<R>::threads) void k_dev_step(DevStepAr
 // at that node, in array order (one wave per segment), from the slots the
 // step kernel recorded; the staged entries and per-slot counts feed the
 // contact scan
-__global__ __launch_bounds__(kDevStepThreads) void k_dev_rank(DevStepArgs<float> a) {
-  extern __shared__ uint32_t rank_cnt[];  // [wave][slot]
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  uint32_t *mycnt = rank_cnt + wave * a.n_slots;
+template <typename R>
+__device__ __forceinline__ void dev_rank(const DevStepArgs<R> &a, long long n_seg, long long seg,
+                                         uint32_t *mycnt) {
+  const int lane = threadIdx.x & 63;
   for (int k = lane; k < a.n_slots; k += 64)
     mycnt[k] = 0u;
   const long long n_total = a.state->n_total;
-  const long long n_seg = (long long)gridDim.x * kDevStepWaves;
-  const long long seg = (long long)blockIdx.x * kDevStepWaves + wave;
   long long lo, hi;
   segment_range(n_total, n_seg, seg, lo, hi);
   const uint64_t lt = (1ull << lane) - 1ull;
@@ -622,6 +620,41 @@ __global__ __launch_bounds__(kDevStepThreads) void k_dev_rank(DevStepArgs<float>
     a.seg_cnt[seg * a.n_slots + k] = mycnt[k];
   if (lane == 0)
     a.seg_nst[seg] = nst;
+}
+
+// first pass of the FP64 sum of the block grids: rows [b0, b1) of node k
+template <typename R>
+__device__ __forceinline__ void dev_reduce_rows(const DevContactArgs<R> &a, int k, int chunk) {
+  const int nn = a.g.nn;
+  if (k >= nn)
+    return;
+  const int b0 = chunk * a.rows_per_chunk;
+  const int b1 = b0 + a.rows_per_chunk < a.n_blocks_dep ? b0 + a.rows_per_chunk : a.n_blocks_dep;
+  double s = 0.0;
+  for (int b = b0; b < b1; ++b)
+    s += (double)a.dep[(long long)b * nn + k];
+  a.dpart[(long long)chunk * nn + k] = s;
+  if (a.n_blocks_dep == 1)  // the atomic deposit accumulates in row 0
+    a.dep[k] = R(0);
+}
+
+// after the step kernel, two independent jobs in one launch: blocks
+// [0, n_seg / 8) rank the contact particles (one wave per segment), the
+// others sum the block grids in chunks of rows (one node per thread)
+template <typename R>
+__global__ __launch_bounds__(kDevStepThreads) void k_dev_rank_rows(DevStepArgs<R> a,
+                                                                   DevContactArgs<R> ca,
+                                                                   int n_rank_blocks) {
+  extern __shared__ uint32_t rank_cnt[];  // [wave][slot]
+  const int tid = threadIdx.x, wave = tid >> 6;
+  if ((int)blockIdx.x < n_rank_blocks) {
+    dev_rank(a, (long long)n_rank_blocks * kDevStepWaves,
+             (long long)blockIdx.x * kDevStepWaves + wave, rank_cnt + wave * a.n_slots);
+    return;
+  }
+  const int r = (int)blockIdx.x - n_rank_blocks;
+  const int node_blocks = (ca.g.nn + kDevStepThreads - 1) / kDevStepThreads;
+  dev_reduce_rows(ca, (r % node_blocks) * kDevStepThreads + tid, r / node_blocks);
 }
 
 // ---------------------------------------------------------------------------
@@ -822,22 +855,6 @@ __global__ __launch_bounds__(256) void k_dev_resolve(DevContactArgs<R> a) {
 // corrections; concentration (updateCurrentParticleConcentrations,
 // emcSimulationResults.hpp:98-116); commit of the step's counters
 // first pass: chunk c of the block grids' rows summed per node (FP64)
-template <typename R>
-__global__ __launch_bounds__(256) void k_dev_reduce_rows(DevContactArgs<R> a) {
-  const int k = blockIdx.x * 256 + threadIdx.x;
-  const int nn = a.g.nn;
-  if (k >= nn)
-    return;
-  const int b0 = blockIdx.y * a.rows_per_chunk;
-  const int b1 = b0 + a.rows_per_chunk < a.n_blocks_dep ? b0 + a.rows_per_chunk : a.n_blocks_dep;
-  double s = 0.0;
-  for (int b = b0; b < b1; ++b)
-    s += (double)a.dep[(long long)b * nn + k];
-  a.dpart[(long long)blockIdx.y * nn + k] = s;
-  if (a.n_blocks_dep == 1)  // the atomic deposit accumulates in row 0
-    a.dep[k] = R(0);
-}
-
 template <typename R>
 __global__ __launch_bounds__(256) void k_dev_reduce(DevContactArgs<R> a, int commit) {
   const int k = blockIdx.x * 256 + threadIdx.x;
@@ -1581,8 +1598,10 @@ __global__ __launch_bounds__(kDevThreads) void k_dev_field(const DevFieldArgs<R>
     }
     return;
   }
-  // the dense corrections before this launch: their checks say how many
-  // ran and whether one already meets the stopping rule
+  // the Newton corrections before this launch: their checks say how many
+  // ran and whether one already meets the stopping rule (the last check
+  // follows the last correction; a separate launch over all CUs forms it,
+  // which is cheaper than one workgroup here)
   int green_done = 0;
   bool green_ok = false;
   double green_fm = 0.0, green_scale = 1.0;
@@ -1888,8 +1907,8 @@ hipError_t dev_launch_field(const DevFieldArgs<R> *args_dev, size_t lds, hipStre
 }
 
 template <typename R>
-hipError_t dev_launch_step(const DevStepArgs<R> &a, int blocks, int n_seg, size_t lds,
-                           hipStream_t st) {
+hipError_t dev_launch_step(const DevStepArgs<R> &a, const DevContactArgs<R> &ca, int blocks,
+                           int n_seg, size_t lds, hipStream_t st) {
   if (lds > (size_t)kDevMaxLds)
     return hipErrorInvalidValue;
   if (lds > 65536) {
@@ -1901,18 +1920,11 @@ hipError_t dev_launch_step(const DevStepArgs<R> &a, int blocks, int n_seg, size_
     }
   }
   hipLaunchKernelGGL((k_dev_step<R>), dim3(blocks), dim3(DevStepShape<R>::threads), lds, st, a);
-  // the ranks read only the slots, segments and stage (no precision-typed
-  // field): one instantiation serves both precisions
-  DevStepArgs<float> ra;
-  std::memset(&ra, 0, sizeof ra);
-  ra.state = a.state;
-  ra.n_slots = a.n_slots;
-  ra.seg_cnt = a.seg_cnt;
-  ra.seg_nst = a.seg_nst;
-  ra.stage = a.stage;
-  ra.pslot = a.pslot;
-  hipLaunchKernelGGL(k_dev_rank, dim3((unsigned)(n_seg / kDevStepWaves)), dim3(kDevStepThreads),
-                     sizeof(uint32_t) * kDevStepWaves * (size_t)a.n_slots, st, ra);
+  const int n_rank = n_seg / kDevStepWaves;
+  const int node_blocks = (ca.g.nn + kDevStepThreads - 1) / kDevStepThreads;
+  hipLaunchKernelGGL((k_dev_rank_rows<R>), dim3((unsigned)(n_rank + node_blocks * ca.n_chunks)),
+                     dim3(kDevStepThreads), sizeof(uint32_t) * kDevStepWaves * (size_t)a.n_slots,
+                     st, a, ca, n_rank);
   return hipGetLastError();
 }
 
@@ -1936,8 +1948,6 @@ hipError_t dev_launch_contacts(const DevContactArgs<R> &a, long long max_inj, hi
 
 template <typename R>
 hipError_t dev_launch_reduce(const DevContactArgs<R> &a, int commit, hipStream_t st) {
-  hipLaunchKernelGGL((k_dev_reduce_rows<R>), dim3((a.g.nn + 255) / 256, a.n_chunks), dim3(256), 0,
-                     st, a);
   hipLaunchKernelGGL((k_dev_reduce<R>), dim3((a.g.nn + 255) / 256), dim3(256), 0, st, a, commit);
   return hipGetLastError();
 }
@@ -2334,7 +2344,8 @@ hipError_t dev_launch_green_expand(const double *Ainv, long long lda, const int 
 #define EMCGPU_DEV_INST(R)                                                                   \
   template hipError_t dev_launch_field<R>(const DevFieldArgs<R> *, size_t, hipStream_t);      \
   template hipError_t dev_launch_efield<R>(const DevFieldArgs<R> *, int, hipStream_t);        \
-  template hipError_t dev_launch_step<R>(const DevStepArgs<R> &, int, int, size_t, hipStream_t); \
+  template hipError_t dev_launch_step<R>(const DevStepArgs<R> &, const DevContactArgs<R> &, int, \
+                                         int, size_t, hipStream_t);                           \
   template hipError_t dev_launch_drift<R>(const DevStepArgs<R> &, int, hipStream_t);          \
   template hipError_t dev_launch_contacts<R>(const DevContactArgs<R> &, long long,           \
                                              hipStream_t);                                    \
