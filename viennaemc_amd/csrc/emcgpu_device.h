@@ -238,7 +238,6 @@ constexpr int kDevSegPerCU = 16;       // contact-rank segments per CU (4 measur
 // 8 (its registers would spill at 16)
 template <typename R>
 struct DevStepShape {
-  static constexpr int ppt = 1;
   static constexpr int threads = sizeof(R) == 4 ? 1024 : 512;
   static constexpr int waves = threads / 64;
 };

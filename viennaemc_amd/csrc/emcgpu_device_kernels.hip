@@ -37,9 +37,10 @@ __device__ __forceinline__ int pos_coord(R p, R h) {
 }
 
 template <typename R>
-__device__ __forceinline__ void acc_at(const DevGeom<R> &g, int ix, int iy, R &ax, R &ay) {
+__device__ __forceinline__ void acc_at(const DevGeom<R> &g, const R *acc, int ix, int iy, R &ax,
+                                       R &ay) {
   if ((unsigned)ix < (unsigned)g.nx && (unsigned)iy < (unsigned)g.ny) {
-    const R *q = g.acc + 2 * ((long long)iy * g.nx + ix);
+    const R *q = acc + 2 * (iy * g.nx + ix);
     ax = q[0];
     ay = q[1];
   } else {
@@ -48,15 +49,16 @@ __device__ __forceinline__ void acc_at(const DevGeom<R> &g, int ix, int iy, R &a
   }
 }
 
-// interpolateForce of the schemes on the grid g.acc (nodes outside the grid
-// count zero): NGP emcNGPScheme.hpp:55-68, CIC emcCICScheme.hpp:122-141
-// with the (1 - wY)(1 - wY) weight of its last node, NEC emcNECScheme.hpp:
-// 99-113
+// interpolateForce of the schemes on the force grid acc (g.acc, or a copy
+// of it in LDS; nodes outside the grid count zero): NGP emcNGPScheme.hpp:
+// 55-68, CIC emcCICScheme.hpp:122-141 with the (1 - wY)(1 - wY) weight of
+// its last node, NEC emcNECScheme.hpp:99-113
 template <typename R>
-__device__ __forceinline__ void interp(const DevGeom<R> &g, R x, R y, R &fx, R &fy) {
+__device__ __forceinline__ void interp(const DevGeom<R> &g, const R *acc, R x, R y, R &fx,
+                                       R &fy) {
 #pragma clang fp contract(off)
   if (g.pm == EMCGPU_PM_NGP) {
-    acc_at(g, pos_coord(x, g.hx), pos_coord(y, g.hy), fx, fy);
+    acc_at(g, acc, pos_coord(x, g.hx), pos_coord(y, g.hy), fx, fy);
     return;
   }
   const R sx = Math<R>::div(x, g.hx), sy = Math<R>::div(y, g.hy);
@@ -65,22 +67,26 @@ __device__ __forceinline__ void interp(const DevGeom<R> &g, R x, R y, R &fx, R &
   if (g.pm == EMCGPU_PM_CIC) {
     const R wx = sx - flx, wy = sy - fly;
     R a0x, a0y, a1x, a1y, a2x, a2y, a3x, a3y;
-    acc_at(g, mx, my, a0x, a0y);
-    acc_at(g, mx + 1, my, a1x, a1y);
-    acc_at(g, mx, my + 1, a2x, a2y);
-    acc_at(g, mx + 1, my + 1, a3x, a3y);
+    acc_at(g, acc, mx, my, a0x, a0y);
+    acc_at(g, acc, mx + 1, my, a1x, a1y);
+    acc_at(g, acc, mx, my + 1, a2x, a2y);
+    acc_at(g, acc, mx + 1, my + 1, a3x, a3y);
     fx = a0x * wx * wy + a1x * (R(1) - wx) * wy + a2x * wx * (R(1) - wy) +
          a3x * (R(1) - wy) * (R(1) - wy);
     fy = a0y * wx * wy + a1y * (R(1) - wx) * wy + a2y * wx * (R(1) - wy) +
          a3y * (R(1) - wy) * (R(1) - wy);
   } else {
     R e0x, e0y, e1x, e1y, e2x, e2y;
-    acc_at(g, mx, my, e0x, e0y);
-    acc_at(g, mx, my + 1, e1x, e1y);
-    acc_at(g, mx + 1, my, e2x, e2y);
+    acc_at(g, acc, mx, my, e0x, e0y);
+    acc_at(g, acc, mx, my + 1, e1x, e1y);
+    acc_at(g, acc, mx + 1, my, e2x, e2y);
     fx = (e0x + e1x) / R(2);
     fy = (e0y + e2y) / R(2);
   }
+}
+template <typename R>
+__device__ __forceinline__ void interp(const DevGeom<R> &g, R x, R y, R &fx, R &fy) {
+  interp(g, g.acc, x, y, fx, fy);
 }
 
 // assignToMesh of one particle (one carrier): add(ix, iy, weight) for the
@@ -189,12 +195,12 @@ __device__ __forceinline__ double vx_dev(const InstDev<R> *P, int v, int sub, R 
 // finally tau -= dt
 template <typename R, int MODE>
 __device__ __forceinline__ bool move_dev(const InstDev<R> *P, const R *__restrict__ cum,
-                                         const DevGeom<R> &g, R &x, R &y, R &kx, R &ky, R &kz,
-                                         R &E, R &tau, int &v, int &sub, StepRng<R, MODE> &rng,
-                                         int &nev) {
+                                         const DevGeom<R> &g, const R *acc, R &x, R &y, R &kx,
+                                         R &ky, R &kz, R &E, R &tau, int &v, int &sub,
+                                         StepRng<R, MODE> &rng, int &nev) {
   const R dt = P->dt;
   R ax, ay;
-  interp(g, x, y, ax, ay);
+  interp(g, acc, x, y, ax, ay);
   bool removed;
   uint32_t e = 0;
   int nreal = 0;
@@ -208,7 +214,7 @@ __device__ __forceinline__ bool move_dev(const InstDev<R> *P, const R *__restric
     while (rem > R(0) && !removed && e < kMaxEventsPerStep) {
       const R newTau = scatter_event<R, MODE>(P, cum, kx, ky, kz, E, v, sub, e, rng, nreal);
       tau += newTau;
-      interp(g, x, y, ax, ay);
+      interp(g, acc, x, y, ax, ay);
       removed = drift_dev(P, g, (newTau < rem) ? newTau : rem, ax, ay, x, y, kx, ky, kz, E, v,
                           sub);
       rem -= newTau;
@@ -291,7 +297,7 @@ __global__ __launch_bounds__(256) void k_dev_drift(DevStepArgs<R> a) {
     if (!out) {
       if constexpr (MODE == 0)
         rng.step = a.step + (uint32_t)s;
-      const bool removed = move_dev<R, MODE>(P, a.cum, a.g, x, y, kx, ky, kz, E, tau, v, sub,
+      const bool removed = move_dev<R, MODE>(P, a.cum, a.g, a.g.acc, x, y, kx, ky, kz, E, tau, v, sub,
                                              rng, nev);
       if constexpr (MODE == 1) {
         // grain bookkeeping (emcBasicParticleHandler.hpp:134-138), no grain
@@ -348,17 +354,14 @@ __device__ __forceinline__ void segment_range(long long n, long long n_seg, long
 }
 
 // The step kernel runs one block per CU on a contiguous range of chunks of
-// 64 * PPT consecutive particles (particle j of lane t of a chunk is
-// chunk_first + 64 j + t); the block's waves take the chunks one at a time
-// from an LDS counter, so waves that progress faster take more of them.
-// The PPT particles of a lane move together: their force gathers are in
-// flight at once, then their first flights, then the lane works through its
-// particles' scatter events one at a time (lowest j first).
+// 64 consecutive particles (one per lane); the block's waves take the
+// chunks one at a time from an LDS counter, so waves that progress faster
+// take more of them, and load the state of their next chunk while the
+// current one moves.  (A copy of the force grid in LDS measured slower: the
+// copy costs more than the L2 gathers it saves.)
 template <typename R>
 __global__ __launch_bounds__(DevStepShape<R>::threads) void k_dev_step(DevStepArgs<R> a) {
-  constexpr int PPT = DevStepShape<R>::ppt, NT = DevStepShape<R>::threads;
-  constexpr int NW = DevStepShape<R>::waves;
-  constexpr long long CH = 64LL * PPT;
+  constexpr int NT = DevStepShape<R>::threads, NW = DevStepShape<R>::waves;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   __shared__ int ex_cnt[kMaxContacts];
   __shared__ double vx_w[NW];
@@ -377,11 +380,10 @@ __global__ __launch_bounds__(DevStepShape<R>::threads) void k_dev_step(DevStepAr
 
   const long long n_total = a.state->n_total;
   const uint32_t stw = (uint32_t)a.state->step;
-  const long long n_chunks = (n_total + CH - 1) / CH;
+  const long long n_chunks = (n_total + 63) / 64;
   const long long c_lo = n_chunks * blockIdx.x / gridDim.x;
   const long long c_hi = n_chunks * (blockIdx.x + 1) / gridDim.x;
   const InstDev<R> *P = a.inst;
-  const R dt = P->dt;
   double vxs = 0.0;  // channel current: this lane's sum of v_x
   auto take = [&]() -> long long {
     int t = 0;
@@ -389,156 +391,80 @@ __global__ __launch_bounds__(DevStepShape<R>::threads) void k_dev_step(DevStepAr
       t = atomicAdd(&q_next, 1);
     return c_lo + __shfl(t, 0, 64);
   };
-  for (long long c = take(); c < c_hi; c = take()) {
-    const long long i0 = c * CH + lane;
-    R x[PPT], y[PPT], kx[PPT], ky[PPT], kz[PPT], E[PPT], tau[PPT], rem[PPT];
-    int v[PPT], sub[PPT];
-    uint32_t pid[PPT], ev[PPT];
-    bool live[PPT], gone[PPT];
-#pragma unroll
-    for (int j = 0; j < PPT; ++j) {
-      const long long i = i0 + 64LL * j;
-      const uint32_t vs = i < n_total ? a.p.vs[i] : kHole;
-      live[j] = vs != kHole;
-      gone[j] = false;
-      v[j] = (int)(vs >> 16);
-      sub[j] = (int)(vs & 0xffffu);
-      x[j] = y[j] = kx[j] = ky[j] = kz[j] = E[j] = tau[j] = rem[j] = R(0);
-      pid[j] = 0u;
-      ev[j] = 0u;
-      if (live[j]) {
-        x[j] = a.p.x[i];
-        y[j] = a.p.y[i];
-        if (a.move) {
-          kx[j] = a.p.kx[i];
-          ky[j] = a.p.ky[i];
-          kz[j] = a.p.kz[i];
-          E[j] = a.p.E[i];
-          tau[j] = a.p.tau[i];
-          pid[j] = a.p.pid[i];
-        }
+  struct Part {
+    R x, y, kx, ky, kz, E, tau;
+    uint32_t vs, pid;
+  };
+  auto load = [&](long long c, Part &q) {
+    const long long i = c * 64 + lane;
+    q.vs = (c < c_hi && i < n_total) ? a.p.vs[i] : kHole;
+    if (q.vs != kHole) {
+      q.x = a.p.x[i];
+      q.y = a.p.y[i];
+      if (a.move) {
+        q.kx = a.p.kx[i];
+        q.ky = a.p.ky[i];
+        q.kz = a.p.kz[i];
+        q.E = a.p.E[i];
+        q.tau = a.p.tau[i];
+        q.pid = a.p.pid[i];
       }
     }
-    if (a.move) {
-      // driftScatterParticles (emcBasicParticleHandler.hpp:100-131): force at
-      // the current position, drift min(tau, dt); while time remains and the
-      // particle is inside: scatter, tau += tau', force again, drift
-      // min(rem, tau'), rem -= tau'; finally tau -= dt
-      R ax[PPT], ay[PPT];
-#pragma unroll
-      for (int j = 0; j < PPT; ++j) {
-        ax[j] = ay[j] = R(0);
-        if (live[j])
-          interp(a.g, x[j], y[j], ax[j], ay[j]);
-      }
-#pragma unroll
-      for (int j = 0; j < PPT; ++j)
-        if (live[j]) {
-          const bool whole = tau[j] >= dt;
-          gone[j] = drift_dev(P, a.g, whole ? dt : tau[j], ax[j], ay[j], x[j], y[j], kx[j], ky[j],
-                              kz[j], E[j], v[j], sub[j]);
-          rem[j] = whole ? R(0) : dt - tau[j];
-        }
-      StepRng<R, 0> rng;
-      rng.step = stw;
-      rng.inst = kDomMove;
-      rng.k0 = a.key0;
-      rng.k1 = a.key1;
-      int nreal = 0;
-      for (;;) {
-        // every pass draws tau' > 0, so rem strictly decreases; the event
-        // cap bounds the loop regardless
-        int sel = -1;
-#pragma unroll
-        for (int j = PPT - 1; j >= 0; --j)
-          if (live[j] && !gone[j] && rem[j] > R(0) && ev[j] < kMaxEventsPerStep)
-            sel = j;
-        if (sel < 0)
-          break;
-        R px = x[0], py = y[0], qx = kx[0], qy = ky[0], qz = kz[0], en = E[0], ta = tau[0],
-          rm = rem[0];
-        int vv = v[0], ss = sub[0];
-        uint32_t ee = ev[0], pp = pid[0];
-#pragma unroll
-        for (int j = 1; j < PPT; ++j)
-          if (sel == j) {
-            px = x[j];
-            py = y[j];
-            qx = kx[j];
-            qy = ky[j];
-            qz = kz[j];
-            en = E[j];
-            ta = tau[j];
-            rm = rem[j];
-            vv = v[j];
-            ss = sub[j];
-            ee = ev[j];
-            pp = pid[j];
-          }
-        rng.pid = pp;
-        const R newTau = scatter_event<R, 0>(P, a.cum, qx, qy, qz, en, vv, ss, ee, rng, nreal);
-        ta += newTau;
-        R fx, fy;
-        interp(a.g, px, py, fx, fy);
-        const bool out = drift_dev(P, a.g, (newTau < rm) ? newTau : rm, fx, fy, px, py, qx, qy, qz,
-                                   en, vv, ss);
-        rm -= newTau;
-#pragma unroll
-        for (int j = 0; j < PPT; ++j)
-          if (sel == j) {
-            x[j] = px;
-            y[j] = py;
-            kx[j] = qx;
-            ky[j] = qy;
-            kz[j] = qz;
-            E[j] = en;
-            tau[j] = ta;
-            rem[j] = rm;
-            v[j] = vv;
-            sub[j] = ss;
-            ev[j] = ee;
-            gone[j] = out;
-          }
-      }
-#pragma unroll
-      for (int j = 0; j < PPT; ++j)
-        tau[j] -= dt;
-    }
-#pragma unroll
-    for (int j = 0; j < PPT; ++j) {
-      const long long i = i0 + 64LL * j;
-      if (i >= n_total)
-        continue;
-      int slot = -1;
-      if (live[j]) {
-        if (gone[j]) {
+  };
+  long long c = take();
+  Part nxt;
+  load(c, nxt);
+  while (c < c_hi) {
+    const long long cn = take();
+    const Part cur = nxt;
+    load(cn, nxt);
+    const long long i = c * 64 + lane;
+    int slot = -1;
+    if (cur.vs != kHole) {
+      R x = cur.x, y = cur.y;
+      bool removed = false;
+      if (a.move) {
+        R kx = cur.kx, ky = cur.ky, kz = cur.kz, E = cur.E, tau = cur.tau;
+        int v = (int)(cur.vs >> 16), sub = (int)(cur.vs & 0xffffu);
+        StepRng<R, 0> rng;
+        rng.step = stw;
+        rng.pid = cur.pid;
+        rng.inst = kDomMove;
+        rng.k0 = a.key0;
+        rng.k1 = a.key1;
+        int nev;
+        removed =
+            move_dev<R, 0>(P, a.cum, a.g, a.g.acc, x, y, kx, ky, kz, E, tau, v, sub, rng, nev);
+        if (removed) {
           a.p.vs[i] = kHole;
-          const int cidx = a.g.cnode[pos_coord(y[j], a.g.hy) * a.g.nx + pos_coord(x[j], a.g.hx)];
+          const int cidx = a.g.cnode[pos_coord(y, a.g.hy) * a.g.nx + pos_coord(x, a.g.hx)];
           atomicAdd(&ex_cnt[cidx], 1);
         } else {
-          if (a.move) {
-            a.p.x[i] = x[j];
-            a.p.y[i] = y[j];
-            a.p.kx[i] = kx[j];
-            a.p.ky[i] = ky[j];
-            a.p.kz[i] = kz[j];
-            a.p.E[i] = E[j];
-            a.p.tau[i] = tau[j];
-            a.p.vs[i] = ((uint32_t)v[j] << 16) | (uint32_t)sub[j];
-            if (a.chan && x[j] >= a.chan_x0 && x[j] <= a.chan_x1)
-              vxs += vx_dev(P, v[j], sub[j], kx[j], ky[j], kz[j], E[j]);
-          }
-          if (a.lds_dep)
-            assign(a.g, x[j], y[j], [&](int k, R w) { atomicAdd(&grid[k], w); });
-          else
-            assign(a.g, x[j], y[j], [&](int k, R w) { atomicAdd(&a.dep[k], w); });
-          const int cx = pos_coord(x[j], a.g.hx), cy = pos_coord(y[j], a.g.hy);
-          if ((unsigned)cx < (unsigned)a.g.nx && (unsigned)cy < (unsigned)a.g.ny)
-            slot = a.g.cslot[cy * a.g.nx + cx];
+          a.p.x[i] = x;
+          a.p.y[i] = y;
+          a.p.kx[i] = kx;
+          a.p.ky[i] = ky;
+          a.p.kz[i] = kz;
+          a.p.E[i] = E;
+          a.p.tau[i] = tau;
+          a.p.vs[i] = ((uint32_t)v << 16) | (uint32_t)sub;
+          if (a.chan && x >= a.chan_x0 && x <= a.chan_x1)
+            vxs += vx_dev(P, v, sub, kx, ky, kz, E);
         }
       }
-      a.pslot[i] = (int16_t)slot;
+      if (!removed) {
+        if (a.lds_dep)
+          assign(a.g, x, y, [&](int k, R w) { atomicAdd(&grid[k], w); });
+        else
+          assign(a.g, x, y, [&](int k, R w) { atomicAdd(&a.dep[k], w); });
+        const int cx = pos_coord(x, a.g.hx), cy = pos_coord(y, a.g.hy);
+        if ((unsigned)cx < (unsigned)a.g.nx && (unsigned)cy < (unsigned)a.g.ny)
+          slot = a.g.cslot[cy * a.g.nx + cx];
+      }
     }
+    if (i < n_total)
+      a.pslot[i] = (int16_t)slot;
+    c = cn;
   }
   if (a.chan) {
 #pragma unroll
@@ -1985,6 +1911,12 @@ hipError_t dev_launch_mark(DevState *state, hipStream_t st) {
 template <typename R>
 int dev_step_blocks_per_cu(size_t lds) {
   int n = 0;
+  if (lds > 65536 &&
+      hipFuncSetAttribute((const void *)k_dev_step<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)lds) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_dev_step<R>, DevStepShape<R>::threads,
                                                    lds) !=
       hipSuccess)
