@@ -412,8 +412,9 @@ def leg_hot(emcgpu, n, warmup, steps, precision, screened=False):
 def leg_device(emcgpu, n, warmup, steps, precision, seed=SEED):
     """Self-consistent 2-D resistor: 200 x 50 nodes at 1 nm, N_D 1e17 cm^-3
     with 1e19 cm^-3 ohmic regions, 0.5 V, parabolic m* = 0.26 valley with
-    acoustic + intervalley phonons, CIC, Poisson (Newton + FP32 multigrid,
-    FP64 residual to 1e-10) every 0.1 fs step; whole-job particle-updates/s
+    acoustic + intervalley phonons, CIC, Poisson (Newton corrections from the
+    separable inverse of the discrete operator, multigrid as check and
+    fallback, FP64 residual to 1e-10) every 0.1 fs step; whole-job particle-updates/s
     (particles moved per step x steps / wall time, HIP-graph launch)."""
     from viennaemc_amd.workloads import MODELS_DEVICE, resistor_device
     dev = resistor_device(n_particles=n)
