@@ -205,6 +205,21 @@ int orc_device_is_ohmic(const orc_device *d, const size_t c[2]) {
   return ci != -1 && d->contactType[ci] == 0;
 }
 
+const orc_model *orc_device_model_at(const orc_device *d, const orc_model *m, int region) {
+  if (d->nRegionModels > 0 && region >= 0 && region < d->nRegionModels)
+    return d->regionModel[region];
+  return m;
+}
+
+int orc_device_set_region_models(orc_device *d, const orc_model *const *models, int n) {
+  if (n < 0 || n > ORC_MAX_REGIONS)
+    return -1;
+  for (int r = 0; r < n; r++)
+    d->regionModel[r] = models[r];
+  d->nRegionModels = n;
+  return 0;
+}
+
 /* ---- particle-mesh schemes ----------------------------------------------- */
 
 static int orc_valid(const int ext[2], size_t ix, size_t iy) {
@@ -557,9 +572,12 @@ int orc_device_move_particle(const orc_model *m, const orc_device *d, int pm,
     removed = orc_device_drift(m, d, p->tau, p, pos, force);
     double tRem = tStep - p->tau;
     while (tRem > 0 && !removed) {
-      orc_scatter(m, p, rng);
+      /* the tables of the particle's (valley, region), the region of its
+       * last drift (driftParticle, emcAbstractParticleHandler.hpp:244-247) */
+      const orc_model *mr = orc_device_model_at(d, m, p->region);
+      orc_scatter(mr, p, rng);
       nev++;
-      double newTau = orc_new_tau(m, p->valley, rng);
+      double newTau = orc_new_tau(mr, p->valley, rng);
       p->tau += newTau;
       orc_pm_force(pm, ex, ey, pos, d->h, d->ext, charge, force);
       removed = orc_device_drift(m, d, orc_min_d(tRem, newTau), p, pos, force);
@@ -637,7 +655,7 @@ static void orc_device_add_particle(const orc_model *m, const orc_device *d,
   for (int i = 0; i < 2; i++)
     if ((c[i] == 0 && p.k[i] < 0) || (c[i] == (size_t)(d->ext[i] - 1) && p.k[i] > 0))
       p.k[i] *= -1;
-  p.tau = orc_new_tau(m, p.valley, rng);
+  p.tau = orc_new_tau(orc_device_model_at(d, m, p.region), p.valley, rng);
   p.grainTau = -log(orc_rng_uniform(rng, 1e-6, 1.)) * 1.0;
   orc_ens_push(e, &p, pos);
 }
@@ -905,7 +923,8 @@ int orc_device_move_particle_philox(const orc_model *m, const orc_device *d, int
     removed = orc_device_drift(m, d, p->tau, p, pos, force);
     double tRem = tStep - p->tau;
     while (tRem > 0 && !removed) {
-      double newTau = orc_philox_event(m, p, seed, ORC_DOM_MOVE, pid, step, e++);
+      double newTau = orc_philox_event(orc_device_model_at(d, m, p->region), p, seed,
+                                       ORC_DOM_MOVE, pid, step, e++);
       p->tau += newTau;
       orc_pm_force(pm, ex, ey, pos, d->h, d->ext, charge, force);
       removed = orc_device_drift(m, d, orc_min_d(tRem, newTau), p, pos, force);
@@ -953,7 +972,7 @@ static void orc_device_new_philox(const orc_model *m, const orc_device *d, const
   for (int i = 0; i < 2; i++)
     if ((c[i] == 0 && p->k[i] < 0) || (c[i] == (size_t)(d->ext[i] - 1) && p->k[i] > 0))
       p->k[i] *= -1;
-  p->tau = -log(u[7] * sc + lo) * m->tau0[p->valley];
+  p->tau = -log(u[7] * sc + lo) * orc_device_model_at(d, m, p->region)->tau0[p->valley];
   p->grainTau = 1.0;
 }
 

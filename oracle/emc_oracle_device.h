@@ -45,7 +45,19 @@ typedef struct {
    * thickness [m], barrier height [V] */
   int contactType[ORC_MAX_CONTACTS];
   double gateEps[ORC_MAX_CONTACTS], gateThick[ORC_MAX_CONTACTS], gateBarrier[ORC_MAX_CONTACTS];
+  /* per doping region, the particle type's tables of that region (the
+   * (valley, region) table sets of emcScatterHandler, emcScatterHandler.hpp:
+   * 53-68, 130-145): when set, scattering and getNewTau of a particle use
+   * regionModel[particle.region]; otherwise the run's single model */
+  const orc_model *regionModel[ORC_MAX_REGIONS];
+  int nRegionModels;
 } orc_device;
+
+/* the model of a particle in region r (the run's model m without region
+ * models) */
+const orc_model *orc_device_model_at(const orc_device *d, const orc_model *m, int region);
+/* region models, n <= ORC_MAX_REGIONS (n = 0: none) */
+int orc_device_set_region_models(orc_device *d, const orc_model *const *models, int n);
 
 /* emcDevice ctor (emcDevice.hpp:83-99), maxPosToExtent (emcUtil.hpp:112-121) */
 void orc_device_init(orc_device *d, double maxX, double maxY, double hx,

@@ -106,6 +106,7 @@ def load():
         "orc_device_add_doping": (None, [vp, _PD, _PD, _D]),
         "orc_device_add_ohmic": (_I, [vp, _I, _D, _D, _D]),
         "orc_device_add_gate": (_I, [vp, _I, _D, _D, _D, _D, _D, _D]),
+        "orc_device_set_region_models": (_I, [vp, C.c_void_p, _I]),
         "orc_device_info": (None, [vp, _PI32, _PD]),
         "orc_device_get_grids": (None, [vp, C.c_void_p, C.c_void_p, C.c_void_p]),
         "orc_sor_solve": (C.c_longlong, [vp, _PD, C.c_void_p, _D, _D, _I]),
@@ -508,6 +509,15 @@ class OracleDevice:
             self.L.orc_device_delete(self.h)
         except Exception:
             pass
+
+    def set_region_models(self, models):
+        """per doping region the particle type's tables of that region
+        (OracleModel list; [] clears): scattering and new free-flight times
+        of a particle then use the model of its region"""
+        self._region_models = list(models)  # keep the handles alive
+        arr = (C.c_void_p * max(len(models), 1))(*[m.h for m in models])
+        if self.L.orc_device_set_region_models(self.h, C.cast(arr, C.c_void_p), len(models)) != 0:
+            raise ValueError("too many region models")
 
     def grids(self):
         d = np.zeros(self.n_nodes)

@@ -821,15 +821,22 @@ static std::unique_ptr<Device2> makeDevice2() {
   return dev;
 }
 
-static std::unique_ptr<Electron2> makeDeviceType(Device2 &dev, bool usePotential = false) {
+// regionTables: the intervalley absorption only in the channel (region 0),
+// the emission in the channel and the drain region (2), so the three doping
+// regions have three different (valley, region) table sets
+static std::unique_ptr<Electron2> makeDeviceType(Device2 &dev, bool usePotential = false,
+                                                 bool regionTables = false) {
   auto type = std::make_unique<Electron2>(1000, 1.5, usePotential);
   type->addValley(std::make_unique<emcParabolicIsotropValley<T>>(0.26, type->getMass(), 1));
   std::map<SizeType, std::vector<SizeType>> self = {{0, {0}}};
   using ZAb = emcZeroOrderInterValleyAbsorptionScatterMechanism<T>;
   using ZEm = emcZeroOrderInterValleyEmissionScatterMechanism<T>;
-  type->addScatterMechanism({0, 1, 2}, std::make_unique<emcAcousticScatterMechanism<T>>(0, 9., dev));
-  type->addScatterMechanism({0, 1, 2}, std::make_unique<ZAb>("P", 0, self, 5e10, 0.05, dev));
-  type->addScatterMechanism({0, 1, 2}, std::make_unique<ZEm>("P", 0, self, 5e10, 0.05, dev));
+  const std::vector<int> all = {0, 1, 2};
+  const std::vector<int> ab = regionTables ? std::vector<int>{0} : all;
+  const std::vector<int> em = regionTables ? std::vector<int>{0, 2} : all;
+  type->addScatterMechanism(all, std::make_unique<emcAcousticScatterMechanism<T>>(0, 9., dev));
+  type->addScatterMechanism(ab, std::make_unique<ZAb>("P", 0, self, 5e10, 0.05, dev));
+  type->addScatterMechanism(em, std::make_unique<ZEm>("P", 0, self, 5e10, 0.05, dev));
   return type;
 }
 
@@ -844,11 +851,12 @@ static const double kChan[3] = {5e-9, 15e-9, 10e-9};
 
 template <class PM>
 static void dumpDeviceRun(const std::string &tag, uint64_t seed, int nSteps, double dt,
-                          double accV, double omega, int interval = 1, bool usePotential = false) {
+                          double accV, double omega, int interval = 1, bool usePotential = false,
+                          bool regionTables = false) {
   auto dev = makeDevice2();
   PM pm;
   TypeMap2 types;
-  types[0] = makeDeviceType(*dev, usePotential);
+  types[0] = makeDeviceType(*dev, usePotential, regionTables);
   using Handler2 = emcBasicParticleHandler<T, Device2, PM>;
   using Solver2 = emcSORSolver<T, Device2, Handler2>;
   Handler2 handler(*dev, pm, types, 1, seed);
@@ -1235,6 +1243,8 @@ int main(int argc, char **argv) {
   dumpDeviceRun<emcNGPScheme<T, Device2>>("ngp", seed + 13, 40, 1e-15, 1e-8, 1.8);
   dumpDeviceRun<emcCICScheme<T, Device2>>("cic_sub3", seed + 17, 40, 1e-15, 1e-8, 1.8, 3);
   dumpDeviceRun<emcCICScheme<T, Device2>>("cic_potinit", seed + 23, 20, 1e-15, 1e-8, 1.8, 1, true);
+  dumpDeviceRun<emcCICScheme<T, Device2>>("cic_regions", seed + 29, 40, 1e-15, 1e-8, 1.8, 1, false,
+                                          true);
   dumpSimulationFiles(seed + 19);
   for (const auto &m : models) {
     double mp = 4e-8;

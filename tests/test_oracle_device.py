@@ -25,7 +25,7 @@ solves the discrete Poisson equation.
 import numpy as np
 import pytest
 
-from emc_models import DEVICE_GATED, DEVICE_SMALL, MODELS_DEVICE
+from emc_models import DEVICE_GATED, DEVICE_SMALL, MODELS_DEVICE, device_region_specs
 from oracle import oracle_py as O
 
 
@@ -219,6 +219,30 @@ def test_device_simulation_bit_exact(dev, golden, pm, tag):
     assert ((out["injs"] - out["dels"]).reshape(-1) == golden[b + "ohmic"]).all()
     assert (out["nr_part"] == golden[b + "nrpart_last"]).all()
     assert out["exits"].sum() > 20 and out["injs"].sum() > 20
+
+
+def test_device_simulation_with_region_tables_bit_exact(golden):
+    """Mechanisms registered for some doping regions only: each region has its
+    own (valley, region) tables and tau0, and a particle scatters with the
+    tables of the region of its last drift."""
+    b = "device.cic_regions."
+    seed, nsteps, dt, acc, omega, interval, pot_init = golden[b + "cfg"]
+    dev = O.OracleDevice(DEVICE_SMALL)
+    models = [O.OracleModel(sp) for sp in device_region_specs()]
+    assert len({M.tau0(0) for M in models}) == 3
+    dev.set_region_models(models)
+    out = dev.run(models[0], O.PM_CIC, acc, omega, dt, int(nsteps), int(seed), channel=CHANNEL)
+    assert (out["chan"] == golden[b + "chan"]).all()
+    assert out["n_init"][0] == golden[b + "ninit"][0]
+    assert (out["pot"].reshape(-1) == golden[b + "pot"]).all()
+    assert (out["n_part"] == golden[b + "npart"]).all()
+    assert (out["exits"].reshape(-1) == golden[b + "exits"]).all()
+    assert ((out["injs"] - out["dels"]).reshape(-1) == golden[b + "ohmic"]).all()
+    assert (out["nr_part"] == golden[b + "nrpart_last"]).all()
+    # one table set for every region gives another run
+    dev.set_region_models([])
+    other = dev.run(models[0], O.PM_CIC, acc, omega, dt, int(nsteps), int(seed), channel=CHANNEL)
+    assert not (other["chan"] == golden[b + "chan"]).all()
 
 
 def test_contact_rule_keeps_the_first_in_array_order(dev):

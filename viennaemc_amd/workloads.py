@@ -286,6 +286,15 @@ MODELS_DEVICE = {
                dict(kind=ZERO_IV_EM, valley=0, final=0, defpot=5e10, hw=0.05, fmap=[[0]])]),
 }
 
+
+def device_region_specs(spec=None):
+    """The device type with per-region scatter tables (addScatterMechanism
+    with region lists): for doping regions 0 (the channel), 1 (source) and 2
+    (drain) of DEVICE_SMALL the mechanisms registered there, in registration
+    order: all three; acoustic only; acoustic and intervalley emission."""
+    spec = spec or MODELS_DEVICE["device_para"]
+    return [dict(spec, mechs=[spec["mechs"][i] for i in idx]) for idx in ([0, 1, 2], [0], [0, 2])]
+
 # the 21 x 7 node n+ n n+ resistor of the golden fixtures: doping regions
 # (min corner, max corner, doping [1/m^3]) in order, ohmic contacts
 # (boundary 0=XMIN 1=XMAX 2=YMIN 3=YMAX, voltage, min, max along the boundary)
