@@ -477,6 +477,16 @@ typedef struct {
    * carriers per node from exp(equilibrium potential) * n_i instead of the
    * doping */
   int init_from_potential;
+  /* per-region scatter tables (the (valley, region) table sets of
+   * emcScatterHandler, emcScatterHandler.hpp:53-68, 130-145): nonzero makes
+   * instance r hold the tables of doping region r (the mechanisms registered
+   * for r, addScatterMechanism(regions, ...)); a particle then scatters with
+   * the tables, and draws its free-flight times with the tau0, of the region
+   * of its position (updated after every drift, emcAbstractParticleHandler.
+   * hpp:244-247).  Every node must lie in a doping region, the context needs
+   * an instance per region, each with the valleys of instance 0 and its
+   * tables built.  0: instance 0's tables everywhere. */
+  int region_tables;
 } emcgpu_device_desc;
 
 /* geometry (host) and device grids; call after emcgpu_set_model and

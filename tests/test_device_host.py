@@ -16,7 +16,8 @@ import tempfile
 
 import pytest
 
-from emc_models import DEVICE_SMALL, MODELS_DEVICE, SI_EPS_R, SI_NI, resistor_device
+from emc_models import (DEVICE_SMALL, MODELS_DEVICE, SI_EPS_R, SI_NI, device_region_specs,
+                        resistor_device)
 from oracle import oracle_py as O
 from viennaemc_amd import emcgpu
 
@@ -120,3 +121,23 @@ def test_ctypes_layouts_match_the_c_compiler():
         if "." in key:
             t, f = key.split(".")
             assert getattr(py[t], f).offset == int(val), key
+
+
+def test_region_tables_need_an_instance_and_a_region_per_node():
+    """Per-region tables: instance r holds the tables of doping region r, so
+    the context needs an instance per region and every node a region."""
+    specs = device_region_specs()
+    ctx = emcgpu.Context(precision=emcgpu.FP64, n_instances=2, device=-1)
+    for r, sp in enumerate(specs[:2]):
+        ctx.setup(r, sp)
+    with pytest.raises(emcgpu.EmcGpuError, match="instance per doping region"):
+        ctx.device_setup(DEVICE_SMALL, region_tables=True)
+    ctx.close()
+    ctx = emcgpu.Context(precision=emcgpu.FP64, n_instances=3, device=-1)
+    for r, sp in enumerate(specs):
+        ctx.setup(r, sp)
+    ctx.device_setup(DEVICE_SMALL, region_tables=True)
+    holes = dict(DEVICE_SMALL, doping=DEVICE_SMALL["doping"][1:])  # the channel has no region
+    with pytest.raises(emcgpu.EmcGpuError, match="lies in no doping region"):
+        ctx.device_setup(holes, region_tables=True)
+    ctx.close()

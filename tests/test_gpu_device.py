@@ -27,7 +27,8 @@
 import numpy as np
 import pytest
 
-from emc_models import DEVICE_GATED, DEVICE_SMALL, MODELS_DEVICE, resistor_device
+from emc_models import (DEVICE_GATED, DEVICE_SMALL, MODELS_DEVICE, device_region_specs,
+                        resistor_device)
 from oracle import oracle_py as O
 
 pytestmark = pytest.mark.gpu
@@ -447,6 +448,48 @@ def test_gated_run_matches_oracle_twin(gates, mode):
     ctx.close()
 
 
+def test_region_tables_run_matches_oracle_twin():
+    """Mechanisms registered for some doping regions only (region_tables:
+    instance r holds the tables of region r): the loop against the oracle's
+    Philox twin with the same per-region tables, counts per step exact and
+    the final ensemble within 1e-9; instance 0's tables everywhere give
+    another run."""
+    emcgpu = _emc()
+    dev = dict(DEVICE_SMALL, width=1e-5)
+    seed, ns, dt = 2468, 50, 1e-15
+    specs = device_region_specs()
+    od = O.OracleDevice(dev)
+    models = [O.OracleModel(sp) for sp in specs]
+    od.set_region_models(models)
+    ref = od.run_philox(models[0], O.PM_CIC, 1e-14, 1.8, dt, ns, seed, O.thermal_voltage(300.0),
+                        max_state=100000)
+    print("region-table twin: %d particles, %d steps" % (ref["n_init"][0], ns), flush=True)
+    res = {}
+    for per_region in (True, False):
+        ctx = emcgpu.Context(precision=0, n_instances=3)
+        for r, sp in enumerate(specs):
+            ctx.setup(r, sp)
+        ctx.device_setup(dev, pm=emcgpu.PM_CIC, poisson_tol=1e-14, poisson_max_iter=80,
+                         region_tables=per_region)
+        assert ctx.device_init(seed) == ref["n_init"][0]
+        out = ctx.device_step(dt, ns, seed=seed)
+        res[per_region] = (out, ctx.device_get_particles())
+        ctx.close()
+    out, p = res[True]
+    assert (out["n_particles"] == ref["n_part"]).all()
+    assert (out["exits"] == ref["exits"]).all()
+    assert (out["deleted"] == ref["dels"]).all()
+    assert (out["injected"] == ref["injs"]).all()
+    st = ref["state"]
+    assert p["E"].size == st.shape[0]
+    dpos = np.abs(p["pos"] - st[:, 0:2]).max() / 20e-9
+    dE = (np.abs(p["E"] - st[:, 5]) / st[:, 5]).max()
+    print("final ensemble: |dpos| %.2e, |dE| %.2e" % (dpos, dE), flush=True)
+    assert dpos < 1e-9 and dE < 1e-9
+    other = res[False][1]
+    assert other["E"].size != st.shape[0] or np.abs(other["E"] - st[:, 5]).max() > 1e-6
+
+
 @pytest.mark.parametrize("region", ["middle", "whole"])
 def test_poisson_interval_and_channel_current_match_oracle_twin(region):
     """setPoissonInterval(3) + setChannelCurrentRegion against the oracle's
@@ -575,6 +618,39 @@ def test_cpp_simulation_writes_the_reference_results(golden, tmp_path):
         assert np.allclose(r[3:5], cur[s], rtol=1e-5, atol=0), s
     assert "drift current" in txt
     ctx.close()
+
+
+def test_cpp_simulation_with_region_tables(tmp_path):
+    """resistor2DGpu with mechanisms registered for some doping regions only:
+    the C++ host API gives each region its own tables (one instance per
+    region), and its per-step contact tallies equal the Python binding's run
+    with the same per-region tables."""
+    import os
+    import subprocess
+    emcgpu = _emc()
+    exe = os.path.join(os.path.dirname(emcgpu.__file__), "examples", "resistor2DGpu")
+    seed = 4141
+    prefix = str(tmp_path / "sim_")
+    print("cpp device example with region tables", flush=True)
+    subprocess.check_output([exe, prefix, "small", "fp64", str(seed), "regions"],
+                            env=dict(os.environ, EMCGPU_THROW="1"), timeout=120)
+    rows = [[float(x) for x in l.split()]
+            for l in (tmp_path / "sim_ElectronsCurrent.txt").read_text().strip().split("\n")]
+    nettos = {}
+    for per_region in (True, False):
+        ctx = emcgpu.Context(precision=emcgpu.FP64, n_instances=3)
+        for r, sp in enumerate(device_region_specs()):
+            ctx.setup(r, sp)
+        ctx.device_setup(DEVICE_SMALL, pm=emcgpu.PM_CIC, poisson_tol=1e-10, poisson_max_iter=50,
+                         region_tables=per_region)
+        ctx.device_init(seed)
+        out = ctx.device_step(1e-15, 30, seed=seed, poisson_interval=2)
+        nettos[per_region] = (out["exits"] - (out["injected"] - out["deleted"]))[10:]
+        ctx.close()
+    assert len(rows) == 20
+    for s_, r in enumerate(rows):
+        assert r[1:3] == nettos[True][s_].tolist(), s_
+    assert (nettos[True] != nettos[False]).any()
 
 
 def test_initial_particles_from_the_equilibrium_potential_match_oracle_twin():

@@ -295,6 +295,9 @@ struct DevGeom {
   const int *cnode;  // ohmic contact index per node, -1 elsewhere
   const int *cslot;  // contact slot per node, -1 elsewhere
   const R *acc;      // [node][2] interpolated force / hbar [1/nm per fs]
+  // per-region tables: the doping region of each node, whose instance holds
+  // the tables of that region (null: instance 0 everywhere)
+  const int8_t *region;
 };
 
 // one multigrid level: node counts, offset into the level-stacked arrays,

@@ -82,6 +82,8 @@ struct DeviceSim {
   bool has_gate = false;
   int pm = EMCGPU_PM_CIC;
   bool init_pot = false;  // initial carriers from the equilibrium potential
+  bool region_tables = false;  // instance r holds the tables of doping region r
+  int n_regions = 0;           // doping regions (instances the tables need)
   double tol = 1e-10;
   int max_iter = 50;
   // ---- per node / per ohmic node ----------------------------------------
@@ -110,6 +112,7 @@ struct DeviceSim {
   // ---- device memory ------------------------------------------------------
   bool on_gpu = false;
   int *d_cnode = nullptr, *d_cslot = nullptr;
+  int8_t *d_region = nullptr;
   signed char *d_bcx = nullptr, *d_bcy = nullptr;
   uint8_t *d_dmask = nullptr;
   double *d_dopn = nullptr, *d_bc_eq = nullptr, *d_bc_neq = nullptr, *d_cfac = nullptr;
@@ -214,6 +217,7 @@ void device_sim_destroy(DeviceSim *d) {
   if (d->on_gpu) {
     dev_free_particles(d);
     dfreet(d->d_cnode);
+    dfreet(d->d_region);
     dfreet(d->d_cslot);
     dfreet(d->d_bcx);
     dfreet(d->d_bcy);
@@ -343,6 +347,7 @@ emcgpu_status build_geometry(emcgpu_ctx *c, DeviceSim &d, const emcgpu_device_de
   d.tol = g.poisson_tol > 0 ? g.poisson_tol : 1e-10;
   d.max_iter = g.poisson_max_iter > 0 ? g.poisson_max_iter : 50;
   d.init_pot = g.init_from_potential != 0;
+  d.region_tables = g.region_tables != 0;
   // emcDevice ctor (emcDevice.hpp:83-99): thermal voltage, intrinsic Debye
   // length; calcCellVolume (:333-343)
   d.Vt = phys::kB / phys::q * d.T;
@@ -381,6 +386,16 @@ emcgpu_status build_geometry(emcgpu_ctx *c, DeviceSim &d, const emcgpu_device_de
           d.doping[idx(d, ix, iy)] = dr.doping;
           d.region[idx(d, ix, iy)] = r;
         }
+  }
+  d.n_regions = g.n_doping;
+  if (d.region_tables) {
+    if (g.n_doping > c->n_inst || g.n_doping > 127)
+      return fail(c, EMCGPU_ERR_ARG,
+                  "per-region tables need an instance per doping region (%d regions, %d instances)",
+                  g.n_doping, c->n_inst);
+    for (int k = 0; k < d.nn; k++)
+      if (d.region[k] < 0)
+        return fail(c, EMCGPU_ERR_ARG, "per-region tables: node %d lies in no doping region", k);
   }
   // contacts (addContact, emcSurface.hpp:199-222): an ohmic contact is
   // marked on every boundary its nodes lie on (updateAllOccurences,
@@ -790,6 +805,12 @@ emcgpu_status dput(emcgpu_ctx *c, T *&p, const std::vector<T> &v) {
 emcgpu_status upload_device(emcgpu_ctx *c, DeviceSim &d) {
   const size_t nn = d.nn, rs = real_size(c);
   ST(dput(c, d.d_cnode, d.cnode));
+  if (d.region_tables) {
+    std::vector<int8_t> reg(nn);
+    for (size_t k = 0; k < nn; k++)
+      reg[k] = (int8_t)d.region[k];
+    ST(dput(c, d.d_region, reg));
+  }
   ST(dput(c, d.d_cslot, d.cslot));
   ST(dput(c, d.d_bcx, d.bcx));
   ST(dput(c, d.d_bcy, d.bcy));
@@ -1008,6 +1029,7 @@ DevGeom<R> geom_nm(const DeviceSim &d) {
   g.cnode = d.d_cnode;
   g.cslot = d.d_cslot;
   g.acc = (const R *)d.d_acc;
+  g.region = d.region_tables ? d.d_region : nullptr;
   return g;
 }
 
@@ -1201,6 +1223,38 @@ emcgpu_status need_model(emcgpu_ctx *c, double dt_s) {
   if (c->inst[0].plasma())
     return fail(c, EMCGPU_ERR_ARG, "plasma screening is not supported in device runs");
   return upload_model(c, dt_s);
+}
+
+// per-region tables: every region's instance has tables, the valleys of
+// instance 0 and no mechanism the device path refuses
+emcgpu_status need_regions(emcgpu_ctx *c, const DeviceSim &d) {
+  if (!d.region_tables)
+    return EMCGPU_OK;
+  for (int r = 0; r < d.n_regions; r++) {
+    const auto &in = c->inst[r];
+    if (!in.has_tables)
+      return fail(c, EMCGPU_ERR_STATE, "instance %d (tables of region %d) has no tables", r, r);
+    bool same = in.valleys.size() == c->inst[0].valleys.size();
+    for (size_t v = 0; same && v < in.valleys.size(); v++) {
+      const emcgpu_valley_desc &a = in.valleys[v], &b = c->inst[0].valleys[v];
+      same = a.kind == b.kind && a.particle_mass == b.particle_mass &&
+             a.degeneracy == b.degeneracy && a.alpha == b.alpha &&
+             a.bottom_energy == b.bottom_energy &&
+             std::memcmp(a.rel_mass, b.rel_mass, sizeof a.rel_mass) == 0 &&
+             std::memcmp(a.axes, b.axes, sizeof a.axes) == 0;
+    }
+    if (!same)
+      return fail(c, EMCGPU_ERR_ARG, "instance %d: per-region tables need the valleys of instance 0",
+                  r);
+    for (const auto &m : in.mechs)
+      if (m.kind == EMCGPU_MECH_HOT_FROEHLICH_AB || m.kind == EMCGPU_MECH_HOT_FROEHLICH_EM ||
+          m.kind == EMCGPU_MECH_SCREENED_HOT_FROEHLICH_AB ||
+          m.kind == EMCGPU_MECH_SCREENED_HOT_FROEHLICH_EM)
+        return fail(c, EMCGPU_ERR_ARG, "hot-phonon mechanisms are not supported in device runs");
+    if (in.plasma())
+      return fail(c, EMCGPU_ERR_ARG, "plasma screening is not supported in device runs");
+  }
+  return EMCGPU_OK;
 }
 
 // device slot of the field-kernel arguments: 0 steps, 1 the first step
@@ -1637,6 +1691,7 @@ emcgpu_status compact(emcgpu_ctx *c, DeviceSim &d, long long n_total) {
 template <typename R>
 emcgpu_status run_device_steps(emcgpu_ctx *c, DeviceSim &d, const emcgpu_device_step_args *s) {
   ST(need_model(c, s->dt));
+  ST(need_regions(c, d));
   if (s->poisson_interval < 0)
     return fail(c, EMCGPU_ERR_ARG, "poisson_interval must be >= 0");
   d.interval = s->poisson_interval > 1 ? s->poisson_interval : 1;
@@ -1772,6 +1827,7 @@ double host_uniform(uint32_t j, uint32_t step, uint32_t pid, uint32_t dom, uint6
 template <typename R>
 emcgpu_status device_init(emcgpu_ctx *c, DeviceSim &d, uint64_t seed, size_t *n_out) {
   ST(need_model(c, c->params_dt > 0 ? c->params_dt : 1e-15));
+  ST(need_regions(c, d));
   d.seed = seed;
   // equilibrium potential (calcEquilibriumCharacteristics,
   // emcSimulation.hpp:139-146) from the charge-neutral guess asinh(C/2)

@@ -212,7 +212,11 @@ __device__ __forceinline__ bool move_dev(const InstDev<R> *P, const R *__restric
     // every pass draws tau' > 0, so rem strictly decreases; the event cap
     // bounds the loop regardless
     while (rem > R(0) && !removed && e < kMaxEventsPerStep) {
-      const R newTau = scatter_event<R, MODE>(P, cum, kx, ky, kz, E, v, sub, e, rng, nreal);
+      // the tables of the region of the last drift (driftParticle,
+      // emcAbstractParticleHandler.hpp:244-247)
+      const InstDev<R> *Pr =
+          g.region ? P + g.region[pos_coord(y, g.hy) * g.nx + pos_coord(x, g.hx)] : P;
+      const R newTau = scatter_event<R, MODE>(Pr, cum, kx, ky, kz, E, v, sub, e, rng, nreal);
       tau += newTau;
       interp(g, acc, x, y, ax, ay);
       removed = drift_dev(P, g, (newTau < rem) ? newTau : rem, ax, ay, x, y, kx, ky, kz, E, v,
@@ -260,7 +264,9 @@ __device__ __forceinline__ void new_particle(const InstDev<R> *P, const DevGeom<
     kx = -kx;
   if ((cy == 0 && ky < R(0)) || (cy == g.ny - 1 && ky > R(0)))
     ky = -ky;
-  tau = -Math<R>::log(u[7] * sc + lo) * P->tau0[v];
+  // getNewTau with the tau0 of the node's region (emcElectron.hpp:79-87)
+  const InstDev<R> *Pr = g.region ? P + g.region[cy * g.nx + cx] : P;
+  tau = -Math<R>::log(u[7] * sc + lo) * Pr->tau0[v];
   vs = ((uint32_t)v << 16) | (uint32_t)sub;
 }
 

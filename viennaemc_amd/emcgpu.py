@@ -107,7 +107,8 @@ class DeviceDesc(C.Structure):
                 ("temperature", C.c_double), ("eps_r", C.c_double), ("ni", C.c_double),
                 ("n_doping", C.c_int), ("doping", C.c_void_p), ("n_contacts", C.c_int),
                 ("contacts", C.c_void_p), ("pm_scheme", C.c_int), ("poisson_tol", C.c_double),
-                ("poisson_max_iter", C.c_int), ("init_from_potential", C.c_int)]
+                ("poisson_max_iter", C.c_int), ("init_from_potential", C.c_int),
+                ("region_tables", C.c_int)]
 
 
 class DeviceDriftArgs(C.Structure):
@@ -651,7 +652,7 @@ class Context:
 
     # -- 2-D device (instance 0) ------------------------------------------
     def device_setup(self, dev, pm=PM_CIC, poisson_tol=1e-10, poisson_max_iter=50,
-                     init_from_potential=False):
+                     init_from_potential=False, region_tables=False):
         """Geometry from a device dictionary (viennaemc_amd/workloads.py format):
         max_pos, spacing, width, T, eps_r, ni, doping [(min, max, N)],
         contacts [(boundary, voltage, min, max)] for ohmic contacts and
@@ -687,6 +688,7 @@ class Context:
         d.poisson_tol = poisson_tol
         d.poisson_max_iter = poisson_max_iter
         d.init_from_potential = 1 if init_from_potential else 0
+        d.region_tables = 1 if region_tables else 0
         self._chk(self.L.emcgpu_device_setup(self.h, C.byref(d)))
         self.n_contacts = len(dev["contacts"])
         ext = np.zeros(2, dtype=np.int32)

@@ -7,10 +7,12 @@
 // Physics: one parabolic valley (m* = 0.26), acoustic + zero-order
 // intervalley phonons, 300 K, Si material of examples/SiliconFunctions.hpp.
 // (resistor2D itself uses Si X valleys with ionized-impurity scattering,
-// whose rates depend on the doping region; the GPU path has one table set
-// per valley.)
+// whose rates depend on the doping region.)  With "regions" the intervalley
+// absorption is registered for the channel region only and the emission for
+// the channel and the drain region, so each doping region gets its own
+// (valley, region) tables.
 //
-// usage: resistor2DGpu <name prefix> [small|c4] [fp32|fp64] [seed]
+// usage: resistor2DGpu <name prefix> [small|c4] [fp32|fp64] [seed] [regions]
 //   small: 20 x 6 nm, 1 nm spacing, 1e24 / m^3 with 1e25 / m^3 contact
 //          regions, 0.2 V; 30 fs in 1 fs steps, 10 fs transient, the last 8
 //          steps averaged, Poisson every second step, channel current over
@@ -41,6 +43,7 @@ int main(int argc, char **argv) {
   const bool c4 = argc > 2 && std::strcmp(argv[2], "c4") == 0;
   const bool fp64 = argc > 3 && std::strcmp(argv[3], "fp64") == 0;
   const unsigned long seed = argc > 4 ? std::strtoul(argv[4], nullptr, 10) : 20261221;
+  const bool regions = argc > 5 && std::strcmp(argv[5], "regions") == 0;
   if (std::getenv("EMCGPU_THROW"))
     setErrorPolicy(ErrorPolicy::Throw);
 
@@ -96,14 +99,17 @@ int main(int argc, char **argv) {
   electrons->addValley(
       std::make_unique<emcParabolicIsotropValley<NumType>>(0.26, electrons->getMass(), 1));
   const std::map<SizeType, std::vector<SizeType>> self = {{0, {0}}};
+  const std::vector<int> all = {0, 1, 2};
   electrons->addScatterMechanism(
-      {0, 1, 2}, std::make_unique<emcAcousticScatterMechanism<NumType>>(0, 9., device));
+      all, std::make_unique<emcAcousticScatterMechanism<NumType>>(0, 9., device));
   electrons->addScatterMechanism(
-      {0, 1, 2}, std::make_unique<emcZeroOrderInterValleyAbsorptionScatterMechanism<NumType>>(
-                     "P", 0, self, 5e10, 0.05, device));
+      regions ? std::vector<int>{0} : all,
+      std::make_unique<emcZeroOrderInterValleyAbsorptionScatterMechanism<NumType>>(
+          "P", 0, self, 5e10, 0.05, device));
   electrons->addScatterMechanism(
-      {0, 1, 2}, std::make_unique<emcZeroOrderInterValleyEmissionScatterMechanism<NumType>>(
-                     "P", 0, self, 5e10, 0.05, device));
+      regions ? std::vector<int>{0, 2} : all,
+      std::make_unique<emcZeroOrderInterValleyEmissionScatterMechanism<NumType>>(
+          "P", 0, self, 5e10, 0.05, device));
   param.addParticleType(std::move(electrons));
 
   emcgpuSimulation<NumType, emcgpuCICScheme<NumType>> simulation(param, device, solver, pmScheme);

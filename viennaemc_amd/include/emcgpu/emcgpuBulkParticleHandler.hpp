@@ -116,8 +116,9 @@ public:
       m->setPtrValley(valleys);
   }
 
-  /// the tables use region 0 (the GPU path has one table set per valley);
-  /// the regions name the table dumps of writeTablesToFiles
+  /// the mechanism scatters in the listed doping regions (a bulk run uses
+  /// region 0; a device run builds one table set per region when the lists
+  /// differ); the regions also name the table dumps of writeTablesToFiles
   template <class DerivedMechanism>
   void addScatterMechanism(const std::vector<int> &regions,
                            std::unique_ptr<DerivedMechanism> &&m) {
@@ -160,6 +161,43 @@ public:
       for (SizeType l = 0; l < nLevels; l++)
         r[j * nLevels + l] = mechs[j]->getScatterRate((T)(l + 1) * dE, 0);
     return r;
+  }
+
+  /// the (valley, region) table sets of emcScatterHandler
+  /// (emcScatterHandler.hpp:130-145): the mechanisms registered for region r
+  /// in registration order, their descriptors and their rates at r
+  std::vector<SizeType> mechanismsOfRegion(int region) const {
+    std::vector<SizeType> idx;
+    for (SizeType j = 0; j < mechs.size(); j++)
+      for (int r : mechRegions[j])
+        if (r == region) {
+          idx.push_back(j);
+          break;
+        }
+    return idx;
+  }
+  std::vector<emcgpu_mech_desc> mechanismDescriptors(int region) const {
+    const auto all = mechanismDescriptors();
+    std::vector<emcgpu_mech_desc> d;
+    for (SizeType j : mechanismsOfRegion(region))
+      d.push_back(all[j]);
+    return d;
+  }
+  std::vector<double> rawRates(int region) const {
+    const T dE = maxE / (T)nLevels;
+    const auto idx = mechanismsOfRegion(region);
+    std::vector<double> r(idx.size() * nLevels);
+    for (SizeType i = 0; i < idx.size(); i++)
+      for (SizeType l = 0; l < nLevels; l++)
+        r[i * nLevels + l] = mechs[idx[i]]->getScatterRate((T)(l + 1) * dE, (SizeType)region);
+    return r;
+  }
+  /// true when the first n regions do not all hold every mechanism
+  bool regionDependent(int n) const {
+    for (int r = 0; r < n; r++)
+      if (mechanismsOfRegion(r).size() != mechs.size())
+        return true;
+    return false;
   }
 
   std::vector<emcgpu_valley_desc> valleyDescriptors() const {

@@ -303,23 +303,32 @@ private:
   void setup() {
     if (ctx)
       return;
-    emcgpu_config cfg{param.gpu, (int)param.precision, 1};
-    check(emcgpu_create(&cfg, &ctx), nullptr, "emcgpu_create");
     auto &type = *param.particleType;
-    auto vd = type.valleyDescriptors();
-    auto md = type.mechanismDescriptors();
-    check(emcgpu_set_model(ctx, 0, vd.data(), (int)vd.size(), md.data(), (int)md.size()), ctx,
-          "emcgpu_set_model");
-    emcgpu_screening_desc sd;
-    if (type.screeningDescriptor(sd))
-      check(emcgpu_set_screening(ctx, 0, &sd), ctx, "emcgpu_set_screening");
-    auto rates = type.rawRates();
-    check(emcgpu_set_rate_tables(ctx, 0, (int)type.getNrEnergyLevels(), type.getMaxEnergy(),
-                                 rates.data()),
-          ctx, "emcgpu_set_rate_tables");
     emcgpu_device_desc d =
         device.descriptor(PMScheme::id, solver.getTolerance(), solver.getMaxIter());
     d.init_from_potential = type.usesPotentialForInit() ? 1 : 0;
+    // mechanisms registered for some doping regions only: one instance per
+    // region holds that region's tables (the (valley, region) table sets of
+    // emcScatterHandler)
+    const int nRegions = d.n_doping;
+    const bool perRegion = nRegions > 1 && type.regionDependent(nRegions);
+    const int nInst = perRegion ? nRegions : 1;
+    emcgpu_config cfg{param.gpu, (int)param.precision, nInst};
+    check(emcgpu_create(&cfg, &ctx), nullptr, "emcgpu_create");
+    auto vd = type.valleyDescriptors();
+    for (int i = 0; i < nInst; i++) {
+      auto md = perRegion ? type.mechanismDescriptors(i) : type.mechanismDescriptors();
+      check(emcgpu_set_model(ctx, i, vd.data(), (int)vd.size(), md.data(), (int)md.size()), ctx,
+            "emcgpu_set_model");
+      emcgpu_screening_desc sd;
+      if (type.screeningDescriptor(sd))
+        check(emcgpu_set_screening(ctx, i, &sd), ctx, "emcgpu_set_screening");
+      auto rates = perRegion ? type.rawRates(i) : type.rawRates();
+      check(emcgpu_set_rate_tables(ctx, i, (int)type.getNrEnergyLevels(), type.getMaxEnergy(),
+                                   rates.data()),
+            ctx, "emcgpu_set_rate_tables");
+    }
+    d.region_tables = perRegion ? 1 : 0;
     check(emcgpu_device_setup(ctx, &d), ctx, "emcgpu_device_setup");
     check(emcgpu_device_info(ctx, ext, scal), ctx, "emcgpu_device_info");
     nContacts = (int)device.getNrContacts();
