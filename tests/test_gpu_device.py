@@ -16,6 +16,10 @@
   Poisson convergence, graph launch equal to eager launch;
 * the same loop with two gates (reflecting, Robin terms in every solve), by
   dense corrections and by multigrid;
+* the mosfet2D example's 126 x 101 grid, whose multigrid data exceed the LDS
+  (the solver keeps them in a global workspace): Poisson against the SOR
+  fixed point and the discrete equations, and a self-consistent run against
+  the oracle's twin;
 * Poisson every third step and the drift-velocity channel current (over the
   middle of the device, and over all of it, where deleted and injected
   particles count) against the oracle's Philox twin; the running averages
@@ -28,7 +32,7 @@ import numpy as np
 import pytest
 
 from emc_models import (DEVICE_GATED, DEVICE_SMALL, MODELS_DEVICE, device_region_specs,
-                        resistor_device)
+                        mosfet_device, resistor_device)
 from oracle import oracle_py as O
 
 pytestmark = pytest.mark.gpu
@@ -444,6 +448,85 @@ def test_gated_run_matches_oracle_twin(gates, mode):
         dpos, dE, np.bincount(out["poisson_iters"])), flush=True)
     assert dpos < 1e-9 and dE < 1e-9
     # the potential of the last step's solve
+    assert np.abs(ctx.device_fields()["pot"] - ref["pot"][-1]).max() < 1e-9
+    ctx.close()
+
+
+def test_poisson_beyond_lds_matches_sor():
+    """The mosfet2D example's 126 x 101 grid (p-type bulk, n+ source and
+    drain, a gate between them): its multigrid levels exceed the LDS, so
+    the solver keeps them in a global workspace.  Equilibrium and biased
+    solutions against the oracle's SOR at 1e-14 and against the discrete
+    equations with the gate's Robin term."""
+    print("Poisson beyond the LDS (mosfet2D grid)", flush=True)
+    dev = mosfet_device()
+    od = O.OracleDevice(dev)
+    assert tuple(od.ext) == (126, 101)
+    ctx = _ctx(dev=dev, tol=1e-14, max_iter=200)
+    pot0 = od.init_potential()
+    ref_eq, _ = od.sor(pot0, accuracy=1e-14, omega=1.8)
+    eq, it, res = ctx.poisson(pot0, equilibrium=True, reset_bc=True)
+    print("equilibrium: %d iterations, residual %.2e, max |d| %.2e" % (
+        it, res, np.abs(eq - ref_eq).max()), flush=True)
+    assert res < 1e-13
+    assert np.abs(eq - ref_eq).max() < 1e-9
+    conc = np.exp(ref_eq)  # equilibrium electrons under the applied voltages
+    ref_neq, _ = od.sor(ref_eq, conc=conc, accuracy=1e-14, omega=1.8)
+    neq, it2, res2 = ctx.poisson(ref_eq, conc=conc, equilibrium=False, reset_bc=True)
+    print("biased: %d iterations, residual %.2e, max |d| %.2e" % (
+        it2, res2, np.abs(neq - ref_neq).max()), flush=True)
+    assert res2 < 1e-13
+    assert np.abs(neq - ref_neq).max() < 1e-9
+    C = od.grids()["doping"] / dev["ni"]
+    assert _residual(od, eq, None, C, eq=True, spec=dev) < 1e-12
+    assert _residual(od, neq, conc, C, eq=False, spec=dev) < 1e-12
+    # the drain and gate voltages move the potential
+    assert np.abs(neq - eq).max() > 10.0
+    ctx.close()
+
+
+@pytest.fixture(scope="module")
+def mosfet_twin():
+    """The oracle's Philox twin of the mosfet2D device at a width of 20 nm
+    (about 3000 electrons), 6 steps of 1 fs."""
+    dev = mosfet_device(width=2e-8)
+    seed, ns, dt = 515, 6, 1e-15
+    od = O.OracleDevice(dev)
+    M = O.OracleModel(MODELS_DEVICE["device_para"])
+    ref = od.run_philox(M, O.PM_CIC, 1e-14, 1.8, dt, ns, seed, O.thermal_voltage(300.0),
+                        max_state=100000)
+    return dev, seed, ns, dt, ref
+
+
+@pytest.mark.parametrize("mode", ["auto", "multigrid"])
+def test_mosfet_run_matches_oracle_twin(mosfet_twin, mode):
+    """The self-consistent loop on the mosfet2D grid (multigrid data in the
+    global workspace; auto: dense Newton corrections, then the multigrid
+    check) against the oracle's Philox twin: ensemble counts per step and
+    per contact exact, final ensemble and potential within 1e-9."""
+    emcgpu = _emc()
+    dev, seed, ns, dt, ref = mosfet_twin
+    print("mosfet twin (%s): %d particles, %d steps, exits %s" % (
+        mode, ref["n_init"][0], ns, ref["exits"].sum(axis=0)), flush=True)
+    ctx = _ctx(dev=dev, tol=1e-14, max_iter=200)
+    assert ctx.device_init(seed) == ref["n_init"][0]
+    assert np.abs(ctx.device_fields()["pot"] - ref["eq_pot"]).max() < 1e-9
+    pmode = emcgpu.POISSON_AUTO if mode == "auto" else emcgpu.POISSON_MULTIGRID
+    out = ctx.device_step(dt, ns, seed=seed, poisson_mode=pmode)
+    assert (out["n_particles"] == ref["n_part"]).all()
+    assert (out["exits"] == ref["exits"]).all()
+    assert (out["deleted"] == ref["dels"]).all()
+    assert (out["injected"] == ref["injs"]).all()
+    assert (out["poisson_resid"] <= 1e-13).all()
+    p = ctx.device_get_particles()
+    st = ref["state"]
+    assert p["E"].size == st.shape[0]
+    assert (p["valley"] == st[:, 7]).all() and (p["sub"] == st[:, 8]).all()
+    dpos = np.abs(p["pos"] - st[:, 0:2]).max() / 125e-9
+    dE = (np.abs(p["E"] - st[:, 5]) / st[:, 5]).max()
+    print("final ensemble: |dpos| %.2e, |dE| %.2e, Poisson work %s" % (
+        dpos, dE, np.bincount(out["poisson_iters"])), flush=True)
+    assert dpos < 1e-9 and dE < 1e-9
     assert np.abs(ctx.device_fields()["pot"] - ref["pot"][-1]).max() < 1e-9
     ctx.close()
 

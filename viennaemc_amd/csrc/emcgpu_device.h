@@ -356,6 +356,9 @@ struct DevFieldArgs {
   // dynamic LDS layout (float offsets; tab and cmask in bytes)
   int lds_res, lds_rc, lds_sc, lds_tab, lds_cmask;
   int lds_dummy;  // float offset of one scratch word per thread
+  // the same layout in global memory (L2-resident) when it exceeds the
+  // LDS, null when it fits
+  float *mg_ws;
   // fine-grid thread map: a thread owns column (tid mod 2^cw_log2) of a row
   // block; half-row map (red-black sweeps, node pairs) and full-row map,
   // with the number of row blocks (passes) covering the grid

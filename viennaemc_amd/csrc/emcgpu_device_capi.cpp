@@ -22,7 +22,7 @@
 
 namespace emcgpu {
 template <typename R>
-hipError_t dev_launch_field(const DevFieldArgs<R> *args_dev, size_t lds, hipStream_t st);
+hipError_t dev_launch_field(const DevFieldArgs<R> *args_dev, size_t lds, bool ws, hipStream_t st);
 template <typename R>
 hipError_t dev_launch_efield(const DevFieldArgs<R> *args_dev, int nn, hipStream_t st);
 template <typename R>
@@ -102,6 +102,7 @@ struct DeviceSim {
   std::vector<DimTab> tab;
   std::vector<uint8_t> lmask;
   size_t mg_lds = 0;
+  bool mg_global = false;  // the multigrid data exceed the LDS: global workspace d_mg_ws
   int mg_nodes = 0;  // nodes of all levels
   int lds_res = 0, lds_rc = 0, lds_sc = 0, lds_tab = 0, lds_cmask = 0, wave_level = 0;
   int lds_dummy = 0;
@@ -109,6 +110,7 @@ struct DeviceSim {
   int hcw_log2 = 0, nph = 0, fcw_log2 = 0, npf = 0;
   std::vector<float> cinv_t;
   float *d_cinv_t = nullptr;
+  float *d_mg_ws = nullptr;
   // ---- device memory ------------------------------------------------------
   bool on_gpu = false;
   int *d_cnode = nullptr, *d_cslot = nullptr;
@@ -244,6 +246,7 @@ void device_sim_destroy(DeviceSim *d) {
     dfree(d->d_fargs);
     dfreet(d->d_hw0);
     dfreet(d->d_cinv_t);
+    dfreet(d->d_mg_ws);
     dfreet(d->d_green);
     dfreet(d->d_green_flag);
     dfreet(d->d_green_acc);
@@ -763,11 +766,13 @@ emcgpu_status build_multigrid(emcgpu_ctx *c, DeviceSim &d) {
   const int rh = kDevThreads >> d.hcw_log2, rf = kDevThreads >> d.fcw_log2;
   d.nph = (d.ext[1] + rh - 1) / rh;
   d.npf = (d.ext[1] + rf - 1) / rf;
-  if (d.mg_lds > (size_t)kDevMaxLds)
-    return fail(c, EMCGPU_ERR_ARG,
-                "a %d x %d grid exceeds the single-workgroup Poisson solver (%zu bytes of LDS > "
-                "%d)",
-                d.ext[0], d.ext[1], d.mg_lds, kDevMaxLds);
+  // levels beyond the LDS: the same layout in a global workspace (the
+  // workgroup then reads them through L2)
+  d.mg_global = d.mg_lds > (size_t)kDevMaxLds;
+  // the Poisson passes' index maths (idiv) is exact below 2^22 nodes
+  if ((size_t)d.mg_nodes + d.mg_lds / sizeof(float) >= ((size_t)1 << 22))
+    return fail(c, EMCGPU_ERR_ARG, "a %d x %d grid exceeds the Poisson solver (%d nodes)",
+                d.ext[0], d.ext[1], d.mg_nodes);
   return EMCGPU_OK;
 }
 
@@ -849,6 +854,8 @@ emcgpu_status upload_device(emcgpu_ctx *c, DeviceSim &d) {
   ST(dalloc(c, d.d_q64, np));
   ST(dalloc(c, d.d_s64, np));
   ST(dalloc(c, d.d_rs0, np));
+  if (d.mg_global)
+    ST(dalloc(c, d.d_mg_ws, (d.mg_lds + sizeof(float) - 1) / sizeof(float)));
   {
     std::vector<float> w(np, 0.0f), hw(np, 0.0f);
     std::vector<float2> rs(np, make_float2(0.0f, 1e30f));
@@ -1113,6 +1120,7 @@ DevFieldArgs<R> field_args(const emcgpu_ctx *c, const DeviceSim &d) {
   a.lds_tab = d.lds_tab;
   a.lds_cmask = d.lds_cmask;
   a.lds_dummy = d.lds_dummy;
+  a.mg_ws = d.mg_global ? d.d_mg_ws : nullptr;
   a.hcw_log2 = d.hcw_log2;
   a.nph = d.nph;
   a.fcw_log2 = d.fcw_log2;
@@ -1277,7 +1285,7 @@ emcgpu_status put_fargs(emcgpu_ctx *c, DeviceSim &d, int slot, const DevFieldArg
 template <typename R>
 emcgpu_status run_field(emcgpu_ctx *c, DeviceSim &d, DevFieldArgs<R> a) {
   ST(put_fargs<R>(c, d, 2, a));
-  HIPCHK(c, dev_launch_field<R>(fargs_slot<R>(d, 2), d.mg_lds, c->stream));
+  HIPCHK(c, dev_launch_field<R>(fargs_slot<R>(d, 2), d.mg_lds, d.mg_global, c->stream));
   return EMCGPU_OK;
 }
 
@@ -1668,7 +1676,8 @@ template <typename R>
 emcgpu_status launch_step(emcgpu_ctx *c, DeviceSim &d, bool reset_bc) {
   if (d.green_use && !reset_bc)
     HIPCHK(c, dev_launch_green(green_args(d), green_iters(d), d.green_blocks, c->stream));
-  HIPCHK(c, dev_launch_field<R>(fargs_slot<R>(d, reset_bc ? 1 : 0), d.mg_lds, c->stream));
+  HIPCHK(c, dev_launch_field<R>(fargs_slot<R>(d, reset_bc ? 1 : 0), d.mg_lds, d.mg_global,
+                                 c->stream));
   if (split_efield(d))
     HIPCHK(c, dev_launch_efield<R>(fargs_slot<R>(d, reset_bc ? 1 : 0), (int)d.nn, c->stream));
   const DevContactArgs<R> ca = contact_args<R>(c, d);

@@ -1500,10 +1500,13 @@ __device__ __forceinline__ void mg_shift_down(const DevFieldArgs<R> &a, const MG
 // vectors and the update are FP64; the CG preconditioner is one FP32
 // multigrid V-cycle.  Then calcEField (PMSchemes/emcEFieldCalculation.hpp:
 // 13-81).
-template <typename R>
+template <typename R, bool WS>
 __global__ __launch_bounds__(kDevThreads) void k_dev_field(const DevFieldArgs<R> *__restrict__ args) {
   const DevFieldArgs<R> &a = *args;
   extern __shared__ __align__(16) float lds[];
+  // the multigrid data: in LDS, or with WS (grids whose levels exceed the
+  // LDS) in a global workspace of the same layout, kept in L2
+  float *const mg = WS ? a.mg_ws : lds;
   __shared__ double red[kDevThreads / 64];
   const int tid = threadIdx.x;
   const int nx = a.nx, ny = a.ny, nn = a.nn;
@@ -1568,12 +1571,12 @@ __global__ __launch_bounds__(kDevThreads) void k_dev_field(const DevFieldArgs<R>
   }
   if (a.do_poisson && !green_ok) {
     MGShared m;
-    m.d = lds;
-    m.res = lds + a.lds_res;
-    m.rc = lds + a.lds_rc;
-    m.sc = lds + a.lds_sc;
-    DimTab *tab = (DimTab *)((unsigned char *)lds + a.lds_tab);
-    uint8_t *cmask = (uint8_t *)lds + a.lds_cmask;
+    m.d = mg;
+    m.res = mg + a.lds_res;
+    m.rc = mg + a.lds_rc;
+    m.sc = mg + a.lds_sc;
+    DimTab *tab = (DimTab *)((unsigned char *)mg + a.lds_tab);
+    uint8_t *cmask = (uint8_t *)mg + a.lds_cmask;
     m.tab = tab;
     m.cmask = cmask;
     m.n0 = nn;
@@ -1648,8 +1651,13 @@ __global__ __launch_bounds__(kDevThreads) void k_dev_field(const DevFieldArgs<R>
         mg_shift_down(a, m);
       // conjugate gradients on A x = F (A = -Jacobian, symmetric in the
       // control-volume inner product), x0 = 0
+      // inexact Newton away from the root: the FP32-preconditioned CG
+      // solve stops at a tenth of the residual and the system is
+      // linearised again (solving each Newton system to the stopping rule
+      // takes several times the V-cycles once the contacts' voltages or a
+      // p-type region's hole term pull the potential far from the start)
       double target = 0.5 * a.tol * scale;
-      if (a.equilibrium && 0.1 * fm > target)  // inexact Newton away from the root
+      if (0.1 * fm > target)
         target = 0.1 * fm;
       double rz_old = 1.0;
       for (int k = tid; k < np; k += kDevThreads) {
@@ -1713,11 +1721,14 @@ __global__ __launch_bounds__(kDevThreads) void k_dev_field(const DevFieldArgs<R>
           break;
       }
       for (int k = tid; k < nn; k += kDevThreads) {
+        // damped Newton: a correction beyond 2 V_T is clipped (from the
+        // charge-neutral guess, and where new contact voltages move the
+        // potential across a junction, the hole term exp(-phi) would
+        // otherwise overshoot); near the root the corrections are far
+        // smaller and the step is Newton's
         double d = x64[k];  // zero at Dirichlet nodes
-        if (a.equilibrium) {  // damped Newton from the charge-neutral guess
-          d = d > 2.0 ? 2.0 : d;
-          d = d < -2.0 ? -2.0 : d;
-        }
+        d = d > 2.0 ? 2.0 : d;
+        d = d < -2.0 ? -2.0 : d;
         potg[k] += d;
       }
       __syncthreads();
@@ -1823,18 +1834,22 @@ hipError_t dev_launch_efield(const DevFieldArgs<R> *args_dev, int nn, hipStream_
 // the caller): dynamic indexing of the level table would otherwise copy the
 // whole block to scratch
 template <typename R>
-hipError_t dev_launch_field(const DevFieldArgs<R> *args_dev, size_t lds, hipStream_t st) {
+hipError_t dev_launch_field(const DevFieldArgs<R> *args_dev, size_t lds, bool ws, hipStream_t st) {
+  if (ws) {  // the multigrid data in the global workspace (args_dev->mg_ws)
+    hipLaunchKernelGGL((k_dev_field<R, true>), dim3(1), dim3(kDevThreads), 0, st, args_dev);
+    return hipGetLastError();
+  }
   if (lds > (size_t)kDevMaxLds)
     return hipErrorInvalidValue;
   if (lds > 65536) {
-    hipError_t e = hipFuncSetAttribute((const void *)k_dev_field<R>,
+    hipError_t e = hipFuncSetAttribute((const void *)k_dev_field<R, false>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) {
       (void)hipGetLastError();
       return e;
     }
   }
-  hipLaunchKernelGGL((k_dev_field<R>), dim3(1), dim3(kDevThreads), lds, st, args_dev);
+  hipLaunchKernelGGL((k_dev_field<R, false>), dim3(1), dim3(kDevThreads), lds, st, args_dev);
   return hipGetLastError();
 }
 
@@ -2280,7 +2295,7 @@ hipError_t dev_launch_green_expand(const double *Ainv, long long lda, const int 
 }
 
 #define EMCGPU_DEV_INST(R)                                                                   \
-  template hipError_t dev_launch_field<R>(const DevFieldArgs<R> *, size_t, hipStream_t);      \
+  template hipError_t dev_launch_field<R>(const DevFieldArgs<R> *, size_t, bool, hipStream_t);      \
   template hipError_t dev_launch_efield<R>(const DevFieldArgs<R> *, int, hipStream_t);        \
   template hipError_t dev_launch_step<R>(const DevStepArgs<R> &, const DevContactArgs<R> &, int, \
                                          int, size_t, hipStream_t);                           \

@@ -312,6 +312,26 @@ DEVICE_GATED = dict(DEVICE_SMALL, contacts=DEVICE_SMALL["contacts"] + [
     (3, 0.4, 5e-9, 15e-9, 3.9, 1e-9, 0.3),
     (2, -0.2, 8e-9, 12e-9, 25.0, 2e-9, 0.1)])
 
+SI_BANDGAP = 1.15  # examples/SiliconFunctions.hpp:25 [eV]
+
+
+def mosfet_device(vg=1.0, vd=1.0, width=1e-6):
+    """The n-channel MOSFET of the mosfet2D example (mosfet2D.cpp:62-87):
+    125 x 100 nm at 1 nm (126 x 101 nodes), p-type bulk 5e23 /m^3 below
+    30 nm, n+ source [0, 51] nm and drain [75, 125] nm at 5e25 /m^3, p-type
+    5e24 /m^3 under the gate; ohmic substrate (YMAX), source and drain (YMIN,
+    drain at vd), a gate on YMIN over [51, 75] nm (oxide eps_r 3.9, 1.2 nm,
+    barrier half the band gap) at vg.  The reference runs it at a width of
+    1 um; a smaller width scales the particle count down."""
+    return dict(max_pos=(125e-9, 100e-9), spacing=(1e-9, 1e-9), width=width, T=300.0,
+                eps_r=SI_EPS_R, ni=SI_NI,
+                doping=[((0, 30e-9), (125e-9, 100e-9), -5e23), ((0, 0), (51e-9, 30e-9), 5e25),
+                        ((51e-9, 0), (75e-9, 30e-9), -5e24), ((75e-9, 0), (125e-9, 30e-9), 5e25)],
+                contacts=[(3, 0.0, 0.0, 125e-9), (2, 0.0, 0.0, 51e-9),
+                          (2, vg, 51e-9, 75e-9, 3.9, 1.2e-9, SI_BANDGAP / 2),
+                          (2, vd, 75e-9, 125e-9)])
+
+
 def resistor_device(nx=200, ny=50, h=1e-9, n_particles=2_000_000, contact_nodes=10, bias=0.5):
     """The 2-D resistor workload: nx x ny nodes at spacing h, N_D = 1e23 /m^3
     with 1e25 /m^3 ohmic contact regions of contact_nodes columns at both
