@@ -17,7 +17,7 @@ import tempfile
 import pytest
 
 from emc_models import (DEVICE_SMALL, MODELS_DEVICE, SI_EPS_R, SI_NI, device_region_specs,
-                        resistor_device)
+                        mosfet_device, resistor_device)
 from oracle import oracle_py as O
 from viennaemc_amd import emcgpu
 
@@ -77,6 +77,26 @@ def test_rejects_regions_and_contacts_outside():
     bad = dict(DEVICE_SMALL, max_pos=(1e-9, 6e-9))  # fewer than 3 nodes in x
     with pytest.raises(emcgpu.EmcGpuError):
         _host(bad)
+
+
+
+def test_grid_limits_of_the_poisson_solver():
+    """The mosfet2D example's 126 x 101 grid (multigrid levels beyond the
+    LDS: kept in a global workspace) is accepted and its geometry matches
+    the oracle's; a grid wider than the solver's 1024-node row map is
+    rejected."""
+    dev = mosfet_device()
+    ctx = _host(dev)
+    od = O.OracleDevice(dev)
+    assert ctx.ext == (126, 101) == tuple(od.ext)
+    g, og = ctx.device_grids(), od.grids()
+    assert (g["doping"] == og["doping"]).all() and (g["contact"] == og["contact"]).all()
+    ctx.close()
+    wide = dict(DEVICE_SMALL, max_pos=(1100e-9, 3e-9),
+                doping=[((0, 0), (1100e-9, 3e-9), 1e24)],
+                contacts=[(0, 0.0, 0.0, 3e-9), (1, 0.0, 0.0, 3e-9)])
+    with pytest.raises(emcgpu.EmcGpuError, match="1024"):
+        _host(wide)
 
 
 C_LAYOUT = r"""
