@@ -491,7 +491,9 @@ typedef struct {
 
 /* geometry (host) and device grids; call after emcgpu_set_model and
  * emcgpu_build_tables of instance 0.  Needs a GPU context except for the
- * geometry getters below. */
+ * geometry getters below.  The Poisson solver runs in one workgroup: at
+ * most 1024 nodes along x; grids whose multigrid levels exceed the LDS
+ * (above about 11k nodes) keep them in device memory instead. */
 emcgpu_status emcgpu_device_setup(emcgpu_ctx *ctx, const emcgpu_device_desc *desc);
 /* ext[2] node counts; scal[4] = V_T [V], intrinsic Debye length [m], cell
  * volume [m^3], normalised cell volume (emcDevice.hpp:236-343) */
