@@ -520,6 +520,40 @@ emcgpu_status emcgpu_deposit(emcgpu_ctx *ctx, const double *pos, size_t n, doubl
  * of the given charge from ex, ey (V/m, as emcgpu_efield lays them out) */
 emcgpu_status emcgpu_interpolate(emcgpu_ctx *ctx, const double *ex, const double *ey,
                                  const double *pos, size_t n, double charge, double *force);
+
+/* ---- 3-D particle-mesh operators ----------------------------------------
+ * The reference's 3-D schemes on a caller grid (emcDevice<T, 3> with
+ * emcNGPScheme / emcCICScheme; its NEC scheme is 2-D only): ext[0] x ext[1]
+ * x ext[2] nodes, x fastest (emcGrid.hpp:241-253), positions [n][3] in
+ * metres.  FP64 whatever the context's precision; host buffers; need a GPU
+ * context, no device setup. */
+typedef struct {
+  int ext[3];         /* nodes per dimension */
+  double spacing[3];  /* [m] */
+  int pm_scheme;      /* EMCGPU_PM_NGP or EMCGPU_PM_CIC */
+} emcgpu_grid3;
+/* assignToMesh, one carrier per position, accumulated into grid[nodes]
+ * (emcNGPScheme.hpp:36-48; emcCICScheme.hpp:97-120, the node at the floor
+ * corner taking wX wY wZ); nodes outside the grid are skipped */
+emcgpu_status emcgpu_deposit3(emcgpu_ctx *ctx, const emcgpu_grid3 *grid3, const double *pos,
+                              size_t n, double *grid);
+/* interpolateForce: force[n][3] [N] on a particle of the given charge from
+ * the nodal field ex, ey, ez [V/m] (emcNGPScheme.hpp:51-66;
+ * emcCICScheme.hpp:143-173, including its (1 - wY)(1 - wY) weight of the
+ * (x+1, y+1) nodes) */
+emcgpu_status emcgpu_interpolate3(emcgpu_ctx *ctx, const emcgpu_grid3 *grid3, const double *ex,
+                                  const double *ey, const double *ez, const double *pos, size_t n,
+                                  double charge, double *force);
+/* calcEFieldAtGridPts with setEFieldBoundaryValues
+ * (emcEFieldCalculation.hpp:13-32, 57-81): ex, ey, ez [V/m] from the
+ * potential pot (normalised by the thermal voltage vt [V]); at least 3
+ * nodes per dimension.  bc[3][nodes]: for component d, a node on a face
+ * normal to d holds -1 (artificial boundary: E_d = 0) or a contact index
+ * >= 0 (E_d = the inner neighbour's), every other node -2 (the status
+ * emcSurface::isArtificialBoundary gives the face points) */
+emcgpu_status emcgpu_efield3(emcgpu_ctx *ctx, const emcgpu_grid3 *grid3, double vt,
+                             const signed char *bc, const double *pot, double *ex, double *ey,
+                             double *ez);
 /* emcgpu_contacts -- the ohmic-contact equilibrium of handleOhmicContacts
  * (emcBasicParticleHandler.hpp:158-192) on positions in array order:
  * keep[n] = 0 for particles deleted as excess, inject[nodes] = particles to

@@ -291,6 +291,119 @@ void orc_pm_force(int scheme, const double *ex, const double *ey,
   }
 }
 
+/* ---- 3-D particle-mesh schemes ---------------------------------------- */
+static int orc_valid3(const int ext[3], size_t i, size_t j, size_t k) {
+  return i < (size_t)ext[0] && j < (size_t)ext[1] && k < (size_t)ext[2];
+}
+static size_t orc_gidx3(const int ext[3], size_t i, size_t j, size_t k) {
+  return i + (size_t)ext[0] * (j + (size_t)ext[1] * k); /* emcGrid.hpp:241-253 */
+}
+static void orc_add3(double *g, const int ext[3], size_t i, size_t j, size_t k, double v) {
+  if (orc_valid3(ext, i, j, k))
+    g[orc_gidx3(ext, i, j, k)] += v;
+}
+static double orc_get3(const double *g, const int ext[3], size_t i, size_t j, size_t k) {
+  return orc_valid3(ext, i, j, k) ? g[orc_gidx3(ext, i, j, k)] : 0.0;
+}
+
+void orc_pm3_assign(int scheme, const int ext[3], const double h[3], const double *pos, int n,
+                    double *grid) {
+  for (int q = 0; q < n; q++) {
+    const double *p = pos + 3 * q;
+    if (scheme == ORC_PM_NGP) { /* emcNGPScheme.hpp:36-48: round(pos / h) */
+      orc_add3(grid, ext, (size_t)round(p[0] / h[0]), (size_t)round(p[1] / h[1]),
+               (size_t)round(p[2] / h[2]), 1.0);
+      continue;
+    }
+    /* emcCICScheme.hpp:99-119: node (floor) takes wX wY wZ, the weights
+     * multiplied left to right, times the carriers (1) */
+    size_t i = (size_t)floor(p[0] / h[0]), j = (size_t)floor(p[1] / h[1]),
+           k = (size_t)floor(p[2] / h[2]);
+    double wX = p[0] / h[0] - (double)i, wY = p[1] / h[1] - (double)j,
+           wZ = p[2] / h[2] - (double)k;
+    orc_add3(grid, ext, i, j, k, wX * wY * wZ * 1.0);
+    orc_add3(grid, ext, i + 1, j, k, (1 - wX) * wY * wZ * 1.0);
+    orc_add3(grid, ext, i, j + 1, k, wX * (1 - wY) * wZ * 1.0);
+    orc_add3(grid, ext, i + 1, j + 1, k, (1 - wX) * (1 - wY) * wZ * 1.0);
+    orc_add3(grid, ext, i, j, k + 1, wX * wY * (1 - wZ) * 1.0);
+    orc_add3(grid, ext, i + 1, j, k + 1, (1 - wX) * wY * (1 - wZ) * 1.0);
+    orc_add3(grid, ext, i, j + 1, k + 1, wX * (1 - wY) * (1 - wZ) * 1.0);
+    orc_add3(grid, ext, i + 1, j + 1, k + 1, (1 - wX) * (1 - wY) * (1 - wZ) * 1.0);
+  }
+}
+
+void orc_pm3_force(int scheme, const int ext[3], const double h[3], const double *ex,
+                   const double *ey, const double *ez, const double *pos, int n, double charge,
+                   double *force) {
+  const double *E[3] = {ex, ey, ez};
+  for (int q = 0; q < n; q++) {
+    const double *p = pos + 3 * q;
+    double *f = force + 3 * q;
+    if (scheme == ORC_PM_NGP) { /* emcNGPScheme.hpp:51-66 */
+      size_t i = (size_t)round(p[0] / h[0]), j = (size_t)round(p[1] / h[1]),
+             k = (size_t)round(p[2] / h[2]);
+      for (int d = 0; d < 3; d++)
+        f[d] = charge * orc_get3(E[d], ext, i, j, k);
+      continue;
+    }
+    /* emcCICScheme.hpp:143-173: eight nodes, the (x+1, y+1) nodes of both
+     * z layers weighted (1 - wY)(1 - wY) as in the reference, the sum times
+     * the charge */
+    size_t i = (size_t)floor(p[0] / h[0]), j = (size_t)floor(p[1] / h[1]),
+           k = (size_t)floor(p[2] / h[2]);
+    double wX = p[0] / h[0] - (double)i, wY = p[1] / h[1] - (double)j,
+           wZ = p[2] / h[2] - (double)k;
+    for (int d = 0; d < 3; d++) {
+      const double *g = E[d];
+      f[d] = (orc_get3(g, ext, i, j, k) * wX * wY * wZ +
+              orc_get3(g, ext, i + 1, j, k) * (1 - wX) * wY * wZ +
+              orc_get3(g, ext, i, j + 1, k) * wX * (1 - wY) * wZ +
+              orc_get3(g, ext, i + 1, j + 1, k) * (1 - wY) * (1 - wY) * wZ +
+              orc_get3(g, ext, i, j, k + 1) * wX * wY * (1 - wZ) +
+              orc_get3(g, ext, i + 1, j, k + 1) * (1 - wX) * wY * (1 - wZ) +
+              orc_get3(g, ext, i, j + 1, k + 1) * wX * (1 - wY) * (1 - wZ) +
+              orc_get3(g, ext, i + 1, j + 1, k + 1) * (1 - wY) * (1 - wY) * (1 - wZ)) *
+             charge;
+    }
+  }
+}
+
+/* calcEFieldAtGridPts (emcEFieldCalculation.hpp:13-32) in 3-D: central
+ * differences (prev - next) * vt / (2 h) off the component's faces; then
+ * setEFieldBoundaryValues (:57-81): on a face of the component's
+ * dimension, 0 at an artificial boundary, the inner neighbour's value at a
+ * contact */
+void orc_pm3_efield(const int ext[3], const double h[3], double vt, const int *bc,
+                    const double *pot, double *ex, double *ey, double *ez) {
+  double *E[3] = {ex, ey, ez};
+  const size_t nn = (size_t)ext[0] * ext[1] * ext[2];
+  const size_t stride[3] = {1, (size_t)ext[0], (size_t)ext[0] * ext[1]};
+  for (int d = 0; d < 3; d++) {
+    for (size_t k = 0; k < (size_t)ext[2]; k++)
+      for (size_t j = 0; j < (size_t)ext[1]; j++)
+        for (size_t i = 0; i < (size_t)ext[0]; i++) {
+          const size_t c[3] = {i, j, k};
+          if (c[d] == 0 || c[d] == (size_t)(ext[d] - 1))
+            continue;
+          const size_t m = orc_gidx3(ext, i, j, k);
+          E[d][m] = (pot[m - stride[d]] - pot[m + stride[d]]) * vt / (2 * h[d]);
+        }
+    for (size_t k = 0; k < (size_t)ext[2]; k++)
+      for (size_t j = 0; j < (size_t)ext[1]; j++)
+        for (size_t i = 0; i < (size_t)ext[0]; i++) {
+          const size_t c[3] = {i, j, k};
+          const size_t m = orc_gidx3(ext, i, j, k);
+          const int b = bc[(size_t)d * nn + m];
+          if (b == -2)
+            continue;
+          if (b == -1)
+            E[d][m] = 0;
+          else
+            E[d][m] = c[d] == 0 ? E[d][m + stride[d]] : E[d][m - stride[d]];
+        }
+  }
+}
+
 /* setEFieldBoundaryValues (emcEFieldCalculation.hpp:57-81): the surface
  * loop runs once per dimension, as in the reference */
 static void orc_efield_boundary(const orc_device *d, double *E[2]) {

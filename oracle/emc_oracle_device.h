@@ -101,6 +101,22 @@ void orc_pm_force(int scheme, const double *ex, const double *ey,
 void orc_pm_efield(int scheme, const orc_device *d, const double *pot,
                    double *ex, double *ey);
 
+/* 3-D schemes on a grid of ext[0] x ext[1] x ext[2] nodes (x fastest),
+ * spacing h [m]; NGP or CIC (the reference's NEC is 2-D only).
+ * orc_pm3_assign: assignToMesh of n positions pos[3n], one carrier each,
+ * accumulated into grid.  orc_pm3_force: interpolateForce into force[3n]
+ * from E = ex, ey, ez.  orc_pm3_efield: calcEFieldAtGridPts with vt the
+ * thermal voltage and bc[3][nodes] the boundary status of each node for
+ * the component normal to the faces it lies on (-2 on none, -1 artificial,
+ * >= 0 contact) */
+void orc_pm3_assign(int scheme, const int ext[3], const double h[3], const double *pos, int n,
+                    double *grid);
+void orc_pm3_force(int scheme, const int ext[3], const double h[3], const double *ex,
+                   const double *ey, const double *ez, const double *pos, int n, double charge,
+                   double *force);
+void orc_pm3_efield(const int ext[3], const double h[3], double vt, const int *bc,
+                    const double *pot, double *ex, double *ey, double *ez);
+
 /* ---- SOR Poisson solver (PoissonSolver/emcSORSolver.hpp) --------------- */
 typedef struct {
   double accuracy; /* normalised (accuracy [V] / Vt) */

@@ -113,6 +113,9 @@ def load():
         "orc_pm_assign_h": (None, [vp, _I, _PD, _I, _D, _PD]),
         "orc_pm_forces_h": (None, [vp, _I, _PD, _PD, _PD, _I, _D, _PD]),
         "orc_pm_efield": (None, [_I, vp, _PD, _PD, _PD]),
+        "orc_pm3_assign": (None, [_I, _PI32, _PD, _PD, _I, _PD]),
+        "orc_pm3_force": (None, [_I, _PI32, _PD, _PD, _PD, _PD, _PD, _I, _D, _PD]),
+        "orc_pm3_efield": (None, [_PI32, _PD, _D, _PI32, _PD, _PD, _PD, _PD]),
         "orc_device_init_potential": (None, [vp, _PD]),
         "orc_device_concentration": (None, [vp, _PD, _PD]),
         "orc_device_contact_rule": (None, [vp, _PD, _I, _D, _PI32, _PI32, _PD]),
@@ -463,6 +466,41 @@ def philox(ctr, key):
 def prod_uniform(seed, inst, pid, step, j):
     """uniform j of the production (step, particle, instance) stream"""
     return load().orc_prod_uniform(seed, inst, pid, step, j)
+
+
+def pm3_assign(scheme, ext, h, pos):
+    """3-D assignToMesh (NGP or CIC) of pos[n][3], one carrier each:
+    the grid [ext[2]][ext[1]][ext[0]] flattened x fastest."""
+    L = load()
+    ext = np.ascontiguousarray(ext, dtype=np.int32)
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+    g = np.zeros(int(np.prod(ext)))
+    L.orc_pm3_assign(scheme, ext, np.ascontiguousarray(h, dtype=np.float64), pos, pos.size // 3, g)
+    return g
+
+
+def pm3_force(scheme, ext, h, ex, ey, ez, pos, charge):
+    """3-D interpolateForce of pos[n][3]: force[n][3]."""
+    L = load()
+    ext = np.ascontiguousarray(ext, dtype=np.int32)
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+    f = np.zeros(pos.size)
+    c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    L.orc_pm3_force(scheme, ext, c(h), c(ex), c(ey), c(ez), pos, pos.size // 3, charge, f)
+    return f.reshape(-1, 3)
+
+
+def pm3_efield(ext, h, vt, bc, pot):
+    """3-D calcEFieldAtGridPts: (ex, ey, ez) from the normalised potential,
+    bc[3][nodes] as in orc_pm3_efield."""
+    L = load()
+    ext = np.ascontiguousarray(ext, dtype=np.int32)
+    nn = int(np.prod(ext))
+    E = [np.zeros(nn) for _ in range(3)]
+    L.orc_pm3_efield(ext, np.ascontiguousarray(h, dtype=np.float64), vt,
+                     np.ascontiguousarray(bc, dtype=np.int32).reshape(-1),
+                     np.ascontiguousarray(pot, dtype=np.float64), *E)
+    return E
 
 
 def thermal_voltage(T):

@@ -840,7 +840,7 @@ static std::unique_ptr<Electron2> makeDeviceType(Device2 &dev, bool usePotential
   return type;
 }
 
-template <class T2> static std::vector<double> gridVec(const emcGrid<T2, 2> &g) {
+template <class T2, SizeType D> static std::vector<double> gridVec(const emcGrid<T2, D> &g) {
   return std::vector<double>(g.begin(), g.end());
 }
 
@@ -1018,6 +1018,84 @@ static void dumpDeviceKATs() {
 // SOR with gate (Robin) contacts: the device above plus a gate on each
 // y boundary (the YMAX one over the channel, a narrower one on YMIN with
 // other oxide constants); the ohmic contacts stay Dirichlet
+// 3-D particle-mesh known answers: NGP and CIC assignment and force
+// interpolation, the grid-point E-field with its boundary rule (ohmic
+// contacts over parts of two faces), on an 8 x 6 x 5 grid with a different
+// spacing per dimension
+static void dumpPM3KATs() {
+  const std::array<T, 3> maxPos = {7e-9, 2.5e-9, 4e-9}, h = {1e-9, 0.5e-9, 1e-9};
+  Device dev{silicon(), maxPos, h, 300.};
+  dev.addConstantDopingRegion({0, 0, 0}, maxPos, 1e24);
+  dev.addOhmicContact(emcBoundaryPos::XMIN, 0.1, {0.5e-9, 1e-9}, {2e-9, 3e-9});
+  dev.addOhmicContact(emcBoundaryPos::ZMAX, 0., {2e-9, 0}, {7e-9, 1.5e-9});
+  auto ext = dev.getGridExtent();
+  secu("pm3.ext", {ext[0], ext[1], ext[2]});
+  sec("pm3.spacing", {h[0], h[1], h[2]});
+  sec("pm3.vt", {dev.undoNormalizeVoltage(1.0)});
+  uint64_t st = 0x3D3D3Dull;
+  auto u = [&]() {
+    st = splitmix64(st);
+    return (double)(st >> 11) * 0x1p-53;
+  };
+  using Grid3 = emcGrid<T, 3>;
+  std::array<SizeType, 3> c;
+  // boundary status per component and node: -2 off the component's faces,
+  // -1 artificial boundary, 0 contact
+  std::vector<long long> bc(3 * (size_t)ext[0] * ext[1] * ext[2], -2);
+  {
+    auto &surface = dev.getSurface();
+    std::array<SizeType, 2> cs;
+    emcBoundaryPos bp;
+    for (surface.initCoord(bp, cs); !surface.isEndCoord(bp, cs); surface.advanceCoord(bp, cs)) {
+      auto cd = surface.getCoordDevice(cs, bp);
+      const int d = (int)toUnderlying(bp) / 2;
+      const size_t k = cd[0] + ext[0] * (cd[1] + ext[1] * cd[2]);
+      bc[(size_t)d * ext[0] * ext[1] * ext[2] + k] = surface.isArtificialBoundary(cs, bp) ? -1 : 0;
+    }
+  }
+  seci("pm3.bc", bc);
+  std::vector<Grid3> eField(3, Grid3(ext));
+  for (int d = 0; d < 3; d++)
+    for (c.fill(0); !eField[d].isEndCoord(c); eField[d].advanceCoord(c))
+      eField[d][c] = (u() - 0.5) * 2e7;
+  for (int d = 0; d < 3; d++)
+    sec((std::string("pm3.e") + "xyz"[d]).c_str(), gridVec(eField[d]));
+  std::vector<std::array<T, 3>> pos;
+  std::vector<double> posFlat;
+  for (int i = 0; i < 96; i++) {
+    std::array<T, 3> p = {u() * maxPos[0] * 0.999, u() * maxPos[1] * 0.999, u() * maxPos[2] * 0.999};
+    if (i == 0)
+      p = {0, 0, 0};
+    if (i == 1)
+      p = {h[0] * 0.5, h[1] * 2.5, h[2] * 1.5};  // midway between nodes
+    if (i == 2)
+      p = {h[0] * 3, h[1] * 1, h[2] * 2};  // on a node
+    pos.push_back(p);
+    posFlat.insert(posFlat.end(), p.begin(), p.end());
+  }
+  sec("pm3.pos", posFlat);
+  auto schemes = [&](auto pm, const char *name) {
+    Grid3 g(ext, 0);
+    pm.assignToMesh(pos, 1, h, g);
+    sec((std::string("pm3.assign_") + name).c_str(), gridVec(g));
+    std::vector<double> f;
+    for (auto &p : pos) {
+      auto fr = pm.interpolateForce(eField, p, h, -constants::q);
+      f.insert(f.end(), fr.begin(), fr.end());
+    }
+    sec((std::string("pm3.force_") + name).c_str(), f);
+  };
+  schemes(emcNGPScheme<T, Device>(), "ngp");
+  schemes(emcCICScheme<T, Device>(), "cic");
+  Grid3 pot(ext);
+  for (c.fill(0); !pot.isEndCoord(c); pot.advanceCoord(c))
+    pot[c] = (u() - 0.5) * 10;
+  sec("pm3.pot", gridVec(pot));
+  emcCICScheme<T, Device>().calcEField(eField, pot, dev);
+  for (int d = 0; d < 3; d++)
+    sec((std::string("pm3.efield_") + "xyz"[d]).c_str(), gridVec(eField[d]));
+}
+
 static void dumpDeviceGateKATs() {
   auto dev = makeDevice2();
   dev->addGateContact(emcBoundaryPos::YMAX, 0.4, {5e-9}, {15e-9}, 3.9, 1e-9, 0.3);
@@ -1238,6 +1316,7 @@ int main(int argc, char **argv) {
   dumpBathKATs();
   dumpDeviceKATs();
   dumpDeviceGateKATs();
+  dumpPM3KATs();
   dumpDeviceTrajectories(seed + 5, 64, 150, 1e-15);
   dumpDeviceRun<emcCICScheme<T, Device2>>("cic", seed + 11, 40, 1e-15, 1e-8, 1.8);
   dumpDeviceRun<emcNGPScheme<T, Device2>>("ngp", seed + 13, 40, 1e-15, 1e-8, 1.8);

@@ -14,6 +14,7 @@ import os
 import subprocess
 import tempfile
 
+import numpy as np
 import pytest
 
 from emc_models import (DEVICE_SMALL, MODELS_DEVICE, SI_EPS_R, SI_NI, device_region_specs,
@@ -116,6 +117,7 @@ int main(void) {
   O(emcgpu_autotune_args, spl); O(emcgpu_autotune_args, res_ms_per_step);
   O(emcgpu_autotune_args, best_ms_per_step); O(emcgpu_device_step_args, poisson_interval);
   O(emcgpu_device_step_args, channel_current); O(emcgpu_device_step_args, accumulate);
+  P(emcgpu_grid3); O(emcgpu_grid3, spacing); O(emcgpu_grid3, pm_scheme);
   return 0;
 }
 """
@@ -134,13 +136,42 @@ def test_ctypes_layouts_match_the_c_compiler():
           "emcgpu_contact": emcgpu.ContactDesc, "emcgpu_device_desc": emcgpu.DeviceDesc,
           "emcgpu_device_drift_args": emcgpu.DeviceDriftArgs,
           "emcgpu_device_step_args": emcgpu.DeviceStepArgs,
-          "emcgpu_autotune_args": emcgpu.AutotuneArgs}
+          "emcgpu_autotune_args": emcgpu.AutotuneArgs, "emcgpu_grid3": emcgpu.Grid3}
     for name, cls in py.items():
         assert C.sizeof(cls) == int(out[name]), name
     for key, val in out.items():
         if "." in key:
             t, f = key.split(".")
             assert getattr(py[t], f).offset == int(val), key
+
+
+def test_pm3_operators_check_their_arguments():
+    """The 3-D particle-mesh operators reject the 2-D-only NEC scheme,
+    empty or non-positive grids, an E-field grid without interior nodes and
+    a boundary status that does not fit the faces, before needing a GPU;
+    valid arguments on a host-only context then report the missing GPU."""
+    ctx = emcgpu.Context(precision=emcgpu.FP64, device=-1)
+    ext, h = (4, 3, 3), (1e-9, 1e-9, 1e-9)
+    pos = np.zeros((1, 3))
+    with pytest.raises(emcgpu.EmcGpuError, match="NEC"):
+        ctx.deposit3(ext, h, pos, pm=emcgpu.PM_NEC)
+    with pytest.raises(emcgpu.EmcGpuError, match="ext"):
+        ctx.deposit3((4, 0, 3), h, pos)
+    with pytest.raises(emcgpu.EmcGpuError, match="spacing"):
+        ctx.interpolate3(ext, (1e-9, -1e-9, 1e-9), *[np.zeros(36)] * 3, pos)
+    bc = np.full((3, 36), -2, dtype=np.int8)
+    with pytest.raises(emcgpu.EmcGpuError, match="ext"):
+        ctx.efield3((4, 2, 3), h, 0.025, np.full((3, 24), -1), np.zeros(24))
+    with pytest.raises(emcgpu.EmcGpuError, match="face"):
+        ctx.efield3(ext, h, 0.025, bc, np.zeros(36))  # face nodes marked -2
+    kk, jj, ii = np.meshgrid(range(3), range(3), range(4), indexing="ij")
+    for d, c in ((0, ii), (1, jj), (2, kk)):
+        bc[d, ((c == 0) | (c == ext[d] - 1)).reshape(-1)] = -1
+    with pytest.raises(emcgpu.EmcGpuError, match="GPU"):
+        ctx.efield3(ext, h, 0.025, bc, np.zeros(36))
+    with pytest.raises(emcgpu.EmcGpuError, match="GPU"):
+        ctx.deposit3(ext, h, pos)
+    ctx.close()
 
 
 def test_region_tables_need_an_instance_and_a_region_per_node():

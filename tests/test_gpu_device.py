@@ -2,6 +2,8 @@
 
 * charge assignment, force interpolation (NGP / CIC / NEC) and the E-field
   on the reference's known-answer positions and potentials: bit-exact (FP64);
+  the same for the 3-D NGP and CIC schemes and the 3-D grid-point E-field
+  (emcgpu_deposit3, emcgpu_interpolate3, emcgpu_efield3);
 * Poisson: the GPU's Newton / FP32-multigrid solution of the SOR equations
   against the oracle's SOR fixed point (both converged far below the
   tolerance) and against the discrete equation itself, also with two gate
@@ -92,6 +94,44 @@ def test_efield_bit_exact(golden):
         assert (ex == golden["device.kat.efield_%s_x" % tag]).all()
         assert (ey == golden["device.kat.efield_%s_y" % tag]).all()
         ctx.close()
+
+
+@pytest.mark.parametrize("name", ["ngp", "cic"])
+def test_pm3_known_answers_bit_exact(golden, name):
+    """3-D assignToMesh / interpolateForce on the reference's 96 positions of
+    an 8 x 6 x 5 grid (spacing 1, 0.5, 1 nm): each particle's weights bit
+    for bit against the oracle, the grid (atomic order) against the
+    reference's within 1e-15, the forces bit-exact."""
+    emcgpu = _emc()
+    pm = {"ngp": emcgpu.PM_NGP, "cic": emcgpu.PM_CIC}[name]
+    opm = {"ngp": O.PM_NGP, "cic": O.PM_CIC}[name]
+    print("3-D PM known answers", name, flush=True)
+    ext, h = golden["pm3.ext"].astype(np.int32), golden["pm3.spacing"]
+    pos = golden["pm3.pos"].reshape(-1, 3)
+    ctx = emcgpu.Context(precision=emcgpu.FP32)  # the operators are FP64 in any context
+    for p in pos[:40]:
+        assert (ctx.deposit3(ext, h, p[None, :], pm=pm) == O.pm3_assign(opm, ext, h, p[None, :])).all()
+    g = ctx.deposit3(ext, h, pos, pm=pm)
+    assert np.allclose(g, golden["pm3.assign_" + name], rtol=1e-15, atol=1e-15)
+    assert abs(g.sum() - pos.shape[0]) < 1e-12
+    f = ctx.interpolate3(ext, h, golden["pm3.ex"], golden["pm3.ey"], golden["pm3.ez"], pos,
+                         charge=-O.Q, pm=pm)
+    assert (f.reshape(-1) == golden["pm3.force_" + name]).all()
+    ctx.close()
+
+
+def test_pm3_efield_bit_exact(golden):
+    """3-D calcEFieldAtGridPts with the faces' boundary rule (contacts over
+    parts of XMIN and ZMAX, as the reference's emcSurface reports them)."""
+    emcgpu = _emc()
+    print("3-D E-field known answers", flush=True)
+    ext, h = golden["pm3.ext"].astype(np.int32), golden["pm3.spacing"]
+    ctx = emcgpu.Context(precision=emcgpu.FP64)
+    E = ctx.efield3(ext, h, float(golden["pm3.vt"][0]), golden["pm3.bc"].astype(np.int8),
+                    golden["pm3.pot"])
+    for d, c in enumerate("xyz"):
+        assert (E[d] == golden["pm3.efield_" + c]).all()
+    ctx.close()
 
 
 def _residual(dev, phi, n, C, eq, spec=DEVICE_SMALL):

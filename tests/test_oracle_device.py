@@ -18,6 +18,9 @@ both ends):
 * the reference's own emcSimulation::execute (Poisson every second step,
   transient steps, final averages): its averaged channel current, contact
   current file and averaged potential file from the oracle's run.
+* the 3-D NGP and CIC schemes and the grid-point E-field (an 8 x 6 x 5 grid
+  with a different spacing per dimension, ohmic contacts over parts of two
+  faces): bit-exact.
 The mathematics pins what the fixtures leave open: CIC weights of every
 particle sum to one, NGP mass lands on the nearest node, the SOR fixed point
 solves the discrete Poisson equation.
@@ -85,6 +88,61 @@ def test_efield_bit_exact(dev, golden):
     p = pot.reshape(7, 21)
     want = (p[3, 4] - p[3, 6]) * dev.Vt / (2 * 1e-9)
     assert abs(ex.reshape(7, 21)[3, 5] - want) <= 1e-15 * abs(want)
+
+
+@pytest.mark.parametrize("scheme,name", [(O.PM_NGP, "ngp"), (O.PM_CIC, "cic")])
+def test_pm3_schemes_bit_exact(golden, scheme, name):
+    """3-D assignToMesh and interpolateForce (emcCICScheme.hpp:52-74,
+    143-173; emcNGPScheme.hpp:24-66) against the reference's values."""
+    ext, h = golden["pm3.ext"].astype(np.int32), golden["pm3.spacing"]
+    pos = golden["pm3.pos"].reshape(-1, 3)
+    g = O.pm3_assign(scheme, ext, h, pos)
+    assert (g == golden["pm3.assign_" + name]).all()
+    assert abs(g.sum() - pos.shape[0]) < 1e-12  # one carrier per particle
+    f = O.pm3_force(scheme, ext, h, golden["pm3.ex"], golden["pm3.ey"], golden["pm3.ez"], pos,
+                    -O.Q)
+    assert (f.reshape(-1) == golden["pm3.force_" + name]).all()
+
+
+def test_pm3_weights_and_efield():
+    """What the mathematics fixes: a CIC particle a quarter into a cell
+    along every axis puts (1/4)^3 on the node at the floor corner (the
+    reference's mirrored weight) and (3/4)^3 on the opposite one; NGP puts
+    its carrier on the nearest node; a linear potential gives a constant
+    interior field -dphi/dx * vt, zero at artificial faces and the inner
+    value at contacts."""
+    ext, h = np.array([5, 4, 3], dtype=np.int32), np.array([1e-9, 2e-9, 0.5e-9])
+    g = O.pm3_assign(O.PM_CIC, ext, h, np.array([[1.25e-9, 2.5e-9, 0.125e-9]])).reshape(3, 4, 5)
+    assert abs(g[0, 1, 1] - 0.25 ** 3) < 1e-16 and abs(g[1, 2, 2] - 0.75 ** 3) < 1e-16
+    assert abs(g.sum() - 1.0) < 1e-15
+    n = O.pm3_assign(O.PM_NGP, ext, h, np.array([[2.4e-9, 3.1e-9, 0.7e-9]])).reshape(3, 4, 5)
+    assert n[1, 2, 2] == 1.0 and n.sum() == 1.0
+    nn = int(np.prod(ext))
+    kk, jj, ii = np.meshgrid(range(3), range(4), range(5), indexing="ij")
+    pot = (0.3 * ii).reshape(-1).astype(np.float64)  # slope 0.3 per node along x
+    bc = np.full((3, nn), -2, dtype=np.int32)
+    for d, c in ((0, ii), (1, jj), (2, kk)):
+        on = ((c == 0) | (c == ext[d] - 1)).reshape(-1)
+        bc[d, on] = -1
+    xmin = (ii == 0).reshape(-1)
+    bc[0, xmin] = 0  # XMIN a contact
+    ex, ey, ez = O.pm3_efield(ext, h, 0.025, bc, pot)
+    want = -0.3 * 0.025 / 1e-9
+    inner = ((ii > 0) & (ii < 4)).reshape(-1)
+    assert np.allclose(ex[inner], want, rtol=1e-13)
+    assert np.allclose(ex[xmin], want, rtol=1e-13)      # contact: inner neighbour
+    assert (ex[(ii == 4).reshape(-1)] == 0).all()       # artificial XMAX
+    assert (ey == 0).all() and (ez == 0).all()
+
+
+def test_pm3_efield_bit_exact(golden):
+    """3-D calcEFieldAtGridPts with setEFieldBoundaryValues (emcEFieldCalculation.hpp:
+    13-32, 57-81) against the reference's field, the faces' contact status
+    as the reference's emcSurface reports it."""
+    ext, h = golden["pm3.ext"].astype(np.int32), golden["pm3.spacing"]
+    E = O.pm3_efield(ext, h, float(golden["pm3.vt"][0]), golden["pm3.bc"], golden["pm3.pot"])
+    for d, c in enumerate("xyz"):
+        assert (E[d] == golden["pm3.efield_" + c]).all()
 
 
 def test_sor_bit_exact(dev, golden):

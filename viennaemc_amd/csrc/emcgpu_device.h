@@ -286,6 +286,14 @@ struct DevParticles {
   uint32_t *pid;  // creation index (Philox stream of the particle)
 };
 
+// a 3-D grid of the particle-mesh operators: nx x ny x nz nodes, x fastest
+// (emcGrid.hpp:241-253), spacing [m], scheme EMCGPU_PM_NGP or _CIC
+struct PMGrid3 {
+  int nx, ny, nz;
+  double hx, hy, hz;
+  int pm;
+};
+
 // geometry for the particle kernels (positions in nm)
 template <typename R>
 struct DevGeom {

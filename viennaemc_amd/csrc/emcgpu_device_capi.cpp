@@ -57,6 +57,14 @@ hipError_t dev_launch_assign_pos(const DevGeom<double> &g, const double *pos, lo
                                  double *grid, hipStream_t st);
 hipError_t dev_launch_interp_pos(const DevGeom<double> &g, const double *pos, long long n,
                                  double charge, double *force, hipStream_t st);
+hipError_t dev_launch_pm3_assign(const PMGrid3 &g, const double *pos, long long n, double *grid,
+                                 hipStream_t st);
+hipError_t dev_launch_pm3_interp(const PMGrid3 &g, const double *ex, const double *ey,
+                                 const double *ez, const double *pos, long long n, double charge,
+                                 double *force, hipStream_t st);
+hipError_t dev_launch_pm3_efield(const PMGrid3 &g, double vt, const signed char *bc,
+                                 const double *pot, double *ex, double *ey, double *ez,
+                                 hipStream_t st);
 
 namespace host {
 
@@ -2369,6 +2377,154 @@ emcgpu_status emcgpu_interpolate(emcgpu_ctx *c, const double *ex, const double *
   (void)hipFree(buf);
   if (e != hipSuccess)
     return fail(c, EMCGPU_ERR_HIP, "interpolate: %s", hipGetErrorString(e));
+  return EMCGPU_OK;
+}
+
+// ---- 3-D particle-mesh operators --------------------------------------
+namespace {
+emcgpu_status pm3_grid(emcgpu_ctx *c, const emcgpu_grid3 *g, int min_nodes, PMGrid3 &out,
+                       size_t &nn) {
+  if (!c)
+    return EMCGPU_ERR_ARG;
+  if (!g)
+    return fail(c, EMCGPU_ERR_ARG, "grid must be non-NULL");
+  nn = 1;
+  for (int d = 0; d < 3; d++) {
+    if (g->ext[d] < min_nodes || g->ext[d] > (1 << 20))
+      return fail(c, EMCGPU_ERR_ARG, "ext[%d] = %d must be in [%d, 2^20]", d, g->ext[d],
+                  min_nodes);
+    if (!(g->spacing[d] > 0.0) || !std::isfinite(g->spacing[d]))
+      return fail(c, EMCGPU_ERR_ARG, "spacing[%d] must be positive", d);
+    nn *= (size_t)g->ext[d];
+  }
+  if (nn > ((size_t)1 << 31))
+    return fail(c, EMCGPU_ERR_ARG, "a 3-D grid holds at most 2^31 nodes");
+  if (g->pm_scheme != EMCGPU_PM_NGP && g->pm_scheme != EMCGPU_PM_CIC)
+    return fail(c, EMCGPU_ERR_ARG,
+                "3-D schemes are EMCGPU_PM_NGP and EMCGPU_PM_CIC (the reference's NEC is 2-D)");
+  out = PMGrid3{g->ext[0], g->ext[1], g->ext[2], g->spacing[0], g->spacing[1], g->spacing[2],
+                g->pm_scheme};
+  return EMCGPU_OK;
+}
+// after the arguments are checked (so a host-only context reports them)
+emcgpu_status pm3_gpu(emcgpu_ctx *c) {
+  if (c->device < 0)
+    return fail(c, EMCGPU_ERR_STATE, "the 3-D particle-mesh operators need a GPU context");
+  return EMCGPU_OK;
+}
+}  // namespace
+
+emcgpu_status emcgpu_deposit3(emcgpu_ctx *c, const emcgpu_grid3 *g3, const double *pos, size_t n,
+                              double *grid) {
+  PMGrid3 g;
+  size_t nn = 0;
+  emcgpu_status st = pm3_grid(c, g3, 1, g, nn);
+  if (st)
+    return st;
+  if ((n && !pos) || !grid)
+    return fail(c, EMCGPU_ERR_ARG, "pos and grid must be non-NULL");
+  if ((st = pm3_gpu(c)))
+    return st;
+  double *buf = nullptr;
+  HIPCHK(c, hipMalloc((void **)&buf, (3 * n + nn) * sizeof(double)));
+  double *dpos = buf, *dgrid = buf + 3 * n;
+  hipError_t e = hipMemcpy(dpos, pos, 3 * n * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(dgrid, grid, nn * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = dev_launch_pm3_assign(g, dpos, (long long)n, dgrid, c->stream);
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpy(grid, dgrid, nn * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(buf);
+  if (e != hipSuccess)
+    return fail(c, EMCGPU_ERR_HIP, "deposit3: %s", hipGetErrorString(e));
+  return EMCGPU_OK;
+}
+
+emcgpu_status emcgpu_interpolate3(emcgpu_ctx *c, const emcgpu_grid3 *g3, const double *ex,
+                                  const double *ey, const double *ez, const double *pos, size_t n,
+                                  double charge, double *force) {
+  PMGrid3 g;
+  size_t nn = 0;
+  emcgpu_status st = pm3_grid(c, g3, 1, g, nn);
+  if (st)
+    return st;
+  if (!ex || !ey || !ez || (n && (!pos || !force)))
+    return fail(c, EMCGPU_ERR_ARG, "ex, ey, ez, pos and force must be non-NULL");
+  if ((st = pm3_gpu(c)))
+    return st;
+  double *buf = nullptr;
+  HIPCHK(c, hipMalloc((void **)&buf, (3 * nn + 6 * n) * sizeof(double)));
+  double *de = buf, *dpos = buf + 3 * nn, *dforce = dpos + 3 * n;
+  hipError_t e = hipMemcpy(de, ex, nn * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(de + nn, ey, nn * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(de + 2 * nn, ez, nn * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(dpos, pos, 3 * n * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = dev_launch_pm3_interp(g, de, de + nn, de + 2 * nn, dpos, (long long)n, charge, dforce,
+                              c->stream);
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpy(force, dforce, 3 * n * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(buf);
+  if (e != hipSuccess)
+    return fail(c, EMCGPU_ERR_HIP, "interpolate3: %s", hipGetErrorString(e));
+  return EMCGPU_OK;
+}
+
+emcgpu_status emcgpu_efield3(emcgpu_ctx *c, const emcgpu_grid3 *g3, double vt,
+                             const signed char *bc, const double *pot, double *ex, double *ey,
+                             double *ez) {
+  PMGrid3 g;
+  size_t nn = 0;
+  emcgpu_status st = pm3_grid(c, g3, 3, g, nn);
+  if (st)
+    return st;
+  if (!bc || !pot || !ex || !ey || !ez)
+    return fail(c, EMCGPU_ERR_ARG, "bc, pot, ex, ey and ez must be non-NULL");
+  if (!(vt > 0.0))
+    return fail(c, EMCGPU_ERR_ARG, "vt must be positive");
+  // every node on a face normal to d needs a status (-1 or a contact),
+  // every other node -2
+  for (int d = 0; d < 3; d++)
+    for (size_t m = 0; m < nn; m++) {
+      const size_t nxy = (size_t)g3->ext[0] * g3->ext[1];
+      const size_t cd = d == 0 ? m % g3->ext[0] : d == 1 ? (m % nxy) / g3->ext[0] : m / nxy;
+      const bool face = cd == 0 || cd == (size_t)g3->ext[d] - 1;
+      const int b = bc[d * nn + m];
+      if (face ? b < -1 : b != -2)
+        return fail(c, EMCGPU_ERR_ARG,
+                    "bc[%d][%zu] = %d: a node on a face normal to dimension %d needs -1 or a "
+                    "contact index, any other node -2",
+                    d, m, b, d);
+    }
+  if ((st = pm3_gpu(c)))
+    return st;
+  double *buf = nullptr;
+  HIPCHK(c, hipMalloc((void **)&buf, 4 * nn * sizeof(double) + 3 * nn));
+  double *dpot = buf, *de = buf + nn;
+  signed char *dbc = (signed char *)(buf + 4 * nn);
+  hipError_t e = hipMemcpy(dpot, pot, nn * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(dbc, bc, 3 * nn, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemset(de, 0, 3 * nn * sizeof(double));
+  if (e == hipSuccess)
+    e = dev_launch_pm3_efield(g, vt, dbc, dpot, de, de + nn, de + 2 * nn, c->stream);
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(c->stream);
+  double *outs[3] = {ex, ey, ez};
+  for (int d = 0; d < 3 && e == hipSuccess; d++)
+    e = hipMemcpy(outs[d], de + d * nn, nn * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(buf);
+  if (e != hipSuccess)
+    return fail(c, EMCGPU_ERR_HIP, "efield3: %s", hipGetErrorString(e));
   return EMCGPU_OK;
 }
 

@@ -1006,6 +1006,142 @@ __global__ __launch_bounds__(256) void k_dev_interp_pos(DevGeom<double> g, const
 }
 
 // ---------------------------------------------------------------------------
+// 3-D particle-mesh operators on caller positions and grids (FP64): the
+// reference's 3-D NGP and CIC schemes and the grid-point E-field
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool pm3_in(const PMGrid3 &g, long long i, long long j, long long k) {
+  return i >= 0 && j >= 0 && k >= 0 && i < g.nx && j < g.ny && k < g.nz;
+}
+__device__ __forceinline__ long long pm3_idx(const PMGrid3 &g, long long i, long long j,
+                                             long long k) {
+  return i + (long long)g.nx * (j + (long long)g.ny * k);
+}
+// a node coordinate from round / floor of pos / h; -1 for values no grid
+// holds (negative, huge or not a number), whose nodes are then skipped
+__device__ __forceinline__ long long pm3_coord(double c) {
+  return (c >= 0.0 && c < 1073741824.0) ? (long long)c : -1;
+}
+
+// assignToMesh of one carrier per position: NGP puts it on the node at
+// round(pos / h) (emcNGPScheme.hpp:36-48); CIC gives the node at the floor
+// corner wX wY wZ and its neighbours the complements, each weight a left-
+// to-right product (emcCICScheme.hpp:97-120).  Nodes outside the grid are
+// skipped.  The grid sums in atomic (unordered) order.
+__global__ __launch_bounds__(256) void k_pm3_assign(PMGrid3 g, const double *pos, long long n,
+                                                    double *grid) {
+#pragma clang fp contract(off)
+  const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (q >= n)
+    return;
+  const double sx = pos[3 * q] / g.hx, sy = pos[3 * q + 1] / g.hy, sz = pos[3 * q + 2] / g.hz;
+  auto put = [&](long long i, long long j, long long k, double w) {
+    if (pm3_in(g, i, j, k))
+      atomicAdd(&grid[pm3_idx(g, i, j, k)], w);
+  };
+  if (g.pm == EMCGPU_PM_NGP) {
+    put(pm3_coord(round(sx)), pm3_coord(round(sy)), pm3_coord(round(sz)), 1.0);
+    return;
+  }
+  const double fx = floor(sx), fy = floor(sy), fz = floor(sz);
+  const long long i = pm3_coord(fx), j = pm3_coord(fy), k = pm3_coord(fz);
+  if (i < 0 || j < 0 || k < 0)
+    return;
+  const double wx = sx - fx, wy = sy - fy, wz = sz - fz;
+  put(i, j, k, wx * wy * wz);
+  put(i + 1, j, k, (1.0 - wx) * wy * wz);
+  put(i, j + 1, k, wx * (1.0 - wy) * wz);
+  put(i + 1, j + 1, k, (1.0 - wx) * (1.0 - wy) * wz);
+  put(i, j, k + 1, wx * wy * (1.0 - wz));
+  put(i + 1, j, k + 1, (1.0 - wx) * wy * (1.0 - wz));
+  put(i, j + 1, k + 1, wx * (1.0 - wy) * (1.0 - wz));
+  put(i + 1, j + 1, k + 1, (1.0 - wx) * (1.0 - wy) * (1.0 - wz));
+}
+
+// interpolateForce: NGP the nearest node's field times the charge
+// (emcNGPScheme.hpp:51-66); CIC the eight-node sum with the reference's
+// (1 - wY)(1 - wY) weight on the (x+1, y+1) nodes of both z layers, times
+// the charge (emcCICScheme.hpp:143-173).  Nodes outside the grid count zero.
+__global__ __launch_bounds__(256) void k_pm3_interp(PMGrid3 g, const double *ex,
+                                                    const double *ey, const double *ez,
+                                                    const double *pos, long long n, double charge,
+                                                    double *force) {
+#pragma clang fp contract(off)
+  const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (q >= n)
+    return;
+  const double *E[3] = {ex, ey, ez};
+  const double sx = pos[3 * q] / g.hx, sy = pos[3 * q + 1] / g.hy, sz = pos[3 * q + 2] / g.hz;
+  auto at = [&](const double *e, long long i, long long j, long long k) {
+    return pm3_in(g, i, j, k) ? e[pm3_idx(g, i, j, k)] : 0.0;
+  };
+  if (g.pm == EMCGPU_PM_NGP) {
+    const long long i = pm3_coord(round(sx)), j = pm3_coord(round(sy)), k = pm3_coord(round(sz));
+    for (int d = 0; d < 3; ++d)
+      force[3 * q + d] = charge * at(E[d], i, j, k);
+    return;
+  }
+  const double fx = floor(sx), fy = floor(sy), fz = floor(sz);
+  const long long i = pm3_coord(fx), j = pm3_coord(fy), k = pm3_coord(fz);
+  const double wx = sx - fx, wy = sy - fy, wz = sz - fz;
+  const bool ok = i >= 0 && j >= 0 && k >= 0;
+  for (int d = 0; d < 3; ++d) {
+    const double *e = E[d];
+    double f = 0.0;
+    if (ok)
+      f = (at(e, i, j, k) * wx * wy * wz + at(e, i + 1, j, k) * (1.0 - wx) * wy * wz +
+           at(e, i, j + 1, k) * wx * (1.0 - wy) * wz +
+           at(e, i + 1, j + 1, k) * (1.0 - wy) * (1.0 - wy) * wz +
+           at(e, i, j, k + 1) * wx * wy * (1.0 - wz) +
+           at(e, i + 1, j, k + 1) * (1.0 - wx) * wy * (1.0 - wz) +
+           at(e, i, j + 1, k + 1) * wx * (1.0 - wy) * (1.0 - wz) +
+           at(e, i + 1, j + 1, k + 1) * (1.0 - wy) * (1.0 - wy) * (1.0 - wz)) *
+          charge;
+    force[3 * q + d] = f;
+  }
+}
+
+// calcEFieldAtGridPts (emcEFieldCalculation.hpp:13-32): off the faces of
+// component d, E_d = (phi_prev - phi_next) * vt / (2 h_d); one node per
+// thread
+__global__ __launch_bounds__(256) void k_pm3_efield(PMGrid3 g, double vt, const double *pot,
+                                                    double *ex, double *ey, double *ez) {
+#pragma clang fp contract(off)
+  const long long m = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long nn = (long long)g.nx * g.ny * g.nz;
+  if (m >= nn)
+    return;
+  const long long nxy = (long long)g.nx * g.ny;
+  const long long k = m / nxy, r = m - k * nxy, j = r / g.nx, i = r - j * g.nx;
+  const long long c[3] = {i, j, k}, ext[3] = {g.nx, g.ny, g.nz}, s[3] = {1, g.nx, nxy};
+  const double h[3] = {g.hx, g.hy, g.hz};
+  double *E[3] = {ex, ey, ez};
+  for (int d = 0; d < 3; ++d)
+    if (c[d] != 0 && c[d] != ext[d] - 1)
+      E[d][m] = (pot[m - s[d]] - pot[m + s[d]]) * vt / (2.0 * h[d]);
+}
+
+// setEFieldBoundaryValues (emcEFieldCalculation.hpp:57-81) after the
+// interior pass: on a face normal to d, bc[d][node] -1 (artificial) sets
+// E_d = 0, a contact (>= 0) the inner neighbour's value along d
+__global__ __launch_bounds__(256) void k_pm3_faces(PMGrid3 g, const signed char *bc, double *ex,
+                                                   double *ey, double *ez) {
+  const long long m = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long nn = (long long)g.nx * g.ny * g.nz;
+  if (m >= nn)
+    return;
+  const long long nxy = (long long)g.nx * g.ny;
+  const long long k = m / nxy, r = m - k * nxy, j = r / g.nx, i = r - j * g.nx;
+  const long long c[3] = {i, j, k}, s[3] = {1, g.nx, nxy};
+  double *E[3] = {ex, ey, ez};
+  for (int d = 0; d < 3; ++d) {
+    const int b = bc[(long long)d * nn + m];
+    if (b == -2)
+      continue;
+    E[d][m] = b == -1 ? 0.0 : (c[d] == 0 ? E[d][m + s[d]] : E[d][m - s[d]]);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Poisson (Newton + FP32 multigrid) and E-field on one workgroup
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double block_max(double v, double *red) {
@@ -1979,6 +2115,36 @@ hipError_t dev_launch_interp_pos(const DevGeom<double> &g, const double *pos, lo
     return hipSuccess;
   hipLaunchKernelGGL(k_dev_interp_pos, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, pos,
                      n, charge, force);
+  return hipGetLastError();
+}
+
+hipError_t dev_launch_pm3_assign(const PMGrid3 &g, const double *pos, long long n, double *grid,
+                                 hipStream_t st) {
+  if (n <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_pm3_assign, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, pos, n,
+                     grid);
+  return hipGetLastError();
+}
+hipError_t dev_launch_pm3_interp(const PMGrid3 &g, const double *ex, const double *ey,
+                                 const double *ez, const double *pos, long long n, double charge,
+                                 double *force, hipStream_t st) {
+  if (n <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_pm3_interp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, ex, ey,
+                     ez, pos, n, charge, force);
+  return hipGetLastError();
+}
+hipError_t dev_launch_pm3_efield(const PMGrid3 &g, double vt, const signed char *bc,
+                                 const double *pot, double *ex, double *ey, double *ez,
+                                 hipStream_t st) {
+  const long long nn = (long long)g.nx * g.ny * g.nz;
+  const dim3 grid((unsigned)((nn + 255) / 256));
+  hipLaunchKernelGGL(k_pm3_efield, grid, dim3(256), 0, st, g, vt, pot, ex, ey, ez);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return e;
+  hipLaunchKernelGGL(k_pm3_faces, grid, dim3(256), 0, st, g, bc, ex, ey, ez);
   return hipGetLastError();
 }
 

@@ -111,6 +111,11 @@ class DeviceDesc(C.Structure):
                 ("region_tables", C.c_int)]
 
 
+class Grid3(C.Structure):
+    """emcgpu_grid3: a 3-D grid of the particle-mesh operators."""
+    _fields_ = [("ext", C.c_int * 3), ("spacing", C.c_double * 3), ("pm_scheme", C.c_int)]
+
+
 class DeviceDriftArgs(C.Structure):
     _fields_ = [("dt", C.c_double), ("n_steps", C.c_int), ("mode", C.c_int),
                 ("seed", C.c_uint64), ("step0", C.c_uint32), ("draws", C.c_void_p),
@@ -160,7 +165,8 @@ EXPORTS = [
     "emcgpu_autotune", "emcgpu_device_averages", "emcgpu_set_screening", "emcgpu_get_screening",
     "emcgpu_update_screening", "emcgpu_build_tables_all", "emcgpu_step_noalloc",
     "emcgpu_reserve", "emcgpu_get_bath_energy", "emcgpu_set_box", "emcgpu_set_positions",
-    "emcgpu_get_positions", "emcgpu_observe",
+    "emcgpu_get_positions", "emcgpu_observe", "emcgpu_deposit3", "emcgpu_interpolate3",
+    "emcgpu_efield3",
 ]
 
 
@@ -216,6 +222,9 @@ def load(path=LIB_PATH):
         "emcgpu_efield": (i, [vp, vp, vp, vp]),
         "emcgpu_deposit": (i, [vp, vp, C.c_size_t, vp]),
         "emcgpu_interpolate": (i, [vp, vp, vp, vp, C.c_size_t, d, vp]),
+        "emcgpu_deposit3": (i, [vp, p(Grid3), vp, C.c_size_t, vp]),
+        "emcgpu_interpolate3": (i, [vp, p(Grid3), vp, vp, vp, vp, C.c_size_t, d, vp]),
+        "emcgpu_efield3": (i, [vp, p(Grid3), d, vp, vp, vp, vp, vp]),
         "emcgpu_contacts": (i, [vp, vp, C.c_size_t, vp, vp]),
         "emcgpu_device_set_particles": (i, [vp, C.c_size_t, vp, vp, vp, vp, vp, vp]),
         "emcgpu_device_get_particles": (i, [vp, p(C.c_size_t), vp, vp, vp, vp, vp, vp]),
@@ -734,6 +743,49 @@ class Context:
         self._chk(self.L.emcgpu_interpolate(self.h, _ptr(ex), _ptr(ey), _ptr(pos), pos.shape[0],
                                             charge, _ptr(f)))
         return f
+
+    @staticmethod
+    def _grid3(ext, spacing, pm):
+        g = Grid3()
+        for d in range(3):
+            g.ext[d] = int(ext[d])
+            g.spacing[d] = float(spacing[d])
+        g.pm_scheme = int(pm)
+        return g
+
+    def deposit3(self, ext, spacing, pos, pm=None, grid=None):
+        """3-D assignToMesh (NGP or CIC) of pos[n][3] [m], one carrier each,
+        into a grid of ext nodes (x fastest)."""
+        g3 = self._grid3(ext, spacing, PM_CIC if pm is None else pm)
+        nn = int(np.prod(ext))
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        g = np.zeros(nn) if grid is None else np.array(grid, dtype=np.float64).reshape(-1)
+        self._chk(self.L.emcgpu_deposit3(self.h, C.byref(g3), _ptr(pos), pos.shape[0], _ptr(g)))
+        return g
+
+    def interpolate3(self, ext, spacing, ex, ey, ez, pos, charge=-Q, pm=None):
+        """3-D interpolateForce: force[n][3] [N] from nodal ex, ey, ez [V/m]."""
+        g3 = self._grid3(ext, spacing, PM_CIC if pm is None else pm)
+        c = lambda a: np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        ex, ey, ez = c(ex), c(ey), c(ez)
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        f = np.zeros((pos.shape[0], 3))
+        self._chk(self.L.emcgpu_interpolate3(self.h, C.byref(g3), _ptr(ex), _ptr(ey), _ptr(ez),
+                                             _ptr(pos), pos.shape[0], charge, _ptr(f)))
+        return f
+
+    def efield3(self, ext, spacing, vt, bc, pot):
+        """3-D grid-point E-field (ex, ey, ez) [V/m] from the normalised
+        potential; bc[3][nodes] the faces' boundary status (-2 off the
+        component's faces, -1 artificial, >= 0 contact)."""
+        g3 = self._grid3(ext, spacing, PM_CIC)
+        nn = int(np.prod(ext))
+        bc = np.ascontiguousarray(bc, dtype=np.int8).reshape(-1)
+        pot = np.ascontiguousarray(pot, dtype=np.float64).reshape(-1)
+        E = [np.zeros(nn) for _ in range(3)]
+        self._chk(self.L.emcgpu_efield3(self.h, C.byref(g3), vt, _ptr(bc), _ptr(pot),
+                                        *[_ptr(e) for e in E]))
+        return E
 
     def contacts(self, pos):
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 2)
