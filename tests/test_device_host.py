@@ -171,6 +171,10 @@ def test_pm3_operators_check_their_arguments():
         ctx.efield3(ext, h, 0.025, bc, np.zeros(36))
     with pytest.raises(emcgpu.EmcGpuError, match="GPU"):
         ctx.deposit3(ext, h, pos)
+    with pytest.raises(ValueError, match="grid"):  # host buffers of the wrong size
+        ctx.deposit3(ext, h, pos, grid=np.zeros(35))
+    with pytest.raises(ValueError, match="bc"):
+        ctx.efield3(ext, h, 0.025, bc[:, :30], np.zeros(36))
     ctx.close()
 
 

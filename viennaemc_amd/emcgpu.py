@@ -248,6 +248,12 @@ def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
+def _need_size(name, a, n):
+    """The C ABI takes host buffers by pointer: their sizes are checked here."""
+    if a.size != n:
+        raise ValueError("%s holds %d values, the grid needs %d" % (name, a.size, n))
+
+
 def valley_desc(v, particle_mass=ME):
     """Lower a valley dictionary (viennaemc_amd/workloads.py format) to the C descriptor."""
     d = ValleyDesc()
@@ -717,6 +723,9 @@ class Context:
     def poisson(self, pot, conc=None, equilibrium=False, reset_bc=True):
         pot = np.array(pot, dtype=np.float64).reshape(-1)
         c = None if conc is None else np.ascontiguousarray(conc, dtype=np.float64).reshape(-1)
+        _need_size("pot", pot, self.nn)
+        if c is not None:
+            _need_size("conc", c, self.nn)
         it = C.c_int()
         res = C.c_double()
         self._chk(self.L.emcgpu_poisson(self.h, _ptr(pot), _ptr(c), 1 if equilibrium else 0,
@@ -725,19 +734,23 @@ class Context:
 
     def efield(self, pot):
         pot = np.ascontiguousarray(pot, dtype=np.float64).reshape(-1)
+        _need_size("pot", pot, self.nn)
         ex, ey = np.zeros(self.nn), np.zeros(self.nn)
         self._chk(self.L.emcgpu_efield(self.h, _ptr(pot), _ptr(ex), _ptr(ey)))
         return ex, ey
 
     def deposit(self, pos, grid=None):
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 2)
-        g = np.zeros(self.nn) if grid is None else np.array(grid, dtype=np.float64)
+        g = np.zeros(self.nn) if grid is None else np.array(grid, dtype=np.float64).reshape(-1)
+        _need_size("grid", g, self.nn)
         self._chk(self.L.emcgpu_deposit(self.h, _ptr(pos), pos.shape[0], _ptr(g)))
         return g
 
     def interpolate(self, ex, ey, pos, charge=-Q):
         ex = np.ascontiguousarray(ex, dtype=np.float64).reshape(-1)
         ey = np.ascontiguousarray(ey, dtype=np.float64).reshape(-1)
+        _need_size("ex", ex, self.nn)
+        _need_size("ey", ey, self.nn)
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 2)
         f = np.zeros((pos.shape[0], 2))
         self._chk(self.L.emcgpu_interpolate(self.h, _ptr(ex), _ptr(ey), _ptr(pos), pos.shape[0],
@@ -760,6 +773,7 @@ class Context:
         nn = int(np.prod(ext))
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
         g = np.zeros(nn) if grid is None else np.array(grid, dtype=np.float64).reshape(-1)
+        _need_size("grid", g, nn)
         self._chk(self.L.emcgpu_deposit3(self.h, C.byref(g3), _ptr(pos), pos.shape[0], _ptr(g)))
         return g
 
@@ -768,6 +782,8 @@ class Context:
         g3 = self._grid3(ext, spacing, PM_CIC if pm is None else pm)
         c = lambda a: np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
         ex, ey, ez = c(ex), c(ey), c(ez)
+        for name, a in (("ex", ex), ("ey", ey), ("ez", ez)):
+            _need_size(name, a, int(np.prod(ext)))
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
         f = np.zeros((pos.shape[0], 3))
         self._chk(self.L.emcgpu_interpolate3(self.h, C.byref(g3), _ptr(ex), _ptr(ey), _ptr(ez),
@@ -782,6 +798,8 @@ class Context:
         nn = int(np.prod(ext))
         bc = np.ascontiguousarray(bc, dtype=np.int8).reshape(-1)
         pot = np.ascontiguousarray(pot, dtype=np.float64).reshape(-1)
+        _need_size("bc", bc, 3 * nn)
+        _need_size("pot", pot, nn)
         E = [np.zeros(nn) for _ in range(3)]
         self._chk(self.L.emcgpu_efield3(self.h, C.byref(g3), vt, _ptr(bc), _ptr(pot),
                                         *[_ptr(e) for e in E]))
@@ -820,6 +838,8 @@ class Context:
     def device_set_efield(self, ex, ey):
         ex = np.ascontiguousarray(ex, dtype=np.float64).reshape(-1)
         ey = np.ascontiguousarray(ey, dtype=np.float64).reshape(-1)
+        _need_size("ex", ex, self.nn)
+        _need_size("ey", ey, self.nn)
         self._chk(self.L.emcgpu_device_set_efield(self.h, _ptr(ex), _ptr(ey)))
 
     def device_drift(self, dt, n_steps, n, mode=PRODUCTION, seed=0, step0=0, draws=None,
