@@ -452,7 +452,7 @@ struct DevStepArgs {
   uint32_t key0, key1;
   int n_contacts, n_slots;
   R *dep;                 // [block][node] deposits of the step kernel
-  uint32_t *seg_cnt;      // [segment][slot] contact particles per segment
+  uint32_t *seg_cnt;      // [slot][segment] contact particles per segment
   uint32_t *seg_nst;      // [segment] staged entries
   uint2 *stage;           // (particle, slot << 22 | rank in segment) at segment start
   int16_t *pslot;         // [particle] ohmic slot of the position after the move, -1 if none
@@ -489,7 +489,7 @@ struct DevContactArgs {
   DevHist hist;
   uint32_t key0, key1;
   int n_contacts, n_slots, n_segments, n_blocks_dep;
-  const uint32_t *seg_cnt;
+  const uint32_t *seg_cnt;  // [slot][segment]
   uint32_t *seg_off;       // [slot][segment] exclusive offsets
   const uint32_t *seg_nst;
   const uint2 *stage;

@@ -548,8 +548,10 @@ __device__ __forceinline__ void dev_rank(const DevStepArgs<R> &a, long long n_se
       nst += (uint32_t)__popcll(at);
     }
   }
+  // slot-major, so the contact scan of a slot reads its segments' counts
+  // contiguously
   for (int k = lane; k < a.n_slots; k += 64)
-    a.seg_cnt[seg * a.n_slots + k] = mycnt[k];
+    a.seg_cnt[(long long)k * n_seg + seg] = mycnt[k];
   if (lane == 0)
     a.seg_nst[seg] = nst;
 }
@@ -598,8 +600,7 @@ __global__ __launch_bounds__(256) void k_dev_contact_scan(DevContactArgs<R> a) {
   __shared__ uint32_t sc[256];
   const int slot = blockIdx.x, tid = threadIdx.x;
   const int S = a.n_segments;
-  const uint32_t *cnt = a.seg_cnt + slot;  // column of the [segment][slot] counts
-  const long long cs = a.n_slots;
+  const uint32_t *cnt = a.seg_cnt + (long long)slot * S;  // row of the [slot][segment] counts
   uint32_t *off = a.seg_off + (long long)slot * S;
   const int per = (S + 255) / 256;
   const int b0 = tid * per < S ? tid * per : S;
@@ -610,7 +611,7 @@ __global__ __launch_bounds__(256) void k_dev_contact_scan(DevContactArgs<R> a) {
     uint32_t v[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u)
-      v[u] = b + u < b1 ? cnt[(long long)(b + u) * cs] : 0u;
+      v[u] = b + u < b1 ? cnt[b + u] : 0u;
 #pragma unroll
     for (int u = 0; u < 16; ++u)
       tot += v[u];
@@ -628,7 +629,7 @@ __global__ __launch_bounds__(256) void k_dev_contact_scan(DevContactArgs<R> a) {
     uint32_t v[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u)
-      v[u] = b + u < b1 ? cnt[(long long)(b + u) * cs] : 0u;
+      v[u] = b + u < b1 ? cnt[b + u] : 0u;
 #pragma unroll
     for (int u = 0; u < 16; ++u)
       if (b + u < b1) {
