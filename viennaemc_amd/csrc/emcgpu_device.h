@@ -384,6 +384,9 @@ struct DevFieldArgs {
   // keeps the solve when the last check met the stopping rule, and clears
   // the checks
   int green_iters;
+  // 1: the E-field of a solve kept from the corrections was written by
+  // their last check; computed here only when this launch solves
+  int efield_if_solved;
   const unsigned long long *green_flag;
   unsigned long long *green_acc;
 };
@@ -438,6 +441,15 @@ struct GreenArgs {
   // 1: corrections by the separable inverse (fd; g holds F), 0: by G (g = H F)
   int use_fd;
   FdOps fd;
+  // with fuse_efield, the last check (c == iters) also writes the E-field
+  // and the force grid of the corrected potential (the formulas of
+  // k_dev_efield); the field kernel recomputes them only when it changes
+  // the potential itself
+  int fuse_efield, iters, pm, acc_fp64;
+  double vt, hxm, hym, acc_scale;
+  const signed char *bcx, *bcy;
+  double *ex, *ey;
+  void *force;  // the force grid (DevFieldArgs::acc), float or double [node][2] (acc_fp64)
 };
 
 // particle step of a device ensemble

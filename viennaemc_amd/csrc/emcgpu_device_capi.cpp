@@ -1669,8 +1669,13 @@ DevFieldArgs<R> step_field_args(const emcgpu_ctx *c, const DeviceSim &d, bool re
   f.nrpart = d.d_nrpart;
   f.interval = d.interval;
   // the E-field and force grid of a step over all CUs (k_dev_efield)
-  if (split_efield(d))
+  // grids of at least 3 x 3 take the E-field over all CUs: from the
+  // corrections' last check when the corrections run, else from its own
+  // launch after the field kernel
+  if (split_efield(d) && !(d.green_use && !reset_bc))
     f.do_efield = 0;
+  if (split_efield(d) && d.green_use && !reset_bc)
+    f.efield_if_solved = 1;
   if (d.green_use && !reset_bc) {
     f.green_iters = green_iters(d);
     f.green_flag = d.d_green_flag;
@@ -1682,11 +1687,30 @@ DevFieldArgs<R> step_field_args(const emcgpu_ctx *c, const DeviceSim &d, bool re
 // the kernel sequence of one step (emcSimulation.hpp:164-193)
 template <typename R>
 emcgpu_status launch_step(emcgpu_ctx *c, DeviceSim &d, bool reset_bc) {
-  if (d.green_use && !reset_bc)
-    HIPCHK(c, dev_launch_green(green_args(d), green_iters(d), d.green_blocks, c->stream));
+  const bool corr = d.green_use && !reset_bc;
+  if (corr) {
+    GreenArgs ga = green_args(d);
+    if (split_efield(d)) {
+      const DevFieldArgs<R> f = field_args<R>(c, d);
+      ga.fuse_efield = 1;
+      ga.iters = green_iters(d);
+      ga.pm = f.pm;
+      ga.acc_fp64 = sizeof(R) == sizeof(double);
+      ga.vt = f.vt;
+      ga.hxm = f.hxm;
+      ga.hym = f.hym;
+      ga.acc_scale = f.acc_scale;
+      ga.bcx = f.bcx;
+      ga.bcy = f.bcy;
+      ga.ex = f.ex;
+      ga.ey = f.ey;
+      ga.force = f.acc;
+    }
+    HIPCHK(c, dev_launch_green(ga, green_iters(d), d.green_blocks, c->stream));
+  }
   HIPCHK(c, dev_launch_field<R>(fargs_slot<R>(d, reset_bc ? 1 : 0), d.mg_lds, d.mg_global,
                                  c->stream));
-  if (split_efield(d))
+  if (split_efield(d) && !corr)
     HIPCHK(c, dev_launch_efield<R>(fargs_slot<R>(d, reset_bc ? 1 : 0), (int)d.nn, c->stream));
   const DevContactArgs<R> ca = contact_args<R>(c, d);
   HIPCHK(c, dev_launch_step<R>(step_args<R>(c, d), ca, d.blocks, d.n_seg, d.step_lds, c->stream));

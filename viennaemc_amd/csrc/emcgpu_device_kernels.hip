@@ -1882,7 +1882,7 @@ __global__ __launch_bounds__(kDevThreads) void k_dev_field(const DevFieldArgs<R>
     }
     __syncthreads();
   }
-  if (a.do_efield) {
+  if (a.do_efield && !(a.efield_if_solved && green_ok)) {
 #pragma clang fp contract(off)
     const bool nec = a.pm == EMCGPU_PM_NEC;
     for (int k = tid; k < nn; k += kDevThreads) {
@@ -2186,6 +2186,38 @@ __global__ __launch_bounds__(256) void k_green_res(GreenArgs a, int c) {
     sc = wk * a.hp * (fabs(C) + n + p);
     if (wk != 0.0 && a.hp * p > a.shift_max)
       atomicMax(a.flag, (unsigned long long)a.state->step + 1ull);
+    if (a.fuse_efield && c == a.iters) {
+      // calcEField of the corrected potential (k_dev_efield)
+      const int nx = a.nx, j = idiv(k, nx), i = k - j * nx;
+      const bool nec = a.pm == EMCGPU_PM_NEC;
+      auto fx = [&](int q) {
+        return nec ? (pot[q] - pot[q + 1]) * a.vt / a.hxm
+                   : (pot[q - 1] - pot[q + 1]) * a.vt / (2 * a.hxm);
+      };
+      auto fy = [&](int q) {
+        return nec ? (pot[q] - pot[q + nx]) * a.vt / a.hym
+                   : (pot[q - nx] - pot[q + nx]) * a.vt / (2 * a.hym);
+      };
+      const int bx = a.bcx[k], by = a.bcy[k];
+      double ex, ey;
+      if (bx == -2)
+        ex = fx(k);
+      else
+        ex = bx == -1 ? 0.0 : fx(i == 0 ? k + 1 : k - 1);
+      if (by == -2)
+        ey = fy(k);
+      else
+        ey = by == -1 ? 0.0 : fy(j == 0 ? k + nx : k - nx);
+      a.ex[k] = ex;
+      a.ey[k] = ey;
+      if (a.acc_fp64) {
+        ((double *)a.force)[2 * k] = ex * a.acc_scale;
+        ((double *)a.force)[2 * k + 1] = ey * a.acc_scale;
+      } else {
+        ((float *)a.force)[2 * k] = (float)(ex * a.acc_scale);
+        ((float *)a.force)[2 * k + 1] = (float)(ey * a.acc_scale);
+      }
+    }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
